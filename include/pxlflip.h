@@ -1,0 +1,186 @@
+/*
+ * pxlflip.h -- C ABI of the MI355X-native FLIP liquid step (HIP, gfx950).
+ *
+ * Drop-in boundary for pxlengine's fluid path.  In the reference the caller is
+ * Renderer::render_sequence (crates/pxl-render/src/lib.rs:204-252), which drives
+ * FluidState (crates/pxl-render/src/fluid.rs:74-306) over the Slang kernels of
+ * shaders/fluid.slang:86-220.  Those are crate-private Rust today; each entry point below
+ * names the reference function whose role it takes over.
+ *
+ * Conventions
+ *  - Every function returning int returns PXLFLIP_OK (0) or a negative PXLFLIP_ERR_*;
+ *    the message is available from pxlflip_last_error(handle) (or pxlflip_last_error(NULL)
+ *    for failures of the create functions, kept per host thread).
+ *  - Plain pointers and sizes only.  The library owns all device memory of a handle (HBM,
+ *    released by pxlflip_destroy); the caller owns every host array it passes in.
+ *  - A handle is used by one host thread at a time.  Calls return when their GPU work is
+ *    complete (the reference's submit + fence wait, crates/pxl-gpu/src/lib.rs:266-293).
+ *  - Particle arrays use the reference's particle record: 8 floats per particle,
+ *    {x, y, z, 0, vx, vy, vz, 0} (fluid.slang:9, fluid.rs:143), in seeding order.
+ */
+#ifndef PXLFLIP_H
+#define PXLFLIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PXLFLIP_OK 0
+#define PXLFLIP_ERR_INVALID (-1)  /* bad scene block / arguments (validation message) */
+#define PXLFLIP_ERR_OOM (-2)      /* device or host allocation failed */
+#define PXLFLIP_ERR_DEVICE (-3)   /* HIP runtime / device error */
+#define PXLFLIP_ERR_INTERNAL (-4) /* library bug */
+
+/* Pressure solver.  RBGS is the reference's red-black Gauss-Seidel with
+ * pressure_iterations sweeps (fluid.slang:167-182, fluid.rs:295-300).  CG and MIXED are
+ * extensions: Jacobi-preconditioned conjugate gradients on the same matrix, FP32 with a
+ * fixed iteration count (CG), or FP32 iterations with FP64 dot products inside an FP64
+ * iterative-refinement loop run to a relative FP64 residual (MIXED). */
+#define PXLFLIP_SOLVER_RBGS 0
+#define PXLFLIP_SOLVER_CG 1
+#define PXLFLIP_SOLVER_MIXED 2
+
+typedef struct pxlflip pxlflip;
+
+/* Options the reference has no field for (it always runs RBGS on device 0). */
+typedef struct pxlflip_opts {
+    uint32_t struct_size;     /* sizeof(pxlflip_opts) */
+    int32_t solver;           /* PXLFLIP_SOLVER_* (default RBGS) */
+    uint32_t cg_iterations;   /* CG: iterations per step; 0 = pressure_iterations */
+    uint32_t inner_iterations;/* MIXED: FP32 iterations per refinement pass (default 40) */
+    uint32_t max_outer;       /* MIXED: refinement passes cap (default 50) */
+    double tolerance;         /* MIXED: ||b - A p|| / ||b|| target in FP64 (default 1e-6) */
+    int32_t device;           /* HIP device ordinal (default 0) */
+    uint32_t scene_index;     /* i in "fluids[i] 'name': ..." validation messages */
+    int32_t use_graph;        /* 1 = replay each step as a captured hipGraph (default 1) */
+} pxlflip_opts;
+
+/* Fills *opts with the defaults above. */
+int pxlflip_default_opts(pxlflip_opts *opts);
+
+/* Replaces Fluid deserialisation + Scene::validate's fluid checks
+ * (crates/pxl-scene/src/lib.rs:49-119, 710-735) and FluidState::new
+ * (crates/pxl-render/src/fluid.rs:109-228): parses one `fluids[]` object (strict: unknown
+ * fields are rejected), derives the grid (Fluid::grid, lib.rs:123-128), seeds the particles
+ * (fluid.rs:123-149, seed = render.seed) and allocates the simulation in HBM.
+ * fps is render.fps (lib.rs:626-627).  opts may be NULL. */
+int pxlflip_create_from_json(const char *fluid_json, double fps, uint32_t seed,
+                             const pxlflip_opts *opts, pxlflip **out);
+
+/* The host half of create, touching no device: parse + validate the block (as above) and
+ * report its grid and seeding count -- what `pxl check` prints for a fluid
+ * (crates/pxl-cli/src/check.rs:98-104).  info gets n, h, origin, gravity, flip, dt (from
+ * fps), substeps, pressure_iterations, n_particles (for `seed`), n_faces and n_cells;
+ * device_bytes is the HBM a handle would allocate with `opts`.  info may be NULL. */
+typedef struct pxlflip_info pxlflip_info;
+int pxlflip_check_json(const char *fluid_json, double fps, uint32_t seed, const pxlflip_opts *opts,
+                       pxlflip_info *info);
+
+/* Host seeding only (fluid.rs:123-148): writes the initial particles of `fluid_json`
+ * for `seed` as 8-float records into aos8 (capacity max_particles) and their number into
+ * *count.  With aos8 NULL only counts. */
+int pxlflip_seed_particles(const char *fluid_json, uint32_t seed, float *aos8, uint64_t max_particles,
+                           uint64_t *count);
+
+/* Releases the handle and all of its device memory (drop of FluidState, fluid.rs:74-94). */
+void pxlflip_destroy(pxlflip *h);
+
+/* Replaces fluid_advance's stepping (fluid.rs:265-306): brings the simulation to
+ * steps_for_frame(frame) = round((warmup + frame/fps) * fps * substeps) steps
+ * (fluid.rs:234-236), re-seeding from the initial particles first when the target is
+ * earlier than the steps already done or the state was never initialised. */
+int pxlflip_advance_to_frame(pxlflip *h, double frame);
+
+/* Runs n more steps (the body of the step loop, fluid.rs:289-302). */
+int pxlflip_step(pxlflip *h, uint32_t n);
+
+/* Restores the initial particles and zero pressure (fluid.rs:271-275). */
+int pxlflip_reset(pxlflip *h);
+
+/* FluidState::particle_count (fluid.rs:230-232). */
+uint32_t pxlflip_particle_count(const pxlflip *h);
+
+/* Steps done so far (FluidState::steps_done, fluid.rs:85). */
+uint64_t pxlflip_steps_done(const pxlflip *h);
+
+/* FluidState::reservation (fluid.rs:104-106): vertices and indices for the surface mesh. */
+int pxlflip_reservation(const pxlflip *h, uint32_t *vertices, uint32_t *indices);
+
+/* Grid of the simulation: cells per axis, cell size, origin, dt and the counts of faces
+ * and cells (FluidPush fields, fluid.rs:158-180, and counts(), fluid.rs:238-245). */
+struct pxlflip_info {
+    uint32_t n[3];
+    float h, dt, flip, inv_h;
+    float origin[3];
+    float gravity[3];
+    uint32_t n_particles;
+    uint32_t substeps, pressure_iterations;
+    uint64_t n_faces, n_cells;
+    uint64_t device_bytes; /* HBM held by this handle */
+};
+int pxlflip_get_info(const pxlflip *h, pxlflip_info *info);
+
+/* Particle state in the reference record layout (8 floats per particle, seeding order).
+ * Read = the particle buffer after the last step; write = teacher forcing / checkpoint
+ * restore (the initial upload of fluid.rs:188/272). */
+int pxlflip_read_particles(pxlflip *h, float *aos8);
+int pxlflip_write_particles(pxlflip *h, const float *aos8);
+
+/* Grid fields, in the reference's buffer layout (fluid.slang:22-36): faces are the u block,
+ * then v, then w; cells are x-fastest. */
+#define PXLFLIP_FIELD_VEL 0        /* float[n_faces] */
+#define PXLFLIP_FIELD_VEL_OLD 1    /* float[n_faces] */
+#define PXLFLIP_FIELD_ACC 2        /* int32[n_faces] */
+#define PXLFLIP_FIELD_WACC 3       /* int32[n_faces] */
+#define PXLFLIP_FIELD_COUNT 4      /* uint32[n_cells] */
+#define PXLFLIP_FIELD_TYPE 5       /* uint32[n_cells]: 0 air, 1 fluid, 2 solid */
+#define PXLFLIP_FIELD_PRESSURE 6   /* float[n_cells] */
+#define PXLFLIP_FIELD_DIVERGENCE 7 /* float[n_cells] */
+#define PXLFLIP_FIELD_PRESSURE64 8 /* double[n_cells]: FP64 solution of the MIXED solver */
+int pxlflip_read_field(pxlflip *h, int field, void *out);
+/* Writable: VEL, VEL_OLD, PRESSURE (teacher forcing of the warm start). */
+int pxlflip_write_field(pxlflip *h, int field, const void *in);
+
+/* Counting-sort binning of the current particle positions: the per-cell particle count of
+ * p2g (fluid.slang:113-114) plus the sort the build adds.  cell_of[P] = cell of each
+ * particle (seeding order), counts[C], offsets[C] = exclusive scan of counts,
+ * sorted_ids[P] = particle ids grouped by cell, ascending within a cell.  Any pointer may
+ * be NULL. */
+int pxlflip_read_bins(pxlflip *h, uint32_t *cell_of, uint32_t *counts, uint32_t *offsets,
+                      uint32_t *sorted_ids);
+
+/* Runs only the grid phase of one step on the current particles -- bin, p2g,
+ * normalize/classify, forces, divergence (fluid.rs:290-294) -- leaving the fields readable
+ * for parity checks.  Does not advance steps_done. */
+int pxlflip_grid_phase(pxlflip *h);
+
+/* Statistics of the last pressure solve. */
+typedef struct pxlflip_solve_stats {
+    double rel_residual;  /* MIXED: final FP64 ||b - A p|| / ||b||; else -1 */
+    uint32_t inner_iterations;
+    uint32_t outer_iterations;
+} pxlflip_solve_stats;
+int pxlflip_last_solve(const pxlflip *h, pxlflip_solve_stats *st);
+
+/* Per-kernel timing: runs n more steps without graph replay, every launch bracketed by HIP
+ * events on the simulation's stream, and reports per kernel name the launch count and the
+ * summed event time, largest first (the reference only times whole frames, FrameStats in
+ * crates/pxl-render/src/lib.rs:77-91).  out may hold max_out entries; *n_out gets the
+ * number of distinct kernels. */
+typedef struct pxlflip_kernel_time {
+    char name[32];
+    uint32_t launches;
+    double total_ms;
+} pxlflip_kernel_time;
+int pxlflip_profile_steps(pxlflip *h, uint32_t n, pxlflip_kernel_time *out, uint32_t max_out, uint32_t *n_out);
+
+/* Last error message of a handle; with NULL, the calling thread's last create error. */
+const char *pxlflip_last_error(const pxlflip *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PXLFLIP_H */

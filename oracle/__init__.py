@@ -1,0 +1,267 @@
+"""CPU oracle of pxlengine's FLIP step -- TEST INFRASTRUCTURE ONLY.
+
+Loads ``oracle/liborc_flip.so`` (built from ``flip_oracle.c`` by ``oracle/Makefile``) and
+wraps it with numpy views.  Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s
+cpu_baseline leg may import this module, and only as the checker; the product package
+``pxlengine_amd`` never imports it.
+
+The restated reference semantics live in ``flip_oracle.c``; every function there cites
+``shaders/fluid.slang`` / ``crates/pxl-render/src/fluid.rs`` / ``crates/pxl-scene/src/lib.rs``
+line ranges.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liborc_flip.so")
+
+AIR, FLUID, SOLID = 0, 1, 2
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("nx", C.c_uint32), ("ny", C.c_uint32), ("nz", C.c_uint32), ("n_particles", C.c_uint32),
+        ("h", C.c_float), ("dt", C.c_float), ("flip", C.c_float), ("inv_h", C.c_float),
+        ("origin", C.c_float * 3), ("gravity", C.c_float * 3),
+    ]
+
+
+class _State(C.Structure):
+    _fields_ = [
+        ("p", Params), ("n_faces", C.c_uint64), ("n_cells", C.c_uint64),
+        ("px", C.POINTER(C.c_float)), ("py", C.POINTER(C.c_float)), ("pz", C.POINTER(C.c_float)),
+        ("vx", C.POINTER(C.c_float)), ("vy", C.POINTER(C.c_float)), ("vz", C.POINTER(C.c_float)),
+        ("vel", C.POINTER(C.c_float)), ("vel_old", C.POINTER(C.c_float)),
+        ("acc", C.POINTER(C.c_int32)), ("wacc", C.POINTER(C.c_int32)),
+        ("count", C.POINTER(C.c_uint32)), ("type", C.POINTER(C.c_uint32)),
+        ("pressure", C.POINTER(C.c_float)), ("div", C.POINTER(C.c_float)),
+    ]
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc, -ffp-contract=off)."""
+    cmd = ["make", "-s", "-C", _HERE] + (["-B"] if force else [])
+    subprocess.run(cmd, check=True)
+    return _LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        f32p, u32p, dp = C.POINTER(C.c_float), C.POINTER(C.c_uint32), C.POINTER(C.c_double)
+        SP = C.POINTER(_State)
+        L.orc_fluid_grid.argtypes = [f32p, f32p, C.c_uint32, u32p, f32p]
+        L.orc_lcg.argtypes = [C.POINTER(C.c_uint64)]
+        L.orc_lcg.restype = C.c_float
+        L.orc_seed.argtypes = [u32p, C.c_float, f32p, f32p, C.c_uint32, C.c_uint32, C.c_uint32, f32p, f32p]
+        L.orc_seed.restype = C.c_uint64
+        L.orc_steps_for_frame.argtypes = [C.c_double, C.c_double, C.c_double, C.c_uint32]
+        L.orc_steps_for_frame.restype = C.c_uint64
+        L.orc_dt.argtypes = [C.c_double, C.c_uint32]
+        L.orc_dt.restype = C.c_float
+        L.orc_create.argtypes = [C.POINTER(Params)]
+        L.orc_create.restype = SP
+        L.orc_destroy.argtypes = [SP]
+        for name in ("orc_clear_grid", "orc_p2g", "orc_normalize_grid", "orc_forces",
+                     "orc_divergence", "orc_project", "orc_g2p_advect"):
+            getattr(L, name).argtypes = [SP]
+        L.orc_pressure_rb.argtypes = [SP, C.c_uint32]
+        L.orc_step_rbgs.argtypes = [SP, C.c_uint32]
+        L.orc_step_cg.argtypes = [SP, C.c_uint32]
+        L.orc_pressure_cg32.argtypes = [SP, C.c_uint32]
+        L.orc_bin.argtypes = [SP, u32p, u32p, u32p, u32p]
+        L.orc_poisson_apply64.argtypes = [SP, dp, dp]
+        L.orc_poisson_rhs64.argtypes = [SP, dp]
+        L.orc_poisson_relres64.argtypes = [SP, f32p, dp]
+        L.orc_poisson_relres64.restype = C.c_double
+        L.orc_poisson_solve64.argtypes = [SP, dp, C.c_double, C.c_int, dp]
+        L.orc_poisson_solve64.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _up(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def fluid_grid(lo, hi, resolution):
+    """Fluid::grid (crates/pxl-scene/src/lib.rs:123-128) -> ((nx, ny, nz), h)."""
+    lo = np.asarray(lo, np.float32)
+    hi = np.asarray(hi, np.float32)
+    n = np.zeros(3, np.uint32)
+    h = C.c_float()
+    lib().orc_fluid_grid(_fp(lo), _fp(hi), resolution, _up(n), C.byref(h))
+    return tuple(int(v) for v in n), np.float32(h.value)
+
+
+def lcg_stream(state: int, count: int):
+    s = C.c_uint64(state)
+    out = np.empty(count, np.float32)
+    for i in range(count):
+        out[i] = lib().orc_lcg(C.byref(s))
+    return out, s.value
+
+
+def seed_particles(n, h, origin, blocks, ppc, seed, count_only=False):
+    """Seeding loop of FluidState::new (crates/pxl-render/src/fluid.rs:123-148).
+
+    blocks: list of (min3, max3, velocity3).  Returns (x[P,3], v[P,3]) float32, or the count.
+    """
+    nn = np.asarray(n, np.uint32)
+    org = np.asarray(origin, np.float32)
+    b = np.asarray([list(m) + list(M) + list(v) for (m, M, v) in blocks], np.float32).reshape(-1)
+    L = lib()
+    cnt = L.orc_seed(_up(nn), np.float32(h), _fp(org), _fp(b), len(blocks), ppc, seed, None, None)
+    if count_only:
+        return int(cnt)
+    x = np.zeros((max(cnt, 1), 3), np.float32)
+    v = np.zeros((max(cnt, 1), 3), np.float32)
+    L.orc_seed(_up(nn), np.float32(h), _fp(org), _fp(b), len(blocks), ppc, seed, _fp(x), _fp(v))
+    return x[:cnt], v[:cnt]
+
+
+def steps_for_frame(warmup, frame, fps, substeps):
+    return int(lib().orc_steps_for_frame(float(np.float32(warmup)), frame, fps, substeps))
+
+
+def dt_for(fps, substeps):
+    return np.float32(lib().orc_dt(fps, substeps))
+
+
+class Sim:
+    """One oracle simulation; arrays are numpy views of the C state (SoA particles)."""
+
+    def __init__(self, n, h, origin, dt, flip, gravity, x, v):
+        p = Params()
+        p.nx, p.ny, p.nz = (int(a) for a in n)
+        p.n_particles = len(x)
+        p.h = float(np.float32(h))
+        p.dt = float(np.float32(dt))
+        p.flip = float(np.float32(flip))
+        p.inv_h = float(np.float32(1.0) / np.float32(h))
+        for a in range(3):
+            p.origin[a] = float(np.float32(origin[a]))
+            p.gravity[a] = float(np.float32(gravity[a]))
+        self._L = lib()
+        self._s = self._L.orc_create(C.byref(p))
+        if not self._s:
+            raise MemoryError("oracle allocation failed")
+        st = self._s.contents
+        self.n = tuple(int(a) for a in n)
+        self.P = len(x)
+        self.F = int(st.n_faces)
+        self.C = int(st.n_cells)
+
+        def view(ptr, count, dtype):
+            return np.ctypeslib.as_array(ptr, shape=(max(count, 1),))[:count].view(dtype)
+
+        self.px, self.py, self.pz = (view(getattr(st, k), self.P, np.float32) for k in ("px", "py", "pz"))
+        self.vx, self.vy, self.vz = (view(getattr(st, k), self.P, np.float32) for k in ("vx", "vy", "vz"))
+        self.vel = view(st.vel, self.F, np.float32)
+        self.vel_old = view(st.vel_old, self.F, np.float32)
+        self.acc = view(st.acc, self.F, np.int32)
+        self.wacc = view(st.wacc, self.F, np.int32)
+        self.count = view(st.count, self.C, np.uint32)
+        self.type = view(st.type, self.C, np.uint32)
+        self.pressure = view(st.pressure, self.C, np.float32)
+        self.div = view(st.div, self.C, np.float32)
+        x = np.asarray(x, np.float32)
+        v = np.asarray(v, np.float32)
+        self.px[:], self.py[:], self.pz[:] = x[:, 0], x[:, 1], x[:, 2]
+        self.vx[:], self.vy[:], self.vz[:] = v[:, 0], v[:, 1], v[:, 2]
+        self.params = p
+
+    def __del__(self):
+        s = getattr(self, "_s", None)
+        if s:
+            self._L.orc_destroy(s)
+            self._s = None
+
+    # -- particle access
+    @property
+    def x(self):
+        return np.stack([self.px, self.py, self.pz], axis=1)
+
+    @property
+    def v(self):
+        return np.stack([self.vx, self.vy, self.vz], axis=1)
+
+    def set_particles(self, x, v):
+        x = np.asarray(x, np.float32)
+        v = np.asarray(v, np.float32)
+        self.px[:], self.py[:], self.pz[:] = x[:, 0], x[:, 1], x[:, 2]
+        self.vx[:], self.vy[:], self.vz[:] = v[:, 0], v[:, 1], v[:, 2]
+
+    # -- kernels
+    def clear_grid(self): self._L.orc_clear_grid(self._s)
+    def p2g(self): self._L.orc_p2g(self._s)
+    def normalize_grid(self): self._L.orc_normalize_grid(self._s)
+    def forces(self): self._L.orc_forces(self._s)
+    def divergence(self): self._L.orc_divergence(self._s)
+    def pressure_rb(self, parity): self._L.orc_pressure_rb(self._s, parity)
+    def project(self): self._L.orc_project(self._s)
+    def g2p_advect(self): self._L.orc_g2p_advect(self._s)
+    def step_rbgs(self, iters): self._L.orc_step_rbgs(self._s, iters)
+    def step_cg(self, iters): self._L.orc_step_cg(self._s, iters)
+    def pressure_cg32(self, iters): self._L.orc_pressure_cg32(self._s, iters)
+
+    def grid_phase(self):
+        """clear -> p2g -> normalize -> forces -> divergence (fluid.rs:290-294)."""
+        self.clear_grid()
+        self.p2g()
+        self.normalize_grid()
+        self.forces()
+        self.divergence()
+
+    def bin(self):
+        cell_of = np.zeros(max(self.P, 1), np.uint32)
+        counts = np.zeros(self.C, np.uint32)
+        offsets = np.zeros(self.C, np.uint32)
+        ids = np.zeros(max(self.P, 1), np.uint32)
+        self._L.orc_bin(self._s, _up(cell_of), _up(counts), _up(offsets), _up(ids))
+        return cell_of[:self.P], counts, offsets, ids[:self.P]
+
+    # -- FP64 Poisson oracle
+    def poisson_apply64(self, x):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros(self.C, np.float64)
+        self._L.orc_poisson_apply64(self._s, _dp(x), _dp(y))
+        return y
+
+    def poisson_rhs64(self):
+        b = np.zeros(self.C, np.float64)
+        self._L.orc_poisson_rhs64(self._s, _dp(b))
+        return b
+
+    def poisson_relres64(self, x):
+        x = np.asarray(x)
+        if x.dtype == np.float64:
+            x = np.ascontiguousarray(x)
+            return float(self._L.orc_poisson_relres64(self._s, None, _dp(x)))
+        x = np.ascontiguousarray(x, np.float32)
+        return float(self._L.orc_poisson_relres64(self._s, _fp(x), None))
+
+    def poisson_solve64(self, tol=1e-12, max_iter=100000):
+        x = np.zeros(self.C, np.float64)
+        rr = C.c_double()
+        it = self._L.orc_poisson_solve64(self._s, _dp(x), tol, max_iter, C.byref(rr))
+        return x, int(it), float(rr.value)

@@ -1,0 +1,98 @@
+/*
+ * flip_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C, single-threaded restatement of pxlengine's FLIP liquid step
+ * (shaders/fluid.slang:9-220, driven by crates/pxl-render/src/fluid.rs:96-306)
+ * plus an FP64 oracle of the discretised pressure Poisson system.
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
+ * this library, and only as the checker. The product (pxlengine_amd) never links,
+ * loads or calls it.
+ *
+ * Arithmetic rules (so that every result is a deterministic function of the inputs):
+ *  - compiled with -ffp-contract=off, no fast-math: every float op rounds once,
+ *    in the order the reference source writes it;
+ *  - Slang lerp(a, b, t) is SPIR-V FMix = a*(1-t) + b*t; with t a constant 0/1 (P2G / G2P
+ *    trilinear weights) it is exactly a or b;
+ *  - Slang max(v, 0) / min(v, 0) are SPIR-V FMax/FMin: "y if x < y else x" / "y if y < x else x".
+ */
+#ifndef PXL_FLIP_ORACLE_H
+#define PXL_FLIP_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_AIR = 0, ORC_FLUID = 1, ORC_SOLID = 2 };
+
+/* Mirror of the simulation fields of FluidPush (fluid.slang:11-19 / fluid.rs:45-72). */
+typedef struct orc_params {
+    uint32_t nx, ny, nz, n_particles;
+    float h, dt, flip, inv_h;
+    float origin[3];
+    float gravity[3];
+} orc_params;
+
+/* One simulation's state, SoA particles, reference face/cell layout. */
+typedef struct orc_state {
+    orc_params p;
+    uint64_t n_faces, n_cells;
+    float *px, *py, *pz, *vx, *vy, *vz;   /* particles */
+    float *vel, *vel_old;                 /* faces: u block, v block, w block */
+    int32_t *acc, *wacc;                  /* faces: fixed-point P2G sums */
+    uint32_t *count;                      /* cells: particles per cell */
+    uint32_t *type;                       /* cells: 0 air, 1 fluid, 2 solid */
+    float *pressure, *div;                /* cells */
+} orc_state;
+
+/* ---- scene block / seeding (crates/pxl-scene/src/lib.rs:121-129, fluid.rs:96-180) ---- */
+void orc_fluid_grid(const float lo[3], const float hi[3], uint32_t resolution,
+                    uint32_t n_out[3], float *h_out);
+float orc_lcg(uint64_t *s);
+/* blocks: nb rows of 9 floats (min xyz, max xyz, velocity xyz).  When x is NULL only counts.
+ * x is P*3 positions, v is P*3 velocities (interleaved xyz).  Returns the particle count. */
+uint64_t orc_seed(const uint32_t n[3], float h, const float origin[3], const float *blocks,
+                  uint32_t nb, uint32_t ppc, uint32_t seed, float *x, float *v);
+uint64_t orc_steps_for_frame(double warmup, double frame, double fps, uint32_t substeps);
+float orc_dt(double fps, uint32_t substeps);
+
+/* ---- state ---- */
+orc_state *orc_create(const orc_params *p);
+void orc_destroy(orc_state *s);
+
+/* ---- the step kernels, one function per fluid.slang entry point ---- */
+void orc_clear_grid(orc_state *s);                 /* fluid.slang:88-91  */
+void orc_p2g(orc_state *s);                        /* fluid.slang:94-115 */
+void orc_normalize_grid(orc_state *s);             /* fluid.slang:118-132 */
+void orc_forces(orc_state *s);                     /* fluid.slang:143-151 */
+void orc_divergence(orc_state *s);                 /* fluid.slang:154-165 */
+void orc_pressure_rb(orc_state *s, uint32_t parity); /* fluid.slang:168-182 */
+void orc_project(orc_state *s);                    /* fluid.slang:185-197 */
+void orc_g2p_advect(orc_state *s);                 /* fluid.slang:200-220 */
+/* The reference step (fluid.rs:289-302) with `iters` red-black iterations. */
+void orc_step_rbgs(orc_state *s, uint32_t iters);
+/* The same step with the pressure solve replaced by FP32 Jacobi-PCG, fixed `iters`
+ * iterations, warm-started from the pressure field (fp32 dots, sequential order). */
+void orc_step_cg(orc_state *s, uint32_t iters);
+/* Pressure phase only, FP32 PCG as in orc_step_cg (divergence/type already computed). */
+void orc_pressure_cg32(orc_state *s, uint32_t iters);
+
+/* ---- binning: canonical counting sort (ids ascending inside each cell) ---- */
+void orc_bin(const orc_state *s, uint32_t *cell_of, uint32_t *counts, uint32_t *offsets,
+             uint32_t *sorted_ids);
+
+/* ---- FP64 Poisson oracle of the system the pressure phase solves ----
+ * Unknowns: fluid cells.  (A p)_i = cnt_i p_i - sum_{fluid nbrs j} p_j with cnt_i the number
+ * of non-solid neighbours (fluid.slang:172-181); b_i = -div_i h^2 / dt. */
+void orc_poisson_apply64(const orc_state *s, const double *x, double *y);
+void orc_poisson_rhs64(const orc_state *s, double *b);
+/* ||b - A x|| / ||b|| in FP64 for an fp32 (x32) or fp64 (x64) solution; one of them non-NULL. */
+double orc_poisson_relres64(const orc_state *s, const float *x32, const double *x64);
+/* Jacobi-PCG in FP64 from x = 0 until ||r||/||b|| <= tol or max_iter. Returns iterations. */
+int orc_poisson_solve64(const orc_state *s, double *x, double tol, int max_iter, double *relres);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

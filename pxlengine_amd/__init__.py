@@ -1,0 +1,10 @@
+"""MI355X-native FLIP liquid step for pxlengine scenes (HIP on gfx950).
+
+The compute lives in ``libpxlflip.so`` (sources in ``csrc/``, C ABI in
+``include/pxlflip.h``); this package is its Python binding.
+"""
+from .flip import (FIELDS, SOLVERS, FluidError, FluidState, check, load_library,  # noqa: F401
+                   make_opts, seed_particles)
+
+__all__ = ["FluidState", "FluidError", "check", "seed_particles", "load_library", "make_opts",
+           "FIELDS", "SOLVERS"]

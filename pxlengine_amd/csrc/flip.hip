@@ -1,0 +1,766 @@
+// Host runtime of the FLIP path and its C ABI (include/pxlflip.h).
+//
+// One Sim = one liquid: its particles, grid and solver state live in one HBM arena for the
+// lifetime of the handle.  A step is the reference's dispatch sequence (fluid.rs:289-302)
+// re-cut into fused kernels:
+//   clear -> bin count -> scan -> scatter (counting sort) -> p2g -> faces (normalize+forces)
+//   -> cells (classify+stencil+divergence) -> pressure -> project -> g2p
+// and, for the RBGS and CG solvers, captured once into a hipGraph that each step replays.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/pxlflip.h"
+#include "kernels.hpp"
+#include "spec.hpp"
+
+namespace pxl {
+namespace {
+
+thread_local std::string g_create_error;
+
+struct HipError {
+    hipError_t code;
+    const char *what;
+};
+
+#define HIP_CHECK(expr)                                                   \
+    do {                                                                  \
+        hipError_t e_ = (expr);                                           \
+        if (e_ != hipSuccess) throw ::pxl::HipError{e_, #expr};           \
+    } while (0)
+
+constexpr size_t ALIGN = 256;
+
+// Optional HIP-event bracketing of every launch (pxlflip_profile_steps).  Off while a graph
+// is captured or replayed.
+struct KernelTimer {
+    struct Rec {
+        const char *name;
+        hipEvent_t a, b;
+    };
+    bool on = false;
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+
+    hipEvent_t next() {
+        if (used == pool.size()) {
+            hipEvent_t e;
+            HIP_CHECK(hipEventCreate(&e));
+            pool.push_back(e);
+        }
+        return pool[used++];
+    }
+    template <typename F>
+    void run(const char *name, hipStream_t s, F &&launch) {
+        if (!on) {
+            launch();
+            return;
+        }
+        hipEvent_t a = next(), b = next();
+        HIP_CHECK(hipEventRecord(a, s));
+        launch();
+        HIP_CHECK(hipEventRecord(b, s));
+        recs.push_back({name, a, b});
+    }
+    void clear() {
+        recs.clear();
+        used = 0;
+    }
+    ~KernelTimer() {
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+#define KRUN(name, call) timer.run(name, stream, [&] { call; })
+
+struct Sim {
+    FluidSpec spec;
+    pxlflip_opts opts{};
+    double fps = 24.0;
+    uint32_t seed = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DevSim d{};
+    void *arena = nullptr;
+    size_t arena_bytes = 0;
+    uint32_t iters = 80;          // RBGS sweeps or CG iterations per step
+    uint64_t steps_done = 0;
+    bool initialized = false;
+    hipGraphExec_t step_graph = nullptr;
+    hipGraphExec_t inner_graph[2] = {nullptr, nullptr};
+    double *host_scal = nullptr;  // pinned
+    int x64_cur = 0;
+    pxlflip_solve_stats stats{-1.0, 0, 0};
+    std::string error;
+    KernelTimer timer;
+
+    ~Sim() {
+        if (step_graph) (void)hipGraphExecDestroy(step_graph);
+        for (auto &g : inner_graph)
+            if (g) (void)hipGraphExecDestroy(g);
+        if (host_scal) (void)hipHostFree(host_scal);
+        if (arena) (void)hipFree(arena);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    // ---- allocation
+    using Plan = std::vector<std::pair<void **, size_t>>;
+    // Every device array of the simulation, in arena order.
+    Plan plan() {
+        const size_t P = d.np, F = d.nfaces, C = d.ncells;
+        const size_t nb = d.nblocks_cells;
+        Plan plan;
+        auto add = [&](auto **p, size_t bytes) { plan.emplace_back(reinterpret_cast<void **>(p), bytes); };
+        for (float **p : {&d.ax, &d.ay, &d.az, &d.avx, &d.avy, &d.avz}) add(p, 4 * P);
+        add(&d.aid, 4 * P);
+        for (float **p : {&d.bx, &d.by, &d.bz, &d.bvx, &d.bvy, &d.bvz}) add(p, 4 * P);
+        add(&d.bid, 4 * P);
+        add(&d.key, 4 * P);
+        add(&d.slot, 4 * P);
+        for (float **p : {&d.ix, &d.iy, &d.iz, &d.ivx, &d.ivy, &d.ivz}) add(p, 4 * P);
+        add(&d.vel, 4 * F);
+        add(&d.vel_old, 4 * F);
+        add(&d.acc, 4 * F);
+        add(&d.wacc, 4 * F);
+        add(&d.count, 4 * C);
+        add(&d.offset, 4 * C);
+        add(&d.type, C);
+        add(&d.sten, 2 * C);
+        add(&d.pressure, 4 * C);
+        add(&d.div, 4 * C);
+        for (float **p : {&d.r, &d.d0, &d.d1, &d.q}) add(p, 4 * C);
+        if (opts.solver == PXLFLIP_SOLVER_MIXED) {
+            add(&d.e, 4 * C);
+            add(&d.x64a, 8 * C);
+            add(&d.x64b, 8 * C);
+            add(&d.r64, 8 * C);
+        }
+        add(&d.part32, 4 * 2 * nb);
+        add(&d.part64, 8 * 2 * nb);
+        add(&d.scal, 8 * SC_COUNT);
+        add(&d.ctr, 64);
+        add(&d.scan_tmp, 4 * ((C + 4095) / 4096 + 1));
+        return plan;
+    }
+    static size_t plan_total(const Plan &plan) {
+        size_t total = 0;
+        for (auto &e : plan) total += (e.second + ALIGN - 1) / ALIGN * ALIGN;
+        return total;
+    }
+
+    void allocate(const std::vector<float> &x0, const std::vector<float> &v0) {
+        const size_t P = d.np;
+        Plan plan = this->plan();
+        const size_t total = plan_total(plan);
+        void *base = nullptr;
+        hipError_t err = hipMalloc(&base, total);
+        if (err != hipSuccess) throw std::bad_alloc();
+        arena = base;
+        arena_bytes = total;
+        char *cur = static_cast<char *>(base);
+        for (auto &e : plan) {
+            *e.first = cur;
+            cur += (e.second + ALIGN - 1) / ALIGN * ALIGN;
+        }
+        HIP_CHECK(hipMemsetAsync(arena, 0, arena_bytes, stream));
+        // initial particles (reset source)
+        float *dst[6] = {d.ix, d.iy, d.iz, d.ivx, d.ivy, d.ivz};
+        for (int a = 0; a < 3; a++) {
+            HIP_CHECK(hipMemcpyAsync(dst[a], x0.data() + a * P, 4 * P, hipMemcpyHostToDevice, stream));
+            HIP_CHECK(hipMemcpyAsync(dst[3 + a], v0.data() + a * P, 4 * P, hipMemcpyHostToDevice, stream));
+        }
+        HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&host_scal), 8 * SC_COUNT, hipHostMallocDefault));
+        HIP_CHECK(hipStreamSynchronize(stream));
+    }
+
+    // ---- step pieces
+    void enqueue_grid_phase() {
+        KRUN("clear_grid", launch_clear_grid(d, stream));
+        KRUN("bin_count", launch_bin_count(d, stream));
+        KRUN("scan_offsets", launch_scan_offsets(d, stream));
+        KRUN("scatter", launch_scatter(d, stream));
+        KRUN("p2g", launch_p2g(d, stream));
+        KRUN("faces", launch_faces(d, stream));
+        KRUN("cells", launch_cells(d, stream));
+    }
+    void enqueue_fixed_pressure() {
+        if (opts.solver == PXLFLIP_SOLVER_RBGS) {
+            for (uint32_t it = 0; it < iters; it++) {
+                KRUN("rbgs", launch_rbgs(d, 0, stream));
+                KRUN("rbgs", launch_rbgs(d, 1, stream));
+            }
+        } else {
+            KRUN("cg_init", launch_cg_init32(d, stream));
+            for (uint32_t it = 0; it < iters; it++) {
+                KRUN("cg_dir_spmv", launch_cg_pass_a(d, false, it, stream));
+                KRUN("cg_update", launch_cg_pass_b(d, false, d.pressure, it, stream));
+            }
+        }
+    }
+    void enqueue_post() {
+        KRUN("project", launch_project(d, stream));
+        KRUN("g2p", launch_g2p(d, stream));
+    }
+    void enqueue_fixed_step() {
+        enqueue_grid_phase();
+        enqueue_fixed_pressure();
+        enqueue_post();
+    }
+    void enqueue_inner(int cur) {
+        KRUN("inner_init", launch_inner_init(d, stream));
+        for (uint32_t it = 0; it < opts.inner_iterations; it++) {
+            KRUN("cg64_dir_spmv", launch_cg_pass_a(d, true, it, stream));
+            KRUN("cg64_update", launch_cg_pass_b(d, true, d.e, it, stream));
+        }
+        KRUN("mixed_update", launch_mixed_update(d, x64(cur), x64(cur ^ 1), stream));
+    }
+    void enqueue_inner0() { enqueue_inner(0); }
+    void enqueue_inner1() { enqueue_inner(1); }
+
+    hipGraphExec_t capture(void (Sim::*fn)()) {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ex = nullptr;
+        HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        (this->*fn)();
+        HIP_CHECK(hipStreamEndCapture(stream, &g));
+        hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIP_CHECK(e);
+        return ex;
+    }
+
+    double *x64(int k) const { return k == 0 ? d.x64a : d.x64b; }
+
+    void read_scalars() {
+        HIP_CHECK(hipMemcpyAsync(host_scal, d.scal, 8 * SC_COUNT, hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+    }
+
+    // Mixed-precision pressure solve: FP64 refinement driven from the host, one residual
+    // read-back per refinement pass.
+    void mixed_pressure() {
+        x64_cur = 0;
+        KRUN("mixed_init", launch_mixed_init(d, x64(0), stream));
+        read_scalars();
+        const double bb = host_scal[SC_BB];
+        double rr = host_scal[SC_RR];
+        auto rel = [&](double r) { return bb > 0.0 ? std::sqrt(r / bb) : std::sqrt(r); };
+        uint32_t outer = 0;
+        while (rel(rr) > opts.tolerance && outer < opts.max_outer) {
+            if (opts.use_graph && !timer.on) {
+                if (!inner_graph[x64_cur])
+                    inner_graph[x64_cur] = capture(x64_cur == 0 ? &Sim::enqueue_inner0 : &Sim::enqueue_inner1);
+                HIP_CHECK(hipGraphLaunch(inner_graph[x64_cur], stream));
+            } else {
+                (this->*(x64_cur == 0 ? &Sim::enqueue_inner0 : &Sim::enqueue_inner1))();
+            }
+            x64_cur ^= 1;
+            read_scalars();
+            rr = host_scal[SC_RR];
+            outer++;
+        }
+        KRUN("mixed_finish", launch_mixed_finish(d, x64(x64_cur), stream));
+        stats.rel_residual = rel(rr);
+        stats.outer_iterations = outer;
+        stats.inner_iterations = outer * opts.inner_iterations;
+    }
+
+    void run_steps(uint32_t n) {
+        if (n == 0) return;
+        if (opts.solver == PXLFLIP_SOLVER_MIXED) {
+            for (uint32_t s = 0; s < n; s++) {
+                enqueue_grid_phase();
+                mixed_pressure();
+                enqueue_post();
+                steps_done++;
+            }
+        } else {
+            const bool graph = opts.use_graph && !timer.on;
+            if (graph && !step_graph) step_graph = capture(&Sim::enqueue_fixed_step);
+            for (uint32_t s = 0; s < n; s++) {
+                if (graph) HIP_CHECK(hipGraphLaunch(step_graph, stream));
+                else enqueue_fixed_step();
+            }
+            steps_done += n;
+            stats.rel_residual = -1.0;
+            stats.inner_iterations = iters;
+            stats.outer_iterations = 0;
+        }
+        HIP_CHECK(hipStreamSynchronize(stream));
+    }
+
+    void reset() {
+        launch_reset_particles(d, stream);
+        HIP_CHECK(hipMemsetAsync(d.pressure, 0, 4 * (size_t)d.ncells, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        steps_done = 0;
+        initialized = true;
+    }
+};
+
+int fail(Sim *s, int code, const std::string &msg) {
+    if (s) s->error = msg;
+    return code;
+}
+
+int device_fail(Sim *s, const HipError &e) {
+    return fail(s, PXLFLIP_ERR_DEVICE, std::string("HIP error in ") + e.what + ": " + hipGetErrorString(e.code));
+}
+
+// Runs body() with the handle's device current; maps exceptions to status codes.
+template <typename F>
+int guarded(Sim *s, F &&body) {
+    if (!s) return PXLFLIP_ERR_INVALID;
+    try {
+        HIP_CHECK(hipSetDevice(s->device));
+        body();
+        return PXLFLIP_OK;
+    } catch (const HipError &e) {
+        return device_fail(s, e);
+    } catch (const std::bad_alloc &) {
+        return fail(s, PXLFLIP_ERR_OOM, "out of memory");
+    } catch (const std::exception &e) {
+        return fail(s, PXLFLIP_ERR_INTERNAL, e.what());
+    }
+}
+
+struct DeviceBuffer {
+    void *p = nullptr;
+    explicit DeviceBuffer(size_t bytes) {
+        if (hipMalloc(&p, bytes ? bytes : 4) != hipSuccess) throw std::bad_alloc();
+    }
+    ~DeviceBuffer() {
+        if (p) (void)hipFree(p);
+    }
+    template <typename T>
+    T *as() const { return static_cast<T *>(p); }
+};
+
+}  // namespace
+}  // namespace pxl
+
+using pxl::Sim;
+
+struct pxlflip {
+    Sim sim;
+};
+
+extern "C" {
+
+int pxlflip_default_opts(pxlflip_opts *o) {
+    if (!o) return PXLFLIP_ERR_INVALID;
+    std::memset(o, 0, sizeof *o);
+    o->struct_size = sizeof(pxlflip_opts);
+    o->solver = PXLFLIP_SOLVER_RBGS;
+    o->cg_iterations = 0;
+    o->inner_iterations = 40;
+    o->max_outer = 50;
+    o->tolerance = 1e-6;
+    o->device = 0;
+    o->scene_index = 0;
+    o->use_graph = 1;
+    return PXLFLIP_OK;
+}
+
+}  // extern "C"
+
+namespace pxl {
+namespace {
+
+// Parse + validate + seed + derive the device parameters; no device work.  Returns 0 or an
+// error code with g_create_error set.
+int prepare(const char *fluid_json, double fps, uint32_t seed, const pxlflip_opts *opts, Sim &s,
+            std::vector<float> *x0, std::vector<float> *v0) {
+    g_create_error.clear();
+    if (!fluid_json) {
+        g_create_error = "fluid_json is NULL";
+        return PXLFLIP_ERR_INVALID;
+    }
+    pxlflip_opts o;
+    pxlflip_default_opts(&o);
+    if (opts) {
+        if (opts->struct_size != sizeof(pxlflip_opts)) {
+            g_create_error = "pxlflip_opts.struct_size mismatch (use pxlflip_default_opts)";
+            return PXLFLIP_ERR_INVALID;
+        }
+        o = *opts;
+    }
+    if (o.solver < PXLFLIP_SOLVER_RBGS || o.solver > PXLFLIP_SOLVER_MIXED) {
+        g_create_error = "unknown solver " + std::to_string(o.solver);
+        return PXLFLIP_ERR_INVALID;
+    }
+    if (o.solver == PXLFLIP_SOLVER_MIXED && (o.inner_iterations == 0 || !(o.tolerance > 0.0))) {
+        g_create_error = "mixed solver needs inner_iterations > 0 and tolerance > 0";
+        return PXLFLIP_ERR_INVALID;
+    }
+    if (!(fps > 0.0)) {  // Scene::validate (crates/pxl-scene/src/lib.rs:670-672)
+        g_create_error = "render.fps must be > 0";
+        return PXLFLIP_ERR_INVALID;
+    }
+    FluidSpec spec;
+    std::string err = parse_fluid(fluid_json, spec);
+    if (err.empty()) err = validate_fluid(spec, o.scene_index);
+    if (!err.empty()) {
+        g_create_error = err;
+        return PXLFLIP_ERR_INVALID;
+    }
+    uint64_t np = seed_particles(spec, seed, x0, v0);
+    if (np == 0) {  // fluid.rs:149
+        g_create_error = "fluid '" + spec.name + "': fill blocks contain no grid cells";
+        return PXLFLIP_ERR_INVALID;
+    }
+    if (np > 0xFFFFFFFFull) {
+        g_create_error = "fluid '" + spec.name + "': too many particles";
+        return PXLFLIP_ERR_INVALID;
+    }
+    s.spec = spec;
+    s.opts = o;
+    s.fps = fps;
+    s.seed = seed;
+    s.device = o.device;
+    s.iters = (o.solver == PXLFLIP_SOLVER_CG && o.cg_iterations) ? o.cg_iterations : spec.pressure_iterations;
+    uint32_t n[3];
+    float hcell;
+    fluid_grid(spec, n, &hcell);
+    DevSim &d = s.d;
+    d.nx = (int)n[0];
+    d.ny = (int)n[1];
+    d.nz = (int)n[2];
+    d.np = (uint32_t)np;
+    d.h = hcell;
+    d.dt = fluid_dt(fps, spec.substeps);
+    d.flip = spec.flip;
+    d.inv_h = 1.0f / hcell;
+    d.ox = spec.bounds[0][0];
+    d.oy = spec.bounds[0][1];
+    d.oz = spec.bounds[0][2];
+    d.gx = spec.gravity[0];
+    d.gy = spec.gravity[1];
+    d.gz = spec.gravity[2];
+    d.nu = (n[0] + 1) * n[1] * n[2];
+    d.nv = n[0] * (n[1] + 1) * n[2];
+    d.nw = n[0] * n[1] * (n[2] + 1);
+    d.nfaces = d.nu + d.nv + d.nw;
+    d.ncells = n[0] * n[1] * n[2];
+    d.nblocks_cells = ((n[0] + 63) / 64) * ((n[1] + 3) / 4) * n[2];
+    return PXLFLIP_OK;
+}
+
+void fill_info(const Sim &s, pxlflip_info *info) {
+    const DevSim &d = s.d;
+    info->n[0] = (uint32_t)d.nx;
+    info->n[1] = (uint32_t)d.ny;
+    info->n[2] = (uint32_t)d.nz;
+    info->h = d.h;
+    info->dt = d.dt;
+    info->flip = d.flip;
+    info->inv_h = d.inv_h;
+    info->origin[0] = d.ox;
+    info->origin[1] = d.oy;
+    info->origin[2] = d.oz;
+    info->gravity[0] = d.gx;
+    info->gravity[1] = d.gy;
+    info->gravity[2] = d.gz;
+    info->n_particles = d.np;
+    info->substeps = s.spec.substeps;
+    info->pressure_iterations = s.iters;
+    info->n_faces = d.nfaces;
+    info->n_cells = d.ncells;
+    info->device_bytes = s.arena_bytes;
+}
+
+}  // namespace
+}  // namespace pxl
+
+extern "C" {
+
+int pxlflip_check_json(const char *fluid_json, double fps, uint32_t seed, const pxlflip_opts *opts,
+                       pxlflip_info *info) {
+    pxl::Sim s;
+    int rc = pxl::prepare(fluid_json, fps, seed, opts, s, nullptr, nullptr);
+    if (rc != PXLFLIP_OK) return rc;
+    s.arena_bytes = pxl::Sim::plan_total(s.plan());
+    if (info) pxl::fill_info(s, info);
+    return PXLFLIP_OK;
+}
+
+int pxlflip_seed_particles(const char *fluid_json, uint32_t seed, float *aos8, uint64_t max_particles,
+                           uint64_t *count) {
+    pxl::Sim s;
+    std::vector<float> x0, v0;
+    int rc = pxl::prepare(fluid_json, 24.0, seed, nullptr, s, aos8 ? &x0 : nullptr, aos8 ? &v0 : nullptr);
+    if (rc != PXLFLIP_OK) return rc;
+    const uint64_t P = s.d.np;
+    if (count) *count = P;
+    if (aos8) {
+        if (max_particles < P) {
+            pxl::g_create_error = "aos8 holds " + std::to_string(max_particles) + " particles, need " + std::to_string(P);
+            return PXLFLIP_ERR_INVALID;
+        }
+        for (uint64_t i = 0; i < P; i++) {
+            float *r = aos8 + 8 * i;
+            r[0] = x0[i]; r[1] = x0[P + i]; r[2] = x0[2 * P + i]; r[3] = 0.0f;
+            r[4] = v0[i]; r[5] = v0[P + i]; r[6] = v0[2 * P + i]; r[7] = 0.0f;
+        }
+    }
+    return PXLFLIP_OK;
+}
+
+int pxlflip_create_from_json(const char *fluid_json, double fps, uint32_t seed, const pxlflip_opts *opts,
+                             pxlflip **out) {
+    using namespace pxl;
+    if (!out) {
+        g_create_error = "out is NULL";
+        return PXLFLIP_ERR_INVALID;
+    }
+    *out = nullptr;
+    pxlflip *h = new (std::nothrow) pxlflip();
+    if (!h) {
+        g_create_error = "out of memory";
+        return PXLFLIP_ERR_OOM;
+    }
+    Sim &s = h->sim;
+    std::vector<float> x0, v0;
+    int rc = prepare(fluid_json, fps, seed, opts, s, &x0, &v0);
+    if (rc != PXLFLIP_OK) {
+        delete h;
+        return rc;
+    }
+    try {
+        HIP_CHECK(hipSetDevice(s.device));
+        HIP_CHECK(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+        s.allocate(x0, v0);
+        s.reset();
+        s.initialized = false;  // first advance re-seeds, as fluid.rs:271
+    } catch (const HipError &e) {
+        g_create_error = std::string("HIP error in ") + e.what + ": " + hipGetErrorString(e.code);
+        delete h;
+        return e.code == hipErrorOutOfMemory ? PXLFLIP_ERR_OOM : PXLFLIP_ERR_DEVICE;
+    } catch (const std::bad_alloc &) {
+        g_create_error = "fluid '" + s.spec.name + "': out of device memory";
+        delete h;
+        return PXLFLIP_ERR_OOM;
+    }
+    *out = h;
+    return PXLFLIP_OK;
+}
+
+void pxlflip_destroy(pxlflip *h) {
+    if (!h) return;
+    (void)hipSetDevice(h->sim.device);
+    delete h;
+}
+
+int pxlflip_advance_to_frame(pxlflip *h, double frame) {
+    if (!h) return PXLFLIP_ERR_INVALID;
+    Sim *s = &h->sim;
+    return pxl::guarded(s, [&] {
+        uint64_t target = pxl::steps_for_frame((double)s->spec.warmup, frame, s->fps, s->spec.substeps);
+        if (!s->initialized || target < s->steps_done) s->reset();
+        while (s->steps_done < target) {
+            uint64_t n = std::min<uint64_t>(target - s->steps_done, 1u << 20);
+            s->run_steps((uint32_t)n);
+        }
+    });
+}
+
+int pxlflip_step(pxlflip *h, uint32_t n) {
+    if (!h) return PXLFLIP_ERR_INVALID;
+    Sim *s = &h->sim;
+    return pxl::guarded(s, [&] {
+        if (!s->initialized) s->reset();
+        s->run_steps(n);
+    });
+}
+
+int pxlflip_reset(pxlflip *h) {
+    if (!h) return PXLFLIP_ERR_INVALID;
+    Sim *s = &h->sim;
+    return pxl::guarded(s, [&] { s->reset(); });
+}
+
+uint32_t pxlflip_particle_count(const pxlflip *h) { return h ? h->sim.d.np : 0; }
+
+uint64_t pxlflip_steps_done(const pxlflip *h) { return h ? h->sim.steps_done : 0; }
+
+int pxlflip_reservation(const pxlflip *h, uint32_t *vertices, uint32_t *indices) {
+    if (!h) return PXLFLIP_ERR_INVALID;
+    if (vertices) *vertices = h->sim.spec.max_triangles;
+    if (indices) *indices = h->sim.spec.max_triangles * 3u;
+    return PXLFLIP_OK;
+}
+
+int pxlflip_get_info(const pxlflip *h, pxlflip_info *info) {
+    if (!h || !info) return PXLFLIP_ERR_INVALID;
+    pxl::fill_info(h->sim, info);
+    return PXLFLIP_OK;
+}
+
+int pxlflip_read_particles(pxlflip *h, float *aos8) {
+    if (!h || !aos8) return PXLFLIP_ERR_INVALID;
+    Sim *s = &h->sim;
+    return pxl::guarded(s, [&] {
+        if (!s->initialized) s->reset();
+        pxl::DeviceBuffer buf(32ull * s->d.np);
+        pxl::launch_unpack_aos8(s->d, buf.as<float>(), s->stream);
+        HIP_CHECK(hipMemcpyAsync(aos8, buf.p, 32ull * s->d.np, hipMemcpyDeviceToHost, s->stream));
+        HIP_CHECK(hipStreamSynchronize(s->stream));
+    });
+}
+
+int pxlflip_write_particles(pxlflip *h, const float *aos8) {
+    if (!h || !aos8) return PXLFLIP_ERR_INVALID;
+    Sim *s = &h->sim;
+    return pxl::guarded(s, [&] {
+        pxl::DeviceBuffer buf(32ull * s->d.np);
+        HIP_CHECK(hipMemcpyAsync(buf.p, aos8, 32ull * s->d.np, hipMemcpyHostToDevice, s->stream));
+        pxl::launch_pack_aos8(s->d, buf.as<float>(), s->stream);
+        HIP_CHECK(hipStreamSynchronize(s->stream));
+        s->initialized = true;
+    });
+}
+
+int pxlflip_read_field(pxlflip *h, int field, void *out) {
+    if (!h || !out) return PXLFLIP_ERR_INVALID;
+    Sim *s = &h->sim;
+    const pxl::DevSim &d = s->d;
+    const void *src = nullptr;
+    size_t bytes = 0;
+    switch (field) {
+        case PXLFLIP_FIELD_VEL: src = d.vel; bytes = 4ull * d.nfaces; break;
+        case PXLFLIP_FIELD_VEL_OLD: src = d.vel_old; bytes = 4ull * d.nfaces; break;
+        case PXLFLIP_FIELD_ACC: src = d.acc; bytes = 4ull * d.nfaces; break;
+        case PXLFLIP_FIELD_WACC: src = d.wacc; bytes = 4ull * d.nfaces; break;
+        case PXLFLIP_FIELD_COUNT: src = d.count; bytes = 4ull * d.ncells; break;
+        case PXLFLIP_FIELD_TYPE: bytes = 4ull * d.ncells; break;
+        case PXLFLIP_FIELD_PRESSURE: src = d.pressure; bytes = 4ull * d.ncells; break;
+        case PXLFLIP_FIELD_DIVERGENCE: src = d.div; bytes = 4ull * d.ncells; break;
+        case PXLFLIP_FIELD_PRESSURE64:
+            if (s->opts.solver != PXLFLIP_SOLVER_MIXED)
+                return pxl::fail(s, PXLFLIP_ERR_INVALID, "PRESSURE64 exists only with the MIXED solver");
+            bytes = 8ull * d.ncells;
+            break;
+        default: return pxl::fail(s, PXLFLIP_ERR_INVALID, "unknown field " + std::to_string(field));
+    }
+    return pxl::guarded(s, [&] {
+        if (field == PXLFLIP_FIELD_TYPE) {
+            pxl::DeviceBuffer buf(bytes);
+            pxl::launch_type_u32(d, buf.as<uint32_t>(), s->stream);
+            HIP_CHECK(hipMemcpyAsync(out, buf.p, bytes, hipMemcpyDeviceToHost, s->stream));
+        } else {
+            if (field == PXLFLIP_FIELD_PRESSURE64) src = s->x64(s->x64_cur);
+            HIP_CHECK(hipMemcpyAsync(out, src, bytes, hipMemcpyDeviceToHost, s->stream));
+        }
+        HIP_CHECK(hipStreamSynchronize(s->stream));
+    });
+}
+
+int pxlflip_write_field(pxlflip *h, int field, const void *in) {
+    if (!h || !in) return PXLFLIP_ERR_INVALID;
+    Sim *s = &h->sim;
+    const pxl::DevSim &d = s->d;
+    void *dst = nullptr;
+    size_t bytes = 0;
+    switch (field) {
+        case PXLFLIP_FIELD_VEL: dst = d.vel; bytes = 4ull * d.nfaces; break;
+        case PXLFLIP_FIELD_VEL_OLD: dst = d.vel_old; bytes = 4ull * d.nfaces; break;
+        case PXLFLIP_FIELD_PRESSURE: dst = d.pressure; bytes = 4ull * d.ncells; break;
+        default: return pxl::fail(s, PXLFLIP_ERR_INVALID, "field " + std::to_string(field) + " is not writable");
+    }
+    return pxl::guarded(s, [&] {
+        HIP_CHECK(hipMemcpyAsync(dst, in, bytes, hipMemcpyHostToDevice, s->stream));
+        HIP_CHECK(hipStreamSynchronize(s->stream));
+    });
+}
+
+int pxlflip_read_bins(pxlflip *h, uint32_t *cell_of, uint32_t *counts, uint32_t *offsets, uint32_t *sorted_ids) {
+    if (!h) return PXLFLIP_ERR_INVALID;
+    Sim *s = &h->sim;
+    return pxl::guarded(s, [&] {
+        if (!s->initialized) s->reset();
+        const pxl::DevSim &d = s->d;
+        pxl::DeviceBuffer cbuf(4ull * d.np), sbuf(4ull * d.np);
+        pxl::launch_fill_u32(d.count, d.ncells, 0, s->stream);
+        pxl::launch_bin_count(d, s->stream);
+        pxl::launch_scan_offsets(d, s->stream);
+        pxl::launch_bins_out(d, cbuf.as<uint32_t>(), sbuf.as<uint32_t>(), s->stream);
+        pxl::launch_sort_segments(d, sbuf.as<uint32_t>(), s->stream);
+        if (cell_of) HIP_CHECK(hipMemcpyAsync(cell_of, cbuf.p, 4ull * d.np, hipMemcpyDeviceToHost, s->stream));
+        if (counts) HIP_CHECK(hipMemcpyAsync(counts, d.count, 4ull * d.ncells, hipMemcpyDeviceToHost, s->stream));
+        if (offsets) HIP_CHECK(hipMemcpyAsync(offsets, d.offset, 4ull * d.ncells, hipMemcpyDeviceToHost, s->stream));
+        if (sorted_ids) HIP_CHECK(hipMemcpyAsync(sorted_ids, sbuf.p, 4ull * d.np, hipMemcpyDeviceToHost, s->stream));
+        HIP_CHECK(hipStreamSynchronize(s->stream));
+    });
+}
+
+int pxlflip_grid_phase(pxlflip *h) {
+    if (!h) return PXLFLIP_ERR_INVALID;
+    Sim *s = &h->sim;
+    return pxl::guarded(s, [&] {
+        if (!s->initialized) s->reset();
+        s->enqueue_grid_phase();
+        HIP_CHECK(hipStreamSynchronize(s->stream));
+    });
+}
+
+int pxlflip_profile_steps(pxlflip *h, uint32_t n, pxlflip_kernel_time *out, uint32_t max_out, uint32_t *n_out) {
+    if (!h) return PXLFLIP_ERR_INVALID;
+    Sim *s = &h->sim;
+    std::vector<pxlflip_kernel_time> agg;
+    int rc = pxl::guarded(s, [&] {
+        if (!s->initialized) s->reset();
+        s->timer.clear();
+        s->timer.on = true;
+        try {
+            s->run_steps(n);
+        } catch (...) {
+            s->timer.on = false;
+            throw;
+        }
+        s->timer.on = false;
+        for (const auto &r : s->timer.recs) {
+            float ms = 0.0f;
+            HIP_CHECK(hipEventElapsedTime(&ms, r.a, r.b));
+            auto it = std::find_if(agg.begin(), agg.end(),
+                                   [&](const pxlflip_kernel_time &k) { return std::strcmp(k.name, r.name) == 0; });
+            if (it == agg.end()) {
+                pxlflip_kernel_time k{};
+                std::snprintf(k.name, sizeof k.name, "%s", r.name);
+                agg.push_back(k);
+                it = agg.end() - 1;
+            }
+            it->launches++;
+            it->total_ms += ms;
+        }
+        s->timer.clear();
+    });
+    if (rc != PXLFLIP_OK) return rc;
+    std::sort(agg.begin(), agg.end(),
+              [](const pxlflip_kernel_time &a, const pxlflip_kernel_time &b) { return a.total_ms > b.total_ms; });
+    uint32_t k = 0;
+    for (; k < agg.size() && k < max_out; k++)
+        if (out) out[k] = agg[k];
+    if (n_out) *n_out = (uint32_t)agg.size();
+    return PXLFLIP_OK;
+}
+
+int pxlflip_last_solve(const pxlflip *h, pxlflip_solve_stats *st) {
+    if (!h || !st) return PXLFLIP_ERR_INVALID;
+    *st = h->sim.stats;
+    return PXLFLIP_OK;
+}
+
+const char *pxlflip_last_error(const pxlflip *h) {
+    if (!h) return pxl::g_create_error.c_str();
+    return h->sim.error.c_str();
+}
+
+}  // extern "C"

@@ -1,0 +1,173 @@
+// Grid-side kernels of the FLIP step: normalize + forces (one pass over faces),
+// classify + stencil + divergence (one pass over cells), red-black Gauss-Seidel half sweeps
+// and the pressure-gradient projection.  Semantics follow shaders/fluid.slang (cited per
+// kernel).  Cells and faces are walked as 3-D launches of 64x4 lanes: a wave64 covers 64
+// consecutive x-cells of one row, so every array access of a wave is one contiguous run.
+//
+// Compiled with -ffp-contract=off: results are bitwise the reference formulas in fp32.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace pxl {
+
+namespace {
+
+constexpr int TX = 64, TY = 4;
+
+// SOLID for cells outside the grid or on its one-cell wall layer (fluid.slang:55-58, 126-128).
+__device__ __forceinline__ bool is_solid(const DevSim &S, int i, int j, int k) {
+    return i <= 0 || j <= 0 || k <= 0 || i >= S.nx - 1 || j >= S.ny - 1 || k >= S.nz - 1;
+}
+
+__device__ __forceinline__ uint32_t cidx(const DevSim &S, int i, int j, int k) {
+    return (uint32_t)i + (uint32_t)S.nx * ((uint32_t)j + (uint32_t)S.ny * (uint32_t)k);
+}
+__device__ __forceinline__ uint32_t uidx(const DevSim &S, int i, int j, int k) {
+    return (uint32_t)i + (uint32_t)(S.nx + 1) * ((uint32_t)j + (uint32_t)S.ny * (uint32_t)k);
+}
+__device__ __forceinline__ uint32_t vidx(const DevSim &S, int i, int j, int k) {
+    return S.nu + (uint32_t)i + (uint32_t)S.nx * ((uint32_t)j + (uint32_t)(S.ny + 1) * (uint32_t)k);
+}
+__device__ __forceinline__ uint32_t widx(const DevSim &S, int i, int j, int k) {
+    return S.nu + S.nv + (uint32_t)i + (uint32_t)S.nx * ((uint32_t)j + (uint32_t)S.ny * (uint32_t)k);
+}
+
+// normalize_grid faces + forces (fluid.slang:119-124, 143-151) for one face: v = acc / wacc
+// (0 without weight) -> vel_old; vel = v + g_c * dt, or 0 when either side is SOLID.
+// SOLID depends only on the cell position, so this needs no classified cells.
+__device__ __forceinline__ void face_update(const DevSim &S, uint32_t f, float g, bool keep) {
+    const int32_t w = S.wacc[f];
+    const float v = w > 0 ? (float)S.acc[f] / (float)w : 0.0f;
+    S.vel_old[f] = v;
+    S.vel[f] = keep ? v + g * S.dt : 0.0f;
+}
+
+__global__ __launch_bounds__(TX *TY) void k_faces(DevSim S) {
+    const int i = blockIdx.x * TX + threadIdx.x;
+    const int j = blockIdx.y * TY + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i > S.nx || j > S.ny) return;
+    const bool in_c = !is_solid(S, i, j, k);
+    if (j < S.ny && k < S.nz) face_update(S, uidx(S, i, j, k), S.gx, in_c && !is_solid(S, i - 1, j, k));
+    if (i < S.nx && k < S.nz) face_update(S, vidx(S, i, j, k), S.gy, in_c && !is_solid(S, i, j - 1, k));
+    if (i < S.nx && j < S.ny) face_update(S, widx(S, i, j, k), S.gz, in_c && !is_solid(S, i, j, k - 1));
+}
+
+__device__ __forceinline__ uint8_t type_from_count(const DevSim &S, int i, int j, int k) {
+    if (is_solid(S, i, j, k)) return CELL_SOLID;
+    return S.count[cidx(S, i, j, k)] > 0 ? CELL_FLUID : CELL_AIR;
+}
+
+// normalize_grid cells + compute_divergence (fluid.slang:125-131, 154-165), plus the stencil
+// word the pressure kernels read instead of re-classifying six neighbours.
+__global__ __launch_bounds__(TX *TY) void k_cells(DevSim S) {
+    const int i = blockIdx.x * TX + threadIdx.x;
+    const int j = blockIdx.y * TY + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= S.nx || j >= S.ny) return;
+    const uint32_t id = cidx(S, i, j, k);
+    const uint8_t t = type_from_count(S, i, j, k);
+    S.type[id] = t;
+    uint16_t st = 0;
+    float d = 0.0f;
+    if (t == CELL_FLUID) {
+        const int nb[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k}, {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
+        uint32_t diag = 0;
+#pragma unroll
+        for (int n = 0; n < 6; n++) {
+            const uint8_t tq = type_from_count(S, nb[n][0], nb[n][1], nb[n][2]);
+            if (tq != CELL_SOLID) diag++;
+            if (tq == CELL_FLUID) st |= (uint16_t)(1u << n);
+        }
+        st |= (uint16_t)(STEN_FLUID | (diag << 8));
+        d = d + (S.vel[uidx(S, i + 1, j, k)] - S.vel[uidx(S, i, j, k)]);
+        d = d + (S.vel[vidx(S, i, j + 1, k)] - S.vel[vidx(S, i, j, k)]);
+        d = d + (S.vel[widx(S, i, j, k + 1)] - S.vel[widx(S, i, j, k)]);
+    } else {
+        S.pressure[id] = 0.0f;
+    }
+    S.sten[id] = st;
+    S.div[id] = d * S.inv_h;
+}
+
+// pressure_rb (fluid.slang:168-182): one red-black half sweep.  Lanes walk only the cells
+// of the requested colour: i = 2*ih + ((parity + j + k) & 1).
+__global__ __launch_bounds__(TX *TY) void k_rbgs(DevSim S, uint32_t parity) {
+    const int ih = blockIdx.x * TX + threadIdx.x;
+    const int j = blockIdx.y * TY + threadIdx.y;
+    const int k = blockIdx.z;
+    const int i = 2 * ih + (int)((parity + (uint32_t)j + (uint32_t)k) & 1u);
+    if (i >= S.nx || j >= S.ny) return;
+    const uint32_t id = cidx(S, i, j, k);
+    const uint32_t st = S.sten[id];
+    if (!(st & STEN_FLUID)) return;
+    const uint32_t sx = 1, sy = (uint32_t)S.nx, sz = (uint32_t)S.nx * (uint32_t)S.ny;
+    float *p = S.pressure;
+    float sum = 0.0f;
+    if (st & 1u) sum = sum + p[id - sx];
+    if (st & 2u) sum = sum + p[id + sx];
+    if (st & 4u) sum = sum + p[id - sy];
+    if (st & 8u) sum = sum + p[id + sy];
+    if (st & 16u) sum = sum + p[id - sz];
+    if (st & 32u) sum = sum + p[id + sz];
+    const float cnt = (float)((st >> 8) & 7u);
+    if (cnt > 0.0f) p[id] = (sum - S.div[id] * S.h * S.h / S.dt) / cnt;
+}
+
+__device__ __forceinline__ uint8_t type_at(const DevSim &S, int i, int j, int k) {
+    if (i < 0 || j < 0 || k < 0 || i >= S.nx || j >= S.ny || k >= S.nz) return CELL_SOLID;
+    return S.type[cidx(S, i, j, k)];
+}
+
+// project (fluid.slang:185-197) for one face between cells a (low side) and b (high side).
+__device__ __forceinline__ void project_face(const DevSim &S, uint32_t f, int ai, int aj, int ak, int bi,
+                                             int bj, int bk) {
+    const uint8_t ta = type_at(S, ai, aj, ak), tb = type_at(S, bi, bj, bk);
+    if (ta == CELL_SOLID || tb == CELL_SOLID) {
+        S.vel[f] = 0.0f;
+        return;
+    }
+    if (ta == CELL_FLUID || tb == CELL_FLUID) {
+        const float pa = ta == CELL_FLUID ? S.pressure[cidx(S, ai, aj, ak)] : 0.0f;
+        const float pb = tb == CELL_FLUID ? S.pressure[cidx(S, bi, bj, bk)] : 0.0f;
+        S.vel[f] = S.vel[f] - S.dt * S.inv_h * (pb - pa);
+    }
+}
+
+__global__ __launch_bounds__(TX *TY) void k_project(DevSim S) {
+    const int i = blockIdx.x * TX + threadIdx.x;
+    const int j = blockIdx.y * TY + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i > S.nx || j > S.ny) return;
+    if (j < S.ny && k < S.nz) project_face(S, uidx(S, i, j, k), i - 1, j, k, i, j, k);
+    if (i < S.nx && k < S.nz) project_face(S, vidx(S, i, j, k), i, j - 1, k, i, j, k);
+    if (i < S.nx && j < S.ny) project_face(S, widx(S, i, j, k), i, j, k - 1, i, j, k);
+}
+
+__global__ __launch_bounds__(TX *TY) void k_type_u32(DevSim S, uint32_t *out) {
+    const int i = blockIdx.x * TX + threadIdx.x;
+    const int j = blockIdx.y * TY + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= S.nx || j >= S.ny) return;
+    const uint32_t id = cidx(S, i, j, k);
+    out[id] = S.type[id];
+}
+
+dim3 cell_grid(const DevSim &S) { return dim3((S.nx + TX - 1) / TX, (S.ny + TY - 1) / TY, S.nz); }
+dim3 node_grid(const DevSim &S) { return dim3((S.nx + 1 + TX - 1) / TX, (S.ny + 1 + TY - 1) / TY, S.nz + 1); }
+
+}  // namespace
+
+void launch_faces(const DevSim &S, hipStream_t s) { k_faces<<<node_grid(S), dim3(TX, TY), 0, s>>>(S); }
+void launch_cells(const DevSim &S, hipStream_t s) { k_cells<<<cell_grid(S), dim3(TX, TY), 0, s>>>(S); }
+void launch_rbgs(const DevSim &S, uint32_t parity, hipStream_t s) {
+    const int half = (S.nx + 1) / 2;
+    dim3 g((half + TX - 1) / TX, (S.ny + TY - 1) / TY, S.nz);
+    k_rbgs<<<g, dim3(TX, TY), 0, s>>>(S, parity);
+}
+void launch_project(const DevSim &S, hipStream_t s) { k_project<<<node_grid(S), dim3(TX, TY), 0, s>>>(S); }
+void launch_type_u32(const DevSim &S, uint32_t *out, hipStream_t s) {
+    k_type_u32<<<cell_grid(S), dim3(TX, TY), 0, s>>>(S, out);
+}
+
+}  // namespace pxl

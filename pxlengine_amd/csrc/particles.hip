@@ -1,0 +1,376 @@
+// Particle-side kernels of the FLIP step: counting-sort binning, P2G splat, G2P advection
+// and particle IO.  Semantics follow shaders/fluid.slang (cited per kernel); the layout is
+// SoA fp32 streams sorted by cell once per step, so that the splat and the gathers of
+// neighbouring lanes touch neighbouring faces.
+//
+// Floating point: this file is compiled with -ffp-contract=off.  Every expression keeps the
+// reference's operation order, so the results are bitwise those of the reference formulas
+// evaluated in IEEE fp32 (the P2G sums are integers and therefore order-independent).
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace pxl {
+
+namespace {
+
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_TILE = BLOCK * SCAN_ITEMS;  // 4096 cells per scan workgroup
+
+inline unsigned blocks_for(uint64_t n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
+
+// Cell of a position: clamp(floor((x - origin) * inv_h), 0, n - 1) (fluid.slang:113).
+__device__ __forceinline__ uint32_t cell_of_pos(const DevSim &S, float x, float y, float z) {
+    int cx = (int)floorf((x - S.ox) * S.inv_h);
+    int cy = (int)floorf((y - S.oy) * S.inv_h);
+    int cz = (int)floorf((z - S.oz) * S.inv_h);
+    cx = cx < 0 ? 0 : (cx > S.nx - 1 ? S.nx - 1 : cx);
+    cy = cy < 0 ? 0 : (cy > S.ny - 1 ? S.ny - 1 : cy);
+    cz = cz < 0 ? 0 : (cz > S.nz - 1 ? S.nz - 1 : cz);
+    return (uint32_t)cx + (uint32_t)S.nx * ((uint32_t)cy + (uint32_t)S.ny * (uint32_t)cz);
+}
+
+// ---------------------------------------------------------------- clear / binning
+
+__global__ __launch_bounds__(BLOCK) void k_fill_u32(uint32_t *p, uint32_t n, uint32_t value) {
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) p[i] = value;
+}
+
+// clear_grid (fluid.slang:88-91): acc, wacc over faces, count over cells.
+__global__ __launch_bounds__(BLOCK) void k_clear_grid(DevSim S) {
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < S.nfaces) {
+        S.acc[i] = 0;
+        S.wacc[i] = 0;
+    }
+    if (i < S.ncells) S.count[i] = 0;
+}
+
+// Counting-sort pass 1: cell key of each particle and its arrival slot in that cell
+// (the per-cell count of p2g, fluid.slang:113-114).
+__global__ __launch_bounds__(BLOCK) void k_bin_count(DevSim S) {
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= S.np) return;
+    uint32_t c = cell_of_pos(S, S.ax[i], S.ay[i], S.az[i]);
+    S.key[i] = c;
+    S.slot[i] = atomicAdd(&S.count[c], 1u);
+}
+
+// Counting-sort pass 2: exclusive scan of the counts (reduce / scan of tile sums / down-sweep).
+__global__ __launch_bounds__(BLOCK) void k_scan_reduce(const uint32_t *in, uint32_t n, uint32_t *tile_sum) {
+    __shared__ uint32_t sh[4];
+    const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    uint32_t s = 0;
+    if (base + SCAN_ITEMS <= n) {
+        const uint4 *p = reinterpret_cast<const uint4 *>(in + base);
+#pragma unroll
+        for (int m = 0; m < SCAN_ITEMS / 4; m++) {
+            uint4 q = p[m];
+            s += q.x + q.y + q.z + q.w;
+        }
+    } else {
+        for (int m = 0; m < SCAN_ITEMS; m++)
+            if (base + m < n) s += in[base + m];
+    }
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) tile_sum[blockIdx.x] = s;
+}
+
+// Single workgroup: exclusive scan of the tile sums in place (any count, in chunks).
+__global__ __launch_bounds__(BLOCK) void k_scan_tiles(uint32_t *tile_sum, uint32_t ntiles) {
+    __shared__ uint32_t sh[4];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < ntiles; base += BLOCK * 4) {
+        uint32_t v[4], t = 0;
+        for (int m = 0; m < 4; m++) {
+            uint32_t idx = base + threadIdx.x * 4 + m;
+            v[m] = idx < ntiles ? tile_sum[idx] : 0;
+            t += v[m];
+        }
+        uint32_t total;
+        uint32_t off = block_exclusive_scan(t, &total, sh) + carry;
+        for (int m = 0; m < 4; m++) {
+            uint32_t idx = base + threadIdx.x * 4 + m;
+            if (idx < ntiles) tile_sum[idx] = off;
+            off += v[m];
+        }
+        carry += total;
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_scan_down(const uint32_t *in, uint32_t n, const uint32_t *tile_off,
+                                                     uint32_t *out) {
+    __shared__ uint32_t sh[4];
+    const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    uint32_t t = 0;
+    const bool full = base + SCAN_ITEMS <= n;
+    if (full) {
+        const uint4 *p = reinterpret_cast<const uint4 *>(in + base);
+#pragma unroll
+        for (int m = 0; m < SCAN_ITEMS / 4; m++) {
+            uint4 q = p[m];
+            v[4 * m] = q.x; v[4 * m + 1] = q.y; v[4 * m + 2] = q.z; v[4 * m + 3] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < SCAN_ITEMS; m++) v[m] = base + m < n ? in[base + m] : 0;
+    }
+#pragma unroll
+    for (int m = 0; m < SCAN_ITEMS; m++) t += v[m];
+    uint32_t total;
+    uint32_t run = block_exclusive_scan(t, &total, sh) + tile_off[blockIdx.x];
+    if (full) {
+        uint4 *p = reinterpret_cast<uint4 *>(out + base);
+#pragma unroll
+        for (int m = 0; m < SCAN_ITEMS / 4; m++) {
+            uint4 q;
+            q.x = run; run += v[4 * m];
+            q.y = run; run += v[4 * m + 1];
+            q.z = run; run += v[4 * m + 2];
+            q.w = run; run += v[4 * m + 3];
+            p[m] = q;
+        }
+    } else {
+        for (int m = 0; m < SCAN_ITEMS; m++) {
+            if (base + m < n) out[base + m] = run;
+            run += v[m];
+        }
+    }
+}
+
+// Counting-sort pass 3: move each particle to offset[cell] + slot (current -> scratch).
+__global__ __launch_bounds__(BLOCK) void k_scatter(DevSim S) {
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= S.np) return;
+    uint32_t j = S.offset[S.key[i]] + S.slot[i];
+    S.bx[j] = S.ax[i];
+    S.by[j] = S.ay[i];
+    S.bz[j] = S.az[i];
+    S.bvx[j] = S.avx[i];
+    S.bvy[j] = S.avy[i];
+    S.bvz[j] = S.avz[i];
+    S.bid[j] = S.aid[i];
+}
+
+// ---------------------------------------------------------------- P2G / G2P
+
+__device__ __forceinline__ uint32_t face_base(const DevSim &S, int c) {
+    return c == 0 ? 0u : (c == 1 ? S.nu : S.nu + S.nv);
+}
+
+// p2g (fluid.slang:94-112): trilinear fixed-point splat of the particle velocity onto the
+// 8 faces of each component; out-of-range corners are skipped.  Runs over the cell-sorted
+// scratch copy so neighbouring lanes add into neighbouring faces.
+__global__ __launch_bounds__(BLOCK) void k_p2g(DevSim S) {
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= S.np) return;
+    const float x[3] = {S.bx[i], S.by[i], S.bz[i]};
+    const float v[3] = {S.bvx[i], S.bvy[i], S.bvz[i]};
+    const float o[3] = {S.ox, S.oy, S.oz};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        int dim[3] = {S.nx, S.ny, S.nz};
+        dim[c] += 1;
+        float g[3], f[3];
+        int b[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) g[a] = (x[a] - o[a]) * S.inv_h - 0.5f;
+        g[c] = g[c] + 0.5f;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            b[a] = (int)floorf(g[a]);
+            f[a] = g[a] - (float)b[a];
+        }
+        const uint32_t base = face_base(S, c);
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int ox = k & 1, oy = (k >> 1) & 1, oz = k >> 2;
+            const int qx = b[0] + ox, qy = b[1] + oy, qz = b[2] + oz;
+            if (qx < 0 || qy < 0 || qz < 0 || qx > dim[0] - 1 || qy > dim[1] - 1 || qz > dim[2] - 1) continue;
+            const float wx = ox ? f[0] : 1.0f - f[0];
+            const float wy = oy ? f[1] : 1.0f - f[1];
+            const float wz = oz ? f[2] : 1.0f - f[2];
+            const float w = wx * wy * wz;
+            const uint32_t fi = base + (uint32_t)qx + (uint32_t)dim[0] * ((uint32_t)qy + (uint32_t)dim[1] * (uint32_t)qz);
+            atomicAdd(&S.acc[fi], (int)(w * v[c] * 65536.0f));
+            atomicAdd(&S.wacc[fi], (int)(w * 65536.0f));
+        }
+    }
+}
+
+// sample_face (fluid.slang:67-80): trilinear gather with indices clamped to the face grid.
+__device__ __forceinline__ float sample_face(const DevSim &S, const float *__restrict__ buf, int c, float x,
+                                             float y, float z) {
+    int dim[3] = {S.nx, S.ny, S.nz};
+    dim[c] += 1;
+    float g[3] = {(x - S.ox) * S.inv_h - 0.5f, (y - S.oy) * S.inv_h - 0.5f, (z - S.oz) * S.inv_h - 0.5f};
+    g[c] = g[c] + 0.5f;
+    int b[3];
+    float f[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        b[a] = (int)floorf(g[a]);
+        f[a] = g[a] - (float)b[a];
+    }
+    const uint32_t base = face_base(S, c);
+    float s = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const int ox = k & 1, oy = (k >> 1) & 1, oz = k >> 2;
+        int qx = b[0] + ox, qy = b[1] + oy, qz = b[2] + oz;
+        qx = qx < 0 ? 0 : (qx > dim[0] - 1 ? dim[0] - 1 : qx);
+        qy = qy < 0 ? 0 : (qy > dim[1] - 1 ? dim[1] - 1 : qy);
+        qz = qz < 0 ? 0 : (qz > dim[2] - 1 ? dim[2] - 1 : qz);
+        const float wx = ox ? f[0] : 1.0f - f[0];
+        const float wy = oy ? f[1] : 1.0f - f[1];
+        const float wz = oz ? f[2] : 1.0f - f[2];
+        s = s + wx * wy * wz * buf[base + (uint32_t)qx + (uint32_t)dim[0] * ((uint32_t)qy + (uint32_t)dim[1] * (uint32_t)qz)];
+    }
+    return s;
+}
+
+// g2p_advect (fluid.slang:200-220): PIC/FLIP blend (lerp = a*(1-t) + b*t), RK2 through the
+// projected grid velocity, clamp into the interior with the wall-normal velocity clipped
+// (Slang max/min = SPIR-V FMax/FMin).  Reads the sorted scratch copy, writes the current
+// particle arrays at the same (sorted) index.
+__global__ __launch_bounds__(BLOCK) void k_g2p(DevSim S) {
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= S.np) return;
+    float x[3] = {S.bx[i], S.by[i], S.bz[i]};
+    const float v0[3] = {S.bvx[i], S.bvy[i], S.bvz[i]};
+    float pic[3], old[3], v[3], mid[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) pic[c] = sample_face(S, S.vel, c, x[0], x[1], x[2]);
+#pragma unroll
+    for (int c = 0; c < 3; c++) old[c] = sample_face(S, S.vel_old, c, x[0], x[1], x[2]);
+    const float t = S.flip;
+    const float half_dt = 0.5f * S.dt;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const float flipv = v0[c] + pic[c] - old[c];
+        v[c] = pic[c] * (1.0f - t) + flipv * t;
+        mid[c] = x[c] + half_dt * pic[c];
+    }
+    float adv[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) adv[c] = sample_face(S, S.vel, c, mid[0], mid[1], mid[2]);
+#pragma unroll
+    for (int c = 0; c < 3; c++) x[c] = x[c] + S.dt * adv[c];
+    const float o[3] = {S.ox, S.oy, S.oz};
+    const float nf[3] = {(float)S.nx, (float)S.ny, (float)S.nz};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const float lo = o[a] + S.h * 1.001f;
+        const float hi = o[a] + S.h * (nf[a] - 1.001f);
+        if (x[a] < lo) { x[a] = lo; v[a] = v[a] < 0.0f ? 0.0f : v[a]; }
+        if (x[a] > hi) { x[a] = hi; v[a] = 0.0f < v[a] ? 0.0f : v[a]; }
+    }
+    S.ax[i] = x[0];
+    S.ay[i] = x[1];
+    S.az[i] = x[2];
+    S.avx[i] = v[0];
+    S.avy[i] = v[1];
+    S.avz[i] = v[2];
+    S.aid[i] = S.bid[i];
+}
+
+// ---------------------------------------------------------------- IO / bins
+
+// Bins in seeding order: cell_of[id] and the per-cell id lists at offset + slot.
+__global__ __launch_bounds__(BLOCK) void k_bins_out(DevSim S, uint32_t *cell_of, uint32_t *sorted) {
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= S.np) return;
+    uint32_t id = S.aid[i], c = S.key[i];
+    if (cell_of) cell_of[id] = c;
+    if (sorted) sorted[S.offset[c] + S.slot[i]] = id;
+}
+
+// Canonical order inside each cell: ids ascending (insertion sort of the cell's segment).
+__global__ __launch_bounds__(BLOCK) void k_sort_segments(DevSim S, uint32_t *sorted) {
+    uint32_t c = blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= S.ncells) return;
+    uint32_t *seg = sorted + S.offset[c];
+    uint32_t n = S.count[c];
+    for (uint32_t a = 1; a < n; a++) {
+        uint32_t v = seg[a];
+        uint32_t b = a;
+        while (b > 0 && seg[b - 1] > v) {
+            seg[b] = seg[b - 1];
+            b--;
+        }
+        seg[b] = v;
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_unpack_aos8(DevSim S, float *out) {
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= S.np) return;
+    float *r = out + (size_t)S.aid[i] * 8;
+    float4 a = make_float4(S.ax[i], S.ay[i], S.az[i], 0.0f);
+    float4 b = make_float4(S.avx[i], S.avy[i], S.avz[i], 0.0f);
+    reinterpret_cast<float4 *>(r)[0] = a;
+    reinterpret_cast<float4 *>(r)[1] = b;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_pack_aos8(DevSim S, const float *in) {
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= S.np) return;
+    const float4 a = reinterpret_cast<const float4 *>(in + (size_t)i * 8)[0];
+    const float4 b = reinterpret_cast<const float4 *>(in + (size_t)i * 8)[1];
+    S.ax[i] = a.x; S.ay[i] = a.y; S.az[i] = a.z;
+    S.avx[i] = b.x; S.avy[i] = b.y; S.avz[i] = b.z;
+    S.aid[i] = i;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_reset_particles(DevSim S) {
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= S.np) return;
+    S.ax[i] = S.ix[i]; S.ay[i] = S.iy[i]; S.az[i] = S.iz[i];
+    S.avx[i] = S.ivx[i]; S.avy[i] = S.ivy[i]; S.avz[i] = S.ivz[i];
+    S.aid[i] = i;
+}
+
+}  // namespace
+
+void launch_fill_u32(uint32_t *p, uint32_t n, uint32_t value, hipStream_t s) {
+    if (n) k_fill_u32<<<blocks_for(n), BLOCK, 0, s>>>(p, n, value);
+}
+
+void launch_clear_grid(const DevSim &S, hipStream_t s) {
+    uint32_t n = S.nfaces > S.ncells ? S.nfaces : S.ncells;
+    k_clear_grid<<<blocks_for(n), BLOCK, 0, s>>>(S);
+}
+
+void launch_bin_count(const DevSim &S, hipStream_t s) {
+    k_bin_count<<<blocks_for(S.np), BLOCK, 0, s>>>(S);
+}
+
+void launch_scan_offsets(const DevSim &S, hipStream_t s) {
+    const uint32_t ntiles = (S.ncells + SCAN_TILE - 1) / SCAN_TILE;
+    k_scan_reduce<<<ntiles, BLOCK, 0, s>>>(S.count, S.ncells, S.scan_tmp);
+    k_scan_tiles<<<1, BLOCK, 0, s>>>(S.scan_tmp, ntiles);
+    k_scan_down<<<ntiles, BLOCK, 0, s>>>(S.count, S.ncells, S.scan_tmp, S.offset);
+}
+
+void launch_scatter(const DevSim &S, hipStream_t s) { k_scatter<<<blocks_for(S.np), BLOCK, 0, s>>>(S); }
+void launch_p2g(const DevSim &S, hipStream_t s) { k_p2g<<<blocks_for(S.np), BLOCK, 0, s>>>(S); }
+void launch_g2p(const DevSim &S, hipStream_t s) { k_g2p<<<blocks_for(S.np), BLOCK, 0, s>>>(S); }
+
+void launch_bins_out(const DevSim &S, uint32_t *cell_of, uint32_t *sorted, hipStream_t s) {
+    k_bins_out<<<blocks_for(S.np), BLOCK, 0, s>>>(S, cell_of, sorted);
+}
+void launch_sort_segments(const DevSim &S, uint32_t *sorted, hipStream_t s) {
+    k_sort_segments<<<blocks_for(S.ncells), BLOCK, 0, s>>>(S, sorted);
+}
+void launch_unpack_aos8(const DevSim &S, float *aos8, hipStream_t s) {
+    k_unpack_aos8<<<blocks_for(S.np), BLOCK, 0, s>>>(S, aos8);
+}
+void launch_pack_aos8(const DevSim &S, const float *aos8, hipStream_t s) {
+    k_pack_aos8<<<blocks_for(S.np), BLOCK, 0, s>>>(S, aos8);
+}
+void launch_reset_particles(const DevSim &S, hipStream_t s) {
+    k_reset_particles<<<blocks_for(S.np), BLOCK, 0, s>>>(S);
+}
+
+}  // namespace pxl

@@ -1,0 +1,265 @@
+// Fluid block parsing, validation, grid derivation and seeding (see spec.hpp).
+// Compiled with -ffp-contract=off: every f32 operation below rounds once, in source order,
+// as the reference's Rust (which never contracts) does.
+#include "spec.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <set>
+
+#include "json.hpp"
+
+namespace pxl {
+namespace {
+
+const char *const kFluidFields[] = {"name", "bounds", "resolution", "fill", "particles_per_cell",
+                                    "substeps", "flip", "gravity", "pressure_iterations",
+                                    "surface_refine", "particle_radius", "warmup",
+                                    "max_triangles", "materials"};
+const char *const kBlockFields[] = {"min", "max", "velocity"};
+
+template <size_t N>
+std::string expected_list(const char *const (&fields)[N]) {
+    std::string s;
+    for (size_t k = 0; k < N; k++) {
+        if (k) s += ", ";
+        s += "`";
+        s += fields[k];
+        s += "`";
+    }
+    return s;
+}
+
+template <size_t N>
+bool known(const char *const (&fields)[N], const std::string &key) {
+    for (size_t k = 0; k < N; k++)
+        if (key == fields[k]) return true;
+    return false;
+}
+
+std::string get_u32(const JsonValue &v, uint32_t &out) {
+    if (v.kind != JsonValue::Number) return "invalid type: " + json_describe(v) + ", expected u32";
+    if (!v.is_integer) return "invalid type: " + json_describe(v) + ", expected u32";
+    if (v.negative || !v.uint_fits || v.uint_value > 0xFFFFFFFFull)
+        return "invalid value: " + json_describe(v) + ", expected u32";
+    out = (uint32_t)v.uint_value;
+    return "";
+}
+
+std::string get_f32(const JsonValue &v, float &out) {
+    if (v.kind != JsonValue::Number) return "invalid type: " + json_describe(v) + ", expected f32";
+    out = (float)v.num;  // serde_json: decimal -> f64 (correctly rounded) -> `as f32`
+    return "";
+}
+
+std::string get_vec3(const JsonValue &v, float out[3]) {
+    if (v.kind != JsonValue::Array) return "invalid type: " + json_describe(v) + ", expected an array of length 3";
+    if (v.items.size() != 3)
+        return "invalid length " + std::to_string(v.items.size()) + ", expected an array of length 3";
+    for (int a = 0; a < 3; a++) {
+        std::string e = get_f32(v.items[a], out[a]);
+        if (!e.empty()) return e;
+    }
+    return "";
+}
+
+std::string parse_block(const JsonValue &v, FluidBlockSpec &b) {
+    if (v.kind != JsonValue::Object) return "invalid type: " + json_describe(v) + ", expected struct FluidBlock";
+    std::set<std::string> seen;
+    bool has_min = false, has_max = false;
+    for (const auto &m : v.members) {
+        const std::string &k = m.first;
+        if (!known(kBlockFields, k)) return "unknown field `" + k + "`, expected one of " + expected_list(kBlockFields);
+        if (!seen.insert(k).second) return "duplicate field `" + k + "`";
+        std::string e;
+        if (k == "min") { e = get_vec3(m.second, b.min); has_min = true; }
+        else if (k == "max") { e = get_vec3(m.second, b.max); has_max = true; }
+        else { e = get_vec3(m.second, b.velocity); }
+        if (!e.empty()) return e;
+    }
+    if (!has_min) return "missing field `min`";
+    if (!has_max) return "missing field `max`";
+    return "";
+}
+
+bool all_gt(const float a[3], const float b[3]) { return a[0] > b[0] && a[1] > b[1] && a[2] > b[2]; }
+bool any_lt(const float a[3], const float b[3]) { return a[0] < b[0] || a[1] < b[1] || a[2] < b[2]; }
+
+}  // namespace
+
+std::string parse_fluid(const std::string &json, FluidSpec &f) {
+    JsonValue root;
+    JsonError jerr;
+    if (!json_parse(json, root, jerr))
+        return "invalid fluid: " + jerr.message + " at line " + std::to_string(jerr.line) + " column " +
+               std::to_string(jerr.column);
+    if (root.kind != JsonValue::Object) return "invalid fluid: invalid type: " + json_describe(root) + ", expected struct Fluid";
+    f = FluidSpec();
+    std::set<std::string> seen;
+    bool has_name = false, has_bounds = false, has_fill = false;
+    for (const auto &m : root.members) {
+        const std::string &k = m.first;
+        const JsonValue &v = m.second;
+        std::string e;
+        if (!known(kFluidFields, k)) {
+            e = "unknown field `" + k + "`, expected one of " + expected_list(kFluidFields);
+        } else if (!seen.insert(k).second) {
+            e = "duplicate field `" + k + "`";
+        } else if (k == "name") {
+            if (v.kind != JsonValue::String) e = "invalid type: " + json_describe(v) + ", expected a string";
+            else f.name = v.text;
+            has_name = true;
+        } else if (k == "bounds") {
+            if (v.kind != JsonValue::Array) e = "invalid type: " + json_describe(v) + ", expected an array of length 2";
+            else if (v.items.size() != 2) e = "invalid length " + std::to_string(v.items.size()) + ", expected an array of length 2";
+            else {
+                e = get_vec3(v.items[0], f.bounds[0]);
+                if (e.empty()) e = get_vec3(v.items[1], f.bounds[1]);
+            }
+            has_bounds = true;
+        } else if (k == "resolution") {
+            e = get_u32(v, f.resolution);
+        } else if (k == "fill") {
+            if (v.kind != JsonValue::Array) e = "invalid type: " + json_describe(v) + ", expected a sequence";
+            else {
+                for (const auto &item : v.items) {
+                    FluidBlockSpec b;
+                    e = parse_block(item, b);
+                    if (!e.empty()) break;
+                    f.fill.push_back(b);
+                }
+            }
+            has_fill = true;
+        } else if (k == "particles_per_cell") {
+            e = get_u32(v, f.particles_per_cell);
+        } else if (k == "substeps") {
+            e = get_u32(v, f.substeps);
+        } else if (k == "flip") {
+            e = get_f32(v, f.flip);
+        } else if (k == "gravity") {
+            e = get_vec3(v, f.gravity);
+        } else if (k == "pressure_iterations") {
+            e = get_u32(v, f.pressure_iterations);
+        } else if (k == "surface_refine") {
+            e = get_u32(v, f.surface_refine);
+        } else if (k == "particle_radius") {
+            e = get_f32(v, f.particle_radius);
+        } else if (k == "warmup") {
+            e = get_f32(v, f.warmup);
+        } else if (k == "max_triangles") {
+            e = get_u32(v, f.max_triangles);
+        } else if (k == "materials") {
+            // Rendering materials (OpenPBR inputs) do not enter the simulation; the list is
+            // accepted as a sequence of objects and otherwise not interpreted.
+            if (v.kind != JsonValue::Array) e = "invalid type: " + json_describe(v) + ", expected a sequence";
+            else {
+                for (const auto &item : v.items)
+                    if (item.kind != JsonValue::Object) {
+                        e = "invalid type: " + json_describe(item) + ", expected struct Material";
+                        break;
+                    }
+                f.n_materials = (uint32_t)v.items.size();
+            }
+        }
+        if (!e.empty()) return "invalid fluid: " + e;
+    }
+    if (!has_name) return "invalid fluid: missing field `name`";
+    if (!has_bounds) return "invalid fluid: missing field `bounds`";
+    if (!has_fill) return "invalid fluid: missing field `fill`";
+    return "";
+}
+
+void fluid_grid(const FluidSpec &f, uint32_t n[3], float *h_out) {
+    float size[3];
+    for (int a = 0; a < 3; a++) size[a] = f.bounds[1][a] - f.bounds[0][a];
+    float m = size[0];
+    if (size[1] > m) m = size[1];
+    if (size[2] > m) m = size[2];
+    float h = m / (float)f.resolution;
+    for (int a = 0; a < 3; a++) {
+        float c = std::round(size[a] / h);  // f32::round (half away from zero)
+        if (c < 4.0f) c = 4.0f;
+        n[a] = (uint32_t)c;
+    }
+    *h_out = h;
+}
+
+std::string validate_fluid(const FluidSpec &f, uint32_t index) {
+    std::string ctx = "fluids[" + std::to_string(index) + "] '" + f.name + "': ";
+    const float *lo = f.bounds[0], *hi = f.bounds[1];
+    if (!all_gt(hi, lo)) return ctx + "bounds must be [min, max] with max > min on every axis";
+    bool flip_ok = f.flip >= 0.0f && f.flip <= 1.0f;  // NaN fails, as RangeInclusive::contains
+    if (!(f.resolution >= 16 && f.resolution <= 256) ||
+        !(f.particles_per_cell >= 1 && f.particles_per_cell <= 64) ||
+        !(f.substeps >= 1 && f.substeps <= 64) || !(f.surface_refine >= 1 && f.surface_refine <= 3) ||
+        !flip_ok || f.pressure_iterations == 0)
+        return ctx + "resolution 16..256, particles_per_cell 1..64, substeps 1..64, surface_refine 1..3, flip 0..1";
+    if (f.fill.empty()) return ctx + "needs at least one `fill` block of liquid";
+    for (const auto &b : f.fill)
+        if (!all_gt(b.max, b.min) || any_lt(b.min, lo) || any_lt(hi, b.max))
+            return ctx + "each fill block must have max > min and lie inside `bounds`";
+    uint32_t n[3];
+    float h;
+    fluid_grid(f, n, &h);
+    if ((uint64_t)n[0] * n[1] * n[2] > 16000000ull) return ctx + "grid too large (lower resolution)";
+    return "";
+}
+
+float fluid_dt(double fps, uint32_t substeps) { return (float)(1.0 / (fps * (double)substeps)); }
+
+uint64_t steps_for_frame(double warmup, double frame, double fps, uint32_t substeps) {
+    double t = std::round((warmup + frame / fps) * fps * (double)substeps);
+    return t > 0.0 ? (uint64_t)t : 0;
+}
+
+namespace {
+inline float lcg_next(uint64_t &s) {
+    s = s * 6364136223846793005ull + 1442695040888963407ull;
+    return (float)(s >> 40) / (float)(1ull << 24);
+}
+}  // namespace
+
+uint64_t seed_particles(const FluidSpec &f, uint32_t seed, std::vector<float> *x, std::vector<float> *v) {
+    uint32_t n[3];
+    float h;
+    fluid_grid(f, n, &h);
+    const float *o = f.bounds[0];
+    uint64_t rng = 0x9e3779b97f4a7c15ull ^ (uint64_t)seed;
+    uint64_t count = 0;
+    std::vector<float> xs[3], vs[3];
+    for (const auto &b : f.fill) {
+        for (uint32_t k = 1; k + 1 < n[2]; k++)
+            for (uint32_t j = 1; j + 1 < n[1]; j++)
+                for (uint32_t i = 1; i + 1 < n[0]; i++) {
+                    float cx = o[0] + ((float)i + 0.5f) * h;
+                    float cy = o[1] + ((float)j + 0.5f) * h;
+                    float cz = o[2] + ((float)k + 0.5f) * h;
+                    if (cx < b.min[0] || cx > b.max[0] || cy < b.min[1] || cy > b.max[1] ||
+                        cz < b.min[2] || cz > b.max[2])
+                        continue;
+                    for (uint32_t q = 0; q < f.particles_per_cell; q++) {
+                        float px = o[0] + ((float)i + lcg_next(rng)) * h;
+                        float py = o[1] + ((float)j + lcg_next(rng)) * h;
+                        float pz = o[2] + ((float)k + lcg_next(rng)) * h;
+                        if (x) {
+                            xs[0].push_back(px); xs[1].push_back(py); xs[2].push_back(pz);
+                            for (int a = 0; a < 3; a++) vs[a].push_back(b.velocity[a]);
+                        }
+                        count++;
+                    }
+                }
+    }
+    if (x) {
+        x->clear();
+        v->clear();
+        x->reserve(3 * count);
+        v->reserve(3 * count);
+        for (int a = 0; a < 3; a++) {
+            x->insert(x->end(), xs[a].begin(), xs[a].end());
+            v->insert(v->end(), vs[a].begin(), vs[a].end());
+        }
+    }
+    return count;
+}
+
+}  // namespace pxl

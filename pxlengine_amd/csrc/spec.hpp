@@ -1,0 +1,58 @@
+// The scene's fluid block and the host-side derivations of the FLIP path.
+//
+// FluidSpec mirrors pxl_scene::Fluid / FluidBlock (crates/pxl-scene/src/lib.rs:49-119),
+// parse_fluid() its strict serde deserialisation, validate_fluid() the fluid checks of
+// Scene::validate (lib.rs:710-735), fluid_grid() Fluid::grid (lib.rs:123-128), and
+// seed_particles() the seeding loop of FluidState::new (crates/pxl-render/src/fluid.rs:96-149).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace pxl {
+
+struct FluidBlockSpec {
+    float min[3] = {0, 0, 0};
+    float max[3] = {0, 0, 0};
+    float velocity[3] = {0, 0, 0};
+};
+
+struct FluidSpec {
+    std::string name;
+    float bounds[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    uint32_t resolution = 64;
+    std::vector<FluidBlockSpec> fill;
+    uint32_t particles_per_cell = 8;
+    uint32_t substeps = 8;
+    float flip = 0.95f;
+    float gravity[3] = {0.0f, -9.81f, 0.0f};
+    uint32_t pressure_iterations = 80;
+    uint32_t surface_refine = 2;
+    float particle_radius = 2.0f;
+    float warmup = 0.0f;
+    uint32_t max_triangles = 2000000;
+    uint32_t n_materials = 0;  // rendering-only; accepted, not interpreted
+};
+
+// Strict parse of one fluids[] object.  Returns "" on success, else the error message.
+std::string parse_fluid(const std::string &json, FluidSpec &out);
+
+// Scene::validate's fluid checks; index is i in "fluids[i] 'name': ...".  "" when valid.
+std::string validate_fluid(const FluidSpec &f, uint32_t index);
+
+// Fluid::grid: cells per axis and cell size, glam Vec3 f32 arithmetic.
+void fluid_grid(const FluidSpec &f, uint32_t n[3], float *h);
+
+// (1.0 / (fps * substeps)) as f32 (fluid.rs:164).
+float fluid_dt(double fps, uint32_t substeps);
+
+// round((warmup + frame / fps) * fps * substeps).max(0) (fluid.rs:234-236).
+uint64_t steps_for_frame(double warmup, double frame, double fps, uint32_t substeps);
+
+// Reference seeding into SoA arrays (x[3][P], v[3][P]); returns the particle count.
+// With out == nullptr only counts.
+uint64_t seed_particles(const FluidSpec &f, uint32_t seed, std::vector<float> *x,
+                        std::vector<float> *v);
+
+}  // namespace pxl

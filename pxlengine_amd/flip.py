@@ -1,0 +1,296 @@
+"""ctypes binding of ``libpxlflip.so`` (the C ABI in ``include/pxlflip.h``).
+
+``FluidState`` mirrors the reference's per-fluid driver, ``FluidState`` in
+crates/pxl-render/src/fluid.rs:74-306: it is created from one scene ``fluids[]`` block plus
+``render.fps`` / ``render.seed`` (``FluidState::new``, fluid.rs:109), reports
+``particle_count`` and ``reservation`` (fluid.rs:104-106, 230-232) and is advanced to a frame
+with ``advance`` (``fluid_advance``'s step loop, fluid.rs:265-306).  Everything runs in the
+HIP library on the GPU; there is no fallback: if the library is missing the import fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpxlflip.so")
+
+OK = 0
+ERR_INVALID, ERR_OOM, ERR_DEVICE, ERR_INTERNAL = -1, -2, -3, -4
+
+SOLVERS = {"rbgs": 0, "cg": 1, "mixed": 2}
+
+FIELDS = {
+    "vel": (0, np.float32, "faces"),
+    "vel_old": (1, np.float32, "faces"),
+    "acc": (2, np.int32, "faces"),
+    "wacc": (3, np.int32, "faces"),
+    "count": (4, np.uint32, "cells"),
+    "type": (5, np.uint32, "cells"),
+    "pressure": (6, np.float32, "cells"),
+    "divergence": (7, np.float32, "cells"),
+    "pressure64": (8, np.float64, "cells"),
+}
+WRITABLE = ("vel", "vel_old", "pressure")
+
+
+class FluidError(RuntimeError):
+    """A status code other than PXLFLIP_OK, with the library's message."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(message)
+        self.code = code
+
+
+class Opts(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_uint32), ("solver", C.c_int32), ("cg_iterations", C.c_uint32),
+        ("inner_iterations", C.c_uint32), ("max_outer", C.c_uint32), ("tolerance", C.c_double),
+        ("device", C.c_int32), ("scene_index", C.c_uint32), ("use_graph", C.c_int32),
+    ]
+
+
+class Info(C.Structure):
+    _fields_ = [
+        ("n", C.c_uint32 * 3), ("h", C.c_float), ("dt", C.c_float), ("flip", C.c_float),
+        ("inv_h", C.c_float), ("origin", C.c_float * 3), ("gravity", C.c_float * 3),
+        ("n_particles", C.c_uint32), ("substeps", C.c_uint32), ("pressure_iterations", C.c_uint32),
+        ("n_faces", C.c_uint64), ("n_cells", C.c_uint64), ("device_bytes", C.c_uint64),
+    ]
+
+    def as_dict(self):
+        return {
+            "n": tuple(int(v) for v in self.n), "h": np.float32(self.h), "dt": np.float32(self.dt),
+            "flip": np.float32(self.flip), "inv_h": np.float32(self.inv_h),
+            "origin": tuple(np.float32(v) for v in self.origin),
+            "gravity": tuple(np.float32(v) for v in self.gravity),
+            "n_particles": int(self.n_particles), "substeps": int(self.substeps),
+            "pressure_iterations": int(self.pressure_iterations), "n_faces": int(self.n_faces),
+            "n_cells": int(self.n_cells), "device_bytes": int(self.device_bytes),
+        }
+
+
+class KernelTime(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("launches", C.c_uint32), ("total_ms", C.c_double)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("rel_residual", C.c_double), ("inner_iterations", C.c_uint32),
+                ("outer_iterations", C.c_uint32)]
+
+
+# Exported symbols with their ctypes signatures (must match include/pxlflip.h).
+_H = C.c_void_p
+_SIGNATURES = {
+    "pxlflip_default_opts": (C.c_int, [C.POINTER(Opts)]),
+    "pxlflip_check_json": (C.c_int, [C.c_char_p, C.c_double, C.c_uint32, C.POINTER(Opts), C.POINTER(Info)]),
+    "pxlflip_seed_particles": (C.c_int, [C.c_char_p, C.c_uint32, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "pxlflip_create_from_json": (C.c_int, [C.c_char_p, C.c_double, C.c_uint32, C.POINTER(Opts), C.POINTER(_H)]),
+    "pxlflip_destroy": (None, [_H]),
+    "pxlflip_advance_to_frame": (C.c_int, [_H, C.c_double]),
+    "pxlflip_step": (C.c_int, [_H, C.c_uint32]),
+    "pxlflip_reset": (C.c_int, [_H]),
+    "pxlflip_particle_count": (C.c_uint32, [_H]),
+    "pxlflip_steps_done": (C.c_uint64, [_H]),
+    "pxlflip_reservation": (C.c_int, [_H, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "pxlflip_get_info": (C.c_int, [_H, C.POINTER(Info)]),
+    "pxlflip_read_particles": (C.c_int, [_H, C.c_void_p]),
+    "pxlflip_write_particles": (C.c_int, [_H, C.c_void_p]),
+    "pxlflip_read_field": (C.c_int, [_H, C.c_int, C.c_void_p]),
+    "pxlflip_write_field": (C.c_int, [_H, C.c_int, C.c_void_p]),
+    "pxlflip_read_bins": (C.c_int, [_H, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pxlflip_grid_phase": (C.c_int, [_H]),
+    "pxlflip_last_solve": (C.c_int, [_H, C.POINTER(SolveStats)]),
+    "pxlflip_profile_steps": (C.c_int, [_H, C.c_uint32, C.POINTER(KernelTime), C.c_uint32, C.POINTER(C.c_uint32)]),
+    "pxlflip_last_error": (C.c_char_p, [_H]),
+}
+
+_lib = None
+
+
+def load_library(path: Optional[str] = None):
+    """Loads the HIP library; raises ImportError when it has not been built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise ImportError(f"{p} is missing: build it with `python -m pxlengine_amd.build` "
+                          "(the FLIP path has no CPU fallback)")
+    lib = C.CDLL(p)
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _err(handle=None) -> str:
+    msg = load_library().pxlflip_last_error(handle)
+    return msg.decode() if msg else ""
+
+
+def _check(rc: int, handle=None):
+    if rc != OK:
+        raise FluidError(rc, _err(handle))
+
+
+def _json(fluid) -> bytes:
+    if isinstance(fluid, (bytes, bytearray)):
+        return bytes(fluid)
+    if isinstance(fluid, str):
+        return fluid.encode()
+    return json.dumps(fluid).encode()
+
+
+def make_opts(solver="rbgs", cg_iterations=0, inner_iterations=40, max_outer=50, tolerance=1e-6,
+              device=0, scene_index=0, use_graph=True) -> Opts:
+    o = Opts()
+    load_library().pxlflip_default_opts(C.byref(o))
+    o.solver = SOLVERS[solver] if isinstance(solver, str) else int(solver)
+    o.cg_iterations = cg_iterations
+    o.inner_iterations = inner_iterations
+    o.max_outer = max_outer
+    o.tolerance = tolerance
+    o.device = device
+    o.scene_index = scene_index
+    o.use_graph = 1 if use_graph else 0
+    return o
+
+
+def check(fluid, fps: float = 24.0, seed: int = 0, **opts) -> dict:
+    """Parse + validate a fluid block without touching the GPU (Scene::validate's fluid
+    checks, crates/pxl-scene/src/lib.rs:710-735, plus Fluid::grid and the particle count)."""
+    info = Info()
+    o = make_opts(**opts)
+    _check(load_library().pxlflip_check_json(_json(fluid), fps, seed, C.byref(o), C.byref(info)))
+    return info.as_dict()
+
+
+def seed_particles(fluid, seed: int = 0) -> np.ndarray:
+    """Initial particles as (P, 8) float32 records (fluid.rs:123-148), computed on the host."""
+    L = load_library()
+    cnt = C.c_uint64()
+    _check(L.pxlflip_seed_particles(_json(fluid), seed, None, 0, C.byref(cnt)))
+    out = np.zeros((cnt.value, 8), np.float32)
+    _check(L.pxlflip_seed_particles(_json(fluid), seed, out.ctypes.data, cnt.value, C.byref(cnt)))
+    return out
+
+
+class FluidState:
+    """One simulated liquid resident in HBM (crates/pxl-render/src/fluid.rs:74-94)."""
+
+    def __init__(self, fluid, fps: float = 24.0, seed: int = 0, **opts):
+        self._lib = load_library()
+        self._h = C.c_void_p()
+        o = make_opts(**opts)
+        _check(self._lib.pxlflip_create_from_json(_json(fluid), fps, seed, C.byref(o), C.byref(self._h)))
+        info = Info()
+        _check(self._lib.pxlflip_get_info(self._h, C.byref(info)), self._h)
+        self.info = info.as_dict()
+        self.fps = fps
+        self.seed = seed
+
+    # -- lifetime
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            self._lib.pxlflip_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _call(self, fn, *args):
+        if not self._h.value:
+            raise FluidError(ERR_INVALID, "handle is closed")
+        _check(fn(self._h, *args), self._h)
+
+    # -- reference interface
+    def particle_count(self) -> int:
+        return int(self._lib.pxlflip_particle_count(self._h))
+
+    def reservation(self):
+        v, i = C.c_uint32(), C.c_uint32()
+        self._call(self._lib.pxlflip_reservation, C.byref(v), C.byref(i))
+        return int(v.value), int(i.value)
+
+    def advance(self, frame: float):
+        self._call(self._lib.pxlflip_advance_to_frame, float(frame))
+
+    @property
+    def steps_done(self) -> int:
+        return int(self._lib.pxlflip_steps_done(self._h))
+
+    # -- build extensions
+    def step(self, n: int = 1):
+        self._call(self._lib.pxlflip_step, int(n))
+
+    def reset(self):
+        self._call(self._lib.pxlflip_reset)
+
+    def grid_phase(self):
+        self._call(self._lib.pxlflip_grid_phase)
+
+    def read_particles(self) -> np.ndarray:
+        out = np.empty((self.info["n_particles"], 8), np.float32)
+        self._call(self._lib.pxlflip_read_particles, out.ctypes.data)
+        return out
+
+    def write_particles(self, aos8: np.ndarray):
+        a = np.ascontiguousarray(aos8, np.float32)
+        if a.shape != (self.info["n_particles"], 8):
+            raise ValueError(f"expected ({self.info['n_particles']}, 8) records, got {a.shape}")
+        self._call(self._lib.pxlflip_write_particles, a.ctypes.data)
+
+    def read_field(self, name: str) -> np.ndarray:
+        code, dtype, kind = FIELDS[name]
+        n = self.info["n_faces"] if kind == "faces" else self.info["n_cells"]
+        out = np.empty(n, dtype)
+        self._call(self._lib.pxlflip_read_field, code, out.ctypes.data)
+        return out
+
+    def write_field(self, name: str, values: np.ndarray):
+        if name not in WRITABLE:
+            raise ValueError(f"{name} is not writable")
+        code, dtype, kind = FIELDS[name]
+        n = self.info["n_faces"] if kind == "faces" else self.info["n_cells"]
+        a = np.ascontiguousarray(values, dtype)
+        if a.shape != (n,):
+            raise ValueError(f"expected {n} values, got {a.shape}")
+        self._call(self._lib.pxlflip_write_field, code, a.ctypes.data)
+
+    def read_bins(self):
+        P, Cc = self.info["n_particles"], self.info["n_cells"]
+        cell_of = np.empty(P, np.uint32)
+        counts = np.empty(Cc, np.uint32)
+        offsets = np.empty(Cc, np.uint32)
+        ids = np.empty(P, np.uint32)
+        self._call(self._lib.pxlflip_read_bins, cell_of.ctypes.data, counts.ctypes.data, offsets.ctypes.data,
+                   ids.ctypes.data)
+        return cell_of, counts, offsets, ids
+
+    def profile_steps(self, n: int = 1) -> list:
+        """Runs n steps with every launch timed by HIP events; [(name, launches, total_ms)]."""
+        arr = (KernelTime * 64)()
+        cnt = C.c_uint32()
+        self._call(self._lib.pxlflip_profile_steps, int(n), arr, 64, C.byref(cnt))
+        return [(arr[k].name.decode(), int(arr[k].launches), float(arr[k].total_ms))
+                for k in range(min(cnt.value, 64))]
+
+    def last_solve(self) -> dict:
+        st = SolveStats()
+        self._call(self._lib.pxlflip_last_solve, C.byref(st))
+        return {"rel_residual": st.rel_residual, "inner_iterations": int(st.inner_iterations),
+                "outer_iterations": int(st.outer_iterations)}

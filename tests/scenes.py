@@ -1,0 +1,59 @@
+"""Fluid blocks used by the tests and the bench, and an oracle simulation built from one.
+
+The oracle side derives everything itself (grid, dt, seeding) from the block, so a product
+mistake in parsing or seeding shows up as a mismatch instead of being copied over.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+FPS = 24.0
+
+
+def dam_break(res: int, substeps: int = 5, flip: float = 0.95, ppc: int = 8, iters: int = 80,
+              fill_max=(0.4, 0.6, 1.0)) -> dict:
+    """The dam break of the configs: unit cube, liquid block x < 0.4, y < 0.6, at rest.
+    substeps 5 at 24 fps gives dt = f32(1/120)."""
+    return {
+        "name": f"dam{res}", "bounds": [[0, 0, 0], [1, 1, 1]], "resolution": res,
+        "fill": [{"min": [0, 0, 0], "max": list(fill_max)}],
+        "particles_per_cell": ppc, "substeps": substeps, "flip": flip,
+        "gravity": [0, -9.81, 0], "pressure_iterations": iters,
+    }
+
+
+# examples/dam-break.json's fluid block (the reference's only fluid scene).
+EXAMPLE_DAM_BREAK = {
+    "name": "water", "bounds": [[-1, 0, -0.5], [1, 1.2, 0.5]], "resolution": 64,
+    "fill": [{"min": [-0.97, 0.03, -0.47], "max": [-0.35, 0.95, 0.47]}],
+    "substeps": 8, "flip": 0.95,
+    "materials": [{"inputs": {"transmission_weight": 1, "specular_ior": 1.33, "specular_roughness": 0,
+                              "transmission_color": [0.45, 0.85, 0.92], "transmission_depth": 0.8}}],
+}
+
+
+def two_blocks(res: int, ppc: int = 8, iters: int = 40) -> dict:
+    """Two liquid blocks moving toward each other (mixed-precision config shape)."""
+    return {
+        "name": f"collide{res}", "bounds": [[0, 0, 0], [1, 1, 1]], "resolution": res,
+        "fill": [{"min": [0.02, 0.02, 0.02], "max": [0.35, 0.7, 0.98], "velocity": [1.5, 0, 0]},
+                 {"min": [0.65, 0.02, 0.02], "max": [0.98, 0.7, 0.98], "velocity": [-1.5, 0, 0]}],
+        "particles_per_cell": ppc, "substeps": 5, "flip": 0.95, "pressure_iterations": iters,
+    }
+
+
+def oracle_sim(orc, fluid: dict, fps: float = FPS, seed: int = 0):
+    """Builds the oracle simulation for a fluid block (defaults as crates/pxl-scene/src/lib.rs:96-119)."""
+    lo, hi = fluid["bounds"]
+    n, h = orc.fluid_grid(lo, hi, fluid.get("resolution", 64))
+    blocks = [(b["min"], b["max"], b.get("velocity", [0, 0, 0])) for b in fluid["fill"]]
+    x, v = orc.seed_particles(n, h, lo, blocks, fluid.get("particles_per_cell", 8), seed)
+    dt = orc.dt_for(fps, fluid.get("substeps", 8))
+    return orc.Sim(n, h, lo, dt, fluid.get("flip", 0.95), fluid.get("gravity", [0, -9.81, 0]), x, v)
+
+
+def aos8(x: np.ndarray, v: np.ndarray) -> np.ndarray:
+    out = np.zeros((len(x), 8), np.float32)
+    out[:, 0:3] = x
+    out[:, 4:7] = v
+    return out
