@@ -1,0 +1,156 @@
+"""C ABI on the CPU: the library loads, exports every declared entry point, and its host half
+(strict fluid-block parsing, Scene::validate's fluid checks, Fluid::grid, seeding) behaves
+like the reference -- none of this touches a GPU."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from tests.scenes import EXAMPLE_DAM_BREAK, dam_break
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def pxl():
+    import pxlengine_amd
+    pxlengine_amd.load_library()
+    return pxlengine_amd
+
+
+def _declared_symbols():
+    text = open(os.path.join(ROOT, "include", "pxlflip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pxlflip_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol(pxl):
+    lib = ctypes.CDLL(pxl.flip.LIB_PATH)
+    names = _declared_symbols()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+    # and the Python binding declares a signature for each of them
+    assert set(names) == set(pxl.flip._SIGNATURES)
+
+
+def _err(pxl, fluid, **kw):
+    with pytest.raises(pxl.FluidError) as e:
+        pxl.check(fluid, **kw)
+    return e.value
+
+
+def test_fluids_validate_reference_case(pxl):
+    """crates/pxl-scene/src/lib.rs:886-899."""
+    ok = {"name": "w", "bounds": [[0, 0, 0], [2, 1, 1]], "resolution": 32,
+          "fill": [{"min": [0.1, 0.1, 0.1], "max": [0.5, 0.5, 0.5]}]}
+    info = pxl.check(ok)
+    assert info["n"] == (32, 16, 16)
+    assert abs(info["h"] - 0.0625) < 1e-6
+    outside = dict(ok, fill=[{"min": [0.1, 0.1, 0.1], "max": [0.5, 0.5, 1.5]}])
+    e = _err(pxl, outside)
+    assert "inside" in str(e) and e.code == pxl.flip.ERR_INVALID
+    assert str(e) == "fluids[0] 'w': each fill block must have max > min and lie inside `bounds`"
+
+
+def test_defaults_match_reference(pxl):
+    """Defaults of crates/pxl-scene/src/lib.rs:96-119 (resolution 64, 8 ppc, 8 substeps,
+    80 pressure iterations, gravity -9.81 y) and dt = f32(1/(fps*substeps))."""
+    info = pxl.check({"name": "d", "bounds": [[0, 0, 0], [1, 1, 1]], "fill": [{"min": [0, 0, 0], "max": [0.5, 0.5, 0.5]}]})
+    assert info["n"] == (64, 64, 64)
+    assert info["substeps"] == 8 and info["pressure_iterations"] == 80
+    assert info["gravity"] == (0.0, np.float32(-9.81), 0.0)
+    assert info["flip"] == np.float32(0.95)
+    assert info["dt"] == np.float32(1 / (24 * 8))
+    nx, ny, nz = info["n"]
+    assert info["n_faces"] == (nx + 1) * ny * nz + nx * (ny + 1) * nz + nx * ny * (nz + 1)
+    assert info["n_cells"] == nx * ny * nz
+
+
+@pytest.mark.parametrize("change,message", [
+    ({"resolution": 8}, "resolution 16..256, particles_per_cell 1..64, substeps 1..64, surface_refine 1..3, flip 0..1"),
+    ({"resolution": 300}, "resolution 16..256"),
+    ({"particles_per_cell": 0}, "particles_per_cell 1..64"),
+    ({"substeps": 65}, "substeps 1..64"),
+    ({"surface_refine": 4}, "surface_refine 1..3"),
+    ({"flip": 1.5}, "flip 0..1"),
+    ({"pressure_iterations": 0}, "flip 0..1"),
+    ({"fill": []}, "needs at least one `fill` block of liquid"),
+    ({"bounds": [[0, 0, 0], [1, 0, 1]]}, "bounds must be [min, max] with max > min on every axis"),
+    ({"fill": [{"min": [0.5, 0, 0], "max": [0.4, 1, 1]}]}, "each fill block must have max > min"),
+])
+def test_validation_messages(pxl, change, message):
+    f = dict(dam_break(32), **change)
+    e = _err(pxl, f)
+    assert message in str(e)
+    assert str(e).startswith("fluids[0] 'dam32': ")
+
+
+def test_grid_too_large(pxl):
+    # 256 x 256 x 256 = 16.7M cells > 16,000,000 (lib.rs:731-734)
+    f = dict(dam_break(256))
+    assert "grid too large (lower resolution)" in str(_err(pxl, f))
+    f["bounds"] = [[0, 0, 0], [1, 1, 0.9]]
+    f["fill"] = [{"min": [0, 0, 0], "max": [0.4, 0.6, 0.9]}]
+    assert pxl.check(f)["n"] == (256, 256, 230)
+
+
+def test_strict_json(pxl):
+    """deny_unknown_fields (lib.rs:50-51, 88): typos are rejected by name (cf. the reference
+    test rejects_typos); required fields and types are enforced."""
+    e = _err(pxl, dict(dam_break(32), resolutoin=32))
+    assert "unknown field `resolutoin`" in str(e)
+    e = _err(pxl, {"name": "x", "bounds": [[0, 0, 0], [1, 1, 1]], "fill": [{"min": [0, 0, 0], "max": [1, 1, 1], "vel": [0, 0, 0]}]})
+    assert "unknown field `vel`" in str(e)
+    assert "missing field `name`" in str(_err(pxl, {"bounds": [[0, 0, 0], [1, 1, 1]], "fill": []}))
+    assert "missing field `fill`" in str(_err(pxl, {"name": "x", "bounds": [[0, 0, 0], [1, 1, 1]]}))
+    assert "expected u32" in str(_err(pxl, dict(dam_break(32), resolution=32.5)))
+    assert "expected u32" in str(_err(pxl, dict(dam_break(32), resolution=-1)))
+    assert "invalid length 2" in str(_err(pxl, dict(dam_break(32), gravity=[0, 1])))
+    assert "duplicate field" in str(_err(pxl, '{"name":"a","name":"b","bounds":[[0,0,0],[1,1,1]],"fill":[]}'))
+    assert "invalid fluid" in str(_err(pxl, '{"name": "a", '))
+    # the rendering block of examples/dam-break.json is accepted
+    assert pxl.check(EXAMPLE_DAM_BREAK)["n_particles"] == 139200
+
+
+def test_fps_and_empty_fill(pxl):
+    assert str(_err(pxl, dam_break(32), fps=0.0)) == "render.fps must be > 0"
+    # a fill block inside bounds that contains no cell centre (fluid.rs:149)
+    f = dict(dam_break(32), fill=[{"min": [0.5, 0.5, 0.5], "max": [0.51, 0.51, 0.51]}])
+    assert str(_err(pxl, f)) == "fluid 'dam32': fill blocks contain no grid cells"
+
+
+@pytest.mark.parametrize("fluid,seed", [(dam_break(24, ppc=3), 0), (EXAMPLE_DAM_BREAK, 1)], ids=["dam24", "example"])
+def test_host_seeding_matches_oracle(pxl, orc, fluid, seed):
+    from tests.scenes import oracle_sim
+    a = pxl.seed_particles(fluid, seed)
+    sim = oracle_sim(orc, fluid, seed=seed)
+    assert a.shape == (sim.P, 8)
+    np.testing.assert_array_equal(a[:, 0:3], sim.x)
+    np.testing.assert_array_equal(a[:, 4:7], sim.v)
+    assert not a[:, 3].any() and not a[:, 7].any()
+
+
+def test_multiple_blocks_with_velocity(pxl, orc):
+    from tests.scenes import oracle_sim, two_blocks
+    f = two_blocks(20)
+    a = pxl.seed_particles(f, 4)
+    sim = oracle_sim(orc, f, seed=4)
+    np.testing.assert_array_equal(a[:, 0:3], sim.x)
+    np.testing.assert_array_equal(a[:, 4:7], sim.v)
+    assert set(np.unique(a[:, 4])) == {np.float32(1.5), np.float32(-1.5)}
+
+
+def test_device_bytes_reported(pxl):
+    small = pxl.check(dam_break(32))["device_bytes"]
+    mixed = pxl.check(dam_break(32), solver="mixed")["device_bytes"]
+    assert 0 < small < mixed
+
+
+def test_opts_validation(pxl):
+    with pytest.raises(pxl.FluidError):
+        pxl.check(dam_break(32), solver=7)
+    with pytest.raises(pxl.FluidError):
+        pxl.check(dam_break(32), solver="mixed", tolerance=0.0)
