@@ -38,9 +38,9 @@ def algorithmic_bytes(name: str, P: int, F: int, C: int, Cf: int) -> float:
         "faces": 16 * F,
         "cells": 4 * C + 4 * F + 7 * C + 4 * (C - Cf),
         "rbgs": C + 8 * Cf,
-        "cg_init": 2 * C + 4 * Cf + 4 * Cf + 8 * Cf,
-        "cg_dir_spmv": 2 * C + 16 * Cf,
-        "cg_update": 2 * C + 24 * Cf,
+        "cg_start": 2 * C + 4 * Cf + 4 * Cf + 4 * Cf + 2 * C + 4 * Cf + 12 * Cf,
+        "cg_iter": 2 * C + 40 * Cf,
+        "cg_finalize": 0,
         "project": 8 * F + 5 * C,
         "g2p": 28 * P + 8 * F + 28 * P,
     }

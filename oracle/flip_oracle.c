@@ -413,48 +413,72 @@ static float apply32(const orc_state *s, const float *x, const int c[3], uint32_
     return cell_diag(s, c) * x[id] - sum;
 }
 
+static inline float precond32(float r, float dg) { return dg > 0.0f ? r / dg : 0.0f; }
+
 /* Jacobi-preconditioned CG on A p = b, b = -((div*h)*h)/dt, warm-started from pressure,
- * exactly `iters` iterations (an iteration with d.q == 0 or rz == 0 leaves x unchanged). */
+ * exactly `iters` iterations, in the Chronopoulos-Gear arrangement (one pair of dot
+ * products per iteration):
+ *   r0 = b - A x, u0 = r0 / diag, w0 = A u0, gamma = (r,u), delta = (w,u),
+ *   alpha = gamma / delta, beta = 0; then per iteration
+ *   p = u + beta p, s = w + beta s, x += alpha p, r -= alpha s, u = r / diag, w = A u,
+ *   beta = gamma' / gamma, alpha = gamma' / (delta' - beta gamma' / alpha).
+ * Dot products are fp32 sums in cell order.  Guards: a zero gamma, old gamma or alpha leaves
+ * alpha = beta = 0 (no further change). */
 void orc_pressure_cg32(orc_state *s, uint32_t iters) {
     const orc_params *p = &s->p;
     uint32_t C = (uint32_t)s->n_cells;
-    float *r = calloc(C, 4), *d = calloc(C, 4), *q = calloc(C, 4), *dg = calloc(C, 4);
+    float *r = calloc(C, 4), *w = calloc(C, 4), *sv = calloc(C, 4), *pv = calloc(C, 4), *dg = calloc(C, 4);
+    float *u = calloc(C, 4);
     float *x = s->pressure;
-    float rz = 0.0f;
     for (uint32_t id = 0; id < C; id++) {
         if (s->type[id] != ORC_FLUID) { x[id] = 0.0f; continue; }
         int c[3] = {(int)(id % p->nx), (int)((id / p->nx) % p->ny), (int)(id / (p->nx * p->ny))};
         dg[id] = cell_diag(s, c);
+    }
+    for (uint32_t id = 0; id < C; id++) {
+        if (s->type[id] != ORC_FLUID) continue;
+        int c[3] = {(int)(id % p->nx), (int)((id / p->nx) % p->ny), (int)(id / (p->nx * p->ny))};
         float b = -(s->div[id] * p->h * p->h / p->dt);
         r[id] = b - apply32(s, x, c, id);
     }
-    for (uint32_t id = 0; id < C; id++)
-        if (dg[id] > 0.0f) rz = rz + r[id] * (r[id] / dg[id]);
-    float beta = 0.0f;
+    float gamma = 0.0f, delta = 0.0f;
+    for (uint32_t id = 0; id < C; id++) u[id] = s->type[id] == ORC_FLUID ? precond32(r[id], dg[id]) : 0.0f;
+    for (uint32_t id = 0; id < C; id++) {
+        if (s->type[id] != ORC_FLUID) continue;
+        int c[3] = {(int)(id % p->nx), (int)((id / p->nx) % p->ny), (int)(id / (p->nx * p->ny))};
+        w[id] = apply32(s, u, c, id);
+        gamma = gamma + r[id] * u[id];
+        delta = delta + w[id] * u[id];
+    }
+    float alpha = (gamma != 0.0f && delta != 0.0f) ? gamma / delta : 0.0f, beta = 0.0f;
     for (uint32_t it = 0; it < iters; it++) {
         for (uint32_t id = 0; id < C; id++) {
-            float z = dg[id] > 0.0f ? r[id] / dg[id] : 0.0f;
-            d[id] = z + beta * d[id];
+            if (s->type[id] != ORC_FLUID) continue;
+            pv[id] = precond32(r[id], dg[id]) + beta * pv[id];
+            sv[id] = w[id] + beta * sv[id];
+            x[id] = x[id] + alpha * pv[id];
+            r[id] = r[id] - alpha * sv[id];
         }
-        float dq = 0.0f;
-        for (uint32_t id = 0; id < C; id++) {
-            if (s->type[id] != ORC_FLUID) { q[id] = 0.0f; continue; }
-            int c[3] = {(int)(id % p->nx), (int)((id / p->nx) % p->ny), (int)(id / (p->nx * p->ny))};
-            q[id] = apply32(s, d, c, id);
-            dq = dq + d[id] * q[id];
-        }
-        float alpha = (dq != 0.0f && rz != 0.0f) ? rz / dq : 0.0f;
-        float rz_new = 0.0f;
+        for (uint32_t id = 0; id < C; id++) u[id] = s->type[id] == ORC_FLUID ? precond32(r[id], dg[id]) : 0.0f;
+        float g = 0.0f, d = 0.0f;
         for (uint32_t id = 0; id < C; id++) {
             if (s->type[id] != ORC_FLUID) continue;
-            x[id] = x[id] + alpha * d[id];
-            r[id] = r[id] - alpha * q[id];
-            if (dg[id] > 0.0f) rz_new = rz_new + r[id] * (r[id] / dg[id]);
+            int c[3] = {(int)(id % p->nx), (int)((id / p->nx) % p->ny), (int)(id / (p->nx * p->ny))};
+            w[id] = apply32(s, u, c, id);
+            g = g + r[id] * u[id];
+            d = d + w[id] * u[id];
         }
-        beta = rz != 0.0f ? rz_new / rz : 0.0f;
-        rz = rz_new;
+        float a_new = 0.0f, b_new = 0.0f;
+        if (g != 0.0f && gamma != 0.0f && alpha != 0.0f) {
+            b_new = g / gamma;
+            float den = d - b_new * g / alpha;
+            a_new = den != 0.0f ? g / den : 0.0f;
+        }
+        alpha = a_new;
+        beta = b_new;
+        gamma = g;
     }
-    free(r); free(d); free(q); free(dg);
+    free(r); free(w); free(sv); free(pv); free(dg); free(u);
 }
 
 void orc_step_cg(orc_state *s, uint32_t iters) {

@@ -115,8 +115,8 @@ struct Sim {
     using Plan = std::vector<std::pair<void **, size_t>>;
     // Every device array of the simulation, in arena order.
     Plan plan() {
-        const size_t P = d.np, F = d.nfaces, C = d.ncells;
-        const size_t nb = d.nblocks_cells;
+        const size_t P = d.np, F = d.nfaces, C = d.ncells, K = d.nkeys;
+        const size_t nb = solver_blocks(d);
         Plan plan;
         auto add = [&](auto **p, size_t bytes) { plan.emplace_back(reinterpret_cast<void **>(p), bytes); };
         for (float **p : {&d.ax, &d.ay, &d.az, &d.avx, &d.avy, &d.avz}) add(p, 4 * P);
@@ -130,13 +130,13 @@ struct Sim {
         add(&d.vel_old, 4 * F);
         add(&d.acc, 4 * F);
         add(&d.wacc, 4 * F);
-        add(&d.count, 4 * C);
-        add(&d.offset, 4 * C);
+        add(&d.count, 4 * K);
+        add(&d.offset, 4 * (K + 1));
         add(&d.type, C);
         add(&d.sten, 2 * C);
         add(&d.pressure, 4 * C);
         add(&d.div, 4 * C);
-        for (float **p : {&d.r, &d.d0, &d.d1, &d.q}) add(p, 4 * C);
+        for (float **p : {&d.cg_r0, &d.cg_r1, &d.cg_w0, &d.cg_w1, &d.cg_s0, &d.cg_s1, &d.cg_p}) add(p, 4 * C);
         if (opts.solver == PXLFLIP_SOLVER_MIXED) {
             add(&d.e, 4 * C);
             add(&d.x64a, 8 * C);
@@ -147,7 +147,7 @@ struct Sim {
         add(&d.part64, 8 * 2 * nb);
         add(&d.scal, 8 * SC_COUNT);
         add(&d.ctr, 64);
-        add(&d.scan_tmp, 4 * ((C + 4095) / 4096 + 1));
+        add(&d.scan_tmp, 4 * (scan_tiles(K) + 1));
         return plan;
     }
     static size_t plan_total(const Plan &plan) {
@@ -171,6 +171,9 @@ struct Sim {
             cur += (e.second + ALIGN - 1) / ALIGN * ALIGN;
         }
         HIP_CHECK(hipMemsetAsync(arena, 0, arena_bytes, stream));
+        // the end of the last tile's particle run (the scan fills offset[0 .. nkeys-1])
+        const uint32_t np32 = d.np;
+        HIP_CHECK(hipMemcpyAsync(d.offset + d.nkeys, &np32, 4, hipMemcpyHostToDevice, stream));
         // initial particles (reset source)
         float *dst[6] = {d.ix, d.iy, d.iz, d.ivx, d.ivy, d.ivz};
         for (int a = 0; a < 3; a++) {
@@ -198,10 +201,11 @@ struct Sim {
                 KRUN("rbgs", launch_rbgs(d, 1, stream));
             }
         } else {
-            KRUN("cg_init", launch_cg_init32(d, stream));
+            KRUN("cg_start", launch_cg_start(d, false, d.pressure, false, stream));
+            KRUN("cg_finalize", launch_cg_finalize(d, false, true, stream));
             for (uint32_t it = 0; it < iters; it++) {
-                KRUN("cg_dir_spmv", launch_cg_pass_a(d, false, it, stream));
-                KRUN("cg_update", launch_cg_pass_b(d, false, d.pressure, it, stream));
+                KRUN("cg_iter", launch_cg_iter(d, false, d.pressure, it, stream));
+                KRUN("cg_finalize", launch_cg_finalize(d, false, false, stream));
             }
         }
     }
@@ -215,10 +219,11 @@ struct Sim {
         enqueue_post();
     }
     void enqueue_inner(int cur) {
-        KRUN("inner_init", launch_inner_init(d, stream));
+        KRUN("cg64_start", launch_cg_start(d, true, d.e, true, stream));
+        KRUN("cg64_finalize", launch_cg_finalize(d, true, true, stream));
         for (uint32_t it = 0; it < opts.inner_iterations; it++) {
-            KRUN("cg64_dir_spmv", launch_cg_pass_a(d, true, it, stream));
-            KRUN("cg64_update", launch_cg_pass_b(d, true, d.e, it, stream));
+            KRUN("cg64_iter", launch_cg_iter(d, true, d.e, it, stream));
+            KRUN("cg64_finalize", launch_cg_finalize(d, true, false, stream));
         }
         KRUN("mixed_update", launch_mixed_update(d, x64(cur), x64(cur ^ 1), stream));
     }
@@ -450,7 +455,10 @@ int prepare(const char *fluid_json, double fps, uint32_t seed, const pxlflip_opt
     d.nw = n[0] * n[1] * (n[2] + 1);
     d.nfaces = d.nu + d.nv + d.nw;
     d.ncells = n[0] * n[1] * n[2];
-    d.nblocks_cells = ((n[0] + 63) / 64) * ((n[1] + 3) / 4) * n[2];
+    d.ntx = (int)((n[0] + TILE - 1) / TILE);
+    d.nty = (int)((n[1] + TILE - 1) / TILE);
+    d.ntz = (int)((n[2] + TILE - 1) / TILE);
+    d.nkeys = (uint32_t)d.ntx * (uint32_t)d.nty * (uint32_t)d.ntz * TILE_CELLS;
     return PXLFLIP_OK;
 }
 
@@ -639,7 +647,7 @@ int pxlflip_read_field(pxlflip *h, int field, void *out) {
         case PXLFLIP_FIELD_VEL_OLD: src = d.vel_old; bytes = 4ull * d.nfaces; break;
         case PXLFLIP_FIELD_ACC: src = d.acc; bytes = 4ull * d.nfaces; break;
         case PXLFLIP_FIELD_WACC: src = d.wacc; bytes = 4ull * d.nfaces; break;
-        case PXLFLIP_FIELD_COUNT: src = d.count; bytes = 4ull * d.ncells; break;
+        case PXLFLIP_FIELD_COUNT: bytes = 4ull * d.ncells; break;
         case PXLFLIP_FIELD_TYPE: bytes = 4ull * d.ncells; break;
         case PXLFLIP_FIELD_PRESSURE: src = d.pressure; bytes = 4ull * d.ncells; break;
         case PXLFLIP_FIELD_DIVERGENCE: src = d.div; bytes = 4ull * d.ncells; break;
@@ -651,9 +659,10 @@ int pxlflip_read_field(pxlflip *h, int field, void *out) {
         default: return pxl::fail(s, PXLFLIP_ERR_INVALID, "unknown field " + std::to_string(field));
     }
     return pxl::guarded(s, [&] {
-        if (field == PXLFLIP_FIELD_TYPE) {
+        if (field == PXLFLIP_FIELD_TYPE || field == PXLFLIP_FIELD_COUNT) {
             pxl::DeviceBuffer buf(bytes);
-            pxl::launch_type_u32(d, buf.as<uint32_t>(), s->stream);
+            if (field == PXLFLIP_FIELD_TYPE) pxl::launch_type_u32(d, buf.as<uint32_t>(), s->stream);
+            else pxl::launch_count_linear(d, buf.as<uint32_t>(), s->stream);
             HIP_CHECK(hipMemcpyAsync(out, buf.p, bytes, hipMemcpyDeviceToHost, s->stream));
         } else {
             if (field == PXLFLIP_FIELD_PRESSURE64) src = s->x64(s->x64_cur);
@@ -687,15 +696,20 @@ int pxlflip_read_bins(pxlflip *h, uint32_t *cell_of, uint32_t *counts, uint32_t 
     return pxl::guarded(s, [&] {
         if (!s->initialized) s->reset();
         const pxl::DevSim &d = s->d;
-        pxl::DeviceBuffer cbuf(4ull * d.np), sbuf(4ull * d.np);
-        pxl::launch_fill_u32(d.count, d.ncells, 0, s->stream);
+        const size_t C = d.ncells;
+        pxl::DeviceBuffer cbuf(4ull * d.np), sbuf(4ull * d.np), cnt(4 * C), off(4 * C),
+            tmp(4ull * (pxl::scan_tiles((uint32_t)C) + 1));
+        // the step's binning (tiled keys) on the current positions ...
+        pxl::launch_fill_u32(d.count, d.nkeys, 0, s->stream);
         pxl::launch_bin_count(d, s->stream);
-        pxl::launch_scan_offsets(d, s->stream);
-        pxl::launch_bins_out(d, cbuf.as<uint32_t>(), sbuf.as<uint32_t>(), s->stream);
-        pxl::launch_sort_segments(d, sbuf.as<uint32_t>(), s->stream);
+        // ... reported in reference cell numbering
+        pxl::launch_count_linear(d, cnt.as<uint32_t>(), s->stream);
+        pxl::launch_exclusive_scan(cnt.as<uint32_t>(), (uint32_t)C, tmp.as<uint32_t>(), off.as<uint32_t>(), s->stream);
+        pxl::launch_bins_out(d, off.as<uint32_t>(), cbuf.as<uint32_t>(), sbuf.as<uint32_t>(), s->stream);
+        pxl::launch_sort_segments((uint32_t)C, off.as<uint32_t>(), cnt.as<uint32_t>(), sbuf.as<uint32_t>(), s->stream);
         if (cell_of) HIP_CHECK(hipMemcpyAsync(cell_of, cbuf.p, 4ull * d.np, hipMemcpyDeviceToHost, s->stream));
-        if (counts) HIP_CHECK(hipMemcpyAsync(counts, d.count, 4ull * d.ncells, hipMemcpyDeviceToHost, s->stream));
-        if (offsets) HIP_CHECK(hipMemcpyAsync(offsets, d.offset, 4ull * d.ncells, hipMemcpyDeviceToHost, s->stream));
+        if (counts) HIP_CHECK(hipMemcpyAsync(counts, cnt.p, 4 * C, hipMemcpyDeviceToHost, s->stream));
+        if (offsets) HIP_CHECK(hipMemcpyAsync(offsets, off.p, 4 * C, hipMemcpyDeviceToHost, s->stream));
         if (sorted_ids) HIP_CHECK(hipMemcpyAsync(sorted_ids, sbuf.p, 4ull * d.np, hipMemcpyDeviceToHost, s->stream));
         HIP_CHECK(hipStreamSynchronize(s->stream));
     });

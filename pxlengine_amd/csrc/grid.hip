@@ -55,7 +55,7 @@ __global__ __launch_bounds__(TX *TY) void k_faces(DevSim S) {
 
 __device__ __forceinline__ uint8_t type_from_count(const DevSim &S, int i, int j, int k) {
     if (is_solid(S, i, j, k)) return CELL_SOLID;
-    return S.count[cidx(S, i, j, k)] > 0 ? CELL_FLUID : CELL_AIR;
+    return S.count[tile_key(S, i, j, k)] > 0 ? CELL_FLUID : CELL_AIR;
 }
 
 // normalize_grid cells + compute_divergence (fluid.slang:125-131, 154-165), plus the stencil

@@ -19,14 +19,13 @@ constexpr int SCAN_TILE = BLOCK * SCAN_ITEMS;  // 4096 cells per scan workgroup
 inline unsigned blocks_for(uint64_t n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
 
 // Cell of a position: clamp(floor((x - origin) * inv_h), 0, n - 1) (fluid.slang:113).
-__device__ __forceinline__ uint32_t cell_of_pos(const DevSim &S, float x, float y, float z) {
-    int cx = (int)floorf((x - S.ox) * S.inv_h);
-    int cy = (int)floorf((y - S.oy) * S.inv_h);
-    int cz = (int)floorf((z - S.oz) * S.inv_h);
+__device__ __forceinline__ void cell_of_pos(const DevSim &S, float x, float y, float z, int &cx, int &cy, int &cz) {
+    cx = (int)floorf((x - S.ox) * S.inv_h);
+    cy = (int)floorf((y - S.oy) * S.inv_h);
+    cz = (int)floorf((z - S.oz) * S.inv_h);
     cx = cx < 0 ? 0 : (cx > S.nx - 1 ? S.nx - 1 : cx);
     cy = cy < 0 ? 0 : (cy > S.ny - 1 ? S.ny - 1 : cy);
     cz = cz < 0 ? 0 : (cz > S.nz - 1 ? S.nz - 1 : cz);
-    return (uint32_t)cx + (uint32_t)S.nx * ((uint32_t)cy + (uint32_t)S.ny * (uint32_t)cz);
 }
 
 // ---------------------------------------------------------------- clear / binning
@@ -36,27 +35,47 @@ __global__ __launch_bounds__(BLOCK) void k_fill_u32(uint32_t *p, uint32_t n, uin
     if (i < n) p[i] = value;
 }
 
-// clear_grid (fluid.slang:88-91): acc, wacc over faces, count over cells.
+// clear_grid (fluid.slang:88-91): acc, wacc over faces, count over the (tiled) cells.
 __global__ __launch_bounds__(BLOCK) void k_clear_grid(DevSim S) {
     uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i < S.nfaces) {
         S.acc[i] = 0;
         S.wacc[i] = 0;
     }
-    if (i < S.ncells) S.count[i] = 0;
+    if (i < S.nkeys) S.count[i] = 0;
 }
 
-// Counting-sort pass 1: cell key of each particle and its arrival slot in that cell
-// (the per-cell count of p2g, fluid.slang:113-114).
+// Counting-sort pass 1: tiled cell key of each particle and its arrival slot in that cell
+// (the per-cell count of p2g, fluid.slang:113-114).  Particles arrive in last step's cell
+// order, so neighbouring lanes mostly share a key: each run of equal keys in a wave64 takes
+// its slots with ONE atomic by the run's first lane.
 __global__ __launch_bounds__(BLOCK) void k_bin_count(DevSim S) {
-    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= S.np) return;
-    uint32_t c = cell_of_pos(S, S.ax[i], S.ay[i], S.az[i]);
-    S.key[i] = c;
-    S.slot[i] = atomicAdd(&S.count[c], 1u);
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    const bool valid = i < S.np;
+    uint32_t key = 0xFFFFFFFFu;
+    if (valid) {
+        int cx, cy, cz;
+        cell_of_pos(S, S.ax[i], S.ay[i], S.az[i], cx, cy, cz);
+        key = tile_key(S, cx, cy, cz);
+    }
+    const int lane = threadIdx.x & (WAVE - 1);
+    const uint32_t prev = __shfl_up(key, 1, WAVE);
+    const bool head = lane == 0 || prev != key;
+    const uint64_t heads = __ballot(head);
+    const uint64_t le = lane == WAVE - 1 ? ~0ull : ((2ull << lane) - 1ull);
+    const int leader = 63 - __clzll(heads & le);
+    const uint64_t after = heads & ~le;
+    const int next = after ? __ffsll((long long)after) - 1 : WAVE;
+    uint32_t base = 0;
+    if (valid && lane == leader) base = atomicAdd(&S.count[key], (uint32_t)(next - leader));
+    base = __shfl(base, leader, WAVE);
+    if (valid) {
+        S.key[i] = key;
+        S.slot[i] = base + (uint32_t)(lane - leader);
+    }
 }
 
-// Counting-sort pass 2: exclusive scan of the counts (reduce / scan of tile sums / down-sweep).
+// Counting-sort pass 2: exclusive scan of the counts (reduce / scan of chunk sums / down-sweep).
 __global__ __launch_bounds__(BLOCK) void k_scan_reduce(const uint32_t *in, uint32_t n, uint32_t *tile_sum) {
     __shared__ uint32_t sh[4];
     const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
@@ -159,43 +178,108 @@ __device__ __forceinline__ uint32_t face_base(const DevSim &S, int c) {
     return c == 0 ? 0u : (c == 1 ? S.nu : S.nu + S.nv);
 }
 
-// p2g (fluid.slang:94-112): trilinear fixed-point splat of the particle velocity onto the
-// 8 faces of each component; out-of-range corners are skipped.  Runs over the cell-sorted
-// scratch copy so neighbouring lanes add into neighbouring faces.
-__global__ __launch_bounds__(BLOCK) void k_p2g(DevSim S) {
-    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= S.np) return;
-    const float x[3] = {S.bx[i], S.by[i], S.bz[i]};
-    const float v[3] = {S.bvx[i], S.bvy[i], S.bvz[i]};
+// Fixed-point weight contributions of one particle for component c (fluid.slang:97-111):
+// corners out of the face grid are skipped; `emit(q, acc, wacc)` receives the others.
+template <typename Emit>
+__device__ __forceinline__ void splat_component(const DevSim &S, int c, const float x[3], float vc, Emit &&emit) {
+    int dim[3] = {S.nx, S.ny, S.nz};
+    dim[c] += 1;
     const float o[3] = {S.ox, S.oy, S.oz};
+    float g[3], f[3];
+    int b[3];
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-        int dim[3] = {S.nx, S.ny, S.nz};
-        dim[c] += 1;
-        float g[3], f[3];
-        int b[3];
+    for (int a = 0; a < 3; a++) g[a] = (x[a] - o[a]) * S.inv_h - 0.5f;
+    g[c] = g[c] + 0.5f;
 #pragma unroll
-        for (int a = 0; a < 3; a++) g[a] = (x[a] - o[a]) * S.inv_h - 0.5f;
-        g[c] = g[c] + 0.5f;
+    for (int a = 0; a < 3; a++) {
+        b[a] = (int)floorf(g[a]);
+        f[a] = g[a] - (float)b[a];
+    }
 #pragma unroll
-        for (int a = 0; a < 3; a++) {
-            b[a] = (int)floorf(g[a]);
-            f[a] = g[a] - (float)b[a];
+    for (int k = 0; k < 8; k++) {
+        const int ox = k & 1, oy = (k >> 1) & 1, oz = k >> 2;
+        const int qx = b[0] + ox, qy = b[1] + oy, qz = b[2] + oz;
+        if (qx < 0 || qy < 0 || qz < 0 || qx > dim[0] - 1 || qy > dim[1] - 1 || qz > dim[2] - 1) continue;
+        const float wx = ox ? f[0] : 1.0f - f[0];
+        const float wy = oy ? f[1] : 1.0f - f[1];
+        const float wz = oz ? f[2] : 1.0f - f[2];
+        const float w = wx * wy * wz;
+        emit(qx, qy, qz, (int)(w * vc * 65536.0f), (int)(w * 65536.0f));
+    }
+}
+
+// LDS box of one tile's faces for component c: the faces any particle of the tile's
+// cells can reach, i.e. cells [t0 - 1, t0 + TILE] on the axes other than c and faces
+// [t0, t0 + TILE] along c.  Dimensions 9 x 10 x 10 (permuted per component) = 900.
+constexpr int PBOX = 900;
+
+__device__ __forceinline__ void box_of(int c, int i0, int j0, int k0, int lo[3], int d[3]) {
+    lo[0] = i0 - 1; lo[1] = j0 - 1; lo[2] = k0 - 1;
+    d[0] = d[1] = d[2] = TILE + 2;
+    lo[c] += 1;
+    d[c] = TILE + 1;
+}
+
+__device__ __forceinline__ uint32_t face_index(const DevSim &S, int c, int qx, int qy, int qz) {
+    const uint32_t dx = (uint32_t)S.nx + (c == 0), dy = (uint32_t)S.ny + (c == 1);
+    return face_base(S, c) + (uint32_t)qx + dx * ((uint32_t)qy + dy * (uint32_t)qz);
+}
+
+// p2g (fluid.slang:94-112) for one 8^3 tile of cells per workgroup: the tile's particles are
+// contiguous in the cell-sorted scratch copy; their fixed-point splats are summed with LDS
+// atomics into the tile's face box, then the box is added to the global faces (one global
+// atomic per touched face and tile instead of one per particle and corner).  Integer sums:
+// the result does not depend on the order of additions.
+__global__ __launch_bounds__(BLOCK) void k_p2g_tile(DevSim S) {
+    __shared__ int lacc[3 * PBOX];
+    __shared__ int lwt[3 * PBOX];
+    const uint32_t t = blockIdx.x;
+    const uint32_t beg = S.offset[t * TILE_CELLS], end = S.offset[(t + 1) * TILE_CELLS];
+    if (beg == end) return;
+    const int ti = (int)(t % (uint32_t)S.ntx), tj = (int)((t / (uint32_t)S.ntx) % (uint32_t)S.nty),
+              tk = (int)(t / ((uint32_t)S.ntx * (uint32_t)S.nty));
+    const int i0 = ti * TILE, j0 = tj * TILE, k0 = tk * TILE;
+    for (int q = threadIdx.x; q < 3 * PBOX; q += BLOCK) {
+        lacc[q] = 0;
+        lwt[q] = 0;
+    }
+    __syncthreads();
+    // lanes of a wave take every 8th particle of a 256-run, so they land in different cells
+    const uint32_t local = (threadIdx.x & 31u) * 8u + (threadIdx.x >> 5);
+    for (uint32_t base = beg; base < end; base += BLOCK) {
+        const uint32_t p = base + local;
+        if (p >= end) continue;
+        const float x[3] = {S.bx[p], S.by[p], S.bz[p]};
+        const float v[3] = {S.bvx[p], S.bvy[p], S.bvz[p]};
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            int lo[3], d[3];
+            box_of(c, i0, j0, k0, lo, d);
+            splat_component(S, c, x, v[c], [&](int qx, int qy, int qz, int a, int w) {
+                const int lx = qx - lo[0], ly = qy - lo[1], lz = qz - lo[2];
+                if ((unsigned)lx < (unsigned)d[0] && (unsigned)ly < (unsigned)d[1] && (unsigned)lz < (unsigned)d[2]) {
+                    const int li = c * PBOX + lx + d[0] * (ly + d[1] * lz);
+                    atomicAdd(&lacc[li], a);
+                    atomicAdd(&lwt[li], w);
+                } else {  // outside the box (cannot happen for a particle binned in this tile)
+                    const uint32_t fi = face_index(S, c, qx, qy, qz);
+                    atomicAdd(&S.acc[fi], a);
+                    atomicAdd(&S.wacc[fi], w);
+                }
+            });
         }
-        const uint32_t base = face_base(S, c);
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            const int ox = k & 1, oy = (k >> 1) & 1, oz = k >> 2;
-            const int qx = b[0] + ox, qy = b[1] + oy, qz = b[2] + oz;
-            if (qx < 0 || qy < 0 || qz < 0 || qx > dim[0] - 1 || qy > dim[1] - 1 || qz > dim[2] - 1) continue;
-            const float wx = ox ? f[0] : 1.0f - f[0];
-            const float wy = oy ? f[1] : 1.0f - f[1];
-            const float wz = oz ? f[2] : 1.0f - f[2];
-            const float w = wx * wy * wz;
-            const uint32_t fi = base + (uint32_t)qx + (uint32_t)dim[0] * ((uint32_t)qy + (uint32_t)dim[1] * (uint32_t)qz);
-            atomicAdd(&S.acc[fi], (int)(w * v[c] * 65536.0f));
-            atomicAdd(&S.wacc[fi], (int)(w * 65536.0f));
-        }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < 3 * PBOX; q += BLOCK) {
+        const int a = lacc[q], w = lwt[q];
+        if ((a | w) == 0) continue;
+        const int c = q / PBOX, r = q - c * PBOX;
+        int lo[3], d[3];
+        box_of(c, i0, j0, k0, lo, d);
+        const int qx = lo[0] + r % d[0], qy = lo[1] + (r / d[0]) % d[1], qz = lo[2] + r / (d[0] * d[1]);
+        const uint32_t fi = face_index(S, c, qx, qy, qz);
+        if (a) atomicAdd(&S.acc[fi], a);
+        if (w) atomicAdd(&S.wacc[fi], w);
     }
 }
 
@@ -277,21 +361,27 @@ __global__ __launch_bounds__(BLOCK) void k_g2p(DevSim S) {
 
 // ---------------------------------------------------------------- IO / bins
 
-// Bins in seeding order: cell_of[id] and the per-cell id lists at offset + slot.
-__global__ __launch_bounds__(BLOCK) void k_bins_out(DevSim S, uint32_t *cell_of, uint32_t *sorted) {
+// Bins in seeding order and reference (x-fastest) cell numbering: cell_of[id] and the
+// per-cell id lists at offset_lin[cell] + slot (slots are arrival ranks within a cell).
+__global__ __launch_bounds__(BLOCK) void k_bins_out(DevSim S, const uint32_t *off_lin, uint32_t *cell_of,
+                                                    uint32_t *sorted) {
     uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= S.np) return;
-    uint32_t id = S.aid[i], c = S.key[i];
+    int cx, cy, cz;
+    cell_of_pos(S, S.ax[i], S.ay[i], S.az[i], cx, cy, cz);
+    const uint32_t c = (uint32_t)cx + (uint32_t)S.nx * ((uint32_t)cy + (uint32_t)S.ny * (uint32_t)cz);
+    const uint32_t id = S.aid[i];
     if (cell_of) cell_of[id] = c;
-    if (sorted) sorted[S.offset[c] + S.slot[i]] = id;
+    if (sorted) sorted[off_lin[c] + S.slot[i]] = id;
 }
 
 // Canonical order inside each cell: ids ascending (insertion sort of the cell's segment).
-__global__ __launch_bounds__(BLOCK) void k_sort_segments(DevSim S, uint32_t *sorted) {
+__global__ __launch_bounds__(BLOCK) void k_sort_segments(uint32_t ncells, const uint32_t *offsets,
+                                                         const uint32_t *counts, uint32_t *sorted) {
     uint32_t c = blockIdx.x * BLOCK + threadIdx.x;
-    if (c >= S.ncells) return;
-    uint32_t *seg = sorted + S.offset[c];
-    uint32_t n = S.count[c];
+    if (c >= ncells) return;
+    uint32_t *seg = sorted + offsets[c];
+    uint32_t n = counts[c];
     for (uint32_t a = 1; a < n; a++) {
         uint32_t v = seg[a];
         uint32_t b = a;
@@ -301,6 +391,15 @@ __global__ __launch_bounds__(BLOCK) void k_sort_segments(DevSim S, uint32_t *sor
         }
         seg[b] = v;
     }
+}
+
+// Per-cell counts in reference cell numbering.
+__global__ __launch_bounds__(BLOCK) void k_count_linear(DevSim S, uint32_t *out) {
+    uint32_t c = blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= S.ncells) return;
+    const uint32_t i = c % (uint32_t)S.nx, j = (c / (uint32_t)S.nx) % (uint32_t)S.ny,
+                   k = c / ((uint32_t)S.nx * (uint32_t)S.ny);
+    out[c] = S.count[tile_key(S, (int)i, (int)j, (int)k)];
 }
 
 __global__ __launch_bounds__(BLOCK) void k_unpack_aos8(DevSim S, float *out) {
@@ -338,7 +437,7 @@ void launch_fill_u32(uint32_t *p, uint32_t n, uint32_t value, hipStream_t s) {
 }
 
 void launch_clear_grid(const DevSim &S, hipStream_t s) {
-    uint32_t n = S.nfaces > S.ncells ? S.nfaces : S.ncells;
+    uint32_t n = S.nfaces > S.nkeys ? S.nfaces : S.nkeys;
     k_clear_grid<<<blocks_for(n), BLOCK, 0, s>>>(S);
 }
 
@@ -346,22 +445,31 @@ void launch_bin_count(const DevSim &S, hipStream_t s) {
     k_bin_count<<<blocks_for(S.np), BLOCK, 0, s>>>(S);
 }
 
-void launch_scan_offsets(const DevSim &S, hipStream_t s) {
-    const uint32_t ntiles = (S.ncells + SCAN_TILE - 1) / SCAN_TILE;
-    k_scan_reduce<<<ntiles, BLOCK, 0, s>>>(S.count, S.ncells, S.scan_tmp);
-    k_scan_tiles<<<1, BLOCK, 0, s>>>(S.scan_tmp, ntiles);
-    k_scan_down<<<ntiles, BLOCK, 0, s>>>(S.count, S.ncells, S.scan_tmp, S.offset);
+uint32_t scan_tiles(uint32_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
+
+void launch_exclusive_scan(const uint32_t *in, uint32_t n, uint32_t *tile_tmp, uint32_t *out, hipStream_t s) {
+    const uint32_t ntiles = scan_tiles(n);
+    if (ntiles == 0) return;
+    k_scan_reduce<<<ntiles, BLOCK, 0, s>>>(in, n, tile_tmp);
+    k_scan_tiles<<<1, BLOCK, 0, s>>>(tile_tmp, ntiles);
+    k_scan_down<<<ntiles, BLOCK, 0, s>>>(in, n, tile_tmp, out);
 }
+
+void launch_scan_offsets(const DevSim &S, hipStream_t s) { launch_exclusive_scan(S.count, S.nkeys, S.scan_tmp, S.offset, s); }
 
 void launch_scatter(const DevSim &S, hipStream_t s) { k_scatter<<<blocks_for(S.np), BLOCK, 0, s>>>(S); }
-void launch_p2g(const DevSim &S, hipStream_t s) { k_p2g<<<blocks_for(S.np), BLOCK, 0, s>>>(S); }
+void launch_p2g(const DevSim &S, hipStream_t s) { k_p2g_tile<<<S.nkeys / TILE_CELLS, BLOCK, 0, s>>>(S); }
 void launch_g2p(const DevSim &S, hipStream_t s) { k_g2p<<<blocks_for(S.np), BLOCK, 0, s>>>(S); }
 
-void launch_bins_out(const DevSim &S, uint32_t *cell_of, uint32_t *sorted, hipStream_t s) {
-    k_bins_out<<<blocks_for(S.np), BLOCK, 0, s>>>(S, cell_of, sorted);
+void launch_bins_out(const DevSim &S, const uint32_t *off_lin, uint32_t *cell_of, uint32_t *sorted, hipStream_t s) {
+    k_bins_out<<<blocks_for(S.np), BLOCK, 0, s>>>(S, off_lin, cell_of, sorted);
 }
-void launch_sort_segments(const DevSim &S, uint32_t *sorted, hipStream_t s) {
-    k_sort_segments<<<blocks_for(S.ncells), BLOCK, 0, s>>>(S, sorted);
+void launch_sort_segments(uint32_t ncells, const uint32_t *offsets, const uint32_t *counts, uint32_t *sorted,
+                          hipStream_t s) {
+    k_sort_segments<<<blocks_for(ncells), BLOCK, 0, s>>>(ncells, offsets, counts, sorted);
+}
+void launch_count_linear(const DevSim &S, uint32_t *out, hipStream_t s) {
+    k_count_linear<<<blocks_for(S.ncells), BLOCK, 0, s>>>(S, out);
 }
 void launch_unpack_aos8(const DevSim &S, float *aos8, hipStream_t s) {
     k_unpack_aos8<<<blocks_for(S.np), BLOCK, 0, s>>>(S, aos8);

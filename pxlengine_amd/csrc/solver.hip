@@ -2,17 +2,26 @@
 // for FLUID cell i, (A p)_i = cnt_i p_i - sum_{FLUID nbrs} p_j (cnt_i = non-SOLID neighbours,
 // AIR neighbours are Dirichlet 0), b_i = -div_i h^2 / dt.
 //
-// Jacobi-preconditioned CG, two fused passes per iteration:
-//   pass A: d = z + beta*d_prev (z = r / cnt) at the cell and its FLUID neighbours,
-//           q = A d, and the workgroup partial of d.q; the last workgroup forms alpha.
-//   pass B: x += alpha d, r -= alpha q, and the partial of r.z; the last workgroup forms beta.
-// The direction vector ping-pongs between two buffers so pass A can rebuild neighbours' d
-// from unmodified inputs.  Accumulation type Acc = float (plain FP32 CG) or double (FP64
-// dot products of the mixed-precision solve); vector arithmetic is fp32 in both.
+// Conjugate gradients with the Jacobi preconditioner M = diag(A), in the Chronopoulos-Gear
+// arrangement: both dot products of an iteration come out of ONE fused stencil pass, so an
+// iteration is one pass over the cells plus one single-workgroup finalize:
 //
-// Mixed precision: the FP64 outer loop keeps x64 (solution) and r64 = b - A x64 in FP64,
-// solves A e = float(r64) with the FP32 iterations above from e = 0, then x64 += e and
-// recomputes r64 in FP64.
+//   given alpha_i, beta_i:   p_i = u_i + beta_i p_{i-1}      s_i = w_i + beta_i s_{i-1}
+//                            x  += alpha_i p_i               r_{i+1} = r_i - alpha_i s_i
+//                            u_{i+1} = M^-1 r_{i+1}          w_{i+1} = A u_{i+1}
+//                            partials of gamma = (r_{i+1}, u_{i+1}), delta = (w_{i+1}, u_{i+1})
+//   finalize:                beta = gamma / gamma_old, alpha = gamma / (delta - beta gamma / alpha)
+//
+// w_{i+1} needs u_{i+1} at the six neighbours; the pass rebuilds them from r_i, w_i, s_{i-1}
+// read at the neighbours, so r, w and s ping-pong between two buffers each.  Workgroup
+// partials are plain stores; the finalize kernel (one workgroup, launched next on the same
+// stream) sums them in workgroup order, so the scalars do not depend on scheduling.
+//
+// Accumulation type Acc = float (plain FP32 CG) or double (the FP64 dot products of the
+// mixed-precision solve); vector arithmetic is fp32 in both.
+//
+// Mixed precision: the FP64 outer loop keeps x64 and r64 = b - A x64 in FP64, solves
+// A e = float(r64) from e = 0 with the iterations above, then x64 += e and recomputes r64.
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -20,154 +29,193 @@ namespace pxl {
 
 namespace {
 
-constexpr int TX = 64, TY = 4;
-
-struct CellPos {
-    int i, j, k;
-    uint32_t id;
-    bool valid;
-};
-
-__device__ __forceinline__ CellPos cell_pos(const DevSim &S) {
-    CellPos c;
-    c.i = blockIdx.x * TX + threadIdx.x;
-    c.j = blockIdx.y * TY + threadIdx.y;
-    c.k = blockIdx.z;
-    c.valid = c.i < S.nx && c.j < S.ny;
-    c.id = (uint32_t)c.i + (uint32_t)S.nx * ((uint32_t)c.j + (uint32_t)S.ny * (uint32_t)c.k);
-    return c;
-}
+constexpr int TX = 64, TY = 4, ZB = 4;  // a lane walks ZB consecutive z-layers
 
 __device__ __forceinline__ float diag_of(uint32_t st) { return (float)((st >> 8) & 7u); }
 
-// z = r / cnt (0 where cnt = 0).
+// u = M^-1 r (0 where the diagonal is 0).
 __device__ __forceinline__ float precond(float r, uint32_t st) {
     const float dg = diag_of(st);
     return dg > 0.0f ? r / dg : 0.0f;
 }
 
-// Offsets of the six neighbours, order -x,+x,-y,+y,-z,+z.
+struct Lane {
+    int i, j, k0, k1;
+    bool valid;
+};
+
+__device__ __forceinline__ Lane lane_cells(const DevSim &S) {
+    Lane l;
+    l.i = blockIdx.x * TX + threadIdx.x;
+    l.j = blockIdx.y * TY + threadIdx.y;
+    l.k0 = blockIdx.z * ZB;
+    l.k1 = min(l.k0 + ZB, S.nz);
+    l.valid = l.i < S.nx && l.j < S.ny;
+    return l;
+}
+
+__device__ __forceinline__ uint32_t cidx(const DevSim &S, int i, int j, int k) {
+    return (uint32_t)i + (uint32_t)S.nx * ((uint32_t)j + (uint32_t)S.ny * (uint32_t)k);
+}
+
 __device__ __forceinline__ void nb_offsets(const DevSim &S, int32_t off[6]) {
     const int32_t sy = S.nx, sz = S.nx * S.ny;
     off[0] = -1; off[1] = 1; off[2] = -sy; off[3] = sy; off[4] = -sz; off[5] = sz;
 }
 
-template <typename T>
-__device__ __forceinline__ T to_acc(float v) { return (T)v; }
-
-// ---- init: r = b - A x for the pressure system (x = warm-started pressure), rz = r.z
 template <typename Acc>
-__global__ __launch_bounds__(TX *TY) void k_cg_init(DevSim S, float *x, float *dprev) {
-    const CellPos c = cell_pos(S);
-    Acc rz = 0;
-    if (c.valid) {
-        const uint32_t st = S.sten[c.id];
-        if (st & STEN_FLUID) {
-            int32_t off[6];
-            nb_offsets(S, off);
+__device__ __forceinline__ Acc *partials(const DevSim &S);
+template <>
+__device__ __forceinline__ float *partials<float>(const DevSim &S) { return S.part32; }
+template <>
+__device__ __forceinline__ double *partials<double>(const DevSim &S) { return S.part64; }
+
+// Stores the workgroup sums of NV values at part[k * nb + block] (plain stores: they are
+// consumed by the next kernel on the stream).
+template <typename T, int NV>
+__device__ __forceinline__ void store_partials(const T (&v)[NV], T *part) {
+    __shared__ T sh[4 * NV];
+    const uint32_t nb = num_blocks(), b = flat_block();
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        T s = block_sum(v[k], sh + 4 * k);
+        if (flat_tid() == 0) part[(size_t)k * nb + b] = s;
+    }
+}
+
+// Sum of the nb workgroup partials of value k, in workgroup order (one workgroup).
+template <typename T>
+__device__ __forceinline__ T sum_partials(const T *part, uint32_t nb, int k, T *sh) {
+    T a = 0;
+    for (uint32_t i = flat_tid(); i < nb; i += BLOCK) a += part[(size_t)k * nb + i];
+    return block_sum(a, sh);
+}
+
+// ---- init 1: r0 = b - A x (pressure system, x = warm-started pressure) or, for the inner
+// solve of the refinement, r0 = float(r64) with x = e = 0.
+__global__ __launch_bounds__(TX *TY) void k_cg_residual(DevSim S, float *__restrict__ x, float *__restrict__ r,
+                                                        int inner) {
+    const Lane l = lane_cells(S);
+    if (!l.valid) return;
+    int32_t off[6];
+    nb_offsets(S, off);
+    for (int k = l.k0; k < l.k1; k++) {
+        const uint32_t id = cidx(S, l.i, l.j, k);
+        const uint32_t st = S.sten[id];
+        if (!(st & STEN_FLUID)) continue;
+        if (inner) {
+            r[id] = (float)S.r64[id];
+            x[id] = 0.0f;
+            continue;
+        }
+        float sum = 0.0f;
+#pragma unroll
+        for (int n = 0; n < 6; n++)
+            if (st & (1u << n)) sum = sum + x[id + off[n]];
+        const float ax = diag_of(st) * x[id] - sum;
+        const float b = -(S.div[id] * S.h * S.h / S.dt);
+        r[id] = b - ax;
+    }
+}
+
+// ---- init 2: u0 = M^-1 r0, w0 = A u0; p_{-1} = s_{-1} = 0; partials gamma0, delta0.
+template <typename Acc>
+__global__ __launch_bounds__(TX *TY) void k_cg_start(DevSim S, const float *__restrict__ r, float *__restrict__ w,
+                                                     float *__restrict__ s_prev) {
+    const Lane l = lane_cells(S);
+    Acc g = 0, d = 0;
+    if (l.valid) {
+        int32_t off[6];
+        nb_offsets(S, off);
+        for (int k = l.k0; k < l.k1; k++) {
+            const uint32_t id = cidx(S, l.i, l.j, k);
+            const uint32_t st = S.sten[id];
+            if (!(st & STEN_FLUID)) continue;
+            const float uc = precond(r[id], st);
             float sum = 0.0f;
 #pragma unroll
             for (int n = 0; n < 6; n++)
-                if (st & (1u << n)) sum = sum + x[c.id + off[n]];
-            const float ax = diag_of(st) * x[c.id] - sum;
-            const float b = -(S.div[c.id] * S.h * S.h / S.dt);
-            const float r = b - ax;
-            S.r[c.id] = r;
-            dprev[c.id] = 0.0f;
-            if (diag_of(st) > 0.0f) rz = to_acc<Acc>(r) * to_acc<Acc>(r / diag_of(st));
+                if (st & (1u << n)) sum = sum + precond(r[id + off[n]], S.sten[id + off[n]]);
+            const float wc = diag_of(st) * uc - sum;
+            w[id] = wc;
+            S.cg_p[id] = 0.0f;
+            s_prev[id] = 0.0f;
+            g += (Acc)r[id] * (Acc)uc;
+            d += (Acc)wc * (Acc)uc;
         }
     }
-    const Acc v[1] = {rz};
-    Acc tot[1];
-    if (grid_sum<Acc, 1>(v, reinterpret_cast<Acc *>(sizeof(Acc) == 8 ? (void *)S.part64 : (void *)S.part32), S.ctr, tot) &&
-        flat_tid() == 0) {
-        S.scal[SC_RZ] = (double)tot[0];
-        S.scal[SC_BETA] = 0.0;
-    }
+    const Acc v[2] = {g, d};
+    store_partials<Acc, 2>(v, partials<Acc>(S));
 }
 
-// ---- mixed inner init: r = float(r64), e = 0, rz = r.z (FP64)
-__global__ __launch_bounds__(TX *TY) void k_inner_init(DevSim S, float *e, float *dprev) {
-    const CellPos c = cell_pos(S);
-    double rz = 0;
-    if (c.valid) {
-        const uint32_t st = S.sten[c.id];
-        if (st & STEN_FLUID) {
-            const float r = (float)S.r64[c.id];
-            S.r[c.id] = r;
-            e[c.id] = 0.0f;
-            dprev[c.id] = 0.0f;
-            if (diag_of(st) > 0.0f) rz = (double)r * (double)(r / diag_of(st));
-        }
-    }
-    const double v[1] = {rz};
-    double tot[1];
-    if (grid_sum<double, 1>(v, S.part64, S.ctr, tot) && flat_tid() == 0) {
-        S.scal[SC_RZ] = tot[0];
-        S.scal[SC_BETA] = 0.0;
-    }
-}
-
-// ---- pass A: d = z + beta d_prev; q = A d; alpha = rz / (d.q)
+// ---- one fused iteration (see the file comment).
 template <typename Acc>
-__global__ __launch_bounds__(TX *TY) void k_cg_dir_spmv(DevSim S, const float *__restrict__ dprev,
-                                                       float *__restrict__ dcur) {
-    const CellPos c = cell_pos(S);
+__global__ __launch_bounds__(TX *TY) void k_cg_iter(DevSim S, float *__restrict__ x, const float *__restrict__ r_in,
+                                                    float *__restrict__ r_out, const float *__restrict__ w_in,
+                                                    float *__restrict__ w_out, const float *__restrict__ s_prev,
+                                                    float *__restrict__ s_cur) {
+    const Lane l = lane_cells(S);
+    const float alpha = (float)S.scal[SC_ALPHA];
     const float beta = (float)S.scal[SC_BETA];
-    Acc dq = 0;
-    if (c.valid) {
-        const uint32_t st = S.sten[c.id];
-        if (st & STEN_FLUID) {
-            int32_t off[6];
-            nb_offsets(S, off);
-            const float dc = precond(S.r[c.id], st) + beta * dprev[c.id];
+    Acc g = 0, d = 0;
+    if (l.valid) {
+        int32_t off[6];
+        nb_offsets(S, off);
+        for (int k = l.k0; k < l.k1; k++) {
+            const uint32_t id = cidx(S, l.i, l.j, k);
+            const uint32_t st = S.sten[id];
+            if (!(st & STEN_FLUID)) continue;
+            const float ri = r_in[id];
+            const float pc = precond(ri, st) + beta * S.cg_p[id];
+            const float sc = w_in[id] + beta * s_prev[id];
+            x[id] = x[id] + alpha * pc;
+            const float rn = ri - alpha * sc;
+            const float un = precond(rn, st);
             float sum = 0.0f;
 #pragma unroll
             for (int n = 0; n < 6; n++) {
                 if (st & (1u << n)) {
-                    const uint32_t nid = c.id + off[n];
-                    const float dn = precond(S.r[nid], S.sten[nid]) + beta * dprev[nid];
-                    sum = sum + dn;
+                    const uint32_t q = id + off[n];
+                    const float sq = w_in[q] + beta * s_prev[q];
+                    sum = sum + precond(r_in[q] - alpha * sq, S.sten[q]);
                 }
             }
-            const float qc = diag_of(st) * dc - sum;
-            dcur[c.id] = dc;
-            S.q[c.id] = qc;
-            dq = to_acc<Acc>(dc) * to_acc<Acc>(qc);
+            const float wn = diag_of(st) * un - sum;
+            S.cg_p[id] = pc;
+            s_cur[id] = sc;
+            r_out[id] = rn;
+            w_out[id] = wn;
+            g += (Acc)rn * (Acc)un;
+            d += (Acc)wn * (Acc)un;
         }
     }
-    const Acc v[1] = {dq};
-    Acc tot[1];
-    if (grid_sum<Acc, 1>(v, reinterpret_cast<Acc *>(sizeof(Acc) == 8 ? (void *)S.part64 : (void *)S.part32), S.ctr, tot) &&
-        flat_tid() == 0) {
-        const Acc rz = (Acc)S.scal[SC_RZ];
-        S.scal[SC_ALPHA] = (tot[0] != (Acc)0 && rz != (Acc)0) ? (double)(rz / tot[0]) : 0.0;
-    }
+    const Acc v[2] = {g, d};
+    store_partials<Acc, 2>(v, partials<Acc>(S));
 }
 
-// ---- pass B: x += alpha d; r -= alpha q; beta = rz_new / rz
+// ---- finalize: alpha, beta from the partials of gamma, delta (one workgroup).
 template <typename Acc>
-__global__ __launch_bounds__(TX *TY) void k_cg_update(DevSim S, float *__restrict__ x, const float *__restrict__ d) {
-    const CellPos c = cell_pos(S);
-    const float alpha = (float)S.scal[SC_ALPHA];
-    Acc rz = 0;
-    if (c.valid) {
-        const uint32_t st = S.sten[c.id];
-        if (st & STEN_FLUID) {
-            x[c.id] = x[c.id] + alpha * d[c.id];
-            const float r = S.r[c.id] - alpha * S.q[c.id];
-            S.r[c.id] = r;
-            if (diag_of(st) > 0.0f) rz = to_acc<Acc>(r) * to_acc<Acc>(r / diag_of(st));
+__global__ __launch_bounds__(BLOCK) void k_cg_finalize(DevSim S, uint32_t nb, int first) {
+    __shared__ Acc sh[8];
+    const Acc *part = partials<Acc>(S);
+    const Acc g = sum_partials<Acc>(part, nb, 0, sh);
+    const Acc d = sum_partials<Acc>(part, nb, 1, sh + 4);
+    if (threadIdx.x != 0) return;
+    Acc alpha = 0, beta = 0;
+    if (first) {
+        alpha = (g != (Acc)0 && d != (Acc)0) ? g / d : (Acc)0;
+    } else {
+        const Acc g_old = (Acc)S.scal[SC_RZ];
+        const Acc a_old = (Acc)S.scal[SC_ALPHA];
+        if (g != (Acc)0 && g_old != (Acc)0 && a_old != (Acc)0) {
+            beta = g / g_old;
+            const Acc den = d - beta * g / a_old;
+            alpha = den != (Acc)0 ? g / den : (Acc)0;
         }
     }
-    const Acc v[1] = {rz};
-    Acc tot[1];
-    if (grid_sum<Acc, 1>(v, reinterpret_cast<Acc *>(sizeof(Acc) == 8 ? (void *)S.part64 : (void *)S.part32), S.ctr, tot) &&
-        flat_tid() == 0) {
-        const Acc rz_old = (Acc)S.scal[SC_RZ];
-        S.scal[SC_BETA] = rz_old != (Acc)0 ? (double)(tot[0] / rz_old) : 0.0;
-        S.scal[SC_RZ] = (double)tot[0];
-    }
+    S.scal[SC_ALPHA] = (double)alpha;
+    S.scal[SC_BETA] = (double)beta;
+    S.scal[SC_RZ] = (double)g;
 }
 
 // ---- FP64 outer loop.  The solution is double-buffered (x_src -> x_dst) so the update pass
@@ -176,107 +224,131 @@ __device__ __forceinline__ double b64_of(const DevSim &S, uint32_t id) {
     return -((double)S.div[id] * (double)S.h * (double)S.h / (double)S.dt);
 }
 
-// init: x64 = pressure, r64 = b - A x64; totals rr and bb.
+// init: x64 = pressure, r64 = b - A x64; partials of rr and bb.
 __global__ __launch_bounds__(TX *TY) void k_mixed_init(DevSim S, double *__restrict__ x64) {
-    const CellPos c = cell_pos(S);
+    const Lane l = lane_cells(S);
     double rr = 0, bb = 0;
-    if (c.valid) {
-        const uint32_t st = S.sten[c.id];
-        if (st & STEN_FLUID) {
-            int32_t off[6];
-            nb_offsets(S, off);
+    if (l.valid) {
+        int32_t off[6];
+        nb_offsets(S, off);
+        for (int k = l.k0; k < l.k1; k++) {
+            const uint32_t id = cidx(S, l.i, l.j, k);
+            const uint32_t st = S.sten[id];
+            if (!(st & STEN_FLUID)) continue;
             double sum = 0.0;
 #pragma unroll
             for (int n = 0; n < 6; n++)
-                if (st & (1u << n)) sum += (double)S.pressure[c.id + off[n]];
-            const double xc = (double)S.pressure[c.id];
-            const double b = b64_of(S, c.id);
+                if (st & (1u << n)) sum += (double)S.pressure[id + off[n]];
+            const double xc = (double)S.pressure[id];
+            const double b = b64_of(S, id);
             const double r = b - ((double)diag_of(st) * xc - sum);
-            x64[c.id] = xc;
-            S.r64[c.id] = r;
-            rr = r * r;
-            bb = b * b;
+            x64[id] = xc;
+            S.r64[id] = r;
+            rr += r * r;
+            bb += b * b;
         }
     }
     const double v[2] = {rr, bb};
-    double tot[2];
-    if (grid_sum<double, 2>(v, S.part64, S.ctr, tot) && flat_tid() == 0) {
-        S.scal[SC_RR] = tot[0];
-        S.scal[SC_BB] = tot[1];
-    }
+    store_partials<double, 2>(v, S.part64);
 }
 
 // update: x_dst = x_src + e (FP64) at the cell and its neighbours, r64 = b - A x_dst; rr.
 __global__ __launch_bounds__(TX *TY) void k_mixed_update(DevSim S, const double *__restrict__ xsrc,
                                                          double *__restrict__ xdst, const float *__restrict__ e) {
-    const CellPos c = cell_pos(S);
+    const Lane l = lane_cells(S);
     double rr = 0;
-    if (c.valid) {
-        const uint32_t st = S.sten[c.id];
-        if (st & STEN_FLUID) {
-            int32_t off[6];
-            nb_offsets(S, off);
+    if (l.valid) {
+        int32_t off[6];
+        nb_offsets(S, off);
+        for (int k = l.k0; k < l.k1; k++) {
+            const uint32_t id = cidx(S, l.i, l.j, k);
+            const uint32_t st = S.sten[id];
+            if (!(st & STEN_FLUID)) continue;
             double sum = 0.0;
 #pragma unroll
             for (int n = 0; n < 6; n++) {
                 if (st & (1u << n)) {
-                    const uint32_t nid = c.id + off[n];
-                    sum += xsrc[nid] + (double)e[nid];
+                    const uint32_t q = id + off[n];
+                    sum += xsrc[q] + (double)e[q];
                 }
             }
-            const double xc = xsrc[c.id] + (double)e[c.id];
-            const double r = b64_of(S, c.id) - ((double)diag_of(st) * xc - sum);
-            xdst[c.id] = xc;
-            S.r64[c.id] = r;
-            rr = r * r;
+            const double xc = xsrc[id] + (double)e[id];
+            const double r = b64_of(S, id) - ((double)diag_of(st) * xc - sum);
+            xdst[id] = xc;
+            S.r64[id] = r;
+            rr += r * r;
         }
     }
     const double v[1] = {rr};
-    double tot[1];
-    if (grid_sum<double, 1>(v, S.part64, S.ctr, tot) && flat_tid() == 0) S.scal[SC_RR] = tot[0];
+    store_partials<double, 1>(v, S.part64);
 }
 
-__global__ __launch_bounds__(TX *TY) void k_mixed_finish(DevSim S, const double *x64) {
-    const CellPos c = cell_pos(S);
-    if (c.valid && (S.sten[c.id] & STEN_FLUID)) S.pressure[c.id] = (float)x64[c.id];
+// Totals of the refinement norms into scal[SC_RR] (and scal[SC_BB] when nv = 2).
+__global__ __launch_bounds__(BLOCK) void k_norms_finalize(DevSim S, uint32_t nb, int nv) {
+    __shared__ double sh[8];
+    const double rr = sum_partials<double>(S.part64, nb, 0, sh);
+    double bb = 0;
+    if (nv > 1) bb = sum_partials<double>(S.part64, nb, 1, sh + 4);
+    if (threadIdx.x != 0) return;
+    S.scal[SC_RR] = rr;
+    if (nv > 1) S.scal[SC_BB] = bb;
 }
 
-dim3 cgrid(const DevSim &S) { return dim3((S.nx + TX - 1) / TX, (S.ny + TY - 1) / TY, S.nz); }
+__global__ __launch_bounds__(TX *TY) void k_mixed_finish(DevSim S, const double *__restrict__ x64) {
+    const Lane l = lane_cells(S);
+    if (!l.valid) return;
+    for (int k = l.k0; k < l.k1; k++) {
+        const uint32_t id = cidx(S, l.i, l.j, k);
+        if (S.sten[id] & STEN_FLUID) S.pressure[id] = (float)x64[id];
+    }
+}
+
+inline float *r_buf(const DevSim &S, uint32_t k) { return (k & 1) ? S.cg_r1 : S.cg_r0; }
+inline float *w_buf(const DevSim &S, uint32_t k) { return (k & 1) ? S.cg_w1 : S.cg_w0; }
+inline float *s_buf(const DevSim &S, uint32_t k) { return (k & 1) ? S.cg_s1 : S.cg_s0; }
 
 }  // namespace
 
-void launch_cg_init32(const DevSim &S, hipStream_t s) {
-    k_cg_init<float><<<cgrid(S), dim3(TX, TY), 0, s>>>(S, S.pressure, S.d1);
+dim3 solver_grid(const DevSim &S) { return dim3((S.nx + TX - 1) / TX, (S.ny + TY - 1) / TY, (S.nz + ZB - 1) / ZB); }
+
+uint32_t solver_blocks(const DevSim &S) {
+    const dim3 g = solver_grid(S);
+    return g.x * g.y * g.z;
 }
 
-void launch_inner_init(const DevSim &S, hipStream_t s) {
-    k_inner_init<<<cgrid(S), dim3(TX, TY), 0, s>>>(S, S.e, S.d1);
+void launch_cg_start(const DevSim &S, bool fp64, float *x, bool inner, hipStream_t s) {
+    const dim3 g = solver_grid(S), b(TX, TY);
+    k_cg_residual<<<g, b, 0, s>>>(S, x, r_buf(S, 0), inner ? 1 : 0);
+    // iteration 0 reads s_{-1} from s_buf(1)
+    if (fp64) k_cg_start<double><<<g, b, 0, s>>>(S, r_buf(S, 0), w_buf(S, 0), s_buf(S, 1));
+    else k_cg_start<float><<<g, b, 0, s>>>(S, r_buf(S, 0), w_buf(S, 0), s_buf(S, 1));
 }
 
-// Iteration `it` reads the direction of iteration it-1 from d1 (even it) or d0 (odd it).
-void launch_cg_pass_a(const DevSim &S, bool fp64, uint32_t it, hipStream_t s) {
-    const float *dprev = (it & 1) ? S.d0 : S.d1;
-    float *dcur = (it & 1) ? S.d1 : S.d0;
-    if (fp64) k_cg_dir_spmv<double><<<cgrid(S), dim3(TX, TY), 0, s>>>(S, dprev, dcur);
-    else k_cg_dir_spmv<float><<<cgrid(S), dim3(TX, TY), 0, s>>>(S, dprev, dcur);
+void launch_cg_finalize(const DevSim &S, bool fp64, bool first, hipStream_t s) {
+    if (fp64) k_cg_finalize<double><<<1, BLOCK, 0, s>>>(S, solver_blocks(S), first ? 1 : 0);
+    else k_cg_finalize<float><<<1, BLOCK, 0, s>>>(S, solver_blocks(S), first ? 1 : 0);
 }
 
-void launch_cg_pass_b(const DevSim &S, bool fp64, float *x, uint32_t it, hipStream_t s) {
-    const float *dcur = (it & 1) ? S.d1 : S.d0;
-    if (fp64) k_cg_update<double><<<cgrid(S), dim3(TX, TY), 0, s>>>(S, x, dcur);
-    else k_cg_update<float><<<cgrid(S), dim3(TX, TY), 0, s>>>(S, x, dcur);
+void launch_cg_iter(const DevSim &S, bool fp64, float *x, uint32_t it, hipStream_t s) {
+    const dim3 g = solver_grid(S), b(TX, TY);
+    float *rin = r_buf(S, it), *rout = r_buf(S, it + 1), *win = w_buf(S, it), *wout = w_buf(S, it + 1);
+    float *sprev = s_buf(S, it + 1), *scur = s_buf(S, it);
+    if (fp64) k_cg_iter<double><<<g, b, 0, s>>>(S, x, rin, rout, win, wout, sprev, scur);
+    else k_cg_iter<float><<<g, b, 0, s>>>(S, x, rin, rout, win, wout, sprev, scur);
 }
 
 void launch_mixed_init(const DevSim &S, double *x64, hipStream_t s) {
-    k_mixed_init<<<cgrid(S), dim3(TX, TY), 0, s>>>(S, x64);
+    k_mixed_init<<<solver_grid(S), dim3(TX, TY), 0, s>>>(S, x64);
+    k_norms_finalize<<<1, BLOCK, 0, s>>>(S, solver_blocks(S), 2);
 }
 
 void launch_mixed_update(const DevSim &S, const double *xsrc, double *xdst, hipStream_t s) {
-    k_mixed_update<<<cgrid(S), dim3(TX, TY), 0, s>>>(S, xsrc, xdst, S.e);
+    k_mixed_update<<<solver_grid(S), dim3(TX, TY), 0, s>>>(S, xsrc, xdst, S.e);
+    k_norms_finalize<<<1, BLOCK, 0, s>>>(S, solver_blocks(S), 1);
 }
 
 void launch_mixed_finish(const DevSim &S, const double *x64, hipStream_t s) {
-    k_mixed_finish<<<cgrid(S), dim3(TX, TY), 0, s>>>(S, x64);
+    k_mixed_finish<<<solver_grid(S), dim3(TX, TY), 0, s>>>(S, x64);
 }
 
 }  // namespace pxl
