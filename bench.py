@@ -38,7 +38,7 @@ def algorithmic_bytes(name: str, P: int, F: int, C: int, Cf: int) -> float:
         "faces": 16 * F,
         "cells": 4 * C + 4 * F + 7 * C + 4 * (C - Cf),
         "rbgs": C + 8 * Cf,
-        "cg_start": 2 * C + 4 * Cf + 4 * Cf + 4 * Cf + 2 * C + 4 * Cf + 12 * Cf,
+        "cg_start": 2 * C + 12 * Cf + 2 * C + 16 * Cf,
         "cg_iter": 2 * C + 40 * Cf,
         "cg_finalize": 0,
         "project": 8 * F + 5 * C,
@@ -56,6 +56,7 @@ def main():
     ap.add_argument("--solver", default="cg", choices=["cg", "rbgs", "mixed"])
     ap.add_argument("--iters", type=int, default=80)
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps timed for cpu_baseline (0 = skip)")
+    ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -74,6 +75,7 @@ def main():
     info = sim.info
     for _ in range(args.warmup):
         sim.step(1)
+    tuned = sim.autotune(2) if not args.no_tune else None  # launch shape from the autotuner
     t0 = time.perf_counter()
     sim.step(args.steps)  # returns when the GPU work is complete
     t1 = time.perf_counter()
@@ -127,7 +129,8 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded dam break, reference LCG seeding)",
         "config": {"workload": f"dam break {args.res}^3, single instance, {solver_desc}",
                    "grid": list(info["n"]), "particles": P, "dt": float(info["dt"]), "flip": float(info["flip"]),
-                   "path": "plain (one simulation, one stream, hipGraph per step)"},
+                   "path": "plain (one simulation, one stream, hipGraph per step)",
+                   "autotune": tuned},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": per_kernel,

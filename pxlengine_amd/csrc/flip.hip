@@ -102,10 +102,18 @@ struct Sim {
     std::string error;
     KernelTimer timer;
 
-    ~Sim() {
+    // Graphs bake kernel arguments (grid shapes, tunables); drop them after a change.
+    void drop_graphs() {
         if (step_graph) (void)hipGraphExecDestroy(step_graph);
-        for (auto &g : inner_graph)
+        step_graph = nullptr;
+        for (auto &g : inner_graph) {
             if (g) (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    }
+
+    ~Sim() {
+        drop_graphs();
         if (host_scal) (void)hipHostFree(host_scal);
         if (arena) (void)hipFree(arena);
         if (stream) (void)hipStreamDestroy(stream);
@@ -116,7 +124,9 @@ struct Sim {
     // Every device array of the simulation, in arena order.
     Plan plan() {
         const size_t P = d.np, F = d.nfaces, C = d.ncells, K = d.nkeys;
-        const size_t nb = solver_blocks(d);
+        DevSim one = d;
+        one.zb = 1;  // partials sized for the smallest z-batch the tuner may pick
+        const size_t nb = solver_blocks(one);
         Plan plan;
         auto add = [&](auto **p, size_t bytes) { plan.emplace_back(reinterpret_cast<void **>(p), bytes); };
         for (float **p : {&d.ax, &d.ay, &d.az, &d.avx, &d.avy, &d.avz}) add(p, 4 * P);
@@ -128,8 +138,7 @@ struct Sim {
         for (float **p : {&d.ix, &d.iy, &d.iz, &d.ivx, &d.ivy, &d.ivz}) add(p, 4 * P);
         add(&d.vel, 4 * F);
         add(&d.vel_old, 4 * F);
-        add(&d.acc, 4 * F);
-        add(&d.wacc, 4 * F);
+        add(&d.box, 8ull * BOX_ENTRIES * d.ntiles);
         add(&d.count, 4 * K);
         add(&d.offset, 4 * (K + 1));
         add(&d.type, C);
@@ -372,6 +381,7 @@ int pxlflip_default_opts(pxlflip_opts *o) {
     o->device = 0;
     o->scene_index = 0;
     o->use_graph = 1;
+    o->solver_zbatch = 0;
     return PXLFLIP_OK;
 }
 
@@ -458,7 +468,9 @@ int prepare(const char *fluid_json, double fps, uint32_t seed, const pxlflip_opt
     d.ntx = (int)((n[0] + TILE - 1) / TILE);
     d.nty = (int)((n[1] + TILE - 1) / TILE);
     d.ntz = (int)((n[2] + TILE - 1) / TILE);
-    d.nkeys = (uint32_t)d.ntx * (uint32_t)d.nty * (uint32_t)d.ntz * TILE_CELLS;
+    d.ntiles = (uint32_t)d.ntx * (uint32_t)d.nty * (uint32_t)d.ntz;
+    d.nkeys = d.ntiles * TILE_CELLS;
+    d.zb = o.solver_zbatch ? (int)o.solver_zbatch : 1;
     return PXLFLIP_OK;
 }
 
@@ -645,8 +657,8 @@ int pxlflip_read_field(pxlflip *h, int field, void *out) {
     switch (field) {
         case PXLFLIP_FIELD_VEL: src = d.vel; bytes = 4ull * d.nfaces; break;
         case PXLFLIP_FIELD_VEL_OLD: src = d.vel_old; bytes = 4ull * d.nfaces; break;
-        case PXLFLIP_FIELD_ACC: src = d.acc; bytes = 4ull * d.nfaces; break;
-        case PXLFLIP_FIELD_WACC: src = d.wacc; bytes = 4ull * d.nfaces; break;
+        case PXLFLIP_FIELD_ACC:
+        case PXLFLIP_FIELD_WACC: bytes = 4ull * d.nfaces; break;
         case PXLFLIP_FIELD_COUNT: bytes = 4ull * d.ncells; break;
         case PXLFLIP_FIELD_TYPE: bytes = 4ull * d.ncells; break;
         case PXLFLIP_FIELD_PRESSURE: src = d.pressure; bytes = 4ull * d.ncells; break;
@@ -659,7 +671,11 @@ int pxlflip_read_field(pxlflip *h, int field, void *out) {
         default: return pxl::fail(s, PXLFLIP_ERR_INVALID, "unknown field " + std::to_string(field));
     }
     return pxl::guarded(s, [&] {
-        if (field == PXLFLIP_FIELD_TYPE || field == PXLFLIP_FIELD_COUNT) {
+        if (field == PXLFLIP_FIELD_ACC || field == PXLFLIP_FIELD_WACC) {
+            pxl::DeviceBuffer a(bytes), w(bytes);
+            pxl::launch_acc_fields(d, a.as<int32_t>(), w.as<int32_t>(), s->stream);
+            HIP_CHECK(hipMemcpyAsync(out, field == PXLFLIP_FIELD_ACC ? a.p : w.p, bytes, hipMemcpyDeviceToHost, s->stream));
+        } else if (field == PXLFLIP_FIELD_TYPE || field == PXLFLIP_FIELD_COUNT) {
             pxl::DeviceBuffer buf(bytes);
             if (field == PXLFLIP_FIELD_TYPE) pxl::launch_type_u32(d, buf.as<uint32_t>(), s->stream);
             else pxl::launch_count_linear(d, buf.as<uint32_t>(), s->stream);
@@ -764,6 +780,57 @@ int pxlflip_profile_steps(pxlflip *h, uint32_t n, pxlflip_kernel_time *out, uint
         if (out) out[k] = agg[k];
     if (n_out) *n_out = (uint32_t)agg.size();
     return PXLFLIP_OK;
+}
+
+int pxlflip_autotune(pxlflip *h, uint32_t steps, uint32_t *chosen_zbatch, double *best_ms) {
+    if (!h || steps == 0) return PXLFLIP_ERR_INVALID;
+    Sim *s = &h->sim;
+    return pxl::guarded(s, [&] {
+        if (!s->initialized) s->reset();
+        pxl::DevSim &d = s->d;
+        // save the state that persists across steps (particles, ids, warm-start pressure)
+        const size_t P = d.np, C = d.ncells;
+        pxl::DeviceBuffer save(28 * P + 4 * C);
+        char *sp = save.as<char>();
+        float *src[6] = {d.ax, d.ay, d.az, d.avx, d.avy, d.avz};
+        for (int a = 0; a < 6; a++)
+            HIP_CHECK(hipMemcpyAsync(sp + 4 * P * a, src[a], 4 * P, hipMemcpyDeviceToDevice, s->stream));
+        HIP_CHECK(hipMemcpyAsync(sp + 24 * P, d.aid, 4 * P, hipMemcpyDeviceToDevice, s->stream));
+        HIP_CHECK(hipMemcpyAsync(sp + 28 * P, d.pressure, 4 * C, hipMemcpyDeviceToDevice, s->stream));
+        const uint64_t steps_before = s->steps_done;
+        const int candidates[] = {1, 2, 4, 8};
+        int best = d.zb;
+        double best_t = 1e300;
+        for (int zb : candidates) {
+            if (zb > d.nz) continue;
+            d.zb = zb;
+            s->timer.clear();
+            s->timer.on = true;
+            s->run_steps(steps);
+            s->timer.on = false;
+            double t = 0.0;
+            for (const auto &r : s->timer.recs) {
+                float ms = 0.0f;
+                HIP_CHECK(hipEventElapsedTime(&ms, r.a, r.b));
+                t += ms;
+            }
+            s->timer.clear();
+            if (t < best_t) {
+                best_t = t;
+                best = zb;
+            }
+            for (int a = 0; a < 6; a++)
+                HIP_CHECK(hipMemcpyAsync(src[a], sp + 4 * P * a, 4 * P, hipMemcpyDeviceToDevice, s->stream));
+            HIP_CHECK(hipMemcpyAsync(d.aid, sp + 24 * P, 4 * P, hipMemcpyDeviceToDevice, s->stream));
+            HIP_CHECK(hipMemcpyAsync(d.pressure, sp + 28 * P, 4 * C, hipMemcpyDeviceToDevice, s->stream));
+            HIP_CHECK(hipStreamSynchronize(s->stream));
+        }
+        d.zb = best;
+        s->drop_graphs();
+        s->steps_done = steps_before;
+        if (chosen_zbatch) *chosen_zbatch = (uint32_t)best;
+        if (best_ms) *best_ms = best_t / steps;
+    });
 }
 
 int pxlflip_last_solve(const pxlflip *h, pxlflip_solve_stats *st) {

@@ -32,12 +32,15 @@ __device__ __forceinline__ uint32_t widx(const DevSim &S, int i, int j, int k) {
     return S.nu + S.nv + (uint32_t)i + (uint32_t)S.nx * ((uint32_t)j + (uint32_t)S.ny * (uint32_t)k);
 }
 
-// normalize_grid faces + forces (fluid.slang:119-124, 143-151) for one face: v = acc / wacc
-// (0 without weight) -> vel_old; vel = v + g_c * dt, or 0 when either side is SOLID.
-// SOLID depends only on the cell position, so this needs no classified cells.
-__device__ __forceinline__ void face_update(const DevSim &S, uint32_t f, float g, bool keep) {
-    const int32_t w = S.wacc[f];
-    const float v = w > 0 ? (float)S.acc[f] / (float)w : 0.0f;
+// normalize_grid faces + forces (fluid.slang:119-124, 143-151) for one face: the P2G sums
+// acc, wacc are gathered from the tile boxes that cover the face; v = acc / wacc (0 without
+// weight) -> vel_old; vel = v + g_c * dt, or 0 when either side is SOLID.  SOLID depends only
+// on the cell position, so this needs no classified cells.
+__device__ __forceinline__ void face_update(const DevSim &S, int c, int i, int j, int k, uint32_t f, float g,
+                                            bool keep) {
+    const int64_t packed = gather_face(S, c, i, j, k);
+    const int32_t w = packed_wacc(packed);
+    const float v = w > 0 ? (float)packed_acc(packed) / (float)w : 0.0f;
     S.vel_old[f] = v;
     S.vel[f] = keep ? v + g * S.dt : 0.0f;
 }
@@ -48,9 +51,21 @@ __global__ __launch_bounds__(TX *TY) void k_faces(DevSim S) {
     const int k = blockIdx.z;
     if (i > S.nx || j > S.ny) return;
     const bool in_c = !is_solid(S, i, j, k);
-    if (j < S.ny && k < S.nz) face_update(S, uidx(S, i, j, k), S.gx, in_c && !is_solid(S, i - 1, j, k));
-    if (i < S.nx && k < S.nz) face_update(S, vidx(S, i, j, k), S.gy, in_c && !is_solid(S, i, j - 1, k));
-    if (i < S.nx && j < S.ny) face_update(S, widx(S, i, j, k), S.gz, in_c && !is_solid(S, i, j, k - 1));
+    if (j < S.ny && k < S.nz) face_update(S, 0, i, j, k, uidx(S, i, j, k), S.gx, in_c && !is_solid(S, i - 1, j, k));
+    if (i < S.nx && k < S.nz) face_update(S, 1, i, j, k, vidx(S, i, j, k), S.gy, in_c && !is_solid(S, i, j - 1, k));
+    if (i < S.nx && j < S.ny) face_update(S, 2, i, j, k, widx(S, i, j, k), S.gz, in_c && !is_solid(S, i, j, k - 1));
+}
+
+// The P2G sums as the reference's acc / wacc face buffers (for read-back).
+__global__ __launch_bounds__(TX *TY) void k_acc_fields(DevSim S, int32_t *acc, int32_t *wacc) {
+    const int i = blockIdx.x * TX + threadIdx.x;
+    const int j = blockIdx.y * TY + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i > S.nx || j > S.ny) return;
+    int64_t v;
+    if (j < S.ny && k < S.nz) { v = gather_face(S, 0, i, j, k); acc[uidx(S, i, j, k)] = packed_acc(v); wacc[uidx(S, i, j, k)] = packed_wacc(v); }
+    if (i < S.nx && k < S.nz) { v = gather_face(S, 1, i, j, k); acc[vidx(S, i, j, k)] = packed_acc(v); wacc[vidx(S, i, j, k)] = packed_wacc(v); }
+    if (i < S.nx && j < S.ny) { v = gather_face(S, 2, i, j, k); acc[widx(S, i, j, k)] = packed_acc(v); wacc[widx(S, i, j, k)] = packed_wacc(v); }
 }
 
 __device__ __forceinline__ uint8_t type_from_count(const DevSim &S, int i, int j, int k) {
@@ -166,6 +181,9 @@ void launch_rbgs(const DevSim &S, uint32_t parity, hipStream_t s) {
     k_rbgs<<<g, dim3(TX, TY), 0, s>>>(S, parity);
 }
 void launch_project(const DevSim &S, hipStream_t s) { k_project<<<node_grid(S), dim3(TX, TY), 0, s>>>(S); }
+void launch_acc_fields(const DevSim &S, int32_t *acc, int32_t *wacc, hipStream_t s) {
+    k_acc_fields<<<node_grid(S), dim3(TX, TY), 0, s>>>(S, acc, wacc);
+}
 void launch_type_u32(const DevSim &S, uint32_t *out, hipStream_t s) {
     k_type_u32<<<cell_grid(S), dim3(TX, TY), 0, s>>>(S, out);
 }

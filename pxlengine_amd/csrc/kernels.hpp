@@ -45,6 +45,7 @@ struct DevSim {
     uint32_t ncells, nfaces;
     int ntx, nty, ntz;  // tiles per axis
     uint32_t nkeys;     // ntx * nty * ntz * TILE_CELLS
+    int zb;             // z-layers per lane in the solver kernels (tunable)
     // particles: current state (a) and sorted scratch (b)
     float *ax, *ay, *az, *avx, *avy, *avz;
     uint32_t *aid;
@@ -53,7 +54,11 @@ struct DevSim {
     uint32_t *key, *slot;
     // faces
     float *vel, *vel_old;
-    int32_t *acc, *wacc;
+    // per-tile P2G sums: for each tile, component c and face of the tile's face box (see
+    // tile_box), the fixed-point sums packed as (sum_acc << 32) + sum_wacc in one int64
+    // (sum_wacc >= 0 and < 2^32, so the halves never carry into each other)
+    int64_t *box;
+    uint32_t ntiles;
     // cells (count/offset: tiled key order, offset[nkeys] = np)
     uint32_t *count, *offset;
     uint8_t *type;
@@ -71,6 +76,57 @@ struct DevSim {
     float *ix, *iy, *iz, *ivx, *ivy, *ivz;
 };
 
+// Face box of a tile for component c: every face a particle binned in the tile can touch.
+// Along c the faces [8t, 8t + 8] (9), along the other axes the faces at cells [8t - 1, 8t + 8]
+// (10): 900 faces per component, 2700 per tile.
+constexpr int BOX_FACES = 900;
+constexpr int BOX_ENTRIES = 3 * BOX_FACES;
+
+__device__ __forceinline__ void tile_box(int c, int ti, int tj, int tk, int lo[3], int d[3]) {
+    lo[0] = ti * TILE - 1;
+    lo[1] = tj * TILE - 1;
+    lo[2] = tk * TILE - 1;
+    d[0] = d[1] = d[2] = TILE + 2;
+    lo[c] += 1;
+    d[c] = TILE + 1;
+}
+
+__device__ __forceinline__ bool tile_active(const DevSim &S, uint32_t t) {
+    return S.offset[t * TILE_CELLS] != S.offset[(t + 1) * TILE_CELLS];
+}
+
+// Packed P2G sum of face q of component c: the sum over the (1 to 8) boxes of active tiles
+// that contain it.  Along c a face 8m lies in the boxes of tiles m-1 and m; along another
+// axis the face at cell 8m lies in tiles m-1 and m, at cell 8m+7 in tiles m and m+1.
+__device__ __forceinline__ int64_t gather_face(const DevSim &S, int c, int qx, int qy, int qz) {
+    const int q[3] = {qx, qy, qz};
+    const int nt[3] = {S.ntx, S.nty, S.ntz};
+    int tlo[3], thi[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const int m = q[a] >> 3, r = q[a] & 7;
+        tlo[a] = m - (r == 0 ? 1 : 0);
+        thi[a] = m + ((a != c && r == 7) ? 1 : 0);
+        tlo[a] = tlo[a] < 0 ? 0 : tlo[a];
+        thi[a] = thi[a] > nt[a] - 1 ? nt[a] - 1 : thi[a];
+    }
+    int64_t sum = 0;
+    for (int tz = tlo[2]; tz <= thi[2]; tz++)
+        for (int ty = tlo[1]; ty <= thi[1]; ty++)
+            for (int tx = tlo[0]; tx <= thi[0]; tx++) {
+                const uint32_t t = (uint32_t)tx + (uint32_t)S.ntx * ((uint32_t)ty + (uint32_t)S.nty * (uint32_t)tz);
+                if (!tile_active(S, t)) continue;
+                int lo[3], d[3];
+                tile_box(c, tx, ty, tz, lo, d);
+                const int li = (qx - lo[0]) + d[0] * ((qy - lo[1]) + d[1] * (qz - lo[2]));
+                sum += S.box[(size_t)t * BOX_ENTRIES + (size_t)c * BOX_FACES + li];
+            }
+    return sum;
+}
+
+__device__ __forceinline__ int32_t packed_acc(int64_t v) { return (int32_t)(v >> 32); }
+__device__ __forceinline__ int32_t packed_wacc(int64_t v) { return (int32_t)(uint32_t)(uint64_t)v; }
+
 __device__ __forceinline__ uint32_t tile_key(const DevSim &S, int i, int j, int k) {
     const uint32_t t = (uint32_t)(i >> 3) + (uint32_t)S.ntx * ((uint32_t)(j >> 3) + (uint32_t)S.nty * (uint32_t)(k >> 3));
     return t * TILE_CELLS + (uint32_t)((i & 7) | ((j & 7) << 3) | ((k & 7) << 6));
@@ -78,13 +134,13 @@ __device__ __forceinline__ uint32_t tile_key(const DevSim &S, int i, int j, int 
 
 // ---- launch helpers (all asynchronous on `s`) ----
 void launch_fill_u32(uint32_t *p, uint32_t n, uint32_t value, hipStream_t s);
-void launch_clear_grid(const DevSim &S, hipStream_t s);       // count, acc, wacc = 0
+void launch_clear_grid(const DevSim &S, hipStream_t s);       // count = 0
 void launch_bin_count(const DevSim &S, hipStream_t s);        // key, slot, count
 void launch_scan_offsets(const DevSim &S, hipStream_t s);     // offset = exclusive scan(count)
 uint32_t scan_tiles(uint32_t n);                              // scratch entries a scan of n needs
 void launch_exclusive_scan(const uint32_t *in, uint32_t n, uint32_t *tile_tmp, uint32_t *out, hipStream_t s);
 void launch_scatter(const DevSim &S, hipStream_t s);          // a -> b in cell order
-void launch_p2g(const DevSim &S, hipStream_t s);              // b -> acc, wacc
+void launch_p2g(const DevSim &S, hipStream_t s);              // b -> per-tile boxes
 void launch_faces(const DevSim &S, hipStream_t s);            // normalize + forces
 void launch_cells(const DevSim &S, hipStream_t s);            // type, stencil, p=0, div
 void launch_rbgs(const DevSim &S, uint32_t parity, hipStream_t s);
@@ -113,5 +169,6 @@ void launch_unpack_aos8(const DevSim &S, float *aos8, hipStream_t s);        // 
 void launch_pack_aos8(const DevSim &S, const float *aos8, hipStream_t s);    // AoS8 -> a, id = i
 void launch_reset_particles(const DevSim &S, hipStream_t s);                 // initial -> a
 void launch_type_u32(const DevSim &S, uint32_t *out, hipStream_t s);
+void launch_acc_fields(const DevSim &S, int32_t *acc, int32_t *wacc, hipStream_t s);  // boxes -> face sums
 
 }  // namespace pxl

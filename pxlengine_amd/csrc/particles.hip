@@ -35,13 +35,10 @@ __global__ __launch_bounds__(BLOCK) void k_fill_u32(uint32_t *p, uint32_t n, uin
     if (i < n) p[i] = value;
 }
 
-// clear_grid (fluid.slang:88-91): acc, wacc over faces, count over the (tiled) cells.
+// clear_grid (fluid.slang:88-91).  The P2G sums live in per-tile boxes that P2G overwrites
+// whole, so only the cell counts need clearing.
 __global__ __launch_bounds__(BLOCK) void k_clear_grid(DevSim S) {
     uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i < S.nfaces) {
-        S.acc[i] = 0;
-        S.wacc[i] = 0;
-    }
     if (i < S.nkeys) S.count[i] = 0;
 }
 
@@ -208,41 +205,30 @@ __device__ __forceinline__ void splat_component(const DevSim &S, int c, const fl
     }
 }
 
-// LDS box of one tile's faces for component c: the faces any particle of the tile's
-// cells can reach, i.e. cells [t0 - 1, t0 + TILE] on the axes other than c and faces
-// [t0, t0 + TILE] along c.  Dimensions 9 x 10 x 10 (permuted per component) = 900.
-constexpr int PBOX = 900;
-
-__device__ __forceinline__ void box_of(int c, int i0, int j0, int k0, int lo[3], int d[3]) {
-    lo[0] = i0 - 1; lo[1] = j0 - 1; lo[2] = k0 - 1;
-    d[0] = d[1] = d[2] = TILE + 2;
-    lo[c] += 1;
-    d[c] = TILE + 1;
-}
-
 __device__ __forceinline__ uint32_t face_index(const DevSim &S, int c, int qx, int qy, int qz) {
     const uint32_t dx = (uint32_t)S.nx + (c == 0), dy = (uint32_t)S.ny + (c == 1);
     return face_base(S, c) + (uint32_t)qx + dx * ((uint32_t)qy + dy * (uint32_t)qz);
 }
 
-// p2g (fluid.slang:94-112) for one 8^3 tile of cells per workgroup: the tile's particles are
-// contiguous in the cell-sorted scratch copy; their fixed-point splats are summed with LDS
-// atomics into the tile's face box, then the box is added to the global faces (one global
-// atomic per touched face and tile instead of one per particle and corner).  Integer sums:
-// the result does not depend on the order of additions.
+__device__ __forceinline__ void tile_coords(const DevSim &S, uint32_t t, int &ti, int &tj, int &tk) {
+    ti = (int)(t % (uint32_t)S.ntx);
+    tj = (int)((t / (uint32_t)S.ntx) % (uint32_t)S.nty);
+    tk = (int)(t / ((uint32_t)S.ntx * (uint32_t)S.nty));
+}
+
+// p2g (fluid.slang:94-112) for one 8^3 tile of cells per workgroup.  The tile's particles are
+// one contiguous run of the cell-sorted scratch copy; their fixed-point splats (acc and wacc
+// packed into one int64) are summed with LDS atomics into the tile's face box, and the box is
+// stored whole to HBM with plain stores.  The face pass adds the boxes that cover each face,
+// so no global atomics are needed.  Integer sums: the order of additions does not matter.
 __global__ __launch_bounds__(BLOCK) void k_p2g_tile(DevSim S) {
-    __shared__ int lacc[3 * PBOX];
-    __shared__ int lwt[3 * PBOX];
+    __shared__ unsigned long long lbox[BOX_ENTRIES];
     const uint32_t t = blockIdx.x;
     const uint32_t beg = S.offset[t * TILE_CELLS], end = S.offset[(t + 1) * TILE_CELLS];
     if (beg == end) return;
-    const int ti = (int)(t % (uint32_t)S.ntx), tj = (int)((t / (uint32_t)S.ntx) % (uint32_t)S.nty),
-              tk = (int)(t / ((uint32_t)S.ntx * (uint32_t)S.nty));
-    const int i0 = ti * TILE, j0 = tj * TILE, k0 = tk * TILE;
-    for (int q = threadIdx.x; q < 3 * PBOX; q += BLOCK) {
-        lacc[q] = 0;
-        lwt[q] = 0;
-    }
+    int ti, tj, tk;
+    tile_coords(S, t, ti, tj, tk);
+    for (int q = threadIdx.x; q < BOX_ENTRIES; q += BLOCK) lbox[q] = 0ull;
     __syncthreads();
     // lanes of a wave take every 8th particle of a 256-run, so they land in different cells
     const uint32_t local = (threadIdx.x & 31u) * 8u + (threadIdx.x >> 5);
@@ -254,38 +240,28 @@ __global__ __launch_bounds__(BLOCK) void k_p2g_tile(DevSim S) {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             int lo[3], d[3];
-            box_of(c, i0, j0, k0, lo, d);
+            tile_box(c, ti, tj, tk, lo, d);
             splat_component(S, c, x, v[c], [&](int qx, int qy, int qz, int a, int w) {
                 const int lx = qx - lo[0], ly = qy - lo[1], lz = qz - lo[2];
+                // every in-grid corner of a particle binned in this tile lies in the box
                 if ((unsigned)lx < (unsigned)d[0] && (unsigned)ly < (unsigned)d[1] && (unsigned)lz < (unsigned)d[2]) {
-                    const int li = c * PBOX + lx + d[0] * (ly + d[1] * lz);
-                    atomicAdd(&lacc[li], a);
-                    atomicAdd(&lwt[li], w);
-                } else {  // outside the box (cannot happen for a particle binned in this tile)
-                    const uint32_t fi = face_index(S, c, qx, qy, qz);
-                    atomicAdd(&S.acc[fi], a);
-                    atomicAdd(&S.wacc[fi], w);
+                    const unsigned long long packed =
+                        ((unsigned long long)(uint32_t)a << 32) + (unsigned long long)(uint32_t)w;
+                    atomicAdd(&lbox[c * BOX_FACES + lx + d[0] * (ly + d[1] * lz)], packed);
                 }
             });
         }
     }
     __syncthreads();
-    for (int q = threadIdx.x; q < 3 * PBOX; q += BLOCK) {
-        const int a = lacc[q], w = lwt[q];
-        if ((a | w) == 0) continue;
-        const int c = q / PBOX, r = q - c * PBOX;
-        int lo[3], d[3];
-        box_of(c, i0, j0, k0, lo, d);
-        const int qx = lo[0] + r % d[0], qy = lo[1] + (r / d[0]) % d[1], qz = lo[2] + r / (d[0] * d[1]);
-        const uint32_t fi = face_index(S, c, qx, qy, qz);
-        if (a) atomicAdd(&S.acc[fi], a);
-        if (w) atomicAdd(&S.wacc[fi], w);
-    }
+    int64_t *out = S.box + (size_t)t * BOX_ENTRIES;
+    for (int q = threadIdx.x; q < BOX_ENTRIES; q += BLOCK) out[q] = (int64_t)lbox[q];
 }
 
-// sample_face (fluid.slang:67-80): trilinear gather with indices clamped to the face grid.
-__device__ __forceinline__ float sample_face(const DevSim &S, const float *__restrict__ buf, int c, float x,
-                                             float y, float z) {
+// sample_face from the tile's LDS copy of a face field when all 8 clamped corners are in the
+// box (always true at the particle's own position), else from HBM.  Same arithmetic either way.
+__device__ __forceinline__ float sample_face_box(const DevSim &S, const float *__restrict__ lds,
+                                                 const float *__restrict__ buf, int c, int ti, int tj, int tk,
+                                                 float x, float y, float z) {
     int dim[3] = {S.nx, S.ny, S.nz};
     dim[c] += 1;
     float g[3] = {(x - S.ox) * S.inv_h - 0.5f, (y - S.oy) * S.inv_h - 0.5f, (z - S.oz) * S.inv_h - 0.5f};
@@ -297,66 +273,102 @@ __device__ __forceinline__ float sample_face(const DevSim &S, const float *__res
         b[a] = (int)floorf(g[a]);
         f[a] = g[a] - (float)b[a];
     }
+    int lo[3], d[3];
+    tile_box(c, ti, tj, tk, lo, d);
+    int q0[3], q1[3];
+    bool in_box = true;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        q0[a] = b[a] < 0 ? 0 : (b[a] > dim[a] - 1 ? dim[a] - 1 : b[a]);
+        q1[a] = b[a] + 1 < 0 ? 0 : (b[a] + 1 > dim[a] - 1 ? dim[a] - 1 : b[a] + 1);
+        in_box = in_box && q0[a] >= lo[a] && q1[a] < lo[a] + d[a];
+    }
     const uint32_t base = face_base(S, c);
     float s = 0.0f;
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         const int ox = k & 1, oy = (k >> 1) & 1, oz = k >> 2;
-        int qx = b[0] + ox, qy = b[1] + oy, qz = b[2] + oz;
-        qx = qx < 0 ? 0 : (qx > dim[0] - 1 ? dim[0] - 1 : qx);
-        qy = qy < 0 ? 0 : (qy > dim[1] - 1 ? dim[1] - 1 : qy);
-        qz = qz < 0 ? 0 : (qz > dim[2] - 1 ? dim[2] - 1 : qz);
+        const int qx = ox ? q1[0] : q0[0], qy = oy ? q1[1] : q0[1], qz = oz ? q1[2] : q0[2];
         const float wx = ox ? f[0] : 1.0f - f[0];
         const float wy = oy ? f[1] : 1.0f - f[1];
         const float wz = oz ? f[2] : 1.0f - f[2];
-        s = s + wx * wy * wz * buf[base + (uint32_t)qx + (uint32_t)dim[0] * ((uint32_t)qy + (uint32_t)dim[1] * (uint32_t)qz)];
+        const float val = in_box ? lds[(qx - lo[0]) + d[0] * ((qy - lo[1]) + d[1] * (qz - lo[2]))]
+                                 : buf[base + (uint32_t)qx + (uint32_t)dim[0] * ((uint32_t)qy + (uint32_t)dim[1] * (uint32_t)qz)];
+        s = s + wx * wy * wz * val;
     }
     return s;
 }
 
-// g2p_advect (fluid.slang:200-220): PIC/FLIP blend (lerp = a*(1-t) + b*t), RK2 through the
-// projected grid velocity, clamp into the interior with the wall-normal velocity clipped
-// (Slang max/min = SPIR-V FMax/FMin).  Reads the sorted scratch copy, writes the current
-// particle arrays at the same (sorted) index.
-__global__ __launch_bounds__(BLOCK) void k_g2p(DevSim S) {
-    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= S.np) return;
-    float x[3] = {S.bx[i], S.by[i], S.bz[i]};
-    const float v0[3] = {S.bvx[i], S.bvy[i], S.bvz[i]};
-    float pic[3], old[3], v[3], mid[3];
-#pragma unroll
-    for (int c = 0; c < 3; c++) pic[c] = sample_face(S, S.vel, c, x[0], x[1], x[2]);
-#pragma unroll
-    for (int c = 0; c < 3; c++) old[c] = sample_face(S, S.vel_old, c, x[0], x[1], x[2]);
-    const float t = S.flip;
+// g2p_advect (fluid.slang:200-220), one 8^3 tile per workgroup: the tile's vel and vel_old
+// face boxes are staged in LDS, then every particle of the tile does the PIC/FLIP blend
+// (lerp = a*(1-t) + b*t), RK2 through the projected grid velocity (the midpoint sample
+// falls back to HBM when it leaves the box) and the wall clamp with the wall-normal velocity
+// clipped (Slang max/min = SPIR-V FMax/FMin).  Reads the sorted scratch copy, writes the
+// current particle arrays at the same (sorted) index.
+__global__ __launch_bounds__(BLOCK) void k_g2p_tile(DevSim S) {
+    __shared__ float lvel[BOX_ENTRIES];
+    __shared__ float lold[BOX_ENTRIES];
+    const uint32_t t = blockIdx.x;
+    const uint32_t beg = S.offset[t * TILE_CELLS], end = S.offset[(t + 1) * TILE_CELLS];
+    if (beg == end) return;
+    int ti, tj, tk;
+    tile_coords(S, t, ti, tj, tk);
+    for (int q = threadIdx.x; q < BOX_ENTRIES; q += BLOCK) {
+        const int c = q / BOX_FACES, r = q - c * BOX_FACES;
+        int lo[3], d[3];
+        tile_box(c, ti, tj, tk, lo, d);
+        const int qx = lo[0] + r % d[0], qy = lo[1] + (r / d[0]) % d[1], qz = lo[2] + r / (d[0] * d[1]);
+        const int dx = S.nx + (c == 0), dy = S.ny + (c == 1), dz = S.nz + (c == 2);
+        float a = 0.0f, o = 0.0f;
+        if (qx >= 0 && qy >= 0 && qz >= 0 && qx < dx && qy < dy && qz < dz) {
+            const uint32_t fi = face_index(S, c, qx, qy, qz);
+            a = S.vel[fi];
+            o = S.vel_old[fi];
+        }
+        lvel[q] = a;
+        lold[q] = o;
+    }
+    __syncthreads();
+    const float tb = S.flip;
     const float half_dt = 0.5f * S.dt;
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        const float flipv = v0[c] + pic[c] - old[c];
-        v[c] = pic[c] * (1.0f - t) + flipv * t;
-        mid[c] = x[c] + half_dt * pic[c];
-    }
-    float adv[3];
-#pragma unroll
-    for (int c = 0; c < 3; c++) adv[c] = sample_face(S, S.vel, c, mid[0], mid[1], mid[2]);
-#pragma unroll
-    for (int c = 0; c < 3; c++) x[c] = x[c] + S.dt * adv[c];
-    const float o[3] = {S.ox, S.oy, S.oz};
+    const float org[3] = {S.ox, S.oy, S.oz};
     const float nf[3] = {(float)S.nx, (float)S.ny, (float)S.nz};
+    for (uint32_t i = beg + threadIdx.x; i < end; i += BLOCK) {
+        float x[3] = {S.bx[i], S.by[i], S.bz[i]};
+        const float v0[3] = {S.bvx[i], S.bvy[i], S.bvz[i]};
+        float pic[3], old[3], v[3], mid[3], adv[3];
 #pragma unroll
-    for (int a = 0; a < 3; a++) {
-        const float lo = o[a] + S.h * 1.001f;
-        const float hi = o[a] + S.h * (nf[a] - 1.001f);
-        if (x[a] < lo) { x[a] = lo; v[a] = v[a] < 0.0f ? 0.0f : v[a]; }
-        if (x[a] > hi) { x[a] = hi; v[a] = 0.0f < v[a] ? 0.0f : v[a]; }
+        for (int c = 0; c < 3; c++)
+            pic[c] = sample_face_box(S, lvel + c * BOX_FACES, S.vel, c, ti, tj, tk, x[0], x[1], x[2]);
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            old[c] = sample_face_box(S, lold + c * BOX_FACES, S.vel_old, c, ti, tj, tk, x[0], x[1], x[2]);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float flipv = v0[c] + pic[c] - old[c];
+            v[c] = pic[c] * (1.0f - tb) + flipv * tb;
+            mid[c] = x[c] + half_dt * pic[c];
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            adv[c] = sample_face_box(S, lvel + c * BOX_FACES, S.vel, c, ti, tj, tk, mid[0], mid[1], mid[2]);
+#pragma unroll
+        for (int c = 0; c < 3; c++) x[c] = x[c] + S.dt * adv[c];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const float lo = org[a] + S.h * 1.001f;
+            const float hi = org[a] + S.h * (nf[a] - 1.001f);
+            if (x[a] < lo) { x[a] = lo; v[a] = v[a] < 0.0f ? 0.0f : v[a]; }
+            if (x[a] > hi) { x[a] = hi; v[a] = 0.0f < v[a] ? 0.0f : v[a]; }
+        }
+        S.ax[i] = x[0];
+        S.ay[i] = x[1];
+        S.az[i] = x[2];
+        S.avx[i] = v[0];
+        S.avy[i] = v[1];
+        S.avz[i] = v[2];
+        S.aid[i] = S.bid[i];
     }
-    S.ax[i] = x[0];
-    S.ay[i] = x[1];
-    S.az[i] = x[2];
-    S.avx[i] = v[0];
-    S.avy[i] = v[1];
-    S.avz[i] = v[2];
-    S.aid[i] = S.bid[i];
 }
 
 // ---------------------------------------------------------------- IO / bins
@@ -436,10 +448,7 @@ void launch_fill_u32(uint32_t *p, uint32_t n, uint32_t value, hipStream_t s) {
     if (n) k_fill_u32<<<blocks_for(n), BLOCK, 0, s>>>(p, n, value);
 }
 
-void launch_clear_grid(const DevSim &S, hipStream_t s) {
-    uint32_t n = S.nfaces > S.nkeys ? S.nfaces : S.nkeys;
-    k_clear_grid<<<blocks_for(n), BLOCK, 0, s>>>(S);
-}
+void launch_clear_grid(const DevSim &S, hipStream_t s) { k_clear_grid<<<blocks_for(S.nkeys), BLOCK, 0, s>>>(S); }
 
 void launch_bin_count(const DevSim &S, hipStream_t s) {
     k_bin_count<<<blocks_for(S.np), BLOCK, 0, s>>>(S);
@@ -458,8 +467,8 @@ void launch_exclusive_scan(const uint32_t *in, uint32_t n, uint32_t *tile_tmp, u
 void launch_scan_offsets(const DevSim &S, hipStream_t s) { launch_exclusive_scan(S.count, S.nkeys, S.scan_tmp, S.offset, s); }
 
 void launch_scatter(const DevSim &S, hipStream_t s) { k_scatter<<<blocks_for(S.np), BLOCK, 0, s>>>(S); }
-void launch_p2g(const DevSim &S, hipStream_t s) { k_p2g_tile<<<S.nkeys / TILE_CELLS, BLOCK, 0, s>>>(S); }
-void launch_g2p(const DevSim &S, hipStream_t s) { k_g2p<<<blocks_for(S.np), BLOCK, 0, s>>>(S); }
+void launch_p2g(const DevSim &S, hipStream_t s) { k_p2g_tile<<<S.ntiles, BLOCK, 0, s>>>(S); }
+void launch_g2p(const DevSim &S, hipStream_t s) { k_g2p_tile<<<S.ntiles, BLOCK, 0, s>>>(S); }
 
 void launch_bins_out(const DevSim &S, const uint32_t *off_lin, uint32_t *cell_of, uint32_t *sorted, hipStream_t s) {
     k_bins_out<<<blocks_for(S.np), BLOCK, 0, s>>>(S, off_lin, cell_of, sorted);

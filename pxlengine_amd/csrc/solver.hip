@@ -29,7 +29,7 @@ namespace pxl {
 
 namespace {
 
-constexpr int TX = 64, TY = 4, ZB = 4;  // a lane walks ZB consecutive z-layers
+constexpr int TX = 64, TY = 4;  // a lane walks S.zb consecutive z-layers
 
 __device__ __forceinline__ float diag_of(uint32_t st) { return (float)((st >> 8) & 7u); }
 
@@ -48,8 +48,8 @@ __device__ __forceinline__ Lane lane_cells(const DevSim &S) {
     Lane l;
     l.i = blockIdx.x * TX + threadIdx.x;
     l.j = blockIdx.y * TY + threadIdx.y;
-    l.k0 = blockIdx.z * ZB;
-    l.k1 = min(l.k0 + ZB, S.nz);
+    l.k0 = blockIdx.z * S.zb;
+    l.k1 = min(l.k0 + S.zb, S.nz);
     l.valid = l.i < S.nx && l.j < S.ny;
     return l;
 }
@@ -309,7 +309,7 @@ inline float *s_buf(const DevSim &S, uint32_t k) { return (k & 1) ? S.cg_s1 : S.
 
 }  // namespace
 
-dim3 solver_grid(const DevSim &S) { return dim3((S.nx + TX - 1) / TX, (S.ny + TY - 1) / TY, (S.nz + ZB - 1) / ZB); }
+dim3 solver_grid(const DevSim &S) { return dim3((S.nx + TX - 1) / TX, (S.ny + TY - 1) / TY, (S.nz + S.zb - 1) / S.zb); }
 
 uint32_t solver_blocks(const DevSim &S) {
     const dim3 g = solver_grid(S);

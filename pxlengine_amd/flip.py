@@ -51,6 +51,7 @@ class Opts(C.Structure):
         ("struct_size", C.c_uint32), ("solver", C.c_int32), ("cg_iterations", C.c_uint32),
         ("inner_iterations", C.c_uint32), ("max_outer", C.c_uint32), ("tolerance", C.c_double),
         ("device", C.c_int32), ("scene_index", C.c_uint32), ("use_graph", C.c_int32),
+        ("solver_zbatch", C.c_uint32),
     ]
 
 
@@ -106,6 +107,7 @@ _SIGNATURES = {
     "pxlflip_grid_phase": (C.c_int, [_H]),
     "pxlflip_last_solve": (C.c_int, [_H, C.POINTER(SolveStats)]),
     "pxlflip_profile_steps": (C.c_int, [_H, C.c_uint32, C.POINTER(KernelTime), C.c_uint32, C.POINTER(C.c_uint32)]),
+    "pxlflip_autotune": (C.c_int, [_H, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_double)]),
     "pxlflip_last_error": (C.c_char_p, [_H]),
 }
 
@@ -150,7 +152,7 @@ def _json(fluid) -> bytes:
 
 
 def make_opts(solver="rbgs", cg_iterations=0, inner_iterations=40, max_outer=50, tolerance=1e-6,
-              device=0, scene_index=0, use_graph=True) -> Opts:
+              device=0, scene_index=0, use_graph=True, solver_zbatch=0) -> Opts:
     o = Opts()
     load_library().pxlflip_default_opts(C.byref(o))
     o.solver = SOLVERS[solver] if isinstance(solver, str) else int(solver)
@@ -161,6 +163,7 @@ def make_opts(solver="rbgs", cg_iterations=0, inner_iterations=40, max_outer=50,
     o.device = device
     o.scene_index = scene_index
     o.use_graph = 1 if use_graph else 0
+    o.solver_zbatch = solver_zbatch
     return o
 
 
@@ -288,6 +291,12 @@ class FluidState:
         self._call(self._lib.pxlflip_profile_steps, int(n), arr, 64, C.byref(cnt))
         return [(arr[k].name.decode(), int(arr[k].launches), float(arr[k].total_ms))
                 for k in range(min(cnt.value, 64))]
+
+    def autotune(self, steps: int = 2) -> dict:
+        """Picks the solver z-batch by timing `steps` steps per candidate; state unchanged."""
+        zb, ms = C.c_uint32(), C.c_double()
+        self._call(self._lib.pxlflip_autotune, int(steps), C.byref(zb), C.byref(ms))
+        return {"solver_zbatch": int(zb.value), "ms_per_step": float(ms.value)}
 
     def last_solve(self) -> dict:
         st = SolveStats()
