@@ -40,7 +40,6 @@ def algorithmic_bytes(name: str, P: int, F: int, C: int, Cf: int) -> float:
         "rbgs": C + 8 * Cf,
         "cg_start": 2 * C + 12 * Cf + 2 * C + 16 * Cf,
         "cg_iter": 2 * C + 40 * Cf,
-        "cg_finalize": 0,
         "project": 8 * F + 5 * C,
         "g2p": 28 * P + 8 * F + 28 * P,
     }

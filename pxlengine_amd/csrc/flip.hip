@@ -152,9 +152,9 @@ struct Sim {
             add(&d.x64b, 8 * C);
             add(&d.r64, 8 * C);
         }
-        add(&d.part32, 4 * 2 * nb);
-        add(&d.part64, 8 * 2 * nb);
-        add(&d.scal, 8 * SC_COUNT);
+        add(&d.part32, 4 * 4 * nb);
+        add(&d.part64, 8 * 4 * nb);
+        add(&d.scal, 8 * 2 * SC_COUNT);
         add(&d.ctr, 64);
         add(&d.scan_tmp, 4 * (scan_tiles(K) + 1));
         return plan;
@@ -211,11 +211,7 @@ struct Sim {
             }
         } else {
             KRUN("cg_start", launch_cg_start(d, false, d.pressure, false, stream));
-            KRUN("cg_finalize", launch_cg_finalize(d, false, true, stream));
-            for (uint32_t it = 0; it < iters; it++) {
-                KRUN("cg_iter", launch_cg_iter(d, false, d.pressure, it, stream));
-                KRUN("cg_finalize", launch_cg_finalize(d, false, false, stream));
-            }
+            for (uint32_t it = 0; it < iters; it++) KRUN("cg_iter", launch_cg_iter(d, false, d.pressure, it, stream));
         }
     }
     void enqueue_post() {
@@ -229,11 +225,7 @@ struct Sim {
     }
     void enqueue_inner(int cur) {
         KRUN("cg64_start", launch_cg_start(d, true, d.e, true, stream));
-        KRUN("cg64_finalize", launch_cg_finalize(d, true, true, stream));
-        for (uint32_t it = 0; it < opts.inner_iterations; it++) {
-            KRUN("cg64_iter", launch_cg_iter(d, true, d.e, it, stream));
-            KRUN("cg64_finalize", launch_cg_finalize(d, true, false, stream));
-        }
+        for (uint32_t it = 0; it < opts.inner_iterations; it++) KRUN("cg64_iter", launch_cg_iter(d, true, d.e, it, stream));
         KRUN("mixed_update", launch_mixed_update(d, x64(cur), x64(cur ^ 1), stream));
     }
     void enqueue_inner0() { enqueue_inner(0); }

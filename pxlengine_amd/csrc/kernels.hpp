@@ -31,7 +31,8 @@ constexpr uint16_t STEN_FLUID = 0x8000;
 constexpr int TILE = 8;
 constexpr uint32_t TILE_CELLS = 512;
 
-// Solver scalar slots (fp64 array).
+// Solver scalar slots (fp64 array): two slots of SC_COUNT by CG-iteration parity for the
+// (alpha, beta, gamma) history; the refinement norms live in slot 0.
 enum ScalarSlot { SC_RZ = 0, SC_ALPHA = 1, SC_BETA = 2, SC_RR = 3, SC_BB = 4, SC_COUNT = 8 };
 
 struct DevSim {
@@ -67,7 +68,7 @@ struct DevSim {
     // solver
     float *cg_r0, *cg_r1, *cg_w0, *cg_w1, *cg_s0, *cg_s1, *cg_p, *e;
     double *x64a, *x64b, *r64;
-    float *part32;
+    float *part32;   // two slots of 2 x (workgroups at z-batch 1)
     double *part64;
     double *scal;
     unsigned *ctr;
@@ -146,15 +147,13 @@ void launch_cells(const DevSim &S, hipStream_t s);            // type, stencil, 
 void launch_rbgs(const DevSim &S, uint32_t parity, hipStream_t s);
 void launch_project(const DevSim &S, hipStream_t s);
 void launch_g2p(const DevSim &S, hipStream_t s);              // b -> a, advected
-// Jacobi-preconditioned CG (solver.hip), one fused pass + one finalize per iteration:
-//   launch_cg_start(x, inner), launch_cg_finalize(first = true),
-//   then for it = 0..iters-1: launch_cg_iter(x, it), launch_cg_finalize(first = false).
+// Jacobi-preconditioned CG (solver.hip), one fused launch per iteration:
+//   launch_cg_start(x, inner), then for it = 0..iters-1: launch_cg_iter(x, it).
 // Plain solve: fp64 = false, x = pressure, inner = false.  Inner solve of the refinement:
 // fp64 = true (FP64 dot products), x = e, inner = true (b = float(r64), e starts at 0).
 dim3 solver_grid(const DevSim &S);
 uint32_t solver_blocks(const DevSim &S);  // workgroups of a solver launch (partials per value)
 void launch_cg_start(const DevSim &S, bool fp64, float *x, bool inner, hipStream_t s);
-void launch_cg_finalize(const DevSim &S, bool fp64, bool first, hipStream_t s);
 void launch_cg_iter(const DevSim &S, bool fp64, float *x, uint32_t it, hipStream_t s);
 // FP64 refinement (solver.hip); init/update also total the norms into scal[SC_RR / SC_BB].
 void launch_mixed_init(const DevSim &S, double *x64, hipStream_t s);  // x64 = p, r64 = b - A x64

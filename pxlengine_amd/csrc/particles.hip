@@ -14,6 +14,7 @@ namespace pxl {
 namespace {
 
 constexpr int SCAN_ITEMS = 16;
+constexpr int TILE_BLOCK = 1024;  // lanes of the per-tile P2G / G2P workgroups
 constexpr int SCAN_TILE = BLOCK * SCAN_ITEMS;  // 4096 cells per scan workgroup
 
 inline unsigned blocks_for(uint64_t n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
@@ -221,18 +222,19 @@ __device__ __forceinline__ void tile_coords(const DevSim &S, uint32_t t, int &ti
 // packed into one int64) are summed with LDS atomics into the tile's face box, and the box is
 // stored whole to HBM with plain stores.  The face pass adds the boxes that cover each face,
 // so no global atomics are needed.  Integer sums: the order of additions does not matter.
-__global__ __launch_bounds__(BLOCK) void k_p2g_tile(DevSim S) {
+__global__ __launch_bounds__(TILE_BLOCK) void k_p2g_tile(DevSim S) {
     __shared__ unsigned long long lbox[BOX_ENTRIES];
     const uint32_t t = blockIdx.x;
     const uint32_t beg = S.offset[t * TILE_CELLS], end = S.offset[(t + 1) * TILE_CELLS];
     if (beg == end) return;
     int ti, tj, tk;
     tile_coords(S, t, ti, tj, tk);
-    for (int q = threadIdx.x; q < BOX_ENTRIES; q += BLOCK) lbox[q] = 0ull;
+    for (int q = threadIdx.x; q < BOX_ENTRIES; q += TILE_BLOCK) lbox[q] = 0ull;
     __syncthreads();
-    // lanes of a wave take every 8th particle of a 256-run, so they land in different cells
-    const uint32_t local = (threadIdx.x & 31u) * 8u + (threadIdx.x >> 5);
-    for (uint32_t base = beg; base < end; base += BLOCK) {
+    // lanes of a wave take every 8th particle of a run, so concurrent LDS atomics land in
+    // different cells
+    const uint32_t local = (threadIdx.x & (TILE_BLOCK / 8 - 1)) * 8u + (threadIdx.x / (TILE_BLOCK / 8));
+    for (uint32_t base = beg; base < end; base += TILE_BLOCK) {
         const uint32_t p = base + local;
         if (p >= end) continue;
         const float x[3] = {S.bx[p], S.by[p], S.bz[p]};
@@ -254,7 +256,7 @@ __global__ __launch_bounds__(BLOCK) void k_p2g_tile(DevSim S) {
     }
     __syncthreads();
     int64_t *out = S.box + (size_t)t * BOX_ENTRIES;
-    for (int q = threadIdx.x; q < BOX_ENTRIES; q += BLOCK) out[q] = (int64_t)lbox[q];
+    for (int q = threadIdx.x; q < BOX_ENTRIES; q += TILE_BLOCK) out[q] = (int64_t)lbox[q];
 }
 
 // sample_face from the tile's LDS copy of a face field when all 8 clamped corners are in the
@@ -305,7 +307,7 @@ __device__ __forceinline__ float sample_face_box(const DevSim &S, const float *_
 // falls back to HBM when it leaves the box) and the wall clamp with the wall-normal velocity
 // clipped (Slang max/min = SPIR-V FMax/FMin).  Reads the sorted scratch copy, writes the
 // current particle arrays at the same (sorted) index.
-__global__ __launch_bounds__(BLOCK) void k_g2p_tile(DevSim S) {
+__global__ __launch_bounds__(TILE_BLOCK) void k_g2p_tile(DevSim S) {
     __shared__ float lvel[BOX_ENTRIES];
     __shared__ float lold[BOX_ENTRIES];
     const uint32_t t = blockIdx.x;
@@ -313,7 +315,7 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_tile(DevSim S) {
     if (beg == end) return;
     int ti, tj, tk;
     tile_coords(S, t, ti, tj, tk);
-    for (int q = threadIdx.x; q < BOX_ENTRIES; q += BLOCK) {
+    for (int q = threadIdx.x; q < BOX_ENTRIES; q += TILE_BLOCK) {
         const int c = q / BOX_FACES, r = q - c * BOX_FACES;
         int lo[3], d[3];
         tile_box(c, ti, tj, tk, lo, d);
@@ -333,7 +335,7 @@ __global__ __launch_bounds__(BLOCK) void k_g2p_tile(DevSim S) {
     const float half_dt = 0.5f * S.dt;
     const float org[3] = {S.ox, S.oy, S.oz};
     const float nf[3] = {(float)S.nx, (float)S.ny, (float)S.nz};
-    for (uint32_t i = beg + threadIdx.x; i < end; i += BLOCK) {
+    for (uint32_t i = beg + threadIdx.x; i < end; i += TILE_BLOCK) {
         float x[3] = {S.bx[i], S.by[i], S.bz[i]};
         const float v0[3] = {S.bvx[i], S.bvy[i], S.bvz[i]};
         float pic[3], old[3], v[3], mid[3], adv[3];
@@ -467,8 +469,8 @@ void launch_exclusive_scan(const uint32_t *in, uint32_t n, uint32_t *tile_tmp, u
 void launch_scan_offsets(const DevSim &S, hipStream_t s) { launch_exclusive_scan(S.count, S.nkeys, S.scan_tmp, S.offset, s); }
 
 void launch_scatter(const DevSim &S, hipStream_t s) { k_scatter<<<blocks_for(S.np), BLOCK, 0, s>>>(S); }
-void launch_p2g(const DevSim &S, hipStream_t s) { k_p2g_tile<<<S.ntiles, BLOCK, 0, s>>>(S); }
-void launch_g2p(const DevSim &S, hipStream_t s) { k_g2p_tile<<<S.ntiles, BLOCK, 0, s>>>(S); }
+void launch_p2g(const DevSim &S, hipStream_t s) { k_p2g_tile<<<S.ntiles, TILE_BLOCK, 0, s>>>(S); }
+void launch_g2p(const DevSim &S, hipStream_t s) { k_g2p_tile<<<S.ntiles, TILE_BLOCK, 0, s>>>(S); }
 
 void launch_bins_out(const DevSim &S, const uint32_t *off_lin, uint32_t *cell_of, uint32_t *sorted, hipStream_t s) {
     k_bins_out<<<blocks_for(S.np), BLOCK, 0, s>>>(S, off_lin, cell_of, sorted);

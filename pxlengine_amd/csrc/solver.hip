@@ -4,18 +4,22 @@
 //
 // Conjugate gradients with the Jacobi preconditioner M = diag(A), in the Chronopoulos-Gear
 // arrangement: both dot products of an iteration come out of ONE fused stencil pass, so an
-// iteration is one pass over the cells plus one single-workgroup finalize:
+// iteration is ONE kernel launch:
 //
-//   given alpha_i, beta_i:   p_i = u_i + beta_i p_{i-1}      s_i = w_i + beta_i s_{i-1}
-//                            x  += alpha_i p_i               r_{i+1} = r_i - alpha_i s_i
-//                            u_{i+1} = M^-1 r_{i+1}          w_{i+1} = A u_{i+1}
-//                            partials of gamma = (r_{i+1}, u_{i+1}), delta = (w_{i+1}, u_{i+1})
-//   finalize:                beta = gamma / gamma_old, alpha = gamma / (delta - beta gamma / alpha)
+//   scalars:  gamma, delta = sums of the previous pass's workgroup partials;
+//             beta = gamma / gamma_old, alpha = gamma / (delta - beta gamma / alpha_old)
+//             (first iteration: alpha = gamma / delta, beta = 0)
+//   update:   p_i = u_i + beta p_{i-1}      s_i = w_i + beta s_{i-1}
+//             x  += alpha p_i               r_{i+1} = r_i - alpha s_i
+//             u_{i+1} = M^-1 r_{i+1}        w_{i+1} = A u_{i+1}
+//   partials of gamma = (r_{i+1}, u_{i+1}) and delta = (w_{i+1}, u_{i+1}).
 //
+// Every workgroup sums the previous partials itself, in workgroup order with a fixed tree,
+// so all workgroups derive bitwise the same alpha and beta and no finalize launch or
+// cross-workgroup synchronisation is needed (the kernel boundary orders the passes).
+// Partials and the (alpha, gamma) history ping-pong between two slots by iteration parity.
 // w_{i+1} needs u_{i+1} at the six neighbours; the pass rebuilds them from r_i, w_i, s_{i-1}
-// read at the neighbours, so r, w and s ping-pong between two buffers each.  Workgroup
-// partials are plain stores; the finalize kernel (one workgroup, launched next on the same
-// stream) sums them in workgroup order, so the scalars do not depend on scheduling.
+// read at the neighbours, so r, w and s ping-pong between two buffers each.
 //
 // Accumulation type Acc = float (plain FP32 CG) or double (the FP64 dot products of the
 // mixed-precision solve); vector arithmetic is fp32 in both.
@@ -148,74 +152,102 @@ __global__ __launch_bounds__(TX *TY) void k_cg_start(DevSim S, const float *__re
     store_partials<Acc, 2>(v, partials<Acc>(S));
 }
 
-// ---- one fused iteration (see the file comment).
+// Scalars of iteration `it` from the previous pass's partials (identical in every workgroup).
 template <typename Acc>
-__global__ __launch_bounds__(TX *TY) void k_cg_iter(DevSim S, float *__restrict__ x, const float *__restrict__ r_in,
-                                                    float *__restrict__ r_out, const float *__restrict__ w_in,
-                                                    float *__restrict__ w_out, const float *__restrict__ s_prev,
-                                                    float *__restrict__ s_cur) {
-    const Lane l = lane_cells(S);
-    const float alpha = (float)S.scal[SC_ALPHA];
-    const float beta = (float)S.scal[SC_BETA];
+__device__ __forceinline__ void cg_scalars(const DevSim &S, uint32_t it, uint32_t nb, float &alpha, float &beta) {
+    __shared__ Acc sh[8];
+    const Acc *part = partials<Acc>(S) + (size_t)(it & 1) * 2 * nb;
+    const Acc g = sum_partials<Acc>(part, nb, 0, sh);
+    const Acc d = sum_partials<Acc>(part, nb, 1, sh + 4);
+    Acc a = 0, b = 0;
+    if (it == 0) {
+        a = (g != (Acc)0 && d != (Acc)0) ? g / d : (Acc)0;
+    } else {
+        const double *prev = S.scal + SC_COUNT * (it & 1);
+        const Acc g_old = (Acc)prev[SC_RZ], a_old = (Acc)prev[SC_ALPHA];
+        if (g != (Acc)0 && g_old != (Acc)0 && a_old != (Acc)0) {
+            b = g / g_old;
+            const Acc den = d - b * g / a_old;
+            a = den != (Acc)0 ? g / den : (Acc)0;
+        }
+    }
+    if (flat_block() == 0 && flat_tid() == 0) {
+        double *next = S.scal + SC_COUNT * ((it + 1) & 1);
+        next[SC_ALPHA] = (double)a;
+        next[SC_BETA] = (double)b;
+        next[SC_RZ] = (double)g;
+    }
+    alpha = (float)a;
+    beta = (float)b;
+}
+
+// ---- one fused iteration (see the file comment).  ZB cells per lane, fully unrolled; all
+// neighbour loads are issued unconditionally (a neighbour that is not FLUID reads the cell
+// itself, always in bounds) and selected afterwards, so a lane keeps all its loads in flight.
+template <typename Acc, int ZB>
+__global__ __launch_bounds__(TX *TY) void k_cg_iter(DevSim S, uint32_t it, float *__restrict__ x,
+                                                    const float *__restrict__ r_in, float *__restrict__ r_out,
+                                                    const float *__restrict__ w_in, float *__restrict__ w_out,
+                                                    const float *__restrict__ s_prev, float *__restrict__ s_cur) {
+    const uint32_t nb = num_blocks();
+    float alpha, beta;
+    cg_scalars<Acc>(S, it, nb, alpha, beta);
+    const int i = blockIdx.x * TX + threadIdx.x;
+    const int j = blockIdx.y * TY + threadIdx.y;
+    const int k0 = blockIdx.z * ZB;
+    const bool valid = i < S.nx && j < S.ny;
+    const float *__restrict__ pv = S.cg_p;
+    const uint16_t *__restrict__ sten = S.sten;
     Acc g = 0, d = 0;
-    if (l.valid) {
+    if (valid) {
         int32_t off[6];
         nb_offsets(S, off);
-        for (int k = l.k0; k < l.k1; k++) {
-            const uint32_t id = cidx(S, l.i, l.j, k);
-            const uint32_t st = S.sten[id];
-            if (!(st & STEN_FLUID)) continue;
-            const float ri = r_in[id];
-            const float pc = precond(ri, st) + beta * S.cg_p[id];
-            const float sc = w_in[id] + beta * s_prev[id];
-            x[id] = x[id] + alpha * pc;
+        uint32_t id[ZB], st[ZB];
+#pragma unroll
+        for (int z = 0; z < ZB; z++) {
+            const int k = k0 + z < S.nz ? k0 + z : S.nz - 1;
+            id[z] = cidx(S, i, j, k);
+            st[z] = (k0 + z < S.nz) ? (uint32_t)sten[id[z]] : 0u;
+        }
+#pragma unroll
+        for (int z = 0; z < ZB; z++) {
+            if (!(st[z] & STEN_FLUID)) continue;
+            const uint32_t c = id[z];
+            uint32_t q[6];
+#pragma unroll
+            for (int n = 0; n < 6; n++) q[n] = (st[z] & (1u << n)) ? c + off[n] : c;
+            const float ri = r_in[c], wi = w_in[c], si = s_prev[c], pi = pv[c], xi = x[c];
+            float rq[6], wq[6], sq[6];
+            uint32_t stq[6];
+#pragma unroll
+            for (int n = 0; n < 6; n++) {
+                rq[n] = r_in[q[n]];
+                wq[n] = w_in[q[n]];
+                sq[n] = s_prev[q[n]];
+                stq[n] = sten[q[n]];
+            }
+            const float pc = precond(ri, st[z]) + beta * pi;
+            const float sc = wi + beta * si;
             const float rn = ri - alpha * sc;
-            const float un = precond(rn, st);
+            const float un = precond(rn, st[z]);
             float sum = 0.0f;
 #pragma unroll
             for (int n = 0; n < 6; n++) {
-                if (st & (1u << n)) {
-                    const uint32_t q = id + off[n];
-                    const float sq = w_in[q] + beta * s_prev[q];
-                    sum = sum + precond(r_in[q] - alpha * sq, S.sten[q]);
-                }
+                const float un_n = precond(rq[n] - alpha * (wq[n] + beta * sq[n]), stq[n]);
+                sum = (st[z] & (1u << n)) ? sum + un_n : sum;
             }
-            const float wn = diag_of(st) * un - sum;
-            S.cg_p[id] = pc;
-            s_cur[id] = sc;
-            r_out[id] = rn;
-            w_out[id] = wn;
+            const float wn = diag_of(st[z]) * un - sum;
+            x[c] = xi + alpha * pc;
+            S.cg_p[c] = pc;
+            s_cur[c] = sc;
+            r_out[c] = rn;
+            w_out[c] = wn;
             g += (Acc)rn * (Acc)un;
             d += (Acc)wn * (Acc)un;
         }
     }
     const Acc v[2] = {g, d};
-    store_partials<Acc, 2>(v, partials<Acc>(S));
-}
-
-// ---- finalize: alpha, beta from the partials of gamma, delta (one workgroup).
-template <typename Acc>
-__global__ __launch_bounds__(BLOCK) void k_cg_finalize(DevSim S, uint32_t nb, int first) {
-    __shared__ Acc sh[8];
-    const Acc *part = partials<Acc>(S);
-    const Acc g = sum_partials<Acc>(part, nb, 0, sh);
-    const Acc d = sum_partials<Acc>(part, nb, 1, sh + 4);
-    if (threadIdx.x != 0) return;
-    Acc alpha = 0, beta = 0;
-    if (first) {
-        alpha = (g != (Acc)0 && d != (Acc)0) ? g / d : (Acc)0;
-    } else {
-        const Acc g_old = (Acc)S.scal[SC_RZ];
-        const Acc a_old = (Acc)S.scal[SC_ALPHA];
-        if (g != (Acc)0 && g_old != (Acc)0 && a_old != (Acc)0) {
-            beta = g / g_old;
-            const Acc den = d - beta * g / a_old;
-            alpha = den != (Acc)0 ? g / den : (Acc)0;
-        }
-    }
-    S.scal[SC_ALPHA] = (double)alpha;
-    S.scal[SC_BETA] = (double)beta;
-    S.scal[SC_RZ] = (double)g;
+    store_partials<Acc, 2>(v, partials<Acc>(S) + (size_t)((it + 1) & 1) * 2 * nb);
 }
 
 // ---- FP64 outer loop.  The solution is double-buffered (x_src -> x_dst) so the update pass
@@ -324,17 +356,27 @@ void launch_cg_start(const DevSim &S, bool fp64, float *x, bool inner, hipStream
     else k_cg_start<float><<<g, b, 0, s>>>(S, r_buf(S, 0), w_buf(S, 0), s_buf(S, 1));
 }
 
-void launch_cg_finalize(const DevSim &S, bool fp64, bool first, hipStream_t s) {
-    if (fp64) k_cg_finalize<double><<<1, BLOCK, 0, s>>>(S, solver_blocks(S), first ? 1 : 0);
-    else k_cg_finalize<float><<<1, BLOCK, 0, s>>>(S, solver_blocks(S), first ? 1 : 0);
-}
-
-void launch_cg_iter(const DevSim &S, bool fp64, float *x, uint32_t it, hipStream_t s) {
+template <typename Acc, int ZB>
+void launch_iter_zb(const DevSim &S, uint32_t it, float *x, hipStream_t s) {
     const dim3 g = solver_grid(S), b(TX, TY);
     float *rin = r_buf(S, it), *rout = r_buf(S, it + 1), *win = w_buf(S, it), *wout = w_buf(S, it + 1);
     float *sprev = s_buf(S, it + 1), *scur = s_buf(S, it);
-    if (fp64) k_cg_iter<double><<<g, b, 0, s>>>(S, x, rin, rout, win, wout, sprev, scur);
-    else k_cg_iter<float><<<g, b, 0, s>>>(S, x, rin, rout, win, wout, sprev, scur);
+    k_cg_iter<Acc, ZB><<<g, b, 0, s>>>(S, it, x, rin, rout, win, wout, sprev, scur);
+}
+
+template <typename Acc>
+void launch_iter_acc(const DevSim &S, uint32_t it, float *x, hipStream_t s) {
+    switch (S.zb) {
+        case 1: launch_iter_zb<Acc, 1>(S, it, x, s); break;
+        case 2: launch_iter_zb<Acc, 2>(S, it, x, s); break;
+        case 4: launch_iter_zb<Acc, 4>(S, it, x, s); break;
+        default: launch_iter_zb<Acc, 8>(S, it, x, s); break;
+    }
+}
+
+void launch_cg_iter(const DevSim &S, bool fp64, float *x, uint32_t it, hipStream_t s) {
+    if (fp64) launch_iter_acc<double>(S, it, x, s);
+    else launch_iter_acc<float>(S, it, x, s);
 }
 
 void launch_mixed_init(const DevSim &S, double *x64, hipStream_t s) {

@@ -211,9 +211,9 @@ def test_profile_steps_reports_kernels(pxl):
     g = pxl.FluidState(dam_break(32), FPS, seed=0, solver="cg", cg_iterations=10)
     prof = g.profile_steps(2)
     names = {n for n, _, _ in prof}
-    assert {"p2g", "g2p", "cg_iter", "cg_finalize", "scatter"} <= names
+    assert {"p2g", "g2p", "cg_iter", "cg_start", "scatter"} <= names
     d = {n: c for n, c, _ in prof}
-    assert d["cg_iter"] == 20 and d["cg_finalize"] == 22 and d["p2g"] == 2
+    assert d["cg_iter"] == 20 and d["cg_start"] == 2 and d["p2g"] == 2
     assert g.steps_done == 2
     g.close()
 
