@@ -56,7 +56,7 @@ typedef struct pxlflip_opts {
     int32_t device;           /* HIP device ordinal (default 0) */
     uint32_t scene_index;     /* i in "fluids[i] 'name': ..." validation messages */
     int32_t use_graph;        /* 1 = replay each step as a captured hipGraph (default 1) */
-    uint32_t solver_zbatch;   /* z-layers per lane in the solver kernels; 0 = default (1) */
+    uint32_t solver_workgroups; /* workgroups of the pressure-solver launches; 0 = default (1024), max 4096 */
 } pxlflip_opts;
 
 /* Fills *opts with the defaults above. */
@@ -178,11 +178,11 @@ typedef struct pxlflip_kernel_time {
 } pxlflip_kernel_time;
 int pxlflip_profile_steps(pxlflip *h, uint32_t n, pxlflip_kernel_time *out, uint32_t max_out, uint32_t *n_out);
 
-/* Autotuner of the solver launch shape: for each candidate z-batch (z-layers per lane in the
- * pressure kernels: 1, 2, 4, 8) runs `steps` timed steps, keeps the fastest, and restores the
+/* Autotuner of the solver launch shape: for each candidate workgroup count of the pressure
+ * kernels (256 .. 4096) runs `steps` timed steps, keeps the fastest, and restores the
  * particles, ids and pressure so the simulation state and steps_done are unchanged.  Returns
  * the choice and its average event-timed step time. */
-int pxlflip_autotune(pxlflip *h, uint32_t steps, uint32_t *chosen_zbatch, double *best_ms);
+int pxlflip_autotune(pxlflip *h, uint32_t steps, uint32_t *chosen_workgroups, double *best_ms);
 
 /* Last error message of a handle; with NULL, the calling thread's last create error. */
 const char *pxlflip_last_error(const pxlflip *h);

@@ -37,6 +37,8 @@ struct HipError {
     } while (0)
 
 constexpr size_t ALIGN = 256;
+constexpr uint32_t MAX_SOLVER_BLOCKS = 4096;
+constexpr uint32_t DEFAULT_SOLVER_BLOCKS = 1024;
 
 // Optional HIP-event bracketing of every launch (pxlflip_profile_steps).  Off while a graph
 // is captured or replayed.
@@ -124,9 +126,7 @@ struct Sim {
     // Every device array of the simulation, in arena order.
     Plan plan() {
         const size_t P = d.np, F = d.nfaces, C = d.ncells, K = d.nkeys;
-        DevSim one = d;
-        one.zb = 1;  // partials sized for the smallest z-batch the tuner may pick
-        const size_t nb = solver_blocks(one);
+        const size_t nb = MAX_SOLVER_BLOCKS;  // partials sized for the largest grid the tuner may pick
         Plan plan;
         auto add = [&](auto **p, size_t bytes) { plan.emplace_back(reinterpret_cast<void **>(p), bytes); };
         for (float **p : {&d.ax, &d.ay, &d.az, &d.avx, &d.avy, &d.avz}) add(p, 4 * P);
@@ -145,6 +145,11 @@ struct Sim {
         add(&d.sten, 2 * C);
         add(&d.pressure, 4 * C);
         add(&d.div, 4 * C);
+        add(&d.fl_flag, 4 * 2 * C);
+        add(&d.fl_off, 4 * 2 * C);
+        add(&d.fl_list, 4 * C);
+        add(&d.fl_count, 64);
+        add(&d.fl_tmp, 4 * (scan_tiles(2 * (uint32_t)C) + 1));
         for (float **p : {&d.cg_r0, &d.cg_r1, &d.cg_w0, &d.cg_w1, &d.cg_s0, &d.cg_s1, &d.cg_p}) add(p, 4 * C);
         if (opts.solver == PXLFLIP_SOLVER_MIXED) {
             add(&d.e, 4 * C);
@@ -202,6 +207,7 @@ struct Sim {
         KRUN("p2g", launch_p2g(d, stream));
         KRUN("faces", launch_faces(d, stream));
         KRUN("cells", launch_cells(d, stream));
+        KRUN("fluid_list", launch_fluid_list(d, stream));
     }
     void enqueue_fixed_pressure() {
         if (opts.solver == PXLFLIP_SOLVER_RBGS) {
@@ -373,7 +379,7 @@ int pxlflip_default_opts(pxlflip_opts *o) {
     o->device = 0;
     o->scene_index = 0;
     o->use_graph = 1;
-    o->solver_zbatch = 0;
+    o->solver_workgroups = 0;
     return PXLFLIP_OK;
 }
 
@@ -402,6 +408,10 @@ int prepare(const char *fluid_json, double fps, uint32_t seed, const pxlflip_opt
     }
     if (o.solver < PXLFLIP_SOLVER_RBGS || o.solver > PXLFLIP_SOLVER_MIXED) {
         g_create_error = "unknown solver " + std::to_string(o.solver);
+        return PXLFLIP_ERR_INVALID;
+    }
+    if (o.solver_workgroups > MAX_SOLVER_BLOCKS) {
+        g_create_error = "solver_workgroups must be <= " + std::to_string(MAX_SOLVER_BLOCKS);
         return PXLFLIP_ERR_INVALID;
     }
     if (o.solver == PXLFLIP_SOLVER_MIXED && (o.inner_iterations == 0 || !(o.tolerance > 0.0))) {
@@ -462,7 +472,7 @@ int prepare(const char *fluid_json, double fps, uint32_t seed, const pxlflip_opt
     d.ntz = (int)((n[2] + TILE - 1) / TILE);
     d.ntiles = (uint32_t)d.ntx * (uint32_t)d.nty * (uint32_t)d.ntz;
     d.nkeys = d.ntiles * TILE_CELLS;
-    d.zb = o.solver_zbatch ? (int)o.solver_zbatch : 1;
+    d.cg_blocks = (int)(o.solver_workgroups ? o.solver_workgroups : DEFAULT_SOLVER_BLOCKS);
     return PXLFLIP_OK;
 }
 
@@ -774,7 +784,7 @@ int pxlflip_profile_steps(pxlflip *h, uint32_t n, pxlflip_kernel_time *out, uint
     return PXLFLIP_OK;
 }
 
-int pxlflip_autotune(pxlflip *h, uint32_t steps, uint32_t *chosen_zbatch, double *best_ms) {
+int pxlflip_autotune(pxlflip *h, uint32_t steps, uint32_t *chosen_workgroups, double *best_ms) {
     if (!h || steps == 0) return PXLFLIP_ERR_INVALID;
     Sim *s = &h->sim;
     return pxl::guarded(s, [&] {
@@ -790,12 +800,11 @@ int pxlflip_autotune(pxlflip *h, uint32_t steps, uint32_t *chosen_zbatch, double
         HIP_CHECK(hipMemcpyAsync(sp + 24 * P, d.aid, 4 * P, hipMemcpyDeviceToDevice, s->stream));
         HIP_CHECK(hipMemcpyAsync(sp + 28 * P, d.pressure, 4 * C, hipMemcpyDeviceToDevice, s->stream));
         const uint64_t steps_before = s->steps_done;
-        const int candidates[] = {1, 2, 4, 8};
-        int best = d.zb;
+        const int candidates[] = {256, 512, 1024, 2048, 4096};
+        int best = d.cg_blocks;
         double best_t = 1e300;
-        for (int zb : candidates) {
-            if (zb > d.nz) continue;
-            d.zb = zb;
+        for (int g : candidates) {
+            d.cg_blocks = g;
             s->timer.clear();
             s->timer.on = true;
             s->run_steps(steps);
@@ -809,7 +818,7 @@ int pxlflip_autotune(pxlflip *h, uint32_t steps, uint32_t *chosen_zbatch, double
             s->timer.clear();
             if (t < best_t) {
                 best_t = t;
-                best = zb;
+                best = g;
             }
             for (int a = 0; a < 6; a++)
                 HIP_CHECK(hipMemcpyAsync(src[a], sp + 4 * P * a, 4 * P, hipMemcpyDeviceToDevice, s->stream));
@@ -817,10 +826,10 @@ int pxlflip_autotune(pxlflip *h, uint32_t steps, uint32_t *chosen_zbatch, double
             HIP_CHECK(hipMemcpyAsync(d.pressure, sp + 28 * P, 4 * C, hipMemcpyDeviceToDevice, s->stream));
             HIP_CHECK(hipStreamSynchronize(s->stream));
         }
-        d.zb = best;
+        d.cg_blocks = best;
         s->drop_graphs();
         s->steps_done = steps_before;
-        if (chosen_zbatch) *chosen_zbatch = (uint32_t)best;
+        if (chosen_workgroups) *chosen_workgroups = (uint32_t)best;
         if (best_ms) *best_ms = best_t / steps;
     });
 }

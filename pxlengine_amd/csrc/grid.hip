@@ -103,30 +103,20 @@ __global__ __launch_bounds__(TX *TY) void k_cells(DevSim S) {
     }
     S.sten[id] = st;
     S.div[id] = d * S.inv_h;
+    const bool red = (((uint32_t)(i + j + k)) & 1u) == 0;
+    S.fl_flag[id] = (t == CELL_FLUID && red) ? 1u : 0u;
+    S.fl_flag[S.ncells + id] = (t == CELL_FLUID && !red) ? 1u : 0u;
 }
 
-// pressure_rb (fluid.slang:168-182): one red-black half sweep.  Lanes walk only the cells
-// of the requested colour: i = 2*ih + ((parity + j + k) & 1).
-__global__ __launch_bounds__(TX *TY) void k_rbgs(DevSim S, uint32_t parity) {
-    const int ih = blockIdx.x * TX + threadIdx.x;
-    const int j = blockIdx.y * TY + threadIdx.y;
-    const int k = blockIdx.z;
-    const int i = 2 * ih + (int)((parity + (uint32_t)j + (uint32_t)k) & 1u);
-    if (i >= S.nx || j >= S.ny) return;
-    const uint32_t id = cidx(S, i, j, k);
-    const uint32_t st = S.sten[id];
-    if (!(st & STEN_FLUID)) return;
-    const uint32_t sx = 1, sy = (uint32_t)S.nx, sz = (uint32_t)S.nx * (uint32_t)S.ny;
-    float *p = S.pressure;
-    float sum = 0.0f;
-    if (st & 1u) sum = sum + p[id - sx];
-    if (st & 2u) sum = sum + p[id + sx];
-    if (st & 4u) sum = sum + p[id - sy];
-    if (st & 8u) sum = sum + p[id + sy];
-    if (st & 16u) sum = sum + p[id - sz];
-    if (st & 32u) sum = sum + p[id + sz];
-    const float cnt = (float)((st >> 8) & 7u);
-    if (cnt > 0.0f) p[id] = (sum - S.div[id] * S.h * S.h / S.dt) / cnt;
+// Fluid list from the scanned colour-split flags: entry e < C is cell e (red), e >= C is
+// cell e - C (black); fl_count = {red cells, all fluid cells}.
+__global__ __launch_bounds__(BLOCK) void k_fluid_compact(DevSim S) {
+    const uint32_t C = S.ncells, e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= 2 * C) return;
+    const uint32_t f = S.fl_flag[e], o = S.fl_off[e];
+    if (f) S.fl_list[o] = e < C ? e : e - C;
+    if (e == C) S.fl_count[0] = o;
+    if (e == 2 * C - 1) S.fl_count[1] = o + f;
 }
 
 __device__ __forceinline__ uint8_t type_at(const DevSim &S, int i, int j, int k) {
@@ -175,10 +165,9 @@ dim3 node_grid(const DevSim &S) { return dim3((S.nx + 1 + TX - 1) / TX, (S.ny + 
 
 void launch_faces(const DevSim &S, hipStream_t s) { k_faces<<<node_grid(S), dim3(TX, TY), 0, s>>>(S); }
 void launch_cells(const DevSim &S, hipStream_t s) { k_cells<<<cell_grid(S), dim3(TX, TY), 0, s>>>(S); }
-void launch_rbgs(const DevSim &S, uint32_t parity, hipStream_t s) {
-    const int half = (S.nx + 1) / 2;
-    dim3 g((half + TX - 1) / TX, (S.ny + TY - 1) / TY, S.nz);
-    k_rbgs<<<g, dim3(TX, TY), 0, s>>>(S, parity);
+void launch_fluid_list(const DevSim &S, hipStream_t s) {
+    launch_exclusive_scan(S.fl_flag, 2 * S.ncells, S.fl_tmp, S.fl_off, s);
+    k_fluid_compact<<<(2 * S.ncells + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(S);
 }
 void launch_project(const DevSim &S, hipStream_t s) { k_project<<<node_grid(S), dim3(TX, TY), 0, s>>>(S); }
 void launch_acc_fields(const DevSim &S, int32_t *acc, int32_t *wacc, hipStream_t s) {

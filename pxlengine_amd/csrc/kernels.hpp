@@ -46,7 +46,7 @@ struct DevSim {
     uint32_t ncells, nfaces;
     int ntx, nty, ntz;  // tiles per axis
     uint32_t nkeys;     // ntx * nty * ntz * TILE_CELLS
-    int zb;             // z-layers per lane in the solver kernels (tunable)
+    int cg_blocks;      // workgroups of the fluid-list solver launches (tunable)
     // particles: current state (a) and sorted scratch (b)
     float *ax, *ay, *az, *avx, *avy, *avz;
     uint32_t *aid;
@@ -65,6 +65,9 @@ struct DevSim {
     uint8_t *type;
     uint16_t *sten;
     float *pressure, *div;
+    // FLUID-cell list of the step: colour-split flags (2C: red then black), their exclusive
+    // scan, the list (red cells first, each colour in cell order) and {n_red, n_total}
+    uint32_t *fl_flag, *fl_off, *fl_list, *fl_count, *fl_tmp;
     // solver
     float *cg_r0, *cg_r1, *cg_w0, *cg_w1, *cg_s0, *cg_s1, *cg_p, *e;
     double *x64a, *x64b, *r64;
@@ -143,15 +146,15 @@ void launch_exclusive_scan(const uint32_t *in, uint32_t n, uint32_t *tile_tmp, u
 void launch_scatter(const DevSim &S, hipStream_t s);          // a -> b in cell order
 void launch_p2g(const DevSim &S, hipStream_t s);              // b -> per-tile boxes
 void launch_faces(const DevSim &S, hipStream_t s);            // normalize + forces
-void launch_cells(const DevSim &S, hipStream_t s);            // type, stencil, p=0, div
-void launch_rbgs(const DevSim &S, uint32_t parity, hipStream_t s);
+void launch_cells(const DevSim &S, hipStream_t s);            // type, stencil, p=0, div, flags
+void launch_fluid_list(const DevSim &S, hipStream_t s);       // flags -> fl_list, fl_count
+void launch_rbgs(const DevSim &S, uint32_t parity, hipStream_t s);  // over the fluid list
 void launch_project(const DevSim &S, hipStream_t s);
 void launch_g2p(const DevSim &S, hipStream_t s);              // b -> a, advected
-// Jacobi-preconditioned CG (solver.hip), one fused launch per iteration:
+// Jacobi-preconditioned CG (solver.hip, over the fluid list), one fused launch per iteration:
 //   launch_cg_start(x, inner), then for it = 0..iters-1: launch_cg_iter(x, it).
 // Plain solve: fp64 = false, x = pressure, inner = false.  Inner solve of the refinement:
 // fp64 = true (FP64 dot products), x = e, inner = true (b = float(r64), e starts at 0).
-dim3 solver_grid(const DevSim &S);
 uint32_t solver_blocks(const DevSim &S);  // workgroups of a solver launch (partials per value)
 void launch_cg_start(const DevSim &S, bool fp64, float *x, bool inner, hipStream_t s);
 void launch_cg_iter(const DevSim &S, bool fp64, float *x, uint32_t it, hipStream_t s);

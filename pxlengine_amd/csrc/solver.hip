@@ -2,6 +2,12 @@
 // for FLUID cell i, (A p)_i = cnt_i p_i - sum_{FLUID nbrs} p_j (cnt_i = non-SOLID neighbours,
 // AIR neighbours are Dirichlet 0), b_i = -div_i h^2 / dt.
 //
+// Work distribution: every kernel here walks the step's FLUID-cell list (built after
+// classification, red cells first then black, each in cell order).  A launch is a fixed grid
+// of `cg_blocks` workgroups; workgroup b takes the list slice [n*b/G, n*(b+1)/G), so every
+// lane holds liquid and the number of reduction partials is fixed, whatever the liquid's
+// shape.
+//
 // Conjugate gradients with the Jacobi preconditioner M = diag(A), in the Chronopoulos-Gear
 // arrangement: both dot products of an iteration come out of ONE fused stencil pass, so an
 // iteration is ONE kernel launch:
@@ -33,8 +39,6 @@ namespace pxl {
 
 namespace {
 
-constexpr int TX = 64, TY = 4;  // a lane walks S.zb consecutive z-layers
-
 __device__ __forceinline__ float diag_of(uint32_t st) { return (float)((st >> 8) & 7u); }
 
 // u = M^-1 r (0 where the diagonal is 0).
@@ -43,28 +47,16 @@ __device__ __forceinline__ float precond(float r, uint32_t st) {
     return dg > 0.0f ? r / dg : 0.0f;
 }
 
-struct Lane {
-    int i, j, k0, k1;
-    bool valid;
-};
-
-__device__ __forceinline__ Lane lane_cells(const DevSim &S) {
-    Lane l;
-    l.i = blockIdx.x * TX + threadIdx.x;
-    l.j = blockIdx.y * TY + threadIdx.y;
-    l.k0 = blockIdx.z * S.zb;
-    l.k1 = min(l.k0 + S.zb, S.nz);
-    l.valid = l.i < S.nx && l.j < S.ny;
-    return l;
-}
-
-__device__ __forceinline__ uint32_t cidx(const DevSim &S, int i, int j, int k) {
-    return (uint32_t)i + (uint32_t)S.nx * ((uint32_t)j + (uint32_t)S.ny * (uint32_t)k);
-}
-
 __device__ __forceinline__ void nb_offsets(const DevSim &S, int32_t off[6]) {
     const int32_t sy = S.nx, sz = S.nx * S.ny;
     off[0] = -1; off[1] = 1; off[2] = -sy; off[3] = sy; off[4] = -sz; off[5] = sz;
+}
+
+// This workgroup's slice of list entries [lo, hi).
+__device__ __forceinline__ void list_slice(uint32_t lo, uint32_t hi, uint32_t &beg, uint32_t &end) {
+    const uint64_t n = hi - lo, g = gridDim.x, b = blockIdx.x;
+    beg = lo + (uint32_t)(n * b / g);
+    end = lo + (uint32_t)(n * (b + 1) / g);
 }
 
 template <typename Acc>
@@ -79,34 +71,33 @@ __device__ __forceinline__ double *partials<double>(const DevSim &S) { return S.
 template <typename T, int NV>
 __device__ __forceinline__ void store_partials(const T (&v)[NV], T *part) {
     __shared__ T sh[4 * NV];
-    const uint32_t nb = num_blocks(), b = flat_block();
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
 #pragma unroll
     for (int k = 0; k < NV; k++) {
         T s = block_sum(v[k], sh + 4 * k);
-        if (flat_tid() == 0) part[(size_t)k * nb + b] = s;
+        if (threadIdx.x == 0) part[(size_t)k * nb + b] = s;
     }
 }
 
-// Sum of the nb workgroup partials of value k, in workgroup order (one workgroup).
+// Sum of the nb workgroup partials of value k, in workgroup order with a fixed tree.
 template <typename T>
 __device__ __forceinline__ T sum_partials(const T *part, uint32_t nb, int k, T *sh) {
     T a = 0;
-    for (uint32_t i = flat_tid(); i < nb; i += BLOCK) a += part[(size_t)k * nb + i];
+    for (uint32_t i = threadIdx.x; i < nb; i += BLOCK) a += part[(size_t)k * nb + i];
     return block_sum(a, sh);
 }
 
 // ---- init 1: r0 = b - A x (pressure system, x = warm-started pressure) or, for the inner
 // solve of the refinement, r0 = float(r64) with x = e = 0.
-__global__ __launch_bounds__(TX *TY) void k_cg_residual(DevSim S, float *__restrict__ x, float *__restrict__ r,
-                                                        int inner) {
-    const Lane l = lane_cells(S);
-    if (!l.valid) return;
+__global__ __launch_bounds__(BLOCK) void k_cg_residual(DevSim S, float *__restrict__ x, float *__restrict__ r,
+                                                       int inner) {
+    uint32_t beg, end;
+    list_slice(0, S.fl_count[1], beg, end);
     int32_t off[6];
     nb_offsets(S, off);
-    for (int k = l.k0; k < l.k1; k++) {
-        const uint32_t id = cidx(S, l.i, l.j, k);
+    for (uint32_t t = beg + threadIdx.x; t < end; t += BLOCK) {
+        const uint32_t id = S.fl_list[t];
         const uint32_t st = S.sten[id];
-        if (!(st & STEN_FLUID)) continue;
         if (inner) {
             r[id] = (float)S.r64[id];
             x[id] = 0.0f;
@@ -122,31 +113,29 @@ __global__ __launch_bounds__(TX *TY) void k_cg_residual(DevSim S, float *__restr
     }
 }
 
-// ---- init 2: u0 = M^-1 r0, w0 = A u0; p_{-1} = s_{-1} = 0; partials gamma0, delta0.
+// ---- init 2: u0 = M^-1 r0, w0 = A u0; p_{-1} = s_{-1} = 0; partials gamma0, delta0 (slot 0).
 template <typename Acc>
-__global__ __launch_bounds__(TX *TY) void k_cg_start(DevSim S, const float *__restrict__ r, float *__restrict__ w,
-                                                     float *__restrict__ s_prev) {
-    const Lane l = lane_cells(S);
+__global__ __launch_bounds__(BLOCK) void k_cg_start(DevSim S, const float *__restrict__ r, float *__restrict__ w,
+                                                    float *__restrict__ s_prev) {
+    uint32_t beg, end;
+    list_slice(0, S.fl_count[1], beg, end);
+    int32_t off[6];
+    nb_offsets(S, off);
     Acc g = 0, d = 0;
-    if (l.valid) {
-        int32_t off[6];
-        nb_offsets(S, off);
-        for (int k = l.k0; k < l.k1; k++) {
-            const uint32_t id = cidx(S, l.i, l.j, k);
-            const uint32_t st = S.sten[id];
-            if (!(st & STEN_FLUID)) continue;
-            const float uc = precond(r[id], st);
-            float sum = 0.0f;
+    for (uint32_t t = beg + threadIdx.x; t < end; t += BLOCK) {
+        const uint32_t id = S.fl_list[t];
+        const uint32_t st = S.sten[id];
+        const float uc = precond(r[id], st);
+        float sum = 0.0f;
 #pragma unroll
-            for (int n = 0; n < 6; n++)
-                if (st & (1u << n)) sum = sum + precond(r[id + off[n]], S.sten[id + off[n]]);
-            const float wc = diag_of(st) * uc - sum;
-            w[id] = wc;
-            S.cg_p[id] = 0.0f;
-            s_prev[id] = 0.0f;
-            g += (Acc)r[id] * (Acc)uc;
-            d += (Acc)wc * (Acc)uc;
-        }
+        for (int n = 0; n < 6; n++)
+            if (st & (1u << n)) sum = sum + precond(r[id + off[n]], S.sten[id + off[n]]);
+        const float wc = diag_of(st) * uc - sum;
+        w[id] = wc;
+        S.cg_p[id] = 0.0f;
+        s_prev[id] = 0.0f;
+        g += (Acc)r[id] * (Acc)uc;
+        d += (Acc)wc * (Acc)uc;
     }
     const Acc v[2] = {g, d};
     store_partials<Acc, 2>(v, partials<Acc>(S));
@@ -171,7 +160,7 @@ __device__ __forceinline__ void cg_scalars(const DevSim &S, uint32_t it, uint32_
             a = den != (Acc)0 ? g / den : (Acc)0;
         }
     }
-    if (flat_block() == 0 && flat_tid() == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         double *next = S.scal + SC_COUNT * ((it + 1) & 1);
         next[SC_ALPHA] = (double)a;
         next[SC_BETA] = (double)b;
@@ -181,73 +170,61 @@ __device__ __forceinline__ void cg_scalars(const DevSim &S, uint32_t it, uint32_
     beta = (float)b;
 }
 
-// ---- one fused iteration (see the file comment).  ZB cells per lane, fully unrolled; all
-// neighbour loads are issued unconditionally (a neighbour that is not FLUID reads the cell
-// itself, always in bounds) and selected afterwards, so a lane keeps all its loads in flight.
-template <typename Acc, int ZB>
-__global__ __launch_bounds__(TX *TY) void k_cg_iter(DevSim S, uint32_t it, float *__restrict__ x,
-                                                    const float *__restrict__ r_in, float *__restrict__ r_out,
-                                                    const float *__restrict__ w_in, float *__restrict__ w_out,
-                                                    const float *__restrict__ s_prev, float *__restrict__ s_cur) {
-    const uint32_t nb = num_blocks();
+// ---- one fused iteration (see the file comment).  All neighbour loads are issued
+// unconditionally (a neighbour that is not FLUID reads the cell itself; FLUID cells are never
+// on the wall layer, so every neighbour index is in bounds) and selected afterwards, so a lane
+// keeps all its loads in flight.
+template <typename Acc>
+__global__ __launch_bounds__(BLOCK) void k_cg_iter(DevSim S, uint32_t it, float *__restrict__ x,
+                                                   const float *__restrict__ r_in, float *__restrict__ r_out,
+                                                   const float *__restrict__ w_in, float *__restrict__ w_out,
+                                                   const float *__restrict__ s_prev, float *__restrict__ s_cur) {
     float alpha, beta;
-    cg_scalars<Acc>(S, it, nb, alpha, beta);
-    const int i = blockIdx.x * TX + threadIdx.x;
-    const int j = blockIdx.y * TY + threadIdx.y;
-    const int k0 = blockIdx.z * ZB;
-    const bool valid = i < S.nx && j < S.ny;
+    cg_scalars<Acc>(S, it, gridDim.x, alpha, beta);
+    uint32_t beg, end;
+    list_slice(0, S.fl_count[1], beg, end);
     const float *__restrict__ pv = S.cg_p;
     const uint16_t *__restrict__ sten = S.sten;
+    int32_t off[6];
+    nb_offsets(S, off);
     Acc g = 0, d = 0;
-    if (valid) {
-        int32_t off[6];
-        nb_offsets(S, off);
-        uint32_t id[ZB], st[ZB];
+    for (uint32_t t = beg + threadIdx.x; t < end; t += BLOCK) {
+        const uint32_t c = S.fl_list[t];
+        const uint32_t st = sten[c];
+        uint32_t q[6];
 #pragma unroll
-        for (int z = 0; z < ZB; z++) {
-            const int k = k0 + z < S.nz ? k0 + z : S.nz - 1;
-            id[z] = cidx(S, i, j, k);
-            st[z] = (k0 + z < S.nz) ? (uint32_t)sten[id[z]] : 0u;
+        for (int n = 0; n < 6; n++) q[n] = (st & (1u << n)) ? c + off[n] : c;
+        const float ri = r_in[c], wi = w_in[c], si = s_prev[c], pi = pv[c], xi = x[c];
+        float rq[6], wq[6], sq[6];
+        uint32_t stq[6];
+#pragma unroll
+        for (int n = 0; n < 6; n++) {
+            rq[n] = r_in[q[n]];
+            wq[n] = w_in[q[n]];
+            sq[n] = s_prev[q[n]];
+            stq[n] = sten[q[n]];
         }
+        const float pc = precond(ri, st) + beta * pi;
+        const float sc = wi + beta * si;
+        const float rn = ri - alpha * sc;
+        const float un = precond(rn, st);
+        float sum = 0.0f;
 #pragma unroll
-        for (int z = 0; z < ZB; z++) {
-            if (!(st[z] & STEN_FLUID)) continue;
-            const uint32_t c = id[z];
-            uint32_t q[6];
-#pragma unroll
-            for (int n = 0; n < 6; n++) q[n] = (st[z] & (1u << n)) ? c + off[n] : c;
-            const float ri = r_in[c], wi = w_in[c], si = s_prev[c], pi = pv[c], xi = x[c];
-            float rq[6], wq[6], sq[6];
-            uint32_t stq[6];
-#pragma unroll
-            for (int n = 0; n < 6; n++) {
-                rq[n] = r_in[q[n]];
-                wq[n] = w_in[q[n]];
-                sq[n] = s_prev[q[n]];
-                stq[n] = sten[q[n]];
-            }
-            const float pc = precond(ri, st[z]) + beta * pi;
-            const float sc = wi + beta * si;
-            const float rn = ri - alpha * sc;
-            const float un = precond(rn, st[z]);
-            float sum = 0.0f;
-#pragma unroll
-            for (int n = 0; n < 6; n++) {
-                const float un_n = precond(rq[n] - alpha * (wq[n] + beta * sq[n]), stq[n]);
-                sum = (st[z] & (1u << n)) ? sum + un_n : sum;
-            }
-            const float wn = diag_of(st[z]) * un - sum;
-            x[c] = xi + alpha * pc;
-            S.cg_p[c] = pc;
-            s_cur[c] = sc;
-            r_out[c] = rn;
-            w_out[c] = wn;
-            g += (Acc)rn * (Acc)un;
-            d += (Acc)wn * (Acc)un;
+        for (int n = 0; n < 6; n++) {
+            const float un_n = precond(rq[n] - alpha * (wq[n] + beta * sq[n]), stq[n]);
+            sum = (st & (1u << n)) ? sum + un_n : sum;
         }
+        const float wn = diag_of(st) * un - sum;
+        x[c] = xi + alpha * pc;
+        S.cg_p[c] = pc;
+        s_cur[c] = sc;
+        r_out[c] = rn;
+        w_out[c] = wn;
+        g += (Acc)rn * (Acc)un;
+        d += (Acc)wn * (Acc)un;
     }
     const Acc v[2] = {g, d};
-    store_partials<Acc, 2>(v, partials<Acc>(S) + (size_t)((it + 1) & 1) * 2 * nb);
+    store_partials<Acc, 2>(v, partials<Acc>(S) + (size_t)((it + 1) & 1) * 2 * gridDim.x);
 }
 
 // ---- FP64 outer loop.  The solution is double-buffered (x_src -> x_dst) so the update pass
@@ -257,59 +234,55 @@ __device__ __forceinline__ double b64_of(const DevSim &S, uint32_t id) {
 }
 
 // init: x64 = pressure, r64 = b - A x64; partials of rr and bb.
-__global__ __launch_bounds__(TX *TY) void k_mixed_init(DevSim S, double *__restrict__ x64) {
-    const Lane l = lane_cells(S);
+__global__ __launch_bounds__(BLOCK) void k_mixed_init(DevSim S, double *__restrict__ x64) {
+    uint32_t beg, end;
+    list_slice(0, S.fl_count[1], beg, end);
+    int32_t off[6];
+    nb_offsets(S, off);
     double rr = 0, bb = 0;
-    if (l.valid) {
-        int32_t off[6];
-        nb_offsets(S, off);
-        for (int k = l.k0; k < l.k1; k++) {
-            const uint32_t id = cidx(S, l.i, l.j, k);
-            const uint32_t st = S.sten[id];
-            if (!(st & STEN_FLUID)) continue;
-            double sum = 0.0;
+    for (uint32_t t = beg + threadIdx.x; t < end; t += BLOCK) {
+        const uint32_t id = S.fl_list[t];
+        const uint32_t st = S.sten[id];
+        double sum = 0.0;
 #pragma unroll
-            for (int n = 0; n < 6; n++)
-                if (st & (1u << n)) sum += (double)S.pressure[id + off[n]];
-            const double xc = (double)S.pressure[id];
-            const double b = b64_of(S, id);
-            const double r = b - ((double)diag_of(st) * xc - sum);
-            x64[id] = xc;
-            S.r64[id] = r;
-            rr += r * r;
-            bb += b * b;
-        }
+        for (int n = 0; n < 6; n++)
+            if (st & (1u << n)) sum += (double)S.pressure[id + off[n]];
+        const double xc = (double)S.pressure[id];
+        const double b = b64_of(S, id);
+        const double r = b - ((double)diag_of(st) * xc - sum);
+        x64[id] = xc;
+        S.r64[id] = r;
+        rr += r * r;
+        bb += b * b;
     }
     const double v[2] = {rr, bb};
     store_partials<double, 2>(v, S.part64);
 }
 
 // update: x_dst = x_src + e (FP64) at the cell and its neighbours, r64 = b - A x_dst; rr.
-__global__ __launch_bounds__(TX *TY) void k_mixed_update(DevSim S, const double *__restrict__ xsrc,
-                                                         double *__restrict__ xdst, const float *__restrict__ e) {
-    const Lane l = lane_cells(S);
+__global__ __launch_bounds__(BLOCK) void k_mixed_update(DevSim S, const double *__restrict__ xsrc,
+                                                        double *__restrict__ xdst, const float *__restrict__ e) {
+    uint32_t beg, end;
+    list_slice(0, S.fl_count[1], beg, end);
+    int32_t off[6];
+    nb_offsets(S, off);
     double rr = 0;
-    if (l.valid) {
-        int32_t off[6];
-        nb_offsets(S, off);
-        for (int k = l.k0; k < l.k1; k++) {
-            const uint32_t id = cidx(S, l.i, l.j, k);
-            const uint32_t st = S.sten[id];
-            if (!(st & STEN_FLUID)) continue;
-            double sum = 0.0;
+    for (uint32_t t = beg + threadIdx.x; t < end; t += BLOCK) {
+        const uint32_t id = S.fl_list[t];
+        const uint32_t st = S.sten[id];
+        double sum = 0.0;
 #pragma unroll
-            for (int n = 0; n < 6; n++) {
-                if (st & (1u << n)) {
-                    const uint32_t q = id + off[n];
-                    sum += xsrc[q] + (double)e[q];
-                }
+        for (int n = 0; n < 6; n++) {
+            if (st & (1u << n)) {
+                const uint32_t q = id + off[n];
+                sum += xsrc[q] + (double)e[q];
             }
-            const double xc = xsrc[id] + (double)e[id];
-            const double r = b64_of(S, id) - ((double)diag_of(st) * xc - sum);
-            xdst[id] = xc;
-            S.r64[id] = r;
-            rr += r * r;
         }
+        const double xc = xsrc[id] + (double)e[id];
+        const double r = b64_of(S, id) - ((double)diag_of(st) * xc - sum);
+        xdst[id] = xc;
+        S.r64[id] = r;
+        rr += r * r;
     }
     const double v[1] = {rr};
     store_partials<double, 1>(v, S.part64);
@@ -326,12 +299,35 @@ __global__ __launch_bounds__(BLOCK) void k_norms_finalize(DevSim S, uint32_t nb,
     if (nv > 1) S.scal[SC_BB] = bb;
 }
 
-__global__ __launch_bounds__(TX *TY) void k_mixed_finish(DevSim S, const double *__restrict__ x64) {
-    const Lane l = lane_cells(S);
-    if (!l.valid) return;
-    for (int k = l.k0; k < l.k1; k++) {
-        const uint32_t id = cidx(S, l.i, l.j, k);
-        if (S.sten[id] & STEN_FLUID) S.pressure[id] = (float)x64[id];
+__global__ __launch_bounds__(BLOCK) void k_mixed_finish(DevSim S, const double *__restrict__ x64) {
+    uint32_t beg, end;
+    list_slice(0, S.fl_count[1], beg, end);
+    for (uint32_t t = beg + threadIdx.x; t < end; t += BLOCK) {
+        const uint32_t id = S.fl_list[t];
+        S.pressure[id] = (float)x64[id];
+    }
+}
+
+// ---- pressure_rb (fluid.slang:168-182): one red-black half sweep over the cells of one
+// colour (list entries [0, n_red) or [n_red, n)).  Same arithmetic as the reference:
+// neighbour order -x,+x,-y,+y,-z,+z, AIR neighbours contribute 0.
+__global__ __launch_bounds__(BLOCK) void k_rbgs(DevSim S, uint32_t parity) {
+    const uint32_t n_red = S.fl_count[0], n = S.fl_count[1];
+    uint32_t beg, end;
+    if (parity == 0) list_slice(0, n_red, beg, end);
+    else list_slice(n_red, n, beg, end);
+    int32_t off[6];
+    nb_offsets(S, off);
+    float *p = S.pressure;
+    for (uint32_t t = beg + threadIdx.x; t < end; t += BLOCK) {
+        const uint32_t id = S.fl_list[t];
+        const uint32_t st = S.sten[id];
+        float sum = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 6; k++)
+            if (st & (1u << k)) sum = sum + p[id + off[k]];
+        const float cnt = diag_of(st);
+        if (cnt > 0.0f) p[id] = (sum - S.div[id] * S.h * S.h / S.dt) / cnt;
     }
 }
 
@@ -341,56 +337,40 @@ inline float *s_buf(const DevSim &S, uint32_t k) { return (k & 1) ? S.cg_s1 : S.
 
 }  // namespace
 
-dim3 solver_grid(const DevSim &S) { return dim3((S.nx + TX - 1) / TX, (S.ny + TY - 1) / TY, (S.nz + S.zb - 1) / S.zb); }
-
-uint32_t solver_blocks(const DevSim &S) {
-    const dim3 g = solver_grid(S);
-    return g.x * g.y * g.z;
-}
+uint32_t solver_blocks(const DevSim &S) { return (uint32_t)S.cg_blocks; }
 
 void launch_cg_start(const DevSim &S, bool fp64, float *x, bool inner, hipStream_t s) {
-    const dim3 g = solver_grid(S), b(TX, TY);
-    k_cg_residual<<<g, b, 0, s>>>(S, x, r_buf(S, 0), inner ? 1 : 0);
+    const uint32_t g = solver_blocks(S);
+    k_cg_residual<<<g, BLOCK, 0, s>>>(S, x, r_buf(S, 0), inner ? 1 : 0);
     // iteration 0 reads s_{-1} from s_buf(1)
-    if (fp64) k_cg_start<double><<<g, b, 0, s>>>(S, r_buf(S, 0), w_buf(S, 0), s_buf(S, 1));
-    else k_cg_start<float><<<g, b, 0, s>>>(S, r_buf(S, 0), w_buf(S, 0), s_buf(S, 1));
-}
-
-template <typename Acc, int ZB>
-void launch_iter_zb(const DevSim &S, uint32_t it, float *x, hipStream_t s) {
-    const dim3 g = solver_grid(S), b(TX, TY);
-    float *rin = r_buf(S, it), *rout = r_buf(S, it + 1), *win = w_buf(S, it), *wout = w_buf(S, it + 1);
-    float *sprev = s_buf(S, it + 1), *scur = s_buf(S, it);
-    k_cg_iter<Acc, ZB><<<g, b, 0, s>>>(S, it, x, rin, rout, win, wout, sprev, scur);
-}
-
-template <typename Acc>
-void launch_iter_acc(const DevSim &S, uint32_t it, float *x, hipStream_t s) {
-    switch (S.zb) {
-        case 1: launch_iter_zb<Acc, 1>(S, it, x, s); break;
-        case 2: launch_iter_zb<Acc, 2>(S, it, x, s); break;
-        case 4: launch_iter_zb<Acc, 4>(S, it, x, s); break;
-        default: launch_iter_zb<Acc, 8>(S, it, x, s); break;
-    }
+    if (fp64) k_cg_start<double><<<g, BLOCK, 0, s>>>(S, r_buf(S, 0), w_buf(S, 0), s_buf(S, 1));
+    else k_cg_start<float><<<g, BLOCK, 0, s>>>(S, r_buf(S, 0), w_buf(S, 0), s_buf(S, 1));
 }
 
 void launch_cg_iter(const DevSim &S, bool fp64, float *x, uint32_t it, hipStream_t s) {
-    if (fp64) launch_iter_acc<double>(S, it, x, s);
-    else launch_iter_acc<float>(S, it, x, s);
+    const uint32_t g = solver_blocks(S);
+    float *rin = r_buf(S, it), *rout = r_buf(S, it + 1), *win = w_buf(S, it), *wout = w_buf(S, it + 1);
+    float *sprev = s_buf(S, it + 1), *scur = s_buf(S, it);
+    if (fp64) k_cg_iter<double><<<g, BLOCK, 0, s>>>(S, it, x, rin, rout, win, wout, sprev, scur);
+    else k_cg_iter<float><<<g, BLOCK, 0, s>>>(S, it, x, rin, rout, win, wout, sprev, scur);
 }
 
 void launch_mixed_init(const DevSim &S, double *x64, hipStream_t s) {
-    k_mixed_init<<<solver_grid(S), dim3(TX, TY), 0, s>>>(S, x64);
+    k_mixed_init<<<solver_blocks(S), BLOCK, 0, s>>>(S, x64);
     k_norms_finalize<<<1, BLOCK, 0, s>>>(S, solver_blocks(S), 2);
 }
 
 void launch_mixed_update(const DevSim &S, const double *xsrc, double *xdst, hipStream_t s) {
-    k_mixed_update<<<solver_grid(S), dim3(TX, TY), 0, s>>>(S, xsrc, xdst, S.e);
+    k_mixed_update<<<solver_blocks(S), BLOCK, 0, s>>>(S, xsrc, xdst, S.e);
     k_norms_finalize<<<1, BLOCK, 0, s>>>(S, solver_blocks(S), 1);
 }
 
 void launch_mixed_finish(const DevSim &S, const double *x64, hipStream_t s) {
-    k_mixed_finish<<<solver_grid(S), dim3(TX, TY), 0, s>>>(S, x64);
+    k_mixed_finish<<<solver_blocks(S), BLOCK, 0, s>>>(S, x64);
+}
+
+void launch_rbgs(const DevSim &S, uint32_t parity, hipStream_t s) {
+    k_rbgs<<<solver_blocks(S), BLOCK, 0, s>>>(S, parity);
 }
 
 }  // namespace pxl

@@ -51,7 +51,7 @@ class Opts(C.Structure):
         ("struct_size", C.c_uint32), ("solver", C.c_int32), ("cg_iterations", C.c_uint32),
         ("inner_iterations", C.c_uint32), ("max_outer", C.c_uint32), ("tolerance", C.c_double),
         ("device", C.c_int32), ("scene_index", C.c_uint32), ("use_graph", C.c_int32),
-        ("solver_zbatch", C.c_uint32),
+        ("solver_workgroups", C.c_uint32),
     ]
 
 
@@ -152,7 +152,7 @@ def _json(fluid) -> bytes:
 
 
 def make_opts(solver="rbgs", cg_iterations=0, inner_iterations=40, max_outer=50, tolerance=1e-6,
-              device=0, scene_index=0, use_graph=True, solver_zbatch=0) -> Opts:
+              device=0, scene_index=0, use_graph=True, solver_workgroups=0) -> Opts:
     o = Opts()
     load_library().pxlflip_default_opts(C.byref(o))
     o.solver = SOLVERS[solver] if isinstance(solver, str) else int(solver)
@@ -163,7 +163,7 @@ def make_opts(solver="rbgs", cg_iterations=0, inner_iterations=40, max_outer=50,
     o.device = device
     o.scene_index = scene_index
     o.use_graph = 1 if use_graph else 0
-    o.solver_zbatch = solver_zbatch
+    o.solver_workgroups = solver_workgroups
     return o
 
 
@@ -293,10 +293,10 @@ class FluidState:
                 for k in range(min(cnt.value, 64))]
 
     def autotune(self, steps: int = 2) -> dict:
-        """Picks the solver z-batch by timing `steps` steps per candidate; state unchanged."""
-        zb, ms = C.c_uint32(), C.c_double()
-        self._call(self._lib.pxlflip_autotune, int(steps), C.byref(zb), C.byref(ms))
-        return {"solver_zbatch": int(zb.value), "ms_per_step": float(ms.value)}
+        """Picks the solver workgroup count by timing `steps` steps per candidate; state unchanged."""
+        g, ms = C.c_uint32(), C.c_double()
+        self._call(self._lib.pxlflip_autotune, int(steps), C.byref(g), C.byref(ms))
+        return {"solver_workgroups": int(g.value), "ms_per_step": float(ms.value)}
 
     def last_solve(self) -> dict:
         st = SolveStats()

@@ -221,7 +221,7 @@ def test_profile_steps_reports_kernels(pxl):
 def test_autotune_preserves_state(pxl):
     """The autotuner times candidate launch shapes on real steps, then restores the state:
     particles and steps_done are unchanged, and later steps agree with an untuned run (the
-    z-batch only changes the CG reduction order)."""
+    workgroup count only changes the CG reduction order)."""
     fluid = dam_break(32)
     a = pxl.FluidState(fluid, FPS, seed=2, solver="cg", cg_iterations=20)
     b = pxl.FluidState(fluid, FPS, seed=2, solver="cg", cg_iterations=20)
@@ -230,7 +230,7 @@ def test_autotune_preserves_state(pxl):
     before = a.read_particles()
     r = a.autotune(1)
     print(f"autotune {r}", flush=True)
-    assert r["solver_zbatch"] in (1, 2, 4, 8)
+    assert r["solver_workgroups"] in (256, 512, 1024, 2048, 4096)
     np.testing.assert_array_equal(a.read_particles(), before)
     assert a.steps_done == 2
     a.step(1)
@@ -240,14 +240,14 @@ def test_autotune_preserves_state(pxl):
     b.close()
 
 
-@pytest.mark.parametrize("zb", [1, 2, 8])
-def test_rbgs_bitwise_any_zbatch(pxl, orc, zb):
+@pytest.mark.parametrize("wg", [1, 7, 4096])
+def test_rbgs_bitwise_any_workgroups(pxl, orc, wg):
     """RBGS results do not depend on the solver launch shape."""
     fluid = dam_break(24)
-    gpu = pxl.FluidState(fluid, FPS, seed=1, solver_zbatch=zb)
+    gpu = pxl.FluidState(fluid, FPS, seed=1, solver_workgroups=wg)
     sim = oracle_sim(orc, fluid, seed=1)
     gpu.step(2)
     sim.step_rbgs(80)
     sim.step_rbgs(80)
-    _state_equal(gpu, sim, f"zb={zb}")
+    _state_equal(gpu, sim, f"workgroups={wg}")
     gpu.close()
