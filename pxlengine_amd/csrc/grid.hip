@@ -114,7 +114,18 @@ __global__ __launch_bounds__(BLOCK) void k_fluid_compact(DevSim S) {
     const uint32_t C = S.ncells, e = blockIdx.x * BLOCK + threadIdx.x;
     if (e >= 2 * C) return;
     const uint32_t f = S.fl_flag[e], o = S.fl_off[e];
-    if (f) S.fl_list[o] = e < C ? e : e - C;
+    if (f) {
+        const uint32_t id = e < C ? e : e - C;
+        const uint32_t st = S.sten[id];
+        const int32_t sy = S.nx, sz = S.nx * S.ny;
+        const int32_t off[6] = {-1, 1, -sy, sy, -sz, sz};
+        uint32_t word = (st & 63u) | (((st >> 8) & 7u) << 6);
+#pragma unroll
+        for (int n = 0; n < 6; n++)
+            if (st & (1u << n)) word |= (((uint32_t)S.sten[id + off[n]] >> 8) & 7u) << (9 + 3 * n);
+        S.fl_list[o] = id;
+        S.fl_word[o] = word;
+    }
     if (e == C) S.fl_count[0] = o;
     if (e == 2 * C - 1) S.fl_count[1] = o + f;
 }

@@ -68,6 +68,9 @@ struct DevSim {
     // FLUID-cell list of the step: colour-split flags (2C: red then black), their exclusive
     // scan, the list (red cells first, each colour in cell order) and {n_red, n_total}
     uint32_t *fl_flag, *fl_off, *fl_list, *fl_count, *fl_tmp;
+    // per list entry: bits 0-5 FLUID neighbours (-x,+x,-y,+y,-z,+z), bits 6-8 the cell's
+    // non-SOLID neighbour count (matrix diagonal), bits 9+3n the diagonal of FLUID neighbour n
+    uint32_t *fl_word;
     // solver
     float *cg_r0, *cg_r1, *cg_w0, *cg_w1, *cg_s0, *cg_s1, *cg_p, *e;
     double *x64a, *x64b, *r64;

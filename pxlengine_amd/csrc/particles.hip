@@ -285,18 +285,30 @@ __device__ __forceinline__ float sample_face_box(const DevSim &S, const float *_
         q1[a] = b[a] + 1 < 0 ? 0 : (b[a] + 1 > dim[a] - 1 ? dim[a] - 1 : b[a] + 1);
         in_box = in_box && q0[a] >= lo[a] && q1[a] < lo[a] + d[a];
     }
-    const uint32_t base = face_base(S, c);
+    // Two separate loops (not a select of pointers): a select would turn the LDS reads into
+    // generic flat loads.  Both loops add the same terms in the same order.
     float s = 0.0f;
+    if (in_box) {
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
-        const int ox = k & 1, oy = (k >> 1) & 1, oz = k >> 2;
-        const int qx = ox ? q1[0] : q0[0], qy = oy ? q1[1] : q0[1], qz = oz ? q1[2] : q0[2];
-        const float wx = ox ? f[0] : 1.0f - f[0];
-        const float wy = oy ? f[1] : 1.0f - f[1];
-        const float wz = oz ? f[2] : 1.0f - f[2];
-        const float val = in_box ? lds[(qx - lo[0]) + d[0] * ((qy - lo[1]) + d[1] * (qz - lo[2]))]
-                                 : buf[base + (uint32_t)qx + (uint32_t)dim[0] * ((uint32_t)qy + (uint32_t)dim[1] * (uint32_t)qz)];
-        s = s + wx * wy * wz * val;
+        for (int k = 0; k < 8; k++) {
+            const int ox = k & 1, oy = (k >> 1) & 1, oz = k >> 2;
+            const int qx = ox ? q1[0] : q0[0], qy = oy ? q1[1] : q0[1], qz = oz ? q1[2] : q0[2];
+            const float wx = ox ? f[0] : 1.0f - f[0];
+            const float wy = oy ? f[1] : 1.0f - f[1];
+            const float wz = oz ? f[2] : 1.0f - f[2];
+            s = s + wx * wy * wz * lds[(qx - lo[0]) + d[0] * ((qy - lo[1]) + d[1] * (qz - lo[2]))];
+        }
+    } else {
+        const uint32_t base = face_base(S, c);
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int ox = k & 1, oy = (k >> 1) & 1, oz = k >> 2;
+            const int qx = ox ? q1[0] : q0[0], qy = oy ? q1[1] : q0[1], qz = oz ? q1[2] : q0[2];
+            const float wx = ox ? f[0] : 1.0f - f[0];
+            const float wy = oy ? f[1] : 1.0f - f[1];
+            const float wz = oz ? f[2] : 1.0f - f[2];
+            s = s + wx * wy * wz * buf[base + (uint32_t)qx + (uint32_t)dim[0] * ((uint32_t)qy + (uint32_t)dim[1] * (uint32_t)qz)];
+        }
     }
     return s;
 }

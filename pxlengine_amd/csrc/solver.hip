@@ -39,13 +39,13 @@ namespace pxl {
 
 namespace {
 
-__device__ __forceinline__ float diag_of(uint32_t st) { return (float)((st >> 8) & 7u); }
+// Fields of a list entry's stencil word (see DevSim::fl_word).
+__device__ __forceinline__ bool w_fluid(uint32_t w, int n) { return (w >> n) & 1u; }
+__device__ __forceinline__ float w_diag(uint32_t w) { return (float)((w >> 6) & 7u); }
+__device__ __forceinline__ float w_ndiag(uint32_t w, int n) { return (float)((w >> (9 + 3 * n)) & 7u); }
 
 // u = M^-1 r (0 where the diagonal is 0).
-__device__ __forceinline__ float precond(float r, uint32_t st) {
-    const float dg = diag_of(st);
-    return dg > 0.0f ? r / dg : 0.0f;
-}
+__device__ __forceinline__ float precond(float r, float dg) { return dg > 0.0f ? r / dg : 0.0f; }
 
 __device__ __forceinline__ void nb_offsets(const DevSim &S, int32_t off[6]) {
     const int32_t sy = S.nx, sz = S.nx * S.ny;
@@ -87,6 +87,16 @@ __device__ __forceinline__ T sum_partials(const T *part, uint32_t nb, int k, T *
     return block_sum(a, sh);
 }
 
+// The same total computed by each wave on its own (lane l adds entries l, l+64, ...; then a
+// butterfly), so no workgroup barrier is needed; every wave of every workgroup gets the
+// same bits.
+template <typename T>
+__device__ __forceinline__ T wave_sum_partials(const T *part, uint32_t nb, int k) {
+    T a = 0;
+    for (uint32_t i = threadIdx.x & (WAVE - 1); i < nb; i += WAVE) a += part[(size_t)k * nb + i];
+    return wave_sum(a);
+}
+
 // ---- init 1: r0 = b - A x (pressure system, x = warm-started pressure) or, for the inner
 // solve of the refinement, r0 = float(r64) with x = e = 0.
 __global__ __launch_bounds__(BLOCK) void k_cg_residual(DevSim S, float *__restrict__ x, float *__restrict__ r,
@@ -97,7 +107,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_residual(DevSim S, float *__restri
     nb_offsets(S, off);
     for (uint32_t t = beg + threadIdx.x; t < end; t += BLOCK) {
         const uint32_t id = S.fl_list[t];
-        const uint32_t st = S.sten[id];
+        const uint32_t wd = S.fl_word[t];
         if (inner) {
             r[id] = (float)S.r64[id];
             x[id] = 0.0f;
@@ -106,8 +116,8 @@ __global__ __launch_bounds__(BLOCK) void k_cg_residual(DevSim S, float *__restri
         float sum = 0.0f;
 #pragma unroll
         for (int n = 0; n < 6; n++)
-            if (st & (1u << n)) sum = sum + x[id + off[n]];
-        const float ax = diag_of(st) * x[id] - sum;
+            if (w_fluid(wd, n)) sum = sum + x[id + off[n]];
+        const float ax = w_diag(wd) * x[id] - sum;
         const float b = -(S.div[id] * S.h * S.h / S.dt);
         r[id] = b - ax;
     }
@@ -124,13 +134,13 @@ __global__ __launch_bounds__(BLOCK) void k_cg_start(DevSim S, const float *__res
     Acc g = 0, d = 0;
     for (uint32_t t = beg + threadIdx.x; t < end; t += BLOCK) {
         const uint32_t id = S.fl_list[t];
-        const uint32_t st = S.sten[id];
-        const float uc = precond(r[id], st);
+        const uint32_t wd = S.fl_word[t];
+        const float uc = precond(r[id], w_diag(wd));
         float sum = 0.0f;
 #pragma unroll
         for (int n = 0; n < 6; n++)
-            if (st & (1u << n)) sum = sum + precond(r[id + off[n]], S.sten[id + off[n]]);
-        const float wc = diag_of(st) * uc - sum;
+            if (w_fluid(wd, n)) sum = sum + precond(r[id + off[n]], w_ndiag(wd, n));
+        const float wc = w_diag(wd) * uc - sum;
         w[id] = wc;
         S.cg_p[id] = 0.0f;
         s_prev[id] = 0.0f;
@@ -144,10 +154,9 @@ __global__ __launch_bounds__(BLOCK) void k_cg_start(DevSim S, const float *__res
 // Scalars of iteration `it` from the previous pass's partials (identical in every workgroup).
 template <typename Acc>
 __device__ __forceinline__ void cg_scalars(const DevSim &S, uint32_t it, uint32_t nb, float &alpha, float &beta) {
-    __shared__ Acc sh[8];
     const Acc *part = partials<Acc>(S) + (size_t)(it & 1) * 2 * nb;
-    const Acc g = sum_partials<Acc>(part, nb, 0, sh);
-    const Acc d = sum_partials<Acc>(part, nb, 1, sh + 4);
+    const Acc g = wave_sum_partials<Acc>(part, nb, 0);
+    const Acc d = wave_sum_partials<Acc>(part, nb, 1);
     Acc a = 0, b = 0;
     if (it == 0) {
         a = (g != (Acc)0 && d != (Acc)0) ? g / d : (Acc)0;
@@ -179,42 +188,40 @@ __global__ __launch_bounds__(BLOCK) void k_cg_iter(DevSim S, uint32_t it, float 
                                                    const float *__restrict__ r_in, float *__restrict__ r_out,
                                                    const float *__restrict__ w_in, float *__restrict__ w_out,
                                                    const float *__restrict__ s_prev, float *__restrict__ s_cur) {
-    float alpha, beta;
-    cg_scalars<Acc>(S, it, gridDim.x, alpha, beta);
     uint32_t beg, end;
     list_slice(0, S.fl_count[1], beg, end);
     const float *__restrict__ pv = S.cg_p;
-    const uint16_t *__restrict__ sten = S.sten;
     int32_t off[6];
     nb_offsets(S, off);
+    float alpha, beta;
+    cg_scalars<Acc>(S, it, gridDim.x, alpha, beta);
     Acc g = 0, d = 0;
     for (uint32_t t = beg + threadIdx.x; t < end; t += BLOCK) {
         const uint32_t c = S.fl_list[t];
-        const uint32_t st = sten[c];
+        const uint32_t wd = S.fl_word[t];
         uint32_t q[6];
 #pragma unroll
-        for (int n = 0; n < 6; n++) q[n] = (st & (1u << n)) ? c + off[n] : c;
+        for (int n = 0; n < 6; n++) q[n] = w_fluid(wd, n) ? c + off[n] : c;
         const float ri = r_in[c], wi = w_in[c], si = s_prev[c], pi = pv[c], xi = x[c];
         float rq[6], wq[6], sq[6];
-        uint32_t stq[6];
 #pragma unroll
         for (int n = 0; n < 6; n++) {
             rq[n] = r_in[q[n]];
             wq[n] = w_in[q[n]];
             sq[n] = s_prev[q[n]];
-            stq[n] = sten[q[n]];
         }
-        const float pc = precond(ri, st) + beta * pi;
+        const float dg = w_diag(wd);
+        const float pc = precond(ri, dg) + beta * pi;
         const float sc = wi + beta * si;
         const float rn = ri - alpha * sc;
-        const float un = precond(rn, st);
+        const float un = precond(rn, dg);
         float sum = 0.0f;
 #pragma unroll
         for (int n = 0; n < 6; n++) {
-            const float un_n = precond(rq[n] - alpha * (wq[n] + beta * sq[n]), stq[n]);
-            sum = (st & (1u << n)) ? sum + un_n : sum;
+            const float un_n = precond(rq[n] - alpha * (wq[n] + beta * sq[n]), w_ndiag(wd, n));
+            sum = w_fluid(wd, n) ? sum + un_n : sum;
         }
-        const float wn = diag_of(st) * un - sum;
+        const float wn = dg * un - sum;
         x[c] = xi + alpha * pc;
         S.cg_p[c] = pc;
         s_cur[c] = sc;
@@ -242,14 +249,14 @@ __global__ __launch_bounds__(BLOCK) void k_mixed_init(DevSim S, double *__restri
     double rr = 0, bb = 0;
     for (uint32_t t = beg + threadIdx.x; t < end; t += BLOCK) {
         const uint32_t id = S.fl_list[t];
-        const uint32_t st = S.sten[id];
+        const uint32_t wd = S.fl_word[t];
         double sum = 0.0;
 #pragma unroll
         for (int n = 0; n < 6; n++)
-            if (st & (1u << n)) sum += (double)S.pressure[id + off[n]];
+            if (w_fluid(wd, n)) sum += (double)S.pressure[id + off[n]];
         const double xc = (double)S.pressure[id];
         const double b = b64_of(S, id);
-        const double r = b - ((double)diag_of(st) * xc - sum);
+        const double r = b - ((double)w_diag(wd) * xc - sum);
         x64[id] = xc;
         S.r64[id] = r;
         rr += r * r;
@@ -269,17 +276,17 @@ __global__ __launch_bounds__(BLOCK) void k_mixed_update(DevSim S, const double *
     double rr = 0;
     for (uint32_t t = beg + threadIdx.x; t < end; t += BLOCK) {
         const uint32_t id = S.fl_list[t];
-        const uint32_t st = S.sten[id];
+        const uint32_t wd = S.fl_word[t];
         double sum = 0.0;
 #pragma unroll
         for (int n = 0; n < 6; n++) {
-            if (st & (1u << n)) {
+            if (w_fluid(wd, n)) {
                 const uint32_t q = id + off[n];
                 sum += xsrc[q] + (double)e[q];
             }
         }
         const double xc = xsrc[id] + (double)e[id];
-        const double r = b64_of(S, id) - ((double)diag_of(st) * xc - sum);
+        const double r = b64_of(S, id) - ((double)w_diag(wd) * xc - sum);
         xdst[id] = xc;
         S.r64[id] = r;
         rr += r * r;
@@ -321,12 +328,12 @@ __global__ __launch_bounds__(BLOCK) void k_rbgs(DevSim S, uint32_t parity) {
     float *p = S.pressure;
     for (uint32_t t = beg + threadIdx.x; t < end; t += BLOCK) {
         const uint32_t id = S.fl_list[t];
-        const uint32_t st = S.sten[id];
+        const uint32_t wd = S.fl_word[t];
         float sum = 0.0f;
 #pragma unroll
         for (int k = 0; k < 6; k++)
-            if (st & (1u << k)) sum = sum + p[id + off[k]];
-        const float cnt = diag_of(st);
+            if (w_fluid(wd, k)) sum = sum + p[id + off[k]];
+        const float cnt = w_diag(wd);
         if (cnt > 0.0f) p[id] = (sum - S.div[id] * S.h * S.h / S.dt) / cnt;
     }
 }
