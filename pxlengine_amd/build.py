@@ -23,7 +23,8 @@ HOST_SOURCES = ["json.cpp", "spec.cpp"]
 HIP_SOURCES = ["particles.hip", "grid.hip", "solver.hip", "flip.hip"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall"]
-HIP_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+HIP_FLAGS = [f"-I{INCLUDE}", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-fno-gpu-rdc", "-munsafe-fp-atomics"]
 
 

@@ -16,7 +16,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/pxlflip.h"
+#include "pxlflip.h"
 #include "kernels.hpp"
 #include "spec.hpp"
 
