@@ -38,7 +38,7 @@ struct HipError {
 
 constexpr size_t ALIGN = 256;
 constexpr uint32_t MAX_SOLVER_BLOCKS = 4096;
-constexpr uint32_t DEFAULT_SOLVER_BLOCKS = 1024;
+constexpr uint32_t DEFAULT_SOLVER_BLOCKS = 256;
 
 // Optional HIP-event bracketing of every launch (pxlflip_profile_steps).  Off while a graph
 // is captured or replayed.
@@ -474,6 +474,7 @@ int prepare(const char *fluid_json, double fps, uint32_t seed, const pxlflip_opt
     d.ntiles = (uint32_t)d.ntx * (uint32_t)d.nty * (uint32_t)d.ntz;
     d.nkeys = d.ntiles * TILE_CELLS;
     d.cg_blocks = (int)(o.solver_workgroups ? o.solver_workgroups : DEFAULT_SOLVER_BLOCKS);
+    d.split_colours = o.solver == PXLFLIP_SOLVER_RBGS ? 1 : 0;
     return PXLFLIP_OK;
 }
 
@@ -801,7 +802,7 @@ int pxlflip_autotune(pxlflip *h, uint32_t steps, uint32_t *chosen_workgroups, do
         HIP_CHECK(hipMemcpyAsync(sp + 24 * P, d.aid, 4 * P, hipMemcpyDeviceToDevice, s->stream));
         HIP_CHECK(hipMemcpyAsync(sp + 28 * P, d.pressure, 4 * C, hipMemcpyDeviceToDevice, s->stream));
         const uint64_t steps_before = s->steps_done;
-        const int candidates[] = {256, 512, 1024, 2048, 4096};
+        const int candidates[] = {64, 128, 256, 512, 1024};
         int best = d.cg_blocks;
         double best_t = 1e300;
         for (int g : candidates) {

@@ -103,16 +103,21 @@ __global__ __launch_bounds__(TX *TY) void k_cells(DevSim S) {
     }
     S.sten[id] = st;
     S.div[id] = d * S.inv_h;
-    const bool red = (((uint32_t)(i + j + k)) & 1u) == 0;
-    S.fl_flag[id] = (t == CELL_FLUID && red) ? 1u : 0u;
-    S.fl_flag[S.ncells + id] = (t == CELL_FLUID && !red) ? 1u : 0u;
+    if (S.split_colours) {
+        const bool red = (((uint32_t)(i + j + k)) & 1u) == 0;
+        S.fl_flag[id] = (t == CELL_FLUID && red) ? 1u : 0u;
+        S.fl_flag[S.ncells + id] = (t == CELL_FLUID && !red) ? 1u : 0u;
+    } else {
+        S.fl_flag[id] = t == CELL_FLUID ? 1u : 0u;
+    }
 }
 
-// Fluid list from the scanned colour-split flags: entry e < C is cell e (red), e >= C is
-// cell e - C (black); fl_count = {red cells, all fluid cells}.
+// Fluid list from the scanned flags.  Split: entry e < C is cell e (red), e >= C is cell
+// e - C (black).  fl_count = {red cells (0 unless split), all fluid cells}.  Each entry also
+// gets its stencil word (see DevSim::fl_word).
 __global__ __launch_bounds__(BLOCK) void k_fluid_compact(DevSim S) {
-    const uint32_t C = S.ncells, e = blockIdx.x * BLOCK + threadIdx.x;
-    if (e >= 2 * C) return;
+    const uint32_t C = S.ncells, n = S.split_colours ? 2 * C : C, e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= n) return;
     const uint32_t f = S.fl_flag[e], o = S.fl_off[e];
     if (f) {
         const uint32_t id = e < C ? e : e - C;
@@ -126,8 +131,11 @@ __global__ __launch_bounds__(BLOCK) void k_fluid_compact(DevSim S) {
         S.fl_list[o] = id;
         S.fl_word[o] = word;
     }
-    if (e == C) S.fl_count[0] = o;
-    if (e == 2 * C - 1) S.fl_count[1] = o + f;
+    if (S.split_colours && e == C) S.fl_count[0] = o;
+    if (e == n - 1) {
+        S.fl_count[1] = o + f;
+        if (!S.split_colours) S.fl_count[0] = 0;
+    }
 }
 
 __device__ __forceinline__ uint8_t type_at(const DevSim &S, int i, int j, int k) {
@@ -177,8 +185,9 @@ dim3 node_grid(const DevSim &S) { return dim3((S.nx + 1 + TX - 1) / TX, (S.ny + 
 void launch_faces(const DevSim &S, hipStream_t s) { k_faces<<<node_grid(S), dim3(TX, TY), 0, s>>>(S); }
 void launch_cells(const DevSim &S, hipStream_t s) { k_cells<<<cell_grid(S), dim3(TX, TY), 0, s>>>(S); }
 void launch_fluid_list(const DevSim &S, hipStream_t s) {
-    launch_exclusive_scan(S.fl_flag, 2 * S.ncells, S.fl_tmp, S.fl_off, s);
-    k_fluid_compact<<<(2 * S.ncells + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(S);
+    const uint32_t n = S.split_colours ? 2 * S.ncells : S.ncells;
+    launch_exclusive_scan(S.fl_flag, n, S.fl_tmp, S.fl_off, s);
+    k_fluid_compact<<<(n + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(S);
 }
 void launch_project(const DevSim &S, hipStream_t s) { k_project<<<node_grid(S), dim3(TX, TY), 0, s>>>(S); }
 void launch_acc_fields(const DevSim &S, int32_t *acc, int32_t *wacc, hipStream_t s) {

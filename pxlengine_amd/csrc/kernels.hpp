@@ -47,6 +47,7 @@ struct DevSim {
     int ntx, nty, ntz;  // tiles per axis
     uint32_t nkeys;     // ntx * nty * ntz * TILE_CELLS
     int cg_blocks;      // workgroups of the fluid-list solver launches (tunable)
+    int split_colours;  // fluid list red cells first then black (RBGS) or in cell order
     // particles: current state (a) and sorted scratch (b)
     float *ax, *ay, *az, *avx, *avy, *avz;
     uint32_t *aid;
@@ -65,8 +66,9 @@ struct DevSim {
     uint8_t *type;
     uint16_t *sten;
     float *pressure, *div;
-    // FLUID-cell list of the step: colour-split flags (2C: red then black), their exclusive
-    // scan, the list (red cells first, each colour in cell order) and {n_red, n_total}
+    // FLUID-cell list of the step: flags (C, or 2C colour-split: red then black), their
+    // exclusive scan, the list (cell order, or red cells first then black) and
+    // {n_red (0 unless split), n_total}
     uint32_t *fl_flag, *fl_off, *fl_list, *fl_count, *fl_tmp;
     // per list entry: bits 0-5 FLUID neighbours (-x,+x,-y,+y,-z,+z), bits 6-8 the cell's
     // non-SOLID neighbour count (matrix diagonal), bits 9+3n the diagonal of FLUID neighbour n

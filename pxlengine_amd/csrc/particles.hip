@@ -259,56 +259,66 @@ __global__ __launch_bounds__(TILE_BLOCK) void k_p2g_tile(DevSim S) {
     for (int q = threadIdx.x; q < BOX_ENTRIES; q += TILE_BLOCK) out[q] = (int64_t)lbox[q];
 }
 
-// sample_face from the tile's LDS copy of a face field when all 8 clamped corners are in the
-// box (always true at the particle's own position), else from HBM.  Same arithmetic either way.
-__device__ __forceinline__ float sample_face_box(const DevSim &S, const float *__restrict__ lds,
-                                                 const float *__restrict__ buf, int c, int ti, int tj, int tk,
-                                                 float x, float y, float z) {
+// Trilinear stencil of sample_face (fluid.slang:67-80) for component c at one position:
+// the eight corner weights w[k] = (wx * wy) * wz (k = ox + 2 oy + 4 oz, the reference's
+// product order), the clamped corner coordinates, and -- when all corners lie in the tile's
+// face box -- the LDS index of corner 0 and the index steps along x, y, z (0 where a
+// clamp folds both corners together).
+struct Stencil {
+    float w[8];
+    int q0[3], q1[3];
+    int li, sx, sy, sz;
+    bool in_box;
+};
+
+__device__ __forceinline__ void make_stencil(const DevSim &S, int c, int ti, int tj, int tk, float x, float y,
+                                             float z, Stencil &st) {
     int dim[3] = {S.nx, S.ny, S.nz};
     dim[c] += 1;
     float g[3] = {(x - S.ox) * S.inv_h - 0.5f, (y - S.oy) * S.inv_h - 0.5f, (z - S.oz) * S.inv_h - 0.5f};
     g[c] = g[c] + 0.5f;
-    int b[3];
-    float f[3];
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-        b[a] = (int)floorf(g[a]);
-        f[a] = g[a] - (float)b[a];
-    }
+    float wl[3], wh[3];
     int lo[3], d[3];
     tile_box(c, ti, tj, tk, lo, d);
-    int q0[3], q1[3];
-    bool in_box = true;
+    st.in_box = true;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        q0[a] = b[a] < 0 ? 0 : (b[a] > dim[a] - 1 ? dim[a] - 1 : b[a]);
-        q1[a] = b[a] + 1 < 0 ? 0 : (b[a] + 1 > dim[a] - 1 ? dim[a] - 1 : b[a] + 1);
-        in_box = in_box && q0[a] >= lo[a] && q1[a] < lo[a] + d[a];
+        const int b = (int)floorf(g[a]);
+        const float f = g[a] - (float)b;
+        wl[a] = 1.0f - f;
+        wh[a] = f;
+        st.q0[a] = b < 0 ? 0 : (b > dim[a] - 1 ? dim[a] - 1 : b);
+        st.q1[a] = b + 1 < 0 ? 0 : (b + 1 > dim[a] - 1 ? dim[a] - 1 : b + 1);
+        st.in_box = st.in_box && st.q0[a] >= lo[a] && st.q1[a] < lo[a] + d[a];
     }
-    // Two separate loops (not a select of pointers): a select would turn the LDS reads into
-    // generic flat loads.  Both loops add the same terms in the same order.
+    const float wxy[4] = {wl[0] * wl[1], wh[0] * wl[1], wl[0] * wh[1], wh[0] * wh[1]};
+#pragma unroll
+    for (int k = 0; k < 8; k++) st.w[k] = wxy[k & 3] * ((k >> 2) ? wh[2] : wl[2]);
+    st.li = (st.q0[0] - lo[0]) + d[0] * ((st.q0[1] - lo[1]) + d[1] * (st.q0[2] - lo[2]));
+    st.sx = st.q1[0] - st.q0[0];
+    st.sy = (st.q1[1] - st.q0[1]) * d[0];
+    st.sz = (st.q1[2] - st.q0[2]) * d[0] * d[1];
+}
+
+// s = sum_k w[k] * value[k], corners in order k = 0..7 (the reference's accumulation).
+__device__ __forceinline__ float sample_lds(const Stencil &st, const float *__restrict__ lds) {
     float s = 0.0f;
-    if (in_box) {
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            const int ox = k & 1, oy = (k >> 1) & 1, oz = k >> 2;
-            const int qx = ox ? q1[0] : q0[0], qy = oy ? q1[1] : q0[1], qz = oz ? q1[2] : q0[2];
-            const float wx = ox ? f[0] : 1.0f - f[0];
-            const float wy = oy ? f[1] : 1.0f - f[1];
-            const float wz = oz ? f[2] : 1.0f - f[2];
-            s = s + wx * wy * wz * lds[(qx - lo[0]) + d[0] * ((qy - lo[1]) + d[1] * (qz - lo[2]))];
-        }
-    } else {
-        const uint32_t base = face_base(S, c);
+    for (int k = 0; k < 8; k++)
+        s = s + st.w[k] * lds[st.li + (k & 1) * st.sx + ((k >> 1) & 1) * st.sy + (k >> 2) * st.sz];
+    return s;
+}
+
+__device__ __forceinline__ float sample_global(const DevSim &S, int c, const Stencil &st, const float *__restrict__ buf) {
+    const uint32_t dx = (uint32_t)S.nx + (c == 0), dy = (uint32_t)S.ny + (c == 1);
+    const uint32_t base = face_base(S, c);
+    float s = 0.0f;
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            const int ox = k & 1, oy = (k >> 1) & 1, oz = k >> 2;
-            const int qx = ox ? q1[0] : q0[0], qy = oy ? q1[1] : q0[1], qz = oz ? q1[2] : q0[2];
-            const float wx = ox ? f[0] : 1.0f - f[0];
-            const float wy = oy ? f[1] : 1.0f - f[1];
-            const float wz = oz ? f[2] : 1.0f - f[2];
-            s = s + wx * wy * wz * buf[base + (uint32_t)qx + (uint32_t)dim[0] * ((uint32_t)qy + (uint32_t)dim[1] * (uint32_t)qz)];
-        }
+    for (int k = 0; k < 8; k++) {
+        const int qx = (k & 1) ? st.q1[0] : st.q0[0];
+        const int qy = ((k >> 1) & 1) ? st.q1[1] : st.q0[1];
+        const int qz = (k >> 2) ? st.q1[2] : st.q0[2];
+        s = s + st.w[k] * buf[base + (uint32_t)qx + dx * ((uint32_t)qy + dy * (uint32_t)qz)];
     }
     return s;
 }
@@ -352,11 +362,18 @@ __global__ __launch_bounds__(TILE_BLOCK) void k_g2p_tile(DevSim S) {
         const float v0[3] = {S.bvx[i], S.bvy[i], S.bvz[i]};
         float pic[3], old[3], v[3], mid[3], adv[3];
 #pragma unroll
-        for (int c = 0; c < 3; c++)
-            pic[c] = sample_face_box(S, lvel + c * BOX_FACES, S.vel, c, ti, tj, tk, x[0], x[1], x[2]);
-#pragma unroll
-        for (int c = 0; c < 3; c++)
-            old[c] = sample_face_box(S, lold + c * BOX_FACES, S.vel_old, c, ti, tj, tk, x[0], x[1], x[2]);
+        for (int c = 0; c < 3; c++) {
+            Stencil st;
+            make_stencil(S, c, ti, tj, tk, x[0], x[1], x[2], st);
+            // separate paths so the LDS reads compile to ds_read, not generic loads
+            if (st.in_box) {
+                pic[c] = sample_lds(st, lvel + c * BOX_FACES);
+                old[c] = sample_lds(st, lold + c * BOX_FACES);
+            } else {
+                pic[c] = sample_global(S, c, st, S.vel);
+                old[c] = sample_global(S, c, st, S.vel_old);
+            }
+        }
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const float flipv = v0[c] + pic[c] - old[c];
@@ -364,8 +381,12 @@ __global__ __launch_bounds__(TILE_BLOCK) void k_g2p_tile(DevSim S) {
             mid[c] = x[c] + half_dt * pic[c];
         }
 #pragma unroll
-        for (int c = 0; c < 3; c++)
-            adv[c] = sample_face_box(S, lvel + c * BOX_FACES, S.vel, c, ti, tj, tk, mid[0], mid[1], mid[2]);
+        for (int c = 0; c < 3; c++) {
+            Stencil st;
+            make_stencil(S, c, ti, tj, tk, mid[0], mid[1], mid[2], st);
+            if (st.in_box) adv[c] = sample_lds(st, lvel + c * BOX_FACES);
+            else adv[c] = sample_global(S, c, st, S.vel);
+        }
 #pragma unroll
         for (int c = 0; c < 3; c++) x[c] = x[c] + S.dt * adv[c];
 #pragma unroll

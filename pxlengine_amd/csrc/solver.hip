@@ -39,13 +39,19 @@ namespace pxl {
 
 namespace {
 
+constexpr int SBLOCK = 1024;  // lanes per solver workgroup (16 wave64)
+constexpr int SWAVES = SBLOCK / WAVE;
+
 // Fields of a list entry's stencil word (see DevSim::fl_word).
 __device__ __forceinline__ bool w_fluid(uint32_t w, int n) { return (w >> n) & 1u; }
 __device__ __forceinline__ float w_diag(uint32_t w) { return (float)((w >> 6) & 7u); }
 __device__ __forceinline__ float w_ndiag(uint32_t w, int n) { return (float)((w >> (9 + 3 * n)) & 7u); }
 
-// u = M^-1 r (0 where the diagonal is 0).
-__device__ __forceinline__ float precond(float r, float dg) { return dg > 0.0f ? r / dg : 0.0f; }
+// Jacobi preconditioner u = r * (1 / diag) with the hardware reciprocal (a preconditioner
+// need not be exact; 0 where the diagonal is 0).  The diagonal is a small integer 1..6.
+__device__ __forceinline__ float precond(float r, float dg) {
+    return dg > 0.0f ? r * __builtin_amdgcn_rcpf(dg) : 0.0f;
+}
 
 __device__ __forceinline__ void nb_offsets(const DevSim &S, int32_t off[6]) {
     const int32_t sy = S.nx, sz = S.nx * S.ny;
@@ -70,11 +76,11 @@ __device__ __forceinline__ double *partials<double>(const DevSim &S) { return S.
 // consumed by the next kernel on the stream).
 template <typename T, int NV>
 __device__ __forceinline__ void store_partials(const T (&v)[NV], T *part) {
-    __shared__ T sh[4 * NV];
+    __shared__ T sh[SWAVES * NV];
     const uint32_t nb = gridDim.x, b = blockIdx.x;
 #pragma unroll
     for (int k = 0; k < NV; k++) {
-        T s = block_sum(v[k], sh + 4 * k);
+        T s = block_sum<T, SWAVES>(v[k], sh + SWAVES * k);
         if (threadIdx.x == 0) part[(size_t)k * nb + b] = s;
     }
 }
@@ -99,13 +105,13 @@ __device__ __forceinline__ T wave_sum_partials(const T *part, uint32_t nb, int k
 
 // ---- init 1: r0 = b - A x (pressure system, x = warm-started pressure) or, for the inner
 // solve of the refinement, r0 = float(r64) with x = e = 0.
-__global__ __launch_bounds__(BLOCK) void k_cg_residual(DevSim S, float *__restrict__ x, float *__restrict__ r,
+__global__ __launch_bounds__(SBLOCK) void k_cg_residual(DevSim S, float *__restrict__ x, float *__restrict__ r,
                                                        int inner) {
     uint32_t beg, end;
     list_slice(0, S.fl_count[1], beg, end);
     int32_t off[6];
     nb_offsets(S, off);
-    for (uint32_t t = beg + threadIdx.x; t < end; t += BLOCK) {
+    for (uint32_t t = beg + threadIdx.x; t < end; t += SBLOCK) {
         const uint32_t id = S.fl_list[t];
         const uint32_t wd = S.fl_word[t];
         if (inner) {
@@ -125,14 +131,14 @@ __global__ __launch_bounds__(BLOCK) void k_cg_residual(DevSim S, float *__restri
 
 // ---- init 2: u0 = M^-1 r0, w0 = A u0; p_{-1} = s_{-1} = 0; partials gamma0, delta0 (slot 0).
 template <typename Acc>
-__global__ __launch_bounds__(BLOCK) void k_cg_start(DevSim S, const float *__restrict__ r, float *__restrict__ w,
+__global__ __launch_bounds__(SBLOCK) void k_cg_start(DevSim S, const float *__restrict__ r, float *__restrict__ w,
                                                     float *__restrict__ s_prev) {
     uint32_t beg, end;
     list_slice(0, S.fl_count[1], beg, end);
     int32_t off[6];
     nb_offsets(S, off);
     Acc g = 0, d = 0;
-    for (uint32_t t = beg + threadIdx.x; t < end; t += BLOCK) {
+    for (uint32_t t = beg + threadIdx.x; t < end; t += SBLOCK) {
         const uint32_t id = S.fl_list[t];
         const uint32_t wd = S.fl_word[t];
         const float uc = precond(r[id], w_diag(wd));
@@ -184,7 +190,7 @@ __device__ __forceinline__ void cg_scalars(const DevSim &S, uint32_t it, uint32_
 // on the wall layer, so every neighbour index is in bounds) and selected afterwards, so a lane
 // keeps all its loads in flight.
 template <typename Acc>
-__global__ __launch_bounds__(BLOCK) void k_cg_iter(DevSim S, uint32_t it, float *__restrict__ x,
+__global__ __launch_bounds__(SBLOCK) void k_cg_iter(DevSim S, uint32_t it, float *__restrict__ x,
                                                    const float *__restrict__ r_in, float *__restrict__ r_out,
                                                    const float *__restrict__ w_in, float *__restrict__ w_out,
                                                    const float *__restrict__ s_prev, float *__restrict__ s_cur) {
@@ -196,7 +202,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_iter(DevSim S, uint32_t it, float 
     float alpha, beta;
     cg_scalars<Acc>(S, it, gridDim.x, alpha, beta);
     Acc g = 0, d = 0;
-    for (uint32_t t = beg + threadIdx.x; t < end; t += BLOCK) {
+    for (uint32_t t = beg + threadIdx.x; t < end; t += SBLOCK) {
         const uint32_t c = S.fl_list[t];
         const uint32_t wd = S.fl_word[t];
         uint32_t q[6];
@@ -241,13 +247,13 @@ __device__ __forceinline__ double b64_of(const DevSim &S, uint32_t id) {
 }
 
 // init: x64 = pressure, r64 = b - A x64; partials of rr and bb.
-__global__ __launch_bounds__(BLOCK) void k_mixed_init(DevSim S, double *__restrict__ x64) {
+__global__ __launch_bounds__(SBLOCK) void k_mixed_init(DevSim S, double *__restrict__ x64) {
     uint32_t beg, end;
     list_slice(0, S.fl_count[1], beg, end);
     int32_t off[6];
     nb_offsets(S, off);
     double rr = 0, bb = 0;
-    for (uint32_t t = beg + threadIdx.x; t < end; t += BLOCK) {
+    for (uint32_t t = beg + threadIdx.x; t < end; t += SBLOCK) {
         const uint32_t id = S.fl_list[t];
         const uint32_t wd = S.fl_word[t];
         double sum = 0.0;
@@ -267,14 +273,14 @@ __global__ __launch_bounds__(BLOCK) void k_mixed_init(DevSim S, double *__restri
 }
 
 // update: x_dst = x_src + e (FP64) at the cell and its neighbours, r64 = b - A x_dst; rr.
-__global__ __launch_bounds__(BLOCK) void k_mixed_update(DevSim S, const double *__restrict__ xsrc,
+__global__ __launch_bounds__(SBLOCK) void k_mixed_update(DevSim S, const double *__restrict__ xsrc,
                                                         double *__restrict__ xdst, const float *__restrict__ e) {
     uint32_t beg, end;
     list_slice(0, S.fl_count[1], beg, end);
     int32_t off[6];
     nb_offsets(S, off);
     double rr = 0;
-    for (uint32_t t = beg + threadIdx.x; t < end; t += BLOCK) {
+    for (uint32_t t = beg + threadIdx.x; t < end; t += SBLOCK) {
         const uint32_t id = S.fl_list[t];
         const uint32_t wd = S.fl_word[t];
         double sum = 0.0;
@@ -306,10 +312,10 @@ __global__ __launch_bounds__(BLOCK) void k_norms_finalize(DevSim S, uint32_t nb,
     if (nv > 1) S.scal[SC_BB] = bb;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_mixed_finish(DevSim S, const double *__restrict__ x64) {
+__global__ __launch_bounds__(SBLOCK) void k_mixed_finish(DevSim S, const double *__restrict__ x64) {
     uint32_t beg, end;
     list_slice(0, S.fl_count[1], beg, end);
-    for (uint32_t t = beg + threadIdx.x; t < end; t += BLOCK) {
+    for (uint32_t t = beg + threadIdx.x; t < end; t += SBLOCK) {
         const uint32_t id = S.fl_list[t];
         S.pressure[id] = (float)x64[id];
     }
@@ -318,7 +324,7 @@ __global__ __launch_bounds__(BLOCK) void k_mixed_finish(DevSim S, const double *
 // ---- pressure_rb (fluid.slang:168-182): one red-black half sweep over the cells of one
 // colour (list entries [0, n_red) or [n_red, n)).  Same arithmetic as the reference:
 // neighbour order -x,+x,-y,+y,-z,+z, AIR neighbours contribute 0.
-__global__ __launch_bounds__(BLOCK) void k_rbgs(DevSim S, uint32_t parity) {
+__global__ __launch_bounds__(SBLOCK) void k_rbgs(DevSim S, uint32_t parity) {
     const uint32_t n_red = S.fl_count[0], n = S.fl_count[1];
     uint32_t beg, end;
     if (parity == 0) list_slice(0, n_red, beg, end);
@@ -326,7 +332,7 @@ __global__ __launch_bounds__(BLOCK) void k_rbgs(DevSim S, uint32_t parity) {
     int32_t off[6];
     nb_offsets(S, off);
     float *p = S.pressure;
-    for (uint32_t t = beg + threadIdx.x; t < end; t += BLOCK) {
+    for (uint32_t t = beg + threadIdx.x; t < end; t += SBLOCK) {
         const uint32_t id = S.fl_list[t];
         const uint32_t wd = S.fl_word[t];
         float sum = 0.0f;
@@ -348,36 +354,36 @@ uint32_t solver_blocks(const DevSim &S) { return (uint32_t)S.cg_blocks; }
 
 void launch_cg_start(const DevSim &S, bool fp64, float *x, bool inner, hipStream_t s) {
     const uint32_t g = solver_blocks(S);
-    k_cg_residual<<<g, BLOCK, 0, s>>>(S, x, r_buf(S, 0), inner ? 1 : 0);
+    k_cg_residual<<<g, SBLOCK, 0, s>>>(S, x, r_buf(S, 0), inner ? 1 : 0);
     // iteration 0 reads s_{-1} from s_buf(1)
-    if (fp64) k_cg_start<double><<<g, BLOCK, 0, s>>>(S, r_buf(S, 0), w_buf(S, 0), s_buf(S, 1));
-    else k_cg_start<float><<<g, BLOCK, 0, s>>>(S, r_buf(S, 0), w_buf(S, 0), s_buf(S, 1));
+    if (fp64) k_cg_start<double><<<g, SBLOCK, 0, s>>>(S, r_buf(S, 0), w_buf(S, 0), s_buf(S, 1));
+    else k_cg_start<float><<<g, SBLOCK, 0, s>>>(S, r_buf(S, 0), w_buf(S, 0), s_buf(S, 1));
 }
 
 void launch_cg_iter(const DevSim &S, bool fp64, float *x, uint32_t it, hipStream_t s) {
     const uint32_t g = solver_blocks(S);
     float *rin = r_buf(S, it), *rout = r_buf(S, it + 1), *win = w_buf(S, it), *wout = w_buf(S, it + 1);
     float *sprev = s_buf(S, it + 1), *scur = s_buf(S, it);
-    if (fp64) k_cg_iter<double><<<g, BLOCK, 0, s>>>(S, it, x, rin, rout, win, wout, sprev, scur);
-    else k_cg_iter<float><<<g, BLOCK, 0, s>>>(S, it, x, rin, rout, win, wout, sprev, scur);
+    if (fp64) k_cg_iter<double><<<g, SBLOCK, 0, s>>>(S, it, x, rin, rout, win, wout, sprev, scur);
+    else k_cg_iter<float><<<g, SBLOCK, 0, s>>>(S, it, x, rin, rout, win, wout, sprev, scur);
 }
 
 void launch_mixed_init(const DevSim &S, double *x64, hipStream_t s) {
-    k_mixed_init<<<solver_blocks(S), BLOCK, 0, s>>>(S, x64);
+    k_mixed_init<<<solver_blocks(S), SBLOCK, 0, s>>>(S, x64);
     k_norms_finalize<<<1, BLOCK, 0, s>>>(S, solver_blocks(S), 2);
 }
 
 void launch_mixed_update(const DevSim &S, const double *xsrc, double *xdst, hipStream_t s) {
-    k_mixed_update<<<solver_blocks(S), BLOCK, 0, s>>>(S, xsrc, xdst, S.e);
+    k_mixed_update<<<solver_blocks(S), SBLOCK, 0, s>>>(S, xsrc, xdst, S.e);
     k_norms_finalize<<<1, BLOCK, 0, s>>>(S, solver_blocks(S), 1);
 }
 
 void launch_mixed_finish(const DevSim &S, const double *x64, hipStream_t s) {
-    k_mixed_finish<<<solver_blocks(S), BLOCK, 0, s>>>(S, x64);
+    k_mixed_finish<<<solver_blocks(S), SBLOCK, 0, s>>>(S, x64);
 }
 
 void launch_rbgs(const DevSim &S, uint32_t parity, hipStream_t s) {
-    k_rbgs<<<solver_blocks(S), BLOCK, 0, s>>>(S, parity);
+    k_rbgs<<<solver_blocks(S), SBLOCK, 0, s>>>(S, parity);
 }
 
 }  // namespace pxl
