@@ -139,6 +139,8 @@ struct Sim {
         add(&d.vel, 4 * F);
         add(&d.vel_old, 4 * F);
         add(&d.box, 8ull * BOX_ENTRIES * d.ntiles);
+        add(&d.tile_list, 4ull * d.ntiles);
+        add(&d.tile_count, 64);
         add(&d.count, 4 * K);
         add(&d.offset, 4 * (K + 1));
         add(&d.type, C);
@@ -204,6 +206,7 @@ struct Sim {
         KRUN("clear_grid", launch_clear_grid(d, stream));
         KRUN("bin_count", launch_bin_count(d, stream));
         KRUN("scan_offsets", launch_scan_offsets(d, stream));
+        KRUN("tile_list", launch_tile_list(d, stream));
         KRUN("scatter", launch_scatter(d, stream));
         KRUN("p2g", launch_p2g(d, stream));
         KRUN("faces", launch_faces(d, stream));

@@ -61,6 +61,7 @@ struct DevSim {
     // (sum_wacc >= 0 and < 2^32, so the halves never carry into each other)
     int64_t *box;
     uint32_t ntiles;
+    uint32_t *tile_list, *tile_count;  // tiles holding particles this step (any order)
     // cells (count/offset: tiled key order, offset[nkeys] = np)
     uint32_t *count, *offset;
     uint8_t *type;
@@ -148,6 +149,7 @@ void launch_bin_count(const DevSim &S, hipStream_t s);        // key, slot, coun
 void launch_scan_offsets(const DevSim &S, hipStream_t s);     // offset = exclusive scan(count)
 uint32_t scan_tiles(uint32_t n);                              // scratch entries a scan of n needs
 void launch_exclusive_scan(const uint32_t *in, uint32_t n, uint32_t *tile_tmp, uint32_t *out, hipStream_t s);
+void launch_tile_list(const DevSim &S, hipStream_t s);        // active tiles (after the scan)
 void launch_scatter(const DevSim &S, hipStream_t s);          // a -> b in cell order
 void launch_p2g(const DevSim &S, hipStream_t s);              // b -> per-tile boxes
 void launch_faces(const DevSim &S, hipStream_t s);            // normalize + forces

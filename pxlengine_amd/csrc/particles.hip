@@ -41,6 +41,7 @@ __global__ __launch_bounds__(BLOCK) void k_fill_u32(uint32_t *p, uint32_t n, uin
 __global__ __launch_bounds__(BLOCK) void k_clear_grid(DevSim S) {
     uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i < S.nkeys) S.count[i] = 0;
+    if (i == 0) *S.tile_count = 0;
 }
 
 // Counting-sort pass 1: tiled cell key of each particle and its arrival slot in that cell
@@ -176,36 +177,6 @@ __device__ __forceinline__ uint32_t face_base(const DevSim &S, int c) {
     return c == 0 ? 0u : (c == 1 ? S.nu : S.nu + S.nv);
 }
 
-// Fixed-point weight contributions of one particle for component c (fluid.slang:97-111):
-// corners out of the face grid are skipped; `emit(q, acc, wacc)` receives the others.
-template <typename Emit>
-__device__ __forceinline__ void splat_component(const DevSim &S, int c, const float x[3], float vc, Emit &&emit) {
-    int dim[3] = {S.nx, S.ny, S.nz};
-    dim[c] += 1;
-    const float o[3] = {S.ox, S.oy, S.oz};
-    float g[3], f[3];
-    int b[3];
-#pragma unroll
-    for (int a = 0; a < 3; a++) g[a] = (x[a] - o[a]) * S.inv_h - 0.5f;
-    g[c] = g[c] + 0.5f;
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-        b[a] = (int)floorf(g[a]);
-        f[a] = g[a] - (float)b[a];
-    }
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        const int ox = k & 1, oy = (k >> 1) & 1, oz = k >> 2;
-        const int qx = b[0] + ox, qy = b[1] + oy, qz = b[2] + oz;
-        if (qx < 0 || qy < 0 || qz < 0 || qx > dim[0] - 1 || qy > dim[1] - 1 || qz > dim[2] - 1) continue;
-        const float wx = ox ? f[0] : 1.0f - f[0];
-        const float wy = oy ? f[1] : 1.0f - f[1];
-        const float wz = oz ? f[2] : 1.0f - f[2];
-        const float w = wx * wy * wz;
-        emit(qx, qy, qz, (int)(w * vc * 65536.0f), (int)(w * 65536.0f));
-    }
-}
-
 __device__ __forceinline__ uint32_t face_index(const DevSim &S, int c, int qx, int qy, int qz) {
     const uint32_t dx = (uint32_t)S.nx + (c == 0), dy = (uint32_t)S.ny + (c == 1);
     return face_base(S, c) + (uint32_t)qx + dx * ((uint32_t)qy + dy * (uint32_t)qz);
@@ -217,46 +188,92 @@ __device__ __forceinline__ void tile_coords(const DevSim &S, uint32_t t, int &ti
     tk = (int)(t / ((uint32_t)S.ntx * (uint32_t)S.nty));
 }
 
-// p2g (fluid.slang:94-112) for one 8^3 tile of cells per workgroup.  The tile's particles are
-// one contiguous run of the cell-sorted scratch copy; their fixed-point splats (acc and wacc
-// packed into one int64) are summed with LDS atomics into the tile's face box, and the box is
-// stored whole to HBM with plain stores.  The face pass adds the boxes that cover each face,
-// so no global atomics are needed.  Integer sums: the order of additions does not matter.
+// Face-box geometry of component c (compile-time after unrolling): extents d0 x d1 x d2 and
+// the LDS offset of trilinear corner k (k = ox + 2 oy + 4 oz) from corner 0.
+__device__ __forceinline__ int box_d0(int c) { return c == 0 ? TILE + 1 : TILE + 2; }
+__device__ __forceinline__ int box_d1(int c) { return c == 1 ? TILE + 1 : TILE + 2; }
+__device__ __forceinline__ int box_corner(int c, int k) {
+    return (k & 1) + box_d0(c) * (((k >> 1) & 1) + box_d1(c) * (k >> 2));
+}
+
+// Active tiles (tiles holding particles) for the per-tile kernels; list order is arbitrary
+// (the per-tile results do not depend on it).
+__global__ __launch_bounds__(BLOCK) void k_tile_list(DevSim S) {
+    const uint32_t t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= S.ntiles) return;
+    if (S.offset[t * TILE_CELLS] != S.offset[(t + 1) * TILE_CELLS]) S.tile_list[atomicAdd(S.tile_count, 1u)] = t;
+}
+
+// p2g (fluid.slang:94-112), one active 8^3 tile at a time per workgroup.  The tile's
+// particles are one contiguous run of the cell-sorted scratch copy; their fixed-point splats
+// (acc and wacc packed into one int64) are summed with LDS atomics into the tile's face box,
+// and the box is stored whole to HBM with plain stores.  The face pass adds the boxes that
+// cover each face, so no global atomics are needed.  Integer sums: the order of additions
+// does not matter.  Corners outside the face grid are skipped as in the reference; every
+// in-grid corner of a particle binned in the tile lies in the box (the box spans the cells
+// [8t-1, 8t+8] around the tile; positions beyond the grid are binned to its edge cells, whose
+// corners beyond the grid are skipped).
 __global__ __launch_bounds__(TILE_BLOCK) void k_p2g_tile(DevSim S) {
     __shared__ unsigned long long lbox[BOX_ENTRIES];
-    const uint32_t t = blockIdx.x;
-    const uint32_t beg = S.offset[t * TILE_CELLS], end = S.offset[(t + 1) * TILE_CELLS];
-    if (beg == end) return;
-    int ti, tj, tk;
-    tile_coords(S, t, ti, tj, tk);
-    for (int q = threadIdx.x; q < BOX_ENTRIES; q += TILE_BLOCK) lbox[q] = 0ull;
-    __syncthreads();
+    const uint32_t n_active = *S.tile_count;
     // lanes of a wave take every 8th particle of a run, so concurrent LDS atomics land in
     // different cells
     const uint32_t local = (threadIdx.x & (TILE_BLOCK / 8 - 1)) * 8u + (threadIdx.x / (TILE_BLOCK / 8));
-    for (uint32_t base = beg; base < end; base += TILE_BLOCK) {
-        const uint32_t p = base + local;
-        if (p >= end) continue;
-        const float x[3] = {S.bx[p], S.by[p], S.bz[p]};
-        const float v[3] = {S.bvx[p], S.bvy[p], S.bvz[p]};
+    const float o[3] = {S.ox, S.oy, S.oz};
+    for (uint32_t u = blockIdx.x; u < n_active; u += gridDim.x) {
+        const uint32_t t = S.tile_list[u];
+        const uint32_t beg = S.offset[t * TILE_CELLS], end = S.offset[(t + 1) * TILE_CELLS];
+        int ti, tj, tk;
+        tile_coords(S, t, ti, tj, tk);
+        for (int q = threadIdx.x; q < BOX_ENTRIES; q += TILE_BLOCK) lbox[q] = 0ull;
+        __syncthreads();
+        for (uint32_t base = beg; base < end; base += TILE_BLOCK) {
+            const uint32_t p = base + local;
+            if (p >= end) continue;
+            const float x[3] = {S.bx[p], S.by[p], S.bz[p]};
+            const float v[3] = {S.bvx[p], S.bvy[p], S.bvz[p]};
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            int lo[3], d[3];
-            tile_box(c, ti, tj, tk, lo, d);
-            splat_component(S, c, x, v[c], [&](int qx, int qy, int qz, int a, int w) {
-                const int lx = qx - lo[0], ly = qy - lo[1], lz = qz - lo[2];
-                // every in-grid corner of a particle binned in this tile lies in the box
-                if ((unsigned)lx < (unsigned)d[0] && (unsigned)ly < (unsigned)d[1] && (unsigned)lz < (unsigned)d[2]) {
-                    const unsigned long long packed =
-                        ((unsigned long long)(uint32_t)a << 32) + (unsigned long long)(uint32_t)w;
-                    atomicAdd(&lbox[c * BOX_FACES + lx + d[0] * (ly + d[1] * lz)], packed);
+            for (int c = 0; c < 3; c++) {
+                int dim[3] = {S.nx, S.ny, S.nz};
+                dim[c] += 1;
+                int lo[3], d[3];
+                tile_box(c, ti, tj, tk, lo, d);
+                float g[3], wl[3], wh[3];
+                int b[3];
+                bool ok[3][2];
+#pragma unroll
+                for (int a = 0; a < 3; a++) g[a] = (x[a] - o[a]) * S.inv_h - 0.5f;
+                g[c] = g[c] + 0.5f;
+                bool in_box = true;
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    b[a] = (int)floorf(g[a]);
+                    const float f = g[a] - (float)b[a];
+                    wl[a] = 1.0f - f;
+                    wh[a] = f;
+                    ok[a][0] = b[a] >= 0 && b[a] <= dim[a] - 1;
+                    ok[a][1] = b[a] + 1 >= 0 && b[a] + 1 <= dim[a] - 1;
+                    in_box = in_box && b[a] >= lo[a] && b[a] + 1 < lo[a] + d[a];
                 }
-            });
+                if (!in_box) continue;  // unreachable for in-grid corners (see above)
+                const int li = c * BOX_FACES + (b[0] - lo[0]) + d[0] * ((b[1] - lo[1]) + d[1] * (b[2] - lo[2]));
+                const float wxy[4] = {wl[0] * wl[1], wh[0] * wl[1], wl[0] * wh[1], wh[0] * wh[1]};
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    const int ox = k & 1, oy = (k >> 1) & 1, oz = k >> 2;
+                    if (!(ok[0][ox] && ok[1][oy] && ok[2][oz])) continue;
+                    const float w = wxy[k & 3] * (oz ? wh[2] : wl[2]);
+                    const int a = (int)(w * v[c] * 65536.0f), wi = (int)(w * 65536.0f);
+                    const unsigned long long packed = ((unsigned long long)(uint32_t)a << 32) + (unsigned long long)(uint32_t)wi;
+                    atomicAdd(&lbox[li + box_corner(c, k)], packed);
+                }
+            }
         }
+        __syncthreads();
+        int64_t *out = S.box + (size_t)t * BOX_ENTRIES;
+        for (int q = threadIdx.x; q < BOX_ENTRIES; q += TILE_BLOCK) out[q] = (int64_t)lbox[q];
+        __syncthreads();
     }
-    __syncthreads();
-    int64_t *out = S.box + (size_t)t * BOX_ENTRIES;
-    for (int q = threadIdx.x; q < BOX_ENTRIES; q += TILE_BLOCK) out[q] = (int64_t)lbox[q];
 }
 
 // Trilinear stencil of sample_face (fluid.slang:67-80) for component c at one position:
@@ -269,6 +286,7 @@ struct Stencil {
     int q0[3], q1[3];
     int li, sx, sy, sz;
     bool in_box;
+    bool fast;  // no clamping and in the box: corner k is at li + box_corner(c, k)
 };
 
 __device__ __forceinline__ void make_stencil(const DevSim &S, int c, int ti, int tj, int tk, float x, float y,
@@ -295,12 +313,20 @@ __device__ __forceinline__ void make_stencil(const DevSim &S, int c, int ti, int
 #pragma unroll
     for (int k = 0; k < 8; k++) st.w[k] = wxy[k & 3] * ((k >> 2) ? wh[2] : wl[2]);
     st.li = (st.q0[0] - lo[0]) + d[0] * ((st.q0[1] - lo[1]) + d[1] * (st.q0[2] - lo[2]));
+    st.fast = st.in_box && st.q1[0] == st.q0[0] + 1 && st.q1[1] == st.q0[1] + 1 && st.q1[2] == st.q0[2] + 1;
     st.sx = st.q1[0] - st.q0[0];
     st.sy = (st.q1[1] - st.q0[1]) * d[0];
     st.sz = (st.q1[2] - st.q0[2]) * d[0] * d[1];
 }
 
 // s = sum_k w[k] * value[k], corners in order k = 0..7 (the reference's accumulation).
+__device__ __forceinline__ float sample_lds_fast(int c, const Stencil &st, const float *__restrict__ lds) {
+    float s = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; k++) s = s + st.w[k] * lds[st.li + box_corner(c, k)];
+    return s;
+}
+
 __device__ __forceinline__ float sample_lds(const Stencil &st, const float *__restrict__ lds) {
     float s = 0.0f;
 #pragma unroll
@@ -332,9 +358,10 @@ __device__ __forceinline__ float sample_global(const DevSim &S, int c, const Ste
 __global__ __launch_bounds__(TILE_BLOCK) void k_g2p_tile(DevSim S) {
     __shared__ float lvel[BOX_ENTRIES];
     __shared__ float lold[BOX_ENTRIES];
-    const uint32_t t = blockIdx.x;
+    const uint32_t n_active = *S.tile_count;
+    for (uint32_t u = blockIdx.x; u < n_active; u += gridDim.x) {
+    const uint32_t t = S.tile_list[u];
     const uint32_t beg = S.offset[t * TILE_CELLS], end = S.offset[(t + 1) * TILE_CELLS];
-    if (beg == end) return;
     int ti, tj, tk;
     tile_coords(S, t, ti, tj, tk);
     for (int q = threadIdx.x; q < BOX_ENTRIES; q += TILE_BLOCK) {
@@ -366,7 +393,10 @@ __global__ __launch_bounds__(TILE_BLOCK) void k_g2p_tile(DevSim S) {
             Stencil st;
             make_stencil(S, c, ti, tj, tk, x[0], x[1], x[2], st);
             // separate paths so the LDS reads compile to ds_read, not generic loads
-            if (st.in_box) {
+            if (st.fast) {
+                pic[c] = sample_lds_fast(c, st, lvel + c * BOX_FACES);
+                old[c] = sample_lds_fast(c, st, lold + c * BOX_FACES);
+            } else if (st.in_box) {
                 pic[c] = sample_lds(st, lvel + c * BOX_FACES);
                 old[c] = sample_lds(st, lold + c * BOX_FACES);
             } else {
@@ -384,7 +414,8 @@ __global__ __launch_bounds__(TILE_BLOCK) void k_g2p_tile(DevSim S) {
         for (int c = 0; c < 3; c++) {
             Stencil st;
             make_stencil(S, c, ti, tj, tk, mid[0], mid[1], mid[2], st);
-            if (st.in_box) adv[c] = sample_lds(st, lvel + c * BOX_FACES);
+            if (st.fast) adv[c] = sample_lds_fast(c, st, lvel + c * BOX_FACES);
+            else if (st.in_box) adv[c] = sample_lds(st, lvel + c * BOX_FACES);
             else adv[c] = sample_global(S, c, st, S.vel);
         }
 #pragma unroll
@@ -403,6 +434,8 @@ __global__ __launch_bounds__(TILE_BLOCK) void k_g2p_tile(DevSim S) {
         S.avy[i] = v[1];
         S.avz[i] = v[2];
         S.aid[i] = S.bid[i];
+    }
+    __syncthreads();
     }
 }
 
@@ -502,8 +535,13 @@ void launch_exclusive_scan(const uint32_t *in, uint32_t n, uint32_t *tile_tmp, u
 void launch_scan_offsets(const DevSim &S, hipStream_t s) { launch_exclusive_scan(S.count, S.nkeys, S.scan_tmp, S.offset, s); }
 
 void launch_scatter(const DevSim &S, hipStream_t s) { k_scatter<<<blocks_for(S.np), BLOCK, 0, s>>>(S); }
-void launch_p2g(const DevSim &S, hipStream_t s) { k_p2g_tile<<<S.ntiles, TILE_BLOCK, 0, s>>>(S); }
-void launch_g2p(const DevSim &S, hipStream_t s) { k_g2p_tile<<<S.ntiles, TILE_BLOCK, 0, s>>>(S); }
+// Workgroups of the per-tile kernels: enough to keep every CU busy, each walking the active
+// tile list with a grid stride.
+inline unsigned tile_grid(const DevSim &S) { return S.ntiles < 1024u ? S.ntiles : 1024u; }
+
+void launch_tile_list(const DevSim &S, hipStream_t s) { k_tile_list<<<blocks_for(S.ntiles), BLOCK, 0, s>>>(S); }
+void launch_p2g(const DevSim &S, hipStream_t s) { k_p2g_tile<<<tile_grid(S), TILE_BLOCK, 0, s>>>(S); }
+void launch_g2p(const DevSim &S, hipStream_t s) { k_g2p_tile<<<tile_grid(S), TILE_BLOCK, 0, s>>>(S); }
 
 void launch_bins_out(const DevSim &S, const uint32_t *off_lin, uint32_t *cell_of, uint32_t *sorted, hipStream_t s) {
     k_bins_out<<<blocks_for(S.np), BLOCK, 0, s>>>(S, off_lin, cell_of, sorted);
