@@ -46,6 +46,19 @@ def algorithmic_bytes(name: str, P: int, F: int, C: int, Cf: int) -> float:
     return float(table.get(name, 0))
 
 
+def measured_traffic(kernel: str, args):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary of this workload (profiles/), or None when there is none for it."""
+    path = os.path.join(ROOT, "profiles", f"traffic_dam{args.res}_{args.solver}{args.iters}.json")
+    if not os.path.exists(path):
+        return None
+    per = json.load(open(path))["per_dispatch"]
+    for key in (f"k_{kernel}", f"k_{kernel}<float>", f"k_{kernel}_tile"):
+        if key in per:
+            return per[key]["hbm_bytes"]
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,8 +107,8 @@ def main():
     dom = per_kernel[0]
     achieved = dom["achieved_GBs"] or 0.0
     roofline = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": achieved / PEAK_HBM_GBS, "traffic": None, "kernel": dom["kernel"],
-                "avg_us": dom["avg_us"], "alg_bytes_per_launch": dom["alg_bytes"]}
+                "frac": achieved / PEAK_HBM_GBS, "traffic": measured_traffic(dom["kernel"], args),
+                "kernel": dom["kernel"], "avg_us": dom["avg_us"], "alg_bytes_per_launch": dom["alg_bytes"]}
     stats = sim.last_solve()
     sim.close()
 

@@ -58,9 +58,17 @@ __device__ __forceinline__ void nb_offsets(const DevSim &S, int32_t off[6]) {
     off[0] = -1; off[1] = 1; off[2] = -sy; off[3] = sy; off[4] = -sz; off[5] = sz;
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup i runs on XCD i % 8), and each
+// XCD has its own L2.  Slice s of the list goes to workgroup xcd_slot(s), so each XCD gets
+// one contiguous eighth of the list and the neighbour rows and planes a slice reads are
+// mostly in the same L2.
+constexpr uint32_t NXCD = 8;
+
 // This workgroup's slice of list entries [lo, hi).
 __device__ __forceinline__ void list_slice(uint32_t lo, uint32_t hi, uint32_t &beg, uint32_t &end) {
-    const uint64_t n = hi - lo, g = gridDim.x, b = blockIdx.x;
+    const uint64_t n = hi - lo, g = gridDim.x;
+    uint64_t b = blockIdx.x;
+    if (g % NXCD == 0) b = (b % NXCD) * (g / NXCD) + b / NXCD;
     beg = lo + (uint32_t)(n * b / g);
     end = lo + (uint32_t)(n * (b + 1) / g);
 }
