@@ -36,9 +36,10 @@ extern "C" {
 
 /* Pressure solver.  RBGS is the reference's red-black Gauss-Seidel with
  * pressure_iterations sweeps (fluid.slang:167-182, fluid.rs:295-300).  CG and MIXED are
- * extensions: Jacobi-preconditioned conjugate gradients on the same matrix, FP32 with a
- * fixed iteration count (CG), or FP32 iterations with FP64 dot products inside an FP64
- * iterative-refinement loop run to a relative FP64 residual (MIXED). */
+ * extensions on the same matrix: Jacobi-preconditioned conjugate gradients in FP32 with a
+ * fixed iteration count (CG), or multigrid-preconditioned CG iterations in FP32 with FP64
+ * dot products inside an FP64 iterative-refinement loop run to a relative FP64 residual
+ * (MIXED). */
 #define PXLFLIP_SOLVER_RBGS 0
 #define PXLFLIP_SOLVER_CG 1
 #define PXLFLIP_SOLVER_MIXED 2
@@ -50,7 +51,7 @@ typedef struct pxlflip_opts {
     uint32_t struct_size;     /* sizeof(pxlflip_opts) */
     int32_t solver;           /* PXLFLIP_SOLVER_* (default RBGS) */
     uint32_t cg_iterations;   /* CG: iterations per step; 0 = pressure_iterations */
-    uint32_t inner_iterations;/* MIXED: FP32 iterations per refinement pass (default 40) */
+    uint32_t inner_iterations;/* MIXED: FP32 multigrid-CG iterations per refinement pass (default 10) */
     uint32_t max_outer;       /* MIXED: refinement passes cap (default 50) */
     double tolerance;         /* MIXED: ||b - A p|| / ||b|| target in FP64 (default 1e-6) */
     int32_t device;           /* HIP device ordinal (default 0) */

@@ -155,6 +155,14 @@ struct Sim {
         add(&d.fl_tmp, 4 * (scan_tiles(2 * (uint32_t)C) + 1));
         for (float **p : {&d.cg_r0, &d.cg_r1, &d.cg_w0, &d.cg_w1, &d.cg_s0, &d.cg_s1, &d.cg_p}) add(p, 4 * C);
         if (opts.solver == PXLFLIP_SOLVER_MIXED) {
+            add(&d.mg_z1, 4 * C);
+            for (int l = 1; l < d.mg_levels; l++) {
+                const size_t cl = (size_t)d.mg[l].nx * d.mg[l].ny * d.mg[l].nz;
+                add(&d.mg[l].type, cl);
+                add(&d.mg[l].r, 4 * cl);
+                add(&d.mg[l].z1, 4 * cl);
+                add(&d.mg[l].zf, 4 * cl);
+            }
             add(&d.e, 4 * C);
             add(&d.x64a, 8 * C);
             add(&d.x64b, 8 * C);
@@ -212,6 +220,7 @@ struct Sim {
         KRUN("faces", launch_faces(d, stream));
         KRUN("cells", launch_cells(d, stream));
         KRUN("fluid_list", launch_fluid_list(d, stream));
+        if (d.mg_levels) KRUN("mg_types", launch_mg_types(d, stream));
     }
     void enqueue_fixed_pressure() {
         if (opts.solver == PXLFLIP_SOLVER_RBGS) {
@@ -234,8 +243,9 @@ struct Sim {
         enqueue_post();
     }
     void enqueue_inner(int cur) {
-        KRUN("cg64_start", launch_cg_start(d, true, d.e, true, stream));
-        for (uint32_t it = 0; it < opts.inner_iterations; it++) KRUN("cg64_iter", launch_cg_iter(d, true, d.e, it, stream));
+        KRUN("mgcg_start", launch_mgcg_start(d, d.e, stream));
+        for (uint32_t it = 0; it < opts.inner_iterations; it++)
+            KRUN("mgcg_iter", launch_mgcg_iter(d, d.e, it, it + 1 == opts.inner_iterations, stream));
         KRUN("mixed_update", launch_mixed_update(d, x64(cur), x64(cur ^ 1), stream));
     }
     void enqueue_inner0() { enqueue_inner(0); }
@@ -377,7 +387,7 @@ int pxlflip_default_opts(pxlflip_opts *o) {
     o->struct_size = sizeof(pxlflip_opts);
     o->solver = PXLFLIP_SOLVER_RBGS;
     o->cg_iterations = 0;
-    o->inner_iterations = 40;
+    o->inner_iterations = 10;
     o->max_outer = 50;
     o->tolerance = 1e-6;
     o->device = 0;
@@ -478,6 +488,28 @@ int prepare(const char *fluid_json, double fps, uint32_t seed, const pxlflip_opt
     d.nkeys = d.ntiles * TILE_CELLS;
     d.cg_blocks = (int)(o.solver_workgroups ? o.solver_workgroups : DEFAULT_SOLVER_BLOCKS);
     d.split_colours = o.solver == PXLFLIP_SOLVER_RBGS ? 1 : 0;
+    d.mg_levels = 0;
+    if (o.solver == PXLFLIP_SOLVER_MIXED) {
+        int l = 0;
+        d.mg[0].nx = d.nx;
+        d.mg[0].ny = d.ny;
+        d.mg[0].nz = d.nz;
+        d.mg[0].scale = 1.0f;
+        // at least one coarse level, then halve until the coarsest is small
+        while ((l == 0 || (size_t)d.mg[l].nx * d.mg[l].ny * d.mg[l].nz > (size_t)MG_COARSE_CELLS) && l + 1 < MG_MAX) {
+            MGLevel &c = d.mg[l + 1];
+            c.nx = (d.mg[l].nx + 1) / 2;
+            c.ny = (d.mg[l].ny + 1) / 2;
+            c.nz = (d.mg[l].nz + 1) / 2;
+            c.scale = d.mg[l].scale * 4.0f;
+            l++;
+        }
+        d.mg_levels = l + 1;
+        if ((size_t)d.mg[l].nx * d.mg[l].ny * d.mg[l].nz > (size_t)MG_COARSE_MAX) {
+            g_create_error = "fluid '" + spec.name + "': grid too large for the multigrid hierarchy";
+            return PXLFLIP_ERR_INVALID;
+        }
+    }
     return PXLFLIP_OK;
 }
 

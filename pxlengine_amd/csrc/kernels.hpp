@@ -31,6 +31,18 @@ constexpr uint16_t STEN_FLUID = 0x8000;
 constexpr int TILE = 8;
 constexpr uint32_t TILE_CELLS = 512;
 
+// Multigrid levels of the mixed-precision solve (level 0 = the simulation grid).
+constexpr int MG_MAX = 10;
+constexpr int MG_COARSE_CELLS = 512;    // coarsening stops at or below this many cells
+constexpr int MG_COARSE_MAX = 4096;     // capacity of the coarsest-level LDS solve
+
+struct MGLevel {
+    int nx, ny, nz;
+    float scale;         // 4^l: A_l = scale * (unit 7-point stencil on level-l types)
+    uint8_t *type;       // level-l cell types (levels >= 1)
+    float *r, *z1, *zf;  // restricted residual, pre-smoothed value, corrected + post-smoothed
+};
+
 // Solver scalar slots (fp64 array): two slots of SC_COUNT by CG-iteration parity for the
 // (alpha, beta, gamma) history; the refinement norms live in slot 0.
 enum ScalarSlot { SC_RZ = 0, SC_ALPHA = 1, SC_BETA = 2, SC_RR = 3, SC_BB = 4, SC_COUNT = 8 };
@@ -77,6 +89,9 @@ struct DevSim {
     // solver
     float *cg_r0, *cg_r1, *cg_w0, *cg_w1, *cg_s0, *cg_s1, *cg_p, *e;
     double *x64a, *x64b, *r64;
+    MGLevel mg[MG_MAX];
+    int mg_levels;  // levels including level 0 (0 when the multigrid is not allocated)
+    float *mg_z1;   // level-0 pre-smoothed values
     float *part32;   // two slots of 2 x (workgroups at z-batch 1)
     double *part64;
     double *scal;
@@ -165,6 +180,13 @@ void launch_g2p(const DevSim &S, hipStream_t s);              // b -> a, advecte
 uint32_t solver_blocks(const DevSim &S);  // workgroups of a solver launch (partials per value)
 void launch_cg_start(const DevSim &S, bool fp64, float *x, bool inner, hipStream_t s);
 void launch_cg_iter(const DevSim &S, bool fp64, float *x, uint32_t it, hipStream_t s);
+// Multigrid-preconditioned CG (mg.hip + solver.hip), the inner solve of the refinement:
+//   launch_mg_types once per step; launch_mgcg_start(e); then for it = 0..K-1:
+//   launch_mgcg_iter(e, it, it == K-1).  FP64 dot products.
+void launch_mg_types(const DevSim &S, hipStream_t s);
+void launch_mg_vcycle(const DevSim &S, const float *r, float *u, hipStream_t s);  // u = M r
+void launch_mgcg_start(const DevSim &S, float *x, hipStream_t s);
+void launch_mgcg_iter(const DevSim &S, float *x, uint32_t it, bool last, hipStream_t s);
 // FP64 refinement (solver.hip); init/update also total the norms into scal[SC_RR / SC_BB].
 void launch_mixed_init(const DevSim &S, double *x64, hipStream_t s);  // x64 = p, r64 = b - A x64
 void launch_mixed_update(const DevSim &S, const double *xsrc, double *xdst, hipStream_t s);  // xdst = xsrc + e; r64

@@ -248,6 +248,57 @@ __global__ __launch_bounds__(SBLOCK) void k_cg_iter(DevSim S, uint32_t it, float
     store_partials<Acc, 2>(v, partials<Acc>(S) + (size_t)((it + 1) & 1) * 2 * gridDim.x);
 }
 
+// ---- multigrid-preconditioned CG (Chronopoulos-Gear with u = M r from the V-cycle).
+// Vectors: r = cg_r0, u = cg_r1, w = cg_w0, s = cg_s0, p = cg_p; x = the inner unknown.
+// spmv: w = A u and the partials of gamma = (r, u), delta = (w, u) into slot `slot`
+// (first: also p_{-1} = s_{-1} = 0).
+template <typename Acc>
+__global__ __launch_bounds__(SBLOCK) void k_mgcg_spmv(DevSim S, uint32_t slot, int first) {
+    uint32_t beg, end;
+    list_slice(0, S.fl_count[1], beg, end);
+    int32_t off[6];
+    nb_offsets(S, off);
+    const float *__restrict__ u = S.cg_r1;
+    Acc g = 0, d = 0;
+    for (uint32_t t = beg + threadIdx.x; t < end; t += SBLOCK) {
+        const uint32_t c = S.fl_list[t];
+        const uint32_t wd = S.fl_word[t];
+        float sum = 0.0f;
+#pragma unroll
+        for (int n = 0; n < 6; n++)
+            if (w_fluid(wd, n)) sum = sum + u[c + off[n]];
+        const float uc = u[c];
+        const float wc = w_diag(wd) * uc - sum;
+        S.cg_w0[c] = wc;
+        if (first) {
+            S.cg_p[c] = 0.0f;
+            S.cg_s0[c] = 0.0f;
+        }
+        g += (Acc)S.cg_r0[c] * (Acc)uc;
+        d += (Acc)wc * (Acc)uc;
+    }
+    const Acc v[2] = {g, d};
+    store_partials<Acc, 2>(v, partials<Acc>(S) + (size_t)slot * 2 * gridDim.x);
+}
+
+// update of iteration it: scalars from slot it & 1, then p, s, x, r (no neighbours).
+template <typename Acc>
+__global__ __launch_bounds__(SBLOCK) void k_mgcg_update(DevSim S, uint32_t it, float *__restrict__ x) {
+    uint32_t beg, end;
+    list_slice(0, S.fl_count[1], beg, end);
+    float alpha, beta;
+    cg_scalars<Acc>(S, it, gridDim.x, alpha, beta);
+    for (uint32_t t = beg + threadIdx.x; t < end; t += SBLOCK) {
+        const uint32_t c = S.fl_list[t];
+        const float pc = S.cg_r1[c] + beta * S.cg_p[c];
+        const float sc = S.cg_w0[c] + beta * S.cg_s0[c];
+        S.cg_p[c] = pc;
+        S.cg_s0[c] = sc;
+        x[c] = x[c] + alpha * pc;
+        S.cg_r0[c] = S.cg_r0[c] - alpha * sc;
+    }
+}
+
 // ---- FP64 outer loop.  The solution is double-buffered (x_src -> x_dst) so the update pass
 // can rebuild neighbours' x_src + e from values no lane has overwritten.
 __device__ __forceinline__ double b64_of(const DevSim &S, uint32_t id) {
@@ -374,6 +425,21 @@ void launch_cg_iter(const DevSim &S, bool fp64, float *x, uint32_t it, hipStream
     float *sprev = s_buf(S, it + 1), *scur = s_buf(S, it);
     if (fp64) k_cg_iter<double><<<g, SBLOCK, 0, s>>>(S, it, x, rin, rout, win, wout, sprev, scur);
     else k_cg_iter<float><<<g, SBLOCK, 0, s>>>(S, it, x, rin, rout, win, wout, sprev, scur);
+}
+
+void launch_mgcg_start(const DevSim &S, float *x, hipStream_t s) {
+    const uint32_t g = solver_blocks(S);
+    k_cg_residual<<<g, SBLOCK, 0, s>>>(S, x, S.cg_r0, 1);
+    launch_mg_vcycle(S, S.cg_r0, S.cg_r1, s);
+    k_mgcg_spmv<double><<<g, SBLOCK, 0, s>>>(S, 0, 1);
+}
+
+void launch_mgcg_iter(const DevSim &S, float *x, uint32_t it, bool last, hipStream_t s) {
+    const uint32_t g = solver_blocks(S);
+    k_mgcg_update<double><<<g, SBLOCK, 0, s>>>(S, it, x);
+    if (last) return;
+    launch_mg_vcycle(S, S.cg_r0, S.cg_r1, s);
+    k_mgcg_spmv<double><<<g, SBLOCK, 0, s>>>(S, (it + 1) & 1, 0);
 }
 
 void launch_mixed_init(const DevSim &S, double *x64, hipStream_t s) {
