@@ -155,13 +155,12 @@ struct Sim {
         add(&d.fl_tmp, 4 * (scan_tiles(2 * (uint32_t)C) + 1));
         for (float **p : {&d.cg_r0, &d.cg_r1, &d.cg_w0, &d.cg_w1, &d.cg_s0, &d.cg_s1, &d.cg_p}) add(p, 4 * C);
         if (opts.solver == PXLFLIP_SOLVER_MIXED) {
-            add(&d.mg_z1, 4 * C);
             for (int l = 1; l < d.mg_levels; l++) {
                 const size_t cl = (size_t)d.mg[l].nx * d.mg[l].ny * d.mg[l].nz;
                 add(&d.mg[l].type, cl);
+                add(&d.mg[l].sten, 2 * cl);
                 add(&d.mg[l].r, 4 * cl);
-                add(&d.mg[l].z1, 4 * cl);
-                add(&d.mg[l].zf, 4 * cl);
+                add(&d.mg[l].z, 4 * cl);
             }
             add(&d.e, 4 * C);
             add(&d.x64a, 8 * C);
@@ -387,7 +386,7 @@ int pxlflip_default_opts(pxlflip_opts *o) {
     o->struct_size = sizeof(pxlflip_opts);
     o->solver = PXLFLIP_SOLVER_RBGS;
     o->cg_iterations = 0;
-    o->inner_iterations = 10;
+    o->inner_iterations = 8;
     o->max_outer = 50;
     o->tolerance = 1e-6;
     o->device = 0;
@@ -494,18 +493,24 @@ int prepare(const char *fluid_json, double fps, uint32_t seed, const pxlflip_opt
         d.mg[0].nx = d.nx;
         d.mg[0].ny = d.ny;
         d.mg[0].nz = d.nz;
-        d.mg[0].scale = 1.0f;
-        // at least one coarse level, then halve until the coarsest is small
-        while ((l == 0 || (size_t)d.mg[l].nx * d.mg[l].ny * d.mg[l].nz > (size_t)MG_COARSE_CELLS) && l + 1 < MG_MAX) {
+        auto cells_of = [&](int q) { return (size_t)d.mg[q].nx * d.mg[q].ny * d.mg[q].nz; };
+        // at least one coarse level, then halve until no axis exceeds 2 cells
+        while ((l == 0 || std::max(d.mg[l].nx, std::max(d.mg[l].ny, d.mg[l].nz)) > 2) && l + 1 < MG_MAX) {
             MGLevel &c = d.mg[l + 1];
             c.nx = (d.mg[l].nx + 1) / 2;
             c.ny = (d.mg[l].ny + 1) / 2;
             c.nz = (d.mg[l].nz + 1) / 2;
-            c.scale = d.mg[l].scale * 4.0f;
             l++;
         }
         d.mg_levels = l + 1;
-        if ((size_t)d.mg[l].nx * d.mg[l].ny * d.mg[l].nz > (size_t)MG_COARSE_MAX) {
+        // the tail: the longest run of coarsest levels (from level 1 on) that fits the LDS
+        size_t tail = 0;
+        d.mg_tail = l + 1;
+        for (int q = l; q >= 1 && tail + cells_of(q) <= (size_t)MG_TAIL_CELLS; q--) {
+            tail += cells_of(q);
+            d.mg_tail = q;
+        }
+        if (d.mg_tail > l) {
             g_create_error = "fluid '" + spec.name + "': grid too large for the multigrid hierarchy";
             return PXLFLIP_ERR_INVALID;
         }

@@ -31,16 +31,17 @@ constexpr uint16_t STEN_FLUID = 0x8000;
 constexpr int TILE = 8;
 constexpr uint32_t TILE_CELLS = 512;
 
-// Multigrid levels of the mixed-precision solve (level 0 = the simulation grid).
-constexpr int MG_MAX = 10;
-constexpr int MG_COARSE_CELLS = 512;    // coarsening stops at or below this many cells
-constexpr int MG_COARSE_MAX = 4096;     // capacity of the coarsest-level LDS solve
+// Multigrid levels of the mixed-precision solve (level 0 = the simulation grid).  Coarsening
+// halves every axis until no axis exceeds 2 cells; the levels from mg_tail on (at most
+// MG_TAIL_CELLS cells together) are solved inside one workgroup's LDS.
+constexpr int MG_MAX = 12;
+constexpr int MG_TAIL_CELLS = 6144;
 
 struct MGLevel {
     int nx, ny, nz;
-    float scale;         // 4^l: A_l = scale * (unit 7-point stencil on level-l types)
-    uint8_t *type;       // level-l cell types (levels >= 1)
-    float *r, *z1, *zf;  // restricted residual, pre-smoothed value, corrected + post-smoothed
+    uint8_t *type;   // level-l cell types (levels >= 1)
+    uint16_t *sten;  // level-l stencil words (same layout as DevSim::sten; levels >= 1)
+    float *r, *z;    // level-l right-hand side (the scaled restricted residual) and iterate
 };
 
 // Solver scalar slots (fp64 array): two slots of SC_COUNT by CG-iteration parity for the
@@ -91,7 +92,7 @@ struct DevSim {
     double *x64a, *x64b, *r64;
     MGLevel mg[MG_MAX];
     int mg_levels;  // levels including level 0 (0 when the multigrid is not allocated)
-    float *mg_z1;   // level-0 pre-smoothed values
+    int mg_tail;    // first level of the single-workgroup tail (>= 1)
     float *part32;   // two slots of 2 x (workgroups at z-batch 1)
     double *part64;
     double *scal;

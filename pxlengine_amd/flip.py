@@ -151,7 +151,7 @@ def _json(fluid) -> bytes:
     return json.dumps(fluid).encode()
 
 
-def make_opts(solver="rbgs", cg_iterations=0, inner_iterations=10, max_outer=50, tolerance=1e-6,
+def make_opts(solver="rbgs", cg_iterations=0, inner_iterations=8, max_outer=50, tolerance=1e-6,
               device=0, scene_index=0, use_graph=True, solver_workgroups=0) -> Opts:
     o = Opts()
     load_library().pxlflip_default_opts(C.byref(o))
