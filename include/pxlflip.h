@@ -57,7 +57,7 @@ typedef struct pxlflip_opts {
     int32_t device;           /* HIP device ordinal (default 0) */
     uint32_t scene_index;     /* i in "fluids[i] 'name': ..." validation messages */
     int32_t use_graph;        /* 1 = replay each step as a captured hipGraph (default 1) */
-    uint32_t solver_workgroups; /* workgroups of the pressure-solver launches; 0 = default (1024), max 4096 */
+    uint32_t solver_workgroups; /* workgroups of the pressure-solver launches; 0 = default (256), max 4096 */
 } pxlflip_opts;
 
 /* Fills *opts with the defaults above. */
@@ -187,6 +187,89 @@ int pxlflip_autotune(pxlflip *h, uint32_t steps, uint32_t *chosen_workgroups, do
 
 /* Last error message of a handle; with NULL, the calling thread's last create error. */
 const char *pxlflip_last_error(const pxlflip *h);
+
+/* ---- Serving: many independent liquids stepped together.
+ * No counterpart in the reference (it steps one fluid per scene, synchronously, inside
+ * Renderer::render_sequence, crates/pxl-render/src/lib.rs:204-252).  A server owns a device
+ * thread that keeps every admitted simulation resident in HBM and advances all of them one
+ * step at a time with batched launches (one launch per kernel for the whole batch, replayed
+ * as hipGraphs), using the MIXED solver for each.  Requests are admitted first-in first-out
+ * while their HBM footprint fits the budget; a request arriving while others run joins the
+ * batch at the next step boundary (continuous batching), and a finished one leaves at the
+ * step boundary after its last step.  Each simulation computes exactly what it computes
+ * alone: same kernels, same workgroup partition and reduction order, its own convergence.
+ * submit / poll / wait / cancel / stats may be called from any host thread. */
+typedef struct pxlflip_server pxlflip_server;
+
+typedef struct pxlflip_server_opts {
+    uint32_t struct_size;       /* sizeof(pxlflip_server_opts) */
+    int32_t device;             /* HIP device ordinal (default 0) */
+    uint64_t budget_bytes;      /* HBM admission budget (default 200e9) */
+    uint32_t max_sims;          /* most simulations in the batch at once (default 256, max 4096) */
+    uint32_t inner_iterations;  /* as pxlflip_opts (default 8) */
+    uint32_t max_outer;         /* as pxlflip_opts (default 50) */
+    double tolerance;           /* as pxlflip_opts (default 1e-6) */
+    uint32_t solver_workgroups; /* per simulation; 0 = chosen by grid size */
+    int32_t use_graph;          /* 1 = replay the batch phases as hipGraphs (default 1) */
+} pxlflip_server_opts;
+
+#define PXLFLIP_REQ_QUEUED 0
+#define PXLFLIP_REQ_RUNNING 1
+#define PXLFLIP_REQ_DONE 2
+#define PXLFLIP_REQ_FAILED 3
+#define PXLFLIP_REQ_CANCELLED 4
+
+typedef struct pxlflip_request_info {
+    int32_t status;          /* PXLFLIP_REQ_* */
+    uint32_t steps;          /* requested */
+    uint32_t steps_done;
+    uint32_t n_particles;
+    uint64_t device_bytes;   /* HBM charged against the budget */
+    double queue_ms;         /* submit -> admission */
+    double latency_ms;       /* submit -> completion (DONE) */
+    double rel_residual;     /* FP64 relative residual of the last step's pressure solve */
+    uint32_t max_outer;      /* most refinement passes any of its steps took */
+    uint32_t pad;
+} pxlflip_request_info;
+
+typedef struct pxlflip_server_stats {
+    uint64_t submitted, completed, failed, cancelled, queued, running;
+    double p50_ms, p99_ms, mean_ms, max_ms; /* submit -> completion over completed requests */
+    double particle_steps;   /* sum over completed requests of particles x steps */
+    double span_s;           /* first submit -> last completion */
+    double throughput;       /* particle_steps / span_s */
+    double busy_s;           /* device-thread time spent in batch steps */
+    uint64_t batch_steps;    /* batch steps run */
+    double mean_batch;       /* mean simulations per batch step */
+    uint64_t bytes_in_use, peak_bytes;
+    double admit_s;          /* device-thread time spent allocating admitted requests */
+    uint64_t rebuilds;       /* work-list rebuilds (batch composition changes) */
+    uint64_t graph_captures; /* batch graphs captured */
+} pxlflip_server_stats;
+
+int pxlflip_server_default_opts(pxlflip_server_opts *opts);
+/* opts may be NULL.  Fails (PXLFLIP_ERR_DEVICE) when the device cannot be used. */
+int pxlflip_server_create(const pxlflip_server_opts *opts, pxlflip_server **out);
+/* Validates the fluid block (as pxlflip_create_from_json) and queues `steps` steps of it.
+ * keep_particles = 1 keeps the final particles for pxlflip_server_read_particles. */
+int pxlflip_server_submit(pxlflip_server *srv, const char *fluid_json, double fps, uint32_t seed, uint32_t steps,
+                          int32_t keep_particles, uint64_t *id);
+int pxlflip_server_poll(pxlflip_server *srv, uint64_t id, pxlflip_request_info *info);
+/* Blocks until the request is DONE, FAILED or CANCELLED, or timeout_s passes (then returns
+ * PXLFLIP_ERR_INVALID with the info of the moment); timeout_s < 0 waits without limit. */
+int pxlflip_server_wait(pxlflip_server *srv, uint64_t id, double timeout_s, pxlflip_request_info *info);
+/* Blocks until nothing is queued or running, or timeout_s passes. */
+int pxlflip_server_drain(pxlflip_server *srv, double timeout_s);
+int pxlflip_server_cancel(pxlflip_server *srv, uint64_t id);
+/* The final particles of a DONE request submitted with keep_particles (AoS8, seeding order). */
+int pxlflip_server_read_particles(pxlflip_server *srv, uint64_t id, float *aos8, uint64_t max_particles);
+/* Forgets a finished request (its kept particles). */
+int pxlflip_server_release(pxlflip_server *srv, uint64_t id);
+int pxlflip_server_get_stats(pxlflip_server *srv, pxlflip_server_stats *stats);
+/* Message of the last failed call on this server (or of a failed request via poll). */
+const char *pxlflip_server_last_error(const pxlflip_server *srv);
+/* Stops the device thread (requests still queued or running are cancelled) and frees all. */
+void pxlflip_server_destroy(pxlflip_server *srv);
 
 #ifdef __cplusplus
 }

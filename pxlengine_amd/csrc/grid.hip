@@ -13,6 +13,14 @@ namespace pxl {
 namespace {
 
 constexpr int TX = 64, TY = 4;
+constexpr unsigned BATCH_GRID = 2048;  // persistent grid of batched launches
+
+// Node / cell launch grids (x, y blocks; z = one plane per block row).  A batched launch
+// numbers a simulation's blocks b = bx + gx (by + gy bz).
+__host__ __device__ inline uint32_t node_gx(const DevSim &S) { return (uint32_t)(S.nx + 1 + TX - 1) / TX; }
+__host__ __device__ inline uint32_t node_gy(const DevSim &S) { return (uint32_t)(S.ny + 1 + TY - 1) / TY; }
+__host__ __device__ inline uint32_t cell_gx(const DevSim &S) { return (uint32_t)(S.nx + TX - 1) / TX; }
+__host__ __device__ inline uint32_t cell_gy(const DevSim &S) { return (uint32_t)(S.ny + TY - 1) / TY; }
 
 // SOLID for cells outside the grid or on its one-cell wall layer (fluid.slang:55-58, 126-128).
 __device__ __forceinline__ bool is_solid(const DevSim &S, int i, int j, int k) {
@@ -45,15 +53,22 @@ __device__ __forceinline__ void face_update(const DevSim &S, int c, int i, int j
     S.vel[f] = keep ? v + g * S.dt : 0.0f;
 }
 
-__global__ __launch_bounds__(TX *TY) void k_faces(DevSim S) {
-    const int i = blockIdx.x * TX + threadIdx.x;
-    const int j = blockIdx.y * TY + threadIdx.y;
-    const int k = blockIdx.z;
+__device__ __forceinline__ void body_faces(const DevSim &S, uint32_t bx, uint32_t by, uint32_t bz) {
+    const int i = bx * TX + threadIdx.x;
+    const int j = by * TY + threadIdx.y;
+    const int k = bz;
     if (i > S.nx || j > S.ny) return;
     const bool in_c = !is_solid(S, i, j, k);
     if (j < S.ny && k < S.nz) face_update(S, 0, i, j, k, uidx(S, i, j, k), S.gx, in_c && !is_solid(S, i - 1, j, k));
     if (i < S.nx && k < S.nz) face_update(S, 1, i, j, k, vidx(S, i, j, k), S.gy, in_c && !is_solid(S, i, j - 1, k));
     if (i < S.nx && j < S.ny) face_update(S, 2, i, j, k, widx(S, i, j, k), S.gz, in_c && !is_solid(S, i, j, k - 1));
+}
+__global__ __launch_bounds__(TX *TY) void k_faces(DevSim S) { body_faces(S, blockIdx.x, blockIdx.y, blockIdx.z); }
+__global__ __launch_bounds__(TX *TY) void kb_faces(WorkList W) {
+    for_each_work(W, [&](const DevSim &S, uint32_t b) {
+        const uint32_t gx = node_gx(S), gy = node_gy(S);
+        body_faces(S, b % gx, (b / gx) % gy, b / (gx * gy));
+    });
 }
 
 // The P2G sums as the reference's acc / wacc face buffers (for read-back).
@@ -75,10 +90,10 @@ __device__ __forceinline__ uint8_t type_from_count(const DevSim &S, int i, int j
 
 // normalize_grid cells + compute_divergence (fluid.slang:125-131, 154-165), plus the stencil
 // word the pressure kernels read instead of re-classifying six neighbours.
-__global__ __launch_bounds__(TX *TY) void k_cells(DevSim S) {
-    const int i = blockIdx.x * TX + threadIdx.x;
-    const int j = blockIdx.y * TY + threadIdx.y;
-    const int k = blockIdx.z;
+__device__ __forceinline__ void body_cells(const DevSim &S, uint32_t bx, uint32_t by, uint32_t bz) {
+    const int i = bx * TX + threadIdx.x;
+    const int j = by * TY + threadIdx.y;
+    const int k = bz;
     if (i >= S.nx || j >= S.ny) return;
     const uint32_t id = cidx(S, i, j, k);
     const uint8_t t = type_from_count(S, i, j, k);
@@ -111,12 +126,19 @@ __global__ __launch_bounds__(TX *TY) void k_cells(DevSim S) {
         S.fl_flag[id] = t == CELL_FLUID ? 1u : 0u;
     }
 }
+__global__ __launch_bounds__(TX *TY) void k_cells(DevSim S) { body_cells(S, blockIdx.x, blockIdx.y, blockIdx.z); }
+__global__ __launch_bounds__(TX *TY) void kb_cells(WorkList W) {
+    for_each_work(W, [&](const DevSim &S, uint32_t b) {
+        const uint32_t gx = cell_gx(S), gy = cell_gy(S);
+        body_cells(S, b % gx, (b / gx) % gy, b / (gx * gy));
+    });
+}
 
 // Fluid list from the scanned flags.  Split: entry e < C is cell e (red), e >= C is cell
 // e - C (black).  fl_count = {red cells (0 unless split), all fluid cells}.  Each entry also
 // gets its stencil word (see DevSim::fl_word).
-__global__ __launch_bounds__(BLOCK) void k_fluid_compact(DevSim S) {
-    const uint32_t C = S.ncells, n = S.split_colours ? 2 * C : C, e = blockIdx.x * BLOCK + threadIdx.x;
+__device__ __forceinline__ void body_fluid_compact(const DevSim &S, uint32_t b) {
+    const uint32_t C = S.ncells, n = S.split_colours ? 2 * C : C, e = b * BLOCK + threadIdx.x;
     if (e >= n) return;
     const uint32_t f = S.fl_flag[e], o = S.fl_off[e];
     if (f) {
@@ -128,6 +150,9 @@ __global__ __launch_bounds__(BLOCK) void k_fluid_compact(DevSim S) {
 #pragma unroll
         for (int n = 0; n < 6; n++)
             if (st & (1u << n)) word |= (((uint32_t)S.sten[id + off[n]] >> 8) & 7u) << (9 + 3 * n);
+        const uint32_t ci = id % (uint32_t)S.nx, cj = (id / (uint32_t)S.nx) % (uint32_t)S.ny,
+                       ck = id / ((uint32_t)S.nx * (uint32_t)S.ny);
+        word |= ((ci + cj + ck) & 1u) << 27;
         S.fl_list[o] = id;
         S.fl_word[o] = word;
     }
@@ -136,6 +161,10 @@ __global__ __launch_bounds__(BLOCK) void k_fluid_compact(DevSim S) {
         S.fl_count[1] = o + f;
         if (!S.split_colours) S.fl_count[0] = 0;
     }
+}
+__global__ __launch_bounds__(BLOCK) void k_fluid_compact(DevSim S) { body_fluid_compact(S, blockIdx.x); }
+__global__ __launch_bounds__(BLOCK) void kb_fluid_compact(WorkList W) {
+    for_each_work(W, [&](const DevSim &S, uint32_t b) { body_fluid_compact(S, b); });
 }
 
 __device__ __forceinline__ uint8_t type_at(const DevSim &S, int i, int j, int k) {
@@ -158,14 +187,21 @@ __device__ __forceinline__ void project_face(const DevSim &S, uint32_t f, int ai
     }
 }
 
-__global__ __launch_bounds__(TX *TY) void k_project(DevSim S) {
-    const int i = blockIdx.x * TX + threadIdx.x;
-    const int j = blockIdx.y * TY + threadIdx.y;
-    const int k = blockIdx.z;
+__device__ __forceinline__ void body_project(const DevSim &S, uint32_t bx, uint32_t by, uint32_t bz) {
+    const int i = bx * TX + threadIdx.x;
+    const int j = by * TY + threadIdx.y;
+    const int k = bz;
     if (i > S.nx || j > S.ny) return;
     if (j < S.ny && k < S.nz) project_face(S, uidx(S, i, j, k), i - 1, j, k, i, j, k);
     if (i < S.nx && k < S.nz) project_face(S, vidx(S, i, j, k), i, j - 1, k, i, j, k);
     if (i < S.nx && j < S.ny) project_face(S, widx(S, i, j, k), i, j, k - 1, i, j, k);
+}
+__global__ __launch_bounds__(TX *TY) void k_project(DevSim S) { body_project(S, blockIdx.x, blockIdx.y, blockIdx.z); }
+__global__ __launch_bounds__(TX *TY) void kb_project(WorkList W) {
+    for_each_work(W, [&](const DevSim &S, uint32_t b) {
+        const uint32_t gx = node_gx(S), gy = node_gy(S);
+        body_project(S, b % gx, (b / gx) % gy, b / (gx * gy));
+    });
 }
 
 __global__ __launch_bounds__(TX *TY) void k_type_u32(DevSim S, uint32_t *out) {
@@ -196,5 +232,18 @@ void launch_acc_fields(const DevSim &S, int32_t *acc, int32_t *wacc, hipStream_t
 void launch_type_u32(const DevSim &S, uint32_t *out, hipStream_t s) {
     k_type_u32<<<cell_grid(S), dim3(TX, TY), 0, s>>>(S, out);
 }
+
+// ---- batched
+uint32_t node_blocks(const DevSim &S) { return node_gx(S) * node_gy(S) * (uint32_t)(S.nz + 1); }
+uint32_t cell_blocks(const DevSim &S) { return cell_gx(S) * cell_gy(S) * (uint32_t)S.nz; }
+uint32_t flag_count(const DevSim &S) { return S.split_colours ? 2 * S.ncells : S.ncells; }
+
+void batch_faces(const WorkLists &W, hipStream_t s) { kb_faces<<<BATCH_GRID, dim3(TX, TY), 0, s>>>(W[WD_NODES]); }
+void batch_cells(const WorkLists &W, hipStream_t s) { kb_cells<<<BATCH_GRID, dim3(TX, TY), 0, s>>>(W[WD_CELLS]); }
+void batch_fluid_list(const WorkLists &W, hipStream_t s) {
+    batch_scan(W, 1, s);
+    kb_fluid_compact<<<BATCH_GRID, BLOCK, 0, s>>>(W[WD_COMPACT]);
+}
+void batch_project(const WorkLists &W, hipStream_t s) { kb_project<<<BATCH_GRID, dim3(TX, TY), 0, s>>>(W[WD_NODES]); }
 
 }  // namespace pxl

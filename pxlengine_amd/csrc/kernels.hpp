@@ -42,7 +42,11 @@ struct MGLevel {
     uint8_t *type;   // level-l cell types (levels >= 1)
     uint16_t *sten;  // level-l stencil words (same layout as DevSim::sten; levels >= 1)
     float *r, *z;    // level-l right-hand side (the scaled restricted residual) and iterate
+    // FLUID cells of the level (levels 1..mg_tail, rebuilt each step, any order): cell index
+    // with bit 31 set for black cells; *count entries
+    uint32_t *list, *count;
 };
+constexpr uint32_t LIST_BLACK = 1u << 31;
 
 // Solver scalar slots (fp64 array): two slots of SC_COUNT by CG-iteration parity for the
 // (alpha, beta, gamma) history; the refinement norms live in slot 0.
@@ -85,7 +89,8 @@ struct DevSim {
     // {n_red (0 unless split), n_total}
     uint32_t *fl_flag, *fl_off, *fl_list, *fl_count, *fl_tmp;
     // per list entry: bits 0-5 FLUID neighbours (-x,+x,-y,+y,-z,+z), bits 6-8 the cell's
-    // non-SOLID neighbour count (matrix diagonal), bits 9+3n the diagonal of FLUID neighbour n
+    // non-SOLID neighbour count (matrix diagonal), bits 9+3n the diagonal of FLUID neighbour n,
+    // bit 27 the cell's colour (1 = black: i + j + k odd)
     uint32_t *fl_word;
     // solver
     float *cg_r0, *cg_r1, *cg_w0, *cg_w1, *cg_s0, *cg_s1, *cg_p, *e;
@@ -96,6 +101,8 @@ struct DevSim {
     float *part32;   // two slots of 2 x (workgroups at z-batch 1)
     double *part64;
     double *scal;
+    double *norms;  // where the refinement norms {rr, bb} are totalled (scal + SC_RR, or a
+                    // batch's per-sim array)
     unsigned *ctr;
     uint32_t *scan_tmp;
     // initial particles (reset)
@@ -191,7 +198,8 @@ void launch_mgcg_iter(const DevSim &S, float *x, uint32_t it, bool last, hipStre
 // FP64 refinement (solver.hip); init/update also total the norms into scal[SC_RR / SC_BB].
 void launch_mixed_init(const DevSim &S, double *x64, hipStream_t s);  // x64 = p, r64 = b - A x64
 void launch_mixed_update(const DevSim &S, const double *xsrc, double *xdst, hipStream_t s);  // xdst = xsrc + e; r64
-void launch_mixed_finish(const DevSim &S, const double *x64, hipStream_t s);  // p = float(x64)
+// (launch_mixed_update also stores pressure = float(xdst), so the pressure is current after
+// every refinement pass.)
 // bins / IO
 void launch_bins_out(const DevSim &S, const uint32_t *off_lin, uint32_t *cell_of, uint32_t *sorted, hipStream_t s);
 void launch_sort_segments(uint32_t ncells, const uint32_t *offsets, const uint32_t *counts, uint32_t *sorted,
@@ -202,5 +210,85 @@ void launch_pack_aos8(const DevSim &S, const float *aos8, hipStream_t s);    // 
 void launch_reset_particles(const DevSim &S, hipStream_t s);                 // initial -> a
 void launch_type_u32(const DevSim &S, uint32_t *out, hipStream_t s);
 void launch_acc_fields(const DevSim &S, int32_t *acc, int32_t *wacc, hipStream_t s);  // boxes -> face sums
+
+// ---- batched launches: one launch runs a kernel for many simulations.
+// A batched kernel is a persistent grid walking a device-side work list.  Entry
+// (sim << WORK_SIM_SHIFT) | block runs the kernel body for workgroup `block` of simulation
+// `sim`'s own single-sim grid, so every simulation computes exactly what it computes alone
+// (same partition, same reduction order).  `sims` is a device array of DevSim.
+constexpr uint32_t WORK_SIM_SHIFT = 20;
+constexpr uint32_t WORK_BLOCK_MASK = (1u << WORK_SIM_SHIFT) - 1u;
+constexpr uint32_t MAX_BATCH_SIMS = 1u << (32 - WORK_SIM_SHIFT);
+
+struct WorkList {
+    const DevSim *sims;
+    const uint32_t *entries;
+    const uint32_t *count;  // device word: number of entries
+};
+
+template <typename F>
+__device__ __forceinline__ void for_each_work(const WorkList &W, F &&f) {
+    const uint32_t n = *W.count;
+    for (uint32_t e = blockIdx.x; e < n; e += gridDim.x) {
+        const uint32_t w = W.entries[e];
+        f(W.sims[w >> WORK_SIM_SHIFT], w & WORK_BLOCK_MASK);
+    }
+}
+
+// Work domains: which workgroups of a simulation a batched launch covers.  The grid-phase
+// domains hold every simulation of the batch; the solve domains (WD_A_*) only those still
+// iterating their pressure solve, and per multigrid level only those whose level runs as
+// separate launches (level < mg_tail).
+enum WorkDomain : int {
+    WD_SIM = 0,     // one entry per simulation
+    WD_KEYS,        // count array (clear)
+    WD_PART,        // particles
+    WD_SCAN_KEYS,   // scan tiles of the count array
+    WD_TILES,       // tiles (active-tile list)
+    WD_TILEGRID,    // per-tile P2G / G2P workgroups
+    WD_NODES,       // face-node grid (faces, project)
+    WD_CELLS,       // cell grid (cells)
+    WD_SCAN_FLAGS,  // scan tiles of the fluid flags
+    WD_COMPACT,     // fluid flags (compaction)
+    WD_MGTYPES,     // + l, l = 1..MG_MAX-1: cells of level l
+    WD_A_SIM = WD_MGTYPES + MG_MAX,
+    WD_A_SOLVER,     // fluid-list solver workgroups
+    WD_A_LVL,        // + l: the solver workgroups again, for simulations with level l < mg_tail
+    WD_COUNT = WD_A_LVL + MG_MAX
+};
+inline bool wd_solve(int d) { return d >= WD_A_SIM; }
+
+// Workgroups of simulation S in domain d (host side; the batched kernels derive the same
+// counts from S where they need them).
+uint32_t work_blocks(const DevSim &S, int d);
+// per-file pieces of work_blocks
+uint32_t particle_blocks(const DevSim &S);
+uint32_t key_blocks(const DevSim &S);
+uint32_t tile_blocks(const DevSim &S);
+uint32_t tile_grid_blocks(const DevSim &S);
+uint32_t node_blocks(const DevSim &S);
+uint32_t cell_blocks(const DevSim &S);
+uint32_t flag_count(const DevSim &S);
+uint32_t mg_cell_blocks(const DevSim &S, int l);
+
+// Batched launchers: W[d] is the work list of domain d.
+using WorkLists = WorkList[WD_COUNT];
+void batch_clear_grid(const WorkLists &W, hipStream_t s);
+void batch_bin_count(const WorkLists &W, hipStream_t s);
+void batch_scan(const WorkLists &W, int which, hipStream_t s);  // 0: count -> offset, 1: fluid flags
+void batch_tile_list(const WorkLists &W, hipStream_t s);
+void batch_scatter(const WorkLists &W, hipStream_t s);
+void batch_p2g(const WorkLists &W, hipStream_t s);
+void batch_faces(const WorkLists &W, hipStream_t s);
+void batch_cells(const WorkLists &W, hipStream_t s);
+void batch_fluid_list(const WorkLists &W, hipStream_t s);
+void batch_mg_types(const WorkLists &W, int max_levels, hipStream_t s);
+void batch_project(const WorkLists &W, hipStream_t s);
+void batch_g2p(const WorkLists &W, hipStream_t s);
+void batch_mg_vcycle(const WorkLists &W, int max_tail, hipStream_t s);  // u = M r for the solving sims
+void batch_grid_phase(const WorkLists &W, int max_levels, hipStream_t s);    // clear .. fluid list, mg types
+void batch_mixed_init(const WorkLists &W, hipStream_t s);                    // x64a = p, r64, norms
+void batch_inner_pass(const WorkLists &W, int cur, uint32_t inner, int max_tail, hipStream_t s);
+void batch_post(const WorkLists &W, hipStream_t s);                          // project, g2p
 
 }  // namespace pxl

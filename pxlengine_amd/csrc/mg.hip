@@ -21,10 +21,14 @@
 //             right-hand side sums the red children's residuals only
 //   post:     black update from red neighbours' z + prolonged correction (the red cells'
 //             corrected values are only ever read here), then sweep(R), sweep(B), sweep(R).
-// Levels from mg_tail on (a few thousand cells together) run the same cycle in one
+// Levels below mg_tail are walked through lists of their FLUID cells (the step's fluid list
+// on level 0, per-level lists built with the coarse types), so the work follows the liquid,
+// not the grid; a launch takes the list in cg_blocks slices, like the solver kernels.
+// Coarse z is kept 0 on non-FLUID cells (the prolongation reads the parent of any FLUID
+// cell).  Levels from mg_tail on (a few thousand cells together) run the same cycle in one
 // workgroup with everything in LDS.  Arithmetic is fp32 (the FP64 refinement outside fixes
-// the rounding).  Kernels take the levels they touch as MGLevel values: a runtime index into
-// DevSim's level array would copy the whole structure into scratch memory.
+// the rounding).  Single-sim kernels take the levels they touch as MGLevel values: a runtime
+// index into DevSim's level array would copy the whole structure into scratch memory.
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -32,9 +36,13 @@ namespace pxl {
 
 namespace {
 
-constexpr int MBLOCK = 256;
+constexpr int MBLOCK = 256;         // grid-walking level kernels (types, stencils, lists)
+constexpr int LBLOCK = 1024;        // list-walking level kernels
 constexpr int TBLOCK = 1024;        // the tail workgroup
 constexpr int COARSEST_SWEEPS = 1;  // sweeps R-B then as many B-R on the coarsest level
+constexpr unsigned BATCH_GRID = 2048;      // persistent grid of batched grid-walking launches
+constexpr unsigned BATCH_LGRID = 1024;     // ... of batched list-walking launches
+constexpr unsigned BATCH_TAIL_GRID = 512;  // ... of the batched tail
 
 __device__ __forceinline__ float rcp_or0(float d) { return d > 0.0f ? __builtin_amdgcn_rcpf(d) : 0.0f; }
 __device__ __forceinline__ float st_diag(uint32_t st) { return (float)((st >> 8) & 7u); }
@@ -56,21 +64,21 @@ __device__ __forceinline__ uint8_t level_type(const MGLevel &v, int i, int j, in
 __device__ __forceinline__ uint32_t parent(int cnx, int cny, int i, int j, int k) {
     return (uint32_t)(i >> 1) + (uint32_t)cnx * ((uint32_t)(j >> 1) + (uint32_t)cny * (uint32_t)(k >> 1));
 }
-// Lane t of a colour launch (ceil(nx/2) lanes per row) -> cell (i, j, k) of that colour;
-// false past the grid.
-__device__ __forceinline__ bool colour_cell(const MGLevel &v, uint32_t t, int colour, int &i, int &j, int &k) {
-    const uint32_t nxh = (uint32_t)(v.nx + 1) >> 1;
-    const uint32_t row = t / nxh;
-    if (row >= (uint32_t)(v.ny * v.nz)) return false;
-    j = (int)(row % (uint32_t)v.ny);
-    k = (int)(row / (uint32_t)v.ny);
-    i = 2 * (int)(t % nxh) + ((j + k + colour) & 1);
-    return i < v.nx;
+// Workgroup b's (of nb) slice [beg, end) of an n-entry list.
+__device__ __forceinline__ void slice(uint32_t n, uint32_t b, uint32_t nb, uint32_t &beg, uint32_t &end) {
+    beg = (uint32_t)((uint64_t)n * b / nb);
+    end = (uint32_t)((uint64_t)n * (b + 1) / nb);
 }
 
-// ---- level types: c from its children in f
-__global__ __launch_bounds__(MBLOCK) void k_mg_types(MGLevel f, MGLevel c) {
-    const uint32_t id = blockIdx.x * MBLOCK + threadIdx.x;
+// Fields of a level-0 list entry's word (DevSim::fl_word).
+__device__ __forceinline__ float w_diag(uint32_t w) { return (float)((w >> 6) & 7u); }
+__device__ __forceinline__ float w_ndiag(uint32_t w, int n) { return (float)((w >> (9 + 3 * n)) & 7u); }
+__device__ __forceinline__ uint32_t w_black(uint32_t w) { return (w >> 27) & 1u; }
+
+// ---- level types (c from its children in f); zeroes c's list count
+__device__ __forceinline__ void body_mg_types(const MGLevel &f, const MGLevel &c, uint32_t b) {
+    const uint32_t id = b * MBLOCK + threadIdx.x;
+    if (id == 0) *c.count = 0;
     if (id >= ncell(c)) return;
     int i, j, k;
     cell_ijk(c.nx, c.ny, id, i, j, k);
@@ -83,139 +91,196 @@ __global__ __launch_bounds__(MBLOCK) void k_mg_types(MGLevel f, MGLevel c) {
     }
     c.type[id] = air ? CELL_AIR : (fluid ? CELL_FLUID : CELL_SOLID);
 }
+__global__ __launch_bounds__(MBLOCK) void k_mg_types(MGLevel f, MGLevel c) { body_mg_types(f, c, blockIdx.x); }
 
-// ---- level stencil words: FLUID bit, non-SOLID-neighbour count (bits 8-10), FLUID-neighbour
-// mask (bits 0-5), the layout of the simulation grid's stencil words
-__global__ __launch_bounds__(MBLOCK) void k_mg_sten(MGLevel v) {
-    const uint32_t id = blockIdx.x * MBLOCK + threadIdx.x;
-    if (id >= ncell(v)) return;
-    int i, j, k;
-    cell_ijk(v.nx, v.ny, id, i, j, k);
-    uint32_t st = 0;
-    if (v.type[id] == CELL_FLUID) {
-        const int nb[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k}, {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
-        uint32_t diag = 0;
+// ---- level stencil words (FLUID bit, non-SOLID-neighbour count in bits 8-10, FLUID-neighbour
+// mask in bits 0-5: the layout of the simulation grid's stencil words), z = 0, and the level's
+// FLUID-cell list (one atomic per wave)
+__device__ __forceinline__ void body_mg_sten(const MGLevel &v, uint32_t b) {
+    const uint32_t id = b * MBLOCK + threadIdx.x;
+    const bool in = id < ncell(v);
+    uint32_t st = 0, entry = 0;
+    if (in) {
+        int i, j, k;
+        cell_ijk(v.nx, v.ny, id, i, j, k);
+        if (v.type[id] == CELL_FLUID) {
+            const int nb[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k}, {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
+            uint32_t diag = 0;
 #pragma unroll
-        for (int n = 0; n < 6; n++) {
-            const uint8_t t = level_type(v, nb[n][0], nb[n][1], nb[n][2]);
-            if (t != CELL_SOLID) diag++;
-            if (t == CELL_FLUID) st |= 1u << n;
+            for (int n = 0; n < 6; n++) {
+                const uint8_t t = level_type(v, nb[n][0], nb[n][1], nb[n][2]);
+                if (t != CELL_SOLID) diag++;
+                if (t == CELL_FLUID) st |= 1u << n;
+            }
+            st |= STEN_FLUID | (diag << 8);
+            entry = id | (((i + j + k) & 1) ? LIST_BLACK : 0u);
         }
-        st |= STEN_FLUID | (diag << 8);
+        v.sten[id] = (uint16_t)st;
+        v.z[id] = 0.0f;
     }
-    v.sten[id] = (uint16_t)st;
+    const bool fl = (st & STEN_FLUID) != 0;
+    const uint64_t mask = __ballot(fl);
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int leader = mask ? __ffsll((long long)mask) - 1 : 0;
+    uint32_t base = 0;
+    if (fl && lane == leader) base = atomicAdd(v.count, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader, WAVE);
+    if (fl) v.list[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = entry;
 }
+__global__ __launch_bounds__(MBLOCK) void k_mg_sten(MGLevel v) { body_mg_sten(v, blockIdx.x); }
 
-// ---- pre0: the first red-black sweep from zero.  One lane per cell; non-FLUID cells get
-// z = 0 when `zero_rest` (coarse levels: the prolongation reads every coarse z).
-__global__ __launch_bounds__(MBLOCK) void k_rb_pre0(MGLevel v, const float *__restrict__ r, float *__restrict__ z,
-                                                    int zero_rest) {
-    const uint32_t c = blockIdx.x * MBLOCK + threadIdx.x;
-    if (c >= ncell(v)) return;
-    int i, j, k;
-    cell_ijk(v.nx, v.ny, c, i, j, k);
-    const bool black = (i + j + k) & 1;
-    const uint32_t last = ncell(v) - 1u;
-    const int32_t off[6] = {-1, 1, -v.nx, v.nx, -v.nx * v.ny, v.nx * v.ny};
-    float rn[6];
-    uint32_t sn[6];
-    if (black) {  // neighbours' r and stencil words, clamped addresses, masked after
+// ---- level 0 (the step's fluid list; r, u are the CG vectors)
+// pre0: red u = r / d, black u = (r + sum FLUID red nbrs r_n / d_n) / d
+__device__ __forceinline__ void body_l0_pre0(const DevSim &S, const float *__restrict__ r, float *__restrict__ u,
+                                             uint32_t b, uint32_t nb) {
+    uint32_t beg, end;
+    slice(S.fl_count[1], b, nb, beg, end);
+    const int32_t off[6] = {-1, 1, -S.nx, S.nx, -S.nx * S.ny, S.nx * S.ny};
+    for (uint32_t t = beg + threadIdx.x; t < end; t += LBLOCK) {
+        const uint32_t id = S.fl_list[t], wd = S.fl_word[t];
+        float sum = 0.0f;
+        if (w_black(wd)) {
 #pragma unroll
-        for (int n = 0; n < 6; n++) {
-            const uint32_t q = min(c + (uint32_t)off[n], last);
-            rn[n] = r[q];
-            sn[n] = v.sten[q];
+            for (int n = 0; n < 6; n++)
+                if ((wd >> n) & 1u) sum += r[id + off[n]] * rcp_or0(w_ndiag(wd, n));
         }
+        u[id] = (r[id] + sum) * rcp_or0(w_diag(wd));
     }
-    const float rc = r[c];
-    const uint32_t st = v.sten[c];
-    if (!(st & STEN_FLUID)) {
-        if (zero_rest) z[c] = 0.0f;
-        return;
-    }
-    float sum = 0.0f;
-    if (black) {
-#pragma unroll
-        for (int n = 0; n < 6; n++) sum += (st & (1u << n)) ? rn[n] * rcp_or0(st_diag(sn[n])) : 0.0f;
-    }
-    z[c] = (rc + sum) * rcp_or0(st_diag(st));
 }
-
-// ---- one colour's Gauss-Seidel update (one lane per cell of that colour)
-__global__ __launch_bounds__(MBLOCK) void k_rb_sweep(MGLevel v, const float *__restrict__ r, float *__restrict__ z,
-                                                     int colour) {
-    int i, j, k;
-    if (!colour_cell(v, blockIdx.x * MBLOCK + threadIdx.x, colour, i, j, k)) return;
-    const uint32_t c = lidx(v, i, j, k);
-    const uint32_t last = ncell(v) - 1u;
-    const int32_t off[6] = {-1, 1, -v.nx, v.nx, -v.nx * v.ny, v.nx * v.ny};
-    // every load issued before the stencil word is known (clamped addresses, masked after)
-    float zn[6];
+// one colour's update
+__device__ __forceinline__ void body_l0_sweep(const DevSim &S, const float *__restrict__ r, float *__restrict__ u,
+                                              uint32_t colour, uint32_t b, uint32_t nb) {
+    uint32_t beg, end;
+    slice(S.fl_count[1], b, nb, beg, end);
+    const int32_t off[6] = {-1, 1, -S.nx, S.nx, -S.nx * S.ny, S.nx * S.ny};
+    for (uint32_t t = beg + threadIdx.x; t < end; t += LBLOCK) {
+        const uint32_t wd = S.fl_word[t];
+        if (w_black(wd) != colour) continue;
+        const uint32_t id = S.fl_list[t];
+        float sum = 0.0f;
 #pragma unroll
-    for (int n = 0; n < 6; n++) zn[n] = z[min(c + (uint32_t)off[n], last)];
-    const float rc = r[c];
-    const uint32_t st = v.sten[c];
-    if (!(st & STEN_FLUID)) return;
-    float sum = 0.0f;
-#pragma unroll
-    for (int n = 0; n < 6; n++) sum += (st & (1u << n)) ? zn[n] : 0.0f;
-    z[c] = (rc + sum) * rcp_or0(st_diag(st));
+        for (int n = 0; n < 6; n++)
+            if ((wd >> n) & 1u) sum += u[id + off[n]];
+        u[id] = (r[id] + sum) * rcp_or0(w_diag(wd));
+    }
 }
-
-// ---- restrict after a black sweep: c.r = 1/2 sum over FLUID red children of r - L z (one lane
-// per coarse cell; the red children are q = 0, 3, 5, 6).  Loads use clamped addresses and
-// masks so that the loads of a lane do not wait on each other.
-__global__ __launch_bounds__(MBLOCK) void k_rb_restrict(MGLevel f, const float *__restrict__ r,
-                                                        const float *__restrict__ z, MGLevel c) {
-    const uint32_t cc = blockIdx.x * MBLOCK + threadIdx.x;
-    if (cc >= ncell(c)) return;
-    int ci, cj, ck;
-    cell_ijk(c.nx, c.ny, cc, ci, cj, ck);
-    const uint32_t nf = ncell(f);
-    const int32_t off[6] = {-1, 1, -f.nx, f.nx, -f.nx * f.ny, f.nx * f.ny};
-    const int red[4] = {0, 3, 5, 6};
-    float acc = 0.0f;
-#pragma unroll
-    for (int m = 0; m < 4; m++) {
-        const int q = red[m];
-        const int fi = 2 * ci + (q & 1), fj = 2 * cj + ((q >> 1) & 1), fk = 2 * ck + (q >> 2);
-        const bool in = fi < f.nx && fj < f.ny && fk < f.nz;
-        const uint32_t id = in ? lidx(f, fi, fj, fk) : 0u;
-        const uint32_t st = in ? (uint32_t)f.sten[id] : 0u;
+// post: black update from red neighbours' u + the prolonged correction c.z
+__device__ __forceinline__ void body_l0_post(const DevSim &S, const MGLevel &c, const float *__restrict__ r,
+                                             float *__restrict__ u, uint32_t b, uint32_t nb) {
+    uint32_t beg, end;
+    slice(S.fl_count[1], b, nb, beg, end);
+    for (uint32_t t = beg + threadIdx.x; t < end; t += LBLOCK) {
+        const uint32_t wd = S.fl_word[t];
+        if (!w_black(wd)) continue;
+        const uint32_t id = S.fl_list[t];
+        int i, j, k;
+        cell_ijk(S.nx, S.ny, id, i, j, k);
+        const int nbc[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k}, {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
         float sum = 0.0f;
 #pragma unroll
         for (int n = 0; n < 6; n++) {
-            const float zn = z[min(id + (uint32_t)off[n], nf - 1u)];
-            sum += (st & (1u << n)) ? zn : 0.0f;
+            if ((wd >> n) & 1u) {
+                const int a = nbc[n][0], bb = nbc[n][1], d = nbc[n][2];
+                sum += u[(uint32_t)a + (uint32_t)S.nx * ((uint32_t)bb + (uint32_t)S.ny * (uint32_t)d)] +
+                       c.z[parent(c.nx, c.ny, a, bb, d)];
+            }
         }
-        const float res = r[id] - st_diag(st) * z[id] + sum;
-        acc += (st & STEN_FLUID) ? res : 0.0f;
+        u[id] = (r[id] + sum) * rcp_or0(w_diag(wd));
     }
-    c.r[cc] = 0.5f * acc;
 }
 
-// ---- post: black update from red neighbours' z + prolonged coarse correction c.z
-__global__ __launch_bounds__(MBLOCK) void k_rb_post(MGLevel f, const float *__restrict__ r, float *__restrict__ z,
-                                                    MGLevel c) {
-    int i, j, k;
-    if (!colour_cell(f, blockIdx.x * MBLOCK + threadIdx.x, 1, i, j, k)) return;
-    const uint32_t id = lidx(f, i, j, k);
-    // neighbour coordinates clamped into the grid: every load is issued before the stencil
-    // word is known and masked after
-    const int im = max(i - 1, 0), ip = min(i + 1, f.nx - 1), jm = max(j - 1, 0), jp = min(j + 1, f.ny - 1),
-              km = max(k - 1, 0), kp = min(k + 1, f.nz - 1);
-    const int nb[6][3] = {{im, j, k}, {ip, j, k}, {i, jm, k}, {i, jp, k}, {i, j, km}, {i, j, kp}};
-    float zn[6];
+// ---- coarse levels 1..mg_tail-1 (the level's list)
+__device__ __forceinline__ void body_lc_pre0(const MGLevel &v, uint32_t b, uint32_t nb) {
+    uint32_t beg, end;
+    slice(*v.count, b, nb, beg, end);
+    const int32_t off[6] = {-1, 1, -v.nx, v.nx, -v.nx * v.ny, v.nx * v.ny};
+    for (uint32_t t = beg + threadIdx.x; t < end; t += LBLOCK) {
+        const uint32_t e = v.list[t], id = e & ~LIST_BLACK;
+        const uint32_t st = v.sten[id];
+        float sum = 0.0f;
+        if (e & LIST_BLACK) {
 #pragma unroll
-    for (int n = 0; n < 6; n++)
-        zn[n] = z[lidx(f, nb[n][0], nb[n][1], nb[n][2])] + c.z[parent(c.nx, c.ny, nb[n][0], nb[n][1], nb[n][2])];
-    const float rc = r[id];
-    const uint32_t st = f.sten[id];
-    if (!(st & STEN_FLUID)) return;
-    float sum = 0.0f;
+            for (int n = 0; n < 6; n++) {
+                if (st & (1u << n)) {
+                    const uint32_t q = id + off[n];
+                    sum += v.r[q] * rcp_or0(st_diag(v.sten[q]));
+                }
+            }
+        }
+        v.z[id] = (v.r[id] + sum) * rcp_or0(st_diag(st));
+    }
+}
+__device__ __forceinline__ void body_lc_sweep(const MGLevel &v, uint32_t colour, uint32_t b, uint32_t nb) {
+    uint32_t beg, end;
+    slice(*v.count, b, nb, beg, end);
+    const int32_t off[6] = {-1, 1, -v.nx, v.nx, -v.nx * v.ny, v.nx * v.ny};
+    const uint32_t want = colour ? LIST_BLACK : 0u;
+    for (uint32_t t = beg + threadIdx.x; t < end; t += LBLOCK) {
+        const uint32_t e = v.list[t];
+        if ((e & LIST_BLACK) != want) continue;
+        const uint32_t id = e & ~LIST_BLACK;
+        const uint32_t st = v.sten[id];
+        float sum = 0.0f;
 #pragma unroll
-    for (int n = 0; n < 6; n++) sum += (st & (1u << n)) ? zn[n] : 0.0f;
-    z[id] = (rc + sum) * rcp_or0(st_diag(st));
+        for (int n = 0; n < 6; n++)
+            if (st & (1u << n)) sum += v.z[id + off[n]];
+        v.z[id] = (v.r[id] + sum) * rcp_or0(st_diag(st));
+    }
+}
+__device__ __forceinline__ void body_lc_post(const MGLevel &v, const MGLevel &c, uint32_t b, uint32_t nb) {
+    uint32_t beg, end;
+    slice(*v.count, b, nb, beg, end);
+    for (uint32_t t = beg + threadIdx.x; t < end; t += LBLOCK) {
+        const uint32_t e = v.list[t];
+        if (!(e & LIST_BLACK)) continue;
+        const uint32_t id = e & ~LIST_BLACK;
+        const uint32_t st = v.sten[id];
+        int i, j, k;
+        cell_ijk(v.nx, v.ny, id, i, j, k);
+        const int nbc[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k}, {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
+        float sum = 0.0f;
+#pragma unroll
+        for (int n = 0; n < 6; n++) {
+            if (st & (1u << n)) {
+                const int a = nbc[n][0], bb = nbc[n][1], d = nbc[n][2];
+                sum += v.z[lidx(v, a, bb, d)] + c.z[parent(c.nx, c.ny, a, bb, d)];
+            }
+        }
+        v.z[id] = (v.r[id] + sum) * rcp_or0(st_diag(st));
+    }
+}
+
+// ---- restrict level f -> c after f's black sweep: for each FLUID cell of c (its list),
+// c.r = 1/2 sum over FLUID red children (q = 0, 3, 5, 6) of r - L z.  f's arrays are given
+// explicitly (level 0: the CG vectors and the grid's stencil words).
+__device__ __forceinline__ void body_restrict(int fnx, int fny, int fnz, const uint16_t *__restrict__ fsten,
+                                              const float *__restrict__ fr, const float *__restrict__ fz,
+                                              const MGLevel &c, uint32_t b, uint32_t nb) {
+    uint32_t beg, end;
+    slice(*c.count, b, nb, beg, end);
+    const int32_t off[6] = {-1, 1, -fnx, fnx, -fnx * fny, fnx * fny};
+    for (uint32_t t = beg + threadIdx.x; t < end; t += LBLOCK) {
+        const uint32_t cc = c.list[t] & ~LIST_BLACK;
+        int ci, cj, ck;
+        cell_ijk(c.nx, c.ny, cc, ci, cj, ck);
+        const int red[4] = {0, 3, 5, 6};
+        float acc = 0.0f;
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            const int q = red[m];
+            const int fi = 2 * ci + (q & 1), fj = 2 * cj + ((q >> 1) & 1), fk = 2 * ck + (q >> 2);
+            if (fi >= fnx || fj >= fny || fk >= fnz) continue;
+            const uint32_t id = (uint32_t)fi + (uint32_t)fnx * ((uint32_t)fj + (uint32_t)fny * (uint32_t)fk);
+            const uint32_t st = fsten[id];
+            if (!(st & STEN_FLUID)) continue;
+            float sum = 0.0f;
+#pragma unroll
+            for (int n = 0; n < 6; n++)
+                if (st & (1u << n)) sum += fz[id + off[n]];
+            acc += fr[id] - st_diag(st) * fz[id] + sum;
+        }
+        c.r[cc] = 0.5f * acc;
+    }
 }
 
 // ---- the tail: levels mg_tail..L in one workgroup.  LDS holds every tail level's z, r and
@@ -295,7 +360,7 @@ __device__ __forceinline__ void t_post(const TailLevel &f, const TailLevel &c, f
     __syncthreads();
 }
 
-__global__ __launch_bounds__(TBLOCK) void k_mg_tail(DevSim S) {
+__device__ __forceinline__ void body_mg_tail(const DevSim &S) {
     __shared__ float lz[MG_TAIL_CELLS], lr[MG_TAIL_CELLS];
     __shared__ uint16_t ls[MG_TAIL_CELLS];
     __shared__ TailLevel tl[MG_MAX];
@@ -356,14 +421,32 @@ __global__ __launch_bounds__(TBLOCK) void k_mg_tail(DevSim S) {
             for (int c = threadIdx.x; c < n; c += TBLOCK) m.z[c] = lz[c];
         }
     }
+    __syncthreads();  // the LDS is reused by the next simulation of a batched launch
+}
+__global__ __launch_bounds__(TBLOCK) void k_mg_tail(DevSim S) { body_mg_tail(S); }
+
+// ---- single-sim launches (grid = cg_blocks workgroups for the list kernels)
+__global__ __launch_bounds__(LBLOCK) void k_l0_pre0(DevSim S) { body_l0_pre0(S, S.cg_r0, S.cg_r1, blockIdx.x, gridDim.x); }
+__global__ __launch_bounds__(LBLOCK) void k_l0_sweep(DevSim S, uint32_t colour) {
+    body_l0_sweep(S, S.cg_r0, S.cg_r1, colour, blockIdx.x, gridDim.x);
+}
+__global__ __launch_bounds__(LBLOCK) void k_l0_post(DevSim S, MGLevel c) {
+    body_l0_post(S, c, S.cg_r0, S.cg_r1, blockIdx.x, gridDim.x);
+}
+__global__ __launch_bounds__(LBLOCK) void k_lc_pre0(MGLevel v) { body_lc_pre0(v, blockIdx.x, gridDim.x); }
+__global__ __launch_bounds__(LBLOCK) void k_lc_sweep(MGLevel v, uint32_t colour) {
+    body_lc_sweep(v, colour, blockIdx.x, gridDim.x);
+}
+__global__ __launch_bounds__(LBLOCK) void k_lc_post(MGLevel v, MGLevel c) { body_lc_post(v, c, blockIdx.x, gridDim.x); }
+__global__ __launch_bounds__(LBLOCK) void k_restrict(MGLevel f, const float *fr, const float *fz, MGLevel c) {
+    body_restrict(f.nx, f.ny, f.nz, f.sten, fr, fz, c, blockIdx.x, gridDim.x);
 }
 
 inline unsigned grid_for(uint32_t n) { return (n + MBLOCK - 1) / MBLOCK; }
 inline uint32_t cells(const MGLevel &m) { return (uint32_t)(m.nx * m.ny * m.nz); }
-inline uint32_t colour_cells(const MGLevel &m) { return (uint32_t)((m.nx + 1) / 2) * (uint32_t)(m.ny * m.nz); }
 
 // The simulation grid as a level (its types and stencil words).
-inline MGLevel level0(const DevSim &S) {
+__host__ __device__ inline MGLevel level0(const DevSim &S) {
     MGLevel v{};
     v.nx = S.nx;
     v.ny = S.ny;
@@ -371,6 +454,65 @@ inline MGLevel level0(const DevSim &S) {
     v.type = S.type;
     v.sten = S.sten;
     return v;
+}
+
+// ---- batched: level l of each simulation in the work list
+__global__ __launch_bounds__(MBLOCK) void kb_mg_types(WorkList W, int l) {
+    for_each_work(W, [&](const DevSim &S, uint32_t b) {
+        const MGLevel f = l == 1 ? level0(S) : S.mg[l - 1], c = S.mg[l];
+        body_mg_types(f, c, b);
+    });
+}
+__global__ __launch_bounds__(MBLOCK) void kb_mg_sten(WorkList W, int l) {
+    for_each_work(W, [&](const DevSim &S, uint32_t b) {
+        const MGLevel v = S.mg[l];
+        body_mg_sten(v, b);
+    });
+}
+__global__ __launch_bounds__(LBLOCK) void kb_pre0(WorkList W, int l) {
+    for_each_work(W, [&](const DevSim &S, uint32_t b) {
+        if (l == 0) {
+            body_l0_pre0(S, S.cg_r0, S.cg_r1, b, S.cg_blocks);
+        } else {
+            const MGLevel v = S.mg[l];
+            body_lc_pre0(v, b, S.cg_blocks);
+        }
+    });
+}
+__global__ __launch_bounds__(LBLOCK) void kb_sweep(WorkList W, int l, uint32_t colour) {
+    for_each_work(W, [&](const DevSim &S, uint32_t b) {
+        if (l == 0) {
+            body_l0_sweep(S, S.cg_r0, S.cg_r1, colour, b, S.cg_blocks);
+        } else {
+            const MGLevel v = S.mg[l];
+            body_lc_sweep(v, colour, b, S.cg_blocks);
+        }
+    });
+}
+__global__ __launch_bounds__(LBLOCK) void kb_restrict(WorkList W, int l) {
+    for_each_work(W, [&](const DevSim &S, uint32_t b) {
+        const MGLevel c = S.mg[l + 1];
+        if (l == 0) {
+            body_restrict(S.nx, S.ny, S.nz, S.sten, S.cg_r0, S.cg_r1, c, b, S.cg_blocks);
+        } else {
+            const MGLevel f = S.mg[l];
+            body_restrict(f.nx, f.ny, f.nz, f.sten, f.r, f.z, c, b, S.cg_blocks);
+        }
+    });
+}
+__global__ __launch_bounds__(LBLOCK) void kb_post(WorkList W, int l) {
+    for_each_work(W, [&](const DevSim &S, uint32_t b) {
+        const MGLevel c = S.mg[l + 1];
+        if (l == 0) {
+            body_l0_post(S, c, S.cg_r0, S.cg_r1, b, S.cg_blocks);
+        } else {
+            const MGLevel v = S.mg[l];
+            body_lc_post(v, c, b, S.cg_blocks);
+        }
+    });
+}
+__global__ __launch_bounds__(TBLOCK) void kb_mg_tail(WorkList W) {
+    for_each_work(W, [&](const DevSim &S, uint32_t) { body_mg_tail(S); });
 }
 
 }  // namespace
@@ -383,27 +525,68 @@ void launch_mg_types(const DevSim &S, hipStream_t s) {
     }
 }
 
+// u = M r with r = cg_r0, u = cg_r1 (the vectors of the MG-CG iteration).
 void launch_mg_vcycle(const DevSim &S, const float *r, float *u, hipStream_t s) {
+    (void)r;
+    (void)u;
     const int T = S.mg_tail;
-    auto lev = [&](int l) { return l == 0 ? level0(S) : S.mg[l]; };
-    auto rhs = [&](int l) { return l == 0 ? r : (const float *)S.mg[l].r; };
-    auto itr = [&](int l) { return l == 0 ? u : S.mg[l].z; };
+    const unsigned g = (unsigned)S.cg_blocks;
     for (int l = 0; l < T; l++) {
-        const MGLevel f = lev(l);
-        const unsigned gc = grid_for(colour_cells(f));
-        k_rb_pre0<<<grid_for(cells(f)), MBLOCK, 0, s>>>(f, rhs(l), itr(l), l > 0);
-        k_rb_sweep<<<gc, MBLOCK, 0, s>>>(f, rhs(l), itr(l), 0);
-        k_rb_sweep<<<gc, MBLOCK, 0, s>>>(f, rhs(l), itr(l), 1);
-        k_rb_restrict<<<grid_for(cells(S.mg[l + 1])), MBLOCK, 0, s>>>(f, rhs(l), itr(l), S.mg[l + 1]);
+        if (l == 0) {
+            k_l0_pre0<<<g, LBLOCK, 0, s>>>(S);
+            k_l0_sweep<<<g, LBLOCK, 0, s>>>(S, 0);
+            k_l0_sweep<<<g, LBLOCK, 0, s>>>(S, 1);
+            k_restrict<<<g, LBLOCK, 0, s>>>(level0(S), S.cg_r0, S.cg_r1, S.mg[1]);
+        } else {
+            k_lc_pre0<<<g, LBLOCK, 0, s>>>(S.mg[l]);
+            k_lc_sweep<<<g, LBLOCK, 0, s>>>(S.mg[l], 0);
+            k_lc_sweep<<<g, LBLOCK, 0, s>>>(S.mg[l], 1);
+            k_restrict<<<g, LBLOCK, 0, s>>>(S.mg[l], S.mg[l].r, S.mg[l].z, S.mg[l + 1]);
+        }
     }
     k_mg_tail<<<1, TBLOCK, 0, s>>>(S);
     for (int l = T - 1; l >= 0; l--) {
-        const MGLevel f = lev(l);
-        const unsigned gc = grid_for(colour_cells(f));
-        k_rb_post<<<gc, MBLOCK, 0, s>>>(f, rhs(l), itr(l), S.mg[l + 1]);
-        k_rb_sweep<<<gc, MBLOCK, 0, s>>>(f, rhs(l), itr(l), 0);
-        k_rb_sweep<<<gc, MBLOCK, 0, s>>>(f, rhs(l), itr(l), 1);
-        k_rb_sweep<<<gc, MBLOCK, 0, s>>>(f, rhs(l), itr(l), 0);
+        if (l == 0) {
+            k_l0_post<<<g, LBLOCK, 0, s>>>(S, S.mg[1]);
+            k_l0_sweep<<<g, LBLOCK, 0, s>>>(S, 0);
+            k_l0_sweep<<<g, LBLOCK, 0, s>>>(S, 1);
+            k_l0_sweep<<<g, LBLOCK, 0, s>>>(S, 0);
+        } else {
+            k_lc_post<<<g, LBLOCK, 0, s>>>(S.mg[l], S.mg[l + 1]);
+            k_lc_sweep<<<g, LBLOCK, 0, s>>>(S.mg[l], 0);
+            k_lc_sweep<<<g, LBLOCK, 0, s>>>(S.mg[l], 1);
+            k_lc_sweep<<<g, LBLOCK, 0, s>>>(S.mg[l], 0);
+        }
+    }
+}
+
+// ---- batched
+uint32_t mg_cell_blocks(const DevSim &S, int l) { return grid_for(cells(l == 0 ? level0(S) : S.mg[l])); }
+
+void batch_mg_types(const WorkLists &W, int max_levels, hipStream_t s) {
+    for (int l = 1; l < max_levels; l++) {
+        kb_mg_types<<<BATCH_GRID, MBLOCK, 0, s>>>(W[WD_MGTYPES + l], l);
+        kb_mg_sten<<<BATCH_GRID, MBLOCK, 0, s>>>(W[WD_MGTYPES + l], l);
+    }
+}
+
+// The V-cycle of every solving simulation: the level-l lists hold the simulations whose
+// level l runs as separate launches (l < mg_tail); the tails run one workgroup each.
+void batch_mg_vcycle(const WorkLists &W, int max_tail, hipStream_t s) {
+    for (int l = 0; l < max_tail; l++) {
+        const WorkList &w = W[WD_A_LVL + l];
+        kb_pre0<<<BATCH_LGRID, LBLOCK, 0, s>>>(w, l);
+        kb_sweep<<<BATCH_LGRID, LBLOCK, 0, s>>>(w, l, 0);
+        kb_sweep<<<BATCH_LGRID, LBLOCK, 0, s>>>(w, l, 1);
+        kb_restrict<<<BATCH_LGRID, LBLOCK, 0, s>>>(w, l);
+    }
+    kb_mg_tail<<<BATCH_TAIL_GRID, TBLOCK, 0, s>>>(W[WD_A_SIM]);
+    for (int l = max_tail - 1; l >= 0; l--) {
+        const WorkList &w = W[WD_A_LVL + l];
+        kb_post<<<BATCH_LGRID, LBLOCK, 0, s>>>(w, l);
+        kb_sweep<<<BATCH_LGRID, LBLOCK, 0, s>>>(w, l, 0);
+        kb_sweep<<<BATCH_LGRID, LBLOCK, 0, s>>>(w, l, 1);
+        kb_sweep<<<BATCH_LGRID, LBLOCK, 0, s>>>(w, l, 0);
     }
 }
 

@@ -18,6 +18,8 @@ constexpr int TILE_BLOCK = 1024;  // lanes of the per-tile P2G / G2P workgroups
 constexpr int SCAN_TILE = BLOCK * SCAN_ITEMS;  // 4096 cells per scan workgroup
 
 inline unsigned blocks_for(uint64_t n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
+constexpr unsigned BATCH_GRID = 2048;       // persistent grid of batched BLOCK-lane kernels
+constexpr unsigned BATCH_TILE_GRID = 1024;  // ... of batched per-tile kernels
 
 // Cell of a position: clamp(floor((x - origin) * inv_h), 0, n - 1) (fluid.slang:113).
 __device__ __forceinline__ void cell_of_pos(const DevSim &S, float x, float y, float z, int &cx, int &cy, int &cz) {
@@ -38,18 +40,22 @@ __global__ __launch_bounds__(BLOCK) void k_fill_u32(uint32_t *p, uint32_t n, uin
 
 // clear_grid (fluid.slang:88-91).  The P2G sums live in per-tile boxes that P2G overwrites
 // whole, so only the cell counts need clearing.
-__global__ __launch_bounds__(BLOCK) void k_clear_grid(DevSim S) {
-    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+__device__ __forceinline__ void body_clear_grid(const DevSim &S, uint32_t b) {
+    uint32_t i = b * BLOCK + threadIdx.x;
     if (i < S.nkeys) S.count[i] = 0;
     if (i == 0) *S.tile_count = 0;
+}
+__global__ __launch_bounds__(BLOCK) void k_clear_grid(DevSim S) { body_clear_grid(S, blockIdx.x); }
+__global__ __launch_bounds__(BLOCK) void kb_clear_grid(WorkList W) {
+    for_each_work(W, [&](const DevSim &S, uint32_t b) { body_clear_grid(S, b); });
 }
 
 // Counting-sort pass 1: tiled cell key of each particle and its arrival slot in that cell
 // (the per-cell count of p2g, fluid.slang:113-114).  Particles arrive in last step's cell
 // order, so neighbouring lanes mostly share a key: each run of equal keys in a wave64 takes
 // its slots with ONE atomic by the run's first lane.
-__global__ __launch_bounds__(BLOCK) void k_bin_count(DevSim S) {
-    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+__device__ __forceinline__ void body_bin_count(const DevSim &S, uint32_t b) {
+    const uint32_t i = b * BLOCK + threadIdx.x;
     const bool valid = i < S.np;
     uint32_t key = 0xFFFFFFFFu;
     if (valid) {
@@ -73,11 +79,15 @@ __global__ __launch_bounds__(BLOCK) void k_bin_count(DevSim S) {
         S.slot[i] = base + (uint32_t)(lane - leader);
     }
 }
+__global__ __launch_bounds__(BLOCK) void k_bin_count(DevSim S) { body_bin_count(S, blockIdx.x); }
+__global__ __launch_bounds__(BLOCK) void kb_bin_count(WorkList W) {
+    for_each_work(W, [&](const DevSim &S, uint32_t b) { body_bin_count(S, b); });
+}
 
 // Counting-sort pass 2: exclusive scan of the counts (reduce / scan of chunk sums / down-sweep).
-__global__ __launch_bounds__(BLOCK) void k_scan_reduce(const uint32_t *in, uint32_t n, uint32_t *tile_sum) {
+__device__ __forceinline__ void body_scan_reduce(const uint32_t *in, uint32_t n, uint32_t *tile_sum, uint32_t b) {
     __shared__ uint32_t sh[4];
-    const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    const uint32_t base = b * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
     uint32_t s = 0;
     if (base + SCAN_ITEMS <= n) {
         const uint4 *p = reinterpret_cast<const uint4 *>(in + base);
@@ -91,11 +101,11 @@ __global__ __launch_bounds__(BLOCK) void k_scan_reduce(const uint32_t *in, uint3
             if (base + m < n) s += in[base + m];
     }
     s = block_sum(s, sh);
-    if (threadIdx.x == 0) tile_sum[blockIdx.x] = s;
+    if (threadIdx.x == 0) tile_sum[b] = s;
 }
 
 // Single workgroup: exclusive scan of the tile sums in place (any count, in chunks).
-__global__ __launch_bounds__(BLOCK) void k_scan_tiles(uint32_t *tile_sum, uint32_t ntiles) {
+__device__ __forceinline__ void body_scan_tiles(uint32_t *tile_sum, uint32_t ntiles) {
     __shared__ uint32_t sh[4];
     uint32_t carry = 0;
     for (uint32_t base = 0; base < ntiles; base += BLOCK * 4) {
@@ -116,10 +126,10 @@ __global__ __launch_bounds__(BLOCK) void k_scan_tiles(uint32_t *tile_sum, uint32
     }
 }
 
-__global__ __launch_bounds__(BLOCK) void k_scan_down(const uint32_t *in, uint32_t n, const uint32_t *tile_off,
-                                                     uint32_t *out) {
+__device__ __forceinline__ void body_scan_down(const uint32_t *in, uint32_t n, const uint32_t *tile_off,
+                                               uint32_t *out, uint32_t b) {
     __shared__ uint32_t sh[4];
-    const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    const uint32_t base = b * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
     uint32_t v[SCAN_ITEMS];
     uint32_t t = 0;
     const bool full = base + SCAN_ITEMS <= n;
@@ -137,7 +147,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan_down(const uint32_t *in, uint32_
 #pragma unroll
     for (int m = 0; m < SCAN_ITEMS; m++) t += v[m];
     uint32_t total;
-    uint32_t run = block_exclusive_scan(t, &total, sh) + tile_off[blockIdx.x];
+    uint32_t run = block_exclusive_scan(t, &total, sh) + tile_off[b];
     if (full) {
         uint4 *p = reinterpret_cast<uint4 *>(out + base);
 #pragma unroll
@@ -157,9 +167,49 @@ __global__ __launch_bounds__(BLOCK) void k_scan_down(const uint32_t *in, uint32_
     }
 }
 
+__global__ __launch_bounds__(BLOCK) void k_scan_reduce(const uint32_t *in, uint32_t n, uint32_t *tile_sum) {
+    body_scan_reduce(in, n, tile_sum, blockIdx.x);
+}
+__global__ __launch_bounds__(BLOCK) void k_scan_tiles(uint32_t *tile_sum, uint32_t ntiles) {
+    body_scan_tiles(tile_sum, ntiles);
+}
+__global__ __launch_bounds__(BLOCK) void k_scan_down(const uint32_t *in, uint32_t n, const uint32_t *tile_off,
+                                                     uint32_t *out) {
+    body_scan_down(in, n, tile_off, out, blockIdx.x);
+}
+
+// The two scans of a step: count -> offset (which = 0) and fluid flags -> their offsets (1).
+struct ScanIO {
+    const uint32_t *in;
+    uint32_t n;
+    uint32_t *tmp, *out;
+};
+__device__ __forceinline__ ScanIO scan_io(const DevSim &S, int which) {
+    if (which == 0) return ScanIO{S.count, S.nkeys, S.scan_tmp, S.offset};
+    return ScanIO{S.fl_flag, S.split_colours ? 2 * S.ncells : S.ncells, S.fl_tmp, S.fl_off};
+}
+__global__ __launch_bounds__(BLOCK) void kb_scan_reduce(WorkList W, int which) {
+    for_each_work(W, [&](const DevSim &S, uint32_t b) {
+        const ScanIO io = scan_io(S, which);
+        body_scan_reduce(io.in, io.n, io.tmp, b);
+    });
+}
+__global__ __launch_bounds__(BLOCK) void kb_scan_tiles(WorkList W, int which) {
+    for_each_work(W, [&](const DevSim &S, uint32_t) {
+        const ScanIO io = scan_io(S, which);
+        body_scan_tiles(io.tmp, (io.n + SCAN_TILE - 1) / SCAN_TILE);
+    });
+}
+__global__ __launch_bounds__(BLOCK) void kb_scan_down(WorkList W, int which) {
+    for_each_work(W, [&](const DevSim &S, uint32_t b) {
+        const ScanIO io = scan_io(S, which);
+        body_scan_down(io.in, io.n, io.tmp, io.out, b);
+    });
+}
+
 // Counting-sort pass 3: move each particle to offset[cell] + slot (current -> scratch).
-__global__ __launch_bounds__(BLOCK) void k_scatter(DevSim S) {
-    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+__device__ __forceinline__ void body_scatter(const DevSim &S, uint32_t b) {
+    uint32_t i = b * BLOCK + threadIdx.x;
     if (i >= S.np) return;
     uint32_t j = S.offset[S.key[i]] + S.slot[i];
     S.bx[j] = S.ax[i];
@@ -169,6 +219,10 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(DevSim S) {
     S.bvy[j] = S.avy[i];
     S.bvz[j] = S.avz[i];
     S.bid[j] = S.aid[i];
+}
+__global__ __launch_bounds__(BLOCK) void k_scatter(DevSim S) { body_scatter(S, blockIdx.x); }
+__global__ __launch_bounds__(BLOCK) void kb_scatter(WorkList W) {
+    for_each_work(W, [&](const DevSim &S, uint32_t b) { body_scatter(S, b); });
 }
 
 // ---------------------------------------------------------------- P2G / G2P
@@ -198,11 +252,19 @@ __device__ __forceinline__ int box_corner(int c, int k) {
 
 // Active tiles (tiles holding particles) for the per-tile kernels; list order is arbitrary
 // (the per-tile results do not depend on it).
-__global__ __launch_bounds__(BLOCK) void k_tile_list(DevSim S) {
-    const uint32_t t = blockIdx.x * BLOCK + threadIdx.x;
+__device__ __forceinline__ void body_tile_list(const DevSim &S, uint32_t b) {
+    const uint32_t t = b * BLOCK + threadIdx.x;
     if (t >= S.ntiles) return;
     if (S.offset[t * TILE_CELLS] != S.offset[(t + 1) * TILE_CELLS]) S.tile_list[atomicAdd(S.tile_count, 1u)] = t;
 }
+__global__ __launch_bounds__(BLOCK) void k_tile_list(DevSim S) { body_tile_list(S, blockIdx.x); }
+__global__ __launch_bounds__(BLOCK) void kb_tile_list(WorkList W) {
+    for_each_work(W, [&](const DevSim &S, uint32_t b) { body_tile_list(S, b); });
+}
+
+// Workgroups of the per-tile kernels: enough to keep every CU busy, each walking the active
+// tile list with a stride of this count.
+__host__ __device__ inline uint32_t tile_grid(const DevSim &S) { return S.ntiles < 1024u ? S.ntiles : 1024u; }
 
 // p2g (fluid.slang:94-112), one active 8^3 tile at a time per workgroup.  The tile's
 // particles are one contiguous run of the cell-sorted scratch copy; their fixed-point splats
@@ -213,14 +275,14 @@ __global__ __launch_bounds__(BLOCK) void k_tile_list(DevSim S) {
 // in-grid corner of a particle binned in the tile lies in the box (the box spans the cells
 // [8t-1, 8t+8] around the tile; positions beyond the grid are binned to its edge cells, whose
 // corners beyond the grid are skipped).
-__global__ __launch_bounds__(TILE_BLOCK) void k_p2g_tile(DevSim S) {
+__device__ __forceinline__ void body_p2g(const DevSim &S, uint32_t b0, uint32_t nb) {
     __shared__ unsigned long long lbox[BOX_ENTRIES];
     const uint32_t n_active = *S.tile_count;
     // lanes of a wave take every 8th particle of a run, so concurrent LDS atomics land in
     // different cells
     const uint32_t local = (threadIdx.x & (TILE_BLOCK / 8 - 1)) * 8u + (threadIdx.x / (TILE_BLOCK / 8));
     const float o[3] = {S.ox, S.oy, S.oz};
-    for (uint32_t u = blockIdx.x; u < n_active; u += gridDim.x) {
+    for (uint32_t u = b0; u < n_active; u += nb) {
         const uint32_t t = S.tile_list[u];
         const uint32_t beg = S.offset[t * TILE_CELLS], end = S.offset[(t + 1) * TILE_CELLS];
         int ti, tj, tk;
@@ -274,6 +336,10 @@ __global__ __launch_bounds__(TILE_BLOCK) void k_p2g_tile(DevSim S) {
         for (int q = threadIdx.x; q < BOX_ENTRIES; q += TILE_BLOCK) out[q] = (int64_t)lbox[q];
         __syncthreads();
     }
+}
+__global__ __launch_bounds__(TILE_BLOCK) void k_p2g_tile(DevSim S) { body_p2g(S, blockIdx.x, gridDim.x); }
+__global__ __launch_bounds__(TILE_BLOCK) void kb_p2g_tile(WorkList W) {
+    for_each_work(W, [&](const DevSim &S, uint32_t b) { body_p2g(S, b, tile_grid(S)); });
 }
 
 // Trilinear stencil of sample_face (fluid.slang:67-80) for component c at one position:
@@ -355,11 +421,11 @@ __device__ __forceinline__ float sample_global(const DevSim &S, int c, const Ste
 // falls back to HBM when it leaves the box) and the wall clamp with the wall-normal velocity
 // clipped (Slang max/min = SPIR-V FMax/FMin).  Reads the sorted scratch copy, writes the
 // current particle arrays at the same (sorted) index.
-__global__ __launch_bounds__(TILE_BLOCK) void k_g2p_tile(DevSim S) {
+__device__ __forceinline__ void body_g2p(const DevSim &S, uint32_t b0, uint32_t nb) {
     __shared__ float lvel[BOX_ENTRIES];
     __shared__ float lold[BOX_ENTRIES];
     const uint32_t n_active = *S.tile_count;
-    for (uint32_t u = blockIdx.x; u < n_active; u += gridDim.x) {
+    for (uint32_t u = b0; u < n_active; u += nb) {
     const uint32_t t = S.tile_list[u];
     const uint32_t beg = S.offset[t * TILE_CELLS], end = S.offset[(t + 1) * TILE_CELLS];
     int ti, tj, tk;
@@ -437,6 +503,10 @@ __global__ __launch_bounds__(TILE_BLOCK) void k_g2p_tile(DevSim S) {
     }
     __syncthreads();
     }
+}
+__global__ __launch_bounds__(TILE_BLOCK) void k_g2p_tile(DevSim S) { body_g2p(S, blockIdx.x, gridDim.x); }
+__global__ __launch_bounds__(TILE_BLOCK) void kb_g2p_tile(WorkList W) {
+    for_each_work(W, [&](const DevSim &S, uint32_t b) { body_g2p(S, b, tile_grid(S)); });
 }
 
 // ---------------------------------------------------------------- IO / bins
@@ -535,9 +605,6 @@ void launch_exclusive_scan(const uint32_t *in, uint32_t n, uint32_t *tile_tmp, u
 void launch_scan_offsets(const DevSim &S, hipStream_t s) { launch_exclusive_scan(S.count, S.nkeys, S.scan_tmp, S.offset, s); }
 
 void launch_scatter(const DevSim &S, hipStream_t s) { k_scatter<<<blocks_for(S.np), BLOCK, 0, s>>>(S); }
-// Workgroups of the per-tile kernels: enough to keep every CU busy, each walking the active
-// tile list with a grid stride.
-inline unsigned tile_grid(const DevSim &S) { return S.ntiles < 1024u ? S.ntiles : 1024u; }
 
 void launch_tile_list(const DevSim &S, hipStream_t s) { k_tile_list<<<blocks_for(S.ntiles), BLOCK, 0, s>>>(S); }
 void launch_p2g(const DevSim &S, hipStream_t s) { k_p2g_tile<<<tile_grid(S), TILE_BLOCK, 0, s>>>(S); }
@@ -562,5 +629,24 @@ void launch_pack_aos8(const DevSim &S, const float *aos8, hipStream_t s) {
 void launch_reset_particles(const DevSim &S, hipStream_t s) {
     k_reset_particles<<<blocks_for(S.np), BLOCK, 0, s>>>(S);
 }
+
+// ---- batched
+uint32_t particle_blocks(const DevSim &S) { return blocks_for(S.np); }
+uint32_t key_blocks(const DevSim &S) { return blocks_for(S.nkeys); }
+uint32_t tile_blocks(const DevSim &S) { return blocks_for(S.ntiles); }
+uint32_t tile_grid_blocks(const DevSim &S) { return tile_grid(S); }
+
+void batch_clear_grid(const WorkLists &W, hipStream_t s) { kb_clear_grid<<<BATCH_GRID, BLOCK, 0, s>>>(W[WD_KEYS]); }
+void batch_bin_count(const WorkLists &W, hipStream_t s) { kb_bin_count<<<BATCH_GRID, BLOCK, 0, s>>>(W[WD_PART]); }
+void batch_scan(const WorkLists &W, int which, hipStream_t s) {
+    const WorkList &t = W[which == 0 ? WD_SCAN_KEYS : WD_SCAN_FLAGS];
+    kb_scan_reduce<<<BATCH_GRID, BLOCK, 0, s>>>(t, which);
+    kb_scan_tiles<<<BATCH_GRID, BLOCK, 0, s>>>(W[WD_SIM], which);
+    kb_scan_down<<<BATCH_GRID, BLOCK, 0, s>>>(t, which);
+}
+void batch_tile_list(const WorkLists &W, hipStream_t s) { kb_tile_list<<<BATCH_GRID, BLOCK, 0, s>>>(W[WD_TILES]); }
+void batch_scatter(const WorkLists &W, hipStream_t s) { kb_scatter<<<BATCH_GRID, BLOCK, 0, s>>>(W[WD_PART]); }
+void batch_p2g(const WorkLists &W, hipStream_t s) { kb_p2g_tile<<<BATCH_TILE_GRID, TILE_BLOCK, 0, s>>>(W[WD_TILEGRID]); }
+void batch_g2p(const WorkLists &W, hipStream_t s) { kb_g2p_tile<<<BATCH_TILE_GRID, TILE_BLOCK, 0, s>>>(W[WD_TILEGRID]); }
 
 }  // namespace pxl

@@ -1,0 +1,834 @@
+// Batched execution of many simulations and the serving loop around it
+// (pxlflip_server_* in include/pxlflip.h).
+//
+// Batch: the resident simulations are stepped together.  Every kernel of the step is
+// launched once for the whole batch as a persistent grid over a device-side work list of
+// (simulation, workgroup) entries (kernels.hpp, WorkList); a simulation's entries are
+// exactly the workgroups of its own single-sim launch, so it computes bitwise what it
+// computes alone.  The step runs as three graph replays and a host-driven refinement loop:
+//   grid graph:   clear .. fluid list, multigrid types, FP64 init (x64, r64, norms)
+//   inner graph:  one refinement pass (MG-CG + FP64 update) for the simulations still
+//                 solving; replayed until every one of them reaches the tolerance (the norms
+//                 of all simulations come back in one copy per pass, and the solve work
+//                 lists are rewritten to hold only the unconverged ones)
+//   post graph:   project, g2p.
+// Work lists live in one device buffer laid out by domain; the layout (and with it the
+// graphs, which bake the list pointers) only changes when a domain outgrows its capacity
+// or the deepest multigrid level changes.
+//
+// Server: a device thread admits queued requests first-in first-out while their HBM
+// footprint fits the budget, allocates them, adds them to the batch, runs batch steps and
+// retires simulations at the step boundary after their last step.
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+
+#include "common.hpp"
+#include "sim.hpp"
+
+namespace pxl {
+
+uint32_t work_blocks(const DevSim &S, int d) {
+    switch (d) {
+        case WD_SIM:
+        case WD_A_SIM: return 1;
+        case WD_KEYS: return key_blocks(S);
+        case WD_PART: return particle_blocks(S);
+        case WD_SCAN_KEYS: return scan_tiles(S.nkeys);
+        case WD_TILES: return tile_blocks(S);
+        case WD_TILEGRID: return tile_grid_blocks(S);
+        case WD_NODES: return node_blocks(S);
+        case WD_CELLS: return cell_blocks(S);
+        case WD_SCAN_FLAGS: return scan_tiles(flag_count(S));
+        case WD_COMPACT: return (flag_count(S) + BLOCK - 1) / BLOCK;
+        case WD_A_SOLVER: return (uint32_t)S.cg_blocks;
+        default: break;
+    }
+    if (d >= WD_MGTYPES && d < WD_MGTYPES + MG_MAX) {
+        const int l = d - WD_MGTYPES;
+        return l >= 1 && l < S.mg_levels ? mg_cell_blocks(S, l) : 0;
+    }
+    if (d >= WD_A_LVL && d < WD_A_LVL + MG_MAX) {
+        const int l = d - WD_A_LVL;
+        return l < S.mg_tail ? (uint32_t)S.cg_blocks : 0;
+    }
+    return 0;
+}
+
+void batch_grid_phase(const WorkLists &W, int max_levels, hipStream_t s) {
+    batch_clear_grid(W, s);
+    batch_bin_count(W, s);
+    batch_scan(W, 0, s);
+    batch_tile_list(W, s);
+    batch_scatter(W, s);
+    batch_p2g(W, s);
+    batch_faces(W, s);
+    batch_cells(W, s);
+    batch_fluid_list(W, s);
+    batch_mg_types(W, max_levels, s);
+}
+
+void batch_post(const WorkLists &W, hipStream_t s) {
+    batch_project(W, s);
+    batch_g2p(W, s);
+}
+
+namespace {
+
+double now_s() {
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+template <typename T>
+void host_alloc(T **p, size_t n) {
+    if (*p) (void)hipHostFree(*p);
+    *p = nullptr;
+    HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(p), sizeof(T) * (n ? n : 1), hipHostMallocDefault));
+}
+template <typename T>
+void dev_alloc(T **p, size_t n) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    if (hipMalloc(reinterpret_cast<void **>(p), sizeof(T) * (n ? n : 1)) != hipSuccess) throw std::bad_alloc();
+}
+
+class Batch {
+public:
+    explicit Batch(const pxlflip_opts &o) : opts_(o) {
+        HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        dev_alloc(&d_counts_, WD_COUNT);
+        host_alloc(&h_counts_, WD_COUNT);
+        std::fill(cap_, cap_ + WD_COUNT, 0);
+        std::fill(off_, off_ + WD_COUNT, 0);
+    }
+    ~Batch() {
+        if (stream_) (void)hipStreamSynchronize(stream_);
+        drop_graphs();
+        for (void *p : {(void *)d_sims_, (void *)d_entries_, (void *)d_counts_, (void *)d_norms_})
+            if (p) (void)hipFree(p);
+        for (void *p : {(void *)h_sims_, (void *)h_entries_, (void *)h_counts_, (void *)h_norms_})
+            if (p) (void)hipHostFree(p);
+        if (stream_) (void)hipStreamDestroy(stream_);
+    }
+
+    hipStream_t stream() const { return stream_; }
+    size_t size() const { return members_.size(); }
+    const std::vector<Sim *> &members() const { return members_; }
+    uint64_t graph_captures() const { return captures_; }
+    uint64_t rebuilds() const { return rebuilds_; }
+
+    void add(Sim *s) {
+        members_.push_back(s);
+        passes_.push_back(0);
+        dirty_ = true;
+    }
+    void remove(Sim *s) {
+        HIP_CHECK(hipStreamSynchronize(stream_));
+        for (size_t i = 0; i < members_.size(); i++) {
+            if (members_[i] == s) {
+                members_.erase(members_.begin() + (long)i);
+                passes_.erase(passes_.begin() + (long)i);
+                break;
+            }
+        }
+        dirty_ = true;
+    }
+
+    // One step of every member.
+    void step() {
+        if (members_.empty()) return;
+        if (dirty_) rebuild();
+        const size_t n = members_.size();
+        std::vector<uint32_t> active(n);
+        for (size_t i = 0; i < n; i++) active[i] = (uint32_t)i;
+        set_active(active);
+        replay(&g_grid_, [&] {
+            batch_grid_phase(lists_, max_levels_, stream_);
+            batch_mixed_init(lists_, stream_);
+        });
+        read_norms();
+        std::vector<double> bb(n), rel(n);
+        for (size_t i = 0; i < n; i++) {
+            bb[i] = h_norms_[2 * i + 1];
+            rel[i] = rel_of(h_norms_[2 * i], bb[i]);
+            passes_[i] = 0;
+        }
+        std::vector<uint32_t> next;
+        for (uint32_t i : active)
+            if (rel[i] > opts_.tolerance) next.push_back(i);
+        uint32_t pass = 0;
+        while (!next.empty() && pass < opts_.max_outer) {
+            if (next.size() != active.size()) set_active(next);
+            active = next;
+            const int cur = (int)(pass & 1);
+            replay(&g_inner_[cur], [&] { batch_inner_pass(lists_, cur, opts_.inner_iterations, max_tail_, stream_); });
+            read_norms();
+            next.clear();
+            for (uint32_t i : active) {
+                passes_[i]++;
+                rel[i] = rel_of(h_norms_[2 * i], bb[i]);
+                if (rel[i] > opts_.tolerance) next.push_back(i);
+            }
+            pass++;
+        }
+        replay(&g_post_, [&] { batch_post(lists_, stream_); });
+        for (size_t i = 0; i < n; i++) {
+            Sim *s = members_[i];
+            s->steps_done++;
+            s->stats.rel_residual = rel[i];
+            s->stats.outer_iterations = passes_[i];
+            s->stats.inner_iterations = passes_[i] * opts_.inner_iterations;
+            s->x64_cur = (int)(passes_[i] & 1);
+        }
+    }
+
+    void sync() { HIP_CHECK(hipStreamSynchronize(stream_)); }
+
+private:
+    static double rel_of(double rr, double bb) { return bb > 0.0 ? std::sqrt(rr / bb) : std::sqrt(rr); }
+
+    void drop_graphs() {
+        for (hipGraphExec_t *g : {&g_grid_, &g_inner_[0], &g_inner_[1], &g_post_}) {
+            if (*g) (void)hipGraphExecDestroy(*g);
+            *g = nullptr;
+        }
+    }
+
+    template <typename F>
+    void replay(hipGraphExec_t *g, F &&enqueue) {
+        if (!opts_.use_graph) {
+            enqueue();
+            return;
+        }
+        if (!*g) {
+            hipGraph_t graph = nullptr;
+            HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+            enqueue();
+            HIP_CHECK(hipStreamEndCapture(stream_, &graph));
+            hipError_t e = hipGraphInstantiate(g, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            HIP_CHECK(e);
+            captures_++;
+        }
+        HIP_CHECK(hipGraphLaunch(*g, stream_));
+    }
+
+    void read_norms() {
+        HIP_CHECK(hipMemcpyAsync(h_norms_, d_norms_, sizeof(double) * 2 * members_.size(), hipMemcpyDeviceToHost,
+                                 stream_));
+        HIP_CHECK(hipStreamSynchronize(stream_));
+    }
+
+    // Fills domain d's entries for the members `ids` into the host staging buffer.
+    uint32_t fill(int d, const std::vector<uint32_t> &ids) {
+        uint32_t *out = h_entries_ + off_[d];
+        uint32_t k = 0;
+        for (uint32_t i : ids) {
+            const uint32_t nb = work_blocks(members_[i]->d, d);
+            for (uint32_t b = 0; b < nb; b++) out[k++] = (i << WORK_SIM_SHIFT) | b;
+        }
+        h_counts_[d] = k;
+        return k;
+    }
+
+    // Solve domains for the members in `ids` (all of them at the start of a step).
+    void set_active(const std::vector<uint32_t> &ids) {
+        for (int d = WD_A_SIM; d < WD_COUNT; d++) fill(d, ids);
+        const size_t a = off_[WD_A_SIM], bytes = 4 * (off_[WD_COUNT - 1] + cap_[WD_COUNT - 1] - a);
+        HIP_CHECK(hipMemcpyAsync(d_entries_ + a, h_entries_ + a, bytes, hipMemcpyHostToDevice, stream_));
+        HIP_CHECK(hipMemcpyAsync(d_counts_, h_counts_, 4 * WD_COUNT, hipMemcpyHostToDevice, stream_));
+    }
+
+    void rebuild() {
+        HIP_CHECK(hipStreamSynchronize(stream_));
+        const size_t n = members_.size();
+        if (n > MAX_BATCH_SIMS) throw std::runtime_error("batch holds more simulations than the work lists address");
+        bool relayout = false;
+        if (n > sims_cap_) {
+            sims_cap_ = std::max<size_t>(2 * sims_cap_, std::max<size_t>(n, 64));
+            sims_cap_ = std::min<size_t>(sims_cap_, MAX_BATCH_SIMS);
+            dev_alloc(&d_sims_, sims_cap_);
+            host_alloc(&h_sims_, sims_cap_);
+            dev_alloc(&d_norms_, 2 * sims_cap_);
+            host_alloc(&h_norms_, 2 * sims_cap_);
+            relayout = true;
+        }
+        size_t need[WD_COUNT];
+        for (int d = 0; d < WD_COUNT; d++) {
+            need[d] = 0;
+            for (Sim *s : members_) need[d] += work_blocks(s->d, d);
+            if (need[d] > cap_[d]) relayout = true;
+        }
+        if (relayout) {
+            size_t total = 0;
+            for (int d = 0; d < WD_COUNT; d++) {
+                if (need[d] > cap_[d]) cap_[d] = need[d] + need[d] / 2 + 64;
+                off_[d] = total;
+                total += cap_[d];
+            }
+            dev_alloc(&d_entries_, total);
+            host_alloc(&h_entries_, total);
+            for (int d = 0; d < WD_COUNT; d++) lists_[d] = WorkList{d_sims_, d_entries_ + off_[d], d_counts_ + d};
+        }
+        // the graphs launch the level kernels of the deepest hierarchy seen so far (levels a
+        // member does not have get empty work lists), so members leaving do not force a
+        // re-capture
+        int levels = max_levels_, tail = max_tail_;
+        for (Sim *s : members_) {
+            levels = std::max(levels, s->d.mg_levels);
+            tail = std::max(tail, s->d.mg_tail);
+        }
+        if (relayout || levels != max_levels_ || tail != max_tail_) drop_graphs();
+        rebuilds_++;
+        max_levels_ = levels;
+        max_tail_ = tail;
+        for (size_t i = 0; i < n; i++) {
+            h_sims_[i] = members_[i]->d;
+            h_sims_[i].norms = d_norms_ + 2 * i;
+        }
+        HIP_CHECK(hipMemcpyAsync(d_sims_, h_sims_, sizeof(DevSim) * n, hipMemcpyHostToDevice, stream_));
+        std::vector<uint32_t> all(n);
+        for (size_t i = 0; i < n; i++) all[i] = (uint32_t)i;
+        for (int d = 0; d < WD_A_SIM; d++) fill(d, all);
+        HIP_CHECK(hipMemcpyAsync(d_entries_, h_entries_, 4 * off_[WD_A_SIM], hipMemcpyHostToDevice, stream_));
+        // the copies above read pinned staging that the next rebuild rewrites
+        HIP_CHECK(hipStreamSynchronize(stream_));
+        dirty_ = false;
+    }
+
+    pxlflip_opts opts_;
+    hipStream_t stream_ = nullptr;
+    std::vector<Sim *> members_;
+    std::vector<uint32_t> passes_;
+    bool dirty_ = true;
+    size_t sims_cap_ = 0;
+    DevSim *d_sims_ = nullptr, *h_sims_ = nullptr;
+    uint32_t *d_entries_ = nullptr, *h_entries_ = nullptr;
+    uint32_t *d_counts_ = nullptr, *h_counts_ = nullptr;
+    double *d_norms_ = nullptr, *h_norms_ = nullptr;
+    size_t off_[WD_COUNT], cap_[WD_COUNT];
+    WorkLists lists_{};
+    int max_levels_ = 0, max_tail_ = 0;
+    hipGraphExec_t g_grid_ = nullptr, g_inner_[2] = {nullptr, nullptr}, g_post_ = nullptr;
+    uint64_t captures_ = 0, rebuilds_ = 0;
+};
+
+// Solver workgroups of a batched simulation: enough lists slices to cover the fluid cells a
+// grid of this size can hold with ~4096 cells per workgroup, a multiple of 8 (XCDs).
+uint32_t batch_solver_blocks(const DevSim &d) {
+    uint32_t g = d.ncells / 8192u;
+    g = std::max<uint32_t>(8u, std::min<uint32_t>(256u, g));
+    return (g + 7u) / 8u * 8u;
+}
+
+struct Request {
+    uint64_t id = 0;
+    std::string json;
+    double fps = 24.0;
+    uint32_t seed = 0, steps = 0;
+    bool keep = false;
+    int status = PXLFLIP_REQ_QUEUED;
+    bool cancel = false;
+    std::string error;
+    double t_submit = 0, t_admit = 0, t_done = 0;
+    uint64_t bytes = 0;
+    uint32_t np = 0;
+    uint32_t steps_done = 0;  // updated by the device thread under the server lock
+    std::string alloc_error;  // set when admission failed to allocate
+    std::unique_ptr<Sim> sim;
+    std::vector<float> result;
+    double rel_residual = -1.0;
+    uint32_t max_outer = 0;
+};
+
+}  // namespace
+}  // namespace pxl
+
+struct pxlflip_server {
+    pxlflip_server_opts opts{};
+    pxlflip_opts sim_opts{};
+    std::unique_ptr<pxl::Batch> batch;
+    std::mutex mu;
+    std::condition_variable cv;       // worker wake-up
+    std::condition_variable done_cv;  // request state changes
+    std::deque<pxl::Request *> queue;
+    std::vector<pxl::Request *> running;
+    std::map<uint64_t, std::unique_ptr<pxl::Request>> all;
+    uint64_t next_id = 1;
+    bool stop = false;
+    std::thread worker;
+    std::string error;
+    // accounting
+    uint64_t bytes_in_use = 0, peak_bytes = 0;
+    uint64_t submitted = 0, completed = 0, failed = 0, cancelled = 0;
+    std::vector<double> latencies_ms;
+    double particle_steps = 0, first_submit = -1, last_done = -1, busy_s = 0;
+    uint64_t batch_steps = 0;
+    double batch_members = 0;
+    double admit_s = 0;  // device-thread time spent allocating admitted requests
+
+    void run();
+    void admit(pxl::Request *r);
+    void finish(pxl::Request *r, int status, const std::string &err);
+};
+
+namespace pxl {
+namespace {
+
+// Allocates a request's simulation (device thread).  Throws on failure.
+void allocate_request(pxlflip_server *srv, Request *r) {
+    auto sim = std::make_unique<Sim>();
+    std::vector<float> x0, v0;
+    if (prepare(r->json.c_str(), r->fps, r->seed, &srv->sim_opts, *sim, &x0, &v0) != PXLFLIP_OK)
+        throw std::runtime_error(g_create_error);
+    sim->d.cg_blocks = (int)(srv->opts.solver_workgroups ? srv->opts.solver_workgroups : batch_solver_blocks(sim->d));
+    HIP_CHECK(hipStreamCreateWithFlags(&sim->stream, hipStreamNonBlocking));
+    sim->allocate(x0, v0);
+    sim->reset();
+    r->sim = std::move(sim);
+}
+
+}  // namespace
+}  // namespace pxl
+
+void pxlflip_server::finish(pxl::Request *r, int status, const std::string &err) {
+    // mu held
+    r->status = status;
+    r->error = err;
+    r->t_done = pxl::now_s();
+    if (status == PXLFLIP_REQ_DONE) {
+        completed++;
+        latencies_ms.push_back(1e3 * (r->t_done - r->t_submit));
+        particle_steps += (double)r->np * (double)r->steps;
+        last_done = r->t_done;
+    } else if (status == PXLFLIP_REQ_FAILED) {
+        failed++;
+    } else {
+        cancelled++;
+    }
+    done_cv.notify_all();
+}
+
+void pxlflip_server::run() {
+    using namespace pxl;
+    try {
+        HIP_CHECK(hipSetDevice(opts.device));
+    } catch (const HipError &) {
+    }
+    std::unique_lock<std::mutex> lk(mu);
+    while (!stop) {
+        // admission: first in, first out, while the budget and the batch size allow.  An
+        // admitted request is `running` from here on (allocated below, without the lock).
+        std::vector<Request *> admitted;
+        while (!queue.empty()) {
+            Request *r = queue.front();
+            if (r->cancel) {
+                queue.pop_front();
+                finish(r, PXLFLIP_REQ_CANCELLED, "");
+                continue;
+            }
+            if (bytes_in_use + r->bytes > opts.budget_bytes || running.size() >= opts.max_sims) break;
+            queue.pop_front();
+            bytes_in_use += r->bytes;
+            peak_bytes = std::max(peak_bytes, bytes_in_use);
+            r->status = PXLFLIP_REQ_RUNNING;
+            r->t_admit = now_s();
+            admitted.push_back(r);
+            running.push_back(r);
+        }
+        if (running.empty()) {
+            cv.wait(lk, [&] { return stop || !queue.empty(); });
+            continue;
+        }
+        lk.unlock();
+        // device work without the lock
+        const double ta = now_s();
+        for (Request *r : admitted) {
+            try {
+                allocate_request(this, r);
+                if (r->steps > 0) batch->add(r->sim.get());
+            } catch (const HipError &e) {
+                r->alloc_error = std::string("HIP error in ") + e.what + ": " + hipGetErrorString(e.code);
+            } catch (const std::bad_alloc &) {
+                r->alloc_error = "out of device memory";
+            } catch (const std::exception &e) {
+                r->alloc_error = e.what();
+            }
+        }
+        const double admit_time = now_s() - ta;
+        std::string step_error;
+        double t0 = now_s(), t1 = t0;
+        const size_t members = batch->size();
+        try {
+            if (members) {
+                batch->step();
+                batch->sync();
+            }
+            t1 = now_s();
+        } catch (const HipError &e) {
+            step_error = std::string("HIP error in ") + e.what + ": " + hipGetErrorString(e.code);
+        } catch (const std::exception &e) {
+            step_error = e.what();
+        }
+        lk.lock();
+        admit_s += admit_time;
+        if (members) {
+            batch_steps++;
+            batch_members += (double)members;
+            busy_s += t1 - t0;
+        }
+        // retire finished, cancelled or failed requests at the step boundary
+        std::vector<Request *> keep;
+        for (Request *r : running) {
+            Sim *s = r->sim.get();
+            if (!s) {  // allocation failed
+                bytes_in_use -= r->bytes;
+                finish(r, PXLFLIP_REQ_FAILED, r->alloc_error);
+                continue;
+            }
+            if (members && r->steps > 0 && step_error.empty()) {
+                r->rel_residual = s->stats.rel_residual;
+                r->max_outer = std::max(r->max_outer, s->stats.outer_iterations);
+                r->steps_done = (uint32_t)s->steps_done;
+            }
+            const bool done = s->steps_done >= r->steps;
+            if (step_error.empty() && !done && !r->cancel) {
+                keep.push_back(r);
+                continue;
+            }
+            lk.unlock();
+            std::string err = step_error;
+            try {
+                if (r->steps > 0) batch->remove(s);
+                if (done && err.empty() && r->keep) {
+                    r->result.assign((size_t)s->d.np * 8, 0.0f);
+                    DeviceBuffer buf((size_t)s->d.np * 32);
+                    launch_unpack_aos8(s->d, buf.as<float>(), s->stream);
+                    HIP_CHECK(hipMemcpyAsync(r->result.data(), buf.p, (size_t)s->d.np * 32, hipMemcpyDeviceToHost,
+                                             s->stream));
+                    HIP_CHECK(hipStreamSynchronize(s->stream));
+                }
+            } catch (const HipError &e) {
+                err = std::string("HIP error in ") + e.what + ": " + hipGetErrorString(e.code);
+            }
+            r->sim.reset();
+            lk.lock();
+            bytes_in_use -= r->bytes;
+            finish(r, !err.empty() ? PXLFLIP_REQ_FAILED : (done ? PXLFLIP_REQ_DONE : PXLFLIP_REQ_CANCELLED), err);
+        }
+        running.swap(keep);
+    }
+    // stop: cancel what is left
+    for (Request *r : running) {
+        lk.unlock();
+        try {
+            if (r->sim && r->steps > 0) batch->remove(r->sim.get());
+        } catch (...) {
+        }
+        r->sim.reset();
+        lk.lock();
+        bytes_in_use -= r->bytes;
+        finish(r, PXLFLIP_REQ_CANCELLED, "server stopped");
+    }
+    running.clear();
+    while (!queue.empty()) {
+        finish(queue.front(), PXLFLIP_REQ_CANCELLED, "server stopped");
+        queue.pop_front();
+    }
+}
+
+namespace {
+
+void fill_request_info(const pxl::Request &r, pxlflip_request_info *info) {
+    std::memset(info, 0, sizeof *info);
+    info->status = r.status;
+    info->steps = r.steps;
+    info->steps_done = r.steps_done;
+    info->n_particles = r.np;
+    info->device_bytes = r.bytes;
+    info->queue_ms = r.t_admit > 0 ? 1e3 * (r.t_admit - r.t_submit) : -1.0;
+    info->latency_ms = r.status == PXLFLIP_REQ_DONE ? 1e3 * (r.t_done - r.t_submit) : -1.0;
+    info->rel_residual = r.rel_residual;
+    info->max_outer = r.max_outer;
+}
+
+double percentile(std::vector<double> v, double q) {
+    if (v.empty()) return 0.0;
+    std::sort(v.begin(), v.end());
+    // nearest-rank percentile
+    size_t k = (size_t)std::ceil(q * (double)v.size());
+    k = k == 0 ? 0 : k - 1;
+    return v[std::min(k, v.size() - 1)];
+}
+
+}  // namespace
+
+extern "C" {
+
+int pxlflip_server_default_opts(pxlflip_server_opts *o) {
+    if (!o) return PXLFLIP_ERR_INVALID;
+    std::memset(o, 0, sizeof *o);
+    o->struct_size = sizeof(pxlflip_server_opts);
+    o->device = 0;
+    o->budget_bytes = 200000000000ull;
+    o->max_sims = 256;
+    o->inner_iterations = 8;
+    o->max_outer = 50;
+    o->tolerance = 1e-6;
+    o->solver_workgroups = 0;
+    o->use_graph = 1;
+    return PXLFLIP_OK;
+}
+
+int pxlflip_server_create(const pxlflip_server_opts *opts, pxlflip_server **out) {
+    using namespace pxl;
+    g_create_error.clear();
+    if (!out) {
+        g_create_error = "out is NULL";
+        return PXLFLIP_ERR_INVALID;
+    }
+    *out = nullptr;
+    pxlflip_server_opts o;
+    pxlflip_server_default_opts(&o);
+    if (opts) {
+        if (opts->struct_size != sizeof(pxlflip_server_opts)) {
+            g_create_error = "pxlflip_server_opts.struct_size mismatch (use pxlflip_server_default_opts)";
+            return PXLFLIP_ERR_INVALID;
+        }
+        o = *opts;
+    }
+    if (o.max_sims == 0 || o.max_sims > MAX_BATCH_SIMS) {
+        g_create_error = "max_sims must be 1.." + std::to_string(MAX_BATCH_SIMS);
+        return PXLFLIP_ERR_INVALID;
+    }
+    if (o.inner_iterations == 0 || o.max_outer == 0 || !(o.tolerance > 0.0)) {
+        g_create_error = "inner_iterations, max_outer and tolerance must be positive";
+        return PXLFLIP_ERR_INVALID;
+    }
+    if (o.solver_workgroups > MAX_SOLVER_BLOCKS) {
+        g_create_error = "solver_workgroups must be at most " + std::to_string(MAX_SOLVER_BLOCKS);
+        return PXLFLIP_ERR_INVALID;
+    }
+    auto srv = std::make_unique<pxlflip_server>();
+    srv->opts = o;
+    pxlflip_default_opts(&srv->sim_opts);
+    srv->sim_opts.solver = PXLFLIP_SOLVER_MIXED;
+    srv->sim_opts.inner_iterations = o.inner_iterations;
+    srv->sim_opts.max_outer = o.max_outer;
+    srv->sim_opts.tolerance = o.tolerance;
+    srv->sim_opts.device = o.device;
+    srv->sim_opts.use_graph = o.use_graph;
+    try {
+        int n = 0;
+        HIP_CHECK(hipGetDeviceCount(&n));
+        if (o.device < 0 || o.device >= n) {
+            g_create_error = "device " + std::to_string(o.device) + " not present";
+            return PXLFLIP_ERR_DEVICE;
+        }
+        HIP_CHECK(hipSetDevice(o.device));
+        srv->batch = std::make_unique<Batch>(srv->sim_opts);
+    } catch (const HipError &e) {
+        g_create_error = std::string("HIP error in ") + e.what + ": " + hipGetErrorString(e.code);
+        return PXLFLIP_ERR_DEVICE;
+    } catch (const std::bad_alloc &) {
+        g_create_error = "out of memory";
+        return PXLFLIP_ERR_OOM;
+    }
+    pxlflip_server *p = srv.release();
+    p->worker = std::thread([p] { p->run(); });
+    *out = p;
+    return PXLFLIP_OK;
+}
+
+int pxlflip_server_submit(pxlflip_server *srv, const char *fluid_json, double fps, uint32_t seed, uint32_t steps,
+                          int32_t keep_particles, uint64_t *id) {
+    using namespace pxl;
+    if (!srv) return PXLFLIP_ERR_INVALID;
+    auto r = std::make_unique<Request>();
+    r->t_submit = now_s();
+    Sim probe;
+    if (prepare(fluid_json, fps, seed, &srv->sim_opts, probe, nullptr, nullptr) != PXLFLIP_OK) {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->error = g_create_error;
+        return PXLFLIP_ERR_INVALID;
+    }
+    probe.d.cg_blocks = (int)(srv->opts.solver_workgroups ? srv->opts.solver_workgroups : batch_solver_blocks(probe.d));
+    r->bytes = Sim::plan_total(probe.plan());
+    r->np = probe.d.np;
+    r->json = fluid_json;
+    r->fps = fps;
+    r->seed = seed;
+    r->steps = steps;
+    r->keep = keep_particles != 0;
+    std::lock_guard<std::mutex> lk(srv->mu);
+    if (r->bytes > srv->opts.budget_bytes) {
+        srv->error = "request needs " + std::to_string(r->bytes) + " bytes of HBM, more than the admission budget (" +
+                     std::to_string(srv->opts.budget_bytes) + ")";
+        return PXLFLIP_ERR_INVALID;
+    }
+    r->id = srv->next_id++;
+    if (id) *id = r->id;
+    if (srv->first_submit < 0) srv->first_submit = r->t_submit;
+    srv->submitted++;
+    srv->queue.push_back(r.get());
+    srv->all.emplace(r->id, std::move(r));
+    srv->cv.notify_all();
+    return PXLFLIP_OK;
+}
+
+int pxlflip_server_poll(pxlflip_server *srv, uint64_t id, pxlflip_request_info *info) {
+    if (!srv) return PXLFLIP_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(srv->mu);
+    auto it = srv->all.find(id);
+    if (it == srv->all.end()) {
+        srv->error = "unknown request " + std::to_string(id);
+        return PXLFLIP_ERR_INVALID;
+    }
+    if (info) fill_request_info(*it->second, info);
+    if (it->second->status == PXLFLIP_REQ_FAILED) srv->error = it->second->error;
+    return PXLFLIP_OK;
+}
+
+int pxlflip_server_wait(pxlflip_server *srv, uint64_t id, double timeout_s, pxlflip_request_info *info) {
+    if (!srv) return PXLFLIP_ERR_INVALID;
+    std::unique_lock<std::mutex> lk(srv->mu);
+    auto it = srv->all.find(id);
+    if (it == srv->all.end()) {
+        srv->error = "unknown request " + std::to_string(id);
+        return PXLFLIP_ERR_INVALID;
+    }
+    pxl::Request *r = it->second.get();
+    auto finished = [&] { return r->status >= PXLFLIP_REQ_DONE; };
+    bool ok = true;
+    if (timeout_s < 0) srv->done_cv.wait(lk, finished);
+    else ok = srv->done_cv.wait_for(lk, std::chrono::duration<double>(timeout_s), finished);
+    if (info) fill_request_info(*r, info);
+    if (!ok) {
+        srv->error = "timed out waiting for request " + std::to_string(id);
+        return PXLFLIP_ERR_INVALID;
+    }
+    if (r->status == PXLFLIP_REQ_FAILED) srv->error = r->error;
+    return PXLFLIP_OK;
+}
+
+int pxlflip_server_drain(pxlflip_server *srv, double timeout_s) {
+    if (!srv) return PXLFLIP_ERR_INVALID;
+    std::unique_lock<std::mutex> lk(srv->mu);
+    auto idle = [&] { return srv->queue.empty() && srv->running.empty(); };
+    bool ok = true;
+    if (timeout_s < 0) srv->done_cv.wait(lk, idle);
+    else ok = srv->done_cv.wait_for(lk, std::chrono::duration<double>(timeout_s), idle);
+    if (!ok) {
+        srv->error = "timed out draining the server";
+        return PXLFLIP_ERR_INVALID;
+    }
+    return PXLFLIP_OK;
+}
+
+int pxlflip_server_cancel(pxlflip_server *srv, uint64_t id) {
+    if (!srv) return PXLFLIP_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(srv->mu);
+    auto it = srv->all.find(id);
+    if (it == srv->all.end()) {
+        srv->error = "unknown request " + std::to_string(id);
+        return PXLFLIP_ERR_INVALID;
+    }
+    it->second->cancel = true;
+    srv->cv.notify_all();
+    return PXLFLIP_OK;
+}
+
+int pxlflip_server_read_particles(pxlflip_server *srv, uint64_t id, float *aos8, uint64_t max_particles) {
+    if (!srv || !aos8) return PXLFLIP_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(srv->mu);
+    auto it = srv->all.find(id);
+    if (it == srv->all.end()) {
+        srv->error = "unknown request " + std::to_string(id);
+        return PXLFLIP_ERR_INVALID;
+    }
+    const pxl::Request &r = *it->second;
+    if (r.status != PXLFLIP_REQ_DONE || !r.keep) {
+        srv->error = "request " + std::to_string(id) + " has no kept particles (not done or submitted without keep)";
+        return PXLFLIP_ERR_INVALID;
+    }
+    if (max_particles < r.np) {
+        srv->error = "aos8 holds " + std::to_string(max_particles) + " particles, need " + std::to_string(r.np);
+        return PXLFLIP_ERR_INVALID;
+    }
+    std::memcpy(aos8, r.result.data(), r.result.size() * sizeof(float));
+    return PXLFLIP_OK;
+}
+
+int pxlflip_server_release(pxlflip_server *srv, uint64_t id) {
+    if (!srv) return PXLFLIP_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(srv->mu);
+    auto it = srv->all.find(id);
+    if (it == srv->all.end()) {
+        srv->error = "unknown request " + std::to_string(id);
+        return PXLFLIP_ERR_INVALID;
+    }
+    if (it->second->status < PXLFLIP_REQ_DONE) {
+        srv->error = "request " + std::to_string(id) + " has not finished";
+        return PXLFLIP_ERR_INVALID;
+    }
+    srv->all.erase(it);
+    return PXLFLIP_OK;
+}
+
+int pxlflip_server_get_stats(pxlflip_server *srv, pxlflip_server_stats *st) {
+    if (!srv || !st) return PXLFLIP_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(srv->mu);
+    std::memset(st, 0, sizeof *st);
+    st->submitted = srv->submitted;
+    st->completed = srv->completed;
+    st->failed = srv->failed;
+    st->cancelled = srv->cancelled;
+    st->queued = srv->queue.size();
+    st->running = srv->running.size();
+    st->p50_ms = percentile(srv->latencies_ms, 0.50);
+    st->p99_ms = percentile(srv->latencies_ms, 0.99);
+    double sum = 0, mx = 0;
+    for (double v : srv->latencies_ms) {
+        sum += v;
+        mx = std::max(mx, v);
+    }
+    st->mean_ms = srv->latencies_ms.empty() ? 0.0 : sum / (double)srv->latencies_ms.size();
+    st->max_ms = mx;
+    st->particle_steps = srv->particle_steps;
+    st->span_s = srv->last_done > srv->first_submit && srv->first_submit >= 0 ? srv->last_done - srv->first_submit : 0.0;
+    st->throughput = st->span_s > 0 ? st->particle_steps / st->span_s : 0.0;
+    st->busy_s = srv->busy_s;
+    st->batch_steps = srv->batch_steps;
+    st->mean_batch = srv->batch_steps ? srv->batch_members / (double)srv->batch_steps : 0.0;
+    st->bytes_in_use = srv->bytes_in_use;
+    st->peak_bytes = srv->peak_bytes;
+    st->admit_s = srv->admit_s;
+    st->rebuilds = srv->batch ? srv->batch->rebuilds() : 0;
+    st->graph_captures = srv->batch ? srv->batch->graph_captures() : 0;
+    return PXLFLIP_OK;
+}
+
+const char *pxlflip_server_last_error(const pxlflip_server *srv) {
+    if (!srv) return pxl::g_create_error.c_str();
+    return srv->error.c_str();
+}
+
+void pxlflip_server_destroy(pxlflip_server *srv) {
+    if (!srv) return;
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->stop = true;
+        srv->cv.notify_all();
+    }
+    if (srv->worker.joinable()) srv->worker.join();
+    (void)hipSetDevice(srv->opts.device);
+    srv->batch.reset();
+    delete srv;
+}
+
+}  // extern "C"

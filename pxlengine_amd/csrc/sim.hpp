@@ -1,0 +1,383 @@
+// Host-side state of one simulation (shared by the single-sim C ABI and the batch server).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "pxlflip.h"
+#include "kernels.hpp"
+#include "spec.hpp"
+
+namespace pxl {
+
+// Error message of the last failed create/check on this thread.
+extern thread_local std::string g_create_error;
+
+struct HipError {
+    hipError_t code;
+    const char *what;
+};
+
+#define HIP_CHECK(expr)                                                   \
+    do {                                                                  \
+        hipError_t e_ = (expr);                                           \
+        if (e_ != hipSuccess) throw ::pxl::HipError{e_, #expr};           \
+    } while (0)
+
+constexpr size_t ALIGN = 256;
+constexpr uint32_t MAX_SOLVER_BLOCKS = 4096;
+constexpr uint32_t DEFAULT_SOLVER_BLOCKS = 256;
+
+// Optional HIP-event bracketing of every launch (pxlflip_profile_steps).  Off while a graph
+// is captured or replayed.
+struct KernelTimer {
+    struct Rec {
+        const char *name;
+        hipEvent_t a, b;
+    };
+    bool on = false;
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+
+    hipEvent_t next() {
+        if (used == pool.size()) {
+            hipEvent_t e;
+            HIP_CHECK(hipEventCreate(&e));
+            pool.push_back(e);
+        }
+        return pool[used++];
+    }
+    template <typename F>
+    void run(const char *name, hipStream_t s, F &&launch) {
+        if (!on) {
+            launch();
+            return;
+        }
+        hipEvent_t a = next(), b = next();
+        HIP_CHECK(hipEventRecord(a, s));
+        launch();
+        HIP_CHECK(hipEventRecord(b, s));
+        recs.push_back({name, a, b});
+    }
+    void clear() {
+        recs.clear();
+        used = 0;
+    }
+    ~KernelTimer() {
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+#define KRUN(name, call) timer.run(name, stream, [&] { call; })
+
+struct Sim {
+    FluidSpec spec;
+    pxlflip_opts opts{};
+    double fps = 24.0;
+    uint32_t seed = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DevSim d{};
+    void *arena = nullptr;
+    size_t arena_bytes = 0;
+    uint32_t iters = 80;          // RBGS sweeps or CG iterations per step
+    uint64_t steps_done = 0;
+    bool initialized = false;
+    hipGraphExec_t step_graph = nullptr;
+    hipGraphExec_t inner_graph[2] = {nullptr, nullptr};
+    hipGraphExec_t grid_graph = nullptr, post_graph = nullptr;  // MIXED: phases around the solve
+    double *host_scal = nullptr;  // pinned
+    int x64_cur = 0;
+    pxlflip_solve_stats stats{-1.0, 0, 0};
+    std::string error;
+    KernelTimer timer;
+
+    // Graphs bake kernel arguments (grid shapes, tunables); drop them after a change.
+    void drop_graphs() {
+        if (step_graph) (void)hipGraphExecDestroy(step_graph);
+        step_graph = nullptr;
+        for (hipGraphExec_t *g : {&inner_graph[0], &inner_graph[1], &grid_graph, &post_graph}) {
+            if (*g) (void)hipGraphExecDestroy(*g);
+            *g = nullptr;
+        }
+    }
+
+    ~Sim() {
+        drop_graphs();
+        if (host_scal) (void)hipHostFree(host_scal);
+        if (arena) (void)hipFree(arena);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    // ---- allocation
+    using Plan = std::vector<std::pair<void **, size_t>>;
+    // Every device array of the simulation, in arena order.
+    Plan plan() {
+        const size_t P = d.np, F = d.nfaces, C = d.ncells, K = d.nkeys;
+        const size_t nb = MAX_SOLVER_BLOCKS;  // partials sized for the largest grid the tuner may pick
+        Plan plan;
+        auto add = [&](auto **p, size_t bytes) { plan.emplace_back(reinterpret_cast<void **>(p), bytes); };
+        for (float **p : {&d.ax, &d.ay, &d.az, &d.avx, &d.avy, &d.avz}) add(p, 4 * P);
+        add(&d.aid, 4 * P);
+        for (float **p : {&d.bx, &d.by, &d.bz, &d.bvx, &d.bvy, &d.bvz}) add(p, 4 * P);
+        add(&d.bid, 4 * P);
+        add(&d.key, 4 * P);
+        add(&d.slot, 4 * P);
+        for (float **p : {&d.ix, &d.iy, &d.iz, &d.ivx, &d.ivy, &d.ivz}) add(p, 4 * P);
+        add(&d.vel, 4 * F);
+        add(&d.vel_old, 4 * F);
+        add(&d.box, 8ull * BOX_ENTRIES * d.ntiles);
+        add(&d.tile_list, 4ull * d.ntiles);
+        add(&d.tile_count, 64);
+        add(&d.count, 4 * K);
+        add(&d.offset, 4 * (K + 1));
+        add(&d.type, C);
+        add(&d.sten, 2 * C);
+        add(&d.pressure, 4 * C);
+        add(&d.div, 4 * C);
+        add(&d.fl_flag, 4 * 2 * C);
+        add(&d.fl_off, 4 * 2 * C);
+        add(&d.fl_list, 4 * C);
+        add(&d.fl_word, 4 * C);
+        add(&d.fl_count, 64);
+        add(&d.fl_tmp, 4 * (scan_tiles(2 * (uint32_t)C) + 1));
+        for (float **p : {&d.cg_r0, &d.cg_r1, &d.cg_w0, &d.cg_w1, &d.cg_s0, &d.cg_s1, &d.cg_p}) add(p, 4 * C);
+        if (opts.solver == PXLFLIP_SOLVER_MIXED) {
+            for (int l = 1; l < d.mg_levels; l++) {
+                const size_t cl = (size_t)d.mg[l].nx * d.mg[l].ny * d.mg[l].nz;
+                add(&d.mg[l].type, cl);
+                add(&d.mg[l].sten, 2 * cl);
+                add(&d.mg[l].r, 4 * cl);
+                add(&d.mg[l].z, 4 * cl);
+                add(&d.mg[l].list, 4 * cl);
+                add(&d.mg[l].count, 64);
+            }
+            add(&d.e, 4 * C);
+            add(&d.x64a, 8 * C);
+            add(&d.x64b, 8 * C);
+            add(&d.r64, 8 * C);
+        }
+        add(&d.part32, 4 * 4 * nb);
+        add(&d.part64, 8 * 4 * nb);
+        add(&d.scal, 8 * 2 * SC_COUNT);
+        add(&d.ctr, 64);
+        add(&d.scan_tmp, 4 * (scan_tiles(K) + 1));
+        return plan;
+    }
+    static size_t plan_total(const Plan &plan) {
+        size_t total = 0;
+        for (auto &e : plan) total += (e.second + ALIGN - 1) / ALIGN * ALIGN;
+        return total;
+    }
+
+    void allocate(const std::vector<float> &x0, const std::vector<float> &v0) {
+        const size_t P = d.np;
+        Plan plan = this->plan();
+        const size_t total = plan_total(plan);
+        void *base = nullptr;
+        hipError_t err = hipMalloc(&base, total);
+        if (err != hipSuccess) throw std::bad_alloc();
+        arena = base;
+        arena_bytes = total;
+        char *cur = static_cast<char *>(base);
+        for (auto &e : plan) {
+            *e.first = cur;
+            cur += (e.second + ALIGN - 1) / ALIGN * ALIGN;
+        }
+        d.norms = d.scal + SC_RR;
+        HIP_CHECK(hipMemsetAsync(arena, 0, arena_bytes, stream));
+        // the end of the last tile's particle run (the scan fills offset[0 .. nkeys-1])
+        const uint32_t np32 = d.np;
+        HIP_CHECK(hipMemcpyAsync(d.offset + d.nkeys, &np32, 4, hipMemcpyHostToDevice, stream));
+        // initial particles (reset source)
+        float *dst[6] = {d.ix, d.iy, d.iz, d.ivx, d.ivy, d.ivz};
+        for (int a = 0; a < 3; a++) {
+            HIP_CHECK(hipMemcpyAsync(dst[a], x0.data() + a * P, 4 * P, hipMemcpyHostToDevice, stream));
+            HIP_CHECK(hipMemcpyAsync(dst[3 + a], v0.data() + a * P, 4 * P, hipMemcpyHostToDevice, stream));
+        }
+        HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&host_scal), 8 * SC_COUNT, hipHostMallocDefault));
+        HIP_CHECK(hipStreamSynchronize(stream));
+    }
+
+    // ---- step pieces
+    void enqueue_grid_phase() {
+        KRUN("clear_grid", launch_clear_grid(d, stream));
+        KRUN("bin_count", launch_bin_count(d, stream));
+        KRUN("scan_offsets", launch_scan_offsets(d, stream));
+        KRUN("tile_list", launch_tile_list(d, stream));
+        KRUN("scatter", launch_scatter(d, stream));
+        KRUN("p2g", launch_p2g(d, stream));
+        KRUN("faces", launch_faces(d, stream));
+        KRUN("cells", launch_cells(d, stream));
+        KRUN("fluid_list", launch_fluid_list(d, stream));
+        if (d.mg_levels) KRUN("mg_types", launch_mg_types(d, stream));
+    }
+    void enqueue_fixed_pressure() {
+        if (opts.solver == PXLFLIP_SOLVER_RBGS) {
+            for (uint32_t it = 0; it < iters; it++) {
+                KRUN("rbgs", launch_rbgs(d, 0, stream));
+                KRUN("rbgs", launch_rbgs(d, 1, stream));
+            }
+        } else {
+            KRUN("cg_start", launch_cg_start(d, false, d.pressure, false, stream));
+            for (uint32_t it = 0; it < iters; it++) KRUN("cg_iter", launch_cg_iter(d, false, d.pressure, it, stream));
+        }
+    }
+    void enqueue_post() {
+        KRUN("project", launch_project(d, stream));
+        KRUN("g2p", launch_g2p(d, stream));
+    }
+    void enqueue_fixed_step() {
+        enqueue_grid_phase();
+        enqueue_fixed_pressure();
+        enqueue_post();
+    }
+    void enqueue_inner(int cur) {
+        KRUN("mgcg_start", launch_mgcg_start(d, d.e, stream));
+        for (uint32_t it = 0; it < opts.inner_iterations; it++)
+            KRUN("mgcg_iter", launch_mgcg_iter(d, d.e, it, it + 1 == opts.inner_iterations, stream));
+        KRUN("mixed_update", launch_mixed_update(d, x64(cur), x64(cur ^ 1), stream));
+    }
+    void enqueue_grid_init() {
+        enqueue_grid_phase();
+        KRUN("mixed_init", launch_mixed_init(d, x64(0), stream));
+    }
+    void enqueue_inner0() { enqueue_inner(0); }
+    void enqueue_inner1() { enqueue_inner(1); }
+
+    hipGraphExec_t capture(void (Sim::*fn)()) {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ex = nullptr;
+        HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        (this->*fn)();
+        HIP_CHECK(hipStreamEndCapture(stream, &g));
+        hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIP_CHECK(e);
+        return ex;
+    }
+
+    double *x64(int k) const { return k == 0 ? d.x64a : d.x64b; }
+
+    void read_scalars() {
+        HIP_CHECK(hipMemcpyAsync(host_scal, d.scal, 8 * SC_COUNT, hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+    }
+
+    // Runs `fn` as a graph captured on first use into *g (when graphs are on), else eagerly.
+    void replay(hipGraphExec_t *g, void (Sim::*fn)()) {
+        if (opts.use_graph && !timer.on) {
+            if (!*g) *g = capture(fn);
+            HIP_CHECK(hipGraphLaunch(*g, stream));
+        } else {
+            (this->*fn)();
+        }
+    }
+
+    // Mixed-precision pressure solve after enqueue_grid_init: FP64 refinement driven from
+    // the host, one residual read-back per refinement pass.
+    void mixed_pressure() {
+        x64_cur = 0;
+        read_scalars();
+        const double bb = host_scal[SC_BB];
+        double rr = host_scal[SC_RR];
+        auto rel = [&](double r) { return bb > 0.0 ? std::sqrt(r / bb) : std::sqrt(r); };
+        uint32_t outer = 0;
+        while (rel(rr) > opts.tolerance && outer < opts.max_outer) {
+            replay(&inner_graph[x64_cur], x64_cur == 0 ? &Sim::enqueue_inner0 : &Sim::enqueue_inner1);
+            x64_cur ^= 1;
+            read_scalars();
+            rr = host_scal[SC_RR];
+            outer++;
+        }
+        stats.rel_residual = rel(rr);
+        stats.outer_iterations = outer;
+        stats.inner_iterations = outer * opts.inner_iterations;
+    }
+
+    void run_steps(uint32_t n) {
+        if (n == 0) return;
+        if (opts.solver == PXLFLIP_SOLVER_MIXED) {
+            for (uint32_t s = 0; s < n; s++) {
+                replay(&grid_graph, &Sim::enqueue_grid_init);
+                mixed_pressure();
+                replay(&post_graph, &Sim::enqueue_post);
+                steps_done++;
+            }
+        } else {
+            const bool graph = opts.use_graph && !timer.on;
+            if (graph && !step_graph) step_graph = capture(&Sim::enqueue_fixed_step);
+            for (uint32_t s = 0; s < n; s++) {
+                if (graph) HIP_CHECK(hipGraphLaunch(step_graph, stream));
+                else enqueue_fixed_step();
+            }
+            steps_done += n;
+            stats.rel_residual = -1.0;
+            stats.inner_iterations = iters;
+            stats.outer_iterations = 0;
+        }
+        HIP_CHECK(hipStreamSynchronize(stream));
+    }
+
+    void reset() {
+        launch_reset_particles(d, stream);
+        HIP_CHECK(hipMemsetAsync(d.pressure, 0, 4 * (size_t)d.ncells, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        steps_done = 0;
+        initialized = true;
+    }
+};
+
+inline int fail(Sim *s, int code, const std::string &msg) {
+    if (s) s->error = msg;
+    return code;
+}
+
+inline int device_fail(Sim *s, const HipError &e) {
+    return fail(s, PXLFLIP_ERR_DEVICE, std::string("HIP error in ") + e.what + ": " + hipGetErrorString(e.code));
+}
+
+// Runs body() with the handle's device current; maps exceptions to status codes.
+template <typename F>
+inline int guarded(Sim *s, F &&body) {
+    if (!s) return PXLFLIP_ERR_INVALID;
+    try {
+        HIP_CHECK(hipSetDevice(s->device));
+        body();
+        return PXLFLIP_OK;
+    } catch (const HipError &e) {
+        return device_fail(s, e);
+    } catch (const std::bad_alloc &) {
+        return fail(s, PXLFLIP_ERR_OOM, "out of memory");
+    } catch (const std::exception &e) {
+        return fail(s, PXLFLIP_ERR_INTERNAL, e.what());
+    }
+}
+
+struct DeviceBuffer {
+    void *p = nullptr;
+    explicit DeviceBuffer(size_t bytes) {
+        if (hipMalloc(&p, bytes ? bytes : 4) != hipSuccess) throw std::bad_alloc();
+    }
+    ~DeviceBuffer() {
+        if (p) (void)hipFree(p);
+    }
+    template <typename T>
+    T *as() const { return static_cast<T *>(p); }
+};
+
+// Parse + validate + seed + derive the device parameters; no device work.  Returns 0 or an
+// error code with g_create_error set.
+int prepare(const char *fluid_json, double fps, uint32_t seed, const pxlflip_opts *opts, Sim &s,
+            std::vector<float> *x0, std::vector<float> *v0);
+void fill_info(const Sim &s, pxlflip_info *info);
+
+}  // namespace pxl

@@ -41,6 +41,7 @@ namespace {
 
 constexpr int SBLOCK = 1024;  // lanes per solver workgroup (16 wave64)
 constexpr int SWAVES = SBLOCK / WAVE;
+constexpr unsigned BATCH_SGRID = 1024;  // persistent grid of batched solver launches
 
 // Fields of a list entry's stencil word (see DevSim::fl_word).
 __device__ __forceinline__ bool w_fluid(uint32_t w, int n) { return (w >> n) & 1u; }
@@ -64,10 +65,11 @@ __device__ __forceinline__ void nb_offsets(const DevSim &S, int32_t off[6]) {
 // mostly in the same L2.
 constexpr uint32_t NXCD = 8;
 
-// This workgroup's slice of list entries [lo, hi).
-__device__ __forceinline__ void list_slice(uint32_t lo, uint32_t hi, uint32_t &beg, uint32_t &end) {
-    const uint64_t n = hi - lo, g = gridDim.x;
-    uint64_t b = blockIdx.x;
+// Workgroup b's (of nb) slice of list entries [lo, hi).
+__device__ __forceinline__ void list_slice(uint32_t lo, uint32_t hi, uint32_t b_in, uint32_t nb, uint32_t &beg,
+                                           uint32_t &end) {
+    const uint64_t n = hi - lo, g = nb;
+    uint64_t b = b_in;
     if (g % NXCD == 0) b = (b % NXCD) * (g / NXCD) + b / NXCD;
     beg = lo + (uint32_t)(n * b / g);
     end = lo + (uint32_t)(n * (b + 1) / g);
@@ -80,12 +82,11 @@ __device__ __forceinline__ float *partials<float>(const DevSim &S) { return S.pa
 template <>
 __device__ __forceinline__ double *partials<double>(const DevSim &S) { return S.part64; }
 
-// Stores the workgroup sums of NV values at part[k * nb + block] (plain stores: they are
-// consumed by the next kernel on the stream).
+// Stores the sums of NV values of workgroup b (of nb) at part[k * nb + b] (plain stores: they
+// are consumed by the next kernel on the stream).
 template <typename T, int NV>
-__device__ __forceinline__ void store_partials(const T (&v)[NV], T *part) {
+__device__ __forceinline__ void store_partials(const T (&v)[NV], T *part, uint32_t b, uint32_t nb) {
     __shared__ T sh[SWAVES * NV];
-    const uint32_t nb = gridDim.x, b = blockIdx.x;
 #pragma unroll
     for (int k = 0; k < NV; k++) {
         T s = block_sum<T, SWAVES>(v[k], sh + SWAVES * k);
@@ -113,10 +114,10 @@ __device__ __forceinline__ T wave_sum_partials(const T *part, uint32_t nb, int k
 
 // ---- init 1: r0 = b - A x (pressure system, x = warm-started pressure) or, for the inner
 // solve of the refinement, r0 = float(r64) with x = e = 0.
-__global__ __launch_bounds__(SBLOCK) void k_cg_residual(DevSim S, float *__restrict__ x, float *__restrict__ r,
-                                                       int inner) {
+__device__ __forceinline__ void body_cg_residual(const DevSim &S, float *__restrict__ x, float *__restrict__ r,
+                                                 int inner, uint32_t b, uint32_t nb) {
     uint32_t beg, end;
-    list_slice(0, S.fl_count[1], beg, end);
+    list_slice(0, S.fl_count[1], b, nb, beg, end);
     int32_t off[6];
     nb_offsets(S, off);
     for (uint32_t t = beg + threadIdx.x; t < end; t += SBLOCK) {
@@ -132,9 +133,13 @@ __global__ __launch_bounds__(SBLOCK) void k_cg_residual(DevSim S, float *__restr
         for (int n = 0; n < 6; n++)
             if (w_fluid(wd, n)) sum = sum + x[id + off[n]];
         const float ax = w_diag(wd) * x[id] - sum;
-        const float b = -(S.div[id] * S.h * S.h / S.dt);
-        r[id] = b - ax;
+        const float bb = -(S.div[id] * S.h * S.h / S.dt);
+        r[id] = bb - ax;
     }
+}
+__global__ __launch_bounds__(SBLOCK) void k_cg_residual(DevSim S, float *__restrict__ x, float *__restrict__ r,
+                                                       int inner) {
+    body_cg_residual(S, x, r, inner, blockIdx.x, gridDim.x);
 }
 
 // ---- init 2: u0 = M^-1 r0, w0 = A u0; p_{-1} = s_{-1} = 0; partials gamma0, delta0 (slot 0).
@@ -142,7 +147,7 @@ template <typename Acc>
 __global__ __launch_bounds__(SBLOCK) void k_cg_start(DevSim S, const float *__restrict__ r, float *__restrict__ w,
                                                     float *__restrict__ s_prev) {
     uint32_t beg, end;
-    list_slice(0, S.fl_count[1], beg, end);
+    list_slice(0, S.fl_count[1], blockIdx.x, gridDim.x, beg, end);
     int32_t off[6];
     nb_offsets(S, off);
     Acc g = 0, d = 0;
@@ -162,12 +167,13 @@ __global__ __launch_bounds__(SBLOCK) void k_cg_start(DevSim S, const float *__re
         d += (Acc)wc * (Acc)uc;
     }
     const Acc v[2] = {g, d};
-    store_partials<Acc, 2>(v, partials<Acc>(S));
+    store_partials<Acc, 2>(v, partials<Acc>(S), blockIdx.x, gridDim.x);
 }
 
 // Scalars of iteration `it` from the previous pass's partials (identical in every workgroup).
 template <typename Acc>
-__device__ __forceinline__ void cg_scalars(const DevSim &S, uint32_t it, uint32_t nb, float &alpha, float &beta) {
+__device__ __forceinline__ void cg_scalars(const DevSim &S, uint32_t it, uint32_t b_self, uint32_t nb, float &alpha,
+                                           float &beta) {
     const Acc *part = partials<Acc>(S) + (size_t)(it & 1) * 2 * nb;
     const Acc g = wave_sum_partials<Acc>(part, nb, 0);
     const Acc d = wave_sum_partials<Acc>(part, nb, 1);
@@ -183,7 +189,7 @@ __device__ __forceinline__ void cg_scalars(const DevSim &S, uint32_t it, uint32_
             a = den != (Acc)0 ? g / den : (Acc)0;
         }
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (b_self == 0 && threadIdx.x == 0) {
         double *next = S.scal + SC_COUNT * ((it + 1) & 1);
         next[SC_ALPHA] = (double)a;
         next[SC_BETA] = (double)b;
@@ -203,12 +209,12 @@ __global__ __launch_bounds__(SBLOCK) void k_cg_iter(DevSim S, uint32_t it, float
                                                    const float *__restrict__ w_in, float *__restrict__ w_out,
                                                    const float *__restrict__ s_prev, float *__restrict__ s_cur) {
     uint32_t beg, end;
-    list_slice(0, S.fl_count[1], beg, end);
+    list_slice(0, S.fl_count[1], blockIdx.x, gridDim.x, beg, end);
     const float *__restrict__ pv = S.cg_p;
     int32_t off[6];
     nb_offsets(S, off);
     float alpha, beta;
-    cg_scalars<Acc>(S, it, gridDim.x, alpha, beta);
+    cg_scalars<Acc>(S, it, blockIdx.x, gridDim.x, alpha, beta);
     Acc g = 0, d = 0;
     for (uint32_t t = beg + threadIdx.x; t < end; t += SBLOCK) {
         const uint32_t c = S.fl_list[t];
@@ -245,7 +251,7 @@ __global__ __launch_bounds__(SBLOCK) void k_cg_iter(DevSim S, uint32_t it, float
         d += (Acc)wn * (Acc)un;
     }
     const Acc v[2] = {g, d};
-    store_partials<Acc, 2>(v, partials<Acc>(S) + (size_t)((it + 1) & 1) * 2 * gridDim.x);
+    store_partials<Acc, 2>(v, partials<Acc>(S) + (size_t)((it + 1) & 1) * 2 * gridDim.x, blockIdx.x, gridDim.x);
 }
 
 // ---- multigrid-preconditioned CG (Chronopoulos-Gear with u = M r from the V-cycle).
@@ -253,9 +259,9 @@ __global__ __launch_bounds__(SBLOCK) void k_cg_iter(DevSim S, uint32_t it, float
 // spmv: w = A u and the partials of gamma = (r, u), delta = (w, u) into slot `slot`
 // (first: also p_{-1} = s_{-1} = 0).
 template <typename Acc>
-__global__ __launch_bounds__(SBLOCK) void k_mgcg_spmv(DevSim S, uint32_t slot, int first) {
+__device__ __forceinline__ void body_mgcg_spmv(const DevSim &S, uint32_t slot, int first, uint32_t b, uint32_t nb) {
     uint32_t beg, end;
-    list_slice(0, S.fl_count[1], beg, end);
+    list_slice(0, S.fl_count[1], b, nb, beg, end);
     int32_t off[6];
     nb_offsets(S, off);
     const float *__restrict__ u = S.cg_r1;
@@ -278,16 +284,17 @@ __global__ __launch_bounds__(SBLOCK) void k_mgcg_spmv(DevSim S, uint32_t slot, i
         d += (Acc)wc * (Acc)uc;
     }
     const Acc v[2] = {g, d};
-    store_partials<Acc, 2>(v, partials<Acc>(S) + (size_t)slot * 2 * gridDim.x);
+    store_partials<Acc, 2>(v, partials<Acc>(S) + (size_t)slot * 2 * nb, b, nb);
 }
 
 // update of iteration it: scalars from slot it & 1, then p, s, x, r (no neighbours).
 template <typename Acc>
-__global__ __launch_bounds__(SBLOCK) void k_mgcg_update(DevSim S, uint32_t it, float *__restrict__ x) {
+__device__ __forceinline__ void body_mgcg_update(const DevSim &S, uint32_t it, float *__restrict__ x, uint32_t b,
+                                                 uint32_t nb) {
     uint32_t beg, end;
-    list_slice(0, S.fl_count[1], beg, end);
+    list_slice(0, S.fl_count[1], b, nb, beg, end);
     float alpha, beta;
-    cg_scalars<Acc>(S, it, gridDim.x, alpha, beta);
+    cg_scalars<Acc>(S, it, b, nb, alpha, beta);
     for (uint32_t t = beg + threadIdx.x; t < end; t += SBLOCK) {
         const uint32_t c = S.fl_list[t];
         const float pc = S.cg_r1[c] + beta * S.cg_p[c];
@@ -298,6 +305,14 @@ __global__ __launch_bounds__(SBLOCK) void k_mgcg_update(DevSim S, uint32_t it, f
         S.cg_r0[c] = S.cg_r0[c] - alpha * sc;
     }
 }
+template <typename Acc>
+__global__ __launch_bounds__(SBLOCK) void k_mgcg_spmv(DevSim S, uint32_t slot, int first) {
+    body_mgcg_spmv<Acc>(S, slot, first, blockIdx.x, gridDim.x);
+}
+template <typename Acc>
+__global__ __launch_bounds__(SBLOCK) void k_mgcg_update(DevSim S, uint32_t it, float *__restrict__ x) {
+    body_mgcg_update<Acc>(S, it, x, blockIdx.x, gridDim.x);
+}
 
 // ---- FP64 outer loop.  The solution is double-buffered (x_src -> x_dst) so the update pass
 // can rebuild neighbours' x_src + e from values no lane has overwritten.
@@ -306,9 +321,9 @@ __device__ __forceinline__ double b64_of(const DevSim &S, uint32_t id) {
 }
 
 // init: x64 = pressure, r64 = b - A x64; partials of rr and bb.
-__global__ __launch_bounds__(SBLOCK) void k_mixed_init(DevSim S, double *__restrict__ x64) {
+__device__ __forceinline__ void body_mixed_init(const DevSim &S, double *__restrict__ x64, uint32_t b_, uint32_t nb) {
     uint32_t beg, end;
-    list_slice(0, S.fl_count[1], beg, end);
+    list_slice(0, S.fl_count[1], b_, nb, beg, end);
     int32_t off[6];
     nb_offsets(S, off);
     double rr = 0, bb = 0;
@@ -328,14 +343,19 @@ __global__ __launch_bounds__(SBLOCK) void k_mixed_init(DevSim S, double *__restr
         bb += b * b;
     }
     const double v[2] = {rr, bb};
-    store_partials<double, 2>(v, S.part64);
+    store_partials<double, 2>(v, S.part64, b_, nb);
+}
+__global__ __launch_bounds__(SBLOCK) void k_mixed_init(DevSim S, double *__restrict__ x64) {
+    body_mixed_init(S, x64, blockIdx.x, gridDim.x);
 }
 
 // update: x_dst = x_src + e (FP64) at the cell and its neighbours, r64 = b - A x_dst; rr.
-__global__ __launch_bounds__(SBLOCK) void k_mixed_update(DevSim S, const double *__restrict__ xsrc,
-                                                        double *__restrict__ xdst, const float *__restrict__ e) {
+// The pressure field gets float(x_dst), so it holds the latest iterate after every pass.
+__device__ __forceinline__ void body_mixed_update(const DevSim &S, const double *__restrict__ xsrc,
+                                                  double *__restrict__ xdst, const float *__restrict__ e, uint32_t b,
+                                                  uint32_t nb) {
     uint32_t beg, end;
-    list_slice(0, S.fl_count[1], beg, end);
+    list_slice(0, S.fl_count[1], b, nb, beg, end);
     int32_t off[6];
     nb_offsets(S, off);
     double rr = 0;
@@ -353,31 +373,52 @@ __global__ __launch_bounds__(SBLOCK) void k_mixed_update(DevSim S, const double 
         const double xc = xsrc[id] + (double)e[id];
         const double r = b64_of(S, id) - ((double)w_diag(wd) * xc - sum);
         xdst[id] = xc;
+        S.pressure[id] = (float)xc;
         S.r64[id] = r;
         rr += r * r;
     }
     const double v[1] = {rr};
-    store_partials<double, 1>(v, S.part64);
+    store_partials<double, 1>(v, S.part64, b, nb);
+}
+__global__ __launch_bounds__(SBLOCK) void k_mixed_update(DevSim S, const double *__restrict__ xsrc,
+                                                        double *__restrict__ xdst, const float *__restrict__ e) {
+    body_mixed_update(S, xsrc, xdst, e, blockIdx.x, gridDim.x);
 }
 
-// Totals of the refinement norms into scal[SC_RR] (and scal[SC_BB] when nv = 2).
-__global__ __launch_bounds__(BLOCK) void k_norms_finalize(DevSim S, uint32_t nb, int nv) {
+// Totals of the refinement norms into norms[0] = rr (and norms[1] = bb when nv = 2).
+__device__ __forceinline__ void body_norms_finalize(const DevSim &S, uint32_t nb, int nv) {
     __shared__ double sh[8];
     const double rr = sum_partials<double>(S.part64, nb, 0, sh);
     double bb = 0;
     if (nv > 1) bb = sum_partials<double>(S.part64, nb, 1, sh + 4);
     if (threadIdx.x != 0) return;
-    S.scal[SC_RR] = rr;
-    if (nv > 1) S.scal[SC_BB] = bb;
+    S.norms[0] = rr;
+    if (nv > 1) S.norms[1] = bb;
+}
+__global__ __launch_bounds__(BLOCK) void k_norms_finalize(DevSim S, uint32_t nb, int nv) {
+    body_norms_finalize(S, nb, nv);
 }
 
-__global__ __launch_bounds__(SBLOCK) void k_mixed_finish(DevSim S, const double *__restrict__ x64) {
-    uint32_t beg, end;
-    list_slice(0, S.fl_count[1], beg, end);
-    for (uint32_t t = beg + threadIdx.x; t < end; t += SBLOCK) {
-        const uint32_t id = S.fl_list[t];
-        S.pressure[id] = (float)x64[id];
-    }
+// ---- batched (the solving simulations of a batch; x = e, r = cg_r0)
+__global__ __launch_bounds__(SBLOCK) void kb_cg_residual(WorkList W) {
+    for_each_work(W, [&](const DevSim &S, uint32_t b) { body_cg_residual(S, S.e, S.cg_r0, 1, b, S.cg_blocks); });
+}
+__global__ __launch_bounds__(SBLOCK) void kb_mgcg_spmv(WorkList W, uint32_t slot, int first) {
+    for_each_work(W, [&](const DevSim &S, uint32_t b) { body_mgcg_spmv<double>(S, slot, first, b, S.cg_blocks); });
+}
+__global__ __launch_bounds__(SBLOCK) void kb_mgcg_update(WorkList W, uint32_t it) {
+    for_each_work(W, [&](const DevSim &S, uint32_t b) { body_mgcg_update<double>(S, it, S.e, b, S.cg_blocks); });
+}
+__global__ __launch_bounds__(SBLOCK) void kb_mixed_init(WorkList W) {
+    for_each_work(W, [&](const DevSim &S, uint32_t b) { body_mixed_init(S, S.x64a, b, S.cg_blocks); });
+}
+__global__ __launch_bounds__(SBLOCK) void kb_mixed_update(WorkList W, int cur) {
+    for_each_work(W, [&](const DevSim &S, uint32_t b) {
+        body_mixed_update(S, cur ? S.x64b : S.x64a, cur ? S.x64a : S.x64b, S.e, b, S.cg_blocks);
+    });
+}
+__global__ __launch_bounds__(BLOCK) void kb_norms_finalize(WorkList W, int nv) {
+    for_each_work(W, [&](const DevSim &S, uint32_t) { body_norms_finalize(S, (uint32_t)S.cg_blocks, nv); });
 }
 
 // ---- pressure_rb (fluid.slang:168-182): one red-black half sweep over the cells of one
@@ -386,8 +427,8 @@ __global__ __launch_bounds__(SBLOCK) void k_mixed_finish(DevSim S, const double 
 __global__ __launch_bounds__(SBLOCK) void k_rbgs(DevSim S, uint32_t parity) {
     const uint32_t n_red = S.fl_count[0], n = S.fl_count[1];
     uint32_t beg, end;
-    if (parity == 0) list_slice(0, n_red, beg, end);
-    else list_slice(n_red, n, beg, end);
+    if (parity == 0) list_slice(0, n_red, blockIdx.x, gridDim.x, beg, end);
+    else list_slice(n_red, n, blockIdx.x, gridDim.x, beg, end);
     int32_t off[6];
     nb_offsets(S, off);
     float *p = S.pressure;
@@ -452,12 +493,31 @@ void launch_mixed_update(const DevSim &S, const double *xsrc, double *xdst, hipS
     k_norms_finalize<<<1, BLOCK, 0, s>>>(S, solver_blocks(S), 1);
 }
 
-void launch_mixed_finish(const DevSim &S, const double *x64, hipStream_t s) {
-    k_mixed_finish<<<solver_blocks(S), SBLOCK, 0, s>>>(S, x64);
-}
 
 void launch_rbgs(const DevSim &S, uint32_t parity, hipStream_t s) {
     k_rbgs<<<solver_blocks(S), SBLOCK, 0, s>>>(S, parity);
+}
+
+// ---- batched
+void batch_mixed_init(const WorkLists &W, hipStream_t s) {
+    kb_mixed_init<<<BATCH_SGRID, SBLOCK, 0, s>>>(W[WD_A_SOLVER]);
+    kb_norms_finalize<<<BATCH_SGRID, BLOCK, 0, s>>>(W[WD_A_SIM], 2);
+}
+
+// One refinement pass for the solving simulations: MG-CG from e = 0 on float(r64), then the
+// FP64 update x64[cur ^ 1] = x64[cur] + e and the residual norm.
+void batch_inner_pass(const WorkLists &W, int cur, uint32_t inner, int max_tail, hipStream_t s) {
+    kb_cg_residual<<<BATCH_SGRID, SBLOCK, 0, s>>>(W[WD_A_SOLVER]);
+    batch_mg_vcycle(W, max_tail, s);
+    kb_mgcg_spmv<<<BATCH_SGRID, SBLOCK, 0, s>>>(W[WD_A_SOLVER], 0, 1);
+    for (uint32_t it = 0; it < inner; it++) {
+        kb_mgcg_update<<<BATCH_SGRID, SBLOCK, 0, s>>>(W[WD_A_SOLVER], it);
+        if (it + 1 == inner) break;
+        batch_mg_vcycle(W, max_tail, s);
+        kb_mgcg_spmv<<<BATCH_SGRID, SBLOCK, 0, s>>>(W[WD_A_SOLVER], (it + 1) & 1, 0);
+    }
+    kb_mixed_update<<<BATCH_SGRID, SBLOCK, 0, s>>>(W[WD_A_SOLVER], cur);
+    kb_norms_finalize<<<BATCH_SGRID, BLOCK, 0, s>>>(W[WD_A_SIM], 1);
 }
 
 }  // namespace pxl

@@ -84,6 +84,45 @@ class SolveStats(C.Structure):
                 ("outer_iterations", C.c_uint32)]
 
 
+class ServerOpts(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_uint32), ("device", C.c_int32), ("budget_bytes", C.c_uint64),
+        ("max_sims", C.c_uint32), ("inner_iterations", C.c_uint32), ("max_outer", C.c_uint32),
+        ("tolerance", C.c_double), ("solver_workgroups", C.c_uint32), ("use_graph", C.c_int32),
+    ]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32), ("steps", C.c_uint32), ("steps_done", C.c_uint32), ("n_particles", C.c_uint32),
+        ("device_bytes", C.c_uint64), ("queue_ms", C.c_double), ("latency_ms", C.c_double),
+        ("rel_residual", C.c_double), ("max_outer", C.c_uint32), ("pad", C.c_uint32),
+    ]
+
+    def as_dict(self) -> dict:
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+        d["status"] = REQUEST_STATUS[self.status]
+        return d
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
+        ("cancelled", C.c_uint64), ("queued", C.c_uint64), ("running", C.c_uint64),
+        ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("max_ms", C.c_double),
+        ("particle_steps", C.c_double), ("span_s", C.c_double), ("throughput", C.c_double),
+        ("busy_s", C.c_double), ("batch_steps", C.c_uint64), ("mean_batch", C.c_double),
+        ("bytes_in_use", C.c_uint64), ("peak_bytes", C.c_uint64), ("admit_s", C.c_double),
+        ("rebuilds", C.c_uint64), ("graph_captures", C.c_uint64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+REQUEST_STATUS = {0: "queued", 1: "running", 2: "done", 3: "failed", 4: "cancelled"}
+
+
 # Exported symbols with their ctypes signatures (must match include/pxlflip.h).
 _H = C.c_void_p
 _SIGNATURES = {
@@ -109,6 +148,19 @@ _SIGNATURES = {
     "pxlflip_profile_steps": (C.c_int, [_H, C.c_uint32, C.POINTER(KernelTime), C.c_uint32, C.POINTER(C.c_uint32)]),
     "pxlflip_autotune": (C.c_int, [_H, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_double)]),
     "pxlflip_last_error": (C.c_char_p, [_H]),
+    "pxlflip_server_default_opts": (C.c_int, [C.POINTER(ServerOpts)]),
+    "pxlflip_server_create": (C.c_int, [C.POINTER(ServerOpts), C.POINTER(_H)]),
+    "pxlflip_server_submit": (C.c_int, [_H, C.c_char_p, C.c_double, C.c_uint32, C.c_uint32, C.c_int32,
+                                        C.POINTER(C.c_uint64)]),
+    "pxlflip_server_poll": (C.c_int, [_H, C.c_uint64, C.POINTER(RequestInfo)]),
+    "pxlflip_server_wait": (C.c_int, [_H, C.c_uint64, C.c_double, C.POINTER(RequestInfo)]),
+    "pxlflip_server_drain": (C.c_int, [_H, C.c_double]),
+    "pxlflip_server_cancel": (C.c_int, [_H, C.c_uint64]),
+    "pxlflip_server_read_particles": (C.c_int, [_H, C.c_uint64, C.c_void_p, C.c_uint64]),
+    "pxlflip_server_release": (C.c_int, [_H, C.c_uint64]),
+    "pxlflip_server_get_stats": (C.c_int, [_H, C.POINTER(ServerStats)]),
+    "pxlflip_server_last_error": (C.c_char_p, [_H]),
+    "pxlflip_server_destroy": (None, [_H]),
 }
 
 _lib = None
@@ -303,3 +355,94 @@ class FluidState:
         self._call(self._lib.pxlflip_last_solve, C.byref(st))
         return {"rel_residual": st.rel_residual, "inner_iterations": int(st.inner_iterations),
                 "outer_iterations": int(st.outer_iterations)}
+
+
+def make_server_opts(device=0, budget_bytes=200_000_000_000, max_sims=256, inner_iterations=8, max_outer=50,
+                     tolerance=1e-6, solver_workgroups=0, use_graph=True) -> ServerOpts:
+    o = ServerOpts()
+    load_library().pxlflip_server_default_opts(C.byref(o))
+    o.device = device
+    o.budget_bytes = int(budget_bytes)
+    o.max_sims = max_sims
+    o.inner_iterations = inner_iterations
+    o.max_outer = max_outer
+    o.tolerance = tolerance
+    o.solver_workgroups = solver_workgroups
+    o.use_graph = 1 if use_graph else 0
+    return o
+
+
+class Server:
+    """Continuous-batching server of independent liquids (pxlflip_server_* in pxlflip.h).
+
+    ``submit`` validates a fluid block and queues ``steps`` steps of it; a device thread admits
+    queued requests while their HBM fits ``budget_bytes``, steps every resident simulation
+    together with batched launches (mixed-precision solve), and retires each after its last
+    step.  Each simulation's result equals its result alone.
+    """
+
+    def __init__(self, **opts):
+        self._lib = load_library()
+        self._h = C.c_void_p()
+        o = make_server_opts(**opts)
+        rc = self._lib.pxlflip_server_create(C.byref(o), C.byref(self._h))
+        if rc != OK:
+            raise FluidError(rc, self._lib.pxlflip_server_last_error(None).decode())
+
+    def _check(self, rc: int):
+        if rc != OK:
+            msg = self._lib.pxlflip_server_last_error(self._h)
+            raise FluidError(rc, msg.decode() if msg else "")
+
+    def submit(self, fluid, fps: float = 24.0, seed: int = 0, steps: int = 1, keep_particles: bool = False) -> int:
+        rid = C.c_uint64()
+        self._check(self._lib.pxlflip_server_submit(self._h, _json(fluid), fps, seed, steps,
+                                                    1 if keep_particles else 0, C.byref(rid)))
+        return int(rid.value)
+
+    def poll(self, rid: int) -> dict:
+        info = RequestInfo()
+        self._check(self._lib.pxlflip_server_poll(self._h, rid, C.byref(info)))
+        return info.as_dict()
+
+    def wait(self, rid: int, timeout: float = -1.0) -> dict:
+        info = RequestInfo()
+        self._check(self._lib.pxlflip_server_wait(self._h, rid, timeout, C.byref(info)))
+        return info.as_dict()
+
+    def drain(self, timeout: float = -1.0):
+        self._check(self._lib.pxlflip_server_drain(self._h, timeout))
+
+    def cancel(self, rid: int):
+        self._check(self._lib.pxlflip_server_cancel(self._h, rid))
+
+    def read_particles(self, rid: int) -> np.ndarray:
+        n = self.poll(rid)["n_particles"]
+        out = np.zeros((n, 8), np.float32)
+        self._check(self._lib.pxlflip_server_read_particles(self._h, rid, out.ctypes.data, n))
+        return out
+
+    def release(self, rid: int):
+        self._check(self._lib.pxlflip_server_release(self._h, rid))
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        self._check(self._lib.pxlflip_server_get_stats(self._h, C.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if self._h:
+            self._lib.pxlflip_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -57,3 +57,20 @@ def aos8(x: np.ndarray, v: np.ndarray) -> np.ndarray:
     out[:, 0:3] = x
     out[:, 4:7] = v
     return out
+
+
+def random_box(index: int, sizes=(32, 48, 64, 96, 128), ppc_range=(4, 8)) -> dict:
+    """A serving request: cubic grid n^3 with n drawn from `sizes`, one liquid box at a random
+    place and size inside the interior, ppc drawn from `ppc_range`, a random initial velocity.
+    Deterministic in `index`."""
+    rng = np.random.default_rng(1000 + index)
+    n = int(rng.choice(sizes))
+    lo = rng.uniform(0.05, 0.45, 3)
+    ext = rng.uniform(0.2, 0.5, 3)
+    hi = np.minimum(lo + ext, 0.95)
+    vel = rng.uniform(-1.0, 1.0, 3)
+    return {
+        "name": f"req{index}", "bounds": [[0, 0, 0], [1, 1, 1]], "resolution": n,
+        "fill": [{"min": [float(a) for a in lo], "max": [float(a) for a in hi], "velocity": [float(a) for a in vel]}],
+        "particles_per_cell": int(rng.integers(ppc_range[0], ppc_range[1] + 1)), "substeps": 5, "flip": 0.95,
+    }

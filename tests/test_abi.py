@@ -154,3 +154,21 @@ def test_opts_validation(pxl):
         pxl.check(dam_break(32), solver=7)
     with pytest.raises(pxl.FluidError):
         pxl.check(dam_break(32), solver="mixed", tolerance=0.0)
+
+
+def test_server_options_are_validated_before_the_device(pxl):
+    with pytest.raises(pxl.FluidError) as e:
+        pxl.Server(max_sims=0)
+    assert e.value.code == pxl.flip.ERR_INVALID and "max_sims" in str(e.value)
+    with pytest.raises(pxl.FluidError) as e:
+        pxl.Server(tolerance=0.0)
+    assert e.value.code == pxl.flip.ERR_INVALID
+
+
+def test_server_without_a_device_fails_cleanly(pxl):
+    try:
+        srv = pxl.Server()
+    except pxl.FluidError as e:  # no GPU in this process
+        assert e.code == pxl.flip.ERR_DEVICE and str(e)
+    else:
+        srv.close()

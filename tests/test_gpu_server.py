@@ -1,0 +1,125 @@
+"""Batched serving on the GPU: every simulation's result in a batch equals its result alone
+(bitwise: same kernels, same workgroup partition, same reduction order, its own convergence),
+each pressure solve reaches the FP64 tolerance, admission respects the HBM budget, requests
+arriving while others run join the batch, and cancellation retires a running request."""
+import time
+
+import numpy as np
+import pytest
+
+from tests.scenes import FPS, dam_break, random_box, two_blocks
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pxl():
+    import pxlengine_amd
+    return pxlengine_amd
+
+WG = 32  # solver workgroups per simulation, fixed so the single-sim path can use the same partition
+
+
+def _server(pxl, **kw):
+    kw.setdefault("solver_workgroups", WG)
+    return pxl.Server(**kw)
+
+
+def _run(pxl, reqs, **kw):
+    """reqs: [(fluid, steps)] submitted together; returns ([particles], [info], stats)."""
+    with _server(pxl, **kw) as srv:
+        ids = [srv.submit(f, FPS, seed=0, steps=n, keep_particles=True) for f, n in reqs]
+        infos = [srv.wait(i, timeout=120) for i in ids]
+        parts = [srv.read_particles(i) for i in ids]
+        return parts, infos, srv.stats()
+
+
+def test_batched_equals_single_sim_path(pxl):
+    f = dam_break(24)
+    (p_batch,), (info,), _ = _run(pxl, [(f, 3)])
+    assert info["status"] == "done" and info["rel_residual"] <= 1e-6
+    sim = pxl.FluidState(f, FPS, seed=0, solver="mixed", solver_workgroups=WG)
+    sim.step(3)
+    assert np.array_equal(p_batch, sim.read_particles())
+    sim.close()
+
+
+def test_alone_equals_in_the_mix(pxl):
+    a, b, c = dam_break(32), two_blocks(24), random_box(3, sizes=(16, 40))
+    (pa,), _, _ = _run(pxl, [(a, 4)])
+    parts, infos, st = _run(pxl, [(b, 2), (a, 4), (c, 5)])
+    for info in infos:
+        assert info["status"] == "done"
+        assert 0 <= info["rel_residual"] <= 1e-6
+    assert np.array_equal(pa, parts[1])
+    assert st["completed"] == 3 and st["mean_batch"] > 1.0
+    (pc,), _, _ = _run(pxl, [(c, 5)])
+    assert np.array_equal(pc, parts[2])
+
+
+def test_many_small_requests_complete_with_stats(pxl):
+    reqs = [(random_box(i, sizes=(16, 24, 32)), 2) for i in range(6)]
+    with _server(pxl) as srv:
+        ids = [srv.submit(f, FPS, seed=0, steps=n) for f, n in reqs]
+        srv.drain(timeout=300)
+        st = srv.stats()
+        assert st["completed"] == len(reqs) and st["failed"] == 0
+        assert st["p50_ms"] > 0 and st["p99_ms"] >= st["p50_ms"]
+        assert st["particle_steps"] > 0 and st["throughput"] > 0
+        for i in ids:
+            info = srv.poll(i)
+            assert info["status"] == "done" and info["rel_residual"] <= 1e-6
+
+
+def test_budget_serialises_admission(pxl):
+    f = dam_break(24)
+    with _server(pxl) as probe:
+        rid = probe.submit(f, FPS, steps=0)
+        probe.wait(rid, timeout=60)
+        one = probe.poll(rid)["device_bytes"]
+    with _server(pxl, budget_bytes=one) as srv:
+        ids = [srv.submit(f, FPS, seed=s, steps=2) for s in range(3)]
+        srv.drain(timeout=300)
+        st = srv.stats()
+        assert st["completed"] == 3 and st["peak_bytes"] <= one
+        assert abs(st["mean_batch"] - 1.0) < 1e-9  # one at a time
+        for i in ids:
+            assert srv.poll(i)["status"] == "done"
+    with _server(pxl, budget_bytes=one - 1) as srv:
+        with pytest.raises(pxl.FluidError) as e:
+            srv.submit(f, FPS, steps=1)
+        assert "budget" in str(e.value)
+
+
+def test_late_arrival_joins_and_finishes_first(pxl):
+    long_f, short_f = dam_break(32), two_blocks(16)
+    (p_short_alone,), _, _ = _run(pxl, [(short_f, 2)])
+    with _server(pxl) as srv:
+        a = srv.submit(long_f, FPS, steps=40)
+        t0 = time.time()
+        while srv.poll(a)["steps_done"] < 2 and time.time() - t0 < 60:
+            time.sleep(0.005)
+        assert srv.poll(a)["status"] == "running"
+        b = srv.submit(short_f, FPS, steps=2, keep_particles=True)
+        ib = srv.wait(b, timeout=120)
+        assert ib["status"] == "done"
+        ia = srv.poll(a)
+        assert ia["status"] == "running" and ia["steps_done"] < 40
+        assert np.array_equal(srv.read_particles(b), p_short_alone)
+        srv.wait(a, timeout=300)
+        assert srv.stats()["mean_batch"] > 1.0
+
+
+def test_cancel_and_invalid_request(pxl):
+    with _server(pxl) as srv:
+        a = srv.submit(dam_break(32), FPS, steps=100000)
+        t0 = time.time()
+        while srv.poll(a)["steps_done"] < 1 and time.time() - t0 < 60:
+            time.sleep(0.005)
+        srv.cancel(a)
+        info = srv.wait(a, timeout=60)
+        assert info["status"] == "cancelled"
+        with pytest.raises(pxl.FluidError) as e:
+            srv.submit({"name": "bad", "resolution": 8}, FPS, steps=1)
+        assert "missing field `bounds`" in str(e.value)
+        assert srv.stats()["cancelled"] == 1
