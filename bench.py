@@ -1,15 +1,28 @@
 #!/usr/bin/env python3
-"""Throughput of the FLIP liquid step on one MI355X: particle-steps/s.
+"""Particle-steps/s of batched mixed-precision FLIP serving vs the plain one-at-a-time FP32
+path on one MI355X, with p50/p99 request latency.
 
-Workload: the single-instance dam break of the baseline configs -- unit cube at 128^3
-(h = 1/128), liquid block x < 0.4, y < 0.6 seeded at 8 particles per cell (3,830,400
-particles, fp32, at rest, seed 0), gravity -9.81, dt = f32(1/120), PIC/FLIP 0.95, and the
-plain FP32 pressure CG with a fixed 80 iterations.  A step is one whole FLIP step of that
-simulation (bin, P2G, grid ops, pressure, project, G2P) with its state resident in HBM.
+Default workload (`--workload serve`): a stream of independent liquid simulations.  Request i
+is a cubic grid n^3 on the unit cube with n drawn from {32, 48, 64, 96, 128}, one liquid box
+of random place and size in the interior with a random initial velocity, 4-8 particles per
+cell (fp32 SoA state), dt = f32(1/120), PIC/FLIP 0.95, seeded with seed = i, and `--steps`
+steps.  Requests arrive as a Poisson process (rate: `--rate`, or 1.25x the service rate measured on
+a warm-up batch, so the device stays loaded) at a server with a 200 GB HBM admission budget.
+Each simulation's pressure solve runs to a relative FP64 residual of 1e-6 (multigrid CG in
+FP32 inside FP64 refinement).  `value` = sum over requests of particles x steps, divided by
+the time from the first submission to the last completion (whole job, inputs created on the
+device at admission).
 
-Prints one JSON line.  `roofline` is for the kernel with the largest share of step time,
-from HIP events bracketing each launch on the simulation's stream; `cpu_baseline` times
-the CPU oracle (single thread) on a bounded sample of the same workload.
+`plain` runs a sample of the same requests one at a time through the single-simulation path
+with the FP32 pressure CG at a fixed iteration count (the reference's default 80), creation
+included, and `ratio_vs_plain` = value / plain.
+
+`roofline` is for the kernel with the largest share of time in a profiled batched step of a
+fixed batch of stream requests (HIP events on the batch stream, launches priced by their
+algorithmic bytes over the simulations each covers).  `cpu_baseline` times the C oracle
+(single thread) on a bounded sample of the stream's requests.
+
+`--workload dam` measures the single-instance 128^3 dam break on the plain path instead.
 """
 from __future__ import annotations
 
@@ -24,11 +37,12 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
+METRIC = "particle-steps/s batched vs plain at ≤1e-6 rel residual; p50/p99 sim latency"
 
 
-def algorithmic_bytes(name: str, P: int, F: int, C: int, Cf: int) -> float:
-    """Minimal bytes one launch of a kernel must move: every logical array once, at its
-    element width (fp32/int32 4 B, cell type 1 B, stencil 2 B, particle x+v 24 B)."""
+def plain_algorithmic_bytes(name: str, P: int, F: int, C: int, Cf: int) -> float:
+    """Minimal bytes one single-sim launch must move: every logical array once, at its element
+    width (fp32/int32 4 B, cell type 1 B, stencil 2 B, particle x+v 24 B)."""
     table = {
         "clear_grid": 8 * F + 4 * C,
         "bin_count": 12 * P + 8 * P + 4 * C,
@@ -46,29 +60,209 @@ def algorithmic_bytes(name: str, P: int, F: int, C: int, Cf: int) -> float:
     return float(table.get(name, 0))
 
 
-def measured_traffic(kernel: str, args):
+def measured_traffic(kernel: str, workload: str):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
     summary of this workload (profiles/), or None when there is none for it."""
-    path = os.path.join(ROOT, "profiles", f"traffic_dam{args.res}_{args.solver}{args.iters}.json")
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if not os.path.exists(path):
         return None
     per = json.load(open(path))["per_dispatch"]
-    for key in (f"k_{kernel}", f"k_{kernel}<float>", f"k_{kernel}_tile"):
+    for key in (kernel, f"k_{kernel}", f"kb_{kernel}", f"k_{kernel}<float>", f"k_{kernel}_tile", f"kb_{kernel}_tile"):
         if key in per:
             return per[key]["hbm_bytes"]
     return None
 
 
+def bench_dam(args, pxl):
+    from tests.scenes import FPS, dam_break
+    fluid = dam_break(128, iters=80)
+    sim = pxl.FluidState(fluid, FPS, seed=0, solver="cg", cg_iterations=80)
+    P = sim.particle_count()
+    info = sim.info
+    for _ in range(args.warmup):
+        sim.step(1)
+    tuned = sim.autotune(2)
+    t0 = time.perf_counter()
+    sim.step(args.steps)
+    elapsed = time.perf_counter() - t0
+    prof = sim.profile_steps(2)
+    Cf = int((sim.read_field("type") == 1).sum())
+    F, Cc = info["n_faces"], info["n_cells"]
+    kernels = []
+    for name, launches, total_ms in prof:
+        avg_s = total_ms / 1e3 / launches
+        b = plain_algorithmic_bytes(name, P, F, Cc, Cf)
+        kernels.append({"kernel": name, "launches_per_step": launches // 2, "avg_us": avg_s * 1e6,
+                        "alg_bytes": b, "achieved_GBs": (b / avg_s / 1e9) if b else None})
+    dom = kernels[0]
+    ach = dom["achieved_GBs"] or 0.0
+    sim.close()
+    cpu = cpu_dam(args, fluid) if args.cpu_seconds > 0 else None
+    return {
+        "metric": METRIC, "value": P * args.steps / elapsed, "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded dam break, reference LCG seeding)",
+        "config": {"workload": "dam break 128^3, single instance, plain FP32 Jacobi-PCG, 80 fixed iterations",
+                   "particles": P, "autotune": tuned},
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": ach / PEAK_HBM_GBS, "traffic": measured_traffic(dom["kernel"], "dam128_cg80"),
+                     "kernel": dom["kernel"], "avg_us": dom["avg_us"], "alg_bytes_per_launch": dom["alg_bytes"]},
+        "cpu_baseline": cpu, "kernels": kernels,
+    }
+
+
+def cpu_dam(args, fluid):
+    import oracle
+    from tests.scenes import oracle_sim
+    osim = oracle_sim(oracle, fluid)
+    t0 = time.perf_counter()
+    n = 0
+    while n == 0 or (time.perf_counter() - t0 < args.cpu_seconds and n < 4):
+        osim.step_cg(80)
+        n += 1
+    dt = time.perf_counter() - t0
+    return {"value": osim.P * n / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"first {n} steps of the 128^3 dam break ({osim.P} particles), FP32 CG 80, C oracle, one thread"}
+
+
+def bench_serve(args, pxl):
+    import numpy as np
+    from tests.scenes import FPS, random_box
+
+    sizes = tuple(int(v) for v in args.sizes.split(","))
+    reqs = [random_box(i, sizes=sizes) for i in range(args.requests)]
+    opts = dict(budget_bytes=args.budget)
+    infos = [pxl.check(f, FPS, seed=i, solver="mixed") for i, f in enumerate(reqs)]
+    P = np.array([inf["n_particles"] for inf in infos], dtype=np.float64)
+    work = P * args.steps
+
+    # warm-up (untimed): device init, kernels, and the service rate for the arrival process
+    warm = [random_box(100000 + i, sizes=sizes) for i in range(args.warmup)]
+    with pxl.Server(**opts) as srv:
+        for i, f in enumerate(warm):
+            srv.submit(f, FPS, seed=i, steps=max(2, args.steps // 4))
+        srv.drain()
+        wst = srv.stats()
+    rate = args.rate
+    if rate <= 0:
+        # requests per second the batch sustained in the warm-up, scaled to this stream's work
+        # with 25% headroom so the server always has queued work
+        rate = 1.25 * wst["throughput"] / max(work.mean(), 1.0) if wst["throughput"] > 0 else 100.0
+    rng = np.random.default_rng(12345)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(reqs)))
+    arrivals -= arrivals[0]
+
+    # timed: the stream through the continuous-batching server
+    with pxl.Server(**opts) as srv:
+        t0 = time.perf_counter()
+        ids = []
+        for i, f in enumerate(reqs):
+            delay = arrivals[i] - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            ids.append(srv.submit(f, FPS, seed=i, steps=args.steps))
+        srv.drain()
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+        per = [srv.poll(i) for i in ids]
+    failed = [p for p in per if p["status"] != "done"]
+    max_res = max(p["rel_residual"] for p in per)
+    value = st["particle_steps"] / st["span_s"]
+
+    # plain: a sample of the same requests, one at a time, FP32 CG with 80 iterations
+    sample = list(range(0, len(reqs), max(1, args.plain_every)))
+    pw, pt, pt_steps = 0.0, 0.0, 0.0
+    for i in sample:
+        a = time.perf_counter()
+        sim = pxl.FluidState(reqs[i], FPS, seed=i, solver="cg", cg_iterations=80)
+        b = time.perf_counter()
+        sim.step(args.steps)
+        c = time.perf_counter()
+        pw += sim.particle_count() * args.steps
+        sim.close()
+        d = time.perf_counter()
+        pt += d - a
+        pt_steps += c - b
+    plain = {"value": pw / pt, "steps_only": pw / pt_steps, "requests": len(sample),
+             "path": "one simulation at a time, FP32 Jacobi-PCG 80 fixed iterations, hipGraph per step, "
+                     "creation included (steps_only: step time alone)"}
+
+    # roofline: one profiled batched step of a fixed batch of stream requests
+    nb = min(args.profile_batch, len(reqs))
+    with pxl.Batch(reqs[:nb], FPS, seeds=list(range(nb))) as bat:
+        bat.step(2)
+        prof = bat.profile_steps(2)
+    prof.sort(key=lambda k: -k["total_ms"])
+    tot = sum(k["total_ms"] for k in prof)
+    kernels = [{"kernel": k["kernel"], "launches": k["launches"], "avg_us": 1e3 * k["total_ms"] / k["launches"],
+                "share": k["total_ms"] / tot,
+                "achieved_GBs": (k["alg_bytes"] / (k["total_ms"] / 1e3) / 1e9) if k["alg_bytes"] else None}
+               for k in prof]
+    dom = kernels[0]
+    ach = dom["achieved_GBs"] or 0.0
+    roofline = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach / PEAK_HBM_GBS,
+                "traffic": measured_traffic(dom["kernel"], f"serve_batch{nb}"), "kernel": dom["kernel"],
+                "avg_us": dom["avg_us"], "alg_bytes_per_launch": prof[0]["alg_bytes"] / prof[0]["launches"],
+                "measured_on": f"profiled batched steps of the first {nb} stream requests"}
+
+    cpu = cpu_serve(args, reqs) if args.cpu_seconds > 0 else None
+    return {
+        "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * st["busy_s"] / max(st["batch_steps"], 1),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (random liquid boxes, reference LCG seeding, seed = request index)",
+        "config": {
+            "workload": f"serving stream: {len(reqs)} requests, cubic grids n in {{{args.sizes}}}, one random "
+                        f"liquid box at 4-8 particles/cell, {args.steps} steps each, Poisson arrivals, "
+                        f"{args.budget / 1e9:.0f} GB admission budget, MIXED solve to 1e-6",
+            "requests": len(reqs), "particles_total": float(P.sum()), "rate_per_s": rate,
+            "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
+            "busy_fraction": st["busy_s"] / st["span_s"] if st["span_s"] else None,
+            "mean_batch": st["mean_batch"], "batch_steps": st["batch_steps"], "admit_s": st["admit_s"],
+            "graph_captures": st["graph_captures"], "peak_bytes": st["peak_bytes"], "wall_s": wall,
+            "failed": len(failed), "max_rel_residual": max_res,
+            "plain": plain, "ratio_vs_plain": value / plain["value"],
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": kernels[:12],
+    }
+
+
+def cpu_serve(args, reqs):
+    """The C oracle (plain FP32 CG 80, one thread) on the stream's requests in order, one step
+    each, until about cpu_seconds of stepping (oracle construction and seeding not timed)."""
+    import oracle
+    from tests.scenes import oracle_sim
+    ps, spent, n = 0, 0.0, 0
+    for f in reqs:
+        osim = oracle_sim(oracle, f)
+        a = time.perf_counter()
+        osim.step_cg(80)
+        spent += time.perf_counter() - a
+        ps += osim.P
+        n += 1
+        del osim
+        if spent > args.cpu_seconds:
+            break
+    return {"value": ps / spent, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"one FP32 CG-80 step of each of the first {n} stream requests, C oracle, one thread"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--res", type=int, default=128)
-    ap.add_argument("--solver", default="cg", choices=["cg", "rbgs", "mixed"])
-    ap.add_argument("--iters", type=int, default=80)
-    ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps timed for cpu_baseline (0 = skip)")
-    ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
+    ap.add_argument("--steps", type=int, default=60, help="steps per request (dam: timed steps)")
+    ap.add_argument("--warmup", type=int, default=32, help="warm-up requests (dam: warm-up steps)")
+    ap.add_argument("--workload", default="serve", choices=["serve", "dam"])
+    ap.add_argument("--requests", type=int, default=512)
+    ap.add_argument("--sizes", default="32,48,64,96,128")
+    ap.add_argument("--rate", type=float, default=0.0, help="arrivals per second (0 = measured service rate)")
+    ap.add_argument("--budget", type=float, default=200e9)
+    ap.add_argument("--plain-every", type=int, default=8, help="plain path on every k-th request")
+    ap.add_argument("--profile-batch", type=int, default=32)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU oracle sample length (0 = skip)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -77,78 +271,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import pxlengine_amd as pxl
-    from tests.scenes import FPS, dam_break
-
-    fluid = dam_break(args.res, iters=args.iters)
-    sim = pxl.FluidState(fluid, FPS, seed=0, solver=args.solver, cg_iterations=args.iters)
-    P = sim.particle_count()
-    info = sim.info
-    for _ in range(args.warmup):
-        sim.step(1)
-    tuned = sim.autotune(2) if not args.no_tune else None  # launch shape from the autotuner
-    t0 = time.perf_counter()
-    sim.step(args.steps)  # returns when the GPU work is complete
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    value = P * args.steps / elapsed
-
-    # per-kernel times from HIP events on the simulation's stream (extra, untimed steps)
-    prof = sim.profile_steps(2)
-    Cf = int((sim.read_field("type") == 1).sum())
-    F, Cc = info["n_faces"], info["n_cells"]
-    per_kernel = []
-    for name, launches, total_ms in prof:
-        avg_s = total_ms / 1e3 / launches
-        b = algorithmic_bytes(name, P, F, Cc, Cf)
-        per_kernel.append({"kernel": name, "launches_per_step": launches // 2, "avg_us": avg_s * 1e6,
-                           "alg_bytes": b, "achieved_GBs": (b / avg_s / 1e9) if b else None})
-    dom = per_kernel[0]
-    achieved = dom["achieved_GBs"] or 0.0
-    roofline = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": achieved / PEAK_HBM_GBS, "traffic": measured_traffic(dom["kernel"], args),
-                "kernel": dom["kernel"], "avg_us": dom["avg_us"], "alg_bytes_per_launch": dom["alg_bytes"]}
-    stats = sim.last_solve()
-    sim.close()
-
-    cpu = None
-    if args.cpu_steps > 0:
-        import oracle
-        from tests.scenes import oracle_sim
-        osim = oracle_sim(oracle, fluid)
-        c0 = time.perf_counter()
-        for _ in range(args.cpu_steps):
-            if args.solver == "rbgs":
-                osim.step_rbgs(args.iters)
-            else:
-                osim.step_cg(args.iters)
-        c1 = time.perf_counter()
-        cpu = {"value": osim.P * args.cpu_steps / (c1 - c0), "unit": "particle-steps/s", "cores": 1,
-               "kind": "port",
-               "sample": f"first {args.cpu_steps} steps of the same {args.res}^3 dam break ({osim.P} particles), "
-                         f"{'RBGS' if args.solver == 'rbgs' else 'FP32 CG'} {args.iters} iterations, "
-                         "C oracle, one thread"}
-        del osim
-
-    solver_desc = {"cg": f"plain FP32 Jacobi-PCG, {args.iters} fixed iterations",
-                   "rbgs": f"reference red-black Gauss-Seidel, {args.iters} iterations",
-                   "mixed": "FP32 CG inside FP64 refinement to 1e-6"}[args.solver]
-    line = {
-        "metric": "particle-steps/s batched vs plain at ≤1e-6 rel residual; p50/p99 sim latency",
-        "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded dam break, reference LCG seeding)",
-        "config": {"workload": f"dam break {args.res}^3, single instance, {solver_desc}",
-                   "grid": list(info["n"]), "particles": P, "dt": float(info["dt"]), "flip": float(info["flip"]),
-                   "path": "plain (one simulation, one stream, hipGraph per step)",
-                   "autotune": tuned},
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-        "kernels": per_kernel,
-        "fluid_cells": Cf,
-        "last_solve": stats,
-    }
+    line = bench_serve(args, pxl) if args.workload == "serve" else bench_dam(args, pxl)
     print(json.dumps(line))
 
 

@@ -176,6 +176,7 @@ typedef struct pxlflip_kernel_time {
     char name[32];
     uint32_t launches;
     double total_ms;
+    double alg_bytes; /* algorithmic bytes summed over the launches (0 where not priced) */
 } pxlflip_kernel_time;
 int pxlflip_profile_steps(pxlflip *h, uint32_t n, pxlflip_kernel_time *out, uint32_t max_out, uint32_t *n_out);
 
@@ -270,6 +271,25 @@ int pxlflip_server_get_stats(pxlflip_server *srv, pxlflip_server_stats *stats);
 const char *pxlflip_server_last_error(const pxlflip_server *srv);
 /* Stops the device thread (requests still queued or running are cancelled) and frees all. */
 void pxlflip_server_destroy(pxlflip_server *srv);
+
+/* ---- A fixed batch: the server's batched step without the serving loop.  n simulations
+ * (fluid_jsons[i] seeded with seeds[i], or seed 0 when seeds is NULL) are allocated together
+ * and advanced in lock step; opts as for the server (budget_bytes and max_sims unused).
+ * No counterpart in the reference.  The handle is used by one host thread at a time. */
+typedef struct pxlflip_batch pxlflip_batch;
+int pxlflip_batch_create(const char *const *fluid_jsons, const uint32_t *seeds, uint32_t n, double fps,
+                         const pxlflip_server_opts *opts, pxlflip_batch **out);
+int pxlflip_batch_step(pxlflip_batch *b, uint32_t steps);
+/* Runs `steps` steps with every launch eager and timed by HIP events on the batch stream;
+ * alg_bytes prices each launch over the simulations it covers (their particles, faces, cells
+ * and FLUID cells per multigrid level of that step). */
+int pxlflip_batch_profile_steps(pxlflip_batch *b, uint32_t steps, pxlflip_kernel_time *out, uint32_t max_out,
+                                uint32_t *n_out);
+int pxlflip_batch_get_info(pxlflip_batch *b, uint32_t index, pxlflip_info *info);
+int pxlflip_batch_read_particles(pxlflip_batch *b, uint32_t index, float *aos8);
+int pxlflip_batch_last_solve(pxlflip_batch *b, uint32_t index, pxlflip_solve_stats *st);
+const char *pxlflip_batch_last_error(const pxlflip_batch *b);
+void pxlflip_batch_destroy(pxlflip_batch *b);
 
 #ifdef __cplusplus
 }

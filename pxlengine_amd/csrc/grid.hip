@@ -240,8 +240,7 @@ uint32_t flag_count(const DevSim &S) { return S.split_colours ? 2 * S.ncells : S
 
 void batch_faces(const WorkLists &W, hipStream_t s) { kb_faces<<<BATCH_GRID, dim3(TX, TY), 0, s>>>(W[WD_NODES]); }
 void batch_cells(const WorkLists &W, hipStream_t s) { kb_cells<<<BATCH_GRID, dim3(TX, TY), 0, s>>>(W[WD_CELLS]); }
-void batch_fluid_list(const WorkLists &W, hipStream_t s) {
-    batch_scan(W, 1, s);
+void batch_fluid_compact(const WorkLists &W, hipStream_t s) {
     kb_fluid_compact<<<BATCH_GRID, BLOCK, 0, s>>>(W[WD_COMPACT]);
 }
 void batch_project(const WorkLists &W, hipStream_t s) { kb_project<<<BATCH_GRID, dim3(TX, TY), 0, s>>>(W[WD_NODES]); }

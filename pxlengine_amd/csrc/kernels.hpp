@@ -273,22 +273,34 @@ uint32_t mg_cell_blocks(const DevSim &S, int l);
 
 // Batched launchers: W[d] is the work list of domain d.
 using WorkLists = WorkList[WD_COUNT];
+// grid phase
 void batch_clear_grid(const WorkLists &W, hipStream_t s);
 void batch_bin_count(const WorkLists &W, hipStream_t s);
-void batch_scan(const WorkLists &W, int which, hipStream_t s);  // 0: count -> offset, 1: fluid flags
+void batch_scan_reduce(const WorkLists &W, int which, hipStream_t s);  // which 0: count -> offset
+void batch_scan_tiles(const WorkLists &W, int which, hipStream_t s);   // which 1: fluid flags
+void batch_scan_down(const WorkLists &W, int which, hipStream_t s);
 void batch_tile_list(const WorkLists &W, hipStream_t s);
 void batch_scatter(const WorkLists &W, hipStream_t s);
 void batch_p2g(const WorkLists &W, hipStream_t s);
 void batch_faces(const WorkLists &W, hipStream_t s);
 void batch_cells(const WorkLists &W, hipStream_t s);
-void batch_fluid_list(const WorkLists &W, hipStream_t s);
-void batch_mg_types(const WorkLists &W, int max_levels, hipStream_t s);
+void batch_fluid_compact(const WorkLists &W, hipStream_t s);
+void batch_mg_types(const WorkLists &W, int l, hipStream_t s);  // level l >= 1: types, list count = 0
+void batch_mg_sten(const WorkLists &W, int l, hipStream_t s);   // stencil words, z = 0, FLUID list
+// solve (WD_A_* lists)
+void batch_mixed_init(const WorkLists &W, hipStream_t s);
+void batch_norms(const WorkLists &W, int nv, hipStream_t s);
+void batch_cg_residual(const WorkLists &W, hipStream_t s);
+void batch_mg_pre0(const WorkLists &W, int l, hipStream_t s);
+void batch_mg_sweep(const WorkLists &W, int l, uint32_t colour, hipStream_t s);
+void batch_mg_restrict(const WorkLists &W, int l, hipStream_t s);  // level l -> l + 1
+void batch_mg_tail(const WorkLists &W, hipStream_t s);
+void batch_mg_post(const WorkLists &W, int l, hipStream_t s);
+void batch_mgcg_spmv(const WorkLists &W, uint32_t slot, int first, hipStream_t s);
+void batch_mgcg_update(const WorkLists &W, uint32_t it, hipStream_t s);
+void batch_mixed_update(const WorkLists &W, int cur, hipStream_t s);
+// post
 void batch_project(const WorkLists &W, hipStream_t s);
 void batch_g2p(const WorkLists &W, hipStream_t s);
-void batch_mg_vcycle(const WorkLists &W, int max_tail, hipStream_t s);  // u = M r for the solving sims
-void batch_grid_phase(const WorkLists &W, int max_levels, hipStream_t s);    // clear .. fluid list, mg types
-void batch_mixed_init(const WorkLists &W, hipStream_t s);                    // x64a = p, r64, norms
-void batch_inner_pass(const WorkLists &W, int cur, uint32_t inner, int max_tail, hipStream_t s);
-void batch_post(const WorkLists &W, hipStream_t s);                          // project, g2p
 
 }  // namespace pxl

@@ -563,31 +563,16 @@ void launch_mg_vcycle(const DevSim &S, const float *r, float *u, hipStream_t s) 
 // ---- batched
 uint32_t mg_cell_blocks(const DevSim &S, int l) { return grid_for(cells(l == 0 ? level0(S) : S.mg[l])); }
 
-void batch_mg_types(const WorkLists &W, int max_levels, hipStream_t s) {
-    for (int l = 1; l < max_levels; l++) {
-        kb_mg_types<<<BATCH_GRID, MBLOCK, 0, s>>>(W[WD_MGTYPES + l], l);
-        kb_mg_sten<<<BATCH_GRID, MBLOCK, 0, s>>>(W[WD_MGTYPES + l], l);
-    }
+void batch_mg_types(const WorkLists &W, int l, hipStream_t s) { kb_mg_types<<<BATCH_GRID, MBLOCK, 0, s>>>(W[WD_MGTYPES + l], l); }
+void batch_mg_sten(const WorkLists &W, int l, hipStream_t s) { kb_mg_sten<<<BATCH_GRID, MBLOCK, 0, s>>>(W[WD_MGTYPES + l], l); }
+void batch_mg_pre0(const WorkLists &W, int l, hipStream_t s) { kb_pre0<<<BATCH_LGRID, LBLOCK, 0, s>>>(W[WD_A_LVL + l], l); }
+void batch_mg_sweep(const WorkLists &W, int l, uint32_t colour, hipStream_t s) {
+    kb_sweep<<<BATCH_LGRID, LBLOCK, 0, s>>>(W[WD_A_LVL + l], l, colour);
 }
-
-// The V-cycle of every solving simulation: the level-l lists hold the simulations whose
-// level l runs as separate launches (l < mg_tail); the tails run one workgroup each.
-void batch_mg_vcycle(const WorkLists &W, int max_tail, hipStream_t s) {
-    for (int l = 0; l < max_tail; l++) {
-        const WorkList &w = W[WD_A_LVL + l];
-        kb_pre0<<<BATCH_LGRID, LBLOCK, 0, s>>>(w, l);
-        kb_sweep<<<BATCH_LGRID, LBLOCK, 0, s>>>(w, l, 0);
-        kb_sweep<<<BATCH_LGRID, LBLOCK, 0, s>>>(w, l, 1);
-        kb_restrict<<<BATCH_LGRID, LBLOCK, 0, s>>>(w, l);
-    }
-    kb_mg_tail<<<BATCH_TAIL_GRID, TBLOCK, 0, s>>>(W[WD_A_SIM]);
-    for (int l = max_tail - 1; l >= 0; l--) {
-        const WorkList &w = W[WD_A_LVL + l];
-        kb_post<<<BATCH_LGRID, LBLOCK, 0, s>>>(w, l);
-        kb_sweep<<<BATCH_LGRID, LBLOCK, 0, s>>>(w, l, 0);
-        kb_sweep<<<BATCH_LGRID, LBLOCK, 0, s>>>(w, l, 1);
-        kb_sweep<<<BATCH_LGRID, LBLOCK, 0, s>>>(w, l, 0);
-    }
+void batch_mg_restrict(const WorkLists &W, int l, hipStream_t s) {
+    kb_restrict<<<BATCH_LGRID, LBLOCK, 0, s>>>(W[WD_A_LVL + l], l);
 }
+void batch_mg_post(const WorkLists &W, int l, hipStream_t s) { kb_post<<<BATCH_LGRID, LBLOCK, 0, s>>>(W[WD_A_LVL + l], l); }
+void batch_mg_tail(const WorkLists &W, hipStream_t s) { kb_mg_tail<<<BATCH_TAIL_GRID, TBLOCK, 0, s>>>(W[WD_A_SIM]); }
 
 }  // namespace pxl

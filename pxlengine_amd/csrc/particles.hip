@@ -638,11 +638,14 @@ uint32_t tile_grid_blocks(const DevSim &S) { return tile_grid(S); }
 
 void batch_clear_grid(const WorkLists &W, hipStream_t s) { kb_clear_grid<<<BATCH_GRID, BLOCK, 0, s>>>(W[WD_KEYS]); }
 void batch_bin_count(const WorkLists &W, hipStream_t s) { kb_bin_count<<<BATCH_GRID, BLOCK, 0, s>>>(W[WD_PART]); }
-void batch_scan(const WorkLists &W, int which, hipStream_t s) {
-    const WorkList &t = W[which == 0 ? WD_SCAN_KEYS : WD_SCAN_FLAGS];
-    kb_scan_reduce<<<BATCH_GRID, BLOCK, 0, s>>>(t, which);
+void batch_scan_reduce(const WorkLists &W, int which, hipStream_t s) {
+    kb_scan_reduce<<<BATCH_GRID, BLOCK, 0, s>>>(W[which == 0 ? WD_SCAN_KEYS : WD_SCAN_FLAGS], which);
+}
+void batch_scan_tiles(const WorkLists &W, int which, hipStream_t s) {
     kb_scan_tiles<<<BATCH_GRID, BLOCK, 0, s>>>(W[WD_SIM], which);
-    kb_scan_down<<<BATCH_GRID, BLOCK, 0, s>>>(t, which);
+}
+void batch_scan_down(const WorkLists &W, int which, hipStream_t s) {
+    kb_scan_down<<<BATCH_GRID, BLOCK, 0, s>>>(W[which == 0 ? WD_SCAN_KEYS : WD_SCAN_FLAGS], which);
 }
 void batch_tile_list(const WorkLists &W, hipStream_t s) { kb_tile_list<<<BATCH_GRID, BLOCK, 0, s>>>(W[WD_TILES]); }
 void batch_scatter(const WorkLists &W, hipStream_t s) { kb_scatter<<<BATCH_GRID, BLOCK, 0, s>>>(W[WD_PART]); }

@@ -59,24 +59,6 @@ uint32_t work_blocks(const DevSim &S, int d) {
     return 0;
 }
 
-void batch_grid_phase(const WorkLists &W, int max_levels, hipStream_t s) {
-    batch_clear_grid(W, s);
-    batch_bin_count(W, s);
-    batch_scan(W, 0, s);
-    batch_tile_list(W, s);
-    batch_scatter(W, s);
-    batch_p2g(W, s);
-    batch_faces(W, s);
-    batch_cells(W, s);
-    batch_fluid_list(W, s);
-    batch_mg_types(W, max_levels, s);
-}
-
-void batch_post(const WorkLists &W, hipStream_t s) {
-    batch_project(W, s);
-    batch_g2p(W, s);
-}
-
 namespace {
 
 double now_s() {
@@ -97,6 +79,55 @@ void dev_alloc(T **p, size_t n) {
     if (hipMalloc(reinterpret_cast<void **>(p), sizeof(T) * (n ? n : 1)) != hipSuccess) throw std::bad_alloc();
 }
 
+// What a profiled batch step needs to price each launch: per member, the FLUID cells of each
+// level (level 0 = the fluid list) and the active P2G tiles of the step.
+struct MemberCounts {
+    uint32_t fluid[MG_MAX] = {};
+    uint32_t tiles = 0;
+};
+
+// Algorithmic bytes of one simulation's share of a launch: every logical array the kernel
+// must touch once, at its element width (fp32/int32 4 B, fp64 8 B, cell type 1 B, stencil
+// word 2 B, particle position + velocity 24 B, id 4 B).  P particles, F faces, C cells, K
+// count keys, Cf / c_l FLUID cells of level 0 / l, N_l cells of level l, t active tiles (a
+// tile's P2G box is 2700 packed 8-byte sums).
+double share_bytes(const char *k, const DevSim &d, const MemberCounts &m, int l) {
+    const double P = d.np, F = d.nfaces, C = d.ncells, K = d.nkeys, Cf = m.fluid[0], t = m.tiles;
+    const double box = 8.0 * BOX_ENTRIES;
+    auto cells_of = [&](int q) { return q == 0 ? C : (double)d.mg[q].nx * d.mg[q].ny * d.mg[q].nz; };
+    const double cl = l < MG_MAX ? m.fluid[l] : 0, cl1 = l + 1 < MG_MAX ? m.fluid[l + 1] : 0;
+    const std::string n(k);
+    if (n == "clear_grid") return 4 * K;
+    if (n == "bin_count") return 20 * P + 4 * C;
+    if (n == "scan_reduce") return 4 * (l == 0 ? K : C);
+    if (n == "scan_down") return 8 * (l == 0 ? K : C);
+    if (n == "tile_list") return 8.0 * d.ntiles;
+    if (n == "scatter") return 68 * P;
+    if (n == "p2g") return 24 * P + box * t;
+    if (n == "faces") return 8 * F + box * t;
+    if (n == "cells") return 4 * F + 15 * C + 4 * (C - Cf);
+    if (n == "fluid_compact") return 10 * C + 8 * Cf;
+    if (n == "mg_types") return cells_of(l - 1) + cells_of(l);
+    if (n == "mg_sten") return 7 * cells_of(l) + 4 * cl;
+    if (n == "mixed_init") return 32 * Cf;
+    if (n == "cg_residual") return 24 * Cf;
+    if (n == "mg_pre0") return 16 * cl;
+    if (n == "mg_sweep") return (l == 0 ? 14 : 13) * cl;
+    if (n == "mg_restrict") return 64 * cl1;
+    if (n == "mg_post") return 13 * cl;
+    if (n == "mg_tail") {
+        double b = 0;
+        for (int q = d.mg_tail; q < d.mg_levels; q++) b += 2 * cells_of(q);
+        return b + 8 * cells_of(d.mg_tail);
+    }
+    if (n == "mgcg_spmv") return 20 * Cf;
+    if (n == "mgcg_update") return 44 * Cf;
+    if (n == "mixed_update") return 44 * Cf;
+    if (n == "project") return 8 * F + 5 * C;
+    if (n == "g2p") return 56 * P + 8 * F;
+    return 0;
+}
+
 class Batch {
 public:
     explicit Batch(const pxlflip_opts &o) : opts_(o) {
@@ -113,6 +144,10 @@ public:
             if (p) (void)hipFree(p);
         for (void *p : {(void *)h_sims_, (void *)h_entries_, (void *)h_counts_, (void *)h_norms_})
             if (p) (void)hipHostFree(p);
+        for (auto &r : recs_) {
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
         if (stream_) (void)hipStreamDestroy(stream_);
     }
 
@@ -148,8 +183,10 @@ public:
         for (size_t i = 0; i < n; i++) active[i] = (uint32_t)i;
         set_active(active);
         replay(&g_grid_, [&] {
-            batch_grid_phase(lists_, max_levels_, stream_);
-            batch_mixed_init(lists_, stream_);
+            enqueue_grid_phase();
+            if (profiling_) read_counts();
+            run("mixed_init", -1, [&] { batch_mixed_init(lists_, stream_); });
+            run("norms", -1, [&] { batch_norms(lists_, 2, stream_); });
         });
         read_norms();
         std::vector<double> bb(n), rel(n);
@@ -166,7 +203,7 @@ public:
             if (next.size() != active.size()) set_active(next);
             active = next;
             const int cur = (int)(pass & 1);
-            replay(&g_inner_[cur], [&] { batch_inner_pass(lists_, cur, opts_.inner_iterations, max_tail_, stream_); });
+            replay(&g_inner_[cur], [&] { enqueue_inner_pass(cur); });
             read_norms();
             next.clear();
             for (uint32_t i : active) {
@@ -176,7 +213,10 @@ public:
             }
             pass++;
         }
-        replay(&g_post_, [&] { batch_post(lists_, stream_); });
+        replay(&g_post_, [&] {
+            run("project", -1, [&] { batch_project(lists_, stream_); });
+            run("g2p", -1, [&] { batch_g2p(lists_, stream_); });
+        });
         for (size_t i = 0; i < n; i++) {
             Sim *s = members_[i];
             s->steps_done++;
@@ -189,8 +229,158 @@ public:
 
     void sync() { HIP_CHECK(hipStreamSynchronize(stream_)); }
 
+    // Profiling: every launch eager and bracketed by HIP events on the batch stream, priced
+    // with share_bytes over the simulations of its work list.
+    void set_profiling(bool on) {
+        profiling_ = on;
+        recs_used_ = 0;
+        agg_.clear();
+    }
+    struct Agg {
+        uint32_t launches = 0;
+        double ms = 0, bytes = 0;
+    };
+    // Per-kernel totals of the profiled steps so far (in first-launch order).
+    std::vector<std::pair<std::string, Agg>> profile() {
+        HIP_CHECK(hipStreamSynchronize(stream_));
+        collect();
+        return agg_;
+    }
+
 private:
     static double rel_of(double rr, double bb) { return bb > 0.0 ? std::sqrt(rr / bb) : std::sqrt(rr); }
+
+    // ---- launch sequences (the same order as the single-sim step)
+    void enqueue_grid_phase() {
+        grid_start_ = recs_used_;
+        run("clear_grid", -1, [&] { batch_clear_grid(lists_, stream_); });
+        run("bin_count", -1, [&] { batch_bin_count(lists_, stream_); });
+        run("scan_reduce", 0, [&] { batch_scan_reduce(lists_, 0, stream_); });
+        run("scan_tiles", 0, [&] { batch_scan_tiles(lists_, 0, stream_); });
+        run("scan_down", 0, [&] { batch_scan_down(lists_, 0, stream_); });
+        run("tile_list", -1, [&] { batch_tile_list(lists_, stream_); });
+        run("scatter", -1, [&] { batch_scatter(lists_, stream_); });
+        run("p2g", -1, [&] { batch_p2g(lists_, stream_); });
+        run("faces", -1, [&] { batch_faces(lists_, stream_); });
+        run("cells", -1, [&] { batch_cells(lists_, stream_); });
+        run("scan_reduce", 1, [&] { batch_scan_reduce(lists_, 1, stream_); });
+        run("scan_tiles", 1, [&] { batch_scan_tiles(lists_, 1, stream_); });
+        run("scan_down", 1, [&] { batch_scan_down(lists_, 1, stream_); });
+        run("fluid_compact", -1, [&] { batch_fluid_compact(lists_, stream_); });
+        for (int l = 1; l < max_levels_; l++) {
+            run("mg_types", l, [&] { batch_mg_types(lists_, l, stream_); });
+            run("mg_sten", l, [&] { batch_mg_sten(lists_, l, stream_); });
+        }
+    }
+    void enqueue_vcycle() {
+        for (int l = 0; l < max_tail_; l++) {
+            run("mg_pre0", l, [&] { batch_mg_pre0(lists_, l, stream_); });
+            run("mg_sweep", l, [&] { batch_mg_sweep(lists_, l, 0, stream_); });
+            run("mg_sweep", l, [&] { batch_mg_sweep(lists_, l, 1, stream_); });
+            run("mg_restrict", l, [&] { batch_mg_restrict(lists_, l, stream_); });
+        }
+        run("mg_tail", -1, [&] { batch_mg_tail(lists_, stream_); });
+        for (int l = max_tail_ - 1; l >= 0; l--) {
+            run("mg_post", l, [&] { batch_mg_post(lists_, l, stream_); });
+            run("mg_sweep", l, [&] { batch_mg_sweep(lists_, l, 0, stream_); });
+            run("mg_sweep", l, [&] { batch_mg_sweep(lists_, l, 1, stream_); });
+            run("mg_sweep", l, [&] { batch_mg_sweep(lists_, l, 0, stream_); });
+        }
+    }
+    // One refinement pass for the solving members: MG-CG from e = 0 on float(r64), then the
+    // FP64 update x64[cur ^ 1] = x64[cur] + e and the residual norm.
+    void enqueue_inner_pass(int cur) {
+        const uint32_t inner = opts_.inner_iterations;
+        run("cg_residual", -1, [&] { batch_cg_residual(lists_, stream_); });
+        enqueue_vcycle();
+        run("mgcg_spmv", -1, [&] { batch_mgcg_spmv(lists_, 0, 1, stream_); });
+        for (uint32_t it = 0; it < inner; it++) {
+            run("mgcg_update", -1, [&] { batch_mgcg_update(lists_, it, stream_); });
+            if (it + 1 == inner) break;
+            enqueue_vcycle();
+            run("mgcg_spmv", -1, [&] { batch_mgcg_spmv(lists_, (it + 1) & 1, 0, stream_); });
+        }
+        run("mixed_update", -1, [&] { batch_mixed_update(lists_, cur, stream_); });
+        run("norms", -1, [&] { batch_norms(lists_, 1, stream_); });
+    }
+
+    // ---- profiling
+    struct Rec {
+        const char *name;
+        int level;
+        hipEvent_t a, b;
+        double bytes;
+    };
+    template <typename F>
+    void run(const char *name, int level, F &&launch) {
+        if (!profiling_) {
+            launch();
+            return;
+        }
+        if (recs_used_ == recs_.size()) {
+            Rec r{};
+            HIP_CHECK(hipEventCreate(&r.a));
+            HIP_CHECK(hipEventCreate(&r.b));
+            recs_.push_back(r);
+        }
+        Rec &r = recs_[recs_used_++];
+        r.name = name;
+        r.level = level;
+        r.bytes = launch_bytes(name, level);
+        HIP_CHECK(hipEventRecord(r.a, stream_));
+        launch();
+        HIP_CHECK(hipEventRecord(r.b, stream_));
+    }
+    // The members a launch covers: the solve kernels take the active ones (per level, those
+    // whose level runs as separate launches), the rest take every member.
+    double launch_bytes(const char *name, int level) {
+        const std::string n(name);
+        const bool solve = n.rfind("mg_", 0) == 0 ? (n != "mg_types" && n != "mg_sten") :
+                           (n == "mixed_init" || n == "norms" || n == "cg_residual" || n.rfind("mgcg", 0) == 0 ||
+                            n == "mixed_update");
+        double b = 0;
+        for (uint32_t i = 0; i < members_.size(); i++) {
+            if (solve && !is_active_[i]) continue;
+            const DevSim &d = members_[i]->d;
+            if (n == "mg_types" || n == "mg_sten") {
+                if (level >= d.mg_levels) continue;
+            } else if (n == "mg_pre0" || n == "mg_sweep" || n == "mg_restrict" || n == "mg_post") {
+                if (level >= d.mg_tail) continue;
+            }
+            b += share_bytes(name, d, counts_[i], level < 0 ? 0 : level);
+        }
+        return b;
+    }
+    void read_counts() {
+        // device -> host copies of each member's list counts (profiling only; not in graphs)
+        counts_.assign(members_.size(), MemberCounts{});
+        for (size_t i = 0; i < members_.size(); i++) {
+            const DevSim &d = members_[i]->d;
+            HIP_CHECK(hipMemcpyAsync(&counts_[i].fluid[0], d.fl_count + 1, 4, hipMemcpyDeviceToHost, stream_));
+            HIP_CHECK(hipMemcpyAsync(&counts_[i].tiles, d.tile_count, 4, hipMemcpyDeviceToHost, stream_));
+            for (int l = 1; l < d.mg_levels && l < MG_MAX; l++)
+                HIP_CHECK(hipMemcpyAsync(&counts_[i].fluid[l], d.mg[l].count, 4, hipMemcpyDeviceToHost, stream_));
+        }
+        HIP_CHECK(hipStreamSynchronize(stream_));
+        // this step's grid-phase launches (recorded before its counts were known) are priced now
+        for (size_t k = grid_start_; k < recs_used_; k++) recs_[k].bytes = launch_bytes(recs_[k].name, recs_[k].level);
+    }
+    void collect() {
+        // events of the pending records -> per-kernel totals
+        for (size_t k = 0; k < recs_used_; k++) {
+            float ms = 0;
+            HIP_CHECK(hipEventElapsedTime(&ms, recs_[k].a, recs_[k].b));
+            auto it = std::find_if(agg_.begin(), agg_.end(), [&](auto &e) { return e.first == recs_[k].name; });
+            if (it == agg_.end()) {
+                agg_.emplace_back(recs_[k].name, Agg{});
+                it = agg_.end() - 1;
+            }
+            it->second.launches++;
+            it->second.ms += ms;
+            it->second.bytes += recs_[k].bytes;
+        }
+        recs_used_ = 0;
+    }
 
     void drop_graphs() {
         for (hipGraphExec_t *g : {&g_grid_, &g_inner_[0], &g_inner_[1], &g_post_}) {
@@ -201,7 +391,7 @@ private:
 
     template <typename F>
     void replay(hipGraphExec_t *g, F &&enqueue) {
-        if (!opts_.use_graph) {
+        if (!opts_.use_graph || profiling_) {
             enqueue();
             return;
         }
@@ -238,6 +428,8 @@ private:
 
     // Solve domains for the members in `ids` (all of them at the start of a step).
     void set_active(const std::vector<uint32_t> &ids) {
+        is_active_.assign(members_.size(), 0);
+        for (uint32_t i : ids) is_active_[i] = 1;
         for (int d = WD_A_SIM; d < WD_COUNT; d++) fill(d, ids);
         const size_t a = off_[WD_A_SIM], bytes = 4 * (off_[WD_COUNT - 1] + cap_[WD_COUNT - 1] - a);
         HIP_CHECK(hipMemcpyAsync(d_entries_ + a, h_entries_ + a, bytes, hipMemcpyHostToDevice, stream_));
@@ -298,6 +490,7 @@ private:
         HIP_CHECK(hipMemcpyAsync(d_entries_, h_entries_, 4 * off_[WD_A_SIM], hipMemcpyHostToDevice, stream_));
         // the copies above read pinned staging that the next rebuild rewrites
         HIP_CHECK(hipStreamSynchronize(stream_));
+        counts_.assign(n, MemberCounts{});
         dirty_ = false;
     }
 
@@ -305,6 +498,7 @@ private:
     hipStream_t stream_ = nullptr;
     std::vector<Sim *> members_;
     std::vector<uint32_t> passes_;
+    std::vector<uint8_t> is_active_;
     bool dirty_ = true;
     size_t sims_cap_ = 0;
     DevSim *d_sims_ = nullptr, *h_sims_ = nullptr;
@@ -316,6 +510,11 @@ private:
     int max_levels_ = 0, max_tail_ = 0;
     hipGraphExec_t g_grid_ = nullptr, g_inner_[2] = {nullptr, nullptr}, g_post_ = nullptr;
     uint64_t captures_ = 0, rebuilds_ = 0;
+    bool profiling_ = false;
+    std::vector<Rec> recs_;
+    size_t recs_used_ = 0, grid_start_ = 0;
+    std::vector<MemberCounts> counts_;
+    std::vector<std::pair<std::string, Agg>> agg_;
 };
 
 // Solver workgroups of a batched simulation: enough lists slices to cover the fluid cells a
@@ -829,6 +1028,166 @@ void pxlflip_server_destroy(pxlflip_server *srv) {
     (void)hipSetDevice(srv->opts.device);
     srv->batch.reset();
     delete srv;
+}
+
+// ---- fixed batch
+}  // extern "C"
+
+struct pxlflip_batch {
+    pxlflip_opts sim_opts{};
+    std::vector<std::unique_ptr<pxl::Sim>> sims;
+    std::unique_ptr<pxl::Batch> batch;
+    int device = 0;
+    std::string error;
+};
+
+namespace {
+
+template <typename F>
+int batch_guarded(pxlflip_batch *b, F &&body) {
+    using namespace pxl;
+    if (!b) return PXLFLIP_ERR_INVALID;
+    try {
+        HIP_CHECK(hipSetDevice(b->device));
+        return body();
+    } catch (const HipError &e) {
+        b->error = std::string("HIP error in ") + e.what + ": " + hipGetErrorString(e.code);
+        return PXLFLIP_ERR_DEVICE;
+    } catch (const std::bad_alloc &) {
+        b->error = "out of memory";
+        return PXLFLIP_ERR_OOM;
+    } catch (const std::exception &e) {
+        b->error = e.what();
+        return PXLFLIP_ERR_INTERNAL;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int pxlflip_batch_create(const char *const *fluid_jsons, const uint32_t *seeds, uint32_t n, double fps,
+                         const pxlflip_server_opts *opts, pxlflip_batch **out) {
+    using namespace pxl;
+    g_create_error.clear();
+    if (!out || (!fluid_jsons && n)) {
+        g_create_error = "fluid_jsons / out is NULL";
+        return PXLFLIP_ERR_INVALID;
+    }
+    *out = nullptr;
+    if (n == 0 || n > MAX_BATCH_SIMS) {
+        g_create_error = "a batch holds 1.." + std::to_string(MAX_BATCH_SIMS) + " simulations";
+        return PXLFLIP_ERR_INVALID;
+    }
+    pxlflip_server_opts o;
+    pxlflip_server_default_opts(&o);
+    if (opts) {
+        if (opts->struct_size != sizeof(pxlflip_server_opts)) {
+            g_create_error = "pxlflip_server_opts.struct_size mismatch (use pxlflip_server_default_opts)";
+            return PXLFLIP_ERR_INVALID;
+        }
+        o = *opts;
+    }
+    auto b = std::make_unique<pxlflip_batch>();
+    b->device = o.device;
+    pxlflip_default_opts(&b->sim_opts);
+    b->sim_opts.solver = PXLFLIP_SOLVER_MIXED;
+    b->sim_opts.inner_iterations = o.inner_iterations;
+    b->sim_opts.max_outer = o.max_outer;
+    b->sim_opts.tolerance = o.tolerance;
+    b->sim_opts.device = o.device;
+    b->sim_opts.use_graph = o.use_graph;
+    try {
+        HIP_CHECK(hipSetDevice(o.device));
+        for (uint32_t i = 0; i < n; i++) {
+            auto sim = std::make_unique<Sim>();
+            std::vector<float> x0, v0;
+            b->sim_opts.scene_index = i;
+            const int rc = prepare(fluid_jsons[i], fps, seeds ? seeds[i] : 0u, &b->sim_opts, *sim, &x0, &v0);
+            if (rc != PXLFLIP_OK) return rc;
+            sim->d.cg_blocks = (int)(o.solver_workgroups ? o.solver_workgroups : batch_solver_blocks(sim->d));
+            HIP_CHECK(hipStreamCreateWithFlags(&sim->stream, hipStreamNonBlocking));
+            sim->allocate(x0, v0);
+            sim->reset();
+            b->sims.push_back(std::move(sim));
+        }
+        b->batch = std::make_unique<Batch>(b->sim_opts);
+        for (auto &sim : b->sims) b->batch->add(sim.get());
+    } catch (const HipError &e) {
+        g_create_error = std::string("HIP error in ") + e.what + ": " + hipGetErrorString(e.code);
+        return e.code == hipErrorOutOfMemory ? PXLFLIP_ERR_OOM : PXLFLIP_ERR_DEVICE;
+    } catch (const std::bad_alloc &) {
+        g_create_error = "out of device memory";
+        return PXLFLIP_ERR_OOM;
+    }
+    *out = b.release();
+    return PXLFLIP_OK;
+}
+
+int pxlflip_batch_step(pxlflip_batch *b, uint32_t steps) {
+    return batch_guarded(b, [&] {
+        for (uint32_t k = 0; k < steps; k++) b->batch->step();
+        b->batch->sync();
+        return PXLFLIP_OK;
+    });
+}
+
+int pxlflip_batch_profile_steps(pxlflip_batch *b, uint32_t steps, pxlflip_kernel_time *out, uint32_t max_out,
+                                uint32_t *n_out) {
+    return batch_guarded(b, [&] {
+        b->batch->set_profiling(true);
+        for (uint32_t k = 0; k < steps; k++) b->batch->step();
+        auto agg = b->batch->profile();
+        b->batch->set_profiling(false);
+        if (n_out) *n_out = (uint32_t)agg.size();
+        for (uint32_t k = 0; k < agg.size() && k < max_out && out; k++) {
+            std::memset(&out[k], 0, sizeof out[k]);
+            std::snprintf(out[k].name, sizeof out[k].name, "%s", agg[k].first.c_str());
+            out[k].launches = agg[k].second.launches;
+            out[k].total_ms = agg[k].second.ms;
+            out[k].alg_bytes = agg[k].second.bytes;
+        }
+        return PXLFLIP_OK;
+    });
+}
+
+int pxlflip_batch_get_info(pxlflip_batch *b, uint32_t index, pxlflip_info *info) {
+    if (!b || !info || index >= b->sims.size()) return PXLFLIP_ERR_INVALID;
+    pxl::fill_info(*b->sims[index], info);
+    return PXLFLIP_OK;
+}
+
+int pxlflip_batch_read_particles(pxlflip_batch *b, uint32_t index, float *aos8) {
+    if (!b || !aos8 || index >= b->sims.size()) return PXLFLIP_ERR_INVALID;
+    return batch_guarded(b, [&] {
+        using namespace pxl;
+        Sim &s = *b->sims[index];
+        b->batch->sync();
+        DeviceBuffer buf((size_t)s.d.np * 32);
+        launch_unpack_aos8(s.d, buf.as<float>(), s.stream);
+        HIP_CHECK(hipMemcpyAsync(aos8, buf.p, (size_t)s.d.np * 32, hipMemcpyDeviceToHost, s.stream));
+        HIP_CHECK(hipStreamSynchronize(s.stream));
+        return PXLFLIP_OK;
+    });
+}
+
+int pxlflip_batch_last_solve(pxlflip_batch *b, uint32_t index, pxlflip_solve_stats *st) {
+    if (!b || !st || index >= b->sims.size()) return PXLFLIP_ERR_INVALID;
+    *st = b->sims[index]->stats;
+    return PXLFLIP_OK;
+}
+
+const char *pxlflip_batch_last_error(const pxlflip_batch *b) {
+    if (!b) return pxl::g_create_error.c_str();
+    return b->error.c_str();
+}
+
+void pxlflip_batch_destroy(pxlflip_batch *b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    b->batch.reset();
+    b->sims.clear();
+    delete b;
 }
 
 }  // extern "C"

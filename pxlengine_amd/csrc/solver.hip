@@ -498,26 +498,20 @@ void launch_rbgs(const DevSim &S, uint32_t parity, hipStream_t s) {
     k_rbgs<<<solver_blocks(S), SBLOCK, 0, s>>>(S, parity);
 }
 
-// ---- batched
-void batch_mixed_init(const WorkLists &W, hipStream_t s) {
-    kb_mixed_init<<<BATCH_SGRID, SBLOCK, 0, s>>>(W[WD_A_SOLVER]);
-    kb_norms_finalize<<<BATCH_SGRID, BLOCK, 0, s>>>(W[WD_A_SIM], 2);
+// ---- batched (the solving simulations: x = e, r = cg_r0, u = cg_r1)
+void batch_mixed_init(const WorkLists &W, hipStream_t s) { kb_mixed_init<<<BATCH_SGRID, SBLOCK, 0, s>>>(W[WD_A_SOLVER]); }
+void batch_norms(const WorkLists &W, int nv, hipStream_t s) {
+    kb_norms_finalize<<<BATCH_SGRID, BLOCK, 0, s>>>(W[WD_A_SIM], nv);
 }
-
-// One refinement pass for the solving simulations: MG-CG from e = 0 on float(r64), then the
-// FP64 update x64[cur ^ 1] = x64[cur] + e and the residual norm.
-void batch_inner_pass(const WorkLists &W, int cur, uint32_t inner, int max_tail, hipStream_t s) {
-    kb_cg_residual<<<BATCH_SGRID, SBLOCK, 0, s>>>(W[WD_A_SOLVER]);
-    batch_mg_vcycle(W, max_tail, s);
-    kb_mgcg_spmv<<<BATCH_SGRID, SBLOCK, 0, s>>>(W[WD_A_SOLVER], 0, 1);
-    for (uint32_t it = 0; it < inner; it++) {
-        kb_mgcg_update<<<BATCH_SGRID, SBLOCK, 0, s>>>(W[WD_A_SOLVER], it);
-        if (it + 1 == inner) break;
-        batch_mg_vcycle(W, max_tail, s);
-        kb_mgcg_spmv<<<BATCH_SGRID, SBLOCK, 0, s>>>(W[WD_A_SOLVER], (it + 1) & 1, 0);
-    }
+void batch_cg_residual(const WorkLists &W, hipStream_t s) { kb_cg_residual<<<BATCH_SGRID, SBLOCK, 0, s>>>(W[WD_A_SOLVER]); }
+void batch_mgcg_spmv(const WorkLists &W, uint32_t slot, int first, hipStream_t s) {
+    kb_mgcg_spmv<<<BATCH_SGRID, SBLOCK, 0, s>>>(W[WD_A_SOLVER], slot, first);
+}
+void batch_mgcg_update(const WorkLists &W, uint32_t it, hipStream_t s) {
+    kb_mgcg_update<<<BATCH_SGRID, SBLOCK, 0, s>>>(W[WD_A_SOLVER], it);
+}
+void batch_mixed_update(const WorkLists &W, int cur, hipStream_t s) {
     kb_mixed_update<<<BATCH_SGRID, SBLOCK, 0, s>>>(W[WD_A_SOLVER], cur);
-    kb_norms_finalize<<<BATCH_SGRID, BLOCK, 0, s>>>(W[WD_A_SIM], 1);
 }
 
 }  // namespace pxl

@@ -76,7 +76,8 @@ class Info(C.Structure):
 
 
 class KernelTime(C.Structure):
-    _fields_ = [("name", C.c_char * 32), ("launches", C.c_uint32), ("total_ms", C.c_double)]
+    _fields_ = [("name", C.c_char * 32), ("launches", C.c_uint32), ("total_ms", C.c_double),
+                ("alg_bytes", C.c_double)]
 
 
 class SolveStats(C.Structure):
@@ -161,6 +162,16 @@ _SIGNATURES = {
     "pxlflip_server_get_stats": (C.c_int, [_H, C.POINTER(ServerStats)]),
     "pxlflip_server_last_error": (C.c_char_p, [_H]),
     "pxlflip_server_destroy": (None, [_H]),
+    "pxlflip_batch_create": (C.c_int, [C.POINTER(C.c_char_p), C.POINTER(C.c_uint32), C.c_uint32, C.c_double,
+                                       C.POINTER(ServerOpts), C.POINTER(_H)]),
+    "pxlflip_batch_step": (C.c_int, [_H, C.c_uint32]),
+    "pxlflip_batch_profile_steps": (C.c_int, [_H, C.c_uint32, C.POINTER(KernelTime), C.c_uint32,
+                                              C.POINTER(C.c_uint32)]),
+    "pxlflip_batch_get_info": (C.c_int, [_H, C.c_uint32, C.POINTER(Info)]),
+    "pxlflip_batch_read_particles": (C.c_int, [_H, C.c_uint32, C.c_void_p]),
+    "pxlflip_batch_last_solve": (C.c_int, [_H, C.c_uint32, C.POINTER(SolveStats)]),
+    "pxlflip_batch_last_error": (C.c_char_p, [_H]),
+    "pxlflip_batch_destroy": (None, [_H]),
 }
 
 _lib = None
@@ -433,6 +444,74 @@ class Server:
     def close(self):
         if self._h:
             self._lib.pxlflip_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """A fixed set of liquids advanced in lock step by the batched launches of the server
+    (pxlflip_batch_* in pxlflip.h), mixed-precision solve; no serving loop."""
+
+    def __init__(self, fluids, fps: float = 24.0, seeds=None, **opts):
+        self._lib = load_library()
+        self._h = C.c_void_p()
+        n = len(fluids)
+        arr = (C.c_char_p * n)(*[_json(f) for f in fluids])
+        sd = (C.c_uint32 * n)(*(seeds if seeds is not None else [0] * n))
+        o = make_server_opts(**opts)
+        rc = self._lib.pxlflip_batch_create(arr, sd, n, fps, C.byref(o), C.byref(self._h))
+        if rc != OK:
+            raise FluidError(rc, self._lib.pxlflip_batch_last_error(None).decode())
+        self.n = n
+        self.infos = []
+        for i in range(n):
+            info = Info()
+            self._check(self._lib.pxlflip_batch_get_info(self._h, i, C.byref(info)))
+            self.infos.append(info.as_dict())
+
+    def _check(self, rc: int):
+        if rc != OK:
+            msg = self._lib.pxlflip_batch_last_error(self._h)
+            raise FluidError(rc, msg.decode() if msg else "")
+
+    def step(self, n: int = 1):
+        self._check(self._lib.pxlflip_batch_step(self._h, int(n)))
+
+    def profile_steps(self, n: int = 1) -> list:
+        """Runs n steps with every launch timed by HIP events:
+        [{"kernel", "launches", "total_ms", "alg_bytes"}] in first-launch order."""
+        arr = (KernelTime * 128)()
+        cnt = C.c_uint32()
+        self._check(self._lib.pxlflip_batch_profile_steps(self._h, int(n), arr, 128, C.byref(cnt)))
+        return [{"kernel": arr[k].name.decode(), "launches": int(arr[k].launches),
+                 "total_ms": float(arr[k].total_ms), "alg_bytes": float(arr[k].alg_bytes)}
+                for k in range(min(cnt.value, 128))]
+
+    def read_particles(self, index: int) -> np.ndarray:
+        out = np.zeros((self.infos[index]["n_particles"], 8), np.float32)
+        self._check(self._lib.pxlflip_batch_read_particles(self._h, index, out.ctypes.data))
+        return out
+
+    def last_solve(self, index: int) -> dict:
+        st = SolveStats()
+        self._check(self._lib.pxlflip_batch_last_solve(self._h, index, C.byref(st)))
+        return {"rel_residual": st.rel_residual, "inner_iterations": int(st.inner_iterations),
+                "outer_iterations": int(st.outer_iterations)}
+
+    def close(self):
+        if self._h:
+            self._lib.pxlflip_batch_destroy(self._h)
             self._h = C.c_void_p()
 
     def __enter__(self):

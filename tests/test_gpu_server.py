@@ -123,3 +123,18 @@ def test_cancel_and_invalid_request(pxl):
             srv.submit({"name": "bad", "resolution": 8}, FPS, steps=1)
         assert "missing field `bounds`" in str(e.value)
         assert srv.stats()["cancelled"] == 1
+
+
+def test_fixed_batch_matches_server_and_prices_launches(pxl):
+    fl = [dam_break(24), two_blocks(16)]
+    with pxl.Batch(fl, FPS, solver_workgroups=WG) as b:
+        b.step(2)
+        pb = [b.read_particles(i) for i in range(2)]
+        for i in range(2):
+            assert 0 <= b.last_solve(i)["rel_residual"] <= 1e-6
+        prof = b.profile_steps(1)
+    parts, _, _ = _run(pxl, [(fl[0], 2), (fl[1], 2)])
+    assert np.array_equal(pb[0], parts[0]) and np.array_equal(pb[1], parts[1])
+    by = {k["kernel"]: k for k in prof}
+    for name in ("p2g", "g2p", "faces", "mg_sweep", "mgcg_update", "mg_tail"):
+        assert by[name]["launches"] > 0 and by[name]["total_ms"] > 0 and by[name]["alg_bytes"] > 0, name
