@@ -141,7 +141,7 @@ def bench_serve(args, pxl):
     warm = [random_box(100000 + i, sizes=sizes) for i in range(args.warmup)]
     with pxl.Server(**opts) as srv:
         for i, f in enumerate(warm):
-            srv.submit(f, FPS, seed=i, steps=max(2, args.steps // 4))
+            srv.submit(f, FPS, seed=i, steps=max(2, args.steps // 2))
         srv.drain()
         wst = srv.stats()
     rate = args.rate
@@ -254,7 +254,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60, help="steps per request (dam: timed steps)")
-    ap.add_argument("--warmup", type=int, default=32, help="warm-up requests (dam: warm-up steps)")
+    ap.add_argument("--warmup", type=int, default=64, help="warm-up requests (dam: warm-up steps)")
     ap.add_argument("--workload", default="serve", choices=["serve", "dam"])
     ap.add_argument("--requests", type=int, default=512)
     ap.add_argument("--sizes", default="32,48,64,96,128")

@@ -123,7 +123,7 @@ int prepare(const char *fluid_json, double fps, uint32_t seed, const pxlflip_opt
     d.ntiles = (uint32_t)d.ntx * (uint32_t)d.nty * (uint32_t)d.ntz;
     d.nkeys = d.ntiles * TILE_CELLS;
     d.cg_blocks = (int)(o.solver_workgroups ? o.solver_workgroups : DEFAULT_SOLVER_BLOCKS);
-    d.split_colours = o.solver == PXLFLIP_SOLVER_RBGS ? 1 : 0;
+    d.split_colours = o.solver == PXLFLIP_SOLVER_CG ? 0 : 1;  // RBGS and the multigrid sweep one colour at a time
     d.mg_levels = 0;
     if (o.solver == PXLFLIP_SOLVER_MIXED) {
         int l = 0;

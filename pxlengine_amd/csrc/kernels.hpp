@@ -42,11 +42,11 @@ struct MGLevel {
     uint8_t *type;   // level-l cell types (levels >= 1)
     uint16_t *sten;  // level-l stencil words (same layout as DevSim::sten; levels >= 1)
     float *r, *z;    // level-l right-hand side (the scaled restricted residual) and iterate
-    // FLUID cells of the level (levels 1..mg_tail, rebuilt each step, any order): cell index
-    // with bit 31 set for black cells; *count entries
+    // FLUID cells of the level (levels >= 1, rebuilt each step): red cells from the front
+    // (count[0] of them), black cells from the back (count[1]: list[cells - 1 - t]), each in
+    // any order
     uint32_t *list, *count;
 };
-constexpr uint32_t LIST_BLACK = 1u << 31;
 
 // Solver scalar slots (fp64 array): two slots of SC_COUNT by CG-iteration parity for the
 // (alpha, beta, gamma) history; the refinement norms live in slot 0.
@@ -64,7 +64,7 @@ struct DevSim {
     int ntx, nty, ntz;  // tiles per axis
     uint32_t nkeys;     // ntx * nty * ntz * TILE_CELLS
     int cg_blocks;      // workgroups of the fluid-list solver launches (tunable)
-    int split_colours;  // fluid list red cells first then black (RBGS) or in cell order
+    int split_colours;  // fluid list red cells first then black (RBGS, MIXED) or in cell order (CG)
     // particles: current state (a) and sorted scratch (b)
     float *ax, *ay, *az, *avx, *avy, *avz;
     uint32_t *aid;

@@ -21,9 +21,11 @@
 //             right-hand side sums the red children's residuals only
 //   post:     black update from red neighbours' z + prolonged correction (the red cells'
 //             corrected values are only ever read here), then sweep(R), sweep(B), sweep(R).
-// Levels below mg_tail are walked through lists of their FLUID cells (the step's fluid list
-// on level 0, per-level lists built with the coarse types), so the work follows the liquid,
-// not the grid; a launch takes the list in cg_blocks slices, like the solver kernels.
+// Levels below mg_tail are walked through lists of their FLUID cells, red cells apart from
+// black ones (the step's colour-split fluid list on level 0, per-level lists built with the
+// coarse types), so the work follows the liquid, not the grid, and a colour sweep keeps every
+// lane busy; a launch takes a list segment in cg_blocks slices, like the solver kernels, and
+// a lane updates two cells with all their loads in flight together.
 // Coarse z is kept 0 on non-FLUID cells (the prolongation reads the parent of any FLUID
 // cell).  Levels from mg_tail on (a few thousand cells together) run the same cycle in one
 // workgroup with everything in LDS.  Arithmetic is fp32 (the FP64 refinement outside fixes
@@ -78,7 +80,10 @@ __device__ __forceinline__ uint32_t w_black(uint32_t w) { return (w >> 27) & 1u;
 // ---- level types (c from its children in f); zeroes c's list count
 __device__ __forceinline__ void body_mg_types(const MGLevel &f, const MGLevel &c, uint32_t b) {
     const uint32_t id = b * MBLOCK + threadIdx.x;
-    if (id == 0) *c.count = 0;
+    if (id == 0) {
+        c.count[0] = 0;
+        c.count[1] = 0;
+    }
     if (id >= ncell(c)) return;
     int i, j, k;
     cell_ijk(c.nx, c.ny, id, i, j, k);
@@ -99,7 +104,8 @@ __global__ __launch_bounds__(MBLOCK) void k_mg_types(MGLevel f, MGLevel c) { bod
 __device__ __forceinline__ void body_mg_sten(const MGLevel &v, uint32_t b) {
     const uint32_t id = b * MBLOCK + threadIdx.x;
     const bool in = id < ncell(v);
-    uint32_t st = 0, entry = 0;
+    uint32_t st = 0;
+    bool black = false;
     if (in) {
         int i, j, k;
         cell_ijk(v.nx, v.ny, id, i, j, k);
@@ -113,23 +119,30 @@ __device__ __forceinline__ void body_mg_sten(const MGLevel &v, uint32_t b) {
                 if (t == CELL_FLUID) st |= 1u << n;
             }
             st |= STEN_FLUID | (diag << 8);
-            entry = id | (((i + j + k) & 1) ? LIST_BLACK : 0u);
+            black = ((i + j + k) & 1) != 0;
         }
         v.sten[id] = (uint16_t)st;
         v.z[id] = 0.0f;
     }
     const bool fl = (st & STEN_FLUID) != 0;
-    const uint64_t mask = __ballot(fl);
     const int lane = threadIdx.x & (WAVE - 1);
-    const int leader = mask ? __ffsll((long long)mask) - 1 : 0;
-    uint32_t base = 0;
-    if (fl && lane == leader) base = atomicAdd(v.count, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader, WAVE);
-    if (fl) v.list[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = entry;
+    const uint64_t below = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int col = 0; col < 2; col++) {
+        const bool mine = fl && black == (col == 1);
+        const uint64_t mask = __ballot(mine);
+        const int leader = mask ? __ffsll((long long)mask) - 1 : 0;
+        uint32_t base = 0;
+        if (mine && lane == leader) base = atomicAdd(&v.count[col], (uint32_t)__popcll(mask));
+        base = __shfl(base, leader, WAVE);
+        const uint32_t slot = base + (uint32_t)__popcll(mask & below);
+        if (mine) v.list[col ? ncell(v) - 1u - slot : slot] = id;
+    }
 }
 __global__ __launch_bounds__(MBLOCK) void k_mg_sten(MGLevel v) { body_mg_sten(v, blockIdx.x); }
 
-// ---- level 0 (the step's fluid list; r, u are the CG vectors)
+// ---- level 0 (the step's colour-split fluid list: red entries [0, n_red), black [n_red, n);
+// r, u are the CG vectors)
 // pre0: red u = r / d, black u = (r + sum FLUID red nbrs r_n / d_n) / d
 __device__ __forceinline__ void body_l0_pre0(const DevSim &S, const float *__restrict__ r, float *__restrict__ u,
                                              uint32_t b, uint32_t nb) {
@@ -147,40 +160,55 @@ __device__ __forceinline__ void body_l0_pre0(const DevSim &S, const float *__res
         u[id] = (r[id] + sum) * rcp_or0(w_diag(wd));
     }
 }
-// one colour's update
+// One colour's update over its list segment.  A lane takes entries t and t + LBLOCK and issues
+// the loads of both before either store (they read only the other colour, so the order of the
+// two updates does not matter).
 __device__ __forceinline__ void body_l0_sweep(const DevSim &S, const float *__restrict__ r, float *__restrict__ u,
                                               uint32_t colour, uint32_t b, uint32_t nb) {
+    const uint32_t n_red = S.fl_count[0], n = S.fl_count[1];
+    const uint32_t lo = colour ? n_red : 0u, hi = colour ? n : n_red;
     uint32_t beg, end;
-    slice(S.fl_count[1], b, nb, beg, end);
+    slice(hi - lo, b, nb, beg, end);
+    beg += lo;
+    end += lo;
     const int32_t off[6] = {-1, 1, -S.nx, S.nx, -S.nx * S.ny, S.nx * S.ny};
-    for (uint32_t t = beg + threadIdx.x; t < end; t += LBLOCK) {
-        const uint32_t wd = S.fl_word[t];
-        if (w_black(wd) != colour) continue;
-        const uint32_t id = S.fl_list[t];
-        float sum = 0.0f;
+    for (uint32_t t = beg + threadIdx.x; t < end; t += 2 * LBLOCK) {
+        const uint32_t t2 = t + LBLOCK;
+        const bool two = t2 < end;
+        const uint32_t wa = S.fl_word[t], ia = S.fl_list[t];
+        const uint32_t wb = two ? S.fl_word[t2] : 0u, ib = two ? S.fl_list[t2] : ia;
+        float sa = 0.0f, sb = 0.0f;
 #pragma unroll
-        for (int n = 0; n < 6; n++)
-            if ((wd >> n) & 1u) sum += u[id + off[n]];
-        u[id] = (r[id] + sum) * rcp_or0(w_diag(wd));
+        for (int q = 0; q < 6; q++) {
+            const float za = ((wa >> q) & 1u) ? u[ia + off[q]] : 0.0f;
+            const float zb = ((wb >> q) & 1u) ? u[ib + off[q]] : 0.0f;
+            sa += za;
+            sb += zb;
+        }
+        const float ra = r[ia], rb = r[ib];
+        u[ia] = (ra + sa) * rcp_or0(w_diag(wa));
+        if (two) u[ib] = (rb + sb) * rcp_or0(w_diag(wb));
     }
 }
 // post: black update from red neighbours' u + the prolonged correction c.z
 __device__ __forceinline__ void body_l0_post(const DevSim &S, const MGLevel &c, const float *__restrict__ r,
                                              float *__restrict__ u, uint32_t b, uint32_t nb) {
+    const uint32_t n_red = S.fl_count[0], n = S.fl_count[1];
     uint32_t beg, end;
-    slice(S.fl_count[1], b, nb, beg, end);
+    slice(n - n_red, b, nb, beg, end);
+    beg += n_red;
+    end += n_red;
     for (uint32_t t = beg + threadIdx.x; t < end; t += LBLOCK) {
         const uint32_t wd = S.fl_word[t];
-        if (!w_black(wd)) continue;
         const uint32_t id = S.fl_list[t];
         int i, j, k;
         cell_ijk(S.nx, S.ny, id, i, j, k);
         const int nbc[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k}, {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
         float sum = 0.0f;
 #pragma unroll
-        for (int n = 0; n < 6; n++) {
-            if ((wd >> n) & 1u) {
-                const int a = nbc[n][0], bb = nbc[n][1], d = nbc[n][2];
+        for (int q = 0; q < 6; q++) {
+            if ((wd >> q) & 1u) {
+                const int a = nbc[q][0], bb = nbc[q][1], d = nbc[q][2];
                 sum += u[(uint32_t)a + (uint32_t)S.nx * ((uint32_t)bb + (uint32_t)S.ny * (uint32_t)d)] +
                        c.z[parent(c.nx, c.ny, a, bb, d)];
             }
@@ -189,21 +217,25 @@ __device__ __forceinline__ void body_l0_post(const DevSim &S, const MGLevel &c, 
     }
 }
 
-// ---- coarse levels 1..mg_tail-1 (the level's list)
+// ---- coarse levels 1..mg_tail-1 (the level's red / black lists)
+__device__ __forceinline__ uint32_t lc_entry(const MGLevel &v, uint32_t t, uint32_t n_red) {
+    return t < n_red ? v.list[t] : v.list[ncell(v) - 1u - (t - n_red)];
+}
 __device__ __forceinline__ void body_lc_pre0(const MGLevel &v, uint32_t b, uint32_t nb) {
+    const uint32_t n_red = v.count[0], n = n_red + v.count[1];
     uint32_t beg, end;
-    slice(*v.count, b, nb, beg, end);
+    slice(n, b, nb, beg, end);
     const int32_t off[6] = {-1, 1, -v.nx, v.nx, -v.nx * v.ny, v.nx * v.ny};
     for (uint32_t t = beg + threadIdx.x; t < end; t += LBLOCK) {
-        const uint32_t e = v.list[t], id = e & ~LIST_BLACK;
+        const uint32_t id = lc_entry(v, t, n_red);
         const uint32_t st = v.sten[id];
         float sum = 0.0f;
-        if (e & LIST_BLACK) {
+        if (t >= n_red) {
 #pragma unroll
-            for (int n = 0; n < 6; n++) {
-                if (st & (1u << n)) {
-                    const uint32_t q = id + off[n];
-                    sum += v.r[q] * rcp_or0(st_diag(v.sten[q]));
+            for (int q = 0; q < 6; q++) {
+                if (st & (1u << q)) {
+                    const uint32_t nq = id + off[q];
+                    sum += v.r[nq] * rcp_or0(st_diag(v.sten[nq]));
                 }
             }
         }
@@ -211,38 +243,47 @@ __device__ __forceinline__ void body_lc_pre0(const MGLevel &v, uint32_t b, uint3
     }
 }
 __device__ __forceinline__ void body_lc_sweep(const MGLevel &v, uint32_t colour, uint32_t b, uint32_t nb) {
+    const uint32_t n = v.count[colour];
+    const uint32_t *list = v.list;
+    const uint32_t last = ncell(v) - 1u;
     uint32_t beg, end;
-    slice(*v.count, b, nb, beg, end);
+    slice(n, b, nb, beg, end);
     const int32_t off[6] = {-1, 1, -v.nx, v.nx, -v.nx * v.ny, v.nx * v.ny};
-    const uint32_t want = colour ? LIST_BLACK : 0u;
-    for (uint32_t t = beg + threadIdx.x; t < end; t += LBLOCK) {
-        const uint32_t e = v.list[t];
-        if ((e & LIST_BLACK) != want) continue;
-        const uint32_t id = e & ~LIST_BLACK;
-        const uint32_t st = v.sten[id];
-        float sum = 0.0f;
+    for (uint32_t t = beg + threadIdx.x; t < end; t += 2 * LBLOCK) {
+        const uint32_t t2 = t + LBLOCK;
+        const bool two = t2 < end;
+        const uint32_t ia = colour ? list[last - t] : list[t];
+        const uint32_t ib = two ? (colour ? list[last - t2] : list[t2]) : ia;
+        const uint32_t sta = v.sten[ia], stb = two ? (uint32_t)v.sten[ib] : 0u;
+        float sa = 0.0f, sb = 0.0f;
 #pragma unroll
-        for (int n = 0; n < 6; n++)
-            if (st & (1u << n)) sum += v.z[id + off[n]];
-        v.z[id] = (v.r[id] + sum) * rcp_or0(st_diag(st));
+        for (int q = 0; q < 6; q++) {
+            const float za = (sta & (1u << q)) ? v.z[ia + off[q]] : 0.0f;
+            const float zb = (stb & (1u << q)) ? v.z[ib + off[q]] : 0.0f;
+            sa += za;
+            sb += zb;
+        }
+        const float ra = v.r[ia], rb = v.r[ib];
+        v.z[ia] = (ra + sa) * rcp_or0(st_diag(sta));
+        if (two) v.z[ib] = (rb + sb) * rcp_or0(st_diag(stb));
     }
 }
 __device__ __forceinline__ void body_lc_post(const MGLevel &v, const MGLevel &c, uint32_t b, uint32_t nb) {
+    const uint32_t n = v.count[1];
+    const uint32_t last = ncell(v) - 1u;
     uint32_t beg, end;
-    slice(*v.count, b, nb, beg, end);
+    slice(n, b, nb, beg, end);
     for (uint32_t t = beg + threadIdx.x; t < end; t += LBLOCK) {
-        const uint32_t e = v.list[t];
-        if (!(e & LIST_BLACK)) continue;
-        const uint32_t id = e & ~LIST_BLACK;
+        const uint32_t id = v.list[last - t];
         const uint32_t st = v.sten[id];
         int i, j, k;
         cell_ijk(v.nx, v.ny, id, i, j, k);
         const int nbc[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k}, {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
         float sum = 0.0f;
 #pragma unroll
-        for (int n = 0; n < 6; n++) {
-            if (st & (1u << n)) {
-                const int a = nbc[n][0], bb = nbc[n][1], d = nbc[n][2];
+        for (int q = 0; q < 6; q++) {
+            if (st & (1u << q)) {
+                const int a = nbc[q][0], bb = nbc[q][1], d = nbc[q][2];
                 sum += v.z[lidx(v, a, bb, d)] + c.z[parent(c.nx, c.ny, a, bb, d)];
             }
         }
@@ -256,11 +297,12 @@ __device__ __forceinline__ void body_lc_post(const MGLevel &v, const MGLevel &c,
 __device__ __forceinline__ void body_restrict(int fnx, int fny, int fnz, const uint16_t *__restrict__ fsten,
                                               const float *__restrict__ fr, const float *__restrict__ fz,
                                               const MGLevel &c, uint32_t b, uint32_t nb) {
+    const uint32_t n_red = c.count[0], n = n_red + c.count[1];
     uint32_t beg, end;
-    slice(*c.count, b, nb, beg, end);
+    slice(n, b, nb, beg, end);
     const int32_t off[6] = {-1, 1, -fnx, fnx, -fnx * fny, fnx * fny};
     for (uint32_t t = beg + threadIdx.x; t < end; t += LBLOCK) {
-        const uint32_t cc = c.list[t] & ~LIST_BLACK;
+        const uint32_t cc = lc_entry(c, t, n_red);
         int ci, cj, ck;
         cell_ijk(c.nx, c.ny, cc, ci, cj, ck);
         const int red[4] = {0, 3, 5, 6};

@@ -99,14 +99,15 @@ double share_bytes(const char *k, const DevSim &d, const MemberCounts &m, int l)
     const std::string n(k);
     if (n == "clear_grid") return 4 * K;
     if (n == "bin_count") return 20 * P + 4 * C;
-    if (n == "scan_reduce") return 4 * (l == 0 ? K : C);
-    if (n == "scan_down") return 8 * (l == 0 ? K : C);
+    const double flags = d.split_colours ? 2 * C : C;
+    if (n == "scan_reduce") return 4 * (l == 0 ? K : flags);
+    if (n == "scan_down") return 8 * (l == 0 ? K : flags);
     if (n == "tile_list") return 8.0 * d.ntiles;
     if (n == "scatter") return 68 * P;
     if (n == "p2g") return 24 * P + box * t;
     if (n == "faces") return 8 * F + box * t;
     if (n == "cells") return 4 * F + 15 * C + 4 * (C - Cf);
-    if (n == "fluid_compact") return 10 * C + 8 * Cf;
+    if (n == "fluid_compact") return 8 * flags + 2 * C + 8 * Cf;
     if (n == "mg_types") return cells_of(l - 1) + cells_of(l);
     if (n == "mg_sten") return 7 * cells_of(l) + 4 * cl;
     if (n == "mixed_init") return 32 * Cf;
@@ -459,7 +460,9 @@ private:
         if (relayout) {
             size_t total = 0;
             for (int d = 0; d < WD_COUNT; d++) {
-                if (need[d] > cap_[d]) cap_[d] = need[d] + need[d] / 2 + 64;
+                // every domain gets headroom at once, so growth rarely relays out (and
+                // re-captures) again
+                cap_[d] = std::max<size_t>(cap_[d], std::max<size_t>(2 * need[d], 16384));
                 off_[d] = total;
                 total += cap_[d];
             }
@@ -517,12 +520,13 @@ private:
     std::vector<std::pair<std::string, Agg>> agg_;
 };
 
-// Solver workgroups of a batched simulation: enough lists slices to cover the fluid cells a
-// grid of this size can hold with ~4096 cells per workgroup, a multiple of 8 (XCDs).
-uint32_t batch_solver_blocks(const DevSim &d) {
-    uint32_t g = d.ncells / 8192u;
-    g = std::max<uint32_t>(8u, std::min<uint32_t>(256u, g));
-    return (g + 7u) / 8u * 8u;
+// Solver workgroups of a batched simulation: one per ~2048 FLUID cells it is expected to hold
+// (its particles over the seeding density), so that a workgroup's slice of a list keeps its
+// lanes busy.
+uint32_t batch_solver_blocks(const Sim &s) {
+    const double fluid = (double)s.d.np / (double)std::max<uint32_t>(1u, s.spec.particles_per_cell);
+    const double g = std::ceil(fluid / 2048.0);
+    return (uint32_t)std::max(1.0, std::min(256.0, g));
 }
 
 struct Request {
@@ -539,6 +543,8 @@ struct Request {
     uint32_t np = 0;
     uint32_t steps_done = 0;  // updated by the device thread under the server lock
     std::string alloc_error;  // set when admission failed to allocate
+    bool allocated = false;   // set by an allocator thread (under the lock) when r->sim is ready
+    bool in_batch = false;    // member of the batch (device thread only)
     std::unique_ptr<Sim> sim;
     std::vector<float> result;
     double rel_residual = -1.0;
@@ -555,12 +561,14 @@ struct pxlflip_server {
     std::mutex mu;
     std::condition_variable cv;       // worker wake-up
     std::condition_variable done_cv;  // request state changes
-    std::deque<pxl::Request *> queue;
-    std::vector<pxl::Request *> running;
+    std::deque<pxl::Request *> queue;        // submitted, waiting for admission
+    std::deque<pxl::Request *> to_allocate;  // admitted, waiting for an allocator thread
+    std::vector<pxl::Request *> running;     // admitted and not finished (allocating or stepping)
     std::map<uint64_t, std::unique_ptr<pxl::Request>> all;
     uint64_t next_id = 1;
     bool stop = false;
     std::thread worker;
+    std::vector<std::thread> allocators;  // allocate + seed admitted requests off the device thread
     std::string error;
     // accounting
     uint64_t bytes_in_use = 0, peak_bytes = 0;
@@ -569,23 +577,23 @@ struct pxlflip_server {
     double particle_steps = 0, first_submit = -1, last_done = -1, busy_s = 0;
     uint64_t batch_steps = 0;
     double batch_members = 0;
-    double admit_s = 0;  // device-thread time spent allocating admitted requests
+    double admit_s = 0;  // allocator-thread time spent allocating admitted requests
 
     void run();
-    void admit(pxl::Request *r);
+    void allocate_loop();
     void finish(pxl::Request *r, int status, const std::string &err);
 };
 
 namespace pxl {
 namespace {
 
-// Allocates a request's simulation (device thread).  Throws on failure.
+// Allocates a request's simulation on its own stream (allocator thread).  Throws on failure.
 void allocate_request(pxlflip_server *srv, Request *r) {
     auto sim = std::make_unique<Sim>();
     std::vector<float> x0, v0;
     if (prepare(r->json.c_str(), r->fps, r->seed, &srv->sim_opts, *sim, &x0, &v0) != PXLFLIP_OK)
         throw std::runtime_error(g_create_error);
-    sim->d.cg_blocks = (int)(srv->opts.solver_workgroups ? srv->opts.solver_workgroups : batch_solver_blocks(sim->d));
+    sim->d.cg_blocks = (int)(srv->opts.solver_workgroups ? srv->opts.solver_workgroups : batch_solver_blocks(*sim));
     HIP_CHECK(hipStreamCreateWithFlags(&sim->stream, hipStreamNonBlocking));
     sim->allocate(x0, v0);
     sim->reset();
@@ -613,6 +621,39 @@ void pxlflip_server::finish(pxl::Request *r, int status, const std::string &err)
     done_cv.notify_all();
 }
 
+void pxlflip_server::allocate_loop() {
+    using namespace pxl;
+    try {
+        HIP_CHECK(hipSetDevice(opts.device));
+    } catch (const HipError &) {
+    }
+    std::unique_lock<std::mutex> lk(mu);
+    while (true) {
+        cv.wait(lk, [&] { return stop || !to_allocate.empty(); });
+        if (stop) break;
+        Request *r = to_allocate.front();
+        to_allocate.pop_front();
+        lk.unlock();
+        const double t0 = now_s();
+        std::string err;
+        try {
+            allocate_request(this, r);
+        } catch (const HipError &e) {
+            err = std::string("HIP error in ") + e.what + ": " + hipGetErrorString(e.code);
+        } catch (const std::bad_alloc &) {
+            err = "out of device memory";
+        } catch (const std::exception &e) {
+            err = e.what();
+        }
+        if (!err.empty()) r->sim.reset();
+        lk.lock();
+        admit_s += now_s() - t0;
+        r->alloc_error = err;
+        r->allocated = true;
+        cv.notify_all();
+    }
+}
+
 void pxlflip_server::run() {
     using namespace pxl;
     try {
@@ -622,8 +663,7 @@ void pxlflip_server::run() {
     std::unique_lock<std::mutex> lk(mu);
     while (!stop) {
         // admission: first in, first out, while the budget and the batch size allow.  An
-        // admitted request is `running` from here on (allocated below, without the lock).
-        std::vector<Request *> admitted;
+        // admitted request is `running` from here on; an allocator thread prepares it.
         while (!queue.empty()) {
             Request *r = queue.front();
             if (r->cancel) {
@@ -637,33 +677,40 @@ void pxlflip_server::run() {
             peak_bytes = std::max(peak_bytes, bytes_in_use);
             r->status = PXLFLIP_REQ_RUNNING;
             r->t_admit = now_s();
-            admitted.push_back(r);
             running.push_back(r);
+            to_allocate.push_back(r);
+            cv.notify_all();
         }
-        if (running.empty()) {
-            cv.wait(lk, [&] { return stop || !queue.empty(); });
+        // allocated requests join the batch at this step boundary
+        std::vector<Request *> joining;
+        size_t in_batch = 0;
+        for (Request *r : running) {
+            if (r->in_batch) in_batch++;
+            else if (r->allocated && r->alloc_error.empty() && r->steps > 0) joining.push_back(r);
+        }
+        bool retire_only = false;
+        for (Request *r : running)
+            if (r->allocated && (!r->alloc_error.empty() || r->steps == 0 || r->cancel)) retire_only = true;
+        if (in_batch + joining.size() == 0 && !retire_only) {
+            cv.wait(lk, [&] {
+                if (stop) return true;
+                for (Request *r : running)
+                    if (r->allocated) return true;
+                return running.size() < opts.max_sims && !queue.empty() &&
+                       bytes_in_use + queue.front()->bytes <= opts.budget_bytes;
+            });
             continue;
         }
         lk.unlock();
-        // device work without the lock
-        const double ta = now_s();
-        for (Request *r : admitted) {
-            try {
-                allocate_request(this, r);
-                if (r->steps > 0) batch->add(r->sim.get());
-            } catch (const HipError &e) {
-                r->alloc_error = std::string("HIP error in ") + e.what + ": " + hipGetErrorString(e.code);
-            } catch (const std::bad_alloc &) {
-                r->alloc_error = "out of device memory";
-            } catch (const std::exception &e) {
-                r->alloc_error = e.what();
-            }
-        }
-        const double admit_time = now_s() - ta;
         std::string step_error;
         double t0 = now_s(), t1 = t0;
-        const size_t members = batch->size();
+        size_t members = 0;
         try {
+            for (Request *r : joining) {
+                batch->add(r->sim.get());
+                r->in_batch = true;
+            }
+            members = batch->size();
             if (members) {
                 batch->step();
                 batch->sync();
@@ -675,7 +722,6 @@ void pxlflip_server::run() {
             step_error = e.what();
         }
         lk.lock();
-        admit_s += admit_time;
         if (members) {
             batch_steps++;
             batch_members += (double)members;
@@ -684,13 +730,17 @@ void pxlflip_server::run() {
         // retire finished, cancelled or failed requests at the step boundary
         std::vector<Request *> keep;
         for (Request *r : running) {
+            if (!r->allocated) {  // still with an allocator thread
+                keep.push_back(r);
+                continue;
+            }
             Sim *s = r->sim.get();
             if (!s) {  // allocation failed
                 bytes_in_use -= r->bytes;
                 finish(r, PXLFLIP_REQ_FAILED, r->alloc_error);
                 continue;
             }
-            if (members && r->steps > 0 && step_error.empty()) {
+            if (r->in_batch && members && step_error.empty()) {
                 r->rel_residual = s->stats.rel_residual;
                 r->max_outer = std::max(r->max_outer, s->stats.outer_iterations);
                 r->steps_done = (uint32_t)s->steps_done;
@@ -703,7 +753,8 @@ void pxlflip_server::run() {
             lk.unlock();
             std::string err = step_error;
             try {
-                if (r->steps > 0) batch->remove(s);
+                if (r->in_batch) batch->remove(s);
+                r->in_batch = false;
                 if (done && err.empty() && r->keep) {
                     r->result.assign((size_t)s->d.np * 8, 0.0f);
                     DeviceBuffer buf((size_t)s->d.np * 32);
@@ -722,11 +773,15 @@ void pxlflip_server::run() {
         }
         running.swap(keep);
     }
-    // stop: cancel what is left
+    // stop: wait for the allocators to leave, then cancel what is left
+    lk.unlock();
+    for (auto &t : allocators)
+        if (t.joinable()) t.join();
+    lk.lock();
     for (Request *r : running) {
         lk.unlock();
         try {
-            if (r->sim && r->steps > 0) batch->remove(r->sim.get());
+            if (r->in_batch) batch->remove(r->sim.get());
         } catch (...) {
         }
         r->sim.reset();
@@ -735,6 +790,7 @@ void pxlflip_server::run() {
         finish(r, PXLFLIP_REQ_CANCELLED, "server stopped");
     }
     running.clear();
+    to_allocate.clear();
     while (!queue.empty()) {
         finish(queue.front(), PXLFLIP_REQ_CANCELLED, "server stopped");
         queue.pop_front();
@@ -839,6 +895,7 @@ int pxlflip_server_create(const pxlflip_server_opts *opts, pxlflip_server **out)
         return PXLFLIP_ERR_OOM;
     }
     pxlflip_server *p = srv.release();
+    for (int k = 0; k < 2; k++) p->allocators.emplace_back([p] { p->allocate_loop(); });
     p->worker = std::thread([p] { p->run(); });
     *out = p;
     return PXLFLIP_OK;
@@ -856,7 +913,7 @@ int pxlflip_server_submit(pxlflip_server *srv, const char *fluid_json, double fp
         srv->error = g_create_error;
         return PXLFLIP_ERR_INVALID;
     }
-    probe.d.cg_blocks = (int)(srv->opts.solver_workgroups ? srv->opts.solver_workgroups : batch_solver_blocks(probe.d));
+    probe.d.cg_blocks = (int)(srv->opts.solver_workgroups ? srv->opts.solver_workgroups : batch_solver_blocks(probe));
     r->bytes = Sim::plan_total(probe.plan());
     r->np = probe.d.np;
     r->json = fluid_json;
@@ -1105,7 +1162,7 @@ int pxlflip_batch_create(const char *const *fluid_jsons, const uint32_t *seeds, 
             b->sim_opts.scene_index = i;
             const int rc = prepare(fluid_jsons[i], fps, seeds ? seeds[i] : 0u, &b->sim_opts, *sim, &x0, &v0);
             if (rc != PXLFLIP_OK) return rc;
-            sim->d.cg_blocks = (int)(o.solver_workgroups ? o.solver_workgroups : batch_solver_blocks(sim->d));
+            sim->d.cg_blocks = (int)(o.solver_workgroups ? o.solver_workgroups : batch_solver_blocks(*sim));
             HIP_CHECK(hipStreamCreateWithFlags(&sim->stream, hipStreamNonBlocking));
             sim->allocate(x0, v0);
             sim->reset();
