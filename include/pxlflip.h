@@ -181,7 +181,7 @@ typedef struct pxlflip_kernel_time {
 int pxlflip_profile_steps(pxlflip *h, uint32_t n, pxlflip_kernel_time *out, uint32_t max_out, uint32_t *n_out);
 
 /* Autotuner of the solver launch shape: for each candidate workgroup count of the pressure
- * kernels (256 .. 4096) runs `steps` timed steps, keeps the fastest, and restores the
+ * kernels (64, 128, 256, 512, 1024) runs `steps` timed steps, keeps the fastest, and restores the
  * particles, ids and pressure so the simulation state and steps_done are unchanged.  Returns
  * the choice and its average event-timed step time. */
 int pxlflip_autotune(pxlflip *h, uint32_t steps, uint32_t *chosen_workgroups, double *best_ms);
