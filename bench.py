@@ -22,7 +22,8 @@ fixed batch of stream requests (HIP events on the batch stream, launches priced 
 algorithmic bytes over the simulations each covers).  `cpu_baseline` times the C oracle
 (single thread) on a bounded sample of the stream's requests.
 
-`--workload dam` measures the single-instance 128^3 dam break on the plain path instead.
+`--workload dam` measures the single-instance 128^3 dam break on the plain path instead;
+`--workload batch` only the fixed batch used for the roofline (for profiling runs).
 """
 from __future__ import annotations
 
@@ -188,23 +189,8 @@ def bench_serve(args, pxl):
              "path": "one simulation at a time, FP32 Jacobi-PCG 80 fixed iterations, hipGraph per step, "
                      "creation included (steps_only: step time alone)"}
 
-    # roofline: one profiled batched step of a fixed batch of stream requests
     nb = min(args.profile_batch, len(reqs))
-    with pxl.Batch(reqs[:nb], FPS, seeds=list(range(nb))) as bat:
-        bat.step(2)
-        prof = bat.profile_steps(2)
-    prof.sort(key=lambda k: -k["total_ms"])
-    tot = sum(k["total_ms"] for k in prof)
-    kernels = [{"kernel": k["kernel"], "launches": k["launches"], "avg_us": 1e3 * k["total_ms"] / k["launches"],
-                "share": k["total_ms"] / tot,
-                "achieved_GBs": (k["alg_bytes"] / (k["total_ms"] / 1e3) / 1e9) if k["alg_bytes"] else None}
-               for k in prof]
-    dom = kernels[0]
-    ach = dom["achieved_GBs"] or 0.0
-    roofline = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach / PEAK_HBM_GBS,
-                "traffic": measured_traffic(dom["kernel"], f"serve_batch{nb}"), "kernel": dom["kernel"],
-                "avg_us": dom["avg_us"], "alg_bytes_per_launch": prof[0]["alg_bytes"] / prof[0]["launches"],
-                "measured_on": f"profiled batched steps of the first {nb} stream requests"}
+    roofline, kernels, _ = batch_roofline(pxl, reqs[:nb], 2, 0)
 
     cpu = cpu_serve(args, reqs) if args.cpu_seconds > 0 else None
     return {
@@ -227,6 +213,52 @@ def bench_serve(args, pxl):
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels[:12],
+    }
+
+
+def batch_roofline(pxl, reqs, warm, timed):
+    """A fixed batch of `reqs` (seed = index): `warm` untimed steps, `timed` timed steps, then
+    two steps with every batched launch timed by HIP events and priced in algorithmic bytes.
+    Returns (roofline of the kernel with the largest time share, per-kernel rows, particle-steps/s
+    of the timed steps or None)."""
+    from tests.scenes import FPS
+    n = len(reqs)
+    rate = None
+    with pxl.Batch(reqs, FPS, seeds=list(range(n))) as bat:
+        bat.step(warm)
+        if timed:
+            t0 = time.perf_counter()
+            bat.step(timed)
+            rate = sum(i["n_particles"] for i in bat.infos) * timed / (time.perf_counter() - t0)
+        prof = bat.profile_steps(2)
+    prof.sort(key=lambda k: -k["total_ms"])
+    tot = sum(k["total_ms"] for k in prof)
+    kernels = [{"kernel": k["kernel"], "launches": k["launches"], "avg_us": 1e3 * k["total_ms"] / k["launches"],
+                "share": k["total_ms"] / tot, "alg_bytes_per_launch": k["alg_bytes"] / k["launches"],
+                "achieved_GBs": (k["alg_bytes"] / (k["total_ms"] / 1e3) / 1e9) if k["alg_bytes"] else None}
+               for k in prof]
+    dom = kernels[0]
+    ach = dom["achieved_GBs"] or 0.0
+    roofline = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach / PEAK_HBM_GBS,
+                "traffic": measured_traffic(dom["kernel"], f"serve_batch{n}"), "kernel": dom["kernel"],
+                "avg_us": dom["avg_us"], "alg_bytes_per_launch": dom["alg_bytes_per_launch"],
+                "measured_on": f"two profiled batched steps of the first {n} stream requests"}
+    return roofline, kernels, rate
+
+
+def bench_batch(args, pxl):
+    """The fixed batch of the first --profile-batch stream requests: particle-steps/s of
+    --steps batched steps after --warmup steps, and the per-kernel roofline."""
+    from tests.scenes import random_box
+    sizes = tuple(int(v) for v in args.sizes.split(","))
+    reqs = [random_box(i, sizes=sizes) for i in range(args.profile_batch)]
+    roofline, kernels, rate = batch_roofline(pxl, reqs, max(args.warmup, 1), args.steps)
+    return {
+        "metric": METRIC, "value": rate, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (random liquid boxes, seed = request index)",
+        "config": {"workload": f"fixed batch of the first {len(reqs)} serving-stream requests, MIXED to 1e-6"},
+        "roofline": roofline, "cpu_baseline": None, "kernels": kernels,
     }
 
 
@@ -255,7 +287,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60, help="steps per request (dam: timed steps)")
     ap.add_argument("--warmup", type=int, default=64, help="warm-up requests (dam: warm-up steps)")
-    ap.add_argument("--workload", default="serve", choices=["serve", "dam"])
+    ap.add_argument("--workload", default="serve", choices=["serve", "dam", "batch"])
     ap.add_argument("--requests", type=int, default=512)
     ap.add_argument("--sizes", default="32,48,64,96,128")
     ap.add_argument("--rate", type=float, default=0.0, help="arrivals per second (0 = measured service rate)")
@@ -272,7 +304,8 @@ def main():
         sys.exit(2)
 
     import pxlengine_amd as pxl
-    line = bench_serve(args, pxl) if args.workload == "serve" else bench_dam(args, pxl)
+    run = {"serve": bench_serve, "dam": bench_dam, "batch": bench_batch}[args.workload]
+    line = run(args, pxl)
     print(json.dumps(line))
 
 

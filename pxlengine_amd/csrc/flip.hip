@@ -10,6 +10,10 @@
 
 namespace pxl {
 thread_local std::string g_create_error;
+LaunchEvents &launch_events() {
+    static thread_local LaunchEvents ev;
+    return ev;
+}
 }  // namespace pxl
 
 using pxl::Sim;

@@ -218,19 +218,19 @@ dim3 node_grid(const DevSim &S) { return dim3((S.nx + 1 + TX - 1) / TX, (S.ny + 
 
 }  // namespace
 
-void launch_faces(const DevSim &S, hipStream_t s) { k_faces<<<node_grid(S), dim3(TX, TY), 0, s>>>(S); }
-void launch_cells(const DevSim &S, hipStream_t s) { k_cells<<<cell_grid(S), dim3(TX, TY), 0, s>>>(S); }
+void launch_faces(const DevSim &S, hipStream_t s) { PXL_LAUNCH(k_faces, node_grid(S), dim3(TX, TY), 0, s, S); }
+void launch_cells(const DevSim &S, hipStream_t s) { PXL_LAUNCH(k_cells, cell_grid(S), dim3(TX, TY), 0, s, S); }
 void launch_fluid_list(const DevSim &S, hipStream_t s) {
     const uint32_t n = S.split_colours ? 2 * S.ncells : S.ncells;
     launch_exclusive_scan(S.fl_flag, n, S.fl_tmp, S.fl_off, s);
-    k_fluid_compact<<<(n + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(S);
+    PXL_LAUNCH(k_fluid_compact, (n + BLOCK - 1) / BLOCK, BLOCK, 0, s, S);
 }
-void launch_project(const DevSim &S, hipStream_t s) { k_project<<<node_grid(S), dim3(TX, TY), 0, s>>>(S); }
+void launch_project(const DevSim &S, hipStream_t s) { PXL_LAUNCH(k_project, node_grid(S), dim3(TX, TY), 0, s, S); }
 void launch_acc_fields(const DevSim &S, int32_t *acc, int32_t *wacc, hipStream_t s) {
-    k_acc_fields<<<node_grid(S), dim3(TX, TY), 0, s>>>(S, acc, wacc);
+    PXL_LAUNCH(k_acc_fields, node_grid(S), dim3(TX, TY), 0, s, S, acc, wacc);
 }
 void launch_type_u32(const DevSim &S, uint32_t *out, hipStream_t s) {
-    k_type_u32<<<cell_grid(S), dim3(TX, TY), 0, s>>>(S, out);
+    PXL_LAUNCH(k_type_u32, cell_grid(S), dim3(TX, TY), 0, s, S, out);
 }
 
 // ---- batched
@@ -238,11 +238,11 @@ uint32_t node_blocks(const DevSim &S) { return node_gx(S) * node_gy(S) * (uint32
 uint32_t cell_blocks(const DevSim &S) { return cell_gx(S) * cell_gy(S) * (uint32_t)S.nz; }
 uint32_t flag_count(const DevSim &S) { return S.split_colours ? 2 * S.ncells : S.ncells; }
 
-void batch_faces(const WorkLists &W, hipStream_t s) { kb_faces<<<BATCH_GRID, dim3(TX, TY), 0, s>>>(W[WD_NODES]); }
-void batch_cells(const WorkLists &W, hipStream_t s) { kb_cells<<<BATCH_GRID, dim3(TX, TY), 0, s>>>(W[WD_CELLS]); }
+void batch_faces(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_faces, BATCH_GRID, dim3(TX, TY), 0, s, W[WD_NODES]); }
+void batch_cells(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_cells, BATCH_GRID, dim3(TX, TY), 0, s, W[WD_CELLS]); }
 void batch_fluid_compact(const WorkLists &W, hipStream_t s) {
-    kb_fluid_compact<<<BATCH_GRID, BLOCK, 0, s>>>(W[WD_COMPACT]);
+    PXL_LAUNCH(kb_fluid_compact, BATCH_GRID, BLOCK, 0, s, W[WD_COMPACT]);
 }
-void batch_project(const WorkLists &W, hipStream_t s) { kb_project<<<BATCH_GRID, dim3(TX, TY), 0, s>>>(W[WD_NODES]); }
+void batch_project(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_project, BATCH_GRID, dim3(TX, TY), 0, s, W[WD_NODES]); }
 
 }  // namespace pxl

@@ -11,6 +11,7 @@
 //               (refinement); workgroup partials, scalars
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -164,6 +165,26 @@ __device__ __forceinline__ uint32_t tile_key(const DevSim &S, int i, int j, int 
     const uint32_t t = (uint32_t)(i >> 3) + (uint32_t)S.ntx * ((uint32_t)(j >> 3) + (uint32_t)S.nty * (uint32_t)(k >> 3));
     return t * TILE_CELLS + (uint32_t)((i & 7) | ((j & 7) << 3) | ((k & 7) << 6));
 }
+
+// ---- launches ----
+// Every launch goes through PXL_LAUNCH.  While a profiler has set this thread's launch events,
+// the first launch records them from its own dispatch (the kernel's start and end, as a
+// kernel trace sees them) and counts itself; later launches of the same profiled phase, and
+// all launches otherwise, are plain.
+struct LaunchEvents {
+    hipEvent_t start = nullptr, stop = nullptr;
+    int launches = 0;
+};
+LaunchEvents &launch_events();  // this thread's slot
+#define PXL_LAUNCH(kernel, grid, block, shmem, stream, ...)                                                    \
+    do {                                                                                                       \
+        ::pxl::LaunchEvents &ev_ = ::pxl::launch_events();                                                     \
+        if (ev_.start && ev_.launches++ == 0)                                                                  \
+            hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), (std::uint32_t)(shmem), stream, ev_.start,  \
+                                  ev_.stop, 0u, __VA_ARGS__);                                                  \
+        else                                                                                                   \
+            hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), shmem, stream, __VA_ARGS__);                  \
+    } while (0)
 
 // ---- launch helpers (all asynchronous on `s`) ----
 void launch_fill_u32(uint32_t *p, uint32_t n, uint32_t value, hipStream_t s);

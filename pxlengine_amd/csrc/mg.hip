@@ -562,8 +562,8 @@ __global__ __launch_bounds__(TBLOCK) void kb_mg_tail(WorkList W) {
 void launch_mg_types(const DevSim &S, hipStream_t s) {
     for (int l = 1; l < S.mg_levels; l++) {
         const MGLevel f = l == 1 ? level0(S) : S.mg[l - 1];
-        k_mg_types<<<grid_for(cells(S.mg[l])), MBLOCK, 0, s>>>(f, S.mg[l]);
-        k_mg_sten<<<grid_for(cells(S.mg[l])), MBLOCK, 0, s>>>(S.mg[l]);
+        PXL_LAUNCH(k_mg_types, grid_for(cells(S.mg[l])), MBLOCK, 0, s, f, S.mg[l]);
+        PXL_LAUNCH(k_mg_sten, grid_for(cells(S.mg[l])), MBLOCK, 0, s, S.mg[l]);
     }
 }
 
@@ -575,29 +575,29 @@ void launch_mg_vcycle(const DevSim &S, const float *r, float *u, hipStream_t s) 
     const unsigned g = (unsigned)S.cg_blocks;
     for (int l = 0; l < T; l++) {
         if (l == 0) {
-            k_l0_pre0<<<g, LBLOCK, 0, s>>>(S);
-            k_l0_sweep<<<g, LBLOCK, 0, s>>>(S, 0);
-            k_l0_sweep<<<g, LBLOCK, 0, s>>>(S, 1);
-            k_restrict<<<g, LBLOCK, 0, s>>>(level0(S), S.cg_r0, S.cg_r1, S.mg[1]);
+            PXL_LAUNCH(k_l0_pre0, g, LBLOCK, 0, s, S);
+            PXL_LAUNCH(k_l0_sweep, g, LBLOCK, 0, s, S, 0);
+            PXL_LAUNCH(k_l0_sweep, g, LBLOCK, 0, s, S, 1);
+            PXL_LAUNCH(k_restrict, g, LBLOCK, 0, s, level0(S), S.cg_r0, S.cg_r1, S.mg[1]);
         } else {
-            k_lc_pre0<<<g, LBLOCK, 0, s>>>(S.mg[l]);
-            k_lc_sweep<<<g, LBLOCK, 0, s>>>(S.mg[l], 0);
-            k_lc_sweep<<<g, LBLOCK, 0, s>>>(S.mg[l], 1);
-            k_restrict<<<g, LBLOCK, 0, s>>>(S.mg[l], S.mg[l].r, S.mg[l].z, S.mg[l + 1]);
+            PXL_LAUNCH(k_lc_pre0, g, LBLOCK, 0, s, S.mg[l]);
+            PXL_LAUNCH(k_lc_sweep, g, LBLOCK, 0, s, S.mg[l], 0);
+            PXL_LAUNCH(k_lc_sweep, g, LBLOCK, 0, s, S.mg[l], 1);
+            PXL_LAUNCH(k_restrict, g, LBLOCK, 0, s, S.mg[l], S.mg[l].r, S.mg[l].z, S.mg[l + 1]);
         }
     }
-    k_mg_tail<<<1, TBLOCK, 0, s>>>(S);
+    PXL_LAUNCH(k_mg_tail, 1, TBLOCK, 0, s, S);
     for (int l = T - 1; l >= 0; l--) {
         if (l == 0) {
-            k_l0_post<<<g, LBLOCK, 0, s>>>(S, S.mg[1]);
-            k_l0_sweep<<<g, LBLOCK, 0, s>>>(S, 0);
-            k_l0_sweep<<<g, LBLOCK, 0, s>>>(S, 1);
-            k_l0_sweep<<<g, LBLOCK, 0, s>>>(S, 0);
+            PXL_LAUNCH(k_l0_post, g, LBLOCK, 0, s, S, S.mg[1]);
+            PXL_LAUNCH(k_l0_sweep, g, LBLOCK, 0, s, S, 0);
+            PXL_LAUNCH(k_l0_sweep, g, LBLOCK, 0, s, S, 1);
+            PXL_LAUNCH(k_l0_sweep, g, LBLOCK, 0, s, S, 0);
         } else {
-            k_lc_post<<<g, LBLOCK, 0, s>>>(S.mg[l], S.mg[l + 1]);
-            k_lc_sweep<<<g, LBLOCK, 0, s>>>(S.mg[l], 0);
-            k_lc_sweep<<<g, LBLOCK, 0, s>>>(S.mg[l], 1);
-            k_lc_sweep<<<g, LBLOCK, 0, s>>>(S.mg[l], 0);
+            PXL_LAUNCH(k_lc_post, g, LBLOCK, 0, s, S.mg[l], S.mg[l + 1]);
+            PXL_LAUNCH(k_lc_sweep, g, LBLOCK, 0, s, S.mg[l], 0);
+            PXL_LAUNCH(k_lc_sweep, g, LBLOCK, 0, s, S.mg[l], 1);
+            PXL_LAUNCH(k_lc_sweep, g, LBLOCK, 0, s, S.mg[l], 0);
         }
     }
 }
@@ -605,16 +605,16 @@ void launch_mg_vcycle(const DevSim &S, const float *r, float *u, hipStream_t s) 
 // ---- batched
 uint32_t mg_cell_blocks(const DevSim &S, int l) { return grid_for(cells(l == 0 ? level0(S) : S.mg[l])); }
 
-void batch_mg_types(const WorkLists &W, int l, hipStream_t s) { kb_mg_types<<<BATCH_GRID, MBLOCK, 0, s>>>(W[WD_MGTYPES + l], l); }
-void batch_mg_sten(const WorkLists &W, int l, hipStream_t s) { kb_mg_sten<<<BATCH_GRID, MBLOCK, 0, s>>>(W[WD_MGTYPES + l], l); }
-void batch_mg_pre0(const WorkLists &W, int l, hipStream_t s) { kb_pre0<<<BATCH_LGRID, LBLOCK, 0, s>>>(W[WD_A_LVL + l], l); }
+void batch_mg_types(const WorkLists &W, int l, hipStream_t s) { PXL_LAUNCH(kb_mg_types, BATCH_GRID, MBLOCK, 0, s, W[WD_MGTYPES + l], l); }
+void batch_mg_sten(const WorkLists &W, int l, hipStream_t s) { PXL_LAUNCH(kb_mg_sten, BATCH_GRID, MBLOCK, 0, s, W[WD_MGTYPES + l], l); }
+void batch_mg_pre0(const WorkLists &W, int l, hipStream_t s) { PXL_LAUNCH(kb_pre0, BATCH_LGRID, LBLOCK, 0, s, W[WD_A_LVL + l], l); }
 void batch_mg_sweep(const WorkLists &W, int l, uint32_t colour, hipStream_t s) {
-    kb_sweep<<<BATCH_LGRID, LBLOCK, 0, s>>>(W[WD_A_LVL + l], l, colour);
+    PXL_LAUNCH(kb_sweep, BATCH_LGRID, LBLOCK, 0, s, W[WD_A_LVL + l], l, colour);
 }
 void batch_mg_restrict(const WorkLists &W, int l, hipStream_t s) {
-    kb_restrict<<<BATCH_LGRID, LBLOCK, 0, s>>>(W[WD_A_LVL + l], l);
+    PXL_LAUNCH(kb_restrict, BATCH_LGRID, LBLOCK, 0, s, W[WD_A_LVL + l], l);
 }
-void batch_mg_post(const WorkLists &W, int l, hipStream_t s) { kb_post<<<BATCH_LGRID, LBLOCK, 0, s>>>(W[WD_A_LVL + l], l); }
-void batch_mg_tail(const WorkLists &W, hipStream_t s) { kb_mg_tail<<<BATCH_TAIL_GRID, TBLOCK, 0, s>>>(W[WD_A_SIM]); }
+void batch_mg_post(const WorkLists &W, int l, hipStream_t s) { PXL_LAUNCH(kb_post, BATCH_LGRID, LBLOCK, 0, s, W[WD_A_LVL + l], l); }
+void batch_mg_tail(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_mg_tail, BATCH_TAIL_GRID, TBLOCK, 0, s, W[WD_A_SIM]); }
 
 }  // namespace pxl

@@ -583,13 +583,13 @@ __global__ __launch_bounds__(BLOCK) void k_reset_particles(DevSim S) {
 }  // namespace
 
 void launch_fill_u32(uint32_t *p, uint32_t n, uint32_t value, hipStream_t s) {
-    if (n) k_fill_u32<<<blocks_for(n), BLOCK, 0, s>>>(p, n, value);
+    if (n) PXL_LAUNCH(k_fill_u32, blocks_for(n), BLOCK, 0, s, p, n, value);
 }
 
-void launch_clear_grid(const DevSim &S, hipStream_t s) { k_clear_grid<<<blocks_for(S.nkeys), BLOCK, 0, s>>>(S); }
+void launch_clear_grid(const DevSim &S, hipStream_t s) { PXL_LAUNCH(k_clear_grid, blocks_for(S.nkeys), BLOCK, 0, s, S); }
 
 void launch_bin_count(const DevSim &S, hipStream_t s) {
-    k_bin_count<<<blocks_for(S.np), BLOCK, 0, s>>>(S);
+    PXL_LAUNCH(k_bin_count, blocks_for(S.np), BLOCK, 0, s, S);
 }
 
 uint32_t scan_tiles(uint32_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
@@ -597,37 +597,37 @@ uint32_t scan_tiles(uint32_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
 void launch_exclusive_scan(const uint32_t *in, uint32_t n, uint32_t *tile_tmp, uint32_t *out, hipStream_t s) {
     const uint32_t ntiles = scan_tiles(n);
     if (ntiles == 0) return;
-    k_scan_reduce<<<ntiles, BLOCK, 0, s>>>(in, n, tile_tmp);
-    k_scan_tiles<<<1, BLOCK, 0, s>>>(tile_tmp, ntiles);
-    k_scan_down<<<ntiles, BLOCK, 0, s>>>(in, n, tile_tmp, out);
+    PXL_LAUNCH(k_scan_reduce, ntiles, BLOCK, 0, s, in, n, tile_tmp);
+    PXL_LAUNCH(k_scan_tiles, 1, BLOCK, 0, s, tile_tmp, ntiles);
+    PXL_LAUNCH(k_scan_down, ntiles, BLOCK, 0, s, in, n, tile_tmp, out);
 }
 
 void launch_scan_offsets(const DevSim &S, hipStream_t s) { launch_exclusive_scan(S.count, S.nkeys, S.scan_tmp, S.offset, s); }
 
-void launch_scatter(const DevSim &S, hipStream_t s) { k_scatter<<<blocks_for(S.np), BLOCK, 0, s>>>(S); }
+void launch_scatter(const DevSim &S, hipStream_t s) { PXL_LAUNCH(k_scatter, blocks_for(S.np), BLOCK, 0, s, S); }
 
-void launch_tile_list(const DevSim &S, hipStream_t s) { k_tile_list<<<blocks_for(S.ntiles), BLOCK, 0, s>>>(S); }
-void launch_p2g(const DevSim &S, hipStream_t s) { k_p2g_tile<<<tile_grid(S), TILE_BLOCK, 0, s>>>(S); }
-void launch_g2p(const DevSim &S, hipStream_t s) { k_g2p_tile<<<tile_grid(S), TILE_BLOCK, 0, s>>>(S); }
+void launch_tile_list(const DevSim &S, hipStream_t s) { PXL_LAUNCH(k_tile_list, blocks_for(S.ntiles), BLOCK, 0, s, S); }
+void launch_p2g(const DevSim &S, hipStream_t s) { PXL_LAUNCH(k_p2g_tile, tile_grid(S), TILE_BLOCK, 0, s, S); }
+void launch_g2p(const DevSim &S, hipStream_t s) { PXL_LAUNCH(k_g2p_tile, tile_grid(S), TILE_BLOCK, 0, s, S); }
 
 void launch_bins_out(const DevSim &S, const uint32_t *off_lin, uint32_t *cell_of, uint32_t *sorted, hipStream_t s) {
-    k_bins_out<<<blocks_for(S.np), BLOCK, 0, s>>>(S, off_lin, cell_of, sorted);
+    PXL_LAUNCH(k_bins_out, blocks_for(S.np), BLOCK, 0, s, S, off_lin, cell_of, sorted);
 }
 void launch_sort_segments(uint32_t ncells, const uint32_t *offsets, const uint32_t *counts, uint32_t *sorted,
                           hipStream_t s) {
-    k_sort_segments<<<blocks_for(ncells), BLOCK, 0, s>>>(ncells, offsets, counts, sorted);
+    PXL_LAUNCH(k_sort_segments, blocks_for(ncells), BLOCK, 0, s, ncells, offsets, counts, sorted);
 }
 void launch_count_linear(const DevSim &S, uint32_t *out, hipStream_t s) {
-    k_count_linear<<<blocks_for(S.ncells), BLOCK, 0, s>>>(S, out);
+    PXL_LAUNCH(k_count_linear, blocks_for(S.ncells), BLOCK, 0, s, S, out);
 }
 void launch_unpack_aos8(const DevSim &S, float *aos8, hipStream_t s) {
-    k_unpack_aos8<<<blocks_for(S.np), BLOCK, 0, s>>>(S, aos8);
+    PXL_LAUNCH(k_unpack_aos8, blocks_for(S.np), BLOCK, 0, s, S, aos8);
 }
 void launch_pack_aos8(const DevSim &S, const float *aos8, hipStream_t s) {
-    k_pack_aos8<<<blocks_for(S.np), BLOCK, 0, s>>>(S, aos8);
+    PXL_LAUNCH(k_pack_aos8, blocks_for(S.np), BLOCK, 0, s, S, aos8);
 }
 void launch_reset_particles(const DevSim &S, hipStream_t s) {
-    k_reset_particles<<<blocks_for(S.np), BLOCK, 0, s>>>(S);
+    PXL_LAUNCH(k_reset_particles, blocks_for(S.np), BLOCK, 0, s, S);
 }
 
 // ---- batched
@@ -636,20 +636,20 @@ uint32_t key_blocks(const DevSim &S) { return blocks_for(S.nkeys); }
 uint32_t tile_blocks(const DevSim &S) { return blocks_for(S.ntiles); }
 uint32_t tile_grid_blocks(const DevSim &S) { return tile_grid(S); }
 
-void batch_clear_grid(const WorkLists &W, hipStream_t s) { kb_clear_grid<<<BATCH_GRID, BLOCK, 0, s>>>(W[WD_KEYS]); }
-void batch_bin_count(const WorkLists &W, hipStream_t s) { kb_bin_count<<<BATCH_GRID, BLOCK, 0, s>>>(W[WD_PART]); }
+void batch_clear_grid(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_clear_grid, BATCH_GRID, BLOCK, 0, s, W[WD_KEYS]); }
+void batch_bin_count(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_bin_count, BATCH_GRID, BLOCK, 0, s, W[WD_PART]); }
 void batch_scan_reduce(const WorkLists &W, int which, hipStream_t s) {
-    kb_scan_reduce<<<BATCH_GRID, BLOCK, 0, s>>>(W[which == 0 ? WD_SCAN_KEYS : WD_SCAN_FLAGS], which);
+    PXL_LAUNCH(kb_scan_reduce, BATCH_GRID, BLOCK, 0, s, W[which == 0 ? WD_SCAN_KEYS : WD_SCAN_FLAGS], which);
 }
 void batch_scan_tiles(const WorkLists &W, int which, hipStream_t s) {
-    kb_scan_tiles<<<BATCH_GRID, BLOCK, 0, s>>>(W[WD_SIM], which);
+    PXL_LAUNCH(kb_scan_tiles, BATCH_GRID, BLOCK, 0, s, W[WD_SIM], which);
 }
 void batch_scan_down(const WorkLists &W, int which, hipStream_t s) {
-    kb_scan_down<<<BATCH_GRID, BLOCK, 0, s>>>(W[which == 0 ? WD_SCAN_KEYS : WD_SCAN_FLAGS], which);
+    PXL_LAUNCH(kb_scan_down, BATCH_GRID, BLOCK, 0, s, W[which == 0 ? WD_SCAN_KEYS : WD_SCAN_FLAGS], which);
 }
-void batch_tile_list(const WorkLists &W, hipStream_t s) { kb_tile_list<<<BATCH_GRID, BLOCK, 0, s>>>(W[WD_TILES]); }
-void batch_scatter(const WorkLists &W, hipStream_t s) { kb_scatter<<<BATCH_GRID, BLOCK, 0, s>>>(W[WD_PART]); }
-void batch_p2g(const WorkLists &W, hipStream_t s) { kb_p2g_tile<<<BATCH_TILE_GRID, TILE_BLOCK, 0, s>>>(W[WD_TILEGRID]); }
-void batch_g2p(const WorkLists &W, hipStream_t s) { kb_g2p_tile<<<BATCH_TILE_GRID, TILE_BLOCK, 0, s>>>(W[WD_TILEGRID]); }
+void batch_tile_list(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_tile_list, BATCH_GRID, BLOCK, 0, s, W[WD_TILES]); }
+void batch_scatter(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_scatter, BATCH_GRID, BLOCK, 0, s, W[WD_PART]); }
+void batch_p2g(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_p2g_tile, BATCH_TILE_GRID, TILE_BLOCK, 0, s, W[WD_TILEGRID]); }
+void batch_g2p(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_g2p_tile, BATCH_TILE_GRID, TILE_BLOCK, 0, s, W[WD_TILEGRID]); }
 
 }  // namespace pxl

@@ -230,8 +230,8 @@ public:
 
     void sync() { HIP_CHECK(hipStreamSynchronize(stream_)); }
 
-    // Profiling: every launch eager and bracketed by HIP events on the batch stream, priced
-    // with share_bytes over the simulations of its work list.
+    // Profiling: every launch eager, timed by HIP events recorded from its own dispatch on the
+    // batch stream, and priced with share_bytes over the simulations of its work list.
     void set_profiling(bool on) {
         profiling_ = on;
         recs_used_ = 0;
@@ -328,9 +328,12 @@ private:
         r.name = name;
         r.level = level;
         r.bytes = launch_bytes(name, level);
-        HIP_CHECK(hipEventRecord(r.a, stream_));
-        launch();
-        HIP_CHECK(hipEventRecord(r.b, stream_));
+        LaunchEvents &ev = launch_events();
+        ev = LaunchEvents{r.a, r.b, 0};
+        launch();  // the kernel records r.a and r.b from its own dispatch
+        const int n = ev.launches;
+        ev = LaunchEvents{};
+        if (n != 1) throw std::logic_error(std::string("profiled batch phase ") + name + " did not launch one kernel");
     }
     // The members a launch covers: the solve kernels take the active ones (per level, those
     // whose level runs as separate launches), the rest take every member.

@@ -35,8 +35,9 @@ constexpr size_t ALIGN = 256;
 constexpr uint32_t MAX_SOLVER_BLOCKS = 4096;
 constexpr uint32_t DEFAULT_SOLVER_BLOCKS = 256;
 
-// Optional HIP-event bracketing of every launch (pxlflip_profile_steps).  Off while a graph
-// is captured or replayed.
+// Optional HIP-event timing of every launch (pxlflip_profile_steps): a single-kernel phase gets
+// its kernel's own start and end; a phase of several kernels runs from its first kernel's start
+// to an event after its last.  Off while a graph is captured or replayed.
 struct KernelTimer {
     struct Rec {
         const char *name;
@@ -62,9 +63,17 @@ struct KernelTimer {
             return;
         }
         hipEvent_t a = next(), b = next();
-        HIP_CHECK(hipEventRecord(a, s));
-        launch();
-        HIP_CHECK(hipEventRecord(b, s));
+        LaunchEvents &ev = launch_events();
+        ev = LaunchEvents{a, b, 0};
+        launch();  // the first kernel records a and b itself
+        const int n = ev.launches;
+        ev = LaunchEvents{};
+        if (n == 0) {  // nothing launched
+            HIP_CHECK(hipEventRecord(a, s));
+            HIP_CHECK(hipEventRecord(b, s));
+        } else if (n > 1) {  // several kernels: stop after the last one
+            HIP_CHECK(hipEventRecord(b, s));
+        }
         recs.push_back({name, a, b});
     }
     void clear() {

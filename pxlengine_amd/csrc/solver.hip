@@ -454,64 +454,64 @@ uint32_t solver_blocks(const DevSim &S) { return (uint32_t)S.cg_blocks; }
 
 void launch_cg_start(const DevSim &S, bool fp64, float *x, bool inner, hipStream_t s) {
     const uint32_t g = solver_blocks(S);
-    k_cg_residual<<<g, SBLOCK, 0, s>>>(S, x, r_buf(S, 0), inner ? 1 : 0);
+    PXL_LAUNCH(k_cg_residual, g, SBLOCK, 0, s, S, x, r_buf(S, 0), inner ? 1 : 0);
     // iteration 0 reads s_{-1} from s_buf(1)
-    if (fp64) k_cg_start<double><<<g, SBLOCK, 0, s>>>(S, r_buf(S, 0), w_buf(S, 0), s_buf(S, 1));
-    else k_cg_start<float><<<g, SBLOCK, 0, s>>>(S, r_buf(S, 0), w_buf(S, 0), s_buf(S, 1));
+    if (fp64) PXL_LAUNCH((k_cg_start<double>), g, SBLOCK, 0, s, S, r_buf(S, 0), w_buf(S, 0), s_buf(S, 1));
+    else PXL_LAUNCH((k_cg_start<float>), g, SBLOCK, 0, s, S, r_buf(S, 0), w_buf(S, 0), s_buf(S, 1));
 }
 
 void launch_cg_iter(const DevSim &S, bool fp64, float *x, uint32_t it, hipStream_t s) {
     const uint32_t g = solver_blocks(S);
     float *rin = r_buf(S, it), *rout = r_buf(S, it + 1), *win = w_buf(S, it), *wout = w_buf(S, it + 1);
     float *sprev = s_buf(S, it + 1), *scur = s_buf(S, it);
-    if (fp64) k_cg_iter<double><<<g, SBLOCK, 0, s>>>(S, it, x, rin, rout, win, wout, sprev, scur);
-    else k_cg_iter<float><<<g, SBLOCK, 0, s>>>(S, it, x, rin, rout, win, wout, sprev, scur);
+    if (fp64) PXL_LAUNCH((k_cg_iter<double>), g, SBLOCK, 0, s, S, it, x, rin, rout, win, wout, sprev, scur);
+    else PXL_LAUNCH((k_cg_iter<float>), g, SBLOCK, 0, s, S, it, x, rin, rout, win, wout, sprev, scur);
 }
 
 void launch_mgcg_start(const DevSim &S, float *x, hipStream_t s) {
     const uint32_t g = solver_blocks(S);
-    k_cg_residual<<<g, SBLOCK, 0, s>>>(S, x, S.cg_r0, 1);
+    PXL_LAUNCH(k_cg_residual, g, SBLOCK, 0, s, S, x, S.cg_r0, 1);
     launch_mg_vcycle(S, S.cg_r0, S.cg_r1, s);
-    k_mgcg_spmv<double><<<g, SBLOCK, 0, s>>>(S, 0, 1);
+    PXL_LAUNCH((k_mgcg_spmv<double>), g, SBLOCK, 0, s, S, 0, 1);
 }
 
 void launch_mgcg_iter(const DevSim &S, float *x, uint32_t it, bool last, hipStream_t s) {
     const uint32_t g = solver_blocks(S);
-    k_mgcg_update<double><<<g, SBLOCK, 0, s>>>(S, it, x);
+    PXL_LAUNCH((k_mgcg_update<double>), g, SBLOCK, 0, s, S, it, x);
     if (last) return;
     launch_mg_vcycle(S, S.cg_r0, S.cg_r1, s);
-    k_mgcg_spmv<double><<<g, SBLOCK, 0, s>>>(S, (it + 1) & 1, 0);
+    PXL_LAUNCH((k_mgcg_spmv<double>), g, SBLOCK, 0, s, S, (it + 1) & 1, 0);
 }
 
 void launch_mixed_init(const DevSim &S, double *x64, hipStream_t s) {
-    k_mixed_init<<<solver_blocks(S), SBLOCK, 0, s>>>(S, x64);
-    k_norms_finalize<<<1, BLOCK, 0, s>>>(S, solver_blocks(S), 2);
+    PXL_LAUNCH(k_mixed_init, solver_blocks(S), SBLOCK, 0, s, S, x64);
+    PXL_LAUNCH(k_norms_finalize, 1, BLOCK, 0, s, S, solver_blocks(S), 2);
 }
 
 void launch_mixed_update(const DevSim &S, const double *xsrc, double *xdst, hipStream_t s) {
-    k_mixed_update<<<solver_blocks(S), SBLOCK, 0, s>>>(S, xsrc, xdst, S.e);
-    k_norms_finalize<<<1, BLOCK, 0, s>>>(S, solver_blocks(S), 1);
+    PXL_LAUNCH(k_mixed_update, solver_blocks(S), SBLOCK, 0, s, S, xsrc, xdst, S.e);
+    PXL_LAUNCH(k_norms_finalize, 1, BLOCK, 0, s, S, solver_blocks(S), 1);
 }
 
 
 void launch_rbgs(const DevSim &S, uint32_t parity, hipStream_t s) {
-    k_rbgs<<<solver_blocks(S), SBLOCK, 0, s>>>(S, parity);
+    PXL_LAUNCH(k_rbgs, solver_blocks(S), SBLOCK, 0, s, S, parity);
 }
 
 // ---- batched (the solving simulations: x = e, r = cg_r0, u = cg_r1)
-void batch_mixed_init(const WorkLists &W, hipStream_t s) { kb_mixed_init<<<BATCH_SGRID, SBLOCK, 0, s>>>(W[WD_A_SOLVER]); }
+void batch_mixed_init(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_mixed_init, BATCH_SGRID, SBLOCK, 0, s, W[WD_A_SOLVER]); }
 void batch_norms(const WorkLists &W, int nv, hipStream_t s) {
-    kb_norms_finalize<<<BATCH_SGRID, BLOCK, 0, s>>>(W[WD_A_SIM], nv);
+    PXL_LAUNCH(kb_norms_finalize, BATCH_SGRID, BLOCK, 0, s, W[WD_A_SIM], nv);
 }
-void batch_cg_residual(const WorkLists &W, hipStream_t s) { kb_cg_residual<<<BATCH_SGRID, SBLOCK, 0, s>>>(W[WD_A_SOLVER]); }
+void batch_cg_residual(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_cg_residual, BATCH_SGRID, SBLOCK, 0, s, W[WD_A_SOLVER]); }
 void batch_mgcg_spmv(const WorkLists &W, uint32_t slot, int first, hipStream_t s) {
-    kb_mgcg_spmv<<<BATCH_SGRID, SBLOCK, 0, s>>>(W[WD_A_SOLVER], slot, first);
+    PXL_LAUNCH(kb_mgcg_spmv, BATCH_SGRID, SBLOCK, 0, s, W[WD_A_SOLVER], slot, first);
 }
 void batch_mgcg_update(const WorkLists &W, uint32_t it, hipStream_t s) {
-    kb_mgcg_update<<<BATCH_SGRID, SBLOCK, 0, s>>>(W[WD_A_SOLVER], it);
+    PXL_LAUNCH(kb_mgcg_update, BATCH_SGRID, SBLOCK, 0, s, W[WD_A_SOLVER], it);
 }
 void batch_mixed_update(const WorkLists &W, int cur, hipStream_t s) {
-    kb_mixed_update<<<BATCH_SGRID, SBLOCK, 0, s>>>(W[WD_A_SOLVER], cur);
+    PXL_LAUNCH(kb_mixed_update, BATCH_SGRID, SBLOCK, 0, s, W[WD_A_SOLVER], cur);
 }
 
 }  // namespace pxl
