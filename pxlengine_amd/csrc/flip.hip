@@ -327,6 +327,7 @@ int pxlflip_write_particles(pxlflip *h, const float *aos8) {
         pxl::DeviceBuffer buf(32ull * s->d.np);
         HIP_CHECK(hipMemcpyAsync(buf.p, aos8, 32ull * s->d.np, hipMemcpyHostToDevice, s->stream));
         pxl::launch_pack_aos8(s->d, buf.as<float>(), s->stream);
+        s->mark_all_tiles();
         HIP_CHECK(hipStreamSynchronize(s->stream));
         s->initialized = true;
     });
@@ -386,6 +387,7 @@ int pxlflip_write_field(pxlflip *h, int field, const void *in) {
     }
     return pxl::guarded(s, [&] {
         HIP_CHECK(hipMemcpyAsync(dst, in, bytes, hipMemcpyHostToDevice, s->stream));
+        s->mark_all_tiles();
         HIP_CHECK(hipStreamSynchronize(s->stream));
     });
 }

@@ -1,8 +1,9 @@
-// Grid-side kernels of the FLIP step: normalize + forces (one pass over faces),
-// classify + stencil + divergence (one pass over cells), red-black Gauss-Seidel half sweeps
-// and the pressure-gradient projection.  Semantics follow shaders/fluid.slang (cited per
-// kernel).  Cells and faces are walked as 3-D launches of 64x4 lanes: a wave64 covers 64
-// consecutive x-cells of one row, so every array access of a wave is one contiguous run.
+// Grid-side kernels of the FLIP step: normalize + forces (one pass over faces), classify +
+// stencil + divergence (one pass over cells), the fluid list and the pressure-gradient
+// projection.  Semantics follow shaders/fluid.slang (cited per kernel).  The face, cell and
+// projection passes walk the step's band of 8^3 tiles (DevSim::band_list; outside it every
+// value is the one the previous step left), each workgroup taking band tiles with a stride;
+// inside a tile lanes run x-fastest.
 //
 // Compiled with -ffp-contract=off: results are bitwise the reference formulas in fp32.
 #include "common.hpp"
@@ -12,15 +13,41 @@ namespace pxl {
 
 namespace {
 
-constexpr int TX = 64, TY = 4;
+constexpr int TX = 64, TY = 4;        // full-grid read-back launches
+constexpr int GBLOCK = 256;           // band-walking workgroups
 constexpr unsigned BATCH_GRID = 2048;  // persistent grid of batched launches
 
-// Node / cell launch grids (x, y blocks; z = one plane per block row).  A batched launch
-// numbers a simulation's blocks b = bx + gx (by + gy bz).
-__host__ __device__ inline uint32_t node_gx(const DevSim &S) { return (uint32_t)(S.nx + 1 + TX - 1) / TX; }
-__host__ __device__ inline uint32_t node_gy(const DevSim &S) { return (uint32_t)(S.ny + 1 + TY - 1) / TY; }
-__host__ __device__ inline uint32_t cell_gx(const DevSim &S) { return (uint32_t)(S.nx + TX - 1) / TX; }
-__host__ __device__ inline uint32_t cell_gy(const DevSim &S) { return (uint32_t)(S.ny + TY - 1) / TY; }
+// Workgroups of the band-walking launches of one simulation (each takes band tiles with this
+// stride), the same count as the per-tile particle kernels.
+__host__ __device__ inline uint32_t band_grid(const DevSim &S) { return S.ntiles < 1024u ? S.ntiles : 1024u; }
+
+// Runs f(i, j, k) over tile t's nodes (faces are stored at the node of their low corner: the
+// tile's 8^3 nodes, plus the grid's upper boundary nodes on the last tile along an axis) or
+// cells (8^3, clipped at the grid edge).  Lanes take the box x-fastest.
+template <typename F>
+__device__ __forceinline__ void for_tile_box(const DevSim &S, uint32_t t, bool nodes, F &&f) {
+    const int ti = (int)(t % (uint32_t)S.ntx), tj = (int)((t / (uint32_t)S.ntx) % (uint32_t)S.nty),
+              tk = (int)(t / ((uint32_t)S.ntx * (uint32_t)S.nty));
+    const int lo[3] = {ti * TILE, tj * TILE, tk * TILE};
+    const int n[3] = {S.nx, S.ny, S.nz}, nt[3] = {S.ntx, S.nty, S.ntz}, tt[3] = {ti, tj, tk};
+    int ext[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const int end = nodes ? (tt[a] == nt[a] - 1 ? n[a] + 1 : lo[a] + TILE) : (lo[a] + TILE < n[a] ? lo[a] + TILE : n[a]);
+        ext[a] = end - lo[a];
+    }
+    const int total = ext[0] * ext[1] * ext[2];
+    for (int q = threadIdx.x; q < total; q += GBLOCK) {
+        const int i = lo[0] + q % ext[0], j = lo[1] + (q / ext[0]) % ext[1], k = lo[2] + q / (ext[0] * ext[1]);
+        f(i, j, k);
+    }
+}
+// Band tiles b, b + nb, ... of simulation S.
+template <typename F>
+__device__ __forceinline__ void for_band(const DevSim &S, uint32_t b, uint32_t nb, F &&f) {
+    const uint32_t n = *S.band_count;
+    for (uint32_t u = b; u < n; u += nb) f(S.band_list[u]);
+}
 
 // SOLID for cells outside the grid or on its one-cell wall layer (fluid.slang:55-58, 126-128).
 __device__ __forceinline__ bool is_solid(const DevSim &S, int i, int j, int k) {
@@ -53,22 +80,18 @@ __device__ __forceinline__ void face_update(const DevSim &S, int c, int i, int j
     S.vel[f] = keep ? v + g * S.dt : 0.0f;
 }
 
-__device__ __forceinline__ void body_faces(const DevSim &S, uint32_t bx, uint32_t by, uint32_t bz) {
-    const int i = bx * TX + threadIdx.x;
-    const int j = by * TY + threadIdx.y;
-    const int k = bz;
-    if (i > S.nx || j > S.ny) return;
+__device__ __forceinline__ void faces_at(const DevSim &S, int i, int j, int k) {
     const bool in_c = !is_solid(S, i, j, k);
     if (j < S.ny && k < S.nz) face_update(S, 0, i, j, k, uidx(S, i, j, k), S.gx, in_c && !is_solid(S, i - 1, j, k));
     if (i < S.nx && k < S.nz) face_update(S, 1, i, j, k, vidx(S, i, j, k), S.gy, in_c && !is_solid(S, i, j - 1, k));
     if (i < S.nx && j < S.ny) face_update(S, 2, i, j, k, widx(S, i, j, k), S.gz, in_c && !is_solid(S, i, j, k - 1));
 }
-__global__ __launch_bounds__(TX *TY) void k_faces(DevSim S) { body_faces(S, blockIdx.x, blockIdx.y, blockIdx.z); }
-__global__ __launch_bounds__(TX *TY) void kb_faces(WorkList W) {
-    for_each_work(W, [&](const DevSim &S, uint32_t b) {
-        const uint32_t gx = node_gx(S), gy = node_gy(S);
-        body_faces(S, b % gx, (b / gx) % gy, b / (gx * gy));
-    });
+__device__ __forceinline__ void body_faces(const DevSim &S, uint32_t b, uint32_t nb) {
+    for_band(S, b, nb, [&](uint32_t t) { for_tile_box(S, t, true, [&](int i, int j, int k) { faces_at(S, i, j, k); }); });
+}
+__global__ __launch_bounds__(GBLOCK) void k_faces(DevSim S) { body_faces(S, blockIdx.x, gridDim.x); }
+__global__ __launch_bounds__(GBLOCK) void kb_faces(WorkList W) {
+    for_each_work(W, [&](const DevSim &S, uint32_t b) { body_faces(S, b, band_grid(S)); });
 }
 
 // The P2G sums as the reference's acc / wacc face buffers (for read-back).
@@ -90,11 +113,7 @@ __device__ __forceinline__ uint8_t type_from_count(const DevSim &S, int i, int j
 
 // normalize_grid cells + compute_divergence (fluid.slang:125-131, 154-165), plus the stencil
 // word the pressure kernels read instead of re-classifying six neighbours.
-__device__ __forceinline__ void body_cells(const DevSim &S, uint32_t bx, uint32_t by, uint32_t bz) {
-    const int i = bx * TX + threadIdx.x;
-    const int j = by * TY + threadIdx.y;
-    const int k = bz;
-    if (i >= S.nx || j >= S.ny) return;
+__device__ __forceinline__ void cells_at(const DevSim &S, int i, int j, int k) {
     const uint32_t id = cidx(S, i, j, k);
     const uint8_t t = type_from_count(S, i, j, k);
     S.type[id] = t;
@@ -126,12 +145,12 @@ __device__ __forceinline__ void body_cells(const DevSim &S, uint32_t bx, uint32_
         S.fl_flag[id] = t == CELL_FLUID ? 1u : 0u;
     }
 }
-__global__ __launch_bounds__(TX *TY) void k_cells(DevSim S) { body_cells(S, blockIdx.x, blockIdx.y, blockIdx.z); }
-__global__ __launch_bounds__(TX *TY) void kb_cells(WorkList W) {
-    for_each_work(W, [&](const DevSim &S, uint32_t b) {
-        const uint32_t gx = cell_gx(S), gy = cell_gy(S);
-        body_cells(S, b % gx, (b / gx) % gy, b / (gx * gy));
-    });
+__device__ __forceinline__ void body_cells(const DevSim &S, uint32_t b, uint32_t nb) {
+    for_band(S, b, nb, [&](uint32_t t) { for_tile_box(S, t, false, [&](int i, int j, int k) { cells_at(S, i, j, k); }); });
+}
+__global__ __launch_bounds__(GBLOCK) void k_cells(DevSim S) { body_cells(S, blockIdx.x, gridDim.x); }
+__global__ __launch_bounds__(GBLOCK) void kb_cells(WorkList W) {
+    for_each_work(W, [&](const DevSim &S, uint32_t b) { body_cells(S, b, band_grid(S)); });
 }
 
 // Fluid list from the scanned flags.  Split: entry e < C is cell e (red), e >= C is cell
@@ -187,21 +206,17 @@ __device__ __forceinline__ void project_face(const DevSim &S, uint32_t f, int ai
     }
 }
 
-__device__ __forceinline__ void body_project(const DevSim &S, uint32_t bx, uint32_t by, uint32_t bz) {
-    const int i = bx * TX + threadIdx.x;
-    const int j = by * TY + threadIdx.y;
-    const int k = bz;
-    if (i > S.nx || j > S.ny) return;
+__device__ __forceinline__ void project_at(const DevSim &S, int i, int j, int k) {
     if (j < S.ny && k < S.nz) project_face(S, uidx(S, i, j, k), i - 1, j, k, i, j, k);
     if (i < S.nx && k < S.nz) project_face(S, vidx(S, i, j, k), i, j - 1, k, i, j, k);
     if (i < S.nx && j < S.ny) project_face(S, widx(S, i, j, k), i, j, k - 1, i, j, k);
 }
-__global__ __launch_bounds__(TX *TY) void k_project(DevSim S) { body_project(S, blockIdx.x, blockIdx.y, blockIdx.z); }
-__global__ __launch_bounds__(TX *TY) void kb_project(WorkList W) {
-    for_each_work(W, [&](const DevSim &S, uint32_t b) {
-        const uint32_t gx = node_gx(S), gy = node_gy(S);
-        body_project(S, b % gx, (b / gx) % gy, b / (gx * gy));
-    });
+__device__ __forceinline__ void body_project(const DevSim &S, uint32_t b, uint32_t nb) {
+    for_band(S, b, nb, [&](uint32_t t) { for_tile_box(S, t, true, [&](int i, int j, int k) { project_at(S, i, j, k); }); });
+}
+__global__ __launch_bounds__(GBLOCK) void k_project(DevSim S) { body_project(S, blockIdx.x, gridDim.x); }
+__global__ __launch_bounds__(GBLOCK) void kb_project(WorkList W) {
+    for_each_work(W, [&](const DevSim &S, uint32_t b) { body_project(S, b, band_grid(S)); });
 }
 
 __global__ __launch_bounds__(TX *TY) void k_type_u32(DevSim S, uint32_t *out) {
@@ -218,14 +233,14 @@ dim3 node_grid(const DevSim &S) { return dim3((S.nx + 1 + TX - 1) / TX, (S.ny + 
 
 }  // namespace
 
-void launch_faces(const DevSim &S, hipStream_t s) { PXL_LAUNCH(k_faces, node_grid(S), dim3(TX, TY), 0, s, S); }
-void launch_cells(const DevSim &S, hipStream_t s) { PXL_LAUNCH(k_cells, cell_grid(S), dim3(TX, TY), 0, s, S); }
+void launch_faces(const DevSim &S, hipStream_t s) { PXL_LAUNCH(k_faces, band_grid(S), GBLOCK, 0, s, S); }
+void launch_cells(const DevSim &S, hipStream_t s) { PXL_LAUNCH(k_cells, band_grid(S), GBLOCK, 0, s, S); }
 void launch_fluid_list(const DevSim &S, hipStream_t s) {
     const uint32_t n = S.split_colours ? 2 * S.ncells : S.ncells;
     launch_exclusive_scan(S.fl_flag, n, S.fl_tmp, S.fl_off, s);
     PXL_LAUNCH(k_fluid_compact, (n + BLOCK - 1) / BLOCK, BLOCK, 0, s, S);
 }
-void launch_project(const DevSim &S, hipStream_t s) { PXL_LAUNCH(k_project, node_grid(S), dim3(TX, TY), 0, s, S); }
+void launch_project(const DevSim &S, hipStream_t s) { PXL_LAUNCH(k_project, band_grid(S), GBLOCK, 0, s, S); }
 void launch_acc_fields(const DevSim &S, int32_t *acc, int32_t *wacc, hipStream_t s) {
     PXL_LAUNCH(k_acc_fields, node_grid(S), dim3(TX, TY), 0, s, S, acc, wacc);
 }
@@ -234,15 +249,13 @@ void launch_type_u32(const DevSim &S, uint32_t *out, hipStream_t s) {
 }
 
 // ---- batched
-uint32_t node_blocks(const DevSim &S) { return node_gx(S) * node_gy(S) * (uint32_t)(S.nz + 1); }
-uint32_t cell_blocks(const DevSim &S) { return cell_gx(S) * cell_gy(S) * (uint32_t)S.nz; }
 uint32_t flag_count(const DevSim &S) { return S.split_colours ? 2 * S.ncells : S.ncells; }
 
-void batch_faces(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_faces, BATCH_GRID, dim3(TX, TY), 0, s, W[WD_NODES]); }
-void batch_cells(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_cells, BATCH_GRID, dim3(TX, TY), 0, s, W[WD_CELLS]); }
+void batch_faces(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_faces, BATCH_GRID, GBLOCK, 0, s, W[WD_TILEGRID]); }
+void batch_cells(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_cells, BATCH_GRID, GBLOCK, 0, s, W[WD_TILEGRID]); }
 void batch_fluid_compact(const WorkLists &W, hipStream_t s) {
     PXL_LAUNCH(kb_fluid_compact, BATCH_GRID, BLOCK, 0, s, W[WD_COMPACT]);
 }
-void batch_project(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_project, BATCH_GRID, dim3(TX, TY), 0, s, W[WD_NODES]); }
+void batch_project(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_project, BATCH_GRID, GBLOCK, 0, s, W[WD_TILEGRID]); }
 
 }  // namespace pxl

@@ -80,6 +80,13 @@ struct DevSim {
     int64_t *box;
     uint32_t ntiles;
     uint32_t *tile_list, *tile_count;  // tiles holding particles this step (any order)
+    // per tile: bit 0 = holds particles this step, bit 1 = held particles the step before
+    // (3 after allocation, reset or a field write: the next two steps then cover every tile)
+    uint8_t *tile_act;
+    // the band: tiles within one tile of a tile active this step or the step before.  Faces
+    // and cells outside it hold values that do not change from step to step (no particle
+    // weight, no FLUID cell near), so the face, cell and projection passes walk only the band.
+    uint32_t *band_list, *band_count;
     // cells (count/offset: tiled key order, offset[nkeys] = np)
     uint32_t *count, *offset;
     uint8_t *type;
@@ -125,9 +132,7 @@ __device__ __forceinline__ void tile_box(int c, int ti, int tj, int tk, int lo[3
     d[c] = TILE + 1;
 }
 
-__device__ __forceinline__ bool tile_active(const DevSim &S, uint32_t t) {
-    return S.offset[t * TILE_CELLS] != S.offset[(t + 1) * TILE_CELLS];
-}
+__device__ __forceinline__ bool tile_active(const DevSim &S, uint32_t t) { return (S.tile_act[t] & 1u) != 0; }
 
 // Packed P2G sum of face q of component c: the sum over the (1 to 8) boxes of active tiles
 // that contain it.  Along c a face 8m lies in the boxes of tiles m-1 and m; along another
@@ -194,6 +199,7 @@ void launch_scan_offsets(const DevSim &S, hipStream_t s);     // offset = exclus
 uint32_t scan_tiles(uint32_t n);                              // scratch entries a scan of n needs
 void launch_exclusive_scan(const uint32_t *in, uint32_t n, uint32_t *tile_tmp, uint32_t *out, hipStream_t s);
 void launch_tile_list(const DevSim &S, hipStream_t s);        // active tiles (after the scan)
+void launch_band(const DevSim &S, hipStream_t s);             // band tiles (after the tile list)
 void launch_scatter(const DevSim &S, hipStream_t s);          // a -> b in cell order
 void launch_p2g(const DevSim &S, hipStream_t s);              // b -> per-tile boxes
 void launch_faces(const DevSim &S, hipStream_t s);            // normalize + forces
@@ -216,6 +222,12 @@ void launch_mg_types(const DevSim &S, hipStream_t s);
 void launch_mg_vcycle(const DevSim &S, const float *r, float *u, hipStream_t s);  // u = M r
 void launch_mgcg_start(const DevSim &S, float *x, hipStream_t s);
 void launch_mgcg_iter(const DevSim &S, float *x, uint32_t it, bool last, hipStream_t s);
+// MG-CG iterations of refinement pass `pass`: `inner` in the first pass, half as many (at least
+// 2) in later ones, which start from the FP32 floor of the first and only need to clear the
+// last digits.
+inline uint32_t pass_iterations(uint32_t inner, uint32_t pass) {
+    return pass == 0 ? inner : (inner + 1) / 2 > 2 ? (inner + 1) / 2 : 2;
+}
 // FP64 refinement (solver.hip); init/update also total the norms into scal[SC_RR / SC_BB].
 void launch_mixed_init(const DevSim &S, double *x64, hipStream_t s);  // x64 = p, r64 = b - A x64
 void launch_mixed_update(const DevSim &S, const double *xsrc, double *xdst, hipStream_t s);  // xdst = xsrc + e; r64
@@ -266,9 +278,7 @@ enum WorkDomain : int {
     WD_PART,        // particles
     WD_SCAN_KEYS,   // scan tiles of the count array
     WD_TILES,       // tiles (active-tile list)
-    WD_TILEGRID,    // per-tile P2G / G2P workgroups
-    WD_NODES,       // face-node grid (faces, project)
-    WD_CELLS,       // cell grid (cells)
+    WD_TILEGRID,    // per-tile workgroups (P2G, G2P, and the band passes: faces, cells, project)
     WD_SCAN_FLAGS,  // scan tiles of the fluid flags
     WD_COMPACT,     // fluid flags (compaction)
     WD_MGTYPES,     // + l, l = 1..MG_MAX-1: cells of level l
@@ -287,8 +297,6 @@ uint32_t particle_blocks(const DevSim &S);
 uint32_t key_blocks(const DevSim &S);
 uint32_t tile_blocks(const DevSim &S);
 uint32_t tile_grid_blocks(const DevSim &S);
-uint32_t node_blocks(const DevSim &S);
-uint32_t cell_blocks(const DevSim &S);
 uint32_t flag_count(const DevSim &S);
 uint32_t mg_cell_blocks(const DevSim &S, int l);
 
@@ -301,6 +309,7 @@ void batch_scan_reduce(const WorkLists &W, int which, hipStream_t s);  // which 
 void batch_scan_tiles(const WorkLists &W, int which, hipStream_t s);   // which 1: fluid flags
 void batch_scan_down(const WorkLists &W, int which, hipStream_t s);
 void batch_tile_list(const WorkLists &W, hipStream_t s);
+void batch_band(const WorkLists &W, hipStream_t s);
 void batch_scatter(const WorkLists &W, hipStream_t s);
 void batch_p2g(const WorkLists &W, hipStream_t s);
 void batch_faces(const WorkLists &W, hipStream_t s);

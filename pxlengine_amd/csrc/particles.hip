@@ -43,7 +43,10 @@ __global__ __launch_bounds__(BLOCK) void k_fill_u32(uint32_t *p, uint32_t n, uin
 __device__ __forceinline__ void body_clear_grid(const DevSim &S, uint32_t b) {
     uint32_t i = b * BLOCK + threadIdx.x;
     if (i < S.nkeys) S.count[i] = 0;
-    if (i == 0) *S.tile_count = 0;
+    if (i == 0) {
+        *S.tile_count = 0;
+        *S.band_count = 0;
+    }
 }
 __global__ __launch_bounds__(BLOCK) void k_clear_grid(DevSim S) { body_clear_grid(S, blockIdx.x); }
 __global__ __launch_bounds__(BLOCK) void kb_clear_grid(WorkList W) {
@@ -255,9 +258,34 @@ __device__ __forceinline__ int box_corner(int c, int k) {
 __device__ __forceinline__ void body_tile_list(const DevSim &S, uint32_t b) {
     const uint32_t t = b * BLOCK + threadIdx.x;
     if (t >= S.ntiles) return;
-    if (S.offset[t * TILE_CELLS] != S.offset[(t + 1) * TILE_CELLS]) S.tile_list[atomicAdd(S.tile_count, 1u)] = t;
+    const bool active = S.offset[t * TILE_CELLS] != S.offset[(t + 1) * TILE_CELLS];
+    S.tile_act[t] = (uint8_t)(((S.tile_act[t] & 1u) << 1) | (active ? 1u : 0u));
+    if (active) S.tile_list[atomicAdd(S.tile_count, 1u)] = t;
 }
 __global__ __launch_bounds__(BLOCK) void k_tile_list(DevSim S) { body_tile_list(S, blockIdx.x); }
+
+// Band: tiles with an active tile (this step or the last) among their 3x3x3 neighbours.
+__device__ __forceinline__ void body_band(const DevSim &S, uint32_t b) {
+    const uint32_t t = b * BLOCK + threadIdx.x;
+    if (t >= S.ntiles) return;
+    int ti, tj, tk;
+    ti = (int)(t % (uint32_t)S.ntx);
+    tj = (int)((t / (uint32_t)S.ntx) % (uint32_t)S.nty);
+    tk = (int)(t / ((uint32_t)S.ntx * (uint32_t)S.nty));
+    uint32_t any = 0;
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+                const int x = ti + dx, y = tj + dy, z = tk + dz;
+                if (x < 0 || y < 0 || z < 0 || x >= S.ntx || y >= S.nty || z >= S.ntz) continue;
+                any |= S.tile_act[(uint32_t)x + (uint32_t)S.ntx * ((uint32_t)y + (uint32_t)S.nty * (uint32_t)z)];
+            }
+    if (any) S.band_list[atomicAdd(S.band_count, 1u)] = t;
+}
+__global__ __launch_bounds__(BLOCK) void k_band(DevSim S) { body_band(S, blockIdx.x); }
+__global__ __launch_bounds__(BLOCK) void kb_band(WorkList W) {
+    for_each_work(W, [&](const DevSim &S, uint32_t b) { body_band(S, b); });
+}
 __global__ __launch_bounds__(BLOCK) void kb_tile_list(WorkList W) {
     for_each_work(W, [&](const DevSim &S, uint32_t b) { body_tile_list(S, b); });
 }
@@ -607,6 +635,7 @@ void launch_scan_offsets(const DevSim &S, hipStream_t s) { launch_exclusive_scan
 void launch_scatter(const DevSim &S, hipStream_t s) { PXL_LAUNCH(k_scatter, blocks_for(S.np), BLOCK, 0, s, S); }
 
 void launch_tile_list(const DevSim &S, hipStream_t s) { PXL_LAUNCH(k_tile_list, blocks_for(S.ntiles), BLOCK, 0, s, S); }
+void launch_band(const DevSim &S, hipStream_t s) { PXL_LAUNCH(k_band, blocks_for(S.ntiles), BLOCK, 0, s, S); }
 void launch_p2g(const DevSim &S, hipStream_t s) { PXL_LAUNCH(k_p2g_tile, tile_grid(S), TILE_BLOCK, 0, s, S); }
 void launch_g2p(const DevSim &S, hipStream_t s) { PXL_LAUNCH(k_g2p_tile, tile_grid(S), TILE_BLOCK, 0, s, S); }
 
@@ -648,6 +677,7 @@ void batch_scan_down(const WorkLists &W, int which, hipStream_t s) {
     PXL_LAUNCH(kb_scan_down, BATCH_GRID, BLOCK, 0, s, W[which == 0 ? WD_SCAN_KEYS : WD_SCAN_FLAGS], which);
 }
 void batch_tile_list(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_tile_list, BATCH_GRID, BLOCK, 0, s, W[WD_TILES]); }
+void batch_band(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_band, BATCH_GRID, BLOCK, 0, s, W[WD_TILES]); }
 void batch_scatter(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_scatter, BATCH_GRID, BLOCK, 0, s, W[WD_PART]); }
 void batch_p2g(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_p2g_tile, BATCH_TILE_GRID, TILE_BLOCK, 0, s, W[WD_TILEGRID]); }
 void batch_g2p(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_g2p_tile, BATCH_TILE_GRID, TILE_BLOCK, 0, s, W[WD_TILEGRID]); }

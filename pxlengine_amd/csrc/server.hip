@@ -41,8 +41,6 @@ uint32_t work_blocks(const DevSim &S, int d) {
         case WD_SCAN_KEYS: return scan_tiles(S.nkeys);
         case WD_TILES: return tile_blocks(S);
         case WD_TILEGRID: return tile_grid_blocks(S);
-        case WD_NODES: return node_blocks(S);
-        case WD_CELLS: return cell_blocks(S);
         case WD_SCAN_FLAGS: return scan_tiles(flag_count(S));
         case WD_COMPACT: return (flag_count(S) + BLOCK - 1) / BLOCK;
         case WD_A_SOLVER: return (uint32_t)S.cg_blocks;
@@ -83,7 +81,7 @@ void dev_alloc(T **p, size_t n) {
 // level (level 0 = the fluid list) and the active P2G tiles of the step.
 struct MemberCounts {
     uint32_t fluid[MG_MAX] = {};
-    uint32_t tiles = 0;
+    uint32_t tiles = 0, band = 0;
 };
 
 // Algorithmic bytes of one simulation's share of a launch: every logical array the kernel
@@ -105,8 +103,11 @@ double share_bytes(const char *k, const DevSim &d, const MemberCounts &m, int l)
     if (n == "tile_list") return 8.0 * d.ntiles;
     if (n == "scatter") return 68 * P;
     if (n == "p2g") return 24 * P + box * t;
-    if (n == "faces") return 8 * F + box * t;
-    if (n == "cells") return 4 * F + 15 * C + 4 * (C - Cf);
+    // band passes: the fraction of the grid in the step's band of tiles
+    const double bf = d.ntiles ? std::min(1.0, (double)m.band / (double)d.ntiles) : 1.0;
+    if (n == "band") return 28.0 * d.ntiles;
+    if (n == "faces") return 8 * F * bf + box * t;
+    if (n == "cells") return bf * (4 * F + 15 * C + 4 * C) - 4 * Cf;
     if (n == "fluid_compact") return 8 * flags + 2 * C + 8 * Cf;
     if (n == "mg_types") return cells_of(l - 1) + cells_of(l);
     if (n == "mg_sten") return 7 * cells_of(l) + 4 * cl;
@@ -124,7 +125,7 @@ double share_bytes(const char *k, const DevSim &d, const MemberCounts &m, int l)
     if (n == "mgcg_spmv") return 20 * Cf;
     if (n == "mgcg_update") return 44 * Cf;
     if (n == "mixed_update") return 44 * Cf;
-    if (n == "project") return 8 * F + 5 * C;
+    if (n == "project") return bf * (8 * F + 5 * C);
     if (n == "g2p") return 56 * P + 8 * F;
     return 0;
 }
@@ -204,7 +205,8 @@ public:
             if (next.size() != active.size()) set_active(next);
             active = next;
             const int cur = (int)(pass & 1);
-            replay(&g_inner_[cur], [&] { enqueue_inner_pass(cur); });
+            const uint32_t inner = pass_iterations(opts_.inner_iterations, pass);
+            replay(&g_inner_[pass > 0 ? 1 : 0][cur], [&] { enqueue_inner_pass(cur, inner); });
             read_norms();
             next.clear();
             for (uint32_t i : active) {
@@ -223,7 +225,9 @@ public:
             s->steps_done++;
             s->stats.rel_residual = rel[i];
             s->stats.outer_iterations = passes_[i];
-            s->stats.inner_iterations = passes_[i] * opts_.inner_iterations;
+            uint32_t total = 0;
+            for (uint32_t q = 0; q < passes_[i]; q++) total += pass_iterations(opts_.inner_iterations, q);
+            s->stats.inner_iterations = total;
             s->x64_cur = (int)(passes_[i] & 1);
         }
     }
@@ -260,6 +264,7 @@ private:
         run("scan_tiles", 0, [&] { batch_scan_tiles(lists_, 0, stream_); });
         run("scan_down", 0, [&] { batch_scan_down(lists_, 0, stream_); });
         run("tile_list", -1, [&] { batch_tile_list(lists_, stream_); });
+        run("band", -1, [&] { batch_band(lists_, stream_); });
         run("scatter", -1, [&] { batch_scatter(lists_, stream_); });
         run("p2g", -1, [&] { batch_p2g(lists_, stream_); });
         run("faces", -1, [&] { batch_faces(lists_, stream_); });
@@ -290,8 +295,7 @@ private:
     }
     // One refinement pass for the solving members: MG-CG from e = 0 on float(r64), then the
     // FP64 update x64[cur ^ 1] = x64[cur] + e and the residual norm.
-    void enqueue_inner_pass(int cur) {
-        const uint32_t inner = opts_.inner_iterations;
+    void enqueue_inner_pass(int cur, uint32_t inner) {
         run("cg_residual", -1, [&] { batch_cg_residual(lists_, stream_); });
         enqueue_vcycle();
         run("mgcg_spmv", -1, [&] { batch_mgcg_spmv(lists_, 0, 1, stream_); });
@@ -362,6 +366,7 @@ private:
             const DevSim &d = members_[i]->d;
             HIP_CHECK(hipMemcpyAsync(&counts_[i].fluid[0], d.fl_count + 1, 4, hipMemcpyDeviceToHost, stream_));
             HIP_CHECK(hipMemcpyAsync(&counts_[i].tiles, d.tile_count, 4, hipMemcpyDeviceToHost, stream_));
+            HIP_CHECK(hipMemcpyAsync(&counts_[i].band, d.band_count, 4, hipMemcpyDeviceToHost, stream_));
             for (int l = 1; l < d.mg_levels && l < MG_MAX; l++)
                 HIP_CHECK(hipMemcpyAsync(&counts_[i].fluid[l], d.mg[l].count, 4, hipMemcpyDeviceToHost, stream_));
         }
@@ -387,7 +392,7 @@ private:
     }
 
     void drop_graphs() {
-        for (hipGraphExec_t *g : {&g_grid_, &g_inner_[0], &g_inner_[1], &g_post_}) {
+        for (hipGraphExec_t *g : {&g_grid_, &g_inner_[0][0], &g_inner_[0][1], &g_inner_[1][0], &g_inner_[1][1], &g_post_}) {
             if (*g) (void)hipGraphExecDestroy(*g);
             *g = nullptr;
         }
@@ -514,7 +519,7 @@ private:
     size_t off_[WD_COUNT], cap_[WD_COUNT];
     WorkLists lists_{};
     int max_levels_ = 0, max_tail_ = 0;
-    hipGraphExec_t g_grid_ = nullptr, g_inner_[2] = {nullptr, nullptr}, g_post_ = nullptr;
+    hipGraphExec_t g_grid_ = nullptr, g_inner_[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}}, g_post_ = nullptr;
     uint64_t captures_ = 0, rebuilds_ = 0;
     bool profiling_ = false;
     std::vector<Rec> recs_;

@@ -101,7 +101,7 @@ struct Sim {
     uint64_t steps_done = 0;
     bool initialized = false;
     hipGraphExec_t step_graph = nullptr;
-    hipGraphExec_t inner_graph[2] = {nullptr, nullptr};
+    hipGraphExec_t inner_graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [later pass][x64 parity]
     hipGraphExec_t grid_graph = nullptr, post_graph = nullptr;  // MIXED: phases around the solve
     double *host_scal = nullptr;  // pinned
     int x64_cur = 0;
@@ -113,7 +113,8 @@ struct Sim {
     void drop_graphs() {
         if (step_graph) (void)hipGraphExecDestroy(step_graph);
         step_graph = nullptr;
-        for (hipGraphExec_t *g : {&inner_graph[0], &inner_graph[1], &grid_graph, &post_graph}) {
+        for (hipGraphExec_t *g : {&inner_graph[0][0], &inner_graph[0][1], &inner_graph[1][0], &inner_graph[1][1],
+                                  &grid_graph, &post_graph}) {
             if (*g) (void)hipGraphExecDestroy(*g);
             *g = nullptr;
         }
@@ -146,6 +147,9 @@ struct Sim {
         add(&d.box, 8ull * BOX_ENTRIES * d.ntiles);
         add(&d.tile_list, 4ull * d.ntiles);
         add(&d.tile_count, 64);
+        add(&d.tile_act, d.ntiles);
+        add(&d.band_list, 4ull * d.ntiles);
+        add(&d.band_count, 64);
         add(&d.count, 4 * K);
         add(&d.offset, 4 * (K + 1));
         add(&d.type, C);
@@ -222,6 +226,7 @@ struct Sim {
         KRUN("bin_count", launch_bin_count(d, stream));
         KRUN("scan_offsets", launch_scan_offsets(d, stream));
         KRUN("tile_list", launch_tile_list(d, stream));
+        KRUN("band", launch_band(d, stream));
         KRUN("scatter", launch_scatter(d, stream));
         KRUN("p2g", launch_p2g(d, stream));
         KRUN("faces", launch_faces(d, stream));
@@ -249,24 +254,22 @@ struct Sim {
         enqueue_fixed_pressure();
         enqueue_post();
     }
-    void enqueue_inner(int cur) {
+    void enqueue_inner(int cur, uint32_t inner) {
         KRUN("mgcg_start", launch_mgcg_start(d, d.e, stream));
-        for (uint32_t it = 0; it < opts.inner_iterations; it++)
-            KRUN("mgcg_iter", launch_mgcg_iter(d, d.e, it, it + 1 == opts.inner_iterations, stream));
+        for (uint32_t it = 0; it < inner; it++)
+            KRUN("mgcg_iter", launch_mgcg_iter(d, d.e, it, it + 1 == inner, stream));
         KRUN("mixed_update", launch_mixed_update(d, x64(cur), x64(cur ^ 1), stream));
     }
     void enqueue_grid_init() {
         enqueue_grid_phase();
         KRUN("mixed_init", launch_mixed_init(d, x64(0), stream));
     }
-    void enqueue_inner0() { enqueue_inner(0); }
-    void enqueue_inner1() { enqueue_inner(1); }
-
-    hipGraphExec_t capture(void (Sim::*fn)()) {
+    template <typename F>
+    hipGraphExec_t capture(F &&fn) {
         hipGraph_t g = nullptr;
         hipGraphExec_t ex = nullptr;
         HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        (this->*fn)();
+        fn();
         HIP_CHECK(hipStreamEndCapture(stream, &g));
         hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
@@ -282,12 +285,13 @@ struct Sim {
     }
 
     // Runs `fn` as a graph captured on first use into *g (when graphs are on), else eagerly.
-    void replay(hipGraphExec_t *g, void (Sim::*fn)()) {
+    template <typename F>
+    void replay(hipGraphExec_t *g, F &&fn) {
         if (opts.use_graph && !timer.on) {
             if (!*g) *g = capture(fn);
             HIP_CHECK(hipGraphLaunch(*g, stream));
         } else {
-            (this->*fn)();
+            fn();
         }
     }
 
@@ -299,31 +303,35 @@ struct Sim {
         const double bb = host_scal[SC_BB];
         double rr = host_scal[SC_RR];
         auto rel = [&](double r) { return bb > 0.0 ? std::sqrt(r / bb) : std::sqrt(r); };
-        uint32_t outer = 0;
+        uint32_t outer = 0, inner_total = 0;
         while (rel(rr) > opts.tolerance && outer < opts.max_outer) {
-            replay(&inner_graph[x64_cur], x64_cur == 0 ? &Sim::enqueue_inner0 : &Sim::enqueue_inner1);
+            const int later = outer > 0 ? 1 : 0;
+            const uint32_t inner = pass_iterations(opts.inner_iterations, outer);
+            const int cur = x64_cur;
+            replay(&inner_graph[later][cur], [this, cur, inner] { enqueue_inner(cur, inner); });
             x64_cur ^= 1;
             read_scalars();
             rr = host_scal[SC_RR];
             outer++;
+            inner_total += inner;
         }
         stats.rel_residual = rel(rr);
         stats.outer_iterations = outer;
-        stats.inner_iterations = outer * opts.inner_iterations;
+        stats.inner_iterations = inner_total;
     }
 
     void run_steps(uint32_t n) {
         if (n == 0) return;
         if (opts.solver == PXLFLIP_SOLVER_MIXED) {
             for (uint32_t s = 0; s < n; s++) {
-                replay(&grid_graph, &Sim::enqueue_grid_init);
+                replay(&grid_graph, [this] { enqueue_grid_init(); });
                 mixed_pressure();
-                replay(&post_graph, &Sim::enqueue_post);
+                replay(&post_graph, [this] { enqueue_post(); });
                 steps_done++;
             }
         } else {
             const bool graph = opts.use_graph && !timer.on;
-            if (graph && !step_graph) step_graph = capture(&Sim::enqueue_fixed_step);
+            if (graph && !step_graph) step_graph = capture([this] { enqueue_fixed_step(); });
             for (uint32_t s = 0; s < n; s++) {
                 if (graph) HIP_CHECK(hipGraphLaunch(step_graph, stream));
                 else enqueue_fixed_step();
@@ -336,9 +344,14 @@ struct Sim {
         HIP_CHECK(hipStreamSynchronize(stream));
     }
 
+    // The next two steps walk every tile (the band then covers the whole grid), so every face
+    // and cell holds this step's values again after a reset or a field write.
+    void mark_all_tiles() { HIP_CHECK(hipMemsetAsync(d.tile_act, 3, d.ntiles, stream)); }
+
     void reset() {
         launch_reset_particles(d, stream);
         HIP_CHECK(hipMemsetAsync(d.pressure, 0, 4 * (size_t)d.ncells, stream));
+        mark_all_tiles();
         HIP_CHECK(hipStreamSynchronize(stream));
         steps_done = 0;
         initialized = true;
