@@ -6,8 +6,11 @@ Default workload (`--workload serve`): a stream of independent liquid simulation
 is a cubic grid n^3 on the unit cube with n drawn from {32, 48, 64, 96, 128}, one liquid box
 of random place and size in the interior with a random initial velocity, 4-8 particles per
 cell (fp32 SoA state), dt = f32(1/120), PIC/FLIP 0.95, seeded with seed = i, and `--steps`
-steps.  Requests arrive as a Poisson process (rate: `--rate`, or 1.25x the service rate measured on
-a warm-up batch, so the device stays loaded) at a server with a 200 GB HBM admission budget.
+steps.  Requests arrive as a Poisson process at a server with a 200 GB HBM admission budget.
+The arrival rate is `--rate`, or by default the steady-state service rate of a fixed batch of
+`--warmup` (64) warm-up requests drawn from the same distribution, in requests per second of
+this stream's mean work: the load a server holding about that many simulations at once
+sustains (the batch grows past it when arrivals bunch up and drains back when they thin).
 Each simulation's pressure solve runs to a relative FP64 residual of 1e-6 (multigrid CG in
 FP32 inside FP64 refinement).  `value` = sum over requests of particles x steps, divided by
 the time from the first submission to the last completion (whole job, inputs created on the
@@ -138,18 +141,19 @@ def bench_serve(args, pxl):
     P = np.array([inf["n_particles"] for inf in infos], dtype=np.float64)
     work = P * args.steps
 
-    # warm-up (untimed): device init, kernels, and the service rate for the arrival process
+    # warm-up (untimed): device init and kernels through a short server run, then the
+    # steady-state service rate of a fixed batch of the warm-up requests
     warm = [random_box(100000 + i, sizes=sizes) for i in range(args.warmup)]
     with pxl.Server(**opts) as srv:
-        for i, f in enumerate(warm):
-            srv.submit(f, FPS, seed=i, steps=max(2, args.steps // 2))
+        for i, f in enumerate(warm[:8]):
+            srv.submit(f, FPS, seed=i, steps=2)
         srv.drain()
-        wst = srv.stats()
-    rate = args.rate
-    if rate <= 0:
-        # requests per second the batch sustained in the warm-up, scaled to this stream's work
-        # with 25% headroom so the server always has queued work
-        rate = 1.25 * wst["throughput"] / max(work.mean(), 1.0) if wst["throughput"] > 0 else 100.0
+    with pxl.Batch(warm, FPS, seeds=list(range(len(warm)))) as bat:
+        bat.step(2)
+        t0 = time.perf_counter()
+        bat.step(args.calib_steps)
+        warm_rate = sum(i["n_particles"] for i in bat.infos) * args.calib_steps / (time.perf_counter() - t0)
+    rate = args.rate if args.rate > 0 else warm_rate / max(work.mean(), 1.0)
     rng = np.random.default_rng(12345)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(reqs)))
     arrivals -= arrivals[0]
@@ -203,10 +207,12 @@ def bench_serve(args, pxl):
                         f"liquid box at 4-8 particles/cell, {args.steps} steps each, Poisson arrivals, "
                         f"{args.budget / 1e9:.0f} GB admission budget, MIXED solve to 1e-6",
             "requests": len(reqs), "particles_total": float(P.sum()), "rate_per_s": rate,
+            "rate_rule": (f"steady-state service rate of a fixed batch of {len(warm)} warm-up requests "
+                          f"({warm_rate:.4g} particle-steps/s) / mean request work") if args.rate <= 0 else "--rate",
             "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
             "busy_fraction": st["busy_s"] / st["span_s"] if st["span_s"] else None,
             "mean_batch": st["mean_batch"], "batch_steps": st["batch_steps"], "admit_s": st["admit_s"],
-            "graph_captures": st["graph_captures"], "peak_bytes": st["peak_bytes"], "wall_s": wall,
+            "graph_captures": st["graph_captures"], "capture_s": st["capture_s"], "peak_bytes": st["peak_bytes"], "wall_s": wall,
             "failed": len(failed), "max_rel_residual": max_res,
             "plain": plain, "ratio_vs_plain": value / plain["value"],
         },
@@ -286,7 +292,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60, help="steps per request (dam: timed steps)")
-    ap.add_argument("--warmup", type=int, default=64, help="warm-up requests (dam: warm-up steps)")
+    ap.add_argument("--warmup", type=int, default=64,
+                    help="warm-up requests: the batch whose service rate sets the arrival rate (dam: warm-up steps)")
+    ap.add_argument("--calib-steps", type=int, default=10, help="timed steps of the warm-up batch")
     ap.add_argument("--workload", default="serve", choices=["serve", "dam", "batch"])
     ap.add_argument("--requests", type=int, default=512)
     ap.add_argument("--sizes", default="32,48,64,96,128")

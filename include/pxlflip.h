@@ -246,6 +246,7 @@ typedef struct pxlflip_server_stats {
     double admit_s;          /* device-thread time spent allocating admitted requests */
     uint64_t rebuilds;       /* work-list rebuilds (batch composition changes) */
     uint64_t graph_captures; /* batch graphs captured */
+    double capture_s;        /* device-thread time spent capturing and instantiating them */
 } pxlflip_server_stats;
 
 int pxlflip_server_default_opts(pxlflip_server_opts *opts);

@@ -91,7 +91,7 @@ __device__ __forceinline__ void body_faces(const DevSim &S, uint32_t b, uint32_t
 }
 __global__ __launch_bounds__(GBLOCK) void k_faces(DevSim S) { body_faces(S, blockIdx.x, gridDim.x); }
 __global__ __launch_bounds__(GBLOCK) void kb_faces(WorkList W) {
-    for_each_work(W, [&](const DevSim &S, uint32_t b) { body_faces(S, b, band_grid(S)); });
+    for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *) { body_faces(S, b, band_grid(S)); });
 }
 
 // The P2G sums as the reference's acc / wacc face buffers (for read-back).
@@ -150,7 +150,7 @@ __device__ __forceinline__ void body_cells(const DevSim &S, uint32_t b, uint32_t
 }
 __global__ __launch_bounds__(GBLOCK) void k_cells(DevSim S) { body_cells(S, blockIdx.x, gridDim.x); }
 __global__ __launch_bounds__(GBLOCK) void kb_cells(WorkList W) {
-    for_each_work(W, [&](const DevSim &S, uint32_t b) { body_cells(S, b, band_grid(S)); });
+    for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *) { body_cells(S, b, band_grid(S)); });
 }
 
 // Fluid list from the scanned flags.  Split: entry e < C is cell e (red), e >= C is cell
@@ -183,7 +183,7 @@ __device__ __forceinline__ void body_fluid_compact(const DevSim &S, uint32_t b) 
 }
 __global__ __launch_bounds__(BLOCK) void k_fluid_compact(DevSim S) { body_fluid_compact(S, blockIdx.x); }
 __global__ __launch_bounds__(BLOCK) void kb_fluid_compact(WorkList W) {
-    for_each_work(W, [&](const DevSim &S, uint32_t b) { body_fluid_compact(S, b); });
+    for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *) { body_fluid_compact(S, b); });
 }
 
 __device__ __forceinline__ uint8_t type_at(const DevSim &S, int i, int j, int k) {
@@ -216,7 +216,7 @@ __device__ __forceinline__ void body_project(const DevSim &S, uint32_t b, uint32
 }
 __global__ __launch_bounds__(GBLOCK) void k_project(DevSim S) { body_project(S, blockIdx.x, gridDim.x); }
 __global__ __launch_bounds__(GBLOCK) void kb_project(WorkList W) {
-    for_each_work(W, [&](const DevSim &S, uint32_t b) { body_project(S, b, band_grid(S)); });
+    for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *) { body_project(S, b, band_grid(S)); });
 }
 
 __global__ __launch_bounds__(TX *TY) void k_type_u32(DevSim S, uint32_t *out) {
@@ -251,11 +251,11 @@ void launch_type_u32(const DevSim &S, uint32_t *out, hipStream_t s) {
 // ---- batched
 uint32_t flag_count(const DevSim &S) { return S.split_colours ? 2 * S.ncells : S.ncells; }
 
-void batch_faces(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_faces, BATCH_GRID, GBLOCK, 0, s, W[WD_TILEGRID]); }
-void batch_cells(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_cells, BATCH_GRID, GBLOCK, 0, s, W[WD_TILEGRID]); }
+void batch_faces(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_faces, batch_grid(W[WD_TILEGRID], BATCH_GRID), GBLOCK, 0, s, W[WD_TILEGRID]); }
+void batch_cells(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_cells, batch_grid(W[WD_TILEGRID], BATCH_GRID), GBLOCK, 0, s, W[WD_TILEGRID]); }
 void batch_fluid_compact(const WorkLists &W, hipStream_t s) {
-    PXL_LAUNCH(kb_fluid_compact, BATCH_GRID, BLOCK, 0, s, W[WD_COMPACT]);
+    PXL_LAUNCH(kb_fluid_compact, batch_grid(W[WD_COMPACT], BATCH_GRID), BLOCK, 0, s, W[WD_COMPACT]);
 }
-void batch_project(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_project, BATCH_GRID, GBLOCK, 0, s, W[WD_TILEGRID]); }
+void batch_project(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_project, batch_grid(W[WD_TILEGRID], BATCH_GRID), GBLOCK, 0, s, W[WD_TILEGRID]); }
 
 }  // namespace pxl

@@ -253,18 +253,41 @@ constexpr uint32_t WORK_SIM_SHIFT = 20;
 constexpr uint32_t WORK_BLOCK_MASK = (1u << WORK_SIM_SHIFT) - 1u;
 constexpr uint32_t MAX_BATCH_SIMS = 1u << (32 - WORK_SIM_SHIFT);
 
+// A batch keeps every member's list counts in one array, COUNT_SLOTS words per simulation:
+// [0..1] = DevSim::fl_count, [2l..2l+1] = MGLevel::count of level l (the members' DevSim
+// copies point there).  A batched kernel derives a member's slots from its work entry alone,
+// so the counts load together with the member's DevSim fields instead of after them.
+constexpr uint32_t COUNT_SLOTS = 2 * MG_MAX;
+// Every work list's storage holds at least this many entries.
+constexpr uint32_t WORK_LIST_MIN_CAP = 16384;
+
 struct WorkList {
     const DevSim *sims;
     const uint32_t *entries;
     const uint32_t *count;  // device word: number of entries
+    uint32_t *slots;        // the batch's list counts (COUNT_SLOTS per simulation)
+    uint32_t grid;          // workgroups a batched launch over this list uses (host; 0 = the kernel's maximum)
 };
+// Persistent grid of a batched launch: the list's grid, at most the kernel's maximum and
+// never more than a list's minimum storage (for_each_work reads entry blockIdx.x unchecked).
+inline unsigned batch_grid(const WorkList &w, unsigned max_grid) {
+    const unsigned g = w.grid == 0 || w.grid > max_grid ? max_grid : w.grid;
+    return g < WORK_LIST_MIN_CAP ? g : WORK_LIST_MIN_CAP;
+}
 
+// Runs f(S, block, slots) for each work entry of this workgroup.  The first entry is loaded
+// together with the entry count (it lies in the list's storage even when the list is
+// shorter), so a workgroup's first item waits for one load instead of two.
 template <typename F>
 __device__ __forceinline__ void for_each_work(const WorkList &W, F &&f) {
     const uint32_t n = *W.count;
-    for (uint32_t e = blockIdx.x; e < n; e += gridDim.x) {
-        const uint32_t w = W.entries[e];
-        f(W.sims[w >> WORK_SIM_SHIFT], w & WORK_BLOCK_MASK);
+    uint32_t e = blockIdx.x;
+    uint32_t w = W.entries[e];
+    while (e < n) {
+        const uint32_t sim = w >> WORK_SIM_SHIFT;
+        f(W.sims[sim], w & WORK_BLOCK_MASK, W.slots + (size_t)sim * COUNT_SLOTS);
+        e += gridDim.x;
+        if (e < n) w = W.entries[e];
     }
 }
 
@@ -299,6 +322,14 @@ uint32_t tile_blocks(const DevSim &S);
 uint32_t tile_grid_blocks(const DevSim &S);
 uint32_t flag_count(const DevSim &S);
 uint32_t mg_cell_blocks(const DevSim &S, int l);
+// Workgroups (list slices) of the multigrid list launches on level l: the solver's count on
+// level 0, an eighth of it per coarsening after that (a coarse level holds about an eighth of
+// the FLUID cells), at least one.
+__host__ __device__ inline uint32_t level_blocks(int cg_blocks, int l) {
+    if (l == 0) return (uint32_t)cg_blocks;
+    const uint32_t g = ((uint32_t)cg_blocks + (1u << (3 * l)) - 1u) >> (3 * l);
+    return g ? g : 1u;
+}
 
 // Batched launchers: W[d] is the work list of domain d.
 using WorkLists = WorkList[WD_COUNT];

@@ -24,8 +24,9 @@
 // Levels below mg_tail are walked through lists of their FLUID cells, red cells apart from
 // black ones (the step's colour-split fluid list on level 0, per-level lists built with the
 // coarse types), so the work follows the liquid, not the grid, and a colour sweep keeps every
-// lane busy; a launch takes a list segment in cg_blocks slices, like the solver kernels, and
-// a lane updates two cells with all their loads in flight together.
+// lane busy; a launch takes a list segment in level_blocks slices (the solver kernels' count on
+// level 0, an eighth of it per coarser level), and a lane updates two cells with all their
+// loads in flight together.
 // Coarse z is kept 0 on non-FLUID cells (the prolongation reads the parent of any FLUID
 // cell).  Levels from mg_tail on (a few thousand cells together) run the same cycle in one
 // workgroup with everything in LDS.  Arithmetic is fp32 (the FP64 refinement outside fixes
@@ -144,10 +145,10 @@ __global__ __launch_bounds__(MBLOCK) void k_mg_sten(MGLevel v) { body_mg_sten(v,
 // ---- level 0 (the step's colour-split fluid list: red entries [0, n_red), black [n_red, n);
 // r, u are the CG vectors)
 // pre0: red u = r / d, black u = (r + sum FLUID red nbrs r_n / d_n) / d
-__device__ __forceinline__ void body_l0_pre0(const DevSim &S, const float *__restrict__ r, float *__restrict__ u,
-                                             uint32_t b, uint32_t nb) {
+__device__ __forceinline__ void body_l0_pre0(const DevSim &S, const uint32_t *flc, const float *__restrict__ r,
+                                             float *__restrict__ u, uint32_t b, uint32_t nb) {
     uint32_t beg, end;
-    slice(S.fl_count[1], b, nb, beg, end);
+    slice(flc[1], b, nb, beg, end);
     const int32_t off[6] = {-1, 1, -S.nx, S.nx, -S.nx * S.ny, S.nx * S.ny};
     for (uint32_t t = beg + threadIdx.x; t < end; t += LBLOCK) {
         const uint32_t id = S.fl_list[t], wd = S.fl_word[t];
@@ -163,9 +164,9 @@ __device__ __forceinline__ void body_l0_pre0(const DevSim &S, const float *__res
 // One colour's update over its list segment.  A lane takes entries t and t + LBLOCK and issues
 // the loads of both before either store (they read only the other colour, so the order of the
 // two updates does not matter).
-__device__ __forceinline__ void body_l0_sweep(const DevSim &S, const float *__restrict__ r, float *__restrict__ u,
-                                              uint32_t colour, uint32_t b, uint32_t nb) {
-    const uint32_t n_red = S.fl_count[0], n = S.fl_count[1];
+__device__ __forceinline__ void body_l0_sweep(const DevSim &S, const uint32_t *flc, const float *__restrict__ r,
+                                              float *__restrict__ u, uint32_t colour, uint32_t b, uint32_t nb) {
+    const uint32_t n_red = flc[0], n = flc[1];
     const uint32_t lo = colour ? n_red : 0u, hi = colour ? n : n_red;
     uint32_t beg, end;
     slice(hi - lo, b, nb, beg, end);
@@ -191,9 +192,10 @@ __device__ __forceinline__ void body_l0_sweep(const DevSim &S, const float *__re
     }
 }
 // post: black update from red neighbours' u + the prolonged correction c.z
-__device__ __forceinline__ void body_l0_post(const DevSim &S, const MGLevel &c, const float *__restrict__ r,
-                                             float *__restrict__ u, uint32_t b, uint32_t nb) {
-    const uint32_t n_red = S.fl_count[0], n = S.fl_count[1];
+__device__ __forceinline__ void body_l0_post(const DevSim &S, const uint32_t *flc, const MGLevel &c,
+                                             const float *__restrict__ r, float *__restrict__ u, uint32_t b,
+                                             uint32_t nb) {
+    const uint32_t n_red = flc[0], n = flc[1];
     uint32_t beg, end;
     slice(n - n_red, b, nb, beg, end);
     beg += n_red;
@@ -226,17 +228,26 @@ __device__ __forceinline__ void body_lc_pre0(const MGLevel &v, uint32_t b, uint3
     uint32_t beg, end;
     slice(n, b, nb, beg, end);
     const int32_t off[6] = {-1, 1, -v.nx, v.nx, -v.nx * v.ny, v.nx * v.ny};
+    const uint32_t last = ncell(v) - 1u;
     for (uint32_t t = beg + threadIdx.x; t < end; t += LBLOCK) {
         const uint32_t id = lc_entry(v, t, n_red);
         const uint32_t st = v.sten[id];
         float sum = 0.0f;
         if (t >= n_red) {
+            // neighbours are loaded before the cell's stencil word arrives (indices clamped
+            // into the level) and the ones that are not FLUID are dropped afterwards
+            float rq[6];
+            uint32_t sq[6];
 #pragma unroll
             for (int q = 0; q < 6; q++) {
-                if (st & (1u << q)) {
-                    const uint32_t nq = id + off[q];
-                    sum += v.r[nq] * rcp_or0(st_diag(v.sten[nq]));
-                }
+                const uint32_t nq = min(id + (uint32_t)off[q], last);
+                rq[q] = v.r[nq];
+                sq[q] = v.sten[nq];
+            }
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                const float tq = rq[q] * rcp_or0(st_diag(sq[q]));
+                sum = (st & (1u << q)) ? sum + tq : sum;
             }
         }
         v.z[id] = (v.r[id] + sum) * rcp_or0(st_diag(st));
@@ -275,18 +286,23 @@ __device__ __forceinline__ void body_lc_post(const MGLevel &v, const MGLevel &c,
     slice(n, b, nb, beg, end);
     for (uint32_t t = beg + threadIdx.x; t < end; t += LBLOCK) {
         const uint32_t id = v.list[last - t];
-        const uint32_t st = v.sten[id];
         int i, j, k;
         cell_ijk(v.nx, v.ny, id, i, j, k);
         const int nbc[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k}, {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
-        float sum = 0.0f;
+        // neighbour and parent values are loaded together with the stencil word (coordinates
+        // clamped into the level); the ones that are not FLUID are dropped afterwards
+        float zq[6], pq[6];
 #pragma unroll
         for (int q = 0; q < 6; q++) {
-            if (st & (1u << q)) {
-                const int a = nbc[q][0], bb = nbc[q][1], d = nbc[q][2];
-                sum += v.z[lidx(v, a, bb, d)] + c.z[parent(c.nx, c.ny, a, bb, d)];
-            }
+            const int a = min(max(nbc[q][0], 0), v.nx - 1), bb = min(max(nbc[q][1], 0), v.ny - 1),
+                      d = min(max(nbc[q][2], 0), v.nz - 1);
+            zq[q] = v.z[lidx(v, a, bb, d)];
+            pq[q] = c.z[parent(c.nx, c.ny, a, bb, d)];
         }
+        const uint32_t st = v.sten[id];
+        float sum = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 6; q++) sum = (st & (1u << q)) ? sum + (zq[q] + pq[q]) : sum;
         v.z[id] = (v.r[id] + sum) * rcp_or0(st_diag(st));
     }
 }
@@ -301,25 +317,39 @@ __device__ __forceinline__ void body_restrict(int fnx, int fny, int fnz, const u
     uint32_t beg, end;
     slice(n, b, nb, beg, end);
     const int32_t off[6] = {-1, 1, -fnx, fnx, -fnx * fny, fnx * fny};
+    const uint32_t flast = (uint32_t)(fnx * fny * fnz) - 1u;
     for (uint32_t t = beg + threadIdx.x; t < end; t += LBLOCK) {
         const uint32_t cc = lc_entry(c, t, n_red);
         int ci, cj, ck;
         cell_ijk(c.nx, c.ny, cc, ci, cj, ck);
         const int red[4] = {0, 3, 5, 6};
-        float acc = 0.0f;
+        // every child's values and neighbours are loaded before its stencil word arrives
+        // (indices clamped into the fine level; children past its edge read cell 0 and are
+        // skipped, neighbours that are not FLUID are dropped afterwards)
+        bool in[4];
+        uint32_t st[4];
+        float rc[4], zc[4], zn[4][6];
 #pragma unroll
         for (int m = 0; m < 4; m++) {
             const int q = red[m];
             const int fi = 2 * ci + (q & 1), fj = 2 * cj + ((q >> 1) & 1), fk = 2 * ck + (q >> 2);
-            if (fi >= fnx || fj >= fny || fk >= fnz) continue;
-            const uint32_t id = (uint32_t)fi + (uint32_t)fnx * ((uint32_t)fj + (uint32_t)fny * (uint32_t)fk);
-            const uint32_t st = fsten[id];
-            if (!(st & STEN_FLUID)) continue;
+            in[m] = fi < fnx && fj < fny && fk < fnz;
+            const uint32_t id =
+                in[m] ? (uint32_t)fi + (uint32_t)fnx * ((uint32_t)fj + (uint32_t)fny * (uint32_t)fk) : 0u;
+            st[m] = fsten[id];
+            rc[m] = fr[id];
+            zc[m] = fz[id];
+#pragma unroll
+            for (int n = 0; n < 6; n++) zn[m][n] = fz[min(id + (uint32_t)off[n], flast)];
+        }
+        float acc = 0.0f;
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            if (!in[m] || !(st[m] & STEN_FLUID)) continue;
             float sum = 0.0f;
 #pragma unroll
-            for (int n = 0; n < 6; n++)
-                if (st & (1u << n)) sum += fz[id + off[n]];
-            acc += fr[id] - st_diag(st) * fz[id] + sum;
+            for (int n = 0; n < 6; n++) sum = (st[m] & (1u << n)) ? sum + zn[m][n] : sum;
+            acc += rc[m] - st_diag(st[m]) * zc[m] + sum;
         }
         c.r[cc] = 0.5f * acc;
     }
@@ -467,13 +497,15 @@ __device__ __forceinline__ void body_mg_tail(const DevSim &S) {
 }
 __global__ __launch_bounds__(TBLOCK) void k_mg_tail(DevSim S) { body_mg_tail(S); }
 
-// ---- single-sim launches (grid = cg_blocks workgroups for the list kernels)
-__global__ __launch_bounds__(LBLOCK) void k_l0_pre0(DevSim S) { body_l0_pre0(S, S.cg_r0, S.cg_r1, blockIdx.x, gridDim.x); }
+// ---- single-sim launches (grid = level_blocks workgroups for the list kernels)
+__global__ __launch_bounds__(LBLOCK) void k_l0_pre0(DevSim S) {
+    body_l0_pre0(S, S.fl_count, S.cg_r0, S.cg_r1, blockIdx.x, gridDim.x);
+}
 __global__ __launch_bounds__(LBLOCK) void k_l0_sweep(DevSim S, uint32_t colour) {
-    body_l0_sweep(S, S.cg_r0, S.cg_r1, colour, blockIdx.x, gridDim.x);
+    body_l0_sweep(S, S.fl_count, S.cg_r0, S.cg_r1, colour, blockIdx.x, gridDim.x);
 }
 __global__ __launch_bounds__(LBLOCK) void k_l0_post(DevSim S, MGLevel c) {
-    body_l0_post(S, c, S.cg_r0, S.cg_r1, blockIdx.x, gridDim.x);
+    body_l0_post(S, S.fl_count, c, S.cg_r0, S.cg_r1, blockIdx.x, gridDim.x);
 }
 __global__ __launch_bounds__(LBLOCK) void k_lc_pre0(MGLevel v) { body_lc_pre0(v, blockIdx.x, gridDim.x); }
 __global__ __launch_bounds__(LBLOCK) void k_lc_sweep(MGLevel v, uint32_t colour) {
@@ -500,61 +532,68 @@ __host__ __device__ inline MGLevel level0(const DevSim &S) {
 
 // ---- batched: level l of each simulation in the work list
 __global__ __launch_bounds__(MBLOCK) void kb_mg_types(WorkList W, int l) {
-    for_each_work(W, [&](const DevSim &S, uint32_t b) {
+    for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *) {
         const MGLevel f = l == 1 ? level0(S) : S.mg[l - 1], c = S.mg[l];
         body_mg_types(f, c, b);
     });
 }
 __global__ __launch_bounds__(MBLOCK) void kb_mg_sten(WorkList W, int l) {
-    for_each_work(W, [&](const DevSim &S, uint32_t b) {
+    for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *) {
         const MGLevel v = S.mg[l];
         body_mg_sten(v, b);
     });
 }
+// A member's level copy with its count pointer taken from the batch's slots (known from the
+// work entry alone, so the count loads alongside the level's fields).
+__device__ __forceinline__ MGLevel level_in_batch(const DevSim &S, uint32_t *cnt, int l) {
+    MGLevel v = S.mg[l];
+    v.count = cnt + 2 * l;
+    return v;
+}
 __global__ __launch_bounds__(LBLOCK) void kb_pre0(WorkList W, int l) {
-    for_each_work(W, [&](const DevSim &S, uint32_t b) {
+    for_each_work(W, [&](const DevSim &S, uint32_t b, uint32_t *cnt) {
         if (l == 0) {
-            body_l0_pre0(S, S.cg_r0, S.cg_r1, b, S.cg_blocks);
+            body_l0_pre0(S, cnt, S.cg_r0, S.cg_r1, b, S.cg_blocks);
         } else {
-            const MGLevel v = S.mg[l];
-            body_lc_pre0(v, b, S.cg_blocks);
+            const MGLevel v = level_in_batch(S, cnt, l);
+            body_lc_pre0(v, b, level_blocks(S.cg_blocks, l));
         }
     });
 }
 __global__ __launch_bounds__(LBLOCK) void kb_sweep(WorkList W, int l, uint32_t colour) {
-    for_each_work(W, [&](const DevSim &S, uint32_t b) {
+    for_each_work(W, [&](const DevSim &S, uint32_t b, uint32_t *cnt) {
         if (l == 0) {
-            body_l0_sweep(S, S.cg_r0, S.cg_r1, colour, b, S.cg_blocks);
+            body_l0_sweep(S, cnt, S.cg_r0, S.cg_r1, colour, b, S.cg_blocks);
         } else {
-            const MGLevel v = S.mg[l];
-            body_lc_sweep(v, colour, b, S.cg_blocks);
+            const MGLevel v = level_in_batch(S, cnt, l);
+            body_lc_sweep(v, colour, b, level_blocks(S.cg_blocks, l));
         }
     });
 }
 __global__ __launch_bounds__(LBLOCK) void kb_restrict(WorkList W, int l) {
-    for_each_work(W, [&](const DevSim &S, uint32_t b) {
-        const MGLevel c = S.mg[l + 1];
+    for_each_work(W, [&](const DevSim &S, uint32_t b, uint32_t *cnt) {
+        const MGLevel c = level_in_batch(S, cnt, l + 1);
         if (l == 0) {
             body_restrict(S.nx, S.ny, S.nz, S.sten, S.cg_r0, S.cg_r1, c, b, S.cg_blocks);
         } else {
             const MGLevel f = S.mg[l];
-            body_restrict(f.nx, f.ny, f.nz, f.sten, f.r, f.z, c, b, S.cg_blocks);
+            body_restrict(f.nx, f.ny, f.nz, f.sten, f.r, f.z, c, b, level_blocks(S.cg_blocks, l));
         }
     });
 }
 __global__ __launch_bounds__(LBLOCK) void kb_post(WorkList W, int l) {
-    for_each_work(W, [&](const DevSim &S, uint32_t b) {
+    for_each_work(W, [&](const DevSim &S, uint32_t b, uint32_t *cnt) {
         const MGLevel c = S.mg[l + 1];
         if (l == 0) {
-            body_l0_post(S, c, S.cg_r0, S.cg_r1, b, S.cg_blocks);
+            body_l0_post(S, cnt, c, S.cg_r0, S.cg_r1, b, S.cg_blocks);
         } else {
-            const MGLevel v = S.mg[l];
-            body_lc_post(v, c, b, S.cg_blocks);
+            const MGLevel v = level_in_batch(S, cnt, l);
+            body_lc_post(v, c, b, level_blocks(S.cg_blocks, l));
         }
     });
 }
 __global__ __launch_bounds__(TBLOCK) void kb_mg_tail(WorkList W) {
-    for_each_work(W, [&](const DevSim &S, uint32_t) { body_mg_tail(S); });
+    for_each_work(W, [&](const DevSim &S, uint32_t, const uint32_t *) { body_mg_tail(S); });
 }
 
 }  // namespace
@@ -572,8 +611,8 @@ void launch_mg_vcycle(const DevSim &S, const float *r, float *u, hipStream_t s) 
     (void)r;
     (void)u;
     const int T = S.mg_tail;
-    const unsigned g = (unsigned)S.cg_blocks;
     for (int l = 0; l < T; l++) {
+        const unsigned g = level_blocks(S.cg_blocks, l);
         if (l == 0) {
             PXL_LAUNCH(k_l0_pre0, g, LBLOCK, 0, s, S);
             PXL_LAUNCH(k_l0_sweep, g, LBLOCK, 0, s, S, 0);
@@ -588,6 +627,7 @@ void launch_mg_vcycle(const DevSim &S, const float *r, float *u, hipStream_t s) 
     }
     PXL_LAUNCH(k_mg_tail, 1, TBLOCK, 0, s, S);
     for (int l = T - 1; l >= 0; l--) {
+        const unsigned g = level_blocks(S.cg_blocks, l);
         if (l == 0) {
             PXL_LAUNCH(k_l0_post, g, LBLOCK, 0, s, S, S.mg[1]);
             PXL_LAUNCH(k_l0_sweep, g, LBLOCK, 0, s, S, 0);
@@ -605,16 +645,16 @@ void launch_mg_vcycle(const DevSim &S, const float *r, float *u, hipStream_t s) 
 // ---- batched
 uint32_t mg_cell_blocks(const DevSim &S, int l) { return grid_for(cells(l == 0 ? level0(S) : S.mg[l])); }
 
-void batch_mg_types(const WorkLists &W, int l, hipStream_t s) { PXL_LAUNCH(kb_mg_types, BATCH_GRID, MBLOCK, 0, s, W[WD_MGTYPES + l], l); }
-void batch_mg_sten(const WorkLists &W, int l, hipStream_t s) { PXL_LAUNCH(kb_mg_sten, BATCH_GRID, MBLOCK, 0, s, W[WD_MGTYPES + l], l); }
-void batch_mg_pre0(const WorkLists &W, int l, hipStream_t s) { PXL_LAUNCH(kb_pre0, BATCH_LGRID, LBLOCK, 0, s, W[WD_A_LVL + l], l); }
+void batch_mg_types(const WorkLists &W, int l, hipStream_t s) { PXL_LAUNCH(kb_mg_types, batch_grid(W[WD_MGTYPES + l], BATCH_GRID), MBLOCK, 0, s, W[WD_MGTYPES + l], l); }
+void batch_mg_sten(const WorkLists &W, int l, hipStream_t s) { PXL_LAUNCH(kb_mg_sten, batch_grid(W[WD_MGTYPES + l], BATCH_GRID), MBLOCK, 0, s, W[WD_MGTYPES + l], l); }
+void batch_mg_pre0(const WorkLists &W, int l, hipStream_t s) { PXL_LAUNCH(kb_pre0, batch_grid(W[WD_A_LVL + l], BATCH_LGRID), LBLOCK, 0, s, W[WD_A_LVL + l], l); }
 void batch_mg_sweep(const WorkLists &W, int l, uint32_t colour, hipStream_t s) {
-    PXL_LAUNCH(kb_sweep, BATCH_LGRID, LBLOCK, 0, s, W[WD_A_LVL + l], l, colour);
+    PXL_LAUNCH(kb_sweep, batch_grid(W[WD_A_LVL + l], BATCH_LGRID), LBLOCK, 0, s, W[WD_A_LVL + l], l, colour);
 }
 void batch_mg_restrict(const WorkLists &W, int l, hipStream_t s) {
-    PXL_LAUNCH(kb_restrict, BATCH_LGRID, LBLOCK, 0, s, W[WD_A_LVL + l], l);
+    PXL_LAUNCH(kb_restrict, batch_grid(W[WD_A_LVL + l], BATCH_LGRID), LBLOCK, 0, s, W[WD_A_LVL + l], l);
 }
-void batch_mg_post(const WorkLists &W, int l, hipStream_t s) { PXL_LAUNCH(kb_post, BATCH_LGRID, LBLOCK, 0, s, W[WD_A_LVL + l], l); }
-void batch_mg_tail(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_mg_tail, BATCH_TAIL_GRID, TBLOCK, 0, s, W[WD_A_SIM]); }
+void batch_mg_post(const WorkLists &W, int l, hipStream_t s) { PXL_LAUNCH(kb_post, batch_grid(W[WD_A_LVL + l], BATCH_LGRID), LBLOCK, 0, s, W[WD_A_LVL + l], l); }
+void batch_mg_tail(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_mg_tail, batch_grid(W[WD_A_SIM], BATCH_TAIL_GRID), TBLOCK, 0, s, W[WD_A_SIM]); }
 
 }  // namespace pxl

@@ -50,7 +50,7 @@ __device__ __forceinline__ void body_clear_grid(const DevSim &S, uint32_t b) {
 }
 __global__ __launch_bounds__(BLOCK) void k_clear_grid(DevSim S) { body_clear_grid(S, blockIdx.x); }
 __global__ __launch_bounds__(BLOCK) void kb_clear_grid(WorkList W) {
-    for_each_work(W, [&](const DevSim &S, uint32_t b) { body_clear_grid(S, b); });
+    for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *) { body_clear_grid(S, b); });
 }
 
 // Counting-sort pass 1: tiled cell key of each particle and its arrival slot in that cell
@@ -84,7 +84,7 @@ __device__ __forceinline__ void body_bin_count(const DevSim &S, uint32_t b) {
 }
 __global__ __launch_bounds__(BLOCK) void k_bin_count(DevSim S) { body_bin_count(S, blockIdx.x); }
 __global__ __launch_bounds__(BLOCK) void kb_bin_count(WorkList W) {
-    for_each_work(W, [&](const DevSim &S, uint32_t b) { body_bin_count(S, b); });
+    for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *) { body_bin_count(S, b); });
 }
 
 // Counting-sort pass 2: exclusive scan of the counts (reduce / scan of chunk sums / down-sweep).
@@ -192,19 +192,19 @@ __device__ __forceinline__ ScanIO scan_io(const DevSim &S, int which) {
     return ScanIO{S.fl_flag, S.split_colours ? 2 * S.ncells : S.ncells, S.fl_tmp, S.fl_off};
 }
 __global__ __launch_bounds__(BLOCK) void kb_scan_reduce(WorkList W, int which) {
-    for_each_work(W, [&](const DevSim &S, uint32_t b) {
+    for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *) {
         const ScanIO io = scan_io(S, which);
         body_scan_reduce(io.in, io.n, io.tmp, b);
     });
 }
 __global__ __launch_bounds__(BLOCK) void kb_scan_tiles(WorkList W, int which) {
-    for_each_work(W, [&](const DevSim &S, uint32_t) {
+    for_each_work(W, [&](const DevSim &S, uint32_t, const uint32_t *) {
         const ScanIO io = scan_io(S, which);
         body_scan_tiles(io.tmp, (io.n + SCAN_TILE - 1) / SCAN_TILE);
     });
 }
 __global__ __launch_bounds__(BLOCK) void kb_scan_down(WorkList W, int which) {
-    for_each_work(W, [&](const DevSim &S, uint32_t b) {
+    for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *) {
         const ScanIO io = scan_io(S, which);
         body_scan_down(io.in, io.n, io.tmp, io.out, b);
     });
@@ -225,7 +225,7 @@ __device__ __forceinline__ void body_scatter(const DevSim &S, uint32_t b) {
 }
 __global__ __launch_bounds__(BLOCK) void k_scatter(DevSim S) { body_scatter(S, blockIdx.x); }
 __global__ __launch_bounds__(BLOCK) void kb_scatter(WorkList W) {
-    for_each_work(W, [&](const DevSim &S, uint32_t b) { body_scatter(S, b); });
+    for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *) { body_scatter(S, b); });
 }
 
 // ---------------------------------------------------------------- P2G / G2P
@@ -284,10 +284,10 @@ __device__ __forceinline__ void body_band(const DevSim &S, uint32_t b) {
 }
 __global__ __launch_bounds__(BLOCK) void k_band(DevSim S) { body_band(S, blockIdx.x); }
 __global__ __launch_bounds__(BLOCK) void kb_band(WorkList W) {
-    for_each_work(W, [&](const DevSim &S, uint32_t b) { body_band(S, b); });
+    for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *) { body_band(S, b); });
 }
 __global__ __launch_bounds__(BLOCK) void kb_tile_list(WorkList W) {
-    for_each_work(W, [&](const DevSim &S, uint32_t b) { body_tile_list(S, b); });
+    for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *) { body_tile_list(S, b); });
 }
 
 // Workgroups of the per-tile kernels: enough to keep every CU busy, each walking the active
@@ -367,7 +367,7 @@ __device__ __forceinline__ void body_p2g(const DevSim &S, uint32_t b0, uint32_t 
 }
 __global__ __launch_bounds__(TILE_BLOCK) void k_p2g_tile(DevSim S) { body_p2g(S, blockIdx.x, gridDim.x); }
 __global__ __launch_bounds__(TILE_BLOCK) void kb_p2g_tile(WorkList W) {
-    for_each_work(W, [&](const DevSim &S, uint32_t b) { body_p2g(S, b, tile_grid(S)); });
+    for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *) { body_p2g(S, b, tile_grid(S)); });
 }
 
 // Trilinear stencil of sample_face (fluid.slang:67-80) for component c at one position:
@@ -534,7 +534,7 @@ __device__ __forceinline__ void body_g2p(const DevSim &S, uint32_t b0, uint32_t 
 }
 __global__ __launch_bounds__(TILE_BLOCK) void k_g2p_tile(DevSim S) { body_g2p(S, blockIdx.x, gridDim.x); }
 __global__ __launch_bounds__(TILE_BLOCK) void kb_g2p_tile(WorkList W) {
-    for_each_work(W, [&](const DevSim &S, uint32_t b) { body_g2p(S, b, tile_grid(S)); });
+    for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *) { body_g2p(S, b, tile_grid(S)); });
 }
 
 // ---------------------------------------------------------------- IO / bins
@@ -665,21 +665,21 @@ uint32_t key_blocks(const DevSim &S) { return blocks_for(S.nkeys); }
 uint32_t tile_blocks(const DevSim &S) { return blocks_for(S.ntiles); }
 uint32_t tile_grid_blocks(const DevSim &S) { return tile_grid(S); }
 
-void batch_clear_grid(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_clear_grid, BATCH_GRID, BLOCK, 0, s, W[WD_KEYS]); }
-void batch_bin_count(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_bin_count, BATCH_GRID, BLOCK, 0, s, W[WD_PART]); }
+void batch_clear_grid(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_clear_grid, batch_grid(W[WD_KEYS], BATCH_GRID), BLOCK, 0, s, W[WD_KEYS]); }
+void batch_bin_count(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_bin_count, batch_grid(W[WD_PART], BATCH_GRID), BLOCK, 0, s, W[WD_PART]); }
 void batch_scan_reduce(const WorkLists &W, int which, hipStream_t s) {
-    PXL_LAUNCH(kb_scan_reduce, BATCH_GRID, BLOCK, 0, s, W[which == 0 ? WD_SCAN_KEYS : WD_SCAN_FLAGS], which);
+    PXL_LAUNCH(kb_scan_reduce, batch_grid(W[which == 0 ? WD_SCAN_KEYS : WD_SCAN_FLAGS], BATCH_GRID), BLOCK, 0, s, W[which == 0 ? WD_SCAN_KEYS : WD_SCAN_FLAGS], which);
 }
 void batch_scan_tiles(const WorkLists &W, int which, hipStream_t s) {
-    PXL_LAUNCH(kb_scan_tiles, BATCH_GRID, BLOCK, 0, s, W[WD_SIM], which);
+    PXL_LAUNCH(kb_scan_tiles, batch_grid(W[WD_SIM], BATCH_GRID), BLOCK, 0, s, W[WD_SIM], which);
 }
 void batch_scan_down(const WorkLists &W, int which, hipStream_t s) {
-    PXL_LAUNCH(kb_scan_down, BATCH_GRID, BLOCK, 0, s, W[which == 0 ? WD_SCAN_KEYS : WD_SCAN_FLAGS], which);
+    PXL_LAUNCH(kb_scan_down, batch_grid(W[which == 0 ? WD_SCAN_KEYS : WD_SCAN_FLAGS], BATCH_GRID), BLOCK, 0, s, W[which == 0 ? WD_SCAN_KEYS : WD_SCAN_FLAGS], which);
 }
-void batch_tile_list(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_tile_list, BATCH_GRID, BLOCK, 0, s, W[WD_TILES]); }
-void batch_band(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_band, BATCH_GRID, BLOCK, 0, s, W[WD_TILES]); }
-void batch_scatter(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_scatter, BATCH_GRID, BLOCK, 0, s, W[WD_PART]); }
-void batch_p2g(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_p2g_tile, BATCH_TILE_GRID, TILE_BLOCK, 0, s, W[WD_TILEGRID]); }
-void batch_g2p(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_g2p_tile, BATCH_TILE_GRID, TILE_BLOCK, 0, s, W[WD_TILEGRID]); }
+void batch_tile_list(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_tile_list, batch_grid(W[WD_TILES], BATCH_GRID), BLOCK, 0, s, W[WD_TILES]); }
+void batch_band(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_band, batch_grid(W[WD_TILES], BATCH_GRID), BLOCK, 0, s, W[WD_TILES]); }
+void batch_scatter(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_scatter, batch_grid(W[WD_PART], BATCH_GRID), BLOCK, 0, s, W[WD_PART]); }
+void batch_p2g(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_p2g_tile, batch_grid(W[WD_TILEGRID], BATCH_TILE_GRID), TILE_BLOCK, 0, s, W[WD_TILEGRID]); }
+void batch_g2p(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_g2p_tile, batch_grid(W[WD_TILEGRID], BATCH_TILE_GRID), TILE_BLOCK, 0, s, W[WD_TILEGRID]); }
 
 }  // namespace pxl

@@ -52,7 +52,7 @@ uint32_t work_blocks(const DevSim &S, int d) {
     }
     if (d >= WD_A_LVL && d < WD_A_LVL + MG_MAX) {
         const int l = d - WD_A_LVL;
-        return l < S.mg_tail ? (uint32_t)S.cg_blocks : 0;
+        return l < S.mg_tail ? level_blocks(S.cg_blocks, l) : 0;
     }
     return 0;
 }
@@ -142,7 +142,7 @@ public:
     ~Batch() {
         if (stream_) (void)hipStreamSynchronize(stream_);
         drop_graphs();
-        for (void *p : {(void *)d_sims_, (void *)d_entries_, (void *)d_counts_, (void *)d_norms_})
+        for (void *p : {(void *)d_sims_, (void *)d_entries_, (void *)d_counts_, (void *)d_norms_, (void *)d_slots_})
             if (p) (void)hipFree(p);
         for (void *p : {(void *)h_sims_, (void *)h_entries_, (void *)h_counts_, (void *)h_norms_})
             if (p) (void)hipHostFree(p);
@@ -157,6 +157,7 @@ public:
     size_t size() const { return members_.size(); }
     const std::vector<Sim *> &members() const { return members_; }
     uint64_t graph_captures() const { return captures_; }
+    double capture_seconds() const { return capture_s_; }
     uint64_t rebuilds() const { return rebuilds_; }
 
     void add(Sim *s) {
@@ -360,17 +361,25 @@ private:
         return b;
     }
     void read_counts() {
-        // device -> host copies of each member's list counts (profiling only; not in graphs)
-        counts_.assign(members_.size(), MemberCounts{});
-        for (size_t i = 0; i < members_.size(); i++) {
+        // device -> host copies of each member's list counts (profiling only; not in graphs):
+        // its count slots (fl_count = {red, all}, level l = {red, black}), tiles and band
+        const size_t n = members_.size();
+        counts_.assign(n, MemberCounts{});
+        std::vector<uint32_t> slots(n * COUNT_SLOTS);
+        if (n)
+            HIP_CHECK(hipMemcpyAsync(slots.data(), d_slots_, 4 * n * COUNT_SLOTS, hipMemcpyDeviceToHost, stream_));
+        for (size_t i = 0; i < n; i++) {
             const DevSim &d = members_[i]->d;
-            HIP_CHECK(hipMemcpyAsync(&counts_[i].fluid[0], d.fl_count + 1, 4, hipMemcpyDeviceToHost, stream_));
             HIP_CHECK(hipMemcpyAsync(&counts_[i].tiles, d.tile_count, 4, hipMemcpyDeviceToHost, stream_));
             HIP_CHECK(hipMemcpyAsync(&counts_[i].band, d.band_count, 4, hipMemcpyDeviceToHost, stream_));
-            for (int l = 1; l < d.mg_levels && l < MG_MAX; l++)
-                HIP_CHECK(hipMemcpyAsync(&counts_[i].fluid[l], d.mg[l].count, 4, hipMemcpyDeviceToHost, stream_));
         }
         HIP_CHECK(hipStreamSynchronize(stream_));
+        for (size_t i = 0; i < n; i++) {
+            const uint32_t *c = slots.data() + i * COUNT_SLOTS;
+            counts_[i].fluid[0] = c[1];
+            for (int l = 1; l < members_[i]->d.mg_levels && l < MG_MAX; l++)
+                counts_[i].fluid[l] = c[2 * l] + c[2 * l + 1];
+        }
         // this step's grid-phase launches (recorded before its counts were known) are priced now
         for (size_t k = grid_start_; k < recs_used_; k++) recs_[k].bytes = launch_bytes(recs_[k].name, recs_[k].level);
     }
@@ -405,6 +414,7 @@ private:
             return;
         }
         if (!*g) {
+            const double t0 = now_s();
             hipGraph_t graph = nullptr;
             HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
             enqueue();
@@ -413,6 +423,7 @@ private:
             (void)hipGraphDestroy(graph);
             HIP_CHECK(e);
             captures_++;
+            capture_s_ += now_s() - t0;
         }
         HIP_CHECK(hipGraphLaunch(*g, stream_));
     }
@@ -457,6 +468,7 @@ private:
             host_alloc(&h_sims_, sims_cap_);
             dev_alloc(&d_norms_, 2 * sims_cap_);
             host_alloc(&h_norms_, 2 * sims_cap_);
+            dev_alloc(&d_slots_, COUNT_SLOTS * sims_cap_);
             relayout = true;
         }
         size_t need[WD_COUNT];
@@ -470,13 +482,14 @@ private:
             for (int d = 0; d < WD_COUNT; d++) {
                 // every domain gets headroom at once, so growth rarely relays out (and
                 // re-captures) again
-                cap_[d] = std::max<size_t>(cap_[d], std::max<size_t>(2 * need[d], 16384));
+                cap_[d] = std::max<size_t>(cap_[d], std::max<size_t>(2 * need[d], WORK_LIST_MIN_CAP));
                 off_[d] = total;
                 total += cap_[d];
             }
             dev_alloc(&d_entries_, total);
             host_alloc(&h_entries_, total);
-            for (int d = 0; d < WD_COUNT; d++) lists_[d] = WorkList{d_sims_, d_entries_ + off_[d], d_counts_ + d};
+            for (int d = 0; d < WD_COUNT; d++)
+                lists_[d] = WorkList{d_sims_, d_entries_ + off_[d], d_counts_ + d, d_slots_, lists_[d].grid};
         }
         // the graphs launch the level kernels of the deepest hierarchy seen so far (levels a
         // member does not have get empty work lists), so members leaving do not force a
@@ -486,13 +499,32 @@ private:
             levels = std::max(levels, s->d.mg_levels);
             tail = std::max(tail, s->d.mg_tail);
         }
-        if (relayout || levels != max_levels_ || tail != max_tail_) drop_graphs();
+        // A launch's persistent grid is its list length rounded up to a power of two (at least
+        // 64) and is baked into the graphs; when a list outgrows its grid or shrinks below an
+        // eighth of it, re-size and re-capture (any grid processes every entry, so this only
+        // tunes speed; power-of-two steps keep re-captures rare as members come and go).
+        auto pow2 = [](size_t v) {
+            uint32_t g = 64;
+            while (g < v) g <<= 1;
+            return g;
+        };
+        bool regrid = false;
+        for (int d = 0; d < WD_COUNT; d++)
+            if (need[d] > lists_[d].grid || (lists_[d].grid > 64 && 8 * need[d] < lists_[d].grid)) regrid = true;
+        if (regrid)
+            for (int d = 0; d < WD_COUNT; d++) lists_[d].grid = pow2(need[d]);
+        if (relayout || regrid || levels != max_levels_ || tail != max_tail_) drop_graphs();
         rebuilds_++;
         max_levels_ = levels;
         max_tail_ = tail;
+        // each member's copy points its norms and list counts into the batch's arrays (the
+        // counts are rebuilt every step before they are read, so slots can move between steps)
         for (size_t i = 0; i < n; i++) {
             h_sims_[i] = members_[i]->d;
             h_sims_[i].norms = d_norms_ + 2 * i;
+            uint32_t *slot = d_slots_ + i * COUNT_SLOTS;
+            h_sims_[i].fl_count = slot;
+            for (int l = 1; l < MG_MAX; l++) h_sims_[i].mg[l].count = slot + 2 * l;
         }
         HIP_CHECK(hipMemcpyAsync(d_sims_, h_sims_, sizeof(DevSim) * n, hipMemcpyHostToDevice, stream_));
         std::vector<uint32_t> all(n);
@@ -516,11 +548,13 @@ private:
     uint32_t *d_entries_ = nullptr, *h_entries_ = nullptr;
     uint32_t *d_counts_ = nullptr, *h_counts_ = nullptr;
     double *d_norms_ = nullptr, *h_norms_ = nullptr;
+    uint32_t *d_slots_ = nullptr;  // list counts, COUNT_SLOTS per member
     size_t off_[WD_COUNT], cap_[WD_COUNT];
     WorkLists lists_{};
     int max_levels_ = 0, max_tail_ = 0;
     hipGraphExec_t g_grid_ = nullptr, g_inner_[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}}, g_post_ = nullptr;
     uint64_t captures_ = 0, rebuilds_ = 0;
+    double capture_s_ = 0.0;
     bool profiling_ = false;
     std::vector<Rec> recs_;
     size_t recs_used_ = 0, grid_start_ = 0;
@@ -1074,6 +1108,7 @@ int pxlflip_server_get_stats(pxlflip_server *srv, pxlflip_server_stats *st) {
     st->admit_s = srv->admit_s;
     st->rebuilds = srv->batch ? srv->batch->rebuilds() : 0;
     st->graph_captures = srv->batch ? srv->batch->graph_captures() : 0;
+    st->capture_s = srv->batch ? srv->batch->capture_seconds() : 0.0;
     return PXLFLIP_OK;
 }
 

@@ -114,10 +114,10 @@ __device__ __forceinline__ T wave_sum_partials(const T *part, uint32_t nb, int k
 
 // ---- init 1: r0 = b - A x (pressure system, x = warm-started pressure) or, for the inner
 // solve of the refinement, r0 = float(r64) with x = e = 0.
-__device__ __forceinline__ void body_cg_residual(const DevSim &S, float *__restrict__ x, float *__restrict__ r,
-                                                 int inner, uint32_t b, uint32_t nb) {
+__device__ __forceinline__ void body_cg_residual(const DevSim &S, const uint32_t *flc, float *__restrict__ x,
+                                                 float *__restrict__ r, int inner, uint32_t b, uint32_t nb) {
     uint32_t beg, end;
-    list_slice(0, S.fl_count[1], b, nb, beg, end);
+    list_slice(0, flc[1], b, nb, beg, end);
     int32_t off[6];
     nb_offsets(S, off);
     for (uint32_t t = beg + threadIdx.x; t < end; t += SBLOCK) {
@@ -139,7 +139,7 @@ __device__ __forceinline__ void body_cg_residual(const DevSim &S, float *__restr
 }
 __global__ __launch_bounds__(SBLOCK) void k_cg_residual(DevSim S, float *__restrict__ x, float *__restrict__ r,
                                                        int inner) {
-    body_cg_residual(S, x, r, inner, blockIdx.x, gridDim.x);
+    body_cg_residual(S, S.fl_count, x, r, inner, blockIdx.x, gridDim.x);
 }
 
 // ---- init 2: u0 = M^-1 r0, w0 = A u0; p_{-1} = s_{-1} = 0; partials gamma0, delta0 (slot 0).
@@ -259,9 +259,10 @@ __global__ __launch_bounds__(SBLOCK) void k_cg_iter(DevSim S, uint32_t it, float
 // spmv: w = A u and the partials of gamma = (r, u), delta = (w, u) into slot `slot`
 // (first: also p_{-1} = s_{-1} = 0).
 template <typename Acc>
-__device__ __forceinline__ void body_mgcg_spmv(const DevSim &S, uint32_t slot, int first, uint32_t b, uint32_t nb) {
+__device__ __forceinline__ void body_mgcg_spmv(const DevSim &S, const uint32_t *flc, uint32_t slot, int first,
+                                               uint32_t b, uint32_t nb) {
     uint32_t beg, end;
-    list_slice(0, S.fl_count[1], b, nb, beg, end);
+    list_slice(0, flc[1], b, nb, beg, end);
     int32_t off[6];
     nb_offsets(S, off);
     const float *__restrict__ u = S.cg_r1;
@@ -289,10 +290,10 @@ __device__ __forceinline__ void body_mgcg_spmv(const DevSim &S, uint32_t slot, i
 
 // update of iteration it: scalars from slot it & 1, then p, s, x, r (no neighbours).
 template <typename Acc>
-__device__ __forceinline__ void body_mgcg_update(const DevSim &S, uint32_t it, float *__restrict__ x, uint32_t b,
-                                                 uint32_t nb) {
+__device__ __forceinline__ void body_mgcg_update(const DevSim &S, const uint32_t *flc, uint32_t it,
+                                                 float *__restrict__ x, uint32_t b, uint32_t nb) {
     uint32_t beg, end;
-    list_slice(0, S.fl_count[1], b, nb, beg, end);
+    list_slice(0, flc[1], b, nb, beg, end);
     float alpha, beta;
     cg_scalars<Acc>(S, it, b, nb, alpha, beta);
     for (uint32_t t = beg + threadIdx.x; t < end; t += SBLOCK) {
@@ -307,11 +308,11 @@ __device__ __forceinline__ void body_mgcg_update(const DevSim &S, uint32_t it, f
 }
 template <typename Acc>
 __global__ __launch_bounds__(SBLOCK) void k_mgcg_spmv(DevSim S, uint32_t slot, int first) {
-    body_mgcg_spmv<Acc>(S, slot, first, blockIdx.x, gridDim.x);
+    body_mgcg_spmv<Acc>(S, S.fl_count, slot, first, blockIdx.x, gridDim.x);
 }
 template <typename Acc>
 __global__ __launch_bounds__(SBLOCK) void k_mgcg_update(DevSim S, uint32_t it, float *__restrict__ x) {
-    body_mgcg_update<Acc>(S, it, x, blockIdx.x, gridDim.x);
+    body_mgcg_update<Acc>(S, S.fl_count, it, x, blockIdx.x, gridDim.x);
 }
 
 // ---- FP64 outer loop.  The solution is double-buffered (x_src -> x_dst) so the update pass
@@ -321,9 +322,10 @@ __device__ __forceinline__ double b64_of(const DevSim &S, uint32_t id) {
 }
 
 // init: x64 = pressure, r64 = b - A x64; partials of rr and bb.
-__device__ __forceinline__ void body_mixed_init(const DevSim &S, double *__restrict__ x64, uint32_t b_, uint32_t nb) {
+__device__ __forceinline__ void body_mixed_init(const DevSim &S, const uint32_t *flc, double *__restrict__ x64,
+                                                uint32_t b_, uint32_t nb) {
     uint32_t beg, end;
-    list_slice(0, S.fl_count[1], b_, nb, beg, end);
+    list_slice(0, flc[1], b_, nb, beg, end);
     int32_t off[6];
     nb_offsets(S, off);
     double rr = 0, bb = 0;
@@ -346,16 +348,16 @@ __device__ __forceinline__ void body_mixed_init(const DevSim &S, double *__restr
     store_partials<double, 2>(v, S.part64, b_, nb);
 }
 __global__ __launch_bounds__(SBLOCK) void k_mixed_init(DevSim S, double *__restrict__ x64) {
-    body_mixed_init(S, x64, blockIdx.x, gridDim.x);
+    body_mixed_init(S, S.fl_count, x64, blockIdx.x, gridDim.x);
 }
 
 // update: x_dst = x_src + e (FP64) at the cell and its neighbours, r64 = b - A x_dst; rr.
 // The pressure field gets float(x_dst), so it holds the latest iterate after every pass.
-__device__ __forceinline__ void body_mixed_update(const DevSim &S, const double *__restrict__ xsrc,
-                                                  double *__restrict__ xdst, const float *__restrict__ e, uint32_t b,
-                                                  uint32_t nb) {
+__device__ __forceinline__ void body_mixed_update(const DevSim &S, const uint32_t *flc,
+                                                  const double *__restrict__ xsrc, double *__restrict__ xdst,
+                                                  const float *__restrict__ e, uint32_t b, uint32_t nb) {
     uint32_t beg, end;
-    list_slice(0, S.fl_count[1], b, nb, beg, end);
+    list_slice(0, flc[1], b, nb, beg, end);
     int32_t off[6];
     nb_offsets(S, off);
     double rr = 0;
@@ -382,7 +384,7 @@ __device__ __forceinline__ void body_mixed_update(const DevSim &S, const double 
 }
 __global__ __launch_bounds__(SBLOCK) void k_mixed_update(DevSim S, const double *__restrict__ xsrc,
                                                         double *__restrict__ xdst, const float *__restrict__ e) {
-    body_mixed_update(S, xsrc, xdst, e, blockIdx.x, gridDim.x);
+    body_mixed_update(S, S.fl_count, xsrc, xdst, e, blockIdx.x, gridDim.x);
 }
 
 // Totals of the refinement norms into norms[0] = rr (and norms[1] = bb when nv = 2).
@@ -399,26 +401,35 @@ __global__ __launch_bounds__(BLOCK) void k_norms_finalize(DevSim S, uint32_t nb,
     body_norms_finalize(S, nb, nv);
 }
 
-// ---- batched (the solving simulations of a batch; x = e, r = cg_r0)
+// ---- batched (the solving simulations of a batch; x = e, r = cg_r0; the fluid-list counts
+// come from the batch's count slots)
 __global__ __launch_bounds__(SBLOCK) void kb_cg_residual(WorkList W) {
-    for_each_work(W, [&](const DevSim &S, uint32_t b) { body_cg_residual(S, S.e, S.cg_r0, 1, b, S.cg_blocks); });
+    for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *cnt) {
+        body_cg_residual(S, cnt, S.e, S.cg_r0, 1, b, S.cg_blocks);
+    });
 }
 __global__ __launch_bounds__(SBLOCK) void kb_mgcg_spmv(WorkList W, uint32_t slot, int first) {
-    for_each_work(W, [&](const DevSim &S, uint32_t b) { body_mgcg_spmv<double>(S, slot, first, b, S.cg_blocks); });
+    for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *cnt) {
+        body_mgcg_spmv<double>(S, cnt, slot, first, b, S.cg_blocks);
+    });
 }
 __global__ __launch_bounds__(SBLOCK) void kb_mgcg_update(WorkList W, uint32_t it) {
-    for_each_work(W, [&](const DevSim &S, uint32_t b) { body_mgcg_update<double>(S, it, S.e, b, S.cg_blocks); });
+    for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *cnt) {
+        body_mgcg_update<double>(S, cnt, it, S.e, b, S.cg_blocks);
+    });
 }
 __global__ __launch_bounds__(SBLOCK) void kb_mixed_init(WorkList W) {
-    for_each_work(W, [&](const DevSim &S, uint32_t b) { body_mixed_init(S, S.x64a, b, S.cg_blocks); });
+    for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *cnt) {
+        body_mixed_init(S, cnt, S.x64a, b, S.cg_blocks);
+    });
 }
 __global__ __launch_bounds__(SBLOCK) void kb_mixed_update(WorkList W, int cur) {
-    for_each_work(W, [&](const DevSim &S, uint32_t b) {
-        body_mixed_update(S, cur ? S.x64b : S.x64a, cur ? S.x64a : S.x64b, S.e, b, S.cg_blocks);
+    for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *cnt) {
+        body_mixed_update(S, cnt, cur ? S.x64b : S.x64a, cur ? S.x64a : S.x64b, S.e, b, S.cg_blocks);
     });
 }
 __global__ __launch_bounds__(BLOCK) void kb_norms_finalize(WorkList W, int nv) {
-    for_each_work(W, [&](const DevSim &S, uint32_t) { body_norms_finalize(S, (uint32_t)S.cg_blocks, nv); });
+    for_each_work(W, [&](const DevSim &S, uint32_t, const uint32_t *) { body_norms_finalize(S, (uint32_t)S.cg_blocks, nv); });
 }
 
 // ---- pressure_rb (fluid.slang:168-182): one red-black half sweep over the cells of one
@@ -499,19 +510,19 @@ void launch_rbgs(const DevSim &S, uint32_t parity, hipStream_t s) {
 }
 
 // ---- batched (the solving simulations: x = e, r = cg_r0, u = cg_r1)
-void batch_mixed_init(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_mixed_init, BATCH_SGRID, SBLOCK, 0, s, W[WD_A_SOLVER]); }
+void batch_mixed_init(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_mixed_init, batch_grid(W[WD_A_SOLVER], BATCH_SGRID), SBLOCK, 0, s, W[WD_A_SOLVER]); }
 void batch_norms(const WorkLists &W, int nv, hipStream_t s) {
-    PXL_LAUNCH(kb_norms_finalize, BATCH_SGRID, BLOCK, 0, s, W[WD_A_SIM], nv);
+    PXL_LAUNCH(kb_norms_finalize, batch_grid(W[WD_A_SIM], BATCH_SGRID), BLOCK, 0, s, W[WD_A_SIM], nv);
 }
-void batch_cg_residual(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_cg_residual, BATCH_SGRID, SBLOCK, 0, s, W[WD_A_SOLVER]); }
+void batch_cg_residual(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_cg_residual, batch_grid(W[WD_A_SOLVER], BATCH_SGRID), SBLOCK, 0, s, W[WD_A_SOLVER]); }
 void batch_mgcg_spmv(const WorkLists &W, uint32_t slot, int first, hipStream_t s) {
-    PXL_LAUNCH(kb_mgcg_spmv, BATCH_SGRID, SBLOCK, 0, s, W[WD_A_SOLVER], slot, first);
+    PXL_LAUNCH(kb_mgcg_spmv, batch_grid(W[WD_A_SOLVER], BATCH_SGRID), SBLOCK, 0, s, W[WD_A_SOLVER], slot, first);
 }
 void batch_mgcg_update(const WorkLists &W, uint32_t it, hipStream_t s) {
-    PXL_LAUNCH(kb_mgcg_update, BATCH_SGRID, SBLOCK, 0, s, W[WD_A_SOLVER], it);
+    PXL_LAUNCH(kb_mgcg_update, batch_grid(W[WD_A_SOLVER], BATCH_SGRID), SBLOCK, 0, s, W[WD_A_SOLVER], it);
 }
 void batch_mixed_update(const WorkLists &W, int cur, hipStream_t s) {
-    PXL_LAUNCH(kb_mixed_update, BATCH_SGRID, SBLOCK, 0, s, W[WD_A_SOLVER], cur);
+    PXL_LAUNCH(kb_mixed_update, batch_grid(W[WD_A_SOLVER], BATCH_SGRID), SBLOCK, 0, s, W[WD_A_SOLVER], cur);
 }
 
 }  // namespace pxl
