@@ -112,8 +112,8 @@ __device__ __forceinline__ uint8_t type_from_count(const DevSim &S, int i, int j
 }
 
 // normalize_grid cells + compute_divergence (fluid.slang:125-131, 154-165), plus the stencil
-// word the pressure kernels read instead of re-classifying six neighbours.
-__device__ __forceinline__ void cells_at(const DevSim &S, int i, int j, int k) {
+// word the pressure kernels read instead of re-classifying six neighbours.  Returns the type.
+__device__ __forceinline__ uint8_t cells_at(const DevSim &S, int i, int j, int k) {
     const uint32_t id = cidx(S, i, j, k);
     const uint8_t t = type_from_count(S, i, j, k);
     S.type[id] = t;
@@ -137,53 +137,103 @@ __device__ __forceinline__ void cells_at(const DevSim &S, int i, int j, int k) {
     }
     S.sten[id] = st;
     S.div[id] = d * S.inv_h;
-    if (S.split_colours) {
-        const bool red = (((uint32_t)(i + j + k)) & 1u) == 0;
-        S.fl_flag[id] = (t == CELL_FLUID && red) ? 1u : 0u;
-        S.fl_flag[S.ncells + id] = (t == CELL_FLUID && !red) ? 1u : 0u;
-    } else {
-        S.fl_flag[id] = t == CELL_FLUID ? 1u : 0u;
+    return t;
+}
+
+// Adds a wave's FLUID cells to the fluid list's row counts.  Lanes hold consecutive cells of
+// a tile box (x fastest), so lanes of one grid row form a run; the run's first lane adds the
+// run's counts with one atomic per colour (integer sums: the result is independent of order).
+// `row` is ~0u on lanes without a cell.  Called by every lane of the wave.
+__device__ __forceinline__ void count_row_runs(const DevSim &S, uint32_t row, bool first_half, bool second_half) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const uint32_t prev = __shfl_up(row, 1, WAVE);
+    const bool start = lane == 0 || prev != row;
+    const uint64_t starts = __ballot(start), ma = __ballot(first_half), mb = __ballot(second_half);
+    if (!start || row == ~0u) return;
+    const uint64_t above = starts & ~((2ull << lane) - 1ull);  // run starts after this lane
+    const uint64_t upto = above ? (above & (~above + 1ull)) - 1ull : ~0ull;
+    const uint64_t run = upto & ~((1ull << lane) - 1ull);
+    const uint32_t na = (uint32_t)__popcll(ma & run), nb = (uint32_t)__popcll(mb & run);
+    if (na) atomicAdd(&S.fl_rows[row], na);
+    if (nb) atomicAdd(&S.fl_rows[(uint32_t)S.ny * (uint32_t)S.nz + row], nb);
+}
+
+// The cell pass over band tile t: every lane of the workgroup runs the same number of rounds
+// (the row counting needs whole waves).
+__device__ __forceinline__ void cells_tile(const DevSim &S, uint32_t t) {
+    const int ti = (int)(t % (uint32_t)S.ntx), tj = (int)((t / (uint32_t)S.ntx) % (uint32_t)S.nty),
+              tk = (int)(t / ((uint32_t)S.ntx * (uint32_t)S.nty));
+    const int lo[3] = {ti * TILE, tj * TILE, tk * TILE};
+    const int ex = min(TILE, S.nx - lo[0]), ey = min(TILE, S.ny - lo[1]), ez = min(TILE, S.nz - lo[2]);
+    const int total = ex * ey * ez;
+    for (int q0 = 0; q0 < total; q0 += GBLOCK) {
+        const int q = q0 + (int)threadIdx.x;
+        uint32_t row = ~0u;
+        bool fluid = false, red = false;
+        if (q < total) {
+            const int i = lo[0] + q % ex, j = lo[1] + (q / ex) % ey, k = lo[2] + q / (ex * ey);
+            fluid = cells_at(S, i, j, k) == CELL_FLUID;
+            red = (((uint32_t)(i + j + k)) & 1u) == 0;
+            row = (uint32_t)j + (uint32_t)S.ny * (uint32_t)k;
+        }
+        const bool split = S.split_colours != 0;
+        count_row_runs(S, row, fluid && (red || !split), fluid && !red && split);
     }
 }
 __device__ __forceinline__ void body_cells(const DevSim &S, uint32_t b, uint32_t nb) {
-    for_band(S, b, nb, [&](uint32_t t) { for_tile_box(S, t, false, [&](int i, int j, int k) { cells_at(S, i, j, k); }); });
+    for_band(S, b, nb, [&](uint32_t t) { cells_tile(S, t); });
 }
 __global__ __launch_bounds__(GBLOCK) void k_cells(DevSim S) { body_cells(S, blockIdx.x, gridDim.x); }
 __global__ __launch_bounds__(GBLOCK) void kb_cells(WorkList W) {
     for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *) { body_cells(S, b, band_grid(S)); });
 }
 
-// Fluid list from the scanned flags.  Split: entry e < C is cell e (red), e >= C is cell
-// e - C (black).  fl_count = {red cells (0 unless split), all fluid cells}.  Each entry also
-// gets its stencil word (see DevSim::fl_word).
-__device__ __forceinline__ void body_fluid_compact(const DevSim &S, uint32_t b) {
-    const uint32_t C = S.ncells, n = S.split_colours ? 2 * C : C, e = b * BLOCK + threadIdx.x;
-    if (e >= n) return;
-    const uint32_t f = S.fl_flag[e], o = S.fl_off[e];
-    if (f) {
-        const uint32_t id = e < C ? e : e - C;
-        const uint32_t st = S.sten[id];
-        const int32_t sy = S.nx, sz = S.nx * S.ny;
-        const int32_t off[6] = {-1, 1, -sy, sy, -sz, sz};
-        uint32_t word = (st & 63u) | (((st >> 8) & 7u) << 6);
-#pragma unroll
-        for (int n = 0; n < 6; n++)
-            if (st & (1u << n)) word |= (((uint32_t)S.sten[id + off[n]] >> 8) & 7u) << (9 + 3 * n);
-        const uint32_t ci = id % (uint32_t)S.nx, cj = (id / (uint32_t)S.nx) % (uint32_t)S.ny,
-                       ck = id / ((uint32_t)S.nx * (uint32_t)S.ny);
-        word |= ((ci + cj + ck) & 1u) << 27;
-        S.fl_list[o] = id;
-        S.fl_word[o] = word;
+// Fluid list from the scanned row counts: one wave per grid row writes the row's FLUID cells
+// in x order at the row's slots (red ones at fl_rowoff[row], black ones at fl_rowoff[R + row]
+// when colours are split), so the list is in cell order within each colour.  Each entry also
+// gets its stencil word (see DevSim::fl_word).  The wave of row 0 writes fl_count = {red
+// cells (0 unless split), all fluid cells}.
+constexpr int ROWS_PER_BLOCK = BLOCK / WAVE;
+__device__ __forceinline__ void body_fluid_rows(const DevSim &S, uint32_t b) {
+    const uint32_t R = (uint32_t)S.ny * (uint32_t)S.nz, row = b * ROWS_PER_BLOCK + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & (WAVE - 1);
+    if (row >= R) return;
+    const bool split = S.split_colours != 0;
+    if (row == 0 && lane == 0) {
+        S.fl_count[0] = split ? S.fl_rowoff[R] : 0u;
+        S.fl_count[1] = S.fl_rowoff[2 * R - 1] + S.fl_rows[2 * R - 1];
     }
-    if (S.split_colours && e == C) S.fl_count[0] = o;
-    if (e == n - 1) {
-        S.fl_count[1] = o + f;
-        if (!S.split_colours) S.fl_count[0] = 0;
+    if (S.fl_rows[row] + S.fl_rows[R + row] == 0) return;
+    uint32_t oa = S.fl_rowoff[row], ob = S.fl_rowoff[R + row];
+    const uint32_t j = row % (uint32_t)S.ny, k = row / (uint32_t)S.ny, base = (uint32_t)S.nx * row;
+    const int32_t sy = S.nx, sz = S.nx * S.ny;
+    const int32_t off[6] = {-1, 1, -sy, sy, -sz, sz};
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (int i0 = 0; i0 < S.nx; i0 += WAVE) {
+        const int i = i0 + lane;
+        const uint32_t id = base + (uint32_t)i;
+        const uint32_t st = i < S.nx ? S.sten[id] : 0u;
+        const bool fl = (st & STEN_FLUID) != 0;
+        const uint32_t black = ((uint32_t)i + j + k) & 1u;
+        const bool a = fl && (!black || !split), bk = fl && black && split;
+        const uint64_t ma = __ballot(a), mb = __ballot(bk);
+        if (a || bk) {
+            const uint32_t o = a ? oa + (uint32_t)__popcll(ma & below) : ob + (uint32_t)__popcll(mb & below);
+            uint32_t word = (st & 63u) | (((st >> 8) & 7u) << 6);
+#pragma unroll
+            for (int n = 0; n < 6; n++)
+                if (st & (1u << n)) word |= (((uint32_t)S.sten[id + off[n]] >> 8) & 7u) << (9 + 3 * n);
+            word |= black << 27;
+            S.fl_list[o] = id;
+            S.fl_word[o] = word;
+        }
+        oa += (uint32_t)__popcll(ma);
+        ob += (uint32_t)__popcll(mb);
     }
 }
-__global__ __launch_bounds__(BLOCK) void k_fluid_compact(DevSim S) { body_fluid_compact(S, blockIdx.x); }
+__global__ __launch_bounds__(BLOCK) void k_fluid_rows(DevSim S) { body_fluid_rows(S, blockIdx.x); }
 __global__ __launch_bounds__(BLOCK) void kb_fluid_compact(WorkList W) {
-    for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *) { body_fluid_compact(S, b); });
+    for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *) { body_fluid_rows(S, b); });
 }
 
 __device__ __forceinline__ uint8_t type_at(const DevSim &S, int i, int j, int k) {
@@ -236,9 +286,8 @@ dim3 node_grid(const DevSim &S) { return dim3((S.nx + 1 + TX - 1) / TX, (S.ny + 
 void launch_faces(const DevSim &S, hipStream_t s) { PXL_LAUNCH(k_faces, band_grid(S), GBLOCK, 0, s, S); }
 void launch_cells(const DevSim &S, hipStream_t s) { PXL_LAUNCH(k_cells, band_grid(S), GBLOCK, 0, s, S); }
 void launch_fluid_list(const DevSim &S, hipStream_t s) {
-    const uint32_t n = S.split_colours ? 2 * S.ncells : S.ncells;
-    launch_exclusive_scan(S.fl_flag, n, S.fl_tmp, S.fl_off, s);
-    PXL_LAUNCH(k_fluid_compact, (n + BLOCK - 1) / BLOCK, BLOCK, 0, s, S);
+    launch_exclusive_scan(S.fl_rows, fl_row_slots(S), S.fl_tmp, S.fl_rowoff, s);
+    PXL_LAUNCH(k_fluid_rows, fluid_row_blocks(S), BLOCK, 0, s, S);
 }
 void launch_project(const DevSim &S, hipStream_t s) { PXL_LAUNCH(k_project, band_grid(S), GBLOCK, 0, s, S); }
 void launch_acc_fields(const DevSim &S, int32_t *acc, int32_t *wacc, hipStream_t s) {
@@ -249,12 +298,14 @@ void launch_type_u32(const DevSim &S, uint32_t *out, hipStream_t s) {
 }
 
 // ---- batched
-uint32_t flag_count(const DevSim &S) { return S.split_colours ? 2 * S.ncells : S.ncells; }
+uint32_t fluid_row_blocks(const DevSim &S) {
+    return ((uint32_t)S.ny * (uint32_t)S.nz + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK;
+}
 
 void batch_faces(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_faces, batch_grid(W[WD_TILEGRID], BATCH_GRID), GBLOCK, 0, s, W[WD_TILEGRID]); }
 void batch_cells(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_cells, batch_grid(W[WD_TILEGRID], BATCH_GRID), GBLOCK, 0, s, W[WD_TILEGRID]); }
 void batch_fluid_compact(const WorkLists &W, hipStream_t s) {
-    PXL_LAUNCH(kb_fluid_compact, batch_grid(W[WD_COMPACT], BATCH_GRID), BLOCK, 0, s, W[WD_COMPACT]);
+    PXL_LAUNCH(kb_fluid_compact, batch_grid(W[WD_ROWS], BATCH_GRID), BLOCK, 0, s, W[WD_ROWS]);
 }
 void batch_project(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_project, batch_grid(W[WD_TILEGRID], BATCH_GRID), GBLOCK, 0, s, W[WD_TILEGRID]); }
 

@@ -92,10 +92,13 @@ struct DevSim {
     uint8_t *type;
     uint16_t *sten;
     float *pressure, *div;
-    // FLUID-cell list of the step: flags (C, or 2C colour-split: red then black), their
-    // exclusive scan, the list (cell order, or red cells first then black) and
+    // FLUID-cell list of the step, built from per-row counts: fl_rows holds the FLUID cells
+    // of each grid row (j, k) (2 ny nz words: rows' red cells, then rows' black cells; with
+    // colours not split, all of a row's cells in the first half and zeros in the second),
+    // fl_rowoff their exclusive scan (each row's first list slot), fl_list the list (cell
+    // order, or red cells first then black, each in cell order) and fl_count
     // {n_red (0 unless split), n_total}
-    uint32_t *fl_flag, *fl_off, *fl_list, *fl_count, *fl_tmp;
+    uint32_t *fl_rows, *fl_rowoff, *fl_list, *fl_count, *fl_tmp;
     // per list entry: bits 0-5 FLUID neighbours (-x,+x,-y,+y,-z,+z), bits 6-8 the cell's
     // non-SOLID neighbour count (matrix diagonal), bits 9+3n the diagonal of FLUID neighbour n,
     // bit 27 the cell's colour (1 = black: i + j + k odd)
@@ -197,6 +200,8 @@ void launch_clear_grid(const DevSim &S, hipStream_t s);       // count = 0
 void launch_bin_count(const DevSim &S, hipStream_t s);        // key, slot, count
 void launch_scan_offsets(const DevSim &S, hipStream_t s);     // offset = exclusive scan(count)
 uint32_t scan_tiles(uint32_t n);                              // scratch entries a scan of n needs
+// Words of the fluid list's per-row counts (two per grid row (j, k)).
+__host__ __device__ inline uint32_t fl_row_slots(const DevSim &S) { return 2u * (uint32_t)S.ny * (uint32_t)S.nz; }
 void launch_exclusive_scan(const uint32_t *in, uint32_t n, uint32_t *tile_tmp, uint32_t *out, hipStream_t s);
 void launch_tile_list(const DevSim &S, hipStream_t s);        // active tiles (after the scan)
 void launch_band(const DevSim &S, hipStream_t s);             // band tiles (after the tile list)
@@ -204,7 +209,7 @@ void launch_scatter(const DevSim &S, hipStream_t s);          // a -> b in cell 
 void launch_p2g(const DevSim &S, hipStream_t s);              // b -> per-tile boxes
 void launch_faces(const DevSim &S, hipStream_t s);            // normalize + forces
 void launch_cells(const DevSim &S, hipStream_t s);            // type, stencil, p=0, div, flags
-void launch_fluid_list(const DevSim &S, hipStream_t s);       // flags -> fl_list, fl_count
+void launch_fluid_list(const DevSim &S, hipStream_t s);       // row counts -> fl_list, fl_count
 void launch_rbgs(const DevSim &S, uint32_t parity, hipStream_t s);  // over the fluid list
 void launch_project(const DevSim &S, hipStream_t s);
 void launch_g2p(const DevSim &S, hipStream_t s);              // b -> a, advected
@@ -302,8 +307,8 @@ enum WorkDomain : int {
     WD_SCAN_KEYS,   // scan tiles of the count array
     WD_TILES,       // tiles (active-tile list)
     WD_TILEGRID,    // per-tile workgroups (P2G, G2P, and the band passes: faces, cells, project)
-    WD_SCAN_FLAGS,  // scan tiles of the fluid flags
-    WD_COMPACT,     // fluid flags (compaction)
+    WD_SCAN_ROWS,   // scan tiles of the fluid list's row counts
+    WD_ROWS,        // grid rows, four per workgroup (fluid-list compaction)
     WD_MGTYPES,     // + l, l = 1..MG_MAX-1: cells of level l
     WD_A_SIM = WD_MGTYPES + MG_MAX,
     WD_A_SOLVER,     // fluid-list solver workgroups
@@ -320,8 +325,8 @@ uint32_t particle_blocks(const DevSim &S);
 uint32_t key_blocks(const DevSim &S);
 uint32_t tile_blocks(const DevSim &S);
 uint32_t tile_grid_blocks(const DevSim &S);
-uint32_t flag_count(const DevSim &S);
 uint32_t mg_cell_blocks(const DevSim &S, int l);
+uint32_t fluid_row_blocks(const DevSim &S);  // workgroups of the fluid-list compaction (a wave per row)
 // Workgroups (list slices) of the multigrid list launches on level l: the solver's count on
 // level 0, an eighth of it per coarsening after that (a coarse level holds about an eighth of
 // the FLUID cells), at least one.
@@ -337,7 +342,7 @@ using WorkLists = WorkList[WD_COUNT];
 void batch_clear_grid(const WorkLists &W, hipStream_t s);
 void batch_bin_count(const WorkLists &W, hipStream_t s);
 void batch_scan_reduce(const WorkLists &W, int which, hipStream_t s);  // which 0: count -> offset
-void batch_scan_tiles(const WorkLists &W, int which, hipStream_t s);   // which 1: fluid flags
+void batch_scan_tiles(const WorkLists &W, int which, hipStream_t s);   // which 1: fluid row counts
 void batch_scan_down(const WorkLists &W, int which, hipStream_t s);
 void batch_tile_list(const WorkLists &W, hipStream_t s);
 void batch_band(const WorkLists &W, hipStream_t s);

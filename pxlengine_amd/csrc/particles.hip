@@ -39,10 +39,12 @@ __global__ __launch_bounds__(BLOCK) void k_fill_u32(uint32_t *p, uint32_t n, uin
 }
 
 // clear_grid (fluid.slang:88-91).  The P2G sums live in per-tile boxes that P2G overwrites
-// whole, so only the cell counts need clearing.
+// whole, so only the cell counts need clearing (and the fluid list's row counts, which the
+// cell pass accumulates).
 __device__ __forceinline__ void body_clear_grid(const DevSim &S, uint32_t b) {
     uint32_t i = b * BLOCK + threadIdx.x;
     if (i < S.nkeys) S.count[i] = 0;
+    for (uint32_t r = i; r < fl_row_slots(S); r += S.nkeys) S.fl_rows[r] = 0;
     if (i == 0) {
         *S.tile_count = 0;
         *S.band_count = 0;
@@ -181,7 +183,8 @@ __global__ __launch_bounds__(BLOCK) void k_scan_down(const uint32_t *in, uint32_
     body_scan_down(in, n, tile_off, out, blockIdx.x);
 }
 
-// The two scans of a step: count -> offset (which = 0) and fluid flags -> their offsets (1).
+// The two scans of a step: count -> offset (which = 0) and the fluid list's row counts ->
+// the rows' first list slots (1).
 struct ScanIO {
     const uint32_t *in;
     uint32_t n;
@@ -189,7 +192,7 @@ struct ScanIO {
 };
 __device__ __forceinline__ ScanIO scan_io(const DevSim &S, int which) {
     if (which == 0) return ScanIO{S.count, S.nkeys, S.scan_tmp, S.offset};
-    return ScanIO{S.fl_flag, S.split_colours ? 2 * S.ncells : S.ncells, S.fl_tmp, S.fl_off};
+    return ScanIO{S.fl_rows, fl_row_slots(S), S.fl_tmp, S.fl_rowoff};
 }
 __global__ __launch_bounds__(BLOCK) void kb_scan_reduce(WorkList W, int which) {
     for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *) {
@@ -668,13 +671,13 @@ uint32_t tile_grid_blocks(const DevSim &S) { return tile_grid(S); }
 void batch_clear_grid(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_clear_grid, batch_grid(W[WD_KEYS], BATCH_GRID), BLOCK, 0, s, W[WD_KEYS]); }
 void batch_bin_count(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_bin_count, batch_grid(W[WD_PART], BATCH_GRID), BLOCK, 0, s, W[WD_PART]); }
 void batch_scan_reduce(const WorkLists &W, int which, hipStream_t s) {
-    PXL_LAUNCH(kb_scan_reduce, batch_grid(W[which == 0 ? WD_SCAN_KEYS : WD_SCAN_FLAGS], BATCH_GRID), BLOCK, 0, s, W[which == 0 ? WD_SCAN_KEYS : WD_SCAN_FLAGS], which);
+    PXL_LAUNCH(kb_scan_reduce, batch_grid(W[which == 0 ? WD_SCAN_KEYS : WD_SCAN_ROWS], BATCH_GRID), BLOCK, 0, s, W[which == 0 ? WD_SCAN_KEYS : WD_SCAN_ROWS], which);
 }
 void batch_scan_tiles(const WorkLists &W, int which, hipStream_t s) {
     PXL_LAUNCH(kb_scan_tiles, batch_grid(W[WD_SIM], BATCH_GRID), BLOCK, 0, s, W[WD_SIM], which);
 }
 void batch_scan_down(const WorkLists &W, int which, hipStream_t s) {
-    PXL_LAUNCH(kb_scan_down, batch_grid(W[which == 0 ? WD_SCAN_KEYS : WD_SCAN_FLAGS], BATCH_GRID), BLOCK, 0, s, W[which == 0 ? WD_SCAN_KEYS : WD_SCAN_FLAGS], which);
+    PXL_LAUNCH(kb_scan_down, batch_grid(W[which == 0 ? WD_SCAN_KEYS : WD_SCAN_ROWS], BATCH_GRID), BLOCK, 0, s, W[which == 0 ? WD_SCAN_KEYS : WD_SCAN_ROWS], which);
 }
 void batch_tile_list(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_tile_list, batch_grid(W[WD_TILES], BATCH_GRID), BLOCK, 0, s, W[WD_TILES]); }
 void batch_band(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_band, batch_grid(W[WD_TILES], BATCH_GRID), BLOCK, 0, s, W[WD_TILES]); }

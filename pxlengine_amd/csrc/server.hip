@@ -41,8 +41,8 @@ uint32_t work_blocks(const DevSim &S, int d) {
         case WD_SCAN_KEYS: return scan_tiles(S.nkeys);
         case WD_TILES: return tile_blocks(S);
         case WD_TILEGRID: return tile_grid_blocks(S);
-        case WD_SCAN_FLAGS: return scan_tiles(flag_count(S));
-        case WD_COMPACT: return (flag_count(S) + BLOCK - 1) / BLOCK;
+        case WD_SCAN_ROWS: return scan_tiles(fl_row_slots(S));
+        case WD_ROWS: return fluid_row_blocks(S);
         case WD_A_SOLVER: return (uint32_t)S.cg_blocks;
         default: break;
     }
@@ -97,9 +97,9 @@ double share_bytes(const char *k, const DevSim &d, const MemberCounts &m, int l)
     const std::string n(k);
     if (n == "clear_grid") return 4 * K;
     if (n == "bin_count") return 20 * P + 4 * C;
-    const double flags = d.split_colours ? 2 * C : C;
-    if (n == "scan_reduce") return 4 * (l == 0 ? K : flags);
-    if (n == "scan_down") return 8 * (l == 0 ? K : flags);
+    const double rows = fl_row_slots(d);
+    if (n == "scan_reduce") return 4 * (l == 0 ? K : rows);
+    if (n == "scan_down") return 8 * (l == 0 ? K : rows);
     if (n == "tile_list") return 8.0 * d.ntiles;
     if (n == "scatter") return 68 * P;
     if (n == "p2g") return 24 * P + box * t;
@@ -108,7 +108,7 @@ double share_bytes(const char *k, const DevSim &d, const MemberCounts &m, int l)
     if (n == "band") return 28.0 * d.ntiles;
     if (n == "faces") return 8 * F * bf + box * t;
     if (n == "cells") return bf * (4 * F + 15 * C + 4 * C) - 4 * Cf;
-    if (n == "fluid_compact") return 8 * flags + 2 * C + 8 * Cf;
+    if (n == "fluid_compact") return 8 * rows + 2 * C * bf + 8 * Cf;
     if (n == "mg_types") return cells_of(l - 1) + cells_of(l);
     if (n == "mg_sten") return 7 * cells_of(l) + 4 * cl;
     if (n == "mixed_init") return 32 * Cf;

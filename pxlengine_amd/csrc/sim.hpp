@@ -156,12 +156,12 @@ struct Sim {
         add(&d.sten, 2 * C);
         add(&d.pressure, 4 * C);
         add(&d.div, 4 * C);
-        add(&d.fl_flag, 4 * 2 * C);
-        add(&d.fl_off, 4 * 2 * C);
+        add(&d.fl_rows, 4ull * fl_row_slots(d));
+        add(&d.fl_rowoff, 4ull * fl_row_slots(d));
         add(&d.fl_list, 4 * C);
         add(&d.fl_word, 4 * C);
         add(&d.fl_count, 64);
-        add(&d.fl_tmp, 4 * (scan_tiles(2 * (uint32_t)C) + 1));
+        add(&d.fl_tmp, 4ull * (scan_tiles(fl_row_slots(d)) + 1));
         for (float **p : {&d.cg_r0, &d.cg_r1, &d.cg_w0, &d.cg_w1, &d.cg_s0, &d.cg_s1, &d.cg_p}) add(p, 4 * C);
         if (opts.solver == PXLFLIP_SOLVER_MIXED) {
             for (int l = 1; l < d.mg_levels; l++) {
