@@ -301,7 +301,7 @@ def main():
     ap.add_argument("--rate", type=float, default=0.0, help="arrivals per second (0 = measured service rate)")
     ap.add_argument("--budget", type=float, default=200e9)
     ap.add_argument("--plain-every", type=int, default=8, help="plain path on every k-th request")
-    ap.add_argument("--profile-batch", type=int, default=32)
+    ap.add_argument("--profile-batch", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU oracle sample length (0 = skip)")
     args = ap.parse_args()
 

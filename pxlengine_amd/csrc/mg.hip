@@ -356,103 +356,158 @@ __device__ __forceinline__ void body_restrict(int fnx, int fny, int fnz, const u
 }
 
 // ---- the tail: levels mg_tail..L in one workgroup.  LDS holds every tail level's z, r and
-// stencil words back to back; the cycle is the one above with barriers in place of launches.
+// stencil words back to back, and per level the LDS indices of its FLUID cells (red ones from
+// the level's start, black ones from its end, listed once at the fill in any order: every
+// phase below updates cells independently of each other); the cycle is the one above with
+// barriers in place of launches.  A lane issues all LDS loads of a cell (word, r, the six
+// neighbours at indices clamped into the level) before using any; neighbours that are not
+// FLUID are dropped afterwards.
 struct TailLevel {
     int nx, ny, nz, o;  // dims and LDS offset
+    int cnt[2];         // FLUID cells of each colour
 };
-constexpr uint32_t LDS_BLACK = 1u << 12;  // LDS stencil words carry the cell's colour
+__device__ __forceinline__ int t_cells(const TailLevel &v) { return v.nx * v.ny * v.nz; }
+// LDS index of the level's t-th FLUID cell (red ones first, then black ones)
+__device__ __forceinline__ int t_entry(const TailLevel &v, const uint16_t *lcl, int t) {
+    return t < v.cnt[0] ? lcl[v.o + t] : lcl[v.o + t_cells(v) - 1 - (t - v.cnt[0])];
+}
 
 __device__ __forceinline__ void t_sweep(const TailLevel &v, float *lz, const float *lr, const uint16_t *ls,
-                                        int colour) {
-    const int n = v.nx * v.ny * v.nz;
+                                        const uint16_t *lcl, int colour) {
+    const int n = v.cnt[colour], last = v.o + t_cells(v) - 1;
     const int32_t off[6] = {-1, 1, -v.nx, v.nx, -v.nx * v.ny, v.nx * v.ny};
-    const uint32_t want = STEN_FLUID | (colour ? LDS_BLACK : 0u);
-    for (int c = threadIdx.x; c < n; c += TBLOCK) {
-        const uint32_t st = ls[v.o + c];
-        if ((st & (STEN_FLUID | LDS_BLACK)) != want) continue;
+    for (int t = threadIdx.x; t < n; t += TBLOCK) {
+        const int c = colour ? lcl[last - t] : lcl[v.o + t];
+        const uint32_t st = ls[c];
+        const float rc = lr[c];
+        float zq[6];
+#pragma unroll
+        for (int q = 0; q < 6; q++) zq[q] = lz[min(max(c + off[q], v.o), last)];
         float sum = 0.0f;
 #pragma unroll
-        for (int q = 0; q < 6; q++)
-            if (st & (1u << q)) sum += lz[v.o + c + off[q]];
-        lz[v.o + c] = (lr[v.o + c] + sum) * rcp_or0(st_diag(st));
+        for (int q = 0; q < 6; q++) sum = (st & (1u << q)) ? sum + zq[q] : sum;
+        lz[c] = (rc + sum) * rcp_or0(st_diag(st));
     }
     __syncthreads();
 }
 
-// restrict f -> c after f's black sweep; also zeroes c's z
+// restrict f -> c after f's black sweep, for c's FLUID cells (c's z is still the fill's zero)
 __device__ __forceinline__ void t_restrict(const TailLevel &f, const TailLevel &c, float *lz, float *lr,
-                                           const uint16_t *ls) {
-    const int n = c.nx * c.ny * c.nz;
+                                           const uint16_t *ls, const uint16_t *lcl) {
+    const int n = c.cnt[0] + c.cnt[1], last = f.o + t_cells(f) - 1;
     const int32_t off[6] = {-1, 1, -f.nx, f.nx, -f.nx * f.ny, f.nx * f.ny};
     const int red[4] = {0, 3, 5, 6};
-    for (int cc = threadIdx.x; cc < n; cc += TBLOCK) {
+    for (int t = threadIdx.x; t < n; t += TBLOCK) {
+        const int cc = t_entry(c, lcl, t);
         int ci, cj, ck;
-        cell_ijk(c.nx, c.ny, (uint32_t)cc, ci, cj, ck);
-        float acc = 0.0f;
+        cell_ijk(c.nx, c.ny, (uint32_t)(cc - c.o), ci, cj, ck);
+        bool in[4];
+        uint32_t st[4];
+        float rc[4], zc[4], zn[4][6];
 #pragma unroll
         for (int m = 0; m < 4; m++) {
             const int q = red[m];
             const int fi = 2 * ci + (q & 1), fj = 2 * cj + ((q >> 1) & 1), fk = 2 * ck + (q >> 2);
-            if (fi >= f.nx || fj >= f.ny || fk >= f.nz) continue;
-            const int id = f.o + fi + f.nx * (fj + f.ny * fk);
-            const uint32_t st = ls[id];
-            if (!(st & STEN_FLUID)) continue;
+            in[m] = fi < f.nx && fj < f.ny && fk < f.nz;
+            const int id = in[m] ? f.o + fi + f.nx * (fj + f.ny * fk) : f.o;
+            st[m] = ls[id];
+            rc[m] = lr[id];
+            zc[m] = lz[id];
+#pragma unroll
+            for (int e = 0; e < 6; e++) zn[m][e] = lz[min(max(id + off[e], f.o), last)];
+        }
+        float acc = 0.0f;
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            if (!in[m] || !(st[m] & STEN_FLUID)) continue;
             float sum = 0.0f;
 #pragma unroll
-            for (int e = 0; e < 6; e++)
-                if (st & (1u << e)) sum += lz[id + off[e]];
-            acc += lr[id] - st_diag(st) * lz[id] + sum;
+            for (int e = 0; e < 6; e++) sum = (st[m] & (1u << e)) ? sum + zn[m][e] : sum;
+            acc += rc[m] - st_diag(st[m]) * zc[m] + sum;
         }
-        lr[c.o + cc] = 0.5f * acc;
-        lz[c.o + cc] = 0.0f;
+        lr[cc] = 0.5f * acc;
     }
     __syncthreads();
 }
 
 // black update of f from red neighbours' z + prolonged correction from c
 __device__ __forceinline__ void t_post(const TailLevel &f, const TailLevel &c, float *lz, const float *lr,
-                                       const uint16_t *ls) {
-    const int n = f.nx * f.ny * f.nz;
-    for (int id = threadIdx.x; id < n; id += TBLOCK) {
-        const uint32_t st = ls[f.o + id];
-        if ((st & (STEN_FLUID | LDS_BLACK)) != (STEN_FLUID | LDS_BLACK)) continue;
+                                       const uint16_t *ls, const uint16_t *lcl) {
+    const int n = f.cnt[1], last = f.o + t_cells(f) - 1;
+    for (int t = threadIdx.x; t < n; t += TBLOCK) {
+        const int id = lcl[last - t];
         int i, j, k;
-        cell_ijk(f.nx, f.ny, (uint32_t)id, i, j, k);
+        cell_ijk(f.nx, f.ny, (uint32_t)(id - f.o), i, j, k);
         const int nb[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k}, {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
-        float sum = 0.0f;
+        const uint32_t st = ls[id];
+        const float rc = lr[id];
+        float zq[6], pq[6];
 #pragma unroll
         for (int q = 0; q < 6; q++) {
-            if (st & (1u << q)) {
-                const int a = nb[q][0], b = nb[q][1], d = nb[q][2];
-                sum += lz[f.o + a + f.nx * (b + f.ny * d)] + lz[c.o + (int)parent(c.nx, c.ny, a, b, d)];
-            }
+            const int a = min(max(nb[q][0], 0), f.nx - 1), b = min(max(nb[q][1], 0), f.ny - 1),
+                      d = min(max(nb[q][2], 0), f.nz - 1);
+            zq[q] = lz[f.o + a + f.nx * (b + f.ny * d)];
+            pq[q] = lz[c.o + (int)parent(c.nx, c.ny, a, b, d)];
         }
-        lz[f.o + id] = (lr[f.o + id] + sum) * rcp_or0(st_diag(st));
+        float sum = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 6; q++) sum = (st & (1u << q)) ? sum + (zq[q] + pq[q]) : sum;
+        lz[id] = (rc + sum) * rcp_or0(st_diag(st));
     }
     __syncthreads();
 }
 
 __device__ __forceinline__ void body_mg_tail(const DevSim &S) {
     __shared__ float lz[MG_TAIL_CELLS], lr[MG_TAIL_CELLS];
-    __shared__ uint16_t ls[MG_TAIL_CELLS];
+    __shared__ uint16_t ls[MG_TAIL_CELLS], lcl[MG_TAIL_CELLS];
     __shared__ TailLevel tl[MG_MAX];
     const int T = S.mg_tail, L = S.mg_levels - 1;
-    int o = 0;
-    // level table and LDS fill (constant level indices: no scratch copy of S)
+    // level table (constant level indices: no scratch copy of S)
+    if (threadIdx.x == 0) {
+        int o = 0;
+#pragma unroll
+        for (int q = 1; q < MG_MAX; q++) {
+            if (q >= T && q <= L) {
+                const MGLevel &m = S.mg[q];
+                tl[q] = TailLevel{m.nx, m.ny, m.nz, o, {0, 0}};
+                o += m.nx * m.ny * m.nz;
+            }
+        }
+    }
+    __syncthreads();
+    // LDS fill and the FLUID lists (one LDS atomic per wave and colour)
+    const int lane = threadIdx.x & (WAVE - 1);
+    const uint64_t below = (1ull << lane) - 1ull;
 #pragma unroll
     for (int q = 1; q < MG_MAX; q++) {
         if (q >= T && q <= L) {
             const MGLevel &m = S.mg[q];
-            const int n = m.nx * m.ny * m.nz;
-            if (threadIdx.x == 0) tl[q] = TailLevel{m.nx, m.ny, m.nz, o};
-            for (int c = threadIdx.x; c < n; c += TBLOCK) {
-                int i, j, k;
-                cell_ijk(m.nx, m.ny, (uint32_t)c, i, j, k);
-                ls[o + c] = (uint16_t)(m.sten[c] | (((i + j + k) & 1) ? LDS_BLACK : 0u));
-                lz[o + c] = 0.0f;
-                if (q == T) lr[o + c] = m.r[c];
+            const int n = m.nx * m.ny * m.nz, o = tl[q].o;
+            for (int c0 = 0; c0 < n; c0 += TBLOCK) {
+                const int c = c0 + (int)threadIdx.x;
+                uint32_t st = 0, black = 0;
+                if (c < n) {
+                    st = m.sten[c];
+                    int i, j, k;
+                    cell_ijk(m.nx, m.ny, (uint32_t)c, i, j, k);
+                    black = (uint32_t)(i + j + k) & 1u;
+                    ls[o + c] = (uint16_t)st;
+                    lz[o + c] = 0.0f;
+                    if (q == T) lr[o + c] = m.r[c];
+                }
+                const bool fl = (st & STEN_FLUID) != 0;
+#pragma unroll
+                for (int col = 0; col < 2; col++) {
+                    const bool mine = fl && black == (uint32_t)col;
+                    const uint64_t mask = __ballot(mine);
+                    const int leader = mask ? __ffsll((long long)mask) - 1 : 0;
+                    int base = 0;
+                    if (mine && lane == leader) base = atomicAdd(&tl[q].cnt[col], __popcll(mask));
+                    base = __shfl(base, leader, WAVE);
+                    const int slot = base + __popcll(mask & below);
+                    if (mine) lcl[col ? o + n - 1 - slot : o + slot] = (uint16_t)(o + c);
+                }
             }
-            o += n;
         }
     }
     __syncthreads();
@@ -460,30 +515,30 @@ __device__ __forceinline__ void body_mg_tail(const DevSim &S) {
     for (int q = T; q < L; q++) {
         const TailLevel f = tl[q], c = tl[q + 1];
         for (int s = 0; s < 2; s++) {
-            t_sweep(f, lz, lr, ls, 0);
-            t_sweep(f, lz, lr, ls, 1);
+            t_sweep(f, lz, lr, ls, lcl, 0);
+            t_sweep(f, lz, lr, ls, lcl, 1);
         }
-        t_restrict(f, c, lz, lr, ls);
+        t_restrict(f, c, lz, lr, ls, lcl);
     }
     // coarsest: symmetric sweeps
     {
         const TailLevel v = tl[L];
         for (int s = 0; s < COARSEST_SWEEPS; s++) {
-            t_sweep(v, lz, lr, ls, 0);
-            t_sweep(v, lz, lr, ls, 1);
+            t_sweep(v, lz, lr, ls, lcl, 0);
+            t_sweep(v, lz, lr, ls, lcl, 1);
         }
         for (int s = 0; s < COARSEST_SWEEPS; s++) {
-            t_sweep(v, lz, lr, ls, 1);
-            t_sweep(v, lz, lr, ls, 0);
+            t_sweep(v, lz, lr, ls, lcl, 1);
+            t_sweep(v, lz, lr, ls, lcl, 0);
         }
     }
     // up: correct + post-smooth (black then red, twice)
     for (int q = L - 1; q >= T; q--) {
         const TailLevel f = tl[q], c = tl[q + 1];
-        t_post(f, c, lz, lr, ls);
-        t_sweep(f, lz, lr, ls, 0);
-        t_sweep(f, lz, lr, ls, 1);
-        t_sweep(f, lz, lr, ls, 0);
+        t_post(f, c, lz, lr, ls, lcl);
+        t_sweep(f, lz, lr, ls, lcl, 0);
+        t_sweep(f, lz, lr, ls, lcl, 1);
+        t_sweep(f, lz, lr, ls, lcl, 0);
     }
 #pragma unroll
     for (int q = 1; q < MG_MAX; q++) {
