@@ -67,27 +67,79 @@ __device__ __forceinline__ uint32_t widx(const DevSim &S, int i, int j, int k) {
     return S.nu + S.nv + (uint32_t)i + (uint32_t)S.nx * ((uint32_t)j + (uint32_t)S.ny * (uint32_t)k);
 }
 
+// Active tiles among the 3x3x3 tiles around tile (ti, tj, tk): bit (dx+1) + 3 (dy+1) + 9 (dz+1)
+// (tiles past the grid are 0).  Each wave computes it with one load per lane and a ballot.
+__device__ __forceinline__ uint32_t active_neighbours(const DevSim &S, int ti, int tj, int tk) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    bool on = false;
+    if (lane < 27) {
+        const int x = ti + lane % 3 - 1, y = tj + (lane / 3) % 3 - 1, z = tk + lane / 9 - 1;
+        if (x >= 0 && y >= 0 && z >= 0 && x < S.ntx && y < S.nty && z < S.ntz)
+            on = tile_active(S, (uint32_t)x + (uint32_t)S.ntx * ((uint32_t)y + (uint32_t)S.nty * (uint32_t)z));
+    }
+    return (uint32_t)__ballot(on);
+}
+
+// Packed P2G sum of face (c, q) for a face pass walking tile t = (t0, t1, t2) (q in t's node
+// box): the boxes that contain the face are t's and, per axis, the tile below when q is on t's
+// lower face plane, or the tile above when q is on its last cell row along an axis other than
+// c (the same tiles gather_face visits).  The active ones come from the mask, so the (up to
+// 8) box loads do not wait on tile flags and are all in flight together.  Integer sums: the
+// order does not matter.
+__device__ __forceinline__ int64_t gather_face_in(const DevSim &S, uint32_t act, int c, const int q[3],
+                                                  const int t[3]) {
+    int alt[3], u[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        u[a] = q[a] - TILE * t[a];
+        alt[a] = u[a] == 0 ? -1 : ((a != c && u[a] == TILE - 1) ? 1 : 0);
+    }
+    const int d0 = c == 0 ? TILE + 1 : TILE + 2, d1 = c == 1 ? TILE + 1 : TILE + 2;
+    const uint32_t t_lin = (uint32_t)t[0] + (uint32_t)S.ntx * ((uint32_t)t[1] + (uint32_t)S.nty * (uint32_t)t[2]);
+    int64_t sum = 0;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const int dx = (e & 1) ? alt[0] : 0, dy = (e & 2) ? alt[1] : 0, dz = (e & 4) ? alt[2] : 0;
+        const bool valid = (!(e & 1) || alt[0]) && (!(e & 2) || alt[1]) && (!(e & 4) || alt[2]);
+        if (!valid || !((act >> ((dx + 1) + 3 * (dy + 1) + 9 * (dz + 1))) & 1u)) continue;
+        // the face's position in the box of tile t + d (a box starts one cell below its tile
+        // along the axes other than c)
+        const int lx = u[0] - TILE * dx + (c != 0), ly = u[1] - TILE * dy + (c != 1), lz = u[2] - TILE * dz + (c != 2);
+        const uint32_t tn = t_lin + (uint32_t)(dx + S.ntx * (dy + S.nty * dz));
+        sum += S.box[(size_t)tn * BOX_ENTRIES + (size_t)c * BOX_FACES + (size_t)(lx + d0 * (ly + d1 * lz))];
+    }
+    return sum;
+}
+
 // normalize_grid faces + forces (fluid.slang:119-124, 143-151) for one face: the P2G sums
 // acc, wacc are gathered from the tile boxes that cover the face; v = acc / wacc (0 without
 // weight) -> vel_old; vel = v + g_c * dt, or 0 when either side is SOLID.  SOLID depends only
 // on the cell position, so this needs no classified cells.
-__device__ __forceinline__ void face_update(const DevSim &S, int c, int i, int j, int k, uint32_t f, float g,
-                                            bool keep) {
-    const int64_t packed = gather_face(S, c, i, j, k);
+__device__ __forceinline__ void face_update(const DevSim &S, int64_t packed, uint32_t f, float g, bool keep) {
     const int32_t w = packed_wacc(packed);
     const float v = w > 0 ? (float)packed_acc(packed) / (float)w : 0.0f;
     S.vel_old[f] = v;
     S.vel[f] = keep ? v + g * S.dt : 0.0f;
 }
 
-__device__ __forceinline__ void faces_at(const DevSim &S, int i, int j, int k) {
+__device__ __forceinline__ void faces_at(const DevSim &S, uint32_t act, const int t[3], int i, int j, int k) {
     const bool in_c = !is_solid(S, i, j, k);
-    if (j < S.ny && k < S.nz) face_update(S, 0, i, j, k, uidx(S, i, j, k), S.gx, in_c && !is_solid(S, i - 1, j, k));
-    if (i < S.nx && k < S.nz) face_update(S, 1, i, j, k, vidx(S, i, j, k), S.gy, in_c && !is_solid(S, i, j - 1, k));
-    if (i < S.nx && j < S.ny) face_update(S, 2, i, j, k, widx(S, i, j, k), S.gz, in_c && !is_solid(S, i, j, k - 1));
+    const int q[3] = {i, j, k};
+    const bool has[3] = {j < S.ny && k < S.nz, i < S.nx && k < S.nz, i < S.nx && j < S.ny};
+    int64_t sum[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) sum[c] = has[c] ? gather_face_in(S, act, c, q, t) : 0;
+    if (has[0]) face_update(S, sum[0], uidx(S, i, j, k), S.gx, in_c && !is_solid(S, i - 1, j, k));
+    if (has[1]) face_update(S, sum[1], vidx(S, i, j, k), S.gy, in_c && !is_solid(S, i, j - 1, k));
+    if (has[2]) face_update(S, sum[2], widx(S, i, j, k), S.gz, in_c && !is_solid(S, i, j, k - 1));
 }
 __device__ __forceinline__ void body_faces(const DevSim &S, uint32_t b, uint32_t nb) {
-    for_band(S, b, nb, [&](uint32_t t) { for_tile_box(S, t, true, [&](int i, int j, int k) { faces_at(S, i, j, k); }); });
+    for_band(S, b, nb, [&](uint32_t tl) {
+        const int t[3] = {(int)(tl % (uint32_t)S.ntx), (int)((tl / (uint32_t)S.ntx) % (uint32_t)S.nty),
+                          (int)(tl / ((uint32_t)S.ntx * (uint32_t)S.nty))};
+        const uint32_t act = active_neighbours(S, t[0], t[1], t[2]);
+        for_tile_box(S, tl, true, [&](int i, int j, int k) { faces_at(S, act, t, i, j, k); });
+    });
 }
 __global__ __launch_bounds__(GBLOCK) void k_faces(DevSim S) { body_faces(S, blockIdx.x, gridDim.x); }
 __global__ __launch_bounds__(GBLOCK) void kb_faces(WorkList W) {
