@@ -243,8 +243,10 @@ __global__ __launch_bounds__(GBLOCK) void kb_cells(WorkList W) {
 // Fluid list from the scanned row counts: one wave per grid row writes the row's FLUID cells
 // in x order at the row's slots (red ones at fl_rowoff[row], black ones at fl_rowoff[R + row]
 // when colours are split), so the list is in cell order within each colour.  Each entry also
-// gets its stencil word (see DevSim::fl_word).  The wave of row 0 writes fl_count = {red
-// cells (0 unless split), all fluid cells}.
+// gets its stencil word (see DevSim::fl_word).  With colours split the row's cells also go,
+// in x order, to the cell-order list fn_list at slot rowoff[row] + rowoff[R + row] - n_red
+// (the FLUID cells of all earlier rows).  The wave of row 0 writes fl_count = {red cells (0
+// unless split), all fluid cells}.
 constexpr int ROWS_PER_BLOCK = BLOCK / WAVE;
 __device__ __forceinline__ void body_fluid_rows(const DevSim &S, uint32_t b) {
     const uint32_t R = (uint32_t)S.ny * (uint32_t)S.nz, row = b * ROWS_PER_BLOCK + (threadIdx.x >> 6);
@@ -257,6 +259,7 @@ __device__ __forceinline__ void body_fluid_rows(const DevSim &S, uint32_t b) {
     }
     if (S.fl_rows[row] + S.fl_rows[R + row] == 0) return;
     uint32_t oa = S.fl_rowoff[row], ob = S.fl_rowoff[R + row];
+    uint32_t on = split ? oa + ob - S.fl_rowoff[R] : 0u;
     const uint32_t j = row % (uint32_t)S.ny, k = row / (uint32_t)S.ny, base = (uint32_t)S.nx * row;
     const int32_t sy = S.nx, sz = S.nx * S.ny;
     const int32_t off[6] = {-1, 1, -sy, sy, -sz, sz};
@@ -278,9 +281,15 @@ __device__ __forceinline__ void body_fluid_rows(const DevSim &S, uint32_t b) {
             word |= black << 27;
             S.fl_list[o] = id;
             S.fl_word[o] = word;
+            if (split) {
+                const uint32_t n = on + (uint32_t)__popcll((ma | mb) & below);
+                S.fn_list[n] = id;
+                S.fn_word[n] = word;
+            }
         }
         oa += (uint32_t)__popcll(ma);
         ob += (uint32_t)__popcll(mb);
+        on += (uint32_t)__popcll(ma | mb);
     }
 }
 __global__ __launch_bounds__(BLOCK) void k_fluid_rows(DevSim S) { body_fluid_rows(S, blockIdx.x); }

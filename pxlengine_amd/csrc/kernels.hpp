@@ -103,6 +103,10 @@ struct DevSim {
     // non-SOLID neighbour count (matrix diagonal), bits 9+3n the diagonal of FLUID neighbour n,
     // bit 27 the cell's colour (1 = black: i + j + k odd)
     uint32_t *fl_word;
+    // the same entries in plain cell order (with colours not split: fl_list / fl_word
+    // themselves), for the kernels that treat every FLUID cell alike: their vector accesses
+    // then run along x instead of skipping every other cell
+    uint32_t *fn_list, *fn_word;
     // solver
     float *cg_r0, *cg_r1, *cg_w0, *cg_w1, *cg_s0, *cg_s1, *cg_p, *e;
     double *x64a, *x64b, *r64;

@@ -151,7 +151,7 @@ __device__ __forceinline__ void body_l0_pre0(const DevSim &S, const uint32_t *fl
     slice(flc[1], b, nb, beg, end);
     const int32_t off[6] = {-1, 1, -S.nx, S.nx, -S.nx * S.ny, S.nx * S.ny};
     for (uint32_t t = beg + threadIdx.x; t < end; t += LBLOCK) {
-        const uint32_t id = S.fl_list[t], wd = S.fl_word[t];
+        const uint32_t id = S.fn_list[t], wd = S.fn_word[t];
         float sum = 0.0f;
         if (w_black(wd)) {
 #pragma unroll

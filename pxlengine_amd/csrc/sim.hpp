@@ -160,6 +160,10 @@ struct Sim {
         add(&d.fl_rowoff, 4ull * fl_row_slots(d));
         add(&d.fl_list, 4 * C);
         add(&d.fl_word, 4 * C);
+        if (d.split_colours) {
+            add(&d.fn_list, 4 * C);
+            add(&d.fn_word, 4 * C);
+        }
         add(&d.fl_count, 64);
         add(&d.fl_tmp, 4ull * (scan_tiles(fl_row_slots(d)) + 1));
         for (float **p : {&d.cg_r0, &d.cg_r1, &d.cg_w0, &d.cg_w1, &d.cg_s0, &d.cg_s1, &d.cg_p}) add(p, 4 * C);
@@ -206,6 +210,10 @@ struct Sim {
             cur += (e.second + ALIGN - 1) / ALIGN * ALIGN;
         }
         d.norms = d.scal + SC_RR;
+        if (!d.split_colours) {  // the list is already in cell order
+            d.fn_list = d.fl_list;
+            d.fn_word = d.fl_word;
+        }
         HIP_CHECK(hipMemsetAsync(arena, 0, arena_bytes, stream));
         // the end of the last tile's particle run (the scan fills offset[0 .. nkeys-1])
         const uint32_t np32 = d.np;

@@ -121,8 +121,8 @@ __device__ __forceinline__ void body_cg_residual(const DevSim &S, const uint32_t
     int32_t off[6];
     nb_offsets(S, off);
     for (uint32_t t = beg + threadIdx.x; t < end; t += SBLOCK) {
-        const uint32_t id = S.fl_list[t];
-        const uint32_t wd = S.fl_word[t];
+        const uint32_t id = S.fn_list[t];
+        const uint32_t wd = S.fn_word[t];
         if (inner) {
             r[id] = (float)S.r64[id];
             x[id] = 0.0f;
@@ -152,8 +152,8 @@ __global__ __launch_bounds__(SBLOCK) void k_cg_start(DevSim S, const float *__re
     nb_offsets(S, off);
     Acc g = 0, d = 0;
     for (uint32_t t = beg + threadIdx.x; t < end; t += SBLOCK) {
-        const uint32_t id = S.fl_list[t];
-        const uint32_t wd = S.fl_word[t];
+        const uint32_t id = S.fn_list[t];
+        const uint32_t wd = S.fn_word[t];
         const float uc = precond(r[id], w_diag(wd));
         float sum = 0.0f;
 #pragma unroll
@@ -217,8 +217,8 @@ __global__ __launch_bounds__(SBLOCK) void k_cg_iter(DevSim S, uint32_t it, float
     cg_scalars<Acc>(S, it, blockIdx.x, gridDim.x, alpha, beta);
     Acc g = 0, d = 0;
     for (uint32_t t = beg + threadIdx.x; t < end; t += SBLOCK) {
-        const uint32_t c = S.fl_list[t];
-        const uint32_t wd = S.fl_word[t];
+        const uint32_t c = S.fn_list[t];
+        const uint32_t wd = S.fn_word[t];
         uint32_t q[6];
 #pragma unroll
         for (int n = 0; n < 6; n++) q[n] = w_fluid(wd, n) ? c + off[n] : c;
@@ -268,8 +268,8 @@ __device__ __forceinline__ void body_mgcg_spmv(const DevSim &S, const uint32_t *
     const float *__restrict__ u = S.cg_r1;
     Acc g = 0, d = 0;
     for (uint32_t t = beg + threadIdx.x; t < end; t += SBLOCK) {
-        const uint32_t c = S.fl_list[t];
-        const uint32_t wd = S.fl_word[t];
+        const uint32_t c = S.fn_list[t];
+        const uint32_t wd = S.fn_word[t];
         float sum = 0.0f;
 #pragma unroll
         for (int n = 0; n < 6; n++)
@@ -297,7 +297,7 @@ __device__ __forceinline__ void body_mgcg_update(const DevSim &S, const uint32_t
     float alpha, beta;
     cg_scalars<Acc>(S, it, b, nb, alpha, beta);
     for (uint32_t t = beg + threadIdx.x; t < end; t += SBLOCK) {
-        const uint32_t c = S.fl_list[t];
+        const uint32_t c = S.fn_list[t];
         const float pc = S.cg_r1[c] + beta * S.cg_p[c];
         const float sc = S.cg_w0[c] + beta * S.cg_s0[c];
         S.cg_p[c] = pc;
@@ -330,8 +330,8 @@ __device__ __forceinline__ void body_mixed_init(const DevSim &S, const uint32_t 
     nb_offsets(S, off);
     double rr = 0, bb = 0;
     for (uint32_t t = beg + threadIdx.x; t < end; t += SBLOCK) {
-        const uint32_t id = S.fl_list[t];
-        const uint32_t wd = S.fl_word[t];
+        const uint32_t id = S.fn_list[t];
+        const uint32_t wd = S.fn_word[t];
         double sum = 0.0;
 #pragma unroll
         for (int n = 0; n < 6; n++)
@@ -362,8 +362,8 @@ __device__ __forceinline__ void body_mixed_update(const DevSim &S, const uint32_
     nb_offsets(S, off);
     double rr = 0;
     for (uint32_t t = beg + threadIdx.x; t < end; t += SBLOCK) {
-        const uint32_t id = S.fl_list[t];
-        const uint32_t wd = S.fl_word[t];
+        const uint32_t id = S.fn_list[t];
+        const uint32_t wd = S.fn_word[t];
         double sum = 0.0;
 #pragma unroll
         for (int n = 0; n < 6; n++) {
