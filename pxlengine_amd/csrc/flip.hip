@@ -126,6 +126,12 @@ int prepare(const char *fluid_json, double fps, uint32_t seed, const pxlflip_opt
     d.ntz = (int)((n[2] + TILE - 1) / TILE);
     d.ntiles = (uint32_t)d.ntx * (uint32_t)d.nty * (uint32_t)d.ntz;
     d.nkeys = d.ntiles * TILE_CELLS;
+    {
+        // four times the tiles the liquid fills at seeding (room for the band around it and
+        // for spreading), at least 64, at most 1024 and the grid's tile count
+        const uint64_t liquid_tiles = ((uint64_t)np / std::max<uint32_t>(spec.particles_per_cell, 1u) + TILE_CELLS - 1) / TILE_CELLS;
+        d.tile_wgs = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(d.ntiles, 1024), std::max<uint64_t>(64, 4 * liquid_tiles));
+    }
     d.cg_blocks = (int)(o.solver_workgroups ? o.solver_workgroups : DEFAULT_SOLVER_BLOCKS);
     d.split_colours = o.solver == PXLFLIP_SOLVER_CG ? 0 : 1;  // RBGS and the multigrid sweep one colour at a time
     d.mg_levels = 0;

@@ -19,7 +19,7 @@ constexpr unsigned BATCH_GRID = 2048;  // persistent grid of batched launches
 
 // Workgroups of the band-walking launches of one simulation (each takes band tiles with this
 // stride), the same count as the per-tile particle kernels.
-__host__ __device__ inline uint32_t band_grid(const DevSim &S) { return S.ntiles < 1024u ? S.ntiles : 1024u; }
+__host__ __device__ inline uint32_t band_grid(const DevSim &S) { return S.tile_wgs; }
 
 // Runs f(i, j, k) over tile t's nodes (faces are stored at the node of their low corner: the
 // tile's 8^3 nodes, plus the grid's upper boundary nodes on the last tile along an axis) or

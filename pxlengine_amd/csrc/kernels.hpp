@@ -79,6 +79,10 @@ struct DevSim {
     // (sum_wacc >= 0 and < 2^32, so the halves never carry into each other)
     int64_t *box;
     uint32_t ntiles;
+    // workgroups of the per-tile (P2G, G2P) and band (faces, cells, projection) launches, each
+    // walking its list of tiles with this stride: sized by the liquid's volume in tiles, not
+    // the grid's, so a small liquid in a large grid does not launch idle workgroups
+    uint32_t tile_wgs;
     uint32_t *tile_list, *tile_count;  // tiles holding particles this step (any order)
     // per tile: bit 0 = holds particles this step, bit 1 = held particles the step before
     // (3 after allocation, reset or a field write: the next two steps then cover every tile)

@@ -293,9 +293,9 @@ __global__ __launch_bounds__(BLOCK) void kb_tile_list(WorkList W) {
     for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *) { body_tile_list(S, b); });
 }
 
-// Workgroups of the per-tile kernels: enough to keep every CU busy, each walking the active
-// tile list with a stride of this count.
-__host__ __device__ inline uint32_t tile_grid(const DevSim &S) { return S.ntiles < 1024u ? S.ntiles : 1024u; }
+// Workgroups of the per-tile kernels, each walking the active tile list with a stride of
+// this count.
+__host__ __device__ inline uint32_t tile_grid(const DevSim &S) { return S.tile_wgs; }
 
 // p2g (fluid.slang:94-112), one active 8^3 tile at a time per workgroup.  The tile's
 // particles are one contiguous run of the cell-sorted scratch copy; their fixed-point splats
