@@ -67,8 +67,12 @@ __device__ __forceinline__ uint8_t level_type(const MGLevel &v, int i, int j, in
 __device__ __forceinline__ uint32_t parent(int cnx, int cny, int i, int j, int k) {
     return (uint32_t)(i >> 1) + (uint32_t)cnx * ((uint32_t)(j >> 1) + (uint32_t)cny * (uint32_t)(k >> 1));
 }
-// Workgroup b's (of nb) slice [beg, end) of an n-entry list.
+// Workgroup b's (of nb) slice [beg, end) of an n-entry list.  Workgroups are dealt to the 8
+// XCDs round-robin (workgroup i runs on XCD i % 8) and each XCD has its own L2: with nb a
+// multiple of 8, XCD x takes the x-th eighth of the list, so the neighbour rows and planes
+// its slices read are mostly in its own L2.
 __device__ __forceinline__ void slice(uint32_t n, uint32_t b, uint32_t nb, uint32_t &beg, uint32_t &end) {
+    if (nb % 8u == 0) b = (b % 8u) * (nb / 8u) + b / 8u;
     beg = (uint32_t)((uint64_t)n * b / nb);
     end = (uint32_t)((uint64_t)n * (b + 1) / nb);
 }

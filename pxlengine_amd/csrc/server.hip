@@ -566,9 +566,12 @@ private:
 // (its particles over the seeding density), so that a workgroup's slice of a list keeps its
 // lanes busy.
 uint32_t batch_solver_blocks(const Sim &s) {
+    // a multiple of 8, so every member's entries start on a multiple of 8 in the solve lists
+    // and a persistent workgroup's XCD (its index mod 8) is the slice index mod 8: the XCD
+    // slice mapping of the list kernels then holds in the batch as it does alone
     const double fluid = (double)s.d.np / (double)std::max<uint32_t>(1u, s.spec.particles_per_cell);
-    const double g = std::ceil(fluid / 2048.0);
-    return (uint32_t)std::max(1.0, std::min(256.0, g));
+    const double g = 8.0 * std::ceil(fluid / (8.0 * 2048.0));
+    return (uint32_t)std::max(8.0, std::min(256.0, g));
 }
 
 struct Request {
