@@ -45,8 +45,9 @@ struct MGLevel {
     float *r, *z;    // level-l right-hand side (the scaled restricted residual) and iterate
     // FLUID cells of the level (levels >= 1, rebuilt each step): red cells from the front
     // (count[0] of them), black cells from the back (count[1]: list[cells - 1 - t]), each in
-    // any order
-    uint32_t *list, *count;
+    // any order; nlist holds all of them (count[2]) in the order the stencil pass meets them
+    // (cell order within each wave's 64 cells), for the kernels that treat both colours alike
+    uint32_t *list, *nlist, *count;
 };
 
 // Solver scalar slots (fp64 array): two slots of SC_COUNT by CG-iteration parity for the
@@ -267,10 +268,12 @@ constexpr uint32_t WORK_BLOCK_MASK = (1u << WORK_SIM_SHIFT) - 1u;
 constexpr uint32_t MAX_BATCH_SIMS = 1u << (32 - WORK_SIM_SHIFT);
 
 // A batch keeps every member's list counts in one array, COUNT_SLOTS words per simulation:
-// [0..1] = DevSim::fl_count, [2l..2l+1] = MGLevel::count of level l (the members' DevSim
+// [0..1] = DevSim::fl_count, [4l..4l+2] = MGLevel::count of level l (the members' DevSim
 // copies point there).  A batched kernel derives a member's slots from its work entry alone,
 // so the counts load together with the member's DevSim fields instead of after them.
-constexpr uint32_t COUNT_SLOTS = 2 * MG_MAX;
+constexpr uint32_t COUNT_SLOTS = 4 * MG_MAX;
+// Level stencil words (levels >= 1) carry the cell's colour in this bit (1 = black).
+constexpr uint16_t STEN_BLACK = 0x0800;
 // Every work list's storage holds at least this many entries.
 constexpr uint32_t WORK_LIST_MIN_CAP = 16384;
 

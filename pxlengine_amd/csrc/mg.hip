@@ -88,6 +88,7 @@ __device__ __forceinline__ void body_mg_types(const MGLevel &f, const MGLevel &c
     if (id == 0) {
         c.count[0] = 0;
         c.count[1] = 0;
+        c.count[2] = 0;
     }
     if (id >= ncell(c)) return;
     int i, j, k;
@@ -104,8 +105,8 @@ __device__ __forceinline__ void body_mg_types(const MGLevel &f, const MGLevel &c
 __global__ __launch_bounds__(MBLOCK) void k_mg_types(MGLevel f, MGLevel c) { body_mg_types(f, c, blockIdx.x); }
 
 // ---- level stencil words (FLUID bit, non-SOLID-neighbour count in bits 8-10, FLUID-neighbour
-// mask in bits 0-5: the layout of the simulation grid's stencil words), z = 0, and the level's
-// FLUID-cell list (one atomic per wave)
+// mask in bits 0-5: the layout of the simulation grid's stencil words, plus the colour in
+// STEN_BLACK), z = 0, and the level's FLUID-cell lists (one atomic per wave and list)
 __device__ __forceinline__ void body_mg_sten(const MGLevel &v, uint32_t b) {
     const uint32_t id = b * MBLOCK + threadIdx.x;
     const bool in = id < ncell(v);
@@ -123,8 +124,8 @@ __device__ __forceinline__ void body_mg_sten(const MGLevel &v, uint32_t b) {
                 if (t != CELL_SOLID) diag++;
                 if (t == CELL_FLUID) st |= 1u << n;
             }
-            st |= STEN_FLUID | (diag << 8);
             black = ((i + j + k) & 1) != 0;
+            st |= STEN_FLUID | (diag << 8) | (black ? STEN_BLACK : 0u);
         }
         v.sten[id] = (uint16_t)st;
         v.z[id] = 0.0f;
@@ -133,15 +134,18 @@ __device__ __forceinline__ void body_mg_sten(const MGLevel &v, uint32_t b) {
     const int lane = threadIdx.x & (WAVE - 1);
     const uint64_t below = (1ull << lane) - 1ull;
 #pragma unroll
-    for (int col = 0; col < 2; col++) {
-        const bool mine = fl && black == (col == 1);
+    for (int col = 0; col < 3; col++) {  // red, black, both (the cell-order list)
+        const bool mine = fl && (col == 2 || black == (col == 1));
         const uint64_t mask = __ballot(mine);
         const int leader = mask ? __ffsll((long long)mask) - 1 : 0;
         uint32_t base = 0;
         if (mine && lane == leader) base = atomicAdd(&v.count[col], (uint32_t)__popcll(mask));
         base = __shfl(base, leader, WAVE);
         const uint32_t slot = base + (uint32_t)__popcll(mask & below);
-        if (mine) v.list[col ? ncell(v) - 1u - slot : slot] = id;
+        if (mine) {
+            if (col == 2) v.nlist[slot] = id;
+            else v.list[col ? ncell(v) - 1u - slot : slot] = id;
+        }
     }
 }
 __global__ __launch_bounds__(MBLOCK) void k_mg_sten(MGLevel v) { body_mg_sten(v, blockIdx.x); }
@@ -223,21 +227,18 @@ __device__ __forceinline__ void body_l0_post(const DevSim &S, const uint32_t *fl
     }
 }
 
-// ---- coarse levels 1..mg_tail-1 (the level's red / black lists)
-__device__ __forceinline__ uint32_t lc_entry(const MGLevel &v, uint32_t t, uint32_t n_red) {
-    return t < n_red ? v.list[t] : v.list[ncell(v) - 1u - (t - n_red)];
-}
+// ---- coarse levels 1..mg_tail-1 (the level's red / black lists; pre0 walks the cell-order list)
 __device__ __forceinline__ void body_lc_pre0(const MGLevel &v, uint32_t b, uint32_t nb) {
-    const uint32_t n_red = v.count[0], n = n_red + v.count[1];
+    const uint32_t n = v.count[0] + v.count[1];
     uint32_t beg, end;
     slice(n, b, nb, beg, end);
     const int32_t off[6] = {-1, 1, -v.nx, v.nx, -v.nx * v.ny, v.nx * v.ny};
     const uint32_t last = ncell(v) - 1u;
     for (uint32_t t = beg + threadIdx.x; t < end; t += LBLOCK) {
-        const uint32_t id = lc_entry(v, t, n_red);
+        const uint32_t id = v.nlist[t];
         const uint32_t st = v.sten[id];
         float sum = 0.0f;
-        if (t >= n_red) {
+        if (st & STEN_BLACK) {
             // neighbours are loaded before the cell's stencil word arrives (indices clamped
             // into the level) and the ones that are not FLUID are dropped afterwards
             float rq[6];
@@ -311,19 +312,20 @@ __device__ __forceinline__ void body_lc_post(const MGLevel &v, const MGLevel &c,
     }
 }
 
-// ---- restrict level f -> c after f's black sweep: for each FLUID cell of c (its list),
-// c.r = 1/2 sum over FLUID red children (q = 0, 3, 5, 6) of r - L z.  f's arrays are given
+// ---- restrict level f -> c after f's black sweep: for each FLUID cell of c (its cell-order
+// list, so neighbouring lanes read neighbouring children), c.r = 1/2 sum over FLUID red
+// children (q = 0, 3, 5, 6) of r - L z.  f's arrays are given
 // explicitly (level 0: the CG vectors and the grid's stencil words).
 __device__ __forceinline__ void body_restrict(int fnx, int fny, int fnz, const uint16_t *__restrict__ fsten,
                                               const float *__restrict__ fr, const float *__restrict__ fz,
                                               const MGLevel &c, uint32_t b, uint32_t nb) {
-    const uint32_t n_red = c.count[0], n = n_red + c.count[1];
+    const uint32_t n = c.count[0] + c.count[1];
     uint32_t beg, end;
     slice(n, b, nb, beg, end);
     const int32_t off[6] = {-1, 1, -fnx, fnx, -fnx * fny, fnx * fny};
     const uint32_t flast = (uint32_t)(fnx * fny * fnz) - 1u;
     for (uint32_t t = beg + threadIdx.x; t < end; t += LBLOCK) {
-        const uint32_t cc = lc_entry(c, t, n_red);
+        const uint32_t cc = c.nlist[t];
         int ci, cj, ck;
         cell_ijk(c.nx, c.ny, cc, ci, cj, ck);
         const int red[4] = {0, 3, 5, 6};
@@ -606,7 +608,7 @@ __global__ __launch_bounds__(MBLOCK) void kb_mg_sten(WorkList W, int l) {
 // work entry alone, so the count loads alongside the level's fields).
 __device__ __forceinline__ MGLevel level_in_batch(const DevSim &S, uint32_t *cnt, int l) {
     MGLevel v = S.mg[l];
-    v.count = cnt + 2 * l;
+    v.count = cnt + 4 * l;
     return v;
 }
 __global__ __launch_bounds__(LBLOCK) void kb_pre0(WorkList W, int l) {

@@ -110,7 +110,7 @@ double share_bytes(const char *k, const DevSim &d, const MemberCounts &m, int l)
     if (n == "cells") return bf * (4 * F + 15 * C + 4 * C) - 4 * Cf;
     if (n == "fluid_compact") return 8 * rows + 2 * C * bf + 8 * Cf;
     if (n == "mg_types") return cells_of(l - 1) + cells_of(l);
-    if (n == "mg_sten") return 7 * cells_of(l) + 4 * cl;
+    if (n == "mg_sten") return 7 * cells_of(l) + 8 * cl;
     if (n == "mixed_init") return 32 * Cf;
     if (n == "cg_residual") return 24 * Cf;
     if (n == "mg_pre0") return 16 * cl;
@@ -378,7 +378,7 @@ private:
             const uint32_t *c = slots.data() + i * COUNT_SLOTS;
             counts_[i].fluid[0] = c[1];
             for (int l = 1; l < members_[i]->d.mg_levels && l < MG_MAX; l++)
-                counts_[i].fluid[l] = c[2 * l] + c[2 * l + 1];
+                counts_[i].fluid[l] = c[4 * l] + c[4 * l + 1];
         }
         // this step's grid-phase launches (recorded before its counts were known) are priced now
         for (size_t k = grid_start_; k < recs_used_; k++) recs_[k].bytes = launch_bytes(recs_[k].name, recs_[k].level);
@@ -524,7 +524,7 @@ private:
             h_sims_[i].norms = d_norms_ + 2 * i;
             uint32_t *slot = d_slots_ + i * COUNT_SLOTS;
             h_sims_[i].fl_count = slot;
-            for (int l = 1; l < MG_MAX; l++) h_sims_[i].mg[l].count = slot + 2 * l;
+            for (int l = 1; l < MG_MAX; l++) h_sims_[i].mg[l].count = slot + 4 * l;
         }
         HIP_CHECK(hipMemcpyAsync(d_sims_, h_sims_, sizeof(DevSim) * n, hipMemcpyHostToDevice, stream_));
         std::vector<uint32_t> all(n);

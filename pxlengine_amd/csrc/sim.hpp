@@ -175,6 +175,7 @@ struct Sim {
                 add(&d.mg[l].r, 4 * cl);
                 add(&d.mg[l].z, 4 * cl);
                 add(&d.mg[l].list, 4 * cl);
+                add(&d.mg[l].nlist, 4 * cl);
                 add(&d.mg[l].count, 64);
             }
             add(&d.e, 4 * C);
