@@ -412,6 +412,9 @@ int pxlflip_read_bins(pxlflip *h, uint32_t *cell_of, uint32_t *counts, uint32_t 
         pxl::launch_bin_count(d, s->stream);
         // ... reported in reference cell numbering
         pxl::launch_count_linear(d, cnt.as<uint32_t>(), s->stream);
+        // the step clears only the counts of the tiles its last tile list holds (bin_count
+        // above emptied that list), so every count must be zero again before the next step
+        pxl::launch_fill_u32(d.count, d.nkeys, 0, s->stream);
         pxl::launch_exclusive_scan(cnt.as<uint32_t>(), (uint32_t)C, tmp.as<uint32_t>(), off.as<uint32_t>(), s->stream);
         pxl::launch_bins_out(d, off.as<uint32_t>(), cbuf.as<uint32_t>(), sbuf.as<uint32_t>(), s->stream);
         pxl::launch_sort_segments((uint32_t)C, off.as<uint32_t>(), cnt.as<uint32_t>(), sbuf.as<uint32_t>(), s->stream);

@@ -205,7 +205,7 @@ LaunchEvents &launch_events();  // this thread's slot
 
 // ---- launch helpers (all asynchronous on `s`) ----
 void launch_fill_u32(uint32_t *p, uint32_t n, uint32_t value, hipStream_t s);
-void launch_clear_grid(const DevSim &S, hipStream_t s);       // count = 0
+void launch_clear_grid(const DevSim &S, hipStream_t s);       // count = 0 (the last step's tiles), row counts = 0
 void launch_bin_count(const DevSim &S, hipStream_t s);        // key, slot, count
 void launch_scan_offsets(const DevSim &S, hipStream_t s);     // offset = exclusive scan(count)
 uint32_t scan_tiles(uint32_t n);                              // scratch entries a scan of n needs
@@ -313,7 +313,6 @@ __device__ __forceinline__ void for_each_work(const WorkList &W, F &&f) {
 // separate launches (level < mg_tail).
 enum WorkDomain : int {
     WD_SIM = 0,     // one entry per simulation
-    WD_KEYS,        // count array (clear)
     WD_PART,        // particles
     WD_SCAN_KEYS,   // scan tiles of the count array
     WD_TILES,       // tiles (active-tile list)
@@ -333,7 +332,6 @@ inline bool wd_solve(int d) { return d >= WD_A_SIM; }
 uint32_t work_blocks(const DevSim &S, int d);
 // per-file pieces of work_blocks
 uint32_t particle_blocks(const DevSim &S);
-uint32_t key_blocks(const DevSim &S);
 uint32_t tile_blocks(const DevSim &S);
 uint32_t tile_grid_blocks(const DevSim &S);
 uint32_t mg_cell_blocks(const DevSim &S, int l);

@@ -38,21 +38,27 @@ __global__ __launch_bounds__(BLOCK) void k_fill_u32(uint32_t *p, uint32_t n, uin
     if (i < n) p[i] = value;
 }
 
+// Workgroups of the per-tile kernels, each walking the active tile list with a stride of
+// this count.
+__host__ __device__ inline uint32_t tile_grid(const DevSim &S) { return S.tile_wgs; }
+
 // clear_grid (fluid.slang:88-91).  The P2G sums live in per-tile boxes that P2G overwrites
-// whole, so only the cell counts need clearing (and the fluid list's row counts, which the
-// cell pass accumulates).
-__device__ __forceinline__ void body_clear_grid(const DevSim &S, uint32_t b) {
-    uint32_t i = b * BLOCK + threadIdx.x;
-    if (i < S.nkeys) S.count[i] = 0;
-    for (uint32_t r = i; r < fl_row_slots(S); r += S.nkeys) S.fl_rows[r] = 0;
-    if (i == 0) {
-        *S.tile_count = 0;
-        *S.band_count = 0;
+// whole, so only the cell counts need clearing, and only in the tiles the previous step's
+// tile list holds: every other count is still zero (allocation zeroes them, and a tile gets
+// counts only in a step that lists it).  Workgroup b of nb takes listed tiles b, b + nb, ...;
+// the list's count is reset by bin_count, after every workgroup here has read it.  Also
+// zeroes the fluid list's row counts, which the cell pass accumulates.
+__device__ __forceinline__ void body_clear_grid(const DevSim &S, uint32_t b, uint32_t nb) {
+    const uint32_t n_prev = *S.tile_count;
+    for (uint32_t u = b; u < n_prev; u += nb) {
+        uint32_t *c = S.count + (size_t)S.tile_list[u] * TILE_CELLS;
+        for (uint32_t q = threadIdx.x; q < TILE_CELLS; q += BLOCK) c[q] = 0;
     }
+    for (uint32_t r = b * BLOCK + threadIdx.x; r < fl_row_slots(S); r += nb * BLOCK) S.fl_rows[r] = 0;
 }
-__global__ __launch_bounds__(BLOCK) void k_clear_grid(DevSim S) { body_clear_grid(S, blockIdx.x); }
+__global__ __launch_bounds__(BLOCK) void k_clear_grid(DevSim S) { body_clear_grid(S, blockIdx.x, gridDim.x); }
 __global__ __launch_bounds__(BLOCK) void kb_clear_grid(WorkList W) {
-    for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *) { body_clear_grid(S, b); });
+    for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *) { body_clear_grid(S, b, tile_grid(S)); });
 }
 
 // Counting-sort pass 1: tiled cell key of each particle and its arrival slot in that cell
@@ -62,6 +68,10 @@ __global__ __launch_bounds__(BLOCK) void kb_clear_grid(WorkList W) {
 __device__ __forceinline__ void body_bin_count(const DevSim &S, uint32_t b) {
     const uint32_t i = b * BLOCK + threadIdx.x;
     const bool valid = i < S.np;
+    if (i == 0) {  // the tile and band lists are rebuilt later in the step (clear_grid has read them)
+        *S.tile_count = 0;
+        *S.band_count = 0;
+    }
     uint32_t key = 0xFFFFFFFFu;
     if (valid) {
         int cx, cy, cz;
@@ -218,6 +228,7 @@ __device__ __forceinline__ void body_scatter(const DevSim &S, uint32_t b) {
     uint32_t i = b * BLOCK + threadIdx.x;
     if (i >= S.np) return;
     uint32_t j = S.offset[S.key[i]] + S.slot[i];
+    if (j >= S.np) return;  // only if the counts were not cleared (a broken invariant): stay in bounds
     S.bx[j] = S.ax[i];
     S.by[j] = S.ay[i];
     S.bz[j] = S.az[i];
@@ -292,10 +303,6 @@ __global__ __launch_bounds__(BLOCK) void kb_band(WorkList W) {
 __global__ __launch_bounds__(BLOCK) void kb_tile_list(WorkList W) {
     for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *) { body_tile_list(S, b); });
 }
-
-// Workgroups of the per-tile kernels, each walking the active tile list with a stride of
-// this count.
-__host__ __device__ inline uint32_t tile_grid(const DevSim &S) { return S.tile_wgs; }
 
 // p2g (fluid.slang:94-112), one active 8^3 tile at a time per workgroup.  The tile's
 // particles are one contiguous run of the cell-sorted scratch copy; their fixed-point splats
@@ -617,7 +624,7 @@ void launch_fill_u32(uint32_t *p, uint32_t n, uint32_t value, hipStream_t s) {
     if (n) PXL_LAUNCH(k_fill_u32, blocks_for(n), BLOCK, 0, s, p, n, value);
 }
 
-void launch_clear_grid(const DevSim &S, hipStream_t s) { PXL_LAUNCH(k_clear_grid, blocks_for(S.nkeys), BLOCK, 0, s, S); }
+void launch_clear_grid(const DevSim &S, hipStream_t s) { PXL_LAUNCH(k_clear_grid, tile_grid(S), BLOCK, 0, s, S); }
 
 void launch_bin_count(const DevSim &S, hipStream_t s) {
     PXL_LAUNCH(k_bin_count, blocks_for(S.np), BLOCK, 0, s, S);
@@ -664,11 +671,12 @@ void launch_reset_particles(const DevSim &S, hipStream_t s) {
 
 // ---- batched
 uint32_t particle_blocks(const DevSim &S) { return blocks_for(S.np); }
-uint32_t key_blocks(const DevSim &S) { return blocks_for(S.nkeys); }
 uint32_t tile_blocks(const DevSim &S) { return blocks_for(S.ntiles); }
 uint32_t tile_grid_blocks(const DevSim &S) { return tile_grid(S); }
 
-void batch_clear_grid(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_clear_grid, batch_grid(W[WD_KEYS], BATCH_GRID), BLOCK, 0, s, W[WD_KEYS]); }
+void batch_clear_grid(const WorkLists &W, hipStream_t s) {
+    PXL_LAUNCH(kb_clear_grid, batch_grid(W[WD_TILEGRID], BATCH_GRID), BLOCK, 0, s, W[WD_TILEGRID]);
+}
 void batch_bin_count(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_bin_count, batch_grid(W[WD_PART], BATCH_GRID), BLOCK, 0, s, W[WD_PART]); }
 void batch_scan_reduce(const WorkLists &W, int which, hipStream_t s) {
     PXL_LAUNCH(kb_scan_reduce, batch_grid(W[which == 0 ? WD_SCAN_KEYS : WD_SCAN_ROWS], BATCH_GRID), BLOCK, 0, s, W[which == 0 ? WD_SCAN_KEYS : WD_SCAN_ROWS], which);

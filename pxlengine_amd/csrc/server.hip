@@ -36,7 +36,6 @@ uint32_t work_blocks(const DevSim &S, int d) {
     switch (d) {
         case WD_SIM:
         case WD_A_SIM: return 1;
-        case WD_KEYS: return key_blocks(S);
         case WD_PART: return particle_blocks(S);
         case WD_SCAN_KEYS: return scan_tiles(S.nkeys);
         case WD_TILES: return tile_blocks(S);
@@ -95,7 +94,7 @@ double share_bytes(const char *k, const DevSim &d, const MemberCounts &m, int l)
     auto cells_of = [&](int q) { return q == 0 ? C : (double)d.mg[q].nx * d.mg[q].ny * d.mg[q].nz; };
     const double cl = l < MG_MAX ? m.fluid[l] : 0, cl1 = l + 1 < MG_MAX ? m.fluid[l + 1] : 0;
     const std::string n(k);
-    if (n == "clear_grid") return 4 * K;
+    if (n == "clear_grid") return 4.0 * TILE_CELLS * t + 4.0 * fl_row_slots(d);  // t: about the last step's tiles
     if (n == "bin_count") return 20 * P + 4 * C;
     const double rows = fl_row_slots(d);
     if (n == "scan_reduce") return 4 * (l == 0 ? K : rows);
