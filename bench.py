@@ -212,7 +212,7 @@ def bench_serve(args, pxl):
             "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
             "busy_fraction": st["busy_s"] / st["span_s"] if st["span_s"] else None,
             "mean_batch": st["mean_batch"], "batch_steps": st["batch_steps"], "admit_s": st["admit_s"],
-            "graph_captures": st["graph_captures"], "capture_s": st["capture_s"], "peak_bytes": st["peak_bytes"], "wall_s": wall,
+            "graph_captures": st["graph_captures"], "capture_s": st["capture_s"], "retire_s": st["retire_s"], "peak_bytes": st["peak_bytes"], "wall_s": wall,
             "failed": len(failed), "max_rel_residual": max_res,
             "plain": plain, "ratio_vs_plain": value / plain["value"],
         },

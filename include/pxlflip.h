@@ -247,6 +247,7 @@ typedef struct pxlflip_server_stats {
     uint64_t rebuilds;       /* work-list rebuilds (batch composition changes) */
     uint64_t graph_captures; /* batch graphs captured */
     double capture_s;        /* device-thread time spent capturing and instantiating them */
+    double retire_s;         /* device-thread time spent retiring requests (copy-out, release) */
 } pxlflip_server_stats;
 
 int pxlflip_server_default_opts(pxlflip_server_opts *opts);

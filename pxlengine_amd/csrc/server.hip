@@ -498,10 +498,10 @@ private:
             levels = std::max(levels, s->d.mg_levels);
             tail = std::max(tail, s->d.mg_tail);
         }
-        // A launch's persistent grid is its list length rounded up to a power of two (at least
-        // 64) and is baked into the graphs; when a list outgrows its grid or shrinks below an
-        // eighth of it, re-size and re-capture (any grid processes every entry, so this only
-        // tunes speed; power-of-two steps keep re-captures rare as members come and go).
+        // A launch's persistent grid is baked into the graphs: twice its list length rounded up
+        // to a power of two (at least 64), grown (and the graphs re-captured) only when a list
+        // outgrows it, never shrunk: any grid processes every entry, so this only tunes speed,
+        // and members coming and going then rarely force a re-capture.
         auto pow2 = [](size_t v) {
             uint32_t g = 64;
             while (g < v) g <<= 1;
@@ -509,9 +509,9 @@ private:
         };
         bool regrid = false;
         for (int d = 0; d < WD_COUNT; d++)
-            if (need[d] > lists_[d].grid || (lists_[d].grid > 64 && 8 * need[d] < lists_[d].grid)) regrid = true;
+            if (need[d] > lists_[d].grid) regrid = true;
         if (regrid)
-            for (int d = 0; d < WD_COUNT; d++) lists_[d].grid = pow2(need[d]);
+            for (int d = 0; d < WD_COUNT; d++) lists_[d].grid = std::max(lists_[d].grid, pow2(2 * need[d]));
         if (relayout || regrid || levels != max_levels_ || tail != max_tail_) drop_graphs();
         rebuilds_++;
         max_levels_ = levels;
@@ -608,23 +608,29 @@ struct pxlflip_server {
     std::deque<pxl::Request *> queue;        // submitted, waiting for admission
     std::deque<pxl::Request *> to_allocate;  // admitted, waiting for an allocator thread
     std::vector<pxl::Request *> running;     // admitted and not finished (allocating or stepping)
+    // simulations of finished requests, freed by the release thread (freeing device memory
+    // waits for the device), with the budget each returns
+    std::deque<std::pair<std::unique_ptr<pxl::Sim>, uint64_t>> to_free;
+    size_t freeing = 0;  // in to_free or being freed
     std::map<uint64_t, std::unique_ptr<pxl::Request>> all;
     uint64_t next_id = 1;
     bool stop = false;
     std::thread worker;
     std::vector<std::thread> allocators;  // allocate + seed admitted requests off the device thread
+    std::thread releaser;                 // frees finished requests' simulations
     std::string error;
     // accounting
     uint64_t bytes_in_use = 0, peak_bytes = 0;
     uint64_t submitted = 0, completed = 0, failed = 0, cancelled = 0;
     std::vector<double> latencies_ms;
-    double particle_steps = 0, first_submit = -1, last_done = -1, busy_s = 0;
+    double particle_steps = 0, first_submit = -1, last_done = -1, busy_s = 0, retire_s = 0;
     uint64_t batch_steps = 0;
     double batch_members = 0;
     double admit_s = 0;  // allocator-thread time spent allocating admitted requests
 
     void run();
     void allocate_loop();
+    void free_loop();
     void finish(pxl::Request *r, int status, const std::string &err);
 };
 
@@ -663,6 +669,30 @@ void pxlflip_server::finish(pxl::Request *r, int status, const std::string &err)
         cancelled++;
     }
     done_cv.notify_all();
+}
+
+// Frees the simulations of finished requests and returns their budget.  Freeing device
+// memory waits for the device, so this runs on its own thread, not the device thread.
+void pxlflip_server::free_loop() {
+    using namespace pxl;
+    try {
+        HIP_CHECK(hipSetDevice(opts.device));
+    } catch (const HipError &) {
+    }
+    std::unique_lock<std::mutex> lk(mu);
+    while (true) {
+        cv.wait(lk, [&] { return !to_free.empty() || stop; });
+        if (to_free.empty()) break;  // stop, and nothing left to free
+        auto item = std::move(to_free.front());
+        to_free.pop_front();
+        lk.unlock();
+        item.first.reset();
+        lk.lock();
+        bytes_in_use -= item.second;
+        freeing--;
+        cv.notify_all();       // the returned budget may admit a queued request
+        done_cv.notify_all();  // drain waits for the memory too
+    }
 }
 
 void pxlflip_server::allocate_loop() {
@@ -772,6 +802,7 @@ void pxlflip_server::run() {
             busy_s += t1 - t0;
         }
         // retire finished, cancelled or failed requests at the step boundary
+        const double t_retire = now_s();
         std::vector<Request *> keep;
         for (Request *r : running) {
             if (!r->allocated) {  // still with an allocator thread
@@ -794,6 +825,8 @@ void pxlflip_server::run() {
                 keep.push_back(r);
                 continue;
             }
+            // leaves the batch now (the batch re-lays out its lists before its next launch);
+            // kept particles are copied out here, the release thread frees the simulation
             lk.unlock();
             std::string err = step_error;
             try {
@@ -810,12 +843,14 @@ void pxlflip_server::run() {
             } catch (const HipError &e) {
                 err = std::string("HIP error in ") + e.what + ": " + hipGetErrorString(e.code);
             }
-            r->sim.reset();
             lk.lock();
-            bytes_in_use -= r->bytes;
+            to_free.emplace_back(std::move(r->sim), r->bytes);
+            freeing++;
+            cv.notify_all();
             finish(r, !err.empty() ? PXLFLIP_REQ_FAILED : (done ? PXLFLIP_REQ_DONE : PXLFLIP_REQ_CANCELLED), err);
         }
         running.swap(keep);
+        retire_s += now_s() - t_retire;
     }
     // stop: wait for the allocators to leave, then cancel what is left
     lk.unlock();
@@ -940,6 +975,7 @@ int pxlflip_server_create(const pxlflip_server_opts *opts, pxlflip_server **out)
     }
     pxlflip_server *p = srv.release();
     for (int k = 0; k < 2; k++) p->allocators.emplace_back([p] { p->allocate_loop(); });
+    p->releaser = std::thread([p] { p->free_loop(); });
     p->worker = std::thread([p] { p->run(); });
     *out = p;
     return PXLFLIP_OK;
@@ -1019,7 +1055,7 @@ int pxlflip_server_wait(pxlflip_server *srv, uint64_t id, double timeout_s, pxlf
 int pxlflip_server_drain(pxlflip_server *srv, double timeout_s) {
     if (!srv) return PXLFLIP_ERR_INVALID;
     std::unique_lock<std::mutex> lk(srv->mu);
-    auto idle = [&] { return srv->queue.empty() && srv->running.empty(); };
+    auto idle = [&] { return srv->queue.empty() && srv->running.empty() && srv->freeing == 0; };
     bool ok = true;
     if (timeout_s < 0) srv->done_cv.wait(lk, idle);
     else ok = srv->done_cv.wait_for(lk, std::chrono::duration<double>(timeout_s), idle);
@@ -1111,6 +1147,7 @@ int pxlflip_server_get_stats(pxlflip_server *srv, pxlflip_server_stats *st) {
     st->rebuilds = srv->batch ? srv->batch->rebuilds() : 0;
     st->graph_captures = srv->batch ? srv->batch->graph_captures() : 0;
     st->capture_s = srv->batch ? srv->batch->capture_seconds() : 0.0;
+    st->retire_s = srv->retire_s;
     return PXLFLIP_OK;
 }
 
@@ -1127,6 +1164,11 @@ void pxlflip_server_destroy(pxlflip_server *srv) {
         srv->cv.notify_all();
     }
     if (srv->worker.joinable()) srv->worker.join();
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->cv.notify_all();
+    }
+    if (srv->releaser.joinable()) srv->releaser.join();  // frees what is left, then leaves
     (void)hipSetDevice(srv->opts.device);
     srv->batch.reset();
     delete srv;

@@ -115,6 +115,7 @@ class ServerStats(C.Structure):
         ("busy_s", C.c_double), ("batch_steps", C.c_uint64), ("mean_batch", C.c_double),
         ("bytes_in_use", C.c_uint64), ("peak_bytes", C.c_uint64), ("admit_s", C.c_double),
         ("rebuilds", C.c_uint64), ("graph_captures", C.c_uint64), ("capture_s", C.c_double),
+        ("retire_s", C.c_double),
     ]
 
     def as_dict(self) -> dict:
