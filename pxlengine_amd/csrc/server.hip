@@ -598,6 +598,10 @@ struct Request {
 }  // namespace
 }  // namespace pxl
 
+// Host threads that seed and allocate admitted requests (seeding a large liquid is several
+// milliseconds of host work).
+constexpr int ALLOCATOR_THREADS = 4;
+
 struct pxlflip_server {
     pxlflip_server_opts opts{};
     pxlflip_opts sim_opts{};
@@ -974,7 +978,7 @@ int pxlflip_server_create(const pxlflip_server_opts *opts, pxlflip_server **out)
         return PXLFLIP_ERR_OOM;
     }
     pxlflip_server *p = srv.release();
-    for (int k = 0; k < 2; k++) p->allocators.emplace_back([p] { p->allocate_loop(); });
+    for (int k = 0; k < ALLOCATOR_THREADS; k++) p->allocators.emplace_back([p] { p->allocate_loop(); });
     p->releaser = std::thread([p] { p->free_loop(); });
     p->worker = std::thread([p] { p->run(); });
     *out = p;
