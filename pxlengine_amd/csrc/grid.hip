@@ -17,8 +17,8 @@ constexpr int TX = 64, TY = 4;        // full-grid read-back launches
 constexpr int GBLOCK = 256;           // band-walking workgroups
 constexpr unsigned BATCH_GRID = 2048;  // persistent grid of batched launches
 
-// Workgroups of the band-walking launches of one simulation (each takes band tiles with this
-// stride), the same count as the per-tile particle kernels.
+// Workgroups of the band-walking launches of one simulation (each takes one contiguous run of
+// the band list), the same count as the per-tile particle kernels.
 __host__ __device__ inline uint32_t band_grid(const DevSim &S) { return S.tile_wgs; }
 
 // Runs f(i, j, k) over tile t's nodes (faces are stored at the node of their low corner: the
@@ -42,11 +42,14 @@ __device__ __forceinline__ void for_tile_box(const DevSim &S, uint32_t t, bool n
         f(i, j, k);
     }
 }
-// Band tiles b, b + nb, ... of simulation S.
+// Workgroup b's (of nb) contiguous run of simulation S's band list: x-neighbouring tiles are
+// mostly adjacent in the list, so one workgroup writes the rest of a cache line row soon after
+// the first part, while it is still in L2.
 template <typename F>
 __device__ __forceinline__ void for_band(const DevSim &S, uint32_t b, uint32_t nb, F &&f) {
     const uint32_t n = *S.band_count;
-    for (uint32_t u = b; u < n; u += nb) f(S.band_list[u]);
+    const uint32_t beg = (uint32_t)((uint64_t)n * b / nb), end = (uint32_t)((uint64_t)n * (b + 1) / nb);
+    for (uint32_t u = beg; u < end; u++) f(S.band_list[u]);
 }
 
 // SOLID for cells outside the grid or on its one-cell wall layer (fluid.slang:55-58, 126-128).

@@ -278,23 +278,31 @@ __device__ __forceinline__ void body_tile_list(const DevSim &S, uint32_t b) {
 }
 __global__ __launch_bounds__(BLOCK) void k_tile_list(DevSim S) { body_tile_list(S, blockIdx.x); }
 
-// Band: tiles with an active tile (this step or the last) among their 3x3x3 neighbours.
+// Band: tiles with an active tile (this step or the last) among their 3x3x3 neighbours.  A
+// wave appends its band tiles in tile order with one atomic, so x-neighbouring tiles are
+// mostly adjacent in the list (the band passes take contiguous runs of it).
 __device__ __forceinline__ void body_band(const DevSim &S, uint32_t b) {
     const uint32_t t = b * BLOCK + threadIdx.x;
-    if (t >= S.ntiles) return;
-    int ti, tj, tk;
-    ti = (int)(t % (uint32_t)S.ntx);
-    tj = (int)((t / (uint32_t)S.ntx) % (uint32_t)S.nty);
-    tk = (int)(t / ((uint32_t)S.ntx * (uint32_t)S.nty));
     uint32_t any = 0;
-    for (int dz = -1; dz <= 1; dz++)
-        for (int dy = -1; dy <= 1; dy++)
-            for (int dx = -1; dx <= 1; dx++) {
-                const int x = ti + dx, y = tj + dy, z = tk + dz;
-                if (x < 0 || y < 0 || z < 0 || x >= S.ntx || y >= S.nty || z >= S.ntz) continue;
-                any |= S.tile_act[(uint32_t)x + (uint32_t)S.ntx * ((uint32_t)y + (uint32_t)S.nty * (uint32_t)z)];
-            }
-    if (any) S.band_list[atomicAdd(S.band_count, 1u)] = t;
+    if (t < S.ntiles) {
+        const int ti = (int)(t % (uint32_t)S.ntx), tj = (int)((t / (uint32_t)S.ntx) % (uint32_t)S.nty),
+                  tk = (int)(t / ((uint32_t)S.ntx * (uint32_t)S.nty));
+        for (int dz = -1; dz <= 1; dz++)
+            for (int dy = -1; dy <= 1; dy++)
+                for (int dx = -1; dx <= 1; dx++) {
+                    const int x = ti + dx, y = tj + dy, z = tk + dz;
+                    if (x < 0 || y < 0 || z < 0 || x >= S.ntx || y >= S.nty || z >= S.ntz) continue;
+                    any |= S.tile_act[(uint32_t)x + (uint32_t)S.ntx * ((uint32_t)y + (uint32_t)S.nty * (uint32_t)z)];
+                }
+    }
+    const int lane = threadIdx.x & (WAVE - 1);
+    const uint64_t mask = __ballot(any != 0);
+    if (!mask) return;
+    const int leader = __ffsll((long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(S.band_count, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader, WAVE);
+    if (any) S.band_list[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = t;
 }
 __global__ __launch_bounds__(BLOCK) void k_band(DevSim S) { body_band(S, blockIdx.x); }
 __global__ __launch_bounds__(BLOCK) void kb_band(WorkList W) {
