@@ -97,7 +97,6 @@ __device__ __forceinline__ int64_t gather_face_in(const DevSim &S, uint32_t act,
         u[a] = q[a] - TILE * t[a];
         alt[a] = u[a] == 0 ? -1 : ((a != c && u[a] == TILE - 1) ? 1 : 0);
     }
-    const int d0 = c == 0 ? TILE + 1 : TILE + 2, d1 = c == 1 ? TILE + 1 : TILE + 2;
     const uint32_t t_lin = (uint32_t)t[0] + (uint32_t)S.ntx * ((uint32_t)t[1] + (uint32_t)S.nty * (uint32_t)t[2]);
     int64_t sum = 0;
 #pragma unroll
@@ -109,7 +108,7 @@ __device__ __forceinline__ int64_t gather_face_in(const DevSim &S, uint32_t act,
         // along the axes other than c)
         const int lx = u[0] - TILE * dx + (c != 0), ly = u[1] - TILE * dy + (c != 1), lz = u[2] - TILE * dz + (c != 2);
         const uint32_t tn = t_lin + (uint32_t)(dx + S.ntx * (dy + S.nty * dz));
-        sum += S.box[(size_t)tn * BOX_ENTRIES + (size_t)c * BOX_FACES + (size_t)(lx + d0 * (ly + d1 * lz))];
+        sum += S.box[(size_t)tn * BOX_ENTRIES + (size_t)box_gidx(c, lx, ly, lz)];
     }
     return sum;
 }
@@ -136,17 +135,18 @@ __device__ __forceinline__ void faces_at(const DevSim &S, uint32_t act, const in
     if (has[1]) face_update(S, sum[1], vidx(S, i, j, k), S.gy, in_c && !is_solid(S, i, j - 1, k));
     if (has[2]) face_update(S, sum[2], widx(S, i, j, k), S.gz, in_c && !is_solid(S, i, j, k - 1));
 }
+__device__ __forceinline__ void faces_tile(const DevSim &S, uint32_t tl) {
+    const int t[3] = {(int)(tl % (uint32_t)S.ntx), (int)((tl / (uint32_t)S.ntx) % (uint32_t)S.nty),
+                      (int)(tl / ((uint32_t)S.ntx * (uint32_t)S.nty))};
+    const uint32_t act = active_neighbours(S, t[0], t[1], t[2]);
+    for_tile_box(S, tl, true, [&](int i, int j, int k) { faces_at(S, act, t, i, j, k); });
+}
 __device__ __forceinline__ void body_faces(const DevSim &S, uint32_t b, uint32_t nb) {
-    for_band(S, b, nb, [&](uint32_t tl) {
-        const int t[3] = {(int)(tl % (uint32_t)S.ntx), (int)((tl / (uint32_t)S.ntx) % (uint32_t)S.nty),
-                          (int)(tl / ((uint32_t)S.ntx * (uint32_t)S.nty))};
-        const uint32_t act = active_neighbours(S, t[0], t[1], t[2]);
-        for_tile_box(S, tl, true, [&](int i, int j, int k) { faces_at(S, act, t, i, j, k); });
-    });
+    for_band(S, b, nb, [&](uint32_t tl) { faces_tile(S, tl); });
 }
 __global__ __launch_bounds__(GBLOCK) void k_faces(DevSim S) { body_faces(S, blockIdx.x, gridDim.x); }
 __global__ __launch_bounds__(GBLOCK) void kb_faces(WorkList W) {
-    for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *) { body_faces(S, b, band_grid(S)); });
+    for_batch_band(W, [&](const DevSim &S, uint32_t t) { faces_tile(S, t); });
 }
 
 // The P2G sums as the reference's acc / wacc face buffers (for read-back).
@@ -240,7 +240,7 @@ __device__ __forceinline__ void body_cells(const DevSim &S, uint32_t b, uint32_t
 }
 __global__ __launch_bounds__(GBLOCK) void k_cells(DevSim S) { body_cells(S, blockIdx.x, gridDim.x); }
 __global__ __launch_bounds__(GBLOCK) void kb_cells(WorkList W) {
-    for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *) { body_cells(S, b, band_grid(S)); });
+    for_batch_band(W, [&](const DevSim &S, uint32_t t) { cells_tile(S, t); });
 }
 
 // Fluid list from the scanned row counts: one wave per grid row writes the row's FLUID cells
@@ -325,12 +325,15 @@ __device__ __forceinline__ void project_at(const DevSim &S, int i, int j, int k)
     if (i < S.nx && k < S.nz) project_face(S, vidx(S, i, j, k), i, j - 1, k, i, j, k);
     if (i < S.nx && j < S.ny) project_face(S, widx(S, i, j, k), i, j, k - 1, i, j, k);
 }
+__device__ __forceinline__ void project_tile(const DevSim &S, uint32_t t) {
+    for_tile_box(S, t, true, [&](int i, int j, int k) { project_at(S, i, j, k); });
+}
 __device__ __forceinline__ void body_project(const DevSim &S, uint32_t b, uint32_t nb) {
-    for_band(S, b, nb, [&](uint32_t t) { for_tile_box(S, t, true, [&](int i, int j, int k) { project_at(S, i, j, k); }); });
+    for_band(S, b, nb, [&](uint32_t t) { project_tile(S, t); });
 }
 __global__ __launch_bounds__(GBLOCK) void k_project(DevSim S) { body_project(S, blockIdx.x, gridDim.x); }
 __global__ __launch_bounds__(GBLOCK) void kb_project(WorkList W) {
-    for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *) { body_project(S, b, band_grid(S)); });
+    for_batch_band(W, [&](const DevSim &S, uint32_t t) { project_tile(S, t); });
 }
 
 __global__ __launch_bounds__(TX *TY) void k_type_u32(DevSim S, uint32_t *out) {

@@ -92,6 +92,10 @@ struct DevSim {
     // and cells outside it hold values that do not change from step to step (no particle
     // weight, no FLUID cell near), so the face, cell and projection passes walk only the band.
     uint32_t *band_list, *band_count;
+    // in a batch: the batch-wide band list ((member << WORK_SIM_SHIFT) | tile, every member's
+    // band tiles) and its count, and this simulation's member index; null alone
+    uint32_t *band_all, *band_all_count;
+    uint32_t member;
     // cells (count/offset: tiled key order, offset[nkeys] = np)
     uint32_t *count, *offset;
     uint8_t *type;
@@ -144,6 +148,21 @@ __device__ __forceinline__ void tile_box(int c, int ti, int tj, int tk, int lo[3
     d[c] = TILE + 1;
 }
 
+// Where face (lx, ly, lz) of component c's box (box coordinates, tile_box dims) is stored in
+// HBM (S.box).  The box's own columns along x (lx 0..7 for c = 0, 1..8 otherwise) come first,
+// [lz][ly][8 x]; then its x-halo columns (lx 8 for c = 0; lx 0 and 9 otherwise), each
+// [lz][ly] contiguous.  The face pass of the x-neighbour tile reads exactly one halo column of
+// this box, so it reads it as one contiguous run instead of one 8-byte entry per 128-byte
+// line.  (LDS keeps the plain [lz][ly][lx] order; P2G permutes when it stores.)
+__host__ __device__ inline int box_gidx(int c, int lx, int ly, int lz) {
+    const int d1 = c == 1 ? TILE + 1 : TILE + 2, d2 = c == 2 ? TILE + 1 : TILE + 2;
+    const int i0 = c == 0 ? 0 : 1;
+    const int yz = ly + d1 * lz;
+    if (lx >= i0 && lx < i0 + TILE) return c * BOX_FACES + (lx - i0) + TILE * yz;
+    const int h = (c == 0 || lx == 0) ? 0 : 1;
+    return c * BOX_FACES + TILE * d1 * d2 + h * d1 * d2 + yz;
+}
+
 __device__ __forceinline__ bool tile_active(const DevSim &S, uint32_t t) { return (S.tile_act[t] & 1u) != 0; }
 
 // Packed P2G sum of face q of component c: the sum over the (1 to 8) boxes of active tiles
@@ -169,8 +188,7 @@ __device__ __forceinline__ int64_t gather_face(const DevSim &S, int c, int qx, i
                 if (!tile_active(S, t)) continue;
                 int lo[3], d[3];
                 tile_box(c, tx, ty, tz, lo, d);
-                const int li = (qx - lo[0]) + d[0] * ((qy - lo[1]) + d[1] * (qz - lo[2]));
-                sum += S.box[(size_t)t * BOX_ENTRIES + (size_t)c * BOX_FACES + li];
+                sum += S.box[(size_t)t * BOX_ENTRIES + (size_t)box_gidx(c, qx - lo[0], qy - lo[1], qz - lo[2])];
             }
     return sum;
 }
@@ -282,6 +300,7 @@ struct WorkList {
     const uint32_t *entries;
     const uint32_t *count;  // device word: number of entries
     uint32_t *slots;        // the batch's list counts (COUNT_SLOTS per simulation)
+    const uint32_t *band, *band_count;  // the batch-wide band list (DevSim::band_all) and its count
     uint32_t grid;          // workgroups a batched launch over this list uses (host; 0 = the kernel's maximum)
 };
 // Persistent grid of a batched launch: the list's grid, at most the kernel's maximum and
@@ -304,6 +323,18 @@ __device__ __forceinline__ void for_each_work(const WorkList &W, F &&f) {
         f(W.sims[sim], w & WORK_BLOCK_MASK, W.slots + (size_t)sim * COUNT_SLOTS);
         e += gridDim.x;
         if (e < n) w = W.entries[e];
+    }
+}
+
+// Runs f(S, tile) for the batch's band tiles, one tile per workgroup iteration (the band kernel
+// appends every member's band tiles to one batch-wide list, so the band passes balance over
+// tiles, not over members' workgroup counts).
+template <typename F>
+__device__ __forceinline__ void for_batch_band(const WorkList &W, F &&f) {
+    const uint32_t n = *W.band_count;
+    for (uint32_t e = blockIdx.x; e < n; e += gridDim.x) {
+        const uint32_t w = W.band[e];
+        f(W.sims[w >> WORK_SIM_SHIFT], w & WORK_BLOCK_MASK);
     }
 }
 

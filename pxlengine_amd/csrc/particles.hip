@@ -71,6 +71,7 @@ __device__ __forceinline__ void body_bin_count(const DevSim &S, uint32_t b) {
     if (i == 0) {  // the tile and band lists are rebuilt later in the step (clear_grid has read them)
         *S.tile_count = 0;
         *S.band_count = 0;
+        if (S.band_all && S.member == 0) *S.band_all_count = 0;
     }
     uint32_t key = 0xFFFFFFFFu;
     if (valid) {
@@ -299,10 +300,18 @@ __device__ __forceinline__ void body_band(const DevSim &S, uint32_t b) {
     const uint64_t mask = __ballot(any != 0);
     if (!mask) return;
     const int leader = __ffsll((long long)mask) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(S.band_count, (uint32_t)__popcll(mask));
+    const uint32_t rank = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    uint32_t base = 0, all = 0;
+    if (lane == leader) {
+        base = atomicAdd(S.band_count, (uint32_t)__popcll(mask));
+        if (S.band_all) all = atomicAdd(S.band_all_count, (uint32_t)__popcll(mask));
+    }
     base = __shfl(base, leader, WAVE);
-    if (any) S.band_list[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = t;
+    all = __shfl(all, leader, WAVE);
+    if (any) {
+        S.band_list[base + rank] = t;
+        if (S.band_all) S.band_all[all + rank] = (S.member << WORK_SIM_SHIFT) | t;
+    }
 }
 __global__ __launch_bounds__(BLOCK) void k_band(DevSim S) { body_band(S, blockIdx.x); }
 __global__ __launch_bounds__(BLOCK) void kb_band(WorkList W) {
@@ -378,8 +387,23 @@ __device__ __forceinline__ void body_p2g(const DevSim &S, uint32_t b0, uint32_t 
             }
         }
         __syncthreads();
+        // stored in the HBM box order (box_gidx): entry g of the store comes from LDS entry
+        // c * BOX_FACES + lx + d0 (ly + d1 lz)
         int64_t *out = S.box + (size_t)t * BOX_ENTRIES;
-        for (int q = threadIdx.x; q < BOX_ENTRIES; q += TILE_BLOCK) out[q] = (int64_t)lbox[q];
+        for (int g = threadIdx.x; g < BOX_ENTRIES; g += TILE_BLOCK) {
+            const int c = g / BOX_FACES, r = g - c * BOX_FACES;
+            const int d0 = box_d0(c), d1 = box_d1(c), d2 = c == 2 ? TILE + 1 : TILE + 2;
+            int lx, yz;
+            if (r < TILE * d1 * d2) {
+                lx = (c == 0 ? 0 : 1) + r % TILE;
+                yz = r / TILE;
+            } else {
+                const int h = (r - TILE * d1 * d2) / (d1 * d2);
+                lx = c == 0 ? TILE : (h ? TILE + 1 : 0);
+                yz = (r - TILE * d1 * d2) - h * d1 * d2;
+            }
+            out[g] = (int64_t)lbox[c * BOX_FACES + lx + d0 * yz];
+        }
         __syncthreads();
     }
 }

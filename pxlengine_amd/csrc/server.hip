@@ -135,13 +135,15 @@ public:
         HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
         dev_alloc(&d_counts_, WD_COUNT);
         host_alloc(&h_counts_, WD_COUNT);
+        dev_alloc(&d_band_count_, 16);
         std::fill(cap_, cap_ + WD_COUNT, 0);
         std::fill(off_, off_ + WD_COUNT, 0);
     }
     ~Batch() {
         if (stream_) (void)hipStreamSynchronize(stream_);
         drop_graphs();
-        for (void *p : {(void *)d_sims_, (void *)d_entries_, (void *)d_counts_, (void *)d_norms_, (void *)d_slots_})
+        for (void *p : {(void *)d_sims_, (void *)d_entries_, (void *)d_counts_, (void *)d_norms_, (void *)d_slots_,
+                        (void *)d_band_, (void *)d_band_count_})
             if (p) (void)hipFree(p);
         for (void *p : {(void *)h_sims_, (void *)h_entries_, (void *)h_counts_, (void *)h_norms_})
             if (p) (void)hipHostFree(p);
@@ -476,6 +478,14 @@ private:
             for (Sim *s : members_) need[d] += work_blocks(s->d, d);
             if (need[d] > cap_[d]) relayout = true;
         }
+        // the batch-wide band list holds at most every member's tiles
+        size_t tiles = 0;
+        for (Sim *s : members_) tiles += s->d.ntiles;
+        if (tiles > band_cap_) {
+            band_cap_ = std::max<size_t>(2 * tiles, 65536);
+            dev_alloc(&d_band_, band_cap_);
+            relayout = true;
+        }
         if (relayout) {
             size_t total = 0;
             for (int d = 0; d < WD_COUNT; d++) {
@@ -488,7 +498,8 @@ private:
             dev_alloc(&d_entries_, total);
             host_alloc(&h_entries_, total);
             for (int d = 0; d < WD_COUNT; d++)
-                lists_[d] = WorkList{d_sims_, d_entries_ + off_[d], d_counts_ + d, d_slots_, lists_[d].grid};
+                lists_[d] = WorkList{d_sims_, d_entries_ + off_[d], d_counts_ + d, d_slots_, d_band_, d_band_count_,
+                                     lists_[d].grid};
         }
         // the graphs launch the level kernels of the deepest hierarchy seen so far (levels a
         // member does not have get empty work lists), so members leaving do not force a
@@ -522,6 +533,9 @@ private:
             h_sims_[i] = members_[i]->d;
             h_sims_[i].norms = d_norms_ + 2 * i;
             uint32_t *slot = d_slots_ + i * COUNT_SLOTS;
+            h_sims_[i].band_all = d_band_;
+            h_sims_[i].band_all_count = d_band_count_;
+            h_sims_[i].member = (uint32_t)i;
             h_sims_[i].fl_count = slot;
             for (int l = 1; l < MG_MAX; l++) h_sims_[i].mg[l].count = slot + 4 * l;
         }
@@ -548,6 +562,8 @@ private:
     uint32_t *d_counts_ = nullptr, *h_counts_ = nullptr;
     double *d_norms_ = nullptr, *h_norms_ = nullptr;
     uint32_t *d_slots_ = nullptr;  // list counts, COUNT_SLOTS per member
+    uint32_t *d_band_ = nullptr, *d_band_count_ = nullptr;  // the batch-wide band list
+    size_t band_cap_ = 0;
     size_t off_[WD_COUNT], cap_[WD_COUNT];
     WorkLists lists_{};
     int max_levels_ = 0, max_tail_ = 0;
