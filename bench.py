@@ -26,7 +26,8 @@ algorithmic bytes over the simulations each covers).  `cpu_baseline` times the C
 (single thread) on a bounded sample of the stream's requests.
 
 `--workload dam` measures the single-instance 128^3 dam break on the plain path instead;
-`--workload batch` only the fixed batch used for the roofline (for profiling runs).
+`--workload batch` only the fixed batch used for the roofline (for profiling runs);
+`--workload mixed256` the 256^3 two-block mixed-precision configuration (~58M particles).
 """
 from __future__ import annotations
 
@@ -113,6 +114,43 @@ def bench_dam(args, pxl):
                      "frac": ach / PEAK_HBM_GBS, "traffic": measured_traffic(dom["kernel"], "dam128_cg80"),
                      "kernel": dom["kernel"], "avg_us": dom["avg_us"], "alg_bytes_per_launch": dom["alg_bytes"]},
         "cpu_baseline": cpu, "kernels": kernels,
+    }
+
+
+def bench_mixed256(args, pxl):
+    """The mixed-precision configuration: 256^3 grid (above the reference's 16M-cell limit, an
+    extension enabled with max_cells), two liquid blocks colliding, ~58M particles, MIXED solve
+    (FP32 multigrid CG inside FP64 refinement) to 1e-6 every step.  Reports particle-steps/s,
+    the FP64 relative residual and refinement passes of every step, and the per-phase times
+    of two profiled steps."""
+    from tests.scenes import FPS, two_blocks
+    fluid = two_blocks(256)
+    sim = pxl.FluidState(fluid, FPS, seed=0, solver="mixed", tolerance=1e-6, max_cells=1 << 25)
+    P = sim.particle_count()
+    for _ in range(args.warmup):
+        sim.step(1)
+    res, outer = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sim.step(1)
+        st = sim.last_solve()
+        res.append(st["rel_residual"])
+        outer.append(st["outer_iterations"])
+    elapsed = time.perf_counter() - t0
+    prof = sim.profile_steps(2)
+    sim.close()
+    return {
+        "metric": METRIC, "value": P * args.steps / elapsed, "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (two colliding liquid blocks, reference LCG seeding)",
+        "config": {"workload": "256^3 grid, two blocks at 8 particles/cell colliding at 1.5 m/s, single instance, "
+                               "MIXED solve to 1e-6 every step",
+                   "particles": P, "max_rel_residual": max(res), "outer_passes": {str(k): outer.count(k) for k in sorted(set(outer))},
+                   "residual_per_step": [float(f"{r:.3g}") for r in res]},
+        "roofline": None, "cpu_baseline": None,
+        "phases": [{"phase": name, "launches_per_step": launches // 2, "ms_per_step": total_ms / 2}
+                   for name, launches, total_ms in prof],
     }
 
 
@@ -295,7 +333,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=64,
                     help="warm-up requests: the batch whose service rate sets the arrival rate (dam: warm-up steps)")
     ap.add_argument("--calib-steps", type=int, default=10, help="timed steps of the warm-up batch")
-    ap.add_argument("--workload", default="serve", choices=["serve", "dam", "batch"])
+    ap.add_argument("--workload", default="serve", choices=["serve", "dam", "batch", "mixed256"])
     ap.add_argument("--requests", type=int, default=512)
     ap.add_argument("--sizes", default="32,48,64,96,128")
     ap.add_argument("--rate", type=float, default=0.0, help="arrivals per second (0 = measured service rate)")
@@ -312,7 +350,7 @@ def main():
         sys.exit(2)
 
     import pxlengine_amd as pxl
-    run = {"serve": bench_serve, "dam": bench_dam, "batch": bench_batch}[args.workload]
+    run = {"serve": bench_serve, "dam": bench_dam, "batch": bench_batch, "mixed256": bench_mixed256}[args.workload]
     line = run(args, pxl)
     print(json.dumps(line))
 

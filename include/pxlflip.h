@@ -58,6 +58,9 @@ typedef struct pxlflip_opts {
     uint32_t scene_index;     /* i in "fluids[i] 'name': ..." validation messages */
     int32_t use_graph;        /* 1 = replay each step as a captured hipGraph (default 1) */
     uint32_t solver_workgroups; /* workgroups of the pressure-solver launches; 0 = default (256), max 4096 */
+    uint32_t max_cells;       /* grid cell limit of the validation: 0 = the reference's 16,000,000
+                                 (crates/pxl-scene/src/lib.rs:732); larger admits bigger grids up to the
+                                 resolution limit, 256^3 (extension) */
 } pxlflip_opts;
 
 /* Fills *opts with the defaults above. */

@@ -78,7 +78,7 @@ int prepare(const char *fluid_json, double fps, uint32_t seed, const pxlflip_opt
     }
     FluidSpec spec;
     std::string err = parse_fluid(fluid_json, spec);
-    if (err.empty()) err = validate_fluid(spec, o.scene_index);
+    if (err.empty()) err = validate_fluid(spec, o.scene_index, o.max_cells ? (uint64_t)o.max_cells : REFERENCE_MAX_CELLS);
     if (!err.empty()) {
         g_create_error = err;
         return PXLFLIP_ERR_INVALID;

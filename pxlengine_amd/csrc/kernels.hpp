@@ -254,11 +254,11 @@ void launch_mg_types(const DevSim &S, hipStream_t s);
 void launch_mg_vcycle(const DevSim &S, const float *r, float *u, hipStream_t s);  // u = M r
 void launch_mgcg_start(const DevSim &S, float *x, hipStream_t s);
 void launch_mgcg_iter(const DevSim &S, float *x, uint32_t it, bool last, hipStream_t s);
-// MG-CG iterations of refinement pass `pass`: `inner` in the first pass, a quarter as many (at
-// least 2) in later ones, which start from the FP32 floor of the first and only need to clear
-// the last digits.
+// MG-CG iterations of refinement pass `pass`: `inner` in the first pass; 2 in the second, which
+// starts from the first pass's FP32 floor and usually only clears the last digits; 2 more in
+// each pass after that (at most `inner`), for the hard steps that need them.
 inline uint32_t pass_iterations(uint32_t inner, uint32_t pass) {
-    return pass == 0 ? inner : (inner + 3) / 4 > 2 ? (inner + 3) / 4 : 2;
+    return pass == 0 ? inner : (2 * pass < inner ? 2 * pass : inner);
 }
 // FP64 refinement (solver.hip); init/update also total the norms into scal[SC_RR / SC_BB].
 void launch_mixed_init(const DevSim &S, double *x64, hipStream_t s);  // x64 = p, r64 = b - A x64

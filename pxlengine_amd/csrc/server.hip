@@ -208,7 +208,7 @@ public:
             active = next;
             const int cur = (int)(pass & 1);
             const uint32_t inner = pass_iterations(opts_.inner_iterations, pass);
-            replay(&g_inner_[pass > 0 ? 1 : 0][cur], [&] { enqueue_inner_pass(cur, inner); });
+            replay(&g_inner_[inner][cur], [&] { enqueue_inner_pass(cur, inner); });
             read_norms();
             next.clear();
             for (uint32_t i : active) {
@@ -402,10 +402,14 @@ private:
     }
 
     void drop_graphs() {
-        for (hipGraphExec_t *g : {&g_grid_, &g_inner_[0][0], &g_inner_[0][1], &g_inner_[1][0], &g_inner_[1][1], &g_post_}) {
+        for (hipGraphExec_t *g : {&g_grid_, &g_post_}) {
             if (*g) (void)hipGraphExecDestroy(*g);
             *g = nullptr;
         }
+        for (auto &e : g_inner_)
+            for (hipGraphExec_t g : e.second)
+                if (g) (void)hipGraphExecDestroy(g);
+        g_inner_.clear();
     }
 
     template <typename F>
@@ -567,7 +571,8 @@ private:
     size_t off_[WD_COUNT], cap_[WD_COUNT];
     WorkLists lists_{};
     int max_levels_ = 0, max_tail_ = 0;
-    hipGraphExec_t g_grid_ = nullptr, g_inner_[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}}, g_post_ = nullptr;
+    hipGraphExec_t g_grid_ = nullptr, g_post_ = nullptr;
+    std::map<uint32_t, std::array<hipGraphExec_t, 2>> g_inner_;  // [MG-CG iterations][x64 parity]
     uint64_t captures_ = 0, rebuilds_ = 0;
     double capture_s_ = 0.0;
     bool profiling_ = false;

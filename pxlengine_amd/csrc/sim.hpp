@@ -6,7 +6,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <array>
 #include <cstring>
+#include <map>
 #include <new>
 #include <string>
 #include <vector>
@@ -101,7 +103,7 @@ struct Sim {
     uint64_t steps_done = 0;
     bool initialized = false;
     hipGraphExec_t step_graph = nullptr;
-    hipGraphExec_t inner_graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [later pass][x64 parity]
+    std::map<uint32_t, std::array<hipGraphExec_t, 2>> inner_graph;  // [MG-CG iterations][x64 parity]
     hipGraphExec_t grid_graph = nullptr, post_graph = nullptr;  // MIXED: phases around the solve
     double *host_scal = nullptr;  // pinned
     int x64_cur = 0;
@@ -113,11 +115,14 @@ struct Sim {
     void drop_graphs() {
         if (step_graph) (void)hipGraphExecDestroy(step_graph);
         step_graph = nullptr;
-        for (hipGraphExec_t *g : {&inner_graph[0][0], &inner_graph[0][1], &inner_graph[1][0], &inner_graph[1][1],
-                                  &grid_graph, &post_graph}) {
+        for (hipGraphExec_t *g : {&grid_graph, &post_graph}) {
             if (*g) (void)hipGraphExecDestroy(*g);
             *g = nullptr;
         }
+        for (auto &e : inner_graph)
+            for (hipGraphExec_t g : e.second)
+                if (g) (void)hipGraphExecDestroy(g);
+        inner_graph.clear();
     }
 
     ~Sim() {
@@ -314,10 +319,9 @@ struct Sim {
         auto rel = [&](double r) { return bb > 0.0 ? std::sqrt(r / bb) : std::sqrt(r); };
         uint32_t outer = 0, inner_total = 0;
         while (rel(rr) > opts.tolerance && outer < opts.max_outer) {
-            const int later = outer > 0 ? 1 : 0;
             const uint32_t inner = pass_iterations(opts.inner_iterations, outer);
             const int cur = x64_cur;
-            replay(&inner_graph[later][cur], [this, cur, inner] { enqueue_inner(cur, inner); });
+            replay(&inner_graph[inner][cur], [this, cur, inner] { enqueue_inner(cur, inner); });
             x64_cur ^= 1;
             read_scalars();
             rr = host_scal[SC_RR];

@@ -184,7 +184,7 @@ void fluid_grid(const FluidSpec &f, uint32_t n[3], float *h_out) {
     *h_out = h;
 }
 
-std::string validate_fluid(const FluidSpec &f, uint32_t index) {
+std::string validate_fluid(const FluidSpec &f, uint32_t index, uint64_t max_cells) {
     std::string ctx = "fluids[" + std::to_string(index) + "] '" + f.name + "': ";
     const float *lo = f.bounds[0], *hi = f.bounds[1];
     if (!all_gt(hi, lo)) return ctx + "bounds must be [min, max] with max > min on every axis";
@@ -201,7 +201,7 @@ std::string validate_fluid(const FluidSpec &f, uint32_t index) {
     uint32_t n[3];
     float h;
     fluid_grid(f, n, &h);
-    if ((uint64_t)n[0] * n[1] * n[2] > 16000000ull) return ctx + "grid too large (lower resolution)";
+    if ((uint64_t)n[0] * n[1] * n[2] > max_cells) return ctx + "grid too large (lower resolution)";
     return "";
 }
 

@@ -39,7 +39,9 @@ struct FluidSpec {
 std::string parse_fluid(const std::string &json, FluidSpec &out);
 
 // Scene::validate's fluid checks; index is i in "fluids[i] 'name': ...".  "" when valid.
-std::string validate_fluid(const FluidSpec &f, uint32_t index);
+// max_cells: the grid cell limit (the reference's 16,000,000 unless a caller raises it).
+constexpr uint64_t REFERENCE_MAX_CELLS = 16000000ull;
+std::string validate_fluid(const FluidSpec &f, uint32_t index, uint64_t max_cells = REFERENCE_MAX_CELLS);
 
 // Fluid::grid: cells per axis and cell size, glam Vec3 f32 arithmetic.
 void fluid_grid(const FluidSpec &f, uint32_t n[3], float *h);

@@ -171,6 +171,35 @@ def test_mixed_solve_vs_fp64_oracle(pxl, orc, fluid):
     gpu.close()
 
 
+def test_mixed_256_grid_vs_fp64_oracle(pxl, orc):
+    """A 256^3 grid (16.8M cells, above the reference's 16M-cell limit; the extension is enabled
+    by max_cells): refused by default with the reference message; with the limit raised, the
+    mixed solve's FP64 solution has relative residual <= 1e-6, recomputed by the oracle from
+    the same grid state, for 2 steps."""
+    f = two_blocks(256)
+    f["fill"] = [{"min": [0.25, 0.05, 0.3], "max": [0.42, 0.3, 0.45], "velocity": [1.5, 0, 0]},
+                 {"min": [0.58, 0.05, 0.3], "max": [0.75, 0.3, 0.45], "velocity": [-1.5, 0, 0]}]
+    with pytest.raises(pxl.FluidError, match="grid too large"):
+        pxl.FluidState(f, FPS, seed=0, solver="mixed")
+    gpu = pxl.FluidState(f, FPS, seed=0, solver="mixed", tolerance=1e-6, max_cells=1 << 25)
+    assert gpu.info["n_cells"] == 256 ** 3
+    sim = oracle_sim(orc, f)
+    for s in range(2):
+        p = gpu.read_particles()
+        sim.set_particles(p[:, 0:3], p[:, 4:7])
+        sim.pressure[:] = gpu.read_field("pressure")
+        gpu.step(1)
+        sim.grid_phase()
+        st = gpu.last_solve()
+        x64 = gpu.read_field("pressure64")
+        rr = sim.poisson_relres64(np.where(sim.type == 1, x64, 0.0))
+        print(f"256^3 mixed step {s + 1}: gpu relres {st['rel_residual']:.3e} oracle relres {rr:.3e} "
+              f"outer {st['outer_iterations']}", flush=True)
+        assert st["rel_residual"] <= 1e-6
+        assert rr <= 1.05e-6
+    gpu.close()
+
+
 def test_advance_to_frame_and_rewind(pxl, orc):
     """fluid_advance semantics (fluid.rs:265-306): advance(f) runs steps_for_frame(f) steps;
     asking for an earlier frame re-seeds and re-simulates, giving the same state."""
