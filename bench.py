@@ -4,12 +4,13 @@ path on one MI355X, with p50/p99 request latency.
 
 Default workload (`--workload serve`): a stream of independent liquid simulations.  Request i
 is a cubic grid n^3 on the unit cube with n drawn from {32, 48, 64, 96, 128}, one liquid box
-of random place and size in the interior with a random initial velocity, 4-8 particles per
+or (even odds) sphere of random place and size in the interior with a random initial
+velocity, 4-8 particles per
 cell (fp32 SoA state), dt = f32(1/120), PIC/FLIP 0.95, seeded with seed = i, and `--steps`
 steps.  Requests arrive as a Poisson process at a server with a 200 GB HBM admission budget.
 The arrival rate is `--rate`, or by default the steady-state service rate of a fixed batch of
-`--warmup` (64) warm-up requests drawn from the same distribution, in requests per second of
-this stream's mean work: the load a server holding about that many simulations at once
+the stream's first `--warmup` (64) requests, run untimed beforehand, in requests per second
+of this stream's mean work: the load a server holding about that many simulations at once
 sustains (the batch grows past it when arrivals bunch up and drains back when they thin).
 Each simulation's pressure solve runs to a relative FP64 residual of 1e-6 (multigrid CG in
 FP32 inside FP64 refinement).  `value` = sum over requests of particles x steps, divided by
@@ -170,22 +171,23 @@ def cpu_dam(args, fluid):
 
 def bench_serve(args, pxl):
     import numpy as np
-    from tests.scenes import FPS, random_box
+    from tests.scenes import FPS, random_request
 
     sizes = tuple(int(v) for v in args.sizes.split(","))
-    reqs = [random_box(i, sizes=sizes) for i in range(args.requests)]
+    reqs = [random_request(i, sizes=sizes) for i in range(args.requests)]
     opts = dict(budget_bytes=args.budget)
     infos = [pxl.check(f, FPS, seed=i, solver="mixed") for i, f in enumerate(reqs)]
     P = np.array([inf["n_particles"] for inf in infos], dtype=np.float64)
     work = P * args.steps
 
-    # warm-up (untimed): device init and kernels through a short server run, then the
-    # steady-state service rate of a fixed batch of the warm-up requests
-    warm = [random_box(100000 + i, sizes=sizes) for i in range(args.warmup)]
+    # warm-up (untimed): device init and kernels through a short server run of other requests
+    # from the same distribution, then the steady-state service rate of a fixed batch of the
+    # stream's first requests
     with pxl.Server(**opts) as srv:
-        for i, f in enumerate(warm[:8]):
-            srv.submit(f, FPS, seed=i, steps=2)
+        for i in range(8):
+            srv.submit(random_request(100000 + i, sizes=sizes), FPS, seed=i, steps=2)
         srv.drain()
+    warm = reqs[:args.warmup]
     with pxl.Batch(warm, FPS, seeds=list(range(len(warm)))) as bat:
         bat.step(2)
         t0 = time.perf_counter()
@@ -239,13 +241,14 @@ def bench_serve(args, pxl):
         "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * st["busy_s"] / max(st["batch_steps"], 1),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic (random liquid boxes, reference LCG seeding, seed = request index)",
+        "data": "synthetic (random liquid boxes or spheres, reference LCG seeding, seed = request index)",
         "config": {
             "workload": f"serving stream: {len(reqs)} requests, cubic grids n in {{{args.sizes}}}, one random "
-                        f"liquid box at 4-8 particles/cell, {args.steps} steps each, Poisson arrivals, "
+                        f"liquid box or sphere (even odds) at 4-8 particles/cell, {args.steps} steps each, Poisson arrivals, "
                         f"{args.budget / 1e9:.0f} GB admission budget, MIXED solve to 1e-6",
-            "requests": len(reqs), "particles_total": float(P.sum()), "rate_per_s": rate,
-            "rate_rule": (f"steady-state service rate of a fixed batch of {len(warm)} warm-up requests "
+            "requests": len(reqs), "sphere_fills": sum("center" in f["fill"][0] for f in reqs),
+            "particles_total": float(P.sum()), "rate_per_s": rate,
+            "rate_rule": (f"steady-state service rate of a fixed batch of the first {len(warm)} requests "
                           f"({warm_rate:.4g} particle-steps/s) / mean request work") if args.rate <= 0 else "--rate",
             "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
             "busy_fraction": st["busy_s"] / st["span_s"] if st["span_s"] else None,
@@ -293,9 +296,9 @@ def batch_roofline(pxl, reqs, warm, timed):
 def bench_batch(args, pxl):
     """The fixed batch of the first --profile-batch stream requests: particle-steps/s of
     --steps batched steps after --warmup steps, and the per-kernel roofline."""
-    from tests.scenes import random_box
+    from tests.scenes import random_request
     sizes = tuple(int(v) for v in args.sizes.split(","))
-    reqs = [random_box(i, sizes=sizes) for i in range(args.profile_batch)]
+    reqs = [random_request(i, sizes=sizes) for i in range(args.profile_batch)]
     roofline, kernels, rate = batch_roofline(pxl, reqs, max(args.warmup, 1), args.steps)
     return {
         "metric": METRIC, "value": rate, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -331,7 +334,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60, help="steps per request (dam: timed steps)")
     ap.add_argument("--warmup", type=int, default=64,
-                    help="warm-up requests: the batch whose service rate sets the arrival rate (dam: warm-up steps)")
+                    help="calibration batch: the stream's first N requests, whose service rate sets the arrival "
+                         "rate (dam: warm-up steps)")
     ap.add_argument("--calib-steps", type=int, default=10, help="timed steps of the warm-up batch")
     ap.add_argument("--workload", default="serve", choices=["serve", "dam", "batch", "mixed256"])
     ap.add_argument("--requests", type=int, default=512)

@@ -124,11 +124,20 @@ def lcg_stream(state: int, count: int):
 def seed_particles(n, h, origin, blocks, ppc, seed, count_only=False):
     """Seeding loop of FluidState::new (crates/pxl-render/src/fluid.rs:123-148).
 
-    blocks: list of (min3, max3, velocity3).  Returns (x[P,3], v[P,3]) float32, or the count.
+    blocks: list of (min3, max3, velocity3) boxes, or ("sphere", centre3, radius, velocity3)
+    sphere fills (an extension).  Returns (x[P,3], v[P,3]) float32, or the count.
     """
     nn = np.asarray(n, np.uint32)
     org = np.asarray(origin, np.float32)
-    b = np.asarray([list(m) + list(M) + list(v) for (m, M, v) in blocks], np.float32).reshape(-1)
+    rows = []
+    for blk in blocks:
+        if len(blk) == 4 and blk[0] == "sphere":
+            _, c, r, v = blk
+            rows.append([1.0] + list(c) + [r, 0.0, 0.0] + list(v))
+        else:
+            m, M, v = blk
+            rows.append([0.0] + list(m) + list(M) + list(v))
+    b = np.asarray(rows, np.float32).reshape(-1)
     L = lib()
     cnt = L.orc_seed(_up(nn), np.float32(h), _fp(org), _fp(b), len(blocks), ppc, seed, None, None)
     if count_only:

@@ -50,15 +50,21 @@ uint64_t orc_seed(const uint32_t n[3], float h, const float origin[3], const flo
     uint64_t rng = 0x9e3779b97f4a7c15ull ^ (uint64_t)seed;
     uint64_t count = 0;
     for (uint32_t bi = 0; bi < nb; bi++) {
-        const float *b = blocks + 9 * bi;
+        const float *b = blocks + ORC_BLOCK_FLOATS * bi;
         for (uint32_t k = 1; k + 1 < n[2]; k++)
             for (uint32_t j = 1; j + 1 < n[1]; j++)
                 for (uint32_t i = 1; i + 1 < n[0]; i++) {
                     float c[3] = {origin[0] + ((float)i + 0.5f) * h, origin[1] + ((float)j + 0.5f) * h,
                                   origin[2] + ((float)k + 0.5f) * h};
                     int out = 0;
-                    for (int a = 0; a < 3; a++)
-                        if (c[a] < b[a] || c[a] > b[3 + a]) out = 1;
+                    if (b[0] == 0.0f) {  /* box: centre within [min, max] on every axis */
+                        for (int a = 0; a < 3; a++)
+                            if (c[a] < b[1 + a] || c[a] > b[4 + a]) out = 1;
+                    } else {  /* sphere (extension): |centre - c|^2 <= r^2, summed x, y, z in f32 */
+                        const float dx = c[0] - b[1], dy = c[1] - b[2], dz = c[2] - b[3];
+                        const float d2 = dx * dx + dy * dy + dz * dz;
+                        out = !(d2 <= b[4] * b[4]);
+                    }
                     if (out) continue;
                     for (uint32_t q = 0; q < ppc; q++) {
                         float rx = orc_lcg(&rng);
@@ -71,9 +77,9 @@ uint64_t orc_seed(const uint32_t n[3], float h, const float origin[3], const flo
                             x[3 * count + 0] = px;
                             x[3 * count + 1] = py;
                             x[3 * count + 2] = pz;
-                            v[3 * count + 0] = b[6];
-                            v[3 * count + 1] = b[7];
-                            v[3 * count + 2] = b[8];
+                            v[3 * count + 0] = b[7];
+                            v[3 * count + 1] = b[8];
+                            v[3 * count + 2] = b[9];
                         }
                         count++;
                     }

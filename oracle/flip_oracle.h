@@ -50,8 +50,12 @@ typedef struct orc_state {
 void orc_fluid_grid(const float lo[3], const float hi[3], uint32_t resolution,
                     uint32_t n_out[3], float *h_out);
 float orc_lcg(uint64_t *s);
-/* blocks: nb rows of 9 floats (min xyz, max xyz, velocity xyz).  When x is NULL only counts.
- * x is P*3 positions, v is P*3 velocities (interleaved xyz).  Returns the particle count. */
+/* blocks: nb rows of ORC_BLOCK_FLOATS floats {kind, a xyz, b xyz, velocity xyz}: kind 0 = the
+ * reference's box (a = min, b = max: cells whose centre lies in [min, max]); kind 1 = a sphere
+ * fill, an extension (a = centre, b[0] = radius: cells whose centre c has
+ * (c-a).x^2 + (c-a).y^2 + (c-a).z^2 <= radius^2, in f32, summed in that order).  When x is NULL
+ * only counts.  x is P*3 positions, v is P*3 velocities (interleaved xyz).  Returns the count. */
+#define ORC_BLOCK_FLOATS 10
 uint64_t orc_seed(const uint32_t n[3], float h, const float origin[3], const float *blocks,
                   uint32_t nb, uint32_t ppc, uint32_t seed, float *x, float *v);
 uint64_t orc_steps_for_frame(double warmup, double frame, double fps, uint32_t substeps);

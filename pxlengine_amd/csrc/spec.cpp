@@ -17,6 +17,7 @@ const char *const kFluidFields[] = {"name", "bounds", "resolution", "fill", "par
                                     "surface_refine", "particle_radius", "warmup",
                                     "max_triangles", "materials"};
 const char *const kBlockFields[] = {"min", "max", "velocity"};
+const char *const kSphereFields[] = {"center", "radius", "velocity"};  // extension
 
 template <size_t N>
 std::string expected_list(const char *const (&fields)[N]) {
@@ -63,8 +64,29 @@ std::string get_vec3(const JsonValue &v, float out[3]) {
     return "";
 }
 
+// A sphere fill (extension): an object with a `center` field.
+std::string parse_sphere(const JsonValue &v, FluidBlockSpec &b) {
+    std::set<std::string> seen;
+    bool has_radius = false;
+    b.sphere = true;
+    for (const auto &m : v.members) {
+        const std::string &k = m.first;
+        if (!known(kSphereFields, k)) return "unknown field `" + k + "`, expected one of " + expected_list(kSphereFields);
+        if (!seen.insert(k).second) return "duplicate field `" + k + "`";
+        std::string e;
+        if (k == "center") e = get_vec3(m.second, b.center);
+        else if (k == "radius") { e = get_f32(m.second, b.radius); has_radius = true; }
+        else e = get_vec3(m.second, b.velocity);
+        if (!e.empty()) return e;
+    }
+    if (!has_radius) return "missing field `radius`";
+    return "";
+}
+
 std::string parse_block(const JsonValue &v, FluidBlockSpec &b) {
     if (v.kind != JsonValue::Object) return "invalid type: " + json_describe(v) + ", expected struct FluidBlock";
+    for (const auto &m : v.members)
+        if (m.first == "center") return parse_sphere(v, b);
     std::set<std::string> seen;
     bool has_min = false, has_max = false;
     for (const auto &m : v.members) {
@@ -195,9 +217,19 @@ std::string validate_fluid(const FluidSpec &f, uint32_t index, uint64_t max_cell
         !flip_ok || f.pressure_iterations == 0)
         return ctx + "resolution 16..256, particles_per_cell 1..64, substeps 1..64, surface_refine 1..3, flip 0..1";
     if (f.fill.empty()) return ctx + "needs at least one `fill` block of liquid";
-    for (const auto &b : f.fill)
-        if (!all_gt(b.max, b.min) || any_lt(b.min, lo) || any_lt(hi, b.max))
+    for (const auto &b : f.fill) {
+        if (b.sphere) {
+            float smin[3], smax[3];
+            for (int a = 0; a < 3; a++) {
+                smin[a] = b.center[a] - b.radius;
+                smax[a] = b.center[a] + b.radius;
+            }
+            if (!(b.radius > 0.0f) || any_lt(smin, lo) || any_lt(hi, smax))
+                return ctx + "each fill sphere must have radius > 0 and lie inside `bounds`";
+        } else if (!all_gt(b.max, b.min) || any_lt(b.min, lo) || any_lt(hi, b.max)) {
             return ctx + "each fill block must have max > min and lie inside `bounds`";
+        }
+    }
     uint32_t n[3];
     float h;
     fluid_grid(f, n, &h);
@@ -234,9 +266,14 @@ uint64_t seed_particles(const FluidSpec &f, uint32_t seed, std::vector<float> *x
                     float cx = o[0] + ((float)i + 0.5f) * h;
                     float cy = o[1] + ((float)j + 0.5f) * h;
                     float cz = o[2] + ((float)k + 0.5f) * h;
-                    if (cx < b.min[0] || cx > b.max[0] || cy < b.min[1] || cy > b.max[1] ||
-                        cz < b.min[2] || cz > b.max[2])
+                    if (b.sphere) {  // extension: centre within the sphere (f32, x + y + z)
+                        const float dx = cx - b.center[0], dy = cy - b.center[1], dz = cz - b.center[2];
+                        const float d2 = dx * dx + dy * dy + dz * dz;
+                        if (!(d2 <= b.radius * b.radius)) continue;
+                    } else if (cx < b.min[0] || cx > b.max[0] || cy < b.min[1] || cy > b.max[1] ||
+                               cz < b.min[2] || cz > b.max[2]) {
                         continue;
+                    }
                     for (uint32_t q = 0; q < f.particles_per_cell; q++) {
                         float px = o[0] + ((float)i + lcg_next(rng)) * h;
                         float py = o[1] + ((float)j + lcg_next(rng)) * h;

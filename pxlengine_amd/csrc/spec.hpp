@@ -12,10 +12,15 @@
 
 namespace pxl {
 
+// A fill region: the reference's box {min, max, velocity}, or a sphere {center, radius,
+// velocity} (an extension: the reference's blocks are boxes only).
 struct FluidBlockSpec {
     float min[3] = {0, 0, 0};
     float max[3] = {0, 0, 0};
     float velocity[3] = {0, 0, 0};
+    bool sphere = false;
+    float center[3] = {0, 0, 0};
+    float radius = 0.0f;
 };
 
 struct FluidSpec {

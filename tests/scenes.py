@@ -42,11 +42,22 @@ def two_blocks(res: int, ppc: int = 8, iters: int = 40) -> dict:
     }
 
 
+def sphere_drop(res: int, ppc: int = 6, iters: int = 40) -> dict:
+    """A liquid sphere (an extension) thrown downward onto a shallow pool."""
+    return {
+        "name": f"sphere{res}", "bounds": [[0, 0, 0], [1, 1, 1]], "resolution": res,
+        "fill": [{"center": [0.45, 0.62, 0.5], "radius": 0.25, "velocity": [0.5, -2.0, 0.2]},
+                 {"min": [0.02, 0.02, 0.02], "max": [0.98, 0.15, 0.98]}],
+        "particles_per_cell": ppc, "substeps": 5, "flip": 0.95, "pressure_iterations": iters,
+    }
+
+
 def oracle_sim(orc, fluid: dict, fps: float = FPS, seed: int = 0):
     """Builds the oracle simulation for a fluid block (defaults as crates/pxl-scene/src/lib.rs:96-119)."""
     lo, hi = fluid["bounds"]
     n, h = orc.fluid_grid(lo, hi, fluid.get("resolution", 64))
-    blocks = [(b["min"], b["max"], b.get("velocity", [0, 0, 0])) for b in fluid["fill"]]
+    blocks = [("sphere", b["center"], b["radius"], b.get("velocity", [0, 0, 0])) if "center" in b
+              else (b["min"], b["max"], b.get("velocity", [0, 0, 0])) for b in fluid["fill"]]
     x, v = orc.seed_particles(n, h, lo, blocks, fluid.get("particles_per_cell", 8), seed)
     dt = orc.dt_for(fps, fluid.get("substeps", 8))
     return orc.Sim(n, h, lo, dt, fluid.get("flip", 0.95), fluid.get("gravity", [0, -9.81, 0]), x, v)
@@ -59,18 +70,26 @@ def aos8(x: np.ndarray, v: np.ndarray) -> np.ndarray:
     return out
 
 
-def random_box(index: int, sizes=(32, 48, 64, 96, 128), ppc_range=(4, 8)) -> dict:
-    """A serving request: cubic grid n^3 with n drawn from `sizes`, one liquid box at a random
-    place and size inside the interior, ppc drawn from `ppc_range`, a random initial velocity.
+def random_request(index: int, sizes=(32, 48, 64, 96, 128), ppc_range=(4, 8), spheres=True) -> dict:
+    """A serving request: cubic grid n^3 with n drawn from `sizes`, one liquid region at a
+    random place and size inside the interior -- a box, or with `spheres` half the time a
+    sphere (an extension) -- ppc drawn from `ppc_range`, a random initial velocity.
     Deterministic in `index`."""
     rng = np.random.default_rng(1000 + index)
     n = int(rng.choice(sizes))
-    lo = rng.uniform(0.05, 0.45, 3)
-    ext = rng.uniform(0.2, 0.5, 3)
-    hi = np.minimum(lo + ext, 0.95)
-    vel = rng.uniform(-1.0, 1.0, 3)
+    sphere = spheres and bool(rng.integers(2))
+    if sphere:
+        r = rng.uniform(0.12, 0.3)
+        c = rng.uniform(0.05 + r, 0.95 - r, 3)
+        vel = rng.uniform(-1.0, 1.0, 3)
+        fill = {"center": [float(a) for a in c], "radius": float(r), "velocity": [float(a) for a in vel]}
+    else:
+        lo = rng.uniform(0.05, 0.45, 3)
+        ext = rng.uniform(0.2, 0.5, 3)
+        hi = np.minimum(lo + ext, 0.95)
+        vel = rng.uniform(-1.0, 1.0, 3)
+        fill = {"min": [float(a) for a in lo], "max": [float(a) for a in hi], "velocity": [float(a) for a in vel]}
     return {
-        "name": f"req{index}", "bounds": [[0, 0, 0], [1, 1, 1]], "resolution": n,
-        "fill": [{"min": [float(a) for a in lo], "max": [float(a) for a in hi], "velocity": [float(a) for a in vel]}],
+        "name": f"req{index}", "bounds": [[0, 0, 0], [1, 1, 1]], "resolution": n, "fill": [fill],
         "particles_per_cell": int(rng.integers(ppc_range[0], ppc_range[1] + 1)), "substeps": 5, "flip": 0.95,
     }

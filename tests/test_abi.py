@@ -143,6 +143,25 @@ def test_multiple_blocks_with_velocity(pxl, orc):
     assert set(np.unique(a[:, 4])) == {np.float32(1.5), np.float32(-1.5)}
 
 
+def test_sphere_fill_seeding_matches_oracle(pxl, orc):
+    """Sphere fills (an extension): strict parsing, bounds checks, seeding equal to the oracle."""
+    from tests.scenes import oracle_sim
+    f = dict(dam_break(24, ppc=3), fill=[{"center": [0.5, 0.4, 0.5], "radius": 0.25, "velocity": [0, -1, 0]},
+                                        {"min": [0.05, 0.05, 0.05], "max": [0.3, 0.3, 0.3]}])
+    a = pxl.seed_particles(f, 3)
+    sim = oracle_sim(orc, f, seed=3)
+    assert a.shape == (sim.P, 8) and sim.P > 0
+    np.testing.assert_array_equal(a[:, 0:3], sim.x)
+    np.testing.assert_array_equal(a[:, 4:7], sim.v)
+    bad = dict(f, fill=[{"center": [0.5, 0.5, 0.5], "radius": 0.6}])
+    assert "each fill sphere must have radius > 0 and lie inside `bounds`" in str(_err(pxl, bad))
+    bad = dict(f, fill=[{"center": [0.5, 0.5, 0.5], "radius": 0.0}])
+    assert "each fill sphere" in str(_err(pxl, bad))
+    assert "missing field `radius`" in str(_err(pxl, dict(f, fill=[{"center": [0.5, 0.5, 0.5]}])))
+    e = _err(pxl, dict(f, fill=[{"center": [0.5, 0.5, 0.5], "radius": 0.2, "min": [0, 0, 0]}]))
+    assert "unknown field `min`" in str(e)
+
+
 def test_device_bytes_reported(pxl):
     small = pxl.check(dam_break(32))["device_bytes"]
     mixed = pxl.check(dam_break(32), solver="mixed")["device_bytes"]

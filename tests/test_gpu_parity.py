@@ -9,7 +9,7 @@ stated tolerances; the mixed solver is checked against the FP64 Poisson oracle.
 import numpy as np
 import pytest
 
-from tests.scenes import FPS, EXAMPLE_DAM_BREAK, dam_break, oracle_sim, two_blocks
+from tests.scenes import FPS, EXAMPLE_DAM_BREAK, dam_break, oracle_sim, sphere_drop, two_blocks
 
 pytestmark = pytest.mark.gpu
 
@@ -44,8 +44,9 @@ def test_grid_phase_bitwise(pxl, orc, res):
     gpu.close()
 
 
-@pytest.mark.parametrize("fluid,steps", [(dam_break(32), 3), (EXAMPLE_DAM_BREAK, 2), (two_blocks(24), 3)],
-                         ids=["dam32", "example", "collide24"])
+@pytest.mark.parametrize("fluid,steps", [(dam_break(32), 3), (EXAMPLE_DAM_BREAK, 2), (two_blocks(24), 3),
+                                         (sphere_drop(28), 3)],
+                         ids=["dam32", "example", "collide24", "sphere28"])
 def test_rbgs_steps_bitwise(pxl, orc, fluid, steps):
     """Whole reference steps (RBGS, pressure_iterations sweeps): particles, pressure and face
     velocities equal the oracle bit for bit after every step."""

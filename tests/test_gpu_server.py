@@ -7,7 +7,7 @@ import time
 import numpy as np
 import pytest
 
-from tests.scenes import FPS, dam_break, random_box, two_blocks
+from tests.scenes import FPS, dam_break, random_request, two_blocks
 
 pytestmark = pytest.mark.gpu
 
@@ -45,20 +45,24 @@ def test_batched_equals_single_sim_path(pxl):
 
 
 def test_alone_equals_in_the_mix(pxl):
-    a, b, c = dam_break(32), two_blocks(24), random_box(3, sizes=(16, 40))
+    a, b, c = dam_break(32), two_blocks(24), random_request(3, sizes=(16, 40))
+    d = random_request(1, sizes=(16, 40))  # a sphere fill
+    assert "center" in d["fill"][0] and "min" in c["fill"][0]
     (pa,), _, _ = _run(pxl, [(a, 4)])
-    parts, infos, st = _run(pxl, [(b, 2), (a, 4), (c, 5)])
+    parts, infos, st = _run(pxl, [(b, 2), (a, 4), (c, 5), (d, 3)])
     for info in infos:
         assert info["status"] == "done"
         assert 0 <= info["rel_residual"] <= 1e-6
     assert np.array_equal(pa, parts[1])
-    assert st["completed"] == 3 and st["mean_batch"] > 1.0
+    assert st["completed"] == 4 and st["mean_batch"] > 1.0
     (pc,), _, _ = _run(pxl, [(c, 5)])
     assert np.array_equal(pc, parts[2])
+    (pd,), _, _ = _run(pxl, [(d, 3)])
+    assert np.array_equal(pd, parts[3])
 
 
 def test_many_small_requests_complete_with_stats(pxl):
-    reqs = [(random_box(i, sizes=(16, 24, 32)), 2) for i in range(6)]
+    reqs = [(random_request(i, sizes=(16, 24, 32)), 2) for i in range(6)]
     with _server(pxl) as srv:
         ids = [srv.submit(f, FPS, seed=0, steps=n) for f, n in reqs]
         srv.drain(timeout=300)

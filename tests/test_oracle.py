@@ -108,6 +108,47 @@ def test_seed_positions_closed_form(orc, res, ppc, seed):
     assert np.all(x <= np.asarray(lo, np.float32) + (idx + 1) * np.float32(h))
 
 
+def _cells_in_sphere(n, h, origin, centre, radius):
+    """Interior cells whose f32 centre c has (cx-x0)^2 + (cy-y0)^2 + (cz-z0)^2 <= r^2 in f32."""
+    out = []
+    o = np.asarray(origin, np.float32)
+    c0 = np.asarray(centre, np.float32)
+    r = np.float32(radius)
+    for k in range(1, n[2] - 1):
+        for j in range(1, n[1] - 1):
+            for i in range(1, n[0] - 1):
+                c = o + (np.array([i, j, k], np.float32) + np.float32(0.5)) * np.float32(h)
+                d = c - c0
+                if (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2] <= r * r:
+                    out.append((i, j, k))
+    return out
+
+
+@pytest.mark.parametrize("res,ppc,seed", [(16, 4, 0), (22, 3, 7)])
+def test_seed_sphere_brute_force(orc, res, ppc, seed):
+    """Sphere fills (an extension): cells by brute force, positions from the exact LCG
+    stream; a box block after the sphere continues the same stream."""
+    lo, hi = [0, 0, 0], [1, 1, 1]
+    n, h = orc.fluid_grid(lo, hi, res)
+    centre, radius = [0.5, 0.45, 0.55], 0.3
+    box = ([0.05, 0.05, 0.05], [0.3, 0.2, 0.9])
+    x, v = orc.seed_particles(n, h, lo, [("sphere", centre, radius, [1, 0, -1]), (box[0], box[1], [0, 2, 0])],
+                              ppc, seed)
+    sc = _cells_in_sphere(n, h, lo, centre, radius)
+    bc = _cells_in_block(n, h, lo, *box)
+    cells = sc + bc
+    assert len(x) == ppc * len(cells)
+    # the number of cells approximates the sphere's volume
+    assert abs(len(sc) * h ** 3 / (4 / 3 * np.pi * radius ** 3) - 1) < 0.1
+    ints, _ = _lcg_ints(0x9E3779B97F4A7C15 ^ seed, 3 * len(x))
+    r = (np.array(ints, np.float64).astype(np.float32) / np.float32(1 << 24)).reshape(-1, 3)
+    idx = np.repeat(np.array(cells, np.float32), ppc, axis=0)
+    np.testing.assert_array_equal(x, np.asarray(lo, np.float32) + (idx + r) * np.float32(h))
+    ns = ppc * len(sc)
+    np.testing.assert_array_equal(v[:ns], np.tile(np.float32([1, 0, -1]), (ns, 1)))
+    np.testing.assert_array_equal(v[ns:], np.tile(np.float32([0, 2, 0]), (len(x) - ns, 1)))
+
+
 def test_seed_counts_configs(orc):
     n, h = orc.fluid_grid([0, 0, 0], [1, 1, 1], 64)
     assert orc.seed_particles(n, h, [0, 0, 0], [([0, 0, 0], [0.4, 0.6, 1], [0, 0, 0])], 8, 0, count_only=True) \
