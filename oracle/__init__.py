@@ -40,6 +40,7 @@ class _State(C.Structure):
         ("acc", C.POINTER(C.c_int32)), ("wacc", C.POINTER(C.c_int32)),
         ("count", C.POINTER(C.c_uint32)), ("type", C.POINTER(C.c_uint32)),
         ("pressure", C.POINTER(C.c_float)), ("div", C.POINTER(C.c_float)),
+        ("obst_id", C.POINTER(C.c_uint32)), ("obst", C.POINTER(C.c_float)), ("n_obst", C.c_uint32),
     ]
 
 
@@ -64,8 +65,12 @@ def lib():
         L.orc_fluid_grid.argtypes = [f32p, f32p, C.c_uint32, u32p, f32p]
         L.orc_lcg.argtypes = [C.POINTER(C.c_uint64)]
         L.orc_lcg.restype = C.c_float
-        L.orc_seed.argtypes = [u32p, C.c_float, f32p, f32p, C.c_uint32, C.c_uint32, C.c_uint32, f32p, f32p]
+        L.orc_seed.argtypes = [u32p, C.c_float, f32p, f32p, C.c_uint32, C.c_uint32, C.c_uint32, u32p, f32p, f32p]
         L.orc_seed.restype = C.c_uint64
+        L.orc_obstacle_cells.argtypes = [u32p, C.c_float, f32p, f32p, C.c_uint32, u32p]
+        L.orc_obstacle_cells.restype = C.c_uint64
+        L.orc_set_obstacles.argtypes = [SP, f32p, C.c_uint32]
+        L.orc_set_obstacles.restype = C.c_int
         L.orc_steps_for_frame.argtypes = [C.c_double, C.c_double, C.c_double, C.c_uint32]
         L.orc_steps_for_frame.restype = C.c_uint64
         L.orc_dt.argtypes = [C.c_double, C.c_uint32]
@@ -121,11 +126,27 @@ def lcg_stream(state: int, count: int):
     return out, s.value
 
 
-def seed_particles(n, h, origin, blocks, ppc, seed, count_only=False):
+def _obstacle_rows(obstacles):
+    return np.asarray([list(c) + [r] for c, r in obstacles], np.float32).reshape(-1)
+
+
+def obstacle_cells(n, h, origin, obstacles):
+    """Obstacle cells (an extension): per cell 1 + the index of the first obstacle (list order)
+    holding its centre, 0 elsewhere.  obstacles: list of (centre3, radius)."""
+    nn = np.asarray(n, np.uint32)
+    org = np.asarray(origin, np.float32)
+    ob = _obstacle_rows(obstacles)
+    out = np.zeros(int(nn[0]) * int(nn[1]) * int(nn[2]), np.uint32)
+    lib().orc_obstacle_cells(_up(nn), np.float32(h), _fp(org), _fp(ob), len(obstacles), _up(out))
+    return out
+
+
+def seed_particles(n, h, origin, blocks, ppc, seed, count_only=False, obstacles=None):
     """Seeding loop of FluidState::new (crates/pxl-render/src/fluid.rs:123-148).
 
     blocks: list of (min3, max3, velocity3) boxes, or ("sphere", centre3, radius, velocity3)
-    sphere fills (an extension).  Returns (x[P,3], v[P,3]) float32, or the count.
+    sphere fills (an extension).  obstacles (an extension): list of (centre3, radius) solid
+    spheres whose cells receive no particles.  Returns (x[P,3], v[P,3]) float32, or the count.
     """
     nn = np.asarray(n, np.uint32)
     org = np.asarray(origin, np.float32)
@@ -139,12 +160,14 @@ def seed_particles(n, h, origin, blocks, ppc, seed, count_only=False):
             rows.append([0.0] + list(m) + list(M) + list(v))
     b = np.asarray(rows, np.float32).reshape(-1)
     L = lib()
-    cnt = L.orc_seed(_up(nn), np.float32(h), _fp(org), _fp(b), len(blocks), ppc, seed, None, None)
+    ids = obstacle_cells(n, h, origin, obstacles) if obstacles else None
+    idp = _up(ids) if ids is not None else None
+    cnt = L.orc_seed(_up(nn), np.float32(h), _fp(org), _fp(b), len(blocks), ppc, seed, idp, None, None)
     if count_only:
         return int(cnt)
     x = np.zeros((max(cnt, 1), 3), np.float32)
     v = np.zeros((max(cnt, 1), 3), np.float32)
-    L.orc_seed(_up(nn), np.float32(h), _fp(org), _fp(b), len(blocks), ppc, seed, _fp(x), _fp(v))
+    L.orc_seed(_up(nn), np.float32(h), _fp(org), _fp(b), len(blocks), ppc, seed, idp, _fp(x), _fp(v))
     return x[:cnt], v[:cnt]
 
 
@@ -159,7 +182,7 @@ def dt_for(fps, substeps):
 class Sim:
     """One oracle simulation; arrays are numpy views of the C state (SoA particles)."""
 
-    def __init__(self, n, h, origin, dt, flip, gravity, x, v):
+    def __init__(self, n, h, origin, dt, flip, gravity, x, v, obstacles=None):
         p = Params()
         p.nx, p.ny, p.nz = (int(a) for a in n)
         p.n_particles = len(x)
@@ -198,6 +221,12 @@ class Sim:
         self.px[:], self.py[:], self.pz[:] = x[:, 0], x[:, 1], x[:, 2]
         self.vx[:], self.vy[:], self.vz[:] = v[:, 0], v[:, 1], v[:, 2]
         self.params = p
+        self.obst_id = None
+        if obstacles:
+            ob = _obstacle_rows(obstacles)
+            if self._L.orc_set_obstacles(self._s, _fp(ob), len(obstacles)) != 0:
+                raise MemoryError("oracle allocation failed")
+            self.obst_id = view(self._s.contents.obst_id, self.C, np.uint32)
 
     def __del__(self):
         s = getattr(self, "_s", None)

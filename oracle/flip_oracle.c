@@ -42,11 +42,43 @@ float orc_lcg(uint64_t *s) {
     return (float)(*s >> 40) / (float)(1ull << 24);
 }
 
+/* Sphere test of the extensions (sphere fills, obstacles): point c lies in the sphere
+ * {centre s[0..2], radius s[3]} when (c-s).x^2 + (c-s).y^2 + (c-s).z^2 <= r^2, in f32, summed
+ * in that order. */
+static int in_sphere(const float c[3], const float *s) {
+    const float dx = c[0] - s[0], dy = c[1] - s[1], dz = c[2] - s[2];
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    return d2 <= s[3] * s[3];
+}
+
+/* Obstacle cells (extension): id_out[cell] = 1 + the index of the first obstacle, in list
+ * order, whose sphere holds the cell's centre origin + (idx + 0.5) * h, for interior cells
+ * (1..n-2 on every axis); 0 elsewhere.  Brute force over every obstacle. */
+uint64_t orc_obstacle_cells(const uint32_t n[3], float h, const float origin[3], const float *obst,
+                            uint32_t n_obst, uint32_t *id_out) {
+    uint64_t count = 0;
+    memset(id_out, 0, (size_t)n[0] * n[1] * n[2] * 4);
+    for (uint32_t k = 1; k + 1 < n[2]; k++)
+        for (uint32_t j = 1; j + 1 < n[1]; j++)
+            for (uint32_t i = 1; i + 1 < n[0]; i++) {
+                float c[3] = {origin[0] + ((float)i + 0.5f) * h, origin[1] + ((float)j + 0.5f) * h,
+                              origin[2] + ((float)k + 0.5f) * h};
+                for (uint32_t o = 0; o < n_obst; o++)
+                    if (in_sphere(c, obst + 4 * o)) {
+                        id_out[i + n[0] * (j + n[1] * k)] = o + 1;
+                        count++;
+                        break;
+                    }
+            }
+    return count;
+}
+
 /* Seeding loop of FluidState::new (fluid.rs:123-148): for each fill block in order, cells
  * k in 1..nz-1, j in 1..ny-1, i in 1..nx-1 whose centre lies inside the block (inclusive)
- * receive ppc particles at origin + (idx + lcg) * h, x then y then z draws. */
+ * receive ppc particles at origin + (idx + lcg) * h, x then y then z draws.  With obstacles
+ * (obst_id non-NULL, an extension) obstacle cells receive none and draw nothing. */
 uint64_t orc_seed(const uint32_t n[3], float h, const float origin[3], const float *blocks,
-                  uint32_t nb, uint32_t ppc, uint32_t seed, float *x, float *v) {
+                  uint32_t nb, uint32_t ppc, uint32_t seed, const uint32_t *obst_id, float *x, float *v) {
     uint64_t rng = 0x9e3779b97f4a7c15ull ^ (uint64_t)seed;
     uint64_t count = 0;
     for (uint32_t bi = 0; bi < nb; bi++) {
@@ -60,12 +92,10 @@ uint64_t orc_seed(const uint32_t n[3], float h, const float origin[3], const flo
                     if (b[0] == 0.0f) {  /* box: centre within [min, max] on every axis */
                         for (int a = 0; a < 3; a++)
                             if (c[a] < b[1 + a] || c[a] > b[4 + a]) out = 1;
-                    } else {  /* sphere (extension): |centre - c|^2 <= r^2, summed x, y, z in f32 */
-                        const float dx = c[0] - b[1], dy = c[1] - b[2], dz = c[2] - b[3];
-                        const float d2 = dx * dx + dy * dy + dz * dz;
-                        out = !(d2 <= b[4] * b[4]);
+                    } else {  /* sphere (extension) */
+                        out = !in_sphere(c, b + 1);
                     }
-                    if (out) continue;
+                    if (out || (obst_id && obst_id[i + n[0] * (j + n[1] * k)])) continue;
                     for (uint32_t q = 0; q < ppc; q++) {
                         float rx = orc_lcg(&rng);
                         float px = origin[0] + ((float)i + rx) * h;
@@ -123,7 +153,22 @@ void orc_destroy(orc_state *s) {
     free(s->px); free(s->py); free(s->pz); free(s->vx); free(s->vy); free(s->vz);
     free(s->vel); free(s->vel_old); free(s->acc); free(s->wacc);
     free(s->count); free(s->type); free(s->pressure); free(s->div);
+    free(s->obst_id); free(s->obst);
     free(s);
+}
+
+int orc_set_obstacles(orc_state *s, const float *obst, uint32_t n_obst) {
+    free(s->obst_id); free(s->obst);
+    s->obst_id = NULL; s->obst = NULL; s->n_obst = 0;
+    if (n_obst == 0) return 0;
+    s->obst_id = (uint32_t *)calloc(s->n_cells, 4);
+    s->obst = (float *)malloc((size_t)n_obst * 16);
+    if (!s->obst_id || !s->obst) return -1;
+    memcpy(s->obst, obst, (size_t)n_obst * 16);
+    s->n_obst = n_obst;
+    const uint32_t n[3] = {s->p.nx, s->p.ny, s->p.nz};
+    orc_obstacle_cells(n, s->p.h, s->p.origin, obst, n_obst, s->obst_id);
+    return 0;
 }
 
 /* ------------------------------------------------------------------ MAC indexing (fluid.slang:43-65) */
@@ -268,6 +313,7 @@ void orc_normalize_grid(orc_state *s) {
     for (uint32_t id = 0; id < (uint32_t)s->n_cells; id++) {
         uint32_t i = id % p->nx, j = (id / p->nx) % p->ny, k = id / (p->nx * p->ny);
         int wall = i == 0 || j == 0 || k == 0 || i == p->nx - 1 || j == p->ny - 1 || k == p->nz - 1;
+        if (s->obst_id && s->obst_id[id]) wall = 1;  /* obstacle cell (extension) */
         uint32_t t = wall ? ORC_SOLID : (s->count[id] > 0 ? ORC_FLUID : ORC_AIR);
         s->type[id] = t;
         if (t != ORC_FLUID) s->pressure[id] = 0.0f;
@@ -346,6 +392,35 @@ void orc_project(orc_state *s) {
     }
 }
 
+/* Obstacle response (extension), after advection and before the wall clamp: a particle whose
+ * cell (the binning cell, clamp(floor((x - origin) * inv_h))) is an obstacle cell moves along
+ * the ray from the obstacle's centre to distance r + h (so that its new cell's centre is
+ * outside that sphere), and the velocity's inward normal component is removed:
+ *   d = x - centre; len = sqrt(d.x^2 + d.y^2 + d.z^2); n = d / len (per component);
+ *   x = centre + n * (r + h); vn = v.x n.x + v.y n.y + v.z n.z; if vn < 0: v = v - vn * n.
+ * Sums left to right, f32, correctly rounded sqrt and division; len == 0 leaves the
+ * particle unchanged. */
+static void obstacle_push(const orc_state *s, float x[3], float v[3]) {
+    const orc_params *p = &s->p;
+    int n[3] = {(int)p->nx, (int)p->ny, (int)p->nz}, cell[3];
+    for (int a = 0; a < 3; a++) {
+        int t = (int)floorf((x[a] - p->origin[a]) * p->inv_h);
+        cell[a] = t < 0 ? 0 : (t > n[a] - 1 ? n[a] - 1 : t);
+    }
+    const uint32_t o = s->obst_id[cell_index(p, cell)];
+    if (!o) return;
+    const float *sp = s->obst + 4 * (o - 1);
+    const float d[3] = {x[0] - sp[0], x[1] - sp[1], x[2] - sp[2]};
+    const float len = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    if (!(len > 0.0f)) return;
+    const float nr[3] = {d[0] / len, d[1] / len, d[2] / len};
+    const float rad = sp[3] + p->h;
+    for (int a = 0; a < 3; a++) x[a] = sp[a] + nr[a] * rad;
+    const float vn = v[0] * nr[0] + v[1] * nr[1] + v[2] * nr[2];
+    if (vn < 0.0f)
+        for (int a = 0; a < 3; a++) v[a] = v[a] - vn * nr[a];
+}
+
 /* g2p_advect (fluid.slang:200-220): PIC/FLIP blend, RK2 through the grid velocity, clamp to
  * the interior with the wall-normal velocity clipped. */
 void orc_g2p_advect(orc_state *s) {
@@ -370,6 +445,7 @@ void orc_g2p_advect(orc_state *s) {
         }
         for (int c = 0; c < 3; c++) adv[c] = sample_face(s, s->vel, c, mid);
         for (int c = 0; c < 3; c++) x[c] = x[c] + p->dt * adv[c];
+        if (s->obst_id) obstacle_push(s, x, v);
         for (int a = 0; a < 3; a++) {
             if (x[a] < lo[a]) { x[a] = lo[a]; v[a] = fmax_spv(v[a], 0.0f); }
             if (x[a] > hi[a]) { x[a] = hi[a]; v[a] = fmin_spv(v[a], 0.0f); }

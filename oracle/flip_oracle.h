@@ -44,6 +44,9 @@ typedef struct orc_state {
     uint32_t *count;                      /* cells: particles per cell */
     uint32_t *type;                       /* cells: 0 air, 1 fluid, 2 solid */
     float *pressure, *div;                /* cells */
+    uint32_t *obst_id;                    /* cells: 1 + obstacle index, 0 = none (extension; NULL = no obstacles) */
+    float *obst;                          /* n_obst rows {cx, cy, cz, r} */
+    uint32_t n_obst;
 } orc_state;
 
 /* ---- scene block / seeding (crates/pxl-scene/src/lib.rs:121-129, fluid.rs:96-180) ---- */
@@ -56,14 +59,25 @@ float orc_lcg(uint64_t *s);
  * (c-a).x^2 + (c-a).y^2 + (c-a).z^2 <= radius^2, in f32, summed in that order).  When x is NULL
  * only counts.  x is P*3 positions, v is P*3 velocities (interleaved xyz).  Returns the count. */
 #define ORC_BLOCK_FLOATS 10
+/* obst_id (extension, may be NULL): the obstacle ids of orc_obstacle_cells; obstacle cells
+ * are skipped. */
 uint64_t orc_seed(const uint32_t n[3], float h, const float origin[3], const float *blocks,
-                  uint32_t nb, uint32_t ppc, uint32_t seed, float *x, float *v);
+                  uint32_t nb, uint32_t ppc, uint32_t seed, const uint32_t *obst_id, float *x, float *v);
+/* Obstacles (extension): solid spheres, rows {cx, cy, cz, r}.  id_out[cell] = 1 + the index of
+ * the first obstacle (list order) holding the centre of interior cell `cell` (the sphere test
+ * of sphere fills), else 0.  Returns the number of obstacle cells. */
+uint64_t orc_obstacle_cells(const uint32_t n[3], float h, const float origin[3], const float *obst,
+                            uint32_t n_obst, uint32_t *id_out);
 uint64_t orc_steps_for_frame(double warmup, double frame, double fps, uint32_t substeps);
 float orc_dt(double fps, uint32_t substeps);
 
 /* ---- state ---- */
 orc_state *orc_create(const orc_params *p);
 void orc_destroy(orc_state *s);
+/* Obstacles (extension): obstacle cells are SOLID in normalize_grid, and g2p_advect pushes a
+ * particle that ends in one out to distance r + h from that obstacle's centre (see
+ * flip_oracle.c).  Returns 0, or -1 when out of memory. */
+int orc_set_obstacles(orc_state *s, const float *obst, uint32_t n_obst);
 
 /* ---- the step kernels, one function per fluid.slang entry point ---- */
 void orc_clear_grid(orc_state *s);                 /* fluid.slang:88-91  */

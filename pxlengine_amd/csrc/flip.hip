@@ -126,6 +126,7 @@ int prepare(const char *fluid_json, double fps, uint32_t seed, const pxlflip_opt
     d.ntz = (int)((n[2] + TILE - 1) / TILE);
     d.ntiles = (uint32_t)d.ntx * (uint32_t)d.nty * (uint32_t)d.ntz;
     d.nkeys = d.ntiles * TILE_CELLS;
+    d.n_obst = (uint32_t)spec.obstacles.size();
     {
         // four times the tiles the liquid fills at seeding (room for the band around it and
         // for spreading), at least 64, at most 1024 and the grid's tile count

@@ -52,13 +52,17 @@ __device__ __forceinline__ void for_band(const DevSim &S, uint32_t b, uint32_t n
     for (uint32_t u = beg; u < end; u++) f(S.band_list[u]);
 }
 
-// SOLID for cells outside the grid or on its one-cell wall layer (fluid.slang:55-58, 126-128).
-__device__ __forceinline__ bool is_solid(const DevSim &S, int i, int j, int k) {
-    return i <= 0 || j <= 0 || k <= 0 || i >= S.nx - 1 || j >= S.ny - 1 || k >= S.nz - 1;
-}
-
 __device__ __forceinline__ uint32_t cidx(const DevSim &S, int i, int j, int k) {
     return (uint32_t)i + (uint32_t)S.nx * ((uint32_t)j + (uint32_t)S.ny * (uint32_t)k);
+}
+
+// SOLID for cells outside the grid or on its one-cell wall layer (fluid.slang:55-58, 126-128),
+// and for obstacle cells (an extension).
+__device__ __forceinline__ bool is_solid(const DevSim &S, int i, int j, int k) {
+    if (i <= 0 || j <= 0 || k <= 0 || i >= S.nx - 1 || j >= S.ny - 1 || k >= S.nz - 1) return true;
+    if (!S.solid_bits) return false;
+    const uint32_t id = cidx(S, i, j, k);
+    return ((S.solid_bits[id >> 5] >> (id & 31u)) & 1u) != 0;
 }
 __device__ __forceinline__ uint32_t uidx(const DevSim &S, int i, int j, int k) {
     return (uint32_t)i + (uint32_t)(S.nx + 1) * ((uint32_t)j + (uint32_t)S.ny * (uint32_t)k);

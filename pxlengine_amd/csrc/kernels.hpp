@@ -101,6 +101,13 @@ struct DevSim {
     uint8_t *type;
     uint16_t *sten;
     float *pressure, *div;
+    // obstacles (an extension; null without): bit (cell & 31) of solid_bits[cell >> 5] is set
+    // for obstacle cells (SOLID like the wall layer), obst_id[cell] = 1 + the index of the
+    // first obstacle in list order holding the cell's centre (0 = none), obst = n_obst rows
+    // {cx, cy, cz, r}
+    uint32_t *solid_bits, *obst_id;
+    float *obst;
+    uint32_t n_obst;
     // FLUID-cell list of the step, built from per-row counts: fl_rows holds the FLUID cells
     // of each grid row (j, k) (2 ny nz words: rows' red cells, then rows' black cells; with
     // colours not split, all of a row's cells in the first half and zeros in the second),

@@ -485,6 +485,34 @@ __device__ __forceinline__ float sample_global(const DevSim &S, int c, const Ste
     return s;
 }
 
+// Obstacle response (an extension), after advection and before the wall clamp: a particle
+// whose binning cell is an obstacle cell moves along the ray from that obstacle's centre to
+// distance r + h (its new cell's centre is then outside the sphere), and an inward normal
+// velocity component is removed: d = x - c, len = sqrt(d.x^2 + d.y^2 + d.z^2), n = d / len,
+// x = c + n (r + h), vn = v.n, v -= vn n when vn < 0.  Sums left to right; IEEE sqrt and
+// division; len == 0 leaves the particle as it is.
+__device__ __forceinline__ void obstacle_push(const DevSim &S, float x[3], float v[3]) {
+    int ci, cj, ck;
+    cell_of_pos(S, x[0], x[1], x[2], ci, cj, ck);
+    const uint32_t id = (uint32_t)ci + (uint32_t)S.nx * ((uint32_t)cj + (uint32_t)S.ny * (uint32_t)ck);
+    if (!((S.solid_bits[id >> 5] >> (id & 31u)) & 1u)) return;
+    const uint32_t o = S.obst_id[id];
+    if (o == 0u || o > S.n_obst) return;
+    const float *sp = S.obst + 4u * (o - 1u);
+    const float d[3] = {x[0] - sp[0], x[1] - sp[1], x[2] - sp[2]};
+    const float len = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    if (!(len > 0.0f)) return;
+    const float nr[3] = {d[0] / len, d[1] / len, d[2] / len};
+    const float rad = sp[3] + S.h;
+#pragma unroll
+    for (int a = 0; a < 3; a++) x[a] = sp[a] + nr[a] * rad;
+    const float vn = v[0] * nr[0] + v[1] * nr[1] + v[2] * nr[2];
+    if (vn < 0.0f) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) v[a] = v[a] - vn * nr[a];
+    }
+}
+
 // g2p_advect (fluid.slang:200-220), one 8^3 tile per workgroup: the tile's vel and vel_old
 // face boxes are staged in LDS, then every particle of the tile does the PIC/FLIP blend
 // (lerp = a*(1-t) + b*t), RK2 through the projected grid velocity (the midpoint sample
@@ -556,6 +584,7 @@ __device__ __forceinline__ void body_g2p(const DevSim &S, uint32_t b0, uint32_t 
         }
 #pragma unroll
         for (int c = 0; c < 3; c++) x[c] = x[c] + S.dt * adv[c];
+        if (S.solid_bits) obstacle_push(S, x, v);
 #pragma unroll
         for (int a = 0; a < 3; a++) {
             const float lo = org[a] + S.h * 1.001f;

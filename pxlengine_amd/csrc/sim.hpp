@@ -161,6 +161,11 @@ struct Sim {
         add(&d.sten, 2 * C);
         add(&d.pressure, 4 * C);
         add(&d.div, 4 * C);
+        if (d.n_obst) {
+            add(&d.solid_bits, 4 * ((C + 31) / 32));
+            add(&d.obst_id, 4 * C);
+            add(&d.obst, 16ull * d.n_obst);
+        }
         add(&d.fl_rows, 4ull * fl_row_slots(d));
         add(&d.fl_rowoff, 4ull * fl_row_slots(d));
         add(&d.fl_list, 4 * C);
@@ -229,6 +234,19 @@ struct Sim {
         for (int a = 0; a < 3; a++) {
             HIP_CHECK(hipMemcpyAsync(dst[a], x0.data() + a * P, 4 * P, hipMemcpyHostToDevice, stream));
             HIP_CHECK(hipMemcpyAsync(dst[3 + a], v0.data() + a * P, 4 * P, hipMemcpyHostToDevice, stream));
+        }
+        // obstacles (extension): cell ids, their bit mask and the spheres
+        std::vector<uint32_t> ids, bits;
+        std::vector<float> spheres;
+        if (d.n_obst) {
+            obstacle_cells(spec, ids);
+            bits.assign(((size_t)d.ncells + 31) / 32, 0u);
+            for (size_t c = 0; c < ids.size(); c++)
+                if (ids[c]) bits[c >> 5] |= 1u << (c & 31);
+            for (const auto &o : spec.obstacles) spheres.insert(spheres.end(), {o.center[0], o.center[1], o.center[2], o.radius});
+            HIP_CHECK(hipMemcpyAsync(d.obst_id, ids.data(), 4 * ids.size(), hipMemcpyHostToDevice, stream));
+            HIP_CHECK(hipMemcpyAsync(d.solid_bits, bits.data(), 4 * bits.size(), hipMemcpyHostToDevice, stream));
+            HIP_CHECK(hipMemcpyAsync(d.obst, spheres.data(), 4 * spheres.size(), hipMemcpyHostToDevice, stream));
         }
         HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&host_scal), 8 * SC_COUNT, hipHostMallocDefault));
         HIP_CHECK(hipStreamSynchronize(stream));

@@ -3,6 +3,7 @@
 // as the reference's Rust (which never contracts) does.
 #include "spec.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <set>
@@ -15,9 +16,10 @@ namespace {
 const char *const kFluidFields[] = {"name", "bounds", "resolution", "fill", "particles_per_cell",
                                     "substeps", "flip", "gravity", "pressure_iterations",
                                     "surface_refine", "particle_radius", "warmup",
-                                    "max_triangles", "materials"};
+                                    "max_triangles", "materials", "obstacles"};
 const char *const kBlockFields[] = {"min", "max", "velocity"};
 const char *const kSphereFields[] = {"center", "radius", "velocity"};  // extension
+const char *const kObstacleFields[] = {"center", "radius"};              // extension
 
 template <size_t N>
 std::string expected_list(const char *const (&fields)[N]) {
@@ -80,6 +82,21 @@ std::string parse_sphere(const JsonValue &v, FluidBlockSpec &b) {
         if (!e.empty()) return e;
     }
     if (!has_radius) return "missing field `radius`";
+    return "";
+}
+
+std::string parse_obstacle(const JsonValue &v, ObstacleSpec &o) {
+    if (v.kind != JsonValue::Object) return "invalid type: " + json_describe(v) + ", expected struct Obstacle";
+    std::set<std::string> seen;
+    for (const auto &m : v.members) {
+        const std::string &k = m.first;
+        if (!known(kObstacleFields, k)) return "unknown field `" + k + "`, expected one of " + expected_list(kObstacleFields);
+        if (!seen.insert(k).second) return "duplicate field `" + k + "`";
+        std::string e = k == "center" ? get_vec3(m.second, o.center) : get_f32(m.second, o.radius);
+        if (!e.empty()) return e;
+    }
+    if (!seen.count("center")) return "missing field `center`";
+    if (!seen.count("radius")) return "missing field `radius`";
     return "";
 }
 
@@ -170,6 +187,16 @@ std::string parse_fluid(const std::string &json, FluidSpec &f) {
             e = get_f32(v, f.warmup);
         } else if (k == "max_triangles") {
             e = get_u32(v, f.max_triangles);
+        } else if (k == "obstacles") {
+            if (v.kind != JsonValue::Array) e = "invalid type: " + json_describe(v) + ", expected a sequence";
+            else {
+                for (const auto &item : v.items) {
+                    ObstacleSpec o;
+                    e = parse_obstacle(item, o);
+                    if (!e.empty()) break;
+                    f.obstacles.push_back(o);
+                }
+            }
         } else if (k == "materials") {
             // Rendering materials (OpenPBR inputs) do not enter the simulation; the list is
             // accepted as a sequence of objects and otherwise not interpreted.
@@ -230,6 +257,10 @@ std::string validate_fluid(const FluidSpec &f, uint32_t index, uint64_t max_cell
             return ctx + "each fill block must have max > min and lie inside `bounds`";
         }
     }
+    if (f.obstacles.size() > MAX_OBSTACLES) return ctx + "at most 65536 obstacles";
+    for (const auto &o : f.obstacles)
+        if (!(o.radius > 0.0f) || any_lt(o.center, lo) || any_lt(hi, o.center))
+            return ctx + "each obstacle must have radius > 0 and its center inside `bounds`";
     uint32_t n[3];
     float h;
     fluid_grid(f, n, &h);
@@ -251,11 +282,51 @@ inline float lcg_next(uint64_t &s) {
 }
 }  // namespace
 
+uint64_t obstacle_cells(const FluidSpec &f, std::vector<uint32_t> &ids) {
+    ids.clear();
+    if (f.obstacles.empty()) return 0;
+    uint32_t n[3];
+    float h;
+    fluid_grid(f, n, &h);
+    const float *org = f.bounds[0];
+    ids.assign((size_t)n[0] * n[1] * n[2], 0u);
+    uint64_t count = 0;
+    // Obstacles in list order, each over the interior cells of its bounding box widened by one
+    // cell (the f32 test below decides); a cell keeps the first obstacle that claims it.
+    for (size_t q = 0; q < f.obstacles.size(); q++) {
+        const ObstacleSpec &ob = f.obstacles[q];
+        int lo[3], hi[3];
+        for (int a = 0; a < 3; a++) {
+            const double c = ((double)ob.center[a] - (double)org[a]) / (double)h - 0.5, r = (double)ob.radius / (double)h;
+            lo[a] = std::max(1, (int)std::floor(c - r) - 1);
+            hi[a] = std::min((int)n[a] - 2, (int)std::ceil(c + r) + 1);
+        }
+        for (int k = lo[2]; k <= hi[2]; k++)
+            for (int j = lo[1]; j <= hi[1]; j++)
+                for (int i = lo[0]; i <= hi[0]; i++) {
+                    const size_t id = (size_t)i + (size_t)n[0] * ((size_t)j + (size_t)n[1] * (size_t)k);
+                    if (ids[id]) continue;
+                    const float cx = org[0] + ((float)i + 0.5f) * h;
+                    const float cy = org[1] + ((float)j + 0.5f) * h;
+                    const float cz = org[2] + ((float)k + 0.5f) * h;
+                    const float dx = cx - ob.center[0], dy = cy - ob.center[1], dz = cz - ob.center[2];
+                    const float d2 = dx * dx + dy * dy + dz * dz;
+                    if (d2 <= ob.radius * ob.radius) {
+                        ids[id] = (uint32_t)q + 1u;
+                        count++;
+                    }
+                }
+    }
+    return count;
+}
+
 uint64_t seed_particles(const FluidSpec &f, uint32_t seed, std::vector<float> *x, std::vector<float> *v) {
     uint32_t n[3];
     float h;
     fluid_grid(f, n, &h);
     const float *o = f.bounds[0];
+    std::vector<uint32_t> obst;
+    obstacle_cells(f, obst);
     uint64_t rng = 0x9e3779b97f4a7c15ull ^ (uint64_t)seed;
     uint64_t count = 0;
     std::vector<float> xs[3], vs[3];
@@ -274,6 +345,8 @@ uint64_t seed_particles(const FluidSpec &f, uint32_t seed, std::vector<float> *x
                                cz < b.min[2] || cz > b.max[2]) {
                         continue;
                     }
+                    if (!obst.empty() && obst[(size_t)i + (size_t)n[0] * ((size_t)j + (size_t)n[1] * (size_t)k)])
+                        continue;  // extension: no liquid inside obstacles
                     for (uint32_t q = 0; q < f.particles_per_cell; q++) {
                         float px = o[0] + ((float)i + lcg_next(rng)) * h;
                         float py = o[1] + ((float)j + lcg_next(rng)) * h;

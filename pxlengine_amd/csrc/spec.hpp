@@ -23,6 +23,13 @@ struct FluidBlockSpec {
     float radius = 0.0f;
 };
 
+// A solid sphere inside the domain (an extension: the reference has no obstacles).
+struct ObstacleSpec {
+    float center[3] = {0, 0, 0};
+    float radius = 0.0f;
+};
+constexpr uint32_t MAX_OBSTACLES = 65536;
+
 struct FluidSpec {
     std::string name;
     float bounds[2][3] = {{0, 0, 0}, {0, 0, 0}};
@@ -38,6 +45,7 @@ struct FluidSpec {
     float warmup = 0.0f;
     uint32_t max_triangles = 2000000;
     uint32_t n_materials = 0;  // rendering-only; accepted, not interpreted
+    std::vector<ObstacleSpec> obstacles;  // extension
 };
 
 // Strict parse of one fluids[] object.  Returns "" on success, else the error message.
@@ -57,8 +65,14 @@ float fluid_dt(double fps, uint32_t substeps);
 // round((warmup + frame / fps) * fps * substeps).max(0) (fluid.rs:234-236).
 uint64_t steps_for_frame(double warmup, double frame, double fps, uint32_t substeps);
 
+// Obstacle cells (extension): ids[cell] = 1 + the index of the first obstacle (list order)
+// whose sphere holds the centre of interior cell `cell` ((c - o).x^2 + (c - o).y^2 +
+// (c - o).z^2 <= r^2 in f32, the sphere-fill test), else 0.  Returns the number of obstacle
+// cells; ids is left empty without obstacles.
+uint64_t obstacle_cells(const FluidSpec &f, std::vector<uint32_t> &ids);
+
 // Reference seeding into SoA arrays (x[3][P], v[3][P]); returns the particle count.
-// With out == nullptr only counts.
+// With out == nullptr only counts.  Obstacle cells (extension) receive no particles.
 uint64_t seed_particles(const FluidSpec &f, uint32_t seed, std::vector<float> *x,
                         std::vector<float> *v);
 

@@ -52,15 +52,50 @@ def sphere_drop(res: int, ppc: int = 6, iters: int = 40) -> dict:
     }
 
 
+def obstacle_scene(res: int, ppc: int = 4, iters: int = 40) -> dict:
+    """Liquid around and above solid spheres (obstacles, an extension), thrown down onto them;
+    the last obstacle overlaps the first."""
+    return {
+        "name": f"obstacles{res}", "bounds": [[0, 0, 0], [1, 1, 1]], "resolution": res,
+        "fill": [{"min": [0.05, 0.05, 0.05], "max": [0.95, 0.6, 0.95], "velocity": [0.3, -1.5, 0.2]}],
+        "obstacles": [{"center": [0.5, 0.3, 0.5], "radius": 0.15}, {"center": [0.25, 0.25, 0.7], "radius": 0.1},
+                      {"center": [0.72, 0.4, 0.28], "radius": 0.12}, {"center": [0.6, 0.35, 0.55], "radius": 0.1}],
+        "particles_per_cell": ppc, "substeps": 5, "flip": 0.95, "pressure_iterations": iters,
+    }
+
+
+def obstacle_slab(res: int = 192, seed: int = 5, solid_fraction: float = 0.05, ppc: int = 8) -> dict:
+    """Obstacle-heavy scene on [0,2] x [0,1] x [0,2] (res cells along x and z, res/2 along y):
+    solid spheres of radius 2-8 cells at seeded random centres until their summed volume
+    reaches `solid_fraction` of the domain, and a liquid slab over the full width from y = 0.25
+    to the top, falling onto them."""
+    rng = np.random.default_rng(seed)
+    h = 2.0 / res
+    obst, vol = [], 0.0
+    while vol < solid_fraction * 2.0 * 1.0 * 2.0:
+        r = float(rng.uniform(2.0, 8.0)) * h
+        c = [float(rng.uniform(r + h, 2.0 - r - h)), float(rng.uniform(r + h, 1.0 - r - h)),
+             float(rng.uniform(r + h, 2.0 - r - h))]
+        obst.append({"center": c, "radius": r})
+        vol += 4.0 / 3.0 * np.pi * r ** 3
+    return {
+        "name": f"slab{res}", "bounds": [[0, 0, 0], [2, 1, 2]], "resolution": res,
+        "fill": [{"min": [0.0, 0.25, 0.0], "max": [2.0, 1.0, 2.0]}], "obstacles": obst,
+        "particles_per_cell": ppc, "substeps": 5, "flip": 0.95,
+    }
+
+
 def oracle_sim(orc, fluid: dict, fps: float = FPS, seed: int = 0):
     """Builds the oracle simulation for a fluid block (defaults as crates/pxl-scene/src/lib.rs:96-119)."""
     lo, hi = fluid["bounds"]
     n, h = orc.fluid_grid(lo, hi, fluid.get("resolution", 64))
     blocks = [("sphere", b["center"], b["radius"], b.get("velocity", [0, 0, 0])) if "center" in b
               else (b["min"], b["max"], b.get("velocity", [0, 0, 0])) for b in fluid["fill"]]
-    x, v = orc.seed_particles(n, h, lo, blocks, fluid.get("particles_per_cell", 8), seed)
+    obst = [(o["center"], o["radius"]) for o in fluid.get("obstacles", [])]
+    x, v = orc.seed_particles(n, h, lo, blocks, fluid.get("particles_per_cell", 8), seed, obstacles=obst)
     dt = orc.dt_for(fps, fluid.get("substeps", 8))
-    return orc.Sim(n, h, lo, dt, fluid.get("flip", 0.95), fluid.get("gravity", [0, -9.81, 0]), x, v)
+    return orc.Sim(n, h, lo, dt, fluid.get("flip", 0.95), fluid.get("gravity", [0, -9.81, 0]), x, v,
+                   obstacles=obst)
 
 
 def aos8(x: np.ndarray, v: np.ndarray) -> np.ndarray:

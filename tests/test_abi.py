@@ -162,6 +162,28 @@ def test_sphere_fill_seeding_matches_oracle(pxl, orc):
     assert "unknown field `min`" in str(e)
 
 
+def test_obstacles_parse_validate_and_seed(pxl, orc):
+    """Obstacles (an extension): strict parsing and checks; no liquid is seeded in obstacle
+    cells (host seeding equal to the oracle's)."""
+    from tests.scenes import obstacle_scene, obstacle_slab, oracle_sim
+    f = obstacle_scene(20)
+    a = pxl.seed_particles(f, 3)
+    sim = oracle_sim(orc, f, seed=3)
+    plain = oracle_sim(orc, dict(f, obstacles=[]), seed=3)
+    assert a.shape == (sim.P, 8) and 0 < sim.P < plain.P
+    np.testing.assert_array_equal(a[:, 0:3], sim.x)
+    np.testing.assert_array_equal(a[:, 4:7], sim.v)
+    slab = obstacle_slab(64)
+    assert pxl.check(slab)["n_particles"] == oracle_sim(orc, slab).P
+    msg = "each obstacle must have radius > 0 and its center inside `bounds`"
+    assert msg in str(_err(pxl, dict(f, obstacles=[{"center": [0.5, 0.5, 0.5], "radius": 0}])))
+    assert msg in str(_err(pxl, dict(f, obstacles=[{"center": [1.5, 0.5, 0.5], "radius": 0.1}])))
+    assert "missing field `radius`" in str(_err(pxl, dict(f, obstacles=[{"center": [0.5, 0.5, 0.5]}])))
+    e = _err(pxl, dict(f, obstacles=[{"center": [0.5, 0.5, 0.5], "radius": 0.1, "velocity": [0, 0, 0]}]))
+    assert "unknown field `velocity`" in str(e)
+    assert "expected a sequence" in str(_err(pxl, dict(f, obstacles={"center": [0.5, 0.5, 0.5]})))
+
+
 def test_device_bytes_reported(pxl):
     small = pxl.check(dam_break(32))["device_bytes"]
     mixed = pxl.check(dam_break(32), solver="mixed")["device_bytes"]

@@ -9,7 +9,7 @@ stated tolerances; the mixed solver is checked against the FP64 Poisson oracle.
 import numpy as np
 import pytest
 
-from tests.scenes import FPS, EXAMPLE_DAM_BREAK, dam_break, oracle_sim, sphere_drop, two_blocks
+from tests.scenes import FPS, EXAMPLE_DAM_BREAK, dam_break, obstacle_scene, oracle_sim, sphere_drop, two_blocks
 
 pytestmark = pytest.mark.gpu
 
@@ -44,9 +44,25 @@ def test_grid_phase_bitwise(pxl, orc, res):
     gpu.close()
 
 
+def test_obstacles_grid_phase_bitwise(pxl, orc):
+    """Obstacles (an extension): after seeding and one grid phase every field, the cell types
+    (obstacle cells SOLID) included, equals the oracle bit for bit."""
+    fluid = obstacle_scene(32)
+    print("obstacles grid phase", flush=True)
+    gpu = pxl.FluidState(fluid, FPS, seed=0)
+    sim = oracle_sim(orc, fluid)
+    assert gpu.particle_count() == sim.P
+    gpu.grid_phase()
+    sim.grid_phase()
+    assert (sim.type[sim.obst_id > 0] == 2).all()
+    for name, ref in [("vel_old", sim.vel_old), ("vel", sim.vel), ("type", sim.type), ("divergence", sim.div)]:
+        np.testing.assert_array_equal(gpu.read_field(name), ref, err_msg=name)
+    gpu.close()
+
+
 @pytest.mark.parametrize("fluid,steps", [(dam_break(32), 3), (EXAMPLE_DAM_BREAK, 2), (two_blocks(24), 3),
-                                         (sphere_drop(28), 3)],
-                         ids=["dam32", "example", "collide24", "sphere28"])
+                                         (sphere_drop(28), 3), (obstacle_scene(24), 4)],
+                         ids=["dam32", "example", "collide24", "sphere28", "obstacles24"])
 def test_rbgs_steps_bitwise(pxl, orc, fluid, steps):
     """Whole reference steps (RBGS, pressure_iterations sweeps): particles, pressure and face
     velocities equal the oracle bit for bit after every step."""
@@ -145,7 +161,8 @@ def test_cg_solution_residual(pxl, orc):
     gpu.close()
 
 
-@pytest.mark.parametrize("fluid", [dam_break(32), two_blocks(32)], ids=["dam32", "collide32"])
+@pytest.mark.parametrize("fluid", [dam_break(32), two_blocks(32), obstacle_scene(32)],
+                         ids=["dam32", "collide32", "obstacles32"])
 def test_mixed_solve_vs_fp64_oracle(pxl, orc, fluid):
     """Mixed precision: FP64 relative residual <= 1e-6 (recomputed by the oracle from the
     GPU's FP64 solution) and relative error vs the oracle's FP64 CG solution (to 1e-12)

@@ -7,7 +7,7 @@ import time
 import numpy as np
 import pytest
 
-from tests.scenes import FPS, dam_break, random_request, two_blocks
+from tests.scenes import FPS, dam_break, obstacle_scene, random_request, two_blocks
 
 pytestmark = pytest.mark.gpu
 
@@ -47,18 +47,18 @@ def test_batched_equals_single_sim_path(pxl):
 def test_alone_equals_in_the_mix(pxl):
     a, b, c = dam_break(32), two_blocks(24), random_request(3, sizes=(16, 40))
     d = random_request(1, sizes=(16, 40))  # a sphere fill
+    e = obstacle_scene(24)
     assert "center" in d["fill"][0] and "min" in c["fill"][0]
     (pa,), _, _ = _run(pxl, [(a, 4)])
-    parts, infos, st = _run(pxl, [(b, 2), (a, 4), (c, 5), (d, 3)])
+    parts, infos, st = _run(pxl, [(b, 2), (a, 4), (c, 5), (d, 3), (e, 4)])
     for info in infos:
         assert info["status"] == "done"
         assert 0 <= info["rel_residual"] <= 1e-6
     assert np.array_equal(pa, parts[1])
-    assert st["completed"] == 4 and st["mean_batch"] > 1.0
-    (pc,), _, _ = _run(pxl, [(c, 5)])
-    assert np.array_equal(pc, parts[2])
-    (pd,), _, _ = _run(pxl, [(d, 3)])
-    assert np.array_equal(pd, parts[3])
+    assert st["completed"] == 5 and st["mean_batch"] > 1.0
+    for k, (f, n) in ((2, (c, 5)), (3, (d, 3)), (4, (e, 4))):
+        (alone,), _, _ = _run(pxl, [(f, n)])
+        assert np.array_equal(alone, parts[k]), f["name"]
 
 
 def test_many_small_requests_complete_with_stats(pxl):

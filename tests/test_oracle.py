@@ -19,7 +19,7 @@ Sources of truth, per function:
 import numpy as np
 import pytest
 
-from tests.scenes import EXAMPLE_DAM_BREAK, FPS, dam_break, oracle_sim
+from tests.scenes import EXAMPLE_DAM_BREAK, FPS, dam_break, obstacle_scene, oracle_sim
 
 G = np.float32(9.81)
 
@@ -147,6 +147,51 @@ def test_seed_sphere_brute_force(orc, res, ppc, seed):
     ns = ppc * len(sc)
     np.testing.assert_array_equal(v[:ns], np.tile(np.float32([1, 0, -1]), (ns, 1)))
     np.testing.assert_array_equal(v[ns:], np.tile(np.float32([0, 2, 0]), (len(x) - ns, 1)))
+
+
+def test_obstacle_cells_brute_force(orc):
+    """Obstacle cells (an extension): interior cells whose f32 centre a numpy brute force finds
+    in a sphere, labelled with the first such sphere in list order (the scene's last sphere
+    overlaps its first)."""
+    f = obstacle_scene(20)
+    lo, hi = f["bounds"]
+    n, h = orc.fluid_grid(lo, hi, 20)
+    obst = [(o["center"], o["radius"]) for o in f["obstacles"]]
+    ids = orc.obstacle_cells(n, h, lo, obst)
+    expect = np.zeros(n[0] * n[1] * n[2], np.uint32)
+    overlap = 0
+    for k in range(1, n[2] - 1):
+        for j in range(1, n[1] - 1):
+            for i in range(1, n[0] - 1):
+                c = np.asarray(lo, np.float32) + (np.array([i, j, k], np.float32) + np.float32(0.5)) * np.float32(h)
+                hits = [q for q, (o, r) in enumerate(obst)
+                        if (lambda d: (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2])(c - np.asarray(o, np.float32))
+                        <= np.float32(r) * np.float32(r)]
+                if hits:
+                    expect[i + n[0] * (j + n[1] * k)] = hits[0] + 1
+                    overlap += len(hits) > 1
+    np.testing.assert_array_equal(ids, expect)
+    assert overlap > 0 and set(np.unique(ids)) == {0, 1, 2, 3, 4}
+
+
+def test_seed_skips_obstacle_cells(orc):
+    """Fill cells that are obstacle cells receive no particles and draw nothing from the LCG:
+    the positions are the closed form over the remaining cells."""
+    f = obstacle_scene(16, ppc=3)
+    lo, hi = f["bounds"]
+    n, h = orc.fluid_grid(lo, hi, 16)
+    blk = f["fill"][0]
+    obst = [(o["center"], o["radius"]) for o in f["obstacles"]]
+    ids = orc.obstacle_cells(n, h, lo, obst)
+    x, v = orc.seed_particles(n, h, lo, [(blk["min"], blk["max"], [0, 0, 0])], 3, 2, obstacles=obst)
+    cells = [c for c in _cells_in_block(n, h, lo, blk["min"], blk["max"])
+             if ids[c[0] + n[0] * (c[1] + n[1] * c[2])] == 0]
+    assert 0 < len(cells) < len(_cells_in_block(n, h, lo, blk["min"], blk["max"]))
+    assert len(x) == 3 * len(cells)
+    ints, _ = _lcg_ints(0x9E3779B97F4A7C15 ^ 2, 3 * len(x))
+    r = (np.array(ints, np.float64).astype(np.float32) / np.float32(1 << 24)).reshape(-1, 3)
+    idx = np.repeat(np.array(cells, np.float32), 3, axis=0)
+    np.testing.assert_array_equal(x, np.asarray(lo, np.float32) + (idx + r) * np.float32(h))
 
 
 def test_seed_counts_configs(orc):
@@ -384,6 +429,116 @@ def test_poisson_matrix_vs_scipy(orc):
     xs = spl.spsolve(A.tocsc(), b)
     xo, it, rr = sim.poisson_solve64(tol=1e-13)
     np.testing.assert_allclose(xo[lin], xs, rtol=1e-8, atol=1e-9 * np.abs(xs).max())
+
+
+def test_obstacle_cells_solid_and_faces_closed(orc):
+    """Obstacle cells are SOLID after normalize (walls too; FLUID/AIR elsewhere from the
+    count), every face touching one has zero velocity after forces, and the FP64 matrix over
+    such a grid matches an independent assembly (diag = non-solid neighbours)."""
+    import scipy.sparse as sp
+    f = obstacle_scene(14)
+    sim = oracle_sim(orc, f)
+    sim.step_rbgs(10)
+    sim.grid_phase()
+    n = sim.n
+    t = sim.type.reshape(n[2], n[1], n[0])
+    ob = (sim.obst_id > 0).reshape(t.shape)
+    assert ob.any() and np.all(t[ob] == 2)
+    inner = np.zeros(t.shape, bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    free = inner & ~ob
+    np.testing.assert_array_equal(t[free], np.where(sim.count.reshape(t.shape)[free] > 0, 1, 0))
+    nx, ny, nz = n
+    nu, nv = (nx + 1) * ny * nz, nx * (ny + 1) * nz
+    solid = np.pad(t == 2, 1, constant_values=True)  # cells outside the grid are SOLID
+    u = sim.vel[:nu].reshape(nz, ny, nx + 1)
+    vv = sim.vel[nu:nu + nv].reshape(nz, ny + 1, nx)
+    w = sim.vel[nu + nv:].reshape(nz + 1, ny, nx)
+    su = solid[1:-1, 1:-1, :-1] | solid[1:-1, 1:-1, 1:]
+    sv = solid[1:-1, :-1, 1:-1] | solid[1:-1, 1:, 1:-1]
+    sw = solid[:-1, 1:-1, 1:-1] | solid[1:, 1:-1, 1:-1]
+    assert su.any() and not u[su].any() and not vv[sv].any() and not w[sw].any()
+    assert u[~su].any() or vv[~sv].any()
+    fluid = np.argwhere(t == 1)
+    lin = fluid[:, 2] + nx * (fluid[:, 1] + ny * fluid[:, 0])
+    pos = {tuple(c): q for q, c in enumerate(fluid)}
+    rows, cols, vals = [], [], []
+    for r, (k, j, i) in enumerate(fluid):
+        diag = 0
+        for dk, dj, di in [(0, 0, -1), (0, 0, 1), (0, -1, 0), (0, 1, 0), (-1, 0, 0), (1, 0, 0)]:
+            q = (k + dk, j + dj, i + di)
+            if not solid[q[0] + 1, q[1] + 1, q[2] + 1]:
+                diag += 1
+            if q in pos:
+                rows.append(r); cols.append(pos[q]); vals.append(-1.0)
+        rows.append(r); cols.append(r); vals.append(float(diag))
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(len(fluid), len(fluid)))
+    xr = np.zeros(sim.C)
+    xr[lin] = np.random.default_rng(4).standard_normal(len(fluid))
+    np.testing.assert_allclose(sim.poisson_apply64(xr)[lin], A @ xr[lin], rtol=1e-13, atol=1e-12)
+
+
+def test_obstacle_push_invariants(orc):
+    """Particles that end a G2P in an obstacle cell (zero grid velocity, so they stay put)
+    move along the ray from that obstacle's centre to distance r + h; their new cell's centre
+    is outside that sphere; an inward normal velocity is removed and the tangential part
+    kept; an outward one is kept whole.  Particles elsewhere are bitwise those of the same
+    G2P without obstacles."""
+    n, h = (16, 16, 16), np.float32(1.0 / 16)
+    centre, radius = np.float32([0.5, 0.45, 0.5]), np.float32(0.2)
+    rng = np.random.default_rng(9)
+    dirs = rng.standard_normal((400, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    x = (centre + dirs * rng.uniform(0.02, 0.3, (400, 1))).astype(np.float32)
+    v = rng.uniform(-1, 1, (400, 3)).astype(np.float32)
+    a = orc.Sim(n, h, [0, 0, 0], np.float32(0.01), np.float32(1.0), [0, 0, 0], x, v,
+                obstacles=[(centre, radius)])
+    b = orc.Sim(n, h, [0, 0, 0], np.float32(0.01), np.float32(1.0), [0, 0, 0], x, v)
+    for s in (a, b):
+        s.vel[:] = 0
+        s.vel_old[:] = 0
+        s.g2p_advect()
+    cell = np.clip(np.floor(x * np.float32(16)).astype(int), 0, 15)
+    inside = a.obst_id[cell[:, 0] + 16 * (cell[:, 1] + 16 * cell[:, 2])] > 0
+    assert 20 < inside.sum() < 380
+    np.testing.assert_array_equal(a.x[~inside], b.x[~inside])
+    np.testing.assert_array_equal(a.v[~inside], b.v[~inside])
+    d0 = (x[inside] - centre).astype(np.float64)
+    d1 = (a.x[inside] - centre).astype(np.float64)
+    nr = d0 / np.linalg.norm(d0, axis=1, keepdims=True)
+    np.testing.assert_allclose(np.linalg.norm(d1, axis=1), radius + h, rtol=1e-6)
+    np.testing.assert_allclose(d1 / np.linalg.norm(d1, axis=1, keepdims=True), nr, atol=1e-6)
+    c1 = np.clip(np.floor(a.x[inside] * np.float32(16)).astype(int), 0, 15)
+    centres = (c1 + 0.5) / 16.0
+    assert np.all(np.linalg.norm(centres - centre, axis=1) > radius)
+    v0 = v[inside].astype(np.float64)
+    v1 = a.v[inside].astype(np.float64)
+    vn0 = np.sum(v0 * nr, axis=1)
+    np.testing.assert_allclose(np.sum(v1 * nr, axis=1), np.maximum(vn0, 0), atol=1e-6)
+    tang0 = v0 - vn0[:, None] * nr
+    np.testing.assert_allclose(v1 - np.sum(v1 * nr, axis=1)[:, None] * nr, tang0, atol=1e-6)
+    out = vn0 >= 0
+    assert out.any() and (~out).any()
+    np.testing.assert_array_equal(a.v[inside][out], v[inside][out])
+
+
+def test_obstacle_scene_exercises_the_push(orc):
+    """The obstacle parity scene (4 reference steps at res 24) moves particles out of
+    obstacles: after each step some sit exactly at distance r + h from an obstacle centre
+    (pushed in that step), and (almost) none ends a step in an obstacle cell (only a push
+    into an overlapping obstacle can leave one there)."""
+    f = obstacle_scene(24)
+    sim = oracle_sim(orc, f, seed=7)
+    pushed, stuck = 0, 0
+    for _ in range(4):
+        sim.step_rbgs(40)
+        x = sim.x.astype(np.float64)
+        for o in f["obstacles"]:
+            d = np.linalg.norm(x - np.asarray(o["center"]), axis=1)
+            pushed += int(np.sum(np.abs(d - (o["radius"] + float(sim.params.h))) < 2e-6))
+        cell = np.clip(np.floor(sim.x * np.float32(24)).astype(int), 0, 23)
+        stuck += int((sim.obst_id[cell[:, 0] + 24 * (cell[:, 1] + 24 * cell[:, 2])] > 0).sum())
+    assert pushed > 20 and stuck <= 2
 
 
 def test_rbgs_half_sweep_touches_one_colour(orc):
