@@ -28,7 +28,9 @@ algorithmic bytes over the simulations each covers).  `cpu_baseline` times the C
 
 `--workload dam` measures the single-instance 128^3 dam break on the plain path instead;
 `--workload batch` only the fixed batch used for the roofline (for profiling runs);
-`--workload mixed256` the 256^3 two-block mixed-precision configuration (~58M particles).
+`--workload mixed256` the 256^3 two-block mixed-precision configuration (~58M particles);
+`--workload obstacles` the 192x96x192 obstacle-heavy configuration (solid spheres, 19.4M
+particles, 200 steps, mixed-precision solve).
 """
 from __future__ import annotations
 
@@ -118,15 +120,13 @@ def bench_dam(args, pxl):
     }
 
 
-def bench_mixed256(args, pxl):
-    """The mixed-precision configuration: 256^3 grid (above the reference's 16M-cell limit, an
-    extension enabled with max_cells), two liquid blocks colliding, ~58M particles, MIXED solve
-    (FP32 multigrid CG inside FP64 refinement) to 1e-6 every step.  Reports particle-steps/s,
-    the FP64 relative residual and refinement passes of every step, and the per-phase times
-    of two profiled steps."""
-    from tests.scenes import FPS, two_blocks
-    fluid = two_blocks(256)
-    sim = pxl.FluidState(fluid, FPS, seed=0, solver="mixed", tolerance=1e-6, max_cells=1 << 25)
+def bench_single_mixed(args, pxl, fluid, data, workload, max_cells=0):
+    """One large simulation with the MIXED solve (FP32 multigrid CG inside FP64 refinement) to
+    1e-6 every step: particle-steps/s over --steps steps after --warmup, the FP64 relative
+    residual and refinement passes of every step, and the per-phase times of two profiled
+    steps."""
+    from tests.scenes import FPS
+    sim = pxl.FluidState(fluid, FPS, seed=0, solver="mixed", tolerance=1e-6, max_cells=max_cells)
     P = sim.particle_count()
     for _ in range(args.warmup):
         sim.step(1)
@@ -144,15 +144,35 @@ def bench_mixed256(args, pxl):
         "metric": METRIC, "value": P * args.steps / elapsed, "unit": "particle-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic (two colliding liquid blocks, reference LCG seeding)",
-        "config": {"workload": "256^3 grid, two blocks at 8 particles/cell colliding at 1.5 m/s, single instance, "
-                               "MIXED solve to 1e-6 every step",
-                   "particles": P, "max_rel_residual": max(res), "outer_passes": {str(k): outer.count(k) for k in sorted(set(outer))},
+        "data": data,
+        "config": {"workload": workload, "particles": P, "max_rel_residual": max(res),
+                   "outer_passes": {str(k): outer.count(k) for k in sorted(set(outer))},
                    "residual_per_step": [float(f"{r:.3g}") for r in res]},
         "roofline": None, "cpu_baseline": None,
         "phases": [{"phase": name, "launches_per_step": launches // 2, "ms_per_step": total_ms / 2}
                    for name, launches, total_ms in prof],
     }
+
+
+def bench_mixed256(args, pxl):
+    """The mixed-precision configuration: 256^3 grid (above the reference's 16M-cell limit, an
+    extension enabled with max_cells), two liquid blocks colliding, ~58M particles."""
+    from tests.scenes import two_blocks
+    return bench_single_mixed(args, pxl, two_blocks(256), "synthetic (two colliding liquid blocks, reference LCG seeding)",
+                              "256^3 grid, two blocks at 8 particles/cell colliding at 1.5 m/s, single instance, "
+                              "MIXED solve to 1e-6 every step", max_cells=1 << 25)
+
+
+def bench_obstacles(args, pxl):
+    """The obstacle-heavy configuration: 192 x 96 x 192 grid on [0,2] x [0,1] x [0,2], solid
+    spheres (an extension) of radius 2-8 cells at seeded random places covering ~5% of the
+    volume, a 19.4M-particle liquid slab falling onto them."""
+    from tests.scenes import obstacle_slab
+    f = obstacle_slab(192)
+    return bench_single_mixed(args, pxl, f, "synthetic (seeded solid spheres, liquid slab, reference LCG seeding)",
+                              f"192x96x192 grid, {len(f['obstacles'])} solid spheres of radius 2-8 cells (~5% of the "
+                              "volume), liquid slab y in [0.25, 1] at 8 particles/cell, single instance, MIXED solve "
+                              "to 1e-6 every step")
 
 
 def cpu_dam(args, fluid):
@@ -332,12 +352,14 @@ def cpu_serve(args, reqs):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=60, help="steps per request (dam: timed steps)")
+    ap.add_argument("--steps", type=int, default=None,
+                    help="steps per request (default 60); dam, mixed256, obstacles: timed steps (default 60, "
+                         "obstacles 200)")
     ap.add_argument("--warmup", type=int, default=64,
                     help="calibration batch: the stream's first N requests, whose service rate sets the arrival "
                          "rate (dam: warm-up steps)")
     ap.add_argument("--calib-steps", type=int, default=10, help="timed steps of the warm-up batch")
-    ap.add_argument("--workload", default="serve", choices=["serve", "dam", "batch", "mixed256"])
+    ap.add_argument("--workload", default="serve", choices=["serve", "dam", "batch", "mixed256", "obstacles"])
     ap.add_argument("--requests", type=int, default=512)
     ap.add_argument("--sizes", default="32,48,64,96,128")
     ap.add_argument("--rate", type=float, default=0.0, help="arrivals per second (0 = measured service rate)")
@@ -346,6 +368,8 @@ def main():
     ap.add_argument("--profile-batch", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU oracle sample length (0 = skip)")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 200 if args.workload == "obstacles" else 60
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: every box here has one MI355X and this build has no "
@@ -354,7 +378,8 @@ def main():
         sys.exit(2)
 
     import pxlengine_amd as pxl
-    run = {"serve": bench_serve, "dam": bench_dam, "batch": bench_batch, "mixed256": bench_mixed256}[args.workload]
+    run = {"serve": bench_serve, "dam": bench_dam, "batch": bench_batch, "mixed256": bench_mixed256,
+           "obstacles": bench_obstacles}[args.workload]
     line = run(args, pxl)
     print(json.dumps(line))
 
