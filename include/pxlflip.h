@@ -145,9 +145,41 @@ int pxlflip_write_particles(pxlflip *h, const float *aos8);
 #define PXLFLIP_FIELD_PRESSURE 6   /* float[n_cells] */
 #define PXLFLIP_FIELD_DIVERGENCE 7 /* float[n_cells] */
 #define PXLFLIP_FIELD_PRESSURE64 8 /* double[n_cells]: FP64 solution of the MIXED solver */
+/* Surface-net fields of the last pxlflip_surface pass (fluid.slang:39-41; nodes (sx+1)(sy+1)(sz+1),
+ * cells sx sy sz, x fastest; sizes in pxlflip_surface_info). */
+#define PXLFLIP_FIELD_DENSITY 9    /* int32[nodes]: splatted density, fixed point 4096 */
+#define PXLFLIP_FIELD_PHI 10       /* float[nodes]: smoothed density / 4096 - iso, -1 on the boundary */
+#define PXLFLIP_FIELD_CELL_VERT 11 /* uint32[cells]: the cell's vertex, 0xffffffff for none */
 int pxlflip_read_field(pxlflip *h, int field, void *out);
 /* Writable: VEL, VEL_OLD, PRESSURE (teacher forcing of the warm start). */
 int pxlflip_write_field(pxlflip *h, int field, const void *in);
+
+/* ---- Surface nets (replaces the per-frame surface pass of fluid_advance,
+ * crates/pxl-render/src/fluid.rs:308-330, running shaders/fluid.slang:222-355) ---- */
+typedef struct pxlflip_surface_info {
+    uint32_t sx, sy, sz;      /* surface cells per axis: n * surface_refine (fluid.rs:151) */
+    uint32_t max_triangles;   /* the fluid's max_triangles: capacity in vertices, and in triangles */
+    uint32_t vertices;        /* vertices written: min(surface cells, max_triangles) */
+    uint32_t triangles;       /* triangles written: min(2 quads, max_triangles), rounded down to even */
+    uint32_t vertices_total;  /* the reference's counters[0] and [1] before clipping */
+    uint32_t triangles_total;
+    uint32_t overflow;        /* 1 when a count was clipped (FluidState::overflow, fluid.rs:328) */
+    float sh, radius, iso;    /* node spacing, splat radius, iso level (fluid.rs:152-156) */
+} pxlflip_surface_info;
+
+/* Meshes the current particles: splat -> phi -> vertices -> quads.  verts and indices are
+ * DEVICE pointers to the renderer's shared buffers (bindings 43 / 44, fluid.rs:257-258):
+ * vertex v is written as 8 floats {x, y, z, 0, nx, ny, nz, 0} at verts + 8 (vbase + v), triangle
+ * t as 3 vertex numbers local to this fluid at indices + ibase + 3 t, as the reference writes
+ * them.  Capacity: max_triangles vertices and 3 max_triangles indices from the bases
+ * (FluidState::reservation, fluid.rs:104-106); verts 16-byte aligned.  Vertices are numbered in surface-cell order and
+ * triangles in (node, axis) order (the reference's follow its atomic counters).  info may be
+ * NULL. */
+int pxlflip_surface(pxlflip *h, float *verts, uint32_t vbase, uint32_t *indices, uint32_t ibase,
+                    pxlflip_surface_info *info);
+/* The same mesh copied to host arrays (capacity as above; only the written counts are copied).
+ * Either array may be NULL. */
+int pxlflip_surface_host(pxlflip *h, float *verts, uint32_t *indices, pxlflip_surface_info *info);
 
 /* Counting-sort binning of the current particle positions: the per-cell particle count of
  * p2g (fluid.slang:113-114) plus the sort the build adds.  cell_of[P] = cell of each

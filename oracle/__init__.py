@@ -44,6 +44,22 @@ class _State(C.Structure):
     ]
 
 
+class SurfaceParams(C.Structure):
+    _fields_ = [
+        ("sx", C.c_uint32), ("sy", C.c_uint32), ("sz", C.c_uint32),
+        ("max_verts", C.c_uint32), ("max_tris", C.c_uint32),
+        ("sh", C.c_float), ("radius", C.c_float), ("iso", C.c_float), ("origin", C.c_float * 3),
+    ]
+
+    @property
+    def nodes(self):
+        return (self.sx + 1) * (self.sy + 1) * (self.sz + 1)
+
+    @property
+    def cells(self):
+        return self.sx * self.sy * self.sz
+
+
 def build(force: bool = False) -> str:
     """Compile the oracle with its Makefile (gcc, -ffp-contract=off)."""
     cmd = ["make", "-s", "-C", _HERE] + (["-B"] if force else [])
@@ -92,6 +108,11 @@ def lib():
         L.orc_poisson_relres64.restype = C.c_double
         L.orc_poisson_solve64.argtypes = [SP, dp, C.c_double, C.c_int, dp]
         L.orc_poisson_solve64.restype = C.c_int
+        SPP = C.POINTER(SurfaceParams)
+        L.orc_surface_params.argtypes = [u32p, C.c_float, f32p, C.c_uint32, C.c_float, C.c_uint32, C.c_uint32, SPP]
+        L.orc_surface_splat.argtypes = [SPP, f32p, f32p, f32p, C.c_uint32, C.POINTER(C.c_int32)]
+        L.orc_surface_phi.argtypes = [SPP, C.POINTER(C.c_int32), f32p]
+        L.orc_surface_mesh.argtypes = [SPP, f32p, u32p, f32p, u32p, u32p]
         _lib = L
     return _lib
 
@@ -169,6 +190,47 @@ def seed_particles(n, h, origin, blocks, ppc, seed, count_only=False, obstacles=
     v = np.zeros((max(cnt, 1), 3), np.float32)
     L.orc_seed(_up(nn), np.float32(h), _fp(org), _fp(b), len(blocks), ppc, seed, idp, _fp(x), _fp(v))
     return x[:cnt], v[:cnt]
+
+
+def surface_params(n, h, origin, refine=2, particle_radius=2.0, ppc=8, max_triangles=2_000_000):
+    """Surface-net parameters of a fluid (crates/pxl-render/src/fluid.rs:151-157)."""
+    nn = np.asarray(n, np.uint32)
+    org = np.asarray(origin, np.float32)
+    sp = SurfaceParams()
+    lib().orc_surface_params(_up(nn), np.float32(h), _fp(org), refine, np.float32(particle_radius), ppc,
+                             max_triangles, C.byref(sp))
+    return sp
+
+
+def surface_splat(sp, x):
+    """s_clear + s_splat: fixed-point node density of the particles x[P, 3]."""
+    x = np.ascontiguousarray(x, np.float32)
+    cols = [np.ascontiguousarray(x[:, a]) for a in range(3)]
+    d = np.zeros(sp.nodes, np.int32)
+    lib().orc_surface_splat(C.byref(sp), _fp(cols[0]), _fp(cols[1]), _fp(cols[2]), len(x),
+                            d.ctypes.data_as(C.POINTER(C.c_int32)))
+    return d
+
+
+def surface_phi(sp, density):
+    d = np.ascontiguousarray(density, np.int32)
+    phi = np.zeros(sp.nodes, np.float32)
+    lib().orc_surface_phi(C.byref(sp), d.ctypes.data_as(C.POINTER(C.c_int32)), _fp(phi))
+    return phi
+
+
+def surface_mesh(sp, phi):
+    """s_verts + s_quads in index order -> (cell_vert, verts[V, 8], indices[T, 3], (vertices,
+    triangles) counters before clipping)."""
+    phi = np.ascontiguousarray(phi, np.float32)
+    cv = np.zeros(sp.cells, np.uint32)
+    verts = np.zeros((max(sp.max_verts, 1), 8), np.float32)
+    idx = np.zeros((max(sp.max_tris, 1), 3), np.uint32)
+    ctr = np.zeros(2, np.uint32)
+    lib().orc_surface_mesh(C.byref(sp), _fp(phi), _up(cv), _fp(verts), _up(idx), _up(ctr))
+    nv = min(int(ctr[0]), sp.max_verts)
+    nt = min(int(ctr[1]), sp.max_tris) // 2 * 2
+    return cv, verts[:nv], idx[:nt], (int(ctr[0]), int(ctr[1]))
 
 
 def steps_for_frame(warmup, frame, fps, substeps):

@@ -574,6 +574,189 @@ void orc_step_cg(orc_state *s, uint32_t iters) {
     orc_g2p_advect(s);
 }
 
+/* ------------------------------------------------------------------ surface nets */
+
+void orc_surface_params(const uint32_t n[3], float h, const float origin[3], uint32_t refine,
+                        float particle_radius, uint32_t ppc, uint32_t max_triangles, orc_surface *out) {
+    out->sx = n[0] * refine;
+    out->sy = n[1] * refine;
+    out->sz = n[2] * refine;
+    out->sh = h / (float)refine;
+    out->radius = particle_radius * out->sh;
+    const float rh = out->radius / h;
+    const float kernel_volume = 0.638f * (rh * rh * rh);
+    out->iso = 0.5f * (float)ppc * kernel_volume;
+    out->max_verts = max_triangles;
+    out->max_tris = max_triangles;
+    for (int a = 0; a < 3; a++) out->origin[a] = origin[a];
+}
+
+static inline uint64_t snode(const orc_surface *sp, int i, int j, int k) {
+    return (uint64_t)i + (uint64_t)(sp->sx + 1) * ((uint64_t)j + (uint64_t)(sp->sy + 1) * (uint64_t)k);
+}
+static inline uint64_t scell(const orc_surface *sp, int i, int j, int k) {
+    return (uint64_t)i + (uint64_t)sp->sx * ((uint64_t)j + (uint64_t)sp->sy * (uint64_t)k);
+}
+
+/* s_clear + s_splat: each particle adds int(q^3 * 4096), q = 1 - |node - g|^2 / r^2 > 0, to
+ * the nodes of its box [ceil(g - r), floor(g + r)] clipped to the grid, g = (x - origin) / sh,
+ * r = radius / sh. */
+void orc_surface_splat(const orc_surface *sp, const float *px, const float *py, const float *pz,
+                       uint32_t np, int32_t *density) {
+    const uint64_t nn = (uint64_t)(sp->sx + 1) * (sp->sy + 1) * (sp->sz + 1);
+    memset(density, 0, nn * 4);
+    const float r = sp->radius / sp->sh;
+    const int smax[3] = {(int)sp->sx, (int)sp->sy, (int)sp->sz};
+    for (uint32_t p = 0; p < np; p++) {
+        const float x[3] = {px[p], py[p], pz[p]};
+        float g[3];
+        int lo[3], hi[3];
+        for (int a = 0; a < 3; a++) {
+            g[a] = (x[a] - sp->origin[a]) / sp->sh;
+            lo[a] = (int)ceilf(g[a] - r);
+            if (lo[a] < 0) lo[a] = 0;
+            hi[a] = (int)floorf(g[a] + r);
+            if (hi[a] > smax[a]) hi[a] = smax[a];
+        }
+        for (int k = lo[2]; k <= hi[2]; k++)
+            for (int j = lo[1]; j <= hi[1]; j++)
+                for (int i = lo[0]; i <= hi[0]; i++) {
+                    const float d0 = (float)i - g[0], d1 = (float)j - g[1], d2 = (float)k - g[2];
+                    const float q = 1.0f - (d0 * d0 + d1 * d1 + d2 * d2) / (r * r);
+                    if (q > 0.0f) density[snode(sp, i, j, k)] += (int32_t)(q * q * q * 4096.0f);
+                }
+    }
+}
+
+/* s_phi: s = 2 density(n) + the in-grid neighbours' densities (-x, +x, -y, +y, -z, +z), w = 2 +
+ * their number; phi = s / w / 4096 - iso, or -1 on the grid's boundary nodes. */
+void orc_surface_phi(const orc_surface *sp, const int32_t *density, float *phi) {
+    const int smax[3] = {(int)sp->sx, (int)sp->sy, (int)sp->sz};
+    for (int k = 0; k <= smax[2]; k++)
+        for (int j = 0; j <= smax[1]; j++)
+            for (int i = 0; i <= smax[0]; i++) {
+                const int n[3] = {i, j, k};
+                float s = 2.0f * (float)density[snode(sp, i, j, k)];
+                float w = 2.0f;
+                for (int a = 0; a < 6; a++) {
+                    int q[3] = {i, j, k};
+                    q[a >> 1] += (a & 1) ? 1 : -1;
+                    if (q[0] < 0 || q[1] < 0 || q[2] < 0 || q[0] > smax[0] || q[1] > smax[1] || q[2] > smax[2])
+                        continue;
+                    s = s + (float)density[snode(sp, q[0], q[1], q[2])];
+                    w = w + 1.0f;
+                }
+                const int edge = n[0] == 0 || n[1] == 0 || n[2] == 0 || n[0] == smax[0] || n[1] == smax[1] ||
+                                 n[2] == smax[2];
+                phi[snode(sp, i, j, k)] = edge ? -1.0f : s / w / 4096.0f - sp->iso;
+            }
+}
+
+/* s_verts + s_quads.  A cell with both signs among its 8 corner phis (bit k = corner
+ * (k & 1, k >> 1 & 1, k >> 2)) gets the vertex at the mean of its edge crossings (edges e:
+ * axis e >> 2, the other axes' corner bits e & 3; t = v0 / (v0 - v1); FMix of the corners) and
+ * the normal -normalize(grad + (0, 1e-9, 0)) of the trilinear phi there (normalize = g / sqrt(g.g)).
+ * Each node edge (n, n + e_axis) whose phis differ in sign, with all 4 cells around it holding a
+ * vertex ((-1,-1), (0,-1), (0,0), (-1,0) along the next two axes), emits two triangles wound
+ * from inside to outside. */
+void orc_surface_mesh(const orc_surface *sp, const float *phi, uint32_t *cell_vert, float *verts,
+                      uint32_t *indices, uint32_t counters[2]) {
+    const int smax[3] = {(int)sp->sx, (int)sp->sy, (int)sp->sz};
+    uint32_t nv = 0, nt = 0;
+    for (int ck = 0; ck < smax[2]; ck++)
+        for (int cj = 0; cj < smax[1]; cj++)
+            for (int ci = 0; ci < smax[0]; ci++) {
+                const int c[3] = {ci, cj, ck};
+                const uint64_t id = scell(sp, ci, cj, ck);
+                float v[8];
+                uint32_t inside = 0;
+                for (int k = 0; k < 8; k++) {
+                    v[k] = phi[snode(sp, ci + (k & 1), cj + ((k >> 1) & 1), ck + (k >> 2))];
+                    if (v[k] > 0.0f) inside |= 1u << k;
+                }
+                cell_vert[id] = ORC_NONE;
+                if (inside == 0 || inside == 255) continue;
+                float sum[3] = {0.0f, 0.0f, 0.0f}, cnt = 0.0f;
+                for (int e = 0; e < 12; e++) {
+                    const int axis = e >> 2, bits = e & 3, o1 = (axis + 1) % 3, o2 = (axis + 2) % 3;
+                    int a0 = 0;
+                    if (bits & 1) a0 |= 1 << o1;
+                    if (bits & 2) a0 |= 1 << o2;
+                    const int a1 = a0 | (1 << axis);
+                    if ((v[a0] > 0.0f) == (v[a1] > 0.0f)) continue;
+                    const float t = v[a0] / (v[a0] - v[a1]);
+                    for (int a = 0; a < 3; a++) {
+                        const float p0 = (float)((a0 >> a) & 1), p1 = (float)((a1 >> a) & 1);
+                        sum[a] = sum[a] + (p0 * (1.0f - t) + p1 * t);
+                    }
+                    cnt = cnt + 1.0f;
+                }
+                float local[3];
+                for (int a = 0; a < 3; a++) local[a] = sum[a] / cnt;
+                float grad[3] = {0.0f, 0.0f, 0.0f};
+                for (int k = 0; k < 8; k++) {
+                    float w[3], sg[3];
+                    for (int a = 0; a < 3; a++) {
+                        const int o = (k >> a) & 1;
+                        w[a] = o ? local[a] : 1.0f - local[a];
+                        sg[a] = o ? 1.0f : -1.0f;
+                    }
+                    const float gv[3] = {sg[0] * w[1] * w[2], sg[1] * w[0] * w[2], sg[2] * w[0] * w[1]};
+                    for (int a = 0; a < 3; a++) grad[a] = grad[a] + v[k] * gv[a];
+                }
+                const uint32_t idx = nv++;
+                if (idx >= sp->max_verts) continue;
+                const float gg[3] = {grad[0], grad[1] + 1e-9f, grad[2]};
+                const float len = sqrtf(gg[0] * gg[0] + gg[1] * gg[1] + gg[2] * gg[2]);
+                float *out = verts + 8 * (uint64_t)idx;
+                for (int a = 0; a < 3; a++) {
+                    out[a] = sp->origin[a] + ((float)c[a] + local[a]) * sp->sh;
+                    out[4 + a] = -(gg[a] / len);
+                }
+                out[3] = 0.0f;
+                out[7] = 0.0f;
+                cell_vert[id] = idx;
+            }
+    static const int offs[4][2] = {{-1, -1}, {0, -1}, {0, 0}, {-1, 0}};
+    for (int k = 0; k <= smax[2]; k++)
+        for (int j = 0; j <= smax[1]; j++)
+            for (int i = 0; i <= smax[0]; i++) {
+                const int in0 = phi[snode(sp, i, j, k)] > 0.0f;
+                for (int axis = 0; axis < 3; axis++) {
+                    int m[3] = {i, j, k};
+                    m[axis] += 1;
+                    if (m[0] > smax[0] || m[1] > smax[1] || m[2] > smax[2]) continue;
+                    if (in0 == (phi[snode(sp, m[0], m[1], m[2])] > 0.0f)) continue;
+                    const int o1 = (axis + 1) % 3, o2 = (axis + 2) % 3;
+                    uint32_t q[4];
+                    int ok = 1;
+                    for (int e = 0; e < 4 && ok; e++) {
+                        int c[3] = {i, j, k};
+                        c[o1] += offs[e][0];
+                        c[o2] += offs[e][1];
+                        if (c[0] < 0 || c[1] < 0 || c[2] < 0 || c[0] >= smax[0] || c[1] >= smax[1] || c[2] >= smax[2]) {
+                            ok = 0;
+                            break;
+                        }
+                        q[e] = cell_vert[scell(sp, c[0], c[1], c[2])];
+                        if (q[e] == ORC_NONE) ok = 0;
+                    }
+                    if (!ok) continue;
+                    const uint32_t t = nt;
+                    nt += 2;
+                    if (t + 2 > sp->max_tris) continue;
+                    uint32_t *b = indices + 3 * (uint64_t)t;
+                    if (in0) {
+                        b[0] = q[0]; b[1] = q[1]; b[2] = q[2]; b[3] = q[0]; b[4] = q[2]; b[5] = q[3];
+                    } else {
+                        b[0] = q[0]; b[1] = q[2]; b[2] = q[1]; b[3] = q[0]; b[4] = q[3]; b[5] = q[2];
+                    }
+                }
+            }
+    counters[0] = nv;
+    counters[1] = nt;
+}
+
 /* ------------------------------------------------------------------ binning */
 
 /* Cell of a particle, as in p2g (fluid.slang:113): clamp(floor((x - origin) * inv_h)). */

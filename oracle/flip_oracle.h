@@ -96,6 +96,33 @@ void orc_step_cg(orc_state *s, uint32_t iters);
 /* Pressure phase only, FP32 PCG as in orc_step_cg (divergence/type already computed). */
 void orc_pressure_cg32(orc_state *s, uint32_t iters);
 
+/* ---- surface nets (fluid.rs:151-157, 308-330; fluid.slang:222-355) ----
+ * The reference numbers vertices and triangles by atomic counters (thread order); this
+ * restatement walks cells and nodes in index order, the canonical order the build's kernels
+ * also produce. */
+typedef struct orc_surface {
+    uint32_t sx, sy, sz;      /* surface cells per axis = n * surface_refine */
+    uint32_t max_verts, max_tris;
+    float sh, radius, iso;    /* node spacing, splat radius, iso level */
+    float origin[3];
+} orc_surface;
+#define ORC_NONE 0xffffffffu
+/* fluid.rs:151-157: sx = n * refine, sh = h / refine, radius = particle_radius * sh,
+ * kernel_volume = 0.638 * (radius / h)^3 (powi: (x * x) * x), iso = 0.5 * ppc * kernel_volume;
+ * max_verts = max_tris = max_triangles. */
+void orc_surface_params(const uint32_t n[3], float h, const float origin[3], uint32_t refine,
+                        float particle_radius, uint32_t ppc, uint32_t max_triangles, orc_surface *out);
+/* s_clear + s_splat (fluid.slang:228-250): density[(sx+1)(sy+1)(sz+1)], fixed point (4096). */
+void orc_surface_splat(const orc_surface *sp, const float *px, const float *py, const float *pz,
+                       uint32_t np, int32_t *density);
+/* s_phi (fluid.slang:252-268): 7-point smoothing, iso offset, boundary nodes -1. */
+void orc_surface_phi(const orc_surface *sp, const int32_t *density, float *phi);
+/* s_verts + s_quads (fluid.slang:270-355): cell_vert[sx sy sz], verts (8 floats per vertex:
+ * position, 0, normal, 0; capacity max_verts), indices (3 per triangle; capacity max_tris).
+ * counters = the reference's counters[0..1] (vertices, triangles, before clipping). */
+void orc_surface_mesh(const orc_surface *sp, const float *phi, uint32_t *cell_vert, float *verts,
+                      uint32_t *indices, uint32_t counters[2]);
+
 /* ---- binning: canonical counting sort (ids ascending inside each cell) ---- */
 void orc_bin(const orc_state *s, uint32_t *cell_of, uint32_t *counts, uint32_t *offsets,
              uint32_t *sorted_ids);

@@ -6,6 +6,8 @@
 //   clear -> bin count -> scan -> scatter (counting sort) -> p2g -> faces (normalize+forces)
 //   -> cells (classify+stencil+divergence) -> pressure -> project -> g2p
 // and, for the RBGS and CG solvers, captured once into a hipGraph that each step replays.
+#include <cstring>
+
 #include "sim.hpp"
 
 namespace pxl {
@@ -127,6 +129,27 @@ int prepare(const char *fluid_json, double fps, uint32_t seed, const pxlflip_opt
     d.ntiles = (uint32_t)d.ntx * (uint32_t)d.nty * (uint32_t)d.ntz;
     d.nkeys = d.ntiles * TILE_CELLS;
     d.n_obst = (uint32_t)spec.obstacles.size();
+    {
+        // surface-net parameters (fluid.rs:151-157): sx = n * refine, sh = h / refine,
+        // radius = particle_radius * sh, iso = 0.5 * ppc * 0.638 * (radius / h)^3
+        DevSurf &sf = s.surf;
+        const uint32_t r = spec.surface_refine;
+        sf.sx = (int)(n[0] * r);
+        sf.sy = (int)(n[1] * r);
+        sf.sz = (int)(n[2] * r);
+        sf.sh = hcell / (float)r;
+        sf.radius = spec.particle_radius * sf.sh;
+        const float rh = sf.radius / hcell;
+        const float kernel_volume = 0.638f * (rh * rh * rh);
+        sf.iso = 0.5f * (float)spec.particles_per_cell * kernel_volume;
+        sf.max_verts = spec.max_triangles;
+        sf.max_tris = spec.max_triangles;
+        sf.ox = spec.bounds[0][0];
+        sf.oy = spec.bounds[0][1];
+        sf.oz = spec.bounds[0][2];
+        sf.nnodes = (uint32_t)(sf.sx + 1) * (uint32_t)(sf.sy + 1) * (uint32_t)(sf.sz + 1);
+        sf.ncells = (uint32_t)sf.sx * (uint32_t)sf.sy * (uint32_t)sf.sz;
+    }
     {
         // four times the tiles the liquid fills at seeding (room for the band around it and
         // for spreading), at least 64, at most 1024 and the grid's tile count
@@ -360,6 +383,15 @@ int pxlflip_read_field(pxlflip *h, int field, void *out) {
                 return pxl::fail(s, PXLFLIP_ERR_INVALID, "PRESSURE64 exists only with the MIXED solver");
             bytes = 8ull * d.ncells;
             break;
+        case PXLFLIP_FIELD_DENSITY:
+        case PXLFLIP_FIELD_PHI:
+        case PXLFLIP_FIELD_CELL_VERT:
+            if (!s->surf_arena) return pxl::fail(s, PXLFLIP_ERR_INVALID, "no surface pass has run on this handle");
+            src = field == PXLFLIP_FIELD_DENSITY ? (const void *)s->surf.density
+                  : field == PXLFLIP_FIELD_PHI   ? (const void *)s->surf.phi
+                                                 : (const void *)s->surf.cell_vert;
+            bytes = 4ull * (field == PXLFLIP_FIELD_CELL_VERT ? s->surf.ncells : s->surf.nnodes);
+            break;
         default: return pxl::fail(s, PXLFLIP_ERR_INVALID, "unknown field " + std::to_string(field));
     }
     return pxl::guarded(s, [&] {
@@ -377,6 +409,62 @@ int pxlflip_read_field(pxlflip *h, int field, void *out) {
             HIP_CHECK(hipMemcpyAsync(out, src, bytes, hipMemcpyDeviceToHost, s->stream));
         }
         HIP_CHECK(hipStreamSynchronize(s->stream));
+    });
+}
+
+namespace {
+void fill_surface_info(const pxl::Sim &s, const uint32_t ctr[2], pxlflip_surface_info *info) {
+    if (!info) return;
+    std::memset(info, 0, sizeof *info);
+    const pxl::DevSurf &sf = s.surf;
+    info->sx = (uint32_t)sf.sx;
+    info->sy = (uint32_t)sf.sy;
+    info->sz = (uint32_t)sf.sz;
+    info->max_triangles = sf.max_tris;
+    info->sh = sf.sh;
+    info->radius = sf.radius;
+    info->iso = sf.iso;
+    info->vertices_total = ctr[0];
+    info->triangles_total = ctr[1];
+    // fluid.rs:328-330
+    info->overflow = (ctr[0] > sf.max_verts || ctr[1] > sf.max_tris) ? 1u : 0u;
+    info->vertices = std::min(ctr[0], sf.max_verts);
+    info->triangles = std::min(ctr[1], sf.max_tris) / 2u * 2u;
+}
+}  // namespace
+
+int pxlflip_surface(pxlflip *h, float *verts, uint32_t vbase, uint32_t *indices, uint32_t ibase,
+                    pxlflip_surface_info *info) {
+    if (!h) return PXLFLIP_ERR_INVALID;
+    Sim *s = &h->sim;
+    if (!verts || !indices) return pxl::fail(s, PXLFLIP_ERR_INVALID, "verts and indices must be device pointers");
+    if (reinterpret_cast<uintptr_t>(verts) % 16 != 0)
+        return pxl::fail(s, PXLFLIP_ERR_INVALID, "verts must be 16-byte aligned (two float4 per vertex)");
+    return pxl::guarded(s, [&] {
+        if (!s->initialized) s->reset();
+        uint32_t ctr[2];
+        s->surface(verts, vbase, indices, ibase, ctr);
+        fill_surface_info(*s, ctr, info);
+    });
+}
+
+int pxlflip_surface_host(pxlflip *h, float *verts, uint32_t *indices, pxlflip_surface_info *info) {
+    if (!h) return PXLFLIP_ERR_INVALID;
+    Sim *s = &h->sim;
+    return pxl::guarded(s, [&] {
+        if (!s->initialized) s->reset();
+        const uint32_t mv = s->surf.max_verts, mt = s->surf.max_tris;
+        pxl::DeviceBuffer vb(32ull * mv), ib(12ull * mt);
+        uint32_t ctr[2];
+        s->surface(vb.as<float>(), 0, ib.as<uint32_t>(), 0, ctr);
+        pxlflip_surface_info tmp;
+        fill_surface_info(*s, ctr, &tmp);
+        if (verts && tmp.vertices)
+            HIP_CHECK(hipMemcpyAsync(verts, vb.p, 32ull * tmp.vertices, hipMemcpyDeviceToHost, s->stream));
+        if (indices && tmp.triangles)
+            HIP_CHECK(hipMemcpyAsync(indices, ib.p, 12ull * tmp.triangles, hipMemcpyDeviceToHost, s->stream));
+        HIP_CHECK(hipStreamSynchronize(s->stream));
+        if (info) *info = tmp;
     });
 }
 

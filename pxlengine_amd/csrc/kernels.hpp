@@ -208,6 +208,29 @@ __device__ __forceinline__ uint32_t tile_key(const DevSim &S, int i, int j, int 
     return t * TILE_CELLS + (uint32_t)((i & 7) | ((j & 7) << 3) | ((k & 7) << 6));
 }
 
+// Surface nets of one simulation (fluid.slang:222-355): the surface grid has sx x sy x sz
+// cells ((sx+1)(sy+1)(sz+1) nodes, x fastest) of spacing sh from the fluid's origin.
+struct DevSurf {
+    int sx, sy, sz;
+    uint32_t nnodes, ncells;
+    uint32_t max_verts, max_tris;
+    float sh, radius, iso;
+    float ox, oy, oz;
+    int32_t *density;     // nodes: splatted density, fixed point (4096)
+    float *phi;           // nodes: smoothed density - iso (> 0 inside), -1 on the boundary
+    uint32_t *cell_vert;  // cells: vertex number, or SURF_NONE
+    uint32_t *qoff;       // nodes: quads at the node's three edges, then their exclusive scan
+    uint32_t *counters;   // [0] surface cells, [1] 2 x quads (the reference's counters, unclipped)
+    uint32_t *scan_tmp;
+};
+constexpr uint32_t SURF_NONE = 0xFFFFFFFFu;
+constexpr float SURF_SD = 4096.0f;  // fixed-point scale of the surface density
+// The whole surface pass on the current positions: clear, splat, phi, vertices (numbered in
+// surface-cell order) and triangles (in (node, axis) order) into verts[8 (vbase + v)] and
+// indices[ibase + 3 t] (device memory).
+void launch_surface(const DevSurf &S, const float *px, const float *py, const float *pz, uint32_t np, float *verts,
+                    uint32_t vbase, uint32_t *indices, uint32_t ibase, hipStream_t s);
+
 // ---- launches ----
 // Every launch goes through PXL_LAUNCH.  While a profiler has set this thread's launch events,
 // the first launch records them from its own dispatch (the kernel's start and end, as a

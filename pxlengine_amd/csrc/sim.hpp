@@ -110,6 +110,42 @@ struct Sim {
     pxlflip_solve_stats stats{-1.0, 0, 0};
     std::string error;
     KernelTimer timer;
+    // surface nets: parameters set at creation, arrays allocated by the first surface pass
+    DevSurf surf{};
+    void *surf_arena = nullptr;
+    uint32_t *host_ctr = nullptr;  // pinned: the surface counters
+
+    void ensure_surface() {
+        if (surf_arena) return;
+        const size_t nn = surf.nnodes, nc = surf.ncells;
+        const size_t tmp = 4ull * (scan_tiles((uint32_t)std::max(nn, nc)) + 1);
+        const size_t sizes[6] = {4 * nn, 4 * nn, 4 * nn, 4 * nc, 64, tmp};
+        size_t total = 0;
+        for (size_t b : sizes) total += (b + ALIGN - 1) / ALIGN * ALIGN;
+        if (hipMalloc(&surf_arena, total) != hipSuccess) {
+            surf_arena = nullptr;
+            throw std::bad_alloc();
+        }
+        char *cur = static_cast<char *>(surf_arena);
+        void **dst[6] = {reinterpret_cast<void **>(&surf.density), reinterpret_cast<void **>(&surf.phi),
+                         reinterpret_cast<void **>(&surf.qoff), reinterpret_cast<void **>(&surf.cell_vert),
+                         reinterpret_cast<void **>(&surf.counters), reinterpret_cast<void **>(&surf.scan_tmp)};
+        for (int q = 0; q < 6; q++) {
+            *dst[q] = cur;
+            cur += (sizes[q] + ALIGN - 1) / ALIGN * ALIGN;
+        }
+        HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&host_ctr), 16, hipHostMallocDefault));
+    }
+
+    // One surface pass on the current particles into device buffers; returns the counters.
+    void surface(float *verts, uint32_t vbase, uint32_t *indices, uint32_t ibase, uint32_t ctr[2]) {
+        ensure_surface();
+        launch_surface(surf, d.ax, d.ay, d.az, d.np, verts, vbase, indices, ibase, stream);
+        HIP_CHECK(hipMemcpyAsync(host_ctr, surf.counters, 8, hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        ctr[0] = host_ctr[0];
+        ctr[1] = host_ctr[1];
+    }
 
     // Graphs bake kernel arguments (grid shapes, tunables); drop them after a change.
     void drop_graphs() {
@@ -128,6 +164,8 @@ struct Sim {
     ~Sim() {
         drop_graphs();
         if (host_scal) (void)hipHostFree(host_scal);
+        if (host_ctr) (void)hipHostFree(host_ctr);
+        if (surf_arena) (void)hipFree(surf_arena);
         if (arena) (void)hipFree(arena);
         if (stream) (void)hipStreamDestroy(stream);
     }

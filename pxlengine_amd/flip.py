@@ -34,6 +34,10 @@ FIELDS = {
     "pressure": (6, np.float32, "cells"),
     "divergence": (7, np.float32, "cells"),
     "pressure64": (8, np.float64, "cells"),
+    # surface nets (after FluidState.surface)
+    "density": (9, np.int32, "snodes"),
+    "phi": (10, np.float32, "snodes"),
+    "cell_vert": (11, np.uint32, "scells"),
 }
 WRITABLE = ("vel", "vel_old", "pressure")
 
@@ -122,6 +126,18 @@ class ServerStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class SurfaceInfo(C.Structure):
+    _fields_ = [
+        ("sx", C.c_uint32), ("sy", C.c_uint32), ("sz", C.c_uint32), ("max_triangles", C.c_uint32),
+        ("vertices", C.c_uint32), ("triangles", C.c_uint32), ("vertices_total", C.c_uint32),
+        ("triangles_total", C.c_uint32), ("overflow", C.c_uint32),
+        ("sh", C.c_float), ("radius", C.c_float), ("iso", C.c_float),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 REQUEST_STATUS = {0: "queued", 1: "running", 2: "done", 3: "failed", 4: "cancelled"}
 
 
@@ -133,6 +149,8 @@ _SIGNATURES = {
     "pxlflip_seed_particles": (C.c_int, [C.c_char_p, C.c_uint32, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
     "pxlflip_create_from_json": (C.c_int, [C.c_char_p, C.c_double, C.c_uint32, C.POINTER(Opts), C.POINTER(_H)]),
     "pxlflip_destroy": (None, [_H]),
+    "pxlflip_surface": (C.c_int, [_H, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.POINTER(SurfaceInfo)]),
+    "pxlflip_surface_host": (C.c_int, [_H, C.c_void_p, C.c_void_p, C.POINTER(SurfaceInfo)]),
     "pxlflip_advance_to_frame": (C.c_int, [_H, C.c_double]),
     "pxlflip_step": (C.c_int, [_H, C.c_uint32]),
     "pxlflip_reset": (C.c_int, [_H]),
@@ -322,9 +340,35 @@ class FluidState:
             raise ValueError(f"expected ({self.info['n_particles']}, 8) records, got {a.shape}")
         self._call(self._lib.pxlflip_write_particles, a.ctypes.data)
 
+    def surface(self):
+        """Surface nets of the current particles (fluid.rs:308-330) -> (verts[V, 8] as {pos, 0,
+        normal, 0}, indices[T, 3], info)."""
+        cap = self.reservation()[0]
+        verts = np.zeros((max(cap, 1), 8), np.float32)
+        idx = np.zeros((max(cap, 1), 3), np.uint32)
+        info = SurfaceInfo()
+        self._call(self._lib.pxlflip_surface_host, verts.ctypes.data, idx.ctypes.data, C.byref(info))
+        self._surf = info.as_dict()
+        return verts[:info.vertices], idx[:info.triangles], self._surf
+
+    def surface_device(self, verts_ptr: int, vbase: int, indices_ptr: int, ibase: int) -> dict:
+        """The same into device buffers (e.g. torch tensors' data_ptr()), as the renderer binds them."""
+        info = SurfaceInfo()
+        self._call(self._lib.pxlflip_surface, C.c_void_p(verts_ptr), vbase, C.c_void_p(indices_ptr), ibase,
+                   C.byref(info))
+        self._surf = info.as_dict()
+        return self._surf
+
     def read_field(self, name: str) -> np.ndarray:
         code, dtype, kind = FIELDS[name]
-        n = self.info["n_faces"] if kind == "faces" else self.info["n_cells"]
+        if kind in ("snodes", "scells"):
+            sf = getattr(self, "_surf", None)
+            if sf is None:
+                raise FluidError("no surface pass has run on this handle")
+            n = ((sf["sx"] + 1) * (sf["sy"] + 1) * (sf["sz"] + 1) if kind == "snodes"
+                 else sf["sx"] * sf["sy"] * sf["sz"])
+        else:
+            n = self.info["n_faces"] if kind == "faces" else self.info["n_cells"]
         out = np.empty(n, dtype)
         self._call(self._lib.pxlflip_read_field, code, out.ctypes.data)
         return out

@@ -1,0 +1,139 @@
+"""GPU surface nets (shaders/fluid.slang:222-355) against the oracle's restatement on the
+same particles: density (integer sums), phi, the cell -> vertex map, every vertex (position
+and normal) and every triangle bit for bit -- the kernels number vertices in surface-cell
+order and triangles in (node, axis) order, the order of the oracle's loops -- plus the
+reference's counters and clipping rules."""
+import numpy as np
+import pytest
+
+from tests.scenes import EXAMPLE_DAM_BREAK, FPS, dam_break, obstacle_scene, sphere_drop
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pxl():
+    import pxlengine_amd
+    return pxlengine_amd
+
+
+def _oracle_surface(orc, gpu, fluid):
+    info = gpu.info
+    sp = orc.surface_params(info["n"], info["h"], fluid["bounds"][0], fluid.get("surface_refine", 2),
+                            fluid.get("particle_radius", 2.0), fluid.get("particles_per_cell", 8),
+                            fluid.get("max_triangles", 2_000_000))
+    p = gpu.read_particles()
+    dens = orc.surface_splat(sp, p[:, 0:3])
+    phi = orc.surface_phi(sp, dens)
+    cv, verts, tris, ctr = orc.surface_mesh(sp, phi)
+    return sp, dens, phi, cv, verts, tris, ctr
+
+
+def _check(orc, gpu, fluid, what):
+    print(f"surface {what}", flush=True)
+    verts, tris, info = gpu.surface()
+    sp, dens, phi, cv, overts, otris, ctr = _oracle_surface(orc, gpu, fluid)
+    assert (info["sx"], info["sy"], info["sz"]) == (sp.sx, sp.sy, sp.sz)
+    assert info["iso"] == sp.iso and info["sh"] == sp.sh and info["radius"] == sp.radius
+    np.testing.assert_array_equal(gpu.read_field("density"), dens, err_msg=f"density {what}")
+    np.testing.assert_array_equal(gpu.read_field("phi"), phi, err_msg=f"phi {what}")
+    np.testing.assert_array_equal(gpu.read_field("cell_vert"), cv, err_msg=f"cell_vert {what}")
+    assert (info["vertices_total"], info["triangles_total"]) == ctr
+    np.testing.assert_array_equal(verts, overts, err_msg=f"vertices {what}")
+    np.testing.assert_array_equal(tris, otris, err_msg=f"triangles {what}")
+    assert info["overflow"] == int(ctr[0] > sp.max_verts or ctr[1] > sp.max_tris)
+    return verts, tris, info
+
+
+@pytest.mark.parametrize("fluid,steps", [
+    (dam_break(24), 3),
+    (dict(dam_break(20), surface_refine=3, particle_radius=2.5), 2),
+    (dict(dam_break(32, ppc=4), surface_refine=1, particle_radius=1.3), 2),
+    (EXAMPLE_DAM_BREAK, 2),
+    (sphere_drop(28), 3),
+    (obstacle_scene(24), 3),
+], ids=["dam24", "refine3", "refine1", "example", "sphere28", "obstacles24"])
+def test_surface_bitwise(pxl, orc, fluid, steps):
+    gpu = pxl.FluidState(fluid, FPS, seed=0)
+    _check(orc, gpu, fluid, "at seeding")
+    gpu.step(steps)
+    verts, tris, info = _check(orc, gpu, fluid, f"after {steps} steps")
+    assert info["vertices"] > 100 and info["triangles"] > 100 and not info["overflow"]
+    assert tris.max() < info["vertices"]
+    gpu.close()
+
+
+def test_surface_overflow_clipping(pxl, orc):
+    """max_triangles below the mesh's size: vertices past it are not stored, quads needing
+    them are dropped, triangles past it are not written, overflow is reported -- as the
+    oracle does it."""
+    fluid = dict(dam_break(24), max_triangles=1500)
+    gpu = pxl.FluidState(fluid, FPS, seed=0)
+    gpu.step(2)
+    verts, tris, info = _check(orc, gpu, fluid, "clipped")
+    assert info["overflow"] == 1 and info["vertices"] == 1500 and info["vertices_total"] > 1500
+    assert len(tris) <= 1500 and len(tris) % 2 == 0
+    gpu.close()
+
+
+class _Hip:
+    """hipMalloc / hipMemcpy / hipFree of the HIP runtime the library uses."""
+
+    def __init__(self):
+        import ctypes as C
+        import os
+        path = "/opt/rocm/lib/libamdhip64.so"
+        self.C = C
+        self.lib = C.CDLL(path if os.path.exists(path) else "libamdhip64.so")
+        self.lib.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.lib.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.lib.hipFree.argtypes = [C.c_void_p]
+
+    def alloc_from(self, host: np.ndarray):
+        p = self.C.c_void_p()
+        assert self.lib.hipMalloc(self.C.byref(p), host.nbytes) == 0
+        assert self.lib.hipMemcpy(p, host.ctypes.data, host.nbytes, 1) == 0  # host to device
+        return p
+
+    def read(self, p, like: np.ndarray) -> np.ndarray:
+        out = np.empty_like(like)
+        assert self.lib.hipMemcpy(out.ctypes.data, p, out.nbytes, 2) == 0  # device to host
+        return out
+
+
+def test_surface_into_device_buffers(pxl):
+    """The device entry point writes the same mesh at vbase / ibase of caller buffers and
+    leaves the rest of them untouched."""
+    hip = _Hip()
+    fluid = dam_break(24)
+    gpu = pxl.FluidState(fluid, FPS, seed=0)
+    gpu.step(2)
+    verts, tris, info = gpu.surface()
+    cap = gpu.reservation()[0]
+    vbase, ibase = 1000, 777
+    vhost = np.full((vbase + cap, 8), -7.0, np.float32)
+    ihost = np.full(ibase + 3 * cap, 0xABCDEF, np.uint32)
+    vdev, idev = hip.alloc_from(vhost), hip.alloc_from(ihost)
+    try:
+        info2 = gpu.surface_device(vdev.value, vbase, idev.value, ibase)
+        v = hip.read(vdev, vhost)
+        ib = hip.read(idev, ihost)
+    finally:
+        hip.lib.hipFree(vdev)
+        hip.lib.hipFree(idev)
+    assert info2 == info
+    np.testing.assert_array_equal(v[vbase:vbase + info["vertices"]], verts)
+    np.testing.assert_array_equal(ib[ibase:ibase + 3 * info["triangles"]].reshape(-1, 3), tris)
+    assert np.all(v[:vbase] == -7.0) and np.all(v[vbase + info["vertices"]:] == -7.0)
+    assert np.all(ib[:ibase] == 0xABCDEF) and np.all(ib[ibase + 3 * info["triangles"]:] == 0xABCDEF)
+    gpu.close()
+
+
+def test_surface_full_size_dam(pxl, orc):
+    """The 128^3 dam break (3.83M particles) after 5 steps, surface grid 256^3 (refine 2)."""
+    fluid = dam_break(128)
+    gpu = pxl.FluidState(fluid, FPS, seed=0, solver="cg", cg_iterations=80)
+    gpu.step(5)
+    verts, tris, info = _check(orc, gpu, fluid, "128^3")
+    assert info["vertices"] > 10_000
+    gpu.close()
