@@ -30,7 +30,8 @@ algorithmic bytes over the simulations each covers).  `cpu_baseline` times the C
 `--workload batch` only the fixed batch used for the roofline (for profiling runs);
 `--workload mixed256` the 256^3 two-block mixed-precision configuration (~58M particles);
 `--workload obstacles` the 192x96x192 obstacle-heavy configuration (solid spheres, 19.4M
-particles, 200 steps, mixed-precision solve).
+particles, 200 steps, mixed-precision solve); `--workload frame` 128^3 dam-break frames of 5
+steps plus one surface-nets pass.
 """
 from __future__ import annotations
 
@@ -117,6 +118,75 @@ def bench_dam(args, pxl):
                      "frac": ach / PEAK_HBM_GBS, "traffic": measured_traffic(dom["kernel"], "dam128_cg80"),
                      "kernel": dom["kernel"], "avg_us": dom["avg_us"], "alg_bytes_per_launch": dom["alg_bytes"]},
         "cpu_baseline": cpu, "kernels": kernels,
+    }
+
+
+def surface_algorithmic_bytes(stage, P, nodes, cells, touched, V, T):
+    """Minimal HBM bytes of one surface stage: positions 12 B per particle, 4 B per node or
+    cell word read or written once, an atomic-add target (read + write) 8 B per touched node,
+    32 B per vertex, 12 B per triangle."""
+    return {"s_clear": 4 * nodes, "s_splat": 12 * P + 8 * touched, "s_phi": 8 * nodes,
+            "s_vflag": 4 * nodes + 4 * cells, "s_vscan": 12 * cells, "s_verts": 4 * nodes + 8 * cells + 32 * V,
+            "s_qcount": 4 * nodes + 4 * nodes, "s_qscan": 12 * nodes,
+            "s_quads": 8 * nodes + 12 * T}.get(stage, 0)
+
+
+def bench_frame(args, pxl):
+    """Frames of the 128^3 dam break: `substeps` (5) plain FP32 CG-80 steps plus one surface-nets
+    pass (refine 2: a 256^3 surface grid) per frame, the reference's per-frame work
+    (fluid.rs:265-330).  value = particle-steps/s with the meshing included; the roofline is
+    the surface pass's dominant stage."""
+    import numpy as np
+    from tests.scenes import FPS, dam_break
+    fluid = dam_break(128, iters=80)
+    sim = pxl.FluidState(fluid, FPS, seed=0, solver="cg", cg_iterations=80)
+    P = sim.particle_count()
+    sub = fluid["substeps"]
+    for _ in range(args.warmup):
+        sim.step(sub)
+        sim.surface()
+    frames = max(1, args.steps // sub)
+    cap = sim.reservation()[0]
+    vbuf, ibuf = pxl.DeviceBuffer(32 * cap), pxl.DeviceBuffer(12 * cap)  # the renderer's buffers
+    t_mesh = 0.0
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        sim.step(sub)
+        a = time.perf_counter()
+        info = sim.surface_device(vbuf.ptr, 0, ibuf.ptr, 0)
+        t_mesh += time.perf_counter() - a
+    elapsed = time.perf_counter() - t0
+    vbuf.free()
+    ibuf.free()
+    prof = sim.profile_surface(3)
+    dens = sim.read_field("density")
+    sx, sy, sz = info["sx"], info["sy"], info["sz"]
+    nodes, cells = (sx + 1) * (sy + 1) * (sz + 1), sx * sy * sz
+    touched = int(np.count_nonzero(dens))
+    V, T = info["vertices"], info["triangles"]
+    stages = []
+    for name, launches, total_ms in prof:
+        avg_s = total_ms / 1e3 / launches
+        b = surface_algorithmic_bytes(name, P, nodes, cells, touched, V, T)
+        stages.append({"kernel": name, "avg_us": avg_s * 1e6, "alg_bytes": b,
+                       "achieved_GBs": b / avg_s / 1e9 if b else None})
+    dom = stages[0]
+    mesh_ms = sum(st["avg_us"] for st in stages) / 1e3
+    sim.close()
+    return {
+        "metric": METRIC, "value": P * frames * sub / elapsed, "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": frames * sub, "warmup": args.warmup, "ms_per_step": elapsed / (frames * sub) * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded dam break, reference LCG seeding)",
+        "config": {"workload": f"dam break 128^3 frames: {sub} plain FP32 CG-80 steps + surface nets (256^3 surface grid) "
+                               f"per frame, {frames} frames",
+                   "particles": P, "surface_ms_per_frame_wall": t_mesh / frames * 1e3,
+                   "surface_ms_per_frame_kernels": mesh_ms, "vertices": V, "triangles": T,
+                   "surface_nodes": nodes, "touched_nodes": touched, "reservation": cap},
+        "roofline": {"bound": "hbm", "achieved": dom["achieved_GBs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": (dom["achieved_GBs"] or 0.0) / PEAK_HBM_GBS, "traffic": None,
+                     "kernel": dom["kernel"], "avg_us": dom["avg_us"], "alg_bytes_per_launch": dom["alg_bytes"]},
+        "cpu_baseline": None, "kernels": stages,
     }
 
 
@@ -362,7 +432,7 @@ def main():
                     help="calibration batch: the stream's first N requests, whose service rate sets the arrival "
                          "rate (dam: warm-up steps)")
     ap.add_argument("--calib-steps", type=int, default=10, help="timed steps of the warm-up batch")
-    ap.add_argument("--workload", default="serve", choices=["serve", "dam", "batch", "mixed256", "obstacles"])
+    ap.add_argument("--workload", default="serve", choices=["serve", "dam", "batch", "mixed256", "obstacles", "frame"])
     ap.add_argument("--requests", type=int, default=512)
     ap.add_argument("--sizes", default="32,48,64,96,128")
     ap.add_argument("--rate", type=float, default=0.0, help="arrivals per second (0 = measured service rate)")
@@ -385,7 +455,7 @@ def main():
 
     import pxlengine_amd as pxl
     run = {"serve": bench_serve, "dam": bench_dam, "batch": bench_batch, "mixed256": bench_mixed256,
-           "obstacles": bench_obstacles}[args.workload]
+           "obstacles": bench_obstacles, "frame": bench_frame}[args.workload]
     line = run(args, pxl)
     print(json.dumps(line))
 

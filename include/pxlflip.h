@@ -214,6 +214,9 @@ typedef struct pxlflip_kernel_time {
     double alg_bytes; /* algorithmic bytes summed over the launches (0 where not priced) */
 } pxlflip_kernel_time;
 int pxlflip_profile_steps(pxlflip *h, uint32_t n, pxlflip_kernel_time *out, uint32_t max_out, uint32_t *n_out);
+/* The same for n surface passes (pxlflip_surface into internal buffers): one entry per stage
+ * (s_clear, s_splat, s_phi, s_vflag, s_vscan, s_verts, s_qcount, s_qscan, s_quads). */
+int pxlflip_profile_surface(pxlflip *h, uint32_t n, pxlflip_kernel_time *out, uint32_t max_out, uint32_t *n_out);
 
 /* Autotuner of the solver launch shape: for each candidate workgroup count of the pressure
  * kernels (64, 128, 256, 512, 1024) runs `steps` timed steps, keeps the fastest, and restores the

@@ -525,16 +525,19 @@ int pxlflip_grid_phase(pxlflip *h) {
     });
 }
 
-int pxlflip_profile_steps(pxlflip *h, uint32_t n, pxlflip_kernel_time *out, uint32_t max_out, uint32_t *n_out) {
-    if (!h) return PXLFLIP_ERR_INVALID;
-    Sim *s = &h->sim;
+}  // extern "C"
+
+namespace {
+// Runs work() with every launch event-timed and reports per-kernel totals, longest first.
+template <typename F>
+int profile_launches(Sim *s, F &&work, pxlflip_kernel_time *out, uint32_t max_out, uint32_t *n_out) {
     std::vector<pxlflip_kernel_time> agg;
     int rc = pxl::guarded(s, [&] {
         if (!s->initialized) s->reset();
         s->timer.clear();
         s->timer.on = true;
         try {
-            s->run_steps(n);
+            work();
         } catch (...) {
             s->timer.on = false;
             throw;
@@ -564,6 +567,28 @@ int pxlflip_profile_steps(pxlflip *h, uint32_t n, pxlflip_kernel_time *out, uint
         if (out) out[k] = agg[k];
     if (n_out) *n_out = (uint32_t)agg.size();
     return PXLFLIP_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int pxlflip_profile_steps(pxlflip *h, uint32_t n, pxlflip_kernel_time *out, uint32_t max_out, uint32_t *n_out) {
+    if (!h) return PXLFLIP_ERR_INVALID;
+    Sim *s = &h->sim;
+    return profile_launches(s, [&] { s->run_steps(n); }, out, max_out, n_out);
+}
+
+int pxlflip_profile_surface(pxlflip *h, uint32_t n, pxlflip_kernel_time *out, uint32_t max_out, uint32_t *n_out) {
+    if (!h) return PXLFLIP_ERR_INVALID;
+    Sim *s = &h->sim;
+    return profile_launches(
+        s,
+        [&] {
+            pxl::DeviceBuffer vb(32ull * s->surf.max_verts), ib(12ull * s->surf.max_tris);
+            uint32_t ctr[2];
+            for (uint32_t k = 0; k < n; k++) s->surface(vb.as<float>(), 0, ib.as<uint32_t>(), 0, ctr);
+        },
+        out, max_out, n_out);
 }
 
 int pxlflip_autotune(pxlflip *h, uint32_t steps, uint32_t *chosen_workgroups, double *best_ms) {

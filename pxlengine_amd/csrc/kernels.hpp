@@ -225,11 +225,15 @@ struct DevSurf {
 };
 constexpr uint32_t SURF_NONE = 0xFFFFFFFFu;
 constexpr float SURF_SD = 4096.0f;  // fixed-point scale of the surface density
-// The whole surface pass on the current positions: clear, splat, phi, vertices (numbered in
-// surface-cell order) and triangles (in (node, axis) order) into verts[8 (vbase + v)] and
-// indices[ibase + 3 t] (device memory).
-void launch_surface(const DevSurf &S, const float *px, const float *py, const float *pz, uint32_t np, float *verts,
-                    uint32_t vbase, uint32_t *indices, uint32_t ibase, hipStream_t s);
+// The surface pass on the current positions, in stages: clear, splat, phi, vertices (numbered
+// in surface-cell order by a scan of the surface flags) and triangles (in (node, axis) order,
+// by a scan of the per-node quad counts) into verts[8 (vbase + v)] and indices[ibase + 3 t]
+// (device memory).
+enum SurfStage { SURF_CLEAR, SURF_SPLAT, SURF_PHI, SURF_VFLAG, SURF_VSCAN, SURF_VERTS, SURF_QCOUNT, SURF_QSCAN,
+                 SURF_QUADS, SURF_STAGES };
+extern const char *const SURF_STAGE_NAMES[SURF_STAGES];
+void launch_surface_stage(int stage, const DevSurf &S, const float *px, const float *py, const float *pz, uint32_t np,
+                          float *verts, uint32_t vbase, uint32_t *indices, uint32_t ibase, hipStream_t s);
 
 // ---- launches ----
 // Every launch goes through PXL_LAUNCH.  While a profiler has set this thread's launch events,

@@ -140,7 +140,8 @@ struct Sim {
     // One surface pass on the current particles into device buffers; returns the counters.
     void surface(float *verts, uint32_t vbase, uint32_t *indices, uint32_t ibase, uint32_t ctr[2]) {
         ensure_surface();
-        launch_surface(surf, d.ax, d.ay, d.az, d.np, verts, vbase, indices, ibase, stream);
+        for (int st = 0; st < SURF_STAGES; st++)
+            KRUN(SURF_STAGE_NAMES[st], launch_surface_stage(st, surf, d.ax, d.ay, d.az, d.np, verts, vbase, indices, ibase, stream));
         HIP_CHECK(hipMemcpyAsync(host_ctr, surf.counters, 8, hipMemcpyDeviceToHost, stream));
         HIP_CHECK(hipStreamSynchronize(stream));
         ctr[0] = host_ctr[0];

@@ -15,6 +15,8 @@
 // not written.
 //
 // Compiled with -ffp-contract=off; division and sqrt are IEEE, as in the oracle.
+#include <climits>
+
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -45,30 +47,111 @@ __device__ __forceinline__ void cell_ijk(const DevSurf &S, uint32_t c, int &i, i
 // s_splat (fluid.slang:235-250): g = (x - origin) / sh, r = radius / sh; every node of
 // [ceil(g - r), floor(g + r)] (clipped to the grid) with q = 1 - |node - g|^2 / r^2 > 0 gets
 // int(q^3 4096).  Integer atomics: the sum does not depend on order.
-__global__ __launch_bounds__(SBLK) void k_s_splat(DevSurf S, const float *px, const float *py, const float *pz,
-                                                  uint32_t np) {
-    const uint32_t p = blockIdx.x * SBLK + threadIdx.x;
-    if (p >= np) return;
-    const float r = S.radius / S.sh;
-    const float x[3] = {px[p], py[p], pz[p]};
-    const float o[3] = {S.ox, S.oy, S.oz};
-    const int smax[3] = {S.sx, S.sy, S.sz};
+// Implementation: a workgroup takes SPLAT_CHUNK consecutive particles (the particle
+// arrays are in the last step's cell order, so a chunk covers a compact piece of the liquid),
+// adds their contributions into an LDS copy of the node box that bounds them, then adds the
+// box's nonzero entries to the grid -- one global atomic per node and chunk instead of one per
+// particle and node.  A chunk whose box exceeds SPLAT_LDS nodes takes the global path.
+constexpr uint32_t SPLAT_CHUNK = 2048;
+constexpr uint32_t SPLAT_PER_LANE = SPLAT_CHUNK / SBLK;
+constexpr int SPLAT_LDS = 16384;
+
+struct SplatBox {
     float g[3];
     int lo[3], hi[3];
+};
+__device__ __forceinline__ SplatBox splat_box(const DevSurf &S, float r, float x, float y, float z) {
+    SplatBox b;
+    const float xs[3] = {x, y, z}, o[3] = {S.ox, S.oy, S.oz};
+    const int smax[3] = {S.sx, S.sy, S.sz};
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        g[a] = (x[a] - o[a]) / S.sh;
-        lo[a] = max((int)ceilf(g[a] - r), 0);
-        hi[a] = min((int)floorf(g[a] + r), smax[a]);
+        b.g[a] = (xs[a] - o[a]) / S.sh;
+        b.lo[a] = max((int)ceilf(b.g[a] - r), 0);
+        b.hi[a] = min((int)floorf(b.g[a] + r), smax[a]);
     }
+    return b;
+}
+
+__global__ __launch_bounds__(SBLK) void k_s_splat_lds(DevSurf S, const float *px, const float *py, const float *pz,
+                                                      uint32_t np) {
+    __shared__ int32_t box[SPLAT_LDS];
+    __shared__ int red[6][SBLK / WAVE];
+    const uint32_t base = blockIdx.x * SPLAT_CHUNK;
+    const float r = S.radius / S.sh;
     const float rr = r * r;
-    for (int k = lo[2]; k <= hi[2]; k++)
-        for (int j = lo[1]; j <= hi[1]; j++)
-            for (int i = lo[0]; i <= hi[0]; i++) {
-                const float d0 = (float)i - g[0], d1 = (float)j - g[1], d2 = (float)k - g[2];
-                const float q = 1.0f - (d0 * d0 + d1 * d1 + d2 * d2) / rr;
-                if (q > 0.0f) atomicAdd(&S.density[node_id(S, i, j, k)], (int32_t)(q * q * q * SURF_SD));
-            }
+    // the chunk's node box (particles whose own box is empty do not count)
+    int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+    for (uint32_t m = 0; m < SPLAT_PER_LANE; m++) {
+        const uint32_t p = base + m * SBLK + threadIdx.x;
+        if (p >= np) break;
+        const SplatBox b = splat_box(S, r, px[p], py[p], pz[p]);
+        if (b.lo[0] > b.hi[0] || b.lo[1] > b.hi[1] || b.lo[2] > b.hi[2]) continue;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            lo[a] = min(lo[a], b.lo[a]);
+            hi[a] = max(hi[a], b.hi[a]);
+        }
+    }
+    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        int l = lo[a], h = hi[a];
+        for (int off = WAVE / 2; off > 0; off >>= 1) {
+            l = min(l, __shfl_xor(l, off, WAVE));
+            h = max(h, __shfl_xor(h, off, WAVE));
+        }
+        if (lane == 0) {
+            red[a][wv] = l;
+            red[3 + a][wv] = h;
+        }
+    }
+    __syncthreads();
+    int blo[3], ext[3];
+    bool empty = false;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        int l = red[a][0], h = red[3 + a][0];
+        for (int w = 1; w < (int)(SBLK / WAVE); w++) {
+            l = min(l, red[a][w]);
+            h = max(h, red[3 + a][w]);
+        }
+        blo[a] = l;
+        ext[a] = h - l + 1;
+        if (h < l) empty = true;
+    }
+    if (empty) return;  // uniform: every lane read the same reduction
+    const long vol = (long)ext[0] * ext[1] * ext[2];
+    const bool lds = vol <= SPLAT_LDS;
+    if (lds) {
+        for (int q = threadIdx.x; q < (int)vol; q += SBLK) box[q] = 0;
+        __syncthreads();
+    }
+    for (uint32_t m = 0; m < SPLAT_PER_LANE; m++) {
+        const uint32_t p = base + m * SBLK + threadIdx.x;
+        if (p >= np) break;
+        const SplatBox b = splat_box(S, r, px[p], py[p], pz[p]);
+        for (int k = b.lo[2]; k <= b.hi[2]; k++)
+            for (int j = b.lo[1]; j <= b.hi[1]; j++)
+                for (int i = b.lo[0]; i <= b.hi[0]; i++) {
+                    const float d0 = (float)i - b.g[0], d1 = (float)j - b.g[1], d2 = (float)k - b.g[2];
+                    const float q = 1.0f - (d0 * d0 + d1 * d1 + d2 * d2) / rr;
+                    if (!(q > 0.0f)) continue;
+                    const int32_t v = (int32_t)(q * q * q * SURF_SD);
+                    if (lds)
+                        atomicAdd(&box[(i - blo[0]) + ext[0] * ((j - blo[1]) + ext[1] * (k - blo[2]))], v);
+                    else
+                        atomicAdd(&S.density[node_id(S, i, j, k)], v);
+                }
+    }
+    if (!lds) return;
+    __syncthreads();
+    for (int q = threadIdx.x; q < (int)vol; q += SBLK) {
+        const int32_t v = box[q];
+        if (v == 0) continue;
+        const int i = blo[0] + q % ext[0], j = blo[1] + (q / ext[0]) % ext[1], k = blo[2] + q / (ext[0] * ext[1]);
+        atomicAdd(&S.density[node_id(S, i, j, k)], v);
+    }
 }
 
 // s_phi (fluid.slang:253-268).
@@ -257,17 +340,25 @@ inline uint32_t grid_for(uint64_t n) { return (uint32_t)((n + SBLK - 1) / SBLK);
 
 }  // namespace
 
-void launch_surface(const DevSurf &S, const float *px, const float *py, const float *pz, uint32_t np, float *verts,
-                    uint32_t vbase, uint32_t *indices, uint32_t ibase, hipStream_t s) {
-    launch_fill_u32(reinterpret_cast<uint32_t *>(S.density), S.nnodes, 0u, s);
-    if (np) PXL_LAUNCH(k_s_splat, grid_for(np), SBLK, 0, s, S, px, py, pz, np);
-    PXL_LAUNCH(k_s_phi, grid_for(S.nnodes), SBLK, 0, s, S);
-    PXL_LAUNCH(k_s_vflag, grid_for(S.ncells), SBLK, 0, s, S);
-    launch_exclusive_scan(S.cell_vert, S.ncells, S.scan_tmp, S.cell_vert, s);
-    PXL_LAUNCH(k_s_verts, grid_for(S.ncells), SBLK, 0, s, S, verts, vbase);
-    PXL_LAUNCH(k_s_qcount, grid_for(S.nnodes), SBLK, 0, s, S);
-    launch_exclusive_scan(S.qoff, S.nnodes, S.scan_tmp, S.qoff, s);
-    PXL_LAUNCH(k_s_quads, grid_for(S.nnodes), SBLK, 0, s, S, indices, ibase);
+const char *const SURF_STAGE_NAMES[SURF_STAGES] = {"s_clear", "s_splat", "s_phi",   "s_vflag", "s_vscan",
+                                                   "s_verts", "s_qcount", "s_qscan", "s_quads"};
+
+void launch_surface_stage(int stage, const DevSurf &S, const float *px, const float *py, const float *pz, uint32_t np,
+                          float *verts, uint32_t vbase, uint32_t *indices, uint32_t ibase, hipStream_t s) {
+    switch (stage) {
+        case SURF_CLEAR: launch_fill_u32(reinterpret_cast<uint32_t *>(S.density), S.nnodes, 0u, s); break;
+        case SURF_SPLAT:
+            if (np) PXL_LAUNCH(k_s_splat_lds, (np + SPLAT_CHUNK - 1) / SPLAT_CHUNK, SBLK, 0, s, S, px, py, pz, np);
+            break;
+        case SURF_PHI: PXL_LAUNCH(k_s_phi, grid_for(S.nnodes), SBLK, 0, s, S); break;
+        case SURF_VFLAG: PXL_LAUNCH(k_s_vflag, grid_for(S.ncells), SBLK, 0, s, S); break;
+        case SURF_VSCAN: launch_exclusive_scan(S.cell_vert, S.ncells, S.scan_tmp, S.cell_vert, s); break;
+        case SURF_VERTS: PXL_LAUNCH(k_s_verts, grid_for(S.ncells), SBLK, 0, s, S, verts, vbase); break;
+        case SURF_QCOUNT: PXL_LAUNCH(k_s_qcount, grid_for(S.nnodes), SBLK, 0, s, S); break;
+        case SURF_QSCAN: launch_exclusive_scan(S.qoff, S.nnodes, S.scan_tmp, S.qoff, s); break;
+        case SURF_QUADS: PXL_LAUNCH(k_s_quads, grid_for(S.nnodes), SBLK, 0, s, S, indices, ibase); break;
+        default: break;
+    }
 }
 
 }  // namespace pxl

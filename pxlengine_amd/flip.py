@@ -126,6 +126,53 @@ class ServerStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class DeviceBuffer:
+    """Device memory from the HIP runtime the library uses (for the device entry points, e.g.
+    the renderer-side vertex / index buffers of FluidState.surface_device)."""
+
+    _hip = None
+
+    def __init__(self, nbytes: int):
+        if DeviceBuffer._hip is None:
+            import os
+            path = "/opt/rocm/lib/libamdhip64.so"
+            h = C.CDLL(path if os.path.exists(path) else "libamdhip64.so")
+            h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+            h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+            h.hipFree.argtypes = [C.c_void_p]
+            DeviceBuffer._hip = h
+        self.nbytes = int(nbytes)
+        self._p = C.c_void_p()
+        if DeviceBuffer._hip.hipMalloc(C.byref(self._p), max(self.nbytes, 4)) != 0:
+            raise MemoryError(f"hipMalloc of {nbytes} bytes failed")
+
+    @property
+    def ptr(self) -> int:
+        return int(self._p.value)
+
+    def upload(self, host: np.ndarray):
+        assert host.nbytes <= self.nbytes
+        if DeviceBuffer._hip.hipMemcpy(self._p, host.ctypes.data, host.nbytes, 1) != 0:
+            raise FluidError("hipMemcpy to the device failed")
+
+    def download(self, out: np.ndarray) -> np.ndarray:
+        assert out.nbytes <= self.nbytes
+        if DeviceBuffer._hip.hipMemcpy(out.ctypes.data, self._p, out.nbytes, 2) != 0:
+            raise FluidError("hipMemcpy from the device failed")
+        return out
+
+    def free(self):
+        if self._p:
+            DeviceBuffer._hip.hipFree(self._p)
+            self._p = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class SurfaceInfo(C.Structure):
     _fields_ = [
         ("sx", C.c_uint32), ("sy", C.c_uint32), ("sz", C.c_uint32), ("max_triangles", C.c_uint32),
@@ -166,6 +213,7 @@ _SIGNATURES = {
     "pxlflip_grid_phase": (C.c_int, [_H]),
     "pxlflip_last_solve": (C.c_int, [_H, C.POINTER(SolveStats)]),
     "pxlflip_profile_steps": (C.c_int, [_H, C.c_uint32, C.POINTER(KernelTime), C.c_uint32, C.POINTER(C.c_uint32)]),
+    "pxlflip_profile_surface": (C.c_int, [_H, C.c_uint32, C.POINTER(KernelTime), C.c_uint32, C.POINTER(C.c_uint32)]),
     "pxlflip_autotune": (C.c_int, [_H, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_double)]),
     "pxlflip_last_error": (C.c_char_p, [_H]),
     "pxlflip_server_default_opts": (C.c_int, [C.POINTER(ServerOpts)]),
@@ -400,6 +448,14 @@ class FluidState:
         self._call(self._lib.pxlflip_profile_steps, int(n), arr, 64, C.byref(cnt))
         return [(arr[k].name.decode(), int(arr[k].launches), float(arr[k].total_ms))
                 for k in range(min(cnt.value, 64))]
+
+    def profile_surface(self, n: int = 1) -> list:
+        """Runs n surface passes with every stage timed by HIP events; [(name, launches, total_ms)]."""
+        arr = (KernelTime * 16)()
+        cnt = C.c_uint32()
+        self._call(self._lib.pxlflip_profile_surface, int(n), arr, 16, C.byref(cnt))
+        return [(arr[k].name.decode(), int(arr[k].launches), float(arr[k].total_ms))
+                for k in range(min(cnt.value, 16))]
 
     def autotune(self, steps: int = 2) -> dict:
         """Picks the solver workgroup count by timing `steps` steps per candidate; state unchanged."""

@@ -52,8 +52,11 @@ def _check(orc, gpu, fluid, what):
     (EXAMPLE_DAM_BREAK, 2),
     (sphere_drop(28), 3),
     (obstacle_scene(24), 3),
-], ids=["dam24", "refine3", "refine1", "example", "sphere28", "obstacles24"])
+    (dam_break(64, ppc=1), 1),
+], ids=["dam24", "refine3", "refine1", "example", "sphere28", "obstacles24", "sparse64"])
 def test_surface_bitwise(pxl, orc, fluid, steps):
+    """(sparse64: one particle per cell, so a 2048-particle splat chunk spans more nodes than
+    its LDS box holds and takes the global-atomic path.)"""
     gpu = pxl.FluidState(fluid, FPS, seed=0)
     _check(orc, gpu, fluid, "at seeding")
     gpu.step(steps)
@@ -76,35 +79,9 @@ def test_surface_overflow_clipping(pxl, orc):
     gpu.close()
 
 
-class _Hip:
-    """hipMalloc / hipMemcpy / hipFree of the HIP runtime the library uses."""
-
-    def __init__(self):
-        import ctypes as C
-        import os
-        path = "/opt/rocm/lib/libamdhip64.so"
-        self.C = C
-        self.lib = C.CDLL(path if os.path.exists(path) else "libamdhip64.so")
-        self.lib.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
-        self.lib.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
-        self.lib.hipFree.argtypes = [C.c_void_p]
-
-    def alloc_from(self, host: np.ndarray):
-        p = self.C.c_void_p()
-        assert self.lib.hipMalloc(self.C.byref(p), host.nbytes) == 0
-        assert self.lib.hipMemcpy(p, host.ctypes.data, host.nbytes, 1) == 0  # host to device
-        return p
-
-    def read(self, p, like: np.ndarray) -> np.ndarray:
-        out = np.empty_like(like)
-        assert self.lib.hipMemcpy(out.ctypes.data, p, out.nbytes, 2) == 0  # device to host
-        return out
-
-
 def test_surface_into_device_buffers(pxl):
     """The device entry point writes the same mesh at vbase / ibase of caller buffers and
     leaves the rest of them untouched."""
-    hip = _Hip()
     fluid = dam_break(24)
     gpu = pxl.FluidState(fluid, FPS, seed=0)
     gpu.step(2)
@@ -113,14 +90,16 @@ def test_surface_into_device_buffers(pxl):
     vbase, ibase = 1000, 777
     vhost = np.full((vbase + cap, 8), -7.0, np.float32)
     ihost = np.full(ibase + 3 * cap, 0xABCDEF, np.uint32)
-    vdev, idev = hip.alloc_from(vhost), hip.alloc_from(ihost)
+    vdev, idev = pxl.DeviceBuffer(vhost.nbytes), pxl.DeviceBuffer(ihost.nbytes)
     try:
-        info2 = gpu.surface_device(vdev.value, vbase, idev.value, ibase)
-        v = hip.read(vdev, vhost)
-        ib = hip.read(idev, ihost)
+        vdev.upload(vhost)
+        idev.upload(ihost)
+        info2 = gpu.surface_device(vdev.ptr, vbase, idev.ptr, ibase)
+        v = vdev.download(np.empty_like(vhost))
+        ib = idev.download(np.empty_like(ihost))
     finally:
-        hip.lib.hipFree(vdev)
-        hip.lib.hipFree(idev)
+        vdev.free()
+        idev.free()
     assert info2 == info
     np.testing.assert_array_equal(v[vbase:vbase + info["vertices"]], verts)
     np.testing.assert_array_equal(ib[ibase:ibase + 3 * info["triangles"]].reshape(-1, 3), tris)
