@@ -131,6 +131,11 @@ def surface_algorithmic_bytes(stage, P, nodes, cells, touched, V, T):
             "s_quads": 8 * nodes + 12 * T}.get(stage, 0)
 
 
+# kernel symbols of the surface stages (rocprofv3 names)
+SURF_KERNELS = {"s_splat": "k_s_splat_lds", "s_phi": "k_s_phi", "s_vflag": "k_s_vflag", "s_verts": "k_s_verts",
+                "s_qcount": "k_s_qcount", "s_quads": "k_s_quads"}
+
+
 def bench_frame(args, pxl):
     """Frames of the 128^3 dam break: `substeps` (5) plain FP32 CG-80 steps plus one surface-nets
     pass (refine 2: a 256^3 surface grid) per frame, the reference's per-frame work
@@ -184,7 +189,8 @@ def bench_frame(args, pxl):
                    "surface_ms_per_frame_kernels": mesh_ms, "vertices": V, "triangles": T,
                    "surface_nodes": nodes, "touched_nodes": touched, "reservation": cap},
         "roofline": {"bound": "hbm", "achieved": dom["achieved_GBs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": (dom["achieved_GBs"] or 0.0) / PEAK_HBM_GBS, "traffic": None,
+                     "frac": (dom["achieved_GBs"] or 0.0) / PEAK_HBM_GBS,
+                     "traffic": measured_traffic(SURF_KERNELS.get(dom["kernel"], dom["kernel"]), "frame128"),
                      "kernel": dom["kernel"], "avg_us": dom["avg_us"], "alg_bytes_per_launch": dom["alg_bytes"]},
         "cpu_baseline": None, "kernels": stages,
     }

@@ -145,6 +145,10 @@ int pxlflip_write_particles(pxlflip *h, const float *aos8);
 #define PXLFLIP_FIELD_PRESSURE 6   /* float[n_cells] */
 #define PXLFLIP_FIELD_DIVERGENCE 7 /* float[n_cells] */
 #define PXLFLIP_FIELD_PRESSURE64 8 /* double[n_cells]: FP64 solution of the MIXED solver */
+/* CFL cap (extension, the fluid block's `cfl` field; no reference counterpart): the number k of
+ * equal substeps (dt / k each, f32) the last step ran; 1 without the cap. */
+int pxlflip_last_substeps(const pxlflip *h, uint32_t *k);
+
 /* Surface-net fields of the last pxlflip_surface pass (fluid.slang:39-41; nodes (sx+1)(sy+1)(sz+1),
  * cells sx sy sz, x fastest; sizes in pxlflip_surface_info). */
 #define PXLFLIP_FIELD_DENSITY 9    /* int32[nodes]: splatted density, fixed point 4096 */

@@ -108,6 +108,8 @@ def lib():
         L.orc_poisson_relres64.restype = C.c_double
         L.orc_poisson_solve64.argtypes = [SP, dp, C.c_double, C.c_int, dp]
         L.orc_poisson_solve64.restype = C.c_int
+        L.orc_cfl_substeps.argtypes = [SP, C.c_float, C.c_float]
+        L.orc_cfl_substeps.restype = C.c_uint32
         SPP = C.POINTER(SurfaceParams)
         L.orc_surface_params.argtypes = [u32p, C.c_float, f32p, C.c_uint32, C.c_float, C.c_uint32, C.c_uint32, SPP]
         L.orc_surface_splat.argtypes = [SPP, f32p, f32p, f32p, C.c_uint32, C.POINTER(C.c_int32)]
@@ -310,6 +312,27 @@ class Sim:
         v = np.asarray(v, np.float32)
         self.px[:], self.py[:], self.pz[:] = x[:, 0], x[:, 1], x[:, 2]
         self.vx[:], self.vy[:], self.vz[:] = v[:, 0], v[:, 1], v[:, 2]
+
+    # -- CFL cap (extension)
+    def cfl_substeps(self, cfl):
+        return int(self._L.orc_cfl_substeps(self._s, np.float32(self.params.dt), np.float32(cfl)))
+
+    def set_dt(self, dt):
+        self._s.contents.p.dt = float(np.float32(dt))
+        self.params.dt = float(np.float32(dt))
+
+    def step_rbgs_capped(self, iters, cfl):
+        """One step of dt under the CFL cap: k = cfl_substeps, then k reference steps of dt / k."""
+        dt = np.float32(self.params.dt)
+        k = self.cfl_substeps(cfl)
+        if k > 1:
+            self.set_dt(dt / np.float32(k))
+        try:
+            for _ in range(k):
+                self.step_rbgs(iters)
+        finally:
+            self.set_dt(dt)
+        return k
 
     # -- kernels
     def clear_grid(self): self._L.orc_clear_grid(self._s)

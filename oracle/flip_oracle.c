@@ -574,6 +574,20 @@ void orc_step_cg(orc_state *s, uint32_t iters) {
     orc_g2p_advect(s);
 }
 
+/* ------------------------------------------------------------------ CFL cap (extension) */
+
+uint32_t orc_cfl_substeps(const orc_state *s, float dt, float cfl) {
+    float vmax2 = 0.0f;
+    for (uint32_t i = 0; i < s->p.n_particles; i++) {
+        const float v2 = s->vx[i] * s->vx[i] + s->vy[i] * s->vy[i] + s->vz[i] * s->vz[i];
+        if (v2 > vmax2) vmax2 = v2;
+    }
+    const double cells = sqrt((double)vmax2) * (double)dt / ((double)cfl * (double)s->p.h);
+    if (!(cells > 1.0)) return 1;
+    if (cells >= 64.0) return 64;
+    return (uint32_t)ceil(cells);
+}
+
 /* ------------------------------------------------------------------ surface nets */
 
 void orc_surface_params(const uint32_t n[3], float h, const float origin[3], uint32_t refine,

@@ -123,6 +123,12 @@ void orc_surface_phi(const orc_surface *sp, const int32_t *density, float *phi);
 void orc_surface_mesh(const orc_surface *sp, const float *phi, uint32_t *cell_vert, float *verts,
                       uint32_t *indices, uint32_t counters[2]);
 
+/* ---- CFL cap (extension; the reference steps with a fixed dt) ----
+ * k = ceil(|v|max * dt / (cfl * h)) in double, |v|max^2 = max over particles of
+ * (vx vx + vy vy) + vz vz in f32; 1 when that is <= 1, at most 64.  A capped step runs k
+ * steps of dt / k (f32). */
+uint32_t orc_cfl_substeps(const orc_state *s, float dt, float cfl);
+
 /* ---- binning: canonical counting sort (ids ascending inside each cell) ---- */
 void orc_bin(const orc_state *s, uint32_t *cell_of, uint32_t *counts, uint32_t *offsets,
              uint32_t *sorted_ids);

@@ -433,6 +433,12 @@ void fill_surface_info(const pxl::Sim &s, const uint32_t ctr[2], pxlflip_surface
 }
 }  // namespace
 
+int pxlflip_last_substeps(const pxlflip *h, uint32_t *k) {
+    if (!h || !k) return PXLFLIP_ERR_INVALID;
+    *k = h->sim.last_substeps;
+    return PXLFLIP_OK;
+}
+
 int pxlflip_surface(pxlflip *h, float *verts, uint32_t vbase, uint32_t *indices, uint32_t ibase,
                     pxlflip_surface_info *info) {
     if (!h) return PXLFLIP_ERR_INVALID;

@@ -134,7 +134,7 @@ struct DevSim {
     double *scal;
     double *norms;  // where the refinement norms {rr, bb} are totalled (scal + SC_RR, or a
                     // batch's per-sim array)
-    unsigned *ctr;
+    unsigned *ctr;  // [0]: the CFL cap's largest squared particle speed (launch_vmax2)
     uint32_t *scan_tmp;
     // initial particles (reset)
     float *ix, *iy, *iz, *ivx, *ivy, *ivz;
@@ -257,6 +257,9 @@ LaunchEvents &launch_events();  // this thread's slot
 
 // ---- launch helpers (all asynchronous on `s`) ----
 void launch_fill_u32(uint32_t *p, uint32_t n, uint32_t value, hipStream_t s);
+// *out = the bits of max over the particles of (vx vx + vy vy) + vz vz (f32; non-negative
+// floats order as their bit patterns); clears *out first.
+void launch_vmax2(const DevSim &S, uint32_t *out, hipStream_t s);
 void launch_clear_grid(const DevSim &S, hipStream_t s);       // count = 0 (the last step's tiles), row counts = 0
 void launch_bin_count(const DevSim &S, hipStream_t s);        // key, slot, count
 void launch_scan_offsets(const DevSim &S, hipStream_t s);     // offset = exclusive scan(count)

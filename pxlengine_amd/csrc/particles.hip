@@ -681,6 +681,24 @@ __global__ __launch_bounds__(BLOCK) void k_reset_particles(DevSim S) {
 
 }  // namespace
 
+// Largest squared particle speed, for the CFL cap: a wave maximum, then one atomic per wave.
+__global__ __launch_bounds__(BLOCK) void k_vmax2(DevSim S, uint32_t *out) {
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    float v2 = 0.0f;
+    if (i < S.np) {
+        const float vx = S.avx[i], vy = S.avy[i], vz = S.avz[i];
+        v2 = vx * vx + vy * vy + vz * vz;
+        if (!(v2 >= 0.0f)) v2 = 0.0f;  // NaN
+    }
+    for (int off = WAVE / 2; off > 0; off >>= 1) v2 = fmaxf(v2, __shfl_xor(v2, off, WAVE));
+    if ((threadIdx.x & (WAVE - 1)) == 0) atomicMax(out, __float_as_uint(v2));
+}
+
+void launch_vmax2(const DevSim &S, uint32_t *out, hipStream_t s) {
+    launch_fill_u32(out, 1, 0u, s);
+    if (S.np) PXL_LAUNCH(k_vmax2, blocks_for(S.np), BLOCK, 0, s, S, out);
+}
+
 void launch_fill_u32(uint32_t *p, uint32_t n, uint32_t value, hipStream_t s) {
     if (n) PXL_LAUNCH(k_fill_u32, blocks_for(n), BLOCK, 0, s, p, n, value);
 }

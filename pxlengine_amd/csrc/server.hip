@@ -1006,6 +1006,17 @@ int pxlflip_server_create(const pxlflip_server_opts *opts, pxlflip_server **out)
     return PXLFLIP_OK;
 }
 
+namespace {
+// Batched members step in lockstep with one dt; the CFL cap's per-simulation substeps do not fit.
+bool batchable(const pxl::Sim &s) {
+    if (s.spec.cfl > 0.0f) {
+        pxl::g_create_error = "fluid '" + s.spec.name + "': cfl (the CFL cap) is not supported in batched execution";
+        return false;
+    }
+    return true;
+}
+}  // namespace
+
 int pxlflip_server_submit(pxlflip_server *srv, const char *fluid_json, double fps, uint32_t seed, uint32_t steps,
                           int32_t keep_particles, uint64_t *id) {
     using namespace pxl;
@@ -1013,7 +1024,8 @@ int pxlflip_server_submit(pxlflip_server *srv, const char *fluid_json, double fp
     auto r = std::make_unique<Request>();
     r->t_submit = now_s();
     Sim probe;
-    if (prepare(fluid_json, fps, seed, &srv->sim_opts, probe, nullptr, nullptr) != PXLFLIP_OK) {
+    if (prepare(fluid_json, fps, seed, &srv->sim_opts, probe, nullptr, nullptr) != PXLFLIP_OK ||
+        !batchable(probe)) {
         std::lock_guard<std::mutex> lk(srv->mu);
         srv->error = g_create_error;
         return PXLFLIP_ERR_INVALID;
@@ -1274,6 +1286,7 @@ int pxlflip_batch_create(const char *const *fluid_jsons, const uint32_t *seeds, 
             b->sim_opts.scene_index = i;
             const int rc = prepare(fluid_jsons[i], fps, seeds ? seeds[i] : 0u, &b->sim_opts, *sim, &x0, &v0);
             if (rc != PXLFLIP_OK) return rc;
+            if (!batchable(*sim)) return PXLFLIP_ERR_INVALID;
             sim->d.cg_blocks = (int)(o.solver_workgroups ? o.solver_workgroups : batch_solver_blocks(*sim));
             HIP_CHECK(hipStreamCreateWithFlags(&sim->stream, hipStreamNonBlocking));
             sim->allocate(x0, v0);

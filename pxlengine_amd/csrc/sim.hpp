@@ -287,7 +287,7 @@ struct Sim {
             HIP_CHECK(hipMemcpyAsync(d.solid_bits, bits.data(), 4 * bits.size(), hipMemcpyHostToDevice, stream));
             HIP_CHECK(hipMemcpyAsync(d.obst, spheres.data(), 4 * spheres.size(), hipMemcpyHostToDevice, stream));
         }
-        HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&host_scal), 8 * SC_COUNT, hipHostMallocDefault));
+        HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&host_scal), 8 * (SC_COUNT + 1), hipHostMallocDefault));
         HIP_CHECK(hipStreamSynchronize(stream));
     }
 
@@ -390,8 +390,68 @@ struct Sim {
         stats.inner_iterations = inner_total;
     }
 
+    // CFL cap (extension): substeps of the next step from the particles' largest speed.
+    uint32_t last_substeps = 1;
+    uint32_t cfl_step_count() {
+        launch_vmax2(d, d.ctr, stream);
+        HIP_CHECK(hipMemcpyAsync(host_scal + SC_COUNT, d.ctr, 4, hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        uint32_t bits;
+        std::memcpy(&bits, host_scal + SC_COUNT, 4);
+        float v2;
+        std::memcpy(&v2, &bits, 4);
+        return cfl_substeps(v2, d.dt, spec.cfl, d.h);
+    }
+    // One step of the current d.dt (graphs when allowed, else eager).
+    void step_once() {
+        if (opts.solver == PXLFLIP_SOLVER_MIXED) {
+            replay(&grid_graph, [this] { enqueue_grid_init(); });
+            mixed_pressure();
+            replay(&post_graph, [this] { enqueue_post(); });
+        } else {
+            const bool graph = opts.use_graph && !timer.on;
+            if (graph && !step_graph) step_graph = capture([this] { enqueue_fixed_step(); });
+            if (graph) HIP_CHECK(hipGraphLaunch(step_graph, stream));
+            else enqueue_fixed_step();
+            stats.rel_residual = -1.0;
+            stats.inner_iterations = iters;
+            stats.outer_iterations = 0;
+        }
+    }
+    // With the CFL cap each step of dt runs as k substeps of dt / k (f32).  Graphs hold the
+    // full dt, so the substeps of a split step launch eagerly.
+    void run_capped_steps(uint32_t n) {
+        for (uint32_t s = 0; s < n; s++) {
+            const uint32_t k = cfl_step_count();
+            last_substeps = k;
+            if (k == 1) {
+                step_once();
+            } else {
+                const float dt0 = d.dt;
+                const int g = opts.use_graph;
+                d.dt = dt0 / (float)k;
+                opts.use_graph = 0;
+                try {
+                    for (uint32_t q = 0; q < k; q++) step_once();
+                } catch (...) {
+                    d.dt = dt0;
+                    opts.use_graph = g;
+                    throw;
+                }
+                d.dt = dt0;
+                opts.use_graph = g;
+            }
+            steps_done++;
+        }
+        HIP_CHECK(hipStreamSynchronize(stream));
+    }
+
     void run_steps(uint32_t n) {
         if (n == 0) return;
+        if (spec.cfl > 0.0f) {
+            run_capped_steps(n);
+            return;
+        }
         if (opts.solver == PXLFLIP_SOLVER_MIXED) {
             for (uint32_t s = 0; s < n; s++) {
                 replay(&grid_graph, [this] { enqueue_grid_init(); });

@@ -16,7 +16,7 @@ namespace {
 const char *const kFluidFields[] = {"name", "bounds", "resolution", "fill", "particles_per_cell",
                                     "substeps", "flip", "gravity", "pressure_iterations",
                                     "surface_refine", "particle_radius", "warmup",
-                                    "max_triangles", "materials", "obstacles"};
+                                    "max_triangles", "materials", "obstacles", "cfl"};
 const char *const kBlockFields[] = {"min", "max", "velocity"};
 const char *const kSphereFields[] = {"center", "radius", "velocity"};  // extension
 const char *const kObstacleFields[] = {"center", "radius"};              // extension
@@ -187,6 +187,8 @@ std::string parse_fluid(const std::string &json, FluidSpec &f) {
             e = get_f32(v, f.warmup);
         } else if (k == "max_triangles") {
             e = get_u32(v, f.max_triangles);
+        } else if (k == "cfl") {
+            e = get_f32(v, f.cfl);
         } else if (k == "obstacles") {
             if (v.kind != JsonValue::Array) e = "invalid type: " + json_describe(v) + ", expected a sequence";
             else {
@@ -257,6 +259,7 @@ std::string validate_fluid(const FluidSpec &f, uint32_t index, uint64_t max_cell
             return ctx + "each fill block must have max > min and lie inside `bounds`";
         }
     }
+    if (!(f.cfl >= 0.0f) || !std::isfinite(f.cfl)) return ctx + "cfl must be >= 0 (0 = off)";
     if (f.obstacles.size() > MAX_OBSTACLES) return ctx + "at most 65536 obstacles";
     for (const auto &o : f.obstacles)
         if (!(o.radius > 0.0f) || any_lt(o.center, lo) || any_lt(hi, o.center))
@@ -266,6 +269,14 @@ std::string validate_fluid(const FluidSpec &f, uint32_t index, uint64_t max_cell
     fluid_grid(f, n, &h);
     if ((uint64_t)n[0] * n[1] * n[2] > max_cells) return ctx + "grid too large (lower resolution)";
     return "";
+}
+
+uint32_t cfl_substeps(float vmax2, float dt, float cfl, float h) {
+    const double u = std::sqrt((double)vmax2);
+    const double cells = u * (double)dt / ((double)cfl * (double)h);
+    if (!(cells > 1.0)) return 1;  // also NaN
+    if (cells >= (double)MAX_CFL_SUBSTEPS) return MAX_CFL_SUBSTEPS;
+    return (uint32_t)std::ceil(cells);
 }
 
 float fluid_dt(double fps, uint32_t substeps) { return (float)(1.0 / (fps * (double)substeps)); }

@@ -46,7 +46,14 @@ struct FluidSpec {
     uint32_t max_triangles = 2000000;
     uint32_t n_materials = 0;  // rendering-only; accepted, not interpreted
     std::vector<ObstacleSpec> obstacles;  // extension
+    // extension: CFL cap.  0 = off (the reference's fixed dt).  Otherwise each step of dt is run
+    // as k equal substeps, k = ceil(|v|max dt / (cfl h)) (|v|max over the particles at the
+    // step's start), at most MAX_CFL_SUBSTEPS.
+    float cfl = 0.0f;
 };
+constexpr uint32_t MAX_CFL_SUBSTEPS = 64;
+// Substeps of one step under the CFL cap, from the particles' largest squared speed (f32).
+uint32_t cfl_substeps(float vmax2, float dt, float cfl, float h);
 
 // Strict parse of one fluids[] object.  Returns "" on success, else the error message.
 std::string parse_fluid(const std::string &json, FluidSpec &out);

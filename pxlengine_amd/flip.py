@@ -198,6 +198,7 @@ _SIGNATURES = {
     "pxlflip_destroy": (None, [_H]),
     "pxlflip_surface": (C.c_int, [_H, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.POINTER(SurfaceInfo)]),
     "pxlflip_surface_host": (C.c_int, [_H, C.c_void_p, C.c_void_p, C.POINTER(SurfaceInfo)]),
+    "pxlflip_last_substeps": (C.c_int, [_H, C.POINTER(C.c_uint32)]),
     "pxlflip_advance_to_frame": (C.c_int, [_H, C.c_double]),
     "pxlflip_step": (C.c_int, [_H, C.c_uint32]),
     "pxlflip_reset": (C.c_int, [_H]),
@@ -387,6 +388,12 @@ class FluidState:
         if a.shape != (self.info["n_particles"], 8):
             raise ValueError(f"expected ({self.info['n_particles']}, 8) records, got {a.shape}")
         self._call(self._lib.pxlflip_write_particles, a.ctypes.data)
+
+    def last_substeps(self) -> int:
+        """Substeps of the last step under the CFL cap (1 without it)."""
+        k = C.c_uint32()
+        self._call(self._lib.pxlflip_last_substeps, C.byref(k))
+        return int(k.value)
 
     def surface(self):
         """Surface nets of the current particles (fluid.rs:308-330) -> (verts[V, 8] as {pos, 0,

@@ -184,6 +184,13 @@ def test_obstacles_parse_validate_and_seed(pxl, orc):
     assert "expected a sequence" in str(_err(pxl, dict(f, obstacles={"center": [0.5, 0.5, 0.5]})))
 
 
+def test_cfl_field(pxl):
+    """CFL cap (an extension): `cfl` parses as a number, must be >= 0, 0 means off."""
+    assert pxl.check(dict(dam_break(24), cfl=0.5))["n_particles"] == pxl.check(dam_break(24))["n_particles"]
+    assert "cfl must be >= 0" in str(_err(pxl, dict(dam_break(24), cfl=-1.0)))
+    assert "invalid type" in str(_err(pxl, dict(dam_break(24), cfl="fast")))
+
+
 def test_device_bytes_reported(pxl):
     small = pxl.check(dam_break(32))["device_bytes"]
     mixed = pxl.check(dam_break(32), solver="mixed")["device_bytes"]

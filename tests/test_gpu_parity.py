@@ -81,6 +81,29 @@ def test_rbgs_steps_bitwise(pxl, orc, fluid, steps):
     gpu.close()
 
 
+def test_cfl_capped_steps_bitwise(pxl, orc):
+    """CFL cap (an extension): liquid thrown fast enough that steps split into several
+    substeps; each step's substep count equals the oracle's (largest particle speed) and
+    the oracle replaying k reference steps of dt / k gives the GPU's particles, pressure and
+    face velocities bit for bit."""
+    fluid = dict(two_blocks(24), cfl=0.4)
+    fluid["fill"] = [dict(b, velocity=[6.0 if b["velocity"][0] > 0 else -6.0, -2.0, 0.0]) for b in fluid["fill"]]
+    gpu = pxl.FluidState(fluid, FPS, seed=3)
+    sim = oracle_sim(orc, fluid, seed=3)
+    ks = []
+    for s in range(3):
+        print(f"cfl step {s + 1}", flush=True)
+        gpu.step(1)
+        k = sim.step_rbgs_capped(fluid["pressure_iterations"], fluid["cfl"])
+        assert gpu.last_substeps() == k
+        ks.append(k)
+        _state_equal(gpu, sim, f"capped step {s + 1}")
+        np.testing.assert_array_equal(gpu.read_field("pressure"), sim.pressure)
+        np.testing.assert_array_equal(gpu.read_field("vel"), sim.vel)
+    assert max(ks) >= 3 and gpu.steps_done == 3
+    gpu.close()
+
+
 def test_bins_bitwise_teacher_forced(pxl, orc):
     """Binning of identical positions: cell of every particle, counts, offsets and the
     per-cell id lists equal the oracle's counting sort exactly, at seeding and after moving

@@ -126,6 +126,9 @@ def test_cancel_and_invalid_request(pxl):
         with pytest.raises(pxl.FluidError) as e:
             srv.submit({"name": "bad", "resolution": 8}, FPS, steps=1)
         assert "missing field `bounds`" in str(e.value)
+        with pytest.raises(pxl.FluidError) as e:
+            srv.submit(dict(dam_break(16), cfl=0.5), FPS, steps=1)
+        assert "not supported in batched execution" in str(e.value)
         assert srv.stats()["cancelled"] == 1
 
 

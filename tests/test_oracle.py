@@ -605,6 +605,52 @@ def test_free_fall_flip1(orc):
         assert np.abs(sim.pressure).max() < 2e-3 * 9.81 * 0.2
 
 
+def test_cfl_substeps_closed_form(orc):
+    """CFL cap (an extension): k = ceil(|v|max dt / (cfl h)), 1 at or below one cell, at
+    most 64; |v|max is the largest particle speed."""
+    x = np.full((3, 3), 0.5, np.float32)
+    v = np.array([[3, 4, 0], [1, 1, 1], [0, -2, 0]], np.float32)  # |v|max = 5
+    s = orc.Sim((32, 32, 32), np.float32(1 / 32), [0, 0, 0], np.float32(1 / 120), 0.95, [0, -9.81, 0], x, v)
+    assert s.cfl_substeps(1.0) == 2  # 5 * (1/120) * 32 = 1.33 cells
+    assert s.cfl_substeps(0.5) == 3  # 2.67
+    assert s.cfl_substeps(2.0) == 1  # 0.67
+    assert s.cfl_substeps(1e-3) == 64
+    s.vx[:] = 0
+    s.vy[:] = 0
+    s.vz[:] = 0
+    assert s.cfl_substeps(1e-3) == 1
+
+
+def test_cfl_capped_free_fall(orc):
+    """A blob thrown down fast enough to need 3 substeps: the velocity change is -g dt in
+    total (k steps of dt / k); inside the blob the displacement is the free-fall one, and at
+    the blob's leading face, where one long RK2 step samples the air below, the substeps
+    stay closer to it than the uncapped step; a step slow enough for k = 1 is bitwise the
+    reference step."""
+    f = {"name": "blob", "bounds": [[0, 0, 0], [1, 1, 1]], "resolution": 24,
+         "fill": [{"min": [0.35, 0.5, 0.35], "max": [0.65, 0.7, 0.65], "velocity": [0, -6.0, 0]}],
+         "flip": 1.0, "substeps": 5, "particles_per_cell": 4}
+    a, b = oracle_sim(orc, f), oracle_sim(orc, f)
+    dt = float(orc.dt_for(FPS, 5))
+    y0 = a.py.astype(np.float64).copy()
+    assert a.cfl_substeps(1.0) == 2 and a.cfl_substeps(0.45) == 3  # 6 m/s * (1/120) s * 24 / m = 1.2 cells
+    k = a.step_rbgs_capped(80, 0.45)
+    b.step_rbgs(80)
+    assert k == 3
+    np.testing.assert_allclose(a.vy, -6.0 - 9.81 * dt, rtol=2e-3)
+    # inside the blob each (sub)step moves by its length times the post-gravity velocity:
+    # k substeps give -6 dt - g dt^2 (k + 1) / (2 k), one step -6 dt - g dt^2
+    da = a.py - y0 - (-6.0 * dt - 9.81 * dt * dt * 4 / 6)
+    db = b.py - y0 - (-6.0 * dt - 9.81 * dt * dt)
+    assert np.median(np.abs(da)) < 1e-5 and np.median(np.abs(db)) < 1e-5  # f32, fixed-point P2G
+    assert np.abs(da).max() < 0.5 * np.abs(db).max()
+    c, d = oracle_sim(orc, f), oracle_sim(orc, f)
+    assert c.step_rbgs_capped(80, 100.0) == 1
+    d.step_rbgs(80)
+    np.testing.assert_array_equal(c.x, d.x)
+    np.testing.assert_array_equal(c.v, d.v)
+
+
 def test_whole_step_hydrostatic_rest(orc):
     """A resting slab stays at rest through full steps (pressure balances gravity)."""
     f = dam_break(16, fill_max=(1.0, 0.3, 1.0))
