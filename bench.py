@@ -31,7 +31,8 @@ algorithmic bytes over the simulations each covers).  `cpu_baseline` times the C
 `--workload mixed256` the 256^3 two-block mixed-precision configuration (~58M particles);
 `--workload obstacles` the 192x96x192 obstacle-heavy configuration (solid spheres, 19.4M
 particles, 200 steps, mixed-precision solve); `--workload frame` 128^3 dam-break frames of 5
-steps plus one surface-nets pass.
+steps plus one surface-nets pass; `--workload reference64` the reference's own 64^3 dam break
+in reference mode (RBGS 80) with the CFL cap, beside the CPU oracle on the same configuration.
 """
 from __future__ import annotations
 
@@ -193,6 +194,50 @@ def bench_frame(args, pxl):
                      "traffic": measured_traffic(SURF_KERNELS.get(dom["kernel"], dom["kernel"]), "frame128"),
                      "kernel": dom["kernel"], "avg_us": dom["avg_us"], "alg_bytes_per_launch": dom["alg_bytes"]},
         "cpu_baseline": None, "kernels": stages,
+    }
+
+
+def bench_reference64(args, pxl):
+    """The reference's own configuration: the 64^3 dam break (458,800 particles) in reference
+    mode (red-black Gauss-Seidel, 80 iterations per step, bit-exact against the oracle) with
+    the CFL cap at one cell per substep, --steps (100) steps; the CPU baseline is the oracle
+    on the same configuration (first steps, about --cpu-seconds)."""
+    from tests.scenes import FPS, dam_break
+    fluid = dict(dam_break(64, iters=80), cfl=1.0)
+    sim = pxl.FluidState(fluid, FPS, seed=0)
+    P = sim.particle_count()
+    ks = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sim.step(1)
+        ks.append(sim.last_substeps())
+    elapsed = time.perf_counter() - t0
+    prof = sim.profile_steps(2)
+    sim.close()
+    cpu = None
+    if args.cpu_seconds > 0:
+        import oracle
+        from tests.scenes import oracle_sim
+        osim = oracle_sim(oracle, fluid)
+        n, spent = 0, 0.0
+        while n == 0 or (spent < args.cpu_seconds and n < args.steps):
+            a = time.perf_counter()
+            osim.step_rbgs_capped(80, fluid["cfl"])
+            spent += time.perf_counter() - a
+            n += 1
+        cpu = {"value": P * n / spent, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+               "sample": f"first {n} capped RBGS-80 steps of the same 64^3 dam break, C oracle, one thread"}
+    return {
+        "metric": METRIC, "value": P * args.steps / elapsed, "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": 0, "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded dam break, reference LCG seeding)",
+        "config": {"workload": f"dam break 64^3, single instance, reference mode (RBGS 80), CFL cap 1 cell per "
+                               f"substep, {args.steps} steps from seeding",
+                   "particles": P, "substeps_histogram": {str(k): ks.count(k) for k in sorted(set(ks))}},
+        "roofline": None, "cpu_baseline": cpu,
+        "phases": [{"phase": name, "launches_per_step": launches // 2, "ms_per_step": total_ms / 2}
+                   for name, launches, total_ms in prof],
     }
 
 
@@ -438,7 +483,8 @@ def main():
                     help="calibration batch: the stream's first N requests, whose service rate sets the arrival "
                          "rate (dam: warm-up steps)")
     ap.add_argument("--calib-steps", type=int, default=10, help="timed steps of the warm-up batch")
-    ap.add_argument("--workload", default="serve", choices=["serve", "dam", "batch", "mixed256", "obstacles", "frame"])
+    ap.add_argument("--workload", default="serve",
+                    choices=["serve", "dam", "batch", "mixed256", "obstacles", "frame", "reference64"])
     ap.add_argument("--requests", type=int, default=512)
     ap.add_argument("--sizes", default="32,48,64,96,128")
     ap.add_argument("--rate", type=float, default=0.0, help="arrivals per second (0 = measured service rate)")
@@ -451,7 +497,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16, help="serve: host threads of the CPU oracle")
     args = ap.parse_args()
     if args.steps is None:
-        args.steps = 200 if args.workload == "obstacles" else 60
+        args.steps = {"obstacles": 200, "reference64": 100}.get(args.workload, 60)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: every box here has one MI355X and this build has no "
@@ -461,7 +507,7 @@ def main():
 
     import pxlengine_amd as pxl
     run = {"serve": bench_serve, "dam": bench_dam, "batch": bench_batch, "mixed256": bench_mixed256,
-           "obstacles": bench_obstacles, "frame": bench_frame}[args.workload]
+           "obstacles": bench_obstacles, "frame": bench_frame, "reference64": bench_reference64}[args.workload]
     line = run(args, pxl)
     print(json.dumps(line))
 
