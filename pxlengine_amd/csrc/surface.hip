@@ -77,7 +77,13 @@ __global__ __launch_bounds__(SBLK) void k_s_splat_lds(DevSurf S, const float *px
                                                       uint32_t np) {
     __shared__ int32_t box[SPLAT_LDS];
     __shared__ int red[6][SBLK / WAVE];
-    const uint32_t base = blockIdx.x * SPLAT_CHUNK;
+    // Workgroups are dispatched round-robin over the 8 XCDs (block b on XCD b % 8); the grid is
+    // a multiple of 8 and XCD x takes the x-th eighth of the chunks, so the chunks whose boxes
+    // overlap (neighbours in particle order) add into the same L2.
+    const uint32_t per_xcd = gridDim.x / 8u;
+    const uint32_t chunk = (blockIdx.x % 8u) * per_xcd + blockIdx.x / 8u;
+    const uint32_t base = chunk * SPLAT_CHUNK;
+    if (base >= np) return;  // the whole workgroup: padding of the grid
     const float r = S.radius / S.sh;
     const float rr = r * r;
     // the chunk's node box (particles whose own box is empty do not count)
@@ -348,7 +354,8 @@ void launch_surface_stage(int stage, const DevSurf &S, const float *px, const fl
     switch (stage) {
         case SURF_CLEAR: launch_fill_u32(reinterpret_cast<uint32_t *>(S.density), S.nnodes, 0u, s); break;
         case SURF_SPLAT:
-            if (np) PXL_LAUNCH(k_s_splat_lds, (np + SPLAT_CHUNK - 1) / SPLAT_CHUNK, SBLK, 0, s, S, px, py, pz, np);
+            if (np) PXL_LAUNCH(k_s_splat_lds, (np + 8u * SPLAT_CHUNK - 1) / (8u * SPLAT_CHUNK) * 8u, SBLK, 0, s, S, px, py, pz,
+                               np);
             break;
         case SURF_PHI: PXL_LAUNCH(k_s_phi, grid_for(S.nnodes), SBLK, 0, s, S); break;
         case SURF_VFLAG: PXL_LAUNCH(k_s_vflag, grid_for(S.ncells), SBLK, 0, s, S); break;
