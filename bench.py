@@ -32,7 +32,8 @@ algorithmic bytes over the simulations each covers).  `cpu_baseline` times the C
 `--workload obstacles` the 192x96x192 obstacle-heavy configuration (solid spheres, 19.4M
 particles, 200 steps, mixed-precision solve); `--workload frame` 128^3 dam-break frames of 5
 steps plus one surface-nets pass; `--workload reference64` the reference's own 64^3 dam break
-in reference mode (RBGS 80) with the CFL cap, beside the CPU oracle on the same configuration.
+in reference mode (RBGS 80) with the CFL cap, beside the CPU oracle on the same configuration;
+`--workload agents` the agent neighbour grid of a 1M-agent flock.
 """
 from __future__ import annotations
 
@@ -238,6 +239,59 @@ def bench_reference64(args, pxl):
         "roofline": None, "cpu_baseline": cpu,
         "phases": [{"phase": name, "launches_per_step": launches // 2, "ms_per_step": total_ms / 2}
                    for name, launches, total_ms in prof],
+    }
+
+
+def flock_agents(n: int, seed: int = 0):
+    """n agents (64-byte records) in 16 Gaussian clumps of random size and spread, like the
+    reference's 1M-starling flock example (examples/programs/flock.slang)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    centres = rng.uniform(-120.0, 120.0, (16, 3))
+    spreads = rng.uniform(4.0, 25.0, 16)
+    which = rng.integers(0, 16, n)
+    a = np.zeros((n, 16), np.float32)
+    a[:, 0:3] = (centres[which] + rng.normal(0.0, 1.0, (n, 3)) * spreads[which, None]).astype(np.float32)
+    a[:, 4:7] = rng.normal(0.0, 3.0, (n, 3)).astype(np.float32)
+    return a
+
+
+def bench_agents(args, pxl):
+    """The agent neighbour grid (shaders/sim.slang:27-104) of the reference's flock example:
+    1M agents, cell 1.6, table 2^21, built --steps times from device-resident agents; value =
+    agents binned per second."""
+    import numpy as np
+    n, cell = 1_000_000, 1.6
+    a = flock_agents(n)
+    t = pxl.AgentGrid.table_size(n)
+    g = pxl.AgentGrid(n)
+    bufs = [pxl.DeviceBuffer(b) for b in (a.nbytes, 4 * t, 4 * t, 4 * n)]
+    bufs[0].upload(a)
+    for _ in range(max(args.warmup, 1)):
+        g.build_device(bufs[0].ptr, n, cell, t, bufs[1].ptr, bufs[2].ptr, bufs[3].ptr)
+    ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ms.append(g.build_device(bufs[0].ptr, n, cell, t, bufs[1].ptr, bufs[2].ptr, bufs[3].ptr))
+    elapsed = time.perf_counter() - t0
+    cnt = bufs[2].download(np.zeros(t, np.uint32))
+    for b in bufs:
+        b.free()
+    g.close()
+    build_ms = float(np.mean(ms))
+    alg = 12 * n + 4 * n + 8 * t  # positions read, ids written, counts and offsets written
+    ach = alg / (build_ms / 1e3) / 1e9
+    return {
+        "metric": "agent neighbour-grid builds", "value": n * args.steps / elapsed, "unit": "agents/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+        "data": "synthetic (1M agents in 16 Gaussian clumps)",
+        "config": {"workload": f"agent neighbour grid: {n} agents, cell {cell}, table {t}, device-resident",
+                   "build_ms_events": build_ms, "occupied_cells": int(np.count_nonzero(cnt)), "max_per_cell": int(cnt.max())},
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach / PEAK_HBM_GBS,
+                     "traffic": None, "kernel": "grid build (all launches)", "avg_us": build_ms * 1e3,
+                     "alg_bytes_per_launch": alg},
+        "cpu_baseline": None,
     }
 
 
@@ -484,7 +538,7 @@ def main():
                          "rate (dam: warm-up steps)")
     ap.add_argument("--calib-steps", type=int, default=10, help="timed steps of the warm-up batch")
     ap.add_argument("--workload", default="serve",
-                    choices=["serve", "dam", "batch", "mixed256", "obstacles", "frame", "reference64"])
+                    choices=["serve", "dam", "batch", "mixed256", "obstacles", "frame", "reference64", "agents"])
     ap.add_argument("--requests", type=int, default=512)
     ap.add_argument("--sizes", default="32,48,64,96,128")
     ap.add_argument("--rate", type=float, default=0.0, help="arrivals per second (0 = measured service rate)")
@@ -497,7 +551,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16, help="serve: host threads of the CPU oracle")
     args = ap.parse_args()
     if args.steps is None:
-        args.steps = {"obstacles": 200, "reference64": 100}.get(args.workload, 60)
+        args.steps = {"obstacles": 200, "reference64": 100, "agents": 100}.get(args.workload, 60)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: every box here has one MI355X and this build has no "
@@ -507,7 +561,8 @@ def main():
 
     import pxlengine_amd as pxl
     run = {"serve": bench_serve, "dam": bench_dam, "batch": bench_batch, "mixed256": bench_mixed256,
-           "obstacles": bench_obstacles, "frame": bench_frame, "reference64": bench_reference64}[args.workload]
+           "obstacles": bench_obstacles, "frame": bench_frame, "reference64": bench_reference64,
+           "agents": bench_agents}[args.workload]
     line = run(args, pxl)
     print(json.dumps(line))
 

@@ -335,6 +335,32 @@ int pxlflip_batch_last_solve(pxlflip_batch *b, uint32_t index, pxlflip_solve_sta
 const char *pxlflip_batch_last_error(const pxlflip_batch *b);
 void pxlflip_batch_destroy(pxlflip_batch *b);
 
+/* ---- Agent neighbour grid (shaders/sim.slang:27-104, driven per step by
+ * crates/pxl-render/src/sim.rs:224-291): agents grouped by the hash of their grid cell ---- */
+typedef struct pxlgrid pxlgrid;
+/* The table size of a simulation with `count` agents (sim.rs:129): the next power of two of
+ * 2 count, clamped to [1024, 2^23]. */
+uint32_t pxlgrid_table_size(uint32_t count);
+/* Scratch for grids of up to max_agents agents on `device`. */
+int pxlgrid_create(uint32_t max_agents, int device, pxlgrid **out);
+/* grid_clear -> grid_count -> scan -> grid_scatter (sim.slang:31-104) on `count` agents in
+ * DEVICE memory, 64-byte records with the position in floats 0-2 (struct Agent, sim.slang:8;
+ * 16-byte aligned): cell = int3(floor(position / max(cell_size, 1e-6))) (sim.rs:182), hash =
+ * (u32(x) 73856093 ^ u32(y) 19349663 ^ u32(z) 83492791) & (table_size - 1); writes
+ * counts[table_size], offsets[table_size] (exclusive scan: each hash's first slot) and
+ * sorted[count] (agent ids grouped by hash; ascending within a hash, where the reference's
+ * order follows its atomic cursor), all DEVICE pointers.  table_size: a power of two <= 2^23
+ * (normally pxlgrid_table_size(count)).  Returns when done. */
+int pxlgrid_build(pxlgrid *g, const float *agents, uint32_t count, float cell_size, uint32_t table_size,
+                  uint32_t *offsets, uint32_t *counts, uint32_t *sorted);
+/* The same with host arrays (copied through device scratch); outputs may be NULL. */
+int pxlgrid_build_host(pxlgrid *g, const float *agents, uint32_t count, float cell_size, uint32_t table_size,
+                       uint32_t *offsets, uint32_t *counts, uint32_t *sorted);
+/* HIP-event time of the last build's kernels, in ms. */
+double pxlgrid_last_ms(const pxlgrid *g);
+const char *pxlgrid_last_error(const pxlgrid *g);
+void pxlgrid_destroy(pxlgrid *g);
+
 #ifdef __cplusplus
 }
 #endif

@@ -108,6 +108,11 @@ def lib():
         L.orc_poisson_relres64.restype = C.c_double
         L.orc_poisson_solve64.argtypes = [SP, dp, C.c_double, C.c_int, dp]
         L.orc_poisson_solve64.restype = C.c_int
+        L.orc_grid_table_size.argtypes = [C.c_uint32]
+        L.orc_grid_table_size.restype = C.c_uint32
+        L.orc_cell_hash.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_uint32]
+        L.orc_cell_hash.restype = C.c_uint32
+        L.orc_grid_build.argtypes = [f32p, C.c_uint32, C.c_float, C.c_uint32, u32p, u32p, u32p]
         L.orc_cfl_substeps.argtypes = [SP, C.c_float, C.c_float]
         L.orc_cfl_substeps.restype = C.c_uint32
         SPP = C.POINTER(SurfaceParams)
@@ -233,6 +238,26 @@ def surface_mesh(sp, phi):
     nv = min(int(ctr[0]), sp.max_verts)
     nt = min(int(ctr[1]), sp.max_tris) // 2 * 2
     return cv, verts[:nv], idx[:nt], (int(ctr[0]), int(ctr[1]))
+
+
+def grid_table_size(count):
+    return int(lib().orc_grid_table_size(count))
+
+
+def cell_hash(cx, cy, cz, table_size):
+    return int(lib().orc_cell_hash(cx, cy, cz, table_size))
+
+
+def grid_build(agents, cell_size, table_size=None):
+    """Agent neighbour grid of agents[N, 16] (64-byte records) -> (offsets, counts, sorted)."""
+    a = np.ascontiguousarray(agents, np.float32)
+    n = len(a)
+    t = grid_table_size(n) if table_size is None else table_size
+    off = np.zeros(t, np.uint32)
+    cnt = np.zeros(t, np.uint32)
+    srt = np.zeros(max(n, 1), np.uint32)
+    lib().orc_grid_build(_fp(a), n, np.float32(cell_size), t, _up(off), _up(cnt), _up(srt))
+    return off, cnt, srt[:n]
 
 
 def steps_for_frame(warmup, frame, fps, substeps):

@@ -574,6 +574,45 @@ void orc_step_cg(orc_state *s, uint32_t iters) {
     orc_g2p_advect(s);
 }
 
+/* ------------------------------------------------------------------ agent neighbour grid */
+
+uint32_t orc_grid_table_size(uint32_t count) {
+    uint64_t want = (uint64_t)count * 2, t = 1;
+    while (t < want) t <<= 1;
+    if (t < 1024) t = 1024;
+    if (t > (1u << 23)) t = 1u << 23;
+    return (uint32_t)t;
+}
+
+uint32_t orc_cell_hash(int32_t cx, int32_t cy, int32_t cz, uint32_t table_size) {
+    return (((uint32_t)cx * 73856093u) ^ ((uint32_t)cy * 19349663u) ^ ((uint32_t)cz * 83492791u)) &
+           (table_size - 1u);
+}
+
+void orc_grid_build(const float *agents, uint32_t count, float cell_size, uint32_t table_size,
+                    uint32_t *offsets, uint32_t *counts, uint32_t *sorted) {
+    const float cs = cell_size > 1e-6f ? cell_size : 1e-6f; /* f32::max(1e-6) */
+    uint32_t *key = (uint32_t *)malloc((size_t)(count ? count : 1) * 4);
+    memset(counts, 0, (size_t)table_size * 4);
+    for (uint32_t i = 0; i < count; i++) {
+        const float *a = agents + 16 * (uint64_t)i;
+        const int32_t cx = (int32_t)floorf(a[0] / cs), cy = (int32_t)floorf(a[1] / cs), cz = (int32_t)floorf(a[2] / cs);
+        key[i] = orc_cell_hash(cx, cy, cz, table_size);
+        counts[key[i]]++;
+    }
+    uint32_t run = 0;
+    for (uint32_t h = 0; h < table_size; h++) {
+        offsets[h] = run;
+        run += counts[h];
+    }
+    /* stable scatter: ids in increasing order fill their cell's slots in order */
+    uint32_t *cursor = (uint32_t *)malloc((size_t)table_size * 4);
+    memcpy(cursor, offsets, (size_t)table_size * 4);
+    for (uint32_t i = 0; i < count; i++) sorted[cursor[key[i]]++] = i;
+    free(cursor);
+    free(key);
+}
+
 /* ------------------------------------------------------------------ CFL cap (extension) */
 
 uint32_t orc_cfl_substeps(const orc_state *s, float dt, float cfl) {

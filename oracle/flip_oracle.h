@@ -129,6 +129,19 @@ void orc_surface_mesh(const orc_surface *sp, const float *phi, uint32_t *cell_ve
  * steps of dt / k (f32). */
 uint32_t orc_cfl_substeps(const orc_state *s, float dt, float cfl);
 
+/* ---- agent neighbour grid (shaders/sim.slang:27-104; crates/pxl-render/src/sim.rs:129,
+ * 178-186).  The reference scatters with an atomic cursor (order within a cell follows thread
+ * timing); this restatement is the stable counting sort (ids ascending within a cell). ---- */
+/* sim.rs:129: next power of two of 2 count, clamped to [1024, 2^23]. */
+uint32_t orc_grid_table_size(uint32_t count);
+/* cell_hash (sim.slang:28-30): (u32(x) * 73856093 ^ u32(y) * 19349663 ^ u32(z) * 83492791) & (table - 1). */
+uint32_t orc_cell_hash(int32_t cx, int32_t cy, int32_t cz, uint32_t table_size);
+/* grid_clear .. grid_scatter for `count` 64-byte agent records (16 floats, position at 0-2):
+ * cell = int3(floor(position / cell)), cell = max(cell_size, 1e-6) (sim.rs:182); counts and
+ * offsets (exclusive scan) per hash, sorted = agent ids grouped by hash. */
+void orc_grid_build(const float *agents, uint32_t count, float cell_size, uint32_t table_size,
+                    uint32_t *offsets, uint32_t *counts, uint32_t *sorted);
+
 /* ---- binning: canonical counting sort (ids ascending inside each cell) ---- */
 void orc_bin(const orc_state *s, uint32_t *cell_of, uint32_t *counts, uint32_t *offsets,
              uint32_t *sorted_ids);

@@ -134,7 +134,6 @@ class DeviceBuffer:
 
     def __init__(self, nbytes: int):
         if DeviceBuffer._hip is None:
-            import os
             path = "/opt/rocm/lib/libamdhip64.so"
             h = C.CDLL(path if os.path.exists(path) else "libamdhip64.so")
             h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
@@ -153,12 +152,12 @@ class DeviceBuffer:
     def upload(self, host: np.ndarray):
         assert host.nbytes <= self.nbytes
         if DeviceBuffer._hip.hipMemcpy(self._p, host.ctypes.data, host.nbytes, 1) != 0:
-            raise FluidError("hipMemcpy to the device failed")
+            raise FluidError(ERR_DEVICE, "hipMemcpy to the device failed")
 
     def download(self, out: np.ndarray) -> np.ndarray:
         assert out.nbytes <= self.nbytes
         if DeviceBuffer._hip.hipMemcpy(out.ctypes.data, self._p, out.nbytes, 2) != 0:
-            raise FluidError("hipMemcpy from the device failed")
+            raise FluidError(ERR_DEVICE, "hipMemcpy from the device failed")
         return out
 
     def free(self):
@@ -169,6 +168,59 @@ class DeviceBuffer:
     def __del__(self):
         try:
             self.free()
+        except Exception:
+            pass
+
+
+class AgentGrid:
+    """Neighbour grid of an agent simulation (shaders/sim.slang:27-104): agents grouped by the
+    hash of their grid cell."""
+
+    def __init__(self, max_agents: int, device: int = 0):
+        self._lib = load_library()
+        self._h = C.c_void_p()
+        rc = self._lib.pxlgrid_create(int(max_agents), int(device), C.byref(self._h))
+        if rc != 0:
+            raise FluidError(rc, self._lib.pxlgrid_last_error(None).decode())
+        self.max_agents = int(max_agents)
+
+    @staticmethod
+    def table_size(count: int) -> int:
+        return int(load_library().pxlgrid_table_size(int(count)))
+
+    def _check(self, rc):
+        if rc != 0:
+            raise FluidError(rc, self._lib.pxlgrid_last_error(self._h).decode())
+
+    def build(self, agents: np.ndarray, cell_size: float, table_size: int = None):
+        """agents[N, 16] float32 (64-byte records) -> (offsets, counts, sorted)."""
+        a = np.ascontiguousarray(agents, np.float32)
+        assert a.ndim == 2 and a.shape[1] == 16
+        n = len(a)
+        t = self.table_size(n) if table_size is None else int(table_size)
+        off = np.zeros(t, np.uint32)
+        cnt = np.zeros(t, np.uint32)
+        srt = np.zeros(max(n, 1), np.uint32)
+        self._check(self._lib.pxlgrid_build_host(self._h, a.ctypes.data, n, float(np.float32(cell_size)), t,
+                                                 off.ctypes.data, cnt.ctypes.data, srt.ctypes.data))
+        return off, cnt, srt[:n]
+
+    def build_device(self, agents_ptr: int, count: int, cell_size: float, table_size: int, offsets_ptr: int,
+                     counts_ptr: int, sorted_ptr: int) -> float:
+        """The same on device buffers; returns the kernels' time in ms."""
+        self._check(self._lib.pxlgrid_build(self._h, C.c_void_p(agents_ptr), int(count), float(np.float32(cell_size)),
+                                            int(table_size), C.c_void_p(offsets_ptr), C.c_void_p(counts_ptr),
+                                            C.c_void_p(sorted_ptr)))
+        return float(self._lib.pxlgrid_last_ms(self._h))
+
+    def close(self):
+        if self._h:
+            self._lib.pxlgrid_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass
 
@@ -199,6 +251,14 @@ _SIGNATURES = {
     "pxlflip_surface": (C.c_int, [_H, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.POINTER(SurfaceInfo)]),
     "pxlflip_surface_host": (C.c_int, [_H, C.c_void_p, C.c_void_p, C.POINTER(SurfaceInfo)]),
     "pxlflip_last_substeps": (C.c_int, [_H, C.POINTER(C.c_uint32)]),
+    "pxlgrid_table_size": (C.c_uint32, [C.c_uint32]),
+    "pxlgrid_create": (C.c_int, [C.c_uint32, C.c_int, C.POINTER(_H)]),
+    "pxlgrid_build": (C.c_int, [_H, C.c_void_p, C.c_uint32, C.c_float, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pxlgrid_build_host": (C.c_int, [_H, C.c_void_p, C.c_uint32, C.c_float, C.c_uint32, C.c_void_p, C.c_void_p,
+                                     C.c_void_p]),
+    "pxlgrid_last_ms": (C.c_double, [_H]),
+    "pxlgrid_last_error": (C.c_char_p, [_H]),
+    "pxlgrid_destroy": (None, [_H]),
     "pxlflip_advance_to_frame": (C.c_int, [_H, C.c_double]),
     "pxlflip_step": (C.c_int, [_H, C.c_uint32]),
     "pxlflip_reset": (C.c_int, [_H]),

@@ -23,7 +23,7 @@ def pxl():
 def _declared_symbols():
     text = open(os.path.join(ROOT, "include", "pxlflip.h")).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(pxlflip_[a-z0-9_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b((?:pxlflip|pxlgrid)_[a-z0-9_]+)\s*\(", text)))
 
 
 def test_library_exports_every_declared_symbol(pxl):
