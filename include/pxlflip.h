@@ -181,6 +181,16 @@ typedef struct pxlflip_surface_info {
  * NULL. */
 int pxlflip_surface(pxlflip *h, float *verts, uint32_t vbase, uint32_t *indices, uint32_t ibase,
                     pxlflip_surface_info *info);
+/* Vulkan <-> HIP interop for the surface buffers (the renderer's vertex / index buffers live in
+ * Vulkan, fluid.rs:257-258): imports a buffer exported as an opaque file descriptor
+ * (VK_KHR_external_memory_fd, vkGetMemoryFdKHR) of `bytes` bytes on `device` and maps it as
+ * device memory; *dev_ptr can be passed to pxlflip_surface as verts or indices.  On success the
+ * descriptor belongs to HIP.  No reference counterpart (the reference meshes inside Vulkan). */
+typedef struct pxlflip_external pxlflip_external;
+int pxlflip_import_buffer_fd(int fd, uint64_t bytes, int device, void **dev_ptr, pxlflip_external **out);
+/* Unmaps and releases an imported buffer. */
+void pxlflip_release_buffer(pxlflip_external *ext);
+
 /* The same mesh copied to host arrays (capacity as above; only the written counts are copied).
  * Either array may be NULL. */
 int pxlflip_surface_host(pxlflip *h, float *verts, uint32_t *indices, pxlflip_surface_info *info);

@@ -7,6 +7,7 @@
 //   -> cells (classify+stencil+divergence) -> pressure -> project -> g2p
 // and, for the RBGS and CG solvers, captured once into a hipGraph that each step replays.
 #include <cstring>
+#include <new>
 
 #include "sim.hpp"
 
@@ -22,6 +23,13 @@ using pxl::Sim;
 
 struct pxlflip {
     Sim sim;
+};
+
+// A buffer of another API imported through an exported file descriptor.
+struct pxlflip_external {
+    hipExternalMemory_t mem;
+    void *ptr;
+    int device;
 };
 
 extern "C" {
@@ -432,6 +440,57 @@ void fill_surface_info(const pxl::Sim &s, const uint32_t ctr[2], pxlflip_surface
     info->triangles = std::min(ctr[1], sf.max_tris) / 2u * 2u;
 }
 }  // namespace
+
+int pxlflip_import_buffer_fd(int fd, uint64_t bytes, int device, void **dev_ptr, pxlflip_external **out) {
+    pxl::g_create_error.clear();
+    if (!dev_ptr || !out || fd < 0 || bytes == 0) {
+        pxl::g_create_error = "import_buffer_fd needs a file descriptor, a size and output pointers";
+        return PXLFLIP_ERR_INVALID;
+    }
+    *dev_ptr = nullptr;
+    *out = nullptr;
+    auto fail_hip = [](const char *what, hipError_t e) {
+        pxl::g_create_error = std::string(what) + ": " + hipGetErrorString(e);
+        return PXLFLIP_ERR_DEVICE;
+    };
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) return fail_hip("hipSetDevice", e);
+    hipExternalMemoryHandleDesc desc;
+    std::memset(&desc, 0, sizeof desc);
+    desc.type = hipExternalMemoryHandleTypeOpaqueFd;
+    desc.handle.fd = fd;
+    desc.size = bytes;
+    hipExternalMemory_t mem = nullptr;
+    e = hipImportExternalMemory(&mem, &desc);
+    if (e != hipSuccess) return fail_hip("hipImportExternalMemory", e);
+    hipExternalMemoryBufferDesc bd;
+    std::memset(&bd, 0, sizeof bd);
+    bd.offset = 0;
+    bd.size = bytes;
+    void *p = nullptr;
+    e = hipExternalMemoryGetMappedBuffer(&p, mem, &bd);
+    if (e != hipSuccess) {
+        (void)hipDestroyExternalMemory(mem);
+        return fail_hip("hipExternalMemoryGetMappedBuffer", e);
+    }
+    auto *x = new (std::nothrow) pxlflip_external{mem, p, device};
+    if (!x) {
+        (void)hipFree(p);
+        (void)hipDestroyExternalMemory(mem);
+        return PXLFLIP_ERR_OOM;
+    }
+    *dev_ptr = p;
+    *out = x;
+    return PXLFLIP_OK;
+}
+
+void pxlflip_release_buffer(pxlflip_external *x) {
+    if (!x) return;
+    (void)hipSetDevice(x->device);
+    (void)hipFree(x->ptr);
+    (void)hipDestroyExternalMemory(x->mem);
+    delete x;
+}
 
 int pxlflip_last_substeps(const pxlflip *h, uint32_t *k) {
     if (!h || !k) return PXLFLIP_ERR_INVALID;

@@ -172,6 +172,32 @@ class DeviceBuffer:
             pass
 
 
+class ImportedBuffer:
+    """A buffer of another API (e.g. a Vulkan vertex buffer) imported from an exported file
+    descriptor; .ptr is device memory for FluidState.surface_device."""
+
+    def __init__(self, fd: int, nbytes: int, device: int = 0):
+        self._lib = load_library()
+        self._h = C.c_void_p()
+        p = C.c_void_p()
+        rc = self._lib.pxlflip_import_buffer_fd(int(fd), int(nbytes), int(device), C.byref(p), C.byref(self._h))
+        if rc != 0:
+            raise FluidError(rc, self._lib.pxlflip_last_error(None).decode())
+        self.ptr = int(p.value)
+        self.nbytes = int(nbytes)
+
+    def release(self):
+        if self._h:
+            self._lib.pxlflip_release_buffer(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
 class AgentGrid:
     """Neighbour grid of an agent simulation (shaders/sim.slang:27-104): agents grouped by the
     hash of their grid cell."""
@@ -251,6 +277,8 @@ _SIGNATURES = {
     "pxlflip_surface": (C.c_int, [_H, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.POINTER(SurfaceInfo)]),
     "pxlflip_surface_host": (C.c_int, [_H, C.c_void_p, C.c_void_p, C.POINTER(SurfaceInfo)]),
     "pxlflip_last_substeps": (C.c_int, [_H, C.POINTER(C.c_uint32)]),
+    "pxlflip_import_buffer_fd": (C.c_int, [C.c_int, C.c_uint64, C.c_int, C.POINTER(C.c_void_p), C.POINTER(_H)]),
+    "pxlflip_release_buffer": (None, [_H]),
     "pxlgrid_table_size": (C.c_uint32, [C.c_uint32]),
     "pxlgrid_create": (C.c_int, [C.c_uint32, C.c_int, C.POINTER(_H)]),
     "pxlgrid_build": (C.c_int, [_H, C.c_void_p, C.c_uint32, C.c_float, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),

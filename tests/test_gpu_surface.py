@@ -116,3 +116,106 @@ def test_surface_full_size_dam(pxl, orc):
     verts, tris, info = _check(orc, gpu, fluid, "128^3")
     assert info["vertices"] > 10_000
     gpu.close()
+
+
+class _Exporter:
+    """A device allocation made exportable as a POSIX file descriptor through HIP's virtual
+    memory API -- standing in for a Vulkan buffer exported with VK_KHR_external_memory_fd --
+    mapped here too so the test can read what the importer wrote."""
+
+    def __init__(self, nbytes):
+        import ctypes as C
+        import os
+        lib = C.CDLL("/opt/rocm/lib/libamdhip64.so" if os.path.exists("/opt/rocm/lib/libamdhip64.so")
+                     else "libamdhip64.so")
+
+        class Loc(C.Structure):
+            _fields_ = [("type", C.c_int), ("id", C.c_int)]
+
+        class Flags(C.Structure):
+            _fields_ = [("compressionType", C.c_ubyte), ("gpuDirectRDMACapable", C.c_ubyte), ("usage", C.c_ushort)]
+
+        class Prop(C.Structure):
+            _fields_ = [("type", C.c_int), ("requestedHandleType", C.c_int), ("location", Loc),
+                        ("win32HandleMetaData", C.c_void_p), ("allocFlags", Flags)]
+
+        class Access(C.Structure):
+            _fields_ = [("location", Loc), ("flags", C.c_int)]
+
+        self.C, self.lib = C, lib
+        supported = C.c_int(0)
+        # hipDeviceAttributeVirtualMemoryManagementSupported
+        lib.hipDeviceGetAttribute.argtypes = [C.POINTER(C.c_int), C.c_int, C.c_int]
+        prop = Prop(type=1, requestedHandleType=1, location=Loc(type=1, id=0))  # pinned, POSIX fd, device 0
+        gran = C.c_size_t(0)
+        lib.hipMemGetAllocationGranularity.argtypes = [C.POINTER(C.c_size_t), C.POINTER(Prop), C.c_int]
+        if lib.hipMemGetAllocationGranularity(C.byref(gran), C.byref(prop), 0) != 0 or gran.value == 0:
+            pytest.skip("HIP virtual memory management is not available here")
+        self.size = (nbytes + gran.value - 1) // gran.value * gran.value
+        self.handle = C.c_uint64(0)
+        lib.hipMemCreate.argtypes = [C.POINTER(C.c_uint64), C.c_size_t, C.POINTER(Prop), C.c_ulonglong]
+        if lib.hipMemCreate(C.byref(self.handle), self.size, C.byref(prop), 0) != 0:
+            pytest.skip("hipMemCreate with an exportable handle is not available here")
+        fd = C.c_int(-1)
+        lib.hipMemExportToShareableHandle.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_ulonglong]
+        if lib.hipMemExportToShareableHandle(C.byref(fd), self.handle, 1, 0) != 0:
+            pytest.skip("exporting a HIP allocation as a file descriptor is not available here")
+        self.fd = fd.value
+        self.va = C.c_void_p()
+        lib.hipMemAddressReserve.argtypes = [C.POINTER(C.c_void_p), C.c_size_t, C.c_size_t, C.c_void_p, C.c_ulonglong]
+        lib.hipMemMap.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t, C.c_uint64, C.c_ulonglong]
+        lib.hipMemSetAccess.argtypes = [C.c_void_p, C.c_size_t, C.POINTER(Access), C.c_size_t]
+        assert lib.hipMemAddressReserve(C.byref(self.va), self.size, 0, None, 0) == 0
+        assert lib.hipMemMap(self.va, self.size, 0, self.handle, 0) == 0
+        acc = Access(location=Loc(type=1, id=0), flags=3)
+        assert lib.hipMemSetAccess(self.va, self.size, C.byref(acc), 1) == 0
+        lib.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+
+    def write(self, host):
+        assert self.lib.hipMemcpy(self.va, host.ctypes.data, host.nbytes, 1) == 0
+
+    def read(self, out):
+        assert self.lib.hipMemcpy(out.ctypes.data, self.va, out.nbytes, 2) == 0
+        return out
+
+    def close(self):
+        C, lib = self.C, self.lib
+        lib.hipMemUnmap.argtypes = [C.c_void_p, C.c_size_t]
+        lib.hipMemAddressFree.argtypes = [C.c_void_p, C.c_size_t]
+        lib.hipMemRelease.argtypes = [C.c_uint64]
+        lib.hipMemUnmap(self.va, self.size)
+        lib.hipMemAddressFree(self.va, self.size)
+        lib.hipMemRelease(self.handle)
+
+
+def test_surface_into_imported_buffers(pxl):
+    """Interop: the vertex and index buffers are allocations exported as file descriptors
+    (as Vulkan exports the renderer's buffers); the library imports them, meshes into them at
+    vbase / ibase, and the exporter's own mapping sees the mesh."""
+    fluid = dam_break(24)
+    gpu = pxl.FluidState(fluid, FPS, seed=0)
+    gpu.step(2)
+    verts, tris, info = gpu.surface()
+    cap = gpu.reservation()[0]
+    vbase, ibase = 64, 96
+    vex, iex = _Exporter(32 * (vbase + cap)), _Exporter(4 * (ibase + 3 * cap))
+    vimp = iimp = None
+    try:
+        vhost = np.full((vbase + cap, 8), -7.0, np.float32)
+        vex.write(vhost)
+        vimp = pxl.ImportedBuffer(vex.fd, vex.size)
+        iimp = pxl.ImportedBuffer(iex.fd, iex.size)
+        info2 = gpu.surface_device(vimp.ptr, vbase, iimp.ptr, ibase)
+        assert info2 == info
+        v = vex.read(np.empty((vbase + cap, 8), np.float32))
+        ib = iex.read(np.empty(ibase + 3 * cap, np.uint32))
+        np.testing.assert_array_equal(v[vbase:vbase + info["vertices"]], verts)
+        np.testing.assert_array_equal(ib[ibase:ibase + 3 * info["triangles"]].reshape(-1, 3), tris)
+        assert np.all(v[:vbase] == -7.0)
+    finally:
+        for x in (vimp, iimp):
+            if x is not None:
+                x.release()
+        vex.close()
+        iex.close()
+    gpu.close()
