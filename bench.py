@@ -301,7 +301,8 @@ def bench_single_mixed(args, pxl, fluid, data, workload, max_cells=0):
     residual and refinement passes of every step, and the per-phase times of two profiled
     steps."""
     from tests.scenes import FPS
-    sim = pxl.FluidState(fluid, FPS, seed=0, solver="mixed", tolerance=1e-6, max_cells=max_cells)
+    sim = pxl.FluidState(fluid, FPS, seed=0, solver="mixed", tolerance=1e-6, max_cells=max_cells,
+                         inner_max=args.inner_max)
     P = sim.particle_count()
     for _ in range(args.warmup):
         sim.step(1)
@@ -321,6 +322,7 @@ def bench_single_mixed(args, pxl, fluid, data, workload, max_cells=0):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": data,
         "config": {"workload": workload, "particles": P, "max_rel_residual": max(res),
+                   "pass_length": f"adaptive, 2..{args.inner_max}" if args.inner_max else "fixed (8, then 2, 4, ...)",
                    "outer_passes": {str(k): outer.count(k) for k in sorted(set(outer))},
                    "residual_per_step": [float(f"{r:.3g}") for r in res]},
         "roofline": None, "cpu_baseline": None,
@@ -548,6 +550,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="dam: CPU oracle sample length; serve: 0 skips the CPU baseline")
     ap.add_argument("--cpu-requests", type=int, default=192, help="serve: requests in the CPU oracle sample")
+    ap.add_argument("--inner-max", type=int, default=32,
+                    help="mixed256, obstacles: adaptive MG-CG pass length up to this many iterations (0 = fixed)")
     ap.add_argument("--cpu-threads", type=int, default=16, help="serve: host threads of the CPU oracle")
     args = ap.parse_args()
     if args.steps is None:

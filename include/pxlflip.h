@@ -61,6 +61,11 @@ typedef struct pxlflip_opts {
     uint32_t max_cells;       /* grid cell limit of the validation: 0 = the reference's 16,000,000
                                  (crates/pxl-scene/src/lib.rs:732); larger admits bigger grids up to the
                                  resolution limit, 256^3 (extension) */
+    uint32_t inner_max;       /* MIXED: 0 = the fixed pass schedule (inner_iterations, then 2, 4, 6, ...);
+                                 > 0 = adaptive: each pass runs the MG-CG iterations the measured
+                                 per-iteration reduction needs to reach the tolerance from the current
+                                 residual, between 2 and inner_max (the first pass of a simulation:
+                                 inner_iterations) */
 } pxlflip_opts;
 
 /* Fills *opts with the defaults above. */

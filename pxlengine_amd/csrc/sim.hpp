@@ -366,6 +366,21 @@ struct Sim {
         }
     }
 
+    // Adaptive pass length (opts.inner_max > 0): the iterations that take the relative
+    // residual from `rel0` to a third of the tolerance at the last measured per-iteration
+    // reduction rho_est (inner_iterations before any measurement), within [2, inner_max].  A
+    // pass is planned for at most a 1e-5 reduction: the FP32 inner solve does not reliably get
+    // further, and the next refinement pass takes over from there.
+    double rho_est = 0.0;
+    uint32_t adaptive_iterations(double rel0) const {
+        if (!(rho_est > 0.0)) return std::min(opts.inner_iterations, opts.inner_max);
+        const double target = std::max(opts.tolerance / 3.0 / rel0, 1e-5);
+        const double need = std::log(target) / std::log(rho_est);
+        const double k = std::ceil(need);
+        if (!(k >= 2.0)) return std::min<uint32_t>(2u, opts.inner_max);
+        return k >= (double)opts.inner_max ? opts.inner_max : (uint32_t)k;
+    }
+
     // Mixed-precision pressure solve after enqueue_grid_init: FP64 refinement driven from
     // the host, one residual read-back per refinement pass.
     void mixed_pressure() {
@@ -376,7 +391,8 @@ struct Sim {
         auto rel = [&](double r) { return bb > 0.0 ? std::sqrt(r / bb) : std::sqrt(r); };
         uint32_t outer = 0, inner_total = 0;
         while (rel(rr) > opts.tolerance && outer < opts.max_outer) {
-            const uint32_t inner = pass_iterations(opts.inner_iterations, outer);
+            const double before = rel(rr);
+            const uint32_t inner = opts.inner_max ? adaptive_iterations(before) : pass_iterations(opts.inner_iterations, outer);
             const int cur = x64_cur;
             replay(&inner_graph[inner][cur], [this, cur, inner] { enqueue_inner(cur, inner); });
             x64_cur ^= 1;
@@ -384,6 +400,13 @@ struct Sim {
             rr = host_scal[SC_RR];
             outer++;
             inner_total += inner;
+            if (opts.inner_max) {
+                // per-iteration reduction of this pass (a pass that hit the FP32 floor reads slower
+                // than the solver converges, which only makes later passes longer)
+                const double ratio = before > 0.0 ? rel(rr) / before : 1.0;
+                rho_est = ratio > 0.0 && ratio < 1.0 ? std::pow(ratio, 1.0 / inner) : 0.95;
+                rho_est = std::min(0.95, std::max(0.02, rho_est));
+            }
         }
         stats.rel_residual = rel(rr);
         stats.outer_iterations = outer;

@@ -55,7 +55,7 @@ class Opts(C.Structure):
         ("struct_size", C.c_uint32), ("solver", C.c_int32), ("cg_iterations", C.c_uint32),
         ("inner_iterations", C.c_uint32), ("max_outer", C.c_uint32), ("tolerance", C.c_double),
         ("device", C.c_int32), ("scene_index", C.c_uint32), ("use_graph", C.c_int32),
-        ("solver_workgroups", C.c_uint32), ("max_cells", C.c_uint32),
+        ("solver_workgroups", C.c_uint32), ("max_cells", C.c_uint32), ("inner_max", C.c_uint32),
     ]
 
 
@@ -371,7 +371,7 @@ def _json(fluid) -> bytes:
 
 
 def make_opts(solver="rbgs", cg_iterations=0, inner_iterations=8, max_outer=50, tolerance=1e-6,
-              device=0, scene_index=0, use_graph=True, solver_workgroups=0, max_cells=0) -> Opts:
+              device=0, scene_index=0, use_graph=True, solver_workgroups=0, max_cells=0, inner_max=0) -> Opts:
     o = Opts()
     load_library().pxlflip_default_opts(C.byref(o))
     o.solver = SOLVERS[solver] if isinstance(solver, str) else int(solver)
@@ -384,6 +384,7 @@ def make_opts(solver="rbgs", cg_iterations=0, inner_iterations=8, max_outer=50, 
     o.use_graph = 1 if use_graph else 0
     o.solver_workgroups = solver_workgroups
     o.max_cells = max_cells
+    o.inner_max = inner_max
     return o
 
 

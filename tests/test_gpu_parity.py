@@ -185,13 +185,14 @@ def test_cg_solution_residual(pxl, orc):
     gpu.close()
 
 
-@pytest.mark.parametrize("fluid", [dam_break(32), two_blocks(32), obstacle_scene(32)],
-                         ids=["dam32", "collide32", "obstacles32"])
-def test_mixed_solve_vs_fp64_oracle(pxl, orc, fluid):
+@pytest.mark.parametrize("fluid,inner_max", [(dam_break(32), 0), (two_blocks(32), 0), (obstacle_scene(32), 0),
+                                             (two_blocks(32), 24)],
+                         ids=["dam32", "collide32", "obstacles32", "collide32-adaptive"])
+def test_mixed_solve_vs_fp64_oracle(pxl, orc, fluid, inner_max):
     """Mixed precision: FP64 relative residual <= 1e-6 (recomputed by the oracle from the
     GPU's FP64 solution) and relative error vs the oracle's FP64 CG solution (to 1e-12)
-    below 1e-5; both stated per step for 3 steps."""
-    gpu = pxl.FluidState(fluid, FPS, seed=0, solver="mixed", tolerance=1e-6)
+    below 1e-5; both stated per step for 3 steps (also with adaptive pass lengths)."""
+    gpu = pxl.FluidState(fluid, FPS, seed=0, solver="mixed", tolerance=1e-6, inner_max=inner_max)
     sim = oracle_sim(orc, fluid)
     for s in range(3):
         p = gpu.read_particles()
