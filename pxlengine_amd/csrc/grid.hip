@@ -74,17 +74,44 @@ __device__ __forceinline__ uint32_t widx(const DevSim &S, int i, int j, int k) {
     return S.nu + S.nv + (uint32_t)i + (uint32_t)S.nx * ((uint32_t)j + (uint32_t)S.ny * (uint32_t)k);
 }
 
-// Active tiles among the 3x3x3 tiles around tile (ti, tj, tk): bit (dx+1) + 3 (dy+1) + 9 (dz+1)
-// (tiles past the grid are 0).  Each wave computes it with one load per lane and a ballot.
-__device__ __forceinline__ uint32_t active_neighbours(const DevSim &S, int ti, int tj, int tk) {
+// The 3x3x3 tiles around tile (ti, tj, tk), bit (dx+1) + 3 (dy+1) + 9 (dz+1) (tiles past the
+// grid are 0): `act` = tiles holding particles this step (their P2G boxes are current), `core`
+// = tiles holding particles this step or the step before.  One load per lane and two ballots.
+constexpr uint32_t NB_SELF = 1u << 13;
+__device__ __forceinline__ void neighbour_masks(const DevSim &S, int ti, int tj, int tk, uint32_t &act, uint32_t &core) {
     const int lane = threadIdx.x & (WAVE - 1);
-    bool on = false;
+    uint32_t v = 0;
     if (lane < 27) {
         const int x = ti + lane % 3 - 1, y = tj + (lane / 3) % 3 - 1, z = tk + lane / 9 - 1;
         if (x >= 0 && y >= 0 && z >= 0 && x < S.ntx && y < S.nty && z < S.ntz)
-            on = tile_active(S, (uint32_t)x + (uint32_t)S.ntx * ((uint32_t)y + (uint32_t)S.nty * (uint32_t)z));
+            v = S.tile_act[(uint32_t)x + (uint32_t)S.ntx * ((uint32_t)y + (uint32_t)S.nty * (uint32_t)z)];
     }
-    return (uint32_t)__ballot(on);
+    act = (uint32_t)__ballot((v & 1u) != 0);
+    core = (uint32_t)__ballot(v != 0);
+}
+
+// A rim tile of the band (no particles this step or the step before; in the band because a
+// neighbour has) keeps last step's values except where a core neighbour touches it: its P2G
+// faces within one layer of that neighbour (the neighbour's box reaches one cell in; a core
+// neighbour that just emptied leaves faces there to reset), and faces between its cells and
+// FLUID cells below it.  Everything else in it has no particle weight and no FLUID cell near,
+// and was already written with those values when the tile last had a core neighbour there.
+// True when node (u0, u1, u2) of the tile (local coordinates, 8 on the grid's upper planes)
+// lies within one layer of a core neighbour.
+__device__ __forceinline__ bool rim_node_touched(uint32_t core, int u0, int u1, int u2) {
+    const int u[3] = {u0, u1, u2};
+    uint32_t m[3];  // allowed offsets per axis: bit d+1
+#pragma unroll
+    for (int a = 0; a < 3; a++) m[a] = 2u | (u[a] == 0 ? 1u : 0u) | (u[a] >= TILE - 1 ? 4u : 0u);
+    uint32_t allowed = 0;
+#pragma unroll
+    for (int dz = 0; dz < 3; dz++)
+#pragma unroll
+        for (int dy = 0; dy < 3; dy++)
+#pragma unroll
+            for (int dx = 0; dx < 3; dx++)
+                if (((m[0] >> dx) & (m[1] >> dy) & (m[2] >> dz)) & 1u) allowed |= 1u << (dx + 3 * dy + 9 * dz);
+    return (allowed & core & ~NB_SELF) != 0u;
 }
 
 // Packed P2G sum of face (c, q) for a face pass walking tile t = (t0, t1, t2) (q in t's node
@@ -142,8 +169,15 @@ __device__ __forceinline__ void faces_at(const DevSim &S, uint32_t act, const in
 __device__ __forceinline__ void faces_tile(const DevSim &S, uint32_t tl) {
     const int t[3] = {(int)(tl % (uint32_t)S.ntx), (int)((tl / (uint32_t)S.ntx) % (uint32_t)S.nty),
                       (int)(tl / ((uint32_t)S.ntx * (uint32_t)S.nty))};
-    const uint32_t act = active_neighbours(S, t[0], t[1], t[2]);
-    for_tile_box(S, tl, true, [&](int i, int j, int k) { faces_at(S, act, t, i, j, k); });
+    uint32_t act, core;
+    neighbour_masks(S, t[0], t[1], t[2], act, core);
+    if (core & NB_SELF) {
+        for_tile_box(S, tl, true, [&](int i, int j, int k) { faces_at(S, act, t, i, j, k); });
+    } else {  // rim tile: only the layers a core neighbour touches
+        for_tile_box(S, tl, true, [&](int i, int j, int k) {
+            if (rim_node_touched(core, i - TILE * t[0], j - TILE * t[1], k - TILE * t[2])) faces_at(S, act, t, i, j, k);
+        });
+    }
 }
 __device__ __forceinline__ void body_faces(const DevSim &S, uint32_t b, uint32_t nb) {
     for_band(S, b, nb, [&](uint32_t tl) { faces_tile(S, tl); });
@@ -219,7 +253,10 @@ __device__ __forceinline__ void count_row_runs(const DevSim &S, uint32_t row, bo
 
 // The cell pass over band tile t: every lane of the workgroup runs the same number of rounds
 // (the row counting needs whole waves).
+// (A rim tile is skipped: its cells have no particles this step or the last, so they are AIR
+// or SOLID with zero pressure, divergence and stencil, as its last pass as a core tile left them.)
 __device__ __forceinline__ void cells_tile(const DevSim &S, uint32_t t) {
+    if (S.tile_act[t] == 0) return;  // the whole workgroup
     const int ti = (int)(t % (uint32_t)S.ntx), tj = (int)((t / (uint32_t)S.ntx) % (uint32_t)S.nty),
               tk = (int)(t / ((uint32_t)S.ntx * (uint32_t)S.nty));
     const int lo[3] = {ti * TILE, tj * TILE, tk * TILE};
@@ -330,7 +367,23 @@ __device__ __forceinline__ void project_at(const DevSim &S, int i, int j, int k)
     if (i < S.nx && j < S.ny) project_face(S, widx(S, i, j, k), i, j, k - 1, i, j, k);
 }
 __device__ __forceinline__ void project_tile(const DevSim &S, uint32_t t) {
-    for_tile_box(S, t, true, [&](int i, int j, int k) { project_at(S, i, j, k); });
+    if (S.tile_act[t] != 0) {
+        for_tile_box(S, t, true, [&](int i, int j, int k) { project_at(S, i, j, k); });
+        return;
+    }
+    // rim tile: its cells are not FLUID, so only faces on its lower planes can border a FLUID
+    // cell, in the tile below along that axis, which then holds particles this step
+    const int ti = (int)(t % (uint32_t)S.ntx), tj = (int)((t / (uint32_t)S.ntx) % (uint32_t)S.nty),
+              tk = (int)(t / ((uint32_t)S.ntx * (uint32_t)S.nty));
+    uint32_t act, core;
+    neighbour_masks(S, ti, tj, tk, act, core);
+    const bool below[3] = {((act >> 12) & 1u) != 0, ((act >> 10) & 1u) != 0, ((act >> 4) & 1u) != 0};
+    if (!(below[0] || below[1] || below[2])) return;  // uniform: the masks are the workgroup's
+    for_tile_box(S, t, true, [&](int i, int j, int k) {
+        if (below[0] && i == TILE * ti && j < S.ny && k < S.nz) project_face(S, uidx(S, i, j, k), i - 1, j, k, i, j, k);
+        if (below[1] && j == TILE * tj && i < S.nx && k < S.nz) project_face(S, vidx(S, i, j, k), i, j - 1, k, i, j, k);
+        if (below[2] && k == TILE * tk && i < S.nx && j < S.ny) project_face(S, widx(S, i, j, k), i, j, k - 1, i, j, k);
+    });
 }
 __device__ __forceinline__ void body_project(const DevSim &S, uint32_t b, uint32_t nb) {
     for_band(S, b, nb, [&](uint32_t t) { project_tile(S, t); });

@@ -446,6 +446,9 @@ struct Sim {
     void run_capped_steps(uint32_t n) {
         for (uint32_t s = 0; s < n; s++) {
             const uint32_t k = cfl_step_count();
+            // faces away from the liquid hold 0 + g dt from the last pass that covered them:
+            // when dt changes, the next two steps cover every tile again
+            if (k != last_substeps) mark_all_tiles();
             last_substeps = k;
             if (k == 1) {
                 step_once();

@@ -61,12 +61,13 @@ def test_obstacles_grid_phase_bitwise(pxl, orc):
     gpu.close()
 
 
-@pytest.mark.parametrize("fluid,steps", [(dam_break(32), 3), (EXAMPLE_DAM_BREAK, 2), (two_blocks(24), 3),
-                                         (sphere_drop(28), 3), (obstacle_scene(24), 4)],
+@pytest.mark.parametrize("fluid,steps", [(dam_break(32), 8), (EXAMPLE_DAM_BREAK, 2), (two_blocks(24), 6),
+                                         (sphere_drop(28), 6), (obstacle_scene(24), 8)],
                          ids=["dam32", "example", "collide24", "sphere28", "obstacles24"])
 def test_rbgs_steps_bitwise(pxl, orc, fluid, steps):
     """Whole reference steps (RBGS, pressure_iterations sweeps): particles, pressure and face
-    velocities equal the oracle bit for bit after every step."""
+    velocities equal the oracle bit for bit after every step (from the third step on the band's
+    rim tiles are only partly recomputed)."""
     print(f"rbgs steps {fluid['name']}", flush=True)
     gpu = pxl.FluidState(fluid, FPS, seed=7)
     sim = oracle_sim(orc, fluid, seed=7)
