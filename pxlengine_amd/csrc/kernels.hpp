@@ -37,6 +37,10 @@ constexpr uint32_t TILE_CELLS = 512;
 // MG_TAIL_CELLS cells together) are solved inside one workgroup's LDS.
 constexpr int MG_MAX = 12;
 constexpr int MG_TAIL_CELLS = 6144;
+// Smoothing rounds of the list levels 1..tail-1 (level 0 always runs 2): pre-smoothing runs
+// (red, black) that many times, post-smoothing (black, red) as many (the adjoint, so the
+// preconditioner stays symmetric).
+constexpr int MG_COARSE_ROUNDS = 2;
 
 struct MGLevel {
     int nx, ny, nz;

@@ -681,8 +681,10 @@ void launch_mg_vcycle(const DevSim &S, const float *r, float *u, hipStream_t s) 
             PXL_LAUNCH(k_restrict, g, LBLOCK, 0, s, level0(S), S.cg_r0, S.cg_r1, S.mg[1]);
         } else {
             PXL_LAUNCH(k_lc_pre0, g, LBLOCK, 0, s, S.mg[l]);
-            PXL_LAUNCH(k_lc_sweep, g, LBLOCK, 0, s, S.mg[l], 0);
-            PXL_LAUNCH(k_lc_sweep, g, LBLOCK, 0, s, S.mg[l], 1);
+            for (int q = 1; q < MG_COARSE_ROUNDS; q++) {
+                PXL_LAUNCH(k_lc_sweep, g, LBLOCK, 0, s, S.mg[l], 0);
+                PXL_LAUNCH(k_lc_sweep, g, LBLOCK, 0, s, S.mg[l], 1);
+            }
             PXL_LAUNCH(k_restrict, g, LBLOCK, 0, s, S.mg[l], S.mg[l].r, S.mg[l].z, S.mg[l + 1]);
         }
     }
@@ -697,8 +699,10 @@ void launch_mg_vcycle(const DevSim &S, const float *r, float *u, hipStream_t s) 
         } else {
             PXL_LAUNCH(k_lc_post, g, LBLOCK, 0, s, S.mg[l], S.mg[l + 1]);
             PXL_LAUNCH(k_lc_sweep, g, LBLOCK, 0, s, S.mg[l], 0);
-            PXL_LAUNCH(k_lc_sweep, g, LBLOCK, 0, s, S.mg[l], 1);
-            PXL_LAUNCH(k_lc_sweep, g, LBLOCK, 0, s, S.mg[l], 0);
+            for (int q = 1; q < MG_COARSE_ROUNDS; q++) {
+                PXL_LAUNCH(k_lc_sweep, g, LBLOCK, 0, s, S.mg[l], 1);
+                PXL_LAUNCH(k_lc_sweep, g, LBLOCK, 0, s, S.mg[l], 0);
+            }
         }
     }
 }

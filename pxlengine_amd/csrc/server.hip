@@ -281,18 +281,26 @@ private:
         }
     }
     void enqueue_vcycle() {
+        // the same sequence as the single-simulation V-cycle (mg.hip): two smoothing rounds on
+        // level 0, MG_COARSE_ROUNDS on the coarser list levels
         for (int l = 0; l < max_tail_; l++) {
+            const int rounds = l == 0 ? 2 : MG_COARSE_ROUNDS;
             run("mg_pre0", l, [&] { batch_mg_pre0(lists_, l, stream_); });
-            run("mg_sweep", l, [&] { batch_mg_sweep(lists_, l, 0, stream_); });
-            run("mg_sweep", l, [&] { batch_mg_sweep(lists_, l, 1, stream_); });
+            for (int q = 1; q < rounds; q++) {
+                run("mg_sweep", l, [&] { batch_mg_sweep(lists_, l, 0, stream_); });
+                run("mg_sweep", l, [&] { batch_mg_sweep(lists_, l, 1, stream_); });
+            }
             run("mg_restrict", l, [&] { batch_mg_restrict(lists_, l, stream_); });
         }
         run("mg_tail", -1, [&] { batch_mg_tail(lists_, stream_); });
         for (int l = max_tail_ - 1; l >= 0; l--) {
+            const int rounds = l == 0 ? 2 : MG_COARSE_ROUNDS;
             run("mg_post", l, [&] { batch_mg_post(lists_, l, stream_); });
             run("mg_sweep", l, [&] { batch_mg_sweep(lists_, l, 0, stream_); });
-            run("mg_sweep", l, [&] { batch_mg_sweep(lists_, l, 1, stream_); });
-            run("mg_sweep", l, [&] { batch_mg_sweep(lists_, l, 0, stream_); });
+            for (int q = 1; q < rounds; q++) {
+                run("mg_sweep", l, [&] { batch_mg_sweep(lists_, l, 1, stream_); });
+                run("mg_sweep", l, [&] { batch_mg_sweep(lists_, l, 0, stream_); });
+            }
         }
     }
     // One refinement pass for the solving members: MG-CG from e = 0 on float(r64), then the
