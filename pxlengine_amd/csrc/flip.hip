@@ -193,6 +193,9 @@ int prepare(const char *fluid_json, double fps, uint32_t seed, const pxlflip_opt
             g_create_error = "fluid '" + spec.name + "': grid too large for the multigrid hierarchy";
             return PXLFLIP_ERR_INVALID;
         }
+        // the mid levels: the run of list levels just above the tail small enough for one workgroup
+        d.mg_mid = d.mg_tail;
+        for (int q = d.mg_tail - 1; q >= 1 && cells_of(q) <= (size_t)MG_MID_CELLS; q--) d.mg_mid = q;
     }
     return PXLFLIP_OK;
 }

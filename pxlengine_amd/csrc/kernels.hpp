@@ -41,6 +41,10 @@ constexpr int MG_TAIL_CELLS = 6144;
 // (red, black) that many times, post-smoothing (black, red) as many (the adjoint, so the
 // preconditioner stays symmetric).
 constexpr int MG_COARSE_ROUNDS = 2;
+// List levels of at most MG_MID_CELLS cells just above the tail (the "mid" levels) run inside
+// the tail's single workgroup, from global memory, with workgroup barriers between their
+// sweeps instead of one grid-wide launch per sweep.
+constexpr int MG_MID_CELLS = 32768;
 
 struct MGLevel {
     int nx, ny, nz;
@@ -133,6 +137,7 @@ struct DevSim {
     MGLevel mg[MG_MAX];
     int mg_levels;  // levels including level 0 (0 when the multigrid is not allocated)
     int mg_tail;    // first level of the single-workgroup tail (>= 1)
+    int mg_mid;     // first level run inside the tail's workgroup (1 <= mg_mid <= mg_tail)
     float *part32;   // two slots of 2 x (workgroups at z-batch 1)
     double *part64;
     double *scal;

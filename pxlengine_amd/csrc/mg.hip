@@ -556,7 +556,54 @@ __device__ __forceinline__ void body_mg_tail(const DevSim &S) {
     }
     __syncthreads();  // the LDS is reused by the next simulation of a batched launch
 }
-__global__ __launch_bounds__(TBLOCK) void k_mg_tail(DevSim S) { body_mg_tail(S); }
+// The mid levels (mg_mid .. mg_tail-1) and the tail in one workgroup: the mid levels' down
+// half (pre-smoothing, restriction) with the list-level bodies over the whole level (b = 0 of
+// nb = 1) and a workgroup barrier after each, the LDS tail, then their up half.  Each cell is
+// updated by the same arithmetic as in the grid-wide launches, so the result is the same.
+// `cnt` is a batch member's count slots (null: the levels' own counts).  Level indices are
+// compile-time constants (no scratch copy of S).
+__device__ __forceinline__ MGLevel level_counts(const DevSim &S, uint32_t *cnt, int l) {
+    MGLevel v = S.mg[l];
+    if (cnt) v.count = cnt + 4 * l;
+    return v;
+}
+__device__ __forceinline__ void body_mg_midtail(const DevSim &S, uint32_t *cnt) {
+    const int M = S.mg_mid, T = S.mg_tail;
+#pragma unroll
+    for (int q = 1; q < MG_MAX - 1; q++) {
+        if (q >= M && q < T) {
+            const MGLevel v = level_counts(S, cnt, q), c = level_counts(S, cnt, q + 1);
+            body_lc_pre0(v, 0, 1);
+            __syncthreads();
+            for (int r = 1; r < MG_COARSE_ROUNDS; r++) {
+                body_lc_sweep(v, 0, 0, 1);
+                __syncthreads();
+                body_lc_sweep(v, 1, 0, 1);
+                __syncthreads();
+            }
+            body_restrict(v.nx, v.ny, v.nz, v.sten, v.r, v.z, c, 0, 1);
+            __syncthreads();
+        }
+    }
+    body_mg_tail(S);
+#pragma unroll
+    for (int q = MG_MAX - 2; q >= 1; q--) {
+        if (q >= M && q < T) {
+            const MGLevel v = level_counts(S, cnt, q), c = level_counts(S, cnt, q + 1);
+            body_lc_post(v, c, 0, 1);
+            __syncthreads();
+            body_lc_sweep(v, 0, 0, 1);
+            __syncthreads();
+            for (int r = 1; r < MG_COARSE_ROUNDS; r++) {
+                body_lc_sweep(v, 1, 0, 1);
+                __syncthreads();
+                body_lc_sweep(v, 0, 0, 1);
+                __syncthreads();
+            }
+        }
+    }
+}
+__global__ __launch_bounds__(TBLOCK) void k_mg_tail(DevSim S) { body_mg_midtail(S, nullptr); }
 
 // ---- single-sim launches (grid = level_blocks workgroups for the list kernels)
 __global__ __launch_bounds__(LBLOCK) void k_l0_pre0(DevSim S) {
@@ -654,7 +701,7 @@ __global__ __launch_bounds__(LBLOCK) void kb_post(WorkList W, int l) {
     });
 }
 __global__ __launch_bounds__(TBLOCK) void kb_mg_tail(WorkList W) {
-    for_each_work(W, [&](const DevSim &S, uint32_t, const uint32_t *) { body_mg_tail(S); });
+    for_each_work(W, [&](const DevSim &S, uint32_t, uint32_t *cnt) { body_mg_midtail(S, cnt); });
 }
 
 }  // namespace
@@ -671,7 +718,7 @@ void launch_mg_types(const DevSim &S, hipStream_t s) {
 void launch_mg_vcycle(const DevSim &S, const float *r, float *u, hipStream_t s) {
     (void)r;
     (void)u;
-    const int T = S.mg_tail;
+    const int T = S.mg_mid;  // levels from mg_mid on run in the tail's workgroup
     for (int l = 0; l < T; l++) {
         const unsigned g = level_blocks(S.cg_blocks, l);
         if (l == 0) {

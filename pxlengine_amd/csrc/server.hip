@@ -51,7 +51,7 @@ uint32_t work_blocks(const DevSim &S, int d) {
     }
     if (d >= WD_A_LVL && d < WD_A_LVL + MG_MAX) {
         const int l = d - WD_A_LVL;
-        return l < S.mg_tail ? level_blocks(S.cg_blocks, l) : 0;
+        return l < S.mg_mid ? level_blocks(S.cg_blocks, l) : 0;
     }
     return 0;
 }
@@ -116,8 +116,10 @@ double share_bytes(const char *k, const DevSim &d, const MemberCounts &m, int l)
     if (n == "mg_sweep") return (l == 0 ? 14 : 13) * cl;
     if (n == "mg_restrict") return 64 * cl1;
     if (n == "mg_post") return 13 * cl;
-    if (n == "mg_tail") {
+    if (n == "mg_tail") {  // the mid levels' sweeps, restrictions and posts, then the LDS tail
         double b = 0;
+        for (int q = d.mg_mid; q < d.mg_tail; q++)
+            b += (16 + 13 + 13 * (4 * MG_COARSE_ROUNDS - 3)) * (double)m.fluid[q] + 64.0 * m.fluid[q + 1];
         for (int q = d.mg_tail; q < d.mg_levels; q++) b += 2 * cells_of(q);
         return b + 8 * cells_of(d.mg_tail);
     }
@@ -283,7 +285,7 @@ private:
     void enqueue_vcycle() {
         // the same sequence as the single-simulation V-cycle (mg.hip): two smoothing rounds on
         // level 0, MG_COARSE_ROUNDS on the coarser list levels
-        for (int l = 0; l < max_tail_; l++) {
+        for (int l = 0; l < max_mid_; l++) {
             const int rounds = l == 0 ? 2 : MG_COARSE_ROUNDS;
             run("mg_pre0", l, [&] { batch_mg_pre0(lists_, l, stream_); });
             for (int q = 1; q < rounds; q++) {
@@ -293,7 +295,7 @@ private:
             run("mg_restrict", l, [&] { batch_mg_restrict(lists_, l, stream_); });
         }
         run("mg_tail", -1, [&] { batch_mg_tail(lists_, stream_); });
-        for (int l = max_tail_ - 1; l >= 0; l--) {
+        for (int l = max_mid_ - 1; l >= 0; l--) {
             const int rounds = l == 0 ? 2 : MG_COARSE_ROUNDS;
             run("mg_post", l, [&] { batch_mg_post(lists_, l, stream_); });
             run("mg_sweep", l, [&] { batch_mg_sweep(lists_, l, 0, stream_); });
@@ -363,7 +365,7 @@ private:
             if (n == "mg_types" || n == "mg_sten") {
                 if (level >= d.mg_levels) continue;
             } else if (n == "mg_pre0" || n == "mg_sweep" || n == "mg_restrict" || n == "mg_post") {
-                if (level >= d.mg_tail) continue;
+                if (level >= d.mg_mid) continue;
             }
             b += share_bytes(name, d, counts_[i], level < 0 ? 0 : level);
         }
@@ -516,10 +518,10 @@ private:
         // the graphs launch the level kernels of the deepest hierarchy seen so far (levels a
         // member does not have get empty work lists), so members leaving do not force a
         // re-capture
-        int levels = max_levels_, tail = max_tail_;
+        int levels = max_levels_, mid = max_mid_;
         for (Sim *s : members_) {
             levels = std::max(levels, s->d.mg_levels);
-            tail = std::max(tail, s->d.mg_tail);
+            mid = std::max(mid, s->d.mg_mid);  // list levels launched grid-wide: below mg_mid
         }
         // A launch's persistent grid is baked into the graphs: twice its list length rounded up
         // to a power of two (at least 64), grown (and the graphs re-captured) only when a list
@@ -535,10 +537,10 @@ private:
             if (need[d] > lists_[d].grid) regrid = true;
         if (regrid)
             for (int d = 0; d < WD_COUNT; d++) lists_[d].grid = std::max(lists_[d].grid, pow2(2 * need[d]));
-        if (relayout || regrid || levels != max_levels_ || tail != max_tail_) drop_graphs();
+        if (relayout || regrid || levels != max_levels_ || mid != max_mid_) drop_graphs();
         rebuilds_++;
         max_levels_ = levels;
-        max_tail_ = tail;
+        max_mid_ = mid;
         // each member's copy points its norms and list counts into the batch's arrays (the
         // counts are rebuilt every step before they are read, so slots can move between steps)
         for (size_t i = 0; i < n; i++) {
@@ -578,7 +580,7 @@ private:
     size_t band_cap_ = 0;
     size_t off_[WD_COUNT], cap_[WD_COUNT];
     WorkLists lists_{};
-    int max_levels_ = 0, max_tail_ = 0;
+    int max_levels_ = 0, max_mid_ = 0;
     hipGraphExec_t g_grid_ = nullptr, g_post_ = nullptr;
     std::map<uint32_t, std::array<hipGraphExec_t, 2>> g_inner_;  // [MG-CG iterations][x64 parity]
     uint64_t captures_ = 0, rebuilds_ = 0;
