@@ -322,7 +322,7 @@ def bench_single_mixed(args, pxl, fluid, data, workload, max_cells=0):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": data,
         "config": {"workload": workload, "particles": P, "max_rel_residual": max(res),
-                   "pass_length": f"adaptive, 2..{args.inner_max}" if args.inner_max else "fixed (8, then 2, 4, ...)",
+                   "pass_length": f"adaptive, 2..{args.inner_max}" if args.inner_max else "fixed (9, then 2, 4, ...)",
                    "outer_passes": {str(k): outer.count(k) for k in sorted(set(outer))},
                    "residual_per_step": [float(f"{r:.3g}") for r in res]},
         "roofline": None, "cpu_baseline": None,

@@ -51,7 +51,7 @@ typedef struct pxlflip_opts {
     uint32_t struct_size;     /* sizeof(pxlflip_opts) */
     int32_t solver;           /* PXLFLIP_SOLVER_* (default RBGS) */
     uint32_t cg_iterations;   /* CG: iterations per step; 0 = pressure_iterations */
-    uint32_t inner_iterations;/* MIXED: FP32 multigrid-CG iterations of the first refinement pass (default 8; pass k >= 1 runs min(2k, first) iterations) */
+    uint32_t inner_iterations;/* MIXED: FP32 multigrid-CG iterations of the first refinement pass (default 9; pass k >= 1 runs min(2k, first) iterations) */
     uint32_t max_outer;       /* MIXED: refinement passes cap (default 50) */
     double tolerance;         /* MIXED: ||b - A p|| / ||b|| target in FP64 (default 1e-6) */
     int32_t device;           /* HIP device ordinal (default 0) */
@@ -264,7 +264,7 @@ typedef struct pxlflip_server_opts {
     int32_t device;             /* HIP device ordinal (default 0) */
     uint64_t budget_bytes;      /* HBM admission budget (default 200e9) */
     uint32_t max_sims;          /* most simulations in the batch at once (default 256, max 4096) */
-    uint32_t inner_iterations;  /* as pxlflip_opts (default 8) */
+    uint32_t inner_iterations;  /* as pxlflip_opts (default 9) */
     uint32_t max_outer;         /* as pxlflip_opts (default 50) */
     double tolerance;           /* as pxlflip_opts (default 1e-6) */
     uint32_t solver_workgroups; /* per simulation; 0 = chosen by grid size */

@@ -40,7 +40,7 @@ int pxlflip_default_opts(pxlflip_opts *o) {
     o->struct_size = sizeof(pxlflip_opts);
     o->solver = PXLFLIP_SOLVER_RBGS;
     o->cg_iterations = 0;
-    o->inner_iterations = 8;
+    o->inner_iterations = 9;
     o->max_outer = 50;
     o->tolerance = 1e-6;
     o->device = 0;

@@ -946,7 +946,7 @@ int pxlflip_server_default_opts(pxlflip_server_opts *o) {
     o->device = 0;
     o->budget_bytes = 200000000000ull;
     o->max_sims = 256;
-    o->inner_iterations = 8;
+    o->inner_iterations = 9;
     o->max_outer = 50;
     o->tolerance = 1e-6;
     o->solver_workgroups = 0;

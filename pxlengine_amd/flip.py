@@ -370,7 +370,7 @@ def _json(fluid) -> bytes:
     return json.dumps(fluid).encode()
 
 
-def make_opts(solver="rbgs", cg_iterations=0, inner_iterations=8, max_outer=50, tolerance=1e-6,
+def make_opts(solver="rbgs", cg_iterations=0, inner_iterations=9, max_outer=50, tolerance=1e-6,
               device=0, scene_index=0, use_graph=True, solver_workgroups=0, max_cells=0, inner_max=0) -> Opts:
     o = Opts()
     load_library().pxlflip_default_opts(C.byref(o))
@@ -566,7 +566,7 @@ class FluidState:
                 "outer_iterations": int(st.outer_iterations)}
 
 
-def make_server_opts(device=0, budget_bytes=200_000_000_000, max_sims=256, inner_iterations=8, max_outer=50,
+def make_server_opts(device=0, budget_bytes=200_000_000_000, max_sims=256, inner_iterations=9, max_outer=50,
                      tolerance=1e-6, solver_workgroups=0, use_graph=True) -> ServerOpts:
     o = ServerOpts()
     load_library().pxlflip_server_default_opts(C.byref(o))
