@@ -372,13 +372,25 @@ __device__ __forceinline__ void for_each_work(const WorkList &W, F &&f) {
     }
 }
 
+// Virtual index of workgroup b of nb, where b mod 8 is the workgroup's XCD (workgroups are
+// dealt to the 8 XCDs round-robin, and each XCD has its own L2): with nb a multiple of 8 the
+// workgroups of one XCD get consecutive virtual indices, so list entries v and v + 1 taken in
+// the same round of a strided loop (x-neighbouring tiles, mostly) share an L2.
+__device__ __forceinline__ uint32_t xcd_major(uint32_t b, uint32_t nb) {
+    return nb % 8u == 0 ? (b % 8u) * (nb / 8u) + b / 8u : b;
+}
+
 // Runs f(S, tile) for the batch's band tiles, one tile per workgroup iteration (the band kernel
 // appends every member's band tiles to one batch-wide list, so the band passes balance over
-// tiles, not over members' workgroup counts).
+// tiles, not over members' workgroup counts), in the XCD-major order of xcd_major: the rows of
+// a tile's faces and cells are 32 bytes wide, and the x-neighbour tile that writes the rest of
+// each cache line then writes it through the same L2 (measured on the 64-request batch: faces,
+// cells and project 6-10% faster; the same order in the particle tile passes, whose writes are
+// contiguous per tile, made them slower).
 template <typename F>
 __device__ __forceinline__ void for_batch_band(const WorkList &W, F &&f) {
-    const uint32_t n = *W.band_count;
-    for (uint32_t e = blockIdx.x; e < n; e += gridDim.x) {
+    const uint32_t n = *W.band_count, g = gridDim.x;
+    for (uint32_t e = xcd_major(blockIdx.x, g); e < n; e += g) {
         const uint32_t w = W.band[e];
         f(W.sims[w >> WORK_SIM_SHIFT], w & WORK_BLOCK_MASK);
     }
