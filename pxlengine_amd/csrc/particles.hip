@@ -19,7 +19,7 @@ constexpr int SCAN_TILE = BLOCK * SCAN_ITEMS;  // 4096 cells per scan workgroup
 
 inline unsigned blocks_for(uint64_t n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
 constexpr unsigned BATCH_GRID = 2048;       // persistent grid of batched BLOCK-lane kernels
-constexpr unsigned BATCH_TILE_GRID = 1024;  // ... of batched per-tile kernels
+constexpr unsigned BATCH_TILE_GRID = 2048;  // ... of batched per-tile kernels (2048: 2% more batch throughput than 1024)
 
 // Cell of a position: clamp(floor((x - origin) * inv_h), 0, n - 1) (fluid.slang:113).
 __device__ __forceinline__ void cell_of_pos(const DevSim &S, float x, float y, float z, int &cx, int &cy, int &cz) {
