@@ -159,10 +159,11 @@ int prepare(const char *fluid_json, double fps, uint32_t seed, const pxlflip_opt
         sf.ncells = (uint32_t)sf.sx * (uint32_t)sf.sy * (uint32_t)sf.sz;
     }
     {
-        // four times the tiles the liquid fills at seeding (room for the band around it and
-        // for spreading), at least 64, at most 1024 and the grid's tile count
+        // twice the tiles the liquid fills at seeding (room for the band around it and for
+        // spreading), at least 64, at most 1024 and the grid's tile count (measured on the
+        // 64-request batch: 2x is 0.7% faster than 4x, 8x 0.3%)
         const uint64_t liquid_tiles = ((uint64_t)np / std::max<uint32_t>(spec.particles_per_cell, 1u) + TILE_CELLS - 1) / TILE_CELLS;
-        d.tile_wgs = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(d.ntiles, 1024), std::max<uint64_t>(64, 4 * liquid_tiles));
+        d.tile_wgs = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(d.ntiles, 1024), std::max<uint64_t>(64, 2 * liquid_tiles));
     }
     d.cg_blocks = (int)(o.solver_workgroups ? o.solver_workgroups : DEFAULT_SOLVER_BLOCKS);
     d.split_colours = o.solver == PXLFLIP_SOLVER_CG ? 0 : 1;  // RBGS and the multigrid sweep one colour at a time
