@@ -506,6 +506,29 @@ def bench_batch(args, pxl):
     }
 
 
+def bench_tune(args, pxl):
+    """The batch autotuner on the fixed batch of the first --profile-batch stream requests:
+    coordinate descent over the batched launch families' persistent grids (each maximum
+    halved and doubled, --steps timed steps per trial); value = the tuned batch's
+    particle-steps/s.  --grids-out writes the chosen grids as a JSON cache that --grids loads."""
+    from tests.scenes import random_request
+    sizes = tuple(int(v) for v in args.sizes.split(","))
+    reqs = [random_request(i, sizes=sizes) for i in range(args.profile_batch)]
+    steps = min(args.steps, 10)
+    r = pxl.autotune_batch(reqs, steps=steps, warmup=2, cache=args.grids_out)
+    total = sum(int(pxl.check(f)["n_particles"]) for f in reqs)
+    return {
+        "metric": METRIC, "value": total / (r["ms_per_step"] / 1e3), "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": steps, "warmup": 2, "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (random liquid boxes or spheres, seed = request index)",
+        "config": {"workload": f"batch autotuner on the first {len(reqs)} serving-stream requests, MIXED to 1e-6",
+                   "grids": r["grids"], "baseline_ms_per_step": r["baseline_ms_per_step"],
+                   "baseline_value": total / (r["baseline_ms_per_step"] / 1e3), "trials": r["trials"],
+                   "cache": args.grids_out},
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
 def cpu_serve(args, reqs):
     """The C oracle (plain FP32 CG 80) on the stream's first --cpu-requests requests, one step
     each, the simulations spread over --cpu-threads host threads (each simulation on one
@@ -540,7 +563,9 @@ def main():
                          "rate (dam: warm-up steps)")
     ap.add_argument("--calib-steps", type=int, default=10, help="timed steps of the warm-up batch")
     ap.add_argument("--workload", default="serve",
-                    choices=["serve", "dam", "batch", "mixed256", "obstacles", "frame", "reference64", "agents"])
+                    choices=["serve", "dam", "batch", "mixed256", "obstacles", "frame", "reference64", "agents", "tune"])
+    ap.add_argument("--grids", default=None, help="JSON cache of batched launch grids to load first (autotuner output)")
+    ap.add_argument("--grids-out", default=None, help="tune: write the chosen grids to this JSON file")
     ap.add_argument("--requests", type=int, default=512)
     ap.add_argument("--sizes", default="32,48,64,96,128")
     ap.add_argument("--rate", type=float, default=0.0, help="arrivals per second (0 = measured service rate)")
@@ -564,7 +589,9 @@ def main():
         sys.exit(2)
 
     import pxlengine_amd as pxl
-    run = {"serve": bench_serve, "dam": bench_dam, "batch": bench_batch, "mixed256": bench_mixed256,
+    if args.grids:
+        pxl.load_batch_grids(args.grids)
+    run = {"tune": bench_tune, "serve": bench_serve, "dam": bench_dam, "batch": bench_batch, "mixed256": bench_mixed256,
            "obstacles": bench_obstacles, "frame": bench_frame, "reference64": bench_reference64,
            "agents": bench_agents}[args.workload]
     line = run(args, pxl)

@@ -307,6 +307,23 @@ typedef struct pxlflip_server_stats {
     double retire_s;         /* device-thread time spent retiring requests (copy-out, release) */
 } pxlflip_server_stats;
 
+/* Persistent-grid maxima of the batched launch families (no counterpart in the reference:
+ * the batch engine's tuning knobs, the settings the batch autotuner picks).  A batched launch
+ * over a work list uses min(list grid, family maximum) workgroups; results do not depend on
+ * them.  Process-wide; read when a launch is recorded, so a change applies to batch graphs
+ * captured after the call (batches and servers created later).  Values: multiples of 8 in
+ * [8, 16384], else PXLFLIP_ERR_INVALID. */
+#define PXLFLIP_GRID_BAND 0      /* faces, cells, fluid-list compaction, project (default 4096) */
+#define PXLFLIP_GRID_PARTICLE 1  /* binning, scans, scatter (default 2048) */
+#define PXLFLIP_GRID_TILE 2      /* per-tile P2G / G2P (default 2048) */
+#define PXLFLIP_GRID_MG_SETUP 3  /* multigrid level types and stencils (default 2048) */
+#define PXLFLIP_GRID_MG_LIST 4   /* multigrid list levels (default 512) */
+#define PXLFLIP_GRID_MG_TAIL 5   /* multigrid tail (default 512) */
+#define PXLFLIP_GRID_SOLVER 6    /* solver list kernels (default 1024) */
+#define PXLFLIP_GRID_COUNT 7
+int pxlflip_batch_get_grid(int family, uint32_t *workgroups);
+int pxlflip_batch_set_grid(int family, uint32_t workgroups);
+
 int pxlflip_server_default_opts(pxlflip_server_opts *opts);
 /* opts may be NULL.  Fails (PXLFLIP_ERR_DEVICE) when the device cannot be used. */
 int pxlflip_server_create(const pxlflip_server_opts *opts, pxlflip_server **out);

@@ -15,9 +15,6 @@ namespace {
 
 constexpr int TX = 64, TY = 4;        // full-grid read-back launches
 constexpr int GBLOCK = 256;           // band-walking workgroups
-// persistent grid of batched launches (4096: 0.8% more batch throughput than 2048, mostly in
-// the fluid-list compaction; 1024 is 5% slower)
-constexpr unsigned BATCH_GRID = 4096;
 
 // Workgroups of the band-walking launches of one simulation (each takes one contiguous run of
 // the band list), the same count as the per-tile particle kernels.
@@ -428,11 +425,11 @@ uint32_t fluid_row_blocks(const DevSim &S) {
     return ((uint32_t)S.ny * (uint32_t)S.nz + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK;
 }
 
-void batch_faces(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_faces, batch_grid(W[WD_TILEGRID], BATCH_GRID), GBLOCK, 0, s, W[WD_TILEGRID]); }
-void batch_cells(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_cells, batch_grid(W[WD_TILEGRID], BATCH_GRID), GBLOCK, 0, s, W[WD_TILEGRID]); }
+void batch_faces(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_faces, batch_grid(W[WD_TILEGRID], batch_cap(BF_BAND)), GBLOCK, 0, s, W[WD_TILEGRID]); }
+void batch_cells(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_cells, batch_grid(W[WD_TILEGRID], batch_cap(BF_BAND)), GBLOCK, 0, s, W[WD_TILEGRID]); }
 void batch_fluid_compact(const WorkLists &W, hipStream_t s) {
-    PXL_LAUNCH(kb_fluid_compact, batch_grid(W[WD_ROWS], BATCH_GRID), BLOCK, 0, s, W[WD_ROWS]);
+    PXL_LAUNCH(kb_fluid_compact, batch_grid(W[WD_ROWS], batch_cap(BF_BAND)), BLOCK, 0, s, W[WD_ROWS]);
 }
-void batch_project(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_project, batch_grid(W[WD_TILEGRID], BATCH_GRID), GBLOCK, 0, s, W[WD_TILEGRID]); }
+void batch_project(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_project, batch_grid(W[WD_TILEGRID], batch_cap(BF_BAND)), GBLOCK, 0, s, W[WD_TILEGRID]); }
 
 }  // namespace pxl

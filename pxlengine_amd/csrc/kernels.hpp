@@ -349,6 +349,22 @@ struct WorkList {
     const uint32_t *band, *band_count;  // the batch-wide band list (DevSim::band_all) and its count
     uint32_t grid;          // workgroups a batched launch over this list uses (host; 0 = the kernel's maximum)
 };
+// Batched launch families whose persistent-grid maximum is a run-time setting (the batch
+// autotuner's knobs, pxlflip_batch_set_grid; the values are PXLFLIP_GRID_* of pxlflip.h).
+// Results do not depend on them: every kernel walks its work list with a grid stride.
+enum BatchFamily : int {
+    BF_BAND = 0,  // faces, cells, fluid-list compaction, project
+    BF_PART,      // particle and scan kernels (BLOCK lanes)
+    BF_TILE,      // per-tile P2G / G2P
+    BF_MG_GRID,   // multigrid level types and stencils
+    BF_MG_LIST,   // multigrid list levels (pre0, sweep, restrict, post)
+    BF_MG_TAIL,   // multigrid tail
+    BF_SOLVER,    // solver list kernels (spmv, update, residual, refinement)
+    BF_COUNT
+};
+unsigned batch_cap(int family);  // current maximum of a family (a multiple of 8)
+bool set_batch_cap(int family, unsigned workgroups);  // false: unknown family or value out of range
+
 // Persistent grid of a batched launch: the list's grid, at most the kernel's maximum and
 // never more than a list's minimum storage (for_each_work reads entry blockIdx.x unchecked).
 inline unsigned batch_grid(const WorkList &w, unsigned max_grid) {

@@ -43,10 +43,6 @@ constexpr int MBLOCK = 256;         // grid-walking level kernels (types, stenci
 constexpr int LBLOCK = 1024;        // list-walking level kernels
 constexpr int TBLOCK = 1024;        // the tail workgroup
 constexpr int COARSEST_SWEEPS = 1;  // sweeps R-B then as many B-R on the coarsest level
-constexpr unsigned BATCH_GRID = 2048;      // persistent grid of batched grid-walking launches
-constexpr unsigned BATCH_LGRID = 512;      // ... of batched list-walking launches (two 1024-lane
-                                           // workgroups per CU: one round; 1024 made sweeps ~6% slower)
-constexpr unsigned BATCH_TAIL_GRID = 512;  // ... of the batched tail
 
 __device__ __forceinline__ float rcp_or0(float d) { return d > 0.0f ? __builtin_amdgcn_rcpf(d) : 0.0f; }
 __device__ __forceinline__ float st_diag(uint32_t st) { return (float)((st >> 8) & 7u); }
@@ -758,16 +754,16 @@ void launch_mg_vcycle(const DevSim &S, const float *r, float *u, hipStream_t s) 
 // ---- batched
 uint32_t mg_cell_blocks(const DevSim &S, int l) { return grid_for(cells(l == 0 ? level0(S) : S.mg[l])); }
 
-void batch_mg_types(const WorkLists &W, int l, hipStream_t s) { PXL_LAUNCH(kb_mg_types, batch_grid(W[WD_MGTYPES + l], BATCH_GRID), MBLOCK, 0, s, W[WD_MGTYPES + l], l); }
-void batch_mg_sten(const WorkLists &W, int l, hipStream_t s) { PXL_LAUNCH(kb_mg_sten, batch_grid(W[WD_MGTYPES + l], BATCH_GRID), MBLOCK, 0, s, W[WD_MGTYPES + l], l); }
-void batch_mg_pre0(const WorkLists &W, int l, hipStream_t s) { PXL_LAUNCH(kb_pre0, batch_grid(W[WD_A_LVL + l], BATCH_LGRID), LBLOCK, 0, s, W[WD_A_LVL + l], l); }
+void batch_mg_types(const WorkLists &W, int l, hipStream_t s) { PXL_LAUNCH(kb_mg_types, batch_grid(W[WD_MGTYPES + l], batch_cap(BF_MG_GRID)), MBLOCK, 0, s, W[WD_MGTYPES + l], l); }
+void batch_mg_sten(const WorkLists &W, int l, hipStream_t s) { PXL_LAUNCH(kb_mg_sten, batch_grid(W[WD_MGTYPES + l], batch_cap(BF_MG_GRID)), MBLOCK, 0, s, W[WD_MGTYPES + l], l); }
+void batch_mg_pre0(const WorkLists &W, int l, hipStream_t s) { PXL_LAUNCH(kb_pre0, batch_grid(W[WD_A_LVL + l], batch_cap(BF_MG_LIST)), LBLOCK, 0, s, W[WD_A_LVL + l], l); }
 void batch_mg_sweep(const WorkLists &W, int l, uint32_t colour, hipStream_t s) {
-    PXL_LAUNCH(kb_sweep, batch_grid(W[WD_A_LVL + l], BATCH_LGRID), LBLOCK, 0, s, W[WD_A_LVL + l], l, colour);
+    PXL_LAUNCH(kb_sweep, batch_grid(W[WD_A_LVL + l], batch_cap(BF_MG_LIST)), LBLOCK, 0, s, W[WD_A_LVL + l], l, colour);
 }
 void batch_mg_restrict(const WorkLists &W, int l, hipStream_t s) {
-    PXL_LAUNCH(kb_restrict, batch_grid(W[WD_A_LVL + l], BATCH_LGRID), LBLOCK, 0, s, W[WD_A_LVL + l], l);
+    PXL_LAUNCH(kb_restrict, batch_grid(W[WD_A_LVL + l], batch_cap(BF_MG_LIST)), LBLOCK, 0, s, W[WD_A_LVL + l], l);
 }
-void batch_mg_post(const WorkLists &W, int l, hipStream_t s) { PXL_LAUNCH(kb_post, batch_grid(W[WD_A_LVL + l], BATCH_LGRID), LBLOCK, 0, s, W[WD_A_LVL + l], l); }
-void batch_mg_tail(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_mg_tail, batch_grid(W[WD_A_SIM], BATCH_TAIL_GRID), TBLOCK, 0, s, W[WD_A_SIM]); }
+void batch_mg_post(const WorkLists &W, int l, hipStream_t s) { PXL_LAUNCH(kb_post, batch_grid(W[WD_A_LVL + l], batch_cap(BF_MG_LIST)), LBLOCK, 0, s, W[WD_A_LVL + l], l); }
+void batch_mg_tail(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_mg_tail, batch_grid(W[WD_A_SIM], batch_cap(BF_MG_TAIL)), TBLOCK, 0, s, W[WD_A_SIM]); }
 
 }  // namespace pxl

@@ -18,8 +18,6 @@ constexpr int TILE_BLOCK = 1024;  // lanes of the per-tile P2G / G2P workgroups
 constexpr int SCAN_TILE = BLOCK * SCAN_ITEMS;  // 4096 cells per scan workgroup
 
 inline unsigned blocks_for(uint64_t n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
-constexpr unsigned BATCH_GRID = 2048;       // persistent grid of batched BLOCK-lane kernels
-constexpr unsigned BATCH_TILE_GRID = 2048;  // ... of batched per-tile kernels (2048: 2% more batch throughput than 1024)
 
 // Cell of a position: clamp(floor((x - origin) * inv_h), 0, n - 1) (fluid.slang:113).
 __device__ __forceinline__ void cell_of_pos(const DevSim &S, float x, float y, float z, int &cx, int &cy, int &cz) {
@@ -754,22 +752,22 @@ uint32_t tile_blocks(const DevSim &S) { return blocks_for(S.ntiles); }
 uint32_t tile_grid_blocks(const DevSim &S) { return tile_grid(S); }
 
 void batch_clear_grid(const WorkLists &W, hipStream_t s) {
-    PXL_LAUNCH(kb_clear_grid, batch_grid(W[WD_TILEGRID], BATCH_GRID), BLOCK, 0, s, W[WD_TILEGRID]);
+    PXL_LAUNCH(kb_clear_grid, batch_grid(W[WD_TILEGRID], batch_cap(BF_PART)), BLOCK, 0, s, W[WD_TILEGRID]);
 }
-void batch_bin_count(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_bin_count, batch_grid(W[WD_PART], BATCH_GRID), BLOCK, 0, s, W[WD_PART]); }
+void batch_bin_count(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_bin_count, batch_grid(W[WD_PART], batch_cap(BF_PART)), BLOCK, 0, s, W[WD_PART]); }
 void batch_scan_reduce(const WorkLists &W, int which, hipStream_t s) {
-    PXL_LAUNCH(kb_scan_reduce, batch_grid(W[which == 0 ? WD_SCAN_KEYS : WD_SCAN_ROWS], BATCH_GRID), BLOCK, 0, s, W[which == 0 ? WD_SCAN_KEYS : WD_SCAN_ROWS], which);
+    PXL_LAUNCH(kb_scan_reduce, batch_grid(W[which == 0 ? WD_SCAN_KEYS : WD_SCAN_ROWS], batch_cap(BF_PART)), BLOCK, 0, s, W[which == 0 ? WD_SCAN_KEYS : WD_SCAN_ROWS], which);
 }
 void batch_scan_tiles(const WorkLists &W, int which, hipStream_t s) {
-    PXL_LAUNCH(kb_scan_tiles, batch_grid(W[WD_SIM], BATCH_GRID), BLOCK, 0, s, W[WD_SIM], which);
+    PXL_LAUNCH(kb_scan_tiles, batch_grid(W[WD_SIM], batch_cap(BF_PART)), BLOCK, 0, s, W[WD_SIM], which);
 }
 void batch_scan_down(const WorkLists &W, int which, hipStream_t s) {
-    PXL_LAUNCH(kb_scan_down, batch_grid(W[which == 0 ? WD_SCAN_KEYS : WD_SCAN_ROWS], BATCH_GRID), BLOCK, 0, s, W[which == 0 ? WD_SCAN_KEYS : WD_SCAN_ROWS], which);
+    PXL_LAUNCH(kb_scan_down, batch_grid(W[which == 0 ? WD_SCAN_KEYS : WD_SCAN_ROWS], batch_cap(BF_PART)), BLOCK, 0, s, W[which == 0 ? WD_SCAN_KEYS : WD_SCAN_ROWS], which);
 }
-void batch_tile_list(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_tile_list, batch_grid(W[WD_TILES], BATCH_GRID), BLOCK, 0, s, W[WD_TILES]); }
-void batch_band(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_band, batch_grid(W[WD_TILES], BATCH_GRID), BLOCK, 0, s, W[WD_TILES]); }
-void batch_scatter(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_scatter, batch_grid(W[WD_PART], BATCH_GRID), BLOCK, 0, s, W[WD_PART]); }
-void batch_p2g(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_p2g_tile, batch_grid(W[WD_TILEGRID], BATCH_TILE_GRID), TILE_BLOCK, 0, s, W[WD_TILEGRID]); }
-void batch_g2p(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_g2p_tile, batch_grid(W[WD_TILEGRID], BATCH_TILE_GRID), TILE_BLOCK, 0, s, W[WD_TILEGRID]); }
+void batch_tile_list(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_tile_list, batch_grid(W[WD_TILES], batch_cap(BF_PART)), BLOCK, 0, s, W[WD_TILES]); }
+void batch_band(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_band, batch_grid(W[WD_TILES], batch_cap(BF_PART)), BLOCK, 0, s, W[WD_TILES]); }
+void batch_scatter(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_scatter, batch_grid(W[WD_PART], batch_cap(BF_PART)), BLOCK, 0, s, W[WD_PART]); }
+void batch_p2g(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_p2g_tile, batch_grid(W[WD_TILEGRID], batch_cap(BF_TILE)), TILE_BLOCK, 0, s, W[WD_TILEGRID]); }
+void batch_g2p(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_g2p_tile, batch_grid(W[WD_TILEGRID], batch_cap(BF_TILE)), TILE_BLOCK, 0, s, W[WD_TILEGRID]); }
 
 }  // namespace pxl

@@ -19,6 +19,7 @@
 // Server: a device thread admits queued requests first-in first-out while their HBM
 // footprint fits the budget, allocates them, adds them to the batch, runs batch steps and
 // retires simulations at the step boundary after their last step.
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -592,6 +593,18 @@ private:
     std::vector<std::pair<std::string, Agg>> agg_;
 };
 
+// Persistent-grid maxima of the batched launch families, measured on the 64-request serving
+// batch (MI355X; the autotuner can re-pick them): band passes 4096 (0.8% more throughput than
+// 2048, 5% more than 1024), particle kernels 2048, per-tile P2G/G2P 2048 (2% more than 1024),
+// multigrid level setup 2048, multigrid list kernels 512 (two 1024-lane workgroups per CU,
+// so one round on the chip; 1024 made sweeps ~6% slower), tail 512, solver 1024 (2048 no
+// faster).  Read when a launch is recorded, so a change applies to graphs captured later.
+std::atomic<uint32_t> g_batch_caps[BF_COUNT] = {4096, 2048, 2048, 2048, 512, 512, 1024};
+static_assert(BF_BAND == PXLFLIP_GRID_BAND && BF_PART == PXLFLIP_GRID_PARTICLE && BF_TILE == PXLFLIP_GRID_TILE &&
+                  BF_MG_GRID == PXLFLIP_GRID_MG_SETUP && BF_MG_LIST == PXLFLIP_GRID_MG_LIST &&
+                  BF_MG_TAIL == PXLFLIP_GRID_MG_TAIL && BF_SOLVER == PXLFLIP_GRID_SOLVER && BF_COUNT == PXLFLIP_GRID_COUNT,
+              "batch grid families match pxlflip.h");
+
 // Solver workgroups of a batched simulation: one per ~2048 FLUID cells it is expected to hold
 // (its particles over the seeding density), so that a workgroup's slice of a list keeps its
 // lanes busy.
@@ -627,6 +640,17 @@ struct Request {
 };
 
 }  // namespace
+
+unsigned batch_cap(int family) {
+    return family >= 0 && family < BF_COUNT ? g_batch_caps[family].load(std::memory_order_relaxed) : 1024u;
+}
+bool set_batch_cap(int family, unsigned workgroups) {
+    // a multiple of 8 (the XCD mappings assume it) and within a work list's minimum storage
+    if (family < 0 || family >= BF_COUNT || workgroups < 8 || workgroups > WORK_LIST_MIN_CAP || workgroups % 8)
+        return false;
+    g_batch_caps[family].store(workgroups, std::memory_order_relaxed);
+    return true;
+}
 }  // namespace pxl
 
 // Host threads that seed and allocate admitted requests (seeding a large liquid is several
@@ -938,6 +962,16 @@ double percentile(std::vector<double> v, double q) {
 }  // namespace
 
 extern "C" {
+
+int pxlflip_batch_get_grid(int family, uint32_t *workgroups) {
+    if (!workgroups || family < 0 || family >= pxl::BF_COUNT) return PXLFLIP_ERR_INVALID;
+    *workgroups = pxl::batch_cap(family);
+    return PXLFLIP_OK;
+}
+
+int pxlflip_batch_set_grid(int family, uint32_t workgroups) {
+    return pxl::set_batch_cap(family, workgroups) ? PXLFLIP_OK : PXLFLIP_ERR_INVALID;
+}
 
 int pxlflip_server_default_opts(pxlflip_server_opts *o) {
     if (!o) return PXLFLIP_ERR_INVALID;

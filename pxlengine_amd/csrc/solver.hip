@@ -41,7 +41,6 @@ namespace {
 
 constexpr int SBLOCK = 1024;  // lanes per solver workgroup (16 wave64)
 constexpr int SWAVES = SBLOCK / WAVE;
-constexpr unsigned BATCH_SGRID = 1024;  // persistent grid of batched solver launches
 
 // Fields of a list entry's stencil word (see DevSim::fl_word).
 __device__ __forceinline__ bool w_fluid(uint32_t w, int n) { return (w >> n) & 1u; }
@@ -510,19 +509,19 @@ void launch_rbgs(const DevSim &S, uint32_t parity, hipStream_t s) {
 }
 
 // ---- batched (the solving simulations: x = e, r = cg_r0, u = cg_r1)
-void batch_mixed_init(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_mixed_init, batch_grid(W[WD_A_SOLVER], BATCH_SGRID), SBLOCK, 0, s, W[WD_A_SOLVER]); }
+void batch_mixed_init(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_mixed_init, batch_grid(W[WD_A_SOLVER], batch_cap(BF_SOLVER)), SBLOCK, 0, s, W[WD_A_SOLVER]); }
 void batch_norms(const WorkLists &W, int nv, hipStream_t s) {
-    PXL_LAUNCH(kb_norms_finalize, batch_grid(W[WD_A_SIM], BATCH_SGRID), BLOCK, 0, s, W[WD_A_SIM], nv);
+    PXL_LAUNCH(kb_norms_finalize, batch_grid(W[WD_A_SIM], batch_cap(BF_SOLVER)), BLOCK, 0, s, W[WD_A_SIM], nv);
 }
-void batch_cg_residual(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_cg_residual, batch_grid(W[WD_A_SOLVER], BATCH_SGRID), SBLOCK, 0, s, W[WD_A_SOLVER]); }
+void batch_cg_residual(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_cg_residual, batch_grid(W[WD_A_SOLVER], batch_cap(BF_SOLVER)), SBLOCK, 0, s, W[WD_A_SOLVER]); }
 void batch_mgcg_spmv(const WorkLists &W, uint32_t slot, int first, hipStream_t s) {
-    PXL_LAUNCH(kb_mgcg_spmv, batch_grid(W[WD_A_SOLVER], BATCH_SGRID), SBLOCK, 0, s, W[WD_A_SOLVER], slot, first);
+    PXL_LAUNCH(kb_mgcg_spmv, batch_grid(W[WD_A_SOLVER], batch_cap(BF_SOLVER)), SBLOCK, 0, s, W[WD_A_SOLVER], slot, first);
 }
 void batch_mgcg_update(const WorkLists &W, uint32_t it, hipStream_t s) {
-    PXL_LAUNCH(kb_mgcg_update, batch_grid(W[WD_A_SOLVER], BATCH_SGRID), SBLOCK, 0, s, W[WD_A_SOLVER], it);
+    PXL_LAUNCH(kb_mgcg_update, batch_grid(W[WD_A_SOLVER], batch_cap(BF_SOLVER)), SBLOCK, 0, s, W[WD_A_SOLVER], it);
 }
 void batch_mixed_update(const WorkLists &W, int cur, hipStream_t s) {
-    PXL_LAUNCH(kb_mixed_update, batch_grid(W[WD_A_SOLVER], BATCH_SGRID), SBLOCK, 0, s, W[WD_A_SOLVER], cur);
+    PXL_LAUNCH(kb_mixed_update, batch_grid(W[WD_A_SOLVER], batch_cap(BF_SOLVER)), SBLOCK, 0, s, W[WD_A_SOLVER], cur);
 }
 
 }  // namespace pxl

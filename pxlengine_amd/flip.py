@@ -12,6 +12,7 @@ from __future__ import annotations
 import ctypes as C
 import json
 import os
+import time
 from typing import Optional
 
 import numpy as np
@@ -304,6 +305,8 @@ _SIGNATURES = {
     "pxlflip_profile_steps": (C.c_int, [_H, C.c_uint32, C.POINTER(KernelTime), C.c_uint32, C.POINTER(C.c_uint32)]),
     "pxlflip_profile_surface": (C.c_int, [_H, C.c_uint32, C.POINTER(KernelTime), C.c_uint32, C.POINTER(C.c_uint32)]),
     "pxlflip_autotune": (C.c_int, [_H, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_double)]),
+    "pxlflip_batch_get_grid": (C.c_int, [C.c_int, C.POINTER(C.c_uint32)]),
+    "pxlflip_batch_set_grid": (C.c_int, [C.c_int, C.c_uint32]),
     "pxlflip_last_error": (C.c_char_p, [_H]),
     "pxlflip_server_default_opts": (C.c_int, [C.POINTER(ServerOpts)]),
     "pxlflip_server_create": (C.c_int, [C.POINTER(ServerOpts), C.POINTER(_H)]),
@@ -723,3 +726,94 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+# ---- batched launch grids and their autotuner (pxlflip_batch_get_grid / _set_grid)
+
+# PXLFLIP_GRID_* of pxlflip.h, in order
+GRID_FAMILIES = ("band", "particle", "tile", "mg_setup", "mg_list", "mg_tail", "solver")
+
+
+def batch_grids() -> dict:
+    """The persistent-grid maximum of every batched launch family (process-wide)."""
+    lib = load_library()
+    out = {}
+    for i, name in enumerate(GRID_FAMILIES):
+        v = C.c_uint32()
+        _check(lib.pxlflip_batch_get_grid(i, C.byref(v)))
+        out[name] = int(v.value)
+    return out
+
+
+def set_batch_grids(grids: dict):
+    """Sets the maxima of the named families (multiples of 8 in [8, 16384]); they apply to
+    batch graphs captured afterwards.  Unknown names raise KeyError, bad values FluidError."""
+    lib = load_library()
+    for name, v in grids.items():
+        i = GRID_FAMILIES.index(name) if name in GRID_FAMILIES else None
+        if i is None:
+            raise KeyError(f"unknown batch grid family {name!r} (one of {', '.join(GRID_FAMILIES)})")
+        rc = lib.pxlflip_batch_set_grid(i, int(v))
+        if rc != OK:
+            raise FluidError(rc, f"batch grid {name}: {v} is not a multiple of 8 in [8, 16384]")
+
+
+def save_batch_grids(path: str, grids: Optional[dict] = None, **extra):
+    """Writes the grids (default: the current ones) to a JSON cache file."""
+    doc = {"grids": dict(grids if grids is not None else batch_grids())}
+    doc.update(extra)
+    with open(path, "w") as f:
+        json.dump(doc, f, indent=1, sort_keys=True)
+
+
+def load_batch_grids(path: str) -> dict:
+    """Applies the grids of a JSON cache file written by save_batch_grids / autotune_batch."""
+    with open(path) as f:
+        grids = json.load(f)["grids"]
+    set_batch_grids(grids)
+    return grids
+
+
+def _time_batch(fluids, fps, steps, warmup, opts) -> float:
+    with Batch(fluids, fps, **opts) as b:
+        b.step(warmup)
+        t0 = time.perf_counter()
+        b.step(steps)
+        return (time.perf_counter() - t0) / steps
+
+
+def autotune_batch(fluids, fps: float = 24.0, steps: int = 3, warmup: int = 1, families=None,
+                   factors=(0.5, 2.0), min_gain: float = 0.01, cache: Optional[str] = None, **opts) -> dict:
+    """Picks the batched launch grids for a workload by measurement: coordinate descent over
+    the families (default all), trying each one's maximum times `factors` (rounded to a
+    multiple of 8, within [8, 16384]) on a fresh fixed batch of `fluids` (`warmup` untimed
+    and `steps` timed steps each) and keeping a value only when it is at least `min_gain`
+    faster than the best so far.  One untimed trial with the current grids comes first (the
+    process's first batch runs slower).  Leaves the best grids set, writes them to `cache`
+    (JSON) when given, and returns {"grids", "ms_per_step", "baseline_ms_per_step",
+    "trials"}.  Results of a batch do not depend on the grids, only its speed."""
+    fams = tuple(families) if families is not None else GRID_FAMILIES
+    for name in fams:
+        if name not in GRID_FAMILIES:
+            raise KeyError(f"unknown batch grid family {name!r}")
+    best = batch_grids()
+    _time_batch(fluids, fps, 1, warmup, opts)
+    base = best_t = _time_batch(fluids, fps, steps, warmup, opts)
+    trials = [{"grids": dict(best), "ms_per_step": base * 1e3}]
+    for name in fams:
+        start = best[name]
+        for f in factors:
+            v = min(16384, max(8, int(round(start * f / 8.0)) * 8))
+            if v == best[name]:
+                continue
+            set_batch_grids({name: v})
+            t = _time_batch(fluids, fps, steps, warmup, opts)
+            trials.append({"family": name, "workgroups": v, "ms_per_step": t * 1e3})
+            if t < best_t * (1.0 - min_gain):
+                best_t, best[name] = t, v
+            set_batch_grids({name: best[name]})
+    result = {"grids": best, "ms_per_step": best_t * 1e3, "baseline_ms_per_step": base * 1e3, "trials": trials}
+    if cache:
+        save_batch_grids(cache, best, ms_per_step=best_t * 1e3, baseline_ms_per_step=base * 1e3,
+                         members=len(fluids), steps=steps, trials=trials)
+    return result

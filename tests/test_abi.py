@@ -220,3 +220,36 @@ def test_server_without_a_device_fails_cleanly(pxl):
         assert e.code == pxl.flip.ERR_DEVICE and str(e)
     else:
         srv.close()
+
+
+def test_batch_grid_settings(pxl, tmp_path):
+    """The batched launch families' persistent-grid maxima (host-only settings): the measured
+    defaults, validation (multiples of 8 in [8, 16384], known families) and the JSON cache the
+    autotuner writes."""
+    lib = pxl.load_library()
+    saved = pxl.batch_grids()
+    assert saved == {"band": 4096, "particle": 2048, "tile": 2048, "mg_setup": 2048, "mg_list": 512,
+                     "mg_tail": 512, "solver": 1024}
+    try:
+        pxl.set_batch_grids({"tile": 1024, "solver": 8, "band": 16384})
+        g = pxl.batch_grids()
+        assert (g["tile"], g["solver"], g["band"], g["mg_list"]) == (1024, 8, 16384, 512)
+        for bad in (0, 4, 1001, 16392):
+            with pytest.raises(pxl.FluidError):
+                pxl.set_batch_grids({"mg_list": bad})
+        assert pxl.batch_grids()["mg_list"] == 512
+        with pytest.raises(KeyError):
+            pxl.set_batch_grids({"tiles": 8})
+        out = ctypes.c_uint32()
+        assert lib.pxlflip_batch_set_grid(len(pxl.GRID_FAMILIES), 8) == -1
+        assert lib.pxlflip_batch_set_grid(-1, 8) == -1
+        assert lib.pxlflip_batch_get_grid(len(pxl.GRID_FAMILIES), ctypes.byref(out)) == -1
+        assert lib.pxlflip_batch_get_grid(0, None) == -1
+        path = str(tmp_path / "grids.json")
+        pxl.save_batch_grids(path, note="test")
+        pxl.set_batch_grids(saved)
+        assert pxl.load_batch_grids(path) == g
+        assert pxl.batch_grids() == g
+    finally:
+        pxl.set_batch_grids(saved)
+    assert pxl.batch_grids() == saved

@@ -145,3 +145,34 @@ def test_fixed_batch_matches_server_and_prices_launches(pxl):
     by = {k["kernel"]: k for k in prof}
     for name in ("p2g", "g2p", "faces", "mg_sweep", "mgcg_update", "mg_tail"):
         assert by[name]["launches"] > 0 and by[name]["total_ms"] > 0 and by[name]["alg_bytes"] > 0, name
+
+
+def test_batch_grids_change_speed_not_results(pxl, tmp_path):
+    """The batched launches' persistent-grid maxima only change how work is spread: a batch
+    stepped with every family at the minimum (8 workgroups) is bitwise the batch stepped with
+    the defaults.  The autotuner then picks grids by timing and writes its JSON cache."""
+    fluids = [random_request(i, sizes=(32, 48)) for i in range(3)]
+    saved = pxl.batch_grids()
+    with pxl.Batch(fluids, FPS) as b:
+        b.step(2)
+        ref = [b.read_particles(i) for i in range(len(fluids))]
+    try:
+        pxl.set_batch_grids({k: 8 for k in pxl.GRID_FAMILIES})
+        with pxl.Batch(fluids, FPS) as b:
+            b.step(2)
+            got = [b.read_particles(i) for i in range(len(fluids))]
+        for i, (a, c) in enumerate(zip(ref, got)):
+            assert np.array_equal(a.view(np.uint32), c.view(np.uint32)), f"member {i} differs with 8-workgroup grids"
+        pxl.set_batch_grids(saved)
+        cache = str(tmp_path / "tuned.json")
+        r = pxl.autotune_batch(fluids, FPS, steps=1, warmup=1, families=("tile", "mg_list"), cache=cache)
+        print(f"autotune {r['grids']} {r['ms_per_step']:.3f} ms/step (baseline {r['baseline_ms_per_step']:.3f})",
+              flush=True)
+        assert set(r["grids"]) == set(pxl.GRID_FAMILIES)
+        assert all(v % 8 == 0 and 8 <= v <= 16384 for v in r["grids"].values())
+        assert len(r["trials"]) == 5 and r["ms_per_step"] <= r["baseline_ms_per_step"]
+        assert pxl.batch_grids() == r["grids"]
+        pxl.set_batch_grids(saved)
+        assert pxl.load_batch_grids(cache) == r["grids"]
+    finally:
+        pxl.set_batch_grids(saved)
