@@ -29,5 +29,6 @@ def test_multi_rank_launch_is_refused():
 def test_workloads_listed():
     r = _run(["--help"])
     assert r.returncode == 0
-    for w in ("serve", "dam", "batch", "mixed256", "obstacles", "frame", "reference64", "agents"):
+    for w in ("serve", "dam", "batch", "mixed256", "obstacles", "frame", "reference64", "agents", "tune"):
         assert w in r.stdout
+    assert "--grids" in r.stdout and "--grids-out" in r.stdout
