@@ -57,6 +57,22 @@ uint32_t work_blocks(const DevSim &S, int d) {
     return 0;
 }
 
+// A work-list entry holds a member's workgroup in its low WORK_SIM_SHIFT bits, so no domain of a
+// batched simulation may need more workgroups than that field counts (the particle domain
+// reaches it first: 2^20 workgroups of 256 particles).
+bool work_fits(const DevSim &S, std::string *why) {
+    for (int d = 0; d < WD_COUNT; d++) {
+        const uint32_t nb = work_blocks(S, d);
+        if (nb > WORK_BLOCK_MASK + 1u) {
+            if (why)
+                *why = "work domain " + std::to_string(d) + " needs " + std::to_string(nb) +
+                       " workgroups, a batch entry addresses " + std::to_string(WORK_BLOCK_MASK + 1u);
+            return false;
+        }
+    }
+    return true;
+}
+
 namespace {
 
 double now_s() {
@@ -476,6 +492,10 @@ private:
         HIP_CHECK(hipStreamSynchronize(stream_));
         const size_t n = members_.size();
         if (n > MAX_BATCH_SIMS) throw std::runtime_error("batch holds more simulations than the work lists address");
+        for (Sim *s : members_) {
+            std::string why;
+            if (!work_fits(s->d, &why)) throw std::runtime_error("batch member too large: " + why);
+        }
         bool relayout = false;
         if (n > sims_cap_) {
             sims_cap_ = std::max<size_t>(2 * sims_cap_, std::max<size_t>(n, 64));
@@ -674,6 +694,7 @@ struct pxlflip_server {
     std::map<uint64_t, std::unique_ptr<pxl::Request>> all;
     uint64_t next_id = 1;
     bool stop = false;
+    bool worker_done = false;  // the device thread has left: nothing more is queued to free
     std::thread worker;
     std::vector<std::thread> allocators;  // allocate + seed admitted requests off the device thread
     std::thread releaser;                 // frees finished requests' simulations
@@ -731,7 +752,9 @@ void pxlflip_server::finish(pxl::Request *r, int status, const std::string &err)
 }
 
 // Frees the simulations of finished requests and returns their budget.  Freeing device
-// memory waits for the device, so this runs on its own thread, not the device thread.
+// memory waits for the device, so this runs on its own thread, not the device thread.  It
+// leaves only after the device thread has (whose last step can still retire simulations), so
+// every retired simulation is freed here.
 void pxlflip_server::free_loop() {
     using namespace pxl;
     try {
@@ -740,8 +763,8 @@ void pxlflip_server::free_loop() {
     }
     std::unique_lock<std::mutex> lk(mu);
     while (true) {
-        cv.wait(lk, [&] { return !to_free.empty() || stop; });
-        if (to_free.empty()) break;  // stop, and nothing left to free
+        cv.wait(lk, [&] { return !to_free.empty() || worker_done; });
+        if (to_free.empty()) break;  // the device thread has left and nothing is left to free
         auto item = std::move(to_free.front());
         to_free.pop_front();
         lk.unlock();
@@ -933,6 +956,8 @@ void pxlflip_server::run() {
         finish(queue.front(), PXLFLIP_REQ_CANCELLED, "server stopped");
         queue.pop_front();
     }
+    worker_done = true;
+    cv.notify_all();
 }
 
 namespace {
@@ -1051,10 +1076,19 @@ int pxlflip_server_create(const pxlflip_server_opts *opts, pxlflip_server **out)
 }
 
 namespace {
-// Batched members step in lockstep with one dt; the CFL cap's per-simulation substeps do not fit.
-bool batchable(const pxl::Sim &s) {
+// Sets a prepared simulation's batched solver workgroups and checks that it can join a batch:
+// members step in lockstep with one dt (the CFL cap's per-simulation substeps do not fit), and
+// every domain's workgroups must be addressable by a work-list entry's block field.
+bool batchable(pxl::Sim &s, const pxlflip_server_opts &o) {
+    using namespace pxl;
+    s.d.cg_blocks = (int)(o.solver_workgroups ? o.solver_workgroups : batch_solver_blocks(s));
     if (s.spec.cfl > 0.0f) {
-        pxl::g_create_error = "fluid '" + s.spec.name + "': cfl (the CFL cap) is not supported in batched execution";
+        g_create_error = "fluid '" + s.spec.name + "': cfl (the CFL cap) is not supported in batched execution";
+        return false;
+    }
+    std::string why;
+    if (!work_fits(s.d, &why)) {
+        g_create_error = "fluid '" + s.spec.name + "': too large for batched execution (" + why + ")";
         return false;
     }
     return true;
@@ -1069,12 +1103,11 @@ int pxlflip_server_submit(pxlflip_server *srv, const char *fluid_json, double fp
     r->t_submit = now_s();
     Sim probe;
     if (prepare(fluid_json, fps, seed, &srv->sim_opts, probe, nullptr, nullptr) != PXLFLIP_OK ||
-        !batchable(probe)) {
+        !batchable(probe, srv->opts)) {
         std::lock_guard<std::mutex> lk(srv->mu);
         srv->error = g_create_error;
         return PXLFLIP_ERR_INVALID;
     }
-    probe.d.cg_blocks = (int)(srv->opts.solver_workgroups ? srv->opts.solver_workgroups : batch_solver_blocks(probe));
     r->bytes = Sim::plan_total(probe.plan());
     r->np = probe.d.np;
     r->json = fluid_json;
@@ -1244,12 +1277,8 @@ void pxlflip_server_destroy(pxlflip_server *srv) {
         srv->stop = true;
         srv->cv.notify_all();
     }
-    if (srv->worker.joinable()) srv->worker.join();
-    {
-        std::lock_guard<std::mutex> lk(srv->mu);
-        srv->cv.notify_all();
-    }
-    if (srv->releaser.joinable()) srv->releaser.join();  // frees what is left, then leaves
+    if (srv->worker.joinable()) srv->worker.join();  // also joins the allocator threads
+    if (srv->releaser.joinable()) srv->releaser.join();  // frees every retired simulation, then leaves
     (void)hipSetDevice(srv->opts.device);
     srv->batch.reset();
     delete srv;
@@ -1322,6 +1351,14 @@ int pxlflip_batch_create(const char *const *fluid_jsons, const uint32_t *seeds, 
     b->sim_opts.tolerance = o.tolerance;
     b->sim_opts.device = o.device;
     b->sim_opts.use_graph = o.use_graph;
+    // every block is parsed, validated and checked for batching before any device work
+    for (uint32_t i = 0; i < n; i++) {
+        Sim probe;
+        b->sim_opts.scene_index = i;
+        const int rc = prepare(fluid_jsons[i], fps, seeds ? seeds[i] : 0u, &b->sim_opts, probe, nullptr, nullptr);
+        if (rc != PXLFLIP_OK) return rc;
+        if (!batchable(probe, o)) return PXLFLIP_ERR_INVALID;
+    }
     try {
         HIP_CHECK(hipSetDevice(o.device));
         for (uint32_t i = 0; i < n; i++) {
@@ -1330,8 +1367,7 @@ int pxlflip_batch_create(const char *const *fluid_jsons, const uint32_t *seeds, 
             b->sim_opts.scene_index = i;
             const int rc = prepare(fluid_jsons[i], fps, seeds ? seeds[i] : 0u, &b->sim_opts, *sim, &x0, &v0);
             if (rc != PXLFLIP_OK) return rc;
-            if (!batchable(*sim)) return PXLFLIP_ERR_INVALID;
-            sim->d.cg_blocks = (int)(o.solver_workgroups ? o.solver_workgroups : batch_solver_blocks(*sim));
+            if (!batchable(*sim, o)) return PXLFLIP_ERR_INVALID;
             HIP_CHECK(hipStreamCreateWithFlags(&sim->stream, hipStreamNonBlocking));
             sim->allocate(x0, v0);
             sim->reset();

@@ -20,6 +20,13 @@ const char *const kFluidFields[] = {"name", "bounds", "resolution", "fill", "par
 const char *const kBlockFields[] = {"min", "max", "velocity"};
 const char *const kSphereFields[] = {"center", "radius", "velocity"};  // extension
 const char *const kObstacleFields[] = {"center", "radius"};              // extension
+// Material and MaterialInputs (crates/pxl-scene/src/lib.rs:365-444, both deny_unknown_fields)
+const char *const kMaterialFields[] = {"inputs", "program", "role"};
+const char *const kMaterialInputFields[] = {
+    "base_color", "base_color_texture", "specular_roughness", "specular_roughness_texture", "base_metalness",
+    "base_metalness_texture", "emission_color", "emission_color_texture", "emission_luminance", "normal_texture",
+    "normal_scale", "uv_scale", "transmission_weight", "specular_ior", "transmission_color", "transmission_depth"};
+const char *const kRoles[] = {"surface", "shadow_catcher", "holdout"};  // Role, rename_all snake_case
 
 template <size_t N>
 std::string expected_list(const char *const (&fields)[N]) {
@@ -121,6 +128,65 @@ std::string parse_block(const JsonValue &v, FluidBlockSpec &b) {
     return "";
 }
 
+std::string get_floats(const JsonValue &v, float *out, size_t n) {
+    const std::string len = "an array of length " + std::to_string(n);
+    if (v.kind != JsonValue::Array) return "invalid type: " + json_describe(v) + ", expected " + len;
+    if (v.items.size() != n) return "invalid length " + std::to_string(v.items.size()) + ", expected " + len;
+    for (size_t a = 0; a < n; a++) {
+        std::string e = get_f32(v.items[a], out[a]);
+        if (!e.empty()) return e;
+    }
+    return "";
+}
+
+// Option<String>: null or a string.
+std::string get_opt_string(const JsonValue &v) {
+    if (v.kind == JsonValue::Null || v.kind == JsonValue::String) return "";
+    return "invalid type: " + json_describe(v) + ", expected a string";
+}
+
+// The rendering material of a fluid (Material, crates/pxl-scene/src/lib.rs:365-377): parsed
+// with the reference's field names and types so that a block the reference refuses is refused
+// here too; its values do not enter the simulation.
+std::string parse_material(const JsonValue &v) {
+    if (v.kind != JsonValue::Object) return "invalid type: " + json_describe(v) + ", expected struct Material";
+    std::set<std::string> seen;
+    for (const auto &m : v.members) {
+        const std::string &k = m.first;
+        const JsonValue &x = m.second;
+        if (!known(kMaterialFields, k)) return "unknown field `" + k + "`, expected one of " + expected_list(kMaterialFields);
+        if (!seen.insert(k).second) return "duplicate field `" + k + "`";
+        std::string e;
+        if (k == "program") {
+            e = get_opt_string(x);
+        } else if (k == "role") {
+            if (x.kind != JsonValue::String) e = "invalid type: " + json_describe(x) + ", expected enum Role";
+            else if (!known(kRoles, x.text))
+                e = "unknown variant `" + x.text + "`, expected one of " + expected_list(kRoles);
+        } else {  // inputs (MaterialInputs, lib.rs:400-444)
+            if (x.kind != JsonValue::Object) return "invalid type: " + json_describe(x) + ", expected struct MaterialInputs";
+            std::set<std::string> iseen;
+            for (const auto &im : x.members) {
+                const std::string &ik = im.first;
+                const JsonValue &iv = im.second;
+                if (!known(kMaterialInputFields, ik))
+                    return "unknown field `" + ik + "`, expected one of " + expected_list(kMaterialInputFields);
+                if (!iseen.insert(ik).second) return "duplicate field `" + ik + "`";
+                float tmp[3];
+                const size_t len = ik.size();
+                if (len > 8 && ik.compare(len - 8, 8, "_texture") == 0) e = get_opt_string(iv);
+                else if (ik == "base_color" || ik == "emission_color" || ik == "transmission_color")
+                    e = get_floats(iv, tmp, 3);
+                else if (ik == "uv_scale") e = get_floats(iv, tmp, 2);
+                else e = get_f32(iv, tmp[0]);
+                if (!e.empty()) return e;
+            }
+        }
+        if (!e.empty()) return e;
+    }
+    return "";
+}
+
 bool all_gt(const float a[3], const float b[3]) { return a[0] > b[0] && a[1] > b[1] && a[2] > b[2]; }
 bool any_lt(const float a[3], const float b[3]) { return a[0] < b[0] || a[1] < b[1] || a[2] < b[2]; }
 
@@ -200,15 +266,14 @@ std::string parse_fluid(const std::string &json, FluidSpec &f) {
                 }
             }
         } else if (k == "materials") {
-            // Rendering materials (OpenPBR inputs) do not enter the simulation; the list is
-            // accepted as a sequence of objects and otherwise not interpreted.
+            // Rendering materials (OpenPBR inputs): checked field by field like the reference's
+            // Material; they do not enter the simulation.
             if (v.kind != JsonValue::Array) e = "invalid type: " + json_describe(v) + ", expected a sequence";
             else {
-                for (const auto &item : v.items)
-                    if (item.kind != JsonValue::Object) {
-                        e = "invalid type: " + json_describe(item) + ", expected struct Material";
-                        break;
-                    }
+                for (const auto &item : v.items) {
+                    e = parse_material(item);
+                    if (!e.empty()) break;
+                }
                 f.n_materials = (uint32_t)v.items.size();
             }
         }
@@ -358,14 +423,16 @@ uint64_t seed_particles(const FluidSpec &f, uint32_t seed, std::vector<float> *x
                     }
                     if (!obst.empty() && obst[(size_t)i + (size_t)n[0] * ((size_t)j + (size_t)n[1] * (size_t)k)])
                         continue;  // extension: no liquid inside obstacles
+                    if (!x) {  // counting only: the draws decide positions, not the count
+                        count += f.particles_per_cell;
+                        continue;
+                    }
                     for (uint32_t q = 0; q < f.particles_per_cell; q++) {
                         float px = o[0] + ((float)i + lcg_next(rng)) * h;
                         float py = o[1] + ((float)j + lcg_next(rng)) * h;
                         float pz = o[2] + ((float)k + lcg_next(rng)) * h;
-                        if (x) {
-                            xs[0].push_back(px); xs[1].push_back(py); xs[2].push_back(pz);
-                            for (int a = 0; a < 3; a++) vs[a].push_back(b.velocity[a]);
-                        }
+                        xs[0].push_back(px); xs[1].push_back(py); xs[2].push_back(pz);
+                        for (int a = 0; a < 3; a++) vs[a].push_back(b.velocity[a]);
                         count++;
                     }
                 }

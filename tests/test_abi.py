@@ -253,3 +253,50 @@ def test_batch_grid_settings(pxl, tmp_path):
     finally:
         pxl.set_batch_grids(saved)
     assert pxl.batch_grids() == saved
+
+
+def test_material_fields_are_strict(pxl):
+    """A fluid's `materials` follow the reference's Material / MaterialInputs
+    (crates/pxl-scene/src/lib.rs:365-444, deny_unknown_fields): known fields with their types are
+    accepted, anything else is refused with serde's wording."""
+    base = dict(dam_break(24))
+    ok = [{"inputs": {"base_color": [0.1, 0.2, 0.3], "base_color_texture": "water.png", "specular_roughness": 0,
+                      "normal_texture": None, "uv_scale": [2, 2], "transmission_weight": 1, "specular_ior": 1.33,
+                      "transmission_color": [0.45, 0.85, 0.92], "transmission_depth": 0.8, "emission_luminance": 0,
+                      "normal_scale": 1, "base_metalness": 0, "emission_color": [1, 1, 1]},
+           "program": "wet", "role": "shadow_catcher"}, {}, {"role": "holdout", "program": None}]
+    assert pxl.check(dict(base, materials=ok))["n_particles"] == pxl.check(base)["n_particles"]
+    cases = [
+        ([{"inputs": {"transmision_weight": 1}}], "unknown field `transmision_weight`, expected one of `base_color`"),
+        ([{"colour": [1, 1, 1]}], "unknown field `colour`, expected one of `inputs`, `program`, `role`"),
+        ([{"role": "glass"}], "unknown variant `glass`, expected one of `surface`, `shadow_catcher`, `holdout`"),
+        ([{"role": 1}], "invalid type: integer `1`, expected enum Role"),
+        ([{"inputs": {"base_color": [1, 1]}}], "invalid length 2, expected an array of length 3"),
+        ([{"inputs": {"uv_scale": [1, 1, 1]}}], "invalid length 3, expected an array of length 2"),
+        ([{"inputs": {"specular_ior": "1.33"}}], "expected f32"),
+        ([{"inputs": {"normal_texture": 3}}], "expected a string"),
+        ([{"program": 7}], "expected a string"),
+        ([{"inputs": []}], "expected struct MaterialInputs"),
+        (["water"], "expected struct Material"),
+    ]
+    for mats, msg in cases:
+        e = _err(pxl, dict(base, materials=mats))
+        assert msg in str(e), (mats, str(e))
+        assert str(e).startswith("invalid fluid: "), str(e)
+
+
+def test_batch_rejects_members_beyond_the_work_list_field(pxl):
+    """A batch work-list entry keeps a member's workgroup in 20 bits, so a member needing 2^20
+    or more workgroups of 256 particles (over 268,435,456 particles: a full 250^3 grid at 20
+    particles per cell, inside the reference's 16M-cell limit) is refused before any device
+    work, instead of overflowing into the member bits."""
+    big = {"name": "big", "bounds": [[0, 0, 0], [1, 1, 1]], "resolution": 250,
+           "fill": [{"min": [0, 0, 0], "max": [1, 1, 1]}], "particles_per_cell": 20}
+    info = pxl.check(big)
+    assert info["n_particles"] == 248 ** 3 * 20 > 1 << 28
+    with pytest.raises(pxl.FluidError) as e:
+        pxl.Batch([big])
+    assert e.value.code == pxl.flip.ERR_INVALID
+    assert "too large for batched execution" in str(e.value) and "1048576" in str(e.value)
+    ok = dict(big, particles_per_cell=16)  # 244M particles: 953,312 workgroups fit
+    assert pxl.check(ok)["n_particles"] < 1 << 28

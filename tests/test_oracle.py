@@ -572,6 +572,59 @@ def test_g2p_uniform_field(orc):
     np.testing.assert_allclose(sim.x, x + dt * c, rtol=0, atol=1e-6)
 
 
+def test_g2p_linear_field_rk2_midpoint(orc):
+    """A linear grid field u = a (x - x0), v = b (y - y0), w = 0 (vel_old = vel, flip = 0), which
+    trilinear sampling reproduces away from the walls: v' = PIC = the field at x, and the RK2
+    midpoint step gives x' - x0 = (x - x0)(1 + a dt + (a dt)^2 / 2) exactly (y likewise with b).
+    An Euler step (sampling at x instead of the midpoint) misses the (a dt)^2 / 2 term, which
+    here is ~700 f32 ulps of the position: the test tells the two apart."""
+    n, h, dt = (10, 10, 10), np.float32(0.1), np.float32(0.01)
+    a, b, x0, y0 = 3.0, -2.0, 0.45, 0.55
+    rng = np.random.default_rng(5)
+    x = rng.uniform(0.3, 0.7, size=(200, 3)).astype(np.float32)
+    sim = orc.Sim(n, h, [0, 0, 0], dt, 0.0, [0, 0, 0], x, np.zeros_like(x))
+    fu, fv, fw = _face_positions(n, float(h), [0, 0, 0])
+    field = np.concatenate([a * (fu[:, 0] - x0), b * (fv[:, 1] - y0), np.zeros(len(fw))]).astype(np.float32)
+    sim.vel[:] = field
+    sim.vel_old[:] = field
+    sim.g2p_advect()
+    X = x.astype(np.float64)
+    d = float(dt)
+    np.testing.assert_allclose(sim.vx, a * (X[:, 0] - x0), atol=2e-6)
+    np.testing.assert_allclose(sim.vy, b * (X[:, 1] - y0), atol=2e-6)
+    assert np.abs(sim.vz).max() < 1e-6
+    rk2x = x0 + (X[:, 0] - x0) * (1 + a * d + (a * d) ** 2 / 2)
+    rk2y = y0 + (X[:, 1] - y0) * (1 + b * d + (b * d) ** 2 / 2)
+    np.testing.assert_allclose(sim.px, rk2x, rtol=0, atol=1e-6)
+    np.testing.assert_allclose(sim.py, rk2y, rtol=0, atol=1e-6)
+    np.testing.assert_array_equal(sim.pz, x[:, 2])
+    euler = x0 + (X[:, 0] - x0) * (1 + a * d)
+    far = np.abs(X[:, 0] - x0) > 0.05
+    assert np.all(np.abs(sim.px[far] - euler[far]) > 2e-5)
+
+
+def test_g2p_clamps_the_stencil_where_p2g_skips(orc):
+    """Near the low x wall the v- and w-face stencils reach face column -1: sample_face clamps
+    that corner onto column 0 (fluid.slang:75), where p2g skips it (fluid.slang:105).  A
+    constant field c on the v faces is therefore sampled as c there (weights 0.3 + 0.7 on column
+    0, to f32 rounding), not 0.7 c; the same particle's P2G keeps only the 0.7 in-range weight."""
+    n, h, dt = (6, 6, 6), np.float32(0.1), np.float32(0.001)
+    x = np.float32([[0.02, 0.35, 0.33]])
+    c = np.float32(0.8)
+    sim = orc.Sim(n, h, [0, 0, 0], dt, 0.0, [0, 0, 0], x, np.zeros((1, 3), np.float32))
+    nu, nv = 7 * 6 * 6, 6 * 7 * 6
+    sim.vel[:] = 0
+    sim.vel[nu:nu + nv] = c
+    sim.vel_old[:] = sim.vel
+    sim.g2p_advect()
+    assert abs(float(sim.vy[0]) - float(c)) < 1e-6  # f32 rounding of the weight sums; skipping gives 0.56
+    assert sim.vx[0] == 0 and sim.vz[0] == 0
+    sim.set_particles(x, np.ones((1, 3), np.float32))
+    sim.clear_grid()
+    sim.p2g()
+    assert abs(sim.wacc[nu:nu + nv].sum() / 65536 - 0.7) < 1e-3
+
+
 def test_g2p_wall_clamp(orc):
     n, h, dt = (8, 8, 8), np.float32(0.125), np.float32(0.1)
     x = np.float32([[0.2, 0.5, 0.5], [0.85, 0.5, 0.5]])
