@@ -366,51 +366,73 @@ def cpu_dam(args, fluid):
             "sample": f"first {n} steps of the 128^3 dam break ({osim.P} particles), FP32 CG 80, C oracle, one thread"}
 
 
+def _stream(pxl, srv, reqs, arrivals, steps):
+    """Submits reqs[i] (seed i) at arrivals[i] seconds after the start and waits for all of them;
+    returns (wall seconds from the first submission to the last completion seen by drain,
+    request ids, server stats)."""
+    t0 = time.perf_counter()
+    ids = []
+    for i, f in enumerate(reqs):
+        delay = arrivals[i] - (time.perf_counter() - t0)
+        if delay > 0:
+            time.sleep(delay)
+        ids.append(srv.submit(f, FPS_DEFAULT, seed=i, steps=steps))
+    srv.drain()
+    return time.perf_counter() - t0, ids, srv.stats()
+
+
+FPS_DEFAULT = 24.0
+REQUEST_STEPS = 60  # steps per serving request (the serving configuration)
+
+
 def bench_serve(args, pxl):
+    """The serving configuration: 512 requests of 60 steps each.
+
+    value: capacity -- the whole stream submitted at once (offered load above what the server
+    can take, so the device never waits for arrivals), particle-steps over the time from the
+    first submission to the last completion; the timed region is that run, and `steps` is the
+    number of batched steps the server ran in it (ms_per_step x steps = the timed region).
+    Latency: the same stream again with Poisson arrivals at --load (0.7) of that capacity, in
+    requests per second of the stream's mean work; p50/p99 are submit -> completion there.
+    plain: every --plain-every-th request alone on the single-simulation FP32 CG-80 path,
+    creation included.  roofline: two event-timed batched steps of the batch the capacity run
+    held at its start (the requests it admitted at once)."""
     import numpy as np
     from tests.scenes import FPS, random_request
 
     sizes = tuple(int(v) for v in args.sizes.split(","))
     reqs = [random_request(i, sizes=sizes) for i in range(args.requests)]
-    opts = dict(budget_bytes=args.budget)
+    opts = dict(budget_bytes=args.budget, max_sims=args.max_sims)
     infos = [pxl.check(f, FPS, seed=i, solver="mixed") for i, f in enumerate(reqs)]
     P = np.array([inf["n_particles"] for inf in infos], dtype=np.float64)
-    work = P * args.steps
+    nbytes = np.array([inf["device_bytes"] for inf in infos], dtype=np.float64)
+    work = P * REQUEST_STEPS
 
-    # warm-up (untimed): device init and kernels through a short server run of other requests
-    # from the same distribution, then the steady-state service rate of a fixed batch of the
-    # stream's first requests
+    # warm-up (untimed): device initialisation, kernel loading and graph capture on a few
+    # requests of the same distribution, --warmup batched steps
+    with pxl.Batch([random_request(100000 + i, sizes=sizes) for i in range(8)], FPS,
+                   seeds=list(range(8))) as bat:
+        bat.step(max(args.warmup, 1))
+
+    # capacity (timed): every request submitted at once
     with pxl.Server(**opts) as srv:
-        for i in range(8):
-            srv.submit(random_request(100000 + i, sizes=sizes), FPS, seed=i, steps=2)
-        srv.drain()
-    warm = reqs[:args.warmup]
-    with pxl.Batch(warm, FPS, seeds=list(range(len(warm)))) as bat:
-        bat.step(2)
-        t0 = time.perf_counter()
-        bat.step(args.calib_steps)
-        warm_rate = sum(i["n_particles"] for i in bat.infos) * args.calib_steps / (time.perf_counter() - t0)
-    rate = args.rate if args.rate > 0 else warm_rate / max(work.mean(), 1.0)
+        wall, ids, st = _stream(pxl, srv, reqs, np.zeros(len(reqs)), REQUEST_STEPS)
+        per = [srv.poll(i) for i in ids]
+    failed = [p for p in per if p["status"] != "done"]
+    value = st["particle_steps"] / wall
+    # the requests the capacity run admitted at once: first in, first out within max_sims and
+    # the budget
+    first = int(min(args.max_sims, np.searchsorted(np.cumsum(nbytes), args.budget, side="right")))
+
+    # latency: Poisson arrivals at --load of the measured capacity
+    rate = args.load * value / work.mean()
     rng = np.random.default_rng(12345)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(reqs)))
     arrivals -= arrivals[0]
-
-    # timed: the stream through the continuous-batching server
     with pxl.Server(**opts) as srv:
-        t0 = time.perf_counter()
-        ids = []
-        for i, f in enumerate(reqs):
-            delay = arrivals[i] - (time.perf_counter() - t0)
-            if delay > 0:
-                time.sleep(delay)
-            ids.append(srv.submit(f, FPS, seed=i, steps=args.steps))
-        srv.drain()
-        wall = time.perf_counter() - t0
-        st = srv.stats()
-        per = [srv.poll(i) for i in ids]
-    failed = [p for p in per if p["status"] != "done"]
-    max_res = max(p["rel_residual"] for p in per)
-    value = st["particle_steps"] / st["span_s"]
+        lwall, lids, lst = _stream(pxl, srv, reqs, arrivals, REQUEST_STEPS)
+        lper = [srv.poll(i) for i in lids]
+    failed += [p for p in lper if p["status"] != "done"]
 
     # plain: a sample of the same requests, one at a time, FP32 CG with 80 iterations
     sample = list(range(0, len(reqs), max(1, args.plain_every)))
@@ -419,9 +441,9 @@ def bench_serve(args, pxl):
         a = time.perf_counter()
         sim = pxl.FluidState(reqs[i], FPS, seed=i, solver="cg", cg_iterations=80)
         b = time.perf_counter()
-        sim.step(args.steps)
+        sim.step(REQUEST_STEPS)
         c = time.perf_counter()
-        pw += sim.particle_count() * args.steps
+        pw += sim.particle_count() * REQUEST_STEPS
         sim.close()
         d = time.perf_counter()
         pt += d - a
@@ -430,28 +452,38 @@ def bench_serve(args, pxl):
              "path": "one simulation at a time, FP32 Jacobi-PCG 80 fixed iterations, hipGraph per step, "
                      "creation included (steps_only: step time alone)"}
 
-    nb = min(args.profile_batch, len(reqs))
-    roofline, kernels, _ = batch_roofline(pxl, reqs[:nb], 2, 0)
+    roofline, kernels, _ = batch_roofline(pxl, reqs[:first], 2, 0)
+    roofline["measured_on"] = (f"two event-timed batched steps of the capacity run's opening batch "
+                               f"(its first {first} requests, admitted together)")
 
     cpu = cpu_serve(args, reqs) if args.cpu_seconds > 0 else None
     return {
-        "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": 1e3 * st["busy_s"] / max(st["batch_steps"], 1),
+        "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": st["batch_steps"],
+        "warmup": args.warmup, "ms_per_step": 1e3 * wall / max(st["batch_steps"], 1),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (random liquid boxes or spheres, reference LCG seeding, seed = request index)",
         "config": {
             "workload": f"serving stream: {len(reqs)} requests, cubic grids n in {{{args.sizes}}}, one random "
-                        f"liquid box or sphere (even odds) at 4-8 particles/cell, {args.steps} steps each, Poisson arrivals, "
-                        f"{args.budget / 1e9:.0f} GB admission budget, MIXED solve to 1e-6",
+                        f"liquid box or sphere (even odds) at 4-8 particles/cell, {REQUEST_STEPS} steps each, "
+                        f"{args.budget / 1e9:.0f} GB admission budget, at most {args.max_sims} in the batch, "
+                        f"MIXED solve to 1e-6",
+            "steps_note": (f"steps = batched steps of the timed capacity run (--steps {args.steps} does not apply: "
+                           f"each request is {REQUEST_STEPS} steps)"),
             "requests": len(reqs), "sphere_fills": sum("center" in f["fill"][0] for f in reqs),
-            "particles_total": float(P.sum()), "rate_per_s": rate,
-            "rate_rule": (f"steady-state service rate of a fixed batch of the first {len(warm)} requests "
-                          f"({warm_rate:.4g} particle-steps/s) / mean request work") if args.rate <= 0 else "--rate",
-            "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
-            "busy_fraction": st["busy_s"] / st["span_s"] if st["span_s"] else None,
-            "mean_batch": st["mean_batch"], "batch_steps": st["batch_steps"], "admit_s": st["admit_s"],
-            "graph_captures": st["graph_captures"], "capture_s": st["capture_s"], "retire_s": st["retire_s"], "peak_bytes": st["peak_bytes"], "wall_s": wall,
-            "failed": len(failed), "max_rel_residual": max_res,
+            "particles_total": float(P.sum()),
+            "capacity": {"offered": "all requests at t = 0", "wall_s": wall, "span_s": st["span_s"],
+                         "mean_batch": st["mean_batch"], "batch_steps": st["batch_steps"],
+                         "busy_fraction": st["busy_s"] / wall, "admit_s": st["admit_s"],
+                         "graph_captures": st["graph_captures"], "peak_bytes": st["peak_bytes"],
+                         "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"]},
+            "latency": {"arrivals": "Poisson", "load": args.load, "rate_per_s": rate,
+                        "rate_rule": f"{args.load} x capacity / mean request work ({work.mean():.4g} particle-steps)",
+                        "p50_ms": lst["p50_ms"], "p99_ms": lst["p99_ms"], "mean_ms": lst["mean_ms"],
+                        "max_ms": lst["max_ms"], "mean_batch": lst["mean_batch"],
+                        "throughput": lst["particle_steps"] / lwall, "wall_s": lwall,
+                        "busy_fraction": lst["busy_s"] / lwall},
+            "p50_ms": lst["p50_ms"], "p99_ms": lst["p99_ms"],
+            "failed": len(failed), "max_rel_residual": max(p["rel_residual"] for p in per + lper),
             "plain": plain, "ratio_vs_plain": value / plain["value"],
         },
         "roofline": roofline,
@@ -530,10 +562,10 @@ def bench_tune(args, pxl):
 
 
 def cpu_serve(args, reqs):
-    """The C oracle (plain FP32 CG 80) on the stream's first --cpu-requests requests, one step
-    each, the simulations spread over --cpu-threads host threads (each simulation on one
-    thread; the oracle releases the interpreter lock while it runs).  Oracle construction and
-    seeding are not timed."""
+    """The C oracle in reference mode (the reference's red-black Gauss-Seidel, 80 sweeps) on the
+    stream's first --cpu-requests requests, one step each, the simulations spread over
+    --cpu-threads host threads (each simulation on one thread; the oracle releases the
+    interpreter lock while it runs).  Oracle construction and seeding are not timed."""
     import oracle
     from concurrent.futures import ThreadPoolExecutor
     from tests.scenes import oracle_sim
@@ -544,31 +576,30 @@ def cpu_serve(args, reqs):
     sims.sort(key=lambda s: -s.P)  # largest first: the threads finish together
     t0 = time.perf_counter()
     with ThreadPoolExecutor(threads) as ex:
-        list(ex.map(lambda s: s.step_cg(80), sims))
+        list(ex.map(lambda s: s.step_rbgs(80), sims))
     wall = time.perf_counter() - t0
     ps = sum(s.P for s in sims)
     return {"value": ps / wall, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": f"one FP32 CG-80 step of each of the first {len(sims)} stream requests ({ps} particle-steps), "
-                      f"C oracle, {threads} host threads, one simulation per thread"}
+            "sample": f"one reference-mode (RBGS-80) step of each of the first {len(sims)} stream requests "
+                      f"({ps} particle-steps), C oracle, {threads} host threads, one simulation per thread"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
-                    help="steps per request (default 60); dam, mixed256, obstacles: timed steps (default 60, "
-                         "obstacles 200)")
-    ap.add_argument("--warmup", type=int, default=64,
-                    help="calibration batch: the stream's first N requests, whose service rate sets the arrival "
-                         "rate (dam: warm-up steps)")
-    ap.add_argument("--calib-steps", type=int, default=10, help="timed steps of the warm-up batch")
+                    help="dam, mixed256, obstacles, batch: timed steps (default 60, obstacles 200); serve: not used "
+                         "(each request is 60 steps)")
+    ap.add_argument("--warmup", type=int, default=5, help="untimed warm-up steps")
     ap.add_argument("--workload", default="serve",
                     choices=["serve", "dam", "batch", "mixed256", "obstacles", "frame", "reference64", "agents", "tune"])
     ap.add_argument("--grids", default=None, help="JSON cache of batched launch grids to load first (autotuner output)")
     ap.add_argument("--grids-out", default=None, help="tune: write the chosen grids to this JSON file")
     ap.add_argument("--requests", type=int, default=512)
     ap.add_argument("--sizes", default="32,48,64,96,128")
-    ap.add_argument("--rate", type=float, default=0.0, help="arrivals per second (0 = measured service rate)")
+    ap.add_argument("--load", type=float, default=0.7,
+                    help="serve: Poisson arrival rate of the latency run as a fraction of the measured capacity")
+    ap.add_argument("--max-sims", type=int, default=512, help="serve: most simulations in the batch")
     ap.add_argument("--budget", type=float, default=200e9)
     ap.add_argument("--plain-every", type=int, default=8, help="plain path on every k-th request")
     ap.add_argument("--profile-batch", type=int, default=64)

@@ -364,6 +364,11 @@ int pxlflip_batch_profile_steps(pxlflip_batch *b, uint32_t steps, pxlflip_kernel
 int pxlflip_batch_get_info(pxlflip_batch *b, uint32_t index, pxlflip_info *info);
 int pxlflip_batch_read_particles(pxlflip_batch *b, uint32_t index, float *aos8);
 int pxlflip_batch_last_solve(pxlflip_batch *b, uint32_t index, pxlflip_solve_stats *st);
+/* Member `index`'s fields and binning, as pxlflip_read_field / pxlflip_read_bins (the count field
+ * holds the counts of the member's last batched grid phase). */
+int pxlflip_batch_read_field(pxlflip_batch *b, uint32_t index, int field, void *out);
+int pxlflip_batch_read_bins(pxlflip_batch *b, uint32_t index, uint32_t *cell_of, uint32_t *counts, uint32_t *offsets,
+                            uint32_t *sorted_ids);
 const char *pxlflip_batch_last_error(const pxlflip_batch *b);
 void pxlflip_batch_destroy(pxlflip_batch *b);
 

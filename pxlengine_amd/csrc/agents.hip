@@ -132,39 +132,87 @@ __global__ __launch_bounds__(ABLK) void k_grid_counts(const uint32_t *keys, uint
 
 }  // namespace
 
-// Scratch of one neighbour grid: radix ping-pong buffers, tile histograms, scan scratch.
+// Device scratch of one build: radix ping-pong buffers, tile histograms, scan scratch.
+struct GridScratch {
+    uint32_t *keys[2] = {nullptr, nullptr}, *vals[2] = {nullptr, nullptr};
+    uint32_t *hist = nullptr, *scan_tmp = nullptr;
+
+    // Scan scratch covers the larger of the histograms and the largest table (2^23 entries).
+    static uint32_t scan_len(uint32_t max_agents) {
+        const uint32_t hist_len = RADIX * cdiv(max_agents ? max_agents : 1u, RTILE);
+        return std::max<uint32_t>(hist_len, 1u << 23);
+    }
+    // Bytes of each array for up to max_agents agents, in layout order, and their 256-byte
+    // aligned total.
+    static constexpr int PARTS = 6;
+    static void sizes(uint32_t max_agents, size_t out[PARTS]) {
+        const size_t n = std::max<uint32_t>(max_agents, 1u);
+        const size_t hist_len = (size_t)RADIX * cdiv(n, RTILE);
+        const size_t tmp = 4ull * (scan_tiles(scan_len(max_agents)) + 1);
+        const size_t sz[PARTS] = {4 * n, 4 * n, 4 * n, 4 * n, 4 * hist_len, tmp};
+        for (int q = 0; q < PARTS; q++) out[q] = sz[q];
+    }
+    static size_t total(uint32_t max_agents) {
+        size_t sz[PARTS], t = 0;
+        sizes(max_agents, sz);
+        for (size_t b : sz) t += (b + 255) / 256 * 256;
+        return t;
+    }
+    // Lays the arrays out from `base` (256-byte aligned, total(max_agents) bytes).
+    void layout(void *base, uint32_t max_agents) {
+        size_t sz[PARTS];
+        sizes(max_agents, sz);
+        char *cur = static_cast<char *>(base);
+        uint32_t **dst[PARTS] = {&keys[0], &vals[0], &keys[1], &vals[1], &hist, &scan_tmp};
+        for (int q = 0; q < PARTS; q++) {
+            *dst[q] = reinterpret_cast<uint32_t *>(cur);
+            cur += (sz[q] + 255) / 256 * 256;
+        }
+    }
+};
+
+// grid_clear .. grid_scatter (sim.slang:31-104) into device arrays, on `stream`, using `w`.
+void enqueue_grid_build(const GridScratch &w, const float *agents, uint32_t n, float cell_size, uint32_t table_size,
+                        uint32_t *offsets, uint32_t *counts, uint32_t *sorted, hipStream_t stream) {
+    const float cell = cell_size > 1e-6f ? cell_size : 1e-6f;  // sim.rs:182
+    launch_fill_u32(counts, table_size, 0u, stream);
+    if (n) {
+        PXL_LAUNCH(k_grid_keys, cdiv(n, ABLK), ABLK, 0, stream, agents, n, cell, table_size, w.keys[0], w.vals[0]);
+        uint32_t bits = 0;
+        while ((1u << bits) < table_size) bits++;
+        const uint32_t passes = std::max<uint32_t>(1u, (bits + 7) / 8), ntiles = cdiv(n, RTILE);
+        int cur = 0;
+        for (uint32_t p = 0; p < passes; p++) {
+            const uint32_t shift = 8 * p;
+            PXL_LAUNCH(k_radix_hist, ntiles, ABLK, 0, stream, w.keys[cur], n, shift, ntiles, w.hist);
+            launch_exclusive_scan(w.hist, RADIX * ntiles, w.scan_tmp, w.hist, stream);
+            uint32_t *vout = p + 1 == passes ? sorted : w.vals[cur ^ 1];
+            PXL_LAUNCH(k_radix_scatter, ntiles, ABLK, 0, stream, w.keys[cur], w.vals[cur], n, shift, ntiles, w.hist,
+                       w.keys[cur ^ 1], vout);
+            cur ^= 1;
+        }
+        PXL_LAUNCH(k_grid_counts, cdiv(n, ABLK), ABLK, 0, stream, w.keys[cur], n, counts);
+    }
+    launch_exclusive_scan(counts, table_size, w.scan_tmp, offsets, stream);
+}
+
+// One neighbour grid handle: its own stream, events and scratch for up to max_agents agents.
 struct AgentGrid {
     int device = 0;
     hipStream_t stream = nullptr;
     uint32_t max_agents = 0;
     void *arena = nullptr;
-    uint32_t *keys[2] = {nullptr, nullptr}, *vals[2] = {nullptr, nullptr};
-    uint32_t *hist = nullptr, *scan_tmp = nullptr;
+    GridScratch w;
     hipEvent_t ev[2] = {nullptr, nullptr};
     float last_ms = 0.0f;
     std::string error;
 
-    static uint32_t scan_len(uint32_t max_agents) {
-        const uint32_t hist_len = RADIX * cdiv(max_agents ? max_agents : 1u, RTILE);
-        return std::max<uint32_t>(hist_len, 1u << 23);
-    }
     void allocate() {
-        const size_t n = std::max<uint32_t>(max_agents, 1u);
-        const size_t hist_len = (size_t)RADIX * cdiv(n, RTILE);
-        const size_t tmp = 4ull * (scan_tiles(scan_len(max_agents)) + 1);
-        const size_t sizes[6] = {4 * n, 4 * n, 4 * n, 4 * n, 4 * hist_len, tmp};
-        size_t total = 0;
-        for (size_t b : sizes) total += (b + 255) / 256 * 256;
-        if (hipMalloc(&arena, total) != hipSuccess) {
+        if (hipMalloc(&arena, GridScratch::total(max_agents)) != hipSuccess) {
             arena = nullptr;
             throw std::bad_alloc();
         }
-        char *cur = static_cast<char *>(arena);
-        uint32_t **dst[6] = {&keys[0], &vals[0], &keys[1], &vals[1], &hist, &scan_tmp};
-        for (int q = 0; q < 6; q++) {
-            *dst[q] = reinterpret_cast<uint32_t *>(cur);
-            cur += (sizes[q] + 255) / 256 * 256;
-        }
+        w.layout(arena, max_agents);
     }
     ~AgentGrid() {
         for (hipEvent_t e : ev)
@@ -173,30 +221,10 @@ struct AgentGrid {
         if (stream) (void)hipStreamDestroy(stream);
     }
 
-    // grid_clear .. grid_scatter (sim.slang:31-104) into device arrays.
     void build(const float *agents, uint32_t n, float cell_size, uint32_t table_size, uint32_t *offsets,
                uint32_t *counts, uint32_t *sorted) {
-        const float cell = cell_size > 1e-6f ? cell_size : 1e-6f;  // sim.rs:182
         HIP_CHECK(hipEventRecord(ev[0], stream));
-        launch_fill_u32(counts, table_size, 0u, stream);
-        if (n) {
-            PXL_LAUNCH(k_grid_keys, cdiv(n, ABLK), ABLK, 0, stream, agents, n, cell, table_size, keys[0], vals[0]);
-            uint32_t bits = 0;
-            while ((1u << bits) < table_size) bits++;
-            const uint32_t passes = std::max<uint32_t>(1u, (bits + 7) / 8), ntiles = cdiv(n, RTILE);
-            int cur = 0;
-            for (uint32_t p = 0; p < passes; p++) {
-                const uint32_t shift = 8 * p;
-                PXL_LAUNCH(k_radix_hist, ntiles, ABLK, 0, stream, keys[cur], n, shift, ntiles, hist);
-                launch_exclusive_scan(hist, RADIX * ntiles, scan_tmp, hist, stream);
-                uint32_t *vout = p + 1 == passes ? sorted : vals[cur ^ 1];
-                PXL_LAUNCH(k_radix_scatter, ntiles, ABLK, 0, stream, keys[cur], vals[cur], n, shift, ntiles, hist,
-                           keys[cur ^ 1], vout);
-                cur ^= 1;
-            }
-            PXL_LAUNCH(k_grid_counts, cdiv(n, ABLK), ABLK, 0, stream, keys[cur], n, counts);
-        }
-        launch_exclusive_scan(counts, table_size, scan_tmp, offsets, stream);
+        enqueue_grid_build(w, agents, n, cell_size, table_size, offsets, counts, sorted, stream);
         HIP_CHECK(hipEventRecord(ev[1], stream));
         HIP_CHECK(hipStreamSynchronize(stream));
         HIP_CHECK(hipEventElapsedTime(&last_ms, ev[0], ev[1]));

@@ -226,6 +226,81 @@ void fill_info(const Sim &s, pxlflip_info *info) {
     info->device_bytes = s.arena_bytes;
 }
 
+// Why `field` cannot be read from this simulation ("" when it can).
+std::string field_check(const Sim &s, int field) {
+    if (field < PXLFLIP_FIELD_VEL || field > PXLFLIP_FIELD_CELL_VERT) return "unknown field " + std::to_string(field);
+    if (field == PXLFLIP_FIELD_PRESSURE64 && s.opts.solver != PXLFLIP_SOLVER_MIXED)
+        return "PRESSURE64 exists only with the MIXED solver";
+    if (field >= PXLFLIP_FIELD_DENSITY && !s.surf_arena) return "no surface pass has run on this handle";
+    return "";
+}
+
+// Copies one field (pxlflip.h's PXLFLIP_FIELD_*, checked by field_check) in the reference
+// layout into host memory, on the simulation's stream; returns when it is there.
+void read_field_to_host(Sim &s, int field, void *out) {
+    const DevSim &d = s.d;
+    const size_t F = d.nfaces, C = d.ncells;
+    switch (field) {
+        case PXLFLIP_FIELD_ACC:
+        case PXLFLIP_FIELD_WACC: {
+            DeviceBuffer a(4 * F), w(4 * F);
+            launch_acc_fields(d, a.as<int32_t>(), w.as<int32_t>(), s.stream);
+            HIP_CHECK(hipMemcpyAsync(out, field == PXLFLIP_FIELD_ACC ? a.p : w.p, 4 * F, hipMemcpyDeviceToHost, s.stream));
+            HIP_CHECK(hipStreamSynchronize(s.stream));
+            return;
+        }
+        case PXLFLIP_FIELD_TYPE:
+        case PXLFLIP_FIELD_COUNT: {
+            DeviceBuffer buf(4 * C);
+            if (field == PXLFLIP_FIELD_TYPE) launch_type_u32(d, buf.as<uint32_t>(), s.stream);
+            else launch_count_linear(d, buf.as<uint32_t>(), s.stream);
+            HIP_CHECK(hipMemcpyAsync(out, buf.p, 4 * C, hipMemcpyDeviceToHost, s.stream));
+            HIP_CHECK(hipStreamSynchronize(s.stream));
+            return;
+        }
+        default: break;
+    }
+    const void *src = nullptr;
+    size_t bytes = 0;
+    switch (field) {
+        case PXLFLIP_FIELD_VEL: src = d.vel; bytes = 4 * F; break;
+        case PXLFLIP_FIELD_VEL_OLD: src = d.vel_old; bytes = 4 * F; break;
+        case PXLFLIP_FIELD_PRESSURE: src = d.pressure; bytes = 4 * C; break;
+        case PXLFLIP_FIELD_DIVERGENCE: src = d.div; bytes = 4 * C; break;
+        case PXLFLIP_FIELD_PRESSURE64: src = s.x64(s.x64_cur); bytes = 8 * C; break;
+        case PXLFLIP_FIELD_DENSITY: src = s.surf.density; bytes = 4ull * s.surf.nnodes; break;
+        case PXLFLIP_FIELD_PHI: src = s.surf.phi; bytes = 4ull * s.surf.nnodes; break;
+        default: src = s.surf.cell_vert; bytes = 4ull * s.surf.ncells; break;
+    }
+    HIP_CHECK(hipMemcpyAsync(out, src, bytes, hipMemcpyDeviceToHost, s.stream));
+    HIP_CHECK(hipStreamSynchronize(s.stream));
+}
+
+// The counting sort of the current positions in reference cell numbering (pxlflip_read_bins);
+// any output may be NULL.  Leaves every count zero, as the next step expects.
+void read_bins_to_host(Sim &s, uint32_t *cell_of, uint32_t *counts, uint32_t *offsets, uint32_t *sorted_ids) {
+    const DevSim &d = s.d;
+    const size_t C = d.ncells;
+    DeviceBuffer cbuf(4ull * d.np), sbuf(4ull * d.np), cnt(4 * C), off(4 * C),
+        tmp(4ull * (scan_tiles((uint32_t)C) + 1));
+    // the step's binning (tiled keys) on the current positions ...
+    launch_fill_u32(d.count, d.nkeys, 0, s.stream);
+    launch_bin_count(d, s.stream);
+    // ... reported in reference cell numbering
+    launch_count_linear(d, cnt.as<uint32_t>(), s.stream);
+    // the step clears only the counts of the tiles its last tile list holds (bin_count above
+    // emptied that list), so every count must be zero again before the next step
+    launch_fill_u32(d.count, d.nkeys, 0, s.stream);
+    launch_exclusive_scan(cnt.as<uint32_t>(), (uint32_t)C, tmp.as<uint32_t>(), off.as<uint32_t>(), s.stream);
+    launch_bins_out(d, off.as<uint32_t>(), cbuf.as<uint32_t>(), sbuf.as<uint32_t>(), s.stream);
+    launch_sort_segments((uint32_t)C, off.as<uint32_t>(), cnt.as<uint32_t>(), sbuf.as<uint32_t>(), s.stream);
+    if (cell_of) HIP_CHECK(hipMemcpyAsync(cell_of, cbuf.p, 4ull * d.np, hipMemcpyDeviceToHost, s.stream));
+    if (counts) HIP_CHECK(hipMemcpyAsync(counts, cnt.p, 4 * C, hipMemcpyDeviceToHost, s.stream));
+    if (offsets) HIP_CHECK(hipMemcpyAsync(offsets, off.p, 4 * C, hipMemcpyDeviceToHost, s.stream));
+    if (sorted_ids) HIP_CHECK(hipMemcpyAsync(sorted_ids, sbuf.p, 4ull * d.np, hipMemcpyDeviceToHost, s.stream));
+    HIP_CHECK(hipStreamSynchronize(s.stream));
+}
+
 }  // namespace pxl
 
 extern "C" {
@@ -380,50 +455,9 @@ int pxlflip_write_particles(pxlflip *h, const float *aos8) {
 int pxlflip_read_field(pxlflip *h, int field, void *out) {
     if (!h || !out) return PXLFLIP_ERR_INVALID;
     Sim *s = &h->sim;
-    const pxl::DevSim &d = s->d;
-    const void *src = nullptr;
-    size_t bytes = 0;
-    switch (field) {
-        case PXLFLIP_FIELD_VEL: src = d.vel; bytes = 4ull * d.nfaces; break;
-        case PXLFLIP_FIELD_VEL_OLD: src = d.vel_old; bytes = 4ull * d.nfaces; break;
-        case PXLFLIP_FIELD_ACC:
-        case PXLFLIP_FIELD_WACC: bytes = 4ull * d.nfaces; break;
-        case PXLFLIP_FIELD_COUNT: bytes = 4ull * d.ncells; break;
-        case PXLFLIP_FIELD_TYPE: bytes = 4ull * d.ncells; break;
-        case PXLFLIP_FIELD_PRESSURE: src = d.pressure; bytes = 4ull * d.ncells; break;
-        case PXLFLIP_FIELD_DIVERGENCE: src = d.div; bytes = 4ull * d.ncells; break;
-        case PXLFLIP_FIELD_PRESSURE64:
-            if (s->opts.solver != PXLFLIP_SOLVER_MIXED)
-                return pxl::fail(s, PXLFLIP_ERR_INVALID, "PRESSURE64 exists only with the MIXED solver");
-            bytes = 8ull * d.ncells;
-            break;
-        case PXLFLIP_FIELD_DENSITY:
-        case PXLFLIP_FIELD_PHI:
-        case PXLFLIP_FIELD_CELL_VERT:
-            if (!s->surf_arena) return pxl::fail(s, PXLFLIP_ERR_INVALID, "no surface pass has run on this handle");
-            src = field == PXLFLIP_FIELD_DENSITY ? (const void *)s->surf.density
-                  : field == PXLFLIP_FIELD_PHI   ? (const void *)s->surf.phi
-                                                 : (const void *)s->surf.cell_vert;
-            bytes = 4ull * (field == PXLFLIP_FIELD_CELL_VERT ? s->surf.ncells : s->surf.nnodes);
-            break;
-        default: return pxl::fail(s, PXLFLIP_ERR_INVALID, "unknown field " + std::to_string(field));
-    }
-    return pxl::guarded(s, [&] {
-        if (field == PXLFLIP_FIELD_ACC || field == PXLFLIP_FIELD_WACC) {
-            pxl::DeviceBuffer a(bytes), w(bytes);
-            pxl::launch_acc_fields(d, a.as<int32_t>(), w.as<int32_t>(), s->stream);
-            HIP_CHECK(hipMemcpyAsync(out, field == PXLFLIP_FIELD_ACC ? a.p : w.p, bytes, hipMemcpyDeviceToHost, s->stream));
-        } else if (field == PXLFLIP_FIELD_TYPE || field == PXLFLIP_FIELD_COUNT) {
-            pxl::DeviceBuffer buf(bytes);
-            if (field == PXLFLIP_FIELD_TYPE) pxl::launch_type_u32(d, buf.as<uint32_t>(), s->stream);
-            else pxl::launch_count_linear(d, buf.as<uint32_t>(), s->stream);
-            HIP_CHECK(hipMemcpyAsync(out, buf.p, bytes, hipMemcpyDeviceToHost, s->stream));
-        } else {
-            if (field == PXLFLIP_FIELD_PRESSURE64) src = s->x64(s->x64_cur);
-            HIP_CHECK(hipMemcpyAsync(out, src, bytes, hipMemcpyDeviceToHost, s->stream));
-        }
-        HIP_CHECK(hipStreamSynchronize(s->stream));
-    });
+    std::string why = pxl::field_check(*s, field);
+    if (!why.empty()) return pxl::fail(s, PXLFLIP_ERR_INVALID, why);
+    return pxl::guarded(s, [&] { pxl::read_field_to_host(*s, field, out); });
 }
 
 namespace {
@@ -563,26 +597,7 @@ int pxlflip_read_bins(pxlflip *h, uint32_t *cell_of, uint32_t *counts, uint32_t 
     Sim *s = &h->sim;
     return pxl::guarded(s, [&] {
         if (!s->initialized) s->reset();
-        const pxl::DevSim &d = s->d;
-        const size_t C = d.ncells;
-        pxl::DeviceBuffer cbuf(4ull * d.np), sbuf(4ull * d.np), cnt(4 * C), off(4 * C),
-            tmp(4ull * (pxl::scan_tiles((uint32_t)C) + 1));
-        // the step's binning (tiled keys) on the current positions ...
-        pxl::launch_fill_u32(d.count, d.nkeys, 0, s->stream);
-        pxl::launch_bin_count(d, s->stream);
-        // ... reported in reference cell numbering
-        pxl::launch_count_linear(d, cnt.as<uint32_t>(), s->stream);
-        // the step clears only the counts of the tiles its last tile list holds (bin_count
-        // above emptied that list), so every count must be zero again before the next step
-        pxl::launch_fill_u32(d.count, d.nkeys, 0, s->stream);
-        pxl::launch_exclusive_scan(cnt.as<uint32_t>(), (uint32_t)C, tmp.as<uint32_t>(), off.as<uint32_t>(), s->stream);
-        pxl::launch_bins_out(d, off.as<uint32_t>(), cbuf.as<uint32_t>(), sbuf.as<uint32_t>(), s->stream);
-        pxl::launch_sort_segments((uint32_t)C, off.as<uint32_t>(), cnt.as<uint32_t>(), sbuf.as<uint32_t>(), s->stream);
-        if (cell_of) HIP_CHECK(hipMemcpyAsync(cell_of, cbuf.p, 4ull * d.np, hipMemcpyDeviceToHost, s->stream));
-        if (counts) HIP_CHECK(hipMemcpyAsync(counts, cnt.p, 4 * C, hipMemcpyDeviceToHost, s->stream));
-        if (offsets) HIP_CHECK(hipMemcpyAsync(offsets, off.p, 4 * C, hipMemcpyDeviceToHost, s->stream));
-        if (sorted_ids) HIP_CHECK(hipMemcpyAsync(sorted_ids, sbuf.p, 4ull * d.np, hipMemcpyDeviceToHost, s->stream));
-        HIP_CHECK(hipStreamSynchronize(s->stream));
+        pxl::read_bins_to_host(*s, cell_of, counts, offsets, sorted_ids);
     });
 }
 

@@ -1433,6 +1433,31 @@ int pxlflip_batch_read_particles(pxlflip_batch *b, uint32_t index, float *aos8) 
     });
 }
 
+int pxlflip_batch_read_field(pxlflip_batch *b, uint32_t index, int field, void *out) {
+    if (!b || !out || index >= b->sims.size()) return PXLFLIP_ERR_INVALID;
+    pxl::Sim &s = *b->sims[index];
+    const std::string why = pxl::field_check(s, field);
+    if (!why.empty()) {
+        b->error = why;
+        return PXLFLIP_ERR_INVALID;
+    }
+    return batch_guarded(b, [&] {
+        b->batch->sync();
+        pxl::read_field_to_host(s, field, out);
+        return PXLFLIP_OK;
+    });
+}
+
+int pxlflip_batch_read_bins(pxlflip_batch *b, uint32_t index, uint32_t *cell_of, uint32_t *counts, uint32_t *offsets,
+                            uint32_t *sorted_ids) {
+    if (!b || index >= b->sims.size()) return PXLFLIP_ERR_INVALID;
+    return batch_guarded(b, [&] {
+        b->batch->sync();
+        pxl::read_bins_to_host(*b->sims[index], cell_of, counts, offsets, sorted_ids);
+        return PXLFLIP_OK;
+    });
+}
+
 int pxlflip_batch_last_solve(pxlflip_batch *b, uint32_t index, pxlflip_solve_stats *st) {
     if (!b || !st || index >= b->sims.size()) return PXLFLIP_ERR_INVALID;
     *st = b->sims[index]->stats;

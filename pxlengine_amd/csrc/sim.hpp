@@ -514,6 +514,11 @@ struct Sim {
     }
 };
 
+// Host read-back shared by the single-simulation and batch entry points (flip.hip).
+std::string field_check(const Sim &s, int field);
+void read_field_to_host(Sim &s, int field, void *out);
+void read_bins_to_host(Sim &s, uint32_t *cell_of, uint32_t *counts, uint32_t *offsets, uint32_t *sorted_ids);
+
 inline int fail(Sim *s, int code, const std::string &msg) {
     if (s) s->error = msg;
     return code;

@@ -329,6 +329,8 @@ _SIGNATURES = {
     "pxlflip_batch_get_info": (C.c_int, [_H, C.c_uint32, C.POINTER(Info)]),
     "pxlflip_batch_read_particles": (C.c_int, [_H, C.c_uint32, C.c_void_p]),
     "pxlflip_batch_last_solve": (C.c_int, [_H, C.c_uint32, C.POINTER(SolveStats)]),
+    "pxlflip_batch_read_field": (C.c_int, [_H, C.c_uint32, C.c_int, C.c_void_p]),
+    "pxlflip_batch_read_bins": (C.c_int, [_H, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "pxlflip_batch_last_error": (C.c_char_p, [_H]),
     "pxlflip_batch_destroy": (None, [_H]),
 }
@@ -702,6 +704,22 @@ class Batch:
     def read_particles(self, index: int) -> np.ndarray:
         out = np.zeros((self.infos[index]["n_particles"], 8), np.float32)
         self._check(self._lib.pxlflip_batch_read_particles(self._h, index, out.ctypes.data))
+        return out
+
+    def read_field(self, index: int, name: str) -> np.ndarray:
+        """Member `index`'s grid field (the FIELDS names; surface fields are not kept per member)."""
+        code, dtype, kind = FIELDS[name]
+        if kind not in ("faces", "cells"):
+            raise ValueError(f"{name} is not a grid field")
+        out = np.empty(self.infos[index]["n_faces" if kind == "faces" else "n_cells"], dtype)
+        self._check(self._lib.pxlflip_batch_read_field(self._h, index, code, out.ctypes.data))
+        return out
+
+    def read_bins(self, index: int):
+        """Counting sort of member `index`'s current positions: (cell_of, counts, offsets, ids)."""
+        P, Cc = self.infos[index]["n_particles"], self.infos[index]["n_cells"]
+        out = (np.empty(P, np.uint32), np.empty(Cc, np.uint32), np.empty(Cc, np.uint32), np.empty(P, np.uint32))
+        self._check(self._lib.pxlflip_batch_read_bins(self._h, index, *(a.ctypes.data for a in out)))
         return out
 
     def last_solve(self, index: int) -> dict:
