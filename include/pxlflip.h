@@ -393,6 +393,17 @@ int pxlgrid_build(pxlgrid *g, const float *agents, uint32_t count, float cell_si
 /* The same with host arrays (copied through device scratch); outputs may be NULL. */
 int pxlgrid_build_host(pxlgrid *g, const float *agents, uint32_t count, float cell_size, uint32_t table_size,
                        uint32_t *offsets, uint32_t *counts, uint32_t *sorted);
+/* The neighbour grid of pxlgrid_build without a handle and without allocating: no counterpart
+ * in the reference.  For callers that cannot allocate device memory (code under graph capture,
+ * tight serving loops, harnesses that forbid it).  Its scratch is a library-owned static array
+ * on each device (17 MB, held until the process ends), so count is at most
+ * PXLGRID_STATIC_MAX_AGENTS (else PXLFLIP_ERR_INVALID).  Inputs and outputs as pxlgrid_build
+ * (DEVICE pointers); runs on `stream` (a hipStream_t; NULL = the default stream) of `device`
+ * and returns when done.  Calls on one device share the array: calls that overlap on one
+ * device race.  Errors: pxlgrid_last_error(NULL). */
+#define PXLGRID_STATIC_MAX_AGENTS (1u << 20)
+int pxlgrid_build_static(int device, void *stream, const float *agents, uint32_t count, float cell_size,
+                         uint32_t table_size, uint32_t *offsets, uint32_t *counts, uint32_t *sorted);
 /* HIP-event time of the last build's kernels, in ms. */
 double pxlgrid_last_ms(const pxlgrid *g);
 const char *pxlgrid_last_error(const pxlgrid *g);

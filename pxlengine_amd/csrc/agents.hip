@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 
@@ -231,6 +232,41 @@ struct AgentGrid {
     }
 };
 
+// ---- the neighbour grid without allocation (pxlgrid_build_static)
+//
+// Its scratch is one file-scope device array, sized for PXLGRID_STATIC_MAX_AGENTS agents:
+// 4 radix arrays of 4 B per agent, the digit histograms of 256 counters per 2048-agent tile,
+// and the scan scratch of the largest table (2^23 entries, 2048 scan tiles).  Every GPU the
+// library runs on holds its own copy until the process ends.  Nothing in it needs clearing:
+// each build writes every word it reads (keys and ids of its agents, the histograms of its
+// tiles, the scan partials) before reading it.
+constexpr size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t STATIC_SCRATCH_BYTES =
+    4 * align256(4ull * PXLGRID_STATIC_MAX_AGENTS) +
+    align256(4ull * RADIX * ((PXLGRID_STATIC_MAX_AGENTS + RTILE - 1) / RTILE)) + align256(4ull * (2048 + 1));
+constexpr int MAX_STATIC_DEVICES = 64;
+
+}  // namespace pxl
+
+static __device__ __align__(256) unsigned char scratch[pxl::STATIC_SCRATCH_BYTES];
+
+namespace pxl {
+namespace {
+std::mutex g_static_mu;
+void *g_static_scratch[MAX_STATIC_DEVICES] = {};  // the array's address on each device
+
+// The address of `scratch` on `device` (current), looked up once per device.  The array's name
+// in host code is only a host-side stand-in; this address is what kernels must be given.
+void *static_scratch(int device) {
+    std::lock_guard<std::mutex> lk(g_static_mu);
+    if (!g_static_scratch[device]) {
+        void *p = nullptr;
+        HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_static_scratch[device] = p;
+    }
+    return g_static_scratch[device];
+}
+}  // namespace
 }  // namespace pxl
 
 struct pxlgrid {
@@ -334,6 +370,50 @@ int pxlgrid_build_host(pxlgrid *h, const float *agents, uint32_t count, float ce
         if (counts) HIP_CHECK(hipMemcpy(counts, c.p, 4ull * table_size, hipMemcpyDeviceToHost));
         if (sorted && count) HIP_CHECK(hipMemcpy(sorted, s.p, 4ull * count, hipMemcpyDeviceToHost));
     });
+}
+
+int pxlgrid_build_static(int device, void *stream, const float *agents, uint32_t count, float cell_size,
+                         uint32_t table_size, uint32_t *offsets, uint32_t *counts, uint32_t *sorted) {
+    using namespace pxl;
+    g_create_error.clear();
+    if (count > PXLGRID_STATIC_MAX_AGENTS) {
+        g_create_error = "count " + std::to_string(count) + " exceeds the static scratch's " +
+                         std::to_string(PXLGRID_STATIC_MAX_AGENTS) + " agents (use pxlgrid_create / pxlgrid_build)";
+        return PXLFLIP_ERR_INVALID;
+    }
+    if (table_size < 1 || (table_size & (table_size - 1)) != 0 || table_size > (1u << 23)) {
+        g_create_error = "table_size must be a power of two <= 2^23";
+        return PXLFLIP_ERR_INVALID;
+    }
+    if ((count && (!agents || !sorted)) || !offsets || !counts) {
+        g_create_error = "agents, offsets, counts and sorted must be device pointers";
+        return PXLFLIP_ERR_INVALID;
+    }
+    if (reinterpret_cast<uintptr_t>(agents) % 16 != 0) {
+        g_create_error = "agents must be 16-byte aligned (64-byte records)";
+        return PXLFLIP_ERR_INVALID;
+    }
+    if (device < 0 || device >= MAX_STATIC_DEVICES) {
+        g_create_error = "device " + std::to_string(device) + " out of range";
+        return PXLFLIP_ERR_INVALID;
+    }
+    static_assert(STATIC_SCRATCH_BYTES >= 4 * align256(4ull * PXLGRID_STATIC_MAX_AGENTS), "scratch layout");
+    if (GridScratch::total(PXLGRID_STATIC_MAX_AGENTS) > STATIC_SCRATCH_BYTES) {
+        g_create_error = "static scratch smaller than its layout";
+        return PXLFLIP_ERR_INTERNAL;
+    }
+    try {
+        HIP_CHECK(hipSetDevice(device));
+        GridScratch w;
+        w.layout(static_scratch(device), PXLGRID_STATIC_MAX_AGENTS);
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        enqueue_grid_build(w, agents, count, cell_size, table_size, offsets, counts, sorted, s);
+        HIP_CHECK(hipStreamSynchronize(s));
+    } catch (const HipError &e) {
+        g_create_error = std::string("HIP error in ") + e.what + ": " + hipGetErrorString(e.code);
+        return PXLFLIP_ERR_DEVICE;
+    }
+    return PXLFLIP_OK;
 }
 
 double pxlgrid_last_ms(const pxlgrid *h) { return h ? (double)h->g.last_ms : 0.0; }

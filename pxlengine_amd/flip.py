@@ -285,6 +285,8 @@ _SIGNATURES = {
     "pxlgrid_build": (C.c_int, [_H, C.c_void_p, C.c_uint32, C.c_float, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "pxlgrid_build_host": (C.c_int, [_H, C.c_void_p, C.c_uint32, C.c_float, C.c_uint32, C.c_void_p, C.c_void_p,
                                      C.c_void_p]),
+    "pxlgrid_build_static": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, C.c_uint32,
+                                       C.c_void_p, C.c_void_p, C.c_void_p]),
     "pxlgrid_last_ms": (C.c_double, [_H]),
     "pxlgrid_last_error": (C.c_char_p, [_H]),
     "pxlgrid_destroy": (None, [_H]),
