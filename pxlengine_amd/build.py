@@ -20,7 +20,7 @@ LIB = os.path.join(HERE, "libpxlflip.so")
 ARCH = os.environ.get("PXL_OFFLOAD_ARCH", "gfx950")
 
 HOST_SOURCES = ["json.cpp", "spec.cpp"]
-HIP_SOURCES = ["particles.hip", "grid.hip", "solver.hip", "mg.hip", "surface.hip", "agents.hip", "flip.hip",
+HIP_SOURCES = ["particles.hip", "seed.hip", "grid.hip", "solver.hip", "mg.hip", "surface.hip", "agents.hip", "flip.hip",
                "server.hip"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall"]

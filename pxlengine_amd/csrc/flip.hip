@@ -351,8 +351,7 @@ int pxlflip_create_from_json(const char *fluid_json, double fps, uint32_t seed, 
         return PXLFLIP_ERR_OOM;
     }
     Sim &s = h->sim;
-    std::vector<float> x0, v0;
-    int rc = prepare(fluid_json, fps, seed, opts, s, &x0, &v0);
+    int rc = prepare(fluid_json, fps, seed, opts, s, nullptr, nullptr);
     if (rc != PXLFLIP_OK) {
         delete h;
         return rc;
@@ -360,7 +359,7 @@ int pxlflip_create_from_json(const char *fluid_json, double fps, uint32_t seed, 
     try {
         HIP_CHECK(hipSetDevice(s.device));
         HIP_CHECK(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
-        s.allocate(x0, v0);
+        s.allocate();
         s.reset();
         s.initialized = false;  // first advance re-seeds, as fluid.rs:271
     } catch (const HipError &e) {
@@ -371,6 +370,10 @@ int pxlflip_create_from_json(const char *fluid_json, double fps, uint32_t seed, 
         g_create_error = "fluid '" + s.spec.name + "': out of device memory";
         delete h;
         return PXLFLIP_ERR_OOM;
+    } catch (const std::exception &e) {
+        g_create_error = "fluid '" + s.spec.name + "': " + e.what();
+        delete h;
+        return PXLFLIP_ERR_INTERNAL;
     }
     *out = h;
     return PXLFLIP_OK;

@@ -720,12 +720,11 @@ namespace {
 // Allocates a request's simulation on its own stream (allocator thread).  Throws on failure.
 void allocate_request(pxlflip_server *srv, Request *r) {
     auto sim = std::make_unique<Sim>();
-    std::vector<float> x0, v0;
-    if (prepare(r->json.c_str(), r->fps, r->seed, &srv->sim_opts, *sim, &x0, &v0) != PXLFLIP_OK)
+    if (prepare(r->json.c_str(), r->fps, r->seed, &srv->sim_opts, *sim, nullptr, nullptr) != PXLFLIP_OK)
         throw std::runtime_error(g_create_error);
     sim->d.cg_blocks = (int)(srv->opts.solver_workgroups ? srv->opts.solver_workgroups : batch_solver_blocks(*sim));
     HIP_CHECK(hipStreamCreateWithFlags(&sim->stream, hipStreamNonBlocking));
-    sim->allocate(x0, v0);
+    sim->allocate();
     sim->reset();
     r->sim = std::move(sim);
 }
@@ -1363,13 +1362,12 @@ int pxlflip_batch_create(const char *const *fluid_jsons, const uint32_t *seeds, 
         HIP_CHECK(hipSetDevice(o.device));
         for (uint32_t i = 0; i < n; i++) {
             auto sim = std::make_unique<Sim>();
-            std::vector<float> x0, v0;
             b->sim_opts.scene_index = i;
-            const int rc = prepare(fluid_jsons[i], fps, seeds ? seeds[i] : 0u, &b->sim_opts, *sim, &x0, &v0);
+            const int rc = prepare(fluid_jsons[i], fps, seeds ? seeds[i] : 0u, &b->sim_opts, *sim, nullptr, nullptr);
             if (rc != PXLFLIP_OK) return rc;
             if (!batchable(*sim, o)) return PXLFLIP_ERR_INVALID;
             HIP_CHECK(hipStreamCreateWithFlags(&sim->stream, hipStreamNonBlocking));
-            sim->allocate(x0, v0);
+            sim->allocate();
             sim->reset();
             b->sims.push_back(std::move(sim));
         }
@@ -1381,6 +1379,9 @@ int pxlflip_batch_create(const char *const *fluid_jsons, const uint32_t *seeds, 
     } catch (const std::bad_alloc &) {
         g_create_error = "out of device memory";
         return PXLFLIP_ERR_OOM;
+    } catch (const std::exception &e) {
+        g_create_error = e.what();
+        return PXLFLIP_ERR_INTERNAL;
     }
     *out = b.release();
     return PXLFLIP_OK;

@@ -89,6 +89,11 @@ struct KernelTimer {
 
 #define KRUN(name, call) timer.run(name, stream, [&] { call; })
 
+struct Sim;
+// Seeds a simulation's initial particles (ix..ivz) on its stream (seed.hip); flags and rank are
+// device scratch of ncells words each.  Throws if the device's count differs from the host's.
+void seed_on_device(Sim &s, uint32_t *flags, uint32_t *rank);
+
 struct Sim {
     FluidSpec spec;
     pxlflip_opts opts{};
@@ -245,8 +250,8 @@ struct Sim {
         return total;
     }
 
-    void allocate(const std::vector<float> &x0, const std::vector<float> &v0) {
-        const size_t P = d.np;
+    // The arena (zeroed), the obstacle arrays and the initial particles, seeded on the device.
+    void allocate() {
         Plan plan = this->plan();
         const size_t total = plan_total(plan);
         void *base = nullptr;
@@ -268,12 +273,6 @@ struct Sim {
         // the end of the last tile's particle run (the scan fills offset[0 .. nkeys-1])
         const uint32_t np32 = d.np;
         HIP_CHECK(hipMemcpyAsync(d.offset + d.nkeys, &np32, 4, hipMemcpyHostToDevice, stream));
-        // initial particles (reset source)
-        float *dst[6] = {d.ix, d.iy, d.iz, d.ivx, d.ivy, d.ivz};
-        for (int a = 0; a < 3; a++) {
-            HIP_CHECK(hipMemcpyAsync(dst[a], x0.data() + a * P, 4 * P, hipMemcpyHostToDevice, stream));
-            HIP_CHECK(hipMemcpyAsync(dst[3 + a], v0.data() + a * P, 4 * P, hipMemcpyHostToDevice, stream));
-        }
         // obstacles (extension): cell ids, their bit mask and the spheres
         std::vector<uint32_t> ids, bits;
         std::vector<float> spheres;
@@ -288,6 +287,10 @@ struct Sim {
             HIP_CHECK(hipMemcpyAsync(d.obst, spheres.data(), 4 * spheres.size(), hipMemcpyHostToDevice, stream));
         }
         HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&host_scal), 8 * (SC_COUNT + 1), hipHostMallocDefault));
+        // initial particles (the reset source), with two solver vectors as scratch, cleared after
+        seed_on_device(*this, reinterpret_cast<uint32_t *>(d.cg_r0), reinterpret_cast<uint32_t *>(d.cg_r1));
+        launch_fill_u32(reinterpret_cast<uint32_t *>(d.cg_r0), (uint32_t)d.ncells, 0u, stream);
+        launch_fill_u32(reinterpret_cast<uint32_t *>(d.cg_r1), (uint32_t)d.ncells, 0u, stream);
         HIP_CHECK(hipStreamSynchronize(stream));
     }
 

@@ -396,37 +396,93 @@ uint64_t obstacle_cells(const FluidSpec &f, std::vector<uint32_t> &ids) {
     return count;
 }
 
-uint64_t seed_particles(const FluidSpec &f, uint32_t seed, std::vector<float> *x, std::vector<float> *v) {
+SeedRange seed_range(const FluidBlockSpec &b, const uint32_t n[3], float h, const float origin[3]) {
+    SeedRange r;
+    for (int a = 0; a < 3; a++) {
+        r.lo[a] = r.hi[a] = 1;
+        bool any = false;
+        for (uint32_t idx = 1; idx + 1 < n[a]; idx++) {
+            const float c = origin[a] + ((float)idx + 0.5f) * h;
+            bool in;
+            if (b.sphere) {  // bounding box of the ball, one cell wider on each side
+                const double d = (double)c - (double)b.center[a];
+                in = std::fabs(d) <= (double)b.radius + (double)h;
+            } else {
+                in = !(c < b.min[a] || c > b.max[a]);
+            }
+            if (!in) continue;
+            if (!any) r.lo[a] = idx;
+            r.hi[a] = idx + 1;
+            any = true;
+        }
+        if (!any) r.lo[a] = r.hi[a] = 1;
+    }
+    return r;
+}
+
+bool block_holds(const FluidBlockSpec &b, const float o[3], float h, uint32_t i, uint32_t j, uint32_t k) {
+    const float cx = o[0] + ((float)i + 0.5f) * h;
+    const float cy = o[1] + ((float)j + 0.5f) * h;
+    const float cz = o[2] + ((float)k + 0.5f) * h;
+    if (b.sphere) {  // extension: centre within the sphere (f32, x + y + z)
+        const float dx = cx - b.center[0], dy = cy - b.center[1], dz = cz - b.center[2];
+        const float d2 = dx * dx + dy * dy + dz * dz;
+        return d2 <= b.radius * b.radius;
+    }
+    return !(cx < b.min[0] || cx > b.max[0] || cy < b.min[1] || cy > b.max[1] || cz < b.min[2] || cz > b.max[2]);
+}
+
+std::vector<uint64_t> seed_counts(const FluidSpec &f, const std::vector<uint32_t> &obst) {
     uint32_t n[3];
     float h;
     fluid_grid(f, n, &h);
     const float *o = f.bounds[0];
+    std::vector<uint64_t> out;
+    for (const auto &b : f.fill) {
+        const SeedRange r = seed_range(b, n, h, o);
+        uint64_t cells = 0;
+        if (!b.sphere && obst.empty()) {
+            cells = r.cells();  // every cell of a box's range passes
+        } else {
+            for (uint32_t k = r.lo[2]; k < r.hi[2]; k++)
+                for (uint32_t j = r.lo[1]; j < r.hi[1]; j++)
+                    for (uint32_t i = r.lo[0]; i < r.hi[0]; i++) {
+                        if (!block_holds(b, o, h, i, j, k)) continue;
+                        if (!obst.empty() && obst[(size_t)i + (size_t)n[0] * ((size_t)j + (size_t)n[1] * (size_t)k)])
+                            continue;  // extension: no liquid inside obstacles
+                        cells++;
+                    }
+        }
+        out.push_back(cells * f.particles_per_cell);
+    }
+    return out;
+}
+
+uint64_t seed_particles(const FluidSpec &f, uint32_t seed, std::vector<float> *x, std::vector<float> *v) {
     std::vector<uint32_t> obst;
     obstacle_cells(f, obst);
+    if (!x) {
+        uint64_t count = 0;
+        for (uint64_t c : seed_counts(f, obst)) count += c;
+        return count;
+    }
+    uint32_t n[3];
+    float h;
+    fluid_grid(f, n, &h);
+    const float *o = f.bounds[0];
     uint64_t rng = 0x9e3779b97f4a7c15ull ^ (uint64_t)seed;
     uint64_t count = 0;
     std::vector<float> xs[3], vs[3];
+    // the reference loop (k outer, j, i inner over the interior, fluid.rs:123-148), restricted
+    // to each block's seed range: cells outside it fail the block's test and draw nothing
     for (const auto &b : f.fill) {
-        for (uint32_t k = 1; k + 1 < n[2]; k++)
-            for (uint32_t j = 1; j + 1 < n[1]; j++)
-                for (uint32_t i = 1; i + 1 < n[0]; i++) {
-                    float cx = o[0] + ((float)i + 0.5f) * h;
-                    float cy = o[1] + ((float)j + 0.5f) * h;
-                    float cz = o[2] + ((float)k + 0.5f) * h;
-                    if (b.sphere) {  // extension: centre within the sphere (f32, x + y + z)
-                        const float dx = cx - b.center[0], dy = cy - b.center[1], dz = cz - b.center[2];
-                        const float d2 = dx * dx + dy * dy + dz * dz;
-                        if (!(d2 <= b.radius * b.radius)) continue;
-                    } else if (cx < b.min[0] || cx > b.max[0] || cy < b.min[1] || cy > b.max[1] ||
-                               cz < b.min[2] || cz > b.max[2]) {
-                        continue;
-                    }
+        const SeedRange r = seed_range(b, n, h, o);
+        for (uint32_t k = r.lo[2]; k < r.hi[2]; k++)
+            for (uint32_t j = r.lo[1]; j < r.hi[1]; j++)
+                for (uint32_t i = r.lo[0]; i < r.hi[0]; i++) {
+                    if (!block_holds(b, o, h, i, j, k)) continue;
                     if (!obst.empty() && obst[(size_t)i + (size_t)n[0] * ((size_t)j + (size_t)n[1] * (size_t)k)])
                         continue;  // extension: no liquid inside obstacles
-                    if (!x) {  // counting only: the draws decide positions, not the count
-                        count += f.particles_per_cell;
-                        continue;
-                    }
                     for (uint32_t q = 0; q < f.particles_per_cell; q++) {
                         float px = o[0] + ((float)i + lcg_next(rng)) * h;
                         float py = o[1] + ((float)j + lcg_next(rng)) * h;
@@ -437,15 +493,13 @@ uint64_t seed_particles(const FluidSpec &f, uint32_t seed, std::vector<float> *x
                     }
                 }
     }
-    if (x) {
-        x->clear();
-        v->clear();
-        x->reserve(3 * count);
-        v->reserve(3 * count);
-        for (int a = 0; a < 3; a++) {
-            x->insert(x->end(), xs[a].begin(), xs[a].end());
-            v->insert(v->end(), vs[a].begin(), vs[a].end());
-        }
+    x->clear();
+    v->clear();
+    x->reserve(3 * count);
+    v->reserve(3 * count);
+    for (int a = 0; a < 3; a++) {
+        x->insert(x->end(), xs[a].begin(), xs[a].end());
+        v->insert(v->end(), vs[a].begin(), vs[a].end());
     }
     return count;
 }

@@ -78,6 +78,26 @@ uint64_t steps_for_frame(double warmup, double frame, double fps, uint32_t subst
 // cells; ids is left empty without obstacles.
 uint64_t obstacle_cells(const FluidSpec &f, std::vector<uint32_t> &ids);
 
+// The interior cells whose centres can pass fill block b's test: index ranges [lo, hi) per axis
+// inside [1, n - 1).  Exact for a box (a cell centre's f32 coordinate never decreases with its
+// index, so the passing indices of an axis are contiguous); a sphere's bounding box widened by
+// a cell (the f32 sphere test decides inside it).  The seeding loop visits nothing outside.
+struct SeedRange {
+    uint32_t lo[3], hi[3];
+    uint64_t cells() const {
+        return (uint64_t)(hi[0] - lo[0]) * (uint64_t)(hi[1] - lo[1]) * (uint64_t)(hi[2] - lo[2]);
+    }
+};
+SeedRange seed_range(const FluidBlockSpec &b, const uint32_t n[3], float h, const float origin[3]);
+
+// The reference's cell-centre test of a fill block (fluid.rs:127-133; a sphere's f32 distance
+// test for the extension), f32 as the seeding loop evaluates it.
+bool block_holds(const FluidBlockSpec &b, const float origin[3], float h, uint32_t i, uint32_t j, uint32_t k);
+
+// Particles each fill block seeds (the RNG stream continues across blocks, so block b's first
+// particle is the sum over the earlier blocks).  Obstacle ids as obstacle_cells (empty: none).
+std::vector<uint64_t> seed_counts(const FluidSpec &f, const std::vector<uint32_t> &obstacle_ids);
+
 // Reference seeding into SoA arrays (x[3][P], v[3][P]); returns the particle count.
 // With out == nullptr only counts.  Obstacle cells (extension) receive no particles.
 uint64_t seed_particles(const FluidSpec &f, uint32_t seed, std::vector<float> *x,
