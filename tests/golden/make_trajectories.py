@@ -15,6 +15,11 @@ compares digests, step by step.
 
     python tests/golden/make_trajectories.py dam64     # ~2 min
     python tests/golden/make_trajectories.py dam128    # ~30 min
+    python tests/golden/make_trajectories.py dam128 --checkpoint /tmp/dam128.npz   # resumable
+
+With --checkpoint the oracle's carried state (particles and the warm-start pressure: every other
+array is rebuilt by the next step) and the digests so far are saved after every recorded step,
+and a later run with the same path continues from them, bitwise as an uninterrupted run.
 """
 from __future__ import annotations
 
@@ -67,23 +72,39 @@ def fixture_path(name: str) -> str:
     return os.path.join(HERE, f"trajectory_{name}.json")
 
 
-def oracle_steps(name: str, upto: int):
-    """Yields (step, substeps, digests) of the oracle trajectory of `name` up to `upto`."""
+def oracle_steps(name: str, upto: int, checkpoint: str = None, done: dict = None):
+    """Yields (step, substeps, digests) of the oracle trajectory of `name` up to `upto`; with a
+    checkpoint, starts after the state saved there and saves the state after each recorded
+    step (with `done`, the digests so far)."""
     import oracle
     res, record = CONFIGS[name]
     sim = oracle_sim(oracle, scene(res), seed=SEED)
-    for s in range(1, upto + 1):
+    start = 1
+    if checkpoint and os.path.exists(checkpoint):
+        with np.load(checkpoint) as z:
+            sim.set_particles(z["x"], z["v"])
+            sim.pressure[:] = z["pressure"]
+            start = int(z["step"]) + 1
+    for s in range(start, upto + 1):
         k = sim.step_rbgs_capped(ITERS, CFL)
         if s in record:
             yield s, k, state_digests(sim.x, sim.v, sim.pressure, sim.vel, sim.count, sim.bin())
+            if checkpoint:
+                tmp = checkpoint + ".tmp.npz"
+                np.savez(tmp, x=sim.x, v=sim.v, pressure=sim.pressure, step=s,
+                         done=json.dumps(done if done is not None else {}))
+                os.replace(tmp, checkpoint)
 
 
-def main(name: str):
+def main(name: str, checkpoint: str = None):
     res, record = CONFIGS[name]
     out = {"config": name, "scene": scene(res), "fps": FPS, "seed": SEED, "pressure_iterations": ITERS,
            "cfl": CFL, "solver": "rbgs", "digest": "blake2b-128", "steps": {}}
+    if checkpoint and os.path.exists(checkpoint):
+        with np.load(checkpoint) as z:
+            out["steps"] = json.loads(str(z["done"]))
     t0 = time.time()
-    for s, k, d in oracle_steps(name, max(record)):
+    for s, k, d in oracle_steps(name, max(record), checkpoint, out["steps"]):
         out["steps"][str(s)] = dict(d, substeps=k)
         print(f"{name} step {s}: {k} substeps, {time.time() - t0:.0f} s", flush=True)
     with open(fixture_path(name), "w") as f:
@@ -91,4 +112,5 @@ def main(name: str):
 
 
 if __name__ == "__main__":
-    main(sys.argv[1])
+    ck = sys.argv[sys.argv.index("--checkpoint") + 1] if "--checkpoint" in sys.argv else None
+    main(sys.argv[1], ck)
