@@ -418,10 +418,12 @@ def bench_serve(args, pxl):
                    seeds=list(range(8))) as bat:
         bat.step(max(args.warmup, 1))
 
-    # capacity (timed): every request submitted at once
-    with pxl.Server(**opts) as srv:
+    # capacity (timed): every request submitted at once; every --profile-every-th batched step
+    # runs its launches eagerly, each timed by HIP events (the roofline's kernel durations)
+    with pxl.Server(profile_every=args.profile_every, **opts) as srv:
         wall, ids, st = _stream(pxl, srv, reqs, np.zeros(len(reqs)), REQUEST_STEPS)
         per = [srv.poll(i) for i in ids]
+        kstats = srv.kernel_stats()
     failed = [p for p in per if p["status"] != "done"]
     value = st["particle_steps"] / wall
     # the requests the capacity run admitted at once: first in, first out within max_sims and
@@ -456,9 +458,20 @@ def bench_serve(args, pxl):
              "path": "one simulation at a time, FP32 Jacobi-PCG 80 fixed iterations, hipGraph per step, "
                      "creation included (steps_only: step time alone)"}
 
-    roofline, kernels, _ = batch_roofline(pxl, reqs[:first], 2, 0)
-    roofline["measured_on"] = (f"two event-timed batched steps of the capacity run's opening batch "
-                               f"(its first {first} requests, admitted together)")
+    roofline, kernels = roofline_of(kstats)
+    profiled = sum(k["launches"] for k in kstats if k["kernel"] == "g2p")
+    roofline["measured_on"] = (f"HIP events of every launch of {profiled} of the capacity run's {st['batch_steps']} "
+                               f"batched steps (every {args.profile_every}th step run eagerly)")
+    if args.traffic:
+        t = measured_traffic_pmc(roofline["kernel"], ["--profile-batch", str(first), "--sizes", args.sizes])
+        if t:
+            t["alg_bytes_per_launch"] = next((k["alg_bytes_per_launch"] for k in probe_kernels(pxl, reqs[:first])
+                                              if k["kernel"] == roofline["kernel"]), None)
+            if t["alg_bytes_per_launch"]:
+                t["traffic_over_alg"] = t["hbm_bytes"] / t["alg_bytes_per_launch"]
+            roofline["traffic"] = t["hbm_bytes"]
+            roofline["traffic_detail"] = dict(t, per="dispatch, two event-timed batched steps of the same batch "
+                                                     "under rocprofv3 --pmc (two passes)")
 
     cpu = cpu_serve(args, reqs) if args.cpu_seconds > 0 else None
     return {
@@ -496,6 +509,31 @@ def bench_serve(args, pxl):
     }
 
 
+def probe_kernels(pxl, reqs):
+    """Per-kernel rows of the probe's launches (the same batch, two event-timed steps), for the
+    algorithmic bytes that go with its counters."""
+    from tests.scenes import FPS
+    with pxl.Batch(reqs, FPS, seeds=list(range(len(reqs)))) as bat:
+        return roofline_of(bat.profile_steps(2))[1]
+
+
+def roofline_of(prof):
+    """(roofline of the kernel with the largest time share, per-kernel rows) from per-kernel
+    totals [{"kernel", "launches", "total_ms", "alg_bytes"}]."""
+    prof = sorted(prof, key=lambda k: -k["total_ms"])
+    tot = sum(k["total_ms"] for k in prof)
+    kernels = [{"kernel": k["kernel"], "launches": k["launches"], "avg_us": 1e3 * k["total_ms"] / k["launches"],
+                "share": k["total_ms"] / tot, "alg_bytes_per_launch": k["alg_bytes"] / k["launches"],
+                "achieved_GBs": (k["alg_bytes"] / (k["total_ms"] / 1e3) / 1e9) if k["alg_bytes"] else None}
+               for k in prof]
+    dom = kernels[0]
+    ach = dom["achieved_GBs"] or 0.0
+    roofline = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach / PEAK_HBM_GBS,
+                "traffic": None, "kernel": dom["kernel"], "avg_us": dom["avg_us"],
+                "alg_bytes_per_launch": dom["alg_bytes_per_launch"]}
+    return roofline, kernels
+
+
 def batch_roofline(pxl, reqs, warm, timed):
     """A fixed batch of `reqs` (seed = index): `warm` untimed steps, `timed` timed steps, then
     two steps with every batched launch timed by HIP events and priced in algorithmic bytes.
@@ -524,6 +562,109 @@ def batch_roofline(pxl, reqs, warm, timed):
                 "avg_us": dom["avg_us"], "alg_bytes_per_launch": dom["alg_bytes_per_launch"],
                 "measured_on": f"two profiled batched steps of the first {n} stream requests"}
     return roofline, kernels, rate
+
+
+# HBM traffic from rocprofv3 counters.  Two passes (a pass may hold at most 4 TCC counters):
+# EA read requests by size, then EA write requests by size and the DRAM-bound ones.  Bytes =
+# 32 n32 + 64 n64 + 128 n128 (reads), 32 (n - n64) + 64 n64 (writes), each scaled by what the
+# same formula reads for a device-to-device copy of known size (the factor is reported).
+PMC_PASSES = (("TCC_EA0_RDREQ_sum", "TCC_EA0_RDREQ_32B_sum", "TCC_EA0_RDREQ_64B_sum", "TCC_EA0_RDREQ_128B_sum"),
+              ("TCC_EA0_WRREQ_sum", "TCC_EA0_WRREQ_64B_sum", "TCC_EA0_RDREQ_DRAM_sum", "TCC_EA0_WRREQ_DRAM_sum"))
+CALIB_BYTES = 256 << 20
+COPY_KERNEL = "__amd_rocclr_copyBuffer"
+
+
+def kernel_symbol(name: str) -> list:
+    """Symbols the batched launch of phase `name` may carry (kb_sweep for mg_sweep, ...)."""
+    base = name[3:] if name.startswith("mg_") and name not in ("mg_tail", "mg_types", "mg_sten") else name
+    return [f"kb_{name}", f"kb_{base}", f"kb_{name}_tile"]
+
+
+def pmc_counters(csv_dir: str) -> dict:
+    """{kernel symbol: [{counter: value} per dispatch]} from a rocprofv3 counter-collection CSV
+    (symbols without namespaces or arguments: kb_sweep, __amd_rocclr_copyBuffer, ...)."""
+    import csv
+    import glob
+    out = {}
+    for path in glob.glob(os.path.join(csv_dir, "**", "*counter_collection.csv"), recursive=True):
+        per = {}
+        with open(path, newline="") as f:
+            for row in csv.DictReader(f):
+                name = row["Kernel_Name"].replace("(anonymous namespace)", "anon").split("(")[0].split("::")[-1]
+                per.setdefault((name, row["Dispatch_Id"]), {})[row["Counter_Name"]] = float(row["Counter_Value"])
+        for (name, _), ctr in per.items():
+            out.setdefault(name, []).append(ctr)
+    return out
+
+
+def measured_traffic_pmc(kernel: str, probe_args: list, timeout: float = 240.0):
+    """HBM bytes per dispatch of the batched `kernel` in the probe workload, from two rocprofv3
+    --pmc passes run as child processes, with the copy calibration and the per-size split.
+    Returns a dict, or None when rocprofv3 is unavailable or a pass fails."""
+    import shutil
+    import subprocess
+    import tempfile
+
+    import numpy as np
+    prof = shutil.which("rocprofv3")
+    if not prof:
+        return None
+    res, dispatches = {}, 0
+    env = dict(os.environ, TMPDIR="/tmp")
+    for counters in PMC_PASSES:
+        d = tempfile.mkdtemp(prefix="pxl_pmc_", dir="/tmp")
+        cmd = [prof, "--pmc", *counters, "--output-format", "csv", "-d", d, "-o", "run", "--",
+               sys.executable, os.path.join(ROOT, "bench.py"), "--workload", "probe", *probe_args]
+        try:
+            r = subprocess.run(cmd, cwd=ROOT, env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                               timeout=timeout, text=True)
+        except subprocess.TimeoutExpired:
+            shutil.rmtree(d, ignore_errors=True)
+            return None
+        data = pmc_counters(d)
+        shutil.rmtree(d, ignore_errors=True)
+        ks = next((data[s] for s in kernel_symbol(kernel) if s in data), None)
+        copies = data.get(COPY_KERNEL)
+        if r.returncode != 0 or not ks or not copies:
+            return None
+        big = max(copies, key=lambda c: c.get(counters[0], 0.0))  # the calibration copy
+        dispatches = len(ks)
+        for name in counters:
+            res[name] = (float(np.mean([c.get(name, 0.0) for c in ks])), big.get(name, 0.0))
+
+    def rd(i):
+        n32, n64, n128 = (res[f"TCC_EA0_RDREQ_{s}_sum"][i] for s in ("32B", "64B", "128B"))
+        return 32 * n32 + 64 * n64 + 128 * n128
+
+    def wr(i):
+        n, n64 = res["TCC_EA0_WRREQ_sum"][i], res["TCC_EA0_WRREQ_64B_sum"][i]
+        return 32 * (n - n64) + 64 * n64
+    f_rd = CALIB_BYTES / rd(1) if rd(1) else 1.0
+    f_wr = CALIB_BYTES / wr(1) if wr(1) else 1.0
+    read, write = rd(0) * f_rd, wr(0) * f_wr
+    return {"hbm_bytes": read + write, "read_bytes": read, "write_bytes": write, "dispatches": dispatches,
+            "read_factor": f_rd, "write_factor": f_wr,
+            "dram_read_fraction": res["TCC_EA0_RDREQ_DRAM_sum"][0] / max(res["TCC_EA0_RDREQ_sum"][0], 1.0),
+            "dram_write_fraction": res["TCC_EA0_WRREQ_DRAM_sum"][0] / max(res["TCC_EA0_WRREQ_sum"][0], 1.0),
+            "counters": {k: v[0] for k, v in res.items()},
+            "copy_counters": {k: v[1] for k, v in res.items()}}
+
+
+def bench_probe(args, pxl):
+    """Counter probe, run under rocprofv3 --pmc by measured_traffic_pmc: a device-to-device copy
+    of CALIB_BYTES (the calibration), then --probe-steps event-timed batched steps of the fixed
+    batch of the first --profile-batch stream requests (the same launches the roofline prices);
+    prints their per-kernel algorithmic bytes per launch."""
+    from tests.scenes import FPS, random_request
+    sizes = tuple(int(v) for v in args.sizes.split(","))
+    a, b = pxl.DeviceBuffer(CALIB_BYTES), pxl.DeviceBuffer(CALIB_BYTES)
+    b.copy_from(a)
+    a.free()
+    b.free()
+    reqs = [random_request(i, sizes=sizes) for i in range(args.profile_batch)]
+    with pxl.Batch(reqs, FPS, seeds=list(range(len(reqs)))) as bat:
+        prof = bat.profile_steps(args.probe_steps)
+    return {"probe": {k["kernel"]: k["alg_bytes"] / k["launches"] for k in prof}}
 
 
 def bench_batch(args, pxl):
@@ -596,7 +737,8 @@ def main():
                          "(each request is 60 steps)")
     ap.add_argument("--warmup", type=int, default=5, help="untimed warm-up steps")
     ap.add_argument("--workload", default="serve",
-                    choices=["serve", "dam", "batch", "mixed256", "obstacles", "frame", "reference64", "agents", "tune"])
+                    choices=["serve", "dam", "batch", "mixed256", "obstacles", "frame", "reference64", "agents", "tune",
+                             "probe"])
     ap.add_argument("--grids", default=None, help="JSON cache of batched launch grids to load first (autotuner output)")
     ap.add_argument("--grids-out", default=None, help="tune: write the chosen grids to this JSON file")
     ap.add_argument("--requests", type=int, default=512)
@@ -607,6 +749,11 @@ def main():
     ap.add_argument("--budget", type=float, default=200e9)
     ap.add_argument("--plain-every", type=int, default=8, help="plain path on every k-th request")
     ap.add_argument("--profile-batch", type=int, default=64)
+    ap.add_argument("--probe-steps", type=int, default=2, help="probe: event-timed batched steps")
+    ap.add_argument("--profile-every", type=int, default=20,
+                    help="serve: every k-th batched step of the capacity run is event-timed per launch (roofline)")
+    ap.add_argument("--traffic", type=int, default=1, help="serve: measure the dominant kernel's HBM traffic with "
+                                                            "rocprofv3 counters in two child passes (0 = off)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="dam: CPU oracle sample length; serve: 0 skips the CPU baseline")
     ap.add_argument("--cpu-requests", type=int, default=192, help="serve: requests in the CPU oracle sample")
@@ -628,7 +775,7 @@ def main():
         pxl.load_batch_grids(args.grids)
     run = {"tune": bench_tune, "serve": bench_serve, "dam": bench_dam, "batch": bench_batch, "mixed256": bench_mixed256,
            "obstacles": bench_obstacles, "frame": bench_frame, "reference64": bench_reference64,
-           "agents": bench_agents}[args.workload]
+           "agents": bench_agents, "probe": bench_probe}[args.workload]
     line = run(args, pxl)
     print(json.dumps(line))
 

@@ -269,6 +269,9 @@ typedef struct pxlflip_server_opts {
     double tolerance;           /* as pxlflip_opts (default 1e-6) */
     uint32_t solver_workgroups; /* per simulation; 0 = chosen by grid size */
     int32_t use_graph;          /* 1 = replay the batch phases as hipGraphs (default 1) */
+    uint32_t profile_every;     /* > 0: every profile_every-th batch step runs its launches eagerly, each timed
+                                   by HIP events and priced in algorithmic bytes (pxlflip_server_kernel_stats);
+                                   0 = never (default) */
 } pxlflip_server_opts;
 
 #define PXLFLIP_REQ_QUEUED 0
@@ -343,6 +346,9 @@ int pxlflip_server_read_particles(pxlflip_server *srv, uint64_t id, float *aos8,
 /* Forgets a finished request (its kept particles). */
 int pxlflip_server_release(pxlflip_server *srv, uint64_t id);
 int pxlflip_server_get_stats(pxlflip_server *srv, pxlflip_server_stats *stats);
+/* Per-kernel totals of the batch steps profiled so far (profile_every): launches, HIP-event time
+ * and algorithmic bytes over the members each launch covered, in first-launch order. */
+int pxlflip_server_kernel_stats(pxlflip_server *srv, pxlflip_kernel_time *out, uint32_t max_out, uint32_t *n_out);
 /* Message of the last failed call on this server (or of a failed request via poll). */
 const char *pxlflip_server_last_error(const pxlflip_server *srv);
 /* Stops the device thread (requests still queued or running are cancelled) and frees all. */

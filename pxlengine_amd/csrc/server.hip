@@ -706,6 +706,7 @@ struct pxlflip_server {
     double particle_steps = 0, first_submit = -1, last_done = -1, busy_s = 0, retire_s = 0;
     uint64_t batch_steps = 0;
     double batch_members = 0;
+    std::vector<std::pair<std::string, pxl::Batch::Agg>> kernel_stats;  // profiled batch steps
     double admit_s = 0;  // allocator-thread time spent allocating admitted requests
 
     void run();
@@ -867,8 +868,26 @@ void pxlflip_server::run() {
             }
             members = batch->size();
             if (members) {
+                const bool prof = opts.profile_every && (batch_steps % opts.profile_every) == opts.profile_every - 1;
+                if (prof) batch->set_profiling(true);
                 batch->step();
                 batch->sync();
+                if (prof) {
+                    auto agg = batch->profile();
+                    batch->set_profiling(false);
+                    std::lock_guard<std::mutex> g(mu);
+                    for (auto &e : agg) {
+                        auto it = std::find_if(kernel_stats.begin(), kernel_stats.end(),
+                                               [&](auto &k) { return k.first == e.first; });
+                        if (it == kernel_stats.end()) {
+                            kernel_stats.push_back(e);
+                        } else {
+                            it->second.launches += e.second.launches;
+                            it->second.ms += e.second.ms;
+                            it->second.bytes += e.second.bytes;
+                        }
+                    }
+                }
             }
             t1 = now_s();
         } catch (const HipError &e) {
@@ -1009,6 +1028,7 @@ int pxlflip_server_default_opts(pxlflip_server_opts *o) {
     o->tolerance = 1e-6;
     o->solver_workgroups = 0;
     o->use_graph = 1;
+    o->profile_every = 0;
     return PXLFLIP_OK;
 }
 
@@ -1261,6 +1281,21 @@ int pxlflip_server_get_stats(pxlflip_server *srv, pxlflip_server_stats *st) {
     st->graph_captures = srv->batch ? srv->batch->graph_captures() : 0;
     st->capture_s = srv->batch ? srv->batch->capture_seconds() : 0.0;
     st->retire_s = srv->retire_s;
+    return PXLFLIP_OK;
+}
+
+int pxlflip_server_kernel_stats(pxlflip_server *srv, pxlflip_kernel_time *out, uint32_t max_out, uint32_t *n_out) {
+    if (!srv) return PXLFLIP_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(srv->mu);
+    if (n_out) *n_out = (uint32_t)srv->kernel_stats.size();
+    for (uint32_t k = 0; k < srv->kernel_stats.size() && k < max_out && out; k++) {
+        const auto &e = srv->kernel_stats[k];
+        std::memset(&out[k], 0, sizeof out[k]);
+        std::snprintf(out[k].name, sizeof out[k].name, "%s", e.first.c_str());
+        out[k].launches = e.second.launches;
+        out[k].total_ms = e.second.ms;
+        out[k].alg_bytes = e.second.bytes;
+    }
     return PXLFLIP_OK;
 }
 

@@ -95,6 +95,7 @@ class ServerOpts(C.Structure):
         ("struct_size", C.c_uint32), ("device", C.c_int32), ("budget_bytes", C.c_uint64),
         ("max_sims", C.c_uint32), ("inner_iterations", C.c_uint32), ("max_outer", C.c_uint32),
         ("tolerance", C.c_double), ("solver_workgroups", C.c_uint32), ("use_graph", C.c_int32),
+        ("profile_every", C.c_uint32),
     ]
 
 
@@ -154,6 +155,13 @@ class DeviceBuffer:
         assert host.nbytes <= self.nbytes
         if DeviceBuffer._hip.hipMemcpy(self._p, host.ctypes.data, host.nbytes, 1) != 0:
             raise FluidError(ERR_DEVICE, "hipMemcpy to the device failed")
+
+    def copy_from(self, other: "DeviceBuffer", nbytes: int = None):
+        """Device-to-device copy of nbytes (default all of other) into this buffer."""
+        n = other.nbytes if nbytes is None else int(nbytes)
+        assert n <= self.nbytes and n <= other.nbytes
+        if DeviceBuffer._hip.hipMemcpy(self._p, other._p, n, 3) != 0:
+            raise FluidError(ERR_DEVICE, "hipMemcpy device to device failed")
 
     def download(self, out: np.ndarray) -> np.ndarray:
         assert out.nbytes <= self.nbytes
@@ -321,6 +329,7 @@ _SIGNATURES = {
     "pxlflip_server_read_particles": (C.c_int, [_H, C.c_uint64, C.c_void_p, C.c_uint64]),
     "pxlflip_server_release": (C.c_int, [_H, C.c_uint64]),
     "pxlflip_server_get_stats": (C.c_int, [_H, C.POINTER(ServerStats)]),
+    "pxlflip_server_kernel_stats": (C.c_int, [_H, C.POINTER(KernelTime), C.c_uint32, C.POINTER(C.c_uint32)]),
     "pxlflip_server_last_error": (C.c_char_p, [_H]),
     "pxlflip_server_destroy": (None, [_H]),
     "pxlflip_batch_create": (C.c_int, [C.POINTER(C.c_char_p), C.POINTER(C.c_uint32), C.c_uint32, C.c_double,
@@ -574,7 +583,7 @@ class FluidState:
 
 
 def make_server_opts(device=0, budget_bytes=200_000_000_000, max_sims=256, inner_iterations=9, max_outer=50,
-                     tolerance=1e-6, solver_workgroups=0, use_graph=True) -> ServerOpts:
+                     tolerance=1e-6, solver_workgroups=0, use_graph=True, profile_every=0) -> ServerOpts:
     o = ServerOpts()
     load_library().pxlflip_server_default_opts(C.byref(o))
     o.device = device
@@ -585,6 +594,7 @@ def make_server_opts(device=0, budget_bytes=200_000_000_000, max_sims=256, inner
     o.tolerance = tolerance
     o.solver_workgroups = solver_workgroups
     o.use_graph = 1 if use_graph else 0
+    o.profile_every = int(profile_every)
     return o
 
 
@@ -645,6 +655,16 @@ class Server:
         st = ServerStats()
         self._check(self._lib.pxlflip_server_get_stats(self._h, C.byref(st)))
         return st.as_dict()
+
+    def kernel_stats(self) -> list:
+        """Per-kernel totals of the profiled batch steps (profile_every):
+        [{"kernel", "launches", "total_ms", "alg_bytes"}]."""
+        arr = (KernelTime * 128)()
+        cnt = C.c_uint32()
+        self._check(self._lib.pxlflip_server_kernel_stats(self._h, arr, 128, C.byref(cnt)))
+        return [{"kernel": arr[k].name.decode(), "launches": int(arr[k].launches),
+                 "total_ms": float(arr[k].total_ms), "alg_bytes": float(arr[k].alg_bytes)}
+                for k in range(min(cnt.value, 128))]
 
     def close(self):
         if self._h:
