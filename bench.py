@@ -71,19 +71,6 @@ def plain_algorithmic_bytes(name: str, P: int, F: int, C: int, Cf: int) -> float
     return float(table.get(name, 0))
 
 
-def measured_traffic(kernel: str, workload: str):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
-    summary of this workload (profiles/), or None when there is none for it."""
-    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
-    if not os.path.exists(path):
-        return None
-    per = json.load(open(path))["per_dispatch"]
-    for key in (kernel, f"k_{kernel}", f"kb_{kernel}", f"k_{kernel}<float>", f"k_{kernel}_tile", f"kb_{kernel}_tile"):
-        if key in per:
-            return per[key]["hbm_bytes"]
-    return None
-
-
 def bench_dam(args, pxl):
     from tests.scenes import FPS, dam_break
     fluid = dam_break(128, iters=80)
@@ -108,6 +95,9 @@ def bench_dam(args, pxl):
     dom = kernels[0]
     ach = dom["achieved_GBs"] or 0.0
     sim.close()
+    traffic = measured_traffic_pmc(dom["kernel"], ["--probe-kind", "dam"]) if args.traffic else None
+    if traffic and dom["alg_bytes"]:
+        traffic["traffic_over_alg"] = traffic["hbm_bytes"] / dom["alg_bytes"]
     cpu = cpu_dam(args, fluid) if args.cpu_seconds > 0 else None
     return {
         "metric": METRIC, "value": P * args.steps / elapsed, "unit": "particle-steps/s", "n_gpus": 1,
@@ -117,7 +107,8 @@ def bench_dam(args, pxl):
         "config": {"workload": "dam break 128^3, single instance, plain FP32 Jacobi-PCG, 80 fixed iterations",
                    "particles": P, "autotune": tuned},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": ach / PEAK_HBM_GBS, "traffic": measured_traffic(dom["kernel"], "dam128_cg80"),
+                     "frac": ach / PEAK_HBM_GBS, "traffic": traffic["hbm_bytes"] if traffic else None,
+                     "traffic_detail": traffic,
                      "kernel": dom["kernel"], "avg_us": dom["avg_us"], "alg_bytes_per_launch": dom["alg_bytes"]},
         "cpu_baseline": cpu, "kernels": kernels,
     }
@@ -180,6 +171,9 @@ def bench_frame(args, pxl):
     dom = stages[0]
     mesh_ms = sum(st["avg_us"] for st in stages) / 1e3
     sim.close()
+    traffic = measured_traffic_pmc(dom["kernel"], ["--probe-kind", "frame"]) if args.traffic else None
+    if traffic and dom["alg_bytes"]:
+        traffic["traffic_over_alg"] = traffic["hbm_bytes"] / dom["alg_bytes"]
     return {
         "metric": METRIC, "value": P * frames * sub / elapsed, "unit": "particle-steps/s", "n_gpus": 1,
         "steps": frames * sub, "warmup": args.warmup, "ms_per_step": elapsed / (frames * sub) * 1e3,
@@ -192,7 +186,7 @@ def bench_frame(args, pxl):
                    "surface_nodes": nodes, "touched_nodes": touched, "reservation": cap},
         "roofline": {"bound": "hbm", "achieved": dom["achieved_GBs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": (dom["achieved_GBs"] or 0.0) / PEAK_HBM_GBS,
-                     "traffic": measured_traffic(SURF_KERNELS.get(dom["kernel"], dom["kernel"]), "frame128"),
+                     "traffic": traffic["hbm_bytes"] if traffic else None, "traffic_detail": traffic,
                      "kernel": dom["kernel"], "avg_us": dom["avg_us"], "alg_bytes_per_launch": dom["alg_bytes"]},
         "cpu_baseline": None, "kernels": stages,
     }
@@ -558,7 +552,7 @@ def batch_roofline(pxl, reqs, warm, timed):
     dom = kernels[0]
     ach = dom["achieved_GBs"] or 0.0
     roofline = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach / PEAK_HBM_GBS,
-                "traffic": measured_traffic(dom["kernel"], f"serve_batch{n}"), "kernel": dom["kernel"],
+                "traffic": None, "kernel": dom["kernel"],
                 "avg_us": dom["avg_us"], "alg_bytes_per_launch": dom["alg_bytes_per_launch"],
                 "measured_on": f"two profiled batched steps of the first {n} stream requests"}
     return roofline, kernels, rate
@@ -575,9 +569,11 @@ COPY_KERNEL = "__amd_rocclr_copyBuffer"
 
 
 def kernel_symbol(name: str) -> list:
-    """Symbols the batched launch of phase `name` may carry (kb_sweep for mg_sweep, ...)."""
+    """Symbols the launch of phase `name` may carry: batched (kb_sweep for mg_sweep, kb_g2p_tile
+    for g2p, ...), single-simulation (k_cg_iter, k_g2p_tile, ...) or a surface stage (k_s_splat_lds)."""
     base = name[3:] if name.startswith("mg_") and name not in ("mg_tail", "mg_types", "mg_sten") else name
-    return [f"kb_{name}", f"kb_{base}", f"kb_{name}_tile"]
+    out = [f"kb_{name}", f"kb_{base}", f"kb_{name}_tile", f"k_{name}", f"k_{name}_tile", f"k_{name}_lds"]
+    return out + ([SURF_KERNELS[name]] if name in SURF_KERNELS else [])
 
 
 def pmc_counters(csv_dir: str) -> dict:
@@ -590,7 +586,8 @@ def pmc_counters(csv_dir: str) -> dict:
         per = {}
         with open(path, newline="") as f:
             for row in csv.DictReader(f):
-                name = row["Kernel_Name"].replace("(anonymous namespace)", "anon").split("(")[0].split("::")[-1]
+                name = row["Kernel_Name"].replace("(anonymous namespace)", "anon").split("(")[0]
+                name = name.split("<")[0].split("::")[-1].split(" ")[-1]
                 per.setdefault((name, row["Dispatch_Id"]), {})[row["Counter_Name"]] = float(row["Counter_Value"])
         for (name, _), ctr in per.items():
             out.setdefault(name, []).append(ctr)
@@ -652,15 +649,29 @@ def measured_traffic_pmc(kernel: str, probe_args: list, timeout: float = 240.0):
 
 def bench_probe(args, pxl):
     """Counter probe, run under rocprofv3 --pmc by measured_traffic_pmc: a device-to-device copy
-    of CALIB_BYTES (the calibration), then --probe-steps event-timed batched steps of the fixed
-    batch of the first --profile-batch stream requests (the same launches the roofline prices);
-    prints their per-kernel algorithmic bytes per launch."""
-    from tests.scenes import FPS, random_request
-    sizes = tuple(int(v) for v in args.sizes.split(","))
+    of CALIB_BYTES (the calibration), then the launches the roofline of --probe-kind prices:
+    serve -- --probe-steps event-timed batched steps of the fixed batch of the first
+    --profile-batch stream requests; dam -- two steps of the 128^3 dam break (FP32 CG 80) after
+    one; frame -- the 128^3 dam break's surface pass after three frames."""
+    from tests.scenes import FPS, dam_break, random_request
     a, b = pxl.DeviceBuffer(CALIB_BYTES), pxl.DeviceBuffer(CALIB_BYTES)
     b.copy_from(a)
     a.free()
     b.free()
+    if args.probe_kind == "dam":
+        sim = pxl.FluidState(dam_break(128, iters=80), FPS, seed=0, solver="cg", cg_iterations=80)
+        sim.step(1)
+        sim.profile_steps(2)
+        sim.close()
+        return {"probe": "dam"}
+    if args.probe_kind == "frame":
+        f = dam_break(128, iters=80)
+        sim = pxl.FluidState(f, FPS, seed=0, solver="cg", cg_iterations=80)
+        sim.step(3 * f["substeps"])
+        sim.profile_surface(1)
+        sim.close()
+        return {"probe": "frame"}
+    sizes = tuple(int(v) for v in args.sizes.split(","))
     reqs = [random_request(i, sizes=sizes) for i in range(args.profile_batch)]
     with pxl.Batch(reqs, FPS, seeds=list(range(len(reqs)))) as bat:
         prof = bat.profile_steps(args.probe_steps)
@@ -750,6 +761,7 @@ def main():
     ap.add_argument("--plain-every", type=int, default=8, help="plain path on every k-th request")
     ap.add_argument("--profile-batch", type=int, default=64)
     ap.add_argument("--probe-steps", type=int, default=2, help="probe: event-timed batched steps")
+    ap.add_argument("--probe-kind", default="serve", choices=["serve", "dam", "frame"], help="probe: which launches")
     ap.add_argument("--profile-every", type=int, default=20,
                     help="serve: every k-th batched step of the capacity run is event-timed per launch (roofline)")
     ap.add_argument("--traffic", type=int, default=1, help="serve: measure the dominant kernel's HBM traffic with "

@@ -89,6 +89,28 @@ typedef struct pxlflip_info pxlflip_info;
 int pxlflip_check_json(const char *fluid_json, double fps, uint32_t seed, const pxlflip_opts *opts,
                        pxlflip_info *info);
 
+/* Whole scene documents.  Replaces Scene::from_json / Scene::validate (crates/pxl-scene/src/lib.rs:
+ * 646-745) for what the fluid path reads and the summary of `pxl check` (crates/pxl-cli/src/check.rs:
+ * 31-106): the document is parsed strictly at the top level (unknown scene fields are refused, the
+ * version must be 1, a camera node must exist), the render block strictly (resolution, fps > 0,
+ * seed, motion_blur), every fluids[i] block strictly with Scene::validate's messages, mesh
+ * materials' program names checked; the contents of nodes, world, plate, simulations and programs
+ * are outside the fluid path and not interpreted.  pxlflip_scene_check touches no device; line gets
+ * "ok: N nodes, frames a..b, S simulation(s), F fluid(s)" (check.rs:98-104, without compiling
+ * programs).  pxlflip_create_from_scene creates fluids[fluid_index] with the scene's render.fps and
+ * render.seed (lib.rs:624-627).  opts as for pxlflip_create_from_json (max_cells applies to every
+ * fluid; scene_index is set to fluid_index). */
+typedef struct pxlflip_scene_info {
+    uint32_t version, n_nodes, n_simulations, n_fluids;
+    double fps;
+    uint32_t seed;
+    int32_t frame_start, frame_end; /* the animation's frame range; 0..0 without animation */
+    uint32_t pad;
+} pxlflip_scene_info;
+int pxlflip_scene_check(const char *scene_json, const pxlflip_opts *opts, pxlflip_scene_info *info, char *line,
+                        size_t line_len);
+int pxlflip_create_from_scene(const char *scene_json, uint32_t fluid_index, const pxlflip_opts *opts, pxlflip **out);
+
 /* Host seeding only (fluid.rs:123-148): writes the initial particles of `fluid_json`
  * for `seed` as 8-float records into aos8 (capacity max_particles) and their number into
  * *count.  With aos8 NULL only counts. */
@@ -193,7 +215,17 @@ int pxlflip_surface(pxlflip *h, float *verts, uint32_t vbase, uint32_t *indices,
  * descriptor belongs to HIP.  No reference counterpart (the reference meshes inside Vulkan). */
 typedef struct pxlflip_external pxlflip_external;
 int pxlflip_import_buffer_fd(int fd, uint64_t bytes, int device, void **dev_ptr, pxlflip_external **out);
-/* Unmaps and releases an imported buffer. */
+/* The same for a memory block holding several buffers (the renderer's vertex and index buffers
+ * are suballocated by gpu-allocator, crates/pxl-gpu/src/lib.rs:186-213, so each is a byte range of
+ * one VkDeviceMemory): imports the whole exported allocation of `bytes` bytes once (flags:
+ * PXLFLIP_IMPORT_DEDICATED when it was a dedicated allocation, VkMemoryDedicatedAllocateInfo), then
+ * pxlflip_external_map gives device memory for the range [offset, offset + bytes) -- the buffer's
+ * memory offset and size (vkGetBufferMemoryRequirements / the allocation's offset) -- as often as
+ * needed.  pxlflip_release_buffer unmaps every range and releases the import. */
+#define PXLFLIP_IMPORT_DEDICATED 1u
+int pxlflip_import_memory_fd(int fd, uint64_t bytes, uint32_t flags, int device, pxlflip_external **out);
+int pxlflip_external_map(pxlflip_external *ext, uint64_t offset, uint64_t bytes, void **dev_ptr);
+/* Unmaps and releases an imported buffer or memory block. */
 void pxlflip_release_buffer(pxlflip_external *ext);
 
 /* The same mesh copied to host arrays (capacity as above; only the written counts are copied).

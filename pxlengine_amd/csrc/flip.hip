@@ -25,11 +25,14 @@ struct pxlflip {
     Sim sim;
 };
 
-// A buffer of another API imported through an exported file descriptor.
+// Memory of another API imported through an exported file descriptor, and the buffers mapped
+// from it (each a byte range of the allocation, as a suballocated Vulkan buffer is).
 struct pxlflip_external {
     hipExternalMemory_t mem;
-    void *ptr;
+    void *ptr;  // the first mapping (pxlflip_import_buffer_fd)
     int device;
+    uint64_t bytes;
+    std::vector<void *> maps;
 };
 
 extern "C" {
@@ -55,9 +58,9 @@ int pxlflip_default_opts(pxlflip_opts *o) {
 namespace pxl {
 
 int prepare(const char *fluid_json, double fps, uint32_t seed, const pxlflip_opts *opts, Sim &s,
-            std::vector<float> *x0, std::vector<float> *v0) {
+            std::vector<float> *x0, std::vector<float> *v0, const FluidSpec *parsed) {
     g_create_error.clear();
-    if (!fluid_json) {
+    if (!fluid_json && !parsed) {
         g_create_error = "fluid_json is NULL";
         return PXLFLIP_ERR_INVALID;
     }
@@ -87,7 +90,8 @@ int prepare(const char *fluid_json, double fps, uint32_t seed, const pxlflip_opt
         return PXLFLIP_ERR_INVALID;
     }
     FluidSpec spec;
-    std::string err = parse_fluid(fluid_json, spec);
+    std::string err = parsed ? std::string() : parse_fluid(fluid_json, spec);
+    if (parsed) spec = *parsed;
     if (err.empty()) err = validate_fluid(spec, o.scene_index, o.max_cells ? (uint64_t)o.max_cells : REFERENCE_MAX_CELLS);
     if (!err.empty()) {
         g_create_error = err;
@@ -337,8 +341,12 @@ int pxlflip_seed_particles(const char *fluid_json, uint32_t seed, float *aos8, u
     return PXLFLIP_OK;
 }
 
-int pxlflip_create_from_json(const char *fluid_json, double fps, uint32_t seed, const pxlflip_opts *opts,
-                             pxlflip **out) {
+}  // extern "C"
+
+namespace {
+// FluidState::new for a fluid block given as JSON text or already parsed (from a scene).
+int create_sim(const char *fluid_json, const pxl::FluidSpec *parsed, double fps, uint32_t seed,
+               const pxlflip_opts *opts, pxlflip **out) {
     using namespace pxl;
     if (!out) {
         g_create_error = "out is NULL";
@@ -351,7 +359,7 @@ int pxlflip_create_from_json(const char *fluid_json, double fps, uint32_t seed, 
         return PXLFLIP_ERR_OOM;
     }
     Sim &s = h->sim;
-    int rc = prepare(fluid_json, fps, seed, opts, s, nullptr, nullptr);
+    int rc = prepare(fluid_json, fps, seed, opts, s, nullptr, nullptr, parsed);
     if (rc != PXLFLIP_OK) {
         delete h;
         return rc;
@@ -377,6 +385,75 @@ int pxlflip_create_from_json(const char *fluid_json, double fps, uint32_t seed, 
     }
     *out = h;
     return PXLFLIP_OK;
+}
+
+// Scene::from_json with the opts' cell limit; "" or the message.
+std::string load_scene(const char *scene_json, const pxlflip_opts *opts, pxl::SceneSpec &sc) {
+    if (!scene_json) return "scene_json is NULL";
+    if (opts && opts->struct_size != sizeof(pxlflip_opts)) return "pxlflip_opts.struct_size mismatch (use pxlflip_default_opts)";
+    const uint64_t max_cells = opts && opts->max_cells ? (uint64_t)opts->max_cells : pxl::REFERENCE_MAX_CELLS;
+    return pxl::parse_scene(scene_json, sc, max_cells);
+}
+}  // namespace
+
+extern "C" {
+
+int pxlflip_create_from_json(const char *fluid_json, double fps, uint32_t seed, const pxlflip_opts *opts,
+                             pxlflip **out) {
+    return create_sim(fluid_json, nullptr, fps, seed, opts, out);
+}
+
+int pxlflip_scene_check(const char *scene_json, const pxlflip_opts *opts, pxlflip_scene_info *info, char *line,
+                        size_t line_len) {
+    pxl::g_create_error.clear();
+    pxl::SceneSpec sc;
+    const std::string err = load_scene(scene_json, opts, sc);
+    if (!err.empty()) {
+        pxl::g_create_error = err;
+        return PXLFLIP_ERR_INVALID;
+    }
+    // each fluid must also seed particles (fluid.rs:149), as FluidState::new would report
+    for (size_t i = 0; i < sc.fluids.size(); i++) {
+        if (pxl::seed_particles(sc.fluids[i], sc.seed, nullptr, nullptr) == 0) {
+            pxl::g_create_error = "fluid '" + sc.fluids[i].name + "': fill blocks contain no grid cells";
+            return PXLFLIP_ERR_INVALID;
+        }
+    }
+    if (info) {
+        std::memset(info, 0, sizeof *info);
+        info->version = sc.version;
+        info->n_nodes = sc.n_nodes;
+        info->n_simulations = sc.n_simulations;
+        info->n_fluids = (uint32_t)sc.fluids.size();
+        info->fps = (double)sc.fps;
+        info->seed = sc.seed;
+        info->frame_start = sc.animated ? sc.frame_start : 0;
+        info->frame_end = sc.animated ? sc.frame_end : 0;
+    }
+    if (line && line_len) std::snprintf(line, line_len, "%s", pxl::check_line(sc).c_str());
+    return PXLFLIP_OK;
+}
+
+int pxlflip_create_from_scene(const char *scene_json, uint32_t fluid_index, const pxlflip_opts *opts, pxlflip **out) {
+    pxl::g_create_error.clear();
+    if (out) *out = nullptr;
+    pxl::SceneSpec sc;
+    const std::string err = load_scene(scene_json, opts, sc);
+    if (!err.empty()) {
+        pxl::g_create_error = err;
+        return PXLFLIP_ERR_INVALID;
+    }
+    if (fluid_index >= sc.fluids.size()) {
+        pxl::g_create_error = "the scene has " + std::to_string(sc.fluids.size()) + " fluid(s), no fluids[" +
+                              std::to_string(fluid_index) + "]";
+        return PXLFLIP_ERR_INVALID;
+    }
+    pxlflip_opts o;
+    pxlflip_default_opts(&o);
+    if (opts) o = *opts;
+    o.scene_index = fluid_index;
+    // render.fps and render.seed of the scene (lib.rs:624-627; fluid.rs:120, 164)
+    return create_sim(nullptr, &sc.fluids[fluid_index], (double)sc.fps, sc.seed, &o, out);
 }
 
 void pxlflip_destroy(pxlflip *h) {
@@ -484,45 +561,84 @@ void fill_surface_info(const pxl::Sim &s, const uint32_t ctr[2], pxlflip_surface
 }
 }  // namespace
 
-int pxlflip_import_buffer_fd(int fd, uint64_t bytes, int device, void **dev_ptr, pxlflip_external **out) {
+int pxlflip_import_memory_fd(int fd, uint64_t bytes, uint32_t flags, int device, pxlflip_external **out) {
     pxl::g_create_error.clear();
-    if (!dev_ptr || !out || fd < 0 || bytes == 0) {
-        pxl::g_create_error = "import_buffer_fd needs a file descriptor, a size and output pointers";
+    if (!out || fd < 0 || bytes == 0 || (flags & ~PXLFLIP_IMPORT_DEDICATED)) {
+        pxl::g_create_error = "import_memory_fd needs a file descriptor, a size, known flags and an output pointer";
         return PXLFLIP_ERR_INVALID;
     }
-    *dev_ptr = nullptr;
     *out = nullptr;
-    auto fail_hip = [](const char *what, hipError_t e) {
-        pxl::g_create_error = std::string(what) + ": " + hipGetErrorString(e);
-        return PXLFLIP_ERR_DEVICE;
-    };
     hipError_t e = hipSetDevice(device);
-    if (e != hipSuccess) return fail_hip("hipSetDevice", e);
+    if (e != hipSuccess) {
+        pxl::g_create_error = std::string("hipSetDevice: ") + hipGetErrorString(e);
+        return PXLFLIP_ERR_DEVICE;
+    }
     hipExternalMemoryHandleDesc desc;
     std::memset(&desc, 0, sizeof desc);
     desc.type = hipExternalMemoryHandleTypeOpaqueFd;
     desc.handle.fd = fd;
     desc.size = bytes;
+    desc.flags = (flags & PXLFLIP_IMPORT_DEDICATED) ? hipExternalMemoryDedicated : 0u;
     hipExternalMemory_t mem = nullptr;
     e = hipImportExternalMemory(&mem, &desc);
-    if (e != hipSuccess) return fail_hip("hipImportExternalMemory", e);
-    hipExternalMemoryBufferDesc bd;
-    std::memset(&bd, 0, sizeof bd);
-    bd.offset = 0;
-    bd.size = bytes;
-    void *p = nullptr;
-    e = hipExternalMemoryGetMappedBuffer(&p, mem, &bd);
     if (e != hipSuccess) {
-        (void)hipDestroyExternalMemory(mem);
-        return fail_hip("hipExternalMemoryGetMappedBuffer", e);
+        pxl::g_create_error = std::string("hipImportExternalMemory: ") + hipGetErrorString(e);
+        return PXLFLIP_ERR_DEVICE;
     }
-    auto *x = new (std::nothrow) pxlflip_external{mem, p, device};
+    auto *x = new (std::nothrow) pxlflip_external{mem, nullptr, device, bytes, {}};
     if (!x) {
-        (void)hipFree(p);
         (void)hipDestroyExternalMemory(mem);
+        pxl::g_create_error = "out of memory";
         return PXLFLIP_ERR_OOM;
     }
+    *out = x;
+    return PXLFLIP_OK;
+}
+
+int pxlflip_external_map(pxlflip_external *x, uint64_t offset, uint64_t bytes, void **dev_ptr) {
+    pxl::g_create_error.clear();
+    if (!x || !dev_ptr || bytes == 0 || offset > x->bytes || bytes > x->bytes - offset) {
+        pxl::g_create_error = "external_map: the range must lie inside the imported allocation";
+        return PXLFLIP_ERR_INVALID;
+    }
+    *dev_ptr = nullptr;
+    hipError_t e = hipSetDevice(x->device);
+    if (e != hipSuccess) {
+        pxl::g_create_error = std::string("hipSetDevice: ") + hipGetErrorString(e);
+        return PXLFLIP_ERR_DEVICE;
+    }
+    hipExternalMemoryBufferDesc bd;
+    std::memset(&bd, 0, sizeof bd);
+    bd.offset = offset;
+    bd.size = bytes;
+    void *p = nullptr;
+    e = hipExternalMemoryGetMappedBuffer(&p, x->mem, &bd);
+    if (e != hipSuccess) {
+        pxl::g_create_error = std::string("hipExternalMemoryGetMappedBuffer: ") + hipGetErrorString(e);
+        return PXLFLIP_ERR_DEVICE;
+    }
+    x->maps.push_back(p);
     *dev_ptr = p;
+    return PXLFLIP_OK;
+}
+
+int pxlflip_import_buffer_fd(int fd, uint64_t bytes, int device, void **dev_ptr, pxlflip_external **out) {
+    if (!dev_ptr || !out) {
+        pxl::g_create_error = "import_buffer_fd needs a file descriptor, a size and output pointers";
+        return PXLFLIP_ERR_INVALID;
+    }
+    *dev_ptr = nullptr;
+    pxlflip_external *x = nullptr;
+    int rc = pxlflip_import_memory_fd(fd, bytes, 0u, device, &x);
+    if (rc != PXLFLIP_OK) return rc;
+    rc = pxlflip_external_map(x, 0, bytes, dev_ptr);
+    if (rc != PXLFLIP_OK) {
+        const std::string msg = pxl::g_create_error;
+        pxlflip_release_buffer(x);
+        pxl::g_create_error = msg;
+        return rc;
+    }
+    x->ptr = *dev_ptr;
     *out = x;
     return PXLFLIP_OK;
 }
@@ -530,7 +646,7 @@ int pxlflip_import_buffer_fd(int fd, uint64_t bytes, int device, void **dev_ptr,
 void pxlflip_release_buffer(pxlflip_external *x) {
     if (!x) return;
     (void)hipSetDevice(x->device);
-    (void)hipFree(x->ptr);
+    for (void *p : x->maps) (void)hipFree(p);
     (void)hipDestroyExternalMemory(x->mem);
     delete x;
 }

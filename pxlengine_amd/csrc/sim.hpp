@@ -563,7 +563,7 @@ struct DeviceBuffer {
 // Parse + validate + seed + derive the device parameters; no device work.  Returns 0 or an
 // error code with g_create_error set.
 int prepare(const char *fluid_json, double fps, uint32_t seed, const pxlflip_opts *opts, Sim &s,
-            std::vector<float> *x0, std::vector<float> *v0);
+            std::vector<float> *x0, std::vector<float> *v0, const FluidSpec *parsed = nullptr);
 void fill_info(const Sim &s, pxlflip_info *info);
 
 }  // namespace pxl

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cctype>
 #include <set>
 
 #include "json.hpp"
@@ -192,13 +193,10 @@ bool any_lt(const float a[3], const float b[3]) { return a[0] < b[0] || a[1] < b
 
 }  // namespace
 
-std::string parse_fluid(const std::string &json, FluidSpec &f) {
-    JsonValue root;
-    JsonError jerr;
-    if (!json_parse(json, root, jerr))
-        return "invalid fluid: " + jerr.message + " at line " + std::to_string(jerr.line) + " column " +
-               std::to_string(jerr.column);
-    if (root.kind != JsonValue::Object) return "invalid fluid: invalid type: " + json_describe(root) + ", expected struct Fluid";
+namespace {
+// The fields of one Fluid object (error messages without a prefix).
+std::string parse_fluid_value(const JsonValue &root, FluidSpec &f) {
+    if (root.kind != JsonValue::Object) return "invalid type: " + json_describe(root) + ", expected struct Fluid";
     f = FluidSpec();
     std::set<std::string> seen;
     bool has_name = false, has_bounds = false, has_fill = false;
@@ -277,12 +275,23 @@ std::string parse_fluid(const std::string &json, FluidSpec &f) {
                 f.n_materials = (uint32_t)v.items.size();
             }
         }
-        if (!e.empty()) return "invalid fluid: " + e;
+        if (!e.empty()) return e;
     }
-    if (!has_name) return "invalid fluid: missing field `name`";
-    if (!has_bounds) return "invalid fluid: missing field `bounds`";
-    if (!has_fill) return "invalid fluid: missing field `fill`";
+    if (!has_name) return "missing field `name`";
+    if (!has_bounds) return "missing field `bounds`";
+    if (!has_fill) return "missing field `fill`";
     return "";
+}
+}  // namespace
+
+std::string parse_fluid(const std::string &json, FluidSpec &f) {
+    JsonValue root;
+    JsonError jerr;
+    if (!json_parse(json, root, jerr))
+        return "invalid fluid: " + jerr.message + " at line " + std::to_string(jerr.line) + " column " +
+               std::to_string(jerr.column);
+    std::string e = parse_fluid_value(root, f);
+    return e.empty() ? e : "invalid fluid: " + e;
 }
 
 void fluid_grid(const FluidSpec &f, uint32_t n[3], float *h_out) {
@@ -394,6 +403,240 @@ uint64_t obstacle_cells(const FluidSpec &f, std::vector<uint32_t> &ids) {
                 }
     }
     return count;
+}
+
+namespace {
+// Scene (crates/pxl-scene/src/lib.rs:20-47, deny_unknown_fields) and RenderSettings (lib.rs:610-638)
+const char *const kSceneFields[] = {"version", "nodes", "world", "render", "animation", "programs",
+                                    "simulations", "plate", "fluids"};
+const char *const kRenderFields[] = {"resolution", "samples", "bounces", "exposure", "view", "seed",
+                                     "fps", "motion_blur", "denoise", "transparent"};
+const char *const kViews[] = {"agx", "aces", "standard"};  // View, rename_all lowercase
+const char *const kNodeTypes[] = {"mesh", "asset", "light", "camera"};  // Node, tag "type"
+
+// is_ident (lib.rs): a Slang function name
+bool is_ident(const std::string &p) {
+    if (p.empty() || !(std::isalpha((unsigned char)p[0]) || p[0] == '_')) return false;
+    for (char c : p)
+        if (!(std::isalnum((unsigned char)c) || c == '_')) return false;
+    return true;
+}
+
+std::string get_i32(const JsonValue &v, int32_t &out) {
+    if (v.kind != JsonValue::Number || !v.is_integer) return "invalid type: " + json_describe(v) + ", expected i32";
+    if (v.num < -2147483648.0 || v.num > 2147483647.0) return "invalid value: " + json_describe(v) + ", expected i32";
+    out = (int32_t)v.num;
+    return "";
+}
+
+const JsonValue *member(const JsonValue &o, const char *key) {
+    for (const auto &m : o.members)
+        if (m.first == key) return &m.second;
+    return nullptr;
+}
+
+std::string parse_render(const JsonValue &v, SceneSpec &sc) {
+    if (v.kind != JsonValue::Object) return "invalid type: " + json_describe(v) + ", expected struct RenderSettings";
+    std::set<std::string> seen;
+    for (const auto &m : v.members) {
+        const std::string &k = m.first;
+        const JsonValue &x = m.second;
+        if (!known(kRenderFields, k)) return "unknown field `" + k + "`, expected one of " + expected_list(kRenderFields);
+        if (!seen.insert(k).second) return "duplicate field `" + k + "`";
+        std::string e;
+        float f32 = 0.0f;
+        uint32_t u = 0;
+        if (k == "resolution") {
+            if (x.kind != JsonValue::Array) e = "invalid type: " + json_describe(x) + ", expected an array of length 2";
+            else if (x.items.size() != 2) e = "invalid length " + std::to_string(x.items.size()) + ", expected an array of length 2";
+            else {
+                e = get_u32(x.items[0], sc.resolution[0]);
+                if (e.empty()) e = get_u32(x.items[1], sc.resolution[1]);
+            }
+        } else if (k == "samples" || k == "bounces") {
+            if (x.kind != JsonValue::Null) e = get_u32(x, u);
+        } else if (k == "exposure") {
+            e = get_f32(x, f32);
+        } else if (k == "view") {
+            if (x.kind != JsonValue::String) e = "invalid type: " + json_describe(x) + ", expected enum View";
+            else if (!known(kViews, x.text)) e = "unknown variant `" + x.text + "`, expected one of " + expected_list(kViews);
+        } else if (k == "seed") {
+            e = get_u32(x, sc.seed);
+        } else if (k == "fps") {
+            e = get_f32(x, sc.fps);
+        } else if (k == "motion_blur") {
+            if (x.kind != JsonValue::Null) {
+                e = get_f32(x, sc.motion_blur);
+                sc.has_motion_blur = true;
+            }
+        } else if (k == "denoise" || k == "transparent") {
+            const bool opt = k == "denoise";
+            if (!(x.kind == JsonValue::Bool || (opt && x.kind == JsonValue::Null)))
+                e = "invalid type: " + json_describe(x) + ", expected a boolean";
+        }
+        if (!e.empty()) return e;
+    }
+    return "";
+}
+
+// Animation::end (crates/pxl-scene/src/anim.rs:24-35): frame_end, else the last key (or orbit
+// end frame) of any track, at least frame_start, rounded up.
+std::string parse_animation(const JsonValue &v, SceneSpec &sc) {
+    if (v.kind == JsonValue::Null) return "";
+    if (v.kind != JsonValue::Object) return "invalid type: " + json_describe(v) + ", expected struct Animation";
+    sc.animated = true;
+    int32_t start = 0, end = 0;
+    bool has_end = false;
+    if (const JsonValue *x = member(v, "frame_start")) {
+        std::string e = get_i32(*x, start);
+        if (!e.empty()) return e;
+    }
+    if (const JsonValue *x = member(v, "frame_end")) {
+        if (x->kind != JsonValue::Null) {
+            std::string e = get_i32(*x, end);
+            if (!e.empty()) return e;
+            has_end = true;
+        }
+    }
+    if (!has_end) {
+        double last = (double)start;
+        if (const JsonValue *tracks = member(v, "tracks")) {
+            if (tracks->kind != JsonValue::Array) return "invalid type: " + json_describe(*tracks) + ", expected a sequence";
+            for (const auto &t : tracks->items) {
+                if (t.kind != JsonValue::Object) continue;
+                const JsonValue *orbit = member(t, "orbit");
+                double f = 0.0;
+                if (orbit && orbit->kind == JsonValue::Object) {
+                    const JsonValue *fr = member(*orbit, "frames");
+                    if (fr && fr->kind == JsonValue::Array && fr->items.size() == 2 && fr->items[1].kind == JsonValue::Number)
+                        f = fr->items[1].num;
+                } else if (const JsonValue *keys = member(t, "keys")) {
+                    if (keys->kind == JsonValue::Array && !keys->items.empty()) {
+                        const JsonValue &k = keys->items.back();
+                        if (k.kind == JsonValue::Array && !k.items.empty() && k.items[0].kind == JsonValue::Number)
+                            f = k.items[0].num;
+                    }
+                }
+                last = std::max(last, f);
+            }
+        }
+        end = (int32_t)std::ceil(last);
+    }
+    sc.frame_start = start;
+    sc.frame_end = end;
+    return "";
+}
+}  // namespace
+
+std::string parse_scene(const std::string &json, SceneSpec &sc, uint64_t max_cells) {
+    JsonValue root;
+    JsonError jerr;
+    sc = SceneSpec();
+    if (!json_parse(json, root, jerr))
+        return "invalid scene: " + jerr.message + " at line " + std::to_string(jerr.line) + " column " +
+               std::to_string(jerr.column);
+    if (root.kind != JsonValue::Object) return "invalid scene: invalid type: " + json_describe(root) + ", expected struct Scene";
+    std::set<std::string> seen;
+    bool has_version = false, has_programs = false;
+    const JsonValue *nodes = nullptr;
+    for (const auto &m : root.members) {
+        const std::string &k = m.first;
+        const JsonValue &v = m.second;
+        std::string e;
+        if (!known(kSceneFields, k)) {
+            e = "unknown field `" + k + "`, expected one of " + expected_list(kSceneFields);
+        } else if (!seen.insert(k).second) {
+            e = "duplicate field `" + k + "`";
+        } else if (k == "version") {
+            e = get_u32(v, sc.version);
+            has_version = true;
+        } else if (k == "render") {
+            e = parse_render(v, sc);
+        } else if (k == "fluids") {
+            if (v.kind != JsonValue::Array) e = "invalid type: " + json_describe(v) + ", expected a sequence";
+            else {
+                for (const auto &item : v.items) {
+                    FluidSpec f;
+                    e = parse_fluid_value(item, f);
+                    if (!e.empty()) break;
+                    sc.fluids.push_back(f);
+                }
+            }
+        } else if (k == "animation") {
+            e = parse_animation(v, sc);
+        } else if (k == "nodes" || k == "simulations") {
+            if (v.kind != JsonValue::Array) e = "invalid type: " + json_describe(v) + ", expected a sequence";
+            else if (k == "nodes") {
+                nodes = &v;
+                for (const auto &n : v.items) {
+                    const JsonValue *t = n.kind == JsonValue::Object ? member(n, "type") : nullptr;
+                    if (!t) {
+                        e = n.kind == JsonValue::Object ? "missing field `type`"
+                                                        : "invalid type: " + json_describe(n) + ", expected internally tagged enum Node";
+                        break;
+                    }
+                    if (t->kind != JsonValue::String || !known(kNodeTypes, t->text)) {
+                        e = "unknown variant `" + (t->kind == JsonValue::String ? t->text : json_describe(*t)) +
+                            "`, expected one of " + expected_list(kNodeTypes);
+                        break;
+                    }
+                    if (t->text == "camera") sc.has_camera = true;
+                }
+                sc.n_nodes = (uint32_t)v.items.size();
+            } else {
+                sc.n_simulations = (uint32_t)v.items.size();
+            }
+        } else if (k == "programs") {
+            has_programs = v.kind != JsonValue::Null;
+            if (has_programs && v.kind != JsonValue::Object) e = "invalid type: " + json_describe(v) + ", expected struct Programs";
+        } else {  // world, plate: objects the fluid path does not read
+            const bool opt = k == "plate";
+            if (!(v.kind == JsonValue::Object || (opt && v.kind == JsonValue::Null)))
+                e = "invalid type: " + json_describe(v) + ", expected struct " + (k == "world" ? "World" : "Plate");
+        }
+        if (!e.empty()) return "invalid scene: " + e;
+    }
+    if (!has_version) return "invalid scene: missing field `version`";
+    // Scene::validate (lib.rs:659-745), the checks on what the fluid path reads, in its order
+    if (sc.version != SCENE_VERSION)
+        return "scene version " + std::to_string(sc.version) + " is not supported (expected " +
+               std::to_string(SCENE_VERSION) + ")";
+    if (!sc.has_camera)
+        return "scene has no camera node; add {\"type\":\"camera\",\"translation\":[x,y,z],\"target\":[0,0,0]}";
+    const uint32_t w = sc.resolution[0], h = sc.resolution[1];
+    if (w == 0 || h == 0 || w > 16384 || h > 16384)
+        return "render.resolution must be within 1..16384, got [" + std::to_string(w) + "," + std::to_string(h) + "]";
+    if (!(sc.fps > 0.0f)) return "render.fps must be > 0";
+    if (nodes) {  // material programs of mesh nodes (lib.rs:674-686)
+        for (const auto &n : nodes->items) {
+            const JsonValue *t = member(n, "type");
+            const JsonValue *mats = member(n, "materials");
+            if (!t || t->text != "mesh" || !mats || mats->kind != JsonValue::Array) continue;
+            for (const auto &mat : mats->items) {
+                const JsonValue *p = mat.kind == JsonValue::Object ? member(mat, "program") : nullptr;
+                if (!p || p->kind != JsonValue::String) continue;
+                if (!is_ident(p->text)) return "material program '" + p->text + "' must be the name of a Slang function";
+                if (!has_programs)
+                    return "a material uses program '" + p->text + "' but the scene has no `programs` block";
+            }
+        }
+    }
+    for (size_t i = 0; i < sc.fluids.size(); i++) {
+        std::string e = validate_fluid(sc.fluids[i], (uint32_t)i, max_cells);
+        if (!e.empty()) return e;
+    }
+    if (sc.has_motion_blur && !(sc.motion_blur >= 0.0f && sc.motion_blur <= 1.0f)) {
+        char buf[64];
+        std::snprintf(buf, sizeof buf, "%g", (double)sc.motion_blur);
+        return std::string("render.motion_blur is the shutter fraction of a frame, 0..1 (0.5 = 180\u00b0), got ") + buf;
+    }
+    return "";
+}
+
+std::string check_line(const SceneSpec &sc) {
+    const int32_t a = sc.animated ? sc.frame_start : 0, b = sc.animated ? sc.frame_end : 0;
+    return "ok: " + std::to_string(sc.n_nodes) + " nodes, frames " + std::to_string(a) + ".." + std::to_string(b) + ", " +
+           std::to_string(sc.n_simulations) + " simulation(s), " + std::to_string(sc.fluids.size()) + " fluid(s)";
 }
 
 SeedRange seed_range(const FluidBlockSpec &b, const uint32_t n[3], float h, const float origin[3]) {

@@ -63,6 +63,31 @@ std::string parse_fluid(const std::string &json, FluidSpec &out);
 constexpr uint64_t REFERENCE_MAX_CELLS = 16000000ull;
 std::string validate_fluid(const FluidSpec &f, uint32_t index, uint64_t max_cells = REFERENCE_MAX_CELLS);
 
+// A whole scene document as far as the fluid path reads it (Scene, crates/pxl-scene/src/lib.rs:20-47):
+// the render block (fps, seed; RenderSettings, lib.rs:610-638), every fluids[i] block, and what
+// Scene::validate and `pxl check` (crates/pxl-cli/src/check.rs:31-106) report.  Nodes,
+// simulations, world, plate and programs are checked for their outer type and the node tags
+// (a camera must exist; mesh materials' programs must be identifiers with a programs block);
+// their contents are outside the fluid path and are not interpreted.
+constexpr uint32_t SCENE_VERSION = 1;  // lib.rs:18
+struct SceneSpec {
+    uint32_t version = 0;
+    uint32_t resolution[2] = {640, 480};  // lib.rs:820
+    float fps = 24.0f;  // render.fps (lib.rs:813)
+    uint32_t seed = 0;  // render.seed
+    bool has_motion_blur = false;
+    float motion_blur = 0.0f;
+    bool has_camera = false;
+    uint32_t n_nodes = 0, n_simulations = 0;
+    bool animated = false;
+    int32_t frame_start = 0, frame_end = 0;
+    std::vector<FluidSpec> fluids;
+};
+// Scene::from_json's deserialisation + validate for the above; "" when valid.
+std::string parse_scene(const std::string &json, SceneSpec &out, uint64_t max_cells = 16000000ull);
+// The summary line `pxl check` prints (check.rs:98-104, without the compiled-programs part).
+std::string check_line(const SceneSpec &sc);
+
 // Fluid::grid: cells per axis and cell size, glam Vec3 f32 arithmetic.
 void fluid_grid(const FluidSpec &f, uint32_t n[3], float *h);
 

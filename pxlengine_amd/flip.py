@@ -80,6 +80,14 @@ class Info(C.Structure):
         }
 
 
+class SceneInfo(C.Structure):
+    _fields_ = [
+        ("version", C.c_uint32), ("n_nodes", C.c_uint32), ("n_simulations", C.c_uint32), ("n_fluids", C.c_uint32),
+        ("fps", C.c_double), ("seed", C.c_uint32), ("frame_start", C.c_int32), ("frame_end", C.c_int32),
+        ("pad", C.c_uint32),
+    ]
+
+
 class KernelTime(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("launches", C.c_uint32), ("total_ms", C.c_double),
                 ("alg_bytes", C.c_double)]
@@ -177,6 +185,39 @@ class DeviceBuffer:
     def __del__(self):
         try:
             self.free()
+        except Exception:
+            pass
+
+
+class ImportedMemory:
+    """An allocation of another API (e.g. the Vulkan memory block the renderer suballocates its
+    vertex and index buffers from) imported once from an exported file descriptor; map() gives
+    device memory for byte ranges of it."""
+
+    def __init__(self, fd: int, nbytes: int, dedicated: bool = False, device: int = 0):
+        self._lib = load_library()
+        self._h = C.c_void_p()
+        rc = self._lib.pxlflip_import_memory_fd(int(fd), int(nbytes), 1 if dedicated else 0, int(device),
+                                                C.byref(self._h))
+        if rc != 0:
+            raise FluidError(rc, self._lib.pxlflip_last_error(None).decode())
+        self.nbytes = int(nbytes)
+
+    def map(self, offset: int, nbytes: int) -> int:
+        p = C.c_void_p()
+        rc = self._lib.pxlflip_external_map(self._h, int(offset), int(nbytes), C.byref(p))
+        if rc != 0:
+            raise FluidError(rc, self._lib.pxlflip_last_error(None).decode())
+        return int(p.value)
+
+    def release(self):
+        if self._h:
+            self._lib.pxlflip_release_buffer(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.release()
         except Exception:
             pass
 
@@ -280,6 +321,8 @@ _H = C.c_void_p
 _SIGNATURES = {
     "pxlflip_default_opts": (C.c_int, [C.POINTER(Opts)]),
     "pxlflip_check_json": (C.c_int, [C.c_char_p, C.c_double, C.c_uint32, C.POINTER(Opts), C.POINTER(Info)]),
+    "pxlflip_scene_check": (C.c_int, [C.c_char_p, C.POINTER(Opts), C.POINTER(SceneInfo), C.c_char_p, C.c_size_t]),
+    "pxlflip_create_from_scene": (C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(Opts), C.POINTER(_H)]),
     "pxlflip_seed_particles": (C.c_int, [C.c_char_p, C.c_uint32, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
     "pxlflip_create_from_json": (C.c_int, [C.c_char_p, C.c_double, C.c_uint32, C.POINTER(Opts), C.POINTER(_H)]),
     "pxlflip_destroy": (None, [_H]),
@@ -287,6 +330,8 @@ _SIGNATURES = {
     "pxlflip_surface_host": (C.c_int, [_H, C.c_void_p, C.c_void_p, C.POINTER(SurfaceInfo)]),
     "pxlflip_last_substeps": (C.c_int, [_H, C.POINTER(C.c_uint32)]),
     "pxlflip_import_buffer_fd": (C.c_int, [C.c_int, C.c_uint64, C.c_int, C.POINTER(C.c_void_p), C.POINTER(_H)]),
+    "pxlflip_import_memory_fd": (C.c_int, [C.c_int, C.c_uint64, C.c_uint32, C.c_int, C.POINTER(_H)]),
+    "pxlflip_external_map": (C.c_int, [_H, C.c_uint64, C.c_uint64, C.POINTER(C.c_void_p)]),
     "pxlflip_release_buffer": (None, [_H]),
     "pxlgrid_table_size": (C.c_uint32, [C.c_uint32]),
     "pxlgrid_create": (C.c_int, [C.c_uint32, C.c_int, C.POINTER(_H)]),
@@ -413,6 +458,19 @@ def check(fluid, fps: float = 24.0, seed: int = 0, **opts) -> dict:
     return info.as_dict()
 
 
+def check_scene(scene, **opts) -> dict:
+    """Parse + validate a whole scene document for the fluid path without touching the GPU
+    (Scene::from_json / validate, crates/pxl-scene/src/lib.rs:646-745) and the `pxl check`
+    summary line (crates/pxl-cli/src/check.rs:98-104)."""
+    info = SceneInfo()
+    line = C.create_string_buffer(256)
+    o = make_opts(**opts)
+    _check(load_library().pxlflip_scene_check(_json(scene), C.byref(o), C.byref(info), line, 256))
+    return {"version": int(info.version), "nodes": int(info.n_nodes), "simulations": int(info.n_simulations),
+            "fluids": int(info.n_fluids), "fps": float(info.fps), "seed": int(info.seed),
+            "frames": (int(info.frame_start), int(info.frame_end)), "line": line.value.decode()}
+
+
 def seed_particles(fluid, seed: int = 0) -> np.ndarray:
     """Initial particles as (P, 8) float32 records (fluid.rs:123-148), computed on the host."""
     L = load_library()
@@ -431,6 +489,21 @@ class FluidState:
         self._h = C.c_void_p()
         o = make_opts(**opts)
         _check(self._lib.pxlflip_create_from_json(_json(fluid), fps, seed, C.byref(o), C.byref(self._h)))
+        self._finish(fps, seed)
+
+    @classmethod
+    def from_scene(cls, scene, fluid_index: int = 0, **opts) -> "FluidState":
+        """fluids[fluid_index] of a whole scene document, with its render.fps and render.seed."""
+        self = cls.__new__(cls)
+        self._lib = load_library()
+        self._h = C.c_void_p()
+        o = make_opts(**opts)
+        _check(self._lib.pxlflip_create_from_scene(_json(scene), int(fluid_index), C.byref(o), C.byref(self._h)))
+        sc = check_scene(scene, **opts)
+        self._finish(sc["fps"], sc["seed"])
+        return self
+
+    def _finish(self, fps, seed):
         info = Info()
         _check(self._lib.pxlflip_get_info(self._h, C.byref(info)), self._h)
         self.info = info.as_dict()

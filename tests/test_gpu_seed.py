@@ -47,3 +47,21 @@ def test_batch_members_seeded_bitwise(pxl):
         for k, (f, s) in enumerate(zip(fluids, [4, 5, 6, 7])):
             np.testing.assert_array_equal(b.read_particles(k).view(np.uint32),
                                           pxl.seed_particles(f, s).view(np.uint32))
+
+
+def test_create_from_scene_uses_render_block(pxl):
+    """fluids[i] of a whole scene (examples/dam-break.json, render.seed and render.fps set) is
+    created with the scene's seed and fps: its particles are the fluid block's seeding for
+    render.seed and its dt is f32(1 / (fps * substeps))."""
+    import json
+    import os
+    s = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "scenes", "dam-break.json")))
+    s["render"].update(seed=7, fps=30)
+    gpu = pxl.FluidState.from_scene(json.dumps(s), 0)
+    fluid = s["fluids"][0]
+    np.testing.assert_array_equal(gpu.read_particles().view(np.uint32), pxl.seed_particles(fluid, 7).view(np.uint32))
+    assert gpu.info["dt"] == np.float32(1.0 / (30 * 8)) and gpu.fps == 30.0 and gpu.seed == 7
+    gpu.step(1)
+    gpu.close()
+    with pytest.raises(pxl.FluidError, match="no fluids\\[1\\]"):
+        pxl.FluidState.from_scene(json.dumps(s), 1)

@@ -219,3 +219,47 @@ def test_surface_into_imported_buffers(pxl):
         vex.close()
         iex.close()
     gpu.close()
+
+
+@pytest.mark.parametrize("dedicated", [False, True], ids=["suballocated", "dedicated-flag"])
+def test_surface_into_suballocated_buffers(pxl, dedicated):
+    """Interop with a suballocator (the renderer's buffers are ranges of one VkDeviceMemory,
+    crates/pxl-gpu/src/lib.rs:186-213): one exported allocation is imported once and the vertex
+    and index buffers are mapped from it at non-zero memory offsets; meshing into them at
+    vbase / ibase writes exactly the mesh at offset + base, and the rest of the block (before,
+    between and after the buffers) is untouched, as the exporter's own mapping shows."""
+    import os
+    fluid = dam_break(24)
+    gpu = pxl.FluidState(fluid, FPS, seed=0)
+    gpu.step(2)
+    verts, tris, info = gpu.surface()
+    cap = gpu.reservation()[0]
+    vbase, ibase = 40, 72
+    vbytes, ibytes = 32 * (vbase + cap), 4 * (ibase + 3 * cap)
+    voff = 65536
+    ioff = (voff + vbytes + 65535) // 65536 * 65536 + 256
+    ex = _Exporter(ioff + ibytes + 4096)
+    mem = None
+    try:
+        fill = np.full(ex.size // 4, 0x7F7F7F7F, np.uint32)
+        ex.write(fill)
+        mem = pxl.ImportedMemory(os.dup(ex.fd), ex.size, dedicated=dedicated)
+        vptr, iptr = mem.map(voff, vbytes), mem.map(ioff, ibytes)
+        with pytest.raises(pxl.FluidError):
+            mem.map(ex.size - 16, 32)  # past the end of the allocation
+        info2 = gpu.surface_device(vptr, vbase, iptr, ibase)
+        assert info2 == info
+        got = ex.read(np.empty(ex.size // 4, np.uint32))
+        vw = got[voff // 4 + 8 * vbase: voff // 4 + 8 * (vbase + info["vertices"])].view(np.float32).reshape(-1, 8)
+        iw = got[ioff // 4 + ibase: ioff // 4 + ibase + 3 * info["triangles"]].reshape(-1, 3)
+        np.testing.assert_array_equal(vw, verts)
+        np.testing.assert_array_equal(iw, tris)
+        touched = np.zeros(len(got), bool)
+        touched[voff // 4 + 8 * vbase: voff // 4 + 8 * (vbase + info["vertices"])] = True
+        touched[ioff // 4 + ibase: ioff // 4 + ibase + 3 * info["triangles"]] = True
+        assert np.all(got[~touched] == 0x7F7F7F7F)
+    finally:
+        if mem is not None:
+            mem.release()
+        ex.close()
+    gpu.close()
