@@ -710,7 +710,7 @@ def bench_tune(args, pxl):
         "steps": steps, "warmup": 2, "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (random liquid boxes or spheres, seed = request index)",
         "config": {"workload": f"batch autotuner on the first {len(reqs)} serving-stream requests, MIXED to 1e-6",
-                   "grids": r["grids"], "baseline_ms_per_step": r["baseline_ms_per_step"],
+                   "grids": r["grids"], "shapes": r["shapes"], "baseline_ms_per_step": r["baseline_ms_per_step"],
                    "baseline_value": total / (r["baseline_ms_per_step"] / 1e3), "trials": r["trials"],
                    "cache": args.grids_out},
         "roofline": None, "cpu_baseline": None,

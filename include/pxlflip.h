@@ -358,6 +358,17 @@ typedef struct pxlflip_server_stats {
 #define PXLFLIP_GRID_COUNT 7
 int pxlflip_batch_get_grid(int family, uint32_t *workgroups);
 int pxlflip_batch_set_grid(int family, uint32_t workgroups);
+/* Workgroup sizes of the batched launch families compiled in two sizes (the batch autotuner's
+ * shape knobs; no counterpart in the reference): PXLFLIP_SHAPE_TILE, the per-tile P2G / G2P
+ * (1024 lanes, default, or 512), and PXLFLIP_SHAPE_BAND, the band passes faces, cells and
+ * project (256, default, or 512).  Lanes stride over a tile's particles, nodes or cells, so a size
+ * changes the speed and never the results.  Process-wide, read when a launch is recorded (batch
+ * graphs captured after the call); another size is PXLFLIP_ERR_INVALID. */
+#define PXLFLIP_SHAPE_TILE 0
+#define PXLFLIP_SHAPE_BAND 1
+#define PXLFLIP_SHAPE_COUNT 2
+int pxlflip_batch_get_shape(int family, uint32_t *lanes);
+int pxlflip_batch_set_shape(int family, uint32_t lanes);
 
 int pxlflip_server_default_opts(pxlflip_server_opts *opts);
 /* opts may be NULL.  Fails (PXLFLIP_ERR_DEVICE) when the device cannot be used. */

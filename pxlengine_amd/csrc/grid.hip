@@ -23,7 +23,7 @@ __host__ __device__ inline uint32_t band_grid(const DevSim &S) { return S.tile_w
 // Runs f(i, j, k) over tile t's nodes (faces are stored at the node of their low corner: the
 // tile's 8^3 nodes, plus the grid's upper boundary nodes on the last tile along an axis) or
 // cells (8^3, clipped at the grid edge).  Lanes take the box x-fastest.
-template <typename F>
+template <int NT = GBLOCK, typename F>
 __device__ __forceinline__ void for_tile_box(const DevSim &S, uint32_t t, bool nodes, F &&f) {
     const int ti = (int)(t % (uint32_t)S.ntx), tj = (int)((t / (uint32_t)S.ntx) % (uint32_t)S.nty),
               tk = (int)(t / ((uint32_t)S.ntx * (uint32_t)S.nty));
@@ -36,7 +36,7 @@ __device__ __forceinline__ void for_tile_box(const DevSim &S, uint32_t t, bool n
         ext[a] = end - lo[a];
     }
     const int total = ext[0] * ext[1] * ext[2];
-    for (int q = threadIdx.x; q < total; q += GBLOCK) {
+    for (int q = threadIdx.x; q < total; q += NT) {
         const int i = lo[0] + q % ext[0], j = lo[1] + (q / ext[0]) % ext[1], k = lo[2] + q / (ext[0] * ext[1]);
         f(i, j, k);
     }
@@ -168,25 +168,27 @@ __device__ __forceinline__ void faces_at(const DevSim &S, uint32_t act, const in
     if (has[1]) face_update(S, sum[1], vidx(S, i, j, k), S.gy, in_c && !is_solid(S, i, j - 1, k));
     if (has[2]) face_update(S, sum[2], widx(S, i, j, k), S.gz, in_c && !is_solid(S, i, j, k - 1));
 }
+template <int NT>
 __device__ __forceinline__ void faces_tile(const DevSim &S, uint32_t tl) {
     const int t[3] = {(int)(tl % (uint32_t)S.ntx), (int)((tl / (uint32_t)S.ntx) % (uint32_t)S.nty),
                       (int)(tl / ((uint32_t)S.ntx * (uint32_t)S.nty))};
     uint32_t act, core;
     neighbour_masks(S, t[0], t[1], t[2], act, core);
     if (core & NB_SELF) {
-        for_tile_box(S, tl, true, [&](int i, int j, int k) { faces_at(S, act, t, i, j, k); });
+        for_tile_box<NT>(S, tl, true, [&](int i, int j, int k) { faces_at(S, act, t, i, j, k); });
     } else {  // rim tile: only the layers a core neighbour touches
-        for_tile_box(S, tl, true, [&](int i, int j, int k) {
+        for_tile_box<NT>(S, tl, true, [&](int i, int j, int k) {
             if (rim_node_touched(core, i - TILE * t[0], j - TILE * t[1], k - TILE * t[2])) faces_at(S, act, t, i, j, k);
         });
     }
 }
 __device__ __forceinline__ void body_faces(const DevSim &S, uint32_t b, uint32_t nb) {
-    for_band(S, b, nb, [&](uint32_t tl) { faces_tile(S, tl); });
+    for_band(S, b, nb, [&](uint32_t tl) { faces_tile<GBLOCK>(S, tl); });
 }
 __global__ __launch_bounds__(GBLOCK) void k_faces(DevSim S) { body_faces(S, blockIdx.x, gridDim.x); }
-__global__ __launch_bounds__(GBLOCK) void kb_faces(WorkList W) {
-    for_batch_band(W, [&](const DevSim &S, uint32_t t) { faces_tile(S, t); });
+template <int NT>
+__global__ __launch_bounds__(NT) void kb_faces(WorkList W) {
+    for_batch_band(W, [&](const DevSim &S, uint32_t t) { faces_tile<NT>(S, t); });
 }
 
 // The P2G sums as the reference's acc / wacc face buffers (for read-back).
@@ -257,6 +259,7 @@ __device__ __forceinline__ void count_row_runs(const DevSim &S, uint32_t row, bo
 // (the row counting needs whole waves).
 // (A rim tile is skipped: its cells have no particles this step or the last, so they are AIR
 // or SOLID with zero pressure, divergence and stencil, as its last pass as a core tile left them.)
+template <int NT>
 __device__ __forceinline__ void cells_tile(const DevSim &S, uint32_t t) {
     if (S.tile_act[t] == 0) return;  // the whole workgroup
     const int ti = (int)(t % (uint32_t)S.ntx), tj = (int)((t / (uint32_t)S.ntx) % (uint32_t)S.nty),
@@ -264,7 +267,7 @@ __device__ __forceinline__ void cells_tile(const DevSim &S, uint32_t t) {
     const int lo[3] = {ti * TILE, tj * TILE, tk * TILE};
     const int ex = min(TILE, S.nx - lo[0]), ey = min(TILE, S.ny - lo[1]), ez = min(TILE, S.nz - lo[2]);
     const int total = ex * ey * ez;
-    for (int q0 = 0; q0 < total; q0 += GBLOCK) {
+    for (int q0 = 0; q0 < total; q0 += NT) {
         const int q = q0 + (int)threadIdx.x;
         uint32_t row = ~0u;
         bool fluid = false, red = false;
@@ -279,11 +282,12 @@ __device__ __forceinline__ void cells_tile(const DevSim &S, uint32_t t) {
     }
 }
 __device__ __forceinline__ void body_cells(const DevSim &S, uint32_t b, uint32_t nb) {
-    for_band(S, b, nb, [&](uint32_t t) { cells_tile(S, t); });
+    for_band(S, b, nb, [&](uint32_t t) { cells_tile<GBLOCK>(S, t); });
 }
 __global__ __launch_bounds__(GBLOCK) void k_cells(DevSim S) { body_cells(S, blockIdx.x, gridDim.x); }
-__global__ __launch_bounds__(GBLOCK) void kb_cells(WorkList W) {
-    for_batch_band(W, [&](const DevSim &S, uint32_t t) { cells_tile(S, t); });
+template <int NT>
+__global__ __launch_bounds__(NT) void kb_cells(WorkList W) {
+    for_batch_band(W, [&](const DevSim &S, uint32_t t) { cells_tile<NT>(S, t); });
 }
 
 // Fluid list from the scanned row counts: one wave per grid row writes the row's FLUID cells
@@ -368,9 +372,10 @@ __device__ __forceinline__ void project_at(const DevSim &S, int i, int j, int k)
     if (i < S.nx && k < S.nz) project_face(S, vidx(S, i, j, k), i, j - 1, k, i, j, k);
     if (i < S.nx && j < S.ny) project_face(S, widx(S, i, j, k), i, j, k - 1, i, j, k);
 }
+template <int NT>
 __device__ __forceinline__ void project_tile(const DevSim &S, uint32_t t) {
     if (S.tile_act[t] != 0) {
-        for_tile_box(S, t, true, [&](int i, int j, int k) { project_at(S, i, j, k); });
+        for_tile_box<NT>(S, t, true, [&](int i, int j, int k) { project_at(S, i, j, k); });
         return;
     }
     // rim tile: its cells are not FLUID, so only faces on its lower planes can border a FLUID
@@ -381,18 +386,19 @@ __device__ __forceinline__ void project_tile(const DevSim &S, uint32_t t) {
     neighbour_masks(S, ti, tj, tk, act, core);
     const bool below[3] = {((act >> 12) & 1u) != 0, ((act >> 10) & 1u) != 0, ((act >> 4) & 1u) != 0};
     if (!(below[0] || below[1] || below[2])) return;  // uniform: the masks are the workgroup's
-    for_tile_box(S, t, true, [&](int i, int j, int k) {
+    for_tile_box<NT>(S, t, true, [&](int i, int j, int k) {
         if (below[0] && i == TILE * ti && j < S.ny && k < S.nz) project_face(S, uidx(S, i, j, k), i - 1, j, k, i, j, k);
         if (below[1] && j == TILE * tj && i < S.nx && k < S.nz) project_face(S, vidx(S, i, j, k), i, j - 1, k, i, j, k);
         if (below[2] && k == TILE * tk && i < S.nx && j < S.ny) project_face(S, widx(S, i, j, k), i, j, k - 1, i, j, k);
     });
 }
 __device__ __forceinline__ void body_project(const DevSim &S, uint32_t b, uint32_t nb) {
-    for_band(S, b, nb, [&](uint32_t t) { project_tile(S, t); });
+    for_band(S, b, nb, [&](uint32_t t) { project_tile<GBLOCK>(S, t); });
 }
 __global__ __launch_bounds__(GBLOCK) void k_project(DevSim S) { body_project(S, blockIdx.x, gridDim.x); }
-__global__ __launch_bounds__(GBLOCK) void kb_project(WorkList W) {
-    for_batch_band(W, [&](const DevSim &S, uint32_t t) { project_tile(S, t); });
+template <int NT>
+__global__ __launch_bounds__(NT) void kb_project(WorkList W) {
+    for_batch_band(W, [&](const DevSim &S, uint32_t t) { project_tile<NT>(S, t); });
 }
 
 __global__ __launch_bounds__(TX *TY) void k_type_u32(DevSim S, uint32_t *out) {
@@ -428,11 +434,19 @@ uint32_t fluid_row_blocks(const DevSim &S) {
     return ((uint32_t)S.ny * (uint32_t)S.nz + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK;
 }
 
-void batch_faces(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_faces, batch_grid(W[WD_TILEGRID], batch_cap(BF_BAND)), GBLOCK, 0, s, W[WD_TILEGRID]); }
-void batch_cells(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_cells, batch_grid(W[WD_TILEGRID], batch_cap(BF_BAND)), GBLOCK, 0, s, W[WD_TILEGRID]); }
+// The band launches in the workgroup size the band-lane setting picks (256 or 512 lanes: lanes
+// stride over a tile's nodes or cells, so the size changes the speed, not the results).
+#define PXL_BAND_LAUNCH(kernel)                                                              \
+    do {                                                                                     \
+        const unsigned g_ = batch_grid(W[WD_TILEGRID], batch_cap(BF_BAND));                   \
+        if (batch_shape(BS_BAND) == 512) PXL_LAUNCH(kernel<512>, g_, 512, 0, s, W[WD_TILEGRID]); \
+        else PXL_LAUNCH(kernel<GBLOCK>, g_, GBLOCK, 0, s, W[WD_TILEGRID]);                    \
+    } while (0)
+void batch_faces(const WorkLists &W, hipStream_t s) { PXL_BAND_LAUNCH(kb_faces); }
+void batch_cells(const WorkLists &W, hipStream_t s) { PXL_BAND_LAUNCH(kb_cells); }
 void batch_fluid_compact(const WorkLists &W, hipStream_t s) {
     PXL_LAUNCH(kb_fluid_compact, batch_grid(W[WD_ROWS], batch_cap(BF_BAND)), BLOCK, 0, s, W[WD_ROWS]);
 }
-void batch_project(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_project, batch_grid(W[WD_TILEGRID], batch_cap(BF_BAND)), GBLOCK, 0, s, W[WD_TILEGRID]); }
+void batch_project(const WorkLists &W, hipStream_t s) { PXL_BAND_LAUNCH(kb_project); }
 
 }  // namespace pxl

@@ -364,6 +364,13 @@ enum BatchFamily : int {
 };
 unsigned batch_cap(int family);  // current maximum of a family (a multiple of 8)
 bool set_batch_cap(int family, unsigned workgroups);  // false: unknown family or value out of range
+// Batched launch families compiled in more than one workgroup size (the values are
+// PXLFLIP_SHAPE_* of pxlflip.h): the per-tile P2G / G2P (1024 or 512 lanes) and the band
+// passes faces, cells and project (256 or 512 lanes).  Lanes stride over a tile's particles,
+// nodes or cells, so the size changes the speed, not the results.
+enum BatchShape : int { BS_TILE = 0, BS_BAND, BS_COUNT };
+unsigned batch_shape(int family);                 // current lanes per workgroup
+bool set_batch_shape(int family, unsigned lanes);  // false: unknown family or a size not compiled
 
 // Persistent grid of a batched launch: the list's grid, at most the kernel's maximum and
 // never more than a list's minimum storage (for_each_work reads entry blockIdx.x unchecked).

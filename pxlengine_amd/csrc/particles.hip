@@ -328,21 +328,22 @@ __global__ __launch_bounds__(BLOCK) void kb_tile_list(WorkList W) {
 // in-grid corner of a particle binned in the tile lies in the box (the box spans the cells
 // [8t-1, 8t+8] around the tile; positions beyond the grid are binned to its edge cells, whose
 // corners beyond the grid are skipped).
+template <int NT>
 __device__ __forceinline__ void body_p2g(const DevSim &S, uint32_t b0, uint32_t nb) {
     __shared__ unsigned long long lbox[BOX_ENTRIES];
     const uint32_t n_active = *S.tile_count;
     // lanes of a wave take every 8th particle of a run, so concurrent LDS atomics land in
     // different cells
-    const uint32_t local = (threadIdx.x & (TILE_BLOCK / 8 - 1)) * 8u + (threadIdx.x / (TILE_BLOCK / 8));
+    const uint32_t local = (threadIdx.x & (NT / 8 - 1)) * 8u + (threadIdx.x / (NT / 8));
     const float o[3] = {S.ox, S.oy, S.oz};
     for (uint32_t u = b0; u < n_active; u += nb) {
         const uint32_t t = S.tile_list[u];
         const uint32_t beg = S.offset[t * TILE_CELLS], end = S.offset[(t + 1) * TILE_CELLS];
         int ti, tj, tk;
         tile_coords(S, t, ti, tj, tk);
-        for (int q = threadIdx.x; q < BOX_ENTRIES; q += TILE_BLOCK) lbox[q] = 0ull;
+        for (int q = threadIdx.x; q < BOX_ENTRIES; q += NT) lbox[q] = 0ull;
         __syncthreads();
-        for (uint32_t base = beg; base < end; base += TILE_BLOCK) {
+        for (uint32_t base = beg; base < end; base += NT) {
             const uint32_t p = base + local;
             if (p >= end) continue;
             const float x[3] = {S.bx[p], S.by[p], S.bz[p]};
@@ -388,7 +389,7 @@ __device__ __forceinline__ void body_p2g(const DevSim &S, uint32_t b0, uint32_t 
         // stored in the HBM box order (box_gidx): entry g of the store comes from LDS entry
         // c * BOX_FACES + lx + d0 (ly + d1 lz)
         int64_t *out = S.box + (size_t)t * BOX_ENTRIES;
-        for (int g = threadIdx.x; g < BOX_ENTRIES; g += TILE_BLOCK) {
+        for (int g = threadIdx.x; g < BOX_ENTRIES; g += NT) {
             const int c = g / BOX_FACES, r = g - c * BOX_FACES;
             const int d0 = box_d0(c), d1 = box_d1(c), d2 = c == 2 ? TILE + 1 : TILE + 2;
             int lx, yz;
@@ -405,9 +406,10 @@ __device__ __forceinline__ void body_p2g(const DevSim &S, uint32_t b0, uint32_t 
         __syncthreads();
     }
 }
-__global__ __launch_bounds__(TILE_BLOCK) void k_p2g_tile(DevSim S) { body_p2g(S, blockIdx.x, gridDim.x); }
-__global__ __launch_bounds__(TILE_BLOCK) void kb_p2g_tile(WorkList W) {
-    for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *) { body_p2g(S, b, tile_grid(S)); });
+__global__ __launch_bounds__(TILE_BLOCK) void k_p2g_tile(DevSim S) { body_p2g<TILE_BLOCK>(S, blockIdx.x, gridDim.x); }
+template <int NT>
+__global__ __launch_bounds__(NT) void kb_p2g_tile(WorkList W) {
+    for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *) { body_p2g<NT>(S, b, tile_grid(S)); });
 }
 
 // Trilinear stencil of sample_face (fluid.slang:67-80) for component c at one position:
@@ -517,6 +519,7 @@ __device__ __forceinline__ void obstacle_push(const DevSim &S, float x[3], float
 // falls back to HBM when it leaves the box) and the wall clamp with the wall-normal velocity
 // clipped (Slang max/min = SPIR-V FMax/FMin).  Reads the sorted scratch copy, writes the
 // current particle arrays at the same (sorted) index.
+template <int NT>
 __device__ __forceinline__ void body_g2p(const DevSim &S, uint32_t b0, uint32_t nb) {
     __shared__ float lvel[BOX_ENTRIES];
     __shared__ float lold[BOX_ENTRIES];
@@ -526,7 +529,7 @@ __device__ __forceinline__ void body_g2p(const DevSim &S, uint32_t b0, uint32_t 
     const uint32_t beg = S.offset[t * TILE_CELLS], end = S.offset[(t + 1) * TILE_CELLS];
     int ti, tj, tk;
     tile_coords(S, t, ti, tj, tk);
-    for (int q = threadIdx.x; q < BOX_ENTRIES; q += TILE_BLOCK) {
+    for (int q = threadIdx.x; q < BOX_ENTRIES; q += NT) {
         const int c = q / BOX_FACES, r = q - c * BOX_FACES;
         int lo[3], d[3];
         tile_box(c, ti, tj, tk, lo, d);
@@ -546,7 +549,7 @@ __device__ __forceinline__ void body_g2p(const DevSim &S, uint32_t b0, uint32_t 
     const float half_dt = 0.5f * S.dt;
     const float org[3] = {S.ox, S.oy, S.oz};
     const float nf[3] = {(float)S.nx, (float)S.ny, (float)S.nz};
-    for (uint32_t i = beg + threadIdx.x; i < end; i += TILE_BLOCK) {
+    for (uint32_t i = beg + threadIdx.x; i < end; i += NT) {
         float x[3] = {S.bx[i], S.by[i], S.bz[i]};
         const float v0[3] = {S.bvx[i], S.bvy[i], S.bvz[i]};
         float pic[3], old[3], v[3], mid[3], adv[3];
@@ -601,9 +604,10 @@ __device__ __forceinline__ void body_g2p(const DevSim &S, uint32_t b0, uint32_t 
     __syncthreads();
     }
 }
-__global__ __launch_bounds__(TILE_BLOCK) void k_g2p_tile(DevSim S) { body_g2p(S, blockIdx.x, gridDim.x); }
-__global__ __launch_bounds__(TILE_BLOCK) void kb_g2p_tile(WorkList W) {
-    for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *) { body_g2p(S, b, tile_grid(S)); });
+__global__ __launch_bounds__(TILE_BLOCK) void k_g2p_tile(DevSim S) { body_g2p<TILE_BLOCK>(S, blockIdx.x, gridDim.x); }
+template <int NT>
+__global__ __launch_bounds__(NT) void kb_g2p_tile(WorkList W) {
+    for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *) { body_g2p<NT>(S, b, tile_grid(S)); });
 }
 
 // ---------------------------------------------------------------- IO / bins
@@ -767,7 +771,17 @@ void batch_scan_down(const WorkLists &W, int which, hipStream_t s) {
 void batch_tile_list(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_tile_list, batch_grid(W[WD_TILES], batch_cap(BF_PART)), BLOCK, 0, s, W[WD_TILES]); }
 void batch_band(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_band, batch_grid(W[WD_TILES], batch_cap(BF_PART)), BLOCK, 0, s, W[WD_TILES]); }
 void batch_scatter(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_scatter, batch_grid(W[WD_PART], batch_cap(BF_PART)), BLOCK, 0, s, W[WD_PART]); }
-void batch_p2g(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_p2g_tile, batch_grid(W[WD_TILEGRID], batch_cap(BF_TILE)), TILE_BLOCK, 0, s, W[WD_TILEGRID]); }
-void batch_g2p(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_g2p_tile, batch_grid(W[WD_TILEGRID], batch_cap(BF_TILE)), TILE_BLOCK, 0, s, W[WD_TILEGRID]); }
+// The per-tile launches in the workgroup size the tile-lane setting picks (1024 or 512 lanes:
+// lanes stride over a tile's particles, so the size changes the speed, not the results).
+void batch_p2g(const WorkLists &W, hipStream_t s) {
+    const unsigned g = batch_grid(W[WD_TILEGRID], batch_cap(BF_TILE));
+    if (batch_shape(BS_TILE) == 512) PXL_LAUNCH(kb_p2g_tile<512>, g, 512, 0, s, W[WD_TILEGRID]);
+    else PXL_LAUNCH(kb_p2g_tile<TILE_BLOCK>, g, TILE_BLOCK, 0, s, W[WD_TILEGRID]);
+}
+void batch_g2p(const WorkLists &W, hipStream_t s) {
+    const unsigned g = batch_grid(W[WD_TILEGRID], batch_cap(BF_TILE));
+    if (batch_shape(BS_TILE) == 512) PXL_LAUNCH(kb_g2p_tile<512>, g, 512, 0, s, W[WD_TILEGRID]);
+    else PXL_LAUNCH(kb_g2p_tile<TILE_BLOCK>, g, TILE_BLOCK, 0, s, W[WD_TILEGRID]);
+}
 
 }  // namespace pxl

@@ -671,6 +671,22 @@ bool set_batch_cap(int family, unsigned workgroups) {
     g_batch_caps[family].store(workgroups, std::memory_order_relaxed);
     return true;
 }
+
+// Workgroup sizes of the families compiled in two sizes (the first is the default: on the
+// 64-request serving batch it was measured fastest when the sizes were fixed at compile time).
+constexpr unsigned kShapeSizes[BS_COUNT][2] = {{1024, 512}, {256, 512}};
+std::atomic<uint32_t> g_batch_shapes[BS_COUNT] = {kShapeSizes[BS_TILE][0], kShapeSizes[BS_BAND][0]};
+static_assert(BS_TILE == PXLFLIP_SHAPE_TILE && BS_BAND == PXLFLIP_SHAPE_BAND && BS_COUNT == PXLFLIP_SHAPE_COUNT,
+              "batch shape families match pxlflip.h");
+unsigned batch_shape(int family) {
+    return family >= 0 && family < BS_COUNT ? g_batch_shapes[family].load(std::memory_order_relaxed) : 0u;
+}
+bool set_batch_shape(int family, unsigned lanes) {
+    if (family < 0 || family >= BS_COUNT || (lanes != kShapeSizes[family][0] && lanes != kShapeSizes[family][1]))
+        return false;
+    g_batch_shapes[family].store(lanes, std::memory_order_relaxed);
+    return true;
+}
 }  // namespace pxl
 
 // Host threads that seed and allocate admitted requests (seeding a large liquid is several
@@ -1014,6 +1030,16 @@ int pxlflip_batch_get_grid(int family, uint32_t *workgroups) {
 
 int pxlflip_batch_set_grid(int family, uint32_t workgroups) {
     return pxl::set_batch_cap(family, workgroups) ? PXLFLIP_OK : PXLFLIP_ERR_INVALID;
+}
+
+int pxlflip_batch_get_shape(int family, uint32_t *lanes) {
+    if (!lanes || family < 0 || family >= pxl::BS_COUNT) return PXLFLIP_ERR_INVALID;
+    *lanes = pxl::batch_shape(family);
+    return PXLFLIP_OK;
+}
+
+int pxlflip_batch_set_shape(int family, uint32_t lanes) {
+    return pxl::set_batch_shape(family, lanes) ? PXLFLIP_OK : PXLFLIP_ERR_INVALID;
 }
 
 int pxlflip_server_default_opts(pxlflip_server_opts *o) {

@@ -361,6 +361,8 @@ _SIGNATURES = {
     "pxlflip_profile_surface": (C.c_int, [_H, C.c_uint32, C.POINTER(KernelTime), C.c_uint32, C.POINTER(C.c_uint32)]),
     "pxlflip_autotune": (C.c_int, [_H, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_double)]),
     "pxlflip_batch_get_grid": (C.c_int, [C.c_int, C.POINTER(C.c_uint32)]),
+    "pxlflip_batch_get_shape": (C.c_int, [C.c_int, C.POINTER(C.c_uint32)]),
+    "pxlflip_batch_set_shape": (C.c_int, [C.c_int, C.c_uint32]),
     "pxlflip_batch_set_grid": (C.c_int, [C.c_int, C.c_uint32]),
     "pxlflip_last_error": (C.c_char_p, [_H]),
     "pxlflip_server_default_opts": (C.c_int, [C.POINTER(ServerOpts)]),
@@ -845,6 +847,31 @@ class Batch:
 
 # PXLFLIP_GRID_* of pxlflip.h, in order
 GRID_FAMILIES = ("band", "particle", "tile", "mg_setup", "mg_list", "mg_tail", "solver")
+# PXLFLIP_SHAPE_* of pxlflip.h, in order, with the workgroup sizes compiled for each (default first)
+SHAPE_FAMILIES = {"tile_lanes": (1024, 512), "band_lanes": (256, 512)}
+
+
+def batch_shapes() -> dict:
+    """The workgroup size of every batched launch family compiled in more than one size."""
+    lib = load_library()
+    out = {}
+    for i, name in enumerate(SHAPE_FAMILIES):
+        v = C.c_uint32()
+        _check(lib.pxlflip_batch_get_shape(i, C.byref(v)))
+        out[name] = int(v.value)
+    return out
+
+
+def set_batch_shapes(shapes: dict):
+    """Sets the workgroup sizes of the named shape families (one of their compiled sizes)."""
+    lib = load_library()
+    names = list(SHAPE_FAMILIES)
+    for name, v in shapes.items():
+        if name not in SHAPE_FAMILIES:
+            raise KeyError(f"unknown batch shape family {name!r} (one of {', '.join(names)})")
+        rc = lib.pxlflip_batch_set_shape(names.index(name), int(v))
+        if rc != OK:
+            raise FluidError(rc, f"batch shape {name}: {v} is not one of {SHAPE_FAMILIES[name]}")
 
 
 def batch_grids() -> dict:
@@ -871,20 +898,24 @@ def set_batch_grids(grids: dict):
             raise FluidError(rc, f"batch grid {name}: {v} is not a multiple of 8 in [8, 16384]")
 
 
-def save_batch_grids(path: str, grids: Optional[dict] = None, **extra):
-    """Writes the grids (default: the current ones) to a JSON cache file."""
-    doc = {"grids": dict(grids if grids is not None else batch_grids())}
+def save_batch_grids(path: str, grids: Optional[dict] = None, shapes: Optional[dict] = None, **extra):
+    """Writes the grids and workgroup sizes (default: the current ones) to a JSON cache file."""
+    doc = {"grids": dict(grids if grids is not None else batch_grids()),
+           "shapes": dict(shapes if shapes is not None else batch_shapes())}
     doc.update(extra)
     with open(path, "w") as f:
         json.dump(doc, f, indent=1, sort_keys=True)
 
 
 def load_batch_grids(path: str) -> dict:
-    """Applies the grids of a JSON cache file written by save_batch_grids / autotune_batch."""
+    """Applies the grids (and workgroup sizes, when present) of a JSON cache file written by
+    save_batch_grids / autotune_batch; returns the grids."""
     with open(path) as f:
-        grids = json.load(f)["grids"]
-    set_batch_grids(grids)
-    return grids
+        doc = json.load(f)
+    set_batch_grids(doc["grids"])
+    if "shapes" in doc:
+        set_batch_shapes(doc["shapes"])
+    return doc["grids"]
 
 
 def _time_batch(fluids, fps, steps, warmup, opts) -> float:
@@ -897,23 +928,36 @@ def _time_batch(fluids, fps, steps, warmup, opts) -> float:
 
 def autotune_batch(fluids, fps: float = 24.0, steps: int = 3, warmup: int = 1, families=None,
                    factors=(0.5, 2.0), min_gain: float = 0.01, cache: Optional[str] = None, **opts) -> dict:
-    """Picks the batched launch grids for a workload by measurement: coordinate descent over
-    the families (default all), trying each one's maximum times `factors` (rounded to a
-    multiple of 8, within [8, 16384]) on a fresh fixed batch of `fluids` (`warmup` untimed
+    """Picks the batched launch grids and workgroup sizes for a workload by measurement:
+    coordinate descent over the families (default all grid families, then the shape families
+    tile_lanes and band_lanes), trying each grid maximum times `factors` (rounded to a
+    multiple of 8, within [8, 16384]) and each other compiled workgroup size of a shape family
+    on a fresh fixed batch of `fluids` (`warmup` untimed
     and `steps` timed steps each) and keeping a value only when it is at least `min_gain`
     faster than the best so far.  One untimed trial with the current grids comes first (the
     process's first batch runs slower).  Leaves the best grids set, writes them to `cache`
-    (JSON) when given, and returns {"grids", "ms_per_step", "baseline_ms_per_step",
-    "trials"}.  Results of a batch do not depend on the grids, only its speed."""
-    fams = tuple(families) if families is not None else GRID_FAMILIES
+    (JSON) when given, and returns {"grids", "shapes", "ms_per_step", "baseline_ms_per_step",
+    "trials"}.  Results of a batch do not depend on the grids or sizes, only its speed."""
+    fams = tuple(families) if families is not None else GRID_FAMILIES + tuple(SHAPE_FAMILIES)
     for name in fams:
-        if name not in GRID_FAMILIES:
-            raise KeyError(f"unknown batch grid family {name!r}")
-    best = batch_grids()
+        if name not in GRID_FAMILIES and name not in SHAPE_FAMILIES:
+            raise KeyError(f"unknown batch grid or shape family {name!r}")
+    best, shapes = batch_grids(), batch_shapes()
     _time_batch(fluids, fps, 1, warmup, opts)
     base = best_t = _time_batch(fluids, fps, steps, warmup, opts)
-    trials = [{"grids": dict(best), "ms_per_step": base * 1e3}]
+    trials = [{"grids": dict(best), "shapes": dict(shapes), "ms_per_step": base * 1e3}]
     for name in fams:
+        if name in SHAPE_FAMILIES:  # the other compiled workgroup sizes
+            for v in SHAPE_FAMILIES[name]:
+                if v == shapes[name]:
+                    continue
+                set_batch_shapes({name: v})
+                t = _time_batch(fluids, fps, steps, warmup, opts)
+                trials.append({"family": name, "lanes": v, "ms_per_step": t * 1e3})
+                if t < best_t * (1.0 - min_gain):
+                    best_t, shapes[name] = t, v
+                set_batch_shapes({name: shapes[name]})
+            continue
         start = best[name]
         for f in factors:
             v = min(16384, max(8, int(round(start * f / 8.0)) * 8))
@@ -925,8 +969,9 @@ def autotune_batch(fluids, fps: float = 24.0, steps: int = 3, warmup: int = 1, f
             if t < best_t * (1.0 - min_gain):
                 best_t, best[name] = t, v
             set_batch_grids({name: best[name]})
-    result = {"grids": best, "ms_per_step": best_t * 1e3, "baseline_ms_per_step": base * 1e3, "trials": trials}
+    result = {"grids": best, "shapes": shapes, "ms_per_step": best_t * 1e3, "baseline_ms_per_step": base * 1e3,
+              "trials": trials}
     if cache:
-        save_batch_grids(cache, best, ms_per_step=best_t * 1e3, baseline_ms_per_step=base * 1e3,
+        save_batch_grids(cache, best, shapes, ms_per_step=best_t * 1e3, baseline_ms_per_step=base * 1e3,
                          members=len(fluids), steps=steps, trials=trials)
     return result

@@ -300,3 +300,29 @@ def test_batch_rejects_members_beyond_the_work_list_field(pxl):
     assert "too large for batched execution" in str(e.value) and "1048576" in str(e.value)
     ok = dict(big, particles_per_cell=16)  # 244M particles: 953,312 workgroups fit
     assert pxl.check(ok)["n_particles"] < 1 << 28
+
+
+def test_batch_shape_settings(pxl, tmp_path):
+    """The workgroup sizes of the families compiled in two sizes (host-only settings): the
+    defaults, validation (only the compiled sizes) and the JSON cache round trip."""
+    saved = pxl.batch_shapes()
+    assert saved == {"tile_lanes": 1024, "band_lanes": 256}
+    assert pxl.SHAPE_FAMILIES == {"tile_lanes": (1024, 512), "band_lanes": (256, 512)}
+    lib = pxl.load_library()
+    try:
+        pxl.set_batch_shapes({"tile_lanes": 512, "band_lanes": 512})
+        assert pxl.batch_shapes() == {"tile_lanes": 512, "band_lanes": 512}
+        for bad in (256, 768, 0):
+            with pytest.raises(pxl.FluidError):
+                pxl.set_batch_shapes({"tile_lanes": bad})
+        with pytest.raises(KeyError):
+            pxl.set_batch_shapes({"tiles": 512})
+        assert lib.pxlflip_batch_set_shape(2, 512) == -1 and lib.pxlflip_batch_set_shape(-1, 512) == -1
+        path = str(tmp_path / "tuned.json")
+        pxl.save_batch_grids(path)
+        pxl.set_batch_shapes(saved)
+        pxl.load_batch_grids(path)
+        assert pxl.batch_shapes() == {"tile_lanes": 512, "band_lanes": 512}
+    finally:
+        pxl.set_batch_shapes(saved)
+    assert pxl.batch_shapes() == saved

@@ -176,3 +176,33 @@ def test_batch_grids_change_speed_not_results(pxl, tmp_path):
         assert pxl.load_batch_grids(cache) == r["grids"]
     finally:
         pxl.set_batch_grids(saved)
+
+
+def test_batch_workgroup_sizes_change_speed_not_results(pxl, tmp_path):
+    """The per-tile P2G / G2P kernels in 512-lane workgroups and the band passes in 512-lane
+    workgroups step a batch bitwise like the default sizes (1024 and 256); the autotuner times
+    the other sizes and records its choice in the JSON cache."""
+    fluids = [random_request(i, sizes=(32, 48, 64)) for i in range(4)]
+    saved = pxl.batch_shapes()
+    with pxl.Batch(fluids, FPS) as b:
+        b.step(3)
+        ref = [b.read_particles(i) for i in range(len(fluids))]
+        ref_p = [b.read_field(i, "pressure64") for i in range(len(fluids))]
+    try:
+        for shapes in ({"tile_lanes": 512}, {"band_lanes": 512}, {"tile_lanes": 512, "band_lanes": 512}):
+            pxl.set_batch_shapes(dict(saved, **shapes))
+            with pxl.Batch(fluids, FPS) as b:
+                b.step(3)
+                for i in range(len(fluids)):
+                    assert np.array_equal(ref[i].view(np.uint32), b.read_particles(i).view(np.uint32)), (shapes, i)
+                    assert np.array_equal(ref_p[i].view(np.uint64), b.read_field(i, "pressure64").view(np.uint64))
+        pxl.set_batch_shapes(saved)
+        cache = str(tmp_path / "shapes.json")
+        r = pxl.autotune_batch(fluids, FPS, steps=1, warmup=1, families=("tile_lanes", "band_lanes"), cache=cache)
+        print(f"shape autotune {r['shapes']} {r['ms_per_step']:.3f} ms/step", flush=True)
+        assert len(r["trials"]) == 3 and set(r["shapes"]) == {"tile_lanes", "band_lanes"}
+        assert pxl.batch_shapes() == r["shapes"]
+        import json
+        assert json.load(open(cache))["shapes"] == r["shapes"]
+    finally:
+        pxl.set_batch_shapes(saved)
