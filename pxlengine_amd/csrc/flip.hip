@@ -135,6 +135,8 @@ int prepare(const char *fluid_json, double fps, uint32_t seed, const pxlflip_opt
     d.nw = n[0] * n[1] * (n[2] + 1);
     d.nfaces = d.nu + d.nv + d.nw;
     d.ncells = n[0] * n[1] * n[2];
+    d.hx = (n[0] + 1) / 2;
+    d.hsplit = d.hx * n[1] * n[2];
     d.ntx = (int)((n[0] + TILE - 1) / TILE);
     d.nty = (int)((n[1] + TILE - 1) / TILE);
     d.ntz = (int)((n[2] + TILE - 1) / TILE);

@@ -329,12 +329,16 @@ __device__ __forceinline__ void body_fluid_rows(const DevSim &S, uint32_t b) {
             for (int n = 0; n < 6; n++)
                 if (st & (1u << n)) word |= (((uint32_t)S.sten[id + off[n]] >> 8) & 7u) << (9 + 3 * n);
             word |= black << 27;
+            word |= ((uint32_t)i & 1u) << 28;
             S.fl_list[o] = id;
             S.fl_word[o] = word;
+            const uint32_t sid = black * S.hsplit + ((uint32_t)i >> 1) + S.hx * row;
+            if (S.fls_list) S.fls_list[o] = sid;
             if (split) {
                 const uint32_t n = on + (uint32_t)__popcll((ma | mb) & below);
                 S.fn_list[n] = id;
                 S.fn_word[n] = word;
+                if (S.fns_list) S.fns_list[n] = sid;
             }
         }
         oa += (uint32_t)__popcll(ma);

@@ -125,12 +125,19 @@ struct DevSim {
     uint32_t *fl_rows, *fl_rowoff, *fl_list, *fl_count, *fl_tmp;
     // per list entry: bits 0-5 FLUID neighbours (-x,+x,-y,+y,-z,+z), bits 6-8 the cell's
     // non-SOLID neighbour count (matrix diagonal), bits 9+3n the diagonal of FLUID neighbour n,
-    // bit 27 the cell's colour (1 = black: i + j + k odd)
+    // bit 27 the cell's colour (1 = black: i + j + k odd), bit 28 its x parity (i & 1)
     uint32_t *fl_word;
     // the same entries in plain cell order (with colours not split: fl_list / fl_word
     // themselves), for the kernels that treat every FLUID cell alike: their vector accesses
     // then run along x instead of skipping every other cell
     uint32_t *fn_list, *fn_word;
+    // MIXED: the FP32 vectors of the inner solve (cg_r0 = r, cg_r1 = u, cg_w0, cg_s0, cg_p, e)
+    // are stored colour-split: cell (i, j, k) of colour p = (i + j + k) & 1 at
+    // p hsplit + (i >> 1) + hx (j + ny k), hx = ceil(nx / 2), hsplit = hx ny nz, so a colour
+    // sweep reads and writes whole lines of one colour.  fls_list / fns_list hold the split index
+    // of each fl_list / fn_list entry (fl_word bit 28 = i & 1 gives the x neighbours' offsets).
+    uint32_t hx, hsplit;
+    uint32_t *fls_list, *fns_list;
     // solver
     float *cg_r0, *cg_r1, *cg_w0, *cg_w1, *cg_s0, *cg_s1, *cg_p, *e;
     double *x64a, *x64b, *r64;
@@ -148,6 +155,24 @@ struct DevSim {
     // initial particles (reset)
     float *ix, *iy, *iz, *ivx, *ivy, *ivz;
 };
+
+// Colour-split index of cell (i, j, k) (DevSim::hx, hsplit).
+__host__ __device__ inline uint32_t split_index(const DevSim &S, uint32_t i, uint32_t j, uint32_t k) {
+    return (((i + j + k) & 1u) ? S.hsplit : 0u) + (i >> 1) + S.hx * (j + (uint32_t)S.ny * k);
+}
+// Split-index offsets of the six neighbours (-x, +x, -y, +y, -z, +z) of a cell whose list word
+// has colour bit 27 and x-parity bit 28: the other colour's half, the same half-row position
+// along y and z, and along x the position (i -/+ 1) >> 1.
+__device__ __forceinline__ void split_offsets(const DevSim &S, uint32_t word, int32_t off[6]) {
+    const int32_t d = ((word >> 27) & 1u) ? -(int32_t)S.hsplit : (int32_t)S.hsplit;
+    const int32_t o = (int32_t)((word >> 28) & 1u), hy = (int32_t)S.hx, hz = (int32_t)S.hx * S.ny;
+    off[0] = d - 1 + o;
+    off[1] = d + o;
+    off[2] = d - hy;
+    off[3] = d + hy;
+    off[4] = d - hz;
+    off[5] = d + hz;
+}
 
 // Face box of a tile for component c: every face a particle binned in the tile can touch.
 // Along c the faces [8t, 8t + 8] (9), along the other axes the faces at cells [8t - 1, 8t + 8]

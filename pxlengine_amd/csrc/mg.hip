@@ -148,17 +148,19 @@ __device__ __forceinline__ void body_mg_sten(const MGLevel &v, uint32_t b) {
 __global__ __launch_bounds__(MBLOCK) void k_mg_sten(MGLevel v) { body_mg_sten(v, blockIdx.x); }
 
 // ---- level 0 (the step's colour-split fluid list: red entries [0, n_red), black [n_red, n);
-// r, u are the CG vectors)
+// r, u are the CG vectors, stored colour-split: an entry's vector index is its fls_list /
+// fns_list value and its neighbours' are given by split_offsets)
 // pre0: red u = r / d, black u = (r + sum FLUID red nbrs r_n / d_n) / d
 __device__ __forceinline__ void body_l0_pre0(const DevSim &S, const uint32_t *flc, const float *__restrict__ r,
                                              float *__restrict__ u, uint32_t b, uint32_t nb) {
     uint32_t beg, end;
     slice(flc[1], b, nb, beg, end);
-    const int32_t off[6] = {-1, 1, -S.nx, S.nx, -S.nx * S.ny, S.nx * S.ny};
     for (uint32_t t = beg + threadIdx.x; t < end; t += LBLOCK) {
-        const uint32_t id = S.fn_list[t], wd = S.fn_word[t];
+        const uint32_t id = S.fls_list[t], wd = S.fl_word[t];  // colour-split order: runs through each half
         float sum = 0.0f;
         if (w_black(wd)) {
+            int32_t off[6];
+            split_offsets(S, wd, off);
 #pragma unroll
             for (int n = 0; n < 6; n++)
                 if ((wd >> n) & 1u) sum += r[id + off[n]] * rcp_or0(w_ndiag(wd, n));
@@ -168,7 +170,8 @@ __device__ __forceinline__ void body_l0_pre0(const DevSim &S, const uint32_t *fl
 }
 // One colour's update over its list segment.  A lane takes entries t and t + LBLOCK and issues
 // the loads of both before either store (they read only the other colour, so the order of the
-// two updates does not matter).
+// two updates does not matter).  The colour's neighbours all lie in the other colour's half of
+// the split vector: d +/- 1 or d along x, d +/- hx along y, d +/- hx ny along z.
 __device__ __forceinline__ void body_l0_sweep(const DevSim &S, const uint32_t *flc, const float *__restrict__ r,
                                               float *__restrict__ u, uint32_t colour, uint32_t b, uint32_t nb) {
     const uint32_t n_red = flc[0], n = flc[1];
@@ -177,17 +180,21 @@ __device__ __forceinline__ void body_l0_sweep(const DevSim &S, const uint32_t *f
     slice(hi - lo, b, nb, beg, end);
     beg += lo;
     end += lo;
-    const int32_t off[6] = {-1, 1, -S.nx, S.nx, -S.nx * S.ny, S.nx * S.ny};
+    const int32_t d = colour ? -(int32_t)S.hsplit : (int32_t)S.hsplit;
+    const int32_t hy = (int32_t)S.hx, hz = (int32_t)S.hx * S.ny;
     for (uint32_t t = beg + threadIdx.x; t < end; t += 2 * LBLOCK) {
         const uint32_t t2 = t + LBLOCK;
         const bool two = t2 < end;
-        const uint32_t wa = S.fl_word[t], ia = S.fl_list[t];
-        const uint32_t wb = two ? S.fl_word[t2] : 0u, ib = two ? S.fl_list[t2] : ia;
+        const uint32_t wa = S.fl_word[t], ia = S.fls_list[t];
+        const uint32_t wb = two ? S.fl_word[t2] : 0u, ib = two ? S.fls_list[t2] : ia;
+        const int32_t oa = (int32_t)((wa >> 28) & 1u), ob = (int32_t)((wb >> 28) & 1u);
+        const int32_t offa[6] = {d - 1 + oa, d + oa, d - hy, d + hy, d - hz, d + hz};
+        const int32_t offb[6] = {d - 1 + ob, d + ob, d - hy, d + hy, d - hz, d + hz};
         float sa = 0.0f, sb = 0.0f;
 #pragma unroll
         for (int q = 0; q < 6; q++) {
-            const float za = ((wa >> q) & 1u) ? u[ia + off[q]] : 0.0f;
-            const float zb = ((wb >> q) & 1u) ? u[ib + off[q]] : 0.0f;
+            const float za = ((wa >> q) & 1u) ? u[ia + offa[q]] : 0.0f;
+            const float zb = ((wb >> q) & 1u) ? u[ib + offb[q]] : 0.0f;
             sa += za;
             sb += zb;
         }
@@ -196,7 +203,8 @@ __device__ __forceinline__ void body_l0_sweep(const DevSim &S, const uint32_t *f
         if (two) u[ib] = (rb + sb) * rcp_or0(w_diag(wb));
     }
 }
-// post: black update from red neighbours' u + the prolonged correction c.z
+// post: black update from red neighbours' u + the prolonged correction c.z (the parents come
+// from the cell's grid coordinates, fl_list)
 __device__ __forceinline__ void body_l0_post(const DevSim &S, const uint32_t *flc, const MGLevel &c,
                                              const float *__restrict__ r, float *__restrict__ u, uint32_t b,
                                              uint32_t nb) {
@@ -207,20 +215,18 @@ __device__ __forceinline__ void body_l0_post(const DevSim &S, const uint32_t *fl
     end += n_red;
     for (uint32_t t = beg + threadIdx.x; t < end; t += LBLOCK) {
         const uint32_t wd = S.fl_word[t];
-        const uint32_t id = S.fl_list[t];
+        const uint32_t id = S.fl_list[t], sid = S.fls_list[t];
+        int32_t off[6];
+        split_offsets(S, wd, off);
         int i, j, k;
         cell_ijk(S.nx, S.ny, id, i, j, k);
         const int nbc[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k}, {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
         float sum = 0.0f;
 #pragma unroll
         for (int q = 0; q < 6; q++) {
-            if ((wd >> q) & 1u) {
-                const int a = nbc[q][0], bb = nbc[q][1], d = nbc[q][2];
-                sum += u[(uint32_t)a + (uint32_t)S.nx * ((uint32_t)bb + (uint32_t)S.ny * (uint32_t)d)] +
-                       c.z[parent(c.nx, c.ny, a, bb, d)];
-            }
+            if ((wd >> q) & 1u) sum += u[sid + off[q]] + c.z[parent(c.nx, c.ny, nbc[q][0], nbc[q][1], nbc[q][2])];
         }
-        u[id] = (r[id] + sum) * rcp_or0(w_diag(wd));
+        u[sid] = (r[sid] + sum) * rcp_or0(w_diag(wd));
     }
 }
 
@@ -352,6 +358,54 @@ __device__ __forceinline__ void body_restrict(int fnx, int fny, int fnz, const u
             float sum = 0.0f;
 #pragma unroll
             for (int n = 0; n < 6; n++) sum = (st[m] & (1u << n)) ? sum + zn[m][n] : sum;
+            acc += rc[m] - st_diag(st[m]) * zc[m] + sum;
+        }
+        c.r[cc] = 0.5f * acc;
+    }
+}
+
+// The same restriction from level 0, whose r and z are colour-split: red child (fi, fj, fk) of
+// coarse cell (ci, cj, ck) sits at (fi >> 1) + hx (fj + ny fk) = ci + hx (fj + ny fk) in the red
+// half, its (black) neighbours at that index + hsplit + split_offsets(red, fi & 1).
+__device__ __forceinline__ void body_restrict0(const DevSim &S, const MGLevel &c, uint32_t b, uint32_t nb) {
+    const uint32_t n = c.count[0] + c.count[1];
+    uint32_t beg, end;
+    slice(n, b, nb, beg, end);
+    const int fnx = S.nx, fny = S.ny, fnz = S.nz;
+    const int32_t hs = (int32_t)S.hsplit, hy = (int32_t)S.hx, hz = (int32_t)S.hx * fny;
+    const uint32_t vlast = 2u * S.hsplit - 1u;
+    const float *__restrict__ fr = S.cg_r0;
+    const float *__restrict__ fz = S.cg_r1;
+    for (uint32_t t = beg + threadIdx.x; t < end; t += LBLOCK) {
+        const uint32_t cc = c.nlist[t];
+        int ci, cj, ck;
+        cell_ijk(c.nx, c.ny, cc, ci, cj, ck);
+        const int red[4] = {0, 3, 5, 6};
+        bool in[4];
+        uint32_t st[4];
+        float rc[4], zc[4], zn[4][6];
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            const int q = red[m];
+            const int a = q & 1, fi = 2 * ci + a, fj = 2 * cj + ((q >> 1) & 1), fk = 2 * ck + (q >> 2);
+            in[m] = fi < fnx && fj < fny && fk < fnz;
+            const uint32_t gid =
+                in[m] ? (uint32_t)fi + (uint32_t)fnx * ((uint32_t)fj + (uint32_t)fny * (uint32_t)fk) : 0u;
+            const uint32_t sid = in[m] ? (uint32_t)ci + S.hx * ((uint32_t)fj + (uint32_t)fny * (uint32_t)fk) : 0u;
+            st[m] = S.sten[gid];
+            rc[m] = fr[sid];
+            zc[m] = fz[sid];
+            const int32_t off[6] = {hs - 1 + a, hs + a, hs - hy, hs + hy, hs - hz, hs + hz};
+#pragma unroll
+            for (int e = 0; e < 6; e++) zn[m][e] = fz[min((uint32_t)((int32_t)sid + off[e]), vlast)];
+        }
+        float acc = 0.0f;
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            if (!in[m] || !(st[m] & STEN_FLUID)) continue;
+            float sum = 0.0f;
+#pragma unroll
+            for (int e = 0; e < 6; e++) sum = (st[m] & (1u << e)) ? sum + zn[m][e] : sum;
             acc += rc[m] - st_diag(st[m]) * zc[m] + sum;
         }
         c.r[cc] = 0.5f * acc;
@@ -620,6 +674,7 @@ __global__ __launch_bounds__(LBLOCK) void k_lc_post(MGLevel v, MGLevel c) { body
 __global__ __launch_bounds__(LBLOCK) void k_restrict(MGLevel f, const float *fr, const float *fz, MGLevel c) {
     body_restrict(f.nx, f.ny, f.nz, f.sten, fr, fz, c, blockIdx.x, gridDim.x);
 }
+__global__ __launch_bounds__(LBLOCK) void k_restrict0(DevSim S, MGLevel c) { body_restrict0(S, c, blockIdx.x, gridDim.x); }
 
 inline unsigned grid_for(uint32_t n) { return (n + MBLOCK - 1) / MBLOCK; }
 inline uint32_t cells(const MGLevel &m) { return (uint32_t)(m.nx * m.ny * m.nz); }
@@ -679,7 +734,7 @@ __global__ __launch_bounds__(LBLOCK) void kb_restrict(WorkList W, int l) {
     for_each_work(W, [&](const DevSim &S, uint32_t b, uint32_t *cnt) {
         const MGLevel c = level_in_batch(S, cnt, l + 1);
         if (l == 0) {
-            body_restrict(S.nx, S.ny, S.nz, S.sten, S.cg_r0, S.cg_r1, c, b, S.cg_blocks);
+            body_restrict0(S, c, b, S.cg_blocks);
         } else {
             const MGLevel f = S.mg[l];
             body_restrict(f.nx, f.ny, f.nz, f.sten, f.r, f.z, c, b, level_blocks(S.cg_blocks, l));
@@ -722,7 +777,7 @@ void launch_mg_vcycle(const DevSim &S, const float *r, float *u, hipStream_t s) 
             PXL_LAUNCH(k_l0_pre0, g, LBLOCK, 0, s, S);
             PXL_LAUNCH(k_l0_sweep, g, LBLOCK, 0, s, S, 0);
             PXL_LAUNCH(k_l0_sweep, g, LBLOCK, 0, s, S, 1);
-            PXL_LAUNCH(k_restrict, g, LBLOCK, 0, s, level0(S), S.cg_r0, S.cg_r1, S.mg[1]);
+            PXL_LAUNCH(k_restrict0, g, LBLOCK, 0, s, S, S.mg[1]);
         } else {
             PXL_LAUNCH(k_lc_pre0, g, LBLOCK, 0, s, S.mg[l]);
             for (int q = 1; q < MG_COARSE_ROUNDS; q++) {

@@ -220,7 +220,9 @@ struct Sim {
         }
         add(&d.fl_count, 64);
         add(&d.fl_tmp, 4ull * (scan_tiles(fl_row_slots(d)) + 1));
-        for (float **p : {&d.cg_r0, &d.cg_r1, &d.cg_w0, &d.cg_w1, &d.cg_s0, &d.cg_s1, &d.cg_p}) add(p, 4 * C);
+        // the MIXED inner solve stores its vectors colour-split (2 hsplit entries >= C)
+        const size_t V = opts.solver == PXLFLIP_SOLVER_MIXED ? std::max<size_t>(C, 2ull * d.hsplit) : C;
+        for (float **p : {&d.cg_r0, &d.cg_r1, &d.cg_w0, &d.cg_w1, &d.cg_s0, &d.cg_s1, &d.cg_p}) add(p, 4 * V);
         if (opts.solver == PXLFLIP_SOLVER_MIXED) {
             for (int l = 1; l < d.mg_levels; l++) {
                 const size_t cl = (size_t)d.mg[l].nx * d.mg[l].ny * d.mg[l].nz;
@@ -232,7 +234,9 @@ struct Sim {
                 add(&d.mg[l].nlist, 4 * cl);
                 add(&d.mg[l].count, 64);
             }
-            add(&d.e, 4 * C);
+            add(&d.e, 4 * V);
+            add(&d.fls_list, 4 * C);
+            add(&d.fns_list, 4 * C);
             add(&d.x64a, 8 * C);
             add(&d.x64b, 8 * C);
             add(&d.r64, 8 * C);

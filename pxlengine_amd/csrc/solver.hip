@@ -121,12 +121,13 @@ __device__ __forceinline__ void body_cg_residual(const DevSim &S, const uint32_t
     nb_offsets(S, off);
     for (uint32_t t = beg + threadIdx.x; t < end; t += SBLOCK) {
         const uint32_t id = S.fn_list[t];
-        const uint32_t wd = S.fn_word[t];
-        if (inner) {
-            r[id] = (float)S.r64[id];
-            x[id] = 0.0f;
+        if (inner) {  // the inner solve's vectors are colour-split (DevSim::fns_list)
+            const uint32_t sid = S.fns_list[t];
+            r[sid] = (float)S.r64[id];
+            x[sid] = 0.0f;
             continue;
         }
+        const uint32_t wd = S.fn_word[t];
         float sum = 0.0f;
 #pragma unroll
         for (int n = 0; n < 6; n++)
@@ -254,7 +255,11 @@ __global__ __launch_bounds__(SBLOCK) void k_cg_iter(DevSim S, uint32_t it, float
 }
 
 // ---- multigrid-preconditioned CG (Chronopoulos-Gear with u = M r from the V-cycle).
-// Vectors: r = cg_r0, u = cg_r1, w = cg_w0, s = cg_s0, p = cg_p; x = the inner unknown.
+// Vectors: r = cg_r0, u = cg_r1, w = cg_w0, s = cg_s0, p = cg_p; x = the inner unknown; all
+// colour-split.  spmv and update walk the colour-split list (red cells, then black, each in cell
+// order), whose split indices run through each half of the vectors in order, so every line they
+// touch is used whole even on short rows (an entry's index is fls_list[t], its neighbours'
+// split_offsets away).
 // spmv: w = A u and the partials of gamma = (r, u), delta = (w, u) into slot `slot`
 // (first: also p_{-1} = s_{-1} = 0).
 template <typename Acc>
@@ -262,13 +267,13 @@ __device__ __forceinline__ void body_mgcg_spmv(const DevSim &S, const uint32_t *
                                                uint32_t b, uint32_t nb) {
     uint32_t beg, end;
     list_slice(0, flc[1], b, nb, beg, end);
-    int32_t off[6];
-    nb_offsets(S, off);
     const float *__restrict__ u = S.cg_r1;
     Acc g = 0, d = 0;
     for (uint32_t t = beg + threadIdx.x; t < end; t += SBLOCK) {
-        const uint32_t c = S.fn_list[t];
-        const uint32_t wd = S.fn_word[t];
+        const uint32_t c = S.fls_list[t];
+        const uint32_t wd = S.fl_word[t];
+        int32_t off[6];
+        split_offsets(S, wd, off);
         float sum = 0.0f;
 #pragma unroll
         for (int n = 0; n < 6; n++)
@@ -296,7 +301,7 @@ __device__ __forceinline__ void body_mgcg_update(const DevSim &S, const uint32_t
     float alpha, beta;
     cg_scalars<Acc>(S, it, b, nb, alpha, beta);
     for (uint32_t t = beg + threadIdx.x; t < end; t += SBLOCK) {
-        const uint32_t c = S.fn_list[t];
+        const uint32_t c = S.fls_list[t];
         const float pc = S.cg_r1[c] + beta * S.cg_p[c];
         const float sc = S.cg_w0[c] + beta * S.cg_s0[c];
         S.cg_p[c] = pc;
@@ -361,17 +366,16 @@ __device__ __forceinline__ void body_mixed_update(const DevSim &S, const uint32_
     nb_offsets(S, off);
     double rr = 0;
     for (uint32_t t = beg + threadIdx.x; t < end; t += SBLOCK) {
-        const uint32_t id = S.fn_list[t];
+        const uint32_t id = S.fn_list[t], sid = S.fns_list[t];
         const uint32_t wd = S.fn_word[t];
+        int32_t soff[6];
+        split_offsets(S, wd, soff);  // e is the inner solve's colour-split solution
         double sum = 0.0;
 #pragma unroll
         for (int n = 0; n < 6; n++) {
-            if (w_fluid(wd, n)) {
-                const uint32_t q = id + off[n];
-                sum += xsrc[q] + (double)e[q];
-            }
+            if (w_fluid(wd, n)) sum += xsrc[id + off[n]] + (double)e[sid + soff[n]];
         }
-        const double xc = xsrc[id] + (double)e[id];
+        const double xc = xsrc[id] + (double)e[sid];
         const double r = b64_of(S, id) - ((double)w_diag(wd) * xc - sum);
         xdst[id] = xc;
         S.pressure[id] = (float)xc;
