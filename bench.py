@@ -571,7 +571,7 @@ COPY_KERNEL = "__amd_rocclr_copyBuffer"
 def kernel_symbol(name: str) -> list:
     """Symbols the launch of phase `name` may carry: batched (kb_sweep for mg_sweep, kb_g2p_tile
     for g2p, ...), single-simulation (k_cg_iter, k_g2p_tile, ...) or a surface stage (k_s_splat_lds)."""
-    base = name[3:] if name.startswith("mg_") and name not in ("mg_tail", "mg_types", "mg_sten") else name
+    base = name[3:] if name.startswith("mg_") and name not in ("mg_tail", "mg_types", "mg_sten", "mg_scan", "mg_compact") else name
     out = [f"kb_{name}", f"kb_{base}", f"kb_{name}_tile", f"k_{name}", f"k_{name}_tile", f"k_{name}_lds"]
     return out + ([SURF_KERNELS[name]] if name in SURF_KERNELS else [])
 

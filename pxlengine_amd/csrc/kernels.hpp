@@ -51,12 +51,17 @@ struct MGLevel {
     uint8_t *type;   // level-l cell types (levels >= 1)
     uint16_t *sten;  // level-l stencil words (same layout as DevSim::sten; levels >= 1)
     float *r, *z;    // level-l right-hand side (the scaled restricted residual) and iterate
-    // FLUID cells of the level (levels >= 1, rebuilt each step): red cells from the front
-    // (count[0] of them), black cells from the back (count[1]: list[cells - 1 - t]), each in
-    // any order; nlist holds all of them (count[2]) in the order the stencil pass meets them
-    // (cell order within each wave's 64 cells), for the kernels that treat both colours alike
+    // FLUID cells of the level (levels >= 1, rebuilt each step), each list in cell order: red
+    // cells from the front (count[0] of them), black cells from the back (count[1]: the t-th
+    // at list[cells - 1 - t]); nlist holds all of them (count[2]) for the kernels that treat
+    // both colours alike.  Cell order keeps the neighbour rows and planes that a list slice
+    // reads next to the ones its neighbouring slices read.
     uint32_t *list, *nlist, *count;
+    // per 64-cell chunk of the level: its red FLUID cells [0, chunks), then its black ones
+    // [chunks, 2 chunks); the level scan turns both halves into exclusive offsets
+    uint32_t *chunk;
 };
+__host__ __device__ inline uint32_t level_chunks(const MGLevel &v) { return (uint32_t)(v.nx * v.ny * v.nz + 63) / 64u; }
 
 // Solver scalar slots (fp64 array): two slots of SC_COUNT by CG-iteration parity for the
 // (alpha, beta, gamma) history; the refinement norms live in slot 0.
@@ -497,8 +502,10 @@ void batch_p2g(const WorkLists &W, hipStream_t s);
 void batch_faces(const WorkLists &W, hipStream_t s);
 void batch_cells(const WorkLists &W, hipStream_t s);
 void batch_fluid_compact(const WorkLists &W, hipStream_t s);
-void batch_mg_types(const WorkLists &W, int l, hipStream_t s);  // level l >= 1: types, list count = 0
-void batch_mg_sten(const WorkLists &W, int l, hipStream_t s);   // stencil words, z = 0, FLUID list
+void batch_mg_types(const WorkLists &W, int l, hipStream_t s);  // level l >= 1: types
+void batch_mg_sten(const WorkLists &W, int l, hipStream_t s);   // stencil words, z = 0, chunk counts
+void batch_mg_scan(const WorkLists &W, hipStream_t s);           // every level: chunk offsets, list counts
+void batch_mg_compact(const WorkLists &W, int l, hipStream_t s); // level l's FLUID lists in cell order
 // solve (WD_A_* lists)
 void batch_mixed_init(const WorkLists &W, hipStream_t s);
 void batch_norms(const WorkLists &W, int nv, hipStream_t s);

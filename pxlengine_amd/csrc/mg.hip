@@ -79,14 +79,9 @@ __device__ __forceinline__ float w_diag(uint32_t w) { return (float)((w >> 6) & 
 __device__ __forceinline__ float w_ndiag(uint32_t w, int n) { return (float)((w >> (9 + 3 * n)) & 7u); }
 __device__ __forceinline__ uint32_t w_black(uint32_t w) { return (w >> 27) & 1u; }
 
-// ---- level types (c from its children in f); zeroes c's list count
+// ---- level types (c from its children in f)
 __device__ __forceinline__ void body_mg_types(const MGLevel &f, const MGLevel &c, uint32_t b) {
     const uint32_t id = b * MBLOCK + threadIdx.x;
-    if (id == 0) {
-        c.count[0] = 0;
-        c.count[1] = 0;
-        c.count[2] = 0;
-    }
     if (id >= ncell(c)) return;
     int i, j, k;
     cell_ijk(c.nx, c.ny, id, i, j, k);
@@ -103,7 +98,7 @@ __global__ __launch_bounds__(MBLOCK) void k_mg_types(MGLevel f, MGLevel c) { bod
 
 // ---- level stencil words (FLUID bit, non-SOLID-neighbour count in bits 8-10, FLUID-neighbour
 // mask in bits 0-5: the layout of the simulation grid's stencil words, plus the colour in
-// STEN_BLACK), z = 0, and the level's FLUID-cell lists (one atomic per wave and list)
+// STEN_BLACK), z = 0, and each 64-cell chunk's red and black FLUID cells (a wave is a chunk)
 __device__ __forceinline__ void body_mg_sten(const MGLevel &v, uint32_t b) {
     const uint32_t id = b * MBLOCK + threadIdx.x;
     const bool in = id < ncell(v);
@@ -128,24 +123,95 @@ __device__ __forceinline__ void body_mg_sten(const MGLevel &v, uint32_t b) {
         v.z[id] = 0.0f;
     }
     const bool fl = (st & STEN_FLUID) != 0;
-    const int lane = threadIdx.x & (WAVE - 1);
-    const uint64_t below = (1ull << lane) - 1ull;
-#pragma unroll
-    for (int col = 0; col < 3; col++) {  // red, black, both (the cell-order list)
-        const bool mine = fl && (col == 2 || black == (col == 1));
-        const uint64_t mask = __ballot(mine);
-        const int leader = mask ? __ffsll((long long)mask) - 1 : 0;
-        uint32_t base = 0;
-        if (mine && lane == leader) base = atomicAdd(&v.count[col], (uint32_t)__popcll(mask));
-        base = __shfl(base, leader, WAVE);
-        const uint32_t slot = base + (uint32_t)__popcll(mask & below);
-        if (mine) {
-            if (col == 2) v.nlist[slot] = id;
-            else v.list[col ? ncell(v) - 1u - slot : slot] = id;
-        }
+    const uint64_t mr = __ballot(fl && !black), mb = __ballot(fl && black);
+    const uint32_t c = id / WAVE, nch = level_chunks(v);
+    if ((threadIdx.x & (WAVE - 1)) == 0 && c < nch) {
+        v.chunk[c] = (uint32_t)__popcll(mr);
+        v.chunk[nch + c] = (uint32_t)__popcll(mb);
     }
 }
 __global__ __launch_bounds__(MBLOCK) void k_mg_sten(MGLevel v) { body_mg_sten(v, blockIdx.x); }
+
+// ---- the level lists in cell order.  One workgroup of TBLOCK lanes scans a simulation's
+// chunk counts, level by level, into exclusive offsets (each lane sums a run of consecutive
+// chunks, the workgroup scans the lanes' sums, each lane rewrites its run) and sets the list
+// counts; then a wave per chunk writes the chunk's FLUID cells at its offsets.
+// Exclusive scan of a[0, n) in place; returns the total.
+__device__ __forceinline__ uint32_t block_scan_inplace(uint32_t *a, uint32_t n) {
+    __shared__ uint32_t wsum[TBLOCK / WAVE + 1];
+    const uint32_t per = (n + TBLOCK - 1) / TBLOCK;
+    const uint32_t beg = min(threadIdx.x * per, n), end = min(beg + per, n);
+    uint32_t own = 0;
+    for (uint32_t i = beg; i < end; i++) own += a[i];
+    const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+    uint32_t inc = own;  // inclusive scan of the lanes' sums within the wave
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        const uint32_t y = __shfl_up(inc, o, WAVE);
+        if (lane >= o) inc += y;
+    }
+    if (lane == WAVE - 1) wsum[w] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int q = 0; q < TBLOCK / WAVE; q++) {
+            const uint32_t t = wsum[q];
+            wsum[q] = acc;
+            acc += t;
+        }
+        wsum[TBLOCK / WAVE] = acc;
+    }
+    __syncthreads();
+    uint32_t run = wsum[w] + inc - own;
+    for (uint32_t i = beg; i < end; i++) {
+        const uint32_t t = a[i];
+        a[i] = run;
+        run += t;
+    }
+    const uint32_t total = wsum[TBLOCK / WAVE];
+    __syncthreads();  // wsum is reused by the next scan
+    return total;
+}
+// The level copy with its count pointer taken from a batch member's count slots (`cnt`; null:
+// the level's own counts).
+__device__ __forceinline__ MGLevel level_counts(const DevSim &S, uint32_t *cnt, int l) {
+    MGLevel v = S.mg[l];
+    if (cnt) v.count = cnt + 4 * l;
+    return v;
+}
+__device__ __forceinline__ void body_mg_scan(const DevSim &S, uint32_t *cnt) {
+#pragma unroll
+    for (int q = 1; q < MG_MAX; q++) {
+        if (q < S.mg_levels) {
+            const MGLevel v = level_counts(S, cnt, q);
+            const uint32_t nch = level_chunks(v);
+            const uint32_t red = block_scan_inplace(v.chunk, nch);
+            const uint32_t black = block_scan_inplace(v.chunk + nch, nch);
+            if (threadIdx.x == 0) {
+                v.count[0] = red;
+                v.count[1] = black;
+                v.count[2] = red + black;
+            }
+        }
+    }
+}
+__device__ __forceinline__ void body_mg_compact(const MGLevel &v, uint32_t b) {
+    const uint32_t id = b * MBLOCK + threadIdx.x, n = ncell(v), nch = level_chunks(v);
+    const uint32_t c = id / WAVE;  // the same for the whole wave
+    if (c >= nch) return;
+    const uint32_t st = id < n ? (uint32_t)v.sten[id] : 0u;
+    const bool fl = (st & STEN_FLUID) != 0, black = (st & STEN_BLACK) != 0;
+    const uint64_t mr = __ballot(fl && !black), mb = __ballot(fl && black);
+    if (!fl) return;
+    const uint64_t below = (1ull << (threadIdx.x & (WAVE - 1))) - 1ull;
+    const uint32_t ro = v.chunk[c], bo = v.chunk[nch + c];
+    const uint32_t rank = (uint32_t)__popcll((black ? mb : mr) & below);
+    if (black) v.list[n - 1u - (bo + rank)] = id;
+    else v.list[ro + rank] = id;
+    v.nlist[ro + bo + (uint32_t)__popcll((mr | mb) & below)] = id;
+}
+__global__ __launch_bounds__(TBLOCK) void k_mg_scan(DevSim S) { body_mg_scan(S, nullptr); }
+__global__ __launch_bounds__(MBLOCK) void k_mg_compact(MGLevel v) { body_mg_compact(v, blockIdx.x); }
 
 // ---- level 0 (the step's colour-split fluid list: red entries [0, n_red), black [n_red, n);
 // r, u are the CG vectors, stored colour-split: an entry's vector index is its fls_list /
@@ -613,11 +679,6 @@ __device__ __forceinline__ void body_mg_tail(const DevSim &S) {
 // updated by the same arithmetic as in the grid-wide launches, so the result is the same.
 // `cnt` is a batch member's count slots (null: the levels' own counts).  Level indices are
 // compile-time constants (no scratch copy of S).
-__device__ __forceinline__ MGLevel level_counts(const DevSim &S, uint32_t *cnt, int l) {
-    MGLevel v = S.mg[l];
-    if (cnt) v.count = cnt + 4 * l;
-    return v;
-}
 __device__ __forceinline__ void body_mg_midtail(const DevSim &S, uint32_t *cnt) {
     const int M = S.mg_mid, T = S.mg_tail;
 #pragma unroll
@@ -703,6 +764,15 @@ __global__ __launch_bounds__(MBLOCK) void kb_mg_sten(WorkList W, int l) {
         body_mg_sten(v, b);
     });
 }
+__global__ __launch_bounds__(TBLOCK) void kb_mg_scan(WorkList W) {
+    for_each_work(W, [&](const DevSim &S, uint32_t, uint32_t *cnt) { body_mg_scan(S, cnt); });
+}
+__global__ __launch_bounds__(MBLOCK) void kb_mg_compact(WorkList W, int l) {
+    for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *) {
+        const MGLevel v = S.mg[l];
+        body_mg_compact(v, b);
+    });
+}
 // A member's level copy with its count pointer taken from the batch's slots (known from the
 // work entry alone, so the count loads alongside the level's fields).
 __device__ __forceinline__ MGLevel level_in_batch(const DevSim &S, uint32_t *cnt, int l) {
@@ -764,6 +834,8 @@ void launch_mg_types(const DevSim &S, hipStream_t s) {
         PXL_LAUNCH(k_mg_types, grid_for(cells(S.mg[l])), MBLOCK, 0, s, f, S.mg[l]);
         PXL_LAUNCH(k_mg_sten, grid_for(cells(S.mg[l])), MBLOCK, 0, s, S.mg[l]);
     }
+    PXL_LAUNCH(k_mg_scan, 1, TBLOCK, 0, s, S);
+    for (int l = 1; l < S.mg_levels; l++) PXL_LAUNCH(k_mg_compact, grid_for(cells(S.mg[l])), MBLOCK, 0, s, S.mg[l]);
 }
 
 // u = M r with r = cg_r0, u = cg_r1 (the vectors of the MG-CG iteration).
@@ -811,6 +883,8 @@ uint32_t mg_cell_blocks(const DevSim &S, int l) { return grid_for(cells(l == 0 ?
 
 void batch_mg_types(const WorkLists &W, int l, hipStream_t s) { PXL_LAUNCH(kb_mg_types, batch_grid(W[WD_MGTYPES + l], batch_cap(BF_MG_GRID)), MBLOCK, 0, s, W[WD_MGTYPES + l], l); }
 void batch_mg_sten(const WorkLists &W, int l, hipStream_t s) { PXL_LAUNCH(kb_mg_sten, batch_grid(W[WD_MGTYPES + l], batch_cap(BF_MG_GRID)), MBLOCK, 0, s, W[WD_MGTYPES + l], l); }
+void batch_mg_scan(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_mg_scan, batch_grid(W[WD_SIM], batch_cap(BF_MG_TAIL)), TBLOCK, 0, s, W[WD_SIM]); }
+void batch_mg_compact(const WorkLists &W, int l, hipStream_t s) { PXL_LAUNCH(kb_mg_compact, batch_grid(W[WD_MGTYPES + l], batch_cap(BF_MG_GRID)), MBLOCK, 0, s, W[WD_MGTYPES + l], l); }
 void batch_mg_pre0(const WorkLists &W, int l, hipStream_t s) { PXL_LAUNCH(kb_pre0, batch_grid(W[WD_A_LVL + l], batch_cap(BF_MG_LIST)), LBLOCK, 0, s, W[WD_A_LVL + l], l); }
 void batch_mg_sweep(const WorkLists &W, int l, uint32_t colour, hipStream_t s) {
     PXL_LAUNCH(kb_sweep, batch_grid(W[WD_A_LVL + l], batch_cap(BF_MG_LIST)), LBLOCK, 0, s, W[WD_A_LVL + l], l, colour);

@@ -126,7 +126,13 @@ double share_bytes(const char *k, const DevSim &d, const MemberCounts &m, int l)
     if (n == "cells") return bf * (4 * F + 15 * C + 4 * C) - 4 * Cf;
     if (n == "fluid_compact") return 8 * rows + 2 * C * bf + 8 * Cf;
     if (n == "mg_types") return cells_of(l - 1) + cells_of(l);
-    if (n == "mg_sten") return 7 * cells_of(l) + 8 * cl;
+    if (n == "mg_sten") return 7 * cells_of(l) + 8 * std::ceil(cells_of(l) / 64.0);
+    if (n == "mg_scan") {  // every level's chunk counts read twice and written once
+        double b = 0;
+        for (int q = 1; q < d.mg_levels; q++) b += 12 * std::ceil(cells_of(q) / 64.0);
+        return b;
+    }
+    if (n == "mg_compact") return 2 * cells_of(l) + 8 * std::ceil(cells_of(l) / 64.0) + 8 * cl;
     if (n == "mixed_init") return 32 * Cf;
     if (n == "cg_residual") return 24 * Cf;
     if (n == "mg_pre0") return 16 * cl;
@@ -298,6 +304,8 @@ private:
             run("mg_types", l, [&] { batch_mg_types(lists_, l, stream_); });
             run("mg_sten", l, [&] { batch_mg_sten(lists_, l, stream_); });
         }
+        if (max_levels_ > 1) run("mg_scan", -1, [&] { batch_mg_scan(lists_, stream_); });
+        for (int l = 1; l < max_levels_; l++) run("mg_compact", l, [&] { batch_mg_compact(lists_, l, stream_); });
     }
     void enqueue_vcycle() {
         // the same sequence as the single-simulation V-cycle (mg.hip): two smoothing rounds on
@@ -372,14 +380,15 @@ private:
     // whose level runs as separate launches), the rest take every member.
     double launch_bytes(const char *name, int level) {
         const std::string n(name);
-        const bool solve = n.rfind("mg_", 0) == 0 ? (n != "mg_types" && n != "mg_sten") :
+        const bool grid_mg = n == "mg_types" || n == "mg_sten" || n == "mg_scan" || n == "mg_compact";
+        const bool solve = n.rfind("mg_", 0) == 0 ? !grid_mg :
                            (n == "mixed_init" || n == "norms" || n == "cg_residual" || n.rfind("mgcg", 0) == 0 ||
                             n == "mixed_update");
         double b = 0;
         for (uint32_t i = 0; i < members_.size(); i++) {
             if (solve && !is_active_[i]) continue;
             const DevSim &d = members_[i]->d;
-            if (n == "mg_types" || n == "mg_sten") {
+            if (n == "mg_types" || n == "mg_sten" || n == "mg_compact") {
                 if (level >= d.mg_levels) continue;
             } else if (n == "mg_pre0" || n == "mg_sweep" || n == "mg_restrict" || n == "mg_post") {
                 if (level >= d.mg_mid) continue;

@@ -233,6 +233,7 @@ struct Sim {
                 add(&d.mg[l].list, 4 * cl);
                 add(&d.mg[l].nlist, 4 * cl);
                 add(&d.mg[l].count, 64);
+                add(&d.mg[l].chunk, 8 * ((cl + 63) / 64));
             }
             add(&d.e, 4 * V);
             add(&d.fls_list, 4 * C);
