@@ -217,27 +217,48 @@ __global__ __launch_bounds__(MBLOCK) void k_mg_compact(MGLevel v) { body_mg_comp
 // r, u are the CG vectors, stored colour-split: an entry's vector index is its fls_list /
 // fns_list value and its neighbours' are given by split_offsets)
 // pre0: red u = r / d, black u = (r + sum FLUID red nbrs r_n / d_n) / d
+// The level-0 kernels below give a lane entries t0 + k LBLOCK (k < PRE0_MLP, SWEEP_MLP,
+// POST_MLP) and issue the loads of all of them before any store.  An entry's update reads only
+// cells of the other colour (or, in pre0, r), so the order of the updates does not matter.
+// Measured on the 64-request batch: pre0 and post 9% faster than one entry at a time; the
+// sweep is fastest at two.
+constexpr int PRE0_MLP = 4, SWEEP_MLP = 2, POST_MLP = 2;
 __device__ __forceinline__ void body_l0_pre0(const DevSim &S, const uint32_t *flc, const float *__restrict__ r,
                                              float *__restrict__ u, uint32_t b, uint32_t nb) {
     uint32_t beg, end;
     slice(flc[1], b, nb, beg, end);
-    for (uint32_t t = beg + threadIdx.x; t < end; t += LBLOCK) {
-        const uint32_t id = S.fls_list[t], wd = S.fl_word[t];  // colour-split order: runs through each half
-        float sum = 0.0f;
-        if (w_black(wd)) {
-            int32_t off[6];
-            split_offsets(S, wd, off);
+    for (uint32_t t0 = beg + threadIdx.x; t0 < end; t0 += PRE0_MLP * LBLOCK) {
+        uint32_t id[PRE0_MLP], wd[PRE0_MLP];
 #pragma unroll
-            for (int n = 0; n < 6; n++)
-                if ((wd >> n) & 1u) sum += r[id + off[n]] * rcp_or0(w_ndiag(wd, n));
+        for (int k = 0; k < PRE0_MLP; k++) {
+            const uint32_t t = t0 + k * LBLOCK;
+            id[k] = S.fls_list[t < end ? t : t0];  // colour-split order: runs through each half
+            wd[k] = t < end ? S.fl_word[t] : 0u;    // past the slice: red, no neighbours, no store
         }
-        u[id] = (r[id] + sum) * rcp_or0(w_diag(wd));
+        float rn[PRE0_MLP][6], rc[PRE0_MLP];
+#pragma unroll
+        for (int k = 0; k < PRE0_MLP; k++) {
+            int32_t off[6];
+            split_offsets(S, wd[k], off);
+#pragma unroll
+            for (int n = 0; n < 6; n++) rn[k][n] = (w_black(wd[k]) && ((wd[k] >> n) & 1u)) ? r[id[k] + off[n]] : 0.0f;
+            rc[k] = r[id[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < PRE0_MLP; k++) {
+            if (t0 + k * LBLOCK >= end) break;
+            float sum = 0.0f;
+            if (w_black(wd[k])) {
+#pragma unroll
+                for (int n = 0; n < 6; n++)
+                    if ((wd[k] >> n) & 1u) sum += rn[k][n] * rcp_or0(w_ndiag(wd[k], n));
+            }
+            u[id[k]] = (rc[k] + sum) * rcp_or0(w_diag(wd[k]));
+        }
     }
 }
-// One colour's update over its list segment.  A lane takes entries t and t + LBLOCK and issues
-// the loads of both before either store (they read only the other colour, so the order of the
-// two updates does not matter).  The colour's neighbours all lie in the other colour's half of
-// the split vector: d +/- 1 or d along x, d +/- hx along y, d +/- hx ny along z.
+// One colour's update over its list segment.  The colour's neighbours all lie in the other
+// colour's half of the split vector: d +/- 1 or d along x, d +/- hx along y, d +/- hx ny along z.
 __device__ __forceinline__ void body_l0_sweep(const DevSim &S, const uint32_t *flc, const float *__restrict__ r,
                                               float *__restrict__ u, uint32_t colour, uint32_t b, uint32_t nb) {
     const uint32_t n_red = flc[0], n = flc[1];
@@ -248,25 +269,31 @@ __device__ __forceinline__ void body_l0_sweep(const DevSim &S, const uint32_t *f
     end += lo;
     const int32_t d = colour ? -(int32_t)S.hsplit : (int32_t)S.hsplit;
     const int32_t hy = (int32_t)S.hx, hz = (int32_t)S.hx * S.ny;
-    for (uint32_t t = beg + threadIdx.x; t < end; t += 2 * LBLOCK) {
-        const uint32_t t2 = t + LBLOCK;
-        const bool two = t2 < end;
-        const uint32_t wa = S.fl_word[t], ia = S.fls_list[t];
-        const uint32_t wb = two ? S.fl_word[t2] : 0u, ib = two ? S.fls_list[t2] : ia;
-        const int32_t oa = (int32_t)((wa >> 28) & 1u), ob = (int32_t)((wb >> 28) & 1u);
-        const int32_t offa[6] = {d - 1 + oa, d + oa, d - hy, d + hy, d - hz, d + hz};
-        const int32_t offb[6] = {d - 1 + ob, d + ob, d - hy, d + hy, d - hz, d + hz};
-        float sa = 0.0f, sb = 0.0f;
+    for (uint32_t t0 = beg + threadIdx.x; t0 < end; t0 += SWEEP_MLP * LBLOCK) {
+        uint32_t id[SWEEP_MLP], wd[SWEEP_MLP];
 #pragma unroll
-        for (int q = 0; q < 6; q++) {
-            const float za = ((wa >> q) & 1u) ? u[ia + offa[q]] : 0.0f;
-            const float zb = ((wb >> q) & 1u) ? u[ib + offb[q]] : 0.0f;
-            sa += za;
-            sb += zb;
+        for (int k = 0; k < SWEEP_MLP; k++) {
+            const uint32_t t = t0 + k * LBLOCK;
+            id[k] = S.fls_list[t < end ? t : t0];
+            wd[k] = t < end ? S.fl_word[t] : 0u;
         }
-        const float ra = r[ia], rb = r[ib];
-        u[ia] = (ra + sa) * rcp_or0(w_diag(wa));
-        if (two) u[ib] = (rb + sb) * rcp_or0(w_diag(wb));
+        float z[SWEEP_MLP][6], rc[SWEEP_MLP];
+#pragma unroll
+        for (int k = 0; k < SWEEP_MLP; k++) {
+            const int32_t o = (int32_t)((wd[k] >> 28) & 1u);
+            const int32_t off[6] = {d - 1 + o, d + o, d - hy, d + hy, d - hz, d + hz};
+#pragma unroll
+            for (int q = 0; q < 6; q++) z[k][q] = ((wd[k] >> q) & 1u) ? u[id[k] + off[q]] : 0.0f;
+            rc[k] = r[id[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < SWEEP_MLP; k++) {
+            if (t0 + k * LBLOCK >= end) break;
+            float sum = 0.0f;
+#pragma unroll
+            for (int q = 0; q < 6; q++) sum += z[k][q];
+            u[id[k]] = (rc[k] + sum) * rcp_or0(w_diag(wd[k]));
+        }
     }
 }
 // post: black update from red neighbours' u + the prolonged correction c.z (the parents come
@@ -279,20 +306,41 @@ __device__ __forceinline__ void body_l0_post(const DevSim &S, const uint32_t *fl
     slice(n - n_red, b, nb, beg, end);
     beg += n_red;
     end += n_red;
-    for (uint32_t t = beg + threadIdx.x; t < end; t += LBLOCK) {
-        const uint32_t wd = S.fl_word[t];
-        const uint32_t id = S.fl_list[t], sid = S.fls_list[t];
-        int32_t off[6];
-        split_offsets(S, wd, off);
-        int i, j, k;
-        cell_ijk(S.nx, S.ny, id, i, j, k);
-        const int nbc[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k}, {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
-        float sum = 0.0f;
+    for (uint32_t t0 = beg + threadIdx.x; t0 < end; t0 += POST_MLP * LBLOCK) {
+        uint32_t wd[POST_MLP], id[POST_MLP], sid[POST_MLP];
 #pragma unroll
-        for (int q = 0; q < 6; q++) {
-            if ((wd >> q) & 1u) sum += u[sid + off[q]] + c.z[parent(c.nx, c.ny, nbc[q][0], nbc[q][1], nbc[q][2])];
+        for (int k = 0; k < POST_MLP; k++) {
+            const uint32_t t = t0 + k * LBLOCK, tt = t < end ? t : t0;
+            wd[k] = t < end ? S.fl_word[t] : 0u;
+            id[k] = S.fl_list[tt];
+            sid[k] = S.fls_list[tt];
         }
-        u[sid] = (r[sid] + sum) * rcp_or0(w_diag(wd));
+        float zq[POST_MLP][6], pq[POST_MLP][6], rc[POST_MLP];
+#pragma unroll
+        for (int k = 0; k < POST_MLP; k++) {
+            int32_t off[6];
+            split_offsets(S, wd[k], off);
+            int i, j, kk;
+            cell_ijk(S.nx, S.ny, id[k], i, j, kk);
+            const int nbc[6][3] = {{i - 1, j, kk}, {i + 1, j, kk}, {i, j - 1, kk}, {i, j + 1, kk}, {i, j, kk - 1}, {i, j, kk + 1}};
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                const bool f = (wd[k] >> q) & 1u;
+                zq[k][q] = f ? u[sid[k] + off[q]] : 0.0f;
+                pq[k][q] = f ? c.z[parent(c.nx, c.ny, nbc[q][0], nbc[q][1], nbc[q][2])] : 0.0f;
+            }
+            rc[k] = r[sid[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < POST_MLP; k++) {
+            if (t0 + k * LBLOCK >= end) break;
+            float sum = 0.0f;
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                if ((wd[k] >> q) & 1u) sum += zq[k][q] + pq[k][q];
+            }
+            u[sid[k]] = (rc[k] + sum) * rcp_or0(w_diag(wd[k]));
+        }
     }
 }
 
