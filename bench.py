@@ -400,7 +400,7 @@ def bench_serve(args, pxl):
 
     sizes = tuple(int(v) for v in args.sizes.split(","))
     reqs = [random_request(i, sizes=sizes) for i in range(args.requests)]
-    opts = dict(budget_bytes=args.budget, max_sims=args.max_sims)
+    opts = dict(budget_bytes=args.budget, max_sims=args.max_sims, inner_iterations=args.inner_iterations)
     infos = [pxl.check(f, FPS, seed=i, solver="mixed") for i, f in enumerate(reqs)]
     P = np.array([inf["n_particles"] for inf in infos], dtype=np.float64)
     nbytes = np.array([inf["device_bytes"] for inf in infos], dtype=np.float64)
@@ -528,7 +528,7 @@ def roofline_of(prof):
     return roofline, kernels
 
 
-def batch_roofline(pxl, reqs, warm, timed):
+def batch_roofline(pxl, reqs, warm, timed, **opts):
     """A fixed batch of `reqs` (seed = index): `warm` untimed steps, `timed` timed steps, then
     two steps with every batched launch timed by HIP events and priced in algorithmic bytes.
     Returns (roofline of the kernel with the largest time share, per-kernel rows, particle-steps/s
@@ -536,7 +536,7 @@ def batch_roofline(pxl, reqs, warm, timed):
     from tests.scenes import FPS
     n = len(reqs)
     rate = None
-    with pxl.Batch(reqs, FPS, seeds=list(range(n))) as bat:
+    with pxl.Batch(reqs, FPS, seeds=list(range(n)), **opts) as bat:
         bat.step(warm)
         if timed:
             t0 = time.perf_counter()
@@ -684,7 +684,8 @@ def bench_batch(args, pxl):
     from tests.scenes import random_request
     sizes = tuple(int(v) for v in args.sizes.split(","))
     reqs = [random_request(i, sizes=sizes) for i in range(args.profile_batch)]
-    roofline, kernels, rate = batch_roofline(pxl, reqs, max(args.warmup, 1), args.steps)
+    roofline, kernels, rate = batch_roofline(pxl, reqs, max(args.warmup, 1), args.steps,
+                                             inner_iterations=args.inner_iterations)
     return {
         "metric": METRIC, "value": rate, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
@@ -757,6 +758,8 @@ def main():
     ap.add_argument("--load", type=float, default=0.7,
                     help="serve: Poisson arrival rate of the latency run as a fraction of the measured capacity")
     ap.add_argument("--max-sims", type=int, default=512, help="serve: most simulations in the batch")
+    ap.add_argument("--inner-iterations", type=int, default=9,
+                    help="serve, batch: MG-CG iterations of the first refinement pass (later passes 2, 4, ...)")
     ap.add_argument("--budget", type=float, default=200e9)
     ap.add_argument("--plain-every", type=int, default=8, help="plain path on every k-th request")
     ap.add_argument("--profile-batch", type=int, default=64)

@@ -534,8 +534,10 @@ __device__ __forceinline__ void body_restrict0(const DevSim &S, const MGLevel &c
 // neighbours at indices clamped into the level) before using any; neighbours that are not
 // FLUID are dropped afterwards.
 struct TailLevel {
-    int nx, ny, nz, o;  // dims and LDS offset
-    int cnt[2];         // FLUID cells of each colour
+    int nx, ny, nz, o;      // dims and LDS offset
+    int cnt[2];             // FLUID cells of each colour
+    const uint16_t *sten;   // the level's stencil words and right-hand side in global memory
+    const float *r;
 };
 __device__ __forceinline__ int t_cells(const TailLevel &v) { return v.nx * v.ny * v.nz; }
 // LDS index of the level's t-th FLUID cell (red ones first, then black ones)
@@ -632,6 +634,7 @@ __device__ __forceinline__ void body_mg_tail(const DevSim &S) {
     __shared__ float lz[MG_TAIL_CELLS], lr[MG_TAIL_CELLS];
     __shared__ uint16_t ls[MG_TAIL_CELLS], lcl[MG_TAIL_CELLS];
     __shared__ TailLevel tl[MG_MAX];
+    __shared__ int n_tail;
     const int T = S.mg_tail, L = S.mg_levels - 1;
     // level table (constant level indices: no scratch copy of S)
     if (threadIdx.x == 0) {
@@ -640,45 +643,52 @@ __device__ __forceinline__ void body_mg_tail(const DevSim &S) {
         for (int q = 1; q < MG_MAX; q++) {
             if (q >= T && q <= L) {
                 const MGLevel &m = S.mg[q];
-                tl[q] = TailLevel{m.nx, m.ny, m.nz, o, {0, 0}};
+                tl[q] = TailLevel{m.nx, m.ny, m.nz, o, {0, 0}, m.sten, m.r};
                 o += m.nx * m.ny * m.nz;
             }
         }
+        n_tail = o;
     }
     __syncthreads();
-    // LDS fill and the FLUID lists (one LDS atomic per wave and colour)
-    const int lane = threadIdx.x & (WAVE - 1);
-    const uint64_t below = (1ull << lane) - 1ull;
+    // LDS fill and the FLUID lists.  The tail's cells are taken as one range of at most
+    // MG_TAIL_CELLS (TAIL_PER_LANE per lane): every lane issues the global loads of all its
+    // cells first, then writes them to LDS and lists each FLUID cell with an LDS atomic on its
+    // level's colour count (in any order: every phase below updates cells independently).
+    constexpr int TAIL_PER_LANE = (MG_TAIL_CELLS + TBLOCK - 1) / TBLOCK;
+    const int total = n_tail;
+    int lv[TAIL_PER_LANE];
+    uint32_t st[TAIL_PER_LANE];
+    float rv[TAIL_PER_LANE];
 #pragma unroll
-    for (int q = 1; q < MG_MAX; q++) {
-        if (q >= T && q <= L) {
-            const MGLevel &m = S.mg[q];
-            const int n = m.nx * m.ny * m.nz, o = tl[q].o;
-            for (int c0 = 0; c0 < n; c0 += TBLOCK) {
-                const int c = c0 + (int)threadIdx.x;
-                uint32_t st = 0, black = 0;
-                if (c < n) {
-                    st = m.sten[c];
-                    int i, j, k;
-                    cell_ijk(m.nx, m.ny, (uint32_t)c, i, j, k);
-                    black = (uint32_t)(i + j + k) & 1u;
-                    ls[o + c] = (uint16_t)st;
-                    lz[o + c] = 0.0f;
-                    if (q == T) lr[o + c] = m.r[c];
-                }
-                const bool fl = (st & STEN_FLUID) != 0;
+    for (int u = 0; u < TAIL_PER_LANE; u++) {
+        const int g = u * TBLOCK + (int)threadIdx.x;
+        int q = T;
+        while (q < L && g >= tl[q + 1].o) q++;
+        lv[u] = q;
+        st[u] = 0;
+        rv[u] = 0.0f;
+        if (g < total) {
+            const TailLevel &v = tl[q];
+            st[u] = v.sten[g - v.o];
+            if (q == T) rv[u] = v.r[g - v.o];
+        }
+    }
 #pragma unroll
-                for (int col = 0; col < 2; col++) {
-                    const bool mine = fl && black == (uint32_t)col;
-                    const uint64_t mask = __ballot(mine);
-                    const int leader = mask ? __ffsll((long long)mask) - 1 : 0;
-                    int base = 0;
-                    if (mine && lane == leader) base = atomicAdd(&tl[q].cnt[col], __popcll(mask));
-                    base = __shfl(base, leader, WAVE);
-                    const int slot = base + __popcll(mask & below);
-                    if (mine) lcl[col ? o + n - 1 - slot : o + slot] = (uint16_t)(o + c);
-                }
-            }
+    for (int u = 0; u < TAIL_PER_LANE; u++) {
+        const int g = u * TBLOCK + (int)threadIdx.x;
+        if (g >= total) break;
+        const int q = lv[u];
+        TailLevel &v = tl[q];
+        const int c = g - v.o;
+        ls[g] = (uint16_t)st[u];
+        lz[g] = 0.0f;
+        if (q == T) lr[g] = rv[u];
+        if (st[u] & STEN_FLUID) {
+            int i, j, k;
+            cell_ijk(v.nx, v.ny, (uint32_t)c, i, j, k);
+            const int col = (i + j + k) & 1;
+            const int slot = atomicAdd(&v.cnt[col], 1);
+            lcl[col ? v.o + t_cells(v) - 1 - slot : v.o + slot] = (uint16_t)g;
         }
     }
     __syncthreads();
