@@ -392,6 +392,8 @@ def bench_serve(args, pxl):
     number of batched steps the server ran in it (ms_per_step x steps = the timed region).
     Latency: the same stream again with Poisson arrivals at --load (0.7) of that capacity, in
     requests per second of the stream's mean work; p50/p99 are submit -> completion there.
+    small_batch: the first --small-requests requests at once with at most --small-batch in the
+    batch (capacity where the batched step's fixed cost is not amortised).
     plain: every --plain-every-th request alone on the single-simulation FP32 CG-80 path,
     creation included.  roofline: two event-timed batched steps of the batch the capacity run
     held at its start (the requests it admitted at once)."""
@@ -433,6 +435,21 @@ def bench_serve(args, pxl):
         lwall, lids, lst = _stream(pxl, srv, reqs, arrivals, REQUEST_STEPS)
         lper = [srv.poll(i) for i in lids]
     failed += [p for p in lper if p["status"] != "done"]
+
+    # small batches: the first --small-requests requests at once, at most --small-batch in the
+    # batch (the batched step's fixed cost, which a deep batch amortises)
+    small = None
+    if args.small_batch > 0 and args.small_requests > 0:
+        with pxl.Server(**dict(opts, max_sims=args.small_batch)) as srv:
+            swall, sids, sst = _stream(pxl, srv, reqs[:args.small_requests], np.zeros(args.small_requests),
+                                       REQUEST_STEPS)
+            sper = [srv.poll(i) for i in sids]
+        failed += [p for p in sper if p["status"] != "done"]
+        small = {"requests": args.small_requests, "max_batch": args.small_batch, "offered": "all requests at t = 0",
+                 "value": sst["particle_steps"] / swall, "wall_s": swall, "mean_batch": sst["mean_batch"],
+                 "batch_steps": sst["batch_steps"], "ms_per_step": 1e3 * swall / max(sst["batch_steps"], 1),
+                 "busy_fraction": sst["busy_s"] / swall, "p50_ms": sst["p50_ms"], "p99_ms": sst["p99_ms"],
+                 "max_rel_residual": max(p["rel_residual"] for p in sper)}
 
     # plain: a sample of the same requests, one at a time, FP32 CG with 80 iterations
     sample = list(range(0, len(reqs), max(1, args.plain_every)))
@@ -496,6 +513,7 @@ def bench_serve(args, pxl):
             "p50_ms": lst["p50_ms"], "p99_ms": lst["p99_ms"],
             "failed": len(failed), "max_rel_residual": max(p["rel_residual"] for p in per + lper),
             "plain": plain, "ratio_vs_plain": value / plain["value"],
+            "small_batch": dict(small, ratio_vs_plain=small["value"] / plain["value"]) if small else None,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
@@ -652,8 +670,10 @@ def bench_probe(args, pxl):
     of CALIB_BYTES (the calibration), then the launches the roofline of --probe-kind prices:
     serve -- --probe-steps event-timed batched steps of the fixed batch of the first
     --profile-batch stream requests; dam -- two steps of the 128^3 dam break (FP32 CG 80) after
-    one; frame -- the 128^3 dam break's surface pass after three frames."""
-    from tests.scenes import FPS, dam_break, random_request
+    one; frame -- the 128^3 dam break's surface pass after three frames; mixed128 -- three eager
+    steps of the single 128^3 two-block simulation with the MIXED solve (single-simulation
+    kernels, one symbol per multigrid level kind)."""
+    from tests.scenes import FPS, dam_break, random_request, two_blocks
     a, b = pxl.DeviceBuffer(CALIB_BYTES), pxl.DeviceBuffer(CALIB_BYTES)
     b.copy_from(a)
     a.free()
@@ -664,6 +684,11 @@ def bench_probe(args, pxl):
         sim.profile_steps(2)
         sim.close()
         return {"probe": "dam"}
+    if args.probe_kind == "mixed128":
+        sim = pxl.FluidState(two_blocks(128), FPS, seed=0, solver="mixed", use_graph=False)
+        sim.step(3)
+        sim.close()
+        return {"probe": "mixed128"}
     if args.probe_kind == "frame":
         f = dam_break(128, iters=80)
         sim = pxl.FluidState(f, FPS, seed=0, solver="cg", cg_iterations=80)
@@ -758,13 +783,15 @@ def main():
     ap.add_argument("--load", type=float, default=0.7,
                     help="serve: Poisson arrival rate of the latency run as a fraction of the measured capacity")
     ap.add_argument("--max-sims", type=int, default=512, help="serve: most simulations in the batch")
+    ap.add_argument("--small-batch", type=int, default=8, help="serve: batch limit of the small-batch run (0: skip)")
+    ap.add_argument("--small-requests", type=int, default=96, help="serve: requests of the small-batch run")
     ap.add_argument("--inner-iterations", type=int, default=9,
                     help="serve, batch: MG-CG iterations of the first refinement pass (later passes 2, 4, ...)")
     ap.add_argument("--budget", type=float, default=200e9)
     ap.add_argument("--plain-every", type=int, default=8, help="plain path on every k-th request")
     ap.add_argument("--profile-batch", type=int, default=64)
     ap.add_argument("--probe-steps", type=int, default=2, help="probe: event-timed batched steps")
-    ap.add_argument("--probe-kind", default="serve", choices=["serve", "dam", "frame"], help="probe: which launches")
+    ap.add_argument("--probe-kind", default="serve", choices=["serve", "dam", "frame", "mixed128"], help="probe: which launches")
     ap.add_argument("--profile-every", type=int, default=20,
                     help="serve: every k-th batched step of the capacity run is event-timed per launch (roofline)")
     ap.add_argument("--traffic", type=int, default=1, help="serve: measure the dominant kernel's HBM traffic with "
