@@ -188,6 +188,8 @@ int prepare(const char *fluid_json, double fps, uint32_t seed, const pxlflip_opt
             c.nx = (d.mg[l].nx + 1) / 2;
             c.ny = (d.mg[l].ny + 1) / 2;
             c.nz = (d.mg[l].nz + 1) / 2;
+            c.hx = (c.nx + 1) / 2;
+            c.hsplit = (uint32_t)c.hx * (uint32_t)c.ny * (uint32_t)c.nz;
             l++;
         }
         d.mg_levels = l + 1;

@@ -48,14 +48,22 @@ constexpr int MG_MID_CELLS = 32768;
 
 struct MGLevel {
     int nx, ny, nz;
-    uint8_t *type;   // level-l cell types (levels >= 1)
-    uint16_t *sten;  // level-l stencil words (same layout as DevSim::sten; levels >= 1)
+    // Levels >= 1 store sten, r and z colour-split like the MIXED inner vectors: cell (i, j, k)
+    // at sigma = ((i + j + k) & 1) hsplit + (i >> 1) + hx (j + ny k), hx = ceil(nx / 2),
+    // hsplit = hx ny nz (2 hsplit entries), so a colour sweep reads and writes dense runs.
+    int hx;
+    uint32_t hsplit;
+    uint8_t *type;   // level-l cell types in cell order (levels >= 1)
+    uint16_t *sten;  // level-l stencil words (the bits of DevSim::sten plus STEN_BLACK; levels >= 1)
     float *r, *z;    // level-l right-hand side (the scaled restricted residual) and iterate
     // FLUID cells of the level (levels >= 1, rebuilt each step), each list in cell order: red
     // cells from the front (count[0] of them), black cells from the back (count[1]: the t-th
     // at list[cells - 1 - t]); nlist holds all of them (count[2]) for the kernels that treat
-    // both colours alike.  Cell order keeps the neighbour rows and planes that a list slice
-    // reads next to the ones its neighbouring slices read.
+    // both colours alike (pre0 and the restriction into the level: walking the colours one
+    // after the other there reads the finer level's lines twice, measured 1.9x the bytes).
+    // Cell order keeps the neighbour rows and planes that a list slice reads next to the ones
+    // its neighbouring slices read.  An entry is the cell's sigma with i & 1 in bit 31 (the x
+    // neighbours' offsets depend on it).
     uint32_t *list, *nlist, *count;
     // per 64-cell chunk of the level: its red FLUID cells [0, chunks), then its black ones
     // [chunks, 2 chunks); the level scan turns both halves into exclusive offsets

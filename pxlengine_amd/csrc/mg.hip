@@ -60,9 +60,42 @@ __device__ __forceinline__ uint8_t level_type(const MGLevel &v, int i, int j, in
     if (i < 0 || j < 0 || k < 0 || i >= v.nx || j >= v.ny || k >= v.nz) return CELL_SOLID;
     return v.type[lidx(v, i, j, k)];
 }
-// Index of the coarse cell containing fine cell (i, j, k).
+// Cell-order index of the coarse cell containing fine cell (i, j, k) (the tail's LDS levels).
 __device__ __forceinline__ uint32_t parent(int cnx, int cny, int i, int j, int k) {
     return (uint32_t)(i >> 1) + (uint32_t)cnx * ((uint32_t)(j >> 1) + (uint32_t)cny * (uint32_t)(k >> 1));
+}
+// Colour-split index of cell (i, j, k) of a coarse level (MGLevel::hx).
+__device__ __forceinline__ uint32_t lsig(const MGLevel &v, int i, int j, int k) {
+    return (((uint32_t)(i + j + k) & 1u) ? v.hsplit : 0u) + (uint32_t)(i >> 1) +
+           (uint32_t)v.hx * ((uint32_t)j + (uint32_t)v.ny * (uint32_t)k);
+}
+// The colour-split index of the coarse cell containing fine cell (i, j, k) (i, j, k >= 0).
+__device__ __forceinline__ uint32_t parent_sig(const MGLevel &c, int i, int j, int k) {
+    return lsig(c, i >> 1, j >> 1, k >> 1);
+}
+__device__ __forceinline__ uint32_t sig_last(const MGLevel &v) { return 2u * v.hsplit - 1u; }
+// A coarse list entry: sigma | (i & 1) << 31.
+__device__ __forceinline__ uint32_t e_sig(uint32_t e) { return e & 0x7fffffffu; }
+__device__ __forceinline__ int32_t e_odd(uint32_t e) { return (int32_t)(e >> 31); }
+// Offsets from a cell's sigma to its -x, +x, -y, +y, -z, +z neighbours (all of the other
+// colour): d - 1 + o, d + o, d -/+ hx, d -/+ hx ny, d = +hsplit from red, -hsplit from black,
+// o = i & 1.
+__device__ __forceinline__ void sig_offsets(const MGLevel &v, bool black, int32_t o, int32_t off[6]) {
+    const int32_t d = black ? -(int32_t)v.hsplit : (int32_t)v.hsplit, hy = v.hx, hz = v.hx * v.ny;
+    off[0] = d - 1 + o;
+    off[1] = d + o;
+    off[2] = d - hy;
+    off[3] = d + hy;
+    off[4] = d - hz;
+    off[5] = d + hz;
+}
+// Grid coordinates of coarse list entry e.
+__device__ __forceinline__ void e_ijk(const MGLevel &v, uint32_t e, int &i, int &j, int &k) {
+    const uint32_t sg = e_sig(e), q = sg >= v.hsplit ? sg - v.hsplit : sg;
+    const uint32_t row = q / (uint32_t)v.hx;
+    i = 2 * (int)(q - row * (uint32_t)v.hx) + e_odd(e);
+    j = (int)(row % (uint32_t)v.ny);
+    k = (int)(row / (uint32_t)v.ny);
 }
 // Workgroup b's (of nb) slice [beg, end) of an n-entry list.  Workgroups are dealt to the 8
 // XCDs round-robin (workgroup i runs on XCD i % 8) and each XCD has its own L2: with nb a
@@ -107,6 +140,7 @@ __device__ __forceinline__ void body_mg_sten(const MGLevel &v, uint32_t b) {
     if (in) {
         int i, j, k;
         cell_ijk(v.nx, v.ny, id, i, j, k);
+        const uint32_t sg = lsig(v, i, j, k);
         if (v.type[id] == CELL_FLUID) {
             const int nb[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k}, {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
             uint32_t diag = 0;
@@ -119,8 +153,8 @@ __device__ __forceinline__ void body_mg_sten(const MGLevel &v, uint32_t b) {
             black = ((i + j + k) & 1) != 0;
             st |= STEN_FLUID | (diag << 8) | (black ? STEN_BLACK : 0u);
         }
-        v.sten[id] = (uint16_t)st;
-        v.z[id] = 0.0f;
+        v.sten[sg] = (uint16_t)st;
+        v.z[sg] = 0.0f;
     }
     const bool fl = (st & STEN_FLUID) != 0;
     const uint64_t mr = __ballot(fl && !black), mb = __ballot(fl && black);
@@ -199,16 +233,20 @@ __device__ __forceinline__ void body_mg_compact(const MGLevel &v, uint32_t b) {
     const uint32_t id = b * MBLOCK + threadIdx.x, n = ncell(v), nch = level_chunks(v);
     const uint32_t c = id / WAVE;  // the same for the whole wave
     if (c >= nch) return;
-    const uint32_t st = id < n ? (uint32_t)v.sten[id] : 0u;
+    int i = 0, j = 0, k = 0;
+    if (id < n) cell_ijk(v.nx, v.ny, id, i, j, k);
+    const uint32_t sg = lsig(v, i, j, k);
+    const uint32_t st = id < n ? (uint32_t)v.sten[sg] : 0u;
     const bool fl = (st & STEN_FLUID) != 0, black = (st & STEN_BLACK) != 0;
     const uint64_t mr = __ballot(fl && !black), mb = __ballot(fl && black);
     if (!fl) return;
     const uint64_t below = (1ull << (threadIdx.x & (WAVE - 1))) - 1ull;
     const uint32_t ro = v.chunk[c], bo = v.chunk[nch + c];
     const uint32_t rank = (uint32_t)__popcll((black ? mb : mr) & below);
-    if (black) v.list[n - 1u - (bo + rank)] = id;
-    else v.list[ro + rank] = id;
-    v.nlist[ro + bo + (uint32_t)__popcll((mr | mb) & below)] = id;
+    const uint32_t e = sg | ((uint32_t)(i & 1) << 31);
+    if (black) v.list[n - 1u - (bo + rank)] = e;
+    else v.list[ro + rank] = e;
+    v.nlist[ro + bo + (uint32_t)__popcll((mr | mb) & below)] = e;
 }
 __global__ __launch_bounds__(TBLOCK) void k_mg_scan(DevSim S) { body_mg_scan(S, nullptr); }
 __global__ __launch_bounds__(MBLOCK) void k_mg_compact(MGLevel v) { body_mg_compact(v, blockIdx.x); }
@@ -327,7 +365,7 @@ __device__ __forceinline__ void body_l0_post(const DevSim &S, const uint32_t *fl
             for (int q = 0; q < 6; q++) {
                 const bool f = (wd[k] >> q) & 1u;
                 zq[k][q] = f ? u[sid[k] + off[q]] : 0.0f;
-                pq[k][q] = f ? c.z[parent(c.nx, c.ny, nbc[q][0], nbc[q][1], nbc[q][2])] : 0.0f;
+                pq[k][q] = f ? c.z[parent_sig(c, nbc[q][0], nbc[q][1], nbc[q][2])] : 0.0f;
             }
             rc[k] = r[sid[k]];
         }
@@ -349,20 +387,21 @@ __device__ __forceinline__ void body_lc_pre0(const MGLevel &v, uint32_t b, uint3
     const uint32_t n = v.count[0] + v.count[1];
     uint32_t beg, end;
     slice(n, b, nb, beg, end);
-    const int32_t off[6] = {-1, 1, -v.nx, v.nx, -v.nx * v.ny, v.nx * v.ny};
-    const uint32_t last = ncell(v) - 1u;
+    const uint32_t last = sig_last(v);
     for (uint32_t t = beg + threadIdx.x; t < end; t += LBLOCK) {
-        const uint32_t id = v.nlist[t];
+        const uint32_t e = v.nlist[t], id = e_sig(e);
         const uint32_t st = v.sten[id];
         float sum = 0.0f;
-        if (st & STEN_BLACK) {
+        if (id >= v.hsplit) {  // black (its colour follows from the index)
             // neighbours are loaded before the cell's stencil word arrives (indices clamped
             // into the level) and the ones that are not FLUID are dropped afterwards
+            int32_t off[6];
+            sig_offsets(v, true, e_odd(e), off);
             float rq[6];
             uint32_t sq[6];
 #pragma unroll
             for (int q = 0; q < 6; q++) {
-                const uint32_t nq = min(id + (uint32_t)off[q], last);
+                const uint32_t nq = min((uint32_t)((int32_t)id + off[q]), last);
                 rq[q] = v.r[nq];
                 sq[q] = v.sten[nq];
             }
@@ -381,18 +420,22 @@ __device__ __forceinline__ void body_lc_sweep(const MGLevel &v, uint32_t colour,
     const uint32_t last = ncell(v) - 1u;
     uint32_t beg, end;
     slice(n, b, nb, beg, end);
-    const int32_t off[6] = {-1, 1, -v.nx, v.nx, -v.nx * v.ny, v.nx * v.ny};
+    const int32_t d = colour ? -(int32_t)v.hsplit : (int32_t)v.hsplit, hy = v.hx, hz = v.hx * v.ny;
     for (uint32_t t = beg + threadIdx.x; t < end; t += 2 * LBLOCK) {
         const uint32_t t2 = t + LBLOCK;
         const bool two = t2 < end;
-        const uint32_t ia = colour ? list[last - t] : list[t];
-        const uint32_t ib = two ? (colour ? list[last - t2] : list[t2]) : ia;
+        const uint32_t ea = colour ? list[last - t] : list[t];
+        const uint32_t eb = two ? (colour ? list[last - t2] : list[t2]) : ea;
+        const uint32_t ia = e_sig(ea), ib = e_sig(eb);
+        const int32_t oa = e_odd(ea), ob = e_odd(eb);
+        const int32_t offa[6] = {d - 1 + oa, d + oa, d - hy, d + hy, d - hz, d + hz};
+        const int32_t offb[6] = {d - 1 + ob, d + ob, d - hy, d + hy, d - hz, d + hz};
         const uint32_t sta = v.sten[ia], stb = two ? (uint32_t)v.sten[ib] : 0u;
         float sa = 0.0f, sb = 0.0f;
 #pragma unroll
         for (int q = 0; q < 6; q++) {
-            const float za = (sta & (1u << q)) ? v.z[ia + off[q]] : 0.0f;
-            const float zb = (stb & (1u << q)) ? v.z[ib + off[q]] : 0.0f;
+            const float za = (sta & (1u << q)) ? v.z[ia + offa[q]] : 0.0f;
+            const float zb = (stb & (1u << q)) ? v.z[ib + offb[q]] : 0.0f;
             sa += za;
             sb += zb;
         }
@@ -407,9 +450,9 @@ __device__ __forceinline__ void body_lc_post(const MGLevel &v, const MGLevel &c,
     uint32_t beg, end;
     slice(n, b, nb, beg, end);
     for (uint32_t t = beg + threadIdx.x; t < end; t += LBLOCK) {
-        const uint32_t id = v.list[last - t];
+        const uint32_t e = v.list[last - t], id = e_sig(e);
         int i, j, k;
-        cell_ijk(v.nx, v.ny, id, i, j, k);
+        e_ijk(v, e, i, j, k);
         const int nbc[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k}, {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
         // neighbour and parent values are loaded together with the stencil word (coordinates
         // clamped into the level); the ones that are not FLUID are dropped afterwards
@@ -418,8 +461,8 @@ __device__ __forceinline__ void body_lc_post(const MGLevel &v, const MGLevel &c,
         for (int q = 0; q < 6; q++) {
             const int a = min(max(nbc[q][0], 0), v.nx - 1), bb = min(max(nbc[q][1], 0), v.ny - 1),
                       d = min(max(nbc[q][2], 0), v.nz - 1);
-            zq[q] = v.z[lidx(v, a, bb, d)];
-            pq[q] = c.z[parent(c.nx, c.ny, a, bb, d)];
+            zq[q] = v.z[lsig(v, a, bb, d)];
+            pq[q] = c.z[parent_sig(c, a, bb, d)];
         }
         const uint32_t st = v.sten[id];
         float sum = 0.0f;
@@ -431,69 +474,27 @@ __device__ __forceinline__ void body_lc_post(const MGLevel &v, const MGLevel &c,
 
 // ---- restrict level f -> c after f's black sweep: for each FLUID cell of c (its cell-order
 // list, so neighbouring lanes read neighbouring children), c.r = 1/2 sum over FLUID red
-// children (q = 0, 3, 5, 6) of r - L z.  f's arrays are given
-// explicitly (level 0: the CG vectors and the grid's stencil words).
-__device__ __forceinline__ void body_restrict(int fnx, int fny, int fnz, const uint16_t *__restrict__ fsten,
-                                              const float *__restrict__ fr, const float *__restrict__ fz,
-                                              const MGLevel &c, uint32_t b, uint32_t nb) {
+// children (q = 0, 3, 5, 6) of r - L z.  With f colour-split, red child (fi, fj, fk) of coarse
+// cell (ci, cj, ck) sits at (fi >> 1) + hx (fj + ny fk) = ci + hx (fj + ny fk) in f's red half
+// and its (black) neighbours at that index + hsplit + the x parity's offsets; every child's
+// values and neighbours are loaded before its stencil word arrives (indices clamped into the
+// level; children past its edge read index 0 and are skipped, neighbours that are not FLUID
+// are dropped afterwards).  f's arrays are given explicitly (level 0: the CG vectors and the
+// grid's stencil words, whose red cells sit at the same split index).
+template <bool Level0>
+__device__ __forceinline__ void restrict_split(int fnx, int fny, int fnz, uint32_t fhx, uint32_t fhsplit,
+                                               const uint16_t *__restrict__ fsten, const float *__restrict__ fr,
+                                               const float *__restrict__ fz, const MGLevel &c, uint32_t b,
+                                               uint32_t nb) {
     const uint32_t n = c.count[0] + c.count[1];
     uint32_t beg, end;
     slice(n, b, nb, beg, end);
-    const int32_t off[6] = {-1, 1, -fnx, fnx, -fnx * fny, fnx * fny};
-    const uint32_t flast = (uint32_t)(fnx * fny * fnz) - 1u;
+    const int32_t hs = (int32_t)fhsplit, hy = (int32_t)fhx, hz = (int32_t)fhx * fny;
+    const uint32_t vlast = 2u * fhsplit - 1u;
     for (uint32_t t = beg + threadIdx.x; t < end; t += LBLOCK) {
-        const uint32_t cc = c.nlist[t];
+        const uint32_t ce = c.nlist[t];
         int ci, cj, ck;
-        cell_ijk(c.nx, c.ny, cc, ci, cj, ck);
-        const int red[4] = {0, 3, 5, 6};
-        // every child's values and neighbours are loaded before its stencil word arrives
-        // (indices clamped into the fine level; children past its edge read cell 0 and are
-        // skipped, neighbours that are not FLUID are dropped afterwards)
-        bool in[4];
-        uint32_t st[4];
-        float rc[4], zc[4], zn[4][6];
-#pragma unroll
-        for (int m = 0; m < 4; m++) {
-            const int q = red[m];
-            const int fi = 2 * ci + (q & 1), fj = 2 * cj + ((q >> 1) & 1), fk = 2 * ck + (q >> 2);
-            in[m] = fi < fnx && fj < fny && fk < fnz;
-            const uint32_t id =
-                in[m] ? (uint32_t)fi + (uint32_t)fnx * ((uint32_t)fj + (uint32_t)fny * (uint32_t)fk) : 0u;
-            st[m] = fsten[id];
-            rc[m] = fr[id];
-            zc[m] = fz[id];
-#pragma unroll
-            for (int n = 0; n < 6; n++) zn[m][n] = fz[min(id + (uint32_t)off[n], flast)];
-        }
-        float acc = 0.0f;
-#pragma unroll
-        for (int m = 0; m < 4; m++) {
-            if (!in[m] || !(st[m] & STEN_FLUID)) continue;
-            float sum = 0.0f;
-#pragma unroll
-            for (int n = 0; n < 6; n++) sum = (st[m] & (1u << n)) ? sum + zn[m][n] : sum;
-            acc += rc[m] - st_diag(st[m]) * zc[m] + sum;
-        }
-        c.r[cc] = 0.5f * acc;
-    }
-}
-
-// The same restriction from level 0, whose r and z are colour-split: red child (fi, fj, fk) of
-// coarse cell (ci, cj, ck) sits at (fi >> 1) + hx (fj + ny fk) = ci + hx (fj + ny fk) in the red
-// half, its (black) neighbours at that index + hsplit + split_offsets(red, fi & 1).
-__device__ __forceinline__ void body_restrict0(const DevSim &S, const MGLevel &c, uint32_t b, uint32_t nb) {
-    const uint32_t n = c.count[0] + c.count[1];
-    uint32_t beg, end;
-    slice(n, b, nb, beg, end);
-    const int fnx = S.nx, fny = S.ny, fnz = S.nz;
-    const int32_t hs = (int32_t)S.hsplit, hy = (int32_t)S.hx, hz = (int32_t)S.hx * fny;
-    const uint32_t vlast = 2u * S.hsplit - 1u;
-    const float *__restrict__ fr = S.cg_r0;
-    const float *__restrict__ fz = S.cg_r1;
-    for (uint32_t t = beg + threadIdx.x; t < end; t += LBLOCK) {
-        const uint32_t cc = c.nlist[t];
-        int ci, cj, ck;
-        cell_ijk(c.nx, c.ny, cc, ci, cj, ck);
+        e_ijk(c, ce, ci, cj, ck);
         const int red[4] = {0, 3, 5, 6};
         bool in[4];
         uint32_t st[4];
@@ -503,10 +504,12 @@ __device__ __forceinline__ void body_restrict0(const DevSim &S, const MGLevel &c
             const int q = red[m];
             const int a = q & 1, fi = 2 * ci + a, fj = 2 * cj + ((q >> 1) & 1), fk = 2 * ck + (q >> 2);
             in[m] = fi < fnx && fj < fny && fk < fnz;
-            const uint32_t gid =
-                in[m] ? (uint32_t)fi + (uint32_t)fnx * ((uint32_t)fj + (uint32_t)fny * (uint32_t)fk) : 0u;
-            const uint32_t sid = in[m] ? (uint32_t)ci + S.hx * ((uint32_t)fj + (uint32_t)fny * (uint32_t)fk) : 0u;
-            st[m] = S.sten[gid];
+            const uint32_t sid = in[m] ? (uint32_t)ci + fhx * ((uint32_t)fj + (uint32_t)fny * (uint32_t)fk) : 0u;
+            // level 0's stencil words are in cell order; a coarse level's are colour-split
+            const uint32_t wid =
+                Level0 ? (in[m] ? (uint32_t)fi + (uint32_t)fnx * ((uint32_t)fj + (uint32_t)fny * (uint32_t)fk) : 0u)
+                       : sid;
+            st[m] = fsten[wid];
             rc[m] = fr[sid];
             zc[m] = fz[sid];
             const int32_t off[6] = {hs - 1 + a, hs + a, hs - hy, hs + hy, hs - hz, hs + hz};
@@ -522,8 +525,14 @@ __device__ __forceinline__ void body_restrict0(const DevSim &S, const MGLevel &c
             for (int e = 0; e < 6; e++) sum = (st[m] & (1u << e)) ? sum + zn[m][e] : sum;
             acc += rc[m] - st_diag(st[m]) * zc[m] + sum;
         }
-        c.r[cc] = 0.5f * acc;
+        c.r[e_sig(ce)] = 0.5f * acc;
     }
+}
+__device__ __forceinline__ void body_restrict(const MGLevel &f, const MGLevel &c, uint32_t b, uint32_t nb) {
+    restrict_split<false>(f.nx, f.ny, f.nz, (uint32_t)f.hx, f.hsplit, f.sten, f.r, f.z, c, b, nb);
+}
+__device__ __forceinline__ void body_restrict0(const DevSim &S, const MGLevel &c, uint32_t b, uint32_t nb) {
+    restrict_split<true>(S.nx, S.ny, S.nz, (uint32_t)S.hx, S.hsplit, S.sten, S.cg_r0, S.cg_r1, c, b, nb);
 }
 
 // ---- the tail: levels mg_tail..L in one workgroup.  LDS holds every tail level's z, r and
@@ -537,7 +546,9 @@ struct TailLevel {
     int nx, ny, nz, o;      // dims and LDS offset
     int cnt[2];             // FLUID cells of each colour
     const uint16_t *sten;   // the level's stencil words and right-hand side in global memory
-    const float *r;
+    const float *r;         // (colour-split: MGLevel::hx)
+    int hx;
+    uint32_t hsplit;
 };
 __device__ __forceinline__ int t_cells(const TailLevel &v) { return v.nx * v.ny * v.nz; }
 // LDS index of the level's t-th FLUID cell (red ones first, then black ones)
@@ -643,7 +654,7 @@ __device__ __forceinline__ void body_mg_tail(const DevSim &S) {
         for (int q = 1; q < MG_MAX; q++) {
             if (q >= T && q <= L) {
                 const MGLevel &m = S.mg[q];
-                tl[q] = TailLevel{m.nx, m.ny, m.nz, o, {0, 0}, m.sten, m.r};
+                tl[q] = TailLevel{m.nx, m.ny, m.nz, o, {0, 0}, m.sten, m.r, m.hx, m.hsplit};
                 o += m.nx * m.ny * m.nz;
             }
         }
@@ -669,8 +680,12 @@ __device__ __forceinline__ void body_mg_tail(const DevSim &S) {
         rv[u] = 0.0f;
         if (g < total) {
             const TailLevel &v = tl[q];
-            st[u] = v.sten[g - v.o];
-            if (q == T) rv[u] = v.r[g - v.o];
+            int i, j, k;
+            cell_ijk(v.nx, v.ny, (uint32_t)(g - v.o), i, j, k);
+            const uint32_t sg = (((uint32_t)(i + j + k) & 1u) ? v.hsplit : 0u) + (uint32_t)(i >> 1) +
+                                (uint32_t)v.hx * ((uint32_t)j + (uint32_t)v.ny * (uint32_t)k);
+            st[u] = v.sten[sg];
+            if (q == T) rv[u] = v.r[sg];
         }
     }
 #pragma unroll
@@ -726,7 +741,11 @@ __device__ __forceinline__ void body_mg_tail(const DevSim &S) {
         if (q == T) {
             const MGLevel &m = S.mg[q];
             const int n = m.nx * m.ny * m.nz;
-            for (int c = threadIdx.x; c < n; c += TBLOCK) m.z[c] = lz[c];
+            for (int c = threadIdx.x; c < n; c += TBLOCK) {
+                int i, j, k;
+                cell_ijk(m.nx, m.ny, (uint32_t)c, i, j, k);
+                m.z[lsig(m, i, j, k)] = lz[c];
+            }
         }
     }
     __syncthreads();  // the LDS is reused by the next simulation of a batched launch
@@ -751,7 +770,7 @@ __device__ __forceinline__ void body_mg_midtail(const DevSim &S, uint32_t *cnt) 
                 body_lc_sweep(v, 1, 0, 1);
                 __syncthreads();
             }
-            body_restrict(v.nx, v.ny, v.nz, v.sten, v.r, v.z, c, 0, 1);
+            body_restrict(v, c, 0, 1);
             __syncthreads();
         }
     }
@@ -790,8 +809,8 @@ __global__ __launch_bounds__(LBLOCK) void k_lc_sweep(MGLevel v, uint32_t colour)
     body_lc_sweep(v, colour, blockIdx.x, gridDim.x);
 }
 __global__ __launch_bounds__(LBLOCK) void k_lc_post(MGLevel v, MGLevel c) { body_lc_post(v, c, blockIdx.x, gridDim.x); }
-__global__ __launch_bounds__(LBLOCK) void k_restrict(MGLevel f, const float *fr, const float *fz, MGLevel c) {
-    body_restrict(f.nx, f.ny, f.nz, f.sten, fr, fz, c, blockIdx.x, gridDim.x);
+__global__ __launch_bounds__(LBLOCK) void k_restrict(MGLevel f, MGLevel c) {
+    body_restrict(f, c, blockIdx.x, gridDim.x);
 }
 __global__ __launch_bounds__(LBLOCK) void k_restrict0(DevSim S, MGLevel c) { body_restrict0(S, c, blockIdx.x, gridDim.x); }
 
@@ -865,7 +884,7 @@ __global__ __launch_bounds__(LBLOCK) void kb_restrict(WorkList W, int l) {
             body_restrict0(S, c, b, S.cg_blocks);
         } else {
             const MGLevel f = S.mg[l];
-            body_restrict(f.nx, f.ny, f.nz, f.sten, f.r, f.z, c, b, level_blocks(S.cg_blocks, l));
+            body_restrict(f, c, b, level_blocks(S.cg_blocks, l));
         }
     });
 }
@@ -914,7 +933,7 @@ void launch_mg_vcycle(const DevSim &S, const float *r, float *u, hipStream_t s) 
                 PXL_LAUNCH(k_lc_sweep, g, LBLOCK, 0, s, S.mg[l], 0);
                 PXL_LAUNCH(k_lc_sweep, g, LBLOCK, 0, s, S.mg[l], 1);
             }
-            PXL_LAUNCH(k_restrict, g, LBLOCK, 0, s, S.mg[l], S.mg[l].r, S.mg[l].z, S.mg[l + 1]);
+            PXL_LAUNCH(k_restrict, g, LBLOCK, 0, s, S.mg[l], S.mg[l + 1]);
         }
     }
     PXL_LAUNCH(k_mg_tail, 1, TBLOCK, 0, s, S);

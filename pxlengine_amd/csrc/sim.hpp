@@ -226,10 +226,11 @@ struct Sim {
         if (opts.solver == PXLFLIP_SOLVER_MIXED) {
             for (int l = 1; l < d.mg_levels; l++) {
                 const size_t cl = (size_t)d.mg[l].nx * d.mg[l].ny * d.mg[l].nz;
+                const size_t sl = 2ull * d.mg[l].hsplit;  // colour-split arrays (>= cl)
                 add(&d.mg[l].type, cl);
-                add(&d.mg[l].sten, 2 * cl);
-                add(&d.mg[l].r, 4 * cl);
-                add(&d.mg[l].z, 4 * cl);
+                add(&d.mg[l].sten, 2 * sl);
+                add(&d.mg[l].r, 4 * sl);
+                add(&d.mg[l].z, 4 * sl);
                 add(&d.mg[l].list, 4 * cl);
                 add(&d.mg[l].nlist, 4 * cl);
                 add(&d.mg[l].count, 64);
