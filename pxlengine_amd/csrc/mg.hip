@@ -25,8 +25,9 @@
 // black ones (the step's colour-split fluid list on level 0, per-level lists built with the
 // coarse types), so the work follows the liquid, not the grid, and a colour sweep keeps every
 // lane busy; a launch takes a list segment in level_blocks slices (the solver kernels' count on
-// level 0, an eighth of it per coarser level), and a lane updates two cells with all their
-// loads in flight together.
+// level 0, an eighth of it per coarser level), and a lane has the loads of several cells in
+// flight together.  Every level stores its vectors colour-split (red cells in one half, black
+// in the other, each in cell order; MGLevel), so a colour sweep reads and writes dense runs.
 // Coarse z is kept 0 on non-FLUID cells (the prolongation reads the parent of any FLUID
 // cell).  Levels from mg_tail on (a few thousand cells together) run the same cycle in one
 // workgroup with everything in LDS.  Arithmetic is fp32 (the FP64 refinement outside fixes
