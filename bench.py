@@ -792,7 +792,7 @@ def main():
     ap.add_argument("--profile-batch", type=int, default=64)
     ap.add_argument("--probe-steps", type=int, default=2, help="probe: event-timed batched steps")
     ap.add_argument("--probe-kind", default="serve", choices=["serve", "dam", "frame", "mixed128"], help="probe: which launches")
-    ap.add_argument("--profile-every", type=int, default=20,
+    ap.add_argument("--profile-every", type=int, default=40,
                     help="serve: every k-th batched step of the capacity run is event-timed per launch (roofline)")
     ap.add_argument("--traffic", type=int, default=1, help="serve: measure the dominant kernel's HBM traffic with "
                                                             "rocprofv3 counters in two child passes (0 = off)")

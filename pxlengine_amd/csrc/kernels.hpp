@@ -468,7 +468,7 @@ enum WorkDomain : int {
     WD_TILES,       // tiles (active-tile list)
     WD_TILEGRID,    // per-tile workgroups (P2G, G2P, and the band passes: faces, cells, project)
     WD_SCAN_ROWS,   // scan tiles of the fluid list's row counts
-    WD_ROWS,        // grid rows, four per workgroup (fluid-list compaction)
+    WD_ROWS,        // grid rows, 32 per workgroup (fluid-list compaction)
     WD_MGTYPES,     // + l, l = 1..MG_MAX-1: cells of level l
     WD_A_SIM = WD_MGTYPES + MG_MAX,
     WD_A_SOLVER,     // fluid-list solver workgroups
@@ -485,7 +485,7 @@ uint32_t particle_blocks(const DevSim &S);
 uint32_t tile_blocks(const DevSim &S);
 uint32_t tile_grid_blocks(const DevSim &S);
 uint32_t mg_cell_blocks(const DevSim &S, int l);
-uint32_t fluid_row_blocks(const DevSim &S);  // workgroups of the fluid-list compaction (a wave per row)
+uint32_t fluid_row_blocks(const DevSim &S);  // workgroups of the fluid-list compaction (8 rows per wave)
 // Workgroups (list slices) of the multigrid list launches on level l: the solver's count on
 // level 0, an eighth of it per coarsening after that (a coarse level holds about an eighth of
 // the FLUID cells), at least one.
