@@ -517,7 +517,7 @@ def bench_serve(args, pxl):
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "kernels": kernels[:12],
+        "kernels": kernels,
     }
 
 
