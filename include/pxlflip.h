@@ -360,7 +360,7 @@ int pxlflip_batch_get_grid(int family, uint32_t *workgroups);
 int pxlflip_batch_set_grid(int family, uint32_t workgroups);
 /* Workgroup sizes of the batched launch families compiled in two sizes (the batch autotuner's
  * shape knobs; no counterpart in the reference): PXLFLIP_SHAPE_TILE, the per-tile P2G / G2P
- * (1024 lanes, default, or 512), and PXLFLIP_SHAPE_BAND, the band passes faces, cells and
+ * (512 lanes, default, or 1024), and PXLFLIP_SHAPE_BAND, the band passes faces, cells and
  * project (256, default, or 512).  Lanes stride over a tile's particles, nodes or cells, so a size
  * changes the speed and never the results.  Process-wide, read when a launch is recorded (batch
  * graphs captured after the call); another size is PXLFLIP_ERR_INVALID. */

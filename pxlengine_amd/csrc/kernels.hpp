@@ -403,7 +403,7 @@ enum BatchFamily : int {
 unsigned batch_cap(int family);  // current maximum of a family (a multiple of 8)
 bool set_batch_cap(int family, unsigned workgroups);  // false: unknown family or value out of range
 // Batched launch families compiled in more than one workgroup size (the values are
-// PXLFLIP_SHAPE_* of pxlflip.h): the per-tile P2G / G2P (1024 or 512 lanes) and the band
+// PXLFLIP_SHAPE_* of pxlflip.h): the per-tile P2G / G2P (512 or 1024 lanes) and the band
 // passes faces, cells and project (256 or 512 lanes).  Lanes stride over a tile's particles,
 // nodes or cells, so the size changes the speed, not the results.
 enum BatchShape : int { BS_TILE = 0, BS_BAND, BS_COUNT };

@@ -771,7 +771,7 @@ void batch_scan_down(const WorkLists &W, int which, hipStream_t s) {
 void batch_tile_list(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_tile_list, batch_grid(W[WD_TILES], batch_cap(BF_PART)), BLOCK, 0, s, W[WD_TILES]); }
 void batch_band(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_band, batch_grid(W[WD_TILES], batch_cap(BF_PART)), BLOCK, 0, s, W[WD_TILES]); }
 void batch_scatter(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_scatter, batch_grid(W[WD_PART], batch_cap(BF_PART)), BLOCK, 0, s, W[WD_PART]); }
-// The per-tile launches in the workgroup size the tile-lane setting picks (1024 or 512 lanes:
+// The per-tile launches in the workgroup size the tile-lane setting picks (512, the default, or 1024 lanes:
 // lanes stride over a tile's particles, so the size changes the speed, not the results).
 void batch_p2g(const WorkLists &W, hipStream_t s) {
     const unsigned g = batch_grid(W[WD_TILEGRID], batch_cap(BF_TILE));

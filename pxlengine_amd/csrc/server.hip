@@ -683,7 +683,7 @@ bool set_batch_cap(int family, unsigned workgroups) {
 
 // Workgroup sizes of the families compiled in two sizes (the first is the default: on the
 // 64-request serving batch it was measured fastest when the sizes were fixed at compile time).
-constexpr unsigned kShapeSizes[BS_COUNT][2] = {{1024, 512}, {256, 512}};
+constexpr unsigned kShapeSizes[BS_COUNT][2] = {{512, 1024}, {256, 512}};  // default first
 std::atomic<uint32_t> g_batch_shapes[BS_COUNT] = {kShapeSizes[BS_TILE][0], kShapeSizes[BS_BAND][0]};
 static_assert(BS_TILE == PXLFLIP_SHAPE_TILE && BS_BAND == PXLFLIP_SHAPE_BAND && BS_COUNT == PXLFLIP_SHAPE_COUNT,
               "batch shape families match pxlflip.h");

@@ -848,7 +848,7 @@ class Batch:
 # PXLFLIP_GRID_* of pxlflip.h, in order
 GRID_FAMILIES = ("band", "particle", "tile", "mg_setup", "mg_list", "mg_tail", "solver")
 # PXLFLIP_SHAPE_* of pxlflip.h, in order, with the workgroup sizes compiled for each (default first)
-SHAPE_FAMILIES = {"tile_lanes": (1024, 512), "band_lanes": (256, 512)}
+SHAPE_FAMILIES = {"tile_lanes": (512, 1024), "band_lanes": (256, 512)}
 
 
 def batch_shapes() -> dict:

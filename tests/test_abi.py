@@ -306,8 +306,8 @@ def test_batch_shape_settings(pxl, tmp_path):
     """The workgroup sizes of the families compiled in two sizes (host-only settings): the
     defaults, validation (only the compiled sizes) and the JSON cache round trip."""
     saved = pxl.batch_shapes()
-    assert saved == {"tile_lanes": 1024, "band_lanes": 256}
-    assert pxl.SHAPE_FAMILIES == {"tile_lanes": (1024, 512), "band_lanes": (256, 512)}
+    assert saved == {"tile_lanes": 512, "band_lanes": 256}
+    assert pxl.SHAPE_FAMILIES == {"tile_lanes": (512, 1024), "band_lanes": (256, 512)}
     lib = pxl.load_library()
     try:
         pxl.set_batch_shapes({"tile_lanes": 512, "band_lanes": 512})
