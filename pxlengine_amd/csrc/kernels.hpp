@@ -153,6 +153,9 @@ struct DevSim {
     uint32_t *fls_list, *fns_list;
     // solver
     float *cg_r0, *cg_r1, *cg_w0, *cg_w1, *cg_s0, *cg_s1, *cg_p, *e;
+    // MIXED: the MG-CG iterate in colour-split list order (index t of fl_list, like p, s and w
+    // there; storage of cg_s1, which MIXED does not use otherwise)
+    float *cg_x;
     double *x64a, *x64b, *r64;
     MGLevel mg[MG_MAX];
     int mg_levels;  // levels including level 0 (0 when the multigrid is not allocated)
@@ -524,7 +527,7 @@ void batch_mg_restrict(const WorkLists &W, int l, hipStream_t s);  // level l ->
 void batch_mg_tail(const WorkLists &W, hipStream_t s);
 void batch_mg_post(const WorkLists &W, int l, hipStream_t s);
 void batch_mgcg_spmv(const WorkLists &W, uint32_t slot, int first, hipStream_t s);
-void batch_mgcg_update(const WorkLists &W, uint32_t it, hipStream_t s);
+void batch_mgcg_update(const WorkLists &W, uint32_t it, int last, hipStream_t s);  // last: x to e
 void batch_mixed_update(const WorkLists &W, int cur, hipStream_t s);
 // post
 void batch_project(const WorkLists &W, hipStream_t s);

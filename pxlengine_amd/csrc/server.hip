@@ -337,7 +337,7 @@ private:
         enqueue_vcycle();
         run("mgcg_spmv", -1, [&] { batch_mgcg_spmv(lists_, 0, 1, stream_); });
         for (uint32_t it = 0; it < inner; it++) {
-            run("mgcg_update", -1, [&] { batch_mgcg_update(lists_, it, stream_); });
+            run("mgcg_update", -1, [&] { batch_mgcg_update(lists_, it, it + 1 == inner ? 1 : 0, stream_); });
             if (it + 1 == inner) break;
             enqueue_vcycle();
             run("mgcg_spmv", -1, [&] { batch_mgcg_spmv(lists_, (it + 1) & 1, 0, stream_); });

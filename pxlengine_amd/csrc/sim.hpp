@@ -271,6 +271,7 @@ struct Sim {
             cur += (e.second + ALIGN - 1) / ALIGN * ALIGN;
         }
         d.norms = d.scal + SC_RR;
+        d.cg_x = d.cg_s1;
         if (!d.split_colours) {  // the list is already in cell order
             d.fn_list = d.fl_list;
             d.fn_word = d.fl_word;

@@ -116,7 +116,8 @@ __device__ __forceinline__ T wave_sum_partials(const T *part, uint32_t nb, int k
 }
 
 // ---- init 1: r0 = b - A x (pressure system, x = warm-started pressure) or, for the inner
-// solve of the refinement, r0 = float(r64) with x = e = 0.
+// solve of the refinement, r0 = float(r64) (its iterate starts at 0: the MG-CG's first spmv
+// clears it).
 __device__ __forceinline__ void body_cg_residual(const DevSim &S, const uint32_t *flc, float *__restrict__ x,
                                                  float *__restrict__ r, int inner, uint32_t b, uint32_t nb) {
     uint32_t beg, end;
@@ -128,7 +129,6 @@ __device__ __forceinline__ void body_cg_residual(const DevSim &S, const uint32_t
         if (inner) {  // the inner solve's vectors are colour-split (DevSim::fns_list)
             const uint32_t sid = S.fns_list[t];
             r[sid] = (float)S.r64[id];
-            x[sid] = 0.0f;
             continue;
         }
         const uint32_t wd = S.fn_word[t];
@@ -259,13 +259,15 @@ __global__ __launch_bounds__(SBLOCK) void k_cg_iter(DevSim S, uint32_t it, float
 }
 
 // ---- multigrid-preconditioned CG (Chronopoulos-Gear with u = M r from the V-cycle).
-// Vectors: r = cg_r0, u = cg_r1, w = cg_w0, s = cg_s0, p = cg_p; x = the inner unknown; all
-// colour-split.  spmv and update walk the colour-split list (red cells, then black, each in cell
-// order), whose split indices run through each half of the vectors in order, so every line they
-// touch is used whole even on short rows (an entry's index is fls_list[t], its neighbours'
-// split_offsets away).
+// r = cg_r0 and u = cg_r1, which the V-cycle reads with neighbours, are colour-split (an entry's
+// index is fls_list[t], its neighbours' split_offsets away).  w = cg_w0, s = cg_s0, p = cg_p and
+// the iterate x = cg_x are only ever read at the entry itself, so they are stored in list
+// order (index t): runs of a short row or a small grid then leave no gaps in their cache lines
+// (the colour-split layout touches whole lines even for a few FLUID cells of a row).  spmv and
+// update walk the colour-split list (red cells, then black, each in cell order); the last
+// update of a pass also stores x colour-split in e, for the FP64 refinement.
 // spmv: w = A u and the partials of gamma = (r, u), delta = (w, u) into slot `slot`
-// (first: also p_{-1} = s_{-1} = 0).
+// (first: also p_{-1} = s_{-1} = 0 and x = 0).
 template <typename Acc>
 __device__ __forceinline__ void body_mgcg_spmv(const DevSim &S, const uint32_t *flc, uint32_t slot, int first,
                                                uint32_t b, uint32_t nb) {
@@ -302,10 +304,12 @@ __device__ __forceinline__ void body_mgcg_spmv(const DevSim &S, const uint32_t *
             for (int n = 0; n < 6; n++)
                 if (w_fluid(wd[k], n)) sum = sum + un[k][n];
             const float wc = w_diag(wd[k]) * uc[k] - sum;
-            S.cg_w0[c[k]] = wc;
+            const uint32_t t = t0 + k * SBLOCK;
+            S.cg_w0[t] = wc;
             if (first) {
-                S.cg_p[c[k]] = 0.0f;
-                S.cg_s0[c[k]] = 0.0f;
+                S.cg_p[t] = 0.0f;
+                S.cg_s0[t] = 0.0f;
+                S.cg_x[t] = 0.0f;
             }
             g += (Acc)rc[k] * (Acc)uc[k];
             d += (Acc)wc * (Acc)uc[k];
@@ -315,10 +319,11 @@ __device__ __forceinline__ void body_mgcg_spmv(const DevSim &S, const uint32_t *
     store_partials<Acc, 2>(v, partials<Acc>(S) + (size_t)slot * 2 * nb, b, nb);
 }
 
-// update of iteration it: scalars from slot it & 1, then p, s, x, r (no neighbours).
+// update of iteration it: scalars from slot it & 1, then p, s, x, r (no neighbours); `last`:
+// x goes to e (colour-split) instead of cg_x.
 template <typename Acc>
-__device__ __forceinline__ void body_mgcg_update(const DevSim &S, const uint32_t *flc, uint32_t it,
-                                                 float *__restrict__ x, uint32_t b, uint32_t nb) {
+__device__ __forceinline__ void body_mgcg_update(const DevSim &S, const uint32_t *flc, uint32_t it, int last,
+                                                 float *__restrict__ e, uint32_t b, uint32_t nb) {
     uint32_t beg, end;
     list_slice(0, flc[1], b, nb, beg, end);
     float alpha, beta;
@@ -333,21 +338,25 @@ __device__ __forceinline__ void body_mgcg_update(const DevSim &S, const uint32_t
         float uu[UPD_MLP], pp[UPD_MLP], ww[UPD_MLP], ss[UPD_MLP], xx[UPD_MLP], rr[UPD_MLP];
 #pragma unroll
         for (int k = 0; k < UPD_MLP; k++) {
+            const uint32_t t = min(t0 + k * SBLOCK, end - 1u);
             uu[k] = S.cg_r1[c[k]];
-            pp[k] = S.cg_p[c[k]];
-            ww[k] = S.cg_w0[c[k]];
-            ss[k] = S.cg_s0[c[k]];
-            xx[k] = x[c[k]];
+            pp[k] = S.cg_p[t];
+            ww[k] = S.cg_w0[t];
+            ss[k] = S.cg_s0[t];
+            xx[k] = S.cg_x[t];
             rr[k] = S.cg_r0[c[k]];
         }
 #pragma unroll
         for (int k = 0; k < UPD_MLP; k++) {
-            if (t0 + k * SBLOCK >= end) break;
+            const uint32_t t = t0 + k * SBLOCK;
+            if (t >= end) break;
             const float pc = uu[k] + beta * pp[k];
             const float sc = ww[k] + beta * ss[k];
-            S.cg_p[c[k]] = pc;
-            S.cg_s0[c[k]] = sc;
-            x[c[k]] = xx[k] + alpha * pc;
+            S.cg_p[t] = pc;
+            S.cg_s0[t] = sc;
+            const float xn = xx[k] + alpha * pc;
+            if (last) e[c[k]] = xn;
+            else S.cg_x[t] = xn;
             S.cg_r0[c[k]] = rr[k] - alpha * sc;
         }
     }
@@ -357,8 +366,8 @@ __global__ __launch_bounds__(SBLOCK) void k_mgcg_spmv(DevSim S, uint32_t slot, i
     body_mgcg_spmv<Acc>(S, S.fl_count, slot, first, blockIdx.x, gridDim.x);
 }
 template <typename Acc>
-__global__ __launch_bounds__(SBLOCK) void k_mgcg_update(DevSim S, uint32_t it, float *__restrict__ x) {
-    body_mgcg_update<Acc>(S, S.fl_count, it, x, blockIdx.x, gridDim.x);
+__global__ __launch_bounds__(SBLOCK) void k_mgcg_update(DevSim S, uint32_t it, int last, float *__restrict__ e) {
+    body_mgcg_update<Acc>(S, S.fl_count, it, last, e, blockIdx.x, gridDim.x);
 }
 
 // ---- FP64 outer loop.  The solution is double-buffered (x_src -> x_dst) so the update pass
@@ -458,9 +467,9 @@ __global__ __launch_bounds__(SBLOCK) void kb_mgcg_spmv(WorkList W, uint32_t slot
         body_mgcg_spmv<double>(S, cnt, slot, first, b, S.cg_blocks);
     });
 }
-__global__ __launch_bounds__(SBLOCK) void kb_mgcg_update(WorkList W, uint32_t it) {
+__global__ __launch_bounds__(SBLOCK) void kb_mgcg_update(WorkList W, uint32_t it, int last) {
     for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *cnt) {
-        body_mgcg_update<double>(S, cnt, it, S.e, b, S.cg_blocks);
+        body_mgcg_update<double>(S, cnt, it, last, S.e, b, S.cg_blocks);
     });
 }
 __global__ __launch_bounds__(SBLOCK) void kb_mixed_init(WorkList W) {
@@ -533,7 +542,7 @@ void launch_mgcg_start(const DevSim &S, float *x, hipStream_t s) {
 
 void launch_mgcg_iter(const DevSim &S, float *x, uint32_t it, bool last, hipStream_t s) {
     const uint32_t g = solver_blocks(S);
-    PXL_LAUNCH((k_mgcg_update<double>), g, SBLOCK, 0, s, S, it, x);
+    PXL_LAUNCH((k_mgcg_update<double>), g, SBLOCK, 0, s, S, it, last ? 1 : 0, x);
     if (last) return;
     launch_mg_vcycle(S, S.cg_r0, S.cg_r1, s);
     PXL_LAUNCH((k_mgcg_spmv<double>), g, SBLOCK, 0, s, S, (it + 1) & 1, 0);
@@ -563,8 +572,8 @@ void batch_cg_residual(const WorkLists &W, hipStream_t s) { PXL_LAUNCH(kb_cg_res
 void batch_mgcg_spmv(const WorkLists &W, uint32_t slot, int first, hipStream_t s) {
     PXL_LAUNCH(kb_mgcg_spmv, batch_grid(W[WD_A_SOLVER], batch_cap(BF_SOLVER)), SBLOCK, 0, s, W[WD_A_SOLVER], slot, first);
 }
-void batch_mgcg_update(const WorkLists &W, uint32_t it, hipStream_t s) {
-    PXL_LAUNCH(kb_mgcg_update, batch_grid(W[WD_A_SOLVER], batch_cap(BF_SOLVER)), SBLOCK, 0, s, W[WD_A_SOLVER], it);
+void batch_mgcg_update(const WorkLists &W, uint32_t it, int last, hipStream_t s) {
+    PXL_LAUNCH(kb_mgcg_update, batch_grid(W[WD_A_SOLVER], batch_cap(BF_SOLVER)), SBLOCK, 0, s, W[WD_A_SOLVER], it, last);
 }
 void batch_mixed_update(const WorkLists &W, int cur, hipStream_t s) {
     PXL_LAUNCH(kb_mixed_update, batch_grid(W[WD_A_SOLVER], batch_cap(BF_SOLVER)), SBLOCK, 0, s, W[WD_A_SOLVER], cur);
