@@ -154,8 +154,9 @@ struct DevSim {
     // solver
     float *cg_r0, *cg_r1, *cg_w0, *cg_w1, *cg_s0, *cg_s1, *cg_p, *e;
     // MIXED: the MG-CG iterate in colour-split list order (index t of fl_list, like p, s and w
-    // there; storage of cg_s1, which MIXED does not use otherwise)
-    float *cg_x;
+    // there; storage of cg_s1, which MIXED does not use otherwise), and a list-order copy of
+    // r for the kernels that read r only at the entry itself (storage of cg_w1)
+    float *cg_x, *cg_rt;
     double *x64a, *x64b, *r64;
     MGLevel mg[MG_MAX];
     int mg_levels;  // levels including level 0 (0 when the multigrid is not allocated)

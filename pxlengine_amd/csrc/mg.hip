@@ -281,7 +281,7 @@ __device__ __forceinline__ void body_l0_pre0(const DevSim &S, const uint32_t *fl
             split_offsets(S, wd[k], off);
 #pragma unroll
             for (int n = 0; n < 6; n++) rn[k][n] = (w_black(wd[k]) && ((wd[k] >> n) & 1u)) ? r[id[k] + off[n]] : 0.0f;
-            rc[k] = r[id[k]];
+            rc[k] = S.cg_rt[t0 + k * LBLOCK < end ? t0 + k * LBLOCK : t0];
         }
 #pragma unroll
         for (int k = 0; k < PRE0_MLP; k++) {
@@ -323,7 +323,7 @@ __device__ __forceinline__ void body_l0_sweep(const DevSim &S, const uint32_t *f
             const int32_t off[6] = {d - 1 + o, d + o, d - hy, d + hy, d - hz, d + hz};
 #pragma unroll
             for (int q = 0; q < 6; q++) z[k][q] = ((wd[k] >> q) & 1u) ? u[id[k] + off[q]] : 0.0f;
-            rc[k] = r[id[k]];
+            rc[k] = S.cg_rt[t0 + k * LBLOCK < end ? t0 + k * LBLOCK : t0];
         }
 #pragma unroll
         for (int k = 0; k < SWEEP_MLP; k++) {
@@ -368,7 +368,7 @@ __device__ __forceinline__ void body_l0_post(const DevSim &S, const uint32_t *fl
                 zq[k][q] = f ? u[sid[k] + off[q]] : 0.0f;
                 pq[k][q] = f ? c.z[parent_sig(c, nbc[q][0], nbc[q][1], nbc[q][2])] : 0.0f;
             }
-            rc[k] = r[sid[k]];
+            rc[k] = S.cg_rt[t0 + k * LBLOCK < end ? t0 + k * LBLOCK : t0];
         }
 #pragma unroll
         for (int k = 0; k < POST_MLP; k++) {

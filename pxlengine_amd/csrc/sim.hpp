@@ -272,6 +272,7 @@ struct Sim {
         }
         d.norms = d.scal + SC_RR;
         d.cg_x = d.cg_s1;
+        d.cg_rt = d.cg_w1;
         if (!d.split_colours) {  // the list is already in cell order
             d.fn_list = d.fl_list;
             d.fn_word = d.fl_word;

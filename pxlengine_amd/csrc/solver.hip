@@ -125,12 +125,13 @@ __device__ __forceinline__ void body_cg_residual(const DevSim &S, const uint32_t
     int32_t off[6];
     nb_offsets(S, off);
     for (uint32_t t = beg + threadIdx.x; t < end; t += SBLOCK) {
-        const uint32_t id = S.fn_list[t];
-        if (inner) {  // the inner solve's vectors are colour-split (DevSim::fns_list)
-            const uint32_t sid = S.fns_list[t];
-            r[sid] = (float)S.r64[id];
+        if (inner) {  // the inner solve's r: colour-split (DevSim::fls_list) and in list order
+            const float rv = (float)S.r64[S.fl_list[t]];
+            r[S.fls_list[t]] = rv;
+            S.cg_rt[t] = rv;
             continue;
         }
+        const uint32_t id = S.fn_list[t];
         const uint32_t wd = S.fn_word[t];
         float sum = 0.0f;
 #pragma unroll
@@ -260,7 +261,8 @@ __global__ __launch_bounds__(SBLOCK) void k_cg_iter(DevSim S, uint32_t it, float
 
 // ---- multigrid-preconditioned CG (Chronopoulos-Gear with u = M r from the V-cycle).
 // r = cg_r0 and u = cg_r1, which the V-cycle reads with neighbours, are colour-split (an entry's
-// index is fls_list[t], its neighbours' split_offsets away).  w = cg_w0, s = cg_s0, p = cg_p and
+// index is fls_list[t], its neighbours' split_offsets away); r is kept in list order as well
+// (cg_rt) for the kernels that read it only at the entry itself.  w = cg_w0, s = cg_s0, p = cg_p and
 // the iterate x = cg_x are only ever read at the entry itself, so they are stored in list
 // order (index t): runs of a short row or a small grid then leave no gaps in their cache lines
 // (the colour-split layout touches whole lines even for a few FLUID cells of a row).  spmv and
@@ -294,7 +296,7 @@ __device__ __forceinline__ void body_mgcg_spmv(const DevSim &S, const uint32_t *
 #pragma unroll
             for (int n = 0; n < 6; n++) un[k][n] = w_fluid(wd[k], n) ? u[c[k] + off[n]] : 0.0f;
             uc[k] = u[c[k]];
-            rc[k] = S.cg_r0[c[k]];
+            rc[k] = S.cg_rt[t0 + k * SBLOCK < end ? t0 + k * SBLOCK : t0];
         }
 #pragma unroll
         for (int k = 0; k < SPMV_MLP; k++) {
@@ -344,7 +346,7 @@ __device__ __forceinline__ void body_mgcg_update(const DevSim &S, const uint32_t
             ww[k] = S.cg_w0[t];
             ss[k] = S.cg_s0[t];
             xx[k] = S.cg_x[t];
-            rr[k] = S.cg_r0[c[k]];
+            rr[k] = S.cg_rt[t];
         }
 #pragma unroll
         for (int k = 0; k < UPD_MLP; k++) {
@@ -357,7 +359,9 @@ __device__ __forceinline__ void body_mgcg_update(const DevSim &S, const uint32_t
             const float xn = xx[k] + alpha * pc;
             if (last) e[c[k]] = xn;
             else S.cg_x[t] = xn;
-            S.cg_r0[c[k]] = rr[k] - alpha * sc;
+            const float rn = rr[k] - alpha * sc;
+            S.cg_r0[c[k]] = rn;
+            S.cg_rt[t] = rn;
         }
     }
 }
