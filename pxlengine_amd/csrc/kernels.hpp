@@ -421,6 +421,25 @@ inline unsigned batch_grid(const WorkList &w, unsigned max_grid) {
     return g < WORK_LIST_MIN_CAP ? g : WORK_LIST_MIN_CAP;
 }
 
+// p[i] read as global memory, for a pointer the compiler cannot tell is global (one kept in
+// LDS, say), which it would otherwise access with flat instructions.
+template <typename T>
+__device__ __forceinline__ T ld_global(const T *p, size_t i) {
+    return ((const __attribute__((address_space(1))) T *)p)[i];
+}
+
+// A batch member's descriptor, read through the constant address space.  The descriptor
+// array is written only by host copies before a launch, so it is constant for a kernel's
+// lifetime; the compiler then fetches its fields with scalar loads and takes every pointer
+// loaded from it to be a global-memory pointer (AMDGPU's rule for generic pointers read from
+// constant memory).  Through a generic reference the batched bodies would address their arrays
+// with flat instructions: 64-bit per-lane addresses, and waits that also cover the scalar
+// loads, which kept a lane's neighbour loads from being in flight together.
+__device__ __forceinline__ const DevSim &batch_member(const DevSim *sims, uint32_t sim) {
+    const __attribute__((address_space(4))) DevSim *cs = (const __attribute__((address_space(4))) DevSim *)sims;
+    return *(const DevSim *)(cs + sim);
+}
+
 // Runs f(S, block, slots) for each work entry of this workgroup.  The first entry is loaded
 // together with the entry count (it lies in the list's storage even when the list is
 // shorter), so a workgroup's first item waits for one load instead of two.
@@ -431,7 +450,7 @@ __device__ __forceinline__ void for_each_work(const WorkList &W, F &&f) {
     uint32_t w = W.entries[e];
     while (e < n) {
         const uint32_t sim = w >> WORK_SIM_SHIFT;
-        f(W.sims[sim], w & WORK_BLOCK_MASK, W.slots + (size_t)sim * COUNT_SLOTS);
+        f(batch_member(W.sims, sim), w & WORK_BLOCK_MASK, W.slots + (size_t)sim * COUNT_SLOTS);
         e += gridDim.x;
         if (e < n) w = W.entries[e];
     }
@@ -457,7 +476,7 @@ __device__ __forceinline__ void for_batch_band(const WorkList &W, F &&f) {
     const uint32_t n = *W.band_count, g = gridDim.x;
     for (uint32_t e = xcd_major(blockIdx.x, g); e < n; e += g) {
         const uint32_t w = W.band[e];
-        f(W.sims[w >> WORK_SIM_SHIFT], w & WORK_BLOCK_MASK);
+        f(batch_member(W.sims, w >> WORK_SIM_SHIFT), w & WORK_BLOCK_MASK);
     }
 }
 

@@ -266,21 +266,32 @@ __device__ __forceinline__ void body_l0_pre0(const DevSim &S, const uint32_t *fl
                                              float *__restrict__ u, uint32_t b, uint32_t nb) {
     uint32_t beg, end;
     slice(flc[1], b, nb, beg, end);
+    const uint32_t last = 2u * S.hsplit - 1u;
     for (uint32_t t0 = beg + threadIdx.x; t0 < end; t0 += PRE0_MLP * LBLOCK) {
         uint32_t id[PRE0_MLP], wd[PRE0_MLP];
 #pragma unroll
         for (int k = 0; k < PRE0_MLP; k++) {
             const uint32_t t = t0 + k * LBLOCK;
             id[k] = S.fls_list[t < end ? t : t0];  // colour-split order: runs through each half
-            wd[k] = t < end ? S.fl_word[t] : 0u;    // past the slice: red, no neighbours, no store
+            wd[k] = S.fl_word[t < end ? t : t0];  // loaded unconditionally, then dropped
+            wd[k] = t < end ? wd[k] : 0u;    // past the slice: red, no neighbours, no store
         }
         float rn[PRE0_MLP][6], rc[PRE0_MLP];
 #pragma unroll
         for (int k = 0; k < PRE0_MLP; k++) {
             int32_t off[6];
             split_offsets(S, wd[k], off);
+            // red entries read no neighbours (one branch per entry, not per load: the list is
+            // red then black, so a wave mostly takes one side); a black entry loads all six, a
+            // neighbour that is not FLUID at the +x slot (a line its x neighbours share), and
+            // drops them afterwards
+            const uint32_t xs = min(id[k] + off[1], last);
 #pragma unroll
-            for (int n = 0; n < 6; n++) rn[k][n] = (w_black(wd[k]) && ((wd[k] >> n) & 1u)) ? r[id[k] + off[n]] : 0.0f;
+            for (int n = 0; n < 6; n++) rn[k][n] = 0.0f;
+            if (w_black(wd[k])) {
+#pragma unroll
+                for (int n = 0; n < 6; n++) rn[k][n] = r[((wd[k] >> n) & 1u) ? id[k] + off[n] : xs];
+            }
             rc[k] = S.cg_rt[t0 + k * LBLOCK < end ? t0 + k * LBLOCK : t0];
         }
 #pragma unroll
@@ -314,13 +325,16 @@ __device__ __forceinline__ void body_l0_sweep(const DevSim &S, const uint32_t *f
         for (int k = 0; k < SWEEP_MLP; k++) {
             const uint32_t t = t0 + k * LBLOCK;
             id[k] = S.fls_list[t < end ? t : t0];
-            wd[k] = t < end ? S.fl_word[t] : 0u;
+            wd[k] = S.fl_word[t < end ? t : t0];  // loaded unconditionally, then dropped
+            wd[k] = t < end ? wd[k] : 0u;
         }
         float z[SWEEP_MLP][6], rc[SWEEP_MLP];
 #pragma unroll
         for (int k = 0; k < SWEEP_MLP; k++) {
             const int32_t o = (int32_t)((wd[k] >> 28) & 1u);
             const int32_t off[6] = {d - 1 + o, d + o, d - hy, d + hy, d - hz, d + hz};
+            // only FLUID neighbours are loaded: loading the others too (at the +x slot, a line
+            // already fetched) made the batched sweep 10-15% slower on the 512-request batch
 #pragma unroll
             for (int q = 0; q < 6; q++) z[k][q] = ((wd[k] >> q) & 1u) ? u[id[k] + off[q]] : 0.0f;
             rc[k] = S.cg_rt[t0 + k * LBLOCK < end ? t0 + k * LBLOCK : t0];
@@ -350,7 +364,8 @@ __device__ __forceinline__ void body_l0_post(const DevSim &S, const uint32_t *fl
 #pragma unroll
         for (int k = 0; k < POST_MLP; k++) {
             const uint32_t t = t0 + k * LBLOCK, tt = t < end ? t : t0;
-            wd[k] = t < end ? S.fl_word[t] : 0u;
+            wd[k] = S.fl_word[t < end ? t : t0];  // loaded unconditionally, then dropped
+            wd[k] = t < end ? wd[k] : 0u;
             id[k] = S.fl_list[tt];
             sid[k] = S.fls_list[tt];
         }
@@ -425,13 +440,19 @@ __device__ __forceinline__ void body_lc_sweep(const MGLevel &v, uint32_t colour,
     for (uint32_t t = beg + threadIdx.x; t < end; t += 2 * LBLOCK) {
         const uint32_t t2 = t + LBLOCK;
         const bool two = t2 < end;
-        const uint32_t ea = colour ? list[last - t] : list[t];
-        const uint32_t eb = two ? (colour ? list[last - t2] : list[t2]) : ea;
+        // both entries are loaded at once (the second at the first's index past the slice)
+        const uint32_t tb = two ? t2 : t;
+        const uint32_t ea = list[colour ? last - t : t];
+        const uint32_t eb = list[colour ? last - tb : tb];
         const uint32_t ia = e_sig(ea), ib = e_sig(eb);
         const int32_t oa = e_odd(ea), ob = e_odd(eb);
         const int32_t offa[6] = {d - 1 + oa, d + oa, d - hy, d + hy, d - hz, d + hz};
         const int32_t offb[6] = {d - 1 + ob, d + ob, d - hy, d + hy, d - hz, d + hz};
-        const uint32_t sta = v.sten[ia], stb = two ? (uint32_t)v.sten[ib] : 0u;
+        // the second stencil word is loaded unconditionally (at the first entry's cell past the
+        // slice), so both are in flight together; only FLUID neighbours are loaded (loading
+        // every neighbour with the stencil words was slower on large batches)
+        const uint32_t sta = v.sten[ia], stb_raw = v.sten[ib];
+        const uint32_t stb = two ? stb_raw : 0u;
         float sa = 0.0f, sb = 0.0f;
 #pragma unroll
         for (int q = 0; q < 6; q++) {
@@ -685,8 +706,12 @@ __device__ __forceinline__ void body_mg_tail(const DevSim &S) {
             cell_ijk(v.nx, v.ny, (uint32_t)(g - v.o), i, j, k);
             const uint32_t sg = (((uint32_t)(i + j + k) & 1u) ? v.hsplit : 0u) + (uint32_t)(i >> 1) +
                                 (uint32_t)v.hx * ((uint32_t)j + (uint32_t)v.ny * (uint32_t)k);
-            st[u] = v.sten[sg];
-            if (q == T) rv[u] = v.r[sg];
+            // the level table sits in LDS, so its pointers are read as global memory explicitly
+            // (a generic pointer would make these flat loads); every level has its r, so it is
+            // loaded unconditionally and kept only for the first tail level
+            st[u] = ld_global(v.sten, sg);
+            const float r = ld_global(v.r, sg);
+            rv[u] = q == T ? r : 0.0f;
         }
     }
 #pragma unroll

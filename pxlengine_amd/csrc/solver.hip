@@ -286,7 +286,8 @@ __device__ __forceinline__ void body_mgcg_spmv(const DevSim &S, const uint32_t *
         for (int k = 0; k < SPMV_MLP; k++) {
             const uint32_t t = t0 + k * SBLOCK;
             c[k] = S.fls_list[t < end ? t : t0];
-            wd[k] = t < end ? S.fl_word[t] : 0u;  // past the slice: no neighbours, no store
+            wd[k] = S.fl_word[t < end ? t : t0];  // loaded unconditionally, then dropped
+            wd[k] = t < end ? wd[k] : 0u;  // past the slice: no neighbours, no store
         }
         float un[SPMV_MLP][6], uc[SPMV_MLP], rc[SPMV_MLP];
 #pragma unroll
@@ -294,7 +295,11 @@ __device__ __forceinline__ void body_mgcg_spmv(const DevSim &S, const uint32_t *
             int32_t off[6];
             split_offsets(S, wd[k], off);
 #pragma unroll
-            for (int n = 0; n < 6; n++) un[k][n] = w_fluid(wd[k], n) ? u[c[k] + off[n]] : 0.0f;
+            for (int n = 0; n < 6; n++) {
+                const bool f = w_fluid(wd[k], n);
+                const float v = u[f ? c[k] + off[n] : c[k]];
+                un[k][n] = f ? v : 0.0f;
+            }
             uc[k] = u[c[k]];
             rc[k] = S.cg_rt[t0 + k * SBLOCK < end ? t0 + k * SBLOCK : t0];
         }
