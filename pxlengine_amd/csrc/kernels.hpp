@@ -421,6 +421,24 @@ inline unsigned batch_grid(const WorkList &w, unsigned max_grid) {
     return g < WORK_LIST_MIN_CAP ? g : WORK_LIST_MIN_CAP;
 }
 
+// A device array as a raw buffer of `bytes` bytes (a uniform value: the base and size come from
+// a member's descriptor).  A raw buffer load at a byte offset past the size returns 0 without
+// touching memory, so a stencil kernel gives a neighbour that is not FLUID the offset
+// buf_none(...) instead of branching around its load: every load of a lane is then issued
+// together, and the masked ones cost no memory traffic.  0x00020000 is the gfx9-family
+// descriptor word 3 (32-bit data format, no swizzle, no stride).
+struct Buf {
+    __amdgpu_buffer_rsrc_t r;
+    uint32_t bytes;
+};
+__device__ __forceinline__ Buf buf_of(const void *p, uint32_t bytes) {
+    return Buf{__builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000), bytes};
+}
+// Element i of a float buffer, or 0 when `use` is false (then no memory access).
+__device__ __forceinline__ float buf_f32(const Buf &b, uint32_t i, bool use) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(b.r, (int)(use ? 4u * i : b.bytes), 0, 0));
+}
+
 // p[i] read as global memory, for a pointer the compiler cannot tell is global (one kept in
 // LDS, say), which it would otherwise access with flat instructions.
 template <typename T>

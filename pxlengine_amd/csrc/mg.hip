@@ -266,7 +266,7 @@ __device__ __forceinline__ void body_l0_pre0(const DevSim &S, const uint32_t *fl
                                              float *__restrict__ u, uint32_t b, uint32_t nb) {
     uint32_t beg, end;
     slice(flc[1], b, nb, beg, end);
-    const uint32_t last = 2u * S.hsplit - 1u;
+    const Buf br = buf_of(r, 8u * S.hsplit);
     for (uint32_t t0 = beg + threadIdx.x; t0 < end; t0 += PRE0_MLP * LBLOCK) {
         uint32_t id[PRE0_MLP], wd[PRE0_MLP];
 #pragma unroll
@@ -285,13 +285,8 @@ __device__ __forceinline__ void body_l0_pre0(const DevSim &S, const uint32_t *fl
             // red then black, so a wave mostly takes one side); a black entry loads all six, a
             // neighbour that is not FLUID at the +x slot (a line its x neighbours share), and
             // drops them afterwards
-            const uint32_t xs = min(id[k] + off[1], last);
 #pragma unroll
-            for (int n = 0; n < 6; n++) rn[k][n] = 0.0f;
-            if (w_black(wd[k])) {
-#pragma unroll
-                for (int n = 0; n < 6; n++) rn[k][n] = r[((wd[k] >> n) & 1u) ? id[k] + off[n] : xs];
-            }
+            for (int n = 0; n < 6; n++) rn[k][n] = buf_f32(br, id[k] + off[n], w_black(wd[k]) && ((wd[k] >> n) & 1u));
             rc[k] = S.cg_rt[t0 + k * LBLOCK < end ? t0 + k * LBLOCK : t0];
         }
 #pragma unroll
@@ -319,6 +314,7 @@ __device__ __forceinline__ void body_l0_sweep(const DevSim &S, const uint32_t *f
     end += lo;
     const int32_t d = colour ? -(int32_t)S.hsplit : (int32_t)S.hsplit;
     const int32_t hy = (int32_t)S.hx, hz = (int32_t)S.hx * S.ny;
+    const Buf bu = buf_of(u, 8u * S.hsplit);
     for (uint32_t t0 = beg + threadIdx.x; t0 < end; t0 += SWEEP_MLP * LBLOCK) {
         uint32_t id[SWEEP_MLP], wd[SWEEP_MLP];
 #pragma unroll
@@ -333,10 +329,10 @@ __device__ __forceinline__ void body_l0_sweep(const DevSim &S, const uint32_t *f
         for (int k = 0; k < SWEEP_MLP; k++) {
             const int32_t o = (int32_t)((wd[k] >> 28) & 1u);
             const int32_t off[6] = {d - 1 + o, d + o, d - hy, d + hy, d - hz, d + hz};
-            // only FLUID neighbours are loaded: loading the others too (at the +x slot, a line
-            // already fetched) made the batched sweep 10-15% slower on the 512-request batch
+            // only FLUID neighbours are read (loading the others too, at the +x slot, made the
+            // batched sweep 10-15% slower on the 512-request batch)
 #pragma unroll
-            for (int q = 0; q < 6; q++) z[k][q] = ((wd[k] >> q) & 1u) ? u[id[k] + off[q]] : 0.0f;
+            for (int q = 0; q < 6; q++) z[k][q] = buf_f32(bu, id[k] + off[q], (wd[k] >> q) & 1u);
             rc[k] = S.cg_rt[t0 + k * LBLOCK < end ? t0 + k * LBLOCK : t0];
         }
 #pragma unroll
@@ -359,6 +355,7 @@ __device__ __forceinline__ void body_l0_post(const DevSim &S, const uint32_t *fl
     slice(n - n_red, b, nb, beg, end);
     beg += n_red;
     end += n_red;
+    const Buf bu = buf_of(u, 8u * S.hsplit), bc = buf_of(c.z, 8u * c.hsplit);
     for (uint32_t t0 = beg + threadIdx.x; t0 < end; t0 += POST_MLP * LBLOCK) {
         uint32_t wd[POST_MLP], id[POST_MLP], sid[POST_MLP];
 #pragma unroll
@@ -380,8 +377,8 @@ __device__ __forceinline__ void body_l0_post(const DevSim &S, const uint32_t *fl
 #pragma unroll
             for (int q = 0; q < 6; q++) {
                 const bool f = (wd[k] >> q) & 1u;
-                zq[k][q] = f ? u[sid[k] + off[q]] : 0.0f;
-                pq[k][q] = f ? c.z[parent_sig(c, nbc[q][0], nbc[q][1], nbc[q][2])] : 0.0f;
+                zq[k][q] = buf_f32(bu, sid[k] + off[q], f);
+                pq[k][q] = buf_f32(bc, parent_sig(c, nbc[q][0], nbc[q][1], nbc[q][2]), f);
             }
             rc[k] = S.cg_rt[t0 + k * LBLOCK < end ? t0 + k * LBLOCK : t0];
         }
@@ -437,6 +434,7 @@ __device__ __forceinline__ void body_lc_sweep(const MGLevel &v, uint32_t colour,
     uint32_t beg, end;
     slice(n, b, nb, beg, end);
     const int32_t d = colour ? -(int32_t)v.hsplit : (int32_t)v.hsplit, hy = v.hx, hz = v.hx * v.ny;
+    const Buf bz = buf_of(v.z, 8u * v.hsplit);
     for (uint32_t t = beg + threadIdx.x; t < end; t += 2 * LBLOCK) {
         const uint32_t t2 = t + LBLOCK;
         const bool two = t2 < end;
@@ -453,13 +451,17 @@ __device__ __forceinline__ void body_lc_sweep(const MGLevel &v, uint32_t colour,
         // every neighbour with the stencil words was slower on large batches)
         const uint32_t sta = v.sten[ia], stb_raw = v.sten[ib];
         const uint32_t stb = two ? stb_raw : 0u;
+        float za[6], zb[6];
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            za[q] = buf_f32(bz, ia + offa[q], (sta >> q) & 1u);
+            zb[q] = buf_f32(bz, ib + offb[q], (stb >> q) & 1u);
+        }
         float sa = 0.0f, sb = 0.0f;
 #pragma unroll
         for (int q = 0; q < 6; q++) {
-            const float za = (sta & (1u << q)) ? v.z[ia + offa[q]] : 0.0f;
-            const float zb = (stb & (1u << q)) ? v.z[ib + offb[q]] : 0.0f;
-            sa += za;
-            sb += zb;
+            sa += za[q];
+            sb += zb[q];
         }
         const float ra = v.r[ia], rb = v.r[ib];
         v.z[ia] = (ra + sa) * rcp_or0(st_diag(sta));
