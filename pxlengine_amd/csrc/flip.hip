@@ -193,6 +193,12 @@ int prepare(const char *fluid_json, double fps, uint32_t seed, const pxlflip_opt
             l++;
         }
         d.mg_levels = l + 1;
+        // the multigrid kernels address a colour-split vector (2 hsplit floats) as a raw buffer
+        // with 32-bit byte offsets below 2^31
+        if (8ull * (uint64_t)d.hsplit >= 0x80000000ull) {
+            g_create_error = "fluid '" + spec.name + "': grid too large for the mixed-precision solver";
+            return PXLFLIP_ERR_INVALID;
+        }
         // the tail: the longest run of coarsest levels (from level 1 on) that fits the LDS
         size_t tail = 0;
         d.mg_tail = l + 1;
