@@ -119,11 +119,12 @@ __device__ __forceinline__ bool rim_node_touched(uint32_t core, int u0, int u1, 
 // Packed P2G sum of face (c, q) for a face pass walking tile t = (t0, t1, t2) (q in t's node
 // box): the boxes that contain the face are t's and, per axis, the tile below when q is on t's
 // lower face plane, or the tile above when q is on its last cell row along an axis other than
-// c (the same tiles gather_face visits).  The active ones come from the mask, so the (up to
-// 8) box loads do not wait on tile flags and are all in flight together.  Integer sums: the
-// order does not matter.
-__device__ __forceinline__ int64_t gather_face_in(const DevSim &S, uint32_t act, int c, const int q[3],
-                                                  const int t[3]) {
+// c (the same tiles gather_face visits).  The active ones come from the mask, and all 8 box
+// words are read with raw buffer loads (`boxes`: the simulation's tile boxes), a box that is
+// not among them at an offset past the buffer (0, no memory access): no branch, so the loads
+// are in flight together.  Integer sums: the order does not matter.
+__device__ __forceinline__ int64_t gather_face_in(const DevSim &S, const Buf &boxes, uint32_t act, int c,
+                                                  const int q[3], const int t[3]) {
     int alt[3], u[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
@@ -131,18 +132,21 @@ __device__ __forceinline__ int64_t gather_face_in(const DevSim &S, uint32_t act,
         alt[a] = u[a] == 0 ? -1 : ((a != c && u[a] == TILE - 1) ? 1 : 0);
     }
     const uint32_t t_lin = (uint32_t)t[0] + (uint32_t)S.ntx * ((uint32_t)t[1] + (uint32_t)S.nty * (uint32_t)t[2]);
-    int64_t sum = 0;
+    int64_t v[8];
 #pragma unroll
     for (int e = 0; e < 8; e++) {
         const int dx = (e & 1) ? alt[0] : 0, dy = (e & 2) ? alt[1] : 0, dz = (e & 4) ? alt[2] : 0;
         const bool valid = (!(e & 1) || alt[0]) && (!(e & 2) || alt[1]) && (!(e & 4) || alt[2]);
-        if (!valid || !((act >> ((dx + 1) + 3 * (dy + 1) + 9 * (dz + 1))) & 1u)) continue;
+        const bool use = valid && ((act >> ((dx + 1) + 3 * (dy + 1) + 9 * (dz + 1))) & 1u);
         // the face's position in the box of tile t + d (a box starts one cell below its tile
         // along the axes other than c)
         const int lx = u[0] - TILE * dx + (c != 0), ly = u[1] - TILE * dy + (c != 1), lz = u[2] - TILE * dz + (c != 2);
         const uint32_t tn = t_lin + (uint32_t)(dx + S.ntx * (dy + S.nty * dz));
-        sum += S.box[(size_t)tn * BOX_ENTRIES + (size_t)box_gidx(c, lx, ly, lz)];
+        v[e] = buf_i64(boxes, use ? 8u * (tn * (uint32_t)BOX_ENTRIES + (uint32_t)box_gidx(c, lx, ly, lz)) : boxes.bytes);
     }
+    int64_t sum = 0;
+#pragma unroll
+    for (int e = 0; e < 8; e++) sum += v[e];
     return sum;
 }
 
@@ -157,13 +161,14 @@ __device__ __forceinline__ void face_update(const DevSim &S, int64_t packed, uin
     S.vel[f] = keep ? v + g * S.dt : 0.0f;
 }
 
-__device__ __forceinline__ void faces_at(const DevSim &S, uint32_t act, const int t[3], int i, int j, int k) {
+__device__ __forceinline__ void faces_at(const DevSim &S, const Buf &boxes, uint32_t act, const int t[3], int i, int j,
+                                         int k) {
     const bool in_c = !is_solid(S, i, j, k);
     const int q[3] = {i, j, k};
     const bool has[3] = {j < S.ny && k < S.nz, i < S.nx && k < S.nz, i < S.nx && j < S.ny};
     int64_t sum[3];
 #pragma unroll
-    for (int c = 0; c < 3; c++) sum[c] = has[c] ? gather_face_in(S, act, c, q, t) : 0;
+    for (int c = 0; c < 3; c++) sum[c] = gather_face_in(S, boxes, has[c] ? act : 0u, c, q, t);
     if (has[0]) face_update(S, sum[0], uidx(S, i, j, k), S.gx, in_c && !is_solid(S, i - 1, j, k));
     if (has[1]) face_update(S, sum[1], vidx(S, i, j, k), S.gy, in_c && !is_solid(S, i, j - 1, k));
     if (has[2]) face_update(S, sum[2], widx(S, i, j, k), S.gz, in_c && !is_solid(S, i, j, k - 1));
@@ -174,11 +179,13 @@ __device__ __forceinline__ void faces_tile(const DevSim &S, uint32_t tl) {
                       (int)(tl / ((uint32_t)S.ntx * (uint32_t)S.nty))};
     uint32_t act, core;
     neighbour_masks(S, t[0], t[1], t[2], act, core);
+    const Buf boxes = buf_of(S.box, 8u * (uint32_t)BOX_ENTRIES * S.ntiles);
     if (core & NB_SELF) {
-        for_tile_box<NT>(S, tl, true, [&](int i, int j, int k) { faces_at(S, act, t, i, j, k); });
+        for_tile_box<NT>(S, tl, true, [&](int i, int j, int k) { faces_at(S, boxes, act, t, i, j, k); });
     } else {  // rim tile: only the layers a core neighbour touches
         for_tile_box<NT>(S, tl, true, [&](int i, int j, int k) {
-            if (rim_node_touched(core, i - TILE * t[0], j - TILE * t[1], k - TILE * t[2])) faces_at(S, act, t, i, j, k);
+            if (rim_node_touched(core, i - TILE * t[0], j - TILE * t[1], k - TILE * t[2]))
+                faces_at(S, boxes, act, t, i, j, k);
         });
     }
 }
@@ -203,25 +210,47 @@ __global__ __launch_bounds__(TX *TY) void k_acc_fields(DevSim S, int32_t *acc, i
     if (i < S.nx && j < S.ny) { v = gather_face(S, 2, i, j, k); acc[widx(S, i, j, k)] = packed_acc(v); wacc[widx(S, i, j, k)] = packed_wacc(v); }
 }
 
-__device__ __forceinline__ uint8_t type_from_count(const DevSim &S, int i, int j, int k) {
-    if (is_solid(S, i, j, k)) return CELL_SOLID;
-    return S.count[tile_key(S, i, j, k)] > 0 ? CELL_FLUID : CELL_AIR;
+// The types of cell (i, j, k) and its six neighbours (-x, +x, -y, +y, -z, +z): SOLID on or past
+// the wall layer (is_solid) and in obstacle cells, else FLUID where the cell holds particles,
+// else AIR.  Every load is issued together: a cell on or past the wall layer reads tile key 0
+// and solid word 0, and its values are dropped.
+__device__ __forceinline__ void types7(const DevSim &S, int i, int j, int k, uint8_t t[7]) {
+    const int nb[7][3] = {{i, j, k}, {i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k}, {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
+    bool wall[7];
+    uint32_t cnt[7], sb[7];
+#pragma unroll
+    for (int n = 0; n < 7; n++) {
+        const int a = nb[n][0], b = nb[n][1], c = nb[n][2];
+        wall[n] = a <= 0 || b <= 0 || c <= 0 || a >= S.nx - 1 || b >= S.ny - 1 || c >= S.nz - 1;
+        cnt[n] = S.count[wall[n] ? 0u : tile_key(S, a, b, c)];
+        sb[n] = 0;
+    }
+    if (S.solid_bits) {  // obstacles (uniform)
+#pragma unroll
+        for (int n = 0; n < 7; n++) {
+            const uint32_t id = wall[n] ? 0u : cidx(S, nb[n][0], nb[n][1], nb[n][2]);
+            sb[n] = (S.solid_bits[id >> 5] >> (id & 31u)) & 1u;
+        }
+    }
+#pragma unroll
+    for (int n = 0; n < 7; n++) t[n] = (wall[n] || sb[n]) ? CELL_SOLID : (cnt[n] > 0 ? CELL_FLUID : CELL_AIR);
 }
 
 // normalize_grid cells + compute_divergence (fluid.slang:125-131, 154-165), plus the stencil
 // word the pressure kernels read instead of re-classifying six neighbours.  Returns the type.
 __device__ __forceinline__ uint8_t cells_at(const DevSim &S, int i, int j, int k) {
     const uint32_t id = cidx(S, i, j, k);
-    const uint8_t t = type_from_count(S, i, j, k);
+    uint8_t tn[7];
+    types7(S, i, j, k, tn);
+    const uint8_t t = tn[0];
     S.type[id] = t;
     uint16_t st = 0;
     float d = 0.0f;
     if (t == CELL_FLUID) {
-        const int nb[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k}, {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
         uint32_t diag = 0;
 #pragma unroll
         for (int n = 0; n < 6; n++) {
-            const uint8_t tq = type_from_count(S, nb[n][0], nb[n][1], nb[n][2]);
+            const uint8_t tq = tn[n + 1];
             if (tq != CELL_SOLID) diag++;
             if (tq == CELL_FLUID) st |= (uint16_t)(1u << n);
         }
@@ -411,10 +440,46 @@ __device__ __forceinline__ void project_face(const DevSim &S, uint32_t f, int ai
     }
 }
 
+// project_face for the node's three faces (cell b = (i, j, k) on the high side of each), with
+// every type, pressure and velocity the three might need loaded first, at in-grid indices
+// (index 0 for a cell outside the grid, whose type then counts as SOLID), so a lane's loads
+// are in flight together instead of one face's pressures waiting on its types.
 __device__ __forceinline__ void project_at(const DevSim &S, int i, int j, int k) {
-    if (j < S.ny && k < S.nz) project_face(S, uidx(S, i, j, k), i - 1, j, k, i, j, k);
-    if (i < S.nx && k < S.nz) project_face(S, vidx(S, i, j, k), i, j - 1, k, i, j, k);
-    if (i < S.nx && j < S.ny) project_face(S, widx(S, i, j, k), i, j, k - 1, i, j, k);
+    const bool has[3] = {j < S.ny && k < S.nz, i < S.nx && k < S.nz, i < S.nx && j < S.ny};
+    const uint32_t f[3] = {has[0] ? uidx(S, i, j, k) : 0u, has[1] ? vidx(S, i, j, k) : 0u, has[2] ? widx(S, i, j, k) : 0u};
+    const int ca[3][3] = {{i - 1, j, k}, {i, j - 1, k}, {i, j, k - 1}};
+    const bool in_b = i < S.nx && j < S.ny && k < S.nz;
+    const uint32_t ib = in_b ? cidx(S, i, j, k) : 0u;
+    bool in_a[3];
+    uint32_t ia[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        in_a[c] = ca[c][0] >= 0 && ca[c][1] >= 0 && ca[c][2] >= 0 && ca[c][0] < S.nx && ca[c][1] < S.ny && ca[c][2] < S.nz;
+        ia[c] = in_a[c] ? cidx(S, ca[c][0], ca[c][1], ca[c][2]) : 0u;
+    }
+    const uint8_t tb_raw = S.type[ib];
+    const float pb_raw = S.pressure[ib];
+    uint8_t ta_raw[3];
+    float pa_raw[3], v[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        ta_raw[c] = S.type[ia[c]];
+        pa_raw[c] = S.pressure[ia[c]];
+        v[c] = S.vel[f[c]];
+    }
+    const uint8_t tb = in_b ? tb_raw : (uint8_t)CELL_SOLID;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        if (!has[c]) continue;
+        const uint8_t ta = in_a[c] ? ta_raw[c] : (uint8_t)CELL_SOLID;
+        if (ta == CELL_SOLID || tb == CELL_SOLID) {
+            S.vel[f[c]] = 0.0f;
+        } else if (ta == CELL_FLUID || tb == CELL_FLUID) {
+            const float pa = ta == CELL_FLUID ? pa_raw[c] : 0.0f;
+            const float pb = tb == CELL_FLUID ? pb_raw : 0.0f;
+            S.vel[f[c]] = v[c] - S.dt * S.inv_h * (pb - pa);
+        }
+    }
 }
 template <int NT>
 __device__ __forceinline__ void project_tile(const DevSim &S, uint32_t t) {

@@ -439,6 +439,11 @@ __device__ __forceinline__ float buf_f32(const Buf &b, uint32_t i, bool use) {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(b.r, (int)(use ? 4u * i : b.bytes), 0, 0));
 }
 
+// The 8-byte word at byte offset `off` of a buffer, or 0 for an offset past its size.
+__device__ __forceinline__ int64_t buf_i64(const Buf &b, uint32_t off) {
+    return __builtin_bit_cast(int64_t, __builtin_amdgcn_raw_buffer_load_b64(b.r, (int)off, 0, 0));
+}
+
 // p[i] read as global memory, for a pointer the compiler cannot tell is global (one kept in
 // LDS, say), which it would otherwise access with flat instructions.
 template <typename T>
