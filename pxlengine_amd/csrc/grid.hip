@@ -347,12 +347,18 @@ __device__ __forceinline__ void fluid_row(const DevSim &S, uint32_t row, uint32_
         uint32_t st[ROW_CHUNKS], ns[ROW_CHUNKS][6];
 #pragma unroll
         for (int c = 0; c < ROW_CHUNKS; c++) {
+            // unconditional loads at indices clamped into the grid (a branch per load would make
+            // each wait for the one before), values past the row dropped afterwards
             const int i = i0 + c * WAVE + lane;
             const bool in = i < S.nx;
-            const int32_t id = (int32_t)base + i;
-            st[c] = in ? (uint32_t)S.sten[id] : 0u;
+            const int32_t id = min((int32_t)base + i, last);
+            const uint32_t s0 = S.sten[id];
+            st[c] = in ? s0 : 0u;
 #pragma unroll
-            for (int n = 0; n < 6; n++) ns[c][n] = in ? (uint32_t)S.sten[min(max(id + off[n], 0), last)] : 0u;
+            for (int n = 0; n < 6; n++) {
+                const uint32_t sn = S.sten[min(max(id + off[n], 0), last)];
+                ns[c][n] = in ? sn : 0u;
+            }
         }
 #pragma unroll
         for (int c = 0; c < ROW_CHUNKS; c++) {

@@ -444,6 +444,10 @@ __device__ __forceinline__ int64_t buf_i64(const Buf &b, uint32_t off) {
     return __builtin_bit_cast(int64_t, __builtin_amdgcn_raw_buffer_load_b64(b.r, (int)off, 0, 0));
 }
 
+__device__ __forceinline__ double buf_f64(const Buf &b, uint32_t i, bool use) {
+    return __builtin_bit_cast(double, buf_i64(b, use ? 8u * i : b.bytes));
+}
+
 // p[i] read as global memory, for a pointer the compiler cannot tell is global (one kept in
 // LDS, say), which it would otherwise access with flat instructions.
 template <typename T>

@@ -393,13 +393,18 @@ __device__ __forceinline__ void body_mixed_init(const DevSim &S, const uint32_t 
     int32_t off[6];
     nb_offsets(S, off);
     double rr = 0, bb = 0;
+    // FLUID neighbours are read with masked raw buffer loads and added by select (the sum is the
+    // same as adding only those), so a lane's loads are in flight together
+    const Buf bp = buf_of(S.pressure, 4u * S.ncells);
     for (uint32_t t = beg + threadIdx.x; t < end; t += SBLOCK) {
         const uint32_t id = S.fn_list[t];
         const uint32_t wd = S.fn_word[t];
+        float pn[6];
+#pragma unroll
+        for (int n = 0; n < 6; n++) pn[n] = buf_f32(bp, id + off[n], w_fluid(wd, n));
         double sum = 0.0;
 #pragma unroll
-        for (int n = 0; n < 6; n++)
-            if (w_fluid(wd, n)) sum += (double)S.pressure[id + off[n]];
+        for (int n = 0; n < 6; n++) sum = w_fluid(wd, n) ? sum + (double)pn[n] : sum;
         const double xc = (double)S.pressure[id];
         const double b = b64_of(S, id);
         const double r = b - ((double)w_diag(wd) * xc - sum);
@@ -425,16 +430,24 @@ __device__ __forceinline__ void body_mixed_update(const DevSim &S, const uint32_
     int32_t off[6];
     nb_offsets(S, off);
     double rr = 0;
+    // FLUID neighbours are read with masked raw buffer loads and added by select (the sum is the
+    // same as adding only those), so a lane's loads are in flight together
+    const Buf bx = buf_of(xsrc, 8u * S.ncells), be = buf_of(e, 8u * S.hsplit);
     for (uint32_t t = beg + threadIdx.x; t < end; t += SBLOCK) {
         const uint32_t id = S.fn_list[t], sid = S.fns_list[t];
         const uint32_t wd = S.fn_word[t];
         int32_t soff[6];
         split_offsets(S, wd, soff);  // e is the inner solve's colour-split solution
-        double sum = 0.0;
+        double xn[6];
+        float en[6];
 #pragma unroll
         for (int n = 0; n < 6; n++) {
-            if (w_fluid(wd, n)) sum += xsrc[id + off[n]] + (double)e[sid + soff[n]];
+            xn[n] = buf_f64(bx, id + off[n], w_fluid(wd, n));
+            en[n] = buf_f32(be, sid + soff[n], w_fluid(wd, n));
         }
+        double sum = 0.0;
+#pragma unroll
+        for (int n = 0; n < 6; n++) sum = w_fluid(wd, n) ? sum + (xn[n] + (double)en[n]) : sum;
         const double xc = xsrc[id] + (double)e[sid];
         const double r = b64_of(S, id) - ((double)w_diag(wd) * xc - sum);
         xdst[id] = xc;
