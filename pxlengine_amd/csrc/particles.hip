@@ -343,11 +343,25 @@ __device__ __forceinline__ void body_p2g(const DevSim &S, uint32_t b0, uint32_t 
         tile_coords(S, t, ti, tj, tk);
         for (int q = threadIdx.x; q < BOX_ENTRIES; q += NT) lbox[q] = 0ull;
         __syncthreads();
-        for (uint32_t base = beg; base < end; base += NT) {
-            const uint32_t p = base + local;
+        // software-pipelined: the next round's particle is loaded before this round's LDS
+        // atomics (a tile that is listed holds particles, so index beg is valid for a lane past
+        // the end, whose values are not used)
+        uint32_t p = beg + local;
+        float xn[3], vn[3];
+        {
+            const uint32_t pl = p < end ? p : beg;
+            xn[0] = S.bx[pl], xn[1] = S.by[pl], xn[2] = S.bz[pl];
+            vn[0] = S.bvx[pl], vn[1] = S.bvy[pl], vn[2] = S.bvz[pl];
+        }
+        for (uint32_t base = beg; base < end; base += NT, p += NT) {
+            const float x[3] = {xn[0], xn[1], xn[2]};
+            const float v[3] = {vn[0], vn[1], vn[2]};
+            {
+                const uint32_t pn = p + NT, pl = pn < end ? pn : beg;
+                xn[0] = S.bx[pl], xn[1] = S.by[pl], xn[2] = S.bz[pl];
+                vn[0] = S.bvx[pl], vn[1] = S.bvy[pl], vn[2] = S.bvz[pl];
+            }
             if (p >= end) continue;
-            const float x[3] = {S.bx[p], S.by[p], S.bz[p]};
-            const float v[3] = {S.bvx[p], S.bvy[p], S.bvz[p]};
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 int dim[3] = {S.nx, S.ny, S.nz};
@@ -521,28 +535,43 @@ __device__ __forceinline__ void obstacle_push(const DevSim &S, float x[3], float
 // current particle arrays at the same (sorted) index.
 template <int NT>
 __device__ __forceinline__ void body_g2p(const DevSim &S, uint32_t b0, uint32_t nb) {
-    __shared__ float lvel[BOX_ENTRIES];
-    __shared__ float lold[BOX_ENTRIES];
+    // padded to whole fill rounds, so the fill below stores without a condition (a condition
+    // lets the compiler sink each round's loads into it, one round trip at a time)
+    constexpr int FILL_GROUP = 3, FILL_ROUND = FILL_GROUP * NT;
+    constexpr int BOX_PAD = (BOX_ENTRIES + FILL_ROUND - 1) / FILL_ROUND * FILL_ROUND;
+    __shared__ float lvel[BOX_PAD];
+    __shared__ float lold[BOX_PAD];
     const uint32_t n_active = *S.tile_count;
     for (uint32_t u = b0; u < n_active; u += nb) {
     const uint32_t t = S.tile_list[u];
     const uint32_t beg = S.offset[t * TILE_CELLS], end = S.offset[(t + 1) * TILE_CELLS];
     int ti, tj, tk;
     tile_coords(S, t, ti, tj, tk);
-    for (int q = threadIdx.x; q < BOX_ENTRIES; q += NT) {
-        const int c = q / BOX_FACES, r = q - c * BOX_FACES;
-        int lo[3], d[3];
-        tile_box(c, ti, tj, tk, lo, d);
-        const int qx = lo[0] + r % d[0], qy = lo[1] + (r / d[0]) % d[1], qz = lo[2] + r / (d[0] * d[1]);
-        const int dx = S.nx + (c == 0), dy = S.ny + (c == 1), dz = S.nz + (c == 2);
-        float a = 0.0f, o = 0.0f;
-        if (qx >= 0 && qy >= 0 && qz >= 0 && qx < dx && qy < dy && qz < dz) {
-            const uint32_t fi = face_index(S, c, qx, qy, qz);
-            a = S.vel[fi];
-            o = S.vel_old[fi];
+    // the box fill, FILL_GROUP faces of a lane at a time: their loads (faces past the grid or
+    // the box at index 0, dropped) are all issued before any LDS store, so they are in flight
+    // together (all of a lane's faces at once needed 50% more registers than the rest of the
+    // kernel)
+    for (int q0 = (int)threadIdx.x; q0 < BOX_ENTRIES; q0 += FILL_ROUND) {
+        float fa[FILL_GROUP], fo[FILL_GROUP];
+#pragma unroll
+        for (int m = 0; m < FILL_GROUP; m++) {
+            const int q = q0 + m * NT, qq = q < BOX_ENTRIES ? q : 0;
+            const int c = qq / BOX_FACES, r = qq - c * BOX_FACES;
+            int lo[3], d[3];
+            tile_box(c, ti, tj, tk, lo, d);
+            const int qx = lo[0] + r % d[0], qy = lo[1] + (r / d[0]) % d[1], qz = lo[2] + r / (d[0] * d[1]);
+            const int dx = S.nx + (c == 0), dy = S.ny + (c == 1), dz = S.nz + (c == 2);
+            const bool ok = q < BOX_ENTRIES && qx >= 0 && qy >= 0 && qz >= 0 && qx < dx && qy < dy && qz < dz;
+            const uint32_t fi = ok ? face_index(S, c, qx, qy, qz) : 0u;
+            const float a = S.vel[fi], o = S.vel_old[fi];
+            fa[m] = ok ? a : 0.0f;
+            fo[m] = ok ? o : 0.0f;
         }
-        lvel[q] = a;
-        lold[q] = o;
+#pragma unroll
+        for (int m = 0; m < FILL_GROUP; m++) {
+            lvel[q0 + m * NT] = fa[m];  // past the box: the padding
+            lold[q0 + m * NT] = fo[m];
+        }
     }
     __syncthreads();
     const float tb = S.flip;
