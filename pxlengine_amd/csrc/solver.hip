@@ -43,8 +43,10 @@ constexpr int SBLOCK = 1024;  // lanes per solver workgroup (16 wave64)
 constexpr int SWAVES = SBLOCK / WAVE;
 // List entries a lane of the MG-CG kernels has in flight at once (their loads are issued
 // before any of their stores), within 64 VGPRs (two 1024-lane workgroups per CU).  Measured on
-// the 64-request batch: two entries make spmv 15% faster; more than one makes update slower.
-constexpr int SPMV_MLP = 2, UPD_MLP = 1;
+// the 64-request batch: two entries make spmv 15% faster (four: 60% slower).  With the global
+// (not flat) loads of the batched kernels, two entries make update 5-10% faster on the
+// 512-request batch and leave it unchanged on the 64-request batch (four: 20% slower there).
+constexpr int SPMV_MLP = 2, UPD_MLP = 2;
 
 // Fields of a list entry's stencil word (see DevSim::fl_word).
 __device__ __forceinline__ bool w_fluid(uint32_t w, int n) { return (w >> n) & 1u; }
