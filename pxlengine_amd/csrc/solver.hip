@@ -524,12 +524,20 @@ __global__ __launch_bounds__(SBLOCK) void k_rbgs(DevSim S, uint32_t parity) {
     for (uint32_t t = beg + threadIdx.x; t < end; t += SBLOCK) {
         const uint32_t id = S.fl_list[t];
         const uint32_t wd = S.fl_word[t];
+        // all six neighbours are loaded (one that is not FLUID at the cell itself) before any is
+        // added, and added by select: the same additions in the same order as adding only the
+        // FLUID ones, without a branch per load (each of which waited for its load)
+        float pn[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) pn[k] = p[w_fluid(wd, k) ? id + off[k] : id];
+        const float pc = p[id];
         float sum = 0.0f;
 #pragma unroll
-        for (int k = 0; k < 6; k++)
-            if (w_fluid(wd, k)) sum = sum + p[id + off[k]];
+        for (int k = 0; k < 6; k++) sum = w_fluid(wd, k) ? sum + pn[k] : sum;
         const float cnt = w_diag(wd);
-        if (cnt > 0.0f) p[id] = (sum - S.div[id] * S.h * S.h / S.dt) / cnt;
+        // a cell without non-SOLID neighbours keeps its value (stored back unchanged: a
+        // conditional store would let the compiler sink the list load behind the word's)
+        p[id] = cnt > 0.0f ? (sum - S.div[id] * S.h * S.h / S.dt) / cnt : pc;
     }
 }
 
