@@ -446,11 +446,11 @@ __device__ __forceinline__ void project_face(const DevSim &S, uint32_t f, int ai
     }
 }
 
-// project_face for the node's three faces (cell b = (i, j, k) on the high side of each), with
-// every type, pressure and velocity the three might need loaded first, at in-grid indices
-// (index 0 for a cell outside the grid, whose type then counts as SOLID), so a lane's loads
-// are in flight together instead of one face's pressures waiting on its types.
-__device__ __forceinline__ void project_at(const DevSim &S, int i, int j, int k) {
+// project_face for the node's three faces (cell b = (i, j, k) on the high side of each) in two
+// rounds of loads without a branch: the four cells' types (index 0 for a cell outside the grid,
+// whose type then counts as SOLID), then, as masked raw buffer loads (no memory access where
+// not needed), the FLUID cells' pressures and the velocities of the faces that change.
+__device__ __forceinline__ void project_at(const DevSim &S, const Buf &bp, const Buf &bv, int i, int j, int k) {
     const bool has[3] = {j < S.ny && k < S.nz, i < S.nx && k < S.nz, i < S.nx && j < S.ny};
     const uint32_t f[3] = {has[0] ? uidx(S, i, j, k) : 0u, has[1] ? vidx(S, i, j, k) : 0u, has[2] ? widx(S, i, j, k) : 0u};
     const int ca[3][3] = {{i - 1, j, k}, {i, j - 1, k}, {i, j, k - 1}};
@@ -464,33 +464,37 @@ __device__ __forceinline__ void project_at(const DevSim &S, int i, int j, int k)
         ia[c] = in_a[c] ? cidx(S, ca[c][0], ca[c][1], ca[c][2]) : 0u;
     }
     const uint8_t tb_raw = S.type[ib];
-    const float pb_raw = S.pressure[ib];
     uint8_t ta_raw[3];
-    float pa_raw[3], v[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) ta_raw[c] = S.type[ia[c]];
+    const uint8_t tb = in_b ? tb_raw : (uint8_t)CELL_SOLID;
+    uint8_t ta[3];
+    bool solid[3], upd[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        ta_raw[c] = S.type[ia[c]];
-        pa_raw[c] = S.pressure[ia[c]];
-        v[c] = S.vel[f[c]];
+        ta[c] = in_a[c] ? ta_raw[c] : (uint8_t)CELL_SOLID;
+        solid[c] = ta[c] == CELL_SOLID || tb == CELL_SOLID;
+        upd[c] = has[c] && !solid[c] && (ta[c] == CELL_FLUID || tb == CELL_FLUID);
     }
-    const uint8_t tb = in_b ? tb_raw : (uint8_t)CELL_SOLID;
+    const float pb = buf_f32(bp, ib, tb == CELL_FLUID);
+    float pa[3], v[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        pa[c] = buf_f32(bp, ia[c], upd[c] && ta[c] == CELL_FLUID);
+        v[c] = buf_f32(bv, f[c], upd[c]);
+    }
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         if (!has[c]) continue;
-        const uint8_t ta = in_a[c] ? ta_raw[c] : (uint8_t)CELL_SOLID;
-        if (ta == CELL_SOLID || tb == CELL_SOLID) {
-            S.vel[f[c]] = 0.0f;
-        } else if (ta == CELL_FLUID || tb == CELL_FLUID) {
-            const float pa = ta == CELL_FLUID ? pa_raw[c] : 0.0f;
-            const float pb = tb == CELL_FLUID ? pb_raw : 0.0f;
-            S.vel[f[c]] = v[c] - S.dt * S.inv_h * (pb - pa);
-        }
+        if (solid[c]) S.vel[f[c]] = 0.0f;
+        else if (upd[c]) S.vel[f[c]] = v[c] - S.dt * S.inv_h * (pb - pa[c]);
     }
 }
 template <int NT>
 __device__ __forceinline__ void project_tile(const DevSim &S, uint32_t t) {
     if (S.tile_act[t] != 0) {
-        for_tile_box<NT>(S, t, true, [&](int i, int j, int k) { project_at(S, i, j, k); });
+        const Buf bp = buf_of(S.pressure, 4u * S.ncells), bv = buf_of(S.vel, 4u * S.nfaces);
+        for_tile_box<NT>(S, t, true, [&](int i, int j, int k) { project_at(S, bp, bv, i, j, k); });
         return;
     }
     // rim tile: its cells are not FLUID, so only faces on its lower planes can border a FLUID
