@@ -140,6 +140,12 @@ struct DevSim {
     // non-SOLID neighbour count (matrix diagonal), bits 9+3n the diagonal of FLUID neighbour n,
     // bit 27 the cell's colour (1 = black: i + j + k odd), bit 28 its x parity (i & 1)
     uint32_t *fl_word;
+    // reference mode (RBGS, several half sweeps per launch): per core of the grid (rbgs_cores),
+    // both colours' lists of the FLUID cells of its region (rbgs_list_slots entries per core:
+    // region index, x parity and diagonal in rb_list, the b term in rb_b) and the lists'
+    // lengths by distance from the region's faces (8 words per core); null in other modes
+    uint32_t *rb_list, *rb_count;
+    float *rb_b;
     // the same entries in plain cell order (with colours not split: fl_list / fl_word
     // themselves), for the kernels that treat every FLUID cell alike: their vector accesses
     // then run along x instead of skipping every other cell
@@ -326,6 +332,12 @@ void launch_faces(const DevSim &S, hipStream_t s);            // normalize + for
 void launch_cells(const DevSim &S, hipStream_t s);            // type, stencil, p=0, div, flags
 void launch_fluid_list(const DevSim &S, hipStream_t s);       // row counts -> fl_list, fl_count
 void launch_rbgs(const DevSim &S, uint32_t parity, hipStream_t s);  // over the fluid list
+// `iters` red-black iterations, up to two per launch after one launch that lists each core's
+// FLUID cells, ping-ponging between the pressure array and cg_r0 (an even number of launches:
+// the result ends in the pressure array)
+void launch_rbgs_blocked(const DevSim &S, uint32_t iters, hipStream_t s);
+uint32_t rbgs_cores(const DevSim &S);  // cores of the grid (rb_count holds 8 words per core)
+uint32_t rbgs_list_slots();            // list entries per core (rb_list, rb_b)
 void launch_project(const DevSim &S, hipStream_t s);
 void launch_g2p(const DevSim &S, hipStream_t s);              // b -> a, advected
 // Jacobi-preconditioned CG (solver.hip, over the fluid list), one fused launch per iteration:

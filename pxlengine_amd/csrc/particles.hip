@@ -712,22 +712,32 @@ __global__ __launch_bounds__(BLOCK) void k_reset_particles(DevSim S) {
 
 }  // namespace
 
-// Largest squared particle speed, for the CFL cap: a wave maximum, then one atomic per wave.
+// Largest squared particle speed, for the CFL cap: each workgroup takes a strided share of the
+// particles, reduces it (lanes, then waves through LDS) and adds one atomic maximum.  The
+// atomics all go to one word, so their number is kept to the workgroups of one pass over the
+// chip (one per wave serialised some 7,000 atomics for 460K particles).
+constexpr uint32_t VMAX_BLOCKS = 512;
 __global__ __launch_bounds__(BLOCK) void k_vmax2(DevSim S, uint32_t *out) {
-    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    __shared__ float sh[BLOCK / WAVE];
     float v2 = 0.0f;
-    if (i < S.np) {
+    for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < S.np; i += gridDim.x * BLOCK) {
         const float vx = S.avx[i], vy = S.avy[i], vz = S.avz[i];
-        v2 = vx * vx + vy * vy + vz * vz;
-        if (!(v2 >= 0.0f)) v2 = 0.0f;  // NaN
+        const float q = vx * vx + vy * vy + vz * vz;
+        v2 = q >= 0.0f ? fmaxf(v2, q) : v2;  // NaN adds nothing
     }
     for (int off = WAVE / 2; off > 0; off >>= 1) v2 = fmaxf(v2, __shfl_xor(v2, off, WAVE));
-    if ((threadIdx.x & (WAVE - 1)) == 0) atomicMax(out, __float_as_uint(v2));
+    if ((threadIdx.x & (WAVE - 1)) == 0) sh[threadIdx.x / WAVE] = v2;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m = sh[0];
+        for (int w = 1; w < BLOCK / WAVE; w++) m = fmaxf(m, sh[w]);
+        atomicMax(out, __float_as_uint(m));
+    }
 }
 
 void launch_vmax2(const DevSim &S, uint32_t *out, hipStream_t s) {
     launch_fill_u32(out, 1, 0u, s);
-    if (S.np) PXL_LAUNCH(k_vmax2, blocks_for(S.np), BLOCK, 0, s, S, out);
+    if (S.np) PXL_LAUNCH(k_vmax2, std::min(blocks_for(S.np), VMAX_BLOCKS), BLOCK, 0, s, S, out);
 }
 
 void launch_fill_u32(uint32_t *p, uint32_t n, uint32_t value, hipStream_t s) {

@@ -109,6 +109,8 @@ struct Sim {
     bool initialized = false;
     hipGraphExec_t step_graph = nullptr;
     std::map<uint32_t, std::array<hipGraphExec_t, 2>> inner_graph;  // [MG-CG iterations][x64 parity]
+    static constexpr uint32_t MAX_SUBSTEP_GRAPHS = 8;
+    std::map<uint32_t, hipGraphExec_t> sub_graph;  // [CFL substeps]: the step at dt / substeps
     hipGraphExec_t grid_graph = nullptr, post_graph = nullptr;  // MIXED: phases around the solve
     double *host_scal = nullptr;  // pinned
     int x64_cur = 0;
@@ -165,6 +167,9 @@ struct Sim {
             for (hipGraphExec_t g : e.second)
                 if (g) (void)hipGraphExecDestroy(g);
         inner_graph.clear();
+        for (auto &e : sub_graph)
+            if (e.second) (void)hipGraphExecDestroy(e.second);
+        sub_graph.clear();
     }
 
     ~Sim() {
@@ -219,6 +224,12 @@ struct Sim {
             add(&d.fn_word, 4 * C);
         }
         add(&d.fl_count, 64);
+        if (opts.solver == PXLFLIP_SOLVER_RBGS) {
+            const size_t nl = (size_t)rbgs_cores(d) * rbgs_list_slots();
+            add(&d.rb_list, 4 * nl);
+            add(&d.rb_b, 4 * nl);
+            add(&d.rb_count, 32ull * rbgs_cores(d));
+        }
         add(&d.fl_tmp, 4ull * (scan_tiles(fl_row_slots(d)) + 1));
         // the MIXED inner solve stores its vectors colour-split (2 hsplit entries >= C)
         const size_t V = opts.solver == PXLFLIP_SOLVER_MIXED ? std::max<size_t>(C, 2ull * d.hsplit) : C;
@@ -318,10 +329,7 @@ struct Sim {
     }
     void enqueue_fixed_pressure() {
         if (opts.solver == PXLFLIP_SOLVER_RBGS) {
-            for (uint32_t it = 0; it < iters; it++) {
-                KRUN("rbgs", launch_rbgs(d, 0, stream));
-                KRUN("rbgs", launch_rbgs(d, 1, stream));
-            }
+            KRUN("rbgs", launch_rbgs_blocked(d, iters, stream));
         } else {
             KRUN("cg_start", launch_cg_start(d, false, d.pressure, false, stream));
             for (uint32_t it = 0; it < iters; it++) KRUN("cg_iter", launch_cg_iter(d, false, d.pressure, it, stream));
@@ -437,23 +445,26 @@ struct Sim {
         return cfl_substeps(v2, d.dt, spec.cfl, d.h);
     }
     // One step of the current d.dt (graphs when allowed, else eager).
-    void step_once() {
+    // `sg`: the graph slot of the fixed-iteration step at this d.dt (the full step's by default).
+    void step_once(hipGraphExec_t *sg = nullptr) {
         if (opts.solver == PXLFLIP_SOLVER_MIXED) {
             replay(&grid_graph, [this] { enqueue_grid_init(); });
             mixed_pressure();
             replay(&post_graph, [this] { enqueue_post(); });
         } else {
             const bool graph = opts.use_graph && !timer.on;
-            if (graph && !step_graph) step_graph = capture([this] { enqueue_fixed_step(); });
-            if (graph) HIP_CHECK(hipGraphLaunch(step_graph, stream));
+            hipGraphExec_t *gp = sg ? sg : &step_graph;
+            if (graph && !*gp) *gp = capture([this] { enqueue_fixed_step(); });
+            if (graph) HIP_CHECK(hipGraphLaunch(*gp, stream));
             else enqueue_fixed_step();
             stats.rel_residual = -1.0;
             stats.inner_iterations = iters;
             stats.outer_iterations = 0;
         }
     }
-    // With the CFL cap each step of dt runs as k substeps of dt / k (f32).  Graphs hold the
-    // full dt, so the substeps of a split step launch eagerly.
+    // With the CFL cap each step of dt runs as k substeps of dt / k (f32).  The fixed-iteration
+    // solvers replay one graph per substep count up to MAX_SUBSTEP_GRAPHS (their kernels hold
+    // dt / k); larger counts and the MIXED solve (whose graphs hold the full dt) launch eagerly.
     void run_capped_steps(uint32_t n) {
         for (uint32_t s = 0; s < n; s++) {
             const uint32_t k = cfl_step_count();
@@ -467,9 +478,11 @@ struct Sim {
                 const float dt0 = d.dt;
                 const int g = opts.use_graph;
                 d.dt = dt0 / (float)k;
-                opts.use_graph = 0;
+                hipGraphExec_t *sg = nullptr;
+                if (opts.solver != PXLFLIP_SOLVER_MIXED && k <= MAX_SUBSTEP_GRAPHS) sg = &sub_graph[k];
+                else opts.use_graph = 0;
                 try {
-                    for (uint32_t q = 0; q < k; q++) step_once();
+                    for (uint32_t q = 0; q < k; q++) step_once(sg);
                 } catch (...) {
                     d.dt = dt0;
                     opts.use_graph = g;

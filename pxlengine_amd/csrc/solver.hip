@@ -541,6 +541,247 @@ __global__ __launch_bounds__(SBLOCK) void k_rbgs(DevSim S, uint32_t parity) {
     }
 }
 
+// ---- pressure_rb, up to RB_HALO half sweeps per launch (temporal blocking).  A workgroup
+// owns a CX x CY x CZ core of the grid and holds the core plus RB_HALO cells on every side
+// (its region) in LDS, the pressures split by colour.  Half sweep s (1-based) of a launch
+// updates the region's FLUID cells of its colour at least s cells from the region's faces;
+// their six neighbours are then at least s - 1 from the faces and hold the values after half
+// sweep s - 1, so after s <= RB_HALO half sweeps the core holds what s whole-grid half sweeps
+// give.  Each update is k_rbgs's arithmetic in k_rbgs's order.  The sum runs over all six
+// neighbours: cells that are not FLUID hold pressure 0 (the cell pass zeroes them, and no
+// solver writes them), and a sum that starts at +0 is never -0, so adding a zero leaves it
+// unchanged and 0 + p0 + ... + p5 equals k_rbgs's sum over the FLUID neighbours bit for bit.
+// A FLUID cell whose neighbours are all SOLID keeps its value, as there.
+//
+// The FLUID cells do not change during the solve, so one launch per solve lists each core's
+// region cells to update (FLUID, at least one cell inside the region, with a non-SOLID
+// neighbour) per colour, ordered by distance from the faces (4 or more, 3, 2, 1): half sweep
+// s then walks the prefix at distance >= s.  A sweep launch reads `pin` and writes its core
+// to `pout`, so no workgroup reads a value another has already moved past.  A core with no
+// listed cell keeps its values; it is copied by the first launch (whose `pout` holds values
+// of an earlier solve) and skipped by the others.
+constexpr int RB_HALO = 4;
+
+template <int CX, int CY, int CZ, int NT>
+struct RbTile {
+    static constexpr int RX = CX + 2 * RB_HALO, RY = CY + 2 * RB_HALO, RZ = CZ + 2 * RB_HALO;
+    static constexpr int HX = RX / 2;                 // cells of one colour in a region row
+    static constexpr int NC = HX * RY * RZ;           // cells of one colour in the region
+    static constexpr int SL = (NC + NT - 1) / NT;     // of them per lane
+    static constexpr int NQ = RX * RY * RZ;           // region cells
+    static constexpr int LMAX = ((RX - 2) * (RY - 2) * (RZ - 2) + 1) / 2;  // listed cells of a colour, at most
+    static constexpr int K = (LMAX + NT - 1) / NT;    // of them per lane
+    static_assert(CX % TILE == 0 && CY % TILE == 0 && CZ % TILE == 0, "cores are whole 8^3 tiles");
+    static_assert(RX % 2 == 0, "region rows split evenly by colour");
+    static_assert(NC < (1 << 13), "13-bit region indices");
+};
+// The core shape of reference mode: 8 x 8 x 16 cells.  Sweep launches run 1024 lanes per core
+// when the grid has at most RB_WIDE_CORES cores (64^3: 256 cores, one per CU) and 512 above
+// (128^3: 2048 cores; measured 14% faster there, 9% slower at 64^3).
+constexpr int RB_CX = 8, RB_CY = 8, RB_CZ = 16, RB_NT = 1024;
+constexpr uint32_t RB_WIDE_CORES = 1024;
+using RbCore = RbTile<RB_CX, RB_CY, RB_CZ, RB_NT>;
+static_assert(RbTile<RB_CX, RB_CY, RB_CZ, 512>::LMAX == RbCore::LMAX, "one list layout for both widths");
+// list entry: bits 0-12 the region index of its colour, bit 13 x & 1, bits 14-16 the diagonal
+
+__device__ __forceinline__ void rb_core_of(const DevSim &S, uint32_t core, int cx, int cy, int cz, int &x0, int &y0,
+                                           int &z0) {
+    const uint32_t ncx = ((uint32_t)S.nx + cx - 1) / cx, ncy = ((uint32_t)S.ny + cy - 1) / cy;
+    x0 = (int)(core % ncx) * cx;
+    y0 = (int)((core / ncx) % ncy) * cy;
+    z0 = (int)(core / (ncx * ncy)) * cz;
+}
+
+template <int CX, int CY, int CZ, int NT>
+__global__ __launch_bounds__(NT) void k_rbgs_lists(DevSim S) {
+    using T = RbTile<CX, CY, CZ, NT>;
+    constexpr int H = RB_HALO, HX = T::HX, RX = T::RX, RY = T::RY, RZ = T::RZ, NC = T::NC, SL = T::SL, LMAX = T::LMAX;
+    __shared__ uint32_t cnt[2][4], from[2][4];
+    const uint32_t core = blockIdx.x;
+    int x0, y0, z0;
+    rb_core_of(S, core, CX, CY, CZ, x0, y0, z0);
+    const int tid = (int)threadIdx.x, lane = tid & (WAVE - 1);
+    if (tid < 8) cnt[tid >> 2][tid & 3] = 0u;
+    const int ox = x0 - H, oy = y0 - H, oz = z0 - H;
+    const uint32_t ncell = (uint32_t)S.ncells;
+    const Buf bd = buf_of(S.div, 4u * ncell);
+    const __amdgpu_buffer_rsrc_t bs = __builtin_amdgcn_make_buffer_rsrc(S.sten, (short)0, (int)(2u * ncell), 0x00020000);
+    uint32_t st[2][SL];
+    float dv[2][SL];
+    int lvl[2][SL];
+#pragma unroll
+    for (int m = 0; m < SL; m++) {
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            const int u = tid + NT * m, row = u / HX, y = row % RY, z = row / RY, x = 2 * (u % HX) + ((c ^ y ^ z) & 1);
+            const int gx = ox + x, gy = oy + y, gz = oz + z;
+            const bool in = u < NC && gx >= 0 && gy >= 0 && gz >= 0 && gx < S.nx && gy < S.ny && gz < S.nz;
+            const uint32_t id = in ? (uint32_t)gx + (uint32_t)S.nx * ((uint32_t)gy + (uint32_t)S.ny * (uint32_t)gz) : 0u;
+            st[c][m] = __builtin_amdgcn_raw_buffer_load_b16(bs, (int)(in ? 2u * id : 2u * ncell), 0, 0);
+            dv[c][m] = buf_f32(bd, id, in);
+            lvl[c][m] = in ? min(min(min(x, RX - 1 - x), min(y, RY - 1 - y)), min(z, RZ - 1 - z)) : 0;
+        }
+    }
+    __syncthreads();
+    // ranks within (colour, distance) classes: one LDS atomic per wave and class
+    uint32_t rank[2][SL];
+#pragma unroll
+    for (int m = 0; m < SL; m++) {
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            const uint32_t s = st[c][m];
+            const bool listed = (s & STEN_FLUID) && lvl[c][m] >= 1 && ((s >> 8) & 7u) != 0u;
+            const int cls = 4 - min(lvl[c][m], 4);
+            rank[c][m] = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint64_t mask = __ballot(listed && cls == k);
+                if (mask == 0) continue;
+                const int leader = __ffsll((long long)mask) - 1;
+                uint32_t base = 0;
+                if (lane == leader) base = atomicAdd(&cnt[c][k], (uint32_t)__popcll(mask));
+                base = __shfl(base, leader, WAVE);
+                if (listed && cls == k) rank[c][m] = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+            }
+        }
+    }
+    __syncthreads();
+    if (tid < 2) {
+        uint32_t a = 0;
+        for (int k = 0; k < 4; k++) {
+            from[tid][k] = a;
+            a += cnt[tid][k];
+        }
+    }
+    __syncthreads();
+    uint32_t *lw = S.rb_list + (size_t)core * 2 * LMAX;
+    float *lb = S.rb_b + (size_t)core * 2 * LMAX;
+    const float h = S.h, dt = S.dt;
+#pragma unroll
+    for (int m = 0; m < SL; m++) {
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            const uint32_t s = st[c][m];
+            const bool listed = (s & STEN_FLUID) && lvl[c][m] >= 1 && ((s >> 8) & 7u) != 0u;
+            if (listed) {
+                const int u = tid + NT * m, row = u / HX, y = row % RY, z = row / RY;
+                const uint32_t x1 = (uint32_t)((c ^ y ^ z) & 1);
+                const uint32_t pos = c * LMAX + from[c][4 - min(lvl[c][m], 4)] + rank[c][m];
+                lw[pos] = (uint32_t)u | (x1 << 13) | (((s >> 8) & 7u) << 14);
+                lb[pos] = dv[c][m] * h * h / dt;
+            }
+        }
+    }
+    if (tid < 8) S.rb_count[core * 8 + tid] = cnt[tid >> 2][tid & 3];
+}
+
+template <int CX, int CY, int CZ, int NT>
+__global__ __launch_bounds__(NT) void k_rbgs_tile(DevSim S, const float *pin, float *pout, uint32_t first,
+                                                  uint32_t nsweep, int copy_idle) {
+    using T = RbTile<CX, CY, CZ, NT>;
+    constexpr int H = RB_HALO, HX = T::HX, RX = T::RX, RY = T::RY, NC = T::NC, NQ = T::NQ, LMAX = T::LMAX, K = T::K;
+    __shared__ float P[2 * NC];
+    uint32_t core = blockIdx.x;
+    const uint32_t g = gridDim.x;
+    if (g % NXCD == 0) core = (core % NXCD) * (g / NXCD) + core / NXCD;  // neighbouring cores on one XCD
+    int x0, y0, z0;
+    rb_core_of(S, core, CX, CY, CZ, x0, y0, z0);
+    const int tid = (int)threadIdx.x;
+    const int ox = x0 - H, oy = y0 - H, oz = z0 - H;
+    const uint32_t opar = (uint32_t)(ox + oy + oz) & 1u;
+    const int nx = S.nx, ny = S.ny, nz = S.nz;
+    // list lengths by class (distance 4+, 3, 2, 1) and colour
+    uint32_t cn[2][4];
+#pragma unroll
+    for (int k = 0; k < 8; k++) cn[k >> 2][k & 3] = S.rb_count[core * 8 + k];
+    // the region's pressures (x fastest: whole rows of the grid)
+    const Buf bp = buf_of(pin, 4u * (uint32_t)S.ncells);
+    constexpr int QS = (NQ + NT - 1) / NT;
+    float pv[QS];
+#pragma unroll
+    for (int m = 0; m < QS; m++) {
+        const int q = tid + NT * m, x = q % RX, y = (q / RX) % RY, z = q / (RX * RY);
+        const int gx = ox + x, gy = oy + y, gz = oz + z;
+        const bool in = q < NQ && gx >= 0 && gy >= 0 && gz >= 0 && gx < nx && gy < ny && gz < nz;
+        pv[m] = buf_f32(bp, in ? (uint32_t)gx + (uint32_t)nx * ((uint32_t)gy + (uint32_t)ny * (uint32_t)gz) : 0u, in);
+    }
+    const uint32_t ntot[2] = {cn[0][0] + cn[0][1] + cn[0][2] + cn[0][3], cn[1][0] + cn[1][1] + cn[1][2] + cn[1][3]};
+    const bool idle = cn[0][0] + cn[1][0] == 0u;  // no listed core cell
+    if (idle && !copy_idle) return;
+    // this lane's list entries
+    const uint32_t *lw = S.rb_list + (size_t)core * 2 * LMAX;
+    const float *lb = S.rb_b + (size_t)core * 2 * LMAX;
+    constexpr uint32_t SAFE = (uint32_t)(HX * RY + 1);  // lanes past the list read around an interior cell
+    uint32_t w[2][K];
+    float bt[2][K];
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const uint32_t t = (uint32_t)(tid + NT * k);
+            const bool ok = t < ntot[c];
+            w[c][k] = ok ? ld_global(lw, c * LMAX + t) : SAFE;
+            bt[c][k] = ok ? ld_global(lb, c * LMAX + t) : 0.0f;
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < QS; m++) {
+        const int q = tid + NT * m, x = q % RX, y = (q / RX) % RY, z = q / (RX * RY);
+        if (q < NQ) P[((x + y + z) & 1) * NC + (y + RY * z) * HX + (x >> 1)] = pv[m];
+    }
+    __syncthreads();
+
+    const int wave0 = tid & ~(WAVE - 1);
+    for (uint32_t j = 0; j < nsweep; j++) {
+        const uint32_t lc = ((first + j) & 1u) ^ opar;
+        uint32_t n = 0;  // entries at distance > j
+#pragma unroll
+        for (int k = 0; k < 4; k++) n += (k < 4 - (int)j) ? (lc ? cn[1][k] : cn[0][k]) : 0u;
+        const float *Q = P + (lc ^ 1u) * NC;
+        float *Pc = P + lc * NC;
+        float nb[K][6];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            if ((uint32_t)(wave0 + NT * k) < n) {
+                const uint32_t e = lc ? w[1][k] : w[0][k];
+                const int u = (int)(e & 0x1fffu), x1 = (int)((e >> 13) & 1u);
+                nb[k][0] = Q[u - 1 + x1];
+                nb[k][1] = Q[u + x1];
+                nb[k][2] = Q[u - HX];
+                nb[k][3] = Q[u + HX];
+                nb[k][4] = Q[u - HX * RY];
+                nb[k][5] = Q[u + HX * RY];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            if ((uint32_t)(wave0 + NT * k) < n) {
+                const uint32_t e = lc ? w[1][k] : w[0][k];
+                float sum = 0.0f;
+#pragma unroll
+                for (int q = 0; q < 6; q++) sum = sum + nb[k][q];
+                const float v = (sum - (lc ? bt[1][k] : bt[0][k])) / (float)((e >> 14) & 7u);
+                if ((uint32_t)(tid + NT * k) < n) Pc[e & 0x1fffu] = v;
+            }
+        }
+        __syncthreads();
+    }
+
+    // the core
+    constexpr int CS = (CX * CY * CZ + NT - 1) / NT;
+#pragma unroll
+    for (int m = 0; m < CS; m++) {
+        const int q = tid + NT * m;
+        const int x = q % CX, y = (q / CX) % CY, z = q / (CX * CY);
+        const int gx = x0 + x, gy = y0 + y, gz = z0 + z;
+        if (q < CX * CY * CZ && gx < nx && gy < ny && gz < nz) {
+            const int rx = x + H, ry = y + H, rz = z + H;
+            pout[(uint32_t)gx + (uint32_t)nx * ((uint32_t)gy + (uint32_t)ny * (uint32_t)gz)] =
+                P[((rx + ry + rz) & 1) * NC + (ry + RY * rz) * HX + (rx >> 1)];
+        }
+    }
+}
+
 inline float *r_buf(const DevSim &S, uint32_t k) { return (k & 1) ? S.cg_r1 : S.cg_r0; }
 inline float *w_buf(const DevSim &S, uint32_t k) { return (k & 1) ? S.cg_w1 : S.cg_w0; }
 inline float *s_buf(const DevSim &S, uint32_t k) { return (k & 1) ? S.cg_s1 : S.cg_s0; }
@@ -593,6 +834,35 @@ void launch_mixed_update(const DevSim &S, const double *xsrc, double *xdst, hipS
 
 void launch_rbgs(const DevSim &S, uint32_t parity, hipStream_t s) {
     PXL_LAUNCH(k_rbgs, solver_blocks(S), SBLOCK, 0, s, S, parity);
+}
+
+uint32_t rbgs_cores(const DevSim &S) {
+    return (((uint32_t)S.nx + RB_CX - 1) / RB_CX) * (((uint32_t)S.ny + RB_CY - 1) / RB_CY) *
+           (((uint32_t)S.nz + RB_CZ - 1) / RB_CZ);
+}
+uint32_t rbgs_list_slots() { return 2u * RbCore::LMAX; }
+
+void launch_rbgs_blocked(const DevSim &S, uint32_t iters, hipStream_t s) {
+    const uint32_t sweeps = 2 * iters;
+    if (sweeps == 0) return;
+    const uint32_t g = rbgs_cores(S);
+    PXL_LAUNCH((k_rbgs_lists<RB_CX, RB_CY, RB_CZ, RB_NT>), g, RB_NT, 0, s, S);
+    // an even number of launches (the last then writes the pressure array), each of at most
+    // RB_HALO half sweeps
+    uint32_t launches = (sweeps + RB_HALO - 1) / RB_HALO;
+    launches += launches & 1u;
+    const uint32_t base = sweeps / launches, extra = sweeps % launches;
+    float *a = S.pressure, *b = S.cg_r0;  // the CG vectors are free in reference mode
+    uint32_t done = 0;
+    for (uint32_t l = 0; l < launches; l++) {
+        const uint32_t n = base + (l < extra ? 1u : 0u);
+        if (g <= RB_WIDE_CORES)
+            PXL_LAUNCH((k_rbgs_tile<RB_CX, RB_CY, RB_CZ, RB_NT>), g, RB_NT, 0, s, S, a, b, done & 1u, n, l == 0 ? 1 : 0);
+        else
+            PXL_LAUNCH((k_rbgs_tile<RB_CX, RB_CY, RB_CZ, 512>), g, 512, 0, s, S, a, b, done & 1u, n, l == 0 ? 1 : 0);
+        done += n;
+        std::swap(a, b);
+    }
 }
 
 // ---- batched (the solving simulations: x = e, r = cg_r0, u = cg_r1)

@@ -398,6 +398,23 @@ def test_autotune_preserves_state(pxl):
     b.close()
 
 
+@pytest.mark.parametrize("iters", [1, 2, 3, 7])
+def test_rbgs_bitwise_any_iteration_count(pxl, orc, iters):
+    """Iteration counts that do not split into whole launches of two red-black iterations
+    (and the smallest ones, run as single half sweeps) give the oracle's steps bit for bit,
+    on a 20 x 28 x 36 grid that is no multiple of the 8 x 8 x 16 cores."""
+    fluid = dict(dam_break(36), bounds=[[0, 0, 0], [1.0, 1.4, 1.8]], pressure_iterations=iters)
+    print(f"rbgs iterations {iters}", flush=True)
+    gpu = pxl.FluidState(fluid, FPS, seed=5)
+    sim = oracle_sim(orc, fluid, seed=5)
+    for s in range(3):
+        gpu.step(1)
+        sim.step_rbgs(iters)
+        _state_equal(gpu, sim, f"iterations {iters} step {s + 1}")
+        np.testing.assert_array_equal(gpu.read_field("pressure"), sim.pressure)
+    gpu.close()
+
+
 @pytest.mark.parametrize("wg", [1, 7, 4096])
 def test_rbgs_bitwise_any_workgroups(pxl, orc, wg):
     """RBGS results do not depend on the solver launch shape."""
