@@ -62,7 +62,6 @@ def plain_algorithmic_bytes(name: str, P: int, F: int, C: int, Cf: int) -> float
         "p2g": 24 * P + 8 * F,
         "faces": 16 * F,
         "cells": 4 * C + 4 * F + 7 * C + 4 * (C - Cf),
-        "rbgs": C + 8 * Cf,
         "cg_start": 2 * C + 12 * Cf + 2 * C + 16 * Cf,
         "cg_iter": 2 * C + 40 * Cf,
         "project": 8 * F + 5 * C,

@@ -331,7 +331,6 @@ void launch_p2g(const DevSim &S, hipStream_t s);              // b -> per-tile b
 void launch_faces(const DevSim &S, hipStream_t s);            // normalize + forces
 void launch_cells(const DevSim &S, hipStream_t s);            // type, stencil, p=0, div, flags
 void launch_fluid_list(const DevSim &S, hipStream_t s);       // row counts -> fl_list, fl_count
-void launch_rbgs(const DevSim &S, uint32_t parity, hipStream_t s);  // over the fluid list
 // `iters` red-black iterations, up to two per launch after one launch that lists each core's
 // FLUID cells, ping-ponging between the pressure array and cg_r0 (an even number of launches:
 // the result ends in the pressure array)

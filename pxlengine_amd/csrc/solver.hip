@@ -510,48 +510,21 @@ __global__ __launch_bounds__(BLOCK) void kb_norms_finalize(WorkList W, int nv) {
     for_each_work(W, [&](const DevSim &S, uint32_t, const uint32_t *) { body_norms_finalize(S, (uint32_t)S.cg_blocks, nv); });
 }
 
-// ---- pressure_rb (fluid.slang:168-182): one red-black half sweep over the cells of one
-// colour (list entries [0, n_red) or [n_red, n)).  Same arithmetic as the reference:
-// neighbour order -x,+x,-y,+y,-z,+z, AIR neighbours contribute 0.
-__global__ __launch_bounds__(SBLOCK) void k_rbgs(DevSim S, uint32_t parity) {
-    const uint32_t n_red = S.fl_count[0], n = S.fl_count[1];
-    uint32_t beg, end;
-    if (parity == 0) list_slice(0, n_red, blockIdx.x, gridDim.x, beg, end);
-    else list_slice(n_red, n, blockIdx.x, gridDim.x, beg, end);
-    int32_t off[6];
-    nb_offsets(S, off);
-    float *p = S.pressure;
-    for (uint32_t t = beg + threadIdx.x; t < end; t += SBLOCK) {
-        const uint32_t id = S.fl_list[t];
-        const uint32_t wd = S.fl_word[t];
-        // all six neighbours are loaded (one that is not FLUID at the cell itself) before any is
-        // added, and added by select: the same additions in the same order as adding only the
-        // FLUID ones, without a branch per load (each of which waited for its load)
-        float pn[6];
-#pragma unroll
-        for (int k = 0; k < 6; k++) pn[k] = p[w_fluid(wd, k) ? id + off[k] : id];
-        const float pc = p[id];
-        float sum = 0.0f;
-#pragma unroll
-        for (int k = 0; k < 6; k++) sum = w_fluid(wd, k) ? sum + pn[k] : sum;
-        const float cnt = w_diag(wd);
-        // a cell without non-SOLID neighbours keeps its value (stored back unchanged: a
-        // conditional store would let the compiler sink the list load behind the word's)
-        p[id] = cnt > 0.0f ? (sum - S.div[id] * S.h * S.h / S.dt) / cnt : pc;
-    }
-}
-
-// ---- pressure_rb, up to RB_HALO half sweeps per launch (temporal blocking).  A workgroup
-// owns a CX x CY x CZ core of the grid and holds the core plus RB_HALO cells on every side
-// (its region) in LDS, the pressures split by colour.  Half sweep s (1-based) of a launch
-// updates the region's FLUID cells of its colour at least s cells from the region's faces;
-// their six neighbours are then at least s - 1 from the faces and hold the values after half
-// sweep s - 1, so after s <= RB_HALO half sweeps the core holds what s whole-grid half sweeps
-// give.  Each update is k_rbgs's arithmetic in k_rbgs's order.  The sum runs over all six
-// neighbours: cells that are not FLUID hold pressure 0 (the cell pass zeroes them, and no
-// solver writes them), and a sum that starts at +0 is never -0, so adding a zero leaves it
-// unchanged and 0 + p0 + ... + p5 equals k_rbgs's sum over the FLUID neighbours bit for bit.
-// A FLUID cell whose neighbours are all SOLID keeps its value, as there.
+// ---- pressure_rb (fluid.slang:168-182): red-black half sweeps over the FLUID cells, up to
+// RB_HALO of them per launch (temporal blocking).  The update of a FLUID cell is the
+// reference's, in its fp32 order: p = (sum - div h h / dt) / cnt, with sum = 0 + the FLUID
+// neighbours' pressures in the order -x, +x, -y, +y, -z, +z and cnt its non-SOLID neighbours;
+// a cell with cnt = 0 keeps its value.
+//
+// A workgroup owns a CX x CY x CZ core of the grid and holds the core plus RB_HALO cells on
+// every side (its region) in LDS, the pressures split by colour.  Half sweep s (1-based) of a
+// launch updates the region's FLUID cells of its colour at least s cells from the region's
+// faces; their six neighbours are then at least s - 1 from the faces and hold the values
+// after half sweep s - 1, so after s <= RB_HALO half sweeps the core holds what s whole-grid
+// half sweeps give.  The sum runs over all six neighbours: cells that are not FLUID hold
+// pressure 0 (the cell pass zeroes them, and no sweep writes them), and a sum that starts at
+// +0 is never -0, so adding a zero leaves it unchanged and 0 + p0 + ... + p5 equals the sum
+// over the FLUID neighbours bit for bit.
 //
 // The FLUID cells do not change during the solve, so one launch per solve lists each core's
 // region cells to update (FLUID, at least one cell inside the region, with a non-SOLID
@@ -831,10 +804,6 @@ void launch_mixed_update(const DevSim &S, const double *xsrc, double *xdst, hipS
     PXL_LAUNCH(k_norms_finalize, 1, BLOCK, 0, s, S, solver_blocks(S), 1);
 }
 
-
-void launch_rbgs(const DevSim &S, uint32_t parity, hipStream_t s) {
-    PXL_LAUNCH(k_rbgs, solver_blocks(S), SBLOCK, 0, s, S, parity);
-}
 
 uint32_t rbgs_cores(const DevSim &S) {
     return (((uint32_t)S.nx + RB_CX - 1) / RB_CX) * (((uint32_t)S.ny + RB_CY - 1) / RB_CY) *
