@@ -716,7 +716,7 @@ __global__ __launch_bounds__(BLOCK) void k_reset_particles(DevSim S) {
 // particles, reduces it (lanes, then waves through LDS) and adds one atomic maximum.  The
 // atomics all go to one word, so their number is kept to the workgroups of one pass over the
 // chip (one per wave serialised some 7,000 atomics for 460K particles).
-constexpr uint32_t VMAX_BLOCKS = 512;
+constexpr uint32_t VMAX_BLOCKS = 128;
 __global__ __launch_bounds__(BLOCK) void k_vmax2(DevSim S, uint32_t *out) {
     __shared__ float sh[BLOCK / WAVE];
     float v2 = 0.0f;
