@@ -481,10 +481,15 @@ __device__ __forceinline__ const DevSim &batch_member(const DevSim *sims, uint32
 // Runs f(S, block, slots) for each work entry of this workgroup.  The first entry is loaded
 // together with the entry count (it lies in the list's storage even when the list is
 // shorter), so a workgroup's first item waits for one load instead of two.
-template <typename F>
+__device__ __forceinline__ uint32_t xcd_major(uint32_t b, uint32_t nb);
+// XCD_MAJOR: workgroup b starts at entry xcd_major(b) instead of b, so the entries one round
+// deals to an XCD are consecutive (used by the particle scatter, whose consecutive entries
+// write consecutive runs: 15% faster on the 512-request batch; the per-tile and solver walks
+// are slower that way).
+template <bool XCD_MAJOR = false, typename F>
 __device__ __forceinline__ void for_each_work(const WorkList &W, F &&f) {
     const uint32_t n = *W.count;
-    uint32_t e = blockIdx.x;
+    uint32_t e = XCD_MAJOR ? xcd_major(blockIdx.x, gridDim.x) : blockIdx.x;
     uint32_t w = W.entries[e];
     while (e < n) {
         const uint32_t sim = w >> WORK_SIM_SHIFT;

@@ -236,9 +236,9 @@ __device__ __forceinline__ void body_scatter(const DevSim &S, uint32_t b) {
     S.bvz[j] = S.avz[i];
     S.bid[j] = S.aid[i];
 }
-__global__ __launch_bounds__(BLOCK) void k_scatter(DevSim S) { body_scatter(S, blockIdx.x); }
+__global__ __launch_bounds__(BLOCK) void k_scatter(DevSim S) { body_scatter(S, xcd_major(blockIdx.x, gridDim.x)); }
 __global__ __launch_bounds__(BLOCK) void kb_scatter(WorkList W) {
-    for_each_work(W, [&](const DevSim &S, uint32_t b, const uint32_t *) { body_scatter(S, b); });
+    for_each_work<true>(W, [&](const DevSim &S, uint32_t b, const uint32_t *) { body_scatter(S, b); });
 }
 
 // ---------------------------------------------------------------- P2G / G2P
