@@ -475,8 +475,18 @@ private:
         HIP_CHECK(hipStreamSynchronize(stream_));
     }
 
-    // Fills domain d's entries for the members `ids` into the host staging buffer.
-    uint32_t fill(int d, const std::vector<uint32_t> &ids) {
+    // Fills domain d's entries for the members `ids` into the host staging buffer.  The
+    // one-workgroup-per-member domains (the multigrid tail, the scans) list the largest grids
+    // first, so their long workgroups start in the first round and the small ones fill in
+    // behind them (the entry order changes no member's arithmetic).
+    uint32_t fill(int d, const std::vector<uint32_t> &ids_in) {
+        std::vector<uint32_t> sorted;
+        if (d == WD_SIM || d == WD_A_SIM) {
+            sorted = ids_in;
+            std::stable_sort(sorted.begin(), sorted.end(),
+                             [&](uint32_t a, uint32_t b) { return members_[a]->d.ncells > members_[b]->d.ncells; });
+        }
+        const std::vector<uint32_t> &ids = sorted.empty() ? ids_in : sorted;
         uint32_t *out = h_entries_ + off_[d];
         uint32_t k = 0;
         for (uint32_t i : ids) {
