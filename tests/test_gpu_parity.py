@@ -415,6 +415,28 @@ def test_rbgs_bitwise_any_iteration_count(pxl, orc, iters):
     gpu.close()
 
 
+def test_rbgs_after_pressure_write_bitwise(pxl, orc):
+    """A pressure field written through the ABI -- values in every cell, FLUID or not -- and
+    then whole reference steps: the GPU equals the oracle fed the same field bit for bit.  The
+    blocked red-black sweeps sum all six neighbours and rely on cells that are not FLUID
+    holding 0 when the solve starts; the cell pass of the steps after a write restores that."""
+    fluid = dam_break(32)
+    print("rbgs after a pressure write", flush=True)
+    gpu = pxl.FluidState(fluid, FPS, seed=4)
+    sim = oracle_sim(orc, fluid, seed=4)
+    gpu.step(1)
+    sim.step_rbgs(80)
+    p = (np.random.default_rng(0).standard_normal(sim.pressure.shape) * 100.0).astype(np.float32)
+    gpu.write_field("pressure", p)
+    sim.pressure[:] = p
+    for s in range(3):
+        gpu.step(1)
+        sim.step_rbgs(80)
+        _state_equal(gpu, sim, f"step {s + 1} after the write")
+        np.testing.assert_array_equal(gpu.read_field("pressure"), sim.pressure)
+    gpu.close()
+
+
 @pytest.mark.parametrize("wg", [1, 7, 4096])
 def test_rbgs_bitwise_any_workgroups(pxl, orc, wg):
     """RBGS results do not depend on the solver launch shape."""
