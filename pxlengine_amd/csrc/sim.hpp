@@ -324,7 +324,9 @@ struct Sim {
         KRUN("p2g", launch_p2g(d, stream));
         KRUN("faces", launch_faces(d, stream));
         KRUN("cells", launch_cells(d, stream));
-        KRUN("fluid_list", launch_fluid_list(d, stream));
+        // the FLUID-cell list feeds the CG and MIXED solvers; the reference mode's blocked sweeps
+        // list each core's cells themselves
+        if (opts.solver != PXLFLIP_SOLVER_RBGS) KRUN("fluid_list", launch_fluid_list(d, stream));
         if (d.mg_levels) KRUN("mg_types", launch_mg_types(d, stream));
     }
     void enqueue_fixed_pressure() {
