@@ -57,7 +57,7 @@ typedef struct pxlflip_opts {
     int32_t device;           /* HIP device ordinal (default 0) */
     uint32_t scene_index;     /* i in "fluids[i] 'name': ..." validation messages */
     int32_t use_graph;        /* 1 = replay each step as a captured hipGraph (default 1) */
-    uint32_t solver_workgroups; /* workgroups of the pressure-solver launches; 0 = default (256), max 4096 */
+    uint32_t solver_workgroups; /* workgroups of the CG / MIXED solver launches; 0 = default (256), max 4096 (RBGS runs one workgroup per 8x8x16 core) */
     uint32_t max_cells;       /* grid cell limit of the validation: 0 = the reference's 16,000,000
                                  (crates/pxl-scene/src/lib.rs:732); larger admits bigger grids up to the
                                  resolution limit, 256^3 (extension) */
@@ -269,10 +269,11 @@ int pxlflip_profile_steps(pxlflip *h, uint32_t n, pxlflip_kernel_time *out, uint
  * (s_clear, s_splat, s_phi, s_vflag, s_vscan, s_verts, s_qcount, s_qscan, s_quads). */
 int pxlflip_profile_surface(pxlflip *h, uint32_t n, pxlflip_kernel_time *out, uint32_t max_out, uint32_t *n_out);
 
-/* Autotuner of the solver launch shape: for each candidate workgroup count of the pressure
- * kernels (64, 128, 256, 512, 1024) runs `steps` timed steps, keeps the fastest, and restores the
- * particles, ids and pressure so the simulation state and steps_done are unchanged.  Returns
- * the choice and its average event-timed step time. */
+/* Autotuner of the solver launch shape: for each candidate workgroup count of the CG / MIXED
+ * solver kernels (64, 128, 256, 512, 1024) runs `steps` timed steps, keeps the fastest, and
+ * restores the particles, ids and pressure so the simulation state and steps_done are
+ * unchanged.  Returns the choice and its average event-timed step time.  (RBGS sweeps run one
+ * workgroup per 8x8x16 core whatever the choice.) */
 int pxlflip_autotune(pxlflip *h, uint32_t steps, uint32_t *chosen_workgroups, double *best_ms);
 
 /* Last error message of a handle; with NULL, the calling thread's last create error. */
