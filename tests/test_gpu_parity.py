@@ -62,8 +62,8 @@ def test_obstacles_grid_phase_bitwise(pxl, orc):
 
 
 @pytest.mark.parametrize("fluid,steps", [(dam_break(32), 8), (EXAMPLE_DAM_BREAK, 2), (two_blocks(24), 6),
-                                         (sphere_drop(28), 6), (obstacle_scene(24), 8)],
-                         ids=["dam32", "example", "collide24", "sphere28", "obstacles24"])
+                                         (sphere_drop(28), 6), (obstacle_scene(24), 8), (dam_break(16), 6)],
+                         ids=["dam32", "example", "collide24", "sphere28", "obstacles24", "dam16"])
 def test_rbgs_steps_bitwise(pxl, orc, fluid, steps):
     """Whole reference steps (RBGS, pressure_iterations sweeps): particles, pressure and face
     velocities equal the oracle bit for bit after every step (from the third step on the band's
