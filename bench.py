@@ -501,7 +501,8 @@ def bench_serve(args, pxl):
             "capacity": {"offered": "all requests at t = 0", "wall_s": wall, "span_s": st["span_s"],
                          "mean_batch": st["mean_batch"], "batch_steps": st["batch_steps"],
                          "busy_fraction": st["busy_s"] / wall, "admit_s": st["admit_s"],
-                         "graph_captures": st["graph_captures"], "peak_bytes": st["peak_bytes"],
+                         "graph_captures": st["graph_captures"], "capture_s": st["capture_s"],
+                         "peak_bytes": st["peak_bytes"],
                          "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"]},
             "latency": {"arrivals": "Poisson", "load": args.load, "rate_per_s": rate,
                         "rate_rule": f"{args.load} x capacity / mean request work ({work.mean():.4g} particle-steps)",

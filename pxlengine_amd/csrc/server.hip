@@ -709,8 +709,10 @@ bool set_batch_shape(int family, unsigned lanes) {
 }  // namespace pxl
 
 // Host threads that seed and allocate admitted requests (seeding a large liquid is several
-// milliseconds of host work).
-constexpr int ALLOCATOR_THREADS = 4;
+// milliseconds of host work).  The batch fills sooner with more of them: on the 512-request
+// capacity run 8 threads give 3% more particle-steps/s than 4 (mean batch 394-404 against
+// 366-375), and 16 no more than 8.
+constexpr int ALLOCATOR_THREADS = 8;
 
 struct pxlflip_server {
     pxlflip_server_opts opts{};
