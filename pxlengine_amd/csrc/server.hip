@@ -711,7 +711,9 @@ bool set_batch_shape(int family, unsigned lanes) {
 // Host threads that seed and allocate admitted requests (seeding a large liquid is several
 // milliseconds of host work).  The batch fills sooner with more of them: on the 512-request
 // capacity run 8 threads give 3% more particle-steps/s than 4 (mean batch 394-404 against
-// 366-375), and 16 no more than 8.
+// 366-375), and 16 no more than 8.  Each thread prepares its requests on one stream of its
+// own, created with the server: creating a stream per request took ~2 ms of the runtime's
+// time each (1.1 s of the 512 requests' admission), one request after another.
 constexpr int ALLOCATOR_THREADS = 8;
 
 struct pxlflip_server {
@@ -734,6 +736,7 @@ struct pxlflip_server {
     bool worker_done = false;  // the device thread has left: nothing more is queued to free
     std::thread worker;
     std::vector<std::thread> allocators;  // allocate + seed admitted requests off the device thread
+    std::vector<hipStream_t> alloc_streams;  // one per allocator thread
     std::thread releaser;                 // frees finished requests' simulations
     std::string error;
     // accounting
@@ -746,8 +749,12 @@ struct pxlflip_server {
     std::vector<std::pair<std::string, pxl::Batch::Agg>> kernel_stats;  // profiled batch steps
     double admit_s = 0;  // allocator-thread time spent allocating admitted requests
 
+    ~pxlflip_server() {  // the allocator threads have left (or never started)
+        for (hipStream_t st : alloc_streams)
+            if (st) (void)hipStreamDestroy(st);
+    }
     void run();
-    void allocate_loop();
+    void allocate_loop(int k);
     void free_loop();
     void finish(pxl::Request *r, int status, const std::string &err);
 };
@@ -755,13 +762,15 @@ struct pxlflip_server {
 namespace pxl {
 namespace {
 
-// Allocates a request's simulation on its own stream (allocator thread).  Throws on failure.
-void allocate_request(pxlflip_server *srv, Request *r) {
+// Allocates a request's simulation on the allocator thread's stream `st` (allocate and reset
+// end with that stream drained).  Throws on failure.
+void allocate_request(pxlflip_server *srv, Request *r, hipStream_t st) {
     auto sim = std::make_unique<Sim>();
     if (prepare(r->json.c_str(), r->fps, r->seed, &srv->sim_opts, *sim, nullptr, nullptr) != PXLFLIP_OK)
         throw std::runtime_error(g_create_error);
     sim->d.cg_blocks = (int)(srv->opts.solver_workgroups ? srv->opts.solver_workgroups : batch_solver_blocks(*sim));
-    HIP_CHECK(hipStreamCreateWithFlags(&sim->stream, hipStreamNonBlocking));
+    sim->stream = st;
+    sim->owns_stream = false;
     sim->allocate();
     sim->reset();
     r->sim = std::move(sim);
@@ -814,7 +823,7 @@ void pxlflip_server::free_loop() {
     }
 }
 
-void pxlflip_server::allocate_loop() {
+void pxlflip_server::allocate_loop(int k) {
     using namespace pxl;
     try {
         HIP_CHECK(hipSetDevice(opts.device));
@@ -830,7 +839,7 @@ void pxlflip_server::allocate_loop() {
         const double t0 = now_s();
         std::string err;
         try {
-            allocate_request(this, r);
+            allocate_request(this, r, alloc_streams[(size_t)k]);
         } catch (const HipError &e) {
             err = std::string("HIP error in ") + e.what + ": " + hipGetErrorString(e.code);
         } catch (const std::bad_alloc &) {
@@ -972,10 +981,10 @@ void pxlflip_server::run() {
                 if (done && err.empty() && r->keep) {
                     r->result.assign((size_t)s->d.np * 8, 0.0f);
                     DeviceBuffer buf((size_t)s->d.np * 32);
-                    launch_unpack_aos8(s->d, buf.as<float>(), s->stream);
-                    HIP_CHECK(hipMemcpyAsync(r->result.data(), buf.p, (size_t)s->d.np * 32, hipMemcpyDeviceToHost,
-                                             s->stream));
-                    HIP_CHECK(hipStreamSynchronize(s->stream));
+                    const hipStream_t bs = batch->stream();  // the device thread's own
+                    launch_unpack_aos8(s->d, buf.as<float>(), bs);
+                    HIP_CHECK(hipMemcpyAsync(r->result.data(), buf.p, (size_t)s->d.np * 32, hipMemcpyDeviceToHost, bs));
+                    HIP_CHECK(hipStreamSynchronize(bs));
                 }
             } catch (const HipError &e) {
                 err = std::string("HIP error in ") + e.what + ": " + hipGetErrorString(e.code);
@@ -1126,6 +1135,11 @@ int pxlflip_server_create(const pxlflip_server_opts *opts, pxlflip_server **out)
         }
         HIP_CHECK(hipSetDevice(o.device));
         srv->batch = std::make_unique<Batch>(srv->sim_opts);
+        for (int k = 0; k < ALLOCATOR_THREADS; k++) {
+            hipStream_t st = nullptr;
+            HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+            srv->alloc_streams.push_back(st);
+        }
     } catch (const HipError &e) {
         g_create_error = std::string("HIP error in ") + e.what + ": " + hipGetErrorString(e.code);
         return PXLFLIP_ERR_DEVICE;
@@ -1134,7 +1148,7 @@ int pxlflip_server_create(const pxlflip_server_opts *opts, pxlflip_server **out)
         return PXLFLIP_ERR_OOM;
     }
     pxlflip_server *p = srv.release();
-    for (int k = 0; k < ALLOCATOR_THREADS; k++) p->allocators.emplace_back([p] { p->allocate_loop(); });
+    for (int k = 0; k < ALLOCATOR_THREADS; k++) p->allocators.emplace_back([p, k] { p->allocate_loop(k); });
     p->releaser = std::thread([p] { p->free_loop(); });
     p->worker = std::thread([p] { p->run(); });
     *out = p;

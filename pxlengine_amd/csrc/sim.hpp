@@ -107,6 +107,8 @@ struct Sim {
     uint32_t iters = 80;          // RBGS sweeps or CG iterations per step
     uint64_t steps_done = 0;
     bool initialized = false;
+    // false when `stream` is lent by its creator (a serving allocator thread's stream)
+    bool owns_stream = true;
     hipGraphExec_t step_graph = nullptr;
     std::map<uint32_t, std::array<hipGraphExec_t, 2>> inner_graph;  // [MG-CG iterations][x64 parity]
     static constexpr uint32_t MAX_SUBSTEP_GRAPHS = 8;
@@ -178,7 +180,7 @@ struct Sim {
         if (host_ctr) (void)hipHostFree(host_ctr);
         if (surf_arena) (void)hipFree(surf_arena);
         if (arena) (void)hipFree(arena);
-        if (stream) (void)hipStreamDestroy(stream);
+        if (stream && owns_stream) (void)hipStreamDestroy(stream);
     }
 
     // ---- allocation
